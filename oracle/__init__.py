@@ -1,0 +1,246 @@
+"""CPU oracle of the camera-rendering path — TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings of liboracle.so (built from the C sources beside this file by `make -C oracle`). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this package, and only as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        f64, f32, u32, i32, vp = C.c_double, C.c_float, C.c_uint32, C.c_int, C.c_void_p
+        sig = {
+            "or_mulberry32_fill": (None, [u32, vp, C.c_size_t]),
+            "or_pcg": (u32, [u32]),
+            "or_hash3": (u32, [u32, u32, u32]),
+            "or_u01": (f32, [u32]),
+            "or_kr": (f32, [u32, f32, f32]),
+            "or_distort": (f64, [vp, f64]),
+            "or_distort_deriv": (f64, [vp, f64]),
+            "or_undistort_theta": (f64, [vp, f64, i32]),
+            "or_project": (None, [vp, vp, vp]),
+            "or_unproject": (None, [vp, vp, vp]),
+            "or_max_monotone_theta": (f64, [vp, f64]),
+            "or_max_image_theta": (f64, [vp, i32, i32]),
+            "or_unproject_f32": (None, [vp, vp, vp]),
+            "or_build_bvh": (i32, [vp, vp, u32, i32, i32, vp, u32, vp, vp]),
+            "or_pack_triangles": (None, [vp, vp, vp, u32, vp]),
+            "or_sah_cost": (f64, [vp, u32, vp, vp, f64, f64]),
+            "or_trace_batch": (None, [vp, vp, vp, u32, i32, vp, vp, i32]),
+            "or_brute_batch": (None, [vp, u32, vp, u32, vp, i32]),
+            "or_fresnel_dielectric": (f64, [f64, f64]),
+            "or_fresnel_average": (f64, [f64, i32]),
+            "or_ggx_d": (f64, [f64, f64]),
+            "or_ggx_g1": (f64, [f64, f64]),
+            "or_ggx_g2": (f64, [f64, f64, f64]),
+            "or_eval_bsdf": (None, [vp, vp, vp, vp, vp]),
+            "or_pdf_bsdf": (f64, [vp, vp, vp, vp]),
+            "or_sample_bsdf": (f64, [vp, vp, vp, vp, vp]),
+            "or_spec_prob": (f64, [vp, f64]),
+            "or_mat_unpack": (None, [vp, vp]),
+            "or_albedo_table": (None, [vp, i32, i32, vp, vp, vp, vp]),
+            "or_radinv2": (f64, [u32]),
+            "or_profile": (f64, [f64, vp, vp, i32]),
+            "or_capsule_occ": (f64, [vp, vp, vp, vp, f64]),
+            "or_gauss_legendre": (None, [i32, vp, vp]),
+            "or_disc_irradiance": (f64, [vp, vp, vp, vp, f64, f64, i32, i32]),
+            "or_expf": (f32, [f32]),
+            "or_logf": (f32, [f32]),
+            "or_cosf": (f32, [f32]),
+            "or_erfa": (f32, [f32]),
+            "or_sensor_expose": (None, [vp, vp, vp, vp]),
+            "or_ri_map": (None, [vp, i32, i32, f64, vp]),
+            "or_raw_noise_dn": (f64, [f64, f64, f64, f64]),
+            "or_render_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
+            "or_camera_ray": (None, [vp, u32, u32, u32, vp, vp]),
+            "or_jacobi_dense": (i32, [vp, vp, u32, u32, f64, i32, vp, vp]),
+            "or_jacobi_gather": (None, [u32, vp, vp, u32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
+            "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
+            "or_gather_dir": (None, [u32, u32, u32, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Lens(C.Structure):
+    _fields_ = [("fx", C.c_double), ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("alpha", C.c_double), ("k", C.c_double * 4)]
+
+
+def lens(fx, fy, cx, cy, k, alpha=0.0):
+    return Lens(fx, fy, cx, cy, alpha, (C.c_double * 4)(*k))
+
+
+class Mat(C.Structure):
+    _fields_ = [("kind", C.c_int), ("c", C.c_double * 3), ("alpha", C.c_double), ("roughness", C.c_double),
+                ("ior", C.c_double), ("fbar", C.c_double), ("fbar_computed", C.c_double),
+                ("alpha_c", C.c_double), ("tau", C.c_double * 3)]
+
+
+def mat_from_record(rec):
+    m = Mat()
+    r = np.ascontiguousarray(rec, dtype=np.float32)
+    lib().or_mat_unpack(ptr(r), C.byref(m))
+    return m
+
+
+class Sensor(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("cfa", C.c_uint32), ("frame", C.c_uint32),
+                ("optics_k", C.c_float), ("esf_sigma", C.c_float), ("exposure", C.c_float), ("gain", C.c_float),
+                ("m", C.c_float * 9), ("read_noise", C.c_float), ("full_well", C.c_float), ("dark", C.c_float),
+                ("conv_gain", C.c_float), ("max_dn", C.c_float), ("black", C.c_float), ("noise_on", C.c_float),
+                ("seed", C.c_uint32), ("cam_index", C.c_uint32)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("node_count", C.c_uint32), ("tris", C.c_void_p), ("tri_count", C.c_uint32),
+                ("verts", C.c_void_p), ("vert_count", C.c_uint32), ("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
+                ("materials", C.c_void_p), ("mat_count", C.c_uint32), ("lights", C.c_void_p),
+                ("light_count", C.c_uint32), ("emit", C.c_void_p), ("emit_count", C.c_uint32),
+                ("emit_area", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("R", C.c_float * 9), ("pos", C.c_float * 3), ("mode", C.c_uint32), ("lens", C.c_float * 9),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32),
+                ("cam_index", C.c_uint32), ("light_samples", C.c_uint32), ("max_depth", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+# ---------------------------------------------------------------------------- helpers
+
+def build_bvh(pos, idx, leaf_size=4, bins=16):
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    n = idx.size // 3
+    cap = max(4, n + 2)
+    nodes = np.zeros(cap * 16, dtype=np.float32)
+    order = np.zeros(max(n, 1), dtype=np.uint32)
+
+    class Info(C.Structure):
+        _fields_ = [("node_count", C.c_uint32), ("max_depth", C.c_int), ("bmin", C.c_float * 3), ("bmax", C.c_float * 3)]
+    info = Info()
+    rc = lib().or_build_bvh(ptr(pos), ptr(idx), n, leaf_size, bins, ptr(nodes), cap, ptr(order), C.byref(info))
+    assert rc == 0
+    nc = info.node_count
+    return dict(nodes=nodes[: nc * 16].copy(), node_count=nc, order=order[:n].copy(), max_depth=info.max_depth,
+                bmin=np.array(info.bmin[:]), bmax=np.array(info.bmax[:]))
+
+
+def pack_triangles(pos, idx, order):
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    order = np.ascontiguousarray(order, dtype=np.uint32)
+    out = np.zeros(order.size * 16, dtype=np.float32)
+    lib().or_pack_triangles(ptr(pos), ptr(idx), ptr(order), order.size, ptr(out))
+    return out
+
+
+def trace(nodes, tris, rays, any_hit=False, nthreads=8, visits=False):
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    hits = np.zeros((n, 4), dtype=np.float64)
+    vis = np.zeros((n, 2), dtype=np.uint64) if visits else None
+    lib().or_trace_batch(ptr(nodes), ptr(tris), ptr(rays), n, int(any_hit), ptr(hits),
+                         ptr(vis) if visits else None, nthreads)
+    return (hits, vis) if visits else hits
+
+
+def brute(tris, count, rays, nthreads=8):
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    hits = np.zeros((rays.shape[0], 4), dtype=np.float64)
+    lib().or_brute_batch(ptr(tris), count, ptr(rays), rays.shape[0], ptr(hits), nthreads)
+    return hits
+
+
+def scene_struct(sc):
+    """sc: dict of the upload arrays (nodes, tris, verts, meshes, materials, lights, emit, emit_area)."""
+    keep = {k: np.ascontiguousarray(sc[k]) for k in ("nodes", "tris", "verts", "meshes", "materials", "lights", "emit")}
+    s = Scene(ptr(keep["nodes"]), keep["nodes"].size // 16, ptr(keep["tris"]), keep["tris"].size // 16,
+              ptr(keep["verts"]), keep["verts"].size // 8, ptr(keep["meshes"]), keep["meshes"].size // 4,
+              ptr(keep["materials"]), keep["materials"].size // 16, ptr(keep["lights"]), keep["lights"].size // 4,
+              ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]))
+    s._keep = keep
+    return s
+
+
+def camera_struct(cam):
+    c = Camera()
+    c.R[:] = [float(x) for x in cam["R"]]
+    c.pos[:] = [float(x) for x in cam["pos"]]
+    c.mode = int(cam.get("mode", 0))
+    c.lens[:] = [float(x) for x in cam["lens"]]
+    for k in ("width", "height", "spp", "seed", "cam_index", "light_samples", "max_depth", "flags"):
+        setattr(c, k, int(cam.get(k, 0)))
+    return c
+
+
+def render_tile(sc, cam, x0, y0, w, h, spp_begin=0, spp_end=None, nthreads=8):
+    S = scene_struct(sc)
+    Cm = camera_struct(cam)
+    spp_end = cam["spp"] if spp_end is None else spp_end
+    out = np.zeros((h, w, 4), dtype=np.float32)
+    rays = C.c_uint64(0)
+    vis = np.zeros(2, dtype=np.uint64)
+    lib().or_render_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(out), C.byref(rays), ptr(vis),
+                         nthreads)
+    return out, int(rays.value), vis
+
+
+def camera_ray(cam, x, y, i):
+    Cm = camera_struct(cam)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().or_camera_ray(C.byref(Cm), x, y, i, ptr(o), ptr(d))
+    return o, d
+
+
+def sensor_expose(params, radiance, ri):
+    radiance = np.ascontiguousarray(radiance, dtype=np.float32)
+    ri = np.ascontiguousarray(ri, dtype=np.float32)
+    h, w = ri.shape
+    dn = np.zeros((h, w), dtype=np.uint16)
+    s = sensor_struct(params, w, h)
+    lib().or_sensor_expose(C.byref(s), ptr(radiance), ptr(ri), ptr(dn))
+    return dn
+
+
+def sensor_struct(p, w, h):
+    s = Sensor()
+    s.width, s.height, s.cfa, s.frame = w, h, int(p["cfa"]), int(p["frame"])
+    s.optics_k, s.esf_sigma, s.exposure, s.gain = p["optics_k"], p["esf_sigma"], p["exposure"], p["gain"]
+    s.m[:] = [float(x) for x in np.asarray(p["m"]).reshape(-1)]
+    s.read_noise, s.full_well, s.dark, s.conv_gain = p["read_noise"], p["full_well"], p["dark"], p["conv_gain"]
+    s.max_dn, s.black, s.noise_on = p["max_dn"], p["black"], 1.0 if p["noise_on"] else 0.0
+    s.seed, s.cam_index = int(p["seed"]), int(p["cam_index"])
+    return s
+
+
+def ri_map(L, w, h, pupil_exp=1.0):
+    out = np.zeros((h, w), dtype=np.float32)
+    lib().or_ri_map(C.byref(L), w, h, pupil_exp, ptr(out))
+    return out

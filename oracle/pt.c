@@ -1,0 +1,251 @@
+/* Oracle (test infrastructure only): the camera estimator, FP64 geometry and shading on FP32 scene data.
+ *
+ * The estimator is the build's composition of the reference's pieces (the reference itself is a hybrid
+ * that reads a radiosity cache at static surfaces; the configurations ask for paths of depth D with NEE):
+ *   sample pattern   pixSeeds / subSample, gpu/wgsl/kernels.wgsl.js:34-53 (binary32, bit-exact with the GPU)
+ *   camera ray       gpu/lens.js:54-62 unproject (FP64, 20 Newton steps) of the binary32 sample position
+ *   ray queries      gpu/bvh.js:224-250 traverse (closest, any hit), Moller-Trumbore gpu/bvh.js:189-206
+ *   surface          gpu/wgsl/trace.wgsl.js:330-358 (two-sided normals, 2e-6 m origin offset)
+ *   emitter sample   gpu/wgsl/shade.wgsl.js:31-47 (area CDF, sqrt-u point on the triangle)
+ *   NEE + MIS        gpu/wgsl/shade.wgsl.js:160-188 (solid-angle pdf d^2/(cos_e A), power heuristic,
+ *                    shadow ray to dist - 2e-5); emitter hit by a BSDF ray weighted as :224-236
+ *   BSDF             gpu/bsdf.js (+ the Lambertian kind)
+ * Path of depth D: vertices v1..vD. NEE with MIS at v1..v(D-1), NEE without MIS at vD (no continuation from
+ * vD, so its light paths are carried by NEE alone); a BSDF continuation from vk (k < D) that hits an emitter
+ * adds its MIS-weighted emission and ends the path. An emitter seen by the camera ray adds its radiance.
+ * Random numbers: the reference's Kronecker dimensions at v1; counter hashes u01(hash3(hs, i, 256 b + dim))
+ * from v2 on (the same on the GPU). No Russian roulette.
+ * Pins (tests/test_oracle_pt.py): white furnace inside a closed Lambertian sphere (L = Le rho^k sums), the
+ * on-axis disc irradiance closed form through NEE, and convergence of independent seeds to one value.
+ */
+#include "vt_oracle.h"
+#include <math.h>
+#include <string.h>
+
+#define F_EMITTER 2u
+static const float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
+static const float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
+static const float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
+
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void norm3(double *a) {
+  double l = sqrt(dot3(a, a));
+  a[0] /= l; a[1] /= l; a[2] /= l;
+}
+static inline double maxc(const double *a) { return fmax(a[0], fmax(a[1], a[2])); }
+
+typedef struct { uint32_t hs; float s[9]; } pix_t;
+
+static void pix_seeds(const or_camera *C, uint32_t x, uint32_t y, pix_t *P) {
+  P->hs = or_hash3(x, y, C->seed * 64u + C->cam_index);
+  for (int k = 0; k < 9; k++) P->s[k] = or_u01(or_pcg(P->hs + (uint32_t)k));
+}
+static inline double rnd(const pix_t *P, uint32_t i, uint32_t b, uint32_t dim) {
+  return or_u01(or_hash3(P->hs, i, 256u * b + dim));
+}
+
+static void cam_dir(const or_camera *C, const pix_t *P, uint32_t x, uint32_t y, uint32_t i, double d[3]) {
+  float uv[2];
+  {
+    volatile float a = (float)x + or_kr(i, R2A, P->s[0]);
+    volatile float b = (float)y + or_kr(i, R2B, P->s[1]);
+    uv[0] = a - 0.5f; uv[1] = b - 0.5f;
+  }
+  double dc[3];
+  if (C->mode == 0) {
+    or_lens L = {C->lens[0], C->lens[1], C->lens[2], C->lens[3], C->lens[8], {C->lens[4], C->lens[5], C->lens[6], C->lens[7]}};
+    double duv[2] = {uv[0], uv[1]};
+    or_unproject(&L, duv, dc);
+  } else {
+    dc[0] = ((double)uv[0] - C->lens[2]) / C->lens[0];
+    dc[1] = ((double)uv[1] - C->lens[3]) / C->lens[1];
+    dc[2] = 1;
+    norm3(dc);
+  }
+  for (int k = 0; k < 3; k++) d[k] = (double)C->R[k] * dc[0] + (double)C->R[3 + k] * dc[1] + (double)C->R[6 + k] * dc[2];
+  norm3(d);
+}
+
+void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float o[3], float d[3]) {
+  pix_t P;
+  pix_seeds(C, x, y, &P);
+  double dd[3];
+  cam_dir(C, &P, x, y, i, dd);
+  for (int k = 0; k < 3; k++) { o[k] = C->pos[k]; d[k] = (float)dd[k]; }
+}
+
+typedef struct {
+  double p[3], ng[3], ns[3], b[3];
+  uint32_t vi[3], mesh, mat, flags, light;
+} surf_t;
+
+static void surface(const or_scene *S, const or_hit *h, const double o[3], const double d[3], surf_t *s) {
+  const float *r = S->tris + (size_t)h->tri * 16;
+  const uint32_t *ru = (const uint32_t *)r;
+  s->vi[0] = ru[3]; s->vi[1] = ru[7]; s->vi[2] = ru[11];
+  s->mesh = ru[12];
+  const uint32_t *m = S->meshes + 4 * (size_t)s->mesh;
+  s->mat = m[0]; s->flags = m[1]; s->light = m[2];
+  s->b[0] = 1 - h->u - h->v; s->b[1] = h->u; s->b[2] = h->v;
+  double e1[3] = {r[4], r[5], r[6]}, e2[3] = {r[8], r[9], r[10]};
+  s->ng[0] = e1[1] * e2[2] - e1[2] * e2[1]; s->ng[1] = e1[2] * e2[0] - e1[0] * e2[2]; s->ng[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  norm3(s->ng);
+  for (int k = 0; k < 3; k++) {
+    s->ns[k] = 0;
+    for (int j = 0; j < 3; j++) s->ns[k] += s->b[j] * S->verts[8 * (size_t)s->vi[j] + 4 + k];
+  }
+  norm3(s->ns);
+  if (dot3(s->ng, d) > 0) for (int k = 0; k < 3; k++) s->ng[k] = -s->ng[k];
+  if (dot3(s->ns, s->ng) < 0) for (int k = 0; k < 3; k++) s->ns[k] = -s->ns[k];
+  double md[3] = {-d[0], -d[1], -d[2]};
+  if (dot3(s->ns, md) < 1e-4) memcpy(s->ns, s->ng, sizeof(s->ng));
+  for (int k = 0; k < 3; k++) s->p[k] = o[k] + d[k] * h->t;
+}
+
+/* outward emitter normal at a hit: the interpolated vertex normal, unflipped (shade.wgsl.js:217-220) */
+static void emitter_normal(const or_scene *S, const surf_t *s, double n[3]) {
+  for (int k = 0; k < 3; k++) {
+    n[k] = 0;
+    for (int j = 0; j < 3; j++) n[k] += s->b[j] * S->verts[8 * (size_t)s->vi[j] + 4 + k];
+  }
+  norm3(n);
+}
+
+static void offset_ray(const double p[3], const double ng[3], const double dir[3], double out[3]) {
+  double s = dot3(ng, dir) > 0 ? 1 : -1;
+  for (int k = 0; k < 3; k++) out[k] = p[k] + ng[k] * (s * 2e-6);
+}
+
+/* shade.wgsl.js:31-47: area-CDF pick, uniform point; returns the emitter record */
+static const float *sample_emitter(const or_scene *S, const double u[3], double p[3]) {
+  uint32_t lo = 0, hi = S->emit_count - 1;
+  for (int it = 0; it < 40; it++) {
+    if (lo >= hi) break;
+    uint32_t m = (lo + hi) / 2;
+    if ((double)S->emit[16 * (size_t)m + 3] < u[0]) lo = m + 1; else hi = m;
+  }
+  const float *e = S->emit + 16 * (size_t)lo;
+  double su = sqrt(u[1]);
+  for (int k = 0; k < 3; k++) p[k] = (double)e[k] + (double)e[4 + k] * (su * (1 - u[2])) + (double)e[8 + k] * (su * u[2]);
+  return e;
+}
+
+static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
+                              uint32_t i, double out[3], uint64_t *rays, uint64_t *visits) {
+  double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
+  cam_dir(C, P, x, y, i, d);
+  double thr[3] = {1, 1, 1}, L[3] = {0, 0, 0}, pB = 0;
+  const uint32_t nl = C->light_samples < 1 ? 1 : C->light_samples > 16 ? 16 : C->light_samples;
+  const uint32_t D = C->max_depth < 1 ? 1 : C->max_depth;
+  const double A = S->emit_area;
+  for (uint32_t b = 0; b < D; b++) {
+    or_hit h;
+    or_traverse(S->nodes, S->tris, o, d, 0, INFINITY, 0, 0, &h);
+    (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
+    if (h.tri < 0) break;
+    surf_t s;
+    surface(S, &h, o, d, &s);
+    if (s.flags & F_EMITTER) {
+      double ne[3], md[3] = {-d[0], -d[1], -d[2]};
+      emitter_normal(S, &s, ne);
+      double cosE = dot3(md, ne);
+      if (cosE > 0) {
+        const float *Le = S->lights + 4 * (size_t)s.light;
+        double w = 1;
+        if (b > 0) {
+          double pL = h.t * h.t / (cosE * A);
+          w = pB * pB / (pB * pB + nl * nl * pL * pL);
+        }
+        for (int k = 0; k < 3; k++) L[k] += thr[k] * Le[k] * w;
+      }
+      break;
+    }
+    or_mat M;
+    or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
+    double wo[3] = {-d[0], -d[1], -d[2]};
+    const int last = b + 1 == D;
+    double Ln[3] = {0, 0, 0};
+    for (uint32_t j = 0; j < nl; j++) {
+      double u[3];
+      if (b == 0) {
+        uint32_t li = i * nl + j;
+        u[0] = or_kr(li, KL0, P->s[3]); u[1] = or_kr(li, KL1, P->s[4]); u[2] = or_kr(li, KL2, P->s[5]);
+      } else {
+        u[0] = rnd(P, i, b, 3 * j); u[1] = rnd(P, i, b, 3 * j + 1); u[2] = rnd(P, i, b, 3 * j + 2);
+      }
+      double pl[3];
+      const float *e = sample_emitter(S, u, pl);
+      double wi[3] = {pl[0] - s.p[0], pl[1] - s.p[1], pl[2] - s.p[2]};
+      double d2 = dot3(wi, wi), dist = sqrt(d2);
+      for (int k = 0; k < 3; k++) wi[k] /= dist;
+      double en[3] = {e[12], e[13], e[14]};
+      double cosE = -dot3(wi, en);
+      if (cosE <= 0) continue;
+      double cosS = dot3(s.ns, wi);
+      if (cosS <= 0 || dot3(s.ng, wi) <= 0) continue;
+      uint32_t lid;
+      memcpy(&lid, e + 11, 4);
+      const float *Le = S->lights + 4 * (size_t)lid;
+      if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) continue;
+      double f[3];
+      or_eval_bsdf(&M, s.ns, wi, wo, f);
+      if (maxc(f) <= 0) continue;
+      double so[3];
+      offset_ray(s.p, s.ng, wi, so);
+      or_hit sh;
+      or_traverse(S->nodes, S->tris, so, wi, 0, dist - 2e-5, 1, 0, &sh);
+      (*rays)++; visits[0] += sh.node_visits; visits[1] += sh.tri_tests;
+      if (sh.tri >= 0) continue;
+      double pW = d2 / (cosE * A);
+      double w = 1;
+      if (!last) {
+        double pb = or_pdf_bsdf(&M, s.ns, wo, wi);
+        w = (double)nl * nl * pW * pW / ((double)nl * nl * pW * pW + pb * pb);
+      }
+      for (int k = 0; k < 3; k++) Ln[k] += f[k] * Le[k] * (cosS * w / pW);
+    }
+    for (int k = 0; k < 3; k++) L[k] += thr[k] * Ln[k] / nl;
+    if (last) break;
+    double ub[3];
+    if (b == 0) { ub[0] = or_kr(i, KB0, P->s[6]); ub[1] = or_kr(i, KB1, P->s[7]); ub[2] = or_kr(i, KB2, P->s[8]); }
+    else { ub[0] = rnd(P, i, b, 200); ub[1] = rnd(P, i, b, 201); ub[2] = rnd(P, i, b, 202); }
+    double l[3];
+    double pdf = or_sample_bsdf(&M, s.ns, wo, ub, l);
+    if (!(pdf > 0) || dot3(s.ng, l) <= 0) break;
+    double f[3];
+    or_eval_bsdf(&M, s.ns, l, wo, f);
+    double c = dot3(s.ns, l) / pdf;
+    for (int k = 0; k < 3; k++) thr[k] *= f[k] * c;
+    if (maxc(thr) <= 0) break;
+    offset_ray(s.p, s.ng, l, o);
+    memcpy(d, l, sizeof(l));
+    pB = pdf;
+  }
+  memcpy(out, L, sizeof(L));
+  return 0;
+}
+
+void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads) {
+  uint64_t nr = 0, v0 = 0, v1 = 0;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1) reduction(+ : nr, v0, v1) collapse(2)
+  for (uint32_t yy = 0; yy < h; yy++)
+    for (uint32_t xx = 0; xx < w; xx++) {
+      uint32_t x = x0 + xx, y = y0 + yy;
+      pix_t P;
+      pix_seeds(C, x, y, &P);
+      double acc[3] = {0, 0, 0};
+      uint64_t r = 0, vv[2] = {0, 0};
+      for (uint32_t i = spp_begin; i < spp_end; i++) {
+        double L[3];
+        sample_radiance(S, C, &P, x, y, i, L, &r, vv);
+        for (int k = 0; k < 3; k++) acc[k] += L[k];
+      }
+      double inv = spp_end > spp_begin ? 1.0 / (spp_end - spp_begin) : 0;
+      float *o = out + 4 * ((size_t)yy * w + xx);
+      o[0] = (float)(acc[0] * inv); o[1] = (float)(acc[1] * inv); o[2] = (float)(acc[2] * inv); o[3] = 1;
+      nr += r; v0 += vv[0]; v1 += vv[1];
+    }
+  if (rays) *rays = nr;
+  if (visits) { visits[0] = v0; visits[1] = v1; }
+}

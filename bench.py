@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Camera-path benchmark on one MI355X: the full-scale forward frame.
+
+Workload (default): the seeded 2,013,432-triangle scene (dome + 100 cylinders + 100 boxes, 256 materials,
+16 quad lights), three KB4 cameras 1920x1080 at 64 spp, paths of depth 5 with NEE, then the fused sensor
+epilogue (RGGB, Poisson shot + 2 DN Gaussian read noise, 12 bit) per camera. One step = one frame: every
+camera traced in one batched persistent launch, then the three sensor launches. Inputs are resident in HBM
+before the timed region; the timed region is exactly `--steps` frames between two device synchronisations.
+
+Prints one JSON line. `value` is Mrays/s (camera + extension + shadow rays counted on the device).
+`roofline` prices the trace kernel with the algorithmic bytes of the reference-order BVH2 traversal
+(B_ray = 32 ray + 16 hit + 64 per node visit + 48 per triangle test, visit counts from the CPU reference
+traversal of the same seeded paths on the cpu_baseline sample) against 8 TB/s.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "veridien-twin_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s and spp·Mpix/s per frame on 1 MI355X; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
+    """The oracle (C restatement of gpu/bvh.js traversal + the shading pieces, FP64) on a bounded sample of the
+    same workload: 32x32 tiles of camera 0 around the image centre at the camera's spp, until ~budget_s."""
+    import oracle
+
+    t0 = time.time()
+    ob = oracle.build_bvh(sc["pos"], sc["idx"])
+    import numpy as np
+    assert np.array_equal(ob["nodes"].view(np.uint32), sc["nodes"].view(np.uint32))
+    log("oracle BVH built in %.1f s (identical to the uploaded one)" % (time.time() - t0))
+    w, h = cam["width"], cam["height"]
+    tiles = []
+    cx, cy = (w // 2) // 32 * 32, (h // 2) // 32 * 32
+    for r in range(0, 20):
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                if max(abs(dx), abs(dy)) != r:
+                    continue
+                x0, y0 = cx + 32 * dx, cy + 32 * dy
+                if 0 <= x0 <= w - 32 and 0 <= y0 <= h - 32:
+                    tiles.append((x0, y0))
+    rays = 0
+    vis = np.zeros(2, np.uint64)
+    px = 0
+    t1 = time.time()
+    for (x0, y0) in tiles:
+        _, r, v = oracle.render_tile(sc, cam, x0, y0, 32, 32, 0, cam["spp"], nthreads=nthreads)
+        rays += r
+        vis += v
+        px += 32 * 32
+        if time.time() - t1 > budget_s:
+            break
+    dt = time.time() - t1
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=nthreads, kind="port",
+                sample="%d pixels (%d 32x32 tiles at the centre of camera 0) x %d spp, depth %d: %d rays in %.1f s on %s"
+                % (px, px // 1024, cam["spp"], cam["max_depth"], rays, dt, platform.processor() or platform.machine()),
+                rays=rays, node_visits_per_ray=float(vis[0]) / max(rays, 1), tri_tests_per_ray=float(vis[1]) / max(rays, 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["batched", "per-camera"], default="batched")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box); run --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from vtwin import lib, scenes
+
+    t0 = time.time()
+    sc, cams = scenes.c2()
+    log("scene: %d triangles, %d BVH2 nodes, built in %.1f s" % (sc["tris"].size // 16, sc["nodes"].size // 16, time.time() - t0))
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    for k, c in enumerate(cams):
+        ctx.set_camera(k, c)
+    slots = list(range(len(cams)))
+    sensor = [scenes.default_sensor(cfa=0, frame=0, seed=1006) for _ in cams]
+    mode = lib.VT_RENDER_BATCHED if args.mode == "batched" else lib.VT_RENDER_PER_CAMERA
+
+    def frame():
+        ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk)
+        for k in slots:
+            ctx.sensor(k, sensor[k])
+
+    for i in range(args.warmup):
+        frame()
+        ctx.sync()
+        log("warmup %d done" % i)
+    ms_kernel, rays_step, launches = [], [], 0
+    ctx.sync()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        frame()
+        st = ctx.stats()           # syncs; its event times are this frame's
+        ms_kernel.append(st["ms_kernel_avg"])
+        rays_step.append(st["rays"])
+        launches = st["launches"]
+        log("step %d: %.1f ms trace, %.3g rays" % (i, st["ms_render"], st["rays"]))
+    ctx.sync()
+    t2 = time.perf_counter()
+    assert st["stack_overflows"] == 0
+    dt = t2 - t1
+    total_rays = float(np.sum(rays_step))
+    spp_px = sum(c["spp"] * c["width"] * c["height"] for c in cams) * args.steps
+    value = total_rays / dt / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded mulberry32 scene)",
+        "config": {"workload": "full-scale forward: 2,013,432-tri seeded dome scene, 256 materials, 16 lights; "
+                               "3 KB4 cameras 1920x1080, 64 spp, depth 5, NEE; RGGB sensor, shot + 2 DN read noise, 12 bit",
+                   "cameras": len(cams), "spp": 64, "depth": 5, "mode": args.mode},
+        "spp_mpix_per_s": round(spp_px / dt / 1e6, 2),
+        "rays_per_frame": total_rays / args.steps,
+        "trace_ms_per_launch": round(float(np.mean(ms_kernel)), 3),
+        "trace_launches_per_frame": launches,
+    }
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(sc, cams[0], budget_s=args.cpu_budget)
+        out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        vn, vt = cpu["node_visits_per_ray"], cpu["tri_tests_per_ray"]
+        b_ray = 32 + 16 + 64 * vn + 48 * vt
+        rays_launch = float(np.mean(rays_step)) / max(launches, 1)
+        achieved = b_ray * rays_launch / (float(np.mean(ms_kernel)) * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_render<256>", "bytes_per_ray": round(b_ray, 1),
+                           "node_visits_per_ray": round(vn, 2), "tri_tests_per_ray": round(vt, 2)}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,218 @@
+/* vt.h — C ABI of the MI355X camera-rendering library (libvt.so).
+ *
+ * Drop-in boundary for the reference's `TwinRenderer` (gpu/renderer.js:200-1279): scene upload, camera set,
+ * render, sensor, readback. Plain pointers and sizes only. Every function returns an int status (VT_OK = 0,
+ * negative on error); vt_last_error() gives the context's message. The caller owns all host memory; inputs
+ * are copied before a call returns; outputs go to caller buffers whose exact size is checked (VT_ERR_SIZE).
+ * The context owns all device memory. A HIP error latches the context into a lost state (every later call
+ * returns VT_ERR_LOST), as `device.lost` does for the reference (gpu/renderer.js:241). One host thread uses a
+ * context at a time.
+ */
+#ifndef VT_H
+#define VT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VT_OK = 0,
+  VT_ERR_ARG = -1,     /* bad argument (null pointer, out-of-range index, malformed scene) */
+  VT_ERR_SIZE = -2,    /* caller buffer size does not match the documented size */
+  VT_ERR_HIP = -3,     /* a HIP call failed; the context is now lost */
+  VT_ERR_LOST = -4,    /* the context was lost earlier */
+  VT_ERR_STATE = -5,   /* call out of order (no scene, camera not set, nothing rendered) */
+  VT_ERR_NOMEM = -6,
+};
+
+/* Triangle-record words (gpu/scene_gpu.js:351-373) and mesh flags (gpu/wgsl/common.wgsl.js:36-42). */
+#define VT_NODE_WORDS 16
+#define VT_TRI_WORDS 16
+#define VT_VERT_WORDS 8
+#define VT_MAT_WORDS 16
+#define VT_EMIT_WORDS 16
+#define VT_MESH_STATIC 1u
+#define VT_MESH_EMITTER 2u
+#define VT_MAX_CAMERAS 64
+
+typedef struct vt_ctx vt_ctx;
+
+typedef struct {
+  int device;            /* HIP device ordinal */
+  uint32_t flags;        /* reserved, 0 */
+} vt_create_info;
+
+/* The prepared scene in the reference's upload layout (gpu/renderer.js:271-331, gpu/scene_gpu.js:461-469):
+ *   nodes      BVH2, 16 words per node (gpu/bvh.js:5-8); NULL: the library builds it from tris/verts
+ *   tris       16 words per BVH slot: [v0.xyz, i0][e1.xyz, i1][e2.xyz, i2][mesh, flags, -, -]
+ *   verts      8 words per vertex: [pos.xyz, -][normal.xyz, -]
+ *   meshes     4 words per mesh: material, flags (VT_MESH_*), light index (emitters), -
+ *   materials  16 floats per material: [c.rgb, alpha][ior, kind, fbar, roughness][tau.rgb, alpha_c]
+ *              [fbar_computed, -, -, -]; kind 0 dielectric, 1 conductor, 2 coated conductor, 3 Lambert
+ *              (gpu/bsdf.js:68-88 prepareMaterial)
+ *   lights     4 floats per light: radiance rgb, -
+ *   emit       16 floats per emitter triangle: [v0.xyz, cdf][e1.xyz, area][e2.xyz, light (u32 bits)][n.xyz, -]
+ *              (gpu/renderer.js:303-306, with the light index in place of the shared L0)
+ * Triangles referenced by leaf ranges must be stored in BVH-slot order (the order gpu/bvh.js returns). */
+typedef struct {
+  const float *nodes; uint32_t node_count;
+  const float *tris; uint32_t tri_count;
+  const float *verts; uint32_t vert_count;
+  const uint32_t *meshes; uint32_t mesh_count;
+  const float *materials; uint32_t material_count;
+  const float *lights; uint32_t light_count;
+  const float *emit; uint32_t emit_count;
+  float emit_area;       /* total emitter area (m^2), SceneU.L0.w (gpu/renderer.js:646) */
+} vt_scene;
+
+/* One tracking camera, CamU (gpu/wgsl/common.wgsl.js:79-90; gpu/renderer.js:888-898). */
+typedef struct {
+  float R[9];            /* columns: camera x, y, z axes in the world frame (R_D_C) */
+  float pos[3];          /* camera centre in the world frame */
+  uint32_t mode;         /* 0 KB4, 1 pinhole */
+  float lens[9];         /* fx fy cx cy k1 k2 k3 k4 alpha (gpu/lens.js:8-16 lensParams) */
+  uint32_t width, height, spp, seed;
+  uint32_t cam_index;    /* camera index in the pixel-seed hash (gpu/wgsl/kernels.wgsl.js:35) */
+  uint32_t light_samples;/* emitter samples per path vertex, 1..16 */
+  uint32_t max_depth;    /* path vertices, >= 1 */
+  uint32_t flags;        /* reserved, 0 */
+  float scale;           /* output scale (CamU.expo.z), usually 1 */
+} vt_camera;
+
+enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1 };
+typedef struct {
+  uint32_t mode;         /* VT_RENDER_BATCHED: every listed camera in one persistent launch over a global ray
+                            queue; VT_RENDER_PER_CAMERA: one plain launch per camera */
+  uint32_t spp_begin;    /* sub-sample range [spp_begin, spp_end); spp_end 0 = each camera's spp */
+  uint32_t spp_end;
+  uint32_t accumulate;   /* 0: clear the cameras' sums first; 1: add to them (progressive ranges) */
+  uint32_t chunk;        /* queue chunk in 64-item waves (0 = tuned default) */
+  uint32_t block;        /* threads per block, 64..1024, multiple of 64 (0 = tuned default) */
+} vt_render_opts;
+
+/* Sensor parameters, the 256-B sensor uniform (gpu/sensor.js:158-177), raw-DN part. */
+typedef struct {
+  uint32_t cfa;          /* 0 RGGB, 1 GRBG, 2 GBRG, 3 BGGR */
+  uint32_t frame, seed;
+  float optics_k;        /* pi T / (4 N^2) */
+  float esf_sigma;       /* optics blur sigma (px), after the pixel-aperture part */
+  float exposure, gain;
+  float m[9];            /* response matrix rows (e-/s per W m^-2) */
+  float read_noise, full_well, dark, conv_gain;
+  float max_dn, black;
+  uint32_t noise_on;
+  float pupil_exp;       /* relative-illumination cos exponent (gpu/sensor.js:119) */
+} vt_sensor_params;
+
+enum { VT_KIND_LINEAR = 0, VT_KIND_RAW = 1, VT_KIND_RI = 2 };
+
+typedef struct {
+  double rays;           /* rays traced by the last vt_render (camera, extension and shadow) */
+  double samples;        /* path samples of the last vt_render */
+  double ms_render;      /* GPU time of the last vt_render (all its launches) */
+  double ms_sensor;      /* GPU time of the last vt_sensor */
+  double ms_kernel_avg;  /* average duration of the last vt_render's trace launches */
+  uint32_t launches;     /* trace launches of the last vt_render */
+  uint32_t stack_overflows; /* traversal stack overflows since upload (must be 0) */
+} vt_frame_stats;
+
+/* ---------------------------------------------------------------- lifetime (TwinRenderer.create / destroy,
+ * gpu/renderer.js:202-261, 1278) */
+int vt_create(const vt_create_info *info, vt_ctx **out);
+void vt_destroy(vt_ctx *ctx);
+const char *vt_last_error(const vt_ctx *ctx);
+
+/* Binned-SAH BVH2 of gpu/bvh.js:18-161 (16 bins, leaf 4, same node numbering and triangle order), native.
+ * cap_nodes >= ntri + 2 always suffices. order_out gets ntri entries. Replaces the JS buildBVH call in
+ * prepareScene (gpu/scene_gpu.js:305-307). No context needed. */
+int vt_build_bvh(const float *pos, uint32_t nvert, const uint32_t *idx, uint32_t ntri, int leaf_size, int bins,
+                 float *nodes_out, uint32_t cap_nodes, uint32_t *node_count, uint32_t *order_out);
+
+/* ---------------------------------------------------------------- scene and cameras */
+/* Buffers of _buildBuffers (gpu/renderer.js:271-331): copies the scene in, re-lays the BVH2 out as the
+ * device's 128-B four-wide nodes, and uploads. Replaces any earlier scene. */
+int vt_upload_scene(vt_ctx *ctx, const vt_scene *scene);
+/* CamU packing of _packCam (gpu/renderer.js:888-898) for camera slot `cam` (< VT_MAX_CAMERAS). */
+int vt_set_camera(vt_ctx *ctx, uint32_t cam, const vt_camera *camera);
+
+/* ---------------------------------------------------------------- per frame */
+/* renderCamera / renderFrame (gpu/renderer.js:974-996): enqueue the listed cameras' path tracing. */
+int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts);
+/* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */
+int vt_sensor(vt_ctx *ctx, uint32_t cam, const vt_sensor_params *params);
+/* readback(name, kind) (gpu/renderer.js:1191-1231). Blocks until the data is ready.
+ *   VT_KIND_LINEAR  float32 rgba, width*height*16 bytes (mean radiance times scale; a = 1)
+ *   VT_KIND_RAW     uint16 DN, width*height*2 bytes (the reference returns these integers as float32)
+ *   VT_KIND_RI      float32 relative illumination, width*height*4 bytes */
+int vt_readback(vt_ctx *ctx, uint32_t cam, int kind, void *dst, size_t dst_bytes);
+int vt_sync(vt_ctx *ctx);
+/* timings() (gpu/renderer.js:1271-1274): statistics of the last render / sensor call. */
+int vt_frame_stats_get(vt_ctx *ctx, vt_frame_stats *out);
+
+/* ---------------------------------------------------------------- test kernels (the reference's TESTS entry
+ * points, gpu/wgsl/kernels.wgsl.js:487-551, that tests/gpu_test.html compared against the JS) */
+/* traceTest (:534-543): rays 8 floats each (o.xyz, tmin, d.xyz, tmax) -> hits 4 words each
+ * (t as float, tri as uint32 (0xffffffff = miss), u, v). */
+int vt_trace_test(vt_ctx *ctx, const float *rays, uint32_t n, int any_hit, float *hits);
+/* lensTest (:494-505): uv[2n] -> unit camera rays rays[3n] by the FP32 KB4 inverse (8 Newton steps). */
+int vt_lens_test(vt_ctx *ctx, const float lens[9], const float *uv, uint32_t n, float *rays);
+/* camera rays of sub-samples: for each (x, y, i) triple in xyi[3n] of camera `cam`, o.xyz d.xyz in out[6n] */
+int vt_camera_ray_test(vt_ctx *ctx, uint32_t cam, const uint32_t *xyi, uint32_t n, float *out);
+/* bsdfTest (:507-523): cases of 16 floats (n.xyz, -, l.xyz, -, v.xyz, -, u0 u1 u2, material) ->
+ * 8 floats each (f.rgb, pdf(v, l), sampled l.xyz, sampled pdf). */
+int vt_bsdf_test(vt_ctx *ctx, const float *cases, uint32_t n, float *out);
+/* The fused sensor epilogue on caller-given radiance (rgba float32 w*h*4) and RI map (w*h) -> DN (w*h). */
+int vt_sensor_test(vt_ctx *ctx, const vt_sensor_params *params, uint32_t width, uint32_t height,
+                   const float *radiance, const float *ri, uint16_t *dn);
+
+/* ---------------------------------------------------------------- node radiosity */
+/* One Jacobi step of the reference's sparse gather form (jacobiGather + jacobiUpdate,
+ * gpu/wgsl/kernels.wgsl.js:416-461; _submitJacobi gpu/renderer.js:729-749) on host-given tables. node_state
+ * (16 floats per node) is read and written in place. */
+typedef struct {
+  uint32_t n_static, n_active, gather_k, nbr_k, n_mat, albedo_bins;
+  const uint32_t *active;       /* n_active node ids */
+  const uint32_t *gtable;       /* n_static * gather_k hit node ids, 0xffffffff = miss */
+  const float *normals;         /* n_static * 4: node normal xyz, - */
+  const uint32_t *node_mat;     /* n_static material ids */
+  const float *albedo;          /* n_mat * bins * 8: per bin [spec.rgb, -][diff.rgb, -] (gpu/renderer.js:294-297) */
+  const float *hh;              /* n_mat * 4: hemispherical albedo rgb, - */
+  const uint32_t *nbr;          /* n_active * nbr_k active indices */
+  const float *nbr_w;           /* n_active * nbr_k weights (0 ends a row) */
+} vt_gather_tables;
+int vt_jacobi_sparse(vt_ctx *ctx, const vt_gather_tables *t, float *node_state, uint32_t iters,
+                     float *aux_out);
+
+/* Dense node radiosity x = e + F x (the precision configuration): F n x n row-major, e n x r. */
+enum { VT_DENSE_FP32 = 0, VT_DENSE_BF16_REFINE = 1, VT_DENSE_FP64 = 2 };
+typedef struct {
+  uint32_t n, r;
+  uint32_t method;        /* VT_DENSE_* */
+  double tol;             /* stop when ||e + F x - x||_F <= tol ||e||_F (FP64 check) */
+  uint32_t max_sweeps;
+  uint32_t inner_sweeps;  /* BF16 sweeps per FP32 refinement step */
+} vt_dense_opts;
+typedef struct {
+  double residual;        /* final relative residual, FP64 */
+  uint32_t sweeps;        /* Jacobi sweeps done */
+  uint32_t refinements;
+  double ms;              /* GPU time of the solve */
+  double ms_per_sweep;
+} vt_dense_result;
+/* Upload F (float32, n*n) and e (float32, n*r) once; the library keeps F in FP32, BF16 and FP64 copies as the
+ * methods need. */
+int vt_dense_upload(vt_ctx *ctx, uint32_t n, uint32_t r, const float *F, const float *e);
+/* Device-side form-factor build for nodes on the uploaded scene: F_ij = V_ij cos_i cos_j A_j / (pi r^2),
+ * rows scaled to sum <= row_max; V by shadow rays through the scene's BVH. nodes: n * 8 floats (p.xyz, area,
+ * n.xyz, -). Emission e (n * r) is uploaded from the caller. */
+int vt_dense_formfactors(vt_ctx *ctx, uint32_t n, const float *nodes, float row_max, uint32_t r, const float *e);
+int vt_dense_solve(vt_ctx *ctx, const vt_dense_opts *opts, float *x_out, vt_dense_result *res);
+/* F read back (float32 n*n), for tests */
+int vt_dense_download_F(vt_ctx *ctx, float *F_out, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
+
+Bars (north_star): primary-hit triangle ids bit-exact except a stated fraction of grazing or edge rays;
+barycentrics within 1e-5; sensor integers bit-exact given identical radiance and noise draws; radiance within
+1 % relative RMSE.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lib.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c1(ctx):
+    sc, cams = scenes.c1()
+    ctx.upload_scene(sc)
+    ctx.set_camera(0, cams[0])
+    return sc, cams
+
+
+def _camera_rays(cam, step=8, samples=(0, 5, 11)):
+    xyi = []
+    for y in range(0, cam["height"], step):
+        for x in range(0, cam["width"], step):
+            for i in samples:
+                xyi.append((x, y, i))
+    return np.array(xyi, np.uint32)
+
+
+def _oracle_rays(cam, xyi):
+    rays = np.zeros((len(xyi), 8), np.float32)
+    for k, (x, y, i) in enumerate(xyi):
+        o, d = oracle.camera_ray(cam, int(x), int(y), int(i))
+        rays[k, 0:3], rays[k, 4:7] = o, d
+    rays[:, 7] = np.inf
+    return rays
+
+
+def _edge_or_grazing(sc, rays, k, tri_a, tri_b, ua, va, ub, vb):
+    """A disagreement is 'edge' when either hit lies within 1e-4 of its triangle's boundary (barycentric), and
+    'grazing' when the ray meets either triangle at |cos| < 1e-3."""
+    T = sc["tris"].reshape(-1, 16)
+    d = rays[k, 4:7].astype(np.float64)
+    for tri, u, v in ((tri_a, ua, va), (tri_b, ub, vb)):
+        if tri < 0:
+            continue
+        if min(1 - u - v, u, v) < 1e-4:
+            return True
+        n = np.cross(T[tri, 4:7].astype(np.float64), T[tri, 8:11].astype(np.float64))
+        if abs(np.dot(n, d)) / np.linalg.norm(n) < 1e-3:
+            return True
+    return False
+
+
+def test_primary_hits_match_oracle(ctx, c1):
+    sc, cams = c1
+    cam = cams[0]
+    xyi = _camera_rays(cam)
+    rays = _oracle_rays(cam, xyi)
+    want = oracle.trace(sc["nodes"], sc["tris"], rays)
+    got = ctx.trace(rays)
+    gt = got[:, 1].view(np.uint32).astype(np.int64)
+    gt[gt == 0xFFFFFFFF] = -1
+    wt = want[:, 1].astype(np.int64)
+    same = gt == wt
+    bad = np.nonzero(~same)[0]
+    unexplained = [k for k in bad if not _edge_or_grazing(sc, rays, k, wt[k], gt[k], want[k, 2], want[k, 3], got[k, 2], got[k, 3])]
+    print("primary rays %d, id mismatches %d (edge/grazing %d)" % (len(rays), len(bad), len(bad) - len(unexplained)))
+    assert not unexplained, unexplained[:10]
+    assert len(bad) <= 1e-3 * len(rays)
+    hit = same & (wt >= 0)
+    assert np.max(np.abs(got[hit, 2] - want[hit, 2])) < 1e-5
+    assert np.max(np.abs(got[hit, 3] - want[hit, 3])) < 1e-5
+    assert np.max(np.abs(got[hit, 0] - want[hit, 0]) / want[hit, 0]) < 1e-5
+
+
+def test_random_rays_closest_and_any_hit(ctx, c1):
+    sc, _ = c1
+    r = scenes.Rng(31337)
+    n = 20000
+    o = np.stack([r.uniform(-2, 2, n), r.uniform(-2, 2, n), r.uniform(0.05, 2.5, n)], 1)
+    d = np.stack([r.uniform(-1, 1, n), r.uniform(-1, 1, n), r.uniform(-1, 1, n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7] = o, d
+    rays[:, 7] = np.inf
+    want = oracle.trace(sc["nodes"], sc["tris"], rays)
+    got = ctx.trace(rays)
+    gt = got[:, 1].view(np.uint32).astype(np.int64)
+    gt[gt == 0xFFFFFFFF] = -1
+    wt = want[:, 1].astype(np.int64)
+    bad = np.nonzero(gt != wt)[0]
+    unexplained = [k for k in bad if not _edge_or_grazing(sc, rays, k, wt[k], gt[k], want[k, 2], want[k, 3], got[k, 2], got[k, 3])]
+    assert not unexplained, unexplained[:10]
+    assert len(bad) <= 1e-3 * n
+    # shadow queries: segments of random length
+    rays[:, 7] = r.uniform(0.1, 4.0, n).astype(np.float32)
+    wa = oracle.trace(sc["nodes"], sc["tris"], rays, any_hit=True)
+    ga = ctx.trace(rays, any_hit=True)
+    occ_w = wa[:, 1] >= 0
+    occ_g = ga[:, 1].view(np.uint32) != 0xFFFFFFFF
+    assert np.mean(occ_w != occ_g) <= 1e-3
+    assert ctx.stats()["stack_overflows"] == 0
+
+
+def test_kb4_unproject_matches_fp64(ctx, c1):
+    _, cams = c1
+    lens = cams[0]["lens"]
+    u, v = np.meshgrid(np.linspace(-0.5, 639.5, 97), np.linspace(-0.5, 479.5, 71))
+    uv = np.stack([u.ravel(), v.ravel()], 1).astype(np.float32)
+    got = ctx.lens_test(lens, uv)
+    L = oracle.lens(*[float(x) for x in lens[:4]], [float(x) for x in lens[4:8]], float(lens[8]))
+    want = np.zeros((uv.shape[0], 3))
+    for k in range(uv.shape[0]):
+        r = np.zeros(3)
+        oracle.lib().or_unproject(oracle.C.byref(L), oracle.ptr(uv[k].astype(np.float64)), oracle.ptr(r))
+        want[k] = r
+    ang = np.arccos(np.clip(np.sum(got * want, 1) / np.linalg.norm(got, axis=1), -1, 1))
+    assert ang.max() < 2e-6
+
+
+def test_camera_rays_match_oracle(ctx, c1):
+    _, cams = c1
+    cam = cams[0]
+    xyi = _camera_rays(cam, step=37, samples=(0, 3, 15))
+    got = ctx.camera_rays(0, xyi)
+    for k, (x, y, i) in enumerate(xyi):
+        o, d = oracle.camera_ray(cam, int(x), int(y), int(i))
+        assert np.allclose(got[k, 0:3], o)
+        assert np.linalg.norm(got[k, 3:6] - d) < 2e-6
+
+
+def _bsdf_cases(n, mats_count, seed):
+    r = scenes.Rng(seed)
+    cases = np.zeros((n, 16), np.float32)
+    for k in range(n):
+        nn = np.array([r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(0.2, 1)])
+        nn /= np.linalg.norm(nn)
+        l = np.array([r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(-1, 1)])
+        l /= np.linalg.norm(l)
+        if np.dot(l, nn) < 0:
+            l = -l
+        v = np.array([r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(-1, 1)])
+        v /= np.linalg.norm(v)
+        if np.dot(v, nn) < 0.05:
+            v = v - 1.2 * np.dot(v, nn) * nn + 0.1 * nn
+            v /= np.linalg.norm(v)
+        cases[k, 0:3], cases[k, 4:7], cases[k, 8:11] = nn, l, v
+        cases[k, 12:15] = r.uniform(0, 1, 3)
+        cases[k, 15] = k % mats_count
+    return cases
+
+
+def test_bsdf_matches_oracle(ctx):
+    mats = np.stack([scenes.material_record(scenes.KIND_LAMBERT, (0.6, 0.4, 0.3)),
+                     scenes.material_record(scenes.KIND_DIELECTRIC, (0.5, 0.6, 0.2), roughness=0.3),
+                     scenes.material_record(scenes.KIND_CONDUCTOR, (0.9, 0.7, 0.5), roughness=0.15),
+                     scenes.material_record(scenes.KIND_COATED, (0.8, 0.8, 0.9), roughness=0.4, tau=(0.9, 0.8, 0.7)),
+                     scenes.material_record(scenes.KIND_DIELECTRIC, (0.3, 0.3, 0.3), roughness=0.05)])
+    # a tiny scene carrying these materials
+    sc, _ = scenes.c1()
+    sc = dict(sc)
+    sc["materials"] = mats.reshape(-1)
+    sc["meshes"] = np.zeros(20 * 4, np.uint32)
+    sc["meshes"].reshape(-1, 4)[16:, 1] = scenes.MESH_EMITTER
+    sc["meshes"].reshape(-1, 4)[16:, 2] = np.arange(4)
+    ctx.upload_scene(sc)
+    cases = _bsdf_cases(4000, mats.shape[0], 99)
+    got = ctx.bsdf_test(cases)
+    worst_f = worst_p = 0.0
+    mism = 0
+    for k in range(cases.shape[0]):
+        M = oracle.mat_from_record(mats[int(cases[k, 15])])
+        nn, l, v = (cases[k, a:a + 3].astype(np.float64) for a in (0, 4, 8))
+        f = np.zeros(3)
+        oracle.lib().or_eval_bsdf(oracle.C.byref(M), oracle.ptr(nn), oracle.ptr(l), oracle.ptr(v), oracle.ptr(f))
+        p = oracle.lib().or_pdf_bsdf(oracle.C.byref(M), oracle.ptr(nn), oracle.ptr(v), oracle.ptr(l))
+        worst_f = max(worst_f, float(np.max(np.abs(got[k, 0:3] - f) / np.maximum(np.abs(f), 1e-3))))
+        worst_p = max(worst_p, abs(got[k, 3] - p) / max(abs(p), 1e-3))
+        u = cases[k, 12:15].astype(np.float64)
+        ls = np.zeros(3)
+        ps = oracle.lib().or_sample_bsdf(oracle.C.byref(M), oracle.ptr(nn), oracle.ptr(v), oracle.ptr(u), oracle.ptr(ls))
+        if (ps > 0) != (got[k, 7] > 0):
+            mism += 1
+        elif ps > 0:
+            assert np.linalg.norm(got[k, 4:7] - ls) < 2e-4, k
+    assert worst_f < 2e-4 and worst_p < 2e-4
+    assert mism <= 2          # samples exactly on the horizon may fall either side
+    sc2, cams = scenes.c1()
+    ctx.upload_scene(sc2)
+
+
+def _sensor_case(w, h, seed, level):
+    r = scenes.Rng(seed)
+    rad = np.zeros((h, w, 4), np.float32)
+    base = r.uniform(0.0, level, w * h * 3).reshape(h, w, 3)
+    rad[:, :, 0:3] = base
+    rad[:, :, 3] = 1.0
+    ri = r.uniform(0.3, 1.0, w * h).reshape(h, w).astype(np.float32)
+    return rad, ri
+
+
+@pytest.mark.parametrize("cfa,noise,level,exposure", [(0, True, 3.0, 2e-3), (0, True, 3.0, 2e-5), (1, True, 30.0, 1e-3),
+                                                      (3, False, 3.0, 2e-3), (2, True, 0.5, 4e-6)])
+def test_sensor_dn_bit_exact(ctx, cfa, noise, level, exposure):
+    rad, ri = _sensor_case(203, 157, 5 + cfa, level)
+    p = scenes.default_sensor(cfa=cfa, exposure=exposure, noise=noise, frame=7, seed=1006)
+    got = ctx.sensor_test(p, rad, ri)
+    want = oracle.sensor_expose(p, rad, ri)
+    assert got.shape == want.shape
+    diff = np.count_nonzero(got != want)
+    assert diff == 0, "%d of %d DNs differ; max |d| %d" % (diff, got.size, np.max(np.abs(got.astype(int) - want)))
+    assert got.max() > 0
+
+
+def test_render_matches_oracle_same_samples(ctx, c1):
+    sc, cams = c1
+    cam = cams[0]
+    ctx.render([0])
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    st = ctx.stats()
+    assert st["stack_overflows"] == 0
+    assert st["samples"] == 640 * 480 * 16
+    x0, y0, w, h = 256, 160, 64, 48
+    ref, rays, _ = oracle.render_tile(sc, cam, x0, y0, w, h)
+    a = img[y0:y0 + h, x0:x0 + w, 0:3].astype(np.float64)
+    b = ref[:, :, 0:3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("same-sample relative RMSE %.3g, oracle rays %d" % (rel, rays))
+    assert rel < 1e-3
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    assert close > 0.99
+
+
+def test_batched_and_per_camera_agree(ctx, c1):
+    _, cams = c1
+    cam2 = dict(cams[0])
+    cam2["width"], cam2["height"], cam2["spp"], cam2["cam_index"], cam2["seed"] = 320, 240, 4, 1, 1006
+    cam2["lens"] = np.array([150, 150, 159.5, 119.5] + list(cams[0]["lens"][4:]), np.float32)
+    ctx.set_camera(1, cam2)
+    ctx.render([0, 1], mode=lib.VT_RENDER_BATCHED)
+    a0 = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    a1 = ctx.readback(1, lib.VT_KIND_LINEAR, 320, 240)
+    ctx.render([0, 1], mode=lib.VT_RENDER_PER_CAMERA)
+    b0 = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    b1 = ctx.readback(1, lib.VT_KIND_LINEAR, 320, 240)
+    assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+
+
+def test_render_converges_to_oracle(ctx, c1):
+    """Independent sample sets: GPU 1024 spp vs oracle 1024 spp on a tile, compared on 4x4-pixel blocks."""
+    sc, cams = c1
+    cam = dict(cams[0])
+    ctx.set_camera(0, cam)
+    for s in range(0, 1024, 64):
+        ctx.render([0], spp_begin=s, spp_end=s + 64, accumulate=s > 0)
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    x0, y0, w, h = 288, 216, 32, 24
+    ocam = dict(cam)
+    ocam["seed"] = 777
+    ref, _, _ = oracle.render_tile(sc, ocam, x0, y0, w, h, 0, 1024)
+    a = img[y0:y0 + h, x0:x0 + w, 0:3].reshape(h // 4, 4, w // 4, 4, 3).mean(axis=(1, 3))
+    b = ref[:, :, 0:3].astype(np.float64).reshape(h // 4, 4, w // 4, 4, 3).mean(axis=(1, 3))
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("independent 1024 spp block relative RMSE %.4f" % rel)
+    assert rel < 0.01

@@ -1,0 +1,951 @@
+// The C ABI (include/vt.h): context, scene upload and re-layout, cameras, render/sensor/readback, the
+// reference's test entry points and node radiosity. Device memory is owned by the context; one HIP stream
+// (default priority) orders every call's work.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vt.h"
+#include "vt_internal.h"
+
+using namespace vt;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct CamSlot {
+  bool set = false;
+  vt_camera c{};
+  DevBuf acc, dn, ri, lin;
+  bool ri_valid = false;
+  float ri_pupil = -1.0f;
+  uint32_t spp_total = 0;     // samples accumulated in acc (for the mean)
+  bool rendered = false, sensed = false;
+};
+
+}  // namespace
+
+struct vt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool lost = false;
+  std::string err;
+  int cu_count = 0;
+  // scene
+  bool has_scene = false;
+  DevBuf nodes, tri, attr, vnorm, meshes, mats, lights, emit;
+  SceneDev S{};
+  uint32_t tri_count = 0, node4_count = 0;
+  // launch support
+  DevBuf counters, overflow, spill, dcams, test_in, test_out, test_aux;
+  uint64_t spill_lanes = 0;
+  DevCam *h_cams = nullptr;     // pinned staging for the camera table
+  hipEvent_t ev_stage = nullptr;
+  bool stage_pending = false;
+  hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
+  std::vector<hipEvent_t> ev_launch;
+  uint32_t last_launches = 0;
+  bool render_timed = false, sensor_timed = false;
+  CamSlot cams[VT_MAX_CAMERAS];
+  // radiosity
+  DevBuf g_active, g_table, g_normals, g_mat, g_alb, g_hh, g_nbr, g_nbrw, g_state, g_aux;
+  // dense radiosity
+  DevBuf dF32, dF16, dF64, dE, dX, dY, dXd, dR, dPart;
+  uint32_t dn_n = 0, dn_r = 0, dn_rp = 0;
+};
+
+namespace {
+
+int fail(vt_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_fail(vt_ctx *c, hipError_t e, const char *what) {
+  c->lost = true;
+  return fail(c, VT_ERR_HIP, "%s: %s (%d); the context is lost", what, hipGetErrorString(e), (int)e);
+}
+
+#define VT_HIP(c, call)                                 \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+  } while (0)
+
+#define VT_LIVE(c)                                                                       \
+  do {                                                                                   \
+    if (!(c)) return VT_ERR_ARG;                                                         \
+    if ((c)->lost) return fail((c), VT_ERR_LOST, "context lost: %s", (c)->err.c_str()); \
+  } while (0)
+
+int ensure(vt_ctx *c, DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return VT_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+  }
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      return fail(c, VT_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
+    }
+    return hip_fail(c, e, "hipMalloc");
+  }
+  b.bytes = bytes;
+  return VT_OK;
+}
+void release(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int upload(vt_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+  int rc = ensure(c, b, bytes);
+  if (rc) return rc;
+  if (bytes) VT_HIP(c, hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+  return VT_OK;
+}
+
+// grid for the trace kernels: every CU filled to its occupancy, never more lanes than spill slots
+uint32_t max_blocks(vt_ctx *c, uint32_t block) { return (uint32_t)(c->spill_lanes / block); }
+
+// relative illumination per pixel (gpu/sensor.js:103-124), FP64, default vignetting (1 at every angle)
+void ri_map_host(const vt_camera &cam, double q, std::vector<float> &out) {
+  const double fx = cam.lens[0], fy = cam.lens[1], cx = cam.lens[2], cy = cam.lens[3], al = cam.lens[8];
+  const double k[4] = {cam.lens[4], cam.lens[5], cam.lens[6], cam.lens[7]};
+  auto dist = [&](double th) {
+    double t2 = th * th;
+    return th * (1 + t2 * (k[0] + t2 * (k[1] + t2 * (k[2] + t2 * k[3]))));
+  };
+  auto deriv = [&](double th) {
+    double t2 = th * th;
+    return 1 + t2 * (3 * k[0] + t2 * (5 * k[1] + t2 * (7 * k[2] + t2 * 9 * k[3])));
+  };
+  const uint32_t w = cam.width, h = cam.height;
+  out.resize((size_t)w * h);
+  for (uint32_t y = 0; y < h; y++)
+    for (uint32_t x = 0; x < w; x++) {
+      double ray_z;
+      if (cam.mode == 0) {
+        double yd = (y - cy) / fy, xd = (x - cx) / fx - al * yd;
+        double thd = std::hypot(xd, yd);
+        if (thd < 1e-300) {
+          ray_z = 1;
+        } else {
+          double th = thd;
+          for (int i = 0; i < 20; i++) {
+            double step = (dist(th) - thd) / deriv(th);
+            th -= step;
+            if (std::fabs(step) < 1e-15) break;
+          }
+          ray_z = std::cos(th);
+        }
+      } else {
+        double dx = (x - cx) / fx, dy = (y - cy) / fy;
+        ray_z = 1 / std::sqrt(dx * dx + dy * dy + 1);
+      }
+      double th = std::acos(std::max(-1.0, std::min(1.0, ray_z)));
+      double ri = 1;
+      if (th > 1e-6) {
+        if (cam.mode == 0) ri = std::pow(std::cos(th), q) * std::sin(th) / (dist(th) * deriv(th));
+        else ri = std::pow(std::cos(th), q + 3);   // pinhole mapping: cos^3 * cos^q
+      }
+      out[(size_t)y * w + x] = (float)std::max(0.0, ri);
+    }
+}
+
+}  // namespace
+
+// ====================================================================== lifetime
+extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
+  if (!out) return VT_ERR_ARG;
+  *out = nullptr;
+  vt_ctx *c = new vt_ctx();
+  c->device = info ? info->device : 0;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= c->device || c->device < 0) {
+    delete c;
+    return VT_ERR_HIP;
+  }
+  e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, c->device);
+  if (e != hipSuccess) {
+    delete c;
+    return VT_ERR_HIP;
+  }
+  c->cu_count = prop.multiProcessorCount;
+  int occ = 0;
+  // persistent grids: up to 8 resident 256-thread blocks per CU (LDS and VGPRs usually allow fewer)
+  c->spill_lanes = (uint64_t)c->cu_count * 8 * 256;
+  (void)occ;
+  int rc = ensure(c, c->counters, 64);
+  if (!rc) rc = ensure(c, c->overflow, 64);
+  if (!rc) rc = ensure(c, c->spill, c->spill_lanes * STACK_SPILL * sizeof(uint2));
+  if (!rc) rc = ensure(c, c->dcams, sizeof(DevCam) * VT_MAX_CAMERAS);
+  if (!rc && hipHostMalloc((void **)&c->h_cams, sizeof(DevCam) * VT_MAX_CAMERAS, hipHostMallocDefault) != hipSuccess) rc = VT_ERR_HIP;
+  hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1};
+  for (hipEvent_t *ev : evs)
+    if (!rc && hipEventCreate(ev) != hipSuccess) rc = VT_ERR_HIP;
+  if (!rc && hipMemsetAsync(c->overflow.p, 0, 64, c->stream) != hipSuccess) rc = VT_ERR_HIP;
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VT_ERR_HIP;
+  if (rc) {
+    vt_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return VT_OK;
+}
+
+extern "C" void vt_destroy(vt_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->attr, &c->vnorm, &c->meshes, &c->mats, &c->lights, &c->emit,
+                    &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
+                    &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
+                    &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
+                    &c->dXd, &c->dR, &c->dPart};
+  for (DevBuf *b : bufs) release(*b);
+  for (auto &cs : c->cams) {
+    release(cs.acc);
+    release(cs.dn);
+    release(cs.ri);
+    release(cs.lin);
+  }
+  for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
+  hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1};
+  for (hipEvent_t ev : evs)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->h_cams) (void)hipHostFree(c->h_cams);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char *vt_last_error(const vt_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+// ====================================================================== BVH
+extern "C" int vt_build_bvh(const float *pos, uint32_t nvert, const uint32_t *idx, uint32_t ntri, int leaf_size,
+                            int bins, float *nodes_out, uint32_t cap_nodes, uint32_t *node_count,
+                            uint32_t *order_out) {
+  if ((!pos && nvert) || (!idx && ntri) || !nodes_out || !node_count || (!order_out && ntri)) return VT_ERR_ARG;
+  if (leaf_size < 1 || bins < 2 || bins > 256) return VT_ERR_ARG;
+  for (uint64_t k = 0; k < 3ull * ntri; k++)
+    if (idx[k] >= nvert) return VT_ERR_ARG;
+  int64_t n = build_bvh2(pos, idx, ntri, leaf_size, bins, nodes_out, cap_nodes, order_out);
+  if (n < 0) return VT_ERR_SIZE;
+  *node_count = (uint32_t)n;
+  return VT_OK;
+}
+
+// ====================================================================== scene
+extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
+  VT_LIVE(c);
+  if (!s || !s->tris || !s->verts || !s->meshes || !s->materials) return fail(c, VT_ERR_ARG, "scene: null array");
+  if (s->tri_count == 0 || s->vert_count == 0 || s->mesh_count == 0 || s->material_count == 0)
+    return fail(c, VT_ERR_ARG, "scene: empty triangle, vertex, mesh or material table");
+  if (s->emit_count && (!s->emit || !s->lights || s->light_count == 0))
+    return fail(c, VT_ERR_ARG, "scene: emitter triangles without lights");
+  if (s->tri_count >= (1u << 26)) return fail(c, VT_ERR_ARG, "scene: more than 2^26 triangles");
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  const uint32_t T = s->tri_count, V = s->vert_count;
+  const uint32_t *tw = reinterpret_cast<const uint32_t *>(s->tris);
+  // validate the triangle records
+  for (uint32_t k = 0; k < T; k++) {
+    const uint32_t *r = tw + (size_t)k * 16;
+    if (r[3] >= V || r[7] >= V || r[11] >= V) return fail(c, VT_ERR_ARG, "scene: triangle %u vertex id out of range", k);
+    if (r[12] >= s->mesh_count) return fail(c, VT_ERR_ARG, "scene: triangle %u mesh id out of range", k);
+  }
+  for (uint32_t m = 0; m < s->mesh_count; m++) {
+    if (s->meshes[4 * m] >= s->material_count) return fail(c, VT_ERR_ARG, "scene: mesh %u material out of range", m);
+    if ((s->meshes[4 * m + 1] & VT_MESH_EMITTER) && s->meshes[4 * m + 2] >= std::max(1u, s->light_count))
+      return fail(c, VT_ERR_ARG, "scene: mesh %u light out of range", m);
+  }
+  for (uint32_t e = 0; e < s->emit_count; e++) {
+    uint32_t lid;
+    std::memcpy(&lid, s->emit + 16 * (size_t)e + 11, 4);
+    if (lid >= s->light_count) return fail(c, VT_ERR_ARG, "scene: emitter %u light out of range", e);
+  }
+  // BVH: the caller's (reference layout) or built here; slot order is the tris order
+  std::vector<float> nodes2;
+  std::vector<float> tris_sorted;
+  const float *tr = s->tris;
+  uint32_t n2 = s->node_count;
+  const float *np = s->nodes;
+  if (!np) {
+    std::vector<float> pos((size_t)V * 3);
+    for (uint32_t v = 0; v < V; v++)
+      for (int k = 0; k < 3; k++) pos[3 * v + k] = s->verts[8 * (size_t)v + k];
+    std::vector<uint32_t> idx((size_t)T * 3), order(T);
+    for (uint32_t k = 0; k < T; k++)
+      for (int j = 0; j < 3; j++) idx[3 * k + j] = tw[16 * (size_t)k + 4 * j + 3];
+    nodes2.resize((size_t)(T + 2) * 16);
+    int64_t n = build_bvh2(pos.data(), idx.data(), T, 4, 16, nodes2.data(), T + 2, order.data());
+    if (n < 0) return fail(c, VT_ERR_ARG, "scene: BVH build failed");
+    n2 = (uint32_t)n;
+    np = nodes2.data();
+    tris_sorted.resize((size_t)T * 16);
+    for (uint32_t k = 0; k < T; k++) std::memcpy(&tris_sorted[16 * (size_t)k], s->tris + 16 * (size_t)order[k], 64);
+    tr = tris_sorted.data();
+  } else if (n2 == 0) {
+    return fail(c, VT_ERR_ARG, "scene: node_count is 0");
+  }
+  std::vector<Node4> n4;
+  try {
+    n4 = collapse_bvh4(np, n2, T);
+  } catch (const std::exception &ex) {
+    return fail(c, VT_ERR_ARG, "scene: %s", ex.what());
+  }
+  const uint32_t *trw = reinterpret_cast<const uint32_t *>(tr);
+  std::vector<float4> tri((size_t)T * 3), vn(V);
+  std::vector<uint4> attr(T);
+  for (uint32_t k = 0; k < T; k++) {
+    const uint32_t *r = trw + (size_t)k * 16;
+    for (int j = 0; j < 3; j++) {
+      const float *p = s->verts + 8 * (size_t)r[4 * j + 3];
+      tri[3 * (size_t)k + j] = make_float4(p[0], p[1], p[2], 0.0f);
+    }
+    attr[k] = make_uint4(r[3], r[7], r[11], r[12]);
+  }
+  for (uint32_t v = 0; v < V; v++) {
+    const float *p = s->verts + 8 * (size_t)v;
+    vn[v] = make_float4(p[4], p[5], p[6], 0.0f);
+  }
+  int rc;
+  if ((rc = upload(c, c->nodes, n4.data(), n4.size() * sizeof(Node4)))) return rc;
+  if ((rc = upload(c, c->tri, tri.data(), tri.size() * sizeof(float4)))) return rc;
+  if ((rc = upload(c, c->attr, attr.data(), attr.size() * sizeof(uint4)))) return rc;
+  if ((rc = upload(c, c->vnorm, vn.data(), vn.size() * sizeof(float4)))) return rc;
+  if ((rc = upload(c, c->meshes, s->meshes, (size_t)s->mesh_count * 16))) return rc;
+  if ((rc = upload(c, c->mats, s->materials, (size_t)s->material_count * 64))) return rc;
+  float zero4[4] = {0, 0, 0, 0};
+  if ((rc = upload(c, c->lights, s->light_count ? (const void *)s->lights : zero4, std::max<size_t>(16, (size_t)s->light_count * 16)))) return rc;
+  float zero16[16] = {0};
+  if ((rc = upload(c, c->emit, s->emit_count ? (const void *)s->emit : zero16, std::max<size_t>(64, (size_t)s->emit_count * 64)))) return rc;
+  c->S.nodes = (const Node4 *)c->nodes.p;
+  c->S.tri = (const float4 *)c->tri.p;
+  c->S.attr = (const uint4 *)c->attr.p;
+  c->S.vnorm = (const float4 *)c->vnorm.p;
+  c->S.meshes = (const uint4 *)c->meshes.p;
+  c->S.mats = (const float4 *)c->mats.p;
+  c->S.lights = (const float4 *)c->lights.p;
+  c->S.emit = (const float4 *)c->emit.p;
+  c->S.emit_count = s->emit_count;
+  c->S.root = 0;
+  c->S.emit_area = s->emit_area > 0 ? s->emit_area : 1e-12f;
+  c->tri_count = T;
+  c->node4_count = (uint32_t)n4.size();
+  c->has_scene = true;
+  VT_HIP(c, hipMemsetAsync(c->overflow.p, 0, 64, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_set_camera(vt_ctx *c, uint32_t cam, const vt_camera *k) {
+  VT_LIVE(c);
+  if (cam >= VT_MAX_CAMERAS || !k) return fail(c, VT_ERR_ARG, "camera slot %u out of range", cam);
+  if (k->width == 0 || k->height == 0 || k->width > 16384 || k->height > 16384)
+    return fail(c, VT_ERR_ARG, "camera %u: bad size %ux%u", cam, k->width, k->height);
+  if (k->spp == 0) return fail(c, VT_ERR_ARG, "camera %u: spp is 0", cam);
+  if (k->mode > 1) return fail(c, VT_ERR_ARG, "camera %u: mode %u is not kb4 (0) or pinhole (1)", cam, k->mode);
+  if (!(k->lens[0] > 0) || !(k->lens[1] > 0)) return fail(c, VT_ERR_ARG, "camera %u: focal lengths must be > 0", cam);
+  VT_HIP(c, hipSetDevice(c->device));
+  CamSlot &s = c->cams[cam];
+  const size_t px = (size_t)k->width * k->height;
+  const bool resized = !s.set || s.c.width != k->width || s.c.height != k->height;
+  const bool lens_changed = resized || std::memcmp(s.c.lens, k->lens, sizeof k->lens) != 0 || s.c.mode != k->mode;
+  int rc;
+  if ((rc = ensure(c, s.acc, px * sizeof(float4)))) return rc;
+  if ((rc = ensure(c, s.dn, px * sizeof(uint16_t)))) return rc;
+  if ((rc = ensure(c, s.ri, px * sizeof(float)))) return rc;
+  if ((rc = ensure(c, s.lin, px * sizeof(float4)))) return rc;
+  if (lens_changed) s.ri_valid = false;
+  s.c = *k;
+  s.set = true;
+  if (resized) {
+    s.rendered = false;
+    s.sensed = false;
+    s.spp_total = 0;
+  }
+  return VT_OK;
+}
+
+// ====================================================================== render
+extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "render: no scene uploaded");
+  if (!cams || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "render: 1..%d cameras", VT_MAX_CAMERAS);
+  vt_render_opts opt{};
+  if (o) opt = *o;
+  const uint32_t block = opt.block ? opt.block : DEFAULT_BLOCK;
+  if (block != 64 && block != 128 && block != 256) return fail(c, VT_ERR_ARG, "render: block must be 64, 128 or 256");
+  const uint32_t chunk = opt.chunk ? opt.chunk : 4;
+  for (uint32_t a = 0; a < n; a++) {
+    if (cams[a] >= VT_MAX_CAMERAS || !c->cams[cams[a]].set) return fail(c, VT_ERR_STATE, "render: camera %u not set", cams[a]);
+    for (uint32_t b = 0; b < a; b++)
+      if (cams[a] == cams[b]) return fail(c, VT_ERR_ARG, "render: camera %u listed twice", cams[a]);
+  }
+  VT_HIP(c, hipSetDevice(c->device));
+  // the staging table may still feed an earlier copy
+  if (c->stage_pending) VT_HIP(c, hipEventSynchronize(c->ev_stage));
+  uint64_t items = 0;
+  for (uint32_t a = 0; a < n; a++) {
+    CamSlot &s = c->cams[cams[a]];
+    const vt_camera &k = s.c;
+    DevCam &d = c->h_cams[a];
+    std::memcpy(d.R, k.R, sizeof d.R);
+    std::memcpy(d.pos, k.pos, sizeof d.pos);
+    d.mode = k.mode;
+    std::memcpy(d.lens, k.lens, sizeof d.lens);
+    d.width = k.width; d.height = k.height; d.spp = k.spp; d.seed = k.seed;
+    d.cam_index = k.cam_index;
+    d.nl = k.light_samples ? k.light_samples : 1;
+    d.depth = k.max_depth ? k.max_depth : 1;
+    d.tiles_x = (k.width + 7) / 8;
+    d.tiles_y = (k.height + 7) / 8;
+    const uint32_t s0 = opt.spp_begin, s1 = opt.spp_end ? opt.spp_end : k.spp;
+    if (s1 <= s0) return fail(c, VT_ERR_ARG, "render: empty sub-sample range [%u, %u)", s0, s1);
+    d.nspp = s1 - s0;
+    d.spp_begin = s0;
+    d.accumulate = opt.accumulate;
+    d.item_begin = items;
+    d.item_count = (uint64_t)d.tiles_x * d.tiles_y * 64ull * d.nspp;
+    d.acc = (float4 *)s.acc.p;
+    items += d.item_count;
+  }
+  VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_stage, c->stream));
+  c->stage_pending = true;
+  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 8, c->stream));
+  for (uint32_t a = 0; a < n; a++) {
+    CamSlot &s = c->cams[cams[a]];
+    if (!opt.accumulate) {
+      VT_HIP(c, launch_zero_f4((float4 *)s.acc.p, (uint64_t)s.c.width * s.c.height, c->stream));
+      s.spp_total = 0;
+    }
+  }
+  RenderLaunch L{};
+  L.S = c->S;
+  L.counters = (unsigned long long *)c->counters.p;
+  L.overflow = (uint32_t *)c->overflow.p;
+  L.spill = (uint2 *)c->spill.p;
+  L.spill_stride = c->spill_lanes;
+  L.chunk = chunk;
+  L.block = block;
+  const uint32_t gmax = max_blocks(c, block);
+  VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+  uint32_t launches = 0;
+  auto ensure_events = [&](uint32_t k) -> int {
+    while (c->ev_launch.size() < 2 * (size_t)k) {
+      hipEvent_t ev;
+      VT_HIP(c, hipEventCreate(&ev));
+      c->ev_launch.push_back(ev);
+    }
+    return VT_OK;
+  };
+  if (opt.mode == VT_RENDER_BATCHED) {
+    int rc = ensure_events(1);
+    if (rc) return rc;
+    L.cams = (const DevCam *)c->dcams.p;
+    L.ncam = n;
+    L.item_base = 0;
+    L.total_waves = items / 64;
+    L.persistent = 1;
+    uint64_t need = (L.total_waves * 64 + block - 1) / block;
+    L.grid = (uint32_t)std::min<uint64_t>(need, gmax);
+    VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+    VT_HIP(c, launch_render(L, c->stream));
+    VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+    launches = 1;
+  } else {
+    int rc = ensure_events(n);
+    if (rc) return rc;
+    for (uint32_t a = 0; a < n; a++) {
+      const DevCam &d = c->h_cams[a];
+      L.cams = (const DevCam *)c->dcams.p + a;
+      L.ncam = 1;
+      L.item_base = d.item_begin;
+      L.total_waves = d.item_count / 64;
+      L.persistent = 0;
+      uint64_t need = (L.total_waves * 64 + block - 1) / block;
+      L.grid = (uint32_t)std::min<uint64_t>(need, gmax);
+      VT_HIP(c, hipEventRecord(c->ev_launch[2 * a], c->stream));
+      VT_HIP(c, launch_render(L, c->stream));
+      VT_HIP(c, hipEventRecord(c->ev_launch[2 * a + 1], c->stream));
+      launches++;
+    }
+  }
+  VT_HIP(c, hipEventRecord(c->ev_r1, c->stream));
+  c->last_launches = launches;
+  c->render_timed = true;
+  for (uint32_t a = 0; a < n; a++) {
+    CamSlot &s = c->cams[cams[a]];
+    s.spp_total += c->h_cams[a].nspp;
+    s.rendered = true;
+  }
+  return VT_OK;
+}
+
+// ====================================================================== sensor
+static int sensor_params(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t h, uint32_t cam_index,
+                         float inv_spp, SensorDev &P) {
+  if (!p) return fail(c, VT_ERR_ARG, "sensor: null parameters");
+  if (p->cfa > 3) return fail(c, VT_ERR_ARG, "sensor: cfa code %u", p->cfa);
+  if (!(p->esf_sigma >= 0) || p->esf_sigma > 3.8f) return fail(c, VT_ERR_ARG, "sensor: esf_sigma outside [0, 3.8] px");
+  P.w = w; P.h = h; P.cfa = p->cfa; P.frame = p->frame; P.seed = p->seed; P.cam_index = cam_index;
+  P.optics_k = p->optics_k; P.esf_sigma = p->esf_sigma; P.exposure = p->exposure; P.gain = p->gain;
+  std::memcpy(P.m, p->m, sizeof P.m);
+  P.read_noise = p->read_noise; P.full_well = p->full_well; P.dark = p->dark; P.conv_gain = p->conv_gain;
+  P.max_dn = p->max_dn; P.black = p->black; P.noise_on = p->noise_on ? 1u : 0u;
+  P.inv_spp = inv_spp;
+  return VT_OK;
+}
+
+extern "C" int vt_sensor(vt_ctx *c, uint32_t cam, const vt_sensor_params *p) {
+  VT_LIVE(c);
+  if (cam >= VT_MAX_CAMERAS || !c->cams[cam].set) return fail(c, VT_ERR_STATE, "sensor: camera %u not set", cam);
+  CamSlot &s = c->cams[cam];
+  if (!s.rendered || s.spp_total == 0) return fail(c, VT_ERR_STATE, "sensor: camera %u has no radiance", cam);
+  VT_HIP(c, hipSetDevice(c->device));
+  SensorDev P;
+  const float inv = s.c.scale > 0 ? s.c.scale / (float)s.spp_total : 1.0f / (float)s.spp_total;
+  int rc = sensor_params(c, p, s.c.width, s.c.height, s.c.cam_index, inv, P);
+  if (rc) return rc;
+  if (!s.ri_valid || s.ri_pupil != p->pupil_exp) {
+    std::vector<float> ri;
+    ri_map_host(s.c, p->pupil_exp, ri);
+    VT_HIP(c, hipMemcpy(s.ri.p, ri.data(), ri.size() * 4, hipMemcpyHostToDevice));
+    s.ri_valid = true;
+    s.ri_pupil = p->pupil_exp;
+  }
+  VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
+  VT_HIP(c, launch_sensor(P, (const float4 *)s.acc.p, (const float *)s.ri.p, (uint16_t *)s.dn.p, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_s1, c->stream));
+  c->sensor_timed = true;
+  s.sensed = true;
+  return VT_OK;
+}
+
+// ====================================================================== readback
+extern "C" int vt_readback(vt_ctx *c, uint32_t cam, int kind, void *dst, size_t bytes) {
+  VT_LIVE(c);
+  if (cam >= VT_MAX_CAMERAS || !c->cams[cam].set) return fail(c, VT_ERR_STATE, "readback: camera %u not set", cam);
+  if (!dst) return fail(c, VT_ERR_ARG, "readback: null destination");
+  CamSlot &s = c->cams[cam];
+  const size_t px = (size_t)s.c.width * s.c.height;
+  VT_HIP(c, hipSetDevice(c->device));
+  if (kind == VT_KIND_LINEAR) {
+    if (bytes != px * 16) return fail(c, VT_ERR_SIZE, "readback linear: %zu bytes, want %zu", bytes, px * 16);
+    if (!s.rendered || s.spp_total == 0) return fail(c, VT_ERR_STATE, "readback: camera %u not rendered", cam);
+    const float sc = (s.c.scale > 0 ? s.c.scale : 1.0f) / (float)s.spp_total;
+    VT_HIP(c, launch_finalize_linear((const float4 *)s.acc.p, sc, px, (float4 *)s.lin.p, c->stream));
+    VT_HIP(c, hipMemcpyAsync(dst, s.lin.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  } else if (kind == VT_KIND_RAW) {
+    if (bytes != px * 2) return fail(c, VT_ERR_SIZE, "readback raw: %zu bytes, want %zu", bytes, px * 2);
+    if (!s.sensed) return fail(c, VT_ERR_STATE, "readback raw: camera %u has no sensor output", cam);
+    VT_HIP(c, hipMemcpyAsync(dst, s.dn.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  } else if (kind == VT_KIND_RI) {
+    if (bytes != px * 4) return fail(c, VT_ERR_SIZE, "readback ri: %zu bytes, want %zu", bytes, px * 4);
+    if (!s.ri_valid) return fail(c, VT_ERR_STATE, "readback ri: no sensor run yet");
+    VT_HIP(c, hipMemcpyAsync(dst, s.ri.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    return fail(c, VT_ERR_ARG, "readback: unknown kind %d", kind);
+  }
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_sync(vt_ctx *c) {
+  VT_LIVE(c);
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
+  VT_LIVE(c);
+  if (!out) return VT_ERR_ARG;
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  std::memset(out, 0, sizeof *out);
+  unsigned long long cnt[8];
+  VT_HIP(c, hipMemcpy(cnt, c->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+  uint32_t ov[4];
+  VT_HIP(c, hipMemcpy(ov, c->overflow.p, sizeof ov, hipMemcpyDeviceToHost));
+  out->rays = (double)cnt[1];
+  out->samples = (double)cnt[2];
+  out->stack_overflows = ov[0];
+  out->launches = c->last_launches;
+  if (c->render_timed) {
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_r0, c->ev_r1));
+    out->ms_render = ms;
+    double sum = 0;
+    for (uint32_t k = 0; k < c->last_launches; k++) {
+      float m = 0;
+      VT_HIP(c, hipEventElapsedTime(&m, c->ev_launch[2 * k], c->ev_launch[2 * k + 1]));
+      sum += m;
+    }
+    out->ms_kernel_avg = c->last_launches ? sum / c->last_launches : 0;
+  }
+  if (c->sensor_timed) {
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
+    out->ms_sensor = ms;
+  }
+  return VT_OK;
+}
+
+// ====================================================================== test entry points
+extern "C" int vt_trace_test(vt_ctx *c, const float *rays, uint32_t n, int any_hit, float *hits) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "trace_test: no scene");
+  if (n && (!rays || !hits)) return VT_ERR_ARG;
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 32))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n * 16))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, rays, (size_t)n * 32, hipMemcpyHostToDevice));
+  TestLaunch T{c->S, (uint32_t *)c->overflow.p, (uint2 *)c->spill.p, c->spill_lanes,
+               (uint32_t)(c->spill_lanes / DEFAULT_BLOCK)};
+  VT_HIP(c, launch_trace_test(T, (const float *)c->test_in.p, n, any_hit, (float *)c->test_out.p, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(hits, c->test_out.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_lens_test(vt_ctx *c, const float lens[9], const float *uv, uint32_t n, float *rays) {
+  VT_LIVE(c);
+  if (!lens || (n && (!uv || !rays))) return VT_ERR_ARG;
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 8 + 64))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n * 12))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, lens, 36, hipMemcpyHostToDevice));
+  VT_HIP(c, hipMemcpy((char *)c->test_in.p + 64, uv, (size_t)n * 8, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_lens_test((const float *)c->test_in.p, (const float *)((char *)c->test_in.p + 64), n,
+                             (float *)c->test_out.p, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(rays, c->test_out.p, (size_t)n * 12, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_camera_ray_test(vt_ctx *c, uint32_t cam, const uint32_t *xyi, uint32_t n, float *out) {
+  VT_LIVE(c);
+  if (cam >= VT_MAX_CAMERAS || !c->cams[cam].set) return fail(c, VT_ERR_STATE, "camera %u not set", cam);
+  if (n && (!xyi || !out)) return VT_ERR_ARG;
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  const vt_camera &k = c->cams[cam].c;
+  DevCam d{};
+  std::memcpy(d.R, k.R, sizeof d.R);
+  std::memcpy(d.pos, k.pos, sizeof d.pos);
+  d.mode = k.mode;
+  std::memcpy(d.lens, k.lens, sizeof d.lens);
+  d.width = k.width; d.height = k.height; d.spp = k.spp; d.seed = k.seed; d.cam_index = k.cam_index;
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 12 + 256))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n * 24))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, &d, sizeof d, hipMemcpyHostToDevice));
+  VT_HIP(c, hipMemcpy((char *)c->test_in.p + 256, xyi, (size_t)n * 12, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_camray_test((const DevCam *)c->test_in.p, (const uint32_t *)((char *)c->test_in.p + 256), n,
+                               (float *)c->test_out.p, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(out, c->test_out.p, (size_t)n * 24, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_bsdf_test(vt_ctx *c, const float *cases, uint32_t n, float *out) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "bsdf_test: no scene (materials)");
+  if (n && (!cases || !out)) return VT_ERR_ARG;
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 64))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n * 32))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, cases, (size_t)n * 64, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_bsdf_test(c->S.mats, (const float *)c->test_in.p, n, (float *)c->test_out.p, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(out, c->test_out.p, (size_t)n * 32, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_sensor_test(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t h, const float *radiance,
+                              const float *ri, uint16_t *dn) {
+  VT_LIVE(c);
+  if (!radiance || !ri || !dn || w == 0 || h == 0) return VT_ERR_ARG;
+  VT_HIP(c, hipSetDevice(c->device));
+  SensorDev P;
+  int rc = sensor_params(c, p, w, h, 0, 1.0f, P);
+  if (rc) return rc;
+  P.cam_index = 0;
+  const size_t px = (size_t)w * h;
+  if ((rc = ensure(c, c->test_in, px * 16))) return rc;
+  if ((rc = ensure(c, c->test_aux, px * 4))) return rc;
+  if ((rc = ensure(c, c->test_out, px * 2))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, radiance, px * 16, hipMemcpyHostToDevice));
+  VT_HIP(c, hipMemcpy(c->test_aux.p, ri, px * 4, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_sensor(P, (const float4 *)c->test_in.p, (const float *)c->test_aux.p, (uint16_t *)c->test_out.p,
+                          c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(dn, c->test_out.p, px * 2, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+// ====================================================================== node radiosity
+extern "C" int vt_jacobi_sparse(vt_ctx *c, const vt_gather_tables *t, float *node_state, uint32_t iters,
+                                float *aux_out) {
+  VT_LIVE(c);
+  if (!t || !node_state || !t->active || !t->gtable || !t->normals || !t->node_mat || !t->albedo || !t->hh)
+    return fail(c, VT_ERR_ARG, "jacobi: null table");
+  if (t->nbr_k && (!t->nbr || !t->nbr_w)) return fail(c, VT_ERR_ARG, "jacobi: null neighbour table");
+  if (t->gather_k == 0 || t->albedo_bins == 0 || t->n_mat == 0) return fail(c, VT_ERR_ARG, "jacobi: empty sizes");
+  const size_t N = t->n_static, A = t->n_active, K = t->gather_k;
+  for (size_t k = 0; k < A; k++)
+    if (t->active[k] >= N) return fail(c, VT_ERR_ARG, "jacobi: active node out of range");
+  for (size_t k = 0; k < N * K; k++)
+    if (t->gtable[k] != 0xffffffffu && t->gtable[k] >= N) return fail(c, VT_ERR_ARG, "jacobi: gather entry out of range");
+  for (size_t k = 0; k < N; k++)
+    if (t->node_mat[k] >= t->n_mat) return fail(c, VT_ERR_ARG, "jacobi: node material out of range");
+  for (size_t k = 0; k < A * t->nbr_k; k++)
+    if (t->nbr_w[k] > 0 && t->nbr[k] >= A) return fail(c, VT_ERR_ARG, "jacobi: neighbour out of range");
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = upload(c, c->g_active, t->active, A * 4))) return rc;
+  if ((rc = upload(c, c->g_table, t->gtable, N * K * 4))) return rc;
+  if ((rc = upload(c, c->g_normals, t->normals, N * 16))) return rc;
+  if ((rc = upload(c, c->g_mat, t->node_mat, N * 4))) return rc;
+  if ((rc = upload(c, c->g_alb, t->albedo, (size_t)t->n_mat * t->albedo_bins * 32))) return rc;
+  if ((rc = upload(c, c->g_hh, t->hh, (size_t)t->n_mat * 16))) return rc;
+  if (t->nbr_k) {
+    if ((rc = upload(c, c->g_nbr, t->nbr, A * t->nbr_k * 4))) return rc;
+    if ((rc = upload(c, c->g_nbrw, t->nbr_w, A * t->nbr_k * 4))) return rc;
+  }
+  if ((rc = upload(c, c->g_state, node_state, N * 64))) return rc;
+  if ((rc = ensure(c, c->g_aux, std::max<size_t>(A, 1) * 16))) return rc;
+  GatherDev G{};
+  G.n_active = (uint32_t)A; G.gather_k = (uint32_t)K; G.nbr_k = t->nbr_k; G.bins = t->albedo_bins;
+  G.active = (const uint32_t *)c->g_active.p; G.gtable = (const uint32_t *)c->g_table.p;
+  G.node_mat = (const uint32_t *)c->g_mat.p; G.nbr = (const uint32_t *)c->g_nbr.p;
+  G.normals = (const float4 *)c->g_normals.p; G.albedo = (const float4 *)c->g_alb.p; G.hh = (const float4 *)c->g_hh.p;
+  G.nbr_w = (const float *)c->g_nbrw.p; G.node_state = (float4 *)c->g_state.p; G.aux = (float4 *)c->g_aux.p;
+  // ping-pong slots 1 and 2 (gpu/renderer.js:729-749): step k reads slot_in and writes the other
+  uint32_t slot_in = 1;
+  for (uint32_t it = 0; it < iters; it++) {
+    G.slot_in = slot_in;
+    G.slot_out = 3 - slot_in;
+    VT_HIP(c, launch_jacobi_gather(G, c->stream));
+    VT_HIP(c, launch_jacobi_update(G, c->stream));
+    slot_in = 3 - slot_in;
+  }
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(node_state, c->g_state.p, N * 64, hipMemcpyDeviceToHost));
+  if (aux_out) VT_HIP(c, hipMemcpy(aux_out, c->g_aux.p, A * 16, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+static int dense_alloc(vt_ctx *c, uint32_t n, uint32_t r) {
+  const size_t nn = (size_t)n * n, m = (size_t)n * r;
+  int rc;
+  if ((rc = ensure(c, c->dF32, nn * 4))) return rc;
+  if ((rc = ensure(c, c->dE, m * 4))) return rc;
+  if ((rc = ensure(c, c->dX, m * 4))) return rc;
+  if ((rc = ensure(c, c->dY, m * 4))) return rc;
+  if ((rc = ensure(c, c->dR, m * 4))) return rc;
+  if ((rc = ensure(c, c->dXd, m * 16))) return rc;      // x64, y64
+  if ((rc = ensure(c, c->dPart, RES_BLOCKS_H * 4 * 8))) return rc;
+  c->dn_n = n;
+  c->dn_r = r;
+  c->dn_rp = r;
+  release(c->dF16);
+  return VT_OK;
+}
+
+extern "C" int vt_dense_upload(vt_ctx *c, uint32_t n, uint32_t r, const float *F, const float *e) {
+  VT_LIVE(c);
+  if (!F || !e || n == 0 || r == 0) return fail(c, VT_ERR_ARG, "dense: null or empty input");
+  if (r > 48) return fail(c, VT_ERR_ARG, "dense: at most 48 right-hand sides");
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc = dense_alloc(c, n, r);
+  if (rc) return rc;
+  VT_HIP(c, hipMemcpy(c->dF32.p, F, (size_t)n * n * 4, hipMemcpyHostToDevice));
+  VT_HIP(c, hipMemcpy(c->dE.p, e, (size_t)n * r * 4, hipMemcpyHostToDevice));
+  return VT_OK;
+}
+
+extern "C" int vt_dense_formfactors(vt_ctx *c, uint32_t n, const float *nodes, float row_max, uint32_t r,
+                                    const float *e) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "dense: form factors need an uploaded scene");
+  if (!nodes || !e || n == 0 || r == 0 || r > 48) return fail(c, VT_ERR_ARG, "dense: bad form-factor input");
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc = dense_alloc(c, n, r);
+  if (rc) return rc;
+  DevBuf nb;
+  if ((rc = upload(c, nb, nodes, (size_t)n * 32))) return rc;
+  VT_HIP(c, hipMemcpy(c->dE.p, e, (size_t)n * r * 4, hipMemcpyHostToDevice));
+  hipError_t er = launch_formfactors(c->S, (uint32_t *)c->overflow.p, (uint2 *)c->spill.p, c->spill_lanes, n,
+                                     (const float4 *)nb.p, (float *)c->dF32.p, c->stream);
+  if (er == hipSuccess) er = launch_row_scale((float *)c->dF32.p, n, row_max, c->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+  release(nb);
+  if (er != hipSuccess) return hip_fail(c, er, "form factors");
+  return VT_OK;
+}
+
+extern "C" int vt_dense_download_F(vt_ctx *c, float *F_out, size_t bytes) {
+  VT_LIVE(c);
+  if (!c->dn_n) return fail(c, VT_ERR_STATE, "dense: nothing uploaded");
+  if (!F_out || bytes != (size_t)c->dn_n * c->dn_n * 4) return fail(c, VT_ERR_SIZE, "dense: F is n*n*4 bytes");
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipMemcpy(F_out, c->dF32.p, bytes, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+namespace {
+struct Norms {
+  double res2, e2, err2, ref2;
+};
+int read_norms(vt_ctx *c, Norms &o) {
+  double p[RES_BLOCKS_H * 4];
+  VT_HIP(c, hipMemcpyAsync(p, c->dPart.p, sizeof p, hipMemcpyDeviceToHost, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  o = {0, 0, 0, 0};
+  for (int b = 0; b < RES_BLOCKS_H; b++) {
+    o.res2 += p[4 * b];
+    o.e2 += p[4 * b + 1];
+    o.err2 += p[4 * b + 2];
+    o.ref2 += p[4 * b + 3];
+  }
+  return VT_OK;
+}
+}  // namespace
+
+extern "C" int vt_dense_solve(vt_ctx *c, const vt_dense_opts *o, float *x_out, vt_dense_result *res) {
+  VT_LIVE(c);
+  if (!o || !res) return VT_ERR_ARG;
+  if (!c->dn_n) return fail(c, VT_ERR_STATE, "dense: nothing uploaded");
+  if (o->n != c->dn_n || o->r != c->dn_r) return fail(c, VT_ERR_ARG, "dense: n, r differ from the upload");
+  const uint32_t n = c->dn_n, rp = c->dn_rp;
+  const uint64_t m = (uint64_t)n * rp;
+  const uint32_t maxs = o->max_sweeps ? o->max_sweeps : 1000;
+  VT_HIP(c, hipSetDevice(c->device));
+  std::memset(res, 0, sizeof *res);
+  float *F = (float *)c->dF32.p, *E = (float *)c->dE.p, *X = (float *)c->dX.p, *Y = (float *)c->dY.p,
+        *R = (float *)c->dR.p;
+  double *X64 = (double *)c->dXd.p, *Y64 = X64 + m, *part = (double *)c->dPart.p;
+  int rc;
+  Norms nm;
+  VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
+  if (o->method == VT_DENSE_FP32) {
+    VT_HIP(c, hipMemcpyAsync(X, E, m * 4, hipMemcpyDeviceToDevice, c->stream));
+    for (uint32_t s = 0; s < maxs; s++) {
+      VT_HIP(c, launch_sweep_fp32(F, n, rp, X, E, Y, c->stream));
+      VT_HIP(c, launch_resid_f32(Y, X, E, nullptr, m, part, c->stream));
+      std::swap(X, Y);
+      res->sweeps++;
+      if ((rc = read_norms(c, nm))) return rc;
+      if (std::sqrt(nm.res2) <= o->tol * std::sqrt(nm.e2)) break;
+    }
+  } else if (o->method == VT_DENSE_BF16_REFINE) {
+    if (n % 128) return fail(c, VT_ERR_ARG, "dense bf16: n must be a multiple of 128");
+    if (!c->dF16.p) {
+      if ((rc = ensure(c, c->dF16, (size_t)n * n * 2))) return rc;
+      VT_HIP(c, launch_convert_bf16(F, (uint16_t *)c->dF16.p, (uint64_t)n * n, c->stream));
+    }
+    DevBuf xT, D;
+    if ((rc = ensure(c, xT, m * 2))) return rc;
+    if ((rc = ensure(c, D, m * 4))) return rc;
+    float *Dp = (float *)D.p;
+    const uint32_t inner = o->inner_sweeps ? o->inner_sweeps : 32;
+    VT_HIP(c, hipMemsetAsync(X, 0, m * 4, c->stream));
+    bool ok = true;
+    hipError_t er = hipSuccess;
+    while (res->sweeps < maxs) {
+      // FP32 residual r = e + F x - x
+      if ((er = launch_sweep_fp32(F, n, rp, X, E, Y, c->stream)) != hipSuccess) break;
+      if ((er = launch_sub(Y, X, R, m, c->stream)) != hipSuccess) break;
+      if ((er = launch_resid_f32(Y, X, E, nullptr, m, part, c->stream)) != hipSuccess) break;
+      res->sweeps++;
+      if ((rc = read_norms(c, nm))) { ok = false; break; }
+      if (std::sqrt(nm.res2) <= o->tol * std::sqrt(nm.e2)) break;
+      // correction d = r + F d by BF16 sweeps from d = r, then x += d
+      if ((er = hipMemcpyAsync(Dp, R, m * 4, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) break;
+      for (uint32_t k = 0; k < inner && er == hipSuccess; k++) {
+        er = launch_x_to_bf16T(Dp, n, rp, (uint16_t *)xT.p, c->stream);
+        if (er == hipSuccess) er = launch_sweep_bf16_T((const uint16_t *)c->dF16.p, (const uint16_t *)xT.p, n, rp, R, Dp, c->stream);
+        res->sweeps++;
+      }
+      if (er != hipSuccess) break;
+      if ((er = launch_add(X, Dp, m, c->stream)) != hipSuccess) break;
+      res->refinements++;
+    }
+    release(xT);
+    release(D);
+    if (er != hipSuccess) return hip_fail(c, er, "bf16 refinement");
+    if (!ok) return rc;
+  } else if (o->method == VT_DENSE_FP64) {
+    VT_HIP(c, launch_widen(E, X64, m, c->stream));
+    for (uint32_t s = 0; s < maxs; s++) {
+      VT_HIP(c, launch_sweep_fp64(F, n, rp, X64, E, Y64, c->stream));
+      VT_HIP(c, launch_resid_f64(Y64, X64, E, nullptr, m, part, c->stream));
+      std::swap(X64, Y64);
+      res->sweeps++;
+      if ((rc = read_norms(c, nm))) return rc;
+      if (std::sqrt(nm.res2) <= o->tol * std::sqrt(nm.e2)) break;
+    }
+  } else {
+    return fail(c, VT_ERR_ARG, "dense: unknown method %u", o->method);
+  }
+  VT_HIP(c, hipEventRecord(c->ev_s1, c->stream));
+  VT_HIP(c, hipEventSynchronize(c->ev_s1));
+  float ms = 0;
+  VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
+  res->ms = ms;
+  res->ms_per_sweep = res->sweeps ? ms / res->sweeps : 0;
+  // final residual in FP64: r = e + F x - x with F widened exactly
+  double *Xc = X64, *Yc = Y64;
+  if (o->method != VT_DENSE_FP64) VT_HIP(c, launch_widen(X, Xc, m, c->stream));
+  VT_HIP(c, launch_sweep_fp64(F, n, rp, Xc, E, Yc, c->stream));
+  VT_HIP(c, launch_resid_f64(Yc, Xc, E, nullptr, m, part, c->stream));
+  if ((rc = read_norms(c, nm))) return rc;
+  res->residual = std::sqrt(nm.res2) / std::max(1e-300, std::sqrt(nm.e2));
+  if (x_out) {
+    if (o->method == VT_DENSE_FP64) {
+      std::vector<double> tmp(m);
+      VT_HIP(c, hipMemcpy(tmp.data(), Xc, m * 8, hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < m; i++) x_out[i] = (float)tmp[i];
+    } else {
+      VT_HIP(c, hipMemcpy(x_out, X, m * 4, hipMemcpyDeviceToHost));
+    }
+  }
+  // keep the solution in dX for later calls
+  if (X != (float *)c->dX.p && o->method != VT_DENSE_FP64) VT_HIP(c, hipMemcpy(c->dX.p, X, m * 4, hipMemcpyDeviceToDevice));
+  return VT_OK;
+}

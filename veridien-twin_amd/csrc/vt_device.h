@@ -1,0 +1,470 @@
+// Device-side building blocks of the camera path for gfx950: vector helpers, the hash streams and sample
+// pattern of gpu/wgsl/common.wgsl.js:94-113 and kernels.wgsl.js:34-53, the FP32 KB4 inverse of
+// gpu/wgsl/lens.wgsl.js:9-29, the BSDF of gpu/wgsl/bsdf.wgsl.js (plus the Lambertian kind), emitter sampling
+// of gpu/wgsl/shade.wgsl.js:31-47, and the four-wide BVH traversal with a watertight triangle test.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vt {
+
+constexpr float PI_F = 3.14159265358979f;
+constexpr float INV_PI_F = 0.318309886183791f;
+constexpr uint32_t MISS = 0xffffffffu;
+constexpr float EPS_OFF = 2e-6f;      // ray origin offset along the geometric normal (m), common.wgsl.js:33
+constexpr uint32_t F_EMITTER = 2u;
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t EMPTY_REF = 0xffffffffu;
+
+// ------------------------------------------------------------------ small vector type
+struct v3 {
+  float x, y, z;
+};
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 operator*(float s, v3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 operator*(v3 a, v3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ v3 normalize(v3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ float maxc(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ v3 ld3(const float4 &f) { return {f.x, f.y, f.z}; }
+
+// ------------------------------------------------------------------ hashes and the sample pattern
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+  uint32_t s = v * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) { return pcg(a ^ pcg(b ^ pcg(c))); }
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8u) * (1.0f / 16777216.0f); }
+// fract(shift + f32(i) * a), product and sum each rounded: the WGSL sequence, never fused
+__device__ __forceinline__ float kr(uint32_t i, float a, float shift) {
+#pragma clang fp contract(off)
+  float p = (float)i * a;
+  float s = shift + p;
+  return s - floorf(s);
+}
+
+constexpr float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
+constexpr float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
+constexpr float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
+
+// ------------------------------------------------------------------ cameras
+struct DevCam {
+  float R[9];
+  float pos[3];
+  uint32_t mode;
+  float lens[9];                 // fx fy cx cy k1..k4 alpha
+  uint32_t width, height, spp, seed;
+  uint32_t cam_index, nl, depth, tiles_x;
+  uint32_t tiles_y, nspp, spp_begin, accumulate;
+  uint64_t item_begin, item_count;   // in items of this launch's queue (multiples of 64)
+  float4 *acc;                       // per-pixel radiance sum (rgb), sample count (a)
+};
+
+// KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)
+__device__ __forceinline__ v3 kb4_unproject(const float *L, float u, float v) {
+  const float fx = L[0], fy = L[1], cx = L[2], cy = L[3];
+  const float k1 = L[4], k2 = L[5], k3 = L[6], k4 = L[7], al = L[8];
+  float yd = (v - cy) / fy;
+  float xd = (u - cx) / fx - al * yd;
+  float thd = sqrtf(xd * xd + yd * yd);
+  if (thd < 1e-20f) return mk(0.f, 0.f, 1.f);
+  float th = thd;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    float t2 = th * th;
+    float f = th * (1.0f + t2 * (k1 + t2 * (k2 + t2 * (k3 + t2 * k4)))) - thd;
+    float fp = 1.0f + t2 * (3.0f * k1 + t2 * (5.0f * k2 + t2 * (7.0f * k3 + t2 * 9.0f * k4)));
+    th = th - f / fp;
+  }
+  float sn, cs;
+  sincosf(th, &sn, &cs);
+  float s = sn / thd;
+  return mk(xd * s, yd * s, cs);
+}
+
+struct PixSeeds {
+  uint32_t hs;
+  float s[9];
+};
+__device__ __forceinline__ void pix_seeds(const DevCam &C, uint32_t x, uint32_t y, PixSeeds &P) {
+  P.hs = hash3(x, y, C.seed * 64u + C.cam_index);
+#pragma unroll
+  for (int k = 0; k < 9; k++) P.s[k] = u01(pcg(P.hs + (uint32_t)k));
+}
+
+__device__ __forceinline__ v3 camera_dir(const DevCam &C, const PixSeeds &P, uint32_t x, uint32_t y, uint32_t i) {
+  const float u = ((float)x + kr(i, R2A, P.s[0])) - 0.5f;
+  const float v = ((float)y + kr(i, R2B, P.s[1])) - 0.5f;
+  v3 dc;
+  if (C.mode == 0) dc = kb4_unproject(C.lens, u, v);
+  else dc = normalize(mk((u - C.lens[2]) / C.lens[0], (v - C.lens[3]) / C.lens[1], 1.0f));
+  v3 d = mk(C.R[0], C.R[1], C.R[2]) * dc.x + mk(C.R[3], C.R[4], C.R[5]) * dc.y + mk(C.R[6], C.R[7], C.R[8]) * dc.z;
+  return normalize(d);
+}
+
+// ------------------------------------------------------------------ BSDF (bsdf.wgsl.js; kind 3 = Lambert)
+struct Mat {
+  v3 c;
+  float alpha, ior, fbar;
+  int kind;
+  v3 tau;
+  float alpha_c;
+};
+__device__ __forceinline__ Mat load_mat(const float4 *mats, uint32_t m) {
+  float4 a = mats[4 * m], b = mats[4 * m + 1], c = mats[4 * m + 2];
+  Mat M;
+  M.c = mk(a.x, a.y, a.z);
+  M.alpha = a.w;
+  M.ior = b.x;
+  M.kind = (int)b.y;
+  M.fbar = b.z;
+  M.tau = mk(c.x, c.y, c.z);
+  M.alpha_c = c.w;
+  return M;
+}
+__device__ __forceinline__ float fresnel_dielectric(float cosi, float eta) {
+  float c = fminf(fmaxf(cosi, 0.0f), 1.0f);
+  float s2 = (1.0f - c * c) / (eta * eta);
+  if (s2 >= 1.0f) return 1.0f;
+  float ct = sqrtf(1.0f - s2);
+  float rs = (c - eta * ct) / (c + eta * ct);
+  float rp = (eta * c - ct) / (eta * c + ct);
+  return 0.5f * (rs * rs + rp * rp);
+}
+__device__ __forceinline__ float pow5(float x) { float x2 = x * x; return x2 * x2 * x; }
+__device__ __forceinline__ v3 schlick3(v3 f0, float c) {
+  float m = pow5(1.0f - fminf(fmaxf(c, 0.0f), 1.0f));
+  return f0 + (mk(1.f, 1.f, 1.f) - f0) * m;
+}
+__device__ __forceinline__ float ggx_d(float nh, float a) {
+  if (nh <= 0.0f) return 0.0f;
+  float a2 = a * a;
+  float d = nh * nh * (a2 - 1.0f) + 1.0f;
+  return a2 / (PI_F * d * d);
+}
+__device__ __forceinline__ float ggx_lambda(float c, float a) {
+  float c2 = fmaxf(c * c, 1e-12f);
+  float t2 = (1.0f - c2) / c2;
+  return 0.5f * (-1.0f + sqrtf(1.0f + a * a * t2));
+}
+__device__ __forceinline__ float ggx_g1(float c, float a) { return 1.0f / (1.0f + ggx_lambda(c, a)); }
+__device__ __forceinline__ float ggx_g2(float nl, float nv, float a) {
+  return 1.0f / (1.0f + ggx_lambda(nl, a) + ggx_lambda(nv, a));
+}
+__device__ __forceinline__ float luminance(v3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+
+__device__ __forceinline__ v3 eval_bsdf(const Mat &M, v3 n, v3 l, v3 v) {
+  float nl = dot(n, l), nv = dot(n, v);
+  if (nl <= 0.0f || nv <= 0.0f) return mk(0.f, 0.f, 0.f);
+  if (M.kind == 3) return M.c * INV_PI_F;
+  v3 h = normalize(l + v);
+  float nh = dot(n, h), vh = fmaxf(0.0f, dot(v, h));
+  if (M.kind == 2) {
+    float coat = ggx_d(nh, M.alpha_c) * ggx_g2(nl, nv, M.alpha_c) * fresnel_dielectric(vh, M.ior) / (4.0f * nl * nv);
+    float t = (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior));
+    float metal = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+    float e = 1.0f / nl + 1.0f / nv;
+    v3 tp = mk(powf(M.tau.x, e), powf(M.tau.y, e), powf(M.tau.z, e));
+    return mk(coat, coat, coat) + tp * schlick3(M.c, vh) * (t * metal);
+  }
+  float spec = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+  v3 F = M.kind == 1 ? schlick3(M.c, vh) : [&] { float f = fresnel_dielectric(vh, M.ior); return mk(f, f, f); }();
+  v3 out = F * spec;
+  if (M.kind == 0) {
+    float kd = INV_PI_F * (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior)) / (1.0f - M.fbar);
+    out = out + M.c * kd;
+  }
+  return out;
+}
+__device__ __forceinline__ float spec_prob(const Mat &M, float nv) {
+  if (M.kind == 3) return 0.0f;
+  if (M.kind == 2) {
+    float Fc = fresnel_dielectric(nv, M.ior);
+    return Fc / (Fc + (1.0f - Fc) * (M.c.x + M.c.y + M.c.z) / 3.0f);
+  }
+  if (M.kind == 1) return 1.0f;
+  float F = fresnel_dielectric(nv, M.ior);
+  float d = luminance(M.c) * (1.0f - F);
+  float s = fmaxf(F, 0.04f);
+  return fminf(fmaxf(s / (s + d), 0.05f), 0.95f);
+}
+__device__ __forceinline__ float pdf_bsdf(const Mat &M, v3 n, v3 v, v3 l) {
+  float nl = dot(n, l), nv = dot(n, v);
+  if (nl <= 0.0f || nv <= 0.0f) return 0.0f;
+  if (M.kind == 3) return nl * INV_PI_F;
+  v3 h = normalize(l + v);
+  float nh = dot(n, h), vh = dot(v, h);
+  float ps = spec_prob(M, nv);
+  if (M.kind == 2) {
+    if (!(vh > 0.0f)) return 0.0f;
+    return ps * ggx_g1(nv, M.alpha_c) * ggx_d(nh, M.alpha_c) / (4.0f * nv) +
+           (1.0f - ps) * ggx_g1(nv, M.alpha) * ggx_d(nh, M.alpha) / (4.0f * nv);
+  }
+  float pdf_spec = vh > 0.0f ? ggx_g1(nv, M.alpha) * ggx_d(nh, M.alpha) / (4.0f * nv) : 0.0f;
+  return ps * pdf_spec + (1.0f - ps) * nl * INV_PI_F;
+}
+// Duff et al. 2017 basis (common.wgsl.js:146-157)
+__device__ __forceinline__ void basis(v3 n, v3 &t, v3 &b) {
+  float s = n.z >= 0.0f ? 1.0f : -1.0f;
+  float a = -1.0f / (s + n.z);
+  float bb = n.x * n.y * a;
+  t = mk(1.0f + s * n.x * n.x * a, s * bb, -s * n.x);
+  b = mk(bb, s + n.y * n.y * a, -n.y);
+}
+// Heitz 2018 visible-normal sampling (bsdf.wgsl.js:86-101)
+__device__ __forceinline__ v3 sample_vndf(v3 ve, float a, float u1, float u2) {
+  v3 vh = normalize(mk(a * ve.x, a * ve.y, ve.z));
+  float lensq = vh.x * vh.x + vh.y * vh.y;
+  v3 T1 = mk(1.f, 0.f, 0.f);
+  if (lensq > 0.0f) T1 = mk(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq));
+  v3 T2 = cross(vh, T1);
+  float r = sqrtf(u1), phi = 2.0f * PI_F * u2;
+  float sn, cs;
+  sincosf(phi, &sn, &cs);
+  float t1 = r * cs, t2 = r * sn;
+  float s = 0.5f * (1.0f + vh.z);
+  t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
+  float t3 = sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
+  v3 nh = T1 * t1 + T2 * t2 + vh * t3;
+  return normalize(mk(a * nh.x, a * nh.y, fmaxf(1e-7f, nh.z)));
+}
+// returns pdf (0: no sample), writes l (bsdf.wgsl.js:133-155)
+__device__ __forceinline__ float sample_bsdf(const Mat &M, v3 n, v3 v, float u0, float u1, float u2, v3 &l) {
+  float nv = dot(n, v);
+  if (nv <= 0.0f) return 0.0f;
+  v3 t, b;
+  basis(n, t, b);
+  float ps = spec_prob(M, nv);
+  if (M.kind == 2 || (M.kind != 3 && u0 < ps)) {
+    float a = M.kind == 2 ? (u0 < ps ? M.alpha_c : M.alpha) : M.alpha;
+    v3 hl = sample_vndf(mk(dot(t, v), dot(b, v), nv), a, u1, u2);
+    v3 h = t * hl.x + b * hl.y + n * hl.z;
+    l = h * (2.0f * dot(v, h)) - v;
+  } else {
+    float r = sqrtf(u1), phi = 2.0f * PI_F * u2;
+    float sn, cs;
+    sincosf(phi, &sn, &cs);
+    l = t * (r * cs) + b * (r * sn) + n * sqrtf(fmaxf(0.0f, 1.0f - u1));
+  }
+  if (dot(n, l) <= 0.0f) return 0.0f;
+  return pdf_bsdf(M, n, v, l);
+}
+
+// ------------------------------------------------------------------ scene on the device
+struct __attribute__((aligned(128))) Node4 {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint32_t ref[4];      // interior: node index; leaf: LEAF_BIT | (count - 1) << 26 | first slot; EMPTY_REF
+  uint32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is one 128-B cache line");
+
+struct SceneDev {
+  const Node4 *nodes;
+  const float4 *tri;        // 3 float4 per BVH slot: v0, v1, v2 (w unused)
+  const uint4 *attr;        // per slot: vertex ids i0 i1 i2, mesh
+  const float4 *vnorm;      // per vertex: normal xyz
+  const uint4 *meshes;      // material, flags, light, -
+  const float4 *mats;       // 4 float4 per material (16 floats)
+  const float4 *lights;     // radiance rgb
+  const float4 *emit;       // 4 float4 per emitter triangle
+  uint32_t emit_count, root;
+  float emit_area;
+};
+
+struct Hit {
+  float t, u, v;
+  uint32_t tri;
+};
+
+// per-lane traversal stack: SLDS entries in LDS, the rest spilled to a per-lane global area
+struct Stack {
+  uint32_t *lds_node;   // base of this block's node-index array, [slot * BLOCK + tid]
+  float *lds_t;
+  uint2 *spill;         // [(slot - SLDS) * spill_stride + gtid]
+  uint32_t tid, block, slds, spill_depth;
+  uint64_t spill_stride, gtid;
+  uint32_t *overflow;   // global counter of entries that found no slot
+};
+__device__ __forceinline__ void push(Stack &S, uint32_t &sp, uint32_t node, float t) {
+  if (sp < S.slds) {
+    S.lds_node[sp * S.block + S.tid] = node;
+    S.lds_t[sp * S.block + S.tid] = t;
+  } else if (sp < S.slds + S.spill_depth) {
+    S.spill[(uint64_t)(sp - S.slds) * S.spill_stride + S.gtid] = make_uint2(node, __float_as_uint(t));
+  } else {
+    atomicAdd(S.overflow, 1u);
+    return;
+  }
+  sp++;
+}
+__device__ __forceinline__ void pop(const Stack &S, uint32_t sp, uint32_t &node, float &t) {
+  if (sp < S.slds) {
+    node = S.lds_node[sp * S.block + S.tid];
+    t = S.lds_t[sp * S.block + S.tid];
+  } else {
+    uint2 e = S.spill[(uint64_t)(sp - S.slds) * S.spill_stride + S.gtid];
+    node = e.x;
+    t = __uint_as_float(e.y);
+  }
+}
+
+// Watertight ray-triangle test (Woop, Benthin, Wald, JCGT 2013), two-sided, with the FP64 fallback on an
+// exact edge. Edge functions are evaluated unfused so a shared edge gives exactly negated values.
+struct RayW {
+  v3 o;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+__device__ __forceinline__ float comp(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ RayW ray_w(v3 o, v3 d) {
+  RayW R;
+  R.o = o;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  if (comp(d, kz) < 0.0f) { int s = kx; kx = ky; ky = s; }
+  R.kx = kx; R.ky = ky; R.kz = kz;
+  float dz = comp(d, kz);
+  R.Sx = comp(d, kx) / dz;
+  R.Sy = comp(d, ky) / dz;
+  R.Sz = 1.0f / dz;
+  return R;
+}
+__device__ __forceinline__ bool tri_hit(const RayW &R, v3 p0, v3 p1, v3 p2, float tmin, float tmax, float &t,
+                                        float &u, float &v) {
+#pragma clang fp contract(off)
+  v3 A = p0 - R.o, B = p1 - R.o, C = p2 - R.o;
+  float Az = comp(A, R.kz), Bz = comp(B, R.kz), Cz = comp(C, R.kz);
+  float Ax = comp(A, R.kx) - R.Sx * Az, Ay = comp(A, R.ky) - R.Sy * Az;
+  float Bx = comp(B, R.kx) - R.Sx * Bz, By = comp(B, R.ky) - R.Sy * Bz;
+  float Cx = comp(C, R.kx) - R.Sx * Cz, Cy = comp(C, R.ky) - R.Sy * Cz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float T = U * (R.Sz * Az) + V * (R.Sz * Bz) + W * (R.Sz * Cz);
+  float rd = 1.0f / det;
+  float tt = T * rd;
+  if (!(tt > tmin && tt < tmax)) return false;
+  t = tt;
+  u = V * rd;
+  v = W * rd;
+  return true;
+}
+
+__device__ __forceinline__ v3 safe_inv(v3 d) {
+  return mk(1.0f / copysignf(fmaxf(fabsf(d.x), 1e-20f), d.x), 1.0f / copysignf(fmaxf(fabsf(d.y), 1e-20f), d.y),
+            1.0f / copysignf(fmaxf(fabsf(d.z), 1e-20f), d.z));
+}
+
+// Closest (any_hit = false) or any hit in (tmin, tmax). Returns the number of node visits and triangle tests
+// through *work (x: nodes, y: triangles) when counting.
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, Hit &h, Stack &st,
+                                         uint32_t *vis_nodes = nullptr, uint32_t *vis_tris = nullptr) {
+  const v3 inv = safe_inv(d);
+  const v3 oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
+  const RayW R = ray_w(o, d);
+  float best = tmax;
+  bool found = false;
+  uint32_t sp = 0;
+  uint32_t ref = S.root;
+  uint32_t nv = 0, nt = 0;
+  for (;;) {
+    if (!(ref & LEAF_BIT)) {
+      const float4 *np = reinterpret_cast<const float4 *>(S.nodes + ref);
+      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
+      nv++;
+      float tn[4], lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
+      float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
+      float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+      uint32_t rr[4] = {rf.x, rf.y, rf.z, rf.w};
+      const float tfar_lim = best * 1.00000024f;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
+        float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
+        float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
+        float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+        tn[c] = (a <= b && rr[c] != EMPTY_REF) ? a : INFINITY;
+      }
+      // sort the four (t, ref) pairs ascending: a five-comparator network
+#define VT_CSWAP(i, j)                                                   \
+  if (tn[j] < tn[i]) {                                                   \
+    float ft = tn[i]; tn[i] = tn[j]; tn[j] = ft;                         \
+    uint32_t fr = rr[i]; rr[i] = rr[j]; rr[j] = fr;                      \
+  }
+      VT_CSWAP(0, 1) VT_CSWAP(2, 3) VT_CSWAP(0, 2) VT_CSWAP(1, 3) VT_CSWAP(1, 2)
+#undef VT_CSWAP
+      if (tn[0] < INFINITY) {
+        if (tn[3] < INFINITY) push(st, sp, rr[3], tn[3]);
+        if (tn[2] < INFINITY) push(st, sp, rr[2], tn[2]);
+        if (tn[1] < INFINITY) push(st, sp, rr[1], tn[1]);
+        ref = rr[0];
+        continue;
+      }
+    } else {
+      uint32_t first = ref & 0x03ffffffu, cnt = ((ref >> 26) & 31u) + 1u;
+      for (uint32_t k = first; k < first + cnt; k++) {
+        const float4 *tp = S.tri + 3 * (size_t)k;
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        float t, u, v;
+        nt++;
+        if (tri_hit(R, ld3(a), ld3(b), ld3(c), tmin, best, t, u, v)) {
+          best = t;
+          h.t = t; h.u = u; h.v = v; h.tri = k;
+          found = true;
+          if (ANY) {
+            if (vis_nodes) { *vis_nodes += nv; *vis_tris += nt; }
+            return true;
+          }
+        }
+      }
+    }
+    // pop the next entry nearer than the best hit
+    bool next = false;
+    while (sp > 0) {
+      sp--;
+      float t;
+      pop(st, sp, ref, t);
+      if (t < best) { next = true; break; }
+    }
+    if (!next) break;
+  }
+  if (vis_nodes) { *vis_nodes += nv; *vis_tris += nt; }
+  return found;
+}
+
+// emitter sample (shade.wgsl.js:31-47); returns the record index
+__device__ __forceinline__ uint32_t sample_emitter(const SceneDev &S, float ux, float uy, float uz, v3 &p) {
+  uint32_t lo = 0, hi = S.emit_count - 1;
+  for (int it = 0; it < 40; it++) {
+    if (lo >= hi) break;
+    uint32_t m = (lo + hi) / 2;
+    if (S.emit[4 * m].w < ux) lo = m + 1; else hi = m;
+  }
+  float4 v0 = S.emit[4 * lo], e1 = S.emit[4 * lo + 1], e2 = S.emit[4 * lo + 2];
+  float su = sqrtf(uy);
+  p = ld3(v0) + ld3(e1) * (su * (1.0f - uz)) + ld3(e2) * (su * uz);
+  return lo;
+}
+
+}  // namespace vt

@@ -1,0 +1,102 @@
+// Host-side declarations shared by the library's C++ and HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+#include "vt_device.h"
+
+namespace vt {
+
+// tuned launch shapes for the trace kernels (block threads; LDS stack entries per lane)
+constexpr int DEFAULT_BLOCK = 256;
+constexpr int STACK_LDS = 16;
+constexpr int STACK_SPILL = 32;
+
+struct RenderLaunch {
+  SceneDev S;
+  const DevCam *cams;          // device array
+  uint32_t ncam;
+  uint64_t item_base;          // queue item of wave 0 (a camera's offset when it is launched alone)
+  uint64_t total_waves;        // 64-item waves in the queue
+  unsigned long long *counters;// [0] queue head, [1] rays traced, [2] samples
+  uint32_t *overflow;          // stack overflow counter
+  uint2 *spill;                // per-lane spill stacks
+  uint64_t spill_stride;       // lanes with a spill slot (persistent grid size)
+  uint32_t chunk;              // waves per queue grab
+  uint32_t persistent;         // 1: waves take chunks from the atomic queue head; 0: static grid stride
+  uint32_t block;              // threads per block (64, 128, 256)
+  uint32_t grid;               // blocks
+};
+hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
+hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
+
+struct TestLaunch {
+  SceneDev S;
+  uint32_t *overflow;
+  uint2 *spill;
+  uint64_t spill_stride;   // lanes with a spill slot; the grid never exceeds it
+  uint32_t max_blocks;     // spill_stride / DEFAULT_BLOCK
+};
+hipError_t launch_trace_test(const TestLaunch &L, const float *rays, uint32_t n, int any_hit, float *hits,
+                             hipStream_t s);
+hipError_t launch_lens_test(const float *lens9, const float *uv, uint32_t n, float *rays, hipStream_t s);
+hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out, hipStream_t s);
+hipError_t launch_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out, hipStream_t s);
+
+// the fused sensor epilogue: radiance = acc * inv_spp (rgb), RI map, -> DN
+struct SensorDev {
+  uint32_t w, h, cfa, frame, seed, cam_index;
+  float optics_k, esf_sigma, exposure, gain;
+  float m[9];
+  float read_noise, full_well, dark, conv_gain, max_dn, black;
+  uint32_t noise_on;
+  float inv_spp;
+};
+hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, uint16_t *dn, hipStream_t s);
+hipError_t launch_finalize_linear(const float4 *acc, float scale_over_spp, uint64_t n, float4 *out, hipStream_t s);
+
+// node radiosity
+struct GatherDev {
+  uint32_t n_active, gather_k, nbr_k, bins, slot_in, slot_out;
+  const uint32_t *active, *gtable, *node_mat, *nbr;
+  const float4 *normals, *albedo, *hh;
+  const float *nbr_w;
+  float4 *node_state;
+  float4 *aux;
+};
+hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s);
+hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s);
+
+// dense node radiosity: x, y, e are n x rp (row-major, RHS fastest); K split four ways, sums added into y
+hipError_t launch_sweep_fp32(const float *F32, uint32_t n, uint32_t rp, const float *x, const float *e, float *y,
+                             hipStream_t s);
+hipError_t launch_sweep_fp64(const float *F32, uint32_t n, uint32_t rp, const double *x, const float *e, double *y,
+                             hipStream_t s);
+// BF16 F on MFMA with FP32 accumulation; xT is x rounded to BF16 and transposed (rp x n); n % 128 == 0
+hipError_t launch_sweep_bf16_T(const uint16_t *F16, const uint16_t *xT, uint32_t n, uint32_t rp, const float *e,
+                               float *y, hipStream_t s);
+hipError_t launch_x_to_bf16T(const float *x, uint32_t n, uint32_t rp, uint16_t *xT, hipStream_t s);
+hipError_t launch_convert_bf16(const float *src, uint16_t *dst, uint64_t n, hipStream_t s);
+// per-block partial sums (RES_BLOCKS x 4): sum (y - x)^2, sum e^2, sum (x - ref)^2, sum ref^2
+constexpr int RES_BLOCKS_H = 256;
+hipError_t launch_resid_f32(const float *y, const float *x, const float *e, const double *ref, uint64_t m,
+                            double *part, hipStream_t s);
+hipError_t launch_resid_f64(const double *y, const double *x, const float *e, const double *ref, uint64_t m,
+                            double *part, hipStream_t s);
+hipError_t launch_widen(const float *x, double *y, uint64_t m, hipStream_t s);
+hipError_t launch_add(float *a, const float *b, uint64_t m, hipStream_t s);
+hipError_t launch_sub(const float *y, const float *x, float *r, uint64_t m, hipStream_t s);
+hipError_t launch_formfactors(const SceneDev &S, uint32_t *overflow, uint2 *spill, uint64_t spill_stride,
+                              uint32_t n, const float4 *nodes, float *F, hipStream_t s);
+hipError_t launch_row_scale(float *F, uint32_t n, float row_max, hipStream_t s);
+
+// ------------------------------------------------------------------ host BVH utilities (vt_bvh.cpp)
+// the reference's binned-SAH BVH2 (gpu/bvh.js:18-161); returns node count, or -1 if cap_nodes is too small
+int64_t build_bvh2(const float *pos, const uint32_t *idx, uint32_t ntri, int leaf, int bins, float *nodes,
+                   uint32_t cap_nodes, uint32_t *order);
+// BVH2 (reference layout) -> 128-B four-wide nodes; throws std::runtime_error on a malformed tree
+std::vector<Node4> collapse_bvh4(const float *nodes2, uint32_t node_count, uint32_t tri_count);
+
+}  // namespace vt

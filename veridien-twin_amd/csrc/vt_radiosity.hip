@@ -1,0 +1,392 @@
+// Node radiosity on gfx950.
+//  * The reference's sparse gather Jacobi (gpu/wgsl/kernels.wgsl.js:416-461): one lane per active node.
+//  * The dense form x <- e + F x of the precision configuration (n = 32768 nodes, 48 right-hand sides):
+//      FP32 / FP64 sweeps on the vector ALUs (128-row x 48-RHS block tiles, split over K, x staged in LDS);
+//      the BF16 sweep on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), F stored once as BF16 so a
+//      sweep streams 2 B per matrix entry; the form-factor build traces one visibility ray per node pair.
+#include "vt_internal.h"
+
+namespace vt {
+
+// ------------------------------------------------------------------ sparse gather Jacobi
+__device__ __forceinline__ v3 gather_dir(uint32_t i, uint32_t j, uint32_t K, v3 n) {
+  v3 t, b;
+  basis(n, t, b);
+  const float rot = u01(pcg(i * 2654435761u + 17u));
+  const float rr = sqrtf(((float)j + 0.5f) / (float)K);
+  const float ph = 2.0f * PI_F * kr(j, 0.6180339887498949f, rot);
+  float sn, cs;
+  sincosf(ph, &sn, &cs);
+  return normalize(t * (rr * cs) + b * (rr * sn) + n * sqrtf(fmaxf(0.0f, 1.0f - rr * rr)));
+}
+__device__ __forceinline__ v3 albedo_at(const float4 *alb, uint32_t bins, uint32_t m, float mu) {
+  float x = fminf(fmaxf(mu * (float)bins - 0.5f, 0.0f), (float)(bins - 1));
+  uint32_t i = (uint32_t)floorf(x);
+  uint32_t j = min(bins - 1, i + 1);
+  float f = x - (float)i;
+  const float4 *b = alb + (size_t)m * bins * 2;
+  v3 s0 = ld3(b[2 * i]), s1 = ld3(b[2 * j]), d0 = ld3(b[2 * i + 1]), d1 = ld3(b[2 * j + 1]);
+  return (s0 * (1.0f - f) + s1 * f) + (d0 * (1.0f - f) + d1 * f);
+}
+
+__global__ void k_jacobi_gather(GatherDev G) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G.n_active) return;
+  const uint32_t i = G.active[k];
+  const v3 n = normalize(ld3(G.normals[i]));
+  const uint32_t mi = G.node_mat[i];
+  v3 acc = mk(0.f, 0.f, 0.f);
+  for (uint32_t j = 0; j < G.gather_k; j++) {
+    const uint32_t h = G.gtable[(size_t)i * G.gather_k + j];
+    if (h == MISS) continue;
+    const v3 dir = gather_dir(i, j, G.gather_k, n);
+    const float muIn = sqrtf(fmaxf(0.0f, 1.0f - ((float)j + 0.5f) / (float)G.gather_k));
+    const float muOut = fminf(fmaxf(-dot(dir, normalize(ld3(G.normals[h]))), 0.0f), 1.0f);
+    const v3 Ls = ld3(G.node_state[4 * (size_t)h + G.slot_in]);
+    acc = acc + albedo_at(G.albedo, G.bins, mi, muIn) * albedo_at(G.albedo, G.bins, G.node_mat[h], muOut) * Ls;
+  }
+  const float4 hh = G.hh[mi];
+  const float s = PI_F / (float)G.gather_k;
+  G.aux[k] = make_float4(acc.x * s / fmaxf(hh.x, 1e-4f), acc.y * s / fmaxf(hh.y, 1e-4f), acc.z * s / fmaxf(hh.z, 1e-4f), 0.0f);
+}
+
+__global__ void k_jacobi_update(GatherDev G) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G.n_active) return;
+  const uint32_t i = G.active[k];
+  v3 Ef = mk(0.f, 0.f, 0.f);
+  float ws = 0.0f;
+  for (uint32_t j = 0; j < G.nbr_k; j++) {
+    const float w = G.nbr_w[(size_t)k * G.nbr_k + j];
+    if (w <= 0.0f) break;
+    Ef = Ef + ld3(G.aux[G.nbr[(size_t)k * G.nbr_k + j]]) * w;
+    ws += w;
+  }
+  if (ws > 0.0f) Ef = Ef * (1.0f / ws);
+  else Ef = ld3(G.aux[k]);
+  const float4 st = G.node_state[4 * (size_t)i];
+  const float4 B = G.node_state[4 * (size_t)i + 3];
+  const float4 hh = G.hh[G.node_mat[i]];
+  const v3 h3 = mk(fmaxf(hh.x, 1e-4f), fmaxf(hh.y, 1e-4f), fmaxf(hh.z, 1e-4f));
+  const v3 L = mk((B.x + Ef.x * st.w * h3.x) * INV_PI_F / h3.x, (B.y + Ef.y * st.w * h3.y) * INV_PI_F / h3.y,
+                  (B.z + Ef.z * st.w * h3.z) * INV_PI_F / h3.z);
+  G.node_state[4 * (size_t)i] = make_float4(Ef.x, Ef.y, Ef.z, st.w);
+  G.node_state[4 * (size_t)i + G.slot_out] = make_float4(L.x, L.y, L.z, 1.0f);
+}
+
+hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s) {
+  if (G.n_active == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_gather, dim3((G.n_active + 63) / 64), dim3(64), 0, s, G);
+  return hipGetLastError();
+}
+hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s) {
+  if (G.n_active == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_update, dim3((G.n_active + 63) / 64), dim3(64), 0, s, G);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ dense sweeps
+// y = e + F x over a 128-row x 48-RHS block; K is split KSPLIT ways and the partial sums are added to y
+// (which the host first fills with e).
+constexpr int DM = 128, DK = 32, DR = 48, KSPLIT = 4;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_vec(const float *F, uint32_t n, uint32_t rp, const T *x, T *y) {
+  __shared__ T sx[DK * DR];
+  __shared__ float sf[DM * (DK + 1)];
+  const uint32_t row0 = blockIdx.x * DM;
+  const uint32_t kper = (n + KSPLIT - 1) / KSPLIT;
+  const uint32_t kbeg = blockIdx.y * kper, kend = min(n, kbeg + kper);
+  const int t = threadIdx.x;
+  const int rg = t >> 2, cg = t & 3;        // rows 2*rg, 2*rg+1; RHS 12*cg .. 12*cg+11
+  T acc[2][12];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 12; b++) acc[a][b] = (T)0;
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += DK) {
+    __syncthreads();
+    for (int idx = t; idx < DK * DR; idx += 256) {
+      const uint32_t kk = k0 + idx / DR;
+      const int c = idx % DR;
+      sx[idx] = (kk < kend && c < (int)rp) ? x[(size_t)kk * rp + c] : (T)0;
+    }
+    for (int idx = t; idx < DM * DK; idx += 256) {
+      const int r = idx / DK, kk = idx % DK;
+      const uint32_t gr = row0 + r, gk = k0 + kk;
+      sf[r * (DK + 1) + kk] = (gr < n && gk < kend) ? F[(size_t)gr * n + gk] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < DK; kk++) {
+      const T f0 = (T)sf[(2 * rg) * (DK + 1) + kk], f1 = (T)sf[(2 * rg + 1) * (DK + 1) + kk];
+#pragma unroll
+      for (int b = 0; b < 12; b++) {
+        const T xv = sx[kk * DR + 12 * cg + b];
+        acc[0][b] += f0 * xv;
+        acc[1][b] += f1 * xv;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const uint32_t gr = row0 + 2 * rg + a;
+    if (gr >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 12; b++) {
+      const int c = 12 * cg + b;
+      if (c < (int)rp) atomicAdd(&y[(size_t)gr * rp + c], acc[a][b]);
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_fill_e(const float *e, T *y, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (T)e[i];
+}
+
+hipError_t launch_sweep_fp32(const float *F32, uint32_t n, uint32_t rp, const float *x, const float *e, float *y,
+                             hipStream_t s) {
+  const uint64_t m = (uint64_t)n * rp;
+  hipLaunchKernelGGL(k_fill_e<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
+  hipLaunchKernelGGL(k_sweep_vec<float>, dim3((n + DM - 1) / DM, KSPLIT), dim3(256), 0, s, F32, n, rp, x, y);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ BF16 MFMA sweep
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                              // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+__global__ void k_f32_to_bf16(const float *src, uint16_t *dst, uint64_t n) {
+  uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 3 < n) {
+    float4 v = *reinterpret_cast<const float4 *>(src + i);
+    ushort4 o = make_ushort4(f2bf(v.x), f2bf(v.y), f2bf(v.z), f2bf(v.w));
+    *reinterpret_cast<ushort4 *>(dst + i) = o;
+  } else {
+    for (; i < n; i++) dst[i] = f2bf(src[i]);
+  }
+}
+// x (n x rp, f32) -> xT (rp x n, bf16)
+__global__ void k_x_to_bf16T(const float *x, uint32_t n, uint32_t rp, uint16_t *xT) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)n * rp) return;
+  const uint32_t k = (uint32_t)(i / rp), c = (uint32_t)(i % rp);
+  xT[(size_t)c * n + k] = f2bf(x[i]);
+}
+
+// One wave: 64 rows (four 16-row MFMA tiles) x 48 RHS (three 16-column tiles); a block of 4 waves covers 256
+// rows. K advances 32 per step: every lane loads 16 B of each A row tile and 16 B of each B column tile.
+constexpr int BM_WAVE = 64, BWAVES = 4;
+__global__ __launch_bounds__(256) void k_sweep_bf16(const uint16_t *F16, const uint16_t *xT, uint32_t n,
+                                                    uint32_t rp, float *y) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t row0 = blockIdx.x * (BM_WAVE * BWAVES) + wv * BM_WAVE;
+  const uint32_t kper = (n + KSPLIT - 1) / KSPLIT;
+  const uint32_t kbeg = blockIdx.y * kper, kend = min(n, kbeg + kper);
+  const int li = lane & 15, lk = (lane >> 4) * 8;
+  f32x4 acc[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (uint32_t k0 = kbeg; k0 < kend; k0 += 32) {
+    bf16x8 A[4], B[3];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const uint32_t r = row0 + 16 * a + li;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (r < n && k0 + lk + 8 <= kend) v = *reinterpret_cast<const uint4 *>(F16 + (size_t)r * n + k0 + lk);
+      A[a] = __builtin_bit_cast(bf16x8, v);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const uint32_t c = 16 * b + li;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (c < rp && k0 + lk + 8 <= kend) v = *reinterpret_cast<const uint4 *>(xT + (size_t)c * n + k0 + lk);
+      B[b] = __builtin_bit_cast(bf16x8, v);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[b], acc[a][b], 0, 0, 0);
+  }
+  // D layout: lane holds column (lane & 15), rows 4 * (lane >> 4) + v
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const uint32_t c = 16 * b + li;
+      if (c >= rp) continue;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t r = row0 + 16 * a + 4 * (lane >> 4) + v;
+        if (r < n) atomicAdd(&y[(size_t)r * rp + c], acc[a][b][v]);
+      }
+    }
+}
+
+hipError_t launch_convert_bf16(const float *src, uint16_t *dst, uint64_t n, hipStream_t s) {
+  uint64_t th = (n + 3) / 4;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep_bf16_T(const uint16_t *F16, const uint16_t *xT, uint32_t n, uint32_t rp, const float *e,
+                               float *y, hipStream_t s) {
+  const uint64_t m = (uint64_t)n * rp;
+  hipLaunchKernelGGL(k_fill_e<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
+  hipLaunchKernelGGL(k_sweep_bf16, dim3((n + BM_WAVE * BWAVES - 1) / (BM_WAVE * BWAVES), KSPLIT), dim3(256), 0, s,
+                     F16, xT, n, rp, y);
+  return hipGetLastError();
+}
+hipError_t launch_x_to_bf16T(const float *x, uint32_t n, uint32_t rp, uint16_t *xT, hipStream_t s) {
+  const uint64_t m = (uint64_t)n * rp;
+  hipLaunchKernelGGL(k_x_to_bf16T, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, n, rp, xT);
+  return hipGetLastError();
+}
+
+// FP64 sweep: y = e + F x with F widened from FP32 (exact) and FP64 accumulation
+hipError_t launch_sweep_fp64(const float *F32, uint32_t n, uint32_t rp, const double *x, const float *e, double *y,
+                             hipStream_t s) {
+  const uint64_t m = (uint64_t)n * rp;
+  hipLaunchKernelGGL(k_fill_e<double>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
+  hipLaunchKernelGGL(k_sweep_vec<double>, dim3((n + DM - 1) / DM, KSPLIT), dim3(256), 0, s, F32, n, rp, x, y);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ vector helpers
+// partial sums per block: [0] sum (y - x)^2, [1] sum e^2, [2] sum (x - ref)^2, [3] sum ref^2
+template <typename TX, typename TY>
+__global__ void k_resid(const TY *y, const TX *x, const float *e, const double *ref, uint64_t m, double *part) {
+  __shared__ double s[4][256];
+  double a = 0, b = 0, c = 0, d = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+    const double r = (double)y[i] - (double)x[i];
+    a += r * r;
+    b += (double)e[i] * (double)e[i];
+    if (ref) {
+      const double q = (double)x[i] - ref[i];
+      c += q * q;
+      d += ref[i] * ref[i];
+    }
+  }
+  s[0][threadIdx.x] = a; s[1][threadIdx.x] = b; s[2][threadIdx.x] = c; s[3][threadIdx.x] = d;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int q = 0; q < 4; q++) s[q][threadIdx.x] += s[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = s[q][0];
+}
+constexpr int RES_BLOCKS = 256;
+hipError_t launch_resid_f32(const float *y, const float *x, const float *e, const double *ref, uint64_t m,
+                            double *part, hipStream_t s) {
+  hipLaunchKernelGGL((k_resid<float, float>), dim3(RES_BLOCKS), dim3(256), 0, s, y, x, e, ref, m, part);
+  return hipGetLastError();
+}
+hipError_t launch_resid_f64(const double *y, const double *x, const float *e, const double *ref, uint64_t m,
+                            double *part, hipStream_t s) {
+  hipLaunchKernelGGL((k_resid<double, double>), dim3(RES_BLOCKS), dim3(256), 0, s, y, x, e, ref, m, part);
+  return hipGetLastError();
+}
+// x64 = (double) x32, for the FP64 check of an FP32 solution
+__global__ void k_widen(const float *x, double *y, uint64_t m) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) y[i] = (double)x[i];
+}
+hipError_t launch_widen(const float *x, double *y, uint64_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, y, m);
+  return hipGetLastError();
+}
+// a <- a + b ; c <- b (f32)
+__global__ void k_axpy(float *a, const float *b, uint64_t m) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) a[i] += b[i];
+}
+hipError_t launch_add(float *a, const float *b, uint64_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, a, b, m);
+  return hipGetLastError();
+}
+// r = y - x
+__global__ void k_sub(const float *y, const float *x, float *r, uint64_t m) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) r[i] = y[i] - x[i];
+}
+hipError_t launch_sub(const float *y, const float *x, float *r, uint64_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, y, x, r, m);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ form factors
+// F_ij = V_ij cos_i cos_j A_j / (pi r^2); one lane per (i, j) pair, j fastest. V by an any-hit ray from
+// p_i + 1e-4 n_i toward p_j, ending 1e-4 m short of it.
+__global__ __launch_bounds__(DEFAULT_BLOCK) void k_formfactor(SceneDev S, uint32_t *overflow, uint2 *spill,
+                                                              uint64_t spill_stride, uint32_t n, const float4 *nodes,
+                                                              float *F) {
+  __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
+  __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
+  Stack st{s_node, s_t, spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, spill_stride,
+           (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, overflow};
+  const uint64_t total = (uint64_t)n * n;
+  for (uint64_t q = (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x; q < total;
+       q += (uint64_t)gridDim.x * DEFAULT_BLOCK) {
+    const uint32_t i = (uint32_t)(q / n), j = (uint32_t)(q % n);
+    float f = 0.0f;
+    if (i != j) {
+      const float4 pi = nodes[2 * i], ni = nodes[2 * i + 1], pj = nodes[2 * j], nj = nodes[2 * j + 1];
+      v3 d = ld3(pj) - ld3(pi);
+      const float r2 = dot(d, d);
+      const float r = sqrtf(r2);
+      d = d * (1.0f / r);
+      const float ci = dot(ld3(ni), d), cj = -dot(ld3(nj), d);
+      if (ci > 0.0f && cj > 0.0f && r > 2e-4f) {
+        const v3 o = ld3(pi) + ld3(ni) * 1e-4f;
+        v3 dd = ld3(pj) - o;
+        const float rr = sqrtf(dot(dd, dd));
+        dd = dd * (1.0f / rr);
+        Hit h;
+        if (!traverse<true>(S, o, dd, 0.0f, rr - 1e-4f, h, st)) f = ci * cj * pj.w / (PI_F * r2);
+      }
+    }
+    F[q] = f;
+  }
+}
+hipError_t launch_formfactors(const SceneDev &S, uint32_t *overflow, uint2 *spill, uint64_t spill_stride,
+                              uint32_t n, const float4 *nodes, float *F, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)(spill_stride / DEFAULT_BLOCK);
+  hipLaunchKernelGGL(k_formfactor, dim3(blocks), dim3(DEFAULT_BLOCK), 0, s, S, overflow, spill, spill_stride, n,
+                     nodes, F);
+  return hipGetLastError();
+}
+// F_i* *= row_max / max(1, sum_j F_ij): row sums at most row_max
+__global__ void k_row_scale(float *F, uint32_t n, float row_max) {
+  __shared__ double s[256];
+  const uint32_t i = blockIdx.x;
+  double a = 0;
+  for (uint32_t j = threadIdx.x; j < n; j += 256) a += F[(size_t)i * n + j];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  const float sc = (float)((double)row_max / fmax(1.0, s[0]));
+  for (uint32_t j = threadIdx.x; j < n; j += 256) F[(size_t)i * n + j] *= sc;
+}
+hipError_t launch_row_scale(float *F, uint32_t n, float row_max, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_scale, dim3(n), dim3(256), 0, s, F, n, row_max);
+  return hipGetLastError();
+}
+
+}  // namespace vt

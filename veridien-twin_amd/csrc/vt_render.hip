@@ -1,0 +1,313 @@
+// Camera path tracing on gfx950: one lane per (pixel, sub-sample) item, 64 items per wave, items of every
+// camera of a frame in one queue. The persistent form takes waves from the queue with an atomic head (a
+// frame of uneven cameras is one launch); the plain form maps one wave to each 64 items of one camera.
+// A wave's lanes hold consecutive sub-samples of the same pixel(s), so primary rays are coherent and the
+// pixel sum is a butterfly reduction within the wave (deterministic; one writer per pixel).
+#include "vt_internal.h"
+
+namespace vt {
+
+// The camera estimator (the reference's pieces composed into paths of `depth` vertices, as the oracle states):
+// NEE with the power heuristic at every vertex but the last, NEE alone at the last, emitter hits by BSDF rays
+// MIS-weighted (gpu/wgsl/shade.wgsl.js:135-188, 224-236), Kronecker dimensions at the first vertex
+// (gpu/wgsl/kernels.wgsl.js:42-53) and counter hashes after it.
+__device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
+                                            uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
+  v3 o = mk(C.pos[0], C.pos[1], C.pos[2]);
+  v3 d = camera_dir(C, P, x, y, i);
+  v3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
+  float pB = 0.0f;
+  const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+  const float fnl = (float)nl;
+  const uint32_t D = C.depth < 1 ? 1 : C.depth;
+  const float A = S.emit_area;
+  for (uint32_t b = 0; b < D; b++) {
+    Hit h;
+    nrays++;
+    if (!traverse<false>(S, o, d, 0.0f, INFINITY, h, st)) break;
+    const uint4 at = S.attr[h.tri];
+    const uint4 ms = S.meshes[at.w];
+    const float4 *tp = S.tri + 3 * (size_t)h.tri;
+    const v3 p0 = ld3(tp[0]);
+    v3 ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
+    const float w0 = 1.0f - h.u - h.v;
+    const v3 nsi = normalize(ld3(S.vnorm[at.x]) * w0 + ld3(S.vnorm[at.y]) * h.u + ld3(S.vnorm[at.z]) * h.v);
+    v3 ns = nsi;
+    if (dot(ng, d) > 0.0f) ng = -ng;
+    if (dot(ns, ng) < 0.0f) ns = -ns;
+    if (dot(ns, -d) < 1e-4f) ns = ng;
+    const v3 p = o + d * h.t;
+    if (ms.y & F_EMITTER) {
+      const float cosE = -dot(d, nsi);
+      if (cosE > 0.0f) {
+        const v3 Le = ld3(S.lights[ms.z]);
+        float w = 1.0f;
+        if (b > 0) {
+          const float pL = h.t * h.t / (cosE * A);
+          w = pB * pB / (pB * pB + fnl * fnl * pL * pL);
+        }
+        L = L + thr * Le * w;
+      }
+      break;
+    }
+    const Mat M = load_mat(S.mats, ms.x);
+    const v3 wo = -d;
+    const bool last = b + 1 == D;
+    v3 Ln = mk(0.f, 0.f, 0.f);
+    if (S.emit_count > 0) {
+      for (uint32_t j = 0; j < nl; j++) {
+        float u0, u1, u2;
+        if (b == 0) {
+          const uint32_t li = i * nl + j;
+          u0 = kr(li, KL0, P.s[3]); u1 = kr(li, KL1, P.s[4]); u2 = kr(li, KL2, P.s[5]);
+        } else {
+          u0 = u01(hash3(P.hs, i, 256u * b + 3u * j));
+          u1 = u01(hash3(P.hs, i, 256u * b + 3u * j + 1u));
+          u2 = u01(hash3(P.hs, i, 256u * b + 3u * j + 2u));
+        }
+        v3 pl;
+        const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+        v3 wi = pl - p;
+        const float d2 = dot(wi, wi), dist = sqrtf(d2);
+        wi = wi * (1.0f / dist);
+        const float4 en = S.emit[4 * e + 3];
+        const float cosE = -dot(wi, ld3(en));
+        if (cosE <= 0.0f) continue;
+        const float cosS = dot(ns, wi);
+        if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
+        const v3 Le = ld3(S.lights[__float_as_uint(S.emit[4 * e + 2].w)]);
+        if (maxc(Le) <= 0.0f) continue;
+        const v3 f = eval_bsdf(M, ns, wi, wo);
+        if (maxc(f) <= 0.0f) continue;
+        const v3 so = p + ng * (dot(ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
+        Hit sh;
+        nrays++;
+        if (traverse<true>(S, so, wi, 0.0f, dist - 2e-5f, sh, st)) continue;
+        const float pW = d2 / (cosE * A);
+        float w = 1.0f;
+        if (!last) {
+          const float pb = pdf_bsdf(M, ns, wo, wi);
+          w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
+        }
+        Ln = Ln + f * Le * (cosS * w / pW);
+      }
+    }
+    L = L + thr * Ln * (1.0f / fnl);
+    if (last) break;
+    float u0, u1, u2;
+    if (b == 0) {
+      u0 = kr(i, KB0, P.s[6]); u1 = kr(i, KB1, P.s[7]); u2 = kr(i, KB2, P.s[8]);
+    } else {
+      u0 = u01(hash3(P.hs, i, 256u * b + 200u));
+      u1 = u01(hash3(P.hs, i, 256u * b + 201u));
+      u2 = u01(hash3(P.hs, i, 256u * b + 202u));
+    }
+    v3 l;
+    const float pdf = sample_bsdf(M, ns, wo, u0, u1, u2, l);
+    if (!(pdf > 0.0f) || dot(ng, l) <= 0.0f) break;
+    const v3 f = eval_bsdf(M, ns, l, wo);
+    thr = thr * f * (dot(ns, l) / pdf);
+    if (maxc(thr) <= 0.0f) break;
+    o = p + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
+    d = l;
+    pB = pdf;
+  }
+  return L;
+}
+
+__device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
+  for (uint32_t off = 1; off < width; off <<= 1) v += __shfl_xor(v, (int)off, 64);
+  return v;
+}
+
+// one 64-item wave of the queue
+__device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, Stack &st) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t g0 = Lc.item_base + wave * 64ull;
+  // the wave's camera: item ranges are multiples of 64, so every lane shares it
+  uint32_t c = 0;
+  while (c + 1 < Lc.ncam && g0 >= Lc.cams[c + 1].item_begin) c++;
+  const DevCam &C = Lc.cams[c];
+  const uint64_t q = g0 - C.item_begin + lane;
+  const uint32_t nspp = C.nspp;
+  const uint64_t pix = q / nspp;
+  const uint32_t i = C.spp_begin + (uint32_t)(q - pix * nspp);
+  const uint32_t t = (uint32_t)(pix >> 6), r = (uint32_t)(pix & 63u);
+  const uint32_t x = (t % C.tiles_x) * 8u + (r & 7u), y = (t / C.tiles_x) * 8u + (r >> 3);
+  const bool valid = x < C.width && y < C.height;
+  v3 L = mk(0.f, 0.f, 0.f);
+  uint32_t nrays = 0;
+  if (valid) {
+    PixSeeds P;
+    pix_seeds(C, x, y, P);
+    L = path_radiance(Lc.S, C, P, x, y, i, st, nrays);
+  }
+  // per-pixel sums: aligned power-of-two runs reduce by butterfly; other run shapes add atomically
+  const bool pow2 = (nspp & (nspp - 1)) == 0 && nspp <= 64;
+  if (pow2) {
+    float sx = wave_sum_xor(L.x, nspp), sy = wave_sum_xor(L.y, nspp), sz = wave_sum_xor(L.z, nspp);
+    if (valid && (lane & (nspp - 1)) == 0) {
+      float4 *a = C.acc + (size_t)y * C.width + x;
+      float4 v = *a;
+      v.x += sx; v.y += sy; v.z += sz; v.w += (float)nspp;
+      *a = v;
+    }
+  } else if (valid) {
+    float4 *a = C.acc + (size_t)y * C.width + x;
+    atomicAdd(&a->x, L.x);
+    atomicAdd(&a->y, L.y);
+    atomicAdd(&a->z, L.z);
+    atomicAdd(&a->w, 1.0f);
+  }
+  // rays and samples traced by the wave
+  uint32_t rs = nrays;
+  for (int off = 32; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
+  uint32_t ns = valid ? 1u : 0u;
+  for (int off = 32; off >= 1; off >>= 1) ns += __shfl_xor(ns, off, 64);
+  if (lane == 0) {
+    atomicAdd(&Lc.counters[1], (unsigned long long)rs);
+    atomicAdd(&Lc.counters[2], (unsigned long long)ns);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
+  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
+  __shared__ float s_t[STACK_LDS * BLOCK];
+  Stack st;
+  st.lds_node = s_node;
+  st.lds_t = s_t;
+  st.spill = Lc.spill;
+  st.tid = threadIdx.x;
+  st.block = BLOCK;
+  st.slds = STACK_LDS;
+  st.spill_depth = STACK_SPILL;
+  st.spill_stride = Lc.spill_stride;
+  st.gtid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  st.overflow = Lc.overflow;
+  const uint32_t wib = threadIdx.x >> 6;
+  if (Lc.persistent) {
+    for (;;) {
+      unsigned long long base = 0;
+      if ((threadIdx.x & 63u) == 0) base = atomicAdd(&Lc.counters[0], (unsigned long long)Lc.chunk);
+      base = __shfl(base, 0, 64);
+      if (base >= Lc.total_waves) break;
+      const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
+      for (uint64_t w = base; w < end; w++) run_wave(Lc, w, st);
+    }
+  } else {
+    // static schedule: the waves of the grid stride over the queue
+    const uint64_t stride = (uint64_t)gridDim.x * (BLOCK / 64);
+    for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride) run_wave(Lc, w, st);
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+__global__ void k_zero_f4(float4 *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_f4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
+  switch (L.block) {
+    case 64: hipLaunchKernelGGL(k_render<64>, dim3(L.grid), dim3(64), 0, s, L); break;
+    case 128: hipLaunchKernelGGL(k_render<128>, dim3(L.grid), dim3(128), 0, s, L); break;
+    case 256: hipLaunchKernelGGL(k_render<256>, dim3(L.grid), dim3(256), 0, s, L); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ test kernels
+__global__ __launch_bounds__(DEFAULT_BLOCK) void k_trace_test(TestLaunch Lt, const float *rays, uint32_t n,
+                                                              int any_hit, float *hits) {
+  __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
+  __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
+  Stack st{s_node, s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
+           (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, Lt.overflow};
+  for (uint32_t k = blockIdx.x * DEFAULT_BLOCK + threadIdx.x; k < n; k += gridDim.x * DEFAULT_BLOCK) {
+    const float *r = rays + 8 * (size_t)k;
+    Hit h;
+    bool hit = any_hit ? traverse<true>(Lt.S, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st)
+                       : traverse<false>(Lt.S, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st);
+    float *o = hits + 4 * (size_t)k;
+    o[0] = hit ? h.t : INFINITY;
+    o[1] = __uint_as_float(hit ? h.tri : MISS);
+    o[2] = hit ? h.u : 0.0f;
+    o[3] = hit ? h.v : 0.0f;
+  }
+}
+hipError_t launch_trace_test(const TestLaunch &L, const float *rays, uint32_t n, int any_hit, float *hits,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint32_t blocks = (n + DEFAULT_BLOCK - 1) / DEFAULT_BLOCK;
+  if (blocks > L.max_blocks) blocks = L.max_blocks;
+  hipLaunchKernelGGL(k_trace_test, dim3(blocks), dim3(DEFAULT_BLOCK), 0, s, L, rays, n, any_hit, hits);
+  return hipGetLastError();
+}
+
+__global__ void k_lens_test(const float *lens9, const float *uv, uint32_t n, float *rays) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float L[9];
+  for (int j = 0; j < 9; j++) L[j] = lens9[j];
+  v3 r = kb4_unproject(L, uv[2 * k], uv[2 * k + 1]);
+  rays[3 * k] = r.x; rays[3 * k + 1] = r.y; rays[3 * k + 2] = r.z;
+}
+hipError_t launch_lens_test(const float *lens9, const float *uv, uint32_t n, float *rays, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lens_test, dim3((n + 255) / 256), dim3(256), 0, s, lens9, uv, n, rays);
+  return hipGetLastError();
+}
+
+__global__ void k_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const DevCam &C = *cam;
+  PixSeeds P;
+  pix_seeds(C, xyi[3 * k], xyi[3 * k + 1], P);
+  v3 d = camera_dir(C, P, xyi[3 * k], xyi[3 * k + 1], xyi[3 * k + 2]);
+  float *o = out + 6 * (size_t)k;
+  o[0] = C.pos[0]; o[1] = C.pos[1]; o[2] = C.pos[2];
+  o[3] = d.x; o[4] = d.y; o[5] = d.z;
+}
+hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_camray_test, dim3((n + 255) / 256), dim3(256), 0, s, cam, xyi, n, out);
+  return hipGetLastError();
+}
+
+__global__ void k_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float *c = cases + 16 * (size_t)k;
+  v3 nn = mk(c[0], c[1], c[2]), l = mk(c[4], c[5], c[6]), v = mk(c[8], c[9], c[10]);
+  const Mat M = load_mat(mats, (uint32_t)c[15]);
+  v3 f = eval_bsdf(M, nn, l, v);
+  float pdf = pdf_bsdf(M, nn, v, l);
+  v3 ls = mk(0.f, 0.f, 0.f);
+  float ps = sample_bsdf(M, nn, v, c[12], c[13], c[14], ls);
+  float *o = out + 8 * (size_t)k;
+  o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = pdf;
+  o[4] = ps > 0.0f ? ls.x : 0.0f; o[5] = ps > 0.0f ? ls.y : 0.0f; o[6] = ps > 0.0f ? ls.z : 0.0f; o[7] = ps;
+}
+hipError_t launch_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bsdf_test, dim3((n + 255) / 256), dim3(256), 0, s, mats, cases, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace vt

@@ -1,0 +1,296 @@
+"""ctypes binding of libvt.so (include/vt.h). Raises when the HIP library is missing: there is no CPU fallback."""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(PKG, "libvt.so")
+
+VT_OK, VT_ERR_ARG, VT_ERR_SIZE, VT_ERR_HIP, VT_ERR_LOST, VT_ERR_STATE, VT_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
+VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA = 0, 1
+VT_KIND_LINEAR, VT_KIND_RAW, VT_KIND_RI = 0, 1, 2
+VT_DENSE_FP32, VT_DENSE_BF16_REFINE, VT_DENSE_FP64 = 0, 1, 2
+
+EXPORTS = [
+    "vt_create", "vt_destroy", "vt_last_error", "vt_build_bvh", "vt_upload_scene", "vt_set_camera", "vt_render",
+    "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
+    "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
+    "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F",
+]
+
+vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
+
+
+class CreateInfo(C.Structure):
+    _fields_ = [("device", C.c_int), ("flags", u32)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("nodes", vp), ("node_count", u32), ("tris", vp), ("tri_count", u32), ("verts", vp),
+                ("vert_count", u32), ("meshes", vp), ("mesh_count", u32), ("materials", vp),
+                ("material_count", u32), ("lights", vp), ("light_count", u32), ("emit", vp), ("emit_count", u32),
+                ("emit_area", f32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("R", f32 * 9), ("pos", f32 * 3), ("mode", u32), ("lens", f32 * 9), ("width", u32), ("height", u32),
+                ("spp", u32), ("seed", u32), ("cam_index", u32), ("light_samples", u32), ("max_depth", u32),
+                ("flags", u32), ("scale", f32)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("mode", u32), ("spp_begin", u32), ("spp_end", u32), ("accumulate", u32), ("chunk", u32),
+                ("block", u32)]
+
+
+class SensorParams(C.Structure):
+    _fields_ = [("cfa", u32), ("frame", u32), ("seed", u32), ("optics_k", f32), ("esf_sigma", f32),
+                ("exposure", f32), ("gain", f32), ("m", f32 * 9), ("read_noise", f32), ("full_well", f32),
+                ("dark", f32), ("conv_gain", f32), ("max_dn", f32), ("black", f32), ("noise_on", u32),
+                ("pupil_exp", f32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("rays", f64), ("samples", f64), ("ms_render", f64), ("ms_sensor", f64), ("ms_kernel_avg", f64),
+                ("launches", u32), ("stack_overflows", u32)]
+
+
+class GatherTables(C.Structure):
+    _fields_ = [("n_static", u32), ("n_active", u32), ("gather_k", u32), ("nbr_k", u32), ("n_mat", u32),
+                ("albedo_bins", u32), ("active", vp), ("gtable", vp), ("normals", vp), ("node_mat", vp),
+                ("albedo", vp), ("hh", vp), ("nbr", vp), ("nbr_w", vp)]
+
+
+class DenseOpts(C.Structure):
+    _fields_ = [("n", u32), ("r", u32), ("method", u32), ("tol", f64), ("max_sweeps", u32), ("inner_sweeps", u32)]
+
+
+class DenseResult(C.Structure):
+    _fields_ = [("residual", f64), ("sweeps", u32), ("refinements", u32), ("ms", f64), ("ms_per_sweep", f64)]
+
+
+_LIB = None
+
+
+def load():
+    """Load libvt.so; raises if it has not been built (HIP library required, no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("libvt.so is not built (run __graft_entry__.build()); the HIP path has no fallback")
+    L = C.CDLL(LIBPATH)
+    sig = {
+        "vt_create": (i32, [vp, vp]),
+        "vt_destroy": (None, [vp]),
+        "vt_last_error": (C.c_char_p, [vp]),
+        "vt_build_bvh": (i32, [vp, u32, vp, u32, i32, i32, vp, u32, vp, vp]),
+        "vt_upload_scene": (i32, [vp, vp]),
+        "vt_set_camera": (i32, [vp, u32, vp]),
+        "vt_render": (i32, [vp, vp, u32, vp]),
+        "vt_sensor": (i32, [vp, u32, vp]),
+        "vt_readback": (i32, [vp, u32, i32, vp, sz]),
+        "vt_sync": (i32, [vp]),
+        "vt_frame_stats_get": (i32, [vp, vp]),
+        "vt_trace_test": (i32, [vp, vp, u32, i32, vp]),
+        "vt_lens_test": (i32, [vp, vp, vp, u32, vp]),
+        "vt_camera_ray_test": (i32, [vp, u32, vp, u32, vp]),
+        "vt_bsdf_test": (i32, [vp, vp, u32, vp]),
+        "vt_sensor_test": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "vt_jacobi_sparse": (i32, [vp, vp, vp, u32, vp]),
+        "vt_dense_upload": (i32, [vp, u32, u32, vp, vp]),
+        "vt_dense_formfactors": (i32, [vp, u32, vp, f32, u32, vp]),
+        "vt_dense_solve": (i32, [vp, vp, vp, vp]),
+        "vt_dense_download_F": (i32, [vp, vp, sz]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class VTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("vt error %d: %s" % (code, msg))
+        self.code = code
+
+
+def build_bvh(pos, idx, leaf_size=4, bins=16):
+    """vt_build_bvh: the native binned-SAH BVH2 in the reference layout."""
+    L = load()
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    n = idx.size // 3
+    cap = n + 2
+    nodes = np.zeros(cap * 16, np.float32)
+    order = np.zeros(max(n, 1), np.uint32)
+    cnt = C.c_uint32(0)
+    rc = L.vt_build_bvh(ptr(pos), pos.size // 3, ptr(idx), n, leaf_size, bins, ptr(nodes), cap, C.byref(cnt), ptr(order))
+    if rc:
+        raise VTError(rc, "vt_build_bvh")
+    return nodes[: cnt.value * 16].copy(), order[:n].copy()
+
+
+class Context:
+    """One vt_ctx (one device, one stream)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        info = CreateInfo(device, 0)
+        rc = self.L.vt_create(C.byref(info), C.byref(h))
+        if rc:
+            raise VTError(rc, "vt_create failed (no HIP device?)")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            self.L.vt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc:
+            raise VTError(rc, self.L.vt_last_error(self.h).decode())
+
+    # ---------------------------------------------------------------- scene / cameras
+    def upload_scene(self, sc, with_nodes=True):
+        a = {k: np.ascontiguousarray(sc[k]) for k in ("tris", "verts", "meshes", "materials", "lights", "emit")}
+        nodes = np.ascontiguousarray(sc["nodes"]) if with_nodes else None
+        s = Scene(ptr(nodes), 0 if nodes is None else nodes.size // 16, ptr(a["tris"]), a["tris"].size // 16,
+                  ptr(a["verts"]), a["verts"].size // 8, ptr(a["meshes"]), a["meshes"].size // 4,
+                  ptr(a["materials"]), a["materials"].size // 16, ptr(a["lights"]), a["lights"].size // 4,
+                  ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]))
+        self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
+
+    def set_camera(self, slot, cam):
+        c = Camera()
+        c.R[:] = [float(x) for x in cam["R"]]
+        c.pos[:] = [float(x) for x in cam["pos"]]
+        c.mode = int(cam.get("mode", 0))
+        c.lens[:] = [float(x) for x in cam["lens"]]
+        for k in ("width", "height", "spp", "seed", "cam_index", "light_samples", "max_depth", "flags"):
+            setattr(c, k, int(cam.get(k, 0)))
+        c.scale = float(cam.get("scale", 1.0))
+        self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
+
+    def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0):
+        arr = np.ascontiguousarray(slots, dtype=np.uint32)
+        o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block)
+        self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
+
+    def sensor(self, slot, params):
+        p = sensor_struct(params)
+        self.check(self.L.vt_sensor(self.h, slot, C.byref(p)))
+
+    def readback(self, slot, kind, w, h):
+        if kind == VT_KIND_LINEAR:
+            out = np.zeros((h, w, 4), np.float32)
+        elif kind == VT_KIND_RAW:
+            out = np.zeros((h, w), np.uint16)
+        else:
+            out = np.zeros((h, w), np.float32)
+        self.check(self.L.vt_readback(self.h, slot, kind, ptr(out), out.nbytes))
+        return out
+
+    def sync(self):
+        self.check(self.L.vt_sync(self.h))
+
+    def stats(self):
+        s = FrameStats()
+        self.check(self.L.vt_frame_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in FrameStats._fields_}
+
+    # ---------------------------------------------------------------- test entry points
+    def trace(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        self.check(self.L.vt_trace_test(self.h, ptr(rays), rays.shape[0], int(any_hit), ptr(out)))
+        return out
+
+    def lens_test(self, lens9, uv):
+        lens9 = np.ascontiguousarray(lens9, dtype=np.float32)
+        uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 3), np.float32)
+        self.check(self.L.vt_lens_test(self.h, ptr(lens9), ptr(uv), uv.shape[0], ptr(out)))
+        return out
+
+    def camera_rays(self, slot, xyi):
+        xyi = np.ascontiguousarray(xyi, dtype=np.uint32).reshape(-1, 3)
+        out = np.zeros((xyi.shape[0], 6), np.float32)
+        self.check(self.L.vt_camera_ray_test(self.h, slot, ptr(xyi), xyi.shape[0], ptr(out)))
+        return out
+
+    def bsdf_test(self, cases):
+        cases = np.ascontiguousarray(cases, dtype=np.float32).reshape(-1, 16)
+        out = np.zeros((cases.shape[0], 8), np.float32)
+        self.check(self.L.vt_bsdf_test(self.h, ptr(cases), cases.shape[0], ptr(out)))
+        return out
+
+    def sensor_test(self, params, radiance, ri):
+        radiance = np.ascontiguousarray(radiance, dtype=np.float32)
+        ri = np.ascontiguousarray(ri, dtype=np.float32)
+        h, w = ri.shape
+        dn = np.zeros((h, w), np.uint16)
+        p = sensor_struct(params)
+        self.check(self.L.vt_sensor_test(self.h, C.byref(p), w, h, ptr(radiance), ptr(ri), ptr(dn)))
+        return dn
+
+    # ---------------------------------------------------------------- radiosity
+    def jacobi_sparse(self, t, node_state, iters):
+        keep = {k: np.ascontiguousarray(v) for k, v in t.items() if isinstance(v, np.ndarray)}
+        g = GatherTables(t["n_static"], t["n_active"], t["gather_k"], t["nbr_k"], t["n_mat"], t["albedo_bins"],
+                         ptr(keep["active"]), ptr(keep["gtable"]), ptr(keep["normals"]), ptr(keep["node_mat"]),
+                         ptr(keep["albedo"]), ptr(keep["hh"]), ptr(keep.get("nbr")), ptr(keep.get("nbr_w")))
+        st = np.ascontiguousarray(node_state, dtype=np.float32).copy()
+        aux = np.zeros((t["n_active"], 4), np.float32)
+        self.check(self.L.vt_jacobi_sparse(self.h, C.byref(g), ptr(st), iters, ptr(aux)))
+        return st, aux
+
+    def dense_upload(self, F, e):
+        F = np.ascontiguousarray(F, dtype=np.float32)
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        n, r = e.shape
+        self.check(self.L.vt_dense_upload(self.h, n, r, ptr(F), ptr(e)))
+
+    def dense_formfactors(self, nodes, row_max, e):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float32).reshape(-1, 8)
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        self.check(self.L.vt_dense_formfactors(self.h, nodes.shape[0], ptr(nodes), float(row_max), e.shape[1], ptr(e)))
+
+    def dense_solve(self, n, r, method, tol=1e-6, max_sweeps=2000, inner_sweeps=0, want_x=True):
+        o = DenseOpts(n, r, method, tol, max_sweeps, inner_sweeps)
+        res = DenseResult()
+        x = np.zeros((n, r), np.float32) if want_x else None
+        self.check(self.L.vt_dense_solve(self.h, C.byref(o), ptr(x), C.byref(res)))
+        return x, {k: getattr(res, k) for k, _ in DenseResult._fields_}
+
+    def dense_F(self, n):
+        F = np.zeros((n, n), np.float32)
+        self.check(self.L.vt_dense_download_F(self.h, ptr(F), F.nbytes))
+        return F
+
+
+def sensor_struct(p):
+    s = SensorParams()
+    s.cfa, s.frame, s.seed = int(p["cfa"]), int(p["frame"]), int(p["seed"])
+    s.optics_k, s.esf_sigma, s.exposure, s.gain = p["optics_k"], p["esf_sigma"], p["exposure"], p["gain"]
+    s.m[:] = [float(x) for x in np.asarray(p["m"]).reshape(-1)]
+    s.read_noise, s.full_well, s.dark, s.conv_gain = p["read_noise"], p["full_well"], p["dark"], p["conv_gain"]
+    s.max_dn, s.black = p["max_dn"], p["black"]
+    s.noise_on = 1 if p["noise_on"] else 0
+    s.pupil_exp = float(p.get("pupil_exp", 1.0))
+    return s

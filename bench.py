@@ -70,6 +70,86 @@ def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
                 rays=rays, node_visits_per_ray=float(vis[0]) / max(rays, 1), tri_tests_per_ray=float(vis[1]) / max(rays, 1))
 
 
+def bench_c3(args):
+    """Batching: the full-scale scene with 24 uneven cameras, one persistent launch vs one launch per camera."""
+    import numpy as np
+    from vtwin import lib, scenes
+
+    sc, cams = scenes.c3()
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    for k, c in enumerate(cams):
+        ctx.set_camera(k, c)
+    slots = list(range(len(cams)))
+    res = {}
+    for name, mode in (("per_camera", lib.VT_RENDER_PER_CAMERA), ("batched", lib.VT_RENDER_BATCHED)):
+        ctx.render(slots, mode=mode)
+        ctx.sync()
+        t1 = time.perf_counter()
+        rays = 0.0
+        for _ in range(args.steps):
+            ctx.render(slots, mode=mode)
+            st = ctx.stats()
+            rays += st["rays"]
+        t2 = time.perf_counter()
+        res[name] = dict(ms_per_frame=1e3 * (t2 - t1) / args.steps, mrays=rays / (t2 - t1) / 1e6, launches=st["launches"])
+        log("%s: %.1f ms/frame, %.1f Mrays/s" % (name, res[name]["ms_per_frame"], res[name]["mrays"]))
+    # identical images from both forms
+    ctx.render(slots, mode=lib.VT_RENDER_BATCHED)
+    a = ctx.readback(5, lib.VT_KIND_LINEAR, cams[5]["width"], cams[5]["height"])
+    ctx.render(slots, mode=lib.VT_RENDER_PER_CAMERA)
+    b = ctx.readback(5, lib.VT_KIND_LINEAR, cams[5]["width"], cams[5]["height"])
+    spp_px = sum(c["spp"] * c["width"] * c["height"] for c in cams)
+    out = {"metric": METRIC, "config": {"workload": "batching: 24 KB4 cameras 320x240..2048x1536, spp in {1,4,16,64}, depth 5, "
+                                                     "2,013,432-tri scene"},
+           "value": round(res["batched"]["mrays"], 2), "unit": "Mrays/s", "steps": args.steps,
+           "batched_ms_per_frame": round(res["batched"]["ms_per_frame"], 2),
+           "per_camera_ms_per_frame": round(res["per_camera"]["ms_per_frame"], 2),
+           "speedup_batched_over_per_camera": round(res["per_camera"]["ms_per_frame"] / res["batched"]["ms_per_frame"], 3),
+           "spp_mpix_per_s": round(spp_px / (res["batched"]["ms_per_frame"] * 1e-3) / 1e6, 2),
+           "images_identical": bool(np.array_equal(a, b)), "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_c4(args):
+    """Dense node radiosity: 32,768 dome nodes, 48 RHS, Jacobi to 1e-6; FP32 vs BF16 MFMA + FP32 refinement vs FP64."""
+    import numpy as np
+    from vtwin import lib, scenes
+
+    nodes, sc = scenes.c4_nodes()
+    n = nodes.shape[0]
+    e = scenes.c4_emission(n)
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    t0 = time.perf_counter()
+    ctx.dense_formfactors(nodes, 0.9, e)
+    ctx.sync()
+    t_ff = time.perf_counter() - t0
+    log("form factors %d x %d in %.2f s" % (n, n, t_ff))
+    out = {"metric": METRIC, "config": {"workload": "dense node radiosity: N=32768 dome nodes (visibility through the table box), "
+                                                     "3 channels x 16 emitter configs = 48 RHS, Jacobi to 1e-6"},
+           "form_factor_s": round(t_ff, 3), "form_factor_rays": n * n}
+    sols = {}
+    for name, method, inner in (("fp64", lib.VT_DENSE_FP64, 0), ("fp32", lib.VT_DENSE_FP32, 0),
+                                ("bf16_refine", lib.VT_DENSE_BF16_REFINE, args.inner)):
+        ctx.dense_solve(n, e.shape[1], method, tol=1e-6, max_sweeps=600, inner_sweeps=inner, want_x=False)  # warm
+        x, r = ctx.dense_solve(n, e.shape[1], method, tol=1e-6, max_sweeps=600, inner_sweeps=inner)
+        sols[name] = x
+        out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        log("%s: %s" % (name, out[name]))
+    ref = sols["fp64"].astype(np.float64)
+    for name in ("fp32", "bf16_refine"):
+        out[name]["rel_err_vs_fp64"] = float(np.linalg.norm(sols[name] - ref) / np.linalg.norm(ref))
+    out["speedup_bf16_over_fp32"] = round(out["fp32"]["ms"] / out["bf16_refine"]["ms"], 3)
+    sweep_bytes_fp32 = 4.0 * n * n + 2 * 4 * 48 * n
+    out["fp32_sweep_GBs"] = round(sweep_bytes_fp32 / (out["fp32"]["ms_per_sweep"] * 1e-3) / 1e9, 1)
+    out["value"] = out["speedup_bf16_over_fp32"]
+    out["unit"] = "x (BF16+refinement vs FP32 solve time)"
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +160,18 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
+                    help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity")
+    ap.add_argument("--inner", type=int, default=32, help="BF16 sweeps per refinement step (c4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box); run --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.config == "c3":
+        return bench_c3(args)
+    if args.config == "c4":
+        return bench_c4(args)
     import numpy as np
     from vtwin import lib, scenes
 

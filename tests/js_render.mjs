@@ -1,0 +1,28 @@
+// Test driver for the JavaScript TwinRenderer: renders a scene written by tests/test_js_api.py and writes the
+// linear readback (and the raw DN) next to it. Usage: node tests/js_render.mjs <dir>
+import fs from 'fs';
+import path from 'path';
+
+const dir = process.argv[2];
+const man = JSON.parse(fs.readFileSync(path.join(dir, 'manifest.json'), 'utf8'));
+const f32 = (name) => { const b = fs.readFileSync(path.join(dir, name + '.bin')); return new Float32Array(b.buffer, b.byteOffset, b.byteLength / 4); };
+const u32 = (name) => { const b = fs.readFileSync(path.join(dir, name + '.bin')); return new Uint32Array(b.buffer, b.byteOffset, b.byteLength / 4); };
+
+async function main() {
+  const { TwinRenderer } = await import(path.resolve(man.module));
+  const scene = { nodes: f32('nodes'), tris: f32('tris'), verts: f32('verts'), meshes: u32('meshes'), materials: f32('materials'),
+    lights: f32('lights'), emit: f32('emit'), emitArea: man.emitArea };
+  const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed });
+  const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0 } });
+  await out.done;
+  for (const c of man.cameras) {
+    const lin = await r.readback(c.name, 'linear');
+    fs.writeFileSync(path.join(dir, c.name + '.linear.bin'), Buffer.from(lin.buffer, lin.byteOffset, lin.byteLength));
+    const raw = await r.readback(c.name, 'raw');
+    fs.writeFileSync(path.join(dir, c.name + '.raw.bin'), Buffer.from(raw.buffer, raw.byteOffset, raw.byteLength));
+  }
+  const t = await r.timings();
+  console.log(JSON.stringify({ rays: t.last.rays, overflows: t.last.stackOverflows }));
+  r.destroy();
+}
+main().catch((e) => { console.error(e); process.exit(1); });

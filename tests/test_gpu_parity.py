@@ -127,7 +127,9 @@ def test_kb4_unproject_matches_fp64(ctx, c1):
         r = np.zeros(3)
         oracle.lib().or_unproject(oracle.C.byref(L), oracle.ptr(uv[k].astype(np.float64)), oracle.ptr(r))
         want[k] = r
-    ang = np.arccos(np.clip(np.sum(got * want, 1) / np.linalg.norm(got, axis=1), -1, 1))
+    g = got.astype(np.float64)
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    ang = 2 * np.arcsin(np.linalg.norm(g - want, axis=1) / 2)      # well conditioned near 0, unlike acos(dot)
     assert ang.max() < 2e-6
 
 

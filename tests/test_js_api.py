@@ -1,0 +1,102 @@
+"""The JavaScript host API (js/renderer.mjs over the N-API addon): module shape and FP64 projection on the CPU;
+on the GPU, a frame rendered through TwinRenderer equals the same frame through the C ABI from Python."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import lib, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODULE = os.path.join(ROOT, "veridien-twin_amd", "js", "renderer.mjs")
+NODE = shutil.which("node")
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(os.path.join(ROOT, "veridien-twin_amd", "napi", "vt_napi.node")),
+                                reason="node or the N-API addon is not available")
+
+
+def _node(code):
+    r = subprocess.run([NODE, "--input-type=module", "-e", code], capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@needs_node
+def test_module_exports_the_renderer_api():
+    out = _node("import('%s').then(m => { const p = m.TwinRenderer.prototype; console.log(JSON.stringify({exp: Object.keys(m).sort(), "
+                "methods: Object.getOwnPropertyNames(p).sort(), create: typeof m.TwinRenderer.create})); })" % MODULE)
+    d = json.loads(out)
+    assert {"TwinRenderer", "DEFAULTS", "PRESETS", "OPTION_BITS", "lensParams", "fromPrepared"} <= set(d["exp"])
+    for m in ("renderCamera", "renderFrame", "sensor", "readback", "setPoses", "project", "timings", "destroy", "idle"):
+        assert m in d["methods"], m
+    assert d["create"] == "function"
+
+
+@needs_node
+def test_js_projection_matches_oracle():
+    k = [0.031, -0.012, 0.004, -0.002]
+    pts = [[0.1, 0.2, 1.0], [-0.4, 0.3, 0.5], [0.9, -0.7, 0.2], [0.0, 0.0, 1.0]]
+    out = _node("import('%s').then(m => { const L = m.lensParams({model: 'kb4', K: [[310, 0.8, 320.5], [0, 305, 241.25], [0, 0, 1]], D: %s});"
+                "console.log(JSON.stringify(%s.map(p => m.projectKB4(L, p)))); })" % (MODULE, json.dumps(k), json.dumps(pts)))
+    got = json.loads(out)
+    Ls = oracle.lens(310, 305, 320.5, 241.25, k, 0.8 / 310)
+    for p, g in zip(pts, got):
+        uv = np.zeros(2)
+        oracle.lib().or_project(oracle.C.byref(Ls), oracle.ptr(np.array(p, float)), oracle.ptr(uv))
+        assert np.allclose(g, uv, rtol=0, atol=1e-9)
+
+
+@needs_node
+def test_from_prepared_layout():
+    S = ("{nodes: new Float32Array(16), tris: new Float32Array(16), verts: new Float32Array(24), "
+         "meshTable: [{mat: 1, flags: 1}, {mat: 0, flags: 2}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, "
+         "roughness: 0.5, ior: 1.5, fbar: 0.09, fbarComputed: 0.09}, {metallic: 1, f0: [0.9, 0.8, 0.7], alpha: 0.04, roughness: 0.2}], "
+         "emitter: {tris: [{v0: [0, 0, 1], e1: [1, 0, 0], e2: [0, 1, 0], n: [0, 0, -1], area: 0.5, cdf: 1}], area: 0.5, L0: [3, 4, 5]}}")
+    out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({meshes: Array.from(u.meshes), "
+                "mat: Array.from(u.materials), lights: Array.from(u.lights), emit: Array.from(u.emit), area: u.emitArea})); })" % (MODULE, S))
+    d = json.loads(out)
+    assert d["meshes"] == [1, 1, 0, 0, 0, 2, 0, 0]
+    m = np.array(d["mat"], np.float32).reshape(2, 16)
+    assert np.allclose(m[0, :8], [0.5, 0.4, 0.3, 0.25, 1.5, 0, 0.09, 0.5]) and m[1, 5] == 1 and np.allclose(m[1, :3], [0.9, 0.8, 0.7])
+    assert d["lights"] == [3, 4, 5, 0] and d["area"] == 0.5
+    assert np.allclose(d["emit"][:16], [0, 0, 1, 1, 1, 0, 0, 0.5, 0, 1, 0, 0, 0, 0, -1, 0])
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_frame_equals_c_abi_frame(tmp_path):
+    sc, cams = scenes.c1()
+    cam = dict(cams[0])
+    cam["spp"] = 4
+    for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    sp = scenes.default_sensor()
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"],
+               cameras=[dict(name="CAM1", lens=[float(x) for x in cam["lens"]], R=[float(x) for x in cam["R"]],
+                             pos=[float(x) for x in cam["pos"]], size=[cam["width"], cam["height"]])],
+               sensor=dict(m=[float(x) for x in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
+                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
+                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["overflows"] == 0 and info["rays"] > 0
+    js_lin = np.fromfile(tmp_path / "CAM1.linear.bin", np.float32).reshape(cam["height"], cam["width"], 4)
+    js_raw = np.fromfile(tmp_path / "CAM1.raw.bin", np.float32).reshape(cam["height"], cam["width"])
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    cam["cam_index"] = 0
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    ctx.sensor(0, dict(sp, seed=1006, frame=0))
+    raw = ctx.readback(0, lib.VT_KIND_RAW, cam["width"], cam["height"])
+    ctx.close()
+    assert np.array_equal(js_lin, lin)
+    assert np.array_equal(js_raw, raw.astype(np.float32))

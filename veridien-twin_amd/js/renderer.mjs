@@ -1,0 +1,209 @@
+// TwinRenderer over the MI355X library: the method names, argument meanings and return shapes of the
+// reference's gpu/renderer.js (create / setPoses / renderCamera / renderFrame / sensor / readback / project /
+// timings / idle / destroy), backed by the N-API addon napi/vt_napi.node and libvt.so's HIP kernels.
+// There is no WebGPU device: `device` is a HIP device ordinal, and renderCamera returns an opaque handle where
+// the reference returned a GPUTexture (gpu/renderer.js:855-866). readback returns fresh typed arrays as the
+// reference does (gpu/renderer.js:1191-1231): 'linear' Float32Array w*h*4, 'raw' Float32Array DN.
+import { createRequire } from 'module';
+
+const require = createRequire(import.meta.url);
+const native = require('../napi/vt_napi.node');
+
+export const OPTION_BITS = Object.freeze({ NO_MIS: 128 });
+export const DEFAULTS = Object.freeze({
+  spp: 16,            // sub-samples per pixel
+  maxDepth: 3,        // path vertices (camera hit = 1)
+  lightSamples: 1,    // emitter samples per path vertex
+  seed: 0,
+  batched: true,      // every camera of a frame in one persistent launch
+});
+export const PRESETS = Object.freeze({
+  live: { spp: 1, maxDepth: 2, lightSamples: 1 },
+  balanced: { spp: 4, maxDepth: 3, lightSamples: 1 },
+  judge: { spp: 64, maxDepth: 5, lightSamples: 1 },
+});
+
+const KIND = { linear: 0, raw: 1, ri: 2 };
+// Node 12 has no `??`: the same rule as a function
+const dflt = (a, b) => (a === undefined || a === null ? b : a);
+const MESH_EMITTER = 2;
+
+/** gpu/lens.js:8-16: scene lens {model, K, D, size} -> [fx fy cx cy k1..k4 alpha] */
+export function lensParams(lens) {
+  if (Array.isArray(lens) || ArrayBuffer.isView(lens)) return Array.from(lens);
+  if (!lens || (lens.model && lens.model !== 'kb4')) throw new Error(`lens model ${lens && lens.model} is not kb4`);
+  const K = lens.K, fx = K[0][0], fy = K[1][1];
+  const D = (lens.D.flat ? lens.D.flat() : lens.D).map(Number);
+  if (D.length !== 4) throw new Error('kb4 needs 4 coefficients');
+  return [fx, fy, K[0][2], K[1][2], D[0], D[1], D[2], D[3], K[0][1] / fx];
+}
+
+const rowMajor4 = (T) => (Array.isArray(T[0]) ? T : [[T[0], T[4], T[8], T[12]], [T[1], T[5], T[9], T[13]], [T[2], T[6], T[10], T[14]], [T[3], T[7], T[11], T[15]]]);
+
+/** KB4 projection (gpu/lens.js:43-51), FP64 */
+export function projectKB4(L, X) {
+  const [fx, fy, cx, cy, k1, k2, k3, k4, alpha] = L;
+  const rho = Math.hypot(X[0], X[1]);
+  if (rho < 1e-300) return [cx, cy];
+  const th = Math.atan2(rho, X[2]), t2 = th * th;
+  const s = th * (1 + t2 * (k1 + t2 * (k2 + t2 * (k3 + t2 * k4)))) / rho;
+  const xd = X[0] * s, yd = X[1] * s;
+  return [fx * (xd + alpha * yd) + cx, fy * yd + cy];
+}
+
+/**
+ * The reference's prepared scene (gpu/scene_gpu.js:461-469 prepareScene output: nodes, tris, verts, meshTable,
+ * materials, emitter) -> this library's upload arrays. The emitter's angular profile p(theta) is not carried:
+ * the library's emitters are Lambertian with per-light radiance (one light, L0).
+ */
+export function fromPrepared(S) {
+  const meshes = new Uint32Array(S.meshTable.length * 4);
+  S.meshTable.forEach((m, i) => { meshes[4 * i] = m.mat; meshes[4 * i + 1] = m.flags >>> 0; meshes[4 * i + 2] = 0; });
+  const materials = new Float32Array(S.materials.length * 16);
+  S.materials.forEach((m, i) => {
+    const kind = m.coated ? 2 : m.metallic ? 1 : 0;
+    const c = kind ? (m.f0 || [0.56, 0.57, 0.58]) : (m.rho || [0.5, 0.5, 0.5]);
+    const tau = m.tau || [1, 1, 1];
+    materials.set([c[0], c[1], c[2], m.alpha, m.ior || 1.5, kind, m.fbar || 0, m.roughness,
+      tau[0], tau[1], tau[2], m.alphaC || 0, m.fbarComputed || 0, 0, 0, 0], 16 * i);
+  });
+  const em = S.emitter || { tris: [], area: 0, L0: [0, 0, 0] };
+  const emit = new Float32Array(Math.max(1, em.tris.length) * 16), eu = new Uint32Array(emit.buffer);
+  em.tris.forEach((t, k) => {
+    emit.set([...t.v0, t.cdf, ...t.e1, t.area, ...t.e2, 0, ...t.n, 0], 16 * k);
+    eu[16 * k + 11] = 0;
+  });
+  const L0 = em.L0 || [0, 0, 0];
+  return {
+    nodes: S.nodes, tris: S.tris, verts: S.verts, meshes, materials,
+    lights: new Float32Array([L0[0], L0[1], L0[2], 0]), emit: em.tris.length ? emit : new Float32Array(0),
+    emitArea: em.area || 0,
+  };
+}
+
+export class TwinRenderer {
+  /**
+   * opts: {scene: upload arrays ({nodes?, tris, verts, meshes, materials, lights, emit, emitArea}) or
+   * {prepared: the reference's prepareScene output}, cameras: [{name, lens (scene lens block or 9 numbers),
+   * T_D_C (4x4) | {R (9, columns), pos}, size: [w, h]}], device?, preset?, log?, ...DEFAULTS}
+   */
+  static async create(opts) {
+    const r = new TwinRenderer();
+    r._init(opts);
+    return r;
+  }
+
+  _init(opts) {
+    this.opts = { ...DEFAULTS, ...(PRESETS[opts.preset] || {}), ...opts };
+    this.log = opts.log || (() => {});
+    const src = opts.scene && opts.scene.prepared ? fromPrepared(opts.scene.prepared) : opts.scene;
+    if (!src || !src.tris) throw new Error('scene: pass the upload arrays or {prepared}');
+    this.ctx = native.create(opts.device || 0);
+    this.device = { hip: opts.device || 0 };
+    native.uploadScene(this.ctx, src);
+    this.stats = { triangles: src.tris.length / 16 };
+    this.cameras = {};
+    (opts.cameras || []).forEach((c, i) => this._addCamera(c, i));
+    this.frame = 0;
+    this.poses0 = null;
+    this.poses1 = null;
+  }
+
+  _addCamera(c, index) {
+    const lens = lensParams(c.lens);
+    let R, pos;
+    if (c.T_D_C) {
+      const T = rowMajor4(c.T_D_C);
+      R = [T[0][0], T[1][0], T[2][0], T[0][1], T[1][1], T[2][1], T[0][2], T[1][2], T[2][2]];
+      pos = [T[0][3], T[1][3], T[2][3]];
+    } else { R = Array.from(c.R); pos = Array.from(c.pos); }
+    const size = c.size || (c.lens && c.lens.size) || [c.width, c.height];
+    this.cameras[c.name] = { index, name: c.name, lens, R, pos, size, mode: c.mode || 0 };
+  }
+
+  _slot(name) {
+    const cam = this.cameras[name];
+    if (!cam) throw new Error(`unknown camera ${name}`);
+    return cam;
+  }
+
+  _pack(cam, o) {
+    const spp = dflt(o.spp, this.opts.spp);
+    native.setCamera(this.ctx, cam.index, {
+      R: cam.R, pos: cam.pos, lens: cam.lens, mode: cam.mode, width: cam.size[0], height: cam.size[1], spp,
+      seed: dflt(o.seed, this.opts.seed), camIndex: cam.index, lightSamples: dflt(o.lightSamples, this.opts.lightSamples),
+      maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1),
+    });
+  }
+
+  /** poses.bodies[key] = column-major T_D_body (gpu/renderer.js:619-633). Posed bodies are not in this build. */
+  setPoses(poses0, poses1 = null) {
+    const b = (poses0 && poses0.bodies) || {};
+    if (Object.keys(b).length) throw new Error('posed bodies (BLAS instances) are not supported by this build');
+    this.poses0 = poses0; this.poses1 = poses1 || poses0;
+  }
+
+  /** Render one camera; resolves when enqueued. Returns an opaque handle (the reference returns a GPUTexture). */
+  async renderCamera(name, opts = {}) {
+    const cam = this._slot(name);
+    this._pack(cam, opts);
+    native.render(this.ctx, [cam.index], { mode: 1 });
+    return { camera: name, width: cam.size[0], height: cam.size[1] };
+  }
+
+  /** Every listed camera in one batched launch (gpu/renderer.js:986-996); optional sensor stage per camera. */
+  renderFrame(opts = {}) {
+    const names = opts.cameras || Object.keys(this.cameras);
+    const cams = names.map((n) => this._slot(n));
+    cams.forEach((c) => this._pack(c, opts));
+    native.render(this.ctx, cams.map((c) => c.index), { mode: dflt(opts.batched, this.opts.batched) ? 0 : 1 });
+    const out = {};
+    for (const c of cams) out[c.name] = { linear: { camera: c.name, width: c.size[0], height: c.size[1] } };
+    if (opts.sensor) for (const c of cams) out[c.name].raw = this.sensor(c.name, opts.sensorOpts || {});
+    this.frame++;
+    out.done = native.sync(this.ctx);
+    return out;
+  }
+
+  /** Radiance -> raw DN (sensorOptics + sensorExpose, one fused kernel). opts: {seed, frame, noise, exposure_s,
+   * gain, params} with params overriding the sensor uniform fields (gpu/sensor.js:158-177). */
+  sensor(name, opts = {}) {
+    const cam = this._slot(name);
+    const p = { ...(this.opts.sensor || {}), ...(opts.params || {}) };
+    const g = p.m || [1.35e7, 0, 0, 0, 1.35e7, 0, 0, 0, 1.35e7];
+    native.sensor(this.ctx, cam.index, {
+      ...p, m: g.flat ? g.flat() : g, frame: dflt(opts.frame, this.frame), seed: dflt(opts.seed, 0),
+      exposure: dflt(opts.exposure_s, dflt(p.exposure, 1e-3)), gain: dflt(opts.gain, dflt(p.gain, 1)), noise: opts.noise === false ? 0 : 1,
+    });
+    return { camera: name, kind: 'raw' };
+  }
+
+  /** Fresh host copy: 'linear' Float32Array w*h*4, 'raw' Float32Array of integer DN (gpu/renderer.js:1191-1231). */
+  async readback(name, kind = 'linear') {
+    const cam = this._slot(name);
+    if (!(kind in KIND)) throw new Error(`readback kind ${kind} is not supported by this build`);
+    const data = await native.readback(this.ctx, cam.index, KIND[kind], cam.size[0], cam.size[1]);
+    return kind === 'raw' ? Float32Array.from(data) : data;
+  }
+
+  /** KB4 projection of D-frame points into a camera (gpu/renderer.js:1261-1265), FP64. */
+  project(name, pointsD) {
+    const cam = this._slot(name), R = cam.R, c = cam.pos;
+    return pointsD.map((p) => {
+      const d = [p[0] - c[0], p[1] - c[1], p[2] - c[2]];
+      const X = [R[0] * d[0] + R[1] * d[1] + R[2] * d[2], R[3] * d[0] + R[4] * d[1] + R[5] * d[2], R[6] * d[0] + R[7] * d[1] + R[8] * d[2]];
+      return projectKB4(cam.lens, X);
+    });
+  }
+
+  async timings() {
+    const st = await native.stats(this.ctx);
+    return { gpu: true, last: st, stats: this.stats };
+  }
+
+  async idle() { await native.sync(this.ctx); }
+
+  destroy() { if (this.ctx) { native.destroy(this.ctx); this.ctx = null; } }
+}
+
+export default TwinRenderer;

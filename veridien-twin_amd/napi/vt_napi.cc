@@ -1,0 +1,438 @@
+// N-API (version 8) addon over the C ABI of include/vt.h: the native half of the JavaScript TwinRenderer in
+// js/renderer.mjs. Blocking calls (sync, readback, stats) run as napi_async_work and return Promises, as the
+// reference's queue.onSubmittedWorkDone / mapAsync do (gpu/renderer.js:986-996, 1191-1231); enqueue-only calls
+// (upload, setCamera, render, sensor) return synchronously and throw Error on a bad status.
+#define NAPI_VERSION 8
+#include <node_api.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vt.h"
+
+namespace {
+
+#define CHECK_NAPI(call)                                                   \
+  do {                                                                     \
+    if ((call) != napi_ok) {                                               \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call);         \
+      return nullptr;                                                      \
+    }                                                                      \
+  } while (0)
+
+napi_value throw_vt(napi_env env, vt_ctx *c, int rc, const char *what) {
+  std::string m = std::string(what) + " failed (" + std::to_string(rc) + "): " + (c ? vt_last_error(c) : "");
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+vt_ctx *get_ctx(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "first argument must be a vt context");
+    return nullptr;
+  }
+  return static_cast<vt_ctx *>(p);
+}
+
+bool get_prop(napi_env env, napi_value obj, const char *name, napi_value *out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return false;
+  if (napi_get_named_property(env, obj, name, out) != napi_ok) return false;
+  napi_valuetype t;
+  napi_typeof(env, *out, &t);
+  return t != napi_undefined && t != napi_null;
+}
+
+double num(napi_env env, napi_value obj, const char *name, double dflt) {
+  napi_value v;
+  if (!get_prop(env, obj, name, &v)) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+// typed array view of a property: data pointer and element count (nullptr when absent)
+template <typename T>
+const T *typed(napi_env env, napi_value obj, const char *name, size_t *count, napi_typedarray_type want) {
+  napi_value v;
+  *count = 0;
+  if (!get_prop(env, obj, name, &v)) return nullptr;
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) {
+    std::string m = std::string(name) + " must be a typed array";
+    napi_throw_type_error(env, nullptr, m.c_str());
+    return nullptr;
+  }
+  napi_typedarray_type tt;
+  size_t len;
+  void *data;
+  napi_value ab;
+  size_t off;
+  napi_get_typedarray_info(env, v, &tt, &len, &data, &ab, &off);
+  if (tt != want) {
+    std::string m = std::string(name) + (want == napi_float32_array ? " must be a Float32Array" : " must be a Uint32Array");
+    napi_throw_type_error(env, nullptr, m.c_str());
+    return nullptr;
+  }
+  *count = len;
+  return static_cast<const T *>(data);
+}
+
+bool float_list(napi_env env, napi_value obj, const char *name, float *dst, uint32_t n) {
+  napi_value v;
+  if (!get_prop(env, obj, name, &v)) return false;
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    double d = 0;
+    if (napi_get_element(env, v, i, &e) != napi_ok || napi_get_value_double(env, e, &d) != napi_ok) return false;
+    dst[i] = (float)d;
+  }
+  return true;
+}
+
+napi_value Create(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_create_info ci{0, 0};
+  if (argc > 0) {
+    int32_t dev = 0;
+    napi_get_value_int32(env, argv[0], &dev);
+    ci.device = dev;
+  }
+  vt_ctx *c = nullptr;
+  int rc = vt_create(&ci, &c);
+  if (rc) return throw_vt(env, nullptr, rc, "vt_create (no HIP device?)");
+  napi_value out;
+  CHECK_NAPI(napi_create_external(env, c, nullptr, nullptr, &out));
+  return out;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (c) vt_destroy(c);
+  return nullptr;
+}
+
+napi_value BuildBVH(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  bool a = false, b = false;
+  napi_is_typedarray(env, argv[0], &a);
+  napi_is_typedarray(env, argv[1], &b);
+  if (argc < 2 || !a || !b) {
+    napi_throw_type_error(env, nullptr, "buildBVH(pos: Float32Array, idx: Uint32Array)");
+    return nullptr;
+  }
+  napi_typedarray_type t0, t1;
+  size_t n0, n1, o0, o1;
+  void *d0, *d1;
+  napi_value ab0, ab1;
+  napi_get_typedarray_info(env, argv[0], &t0, &n0, &d0, &ab0, &o0);
+  napi_get_typedarray_info(env, argv[1], &t1, &n1, &d1, &ab1, &o1);
+  if (t0 != napi_float32_array || t1 != napi_uint32_array) {
+    napi_throw_type_error(env, nullptr, "buildBVH(pos: Float32Array, idx: Uint32Array)");
+    return nullptr;
+  }
+  const uint32_t ntri = (uint32_t)(n1 / 3), cap = ntri + 2;
+  std::vector<float> nodes((size_t)cap * 16);
+  std::vector<uint32_t> order(ntri ? ntri : 1);
+  uint32_t count = 0;
+  int rc = vt_build_bvh((const float *)d0, (uint32_t)(n0 / 3), (const uint32_t *)d1, ntri, 4, 16, nodes.data(), cap,
+                        &count, order.data());
+  if (rc) return throw_vt(env, nullptr, rc, "vt_build_bvh");
+  napi_value res, nab, nta, oab, ota, ncount;
+  void *dst;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)count * 64, &dst, &nab));
+  std::memcpy(dst, nodes.data(), (size_t)count * 64);
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, (size_t)count * 16, nab, 0, &nta));
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)ntri * 4, &dst, &oab));
+  if (ntri) std::memcpy(dst, order.data(), (size_t)ntri * 4);
+  CHECK_NAPI(napi_create_typedarray(env, napi_uint32_array, ntri, oab, 0, &ota));
+  CHECK_NAPI(napi_create_object(env, &res));
+  CHECK_NAPI(napi_set_named_property(env, res, "nodes", nta));
+  CHECK_NAPI(napi_set_named_property(env, res, "order", ota));
+  CHECK_NAPI(napi_create_uint32(env, count, &ncount));
+  CHECK_NAPI(napi_set_named_property(env, res, "nodeCount", ncount));
+  return res;
+}
+
+napi_value UploadScene(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  size_t n;
+  vt_scene s{};
+  s.nodes = typed<float>(env, argv[1], "nodes", &n, napi_float32_array);
+  s.node_count = (uint32_t)(n / 16);
+  s.tris = typed<float>(env, argv[1], "tris", &n, napi_float32_array);
+  s.tri_count = (uint32_t)(n / 16);
+  s.verts = typed<float>(env, argv[1], "verts", &n, napi_float32_array);
+  s.vert_count = (uint32_t)(n / 8);
+  s.meshes = typed<uint32_t>(env, argv[1], "meshes", &n, napi_uint32_array);
+  s.mesh_count = (uint32_t)(n / 4);
+  s.materials = typed<float>(env, argv[1], "materials", &n, napi_float32_array);
+  s.material_count = (uint32_t)(n / 16);
+  s.lights = typed<float>(env, argv[1], "lights", &n, napi_float32_array);
+  s.light_count = (uint32_t)(n / 4);
+  s.emit = typed<float>(env, argv[1], "emit", &n, napi_float32_array);
+  s.emit_count = (uint32_t)(n / 16);
+  s.emit_area = (float)num(env, argv[1], "emitArea", 0);
+  bool pending = false;
+  napi_is_exception_pending(env, &pending);
+  if (pending) return nullptr;
+  int rc = vt_upload_scene(c, &s);
+  if (rc) return throw_vt(env, c, rc, "vt_upload_scene");
+  return nullptr;
+}
+
+napi_value SetCamera(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t slot = 0;
+  napi_get_value_uint32(env, argv[1], &slot);
+  vt_camera k{};
+  if (!float_list(env, argv[2], "R", k.R, 9) || !float_list(env, argv[2], "pos", k.pos, 3) ||
+      !float_list(env, argv[2], "lens", k.lens, 9)) {
+    napi_throw_type_error(env, nullptr, "camera needs R[9], pos[3], lens[9]");
+    return nullptr;
+  }
+  k.mode = (uint32_t)num(env, argv[2], "mode", 0);
+  k.width = (uint32_t)num(env, argv[2], "width", 0);
+  k.height = (uint32_t)num(env, argv[2], "height", 0);
+  k.spp = (uint32_t)num(env, argv[2], "spp", 1);
+  k.seed = (uint32_t)num(env, argv[2], "seed", 0);
+  k.cam_index = (uint32_t)num(env, argv[2], "camIndex", slot);
+  k.light_samples = (uint32_t)num(env, argv[2], "lightSamples", 1);
+  k.max_depth = (uint32_t)num(env, argv[2], "maxDepth", 1);
+  k.scale = (float)num(env, argv[2], "scale", 1.0);
+  int rc = vt_set_camera(c, slot, &k);
+  if (rc) return throw_vt(env, c, rc, "vt_set_camera");
+  return nullptr;
+}
+
+napi_value Render(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t len = 0;
+  napi_get_array_length(env, argv[1], &len);
+  std::vector<uint32_t> slots(len);
+  for (uint32_t i = 0; i < len; i++) {
+    napi_value e;
+    napi_get_element(env, argv[1], i, &e);
+    napi_get_value_uint32(env, e, &slots[i]);
+  }
+  vt_render_opts o{};
+  if (argc > 2) {
+    o.mode = (uint32_t)num(env, argv[2], "mode", 0);
+    o.spp_begin = (uint32_t)num(env, argv[2], "sppBegin", 0);
+    o.spp_end = (uint32_t)num(env, argv[2], "sppEnd", 0);
+    o.accumulate = (uint32_t)num(env, argv[2], "accumulate", 0);
+    o.chunk = (uint32_t)num(env, argv[2], "chunk", 0);
+    o.block = (uint32_t)num(env, argv[2], "block", 0);
+  }
+  int rc = vt_render(c, slots.data(), len, &o);
+  if (rc) return throw_vt(env, c, rc, "vt_render");
+  return nullptr;
+}
+
+napi_value Sensor(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t slot = 0;
+  napi_get_value_uint32(env, argv[1], &slot);
+  vt_sensor_params p{};
+  napi_value o = argv[2];
+  p.cfa = (uint32_t)num(env, o, "cfa", 0);
+  p.frame = (uint32_t)num(env, o, "frame", 0);
+  p.seed = (uint32_t)num(env, o, "seed", 0);
+  p.optics_k = (float)num(env, o, "opticsK", 0.17671458676442586);
+  p.esf_sigma = (float)num(env, o, "esfSigma", 0.6374);
+  p.exposure = (float)num(env, o, "exposure", 1e-3);
+  p.gain = (float)num(env, o, "gain", 1.0);
+  if (!float_list(env, o, "m", p.m, 9)) {
+    napi_throw_type_error(env, nullptr, "sensor params need m[9] (response matrix rows)");
+    return nullptr;
+  }
+  p.read_noise = (float)num(env, o, "readNoise", 3.0);
+  p.full_well = (float)num(env, o, "fullWell", 10000.0);
+  p.dark = (float)num(env, o, "dark", 0.0);
+  p.conv_gain = (float)num(env, o, "convGain", 0.33);
+  p.max_dn = (float)num(env, o, "maxDn", 4095.0);
+  p.black = (float)num(env, o, "black", 0.0);
+  p.noise_on = (uint32_t)num(env, o, "noise", 1);
+  p.pupil_exp = (float)num(env, o, "pupilExp", 1.0);
+  int rc = vt_sensor(c, slot, &p);
+  if (rc) return throw_vt(env, c, rc, "vt_sensor");
+  return nullptr;
+}
+
+// ------------------------------------------------------------------ async: sync / readback / stats
+struct Job {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  vt_ctx *c = nullptr;
+  int kind = -1;          // -1 sync, -2 stats, >= 0 readback kind
+  uint32_t slot = 0, w = 0, h = 0;
+  std::vector<uint8_t> buf;
+  vt_frame_stats st{};
+  int rc = 0;
+  std::string err;
+};
+
+void job_exec(napi_env, void *data) {
+  Job *j = static_cast<Job *>(data);
+  if (j->kind == -1) j->rc = vt_sync(j->c);
+  else if (j->kind == -2) j->rc = vt_frame_stats_get(j->c, &j->st);
+  else j->rc = vt_readback(j->c, j->slot, j->kind, j->buf.data(), j->buf.size());
+  if (j->rc) j->err = vt_last_error(j->c);
+}
+
+void job_done(napi_env env, napi_status, void *data) {
+  Job *j = static_cast<Job *>(data);
+  napi_value res;
+  if (j->rc) {
+    napi_value msg, e;
+    std::string m = "vt call failed (" + std::to_string(j->rc) + "): " + j->err;
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &e);
+    napi_reject_deferred(env, j->deferred, e);
+  } else if (j->kind == -1) {
+    napi_get_undefined(env, &res);
+    napi_resolve_deferred(env, j->deferred, res);
+  } else if (j->kind == -2) {
+    napi_create_object(env, &res);
+    const char *names[] = {"rays", "samples", "msRender", "msSensor", "msKernelAvg", "launches", "stackOverflows"};
+    double vals[] = {j->st.rays, j->st.samples, j->st.ms_render, j->st.ms_sensor, j->st.ms_kernel_avg,
+                     (double)j->st.launches, (double)j->st.stack_overflows};
+    for (int k = 0; k < 7; k++) {
+      napi_value v;
+      napi_create_double(env, vals[k], &v);
+      napi_set_named_property(env, res, names[k], v);
+    }
+    napi_resolve_deferred(env, j->deferred, res);
+  } else {
+    napi_value ab;
+    void *dst;
+    napi_create_arraybuffer(env, j->buf.size(), &dst, &ab);
+    std::memcpy(dst, j->buf.data(), j->buf.size());
+    napi_typedarray_type t = j->kind == VT_KIND_RAW ? napi_uint16_array : napi_float32_array;
+    size_t elems = j->kind == VT_KIND_RAW ? j->buf.size() / 2 : j->buf.size() / 4;
+    napi_create_typedarray(env, t, elems, ab, 0, &res);
+    napi_resolve_deferred(env, j->deferred, res);
+  }
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+
+napi_value queue_job(napi_env env, Job *j) {
+  napi_value promise, name;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok) {
+    delete j;
+    napi_throw_error(env, nullptr, "promise creation failed");
+    return nullptr;
+  }
+  napi_create_string_utf8(env, "vt", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, nullptr, name, job_exec, job_done, j, &j->work);
+  napi_queue_async_work(env, j->work);
+  return promise;
+}
+
+napi_value Sync(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  Job *j = new Job();
+  j->c = c;
+  j->kind = -1;
+  return queue_job(env, j);
+}
+
+napi_value Stats(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  Job *j = new Job();
+  j->c = c;
+  j->kind = -2;
+  return queue_job(env, j);
+}
+
+napi_value Readback(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  Job *j = new Job();
+  j->c = c;
+  int32_t kind = 0;
+  napi_get_value_uint32(env, argv[1], &j->slot);
+  napi_get_value_int32(env, argv[2], &kind);
+  napi_get_value_uint32(env, argv[3], &j->w);
+  napi_get_value_uint32(env, argv[4], &j->h);
+  j->kind = kind;
+  const size_t px = (size_t)j->w * j->h;
+  j->buf.resize(kind == VT_KIND_LINEAR ? px * 16 : kind == VT_KIND_RAW ? px * 2 : px * 4);
+  return queue_job(env, j);
+}
+
+napi_value LastError(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], out;
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  CHECK_NAPI(napi_create_string_utf8(env, vt_last_error(c), NAPI_AUTO_LENGTH, &out));
+  return out;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct {
+    const char *name;
+    napi_callback fn;
+  } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
+             {"setCamera", SetCamera}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
+             {"readback", Readback}, {"lastError", LastError}};
+  for (auto &f : fns) {
+    napi_value v;
+    napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);
+    napi_set_named_property(env, exports, f.name, v);
+  }
+  napi_value kinds;
+  napi_create_object(env, &kinds);
+  const char *kn[] = {"linear", "raw", "ri"};
+  for (int k = 0; k < 3; k++) {
+    napi_value v;
+    napi_create_int32(env, k, &v);
+    napi_set_named_property(env, kinds, kn[k], v);
+  }
+  napi_set_named_property(env, exports, "KIND", kinds);
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

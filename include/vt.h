@@ -116,6 +116,8 @@ typedef struct {
   double ms_kernel_avg;  /* average duration of the last vt_render's trace launches */
   uint32_t launches;     /* trace launches of the last vt_render */
   uint32_t stack_overflows; /* traversal stack overflows since upload (must be 0) */
+  double ms_backward;    /* GPU time of the last vt_backward */
+  double rays_backward;  /* rays traced by the last vt_backward (two walks per sample) */
 } vt_frame_stats;
 
 /* ---------------------------------------------------------------- lifetime (TwinRenderer.create / destroy,
@@ -137,6 +139,12 @@ int vt_upload_scene(vt_ctx *ctx, const vt_scene *scene);
 /* CamU packing of _packCam (gpu/renderer.js:888-898) for camera slot `cam` (< VT_MAX_CAMERAS). */
 int vt_set_camera(vt_ctx *ctx, uint32_t cam, const vt_camera *camera);
 
+/* Replace the material table or the light radiances of the uploaded scene in place (same counts as the upload),
+ * e.g. between the steps of a calibration loop; the geometry and BVH stay. No reference counterpart: the
+ * reference rebuilds its params buffer in _buildBuffers (gpu/renderer.js:271-331). */
+int vt_update_materials(vt_ctx *ctx, const float *materials, uint32_t material_count);
+int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
+
 /* ---------------------------------------------------------------- per frame */
 /* renderCamera / renderFrame (gpu/renderer.js:974-996): enqueue the listed cameras' path tracing. */
 int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts);
@@ -150,6 +158,24 @@ int vt_readback(vt_ctx *ctx, uint32_t cam, int kind, void *dst, size_t dst_bytes
 int vt_sync(vt_ctx *ctx);
 /* timings() (gpu/renderer.js:1271-1274): statistics of the last render / sensor call. */
 int vt_frame_stats_get(vt_ctx *ctx, vt_frame_stats *out);
+
+/* ---------------------------------------------------------------- adjoint (no reference counterpart: the
+ * differentiable path of the twin's calibration) */
+/* Gradients of the scalar sum_cams sum_pixels adj . I (adj = dLoss/dI, the adjoint image; I the linear image
+ * vt_readback returns) with respect to the material colours (rho_d, or F0 of conductors; 3 floats per material)
+ * and the light radiances (3 floats per light), by path-replay backpropagation: each sample's path is re-traced
+ * from its seeded random numbers, first for its value, then accumulating the derivative of every term (sampling
+ * and MIS weights detached; interior term only, visibility discontinuities excluded). adj: one pointer per listed
+ * camera to width*height*4 floats (rgb used). opts: the sample range as for vt_render. Buffers of vt_grads are
+ * overwritten; their counts must equal the scene's material and light counts. cameras must be NULL (camera
+ * parameter gradients are not provided by this build). */
+typedef struct {
+  float *materials; uint32_t material_count;
+  float *lights; uint32_t light_count;
+  float *cameras; uint32_t camera_count;
+} vt_grads;
+int vt_backward(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const float *const *adj, const vt_render_opts *opts,
+                vt_grads *out);
 
 /* ---------------------------------------------------------------- test kernels (the reference's TESTS entry
  * points, gpu/wgsl/kernels.wgsl.js:487-551, that tests/gpu_test.html compared against the JS) */

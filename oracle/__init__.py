@@ -73,6 +73,7 @@ def lib():
             "or_jacobi_gather": (None, [u32, vp, vp, u32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
             "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
             "or_gather_dir": (None, [u32, u32, u32, vp, vp]),
+            "or_backward_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -244,3 +245,15 @@ def ri_map(L, w, h, pupil_exp=1.0):
     out = np.zeros((h, w), dtype=np.float32)
     lib().or_ri_map(C.byref(L), w, h, pupil_exp, ptr(out))
     return out
+
+
+def backward_tile(sc, cam, x0, y0, w, h, adj, spp_begin=0, spp_end=None, nthreads=8):
+    """FP64 path-replay gradients of sum adj . I over a tile: (materials x 3, lights x 3)."""
+    S = scene_struct(sc)
+    Cm = camera_struct(cam)
+    spp_end = cam["spp"] if spp_end is None else spp_end
+    adj = np.ascontiguousarray(adj, dtype=np.float32)
+    gm = np.zeros(S.mat_count * 3)
+    gl = np.zeros(max(S.light_count, 1) * 3)
+    lib().or_backward_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(adj), ptr(gm), ptr(gl), nthreads)
+    return gm.reshape(-1, 3), gl[: 3 * S.light_count].reshape(-1, 3)

@@ -2,7 +2,7 @@
  *
  * Follows gpu/bsdf.js:25-280 and gpu/shade_ref.js:5-78. OR_LAMBERT (f = rho/pi, cosine sampling) is the
  * textbook Lambertian surface that the synthetic scene configurations name; it is not in the reference.
- * Pins (tests/test_oracle_bsdf.py): reciprocity f(l, v) = f(v, l); the analytic diffuse albedo of
+ * Pins (tests/test_oracle_pins.py): reciprocity f(l, v) = f(v, l); the analytic diffuse albedo of
  * gpu/bsdf.js:240-241 against quadrature; the pdf integrating to 1; sample histograms against the pdf;
  * Fbar by Simpson against scipy quad; the disc irradiance against the closed form pi L a^2 / (a^2 + d^2);
  * capsule occlusion against Monte Carlo over the hemisphere.

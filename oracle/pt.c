@@ -15,11 +15,13 @@
  * adds its MIS-weighted emission and ends the path. An emitter seen by the camera ray adds its radiance.
  * Random numbers: the reference's Kronecker dimensions at v1; counter hashes u01(hash3(hs, i, 256 b + dim))
  * from v2 on (the same on the GPU). No Russian roulette.
- * Pins (tests/test_oracle_pt.py): white furnace inside a closed Lambertian sphere (L = Le rho^k sums), the
- * on-axis disc irradiance closed form through NEE, and convergence of independent seeds to one value.
+ * Pins (tests/test_oracle_pins.py): white furnace inside a closed Lambertian sphere (L = Le rho^k sums), the
+ * on-axis square-emitter irradiance closed form through NEE, MIS continuations against NEE alone; the adjoint
+ * against central finite differences of this forward estimator (tests/test_adjoint.py).
  */
 #include "vt_oracle.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define F_EMITTER 2u
@@ -130,8 +132,40 @@ static const float *sample_emitter(const or_scene *S, const double u[3], double 
   return e;
 }
 
+/* gradient sink of the path-replay adjoint (FP64): d(delta . L)/d(material colour), d/d(light radiance) */
+typedef struct {
+  double *mat, *light;   /* n_mat * 3, n_light * 3 */
+  double delta[3];
+  double Ltot[3];
+} grad_t;
+
+/* d f / d c (c = rho_d of dielectric and Lambert, F0 of conductor and coated), FP64 twin of the GPU's */
+static void eval_bsdf_dc(const or_mat *M, const double n[3], const double l[3], const double v[3], double out[3]) {
+  double nl = dot3(n, l), nv = dot3(n, v);
+  out[0] = out[1] = out[2] = 0;
+  if (nl <= 0 || nv <= 0) return;
+  if (M->kind == OR_LAMBERT) { out[0] = out[1] = out[2] = 1 / M_PI; return; }
+  if (M->kind == OR_DIELECTRIC) {
+    double kd = (1 / M_PI) * (1 - or_fresnel_dielectric(nl, M->ior)) * (1 - or_fresnel_dielectric(nv, M->ior)) / (1 - M->fbar);
+    out[0] = out[1] = out[2] = kd;
+    return;
+  }
+  double h[3] = {l[0] + v[0], l[1] + v[1], l[2] + v[2]};
+  norm3(h);
+  double nh = dot3(n, h), vh = fmax(0, dot3(v, h));
+  double one_m = 1 - pow(1 - fmin(fmax(vh, 0), 1), 5);
+  if (M->kind == OR_CONDUCTOR) {
+    double spec = or_ggx_d(nh, M->alpha) * or_ggx_g2(nl, nv, M->alpha) / (4 * nl * nv);
+    out[0] = out[1] = out[2] = spec * one_m;
+    return;
+  }
+  double t = (1 - or_fresnel_dielectric(nl, M->ior)) * (1 - or_fresnel_dielectric(nv, M->ior));
+  double metal = or_ggx_d(nh, M->alpha) * or_ggx_g2(nl, nv, M->alpha) / (4 * nl * nv);
+  for (int c = 0; c < 3; c++) out[c] = pow(M->tau[c], 1 / nl + 1 / nv) * t * metal * one_m;
+}
+
 static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
-                              uint32_t i, double out[3], uint64_t *rays, uint64_t *visits) {
+                              uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G) {
   double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
   cam_dir(C, P, x, y, i, d);
   double thr[3] = {1, 1, 1}, L[3] = {0, 0, 0}, pB = 0;
@@ -157,6 +191,7 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
           w = pB * pB / (pB * pB + nl * nl * pL * pL);
         }
         for (int k = 0; k < 3; k++) L[k] += thr[k] * Le[k] * w;
+        if (G) for (int k = 0; k < 3; k++) G->light[3 * (size_t)s.light + k] += G->delta[k] * thr[k] * w;
       }
       break;
     }
@@ -202,7 +237,16 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
         double pb = or_pdf_bsdf(&M, s.ns, wo, wi);
         w = (double)nl * nl * pW * pW / ((double)nl * nl * pW * pW + pb * pb);
       }
-      for (int k = 0; k < 3; k++) Ln[k] += f[k] * Le[k] * (cosS * w / pW);
+      const double kk = cosS * w / pW;
+      for (int k = 0; k < 3; k++) Ln[k] += f[k] * Le[k] * kk;
+      if (G) {
+        double dc[3];
+        eval_bsdf_dc(&M, s.ns, wi, wo, dc);
+        for (int k = 0; k < 3; k++) {
+          G->light[3 * (size_t)lid + k] += G->delta[k] * thr[k] / nl * f[k] * kk;
+          G->mat[3 * (size_t)s.mat + k] += G->delta[k] * thr[k] / nl * dc[k] * Le[k] * kk;
+        }
+      }
     }
     for (int k = 0; k < 3; k++) L[k] += thr[k] * Ln[k] / nl;
     if (last) break;
@@ -215,6 +259,14 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
     double f[3];
     or_eval_bsdf(&M, s.ns, l, wo, f);
     double c = dot3(s.ns, l) / pdf;
+    if (G) {
+      double dc[3];
+      eval_bsdf_dc(&M, s.ns, l, wo, dc);
+      for (int k = 0; k < 3; k++) {
+        double a = f[k] * c;
+        if (a > 0) G->mat[3 * (size_t)s.mat + k] += G->delta[k] * (dc[k] * c / a) * (G->Ltot[k] - L[k]);
+      }
+    }
     for (int k = 0; k < 3; k++) thr[k] *= f[k] * c;
     if (maxc(thr) <= 0) break;
     offset_ray(s.p, s.ng, l, o);
@@ -238,7 +290,7 @@ void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t
       uint64_t r = 0, vv[2] = {0, 0};
       for (uint32_t i = spp_begin; i < spp_end; i++) {
         double L[3];
-        sample_radiance(S, C, &P, x, y, i, L, &r, vv);
+        sample_radiance(S, C, &P, x, y, i, L, &r, vv, NULL);
         for (int k = 0; k < 3; k++) acc[k] += L[k];
       }
       double inv = spp_end > spp_begin ? 1.0 / (spp_end - spp_begin) : 0;
@@ -248,4 +300,37 @@ void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t
     }
   if (rays) *rays = nr;
   if (visits) { visits[0] = v0; visits[1] = v1; }
+}
+
+void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      uint32_t spp_begin, uint32_t spp_end, const float *adj, double *grad_mat, double *grad_light,
+                      int nthreads) {
+  const size_t nm = 3 * (size_t)S->mat_count, nli = 3 * (size_t)S->light_count;
+  const double sc = 1.0 / (double)(spp_end - spp_begin);
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    double *gm = calloc(nm, sizeof(double)), *gl = calloc(nli > 0 ? nli : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 1) collapse(2)
+    for (uint32_t yy = 0; yy < h; yy++)
+      for (uint32_t xx = 0; xx < w; xx++) {
+        const float *a = adj + 4 * ((size_t)yy * w + xx);
+        if (a[0] == 0 && a[1] == 0 && a[2] == 0) continue;
+        pix_t P;
+        pix_seeds(C, x0 + xx, y0 + yy, &P);
+        for (uint32_t i = spp_begin; i < spp_end; i++) {
+          double L[3];
+          uint64_t r = 0, vv[2] = {0, 0};
+          sample_radiance(S, C, &P, x0 + xx, y0 + yy, i, L, &r, vv, NULL);
+          grad_t G = {gm, gl, {a[0] * sc, a[1] * sc, a[2] * sc}, {L[0], L[1], L[2]}};
+          sample_radiance(S, C, &P, x0 + xx, y0 + yy, i, L, &r, vv, &G);
+        }
+      }
+#pragma omp critical
+    {
+      for (size_t k = 0; k < nm; k++) grad_mat[k] += gm[k];
+      for (size_t k = 0; k < nli; k++) grad_light[k] += gl[k];
+    }
+    free(gm);
+    free(gl);
+  }
 }

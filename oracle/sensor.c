@@ -7,7 +7,7 @@
  *   log: exponent/mantissa split, mantissa in [sqrt(1/2), sqrt(2)), the atanh series
  *        log m = 2 (s + s^3/3 + ... + s^11/11), s = (m - 1)/(m + 1);
  *   cos: Cody & Waite reduction by pi/2 (three-part constant), Taylor polynomials of sin and cos on [-pi/4, pi/4].
- * Pins (tests/test_oracle_sensor.py): the routines against libm (max error in ulp over dense grids); with noise
+ * Pins (tests/test_oracle_pins.py): the routines against libm (max error in ulp over dense grids); with noise
  * off, DN equals the closed form clamp(floor(e cg gain + black + 0.5), 0, 2^bits - 1); with noise on, the
  * per-level mean and variance match gpu/sensor.js:180-182 (rawNoiseDN); Poisson inversion against scipy's pmf.
  */

@@ -150,6 +150,11 @@ typedef struct {
  * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays. */
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                     uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads);
+/* path-replay adjoint of the tile: adds d(sum_p adj_p . I_p)/d(material colour) (mat_count*3) and
+ * d/d(light radiance) (light_count*3) to the outputs; adj holds w*h*4 floats (rgb used) */
+void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      uint32_t spp_begin, uint32_t spp_end, const float *adj, double *grad_mat, double *grad_light,
+                      int nthreads);
 /* camera ray of sub-sample i of pixel (x, y): o, d (FP32 sample pattern, FP64 unproject, rounded to FP32) */
 void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float o[3], float d[3]);
 

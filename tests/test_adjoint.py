@@ -1,0 +1,88 @@
+"""The path-replay adjoint: the FP64 oracle's gradients against central finite differences of its own forward
+render (common random numbers), and the GPU's gradients against the oracle's on the same samples.
+
+Loss = sum_p adj_p . I_p with a fixed adjoint image adj (the gradient of any image loss at the current image);
+parameters: material colours (rho_d / F0, rgb) and light radiances (rgb). Bar (north_star): within 1e-3
+relative of finite differences.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import lib, scenes
+
+TILE = (272, 176, 48, 32)      # x0, y0, w, h on the c1 camera (table top, dome, light region)
+
+
+def _setup(spp=8):
+    sc, cams = scenes.c1(bvh=lambda p, i: (lambda b: (b["nodes"], b["order"]))(oracle.build_bvh(p, i)))
+    cam = dict(cams[0])
+    cam["spp"] = spp
+    r = scenes.Rng(99)
+    x0, y0, w, h = TILE
+    adj = np.zeros((h, w, 4), np.float32)
+    adj[:, :, 0:3] = r.uniform(-1, 1, w * h * 3).reshape(h, w, 3)
+    return sc, cam, adj
+
+
+def _loss(sc, cam, adj):
+    x0, y0, w, h = TILE
+    img, _, _ = oracle.render_tile(sc, cam, x0, y0, w, h)
+    return float(np.sum(img[:, :, 0:3].astype(np.float64) * adj[:, :, 0:3]))
+
+
+def _with(sc, key, idx, ch, delta):
+    s2 = dict(sc)
+    if key == "lights":
+        a = sc["lights"].reshape(-1, 4).copy()
+        a[idx, ch] += delta
+    else:
+        a = sc["materials"].reshape(-1, 16).copy()
+        a[idx, ch] += delta
+    s2[key] = a.reshape(-1)
+    return s2
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    sc, cam, adj = _setup()
+    x0, y0, w, h = TILE
+    gm, gl = oracle.backward_tile(sc, cam, x0, y0, w, h, adj)
+    # lights: the image is linear in the radiance, so the central difference is exact up to rounding
+    for li in range(4):
+        for ch in range(3):
+            hstep = 0.5
+            fd = (_loss(_with(sc, "lights", li, ch, hstep), cam, adj) - _loss(_with(sc, "lights", li, ch, -hstep), cam, adj)) / (2 * hstep)
+            assert gl[li, ch] == pytest.approx(fd, rel=1e-3, abs=1e-6 * max(1.0, abs(fd)))
+    # Lambertian albedos: sampling does not depend on them, the estimate is a polynomial in each albedo
+    order = np.argsort(-np.abs(gm[:16, 0]))
+    for m in order[:6]:
+        for ch in range(3):
+            hstep = 1e-3
+            fd = (_loss(_with(sc, "materials", m, ch, hstep), cam, adj) - _loss(_with(sc, "materials", m, ch, -hstep), cam, adj)) / (2 * hstep)
+            assert gm[m, ch] == pytest.approx(fd, rel=1e-3, abs=2e-4 * np.max(np.abs(gm)))
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_matches_oracle_and_fd():
+    sc, cam, adj = _setup()
+    x0, y0, w, h = TILE
+    full = np.zeros((cam["height"], cam["width"], 4), np.float32)
+    full[y0:y0 + h, x0:x0 + w] = adj
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    ctx.set_camera(0, cam)
+    n_mat, n_light = sc["materials"].size // 16, sc["lights"].size // 4
+    gm, gl = ctx.backward([0], [full], n_mat, n_light)
+    st = ctx.stats()
+    ctx.close()
+    om, ol = oracle.backward_tile(sc, cam, x0, y0, w, h, adj)
+    print("GPU vs FP64 oracle gradients: materials max rel %.2e, lights max rel %.2e, backward %.2f ms" % (
+        np.max(np.abs(gm - om)) / np.max(np.abs(om)), np.max(np.abs(gl - ol)) / np.max(np.abs(ol)), st["ms_backward"]))
+    assert np.allclose(gl, ol, rtol=1e-3, atol=1e-4 * np.max(np.abs(ol)))
+    assert np.allclose(gm, om, rtol=1e-3, atol=1e-4 * np.max(np.abs(om)))
+    assert st["stack_overflows"] == 0 and st["rays_backward"] > 0
+    # and the oracle's finite differences on the two largest albedo components
+    big = np.dstack(np.unravel_index(np.argsort(-np.abs(om[:16]).ravel())[:2], (16, 3)))[0]
+    for m, ch in big:
+        fd = (_loss(_with(sc, "materials", m, ch, 1e-3), cam, adj) - _loss(_with(sc, "materials", m, ch, -1e-3), cam, adj)) / 2e-3
+        assert gm[m, ch] == pytest.approx(fd, rel=1e-3)

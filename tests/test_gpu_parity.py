@@ -278,3 +278,25 @@ def test_render_converges_to_oracle(ctx, c1):
     rel = np.linalg.norm(a - b) / np.linalg.norm(b)
     print("independent 1024 spp block relative RMSE %.4f" % rel)
     assert rel < 0.01
+
+
+@pytest.mark.parametrize("depth", [1, 4])
+def test_render_white_furnace_closed_form(depth):
+    """The closed-form sphere furnace (emitting cap, Lambertian rest) on the GPU at full resolution, in its own
+    context: mean radiance against the geometric series, and against the oracle on the same samples."""
+    from test_oracle_pins import furnace_sphere
+    sc, cam, want = furnace_sphere(depth=depth)
+    c = lib.Context(0)
+    try:
+        c.upload_scene(sc)
+        c.set_camera(0, cam)
+        c.render([0])
+        img = c.readback(0, lib.VT_KIND_LINEAR, 64, 64)
+        st = c.stats()
+    finally:
+        c.close()
+    assert st["stack_overflows"] == 0
+    assert img[:, :, 0].mean() == pytest.approx(want, rel=4e-3)
+    ref, _, _ = oracle.render_tile(sc, cam, 16, 16, 32, 32, 0, 256)
+    a, b = img[16:48, 16:48, 0:3].astype(np.float64), ref[:, :, 0:3].astype(np.float64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-3

@@ -371,3 +371,37 @@ def test_estimator_mis_bounce_agrees_with_nee_only():
     img2, _, _ = oracle.render_tile(sc, cam, 30, 30, 4, 4, 0, 1024)
     want = 0.5 / math.pi * _square_irradiance(10.0, 0.2, 1.0)
     assert img2[:, :, 0].mean() == pytest.approx(want, rel=0.015)
+
+
+def furnace_sphere(rho=0.8, Le=5.0, n_theta=32, n_phi=64, cap_rows=6, depth=4):
+    """Unit sphere seen from its centre: an emitting cap (radiance Le, inward) and Lambertian albedo rho elsewhere.
+    Inside a sphere the form factor from any point to a patch is area / (4 pi R^2), so with f the cap's share
+    of the area every non-emitting point has L_D = rho Le f sum_{k<D} (rho (1 - f))^k after D bounces."""
+    B = scenes.Builder()
+    P, N, I = scenes.dome(1.0, n_theta, n_phi, pole=True)
+    fan = n_phi + 2 * n_phi * (cap_rows - 1)
+    B.add(P, N, I[:fan], 1)
+    B.add(P, N, I[fan:], 0)
+    flip = np.array([1.0, 1.0, -1.0])
+    B.add(P * flip, N * flip, I[:, ::-1].copy(), 0)
+    P, N, I, M = B.arrays()
+    mats = [scenes.material_record(scenes.KIND_LAMBERT, (rho,) * 3), scenes.material_record(scenes.KIND_LAMBERT, (0, 0, 0))]
+    meshes = [[0, scenes.MESH_STATIC, 0, 0], [1, scenes.MESH_EMITTER, 0, 0]]
+    bvh = lambda p, i: (lambda b: (b["nodes"], b["order"]))(oracle.build_bvh(p, i))
+    sc = scenes.pack_scene(P, N, I, M, meshes, mats, [[Le, Le, Le, 0]], bvh=bvh)
+    a, b, c = P[I[:, 0]], P[I[:, 1]], P[I[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    f = area[M == 1].sum() / area.sum()
+    cam = scenes.camera(scenes.look_at((0, 0, 0), (0, 0, -1), up=(0, 1, 0)), (0, 0, 0),
+                        [40, 40, 31.5, 31.5, 0, 0, 0, 0, 0], 64, 64, 256, 0, depth, mode=1)
+    q = rho * (1 - f)
+    return sc, cam, rho * Le * f * (1 - q ** depth) / (1 - q)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_estimator_white_furnace_sphere(depth):
+    """NEE alone (depth 1), and NEE + MIS-weighted continuations (depth 2, 4) against the geometric series; the
+    inscribed polyhedron departs from the sphere's uniform form factor by about 1e-3."""
+    sc, cam, want = furnace_sphere(depth=depth)
+    img, _, _ = oracle.render_tile(sc, cam, 16, 16, 32, 32, 0, 256)
+    assert img[:, :, 0].mean() == pytest.approx(want, rel=4e-3)

@@ -25,7 +25,7 @@ struct DevBuf {
 struct CamSlot {
   bool set = false;
   vt_camera c{};
-  DevBuf acc, dn, ri, lin;
+  DevBuf acc, dn, ri, lin, adj;
   bool ri_valid = false;
   float ri_pupil = -1.0f;
   uint32_t spp_total = 0;     // samples accumulated in acc (for the mean)
@@ -54,7 +54,11 @@ struct vt_ctx {
   hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
   std::vector<hipEvent_t> ev_launch;
   uint32_t last_launches = 0;
-  bool render_timed = false, sensor_timed = false;
+  bool render_timed = false, sensor_timed = false, backward_timed = false;
+  hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
+  DevBuf grads;
+  uint32_t n_mat = 0, n_light = 0;
+  double rays_backward = 0;
   CamSlot cams[VT_MAX_CAMERAS];
   // radiosity
   DevBuf g_active, g_table, g_normals, g_mat, g_alb, g_hh, g_nbr, g_nbrw, g_state, g_aux;
@@ -206,7 +210,7 @@ extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
   if (!rc) rc = ensure(c, c->spill, c->spill_lanes * STACK_SPILL * sizeof(uint2));
   if (!rc) rc = ensure(c, c->dcams, sizeof(DevCam) * VT_MAX_CAMERAS);
   if (!rc && hipHostMalloc((void **)&c->h_cams, sizeof(DevCam) * VT_MAX_CAMERAS, hipHostMallocDefault) != hipSuccess) rc = VT_ERR_HIP;
-  hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1};
+  hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1, &c->ev_b0, &c->ev_b1};
   for (hipEvent_t *ev : evs)
     if (!rc && hipEventCreate(ev) != hipSuccess) rc = VT_ERR_HIP;
   if (!rc && hipMemsetAsync(c->overflow.p, 0, 64, c->stream) != hipSuccess) rc = VT_ERR_HIP;
@@ -227,16 +231,17 @@ extern "C" void vt_destroy(vt_ctx *c) {
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
-                    &c->dXd, &c->dR, &c->dPart};
+                    &c->dXd, &c->dR, &c->dPart, &c->grads};
   for (DevBuf *b : bufs) release(*b);
   for (auto &cs : c->cams) {
     release(cs.acc);
     release(cs.dn);
     release(cs.ri);
     release(cs.lin);
+    release(cs.adj);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
-  hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1};
+  hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1, c->ev_b0, c->ev_b1};
   for (hipEvent_t ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->h_cams) (void)hipHostFree(c->h_cams);
@@ -358,9 +363,31 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.emit_area = s->emit_area > 0 ? s->emit_area : 1e-12f;
   c->tri_count = T;
   c->node4_count = (uint32_t)n4.size();
+  c->n_mat = s->material_count;
+  c->n_light = s->light_count;
   c->has_scene = true;
   VT_HIP(c, hipMemsetAsync(c->overflow.p, 0, 64, c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_update_materials(vt_ctx *c, const float *materials, uint32_t count) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "update_materials: no scene uploaded");
+  if (!materials || count != c->n_mat) return fail(c, VT_ERR_SIZE, "update_materials: %u materials expected", c->n_mat);
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(c->mats.p, materials, (size_t)count * 64, hipMemcpyHostToDevice));
+  return VT_OK;
+}
+
+extern "C" int vt_update_lights(vt_ctx *c, const float *lights, uint32_t count) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "update_lights: no scene uploaded");
+  if (!lights || count != c->n_light || count == 0) return fail(c, VT_ERR_SIZE, "update_lights: %u lights expected", c->n_light);
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(c->lights.p, lights, (size_t)count * 16, hipMemcpyHostToDevice));
   return VT_OK;
 }
 
@@ -434,6 +461,8 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
     d.item_begin = items;
     d.item_count = (uint64_t)d.tiles_x * d.tiles_y * 64ull * d.nspp;
     d.acc = (float4 *)s.acc.p;
+    d.adj = nullptr;
+    d.scale = k.scale > 0 ? k.scale : 1.0f;
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -616,6 +645,97 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
     VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
     out->ms_sensor = ms;
   }
+  if (c->backward_timed) {
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1));
+    out->ms_backward = ms;
+    out->rays_backward = (double)cnt[5];
+  }
+  return VT_OK;
+}
+
+// ====================================================================== adjoint
+extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const float *const *adj,
+                           const vt_render_opts *o, vt_grads *out) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "backward: no scene uploaded");
+  if (!cams || !adj || !out || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "backward: bad arguments");
+  if (out->cameras) return fail(c, VT_ERR_ARG, "backward: camera-parameter gradients are not provided by this build");
+  if (!out->materials || out->material_count != c->n_mat) return fail(c, VT_ERR_SIZE, "backward: materials needs %u x 3 floats", c->n_mat);
+  if (c->n_light && (!out->lights || out->light_count != c->n_light)) return fail(c, VT_ERR_SIZE, "backward: lights needs %u x 3 floats", c->n_light);
+  vt_render_opts opt{};
+  if (o) opt = *o;
+  const uint32_t chunk = opt.chunk ? opt.chunk : 4;
+  for (uint32_t a = 0; a < n; a++) {
+    if (cams[a] >= VT_MAX_CAMERAS || !c->cams[cams[a]].set) return fail(c, VT_ERR_STATE, "backward: camera %u not set", cams[a]);
+    if (!adj[a]) return fail(c, VT_ERR_ARG, "backward: null adjoint image for camera %u", cams[a]);
+    for (uint32_t b = 0; b < a; b++)
+      if (cams[a] == cams[b]) return fail(c, VT_ERR_ARG, "backward: camera %u listed twice", cams[a]);
+  }
+  VT_HIP(c, hipSetDevice(c->device));
+  if (c->stage_pending) VT_HIP(c, hipEventSynchronize(c->ev_stage));
+  uint64_t items = 0;
+  int rc;
+  for (uint32_t a = 0; a < n; a++) {
+    CamSlot &s = c->cams[cams[a]];
+    const vt_camera &k = s.c;
+    const size_t px = (size_t)k.width * k.height;
+    if ((rc = ensure(c, s.adj, px * 16))) return rc;
+    VT_HIP(c, hipMemcpyAsync(s.adj.p, adj[a], px * 16, hipMemcpyHostToDevice, c->stream));
+    DevCam &d = c->h_cams[a];
+    std::memcpy(d.R, k.R, sizeof d.R);
+    std::memcpy(d.pos, k.pos, sizeof d.pos);
+    d.mode = k.mode;
+    std::memcpy(d.lens, k.lens, sizeof d.lens);
+    d.width = k.width; d.height = k.height; d.spp = k.spp; d.seed = k.seed;
+    d.cam_index = k.cam_index;
+    d.nl = k.light_samples ? k.light_samples : 1;
+    d.depth = k.max_depth ? k.max_depth : 1;
+    d.tiles_x = (k.width + 7) / 8;
+    d.tiles_y = (k.height + 7) / 8;
+    const uint32_t s0 = opt.spp_begin, s1 = opt.spp_end ? opt.spp_end : k.spp;
+    if (s1 <= s0) return fail(c, VT_ERR_ARG, "backward: empty sub-sample range");
+    d.nspp = s1 - s0;
+    d.spp_begin = s0;
+    d.accumulate = 0;
+    d.item_begin = items;
+    d.item_count = (uint64_t)d.tiles_x * d.tiles_y * 64ull * d.nspp;
+    d.acc = (float4 *)s.acc.p;
+    d.adj = (const float4 *)s.adj.p;
+    d.scale = k.scale > 0 ? k.scale : 1.0f;
+    items += d.item_count;
+  }
+  VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_stage, c->stream));
+  c->stage_pending = true;
+  const size_t ng = 3 * (size_t)(c->n_mat + c->n_light);
+  if ((rc = ensure(c, c->grads, ng * 4))) return rc;
+  VT_HIP(c, launch_zero_f4((float4 *)c->grads.p, (ng + 3) / 4, c->stream));
+  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 8, c->stream));
+  RenderLaunch L{};
+  L.S = c->S;
+  L.cams = (const DevCam *)c->dcams.p;
+  L.ncam = n;
+  L.item_base = 0;
+  L.total_waves = items / 64;
+  L.counters = (unsigned long long *)c->counters.p + 4;   // [4] queue head (counters[0] of the launch), [5] rays
+  L.overflow = (uint32_t *)c->overflow.p;
+  L.spill = (uint2 *)c->spill.p;
+  L.spill_stride = c->spill_lanes;
+  L.chunk = chunk;
+  L.persistent = 1;
+  L.block = 256;
+  L.grid = (uint32_t)std::min<uint64_t>((L.total_waves * 64 + 255) / 256, max_blocks(c, 256));
+  float *gm = (float *)c->grads.p;
+  VT_HIP(c, hipEventRecord(c->ev_b0, c->stream));
+  VT_HIP(c, launch_backward(L, gm, gm + 3 * (size_t)c->n_mat, c->n_mat, c->n_light, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_b1, c->stream));
+  c->backward_timed = true;
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<float> g(ng);
+  VT_HIP(c, hipMemcpy(g.data(), c->grads.p, ng * 4, hipMemcpyDeviceToHost));
+  std::memcpy(out->materials, g.data(), 12 * (size_t)c->n_mat);
+  if (c->n_light) std::memcpy(out->lights, g.data() + 3 * (size_t)c->n_mat, 12 * (size_t)c->n_light);
   return VT_OK;
 }
 

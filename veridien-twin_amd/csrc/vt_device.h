@@ -66,6 +66,8 @@ struct DevCam {
   uint32_t tiles_y, nspp, spp_begin, accumulate;
   uint64_t item_begin, item_count;   // in items of this launch's queue (multiples of 64)
   float4 *acc;                       // per-pixel radiance sum (rgb), sample count (a)
+  const float4 *adj;                 // adjoint image dLoss/dI (rgb) for the backward pass, else null
+  float scale;                       // output scale (the image is acc * scale / spp)
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)
@@ -183,6 +185,28 @@ __device__ __forceinline__ v3 eval_bsdf(const Mat &M, v3 n, v3 l, v3 v) {
     out = out + M.c * kd;
   }
   return out;
+}
+// d f / d c per channel (c = rho_d of dielectric and Lambert, F0 of conductor and coated conductor), the
+// derivative the adjoint needs; 0 outside the upper hemisphere like eval_bsdf
+__device__ __forceinline__ v3 eval_bsdf_dc(const Mat &M, v3 n, v3 l, v3 v) {
+  float nl = dot(n, l), nv = dot(n, v);
+  if (nl <= 0.0f || nv <= 0.0f) return mk(0.f, 0.f, 0.f);
+  if (M.kind == 3) return mk(INV_PI_F, INV_PI_F, INV_PI_F);
+  if (M.kind == 0) {
+    float kd = INV_PI_F * (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior)) / (1.0f - M.fbar);
+    return mk(kd, kd, kd);
+  }
+  v3 h = normalize(l + v);
+  float nh = dot(n, h), vh = fmaxf(0.0f, dot(v, h));
+  float one_m = 1.0f - pow5(1.0f - fminf(fmaxf(vh, 0.0f), 1.0f));
+  if (M.kind == 1) {
+    float spec = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+    return mk(spec * one_m, spec * one_m, spec * one_m);
+  }
+  float t = (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior));
+  float metal = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+  float e = 1.0f / nl + 1.0f / nv;
+  return mk(powf(M.tau.x, e), powf(M.tau.y, e), powf(M.tau.z, e)) * (t * metal * one_m);
 }
 __device__ __forceinline__ float spec_prob(const Mat &M, float nv) {
   if (M.kind == 3) return 0.0f;

@@ -29,6 +29,10 @@ struct RenderLaunch {
   uint32_t grid;               // blocks
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
+// path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
+// (block 256, persistent queue; L.cams[*].adj set)
+hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
+                           hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 

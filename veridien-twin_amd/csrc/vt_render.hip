@@ -7,12 +7,36 @@
 
 namespace vt {
 
+// Gradient sink of the adjoint: per-block LDS accumulators (material colour rgb, light radiance rgb), or the
+// global buffer directly when the tables are too large for LDS.
+struct GradSink {
+  float *mat;      // n_mat * 3
+  float *light;    // n_light * 3
+  v3 delta;        // dLoss/dI of this sample's pixel times scale / spp
+};
+__device__ __forceinline__ void grad_add(float *base, uint32_t idx, v3 g) {
+  if (g.x != 0.0f) atomicAdd(base + 3 * idx, g.x);
+  if (g.y != 0.0f) atomicAdd(base + 3 * idx + 1, g.y);
+  if (g.z != 0.0f) atomicAdd(base + 3 * idx + 2, g.z);
+}
+__device__ __forceinline__ float ratio(float d, float a) { return a > 0.0f ? d / a : 0.0f; }
+
 // The camera estimator (the reference's pieces composed into paths of `depth` vertices, as the oracle states):
 // NEE with the power heuristic at every vertex but the last, NEE alone at the last, emitter hits by BSDF rays
 // MIS-weighted (gpu/wgsl/shade.wgsl.js:135-188, 224-236), Kronecker dimensions at the first vertex
 // (gpu/wgsl/kernels.wgsl.js:42-53) and counter hashes after it.
-__device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
-                                            uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
+// GRAD: the path-replay walk of the adjoint. The same path is re-traced from the same random numbers; every
+// term's derivative with respect to the lights and the material colours is added to the sink, sampling and
+// MIS weights detached; a continuation factor a = f cos / pdf at vertex k scales everything after it, whose sum
+// is Ltot - (what was accumulated up to k), so dL/dc gets (da / a) * (Ltot - L_acc) there.
+template <bool GRAD>
+__device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x, uint32_t y,
+                                        uint32_t i, Stack &st, uint32_t &nrays, GradSink *gs = nullptr,
+                                        v3 Ltot = v3{0.f, 0.f, 0.f}) {
+  // Register pressure: every value a shadow ray or an extension ray waits for is computed before its traversal
+  // (the NEE term and its MIS weight, the derivative pieces), and the material record is re-read from the
+  // cached table after it, so only the ray and the path's small state stay live across the traversals.
+  const uint32_t hs = P.hs;
   v3 o = mk(C.pos[0], C.pos[1], C.pos[2]);
   v3 d = camera_dir(C, P, x, y, i);
   v3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
@@ -27,12 +51,15 @@ __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, 
     if (!traverse<false>(S, o, d, 0.0f, INFINITY, h, st)) break;
     const uint4 at = S.attr[h.tri];
     const uint4 ms = S.meshes[at.w];
-    const float4 *tp = S.tri + 3 * (size_t)h.tri;
-    const v3 p0 = ld3(tp[0]);
-    v3 ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
-    const float w0 = 1.0f - h.u - h.v;
-    const v3 nsi = normalize(ld3(S.vnorm[at.x]) * w0 + ld3(S.vnorm[at.y]) * h.u + ld3(S.vnorm[at.z]) * h.v);
-    v3 ns = nsi;
+    v3 ng, ns, nsi;
+    {
+      const float4 *tp = S.tri + 3 * (size_t)h.tri;
+      const v3 p0 = ld3(tp[0]);
+      ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
+      const float w0 = 1.0f - h.u - h.v;
+      nsi = normalize(ld3(S.vnorm[at.x]) * w0 + ld3(S.vnorm[at.y]) * h.u + ld3(S.vnorm[at.z]) * h.v);
+    }
+    ns = nsi;
     if (dot(ng, d) > 0.0f) ng = -ng;
     if (dot(ns, ng) < 0.0f) ns = -ns;
     if (dot(ns, -d) < 1e-4f) ns = ng;
@@ -47,10 +74,10 @@ __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, 
           w = pB * pB / (pB * pB + fnl * fnl * pL * pL);
         }
         L = L + thr * Le * w;
+        if (GRAD) grad_add(gs->light, ms.z, gs->delta * thr * w);
       }
       break;
     }
-    const Mat M = load_mat(S.mats, ms.x);
     const v3 wo = -d;
     const bool last = b + 1 == D;
     v3 Ln = mk(0.f, 0.f, 0.f);
@@ -61,35 +88,53 @@ __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, 
           const uint32_t li = i * nl + j;
           u0 = kr(li, KL0, P.s[3]); u1 = kr(li, KL1, P.s[4]); u2 = kr(li, KL2, P.s[5]);
         } else {
-          u0 = u01(hash3(P.hs, i, 256u * b + 3u * j));
-          u1 = u01(hash3(P.hs, i, 256u * b + 3u * j + 1u));
-          u2 = u01(hash3(P.hs, i, 256u * b + 3u * j + 2u));
+          u0 = u01(hash3(hs, i, 256u * b + 3u * j));
+          u1 = u01(hash3(hs, i, 256u * b + 3u * j + 1u));
+          u2 = u01(hash3(hs, i, 256u * b + 3u * j + 2u));
         }
-        v3 pl;
-        const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
-        v3 wi = pl - p;
-        const float d2 = dot(wi, wi), dist = sqrtf(d2);
-        wi = wi * (1.0f / dist);
-        const float4 en = S.emit[4 * e + 3];
-        const float cosE = -dot(wi, ld3(en));
-        if (cosE <= 0.0f) continue;
-        const float cosS = dot(ns, wi);
-        if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
-        const v3 Le = ld3(S.lights[__float_as_uint(S.emit[4 * e + 2].w)]);
-        if (maxc(Le) <= 0.0f) continue;
-        const v3 f = eval_bsdf(M, ns, wi, wo);
-        if (maxc(f) <= 0.0f) continue;
-        const v3 so = p + ng * (dot(ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
+        v3 so, wi, contrib, gl_part, gm_part;
+        float tmax;
+        uint32_t lid;
+        {
+          v3 pl;
+          const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+          wi = pl - p;
+          const float d2 = dot(wi, wi), dist = sqrtf(d2);
+          wi = wi * (1.0f / dist);
+          const float cosE = -dot(wi, ld3(S.emit[4 * e + 3]));
+          if (cosE <= 0.0f) continue;
+          const float cosS = dot(ns, wi);
+          if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
+          lid = __float_as_uint(S.emit[4 * e + 2].w);
+          const v3 Le = ld3(S.lights[lid]);
+          if (maxc(Le) <= 0.0f) continue;
+          const Mat M = load_mat(S.mats, ms.x);
+          const v3 f = eval_bsdf(M, ns, wi, wo);
+          if (maxc(f) <= 0.0f) continue;
+          const float pW = d2 / (cosE * A);
+          float w = 1.0f;
+          if (!last) {
+            const float pb = pdf_bsdf(M, ns, wo, wi);
+            w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
+          }
+          const float k = cosS * w / pW;
+          contrib = f * Le * k;
+          if (GRAD) {
+            const v3 g = gs->delta * thr * (1.0f / fnl);
+            gl_part = g * f * k;
+            gm_part = g * eval_bsdf_dc(M, ns, wi, wo) * Le * k;
+          }
+          so = p + ng * (dot(ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
+          tmax = dist - 2e-5f;
+        }
         Hit sh;
         nrays++;
-        if (traverse<true>(S, so, wi, 0.0f, dist - 2e-5f, sh, st)) continue;
-        const float pW = d2 / (cosE * A);
-        float w = 1.0f;
-        if (!last) {
-          const float pb = pdf_bsdf(M, ns, wo, wi);
-          w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
+        if (traverse<true>(S, so, wi, 0.0f, tmax, sh, st)) continue;
+        Ln = Ln + contrib;
+        if (GRAD) {
+          grad_add(gs->light, lid, gl_part);
+          grad_add(gs->mat, ms.x, gm_part);
         }
-        Ln = Ln + f * Le * (cosS * w / pW);
       }
     }
     L = L + thr * Ln * (1.0f / fnl);
@@ -98,21 +143,34 @@ __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, 
     if (b == 0) {
       u0 = kr(i, KB0, P.s[6]); u1 = kr(i, KB1, P.s[7]); u2 = kr(i, KB2, P.s[8]);
     } else {
-      u0 = u01(hash3(P.hs, i, 256u * b + 200u));
-      u1 = u01(hash3(P.hs, i, 256u * b + 201u));
-      u2 = u01(hash3(P.hs, i, 256u * b + 202u));
+      u0 = u01(hash3(hs, i, 256u * b + 200u));
+      u1 = u01(hash3(hs, i, 256u * b + 201u));
+      u2 = u01(hash3(hs, i, 256u * b + 202u));
     }
+    const Mat M = load_mat(S.mats, ms.x);
     v3 l;
     const float pdf = sample_bsdf(M, ns, wo, u0, u1, u2, l);
     if (!(pdf > 0.0f) || dot(ng, l) <= 0.0f) break;
     const v3 f = eval_bsdf(M, ns, l, wo);
-    thr = thr * f * (dot(ns, l) / pdf);
+    const float cp = dot(ns, l) / pdf;
+    const v3 a = f * cp;
+    if (GRAD) {
+      const v3 da = eval_bsdf_dc(M, ns, l, wo) * cp;
+      const v3 rest = Ltot - L;
+      grad_add(gs->mat, ms.x, gs->delta * mk(ratio(da.x, a.x) * rest.x, ratio(da.y, a.y) * rest.y, ratio(da.z, a.z) * rest.z));
+    }
+    thr = thr * a;
     if (maxc(thr) <= 0.0f) break;
     o = p + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
     d = l;
     pB = pdf;
   }
   return L;
+}
+
+__device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
+                                            uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
+  return path_walk<false>(S, C, P, x, y, i, st, nrays);
 }
 
 __device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
@@ -168,6 +226,76 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
     atomicAdd(&Lc.counters[1], (unsigned long long)rs);
     atomicAdd(&Lc.counters[2], (unsigned long long)ns);
   }
+}
+
+// backward wave: replay each sample's path twice (primal value, then the gradient walk)
+__device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t g0 = Lc.item_base + wave * 64ull;
+  uint32_t c = 0;
+  while (c + 1 < Lc.ncam && g0 >= Lc.cams[c + 1].item_begin) c++;
+  const DevCam &C = Lc.cams[c];
+  const uint64_t q = g0 - C.item_begin + lane;
+  const uint32_t nspp = C.nspp;
+  const uint64_t pix = q / nspp;
+  const uint32_t i = C.spp_begin + (uint32_t)(q - pix * nspp);
+  const uint32_t t = (uint32_t)(pix >> 6), r = (uint32_t)(pix & 63u);
+  const uint32_t x = (t % C.tiles_x) * 8u + (r & 7u), y = (t / C.tiles_x) * 8u + (r >> 3);
+  uint32_t nrays = 0;
+  if (x < C.width && y < C.height) {
+    const float4 a = C.adj[(size_t)y * C.width + x];
+    const float s = C.scale / (float)nspp;
+    gs.delta = mk(a.x * s, a.y * s, a.z * s);
+    if (gs.delta.x != 0.0f || gs.delta.y != 0.0f || gs.delta.z != 0.0f) {
+      PixSeeds P;
+      pix_seeds(C, x, y, P);
+      const v3 Ltot = path_walk<false>(Lc.S, C, P, x, y, i, st, nrays);
+      path_walk<true>(Lc.S, C, P, x, y, i, st, nrays, &gs, Ltot);
+    }
+  }
+  uint32_t rs = nrays;
+  for (int off = 32; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
+  if (lane == 0) atomicAdd(&Lc.counters[1], (unsigned long long)rs);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
+                                                    uint32_t n_light, uint32_t use_lds) {
+  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
+  __shared__ float s_t[STACK_LDS * BLOCK];
+  extern __shared__ float sgrad[];
+  const uint32_t ng = 3 * (n_mat + n_light);
+  if (use_lds) {
+    for (uint32_t k = threadIdx.x; k < ng; k += BLOCK) sgrad[k] = 0.0f;
+  }
+  __syncthreads();
+  Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
+           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
+  GradSink gs{use_lds ? sgrad : gmat, use_lds ? sgrad + 3 * n_mat : glight, v3{0.f, 0.f, 0.f}};
+  for (;;) {
+    unsigned long long base = 0;
+    if ((threadIdx.x & 63u) == 0) base = atomicAdd(&Lc.counters[0], (unsigned long long)Lc.chunk);
+    base = __shfl(base, 0, 64);
+    if (base >= Lc.total_waves) break;
+    const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
+    for (uint64_t w = base; w < end; w++) run_wave_bwd(Lc, w, st, gs);
+  }
+  __syncthreads();
+  if (use_lds) {
+    for (uint32_t k = threadIdx.x; k < ng; k += BLOCK) {
+      const float v = sgrad[k];
+      if (v != 0.0f) atomicAdd(k < 3 * n_mat ? gmat + k : glight + (k - 3 * n_mat), v);
+    }
+  }
+}
+
+hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
+                           hipStream_t s) {
+  const size_t lds = (size_t)3 * (n_mat + n_light) * sizeof(float);
+  const uint32_t use_lds = lds <= 16384 ? 1u : 0u;
+  hipLaunchKernelGGL(k_backward<256>, dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight, n_mat,
+                     n_light, use_lds);
+  return hipGetLastError();
 }
 
 template <int BLOCK>

@@ -16,7 +16,8 @@ EXPORTS = [
     "vt_create", "vt_destroy", "vt_last_error", "vt_build_bvh", "vt_upload_scene", "vt_set_camera", "vt_render",
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
-    "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F",
+    "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
+    "vt_update_lights",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
@@ -53,7 +54,12 @@ class SensorParams(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("rays", f64), ("samples", f64), ("ms_render", f64), ("ms_sensor", f64), ("ms_kernel_avg", f64),
-                ("launches", u32), ("stack_overflows", u32)]
+                ("launches", u32), ("stack_overflows", u32), ("ms_backward", f64), ("rays_backward", f64)]
+
+
+class Grads(C.Structure):
+    _fields_ = [("materials", vp), ("material_count", u32), ("lights", vp), ("light_count", u32), ("cameras", vp),
+                ("camera_count", u32)]
 
 
 class GatherTables(C.Structure):
@@ -103,6 +109,9 @@ def load():
         "vt_dense_formfactors": (i32, [vp, u32, vp, f32, u32, vp]),
         "vt_dense_solve": (i32, [vp, vp, vp, vp]),
         "vt_dense_download_F": (i32, [vp, vp, sz]),
+        "vt_backward": (i32, [vp, vp, u32, vp, vp, vp]),
+        "vt_update_materials": (i32, [vp, vp, u32]),
+        "vt_update_lights": (i32, [vp, vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -213,6 +222,26 @@ class Context:
         s = FrameStats()
         self.check(self.L.vt_frame_stats_get(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in FrameStats._fields_}
+
+    def update_materials(self, materials):
+        m = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1)
+        self.check(self.L.vt_update_materials(self.h, ptr(m), m.size // 16))
+
+    def update_lights(self, lights):
+        a = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1)
+        self.check(self.L.vt_update_lights(self.h, ptr(a), a.size // 4))
+
+    def backward(self, slots, adj_images, n_mat, n_light, spp_begin=0, spp_end=0, chunk=0):
+        """Gradients of sum adj . I w.r.t. material colours (n_mat x 3) and light radiances (n_light x 3)."""
+        arr = np.ascontiguousarray(slots, dtype=np.uint32)
+        imgs = [np.ascontiguousarray(a, dtype=np.float32) for a in adj_images]
+        ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
+        gm = np.zeros((n_mat, 3), np.float32)
+        gl = np.zeros((max(n_light, 1), 3), np.float32)
+        g = Grads(ptr(gm), n_mat, ptr(gl), n_light, None, 0)
+        o = RenderOpts(0, spp_begin, spp_end, 0, chunk, 0)
+        self.check(self.L.vt_backward(self.h, ptr(arr), arr.size, ptrs, C.byref(o), C.byref(g)))
+        return gm, gl[:n_light]
 
     # ---------------------------------------------------------------- test entry points
     def trace(self, rays, any_hit=False):

@@ -151,9 +151,10 @@ def bench_c4(args):
 
 
 def bench_c5(args):
-    """Differentiation: targets rendered from the true materials and lights, parameters perturbed (albedo +-20 %,
-    emitters +-30 %); one forward frame and one path-replay backward pass of the loss 1/2 ||I - I_target||^2 over
-    three 1280x960 cameras at 32 spp; reports the backward / forward time ratio."""
+    """Differentiation: targets rendered from the true materials, lights and cameras, parameters perturbed (albedo
+    +-20 %, emitters +-30 %, KB4 k1..k4 +-0.02, pose +-1 degree / 5 mm); one forward frame and one path-replay backward pass of the loss 1/2 ||I - I_target||^2 over
+    three 1280x960 cameras at 32 spp, with the gradients of materials, lights and the cameras' intrinsics and
+    pose; reports the backward / forward time ratio."""
     import numpy as np
     from vtwin import lib, scenes
 
@@ -173,13 +174,25 @@ def bench_c5(args):
     lights[:, 0:3] *= (1 + r.uniform(-0.3, 0.3, n_light)[:, None]).astype(np.float32)
     ctx.update_materials(mats)
     ctx.update_lights(lights)
+    # cameras: KB4 k1..k4 +-0.02, orientation +-1 degree about a random axis, centre +-5 mm
+    for k, c in enumerate(cams):
+        c2 = dict(c)
+        lens = np.asarray(c["lens"], np.float32).copy()
+        lens[4:8] += r.uniform(-0.02, 0.02, 4).astype(np.float32)
+        axis = r.uniform(-1, 1, 3)
+        Rm = np.asarray(c["R"], np.float64).reshape(3, 3).T
+        Rm = scenes.rot_axis(axis / np.linalg.norm(axis), np.radians(r.uniform(-1, 1))) @ Rm
+        c2["R"] = Rm.T.reshape(-1).astype(np.float32)
+        c2["pos"] = (np.asarray(c["pos"], np.float64) + r.uniform(-0.005, 0.005, 3)).astype(np.float32)
+        c2["lens"] = lens
+        ctx.set_camera(k, c2)
     fwd, bwd, rays_f, rays_b = [], [], [], []
     for it in range(args.warmup + args.steps):
         ctx.render(slots)
         st = ctx.stats()
         imgs = [ctx.readback(k, lib.VT_KIND_LINEAR, c["width"], c["height"]) for k, c in enumerate(cams)]
         adj = [im - t for im, t in zip(imgs, targets)]
-        gm, gl = ctx.backward(slots, adj, n_mat, n_light)
+        gm, gl, gc = ctx.backward(slots, adj, n_mat, n_light, cameras=True)
         sb = ctx.stats()
         if it >= args.warmup:
             fwd.append(st["ms_render"])
@@ -197,15 +210,46 @@ def bench_c5(args):
     imgs2 = [ctx.readback(k, lib.VT_KIND_LINEAR, c["width"], c["height"]) for k, c in enumerate(cams)]
     loss2 = 0.5 * sum(float(np.sum((im[:, :, :3] - t[:, :, :3]).astype(np.float64) ** 2)) for im, t in zip(imgs2, targets))
     out = {"metric": METRIC, "config": {"workload": "differentiation: 2,013,432-tri scene, 3 KB4 cameras 1280x960, 32 spp, "
-                                                     "depth 5; gradients w.r.t. 256 material colours (rgb) and 16 emitters (rgb)"},
+                                                     "depth 5; gradients w.r.t. 256 material colours (rgb), 16 emitters (rgb) "
+                                                     "and 3 x (8 KB4 intrinsics + 6 pose)"},
            "value": round(float(np.mean(bwd)) / float(np.mean(fwd)), 3), "unit": "x (adjoint / forward GPU time)",
            "forward_ms": round(float(np.mean(fwd)), 2), "backward_ms": round(float(np.mean(bwd)), 2),
            "forward_mrays_s": round(float(np.mean(rays_f)) / (float(np.mean(fwd)) * 1e-3) / 1e6, 1),
            "backward_mrays_s": round(float(np.mean(rays_b)) / (float(np.mean(bwd)) * 1e-3) / 1e6, 1),
            "loss_perturbed": loss, "loss_after_gradient_step": loss2, "steps": args.steps,
-           "camera_parameter_gradients": "not provided (material and emitter gradients only)"}
+           "parameters": int(3 * 256 + 3 * n_light + gc.size),
+           "camera_gradients": [[round(float(v), 6) for v in row] for row in gc]}
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_copy(args):
+    """Counter calibration: one elementwise kernel reading 1 GiB and writing 1 GiB (y = x * 1), three dispatches.
+    Run under rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE); the counter's KiB per dispatch over 1,048,576 KiB is
+    the factor that scales the trace kernel's counters to bytes."""
+    import torch
+
+    n = 256 << 20
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"config": {"workload": "copy calibration: y = x * 1 over 2^28 float32 (1 GiB read, 1 GiB written)"},
+                      "bytes_read_per_dispatch": 4 * n, "bytes_written_per_dispatch": 4 * n,
+                      "check": float(y[-1].item())}), flush=True)
+
+
+def traffic_from_profile(name="k_render_c2_traffic.md"):
+    """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel) for the same workload, read from the committed summary; None when there is none."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
@@ -218,8 +262,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
-                    help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint")
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
+                    help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
+                         "copy: counter calibration kernel")
     ap.add_argument("--inner", type=int, default=32, help="BF16 sweeps per refinement step (c4)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -232,6 +277,8 @@ def main():
         return bench_c4(args)
     if args.config == "c5":
         return bench_c5(args)
+    if args.config == "copy":
+        return bench_copy(args)
     import numpy as np
     from vtwin import lib, scenes
 
@@ -294,7 +341,7 @@ def main():
         rays_launch = float(np.mean(rays_step)) / max(launches, 1)
         achieved = b_ray * rays_launch / (float(np.mean(ms_kernel)) * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                            "kernel": "k_render<256>", "bytes_per_ray": round(b_ray, 1),
                            "node_visits_per_ray": round(vn, 2), "tri_tests_per_ray": round(vt, 2)}
     print(json.dumps(out), flush=True)

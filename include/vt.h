@@ -89,8 +89,23 @@ typedef struct {
   uint32_t spp_end;
   uint32_t accumulate;   /* 0: clear the cameras' sums first; 1: add to them (progressive ranges) */
   uint32_t chunk;        /* queue chunk in 64-item waves (0 = tuned default) */
-  uint32_t block;        /* threads per block, 64..1024, multiple of 64 (0 = tuned default) */
+  uint32_t block;        /* threads per block: 64, 128 or 256 (0 = tuned default) */
+  uint32_t stack_lds;    /* traversal stack entries per lane kept in LDS, 8..24 (0 = tuned default); the rest of
+                            the 48-entry stack spills to global memory */
+  uint32_t queues;       /* batched: 1 global ray queue, or 8 queues (one per XCD, over contiguous eighths of the
+                            frame, drained by the blocks of that XCD first); 0 = tuned default */
+  uint32_t graph;        /* batched: 1 = run the frame's launches as a hipGraph, captured on first use and
+                            replayed while the cameras, sample range and launch shape stay the same */
 } vt_render_opts;
+
+/* Result of vt_autotune: the chosen launch shape (also made the context's default) and the probe timings. */
+typedef struct {
+  uint32_t block, stack_lds, chunk, queues;
+  uint32_t candidates;   /* launch shapes timed */
+  double probe_mrays_best;    /* Mrays/s of the chosen shape on the probe */
+  double probe_mrays_default; /* Mrays/s of the built-in default (256 threads, 16 LDS entries, chunk 4, 1 queue) */
+  double probe_fraction;      /* share of the frame's waves one probe traces */
+} vt_tune_result;
 
 /* Sensor parameters, the 256-B sensor uniform (gpu/sensor.js:158-177), raw-DN part. */
 typedef struct {
@@ -118,6 +133,8 @@ typedef struct {
   uint32_t stack_overflows; /* traversal stack overflows since upload (must be 0) */
   double ms_backward;    /* GPU time of the last vt_backward */
   double rays_backward;  /* rays traced by the last vt_backward (two walks per sample) */
+  uint32_t graph_used;   /* 1 if the last vt_render ran as a hipGraph */
+  uint32_t graph_builds; /* hipGraph captures since the context was created */
 } vt_frame_stats;
 
 /* ---------------------------------------------------------------- lifetime (TwinRenderer.create / destroy,
@@ -148,6 +165,13 @@ int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
 /* ---------------------------------------------------------------- per frame */
 /* renderCamera / renderFrame (gpu/renderer.js:974-996): enqueue the listed cameras' path tracing. */
 int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts);
+/* Autotuner (no reference counterpart; the reference fixes its workgroup shape): times the batched launch of
+ * the listed cameras over block {64, 128, 256} x LDS stack {8, 16, 24} x chunk {1, 4, 16} x queues {1, 8} on a
+ * probe of every `probe_stride`-th wave of the frame (0 = 32), picks the fastest in rays per second and makes it
+ * the context's default for fields of vt_render_opts left 0. opts: the sample range (as for vt_render). The
+ * listed cameras' accumulated images are cleared. Blocks until done. */
+int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts, uint32_t probe_stride,
+                vt_tune_result *out);
 /* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */
 int vt_sensor(vt_ctx *ctx, uint32_t cam, const vt_sensor_params *params);
 /* readback(name, kind) (gpu/renderer.js:1191-1231). Blocks until the data is ready.
@@ -167,12 +191,18 @@ int vt_frame_stats_get(vt_ctx *ctx, vt_frame_stats *out);
  * from its seeded random numbers, first for its value, then accumulating the derivative of every term (sampling
  * and MIS weights detached; interior term only, visibility discontinuities excluded). adj: one pointer per listed
  * camera to width*height*4 floats (rgb used). opts: the sample range as for vt_render. Buffers of vt_grads are
- * overwritten; their counts must equal the scene's material and light counts. cameras must be NULL (camera
- * parameter gradients are not provided by this build). */
+ * overwritten; their counts must equal the scene's material and light counts. cameras (optional): one row of
+ * VT_CAMERA_PARAMS per listed camera, the derivatives with respect to that camera's intrinsics and pose, by
+ * forward-mode tangents of the camera ray carried along the replayed path (same random numbers and triangles;
+ * interior term only: pixel-footprint and silhouette edges excluded). */
+/* Camera parameters of the adjoint, per listed camera: fx fy cx cy k1 k2 k3 k4 (the KB4 intrinsics of
+ * vt_camera.lens), then the pose: rx ry rz (a small rotation about the world axes applied to R, R' = exp([r]x) R)
+ * and tx ty tz (the camera centre in the world frame). */
+#define VT_CAMERA_PARAMS 14
 typedef struct {
   float *materials; uint32_t material_count;
   float *lights; uint32_t light_count;
-  float *cameras; uint32_t camera_count;
+  float *cameras; uint32_t camera_count;   /* VT_CAMERA_PARAMS per listed camera, or NULL */
 } vt_grads;
 int vt_backward(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const float *const *adj, const vt_render_opts *opts,
                 vt_grads *out);

@@ -74,6 +74,7 @@ def lib():
             "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
             "or_gather_dir": (None, [u32, u32, u32, vp, vp]),
             "or_backward_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
+            "or_camera_grad_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, C.c_double, vp, vp, i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -257,3 +258,18 @@ def backward_tile(sc, cam, x0, y0, w, h, adj, spp_begin=0, spp_end=None, nthread
     gl = np.zeros(max(S.light_count, 1) * 3)
     lib().or_backward_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(adj), ptr(gm), ptr(gl), nthreads)
     return gm.reshape(-1, 3), gl[: 3 * S.light_count].reshape(-1, 3)
+
+
+def camera_grad_tile(sc, cam, x0, y0, w, h, adj, rel_step=1e-6, spp_begin=0, spp_end=None, nthreads=8):
+    """FP64 central-difference gradient of sum adj . I over a tile with respect to the camera's 14 parameters
+    (fx fy cx cy k1..k4, rotation about the world axes, centre), interior term: per sample, with the random
+    numbers and the triangles held fixed. Returns (grad[14], pairs left out because the path changed)."""
+    S = scene_struct(sc)
+    Cm = camera_struct(cam)
+    spp_end = cam["spp"] if spp_end is None else spp_end
+    adj = np.ascontiguousarray(adj, dtype=np.float32)
+    g = np.zeros(14)
+    changed = np.zeros(1, np.uint64)
+    lib().or_camera_grad_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(adj), rel_step, ptr(g),
+                              ptr(changed), nthreads)
+    return g, int(changed[0])

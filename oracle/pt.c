@@ -164,10 +164,23 @@ static void eval_bsdf_dc(const or_mat *M, const double n[3], const double l[3], 
   for (int c = 0; c < 3; c++) out[c] = pow(M->tau[c], 1 / nl + 1 / nv) * t * metal * one_m;
 }
 
-static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
-                              uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G) {
+/* seq (optional): a running hash of every ray's result (hit triangle or miss), so a re-run with a perturbed
+ * camera ray can tell whether the path kept its triangles and visibility; ray0 (optional): the camera ray
+ * (o, d) in place of the camera's own */
+static void seq_add(uint64_t *seq, int64_t tri) {
+  if (seq) *seq = (*seq ^ (uint64_t)(tri + 2)) * 0x100000001b3ull;
+}
+
+static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
+                                 uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G,
+                                 const double *ray0, uint64_t *seq) {
   double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
-  cam_dir(C, P, x, y, i, d);
+  if (ray0) {
+    memcpy(o, ray0, sizeof o);
+    memcpy(d, ray0 + 3, sizeof d);
+  } else {
+    cam_dir(C, P, x, y, i, d);
+  }
   double thr[3] = {1, 1, 1}, L[3] = {0, 0, 0}, pB = 0;
   const uint32_t nl = C->light_samples < 1 ? 1 : C->light_samples > 16 ? 16 : C->light_samples;
   const uint32_t D = C->max_depth < 1 ? 1 : C->max_depth;
@@ -176,6 +189,7 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
     or_hit h;
     or_traverse(S->nodes, S->tris, o, d, 0, INFINITY, 0, 0, &h);
     (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
+    seq_add(seq, h.tri);
     if (h.tri < 0) break;
     surf_t s;
     surface(S, &h, o, d, &s);
@@ -230,6 +244,7 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
       or_hit sh;
       or_traverse(S->nodes, S->tris, so, wi, 0, dist - 2e-5, 1, 0, &sh);
       (*rays)++; visits[0] += sh.node_visits; visits[1] += sh.tri_tests;
+      seq_add(seq, sh.tri >= 0 ? 0 : -1);
       if (sh.tri >= 0) continue;
       double pW = d2 / (cosE * A);
       double w = 1;
@@ -275,6 +290,11 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
   }
   memcpy(out, L, sizeof(L));
   return 0;
+}
+
+static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
+                              uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G) {
+  return sample_radiance_ex(S, C, P, x, y, i, out, rays, visits, G, NULL, NULL);
 }
 
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -333,4 +353,106 @@ void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32
     free(gm);
     free(gl);
   }
+}
+
+/* ---------------------------------------------------------------- camera-parameter gradients (interior term)
+ * theta = (fx, fy, cx, cy, k1..k4, r, t): intrinsics of the KB4 (or pinhole) camera, a rotation r about the
+ * world axes applied to R (R' = exp([r]x) R) and the camera centre t. For every sample, central differences
+ * in FP64 of its path radiance with the image point, the random numbers and the triangles held fixed: the
+ * ray of theta +- h e_k is built in FP64 (gpu/lens.js:54-62 unproject), the path re-traced, and the pair is
+ * kept only if both re-traces hit the same triangles with the same shadow-ray outcomes as the unperturbed one
+ * (else h is shrunk up to three times; a sample still changing is counted in *changed and left out: that is
+ * the boundary term, which is excluded). Gradient of sum_p adj_p . I_p (I = mean over the sample range). */
+static void rodrigues(const double r[3], double M[9]) {
+  double th = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  double kx = 0, ky = 0, kz = 0, c = 1, sn = 0;
+  if (th > 0) { kx = r[0] / th; ky = r[1] / th; kz = r[2] / th; c = cos(th); sn = sin(th); }
+  double v = 1 - c;
+  /* row-major M: M[3*i + j] */
+  M[0] = c + kx * kx * v;      M[1] = kx * ky * v - kz * sn; M[2] = kx * kz * v + ky * sn;
+  M[3] = ky * kx * v + kz * sn; M[4] = c + ky * ky * v;      M[5] = ky * kz * v - kx * sn;
+  M[6] = kz * kx * v - ky * sn; M[7] = kz * ky * v + kx * sn; M[8] = c + kz * kz * v;
+}
+
+static void ray_of_theta(const or_camera *C, const double th[14], double u, double v, double ray[6]) {
+  double dc[3];
+  if (C->mode == 0) {
+    or_lens L = {th[0], th[1], th[2], th[3], C->lens[8], {th[4], th[5], th[6], th[7]}};
+    double duv[2] = {u, v};
+    or_unproject(&L, duv, dc);
+  } else {
+    dc[0] = (u - th[2]) / th[0];
+    dc[1] = (v - th[3]) / th[1];
+    dc[2] = 1;
+    norm3(dc);
+  }
+  double d0[3];
+  for (int k = 0; k < 3; k++) d0[k] = (double)C->R[k] * dc[0] + (double)C->R[3 + k] * dc[1] + (double)C->R[6 + k] * dc[2];
+  double M[9];
+  rodrigues(th + 8, M);
+  for (int k = 0; k < 3; k++) ray[3 + k] = M[3 * k] * d0[0] + M[3 * k + 1] * d0[1] + M[3 * k + 2] * d0[2];
+  norm3(ray + 3);
+  for (int k = 0; k < 3; k++) ray[k] = th[11 + k];
+}
+
+void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         uint32_t spp_begin, uint32_t spp_end, const float *adj, double rel_step, double *grad,
+                         uint64_t *changed, int nthreads) {
+  const double sc = 1.0 / (double)(spp_end - spp_begin);
+  double th0[14];
+  for (int k = 0; k < 8; k++) th0[k] = C->lens[k];
+  th0[8] = th0[9] = th0[10] = 0;
+  for (int k = 0; k < 3; k++) th0[11 + k] = C->pos[k];
+  double step[14];
+  for (int k = 0; k < 14; k++) step[k] = rel_step;
+  for (int k = 0; k < 4; k++) step[k] = rel_step * fabs(th0[0]);
+  uint64_t nch = 0;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : nch)
+  {
+    double g[14] = {0};
+#pragma omp for schedule(dynamic, 1) collapse(2)
+    for (uint32_t yy = 0; yy < h; yy++)
+      for (uint32_t xx = 0; xx < w; xx++) {
+        const float *a = adj + 4 * ((size_t)yy * w + xx);
+        if (a[0] == 0 && a[1] == 0 && a[2] == 0) continue;
+        const uint32_t x = x0 + xx, y = y0 + yy;
+        pix_t P;
+        pix_seeds(C, x, y, &P);
+        for (uint32_t i = spp_begin; i < spp_end; i++) {
+          float uvf[2];
+          {
+            volatile float ua = (float)x + or_kr(i, R2A, P.s[0]);
+            volatile float va = (float)y + or_kr(i, R2B, P.s[1]);
+            uvf[0] = ua - 0.5f; uvf[1] = va - 0.5f;
+          }
+          double ray[6], L0[3];
+          uint64_t r = 0, vv[2] = {0, 0}, seq0 = 0;
+          ray_of_theta(C, th0, uvf[0], uvf[1], ray);
+          sample_radiance_ex(S, C, &P, x, y, i, L0, &r, vv, NULL, ray, &seq0);
+          for (int k = 0; k < 14; k++) {
+            double hk = step[k];
+            int ok = 0;
+            double Lp[3], Lm[3];
+            for (int tries = 0; tries < 4 && !ok; tries++, hk /= 16) {
+              double th[14];
+              memcpy(th, th0, sizeof th);
+              uint64_t sp = 0, sm = 0;
+              th[k] = th0[k] + hk;
+              ray_of_theta(C, th, uvf[0], uvf[1], ray);
+              sample_radiance_ex(S, C, &P, x, y, i, Lp, &r, vv, NULL, ray, &sp);
+              th[k] = th0[k] - hk;
+              ray_of_theta(C, th, uvf[0], uvf[1], ray);
+              sample_radiance_ex(S, C, &P, x, y, i, Lm, &r, vv, NULL, ray, &sm);
+              ok = sp == seq0 && sm == seq0;
+              if (ok)
+                for (int c = 0; c < 3; c++) g[k] += a[c] * sc * (Lp[c] - Lm[c]) / (2 * hk);
+            }
+            if (!ok) nch++;
+          }
+        }
+      }
+#pragma omp critical
+    for (int k = 0; k < 14; k++) grad[k] += g[k];
+  }
+  if (changed) *changed = nch;
 }

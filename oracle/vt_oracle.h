@@ -155,6 +155,12 @@ void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t
 void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                       uint32_t spp_begin, uint32_t spp_end, const float *adj, double *grad_mat, double *grad_light,
                       int nthreads);
+/* camera-parameter gradient (interior term) by FP64 central differences per sample; grad[14] accumulates
+ * (fx fy cx cy k1..k4, rotation about the world axes, camera centre); *changed counts (sample, parameter)
+ * pairs whose perturbed paths changed triangles (left out) */
+void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         uint32_t spp_begin, uint32_t spp_end, const float *adj, double rel_step, double *grad,
+                         uint64_t *changed, int nthreads);
 /* camera ray of sub-sample i of pixel (x, y): o, d (FP32 sample pattern, FP64 unproject, rounded to FP32) */
 void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float o[3], float d[3]);
 

@@ -2,8 +2,9 @@
 render (common random numbers), and the GPU's gradients against the oracle's on the same samples.
 
 Loss = sum_p adj_p . I_p with a fixed adjoint image adj (the gradient of any image loss at the current image);
-parameters: material colours (rho_d / F0, rgb) and light radiances (rgb). Bar (north_star): within 1e-3
-relative of finite differences.
+parameters: material colours (rho_d / F0, rgb), light radiances (rgb), and each camera's intrinsics and pose
+(fx fy cx cy k1..k4, rotation, centre; interior term: image points, random numbers and triangles held fixed).
+Bar: within 1e-3 relative of finite differences.
 """
 import numpy as np
 import pytest
@@ -86,3 +87,72 @@ def test_gpu_adjoint_matches_oracle_and_fd():
     for m, ch in big:
         fd = (_loss(_with(sc, "materials", m, ch, 1e-3), cam, adj) - _loss(_with(sc, "materials", m, ch, -1e-3), cam, adj)) / 2e-3
         assert gm[m, ch] == pytest.approx(fd, rel=1e-3)
+
+
+def _cam_grads_gpu(sc, cam, adj_full, n_mat, n_light):
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(sc)
+        ctx.set_camera(0, cam)
+        gm, gl, gc = ctx.backward([0], [adj_full], n_mat, n_light, cameras=True)
+        gm2, gl2 = ctx.backward([0], [adj_full], n_mat, n_light)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    return gm, gl, gc[0], gm2, gl2, st
+
+
+@pytest.mark.gpu
+def test_gpu_camera_gradients_match_oracle():
+    """Camera intrinsics and pose: the GPU's forward-mode tangents (FP32) against the oracle's FP64 per-sample
+    central differences on the same samples (interior term), on natural scales (pixels, radians, metres)."""
+    sc, cam, adj = _setup()
+    x0, y0, w, h = TILE
+    full = np.zeros((cam["height"], cam["width"], 4), np.float32)
+    full[y0:y0 + h, x0:x0 + w] = adj
+    n_mat, n_light = sc["materials"].size // 16, sc["lights"].size // 4
+    gm, gl, gc, gm2, gl2, st = _cam_grads_gpu(sc, cam, full, n_mat, n_light)
+    want, changed = oracle.camera_grad_tile(sc, cam, x0, y0, w, h, adj)
+    scale = np.array([cam["lens"][0]] * 4 + [1.0] * 10)
+    a, b = gc * scale, want * scale
+    print("GPU   ", np.array2string(a, precision=5), "\noracle", np.array2string(b, precision=5), "\nchanged pairs", changed)
+    assert np.all(np.abs(a - b) <= 1e-3 * np.abs(b) + 2e-4 * np.abs(b).max())
+    # the material and light gradients do not depend on whether camera gradients ride along
+    assert np.allclose(gm, gm2, rtol=1e-5, atol=1e-7 * np.abs(gm2).max())
+    assert np.allclose(gl, gl2, rtol=1e-5, atol=1e-7 * np.abs(gl2).max())
+    assert st["stack_overflows"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 0])
+def test_gpu_camera_gradients_closed_form(mode):
+    """The plane under a square emitter: the GPU camera gradient against the closed form's derivative."""
+    from test_oracle_pins import _plane_and_light, _plane_image
+    sc, _ = _plane_and_light()
+    pos = (0.15, 0.05, 0.5)
+    if mode == 1:
+        lens, tile = [400, 400, 31.5, 31.5, 0, 0, 0, 0, 0], (24, 28, 8, 8)
+    else:
+        lens, tile = [100, 100, 31.5, 31.5, 0.03, -0.02, 0.01, -0.005, 0], (50, 40, 8, 8)
+    cam = scenes.camera(scenes.look_at(pos, (0.15, 0.05, 0.0), up=(0, 1, 0)), pos, lens, 64, 64, 256, 0, 1, mode=mode)
+    x0, y0, w, h = tile
+    full = np.zeros((64, 64, 4), np.float32)
+    full[y0:y0 + h, x0:x0 + w, 0:3] = 1.0
+    _, _, gc, _, _, _ = _cam_grads_gpu(sc, cam, full, 2, 1)
+    verts = np.array([[-0.2, -0.2, 1.0], [0.2, -0.2, 1.0], [0.2, 0.2, 1.0], [-0.2, 0.2, 1.0]])
+    ys, xs = np.mgrid[y0:y0 + h, x0:x0 + w].astype(float)
+    xs, ys = xs.ravel(), ys.ravel()
+    th0 = np.zeros(14)
+    th0[0:8] = np.asarray(lens[0:8], float)
+    th0[11:14] = pos
+    want = np.zeros(14)
+    for k in range(14):
+        hk = 1e-6 * (lens[0] if k < 4 else 1.0)
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += hk
+        tm[k] -= hk
+        want[k] = 3 * (_plane_image(tp, cam, xs, ys, verts, 0.5, 10.0).sum() - _plane_image(tm, cam, xs, ys, verts, 0.5, 10.0).sum()) / (2 * hk)
+    scale = np.array([lens[0]] * 4 + [1.0] * 10)
+    a, b = gc * scale, want * scale
+    print("GPU   ", np.array2string(a, precision=4), "\nclosed", np.array2string(b, precision=4))
+    assert np.all(np.abs(a - b) <= 0.03 * np.abs(b) + 0.005 * np.abs(b).max())

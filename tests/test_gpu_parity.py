@@ -300,3 +300,58 @@ def test_render_white_furnace_closed_form(depth):
     ref, _, _ = oracle.render_tile(sc, cam, 16, 16, 32, 32, 0, 256)
     a, b = img[16:48, 16:48, 0:3].astype(np.float64), ref[:, :, 0:3].astype(np.float64)
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-3
+
+
+def _render_img(ctx, **kw):
+    ctx.render([0], **kw)
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    st = ctx.stats()
+    assert st["stack_overflows"] == 0
+    return img, st
+
+
+def test_launch_shapes_give_identical_images(ctx, c1):
+    """Block size, LDS stack depth, queue chunk and the per-XCD queues change only the schedule: with 16 spp every
+    pixel's samples are summed inside one wave, so the images are bit-identical."""
+    _, cams = c1
+    ctx.set_camera(0, cams[0])
+    ref, _ = _render_img(ctx)
+    for block, stack, chunk, queues in [(64, 8, 1, 1), (128, 24, 16, 8), (256, 8, 4, 8), (256, 24, 1, 1)]:
+        img, st = _render_img(ctx, block=block, stack_lds=stack, chunk=chunk, queues=queues)
+        assert np.array_equal(img, ref), (block, stack, chunk, queues)
+    img, _ = _render_img(ctx, mode=lib.VT_RENDER_PER_CAMERA, block=64, stack_lds=8)
+    assert np.array_equal(img, ref)
+
+
+def test_graph_replays_match_eager(ctx, c1):
+    """The batched frame as a hipGraph: captured once, replayed with eager work in between; every replay equals
+    the eager frame, and a new sample range captures a new graph."""
+    _, cams = c1
+    ctx.set_camera(0, cams[0])
+    ref, _ = _render_img(ctx)
+    ref2, _ = _render_img(ctx, spp_begin=4, spp_end=12)
+    b0 = ctx.stats()["graph_builds"]
+    for k in range(3):
+        img, st = _render_img(ctx, graph=True)
+        assert st["graph_used"] == 1 and st["graph_builds"] == b0 + 1
+        assert np.array_equal(img, ref), k
+        # eager work on the same stream between replays
+        e, st = _render_img(ctx, spp_begin=4, spp_end=12)
+        assert st["graph_used"] == 0 and np.array_equal(e, ref2)
+    img, st = _render_img(ctx, graph=True, spp_begin=4, spp_end=12)
+    assert st["graph_builds"] == b0 + 2 and np.array_equal(img, ref2)
+    img, st = _render_img(ctx, graph=True)
+    assert st["graph_builds"] == b0 + 3 and np.array_equal(img, ref)
+
+
+def test_autotune_picks_a_valid_shape(ctx, c1):
+    _, cams = c1
+    ctx.set_camera(0, cams[0])
+    ref, _ = _render_img(ctx, block=256, stack_lds=16, chunk=4, queues=1)
+    r = ctx.autotune([0], probe_stride=8)
+    print("autotune:", r)
+    assert r["block"] in (64, 128, 256) and r["stack_lds"] in (8, 16, 24) and r["chunk"] in (1, 4, 16)
+    assert r["queues"] in (1, 8) and r["candidates"] == 54
+    assert r["probe_mrays_best"] >= r["probe_mrays_default"] > 0
+    img, _ = _render_img(ctx)          # the tuned default
+    assert np.array_equal(img, ref)

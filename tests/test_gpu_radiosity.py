@@ -88,7 +88,8 @@ def _dense_problem(n, r, seed=5):
     return F, e
 
 
-@pytest.mark.parametrize("method,tol,err", [(lib.VT_DENSE_FP64, 1e-12, 1e-10), (lib.VT_DENSE_FP32, 1e-6, 2e-5),
+# the solution is returned in binary32, so the FP64 solve is checked at binary32 rounding (6e-8)
+@pytest.mark.parametrize("method,tol,err", [(lib.VT_DENSE_FP64, 1e-12, 1e-7), (lib.VT_DENSE_FP32, 1e-6, 2e-5),
                                             (lib.VT_DENSE_BF16_REFINE, 1e-6, 1e-5)])
 def test_dense_solves_match_direct_solve(ctx, method, tol, err):
     n, r = 1024, 48

@@ -405,3 +405,91 @@ def test_estimator_white_furnace_sphere(depth):
     sc, cam, want = furnace_sphere(depth=depth)
     img, _, _ = oracle.render_tile(sc, cam, 16, 16, 32, 32, 0, 256)
     assert img[:, :, 0].mean() == pytest.approx(want, rel=4e-3)
+
+
+# ---------------------------------------------------------------------------- camera-parameter gradient
+def _polygon_irradiance(p, verts, Le):
+    """Irradiance on the plane z = const (normal +z) under a Lambertian polygon of radiance Le: Lambert's
+    contour formula E = Le/2 sum_i gamma_i (n . unit(r_i x r_i+1))."""
+    E = np.zeros(p.shape[0])
+    for i in range(len(verts)):
+        a, b = verts[i] - p, verts[(i + 1) % len(verts)] - p
+        an, bn = a / np.linalg.norm(a, axis=1, keepdims=True), b / np.linalg.norm(b, axis=1, keepdims=True)
+        g = np.arccos(np.clip((an * bn).sum(1), -1, 1))
+        c = np.cross(a, b)
+        E += 0.5 * g * (c / np.linalg.norm(c, axis=1, keepdims=True))[:, 2]
+    return Le * np.abs(E)
+
+
+def _rodrigues(r):
+    th = np.linalg.norm(r)
+    if th == 0:
+        return np.eye(3)
+    k = r / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+
+
+def _kb4_dir(u, v, th, alpha=0.0):
+    """camera-frame direction of image point (u, v) under KB4 intrinsics th[0:8], Newton in FP64"""
+    fx, fy, cx, cy, k1, k2, k3, k4 = th[0:8]
+    yd = (v - cy) / fy
+    xd = (u - cx) / fx - alpha * yd
+    thd = np.hypot(xd, yd)
+    t = thd.copy()
+    for _ in range(50):
+        t2 = t * t
+        t = t - (t * (1 + t2 * (k1 + t2 * (k2 + t2 * (k3 + t2 * k4)))) - thd) / (1 + t2 * (3 * k1 + t2 * (5 * k2 + t2 * (7 * k3 + t2 * 9 * k4))))
+    s = np.sin(t) / thd
+    return np.stack([xd * s, yd * s, np.cos(t)], 1)
+
+
+def _plane_image(th, cam, xs, ys, verts, rho, Le):
+    """closed-form image of the Lambertian plane z = 0 under the square emitter (depth 1) at pixel centres"""
+    if cam["mode"] == 1:
+        dc = np.stack([(xs - th[2]) / th[0], (ys - th[3]) / th[1], np.ones_like(xs)], 1)
+        dc /= np.linalg.norm(dc, axis=1, keepdims=True)
+    else:
+        dc = _kb4_dir(xs, ys, th)
+    R = np.asarray(cam["R"], float).reshape(3, 3).T
+    d = (_rodrigues(th[8:11]) @ (R @ dc.T)).T
+    o = th[11:14]
+    p = o[None, :] + d * (-o[2] / d[:, 2])[:, None]
+    return rho / math.pi * _polygon_irradiance(p, verts, Le)
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_camera_gradient_against_closed_form(mode):
+    """The oracle's per-sample central differences (interior term) against the derivative of the closed-form
+    image of a Lambertian plane under a square emitter, for every camera parameter: intrinsics, the rotation
+    about the world axes and the centre. Pinhole, and KB4 with a wide field so k1..k4 matter."""
+    sc, _ = _plane_and_light()
+    pos = (0.15, 0.05, 0.5)
+    if mode == 1:
+        lens, tile = [400, 400, 31.5, 31.5, 0, 0, 0, 0, 0], (24, 28, 8, 8)
+    else:
+        lens, tile = [100, 100, 31.5, 31.5, 0.03, -0.02, 0.01, -0.005, 0], (50, 40, 8, 8)
+    cam = scenes.camera(scenes.look_at(pos, (0.15, 0.05, 0.0), up=(0, 1, 0)), pos, lens, 64, 64, 256, 0, 1, mode=mode)
+    x0, y0, w, h = tile
+    adj = np.ones((h, w, 4), np.float32)
+    adj[:, :, 3] = 0
+    g, changed = oracle.camera_grad_tile(sc, cam, x0, y0, w, h, adj)
+    assert changed == 0
+    verts = np.array([[-0.2, -0.2, 1.0], [0.2, -0.2, 1.0], [0.2, 0.2, 1.0], [-0.2, 0.2, 1.0]])
+    ys, xs = np.mgrid[y0:y0 + h, x0:x0 + w].astype(float)
+    xs, ys = xs.ravel(), ys.ravel()
+    th0 = np.zeros(14)
+    th0[0:8] = np.asarray(lens[0:8], float)
+    th0[11:14] = pos
+    want = np.zeros(14)
+    for k in range(14):
+        hk = 1e-6 * (lens[0] if k < 4 else 1.0)
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += hk
+        tm[k] -= hk
+        want[k] = 3 * (_plane_image(tp, cam, xs, ys, verts, 0.5, 10.0).sum() - _plane_image(tm, cam, xs, ys, verts, 0.5, 10.0).sum()) / (2 * hk)
+    # compare on natural scales (pixels for fx..cy, radians for k and r, metres for t); Monte Carlo noise ~1 %
+    scale = np.array([lens[0]] * 4 + [1.0] * 10)
+    gs, ws = g * scale, want * scale
+    print("oracle", np.array2string(gs, precision=4), "\nclosed", np.array2string(ws, precision=4))
+    assert np.all(np.abs(gs - ws) <= 0.03 * np.abs(ws) + 0.005 * np.abs(ws).max())

@@ -58,6 +58,14 @@ struct vt_ctx {
   hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
   DevBuf grads;
   uint32_t n_mat = 0, n_light = 0;
+  // launch shape used for vt_render_opts fields left 0 (vt_autotune replaces it)
+  uint32_t def_block = DEFAULT_BLOCK, def_stack = STACK_LDS, def_chunk = 4, def_queues = 1;
+  int occ[3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (block 64/128/256, LDS depth)
+  // the frame graph: the batched frame's launches, replayed while their parameters match `gkey`
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey;
+  uint32_t graph_builds = 0;
+  bool graph_used = false;
   double rays_backward = 0;
   CamSlot cams[VT_MAX_CAMERAS];
   // radiosity
@@ -207,7 +215,7 @@ extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
   (void)occ;
   int rc = ensure(c, c->counters, 64);
   if (!rc) rc = ensure(c, c->overflow, 64);
-  if (!rc) rc = ensure(c, c->spill, c->spill_lanes * STACK_SPILL * sizeof(uint2));
+  if (!rc) rc = ensure(c, c->spill, c->spill_lanes * SPILL_ENTRIES * sizeof(uint2));
   if (!rc) rc = ensure(c, c->dcams, sizeof(DevCam) * VT_MAX_CAMERAS);
   if (!rc && hipHostMalloc((void **)&c->h_cams, sizeof(DevCam) * VT_MAX_CAMERAS, hipHostMallocDefault) != hipSuccess) rc = VT_ERR_HIP;
   hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1, &c->ev_b0, &c->ev_b1};
@@ -241,6 +249,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.adj);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1, c->ev_b0, c->ev_b1};
   for (hipEvent_t ev : evs)
     if (ev) (void)hipEventDestroy(ev);
@@ -421,28 +430,59 @@ extern "C" int vt_set_camera(vt_ctx *c, uint32_t cam, const vt_camera *k) {
 }
 
 // ====================================================================== render
-extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o) {
-  VT_LIVE(c);
-  if (!c->has_scene) return fail(c, VT_ERR_STATE, "render: no scene uploaded");
+namespace {
+
+struct Shape {
+  uint32_t block, stack, chunk, queues;
+};
+
+int check_shape(vt_ctx *c, const Shape &sh) {
+  if (sh.block != 64 && sh.block != 128 && sh.block != 256) return fail(c, VT_ERR_ARG, "render: block must be 64, 128 or 256");
+  if (sh.stack < (uint32_t)STACK_LDS_MIN || sh.stack > (uint32_t)STACK_LDS_MAX)
+    return fail(c, VT_ERR_ARG, "render: stack_lds must be %d..%d", STACK_LDS_MIN, STACK_LDS_MAX);
+  if (sh.queues != 1 && sh.queues != (uint32_t)XCD_COUNT) return fail(c, VT_ERR_ARG, "render: queues must be 1 or %d", XCD_COUNT);
+  if (sh.chunk == 0 || sh.chunk > 4096) return fail(c, VT_ERR_ARG, "render: chunk must be 1..4096");
+  return VT_OK;
+}
+
+Shape shape_of(const vt_ctx *c, const vt_render_opts &o) {
+  return Shape{o.block ? o.block : c->def_block, o.stack_lds ? o.stack_lds : c->def_stack,
+               o.chunk ? o.chunk : c->def_chunk, o.queues ? o.queues : c->def_queues};
+}
+
+// blocks of a persistent grid: every CU filled to the kernel's occupancy, within the spill slots
+int persistent_grid(vt_ctx *c, const Shape &sh, uint64_t waves, uint32_t &grid) {
+  const int bi = sh.block == 64 ? 0 : (sh.block == 128 ? 1 : 2);
+  int &occ = c->occ[bi][sh.stack];
+  if (occ == 0) {
+    VT_HIP(c, render_occupancy(sh.block, sh.stack, &occ));
+    if (occ < 1) occ = 1;
+  }
+  const uint64_t need = (waves * 64 + sh.block - 1) / sh.block;
+  uint64_t g = std::min<uint64_t>((uint64_t)occ * c->cu_count, max_blocks(c, sh.block));
+  grid = (uint32_t)std::max<uint64_t>(1, std::min(need, g));
+  return VT_OK;
+}
+
+// the camera table of a frame into the pinned staging area; returns the frame's queue items
+int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts &opt, uint64_t &items) {
   if (!cams || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "render: 1..%d cameras", VT_MAX_CAMERAS);
-  vt_render_opts opt{};
-  if (o) opt = *o;
-  const uint32_t block = opt.block ? opt.block : DEFAULT_BLOCK;
-  if (block != 64 && block != 128 && block != 256) return fail(c, VT_ERR_ARG, "render: block must be 64, 128 or 256");
-  const uint32_t chunk = opt.chunk ? opt.chunk : 4;
   for (uint32_t a = 0; a < n; a++) {
     if (cams[a] >= VT_MAX_CAMERAS || !c->cams[cams[a]].set) return fail(c, VT_ERR_STATE, "render: camera %u not set", cams[a]);
     for (uint32_t b = 0; b < a; b++)
       if (cams[a] == cams[b]) return fail(c, VT_ERR_ARG, "render: camera %u listed twice", cams[a]);
+    const uint32_t s0 = opt.spp_begin, s1 = opt.spp_end ? opt.spp_end : c->cams[cams[a]].c.spp;
+    if (s1 <= s0) return fail(c, VT_ERR_ARG, "render: empty sub-sample range [%u, %u)", s0, s1);
   }
   VT_HIP(c, hipSetDevice(c->device));
   // the staging table may still feed an earlier copy
   if (c->stage_pending) VT_HIP(c, hipEventSynchronize(c->ev_stage));
-  uint64_t items = 0;
+  items = 0;
   for (uint32_t a = 0; a < n; a++) {
     CamSlot &s = c->cams[cams[a]];
     const vt_camera &k = s.c;
     DevCam &d = c->h_cams[a];
+    std::memset(&d, 0, sizeof d);
     std::memcpy(d.R, k.R, sizeof d.R);
     std::memcpy(d.pos, k.pos, sizeof d.pos);
     d.mode = k.mode;
@@ -454,7 +494,6 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
     d.tiles_x = (k.width + 7) / 8;
     d.tiles_y = (k.height + 7) / 8;
     const uint32_t s0 = opt.spp_begin, s1 = opt.spp_end ? opt.spp_end : k.spp;
-    if (s1 <= s0) return fail(c, VT_ERR_ARG, "render: empty sub-sample range [%u, %u)", s0, s1);
     d.nspp = s1 - s0;
     d.spp_begin = s0;
     d.accumulate = opt.accumulate;
@@ -468,50 +507,141 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
   VT_HIP(c, hipEventRecord(c->ev_stage, c->stream));
   c->stage_pending = true;
-  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 8, c->stream));
-  for (uint32_t a = 0; a < n; a++) {
-    CamSlot &s = c->cams[cams[a]];
-    if (!opt.accumulate) {
-      VT_HIP(c, launch_zero_f4((float4 *)s.acc.p, (uint64_t)s.c.width * s.c.height, c->stream));
-      s.spp_total = 0;
-    }
-  }
-  RenderLaunch L{};
+  return VT_OK;
+}
+
+RenderLaunch base_launch(vt_ctx *c, const Shape &sh) {
+  RenderLaunch L;
+  std::memset(&L, 0, sizeof L);     // the launch record is also the frame graph's key: no stray padding bytes
   L.S = c->S;
   L.counters = (unsigned long long *)c->counters.p;
   L.overflow = (uint32_t *)c->overflow.p;
   L.spill = (uint2 *)c->spill.p;
   L.spill_stride = c->spill_lanes;
-  L.chunk = chunk;
-  L.block = block;
-  const uint32_t gmax = max_blocks(c, block);
-  VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
-  uint32_t launches = 0;
-  auto ensure_events = [&](uint32_t k) -> int {
-    while (c->ev_launch.size() < 2 * (size_t)k) {
-      hipEvent_t ev;
-      VT_HIP(c, hipEventCreate(&ev));
-      c->ev_launch.push_back(ev);
+  L.chunk = sh.chunk;
+  L.block = sh.block;
+  L.stack_lds = sh.stack;
+  L.queues = sh.queues;
+  L.wave_stride = 1;
+  return L;
+}
+
+// the batched frame's stream work after staging: clear the queue heads and counters, clear the sums, one
+// persistent launch
+int enqueue_batched(vt_ctx *c, uint32_t n, bool accumulate, const RenderLaunch &L) {
+  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+  if (!accumulate)
+    for (uint32_t a = 0; a < n; a++) {
+      const DevCam &d = c->h_cams[a];
+      VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
     }
-    return VT_OK;
-  };
+  VT_HIP(c, launch_render(L, c->stream));
+  return VT_OK;
+}
+
+int ensure_launch_events(vt_ctx *c, uint32_t k) {
+  while (c->ev_launch.size() < 2 * (size_t)k) {
+    hipEvent_t ev;
+    VT_HIP(c, hipEventCreate(&ev));
+    c->ev_launch.push_back(ev);
+  }
+  return VT_OK;
+}
+
+void drop_graph(vt_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->gkey.clear();
+}
+
+// capture enqueue_batched as a graph (fill kernels only: no memset or copy nodes)
+int build_graph(vt_ctx *c, uint32_t n, bool accumulate, const RenderLaunch &L) {
+  drop_graph(c);
+  VT_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_batched(c, n, accumulate, L);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+  const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    c->gexec = nullptr;
+    return hip_fail(c, ei, "hipGraphInstantiate");
+  }
+  c->graph_builds++;
+  return VT_OK;
+}
+
+}  // namespace
+
+extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "render: no scene uploaded");
+  vt_render_opts opt{};
+  if (o) opt = *o;
+  const Shape sh = shape_of(c, opt);
+  int rc = check_shape(c, sh);
+  if (rc) return rc;
+  if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
+  uint64_t items = 0;
+  if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
+  RenderLaunch L = base_launch(c, sh);
+  uint32_t launches = 0;
+  c->graph_used = false;
   if (opt.mode == VT_RENDER_BATCHED) {
-    int rc = ensure_events(1);
-    if (rc) return rc;
+    if ((rc = ensure_launch_events(c, 1))) return rc;
     L.cams = (const DevCam *)c->dcams.p;
     L.ncam = n;
     L.item_base = 0;
     L.total_waves = items / 64;
     L.persistent = 1;
-    uint64_t need = (L.total_waves * 64 + block - 1) / block;
-    L.grid = (uint32_t)std::min<uint64_t>(need, gmax);
-    VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
-    VT_HIP(c, launch_render(L, c->stream));
-    VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+    if ((rc = persistent_grid(c, sh, L.total_waves, L.grid))) return rc;
+    if (opt.graph) {
+      // key: the launch record, and the sums the graph clears
+      std::vector<uint8_t> key(sizeof L + 1 + n * sizeof(DevCam));
+      std::memcpy(key.data(), &L, sizeof L);
+      key[sizeof L] = opt.accumulate ? 1 : 0;
+      for (uint32_t a = 0; a < n; a++) {
+        const DevCam &d = c->h_cams[a];
+        uint64_t v[2] = {(uint64_t)(uintptr_t)d.acc, (uint64_t)d.width * d.height};
+        std::memcpy(key.data() + sizeof L + 1 + a * sizeof(DevCam), v, sizeof v);
+      }
+      if (!c->gexec || key != c->gkey) {
+        if ((rc = build_graph(c, n, opt.accumulate != 0, L))) return rc;
+        c->gkey = key;
+      }
+      VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+      VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+      VT_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+      VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+      c->graph_used = true;
+    } else {
+      VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+      VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+      if (!opt.accumulate)
+        for (uint32_t a = 0; a < n; a++) {
+          const DevCam &d = c->h_cams[a];
+          VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
+        }
+      VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+      VT_HIP(c, launch_render(L, c->stream));
+      VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+    }
     launches = 1;
   } else {
-    int rc = ensure_events(n);
-    if (rc) return rc;
+    if ((rc = ensure_launch_events(c, n))) return rc;
+    VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+    VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+    if (!opt.accumulate)
+      for (uint32_t a = 0; a < n; a++) {
+        const DevCam &d = c->h_cams[a];
+        VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
+      }
+    const uint32_t gmax = max_blocks(c, sh.block);
     for (uint32_t a = 0; a < n; a++) {
       const DevCam &d = c->h_cams[a];
       L.cams = (const DevCam *)c->dcams.p + a;
@@ -519,7 +649,8 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
       L.item_base = d.item_begin;
       L.total_waves = d.item_count / 64;
       L.persistent = 0;
-      uint64_t need = (L.total_waves * 64 + block - 1) / block;
+      L.queues = 1;
+      const uint64_t need = (L.total_waves * 64 + sh.block - 1) / sh.block;
       L.grid = (uint32_t)std::min<uint64_t>(need, gmax);
       VT_HIP(c, hipEventRecord(c->ev_launch[2 * a], c->stream));
       VT_HIP(c, launch_render(L, c->stream));
@@ -532,9 +663,102 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   c->render_timed = true;
   for (uint32_t a = 0; a < n; a++) {
     CamSlot &s = c->cams[cams[a]];
+    if (!opt.accumulate) s.spp_total = 0;
     s.spp_total += c->h_cams[a].nspp;
     s.rendered = true;
   }
+  return VT_OK;
+}
+
+extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o, uint32_t probe_stride,
+                           vt_tune_result *out) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "autotune: no scene uploaded");
+  if (!out) return fail(c, VT_ERR_ARG, "autotune: null result");
+  vt_render_opts opt{};
+  if (o) opt = *o;
+  opt.accumulate = 0;
+  const uint64_t stride = probe_stride ? probe_stride : 32;
+  uint64_t items = 0;
+  int rc = stage_frame(c, cams, n, opt, items);
+  if (rc) return rc;
+  const uint64_t frame_waves = items / 64;
+  const uint64_t probe_waves = (frame_waves + stride - 1) / stride;
+  if ((rc = ensure_launch_events(c, 1))) return rc;
+  auto probe = [&](const Shape &sh, double &mrays) -> int {
+    RenderLaunch L = base_launch(c, sh);
+    L.cams = (const DevCam *)c->dcams.p;
+    L.ncam = n;
+    L.total_waves = probe_waves;
+    L.wave_stride = stride;
+    L.persistent = 1;
+    int r = persistent_grid(c, sh, probe_waves, L.grid);
+    if (r) return r;
+    VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+    VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+    VT_HIP(c, launch_render(L, c->stream));
+    VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+    VT_HIP(c, hipEventSynchronize(c->ev_launch[1]));
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_launch[0], c->ev_launch[1]));
+    unsigned long long cnt[2];
+    VT_HIP(c, hipMemcpy(cnt, c->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+    mrays = ms > 0 ? (double)cnt[1] / (ms * 1e-3) / 1e6 : 0.0;
+    return VT_OK;
+  };
+  const Shape def{(uint32_t)DEFAULT_BLOCK, (uint32_t)STACK_LDS, 4u, 1u};
+  double m_def = 0, warm = 0;
+  if ((rc = probe(def, warm))) return rc;          // first touch of the scene and the caches
+  if ((rc = probe(def, m_def))) return rc;
+  Shape best = def;
+  double m_best = m_def;
+  uint32_t tried = 1;
+  const uint32_t blocks[] = {64, 128, 256}, stacks[] = {8, 16, 24}, chunks[] = {1, 4, 16}, queues[] = {1, (uint32_t)XCD_COUNT};
+  for (uint32_t b : blocks)
+    for (uint32_t st : stacks)
+      for (uint32_t ch : chunks)
+        for (uint32_t q : queues) {
+          const Shape sh{b, st, ch, q};
+          if (sh.block == def.block && sh.stack == def.stack && sh.chunk == def.chunk && sh.queues == def.queues) continue;
+          double m = 0;
+          if ((rc = probe(sh, m))) return rc;
+          tried++;
+          if (m > m_best) {
+            m_best = m;
+            best = sh;
+          }
+        }
+  // the winner and the default once more, so a single noisy probe does not decide
+  double m2 = 0, d2 = 0;
+  if ((rc = probe(best, m2))) return rc;
+  if ((rc = probe(def, d2))) return rc;
+  m_best = m2;
+  m_def = d2;
+  if (m_best < m_def) {
+    best = def;
+    m_best = m_def;
+  }
+  c->def_block = best.block;
+  c->def_stack = best.stack;
+  c->def_chunk = best.chunk;
+  c->def_queues = best.queues;
+  drop_graph(c);
+  // the probes left partial sums behind: clear the listed cameras
+  for (uint32_t a = 0; a < n; a++) {
+    CamSlot &s = c->cams[cams[a]];
+    VT_HIP(c, launch_zero_f4((float4 *)s.acc.p, (uint64_t)s.c.width * s.c.height, c->stream));
+    s.spp_total = 0;
+    s.rendered = false;
+  }
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  out->block = best.block;
+  out->stack_lds = best.stack;
+  out->chunk = best.chunk;
+  out->queues = best.queues;
+  out->candidates = tried;
+  out->probe_mrays_best = m_best;
+  out->probe_mrays_default = m_def;
+  out->probe_fraction = (double)probe_waves / (double)std::max<uint64_t>(1, frame_waves);
   return VT_OK;
 }
 
@@ -622,6 +846,8 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
   std::memset(out, 0, sizeof *out);
   unsigned long long cnt[8];
   VT_HIP(c, hipMemcpy(cnt, c->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+  out->graph_used = c->graph_used ? 1u : 0u;
+  out->graph_builds = c->graph_builds;
   uint32_t ov[4];
   VT_HIP(c, hipMemcpy(ov, c->overflow.p, sizeof ov, hipMemcpyDeviceToHost));
   out->rays = (double)cnt[1];
@@ -660,7 +886,8 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   VT_LIVE(c);
   if (!c->has_scene) return fail(c, VT_ERR_STATE, "backward: no scene uploaded");
   if (!cams || !adj || !out || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "backward: bad arguments");
-  if (out->cameras) return fail(c, VT_ERR_ARG, "backward: camera-parameter gradients are not provided by this build");
+  if (out->cameras && out->camera_count != n)
+    return fail(c, VT_ERR_SIZE, "backward: cameras needs %u x %d floats (one row per listed camera)", n, VT_CAMERA_PARAMS);
   if (!out->materials || out->material_count != c->n_mat) return fail(c, VT_ERR_SIZE, "backward: materials needs %u x 3 floats", c->n_mat);
   if (c->n_light && (!out->lights || out->light_count != c->n_light)) return fail(c, VT_ERR_SIZE, "backward: lights needs %u x 3 floats", c->n_light);
   vt_render_opts opt{};
@@ -709,8 +936,10 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   VT_HIP(c, hipEventRecord(c->ev_stage, c->stream));
   c->stage_pending = true;
   const size_t ng = 3 * (size_t)(c->n_mat + c->n_light);
-  if ((rc = ensure(c, c->grads, ng * 4))) return rc;
-  VT_HIP(c, launch_zero_f4((float4 *)c->grads.p, (ng + 3) / 4, c->stream));
+  const size_t ngc = out->cameras ? (size_t)VT_CAMERA_PARAMS * n : 0;
+  const size_t ng_pad = (ng + 3) / 4 * 4;          // camera gradients start on a float4 boundary
+  if ((rc = ensure(c, c->grads, (ng_pad + ngc) * 4))) return rc;
+  VT_HIP(c, launch_zero_f4((float4 *)c->grads.p, (ng_pad + ngc + 3) / 4, c->stream));
   VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 8, c->stream));
   RenderLaunch L{};
   L.S = c->S;
@@ -728,14 +957,16 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   L.grid = (uint32_t)std::min<uint64_t>((L.total_waves * 64 + 255) / 256, max_blocks(c, 256));
   float *gm = (float *)c->grads.p;
   VT_HIP(c, hipEventRecord(c->ev_b0, c->stream));
-  VT_HIP(c, launch_backward(L, gm, gm + 3 * (size_t)c->n_mat, c->n_mat, c->n_light, c->stream));
+  VT_HIP(c, launch_backward(L, gm, gm + 3 * (size_t)c->n_mat, c->n_mat, c->n_light, ngc ? gm + ng_pad : nullptr,
+                            c->stream));
   VT_HIP(c, hipEventRecord(c->ev_b1, c->stream));
   c->backward_timed = true;
   VT_HIP(c, hipStreamSynchronize(c->stream));
-  std::vector<float> g(ng);
-  VT_HIP(c, hipMemcpy(g.data(), c->grads.p, ng * 4, hipMemcpyDeviceToHost));
+  std::vector<float> g(ng_pad + ngc);
+  VT_HIP(c, hipMemcpy(g.data(), c->grads.p, g.size() * 4, hipMemcpyDeviceToHost));
   std::memcpy(out->materials, g.data(), 12 * (size_t)c->n_mat);
   if (c->n_light) std::memcpy(out->lights, g.data() + 3 * (size_t)c->n_mat, 12 * (size_t)c->n_light);
+  if (ngc) std::memcpy(out->cameras, g.data() + ng_pad, 4 * ngc);
   return VT_OK;
 }
 

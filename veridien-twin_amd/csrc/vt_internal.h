@@ -8,10 +8,19 @@
 
 namespace vt {
 
-// tuned launch shapes for the trace kernels (block threads; LDS stack entries per lane)
+// launch shapes for the trace kernels (block threads; LDS stack entries per lane). The render kernel takes
+// its LDS depth at run time (STACK_LDS_MIN..STACK_LDS_MAX) and spills to a global area of depth
+// STACK_TOTAL - LDS depth, so every configuration holds STACK_TOTAL entries per lane.
 constexpr int DEFAULT_BLOCK = 256;
 constexpr int STACK_LDS = 16;
 constexpr int STACK_SPILL = 32;
+constexpr int STACK_TOTAL = 48;
+constexpr int STACK_LDS_MIN = 8;
+constexpr int STACK_LDS_MAX = 24;
+constexpr int SPILL_ENTRIES = STACK_TOTAL - STACK_LDS_MIN;   // depth of the global spill area
+constexpr int XCD_COUNT = 8;
+constexpr int MAX_CAMS = 64;                                 // cameras per launch (VT_MAX_CAMERAS)
+constexpr int CAM_PARAMS = 14;                               // VT_CAMERA_PARAMS                                 // blocks are dispatched round-robin over them
 
 struct RenderLaunch {
   SceneDev S;
@@ -19,7 +28,7 @@ struct RenderLaunch {
   uint32_t ncam;
   uint64_t item_base;          // queue item of wave 0 (a camera's offset when it is launched alone)
   uint64_t total_waves;        // 64-item waves in the queue
-  unsigned long long *counters;// [0] queue head, [1] rays traced, [2] samples
+  unsigned long long *counters;// [0] queue head, [1] rays traced, [2] samples, [8..15] per-XCD queue heads
   uint32_t *overflow;          // stack overflow counter
   uint2 *spill;                // per-lane spill stacks
   uint64_t spill_stride;       // lanes with a spill slot (persistent grid size)
@@ -27,12 +36,18 @@ struct RenderLaunch {
   uint32_t persistent;         // 1: waves take chunks from the atomic queue head; 0: static grid stride
   uint32_t block;              // threads per block (64, 128, 256)
   uint32_t grid;               // blocks
+  uint32_t stack_lds;          // LDS stack entries per lane (STACK_LDS_MIN..STACK_LDS_MAX)
+  uint32_t queues;             // persistent: 1 global queue, or XCD_COUNT queues over contiguous eighths of
+                               // the waves (a block drains its XCD's eighth, then the others)
+  uint64_t wave_stride;        // queue wave w runs frame wave w * wave_stride (1; > 1 for tuning probes)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)
+// gcam (optional): CAM_PARAMS gradients per camera of the launch (fx fy cx cy k1..k4, rotation, centre)
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
-                           hipStream_t s);
+                           float *gcam, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 

@@ -3,6 +3,7 @@
 // frame of uneven cameras is one launch); the plain form maps one wave to each 64 items of one camera.
 // A wave's lanes hold consecutive sub-samples of the same pixel(s), so primary rays are coherent and the
 // pixel sum is a butterfly reduction within the wave (deterministic; one writer per pixel).
+#include "vt_dual.h"
 #include "vt_internal.h"
 
 namespace vt {
@@ -173,6 +174,204 @@ __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, 
   return path_walk<false>(S, C, P, x, y, i, st, nrays);
 }
 
+// ------------------------------------------------------------------ camera adjoint (interior term)
+// The sample's image point (the u, v camera_dir traces)
+__device__ __forceinline__ void camera_uv(const PixSeeds &P, uint32_t x, uint32_t y, uint32_t i, float &u, float &v) {
+  u = ((float)x + kr(i, R2A, P.s[0])) - 0.5f;
+  v = ((float)y + kr(i, R2B, P.s[1])) - 0.5f;
+}
+
+// d(camera-frame ray direction) / d(fx, fy, cx, cy, k1, k2, k3, k4) at image point (u, v): lens.wgsl.js:21-29
+// differentiated, the Newton inverse of theta_d(theta) by its implicit derivative
+// d theta = (d theta_d - sum_j theta^(2j+1) d k_j) / theta_d'(theta).
+__device__ __forceinline__ void camdir_jac(const DevCam &C, float u, float v, v3 J[8]) {
+  const float *L = C.lens;
+  const float fx = L[0], fy = L[1], cx = L[2], cy = L[3], al = L[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) J[j] = mk(0.f, 0.f, 0.f);
+  if (C.mode == 0) {
+    const float k1 = L[4], k2 = L[5], k3 = L[6], k4 = L[7];
+    const float yd = (v - cy) / fy, xd = (u - cx) / fx - al * yd;
+    const float thd = sqrtf(xd * xd + yd * yd);
+    if (thd < 1e-20f) return;     // the optical axis itself (measure zero)
+    float th = thd;
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+      const float t2 = th * th;
+      const float f = th * (1.0f + t2 * (k1 + t2 * (k2 + t2 * (k3 + t2 * k4)))) - thd;
+      const float fp = 1.0f + t2 * (3.0f * k1 + t2 * (5.0f * k2 + t2 * (7.0f * k3 + t2 * 9.0f * k4)));
+      th = th - f / fp;
+    }
+    const float t2 = th * th;
+    const float gp = 1.0f + t2 * (3.0f * k1 + t2 * (5.0f * k2 + t2 * (7.0f * k3 + t2 * 9.0f * k4)));
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    const float s = sn / thd;
+    const float dxd[4] = {-(u - cx) / (fx * fx), al * yd / fy, -1.0f / fx, al / fy};
+    const float dyd[4] = {0.0f, -yd / fy, 0.0f, -1.0f / fy};
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      const float dthd = (xd * dxd[p] + yd * dyd[p]) / thd;
+      const float dth = dthd / gp;
+      const float ds = (cs * dth) / thd - sn * dthd / (thd * thd);
+      J[p] = mk(dxd[p] * s + xd * ds, dyd[p] * s + yd * ds, -sn * dth);
+    }
+    float pw = th * t2;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float dth = -pw / gp;
+      const float ds = cs * dth / thd;
+      J[4 + j] = mk(xd * ds, yd * ds, -sn * dth);
+      pw *= t2;
+    }
+  } else {
+    const float qx = (u - cx) / fx, qy = (v - cy) / fy;
+    const float inv = 1.0f / sqrtf(qx * qx + qy * qy + 1.0f);
+    const v3 dc = mk(qx * inv, qy * inv, inv);
+    const v3 dq[4] = {mk(-qx / fx, 0.f, 0.f), mk(0.f, -qy / fy, 0.f), mk(-1.0f / fx, 0.f, 0.f), mk(0.f, -1.0f / fy, 0.f)};
+#pragma unroll
+    for (int p = 0; p < 4; p++) J[p] = (dq[p] - dc * dot(dc, dq[p])) * inv;
+  }
+}
+
+// The estimator of path_walk with forward-mode tangents of the camera ray (origin o0, direction d0): the same
+// random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken
+// on the value. Returns the radiance value; ws gets the six tangents of sum_ch delta_ch L_ch.
+__device__ __noinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t i, v3 o0,
+                                         v3 d0, v3 delta, Stack &st, uint32_t &nrays, float ws[ND]) {
+  const uint32_t hs = P.hs;
+  DV o = dv(o0), d = dv(d0);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    (k == 0 ? o.x : k == 1 ? o.y : o.z).g[k] = 1.0f;
+    (k == 0 ? d.x : k == 1 ? d.y : d.z).g[3 + k] = 1.0f;
+  }
+  DV thr = dv(mk(1.f, 1.f, 1.f));
+  v3 L = mk(0.f, 0.f, 0.f);
+  D Ls = dk(0.0f);                // sum_ch delta_ch L_ch
+  D pB = dk(0.0f);
+  const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+  const float fnl = (float)nl;
+  const uint32_t Dp = C.depth < 1 ? 1 : C.depth;
+  const float A = S.emit_area;
+  for (uint32_t b = 0; b < Dp; b++) {
+    Hit h;
+    nrays++;
+    if (!traverse<false>(S, val(o), val(d), 0.0f, INFINITY, h, st)) break;
+    const uint4 at = S.attr[h.tri];
+    const uint4 ms = S.meshes[at.w];
+    const float4 *tp = S.tri + 3 * (size_t)h.tri;
+    const v3 p0 = ld3(tp[0]), e1 = ld3(tp[1]) - p0, e2 = ld3(tp[2]) - p0;
+    const v3 nt = cross(e1, e2);
+    // the hit on the triangle's plane and its barycentrics, as functions of the ray
+    const D t = ddot(p0 - o, nt) / ddot(d, nt);
+    const DV p = o + d * t;
+    const DV q = p - p0;
+    const float inn = 1.0f / dot(nt, nt);
+    const D bu = ddot(dcross(q, dv(e2)), nt) * inn;
+    const D bv = ddot(dcross(dv(e1), q), nt) * inn;
+    v3 ng = normalize(nt);
+    const DV nsi = dnormalize(ld3(S.vnorm[at.x]) * (1.0f - bu - bv) + ld3(S.vnorm[at.y]) * bu + ld3(S.vnorm[at.z]) * bv);
+    DV ns = nsi;
+    const v3 dval = val(d);
+    if (dot(ng, dval) > 0.0f) ng = -ng;
+    if (dot(val(ns), ng) < 0.0f) ns = -ns;
+    if (-dot(val(ns), dval) < 1e-4f) ns = dv(ng);
+    if (ms.y & F_EMITTER) {
+      const D cosE = -ddot(d, nsi);
+      if (cosE.v > 0.0f) {
+        const v3 Le = ld3(S.lights[ms.z]);
+        D w = dk(1.0f);
+        if (b > 0) {
+          const D pL = t * t / (cosE * A);
+          const D pb2 = pB * pB;
+          w = pb2 / (pb2 + (fnl * fnl) * (pL * pL));
+        }
+        const DV c = thr * Le;
+        L = L + val(c) * w.v;
+        Ls = Ls + (c.x * delta.x + c.y * delta.y + c.z * delta.z) * w;
+      }
+      break;
+    }
+    const DV wo = -d;
+    const bool last = b + 1 == Dp;
+    DV Ln = dv(mk(0.f, 0.f, 0.f));
+    if (S.emit_count > 0) {
+      for (uint32_t j = 0; j < nl; j++) {
+        float u0, u1, u2;
+        if (b == 0) {
+          const uint32_t li = i * nl + j;
+          u0 = kr(li, KL0, P.s[3]); u1 = kr(li, KL1, P.s[4]); u2 = kr(li, KL2, P.s[5]);
+        } else {
+          u0 = u01(hash3(hs, i, 256u * b + 3u * j));
+          u1 = u01(hash3(hs, i, 256u * b + 3u * j + 1u));
+          u2 = u01(hash3(hs, i, 256u * b + 3u * j + 2u));
+        }
+        v3 pl;
+        const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+        DV wi = pl - p;
+        const D d2 = ddot(wi, wi);
+        const D dist = dsqrt(d2);
+        wi = wi * rcp(dist);
+        const D cosE = -ddot(wi, ld3(S.emit[4 * e + 3]));
+        if (cosE.v <= 0.0f) continue;
+        const D cosS = ddot(ns, wi);
+        const v3 wiv = val(wi);
+        if (cosS.v <= 0.0f || dot(ng, wiv) <= 0.0f) continue;
+        const uint32_t lid = __float_as_uint(S.emit[4 * e + 2].w);
+        const v3 Le = ld3(S.lights[lid]);
+        if (maxc(Le) <= 0.0f) continue;
+        const Mat M = load_mat(S.mats, ms.x);
+        const DV f = d_eval_bsdf(M, ns, wi, wo);
+        if (maxc(val(f)) <= 0.0f) continue;
+        const D pW = d2 / (cosE * A);
+        D w = dk(1.0f);
+        if (!last) {
+          const D pb = d_pdf_bsdf(M, ns, wo, wi);
+          const D a2 = (fnl * fnl) * (pW * pW);
+          w = a2 / (a2 + pb * pb);
+        }
+        const D k = cosS * w / pW;
+        const DV contrib = (f * Le) * k;
+        const v3 so = val(p) + ng * (dot(ng, wiv) > 0.0f ? EPS_OFF : -EPS_OFF);
+        Hit sh;
+        nrays++;
+        if (traverse<true>(S, so, wiv, 0.0f, dist.v - 2e-5f, sh, st)) continue;
+        Ln = Ln + contrib;
+      }
+    }
+    {
+      const DV c = thr * Ln * (1.0f / fnl);
+      L = L + val(c);
+      Ls = Ls + c.x * delta.x + c.y * delta.y + c.z * delta.z;
+    }
+    if (last) break;
+    float u0, u1, u2;
+    if (b == 0) {
+      u0 = kr(i, KB0, P.s[6]); u1 = kr(i, KB1, P.s[7]); u2 = kr(i, KB2, P.s[8]);
+    } else {
+      u0 = u01(hash3(hs, i, 256u * b + 200u));
+      u1 = u01(hash3(hs, i, 256u * b + 201u));
+      u2 = u01(hash3(hs, i, 256u * b + 202u));
+    }
+    const Mat M = load_mat(S.mats, ms.x);
+    DV l;
+    const D pdf = d_sample_bsdf(M, ns, wo, u0, u1, u2, l);
+    const v3 lv = val(l);
+    if (!(pdf.v > 0.0f) || dot(ng, lv) <= 0.0f) break;
+    const DV f = d_eval_bsdf(M, ns, l, wo);
+    const D cp = ddot(ns, l) / pdf;
+    thr = thr * (f * cp);
+    if (maxc(val(thr)) <= 0.0f) break;
+    o = p + ng * (dot(ng, lv) > 0.0f ? EPS_OFF : -EPS_OFF);
+    d = l;
+    pB = pdf;
+  }
+#pragma unroll
+  for (int k = 0; k < ND; k++) ws[k] = Ls.g[k];
+  return L;
+}
+
 __device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
   for (uint32_t off = 1; off < width; off <<= 1) v += __shfl_xor(v, (int)off, 64);
   return v;
@@ -228,8 +427,12 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
   }
 }
 
-// backward wave: replay each sample's path twice (primal value, then the gradient walk)
-__device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs) {
+// backward wave: replay each sample's path twice (its value, then the gradient walk). CAM: the first walk
+// carries the camera-ray tangents, contracted with the ray's derivative with respect to the camera's
+// intrinsics and pose, and the wave's sum is added to scam[camera * CAM_PARAMS + k].
+template <bool CAM>
+__device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs,
+                                             float *scam) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t g0 = Lc.item_base + wave * 64ull;
   uint32_t c = 0;
@@ -242,6 +445,9 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
   const uint32_t t = (uint32_t)(pix >> 6), r = (uint32_t)(pix & 63u);
   const uint32_t x = (t % C.tiles_x) * 8u + (r & 7u), y = (t / C.tiles_x) * 8u + (r >> 3);
   uint32_t nrays = 0;
+  float gc[CAM_PARAMS];
+#pragma unroll
+  for (int k = 0; k < CAM_PARAMS; k++) gc[k] = 0.0f;
   if (x < C.width && y < C.height) {
     const float4 a = C.adj[(size_t)y * C.width + x];
     const float s = C.scale / (float)nspp;
@@ -249,8 +455,37 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
     if (gs.delta.x != 0.0f || gs.delta.y != 0.0f || gs.delta.z != 0.0f) {
       PixSeeds P;
       pix_seeds(C, x, y, P);
-      const v3 Ltot = path_walk<false>(Lc.S, C, P, x, y, i, st, nrays);
+      v3 Ltot;
+      if (CAM) {
+        const v3 o0 = mk(C.pos[0], C.pos[1], C.pos[2]);
+        const v3 d0 = camera_dir(C, P, x, y, i);
+        float ws[ND];
+        Ltot = path_walk_tan(Lc.S, C, P, i, o0, d0, gs.delta, st, nrays, ws);
+        // origin: the camera centre itself; direction: rotation r gives d' = d + r x d, intrinsics through R dc
+        const v3 wd = mk(ws[3], ws[4], ws[5]);
+        const v3 gr = cross(d0, wd);
+        float u, v;
+        camera_uv(P, x, y, i, u, v);
+        v3 J[8];
+        camdir_jac(C, u, v, J);
+        const v3 rw = mk(C.R[0] * wd.x + C.R[1] * wd.y + C.R[2] * wd.z, C.R[3] * wd.x + C.R[4] * wd.y + C.R[5] * wd.z,
+                         C.R[6] * wd.x + C.R[7] * wd.y + C.R[8] * wd.z);
+#pragma unroll
+        for (int k = 0; k < 8; k++) gc[k] = dot(rw, J[k]);
+        gc[8] = gr.x; gc[9] = gr.y; gc[10] = gr.z;
+        gc[11] = ws[0]; gc[12] = ws[1]; gc[13] = ws[2];
+      } else {
+        Ltot = path_walk<false>(Lc.S, C, P, x, y, i, st, nrays);
+      }
       path_walk<true>(Lc.S, C, P, x, y, i, st, nrays, &gs, Ltot);
+    }
+  }
+  if (CAM) {
+#pragma unroll
+    for (int k = 0; k < CAM_PARAMS; k++) {
+      float v = gc[k];
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0 && v != 0.0f) atomicAdd(scam + c * CAM_PARAMS + k, v);
     }
   }
   uint32_t rs = nrays;
@@ -258,16 +493,19 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
   if (lane == 0) atomicAdd(&Lc.counters[1], (unsigned long long)rs);
 }
 
-template <int BLOCK>
+template <int BLOCK, bool CAM>
 __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
-                                                    uint32_t n_light, uint32_t use_lds) {
+                                                    uint32_t n_light, uint32_t use_lds, float *gcam) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
   __shared__ float s_t[STACK_LDS * BLOCK];
+  __shared__ float scam[CAM ? MAX_CAMS * CAM_PARAMS : 1];
   extern __shared__ float sgrad[];
   const uint32_t ng = 3 * (n_mat + n_light);
   if (use_lds) {
     for (uint32_t k = threadIdx.x; k < ng; k += BLOCK) sgrad[k] = 0.0f;
   }
+  if (CAM)
+    for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK) scam[k] = 0.0f;
   __syncthreads();
   Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
            (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
@@ -278,7 +516,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
     base = __shfl(base, 0, 64);
     if (base >= Lc.total_waves) break;
     const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
-    for (uint64_t w = base; w < end; w++) run_wave_bwd(Lc, w, st, gs);
+    for (uint64_t w = base; w < end; w++) run_wave_bwd<CAM>(Lc, w, st, gs, scam);
   }
   __syncthreads();
   if (use_lds) {
@@ -287,46 +525,66 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
       if (v != 0.0f) atomicAdd(k < 3 * n_mat ? gmat + k : glight + (k - 3 * n_mat), v);
     }
   }
+  if (CAM)
+    for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK)
+      if (scam[k] != 0.0f) atomicAdd(gcam + k, scam[k]);
 }
 
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
-                           hipStream_t s) {
+                           float *gcam, hipStream_t s) {
   const size_t lds = (size_t)3 * (n_mat + n_light) * sizeof(float);
   const uint32_t use_lds = lds <= 16384 ? 1u : 0u;
-  hipLaunchKernelGGL(k_backward<256>, dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight, n_mat,
-                     n_light, use_lds);
+  if (L.ncam > MAX_CAMS) return hipErrorInvalidValue;
+  if (gcam)
+    hipLaunchKernelGGL((k_backward<256, true>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight,
+                       n_mat, n_light, use_lds, gcam);
+  else
+    hipLaunchKernelGGL((k_backward<256, false>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight,
+                       n_mat, n_light, use_lds, gcam);
   return hipGetLastError();
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
-  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
-  __shared__ float s_t[STACK_LDS * BLOCK];
+  // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
+  extern __shared__ uint32_t s_stack[];
   Stack st;
-  st.lds_node = s_node;
-  st.lds_t = s_t;
+  st.lds_node = s_stack;
+  st.lds_t = reinterpret_cast<float *>(s_stack + Lc.stack_lds * BLOCK);
   st.spill = Lc.spill;
   st.tid = threadIdx.x;
   st.block = BLOCK;
-  st.slds = STACK_LDS;
-  st.spill_depth = STACK_SPILL;
+  st.slds = Lc.stack_lds;
+  st.spill_depth = STACK_TOTAL - Lc.stack_lds;
   st.spill_stride = Lc.spill_stride;
   st.gtid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   st.overflow = Lc.overflow;
   const uint32_t wib = threadIdx.x >> 6;
   if (Lc.persistent) {
-    for (;;) {
-      unsigned long long base = 0;
-      if ((threadIdx.x & 63u) == 0) base = atomicAdd(&Lc.counters[0], (unsigned long long)Lc.chunk);
-      base = __shfl(base, 0, 64);
-      if (base >= Lc.total_waves) break;
-      const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
-      for (uint64_t w = base; w < end; w++) run_wave(Lc, w, st);
+    // Queue q owns waves [total * q / nq, total * (q + 1) / nq). A block starts on its XCD's queue (blocks
+    // are dispatched round-robin over the XCDs), so neighbouring tiles share that XCD's L2, and moves on to
+    // the other queues when its own is empty. Every wave leaves once all nq queues are exhausted.
+    const uint32_t nq = Lc.queues;
+    const uint32_t home = nq > 1 ? blockIdx.x % nq : 0u;
+    for (uint32_t k = 0; k < nq; k++) {
+      const uint32_t qi = (home + k) % nq;
+      const uint64_t q0 = Lc.total_waves * qi / nq, q1 = Lc.total_waves * (qi + 1) / nq;
+      unsigned long long *head = nq > 1 ? &Lc.counters[8 + qi] : &Lc.counters[0];
+      for (;;) {
+        unsigned long long base = 0;
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(head, (unsigned long long)Lc.chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= q1 - q0) break;
+        const uint64_t b = q0 + base;
+        const uint64_t end = b + Lc.chunk < q1 ? b + Lc.chunk : q1;
+        for (uint64_t w = b; w < end; w++) run_wave(Lc, w * Lc.wave_stride, st);
+      }
     }
   } else {
     // static schedule: the waves of the grid stride over the queue
     const uint64_t stride = (uint64_t)gridDim.x * (BLOCK / 64);
-    for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride) run_wave(Lc, w, st);
+    for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride)
+      run_wave(Lc, w * Lc.wave_stride, st);
   }
 }
 
@@ -350,13 +608,28 @@ hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s) {
 }
 
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
+  if (L.stack_lds < STACK_LDS_MIN || L.stack_lds > STACK_LDS_MAX || L.wave_stride == 0 ||
+      (L.queues != 1 && L.queues != XCD_COUNT) || L.chunk == 0)
+    return hipErrorInvalidValue;
+  const size_t lds = (size_t)L.stack_lds * L.block * 8;
   switch (L.block) {
-    case 64: hipLaunchKernelGGL(k_render<64>, dim3(L.grid), dim3(64), 0, s, L); break;
-    case 128: hipLaunchKernelGGL(k_render<128>, dim3(L.grid), dim3(128), 0, s, L); break;
-    case 256: hipLaunchKernelGGL(k_render<256>, dim3(L.grid), dim3(256), 0, s, L); break;
+    case 64: hipLaunchKernelGGL(k_render<64>, dim3(L.grid), dim3(64), lds, s, L); break;
+    case 128: hipLaunchKernelGGL(k_render<128>, dim3(L.grid), dim3(128), lds, s, L); break;
+    case 256: hipLaunchKernelGGL(k_render<256>, dim3(L.grid), dim3(256), lds, s, L); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// resident blocks per CU of the render kernel at this block size and LDS stack depth
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu) {
+  const size_t lds = (size_t)stack_lds * block * 8;
+  switch (block) {
+    case 64: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<64>, 64, lds);
+    case 128: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<128>, 128, lds);
+    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<256>, 256, lds);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // ------------------------------------------------------------------ test kernels

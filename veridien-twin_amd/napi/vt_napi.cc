@@ -1,7 +1,8 @@
 // N-API (version 8) addon over the C ABI of include/vt.h: the native half of the JavaScript TwinRenderer in
-// js/renderer.mjs. Blocking calls (sync, readback, stats) run as napi_async_work and return Promises, as the
-// reference's queue.onSubmittedWorkDone / mapAsync do (gpu/renderer.js:986-996, 1191-1231); enqueue-only calls
-// (upload, setCamera, render, sensor) return synchronously and throw Error on a bad status.
+// js/renderer.mjs. Blocking calls (sync, readback, stats, autotune, backward) run as napi_async_work and return
+// Promises, as the reference's queue.onSubmittedWorkDone / mapAsync do (gpu/renderer.js:986-996, 1191-1231);
+// enqueue-only calls (upload, setCamera, render, sensor, updateMaterials, updateLights) return synchronously and
+// throw Error on a bad status.
 #define NAPI_VERSION 8
 #include <node_api.h>
 
@@ -245,6 +246,9 @@ napi_value Render(napi_env env, napi_callback_info info) {
     o.accumulate = (uint32_t)num(env, argv[2], "accumulate", 0);
     o.chunk = (uint32_t)num(env, argv[2], "chunk", 0);
     o.block = (uint32_t)num(env, argv[2], "block", 0);
+    o.stack_lds = (uint32_t)num(env, argv[2], "stackLds", 0);
+    o.queues = (uint32_t)num(env, argv[2], "queues", 0);
+    o.graph = (uint32_t)num(env, argv[2], "graph", 0);
   }
   int rc = vt_render(c, slots.data(), len, &o);
   if (rc) return throw_vt(env, c, rc, "vt_render");
@@ -290,10 +294,16 @@ struct Job {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
   vt_ctx *c = nullptr;
-  int kind = -1;          // -1 sync, -2 stats, >= 0 readback kind
+  int kind = -1;          // -1 sync, -2 stats, -3 autotune, -4 backward, >= 0 readback kind
   uint32_t slot = 0, w = 0, h = 0;
   std::vector<uint8_t> buf;
   vt_frame_stats st{};
+  std::vector<uint32_t> slots;            // autotune, backward
+  vt_render_opts opts{};
+  uint32_t probe_stride = 0;
+  vt_tune_result tune{};
+  std::vector<std::vector<float>> adj;    // backward: copies of the adjoint images
+  std::vector<float> gmat, glight, gcam;
   int rc = 0;
   std::string err;
 };
@@ -302,7 +312,15 @@ void job_exec(napi_env, void *data) {
   Job *j = static_cast<Job *>(data);
   if (j->kind == -1) j->rc = vt_sync(j->c);
   else if (j->kind == -2) j->rc = vt_frame_stats_get(j->c, &j->st);
-  else j->rc = vt_readback(j->c, j->slot, j->kind, j->buf.data(), j->buf.size());
+  else if (j->kind == -3)
+    j->rc = vt_autotune(j->c, j->slots.data(), (uint32_t)j->slots.size(), &j->opts, j->probe_stride, &j->tune);
+  else if (j->kind == -4) {
+    std::vector<const float *> ptrs;
+    for (auto &a : j->adj) ptrs.push_back(a.data());
+    vt_grads g{j->gmat.data(), (uint32_t)(j->gmat.size() / 3), j->glight.data(), (uint32_t)(j->glight.size() / 3),
+               j->gcam.empty() ? nullptr : j->gcam.data(), (uint32_t)(j->gcam.size() / VT_CAMERA_PARAMS)};
+    j->rc = vt_backward(j->c, j->slots.data(), (uint32_t)j->slots.size(), ptrs.data(), &j->opts, &g);
+  } else j->rc = vt_readback(j->c, j->slot, j->kind, j->buf.data(), j->buf.size());
   if (j->rc) j->err = vt_last_error(j->c);
 }
 
@@ -320,13 +338,42 @@ void job_done(napi_env env, napi_status, void *data) {
     napi_resolve_deferred(env, j->deferred, res);
   } else if (j->kind == -2) {
     napi_create_object(env, &res);
-    const char *names[] = {"rays", "samples", "msRender", "msSensor", "msKernelAvg", "launches", "stackOverflows"};
+    const char *names[] = {"rays", "samples", "msRender", "msSensor", "msKernelAvg", "launches", "stackOverflows",
+                           "msBackward", "raysBackward", "graphUsed", "graphBuilds"};
     double vals[] = {j->st.rays, j->st.samples, j->st.ms_render, j->st.ms_sensor, j->st.ms_kernel_avg,
-                     (double)j->st.launches, (double)j->st.stack_overflows};
-    for (int k = 0; k < 7; k++) {
+                     (double)j->st.launches, (double)j->st.stack_overflows, j->st.ms_backward, j->st.rays_backward,
+                     (double)j->st.graph_used, (double)j->st.graph_builds};
+    for (int k = 0; k < 11; k++) {
       napi_value v;
       napi_create_double(env, vals[k], &v);
       napi_set_named_property(env, res, names[k], v);
+    }
+    napi_resolve_deferred(env, j->deferred, res);
+  } else if (j->kind == -3) {
+    napi_create_object(env, &res);
+    const char *names[] = {"block", "stackLds", "chunk", "queues", "candidates", "probeMraysBest",
+                           "probeMraysDefault", "probeFraction"};
+    double vals[] = {(double)j->tune.block, (double)j->tune.stack_lds, (double)j->tune.chunk, (double)j->tune.queues,
+                     (double)j->tune.candidates, j->tune.probe_mrays_best, j->tune.probe_mrays_default,
+                     j->tune.probe_fraction};
+    for (int k = 0; k < 8; k++) {
+      napi_value v;
+      napi_create_double(env, vals[k], &v);
+      napi_set_named_property(env, res, names[k], v);
+    }
+    napi_resolve_deferred(env, j->deferred, res);
+  } else if (j->kind == -4) {
+    napi_create_object(env, &res);
+    std::vector<float> *parts[] = {&j->gmat, &j->glight, &j->gcam};
+    const char *names[] = {"materials", "lights", "cameras"};
+    for (int k = 0; k < 3; k++) {
+      if (k == 2 && j->gcam.empty()) continue;
+      napi_value ab, ta;
+      void *dst;
+      napi_create_arraybuffer(env, parts[k]->size() * 4, &dst, &ab);
+      if (!parts[k]->empty()) std::memcpy(dst, parts[k]->data(), parts[k]->size() * 4);
+      napi_create_typedarray(env, napi_float32_array, parts[k]->size(), ab, 0, &ta);
+      napi_set_named_property(env, res, names[k], ta);
     }
     napi_resolve_deferred(env, j->deferred, res);
   } else {
@@ -399,6 +446,111 @@ napi_value Readback(napi_env env, napi_callback_info info) {
   return queue_job(env, j);
 }
 
+bool slot_list(napi_env env, napi_value arr, std::vector<uint32_t> &out) {
+  uint32_t len = 0;
+  if (napi_get_array_length(env, arr, &len) != napi_ok) return false;
+  out.resize(len);
+  for (uint32_t i = 0; i < len; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    napi_get_value_uint32(env, e, &out[i]);
+  }
+  return true;
+}
+
+// autotune(ctx, slots, {sppBegin, sppEnd, probeStride}) -> Promise<{block, stackLds, chunk, queues, ...}>
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  Job *j = new Job();
+  j->c = c;
+  j->kind = -3;
+  if (!slot_list(env, argv[1], j->slots)) {
+    delete j;
+    napi_throw_type_error(env, nullptr, "autotune: camera slots must be an array");
+    return nullptr;
+  }
+  if (argc > 2) {
+    j->opts.spp_begin = (uint32_t)num(env, argv[2], "sppBegin", 0);
+    j->opts.spp_end = (uint32_t)num(env, argv[2], "sppEnd", 0);
+    j->probe_stride = (uint32_t)num(env, argv[2], "probeStride", 0);
+  }
+  return queue_job(env, j);
+}
+
+// backward(ctx, slots, adjImages[], nMat, nLight, {sppBegin, sppEnd, cameras}) -> Promise<{materials, lights[, cameras]}>
+napi_value Backward(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  Job *j = new Job();
+  j->c = c;
+  j->kind = -4;
+  uint32_t na = 0, nm = 0, nl = 0;
+  if (argc < 5 || !slot_list(env, argv[1], j->slots) || napi_get_array_length(env, argv[2], &na) != napi_ok ||
+      na != j->slots.size()) {
+    delete j;
+    napi_throw_type_error(env, nullptr, "backward: (ctx, slots, adjImages, nMat, nLight) with one image per slot");
+    return nullptr;
+  }
+  napi_get_value_uint32(env, argv[3], &nm);
+  napi_get_value_uint32(env, argv[4], &nl);
+  for (uint32_t i = 0; i < na; i++) {
+    napi_value e;
+    napi_get_element(env, argv[2], i, &e);
+    bool is_ta = false;
+    napi_is_typedarray(env, e, &is_ta);
+    napi_typedarray_type t;
+    size_t len = 0;
+    void *data = nullptr;
+    if (!is_ta || napi_get_typedarray_info(env, e, &t, &len, &data, nullptr, nullptr) != napi_ok ||
+        t != napi_float32_array) {
+      delete j;
+      napi_throw_type_error(env, nullptr, "backward: adjoint images must be Float32Array (w*h*4)");
+      return nullptr;
+    }
+    j->adj.emplace_back(static_cast<const float *>(data), static_cast<const float *>(data) + len);
+  }
+  j->gmat.assign(3 * (size_t)nm, 0.0f);
+  j->glight.assign(3 * (size_t)nl, 0.0f);
+  if (argc > 5) {
+    j->opts.spp_begin = (uint32_t)num(env, argv[5], "sppBegin", 0);
+    j->opts.spp_end = (uint32_t)num(env, argv[5], "sppEnd", 0);
+    if (num(env, argv[5], "cameras", 0) != 0) j->gcam.assign((size_t)VT_CAMERA_PARAMS * na, 0.0f);
+  }
+  return queue_job(env, j);
+}
+
+// updateMaterials(ctx, Float32Array 16 per material) / updateLights(ctx, Float32Array 4 per light)
+napi_value update_table(napi_env env, napi_callback_info info, bool mats) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  napi_typedarray_type t;
+  size_t len = 0;
+  void *data = nullptr;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[1], &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, argv[1], &t, &len, &data, nullptr, nullptr) != napi_ok ||
+      t != napi_float32_array) {
+    napi_throw_type_error(env, nullptr, "expected a Float32Array");
+    return nullptr;
+  }
+  const int rc = mats ? vt_update_materials(c, static_cast<const float *>(data), (uint32_t)(len / 16))
+                      : vt_update_lights(c, static_cast<const float *>(data), (uint32_t)(len / 4));
+  if (rc) return throw_vt(env, c, rc, mats ? "vt_update_materials" : "vt_update_lights");
+  return nullptr;
+}
+napi_value UpdateMaterials(napi_env env, napi_callback_info info) { return update_table(env, info, true); }
+napi_value UpdateLights(napi_env env, napi_callback_info info) { return update_table(env, info, false); }
+
 napi_value LastError(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
@@ -415,7 +567,8 @@ napi_value Init(napi_env env, napi_value exports) {
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
              {"setCamera", SetCamera}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
-             {"readback", Readback}, {"lastError", LastError}};
+             {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
+             {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights}};
   for (auto &f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

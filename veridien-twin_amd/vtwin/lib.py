@@ -17,10 +17,11 @@ EXPORTS = [
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
-    "vt_update_lights",
+    "vt_update_lights", "vt_autotune",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
+CAMERA_PARAMS = 14      # VT_CAMERA_PARAMS: fx fy cx cy k1 k2 k3 k4, rotation rx ry rz, centre tx ty tz
 
 
 class CreateInfo(C.Structure):
@@ -42,7 +43,12 @@ class Camera(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("mode", u32), ("spp_begin", u32), ("spp_end", u32), ("accumulate", u32), ("chunk", u32),
-                ("block", u32)]
+                ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("block", u32), ("stack_lds", u32), ("chunk", u32), ("queues", u32), ("candidates", u32),
+                ("probe_mrays_best", f64), ("probe_mrays_default", f64), ("probe_fraction", f64)]
 
 
 class SensorParams(C.Structure):
@@ -54,7 +60,8 @@ class SensorParams(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("rays", f64), ("samples", f64), ("ms_render", f64), ("ms_sensor", f64), ("ms_kernel_avg", f64),
-                ("launches", u32), ("stack_overflows", u32), ("ms_backward", f64), ("rays_backward", f64)]
+                ("launches", u32), ("stack_overflows", u32), ("ms_backward", f64), ("rays_backward", f64),
+                ("graph_used", u32), ("graph_builds", u32)]
 
 
 class Grads(C.Structure):
@@ -112,6 +119,7 @@ def load():
         "vt_backward": (i32, [vp, vp, u32, vp, vp, vp]),
         "vt_update_materials": (i32, [vp, vp, u32]),
         "vt_update_lights": (i32, [vp, vp, u32]),
+        "vt_autotune": (i32, [vp, vp, u32, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -196,10 +204,19 @@ class Context:
         c.scale = float(cam.get("scale", 1.0))
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
 
-    def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0):
+    def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,
+               stack_lds=0, queues=0, graph=False):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
-        o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block)
+        o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block, stack_lds, queues,
+                       1 if graph else 0)
         self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
+
+    def autotune(self, slots, spp_begin=0, spp_end=0, probe_stride=0):
+        arr = np.ascontiguousarray(slots, dtype=np.uint32)
+        o = RenderOpts(VT_RENDER_BATCHED, spp_begin, spp_end, 0, 0, 0, 0, 0, 0)
+        r = TuneResult()
+        self.check(self.L.vt_autotune(self.h, ptr(arr), arr.size, C.byref(o), probe_stride, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     def sensor(self, slot, params):
         p = sensor_struct(params)
@@ -231,16 +248,20 @@ class Context:
         a = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1)
         self.check(self.L.vt_update_lights(self.h, ptr(a), a.size // 4))
 
-    def backward(self, slots, adj_images, n_mat, n_light, spp_begin=0, spp_end=0, chunk=0):
-        """Gradients of sum adj . I w.r.t. material colours (n_mat x 3) and light radiances (n_light x 3)."""
+    def backward(self, slots, adj_images, n_mat, n_light, spp_begin=0, spp_end=0, chunk=0, cameras=False):
+        """Gradients of sum adj . I w.r.t. material colours (n_mat x 3) and light radiances (n_light x 3); with
+        cameras=True also w.r.t. each listed camera's 14 parameters (fx fy cx cy k1..k4, rotation, centre)."""
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
         imgs = [np.ascontiguousarray(a, dtype=np.float32) for a in adj_images]
         ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
         gm = np.zeros((n_mat, 3), np.float32)
         gl = np.zeros((max(n_light, 1), 3), np.float32)
-        g = Grads(ptr(gm), n_mat, ptr(gl), n_light, None, 0)
-        o = RenderOpts(0, spp_begin, spp_end, 0, chunk, 0)
+        gc = np.zeros((arr.size, CAMERA_PARAMS), np.float32) if cameras else None
+        g = Grads(ptr(gm), n_mat, ptr(gl), n_light, ptr(gc), arr.size if cameras else 0)
+        o = RenderOpts(0, spp_begin, spp_end, 0, chunk, 0, 0, 0, 0)
         self.check(self.L.vt_backward(self.h, ptr(arr), arr.size, ptrs, C.byref(o), C.byref(g)))
+        if cameras:
+            return gm, gl[:n_light], gc
         return gm, gl[:n_light]
 
     # ---------------------------------------------------------------- test entry points

@@ -240,7 +240,7 @@ def bench_copy(args):
                       "check": float(y[-1].item())}), flush=True)
 
 
-def traffic_from_profile(name="k_render_c2_traffic.md"):
+def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
     path = os.path.join(ROOT, "profiles", name)

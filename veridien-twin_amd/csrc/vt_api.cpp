@@ -875,7 +875,7 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
     float ms = 0;
     VT_HIP(c, hipEventElapsedTime(&ms, c->ev_b0, c->ev_b1));
     out->ms_backward = ms;
-    out->rays_backward = (double)cnt[5];
+    out->rays_backward = (double)cnt[5] + (double)cnt[7];
   }
   return VT_OK;
 }
@@ -957,8 +957,13 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   L.grid = (uint32_t)std::min<uint64_t>((L.total_waves * 64 + 255) / 256, max_blocks(c, 256));
   float *gm = (float *)c->grads.p;
   VT_HIP(c, hipEventRecord(c->ev_b0, c->stream));
-  VT_HIP(c, launch_backward(L, gm, gm + 3 * (size_t)c->n_mat, c->n_mat, c->n_light, ngc ? gm + ng_pad : nullptr,
-                            c->stream));
+  if (ngc) {
+    // the camera walk first, on its own queue head and ray counter ([6], [7])
+    RenderLaunch Lcam = L;
+    Lcam.counters = (unsigned long long *)c->counters.p + 6;
+    VT_HIP(c, launch_camgrad(Lcam, gm + ng_pad, c->stream));
+  }
+  VT_HIP(c, launch_backward(L, gm, gm + 3 * (size_t)c->n_mat, c->n_mat, c->n_light, c->stream));
   VT_HIP(c, hipEventRecord(c->ev_b1, c->stream));
   c->backward_timed = true;
   VT_HIP(c, hipStreamSynchronize(c->stream));

@@ -397,11 +397,27 @@ __device__ __forceinline__ v3 safe_inv(v3 d) {
             1.0f / copysignf(fmaxf(fabsf(d.z), 1e-20f), d.z));
 }
 
-// Closest (any_hit = false) or any hit in (tmin, tmax). Returns the number of node visits and triangle tests
-// through *work (x: nodes, y: triangles) when counting.
+// Closest (any_hit = false) or any hit in (tmin, tmax), "while-while" with postponed leaves (Aila and Laine,
+// HPG 2009): a lane walks interior nodes and sets aside the first leaf it reaches, and the wave tests leaves only
+// when every active lane holds one, so the node phase and the triangle phase each run with full lanes instead of
+// alternating inside one loop.
+constexpr uint32_t TRAV_DONE = 0xfffffffeu;
+__device__ __forceinline__ bool is_leaf_ref(uint32_t r) { return (r & LEAF_BIT) && r != TRAV_DONE; }
+
+// next entry nearer than the best hit, or TRAV_DONE
+__device__ __forceinline__ uint32_t pop_next(const Stack &st, uint32_t &sp, float best) {
+  while (sp > 0) {
+    sp--;
+    uint32_t r;
+    float t;
+    pop(st, sp, r, t);
+    if (t < best) return r;
+  }
+  return TRAV_DONE;
+}
+
 template <bool ANY>
-__device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, Hit &h, Stack &st,
-                                         uint32_t *vis_nodes = nullptr, uint32_t *vis_tris = nullptr) {
+__device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, Hit &h, Stack &st) {
   const v3 inv = safe_inv(d);
   const v3 oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const RayW R = ray_w(o, d);
@@ -409,25 +425,26 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tm
   bool found = false;
   uint32_t sp = 0;
   uint32_t ref = S.root;
-  uint32_t nv = 0, nt = 0;
-  for (;;) {
-    if (!(ref & LEAF_BIT)) {
+  uint32_t leaf = TRAV_DONE;      // postponed leaf
+  while (ref != TRAV_DONE || leaf != TRAV_DONE) {
+    // node phase
+    while (ref != TRAV_DONE && !(ref & LEAF_BIT)) {
       const float4 *np = reinterpret_cast<const float4 *>(S.nodes + ref);
-      float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
-      nv++;
-      float tn[4], lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
-      float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
-      float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      const uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
+      float tn[4];
       uint32_t rr[4] = {rf.x, rf.y, rf.z, rf.w};
+      const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
       const float tfar_lim = best * 1.00000024f;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
-        float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
-        float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
-        float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+        const float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
+        const float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
+        const float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
+        const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
         tn[c] = (a <= b && rr[c] != EMPTY_REF) ? a : INFINITY;
       }
       // sort the four (t, ref) pairs ascending: a five-comparator network
@@ -443,37 +460,42 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tm
         if (tn[2] < INFINITY) push(st, sp, rr[2], tn[2]);
         if (tn[1] < INFINITY) push(st, sp, rr[1], tn[1]);
         ref = rr[0];
-        continue;
+      } else {
+        ref = pop_next(st, sp, best);
       }
-    } else {
-      uint32_t first = ref & 0x03ffffffu, cnt = ((ref >> 26) & 31u) + 1u;
+      // set the first leaf aside and keep walking nodes while other lanes of the wave still need a leaf
+      if (is_leaf_ref(ref) && leaf == TRAV_DONE) {
+        leaf = ref;
+        ref = pop_next(st, sp, best);
+      }
+      if (!__any(leaf == TRAV_DONE)) break;
+    }
+    // triangle phase: the postponed leaf, then any leaf the node phase stopped on
+    while (leaf != TRAV_DONE) {
+      const uint32_t first = leaf & 0x03ffffffu, cnt = ((leaf >> 26) & 31u) + 1u;
       for (uint32_t k = first; k < first + cnt; k++) {
         const float4 *tp = S.tri + 3 * (size_t)k;
-        float4 a = tp[0], b = tp[1], c = tp[2];
+        const float4 a = tp[0], b = tp[1], c = tp[2];
         float t, u, v;
-        nt++;
         if (tri_hit(R, ld3(a), ld3(b), ld3(c), tmin, best, t, u, v)) {
           best = t;
           h.t = t; h.u = u; h.v = v; h.tri = k;
           found = true;
-          if (ANY) {
-            if (vis_nodes) { *vis_nodes += nv; *vis_tris += nt; }
-            return true;
-          }
+          if (ANY) break;
         }
       }
+      if (ANY && found) {
+        leaf = TRAV_DONE;
+        ref = TRAV_DONE;
+        break;
+      }
+      leaf = TRAV_DONE;
+      if (is_leaf_ref(ref)) {
+        leaf = ref;
+        ref = pop_next(st, sp, best);
+      }
     }
-    // pop the next entry nearer than the best hit
-    bool next = false;
-    while (sp > 0) {
-      sp--;
-      float t;
-      pop(st, sp, ref, t);
-      if (t < best) { next = true; break; }
-    }
-    if (!next) break;
   }
-  if (vis_nodes) { *vis_nodes += nv; *vis_tris += nt; }
   return found;
 }
 

@@ -1,13 +1,15 @@
-// Forward-mode derivatives for the camera adjoint: a value with ND tangents (the six components of the
-// camera ray, origin then direction), and the BSDF of vt_device.h restated on such values. Branches are taken
-// on the value, exactly as the float code takes them, so every function below is the derivative of its float
-// counterpart wherever that is differentiable (the interior term: no visibility or branch changes).
+// Forward-mode derivatives for the camera adjoint: a value with ND tangents, and the BSDF of vt_device.h
+// restated on such values. The tangents are four directions of the camera ray: the origin and the direction
+// each moved along two unit vectors perpendicular to the ray (a move along the ray leaves the first hit, and
+// so the path, unchanged, and the camera's unit direction only ever moves perpendicular to itself). Branches
+// are taken on the value, exactly as the float code takes them, so every function below is the derivative of
+// its float counterpart wherever that is differentiable (the interior term: no visibility or branch changes).
 #pragma once
 #include "vt_device.h"
 
 namespace vt {
 
-constexpr int ND = 6;
+constexpr int ND = 4;
 
 struct D {
   float v;

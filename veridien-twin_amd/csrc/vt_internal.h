@@ -45,9 +45,11 @@ hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
 hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)
-// gcam (optional): CAM_PARAMS gradients per camera of the launch (fx fy cx cy k1..k4, rotation, centre)
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
-                           float *gcam, hipStream_t s);
+                           hipStream_t s);
+// camera adjoint: CAM_PARAMS gradients per camera of the launch (fx fy cx cy k1..k4, rotation, centre), by
+// forward-mode tangents of each sample's camera ray (block 256, persistent queue; L.cams[*].adj set)
+hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 

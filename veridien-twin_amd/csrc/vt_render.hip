@@ -236,16 +236,17 @@ __device__ __forceinline__ void camdir_jac(const DevCam &C, float u, float v, v3
 
 // The estimator of path_walk with forward-mode tangents of the camera ray (origin o0, direction d0): the same
 // random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken
-// on the value. Returns the radiance value; ws gets the six tangents of sum_ch delta_ch L_ch.
-__device__ __noinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t i, v3 o0,
-                                         v3 d0, v3 delta, Stack &st, uint32_t &nrays, float ws[ND]) {
+// on the value. Tangents: origin along e1, e2, direction along e1, e2 (e1, e2 perpendicular to d0). Returns
+// the radiance value; ws gets the four tangents of sum_ch delta_ch L_ch.
+__device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t i,
+                                            v3 o0, v3 d0, v3 e1, v3 e2, v3 delta, Stack &st, uint32_t &nrays,
+                                            float ws[ND]) {
   const uint32_t hs = P.hs;
   DV o = dv(o0), d = dv(d0);
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    (k == 0 ? o.x : k == 1 ? o.y : o.z).g[k] = 1.0f;
-    (k == 0 ? d.x : k == 1 ? d.y : d.z).g[3 + k] = 1.0f;
-  }
+  o.x.g[0] = e1.x; o.y.g[0] = e1.y; o.z.g[0] = e1.z;
+  o.x.g[1] = e2.x; o.y.g[1] = e2.y; o.z.g[1] = e2.z;
+  d.x.g[2] = e1.x; d.y.g[2] = e1.y; d.z.g[2] = e1.z;
+  d.x.g[3] = e2.x; d.y.g[3] = e2.y; d.z.g[3] = e2.z;
   DV thr = dv(mk(1.f, 1.f, 1.f));
   v3 L = mk(0.f, 0.f, 0.f);
   D Ls = dk(0.0f);                // sum_ch delta_ch L_ch
@@ -427,65 +428,42 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
   }
 }
 
-// backward wave: replay each sample's path twice (its value, then the gradient walk). CAM: the first walk
-// carries the camera-ray tangents, contracted with the ray's derivative with respect to the camera's
-// intrinsics and pose, and the wave's sum is added to scam[camera * CAM_PARAMS + k].
-template <bool CAM>
-__device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs,
-                                             float *scam) {
-  const uint32_t lane = threadIdx.x & 63u;
+// the sample of lane `lane` in queue wave `wave`: its camera (index c in the launch), pixel and sub-sample
+struct WaveItem {
+  uint32_t c, x, y, i;
+  bool valid;
+};
+__device__ __forceinline__ WaveItem wave_item(const RenderLaunch &Lc, uint64_t wave, uint32_t lane) {
   const uint64_t g0 = Lc.item_base + wave * 64ull;
-  uint32_t c = 0;
-  while (c + 1 < Lc.ncam && g0 >= Lc.cams[c + 1].item_begin) c++;
-  const DevCam &C = Lc.cams[c];
+  WaveItem it;
+  it.c = 0;
+  while (it.c + 1 < Lc.ncam && g0 >= Lc.cams[it.c + 1].item_begin) it.c++;
+  const DevCam &C = Lc.cams[it.c];
   const uint64_t q = g0 - C.item_begin + lane;
-  const uint32_t nspp = C.nspp;
-  const uint64_t pix = q / nspp;
-  const uint32_t i = C.spp_begin + (uint32_t)(q - pix * nspp);
+  const uint64_t pix = q / C.nspp;
+  it.i = C.spp_begin + (uint32_t)(q - pix * C.nspp);
   const uint32_t t = (uint32_t)(pix >> 6), r = (uint32_t)(pix & 63u);
-  const uint32_t x = (t % C.tiles_x) * 8u + (r & 7u), y = (t / C.tiles_x) * 8u + (r >> 3);
+  it.x = (t % C.tiles_x) * 8u + (r & 7u);
+  it.y = (t / C.tiles_x) * 8u + (r >> 3);
+  it.valid = it.x < C.width && it.y < C.height;
+  return it;
+}
+
+// backward wave: replay each sample's path twice (its value, then the gradient walk)
+__device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const WaveItem it = wave_item(Lc, wave, lane);
+  const DevCam &C = Lc.cams[it.c];
   uint32_t nrays = 0;
-  float gc[CAM_PARAMS];
-#pragma unroll
-  for (int k = 0; k < CAM_PARAMS; k++) gc[k] = 0.0f;
-  if (x < C.width && y < C.height) {
-    const float4 a = C.adj[(size_t)y * C.width + x];
-    const float s = C.scale / (float)nspp;
+  if (it.valid) {
+    const float4 a = C.adj[(size_t)it.y * C.width + it.x];
+    const float s = C.scale / (float)C.nspp;
     gs.delta = mk(a.x * s, a.y * s, a.z * s);
     if (gs.delta.x != 0.0f || gs.delta.y != 0.0f || gs.delta.z != 0.0f) {
       PixSeeds P;
-      pix_seeds(C, x, y, P);
-      v3 Ltot;
-      if (CAM) {
-        const v3 o0 = mk(C.pos[0], C.pos[1], C.pos[2]);
-        const v3 d0 = camera_dir(C, P, x, y, i);
-        float ws[ND];
-        Ltot = path_walk_tan(Lc.S, C, P, i, o0, d0, gs.delta, st, nrays, ws);
-        // origin: the camera centre itself; direction: rotation r gives d' = d + r x d, intrinsics through R dc
-        const v3 wd = mk(ws[3], ws[4], ws[5]);
-        const v3 gr = cross(d0, wd);
-        float u, v;
-        camera_uv(P, x, y, i, u, v);
-        v3 J[8];
-        camdir_jac(C, u, v, J);
-        const v3 rw = mk(C.R[0] * wd.x + C.R[1] * wd.y + C.R[2] * wd.z, C.R[3] * wd.x + C.R[4] * wd.y + C.R[5] * wd.z,
-                         C.R[6] * wd.x + C.R[7] * wd.y + C.R[8] * wd.z);
-#pragma unroll
-        for (int k = 0; k < 8; k++) gc[k] = dot(rw, J[k]);
-        gc[8] = gr.x; gc[9] = gr.y; gc[10] = gr.z;
-        gc[11] = ws[0]; gc[12] = ws[1]; gc[13] = ws[2];
-      } else {
-        Ltot = path_walk<false>(Lc.S, C, P, x, y, i, st, nrays);
-      }
-      path_walk<true>(Lc.S, C, P, x, y, i, st, nrays, &gs, Ltot);
-    }
-  }
-  if (CAM) {
-#pragma unroll
-    for (int k = 0; k < CAM_PARAMS; k++) {
-      float v = gc[k];
-      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0 && v != 0.0f) atomicAdd(scam + c * CAM_PARAMS + k, v);
+      pix_seeds(C, it.x, it.y, P);
+      const v3 Ltot = path_walk<false>(Lc.S, C, P, it.x, it.y, it.i, st, nrays);
+      path_walk<true>(Lc.S, C, P, it.x, it.y, it.i, st, nrays, &gs, Ltot);
     }
   }
   uint32_t rs = nrays;
@@ -493,19 +471,66 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
   if (lane == 0) atomicAdd(&Lc.counters[1], (unsigned long long)rs);
 }
 
-template <int BLOCK, bool CAM>
+// camera wave: the tangent walk of each sample, contracted with the ray's derivative with respect to its
+// camera's intrinsics and pose; the wave's sums go to scam[camera * CAM_PARAMS + k]
+__device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wave, Stack &st, float *scam) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const WaveItem it = wave_item(Lc, wave, lane);
+  const DevCam &C = Lc.cams[it.c];
+  uint32_t nrays = 0;
+  float gc[CAM_PARAMS];
+#pragma unroll
+  for (int k = 0; k < CAM_PARAMS; k++) gc[k] = 0.0f;
+  if (it.valid) {
+    const float4 a = C.adj[(size_t)it.y * C.width + it.x];
+    const float s = C.scale / (float)C.nspp;
+    const v3 delta = mk(a.x * s, a.y * s, a.z * s);
+    if (delta.x != 0.0f || delta.y != 0.0f || delta.z != 0.0f) {
+      PixSeeds P;
+      pix_seeds(C, it.x, it.y, P);
+      const v3 o0 = mk(C.pos[0], C.pos[1], C.pos[2]);
+      const v3 d0 = camera_dir(C, P, it.x, it.y, it.i);
+      v3 e1, e2;
+      basis(d0, e1, e2);
+      float ws[ND];
+      (void)path_walk_tan(Lc.S, C, P, it.i, o0, d0, e1, e2, delta, st, nrays, ws);
+      const v3 wo = e1 * ws[0] + e2 * ws[1];
+      const v3 wd = e1 * ws[2] + e2 * ws[3];
+      // direction: rotation r gives d' = d + r x d; intrinsics through d = R dc
+      const v3 gr = cross(d0, wd);
+      float u, v;
+      camera_uv(P, it.x, it.y, it.i, u, v);
+      v3 J[8];
+      camdir_jac(C, u, v, J);
+      const v3 rw = mk(C.R[0] * wd.x + C.R[1] * wd.y + C.R[2] * wd.z, C.R[3] * wd.x + C.R[4] * wd.y + C.R[5] * wd.z,
+                       C.R[6] * wd.x + C.R[7] * wd.y + C.R[8] * wd.z);
+#pragma unroll
+      for (int k = 0; k < 8; k++) gc[k] = dot(rw, J[k]);
+      gc[8] = gr.x; gc[9] = gr.y; gc[10] = gr.z;
+      gc[11] = wo.x; gc[12] = wo.y; gc[13] = wo.z;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CAM_PARAMS; k++) {
+    float v = gc[k];
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0 && v != 0.0f) atomicAdd(scam + it.c * CAM_PARAMS + k, v);
+  }
+  uint32_t rs = nrays;
+  for (int off = 32; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
+  if (lane == 0) atomicAdd(&Lc.counters[1], (unsigned long long)rs);
+}
+
+template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
-                                                    uint32_t n_light, uint32_t use_lds, float *gcam) {
+                                                    uint32_t n_light, uint32_t use_lds) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
   __shared__ float s_t[STACK_LDS * BLOCK];
-  __shared__ float scam[CAM ? MAX_CAMS * CAM_PARAMS : 1];
   extern __shared__ float sgrad[];
   const uint32_t ng = 3 * (n_mat + n_light);
   if (use_lds) {
     for (uint32_t k = threadIdx.x; k < ng; k += BLOCK) sgrad[k] = 0.0f;
   }
-  if (CAM)
-    for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK) scam[k] = 0.0f;
   __syncthreads();
   Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
            (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
@@ -516,7 +541,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
     base = __shfl(base, 0, 64);
     if (base >= Lc.total_waves) break;
     const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
-    for (uint64_t w = base; w < end; w++) run_wave_bwd<CAM>(Lc, w, st, gs, scam);
+    for (uint64_t w = base; w < end; w++) run_wave_bwd(Lc, w, st, gs);
   }
   __syncthreads();
   if (use_lds) {
@@ -525,22 +550,42 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
       if (v != 0.0f) atomicAdd(k < 3 * n_mat ? gmat + k : glight + (k - 3 * n_mat), v);
     }
   }
-  if (CAM)
-    for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK)
-      if (scam[k] != 0.0f) atomicAdd(gcam + k, scam[k]);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, 2) void k_camgrad(RenderLaunch Lc, float *gcam) {
+  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
+  __shared__ float s_t[STACK_LDS * BLOCK];
+  __shared__ float scam[MAX_CAMS * CAM_PARAMS];
+  for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK) scam[k] = 0.0f;
+  __syncthreads();
+  Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
+           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
+  for (;;) {
+    unsigned long long base = 0;
+    if ((threadIdx.x & 63u) == 0) base = atomicAdd(&Lc.counters[0], (unsigned long long)Lc.chunk);
+    base = __shfl(base, 0, 64);
+    if (base >= Lc.total_waves) break;
+    const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
+    for (uint64_t w = base; w < end; w++) run_wave_cam(Lc, w, st, scam);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK)
+    if (scam[k] != 0.0f) atomicAdd(gcam + k, scam[k]);
 }
 
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
-                           float *gcam, hipStream_t s) {
+                           hipStream_t s) {
   const size_t lds = (size_t)3 * (n_mat + n_light) * sizeof(float);
   const uint32_t use_lds = lds <= 16384 ? 1u : 0u;
+  hipLaunchKernelGGL(k_backward<256>, dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight, n_mat,
+                     n_light, use_lds);
+  return hipGetLastError();
+}
+
+hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s) {
   if (L.ncam > MAX_CAMS) return hipErrorInvalidValue;
-  if (gcam)
-    hipLaunchKernelGGL((k_backward<256, true>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight,
-                       n_mat, n_light, use_lds, gcam);
-  else
-    hipLaunchKernelGGL((k_backward<256, false>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight,
-                       n_mat, n_light, use_lds, gcam);
+  hipLaunchKernelGGL(k_camgrad<256>, dim3(L.grid), dim3(256), 0, s, L, gcam);
   return hipGetLastError();
 }
 

@@ -29,8 +29,10 @@ def test_module_exports_the_renderer_api():
     out = _node("import('%s').then(m => { const p = m.TwinRenderer.prototype; console.log(JSON.stringify({exp: Object.keys(m).sort(), "
                 "methods: Object.getOwnPropertyNames(p).sort(), create: typeof m.TwinRenderer.create})); })" % MODULE)
     d = json.loads(out)
-    assert {"TwinRenderer", "DEFAULTS", "PRESETS", "OPTION_BITS", "lensParams", "fromPrepared"} <= set(d["exp"])
-    for m in ("renderCamera", "renderFrame", "sensor", "readback", "setPoses", "project", "timings", "destroy", "idle"):
+    assert {"TwinRenderer", "DEFAULTS", "PRESETS", "OPTION_BITS", "lensParams", "fromPrepared", "RenderFunction",
+            "rotateColumns"} <= set(d["exp"])
+    for m in ("renderCamera", "renderFrame", "sensor", "readback", "setPoses", "project", "timings", "destroy", "idle",
+              "backward", "autotune", "updateMaterials", "updateLights", "setCameraParams"):
         assert m in d["methods"], m
     assert d["create"] == "function"
 
@@ -100,3 +102,40 @@ def test_js_frame_equals_c_abi_frame(tmp_path):
     ctx.close()
     assert np.array_equal(js_lin, lin)
     assert np.array_equal(js_raw, raw.astype(np.float32))
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_calibration_lowers_the_loss(tmp_path):
+    """The autograd-style RenderFunction drives a small calibration: material colours, KB4 intrinsics and the
+    camera centre start perturbed; ten Adam steps on the adjoint's gradients lower the image loss."""
+    sc, cams = scenes.c1()
+    cam = dict(cams[0])
+    for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"], iters=10,
+               nMat=sc["materials"].size // 16,
+               cameras=[dict(name="CAM1", lens=[float(x) for x in cam["lens"]], R=[float(x) for x in cam["R"]],
+                             pos=[float(x) for x in cam["pos"]], size=[cam["width"], cam["height"]])])
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_calibrate.mjs"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    L = info["losses"]
+    print("calibration losses:", ["%.4g" % x for x in L], "fx", info["fx"], "true", info["fx_true"])
+    assert L[-1] < 0.5 * L[0]
+
+
+@needs_node
+def test_js_rotate_columns_is_rodrigues():
+    R = scenes.look_at((0.3, -1.0, 0.5), (0.0, 0.2, 0.1))
+    r = [0.01, -0.02, 0.005]
+    out = _node("import('%s').then(m => console.log(JSON.stringify(m.rotateColumns(%s, %s))))" % (MODULE, json.dumps([float(x) for x in R]), json.dumps(r)))
+    got = np.array(json.loads(out)).reshape(3, 3).T
+    th = np.linalg.norm(r)
+    k = np.array(r) / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    M = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+    assert np.allclose(got, M @ np.asarray(R, float).reshape(3, 3).T, atol=1e-7)

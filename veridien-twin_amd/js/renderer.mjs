@@ -16,6 +16,7 @@ export const DEFAULTS = Object.freeze({
   lightSamples: 1,    // emitter samples per path vertex
   seed: 0,
   batched: true,      // every camera of a frame in one persistent launch
+  graph: true,        // batched frames run as a hipGraph, captured once and replayed
 });
 export const PRESETS = Object.freeze({
   live: { spp: 1, maxDepth: 2, lightSamples: 1 },
@@ -102,6 +103,8 @@ export class TwinRenderer {
     this.device = { hip: opts.device || 0 };
     native.uploadScene(this.ctx, src);
     this.stats = { triangles: src.tris.length / 16 };
+    this.materials = Float32Array.from(src.materials);
+    this.lights = Float32Array.from(src.lights);
     this.cameras = {};
     (opts.cameras || []).forEach((c, i) => this._addCamera(c, i));
     this.frame = 0;
@@ -156,7 +159,9 @@ export class TwinRenderer {
     const names = opts.cameras || Object.keys(this.cameras);
     const cams = names.map((n) => this._slot(n));
     cams.forEach((c) => this._pack(c, opts));
-    native.render(this.ctx, cams.map((c) => c.index), { mode: dflt(opts.batched, this.opts.batched) ? 0 : 1 });
+    const batched = dflt(opts.batched, this.opts.batched);
+    native.render(this.ctx, cams.map((c) => c.index),
+      { mode: batched ? 0 : 1, graph: batched && dflt(opts.graph, this.opts.graph) ? 1 : 0 });
     const out = {};
     for (const c of cams) out[c.name] = { linear: { camera: c.name, width: c.size[0], height: c.size[1] } };
     if (opts.sensor) for (const c of cams) out[c.name].raw = this.sensor(c.name, opts.sensorOpts || {});
@@ -196,6 +201,45 @@ export class TwinRenderer {
     });
   }
 
+  /** Pick block size, LDS stack depth, queue chunk and queue layout for a frame of these cameras (no reference
+   * counterpart); later frames use the result. Resolves to the chosen shape and the probe rates. */
+  async autotune(opts = {}) {
+    const names = opts.cameras || Object.keys(this.cameras);
+    const cams = names.map((n) => this._slot(n));
+    cams.forEach((c) => this._pack(c, opts));
+    this.tuned = await native.autotune(this.ctx, cams.map((c) => c.index), { probeStride: opts.probeStride || 0 });
+    return this.tuned;
+  }
+
+  /** Replace the material table (16 floats per material) or the light radiances (4 per light) in place. */
+  updateMaterials(materials) { native.updateMaterials(this.ctx, Float32Array.from(materials)); this.materials = Float32Array.from(materials); }
+  updateLights(lights) { native.updateLights(this.ctx, Float32Array.from(lights)); this.lights = Float32Array.from(lights); }
+
+  /** Change a camera's intrinsics (9 lens numbers) and / or pose (R: 9, columns; pos: 3) for the next frame. */
+  setCameraParams(name, { lens, R, pos } = {}) {
+    const cam = this._slot(name);
+    if (lens) cam.lens = lensParams(lens);
+    if (R) cam.R = Array.from(R);
+    if (pos) cam.pos = Array.from(pos);
+  }
+
+  /**
+   * Adjoint of the last frame's linear images: adjoints = {name: Float32Array w*h*4 (dLoss/dI, rgb used)} for
+   * cameras rendered with the same options. Path-replay gradients with respect to the material colours and
+   * light radiances, and (opts.cameras, default true) each camera's [fx fy cx cy k1 k2 k3 k4 rx ry rz tx ty tz]
+   * (rotation about the world axes, centre). Interior term only. Resolves to {materials, lights, cameras}.
+   */
+  async backward(adjoints, opts = {}) {
+    const names = Object.keys(adjoints);
+    const cams = names.map((n) => this._slot(n));
+    const withCams = dflt(opts.cameras, true);
+    const g = await native.backward(this.ctx, cams.map((c) => c.index), names.map((n) => adjoints[n]),
+      this.materials.length / 16, this.lights.length / 4, { cameras: withCams ? 1 : 0 });
+    const out = { materials: g.materials, lights: g.lights, cameras: {} };
+    if (withCams) names.forEach((n, k) => { out.cameras[n] = g.cameras.slice(14 * k, 14 * k + 14); });
+    return out;
+  }
+
   async timings() {
     const st = await native.stats(this.ctx);
     return { gpu: true, last: st, stats: this.stats };
@@ -204,6 +248,49 @@ export class TwinRenderer {
   async idle() { await native.sync(this.ctx); }
 
   destroy() { if (this.ctx) { native.destroy(this.ctx); this.ctx = null; } }
+}
+
+/** Rotation exp([r]x) (Rodrigues) applied to R (9 numbers, columns): the pose update of the camera adjoint. */
+export function rotateColumns(R, r) {
+  const th = Math.hypot(r[0], r[1], r[2]);
+  if (th === 0) return Array.from(R);
+  const k = [r[0] / th, r[1] / th, r[2] / th], c = Math.cos(th), s = Math.sin(th), v = 1 - c;
+  const M = [[c + k[0] * k[0] * v, k[0] * k[1] * v - k[2] * s, k[0] * k[2] * v + k[1] * s],
+    [k[1] * k[0] * v + k[2] * s, c + k[1] * k[1] * v, k[1] * k[2] * v - k[0] * s],
+    [k[2] * k[0] * v - k[1] * s, k[2] * k[1] * v + k[0] * s, c + k[2] * k[2] * v]];
+  const out = new Array(9);
+  for (let col = 0; col < 3; col++)
+    for (let i = 0; i < 3; i++) out[3 * col + i] = M[i][0] * R[3 * col] + M[i][1] * R[3 * col + 1] + M[i][2] * R[3 * col + 2];
+  return out;
+}
+
+/**
+ * Autograd-style differentiable render over a TwinRenderer: forward(params) renders the listed cameras and
+ * resolves to their linear images; backward(gradImages) resolves to the gradients of sum(grad . image) with
+ * respect to the material colours, the light radiances and each camera's 14 parameters, by the path-replay
+ * adjoint on the same samples. params: {materials?, lights?, cameras?: {name: {lens?, R?, pos?}}}.
+ */
+export class RenderFunction {
+  constructor(renderer, names, opts = {}) {
+    this.r = renderer;
+    this.names = names || Object.keys(renderer.cameras);
+    this.opts = opts;
+  }
+
+  async forward(params = {}) {
+    if (params.materials) this.r.updateMaterials(params.materials);
+    if (params.lights) this.r.updateLights(params.lights);
+    for (const [n, p] of Object.entries(params.cameras || {})) this.r.setCameraParams(n, p);
+    const out = this.r.renderFrame({ ...this.opts, cameras: this.names });
+    await out.done;
+    const images = {};
+    for (const n of this.names) images[n] = await this.r.readback(n, 'linear');
+    return images;
+  }
+
+  async backward(gradImages) {
+    return this.r.backward(gradImages, { cameras: dflt(this.opts.cameraGradients, true) });
+  }
 }
 
 export default TwinRenderer;

@@ -81,10 +81,12 @@ typedef struct {
   float scale;           /* output scale (CamU.expo.z), usually 1 */
 } vt_camera;
 
-enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1 };
+enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2 };
 typedef struct {
   uint32_t mode;         /* VT_RENDER_BATCHED: every listed camera in one persistent launch over a global ray
-                            queue; VT_RENDER_PER_CAMERA: one plain launch per camera */
+                            queue; VT_RENDER_PER_CAMERA: one plain launch per camera; VT_RENDER_WAVEFRONT: the
+                            frame's paths in batches advanced one vertex at a time (trace, shade, shadow, gather
+                            kernels), live paths radix-sorted by ray direction octant and origin between vertices */
   uint32_t spp_begin;    /* sub-sample range [spp_begin, spp_end); spp_end 0 = each camera's spp */
   uint32_t spp_end;
   uint32_t accumulate;   /* 0: clear the cameras' sums first; 1: add to them (progressive ranges) */
@@ -96,6 +98,7 @@ typedef struct {
                             frame, drained by the blocks of that XCD first); 0 = tuned default */
   uint32_t graph;        /* batched: 1 = run the frame's launches as a hipGraph, captured on first use and
                             replayed while the cameras, sample range and launch shape stay the same */
+  uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 16,777,216) */
 } vt_render_opts;
 
 /* Result of vt_autotune: the chosen launch shape (also made the context's default) and the probe timings. */

@@ -355,3 +355,23 @@ def test_autotune_picks_a_valid_shape(ctx, c1):
     assert r["probe_mrays_best"] >= r["probe_mrays_default"] > 0
     img, _ = _render_img(ctx)          # the tuned default
     assert np.array_equal(img, ref)
+
+
+@pytest.mark.parametrize("batch", [0, 64 * 1000])
+def test_wavefront_matches_persistent(ctx, c1, batch):
+    """The wavefront form (per-vertex kernels, live paths radix-sorted between vertices, batches of paths) is
+    the same estimator on the same samples: its image equals the persistent kernel's to float rounding (the
+    kernels split the arithmetic differently), with the same sample count; and it stays on the oracle."""
+    sc, cams = c1
+    ctx.set_camera(0, cams[0])
+    ref, st0 = _render_img(ctx)
+    img, st = _render_img(ctx, mode=lib.VT_RENDER_WAVEFRONT, batch=batch)
+    assert st["samples"] == st0["samples"]
+    assert abs(st["rays"] - st0["rays"]) <= 1e-4 * st0["rays"]
+    a, b = img[:, :, 0:3].astype(np.float64), ref[:, :, 0:3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("wavefront vs persistent relative RMSE %.3g (batch %d)" % (rel, batch))
+    assert rel < 1e-5
+    # a second frame through the same buffers gives the same image
+    img2, _ = _render_img(ctx, mode=lib.VT_RENDER_WAVEFRONT, batch=batch)
+    assert np.array_equal(img, img2)

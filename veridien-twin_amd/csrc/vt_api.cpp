@@ -61,6 +61,13 @@ struct vt_ctx {
   // launch shape used for vt_render_opts fields left 0 (vt_autotune replaces it)
   uint32_t def_block = DEFAULT_BLOCK, def_stack = STACK_LDS, def_chunk = 4, def_queues = 1;
   int occ[3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (block 64/128/256, LDS depth)
+  // scene box (for the wavefront sort key)
+  float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  // wavefront buffers: paths per batch and shadow slots per path they are sized for
+  DevBuf wf, wf_temp;
+  uint64_t wf_paths = 0;
+  uint32_t wf_nl = 0;
+  size_t wf_temp_bytes = 0;
   // the frame graph: the batched frame's launches, replayed while their parameters match `gkey`
   hipGraphExec_t gexec = nullptr;
   std::vector<uint8_t> gkey;
@@ -239,7 +246,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
-                    &c->dXd, &c->dR, &c->dPart, &c->grads};
+                    &c->dXd, &c->dR, &c->dPart, &c->grads, &c->wf, &c->wf_temp};
   for (DevBuf *b : bufs) release(*b);
   for (auto &cs : c->cams) {
     release(cs.acc);
@@ -370,6 +377,15 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.emit_count = s->emit_count;
   c->S.root = 0;
   c->S.emit_area = s->emit_area > 0 ? s->emit_area : 1e-12f;
+  for (int k = 0; k < 3; k++) {
+    c->bmin[k] = INFINITY;
+    c->bmax[k] = -INFINITY;
+  }
+  for (uint32_t v = 0; v < V; v++)
+    for (int k = 0; k < 3; k++) {
+      c->bmin[k] = std::min(c->bmin[k], s->verts[8 * (size_t)v + k]);
+      c->bmax[k] = std::max(c->bmax[k], s->verts[8 * (size_t)v + k]);
+    }
   c->tri_count = T;
   c->node4_count = (uint32_t)n4.size();
   c->n_mat = s->material_count;
@@ -576,6 +592,80 @@ int build_graph(vt_ctx *c, uint32_t n, bool accumulate, const RenderLaunch &L) {
   return VT_OK;
 }
 
+// the wavefront frame: buffers for one batch, the batches in order, one event pair around all of them
+int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t items) {
+  uint64_t P = opt.batch ? opt.batch : (1ull << 24);
+  P = std::min<uint64_t>((P + 63) / 64 * 64, std::max<uint64_t>(items, 64));
+  if (P > (1ull << 31)) return fail(c, VT_ERR_ARG, "render: wavefront batch of %llu paths", (unsigned long long)P);
+  uint32_t nl = 1, depth = 1;
+  for (uint32_t a = 0; a < n; a++) {
+    nl = std::max(nl, std::min(16u, std::max(1u, c->h_cams[a].nl)));
+    depth = std::max(depth, c->h_cams[a].depth);
+  }
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  // per path 84 B of state, per queue slot 36 B, per shadow slot 48 B
+  const size_t bytes = al(P * 4) + 5 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) + 256;
+  int rc;
+  if (P > c->wf_paths || nl > c->wf_nl) {
+    if ((rc = ensure(c, c->wf, bytes))) return rc;
+    const size_t tb = wf_sort_temp_bytes((uint32_t)P);
+    if ((rc = ensure(c, c->wf_temp, tb))) return rc;
+    c->wf_temp_bytes = tb;
+    c->wf_paths = P;
+    c->wf_nl = nl;
+  }
+  char *base = (char *)c->wf.p;
+  size_t off = 0;
+  auto take = [&](size_t b) { void *p = base + off; off += al(b); return p; };
+  WfLaunch W;
+  std::memset(&W, 0, sizeof W);
+  W.S = c->S;
+  W.cams = (const DevCam *)c->dcams.p;
+  W.ncam = n;
+  W.nl_max = nl;
+  W.cam = (uint32_t *)take(P * 4);
+  W.o = (float4 *)take(P * 16);
+  W.d = (float4 *)take(P * 16);
+  W.thr = (float4 *)take(P * 16);
+  W.thr_next = (float4 *)take(P * 16);
+  W.L = (float4 *)take(P * 16);
+  W.q_next = (uint32_t *)take(P * 4);
+  W.key_next = (uint32_t *)take(P * 4);
+  uint32_t *q_sorted = (uint32_t *)take(P * 4);
+  uint32_t *keys_sorted = (uint32_t *)take(P * 4);
+  W.hit = (float4 *)take(P * 16);
+  W.nee = (uint32_t *)take(P * 4);
+  W.sh_o = (float4 *)take(P * nl * 16);
+  W.sh_d = (float4 *)take(P * nl * 16);
+  W.sh_c = (float4 *)take(P * nl * 16);
+  W.count = (uint32_t *)take(16);
+  float ext[3];
+  for (int k = 0; k < 3; k++) ext[k] = std::max(c->bmax[k] - c->bmin[k], 1e-6f);
+  W.bmin = make_float3(c->bmin[0], c->bmin[1], c->bmin[2]);
+  W.bscale = make_float3(512.0f / ext[0], 512.0f / ext[1], 512.0f / ext[2]);
+  W.counters = (unsigned long long *)c->counters.p;
+  W.overflow = (uint32_t *)c->overflow.p;
+  W.spill = (uint2 *)c->spill.p;
+  W.spill_stride = c->spill_lanes;
+  VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+  if (!opt.accumulate)
+    for (uint32_t a = 0; a < n; a++) {
+      const DevCam &d = c->h_cams[a];
+      VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
+    }
+  VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+  const uint32_t grid_cap = (uint32_t)c->cu_count * 8, trace_cap = max_blocks(c, 256);
+  for (uint64_t bb = 0; bb < items; bb += P) {
+    W.batch_begin = bb;
+    W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
+    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, keys_sorted, q_sorted, c->wf_temp.p,
+                                     c->wf_temp_bytes, c->stream));
+  }
+  VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+  return VT_OK;
+}
+
 }  // namespace
 
 extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o) {
@@ -587,12 +677,17 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   int rc = check_shape(c, sh);
   if (rc) return rc;
   if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
+  if (opt.mode > VT_RENDER_WAVEFRONT) return fail(c, VT_ERR_ARG, "render: mode %u", opt.mode);
   uint64_t items = 0;
   if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
   RenderLaunch L = base_launch(c, sh);
   uint32_t launches = 0;
   c->graph_used = false;
-  if (opt.mode == VT_RENDER_BATCHED) {
+  if (opt.mode == VT_RENDER_WAVEFRONT) {
+    if ((rc = ensure_launch_events(c, 1))) return rc;
+    if ((rc = render_wavefront(c, n, opt, items))) return rc;
+    launches = 1;
+  } else if (opt.mode == VT_RENDER_BATCHED) {
     if ((rc = ensure_launch_events(c, 1))) return rc;
     L.cams = (const DevCam *)c->dcams.p;
     L.ncam = n;

@@ -50,6 +50,39 @@ hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, ui
 // camera adjoint: CAM_PARAMS gradients per camera of the launch (fx fy cx cy k1..k4, rotation, centre), by
 // forward-mode tangents of each sample's camera ray (block 256, persistent queue; L.cams[*].adj set)
 hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s);
+// the wavefront form: one batch of paths (items [batch_begin, batch_begin + n) of the frame queue, n a multiple
+// of 64), one kernel per stage per vertex, live paths radix-sorted by ray_key between vertices
+struct WfLaunch {
+  SceneDev S;
+  const DevCam *cams;
+  uint32_t ncam;
+  uint64_t batch_begin;
+  uint32_t n;
+  uint32_t bounce;
+  uint32_t nl_max;             // shadow slots per queue slot (the largest light_samples of the frame)
+  // path state, by path id (0..n)
+  uint32_t *cam;               // camera of the path (index in cams)
+  float4 *o;                   // next ray origin; w: pdf of the BSDF sample that chose it
+  float4 *d;                   // next ray direction
+  float4 *thr, *thr_next;      // throughput at this vertex (w: alive), and after its continuation
+  float4 *L;                   // radiance so far
+  // queue of live paths, by queue slot (0..count[0])
+  const uint32_t *q;           // path ids (null: identity, vertex 1)
+  uint32_t *q_next, *key_next; // next vertex: path id and sort key per slot (DEAD key: ended)
+  uint32_t *count;             // [0] live slots of this vertex, [1] of the next
+  float4 *hit;                 // t, u, v, triangle (bits) per slot
+  uint32_t *nee;               // shadow slots used per queue slot
+  float4 *sh_o, *sh_d, *sh_c;  // shadow slots [slot * nl_max + j]: origin + tmax, direction, contribution + visible
+  float3 bmin, bscale;         // scene box and 512 / extent, for the origin's Morton code
+  unsigned long long *counters;// [1] rays, [2] samples
+  uint32_t *overflow;
+  uint2 *spill;
+  uint64_t spill_stride;
+};
+size_t wf_sort_temp_bytes(uint32_t n);
+hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
+                                  uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes,
+                                  hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 

@@ -1,0 +1,399 @@
+// The wavefront form of the camera path tracer: a batch of paths advances one vertex at a time through
+// separate kernels (extension trace, shade, shadow trace, NEE gather), and between vertices the live paths are
+// re-ordered by a radix sort on (direction octant, Morton code of the ray origin), so a wave traces rays that
+// start close together and head the same way. The estimator, its random numbers and the order of every sum
+// are those of path_walk (vt_render.hip); the per-pixel sums use the same butterfly over 64-item waves.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "vt_internal.h"
+
+namespace vt {
+
+namespace {
+
+constexpr uint32_t DEAD_KEY = 0xffffffffu;
+
+// the sample of frame item `item`, given its camera
+struct ItemPix {
+  uint32_t x, y, i;
+  bool valid;
+};
+__device__ __forceinline__ ItemPix item_pix(const DevCam &C, uint64_t item) {
+  ItemPix r;
+  const uint64_t q = item - C.item_begin;
+  const uint64_t pix = q / C.nspp;
+  r.i = C.spp_begin + (uint32_t)(q - pix * C.nspp);
+  const uint32_t t = (uint32_t)(pix >> 6), s = (uint32_t)(pix & 63u);
+  r.x = (t % C.tiles_x) * 8u + (s & 7u);
+  r.y = (t / C.tiles_x) * 8u + (s >> 3);
+  r.valid = r.x < C.width && r.y < C.height;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {   // 9 bits -> every third bit of 27
+  v &= 0x1ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t ray_key(v3 o, v3 d, const WfLaunch &W) {
+  const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  const float fx = fminf(fmaxf((o.x - W.bmin.x) * W.bscale.x, 0.0f), 511.0f);
+  const float fy = fminf(fmaxf((o.y - W.bmin.y) * W.bscale.y, 0.0f), 511.0f);
+  const float fz = fminf(fmaxf((o.z - W.bmin.z) * W.bscale.z, 0.0f), 511.0f);
+  const uint32_t m = spread3((uint32_t)fx) | (spread3((uint32_t)fy) << 1) | (spread3((uint32_t)fz) << 2);
+  return (oct << 27) | m;
+}
+
+__device__ __forceinline__ uint32_t wave_count(bool pred) {
+  return (uint32_t)__popcll(__ballot(pred));
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    W.count[0] = W.n;
+    W.count[1] = 0;
+  }
+  for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < W.n; p += gridDim.x * 256) {
+    const uint64_t item = W.batch_begin + p;
+    uint32_t c = 0;
+    while (c + 1 < W.ncam && item >= W.cams[c + 1].item_begin) c++;
+    const DevCam &C = W.cams[c];
+    const ItemPix ip = item_pix(C, item);
+    W.cam[p] = c;
+    v3 d = mk(0.f, 0.f, 1.f);
+    if (ip.valid) {
+      PixSeeds P;
+      pix_seeds(C, ip.x, ip.y, P);
+      d = camera_dir(C, P, ip.x, ip.y, ip.i);
+    }
+    W.o[p] = make_float4(C.pos[0], C.pos[1], C.pos[2], 0.0f);   // w: pdf of the BSDF sample that led here
+    W.d[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    W.thr[p] = make_float4(1.f, 1.f, 1.f, ip.valid ? 1.0f : 0.0f);   // w: 1 while the path is alive
+    W.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_trace(WfLaunch W) {
+  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
+  __shared__ float s_t[STACK_LDS * BLOCK];
+  Stack st{s_node, s_t, W.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, W.spill_stride,
+           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, W.overflow};
+  const uint32_t n = W.count[0];
+  uint32_t nrays = 0;
+  for (uint32_t k0 = blockIdx.x * BLOCK; k0 < n; k0 += gridDim.x * BLOCK) {
+    const uint32_t k = k0 + threadIdx.x;
+    bool traced = false;
+    if (k < n) {
+      const uint32_t p = W.q ? W.q[k] : k;
+      float4 h4 = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
+      if (W.thr[p].w > 0.0f) {
+        const float4 o = W.o[p], d = W.d[p];
+        Hit h;
+        traced = true;
+        if (traverse<false>(W.S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, INFINITY, h, st))
+          h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+      }
+      W.hit[k] = h4;
+    }
+    nrays += traced ? 1u : 0u;
+  }
+  for (int off = 32; off >= 1; off >>= 1) nrays += __shfl_xor(nrays, off, 64);
+  if ((threadIdx.x & 63u) == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
+}
+
+// one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
+__global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
+  // every slot of the batch gets a key: live paths of this vertex theirs, the rest (and the slots past the
+  // live count) the dead key, so the sort of all n slots puts exactly the next vertex's paths first
+  const uint32_t n = W.count[0];
+  const uint32_t b = W.bounce;
+  const float A = W.S.emit_area;
+  for (uint32_t k0 = blockIdx.x * 256; k0 < W.n; k0 += gridDim.x * 256) {
+    const uint32_t k = k0 + threadIdx.x;
+    bool live = false;
+    if (k >= n && k < W.n) {
+      W.key_next[k] = DEAD_KEY;
+      W.q_next[k] = k;
+    }
+    if (k < n) {
+      const uint32_t p = W.q ? W.q[k] : k;
+      const DevCam &C = W.cams[W.cam[p]];
+      const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+      const float fnl = (float)nl;
+      const uint32_t Dp = C.depth < 1 ? 1 : C.depth;
+      uint32_t nee = 0;   // shadow slots written
+      const float4 h4 = W.hit[k];
+      const uint32_t tri = __float_as_uint(h4.w);
+      if (tri != MISS && b < Dp) {
+        const ItemPix ip = item_pix(C, W.batch_begin + p);
+        PixSeeds P;
+        pix_seeds(C, ip.x, ip.y, P);
+        const uint32_t i = ip.i, hs = P.hs;
+        const float4 o4 = W.o[p], d4 = W.d[p];
+        const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+        const float pB = o4.w;
+        float4 t4 = W.thr[p];
+        const v3 thr = mk(t4.x, t4.y, t4.z);
+        const uint4 at = W.S.attr[tri];
+        const uint4 ms = W.S.meshes[at.w];
+        v3 ng, ns, nsi;
+        {
+          const float4 *tp = W.S.tri + 3 * (size_t)tri;
+          const v3 p0 = ld3(tp[0]);
+          ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
+          const float w0 = 1.0f - h4.y - h4.z;
+          nsi = normalize(ld3(W.S.vnorm[at.x]) * w0 + ld3(W.S.vnorm[at.y]) * h4.y + ld3(W.S.vnorm[at.z]) * h4.z);
+        }
+        ns = nsi;
+        if (dot(ng, d) > 0.0f) ng = -ng;
+        if (dot(ns, ng) < 0.0f) ns = -ns;
+        if (dot(ns, -d) < 1e-4f) ns = ng;
+        const v3 pt = o + d * h4.x;
+        if (ms.y & F_EMITTER) {
+          const float cosE = -dot(d, nsi);
+          if (cosE > 0.0f) {
+            const v3 Le = ld3(W.S.lights[ms.z]);
+            float w = 1.0f;
+            if (b > 0) {
+              const float pL = h4.x * h4.x / (cosE * A);
+              w = pB * pB / (pB * pB + fnl * fnl * pL * pL);
+            }
+            float4 Lp = W.L[p];
+            const v3 add = thr * Le * w;
+            Lp.x += add.x; Lp.y += add.y; Lp.z += add.z;
+            W.L[p] = Lp;
+          }
+        } else {
+          const v3 wo = -d;
+          const bool last = b + 1 == Dp;
+          if (W.S.emit_count > 0) {
+            for (uint32_t j = 0; j < nl; j++) {
+              float u0, u1, u2;
+              if (b == 0) {
+                const uint32_t li = i * nl + j;
+                u0 = kr(li, KL0, P.s[3]); u1 = kr(li, KL1, P.s[4]); u2 = kr(li, KL2, P.s[5]);
+              } else {
+                u0 = u01(hash3(hs, i, 256u * b + 3u * j));
+                u1 = u01(hash3(hs, i, 256u * b + 3u * j + 1u));
+                u2 = u01(hash3(hs, i, 256u * b + 3u * j + 2u));
+              }
+              v3 pl;
+              const uint32_t e = sample_emitter(W.S, u0, u1, u2, pl);
+              v3 wi = pl - pt;
+              const float d2 = dot(wi, wi), dist = sqrtf(d2);
+              wi = wi * (1.0f / dist);
+              const float cosE = -dot(wi, ld3(W.S.emit[4 * e + 3]));
+              if (cosE <= 0.0f) continue;
+              const float cosS = dot(ns, wi);
+              if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
+              const uint32_t lid = __float_as_uint(W.S.emit[4 * e + 2].w);
+              const v3 Le = ld3(W.S.lights[lid]);
+              if (maxc(Le) <= 0.0f) continue;
+              const Mat M = load_mat(W.S.mats, ms.x);
+              const v3 f = eval_bsdf(M, ns, wi, wo);
+              if (maxc(f) <= 0.0f) continue;
+              const float pW = d2 / (cosE * A);
+              float w = 1.0f;
+              if (!last) {
+                const float pb = pdf_bsdf(M, ns, wo, wi);
+                w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
+              }
+              const float kk = cosS * w / pW;
+              const v3 contrib = f * Le * kk;
+              const v3 so = pt + ng * (dot(ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
+              // slots in j order; the gather adds the visible ones in that order
+              const size_t s = (size_t)k * W.nl_max + nee;
+              W.sh_o[s] = make_float4(so.x, so.y, so.z, dist - 2e-5f);
+              W.sh_d[s] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+              W.sh_c[s] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+              nee++;
+            }
+          }
+          if (!last) {
+            float u0, u1, u2;
+            if (b == 0) {
+              u0 = kr(i, KB0, P.s[6]); u1 = kr(i, KB1, P.s[7]); u2 = kr(i, KB2, P.s[8]);
+            } else {
+              u0 = u01(hash3(hs, i, 256u * b + 200u));
+              u1 = u01(hash3(hs, i, 256u * b + 201u));
+              u2 = u01(hash3(hs, i, 256u * b + 202u));
+            }
+            const Mat M = load_mat(W.S.mats, ms.x);
+            v3 l;
+            const float pdf = sample_bsdf(M, ns, wo, u0, u1, u2, l);
+            if (pdf > 0.0f && dot(ng, l) > 0.0f) {
+              const v3 f = eval_bsdf(M, ns, l, wo);
+              const float cp = dot(ns, l) / pdf;
+              const v3 a = f * cp;
+              const v3 tn = thr * a;
+              if (maxc(tn) > 0.0f) {
+                const v3 no = pt + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
+                W.thr_next[p] = make_float4(tn.x, tn.y, tn.z, 1.0f);
+                W.o[p] = make_float4(no.x, no.y, no.z, pdf);
+                W.d[p] = make_float4(l.x, l.y, l.z, 0.0f);
+                W.key_next[k] = ray_key(no, l, W);
+                W.q_next[k] = p;
+                live = true;
+              }
+            }
+          }
+        }
+      }
+      W.nee[k] = nee;
+      if (!live) {
+        W.key_next[k] = DEAD_KEY;
+        W.q_next[k] = p;
+      }
+    }
+    const uint32_t nlive = wave_count(live);
+    if ((threadIdx.x & 63u) == 0 && nlive) atomicAdd(&W.count[1], nlive);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfLaunch W) {
+  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
+  __shared__ float s_t[STACK_LDS * BLOCK];
+  Stack st{s_node, s_t, W.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, W.spill_stride,
+           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, W.overflow};
+  const uint32_t n = W.count[0] * W.nl_max;
+  uint32_t nrays = 0;
+  for (uint32_t s0 = blockIdx.x * BLOCK; s0 < n; s0 += gridDim.x * BLOCK) {
+    const uint32_t s = s0 + threadIdx.x;
+    bool traced = false;
+    if (s < n && s % W.nl_max < W.nee[s / W.nl_max]) {
+      const float4 o = W.sh_o[s], d = W.sh_d[s];
+      Hit h;
+      traced = true;
+      if (!traverse<true>(W.S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w, h, st)) W.sh_c[s].w = 1.0f;
+    }
+    nrays += traced ? 1u : 0u;
+  }
+  for (int off = 32; off >= 1; off >>= 1) nrays += __shfl_xor(nrays, off, 64);
+  if ((threadIdx.x & 63u) == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
+}
+
+// L += thr * (sum of the visible NEE contributions, in sample order) / nl; then the continuation's throughput
+__global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
+  const uint32_t n = W.count[0];
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+    const uint32_t p = W.q ? W.q[k] : k;
+    const uint32_t ne = W.nee[k];
+    if (ne) {
+      const DevCam &C = W.cams[W.cam[p]];
+      const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+      v3 Ln = mk(0.f, 0.f, 0.f);
+      for (uint32_t j = 0; j < ne; j++) {
+        const float4 c = W.sh_c[(size_t)k * W.nl_max + j];
+        if (c.w > 0.0f) Ln = Ln + mk(c.x, c.y, c.z);
+      }
+      const float4 t4 = W.thr[p];
+      const v3 add = mk(t4.x, t4.y, t4.z) * Ln * (1.0f / (float)nl);
+      float4 Lp = W.L[p];
+      Lp.x += add.x; Lp.y += add.y; Lp.z += add.z;
+      W.L[p] = Lp;
+    }
+  }
+}
+
+// after the sort: the next vertex's live count, and its throughput in place
+__global__ __launch_bounds__(256) void k_wf_advance(WfLaunch W) {
+  const uint32_t n = W.count[1];
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+    const uint32_t p = W.q_next[k];
+    W.thr[p] = W.thr_next[p];
+  }
+}
+__global__ void k_wf_swap_counts(uint32_t *count) {
+  count[0] = count[1];
+  count[1] = 0;
+}
+
+// per-pixel sums of the batch: the butterfly of run_wave over each 64-item wave
+__global__ __launch_bounds__(256) void k_wf_accum(WfLaunch W) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = W.n / 64;
+  for (uint32_t w = (blockIdx.x * 256 + threadIdx.x) / 64; w < waves; w += gridDim.x * 4) {
+    const uint32_t p = w * 64 + lane;
+    const uint64_t item = W.batch_begin + p;
+    const DevCam &C = W.cams[W.cam[p]];
+    const ItemPix ip = item_pix(C, item);
+    const float4 L4 = W.L[p];
+    const v3 L = ip.valid ? mk(L4.x, L4.y, L4.z) : mk(0.f, 0.f, 0.f);
+    const uint32_t nspp = C.nspp;
+    const bool pow2 = (nspp & (nspp - 1)) == 0 && nspp <= 64;
+    if (pow2) {
+      float sx = L.x, sy = L.y, sz = L.z;
+      for (uint32_t off = 1; off < nspp; off <<= 1) {
+        sx += __shfl_xor(sx, (int)off, 64);
+        sy += __shfl_xor(sy, (int)off, 64);
+        sz += __shfl_xor(sz, (int)off, 64);
+      }
+      if (ip.valid && (lane & (nspp - 1)) == 0) {
+        float4 *a = C.acc + (size_t)ip.y * C.width + ip.x;
+        float4 v = *a;
+        v.x += sx; v.y += sy; v.z += sz; v.w += (float)nspp;
+        *a = v;
+      }
+    } else if (ip.valid) {
+      float4 *a = C.acc + (size_t)ip.y * C.width + ip.x;
+      atomicAdd(&a->x, L.x);
+      atomicAdd(&a->y, L.y);
+      atomicAdd(&a->z, L.z);
+      atomicAdd(&a->w, 1.0f);
+    }
+    uint32_t ns = ip.valid ? 1u : 0u;
+    for (int off = 32; off >= 1; off >>= 1) ns += __shfl_xor(ns, off, 64);
+    if (lane == 0) atomicAdd(&W.counters[2], (unsigned long long)ns);
+  }
+}
+
+uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
+  const uint64_t g = (n + block - 1) / block;
+  return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+}  // namespace
+
+size_t wf_sort_temp_bytes(uint32_t n) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                  (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
+  return bytes;
+}
+
+// One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
+hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
+                                  uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes,
+                                  hipStream_t s) {
+  const uint32_t g = grid_for(W.n, 256, grid_cap);
+  const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
+  const uint32_t gs = grid_for((uint64_t)W.n * W.nl_max, 256, trace_grid_cap);
+  hipLaunchKernelGGL(k_wf_gen, dim3(g), dim3(256), 0, s, W);
+  W.q = nullptr;   // vertex 1: the camera rays in item order (coherent already)
+  for (uint32_t b = 0; b < max_depth; b++) {
+    W.bounce = b;
+    hipLaunchKernelGGL(k_wf_trace<256>, dim3(gt), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(k_wf_shade, dim3(g), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(k_wf_shadow<256>, dim3(gs), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
+    if (b + 1 == max_depth) break;
+    size_t tb = sort_temp_bytes;
+    hipError_t e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
+    if (e != hipSuccess) return e;
+    WfLaunch A = W;
+    A.q_next = q_sorted;
+    hipLaunchKernelGGL(k_wf_advance, dim3(g), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(1), 0, s, W.count);
+    W.q = q_sorted;
+  }
+  hipLaunchKernelGGL(k_wf_accum, dim3(grid_for(W.n, 256, grid_cap)), dim3(256), 0, s, W);
+  return hipGetLastError();
+}
+
+}  // namespace vt

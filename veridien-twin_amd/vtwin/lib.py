@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libvt.so")
 
 VT_OK, VT_ERR_ARG, VT_ERR_SIZE, VT_ERR_HIP, VT_ERR_LOST, VT_ERR_STATE, VT_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
-VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA = 0, 1
+VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA, VT_RENDER_WAVEFRONT = 0, 1, 2
 VT_KIND_LINEAR, VT_KIND_RAW, VT_KIND_RI = 0, 1, 2
 VT_DENSE_FP32, VT_DENSE_BF16_REFINE, VT_DENSE_FP64 = 0, 1, 2
 
@@ -43,7 +43,7 @@ class Camera(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("mode", u32), ("spp_begin", u32), ("spp_end", u32), ("accumulate", u32), ("chunk", u32),
-                ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32)]
+                ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32), ("batch", u32)]
 
 
 class TuneResult(C.Structure):
@@ -205,15 +205,15 @@ class Context:
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
 
     def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,
-               stack_lds=0, queues=0, graph=False):
+               stack_lds=0, queues=0, graph=False, batch=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
         o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block, stack_lds, queues,
-                       1 if graph else 0)
+                       1 if graph else 0, batch)
         self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
 
     def autotune(self, slots, spp_begin=0, spp_end=0, probe_stride=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
-        o = RenderOpts(VT_RENDER_BATCHED, spp_begin, spp_end, 0, 0, 0, 0, 0, 0)
+        o = RenderOpts(VT_RENDER_BATCHED, spp_begin, spp_end, 0, 0, 0, 0, 0, 0, 0)
         r = TuneResult()
         self.check(self.L.vt_autotune(self.h, ptr(arr), arr.size, C.byref(o), probe_stride, C.byref(r)))
         return {k: getattr(r, k) for k, _ in TuneResult._fields_}
@@ -258,7 +258,7 @@ class Context:
         gl = np.zeros((max(n_light, 1), 3), np.float32)
         gc = np.zeros((arr.size, CAMERA_PARAMS), np.float32) if cameras else None
         g = Grads(ptr(gm), n_mat, ptr(gl), n_light, ptr(gc), arr.size if cameras else 0)
-        o = RenderOpts(0, spp_begin, spp_end, 0, chunk, 0, 0, 0, 0)
+        o = RenderOpts(0, spp_begin, spp_end, 0, chunk, 0, 0, 0, 0, 0)
         self.check(self.L.vt_backward(self.h, ptr(arr), arr.size, ptrs, C.byref(o), C.byref(g)))
         if cameras:
             return gm, gl[:n_light], gc

@@ -90,7 +90,8 @@ typedef struct {
   uint32_t spp_begin;    /* sub-sample range [spp_begin, spp_end); spp_end 0 = each camera's spp */
   uint32_t spp_end;
   uint32_t accumulate;   /* 0: clear the cameras' sums first; 1: add to them (progressive ranges) */
-  uint32_t chunk;        /* queue chunk in 64-item waves (0 = tuned default) */
+  uint32_t chunk;        /* queue chunk in 64-item waves (0 = tuned default); wavefront: idle lanes of a wave
+                            that make it fetch new rays, 1..64 (0 = 16) */
   uint32_t block;        /* threads per block: 64, 128 or 256 (0 = tuned default) */
   uint32_t stack_lds;    /* traversal stack entries per lane kept in LDS, 8..24 (0 = tuned default); the rest of
                             the 48-entry stack spills to global memory */

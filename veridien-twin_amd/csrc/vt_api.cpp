@@ -627,7 +627,7 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
   W.o = (float4 *)take(P * 16);
   W.d = (float4 *)take(P * 16);
   W.thr = (float4 *)take(P * 16);
-  W.thr_next = (float4 *)take(P * 16);
+  W.thr_slot = (float4 *)take(P * 16);
   W.L = (float4 *)take(P * 16);
   W.q_next = (uint32_t *)take(P * 4);
   W.key_next = (uint32_t *)take(P * 4);
@@ -655,11 +655,15 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
       VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
     }
   VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
-  const uint32_t grid_cap = (uint32_t)c->cu_count * 8, trace_cap = max_blocks(c, 256);
+  int occ = 0;
+  VT_HIP(c, wf_trace_occupancy(&occ));
+  const uint32_t grid_cap = (uint32_t)c->cu_count * 8;
+  const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
+  const uint32_t refill = opt.chunk ? std::min(64u, opt.chunk) : 16u;
   for (uint64_t bb = 0; bb < items; bb += P) {
     W.batch_begin = bb;
     W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
-    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, keys_sorted, q_sorted, c->wf_temp.p,
+    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, keys_sorted, q_sorted, c->wf_temp.p,
                                      c->wf_temp_bytes, c->stream));
   }
   VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));

@@ -64,13 +64,14 @@ struct WfLaunch {
   uint32_t *cam;               // camera of the path (index in cams)
   float4 *o;                   // next ray origin; w: pdf of the BSDF sample that chose it
   float4 *d;                   // next ray direction
-  float4 *thr, *thr_next;      // throughput at this vertex (w: alive), and after its continuation
+  float4 *thr;                 // throughput (w: alive); the shade kernel moves it past the vertex's continuation
   float4 *L;                   // radiance so far
   // queue of live paths, by queue slot (0..count[0])
   const uint32_t *q;           // path ids (null: identity, vertex 1)
   uint32_t *q_next, *key_next; // next vertex: path id and sort key per slot (DEAD key: ended)
-  uint32_t *count;             // [0] live slots of this vertex, [1] of the next
+  uint32_t *count;             // [0] live slots of this vertex, [1] of the next, [2] [3] trace queue heads
   float4 *hit;                 // t, u, v, triangle (bits) per slot
+  float4 *thr_slot;            // the throughput at this vertex per slot (weight of its NEE in the gather)
   uint32_t *nee;               // shadow slots used per queue slot
   float4 *sh_o, *sh_d, *sh_c;  // shadow slots [slot * nl_max + j]: origin + tmax, direction, contribution + visible
   float3 bmin, bscale;         // scene box and 512 / extent, for the origin's Morton code
@@ -80,9 +81,12 @@ struct WfLaunch {
   uint64_t spill_stride;
 };
 size_t wf_sort_temp_bytes(uint32_t n);
+hipError_t wf_trace_occupancy(int *blocks_per_cu);
+// trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
+// fetch of new rays (1..64)
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes,
-                                  hipStream_t s);
+                                  uint32_t refill, uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp,
+                                  size_t sort_temp_bytes, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 

@@ -379,7 +379,9 @@ __device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
 }
 
 // one 64-item wave of the queue
-__device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, Stack &st) {
+// nrays_acc / nsamp_acc: the lane's running counts, added to the launch counters once when the wave leaves
+__device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, Stack &st, uint32_t &nrays_acc,
+                                         uint32_t &nsamp_acc) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t g0 = Lc.item_base + wave * 64ull;
   // the wave's camera: item ranges are multiples of 64, so every lane shares it
@@ -417,14 +419,20 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
     atomicAdd(&a->z, L.z);
     atomicAdd(&a->w, 1.0f);
   }
-  // rays and samples traced by the wave
-  uint32_t rs = nrays;
-  for (int off = 32; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
-  uint32_t ns = valid ? 1u : 0u;
-  for (int off = 32; off >= 1; off >>= 1) ns += __shfl_xor(ns, off, 64);
-  if (lane == 0) {
-    atomicAdd(&Lc.counters[1], (unsigned long long)rs);
-    atomicAdd(&Lc.counters[2], (unsigned long long)ns);
+  nrays_acc += nrays;
+  nsamp_acc += valid ? 1u : 0u;
+}
+
+// a wave's ray and sample counts into the launch counters (one atomic pair per wave and launch)
+__device__ __forceinline__ void flush_counts(const RenderLaunch &Lc, uint32_t nrays, uint32_t nsamp) {
+  unsigned long long rs = nrays, ns = nsamp;
+  for (int off = 32; off >= 1; off >>= 1) {
+    rs += __shfl_xor(rs, off, 64);
+    ns += __shfl_xor(ns, off, 64);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    if (rs) atomicAdd(&Lc.counters[1], rs);
+    if (ns) atomicAdd(&Lc.counters[2], ns);
   }
 }
 
@@ -605,6 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
   st.gtid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   st.overflow = Lc.overflow;
   const uint32_t wib = threadIdx.x >> 6;
+  uint32_t nrays = 0, nsamp = 0;
   if (Lc.persistent) {
     // Queue q owns waves [total * q / nq, total * (q + 1) / nq). A block starts on its XCD's queue (blocks
     // are dispatched round-robin over the XCDs), so neighbouring tiles share that XCD's L2, and moves on to
@@ -622,15 +631,16 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
         if (base >= q1 - q0) break;
         const uint64_t b = q0 + base;
         const uint64_t end = b + Lc.chunk < q1 ? b + Lc.chunk : q1;
-        for (uint64_t w = b; w < end; w++) run_wave(Lc, w * Lc.wave_stride, st);
+        for (uint64_t w = b; w < end; w++) run_wave(Lc, w * Lc.wave_stride, st, nrays, nsamp);
       }
     }
   } else {
     // static schedule: the waves of the grid stride over the queue
     const uint64_t stride = (uint64_t)gridDim.x * (BLOCK / 64);
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride)
-      run_wave(Lc, w * Lc.wave_stride, st);
+      run_wave(Lc, w * Lc.wave_stride, st, nrays, nsamp);
   }
+  flush_counts(Lc, nrays, nsamp);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {

@@ -56,6 +56,8 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     W.count[0] = W.n;
     W.count[1] = 0;
+    W.count[2] = 0;
+    W.count[3] = 0;
   }
   for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < W.n; p += gridDim.x * 256) {
     const uint64_t item = W.batch_begin + p;
@@ -77,33 +79,163 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
   }
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_trace(WfLaunch W) {
-  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
-  __shared__ float s_t[STACK_LDS * BLOCK];
-  Stack st{s_node, s_t, W.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, W.spill_stride,
-           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, W.overflow};
-  const uint32_t n = W.count[0];
+// Persistent trace with dynamic ray fetch (Aila and Laine, HPG 2009): each lane holds one ray of the queue and
+// walks it with the while-while loop of traverse(); when at least `refill` lanes of the wave have finished, the
+// wave takes that many new rays from the queue head with one atomic and starts them, so lanes do not idle until
+// the slowest ray of a fixed group ends. ANY = false: extension rays of the live slots (count[0], head count[2]),
+// result into hit[slot]; ANY = true: shadow slots (count[0] * nl_max, head count[3]), visibility into sh_c.w.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
+  __shared__ uint32_t s_node[STACK_LDS * 256];
+  __shared__ float s_t[STACK_LDS * 256];
+  Stack st{s_node, s_t, W.spill, threadIdx.x, 256, STACK_LDS, STACK_SPILL, W.spill_stride,
+           (uint64_t)blockIdx.x * 256 + threadIdx.x, W.overflow};
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = ANY ? W.count[0] * W.nl_max : W.count[0];
+  uint32_t *head = &W.count[ANY ? 3 : 2];
+  bool active = false, exhausted = false;
+  uint32_t k = 0;
+  // the lane's ray and traversal state
+  v3 inv = mk(0.f, 0.f, 0.f), oi = mk(0.f, 0.f, 0.f);
+  RayW R;
+  R.o = mk(0.f, 0.f, 0.f); R.kx = 0; R.ky = 1; R.kz = 2; R.Sx = 0.f; R.Sy = 0.f; R.Sz = 0.f;
+  float best = 0.0f;
+  bool found = false;
+  Hit h{0.f, 0.f, 0.f, 0u};
+  uint32_t sp = 0, ref = TRAV_DONE, leaf = TRAV_DONE;
   uint32_t nrays = 0;
-  for (uint32_t k0 = blockIdx.x * BLOCK; k0 < n; k0 += gridDim.x * BLOCK) {
-    const uint32_t k = k0 + threadIdx.x;
-    bool traced = false;
-    if (k < n) {
-      const uint32_t p = W.q ? W.q[k] : k;
-      float4 h4 = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
-      if (W.thr[p].w > 0.0f) {
-        const float4 o = W.o[p], d = W.d[p];
-        Hit h;
-        traced = true;
-        if (traverse<false>(W.S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, INFINITY, h, st))
-          h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+  for (;;) {
+    // refill: the wave's idle lanes take consecutive queue entries
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle >= refill) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(head, nidle);
+      base = __shfl(base, 0, 64);
+      if (base + nidle >= n) exhausted = true;
+      if (!active) {
+        k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (k < n) {
+          v3 o, d;
+          float tmax;
+          bool ok;
+          if (ANY) {
+            ok = k % W.nl_max < W.nee[k / W.nl_max];
+            const float4 o4 = W.sh_o[k], d4 = W.sh_d[k];
+            o = mk(o4.x, o4.y, o4.z);
+            d = mk(d4.x, d4.y, d4.z);
+            tmax = o4.w;
+          } else {
+            const uint32_t p = W.q ? W.q[k] : k;
+            ok = W.thr[p].w > 0.0f;
+            const float4 o4 = W.o[p], d4 = W.d[p];
+            o = mk(o4.x, o4.y, o4.z);
+            d = mk(d4.x, d4.y, d4.z);
+            tmax = INFINITY;
+            if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
+          }
+          if (ok) {
+            inv = safe_inv(d);
+            oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
+            R = ray_w(o, d);
+            best = tmax;
+            found = false;
+            sp = 0;
+            ref = W.S.root;
+            leaf = TRAV_DONE;
+            active = true;
+            nrays++;
+          }
+        }
       }
-      W.hit[k] = h4;
     }
-    nrays += traced ? 1u : 0u;
+    if (!__any(active)) {
+      if (exhausted) break;
+      continue;
+    }
+    // traverse until this lane's ray ends, or until `refill` lanes of the wave are idle
+    if (active) {
+      for (;;) {
+        while (ref != TRAV_DONE && !(ref & LEAF_BIT)) {
+          const float4 *np = reinterpret_cast<const float4 *>(W.S.nodes + ref);
+          const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+          const uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
+          float tn[4];
+          uint32_t rr[4] = {rf.x, rf.y, rf.z, rf.w};
+          const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
+          const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
+          const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+          const float tfar_lim = best * 1.00000024f;
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
+            const float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
+            const float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
+            const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+            const float bb = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
+            tn[c] = (a <= bb && rr[c] != EMPTY_REF) ? a : INFINITY;
+          }
+#define VT_CSWAP(i, j)                                                   \
+  if (tn[j] < tn[i]) {                                                   \
+    float ft = tn[i]; tn[i] = tn[j]; tn[j] = ft;                         \
+    uint32_t fr = rr[i]; rr[i] = rr[j]; rr[j] = fr;                      \
+  }
+          VT_CSWAP(0, 1) VT_CSWAP(2, 3) VT_CSWAP(0, 2) VT_CSWAP(1, 3) VT_CSWAP(1, 2)
+#undef VT_CSWAP
+          if (tn[0] < INFINITY) {
+            if (tn[3] < INFINITY) push(st, sp, rr[3], tn[3]);
+            if (tn[2] < INFINITY) push(st, sp, rr[2], tn[2]);
+            if (tn[1] < INFINITY) push(st, sp, rr[1], tn[1]);
+            ref = rr[0];
+          } else {
+            ref = pop_next(st, sp, best);
+          }
+          if (is_leaf_ref(ref) && leaf == TRAV_DONE) {
+            leaf = ref;
+            ref = pop_next(st, sp, best);
+          }
+          if (!__any(leaf == TRAV_DONE)) break;
+        }
+        while (leaf != TRAV_DONE) {
+          const uint32_t first = leaf & 0x03ffffffu, cnt = ((leaf >> 26) & 31u) + 1u;
+          for (uint32_t t = first; t < first + cnt; t++) {
+            const float4 *tp = W.S.tri + 3 * (size_t)t;
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            float tt, u, v;
+            if (tri_hit(R, ld3(a), ld3(b), ld3(c), 0.0f, best, tt, u, v)) {
+              best = tt;
+              h.t = tt; h.u = u; h.v = v; h.tri = t;
+              found = true;
+              if (ANY) break;
+            }
+          }
+          if (ANY && found) {
+            leaf = TRAV_DONE;
+            ref = TRAV_DONE;
+            break;
+          }
+          leaf = TRAV_DONE;
+          if (is_leaf_ref(ref)) {
+            leaf = ref;
+            ref = pop_next(st, sp, best);
+          }
+        }
+        const bool done = ref == TRAV_DONE && leaf == TRAV_DONE;
+        if (done) {
+          if (ANY) {
+            if (!found) W.sh_c[k].w = 1.0f;
+          } else {
+            W.hit[k] = found ? make_float4(h.t, h.u, h.v, __uint_as_float(h.tri))
+                             : make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
+          }
+          active = false;
+        }
+        if (!active || (uint32_t)__popcll(__ballot(!active)) >= refill) break;
+      }
+    }
   }
   for (int off = 32; off >= 1; off >>= 1) nrays += __shfl_xor(nrays, off, 64);
-  if ((threadIdx.x & 63u) == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
+  if (lane == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
 }
 
 // one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
@@ -137,8 +269,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         const float4 o4 = W.o[p], d4 = W.d[p];
         const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
         const float pB = o4.w;
-        float4 t4 = W.thr[p];
+        const float4 t4 = W.thr[p];
         const v3 thr = mk(t4.x, t4.y, t4.z);
+        W.thr_slot[k] = t4;             // the gather's weight for this vertex's NEE
         const uint4 at = W.S.attr[tri];
         const uint4 ms = W.S.meshes[at.w];
         v3 ng, ns, nsi;
@@ -233,7 +366,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               const v3 tn = thr * a;
               if (maxc(tn) > 0.0f) {
                 const v3 no = pt + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
-                W.thr_next[p] = make_float4(tn.x, tn.y, tn.z, 1.0f);
+                W.thr[p] = make_float4(tn.x, tn.y, tn.z, 1.0f);
                 W.o[p] = make_float4(no.x, no.y, no.z, pdf);
                 W.d[p] = make_float4(l.x, l.y, l.z, 0.0f);
                 W.key_next[k] = ray_key(no, l, W);
@@ -255,29 +388,6 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
   }
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfLaunch W) {
-  __shared__ uint32_t s_node[STACK_LDS * BLOCK];
-  __shared__ float s_t[STACK_LDS * BLOCK];
-  Stack st{s_node, s_t, W.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, W.spill_stride,
-           (uint64_t)blockIdx.x * BLOCK + threadIdx.x, W.overflow};
-  const uint32_t n = W.count[0] * W.nl_max;
-  uint32_t nrays = 0;
-  for (uint32_t s0 = blockIdx.x * BLOCK; s0 < n; s0 += gridDim.x * BLOCK) {
-    const uint32_t s = s0 + threadIdx.x;
-    bool traced = false;
-    if (s < n && s % W.nl_max < W.nee[s / W.nl_max]) {
-      const float4 o = W.sh_o[s], d = W.sh_d[s];
-      Hit h;
-      traced = true;
-      if (!traverse<true>(W.S, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w, h, st)) W.sh_c[s].w = 1.0f;
-    }
-    nrays += traced ? 1u : 0u;
-  }
-  for (int off = 32; off >= 1; off >>= 1) nrays += __shfl_xor(nrays, off, 64);
-  if ((threadIdx.x & 63u) == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
-}
-
 // L += thr * (sum of the visible NEE contributions, in sample order) / nl; then the continuation's throughput
 __global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
   const uint32_t n = W.count[0];
@@ -292,7 +402,7 @@ __global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
         const float4 c = W.sh_c[(size_t)k * W.nl_max + j];
         if (c.w > 0.0f) Ln = Ln + mk(c.x, c.y, c.z);
       }
-      const float4 t4 = W.thr[p];
+      const float4 t4 = W.thr_slot[k];
       const v3 add = mk(t4.x, t4.y, t4.z) * Ln * (1.0f / (float)nl);
       float4 Lp = W.L[p];
       Lp.x += add.x; Lp.y += add.y; Lp.z += add.z;
@@ -301,23 +411,19 @@ __global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
   }
 }
 
-// after the sort: the next vertex's live count, and its throughput in place
-__global__ __launch_bounds__(256) void k_wf_advance(WfLaunch W) {
-  const uint32_t n = W.count[1];
-  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-    const uint32_t p = W.q_next[k];
-    W.thr[p] = W.thr_next[p];
-  }
-}
+// after the sort: the next vertex's live count
 __global__ void k_wf_swap_counts(uint32_t *count) {
   count[0] = count[1];
   count[1] = 0;
+  count[2] = 0;
+  count[3] = 0;
 }
 
 // per-pixel sums of the batch: the butterfly of run_wave over each 64-item wave
 __global__ __launch_bounds__(256) void k_wf_accum(WfLaunch W) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t waves = W.n / 64;
+  uint32_t nsamp = 0;
   for (uint32_t w = (blockIdx.x * 256 + threadIdx.x) / 64; w < waves; w += gridDim.x * 4) {
     const uint32_t p = w * 64 + lane;
     const uint64_t item = W.batch_begin + p;
@@ -347,10 +453,10 @@ __global__ __launch_bounds__(256) void k_wf_accum(WfLaunch W) {
       atomicAdd(&a->z, L.z);
       atomicAdd(&a->w, 1.0f);
     }
-    uint32_t ns = ip.valid ? 1u : 0u;
-    for (int off = 32; off >= 1; off >>= 1) ns += __shfl_xor(ns, off, 64);
-    if (lane == 0) atomicAdd(&W.counters[2], (unsigned long long)ns);
+    nsamp += ip.valid ? 1u : 0u;
   }
+  for (int off = 32; off >= 1; off >>= 1) nsamp += __shfl_xor(nsamp, off, 64);
+  if (lane == 0 && nsamp) atomicAdd(&W.counters[2], (unsigned long long)nsamp);
 }
 
 uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
@@ -359,6 +465,10 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 }
 
 }  // namespace
+
+hipError_t wf_trace_occupancy(int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false>, 256, 0);
+}
 
 size_t wf_sort_temp_bytes(uint32_t n) {
   size_t bytes = 0;
@@ -369,8 +479,9 @@ size_t wf_sort_temp_bytes(uint32_t n) {
 
 // One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes,
-                                  hipStream_t s) {
+                                  uint32_t refill, uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp,
+                                  size_t sort_temp_bytes, hipStream_t s) {
+  if (refill < 1 || refill > 64) return hipErrorInvalidValue;
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
   const uint32_t gs = grid_for((uint64_t)W.n * W.nl_max, 256, trace_grid_cap);
@@ -378,17 +489,14 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
   W.q = nullptr;   // vertex 1: the camera rays in item order (coherent already)
   for (uint32_t b = 0; b < max_depth; b++) {
     W.bounce = b;
-    hipLaunchKernelGGL(k_wf_trace<256>, dim3(gt), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(k_wf_trace_dyn<false>, dim3(gt), dim3(256), 0, s, W, refill);
     hipLaunchKernelGGL(k_wf_shade, dim3(g), dim3(256), 0, s, W);
-    hipLaunchKernelGGL(k_wf_shadow<256>, dim3(gs), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(k_wf_trace_dyn<true>, dim3(gs), dim3(256), 0, s, W, refill);
     hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
     if (b + 1 == max_depth) break;
     size_t tb = sort_temp_bytes;
     hipError_t e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
     if (e != hipSuccess) return e;
-    WfLaunch A = W;
-    A.q_next = q_sorted;
-    hipLaunchKernelGGL(k_wf_advance, dim3(g), dim3(256), 0, s, A);
     hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(1), 0, s, W.count);
     W.q = q_sorted;
   }

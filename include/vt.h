@@ -94,7 +94,7 @@ typedef struct {
                             that make it fetch new rays, 1..64 (0 = 16) */
   uint32_t block;        /* threads per block: 64, 128 or 256 (0 = tuned default) */
   uint32_t stack_lds;    /* traversal stack entries per lane kept in LDS, 8..24 (0 = tuned default); the rest of
-                            the 48-entry stack spills to global memory */
+                            the 48-entry stack spills to global memory. Wavefront: 4, 8 or 16 (0 = 8) */
   uint32_t queues;       /* batched: 1 global ray queue, or 8 queues (one per XCD, over contiguous eighths of the
                             frame, drained by the blocks of that XCD first); 0 = tuned default */
   uint32_t graph;        /* batched: 1 = run the frame's launches as a hipGraph, captured on first use and

@@ -42,7 +42,7 @@ struct vt_ctx {
   int cu_count = 0;
   // scene
   bool has_scene = false;
-  DevBuf nodes, tri, attr, vnorm, meshes, mats, lights, emit;
+  DevBuf nodes, tri, tshade, meshes, mats, lights, emit;
   SceneDev S{};
   uint32_t tri_count = 0, node4_count = 0;
   // launch support
@@ -242,7 +242,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->attr, &c->vnorm, &c->meshes, &c->mats, &c->lights, &c->emit,
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
@@ -358,8 +358,22 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   int rc;
   if ((rc = upload(c, c->nodes, n4.data(), n4.size() * sizeof(Node4)))) return rc;
   if ((rc = upload(c, c->tri, tri.data(), tri.size() * sizeof(float4)))) return rc;
-  if ((rc = upload(c, c->attr, attr.data(), attr.size() * sizeof(uint4)))) return rc;
-  if ((rc = upload(c, c->vnorm, vn.data(), vn.size() * sizeof(float4)))) return rc;
+  {
+    // vertex ids and normals only feed the shading records; they do not stay on the device
+    DevBuf d_attr, d_vn;
+    rc = upload(c, d_attr, attr.data(), attr.size() * sizeof(uint4));
+    if (!rc) rc = upload(c, d_vn, vn.data(), vn.size() * sizeof(float4));
+    if (!rc) rc = ensure(c, c->tshade, (size_t)T * 64);
+    if (!rc) {
+      hipError_t e = launch_tri_shading((const float4 *)c->tri.p, (const uint4 *)d_attr.p, (const float4 *)d_vn.p, T,
+                                        (float4 *)c->tshade.p, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) rc = hip_fail(c, e, "triangle shading records");
+    }
+    release(d_attr);
+    release(d_vn);
+    if (rc) return rc;
+  }
   if ((rc = upload(c, c->meshes, s->meshes, (size_t)s->mesh_count * 16))) return rc;
   if ((rc = upload(c, c->mats, s->materials, (size_t)s->material_count * 64))) return rc;
   float zero4[4] = {0, 0, 0, 0};
@@ -368,8 +382,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   if ((rc = upload(c, c->emit, s->emit_count ? (const void *)s->emit : zero16, std::max<size_t>(64, (size_t)s->emit_count * 64)))) return rc;
   c->S.nodes = (const Node4 *)c->nodes.p;
   c->S.tri = (const float4 *)c->tri.p;
-  c->S.attr = (const uint4 *)c->attr.p;
-  c->S.vnorm = (const float4 *)c->vnorm.p;
+  c->S.tshade = (const float4 *)c->tshade.p;
   c->S.meshes = (const uint4 *)c->meshes.p;
   c->S.mats = (const float4 *)c->mats.p;
   c->S.lights = (const float4 *)c->lights.p;
@@ -655,16 +668,18 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
       VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
     }
   VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+  const uint32_t slds = opt.stack_lds ? opt.stack_lds : 8;
+  if (slds != 4 && slds != 8 && slds != 16) return fail(c, VT_ERR_ARG, "render: wavefront stack_lds must be 4, 8 or 16");
   int occ = 0;
-  VT_HIP(c, wf_trace_occupancy(&occ));
+  VT_HIP(c, wf_trace_occupancy(slds, &occ));
   const uint32_t grid_cap = (uint32_t)c->cu_count * 8;
   const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
   const uint32_t refill = opt.chunk ? std::min(64u, opt.chunk) : 16u;
   for (uint64_t bb = 0; bb < items; bb += P) {
     W.batch_begin = bb;
     W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
-    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, keys_sorted, q_sorted, c->wf_temp.p,
-                                     c->wf_temp_bytes, c->stream));
+    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, slds, keys_sorted, q_sorted,
+                                     c->wf_temp.p, c->wf_temp_bytes, c->stream));
   }
   VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
   return VT_OK;
@@ -678,7 +693,7 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   vt_render_opts opt{};
   if (o) opt = *o;
   const Shape sh = shape_of(c, opt);
-  int rc = check_shape(c, sh);
+  int rc = opt.mode == VT_RENDER_WAVEFRONT ? VT_OK : check_shape(c, sh);   // the wavefront checks its own
   if (rc) return rc;
   if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
   if (opt.mode > VT_RENDER_WAVEFRONT) return fail(c, VT_ERR_ARG, "render: mode %u", opt.mode);

@@ -293,8 +293,8 @@ static_assert(sizeof(Node4) == 128, "Node4 is one 128-B cache line");
 struct SceneDev {
   const Node4 *nodes;
   const float4 *tri;        // 3 float4 per BVH slot: v0, v1, v2 (w unused)
-  const uint4 *attr;        // per slot: vertex ids i0 i1 i2, mesh
-  const float4 *vnorm;      // per vertex: normal xyz
+  const float4 *tshade;     // 4 per slot, one 64-B line: vertex normals n0 (w: mesh id bits), n1, n2, and the
+                            // geometric normal normalize((v1 - v0) x (v2 - v0))
   const uint4 *meshes;      // material, flags, light, -
   const float4 *mats;       // 4 float4 per material (16 floats)
   const float4 *lights;     // radiance rgb

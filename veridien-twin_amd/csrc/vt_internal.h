@@ -81,13 +81,16 @@ struct WfLaunch {
   uint64_t spill_stride;
 };
 size_t wf_sort_temp_bytes(uint32_t n);
-hipError_t wf_trace_occupancy(int *blocks_per_cu);
+hipError_t wf_trace_occupancy(uint32_t slds, int *blocks_per_cu);
 // trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
-// fetch of new rays (1..64)
+// fetch of new rays (1..64); slds: LDS stack entries per lane of the trace kernels (4, 8 or 16)
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t refill, uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp,
-                                  size_t sort_temp_bytes, hipStream_t s);
+                                  uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
+                                  void *sort_temp, size_t sort_temp_bytes, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
+// shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals
+hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out,
+                              hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s);
 
 struct TestLaunch {

@@ -50,15 +50,13 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
     Hit h;
     nrays++;
     if (!traverse<false>(S, o, d, 0.0f, INFINITY, h, st)) break;
-    const uint4 at = S.attr[h.tri];
-    const uint4 ms = S.meshes[at.w];
-    v3 ng, ns, nsi;
+    const float4 *sh = S.tshade + 4 * (size_t)h.tri;
+    const float4 n0 = sh[0];
+    const uint4 ms = S.meshes[__float_as_uint(n0.w)];
+    v3 ng = ld3(sh[3]), ns, nsi;
     {
-      const float4 *tp = S.tri + 3 * (size_t)h.tri;
-      const v3 p0 = ld3(tp[0]);
-      ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
       const float w0 = 1.0f - h.u - h.v;
-      nsi = normalize(ld3(S.vnorm[at.x]) * w0 + ld3(S.vnorm[at.y]) * h.u + ld3(S.vnorm[at.z]) * h.v);
+      nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h.u + ld3(sh[2]) * h.v);
     }
     ns = nsi;
     if (dot(ng, d) > 0.0f) ng = -ng;
@@ -259,8 +257,9 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
     Hit h;
     nrays++;
     if (!traverse<false>(S, val(o), val(d), 0.0f, INFINITY, h, st)) break;
-    const uint4 at = S.attr[h.tri];
-    const uint4 ms = S.meshes[at.w];
+    const float4 *sh = S.tshade + 4 * (size_t)h.tri;
+    const float4 n0 = sh[0];
+    const uint4 ms = S.meshes[__float_as_uint(n0.w)];
     const float4 *tp = S.tri + 3 * (size_t)h.tri;
     const v3 p0 = ld3(tp[0]), e1 = ld3(tp[1]) - p0, e2 = ld3(tp[2]) - p0;
     const v3 nt = cross(e1, e2);
@@ -271,8 +270,8 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
     const float inn = 1.0f / dot(nt, nt);
     const D bu = ddot(dcross(q, dv(e2)), nt) * inn;
     const D bv = ddot(dcross(dv(e1), q), nt) * inn;
-    v3 ng = normalize(nt);
-    const DV nsi = dnormalize(ld3(S.vnorm[at.x]) * (1.0f - bu - bv) + ld3(S.vnorm[at.y]) * bu + ld3(S.vnorm[at.z]) * bv);
+    v3 ng = ld3(sh[3]);
+    const DV nsi = dnormalize(ld3(n0) * (1.0f - bu - bv) + ld3(sh[1]) * bu + ld3(sh[2]) * bv);
     DV ns = nsi;
     const v3 dval = val(d);
     if (dot(ng, dval) > 0.0f) ng = -ng;
@@ -641,6 +640,27 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
       run_wave(Lc, w * Lc.wave_stride, st, nrays, nsamp);
   }
   flush_counts(Lc, nrays, nsamp);
+}
+
+// per-slot shading record: vertex normals (gathered through the slot's vertex ids), mesh id, geometric normal
+__global__ void k_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < T; k += gridDim.x * blockDim.x) {
+    const uint4 at = attr[k];
+    const float4 *tp = tri + 3 * (size_t)k;
+    const v3 p0 = ld3(tp[0]);
+    const v3 ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
+    const float4 a = vnorm[at.x], b = vnorm[at.y], c = vnorm[at.z];
+    out[4 * (size_t)k] = make_float4(a.x, a.y, a.z, __uint_as_float(at.w));
+    out[4 * (size_t)k + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+    out[4 * (size_t)k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    out[4 * (size_t)k + 3] = make_float4(ng.x, ng.y, ng.z, 0.0f);
+  }
+}
+hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out,
+                              hipStream_t s) {
+  const uint32_t g = (T + 255) / 256 < 65535u ? (T + 255) / 256 : 65535u;
+  hipLaunchKernelGGL(k_tri_shading, dim3(g ? g : 1), dim3(256), 0, s, tri, attr, vnorm, T, out);
+  return hipGetLastError();
 }
 
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {

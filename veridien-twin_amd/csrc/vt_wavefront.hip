@@ -84,11 +84,13 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
 // wave takes that many new rays from the queue head with one atomic and starts them, so lanes do not idle until
 // the slowest ray of a fixed group ends. ANY = false: extension rays of the live slots (count[0], head count[2]),
 // result into hit[slot]; ANY = true: shadow slots (count[0] * nl_max, head count[3]), visibility into sh_c.w.
-template <bool ANY>
+// SLDS: stack entries per lane kept in LDS (the rest of the STACK_TOTAL spill to global memory); a shallow
+// LDS stack lets more waves share a CU, which hides the node fetches better than the deeper stack saves.
+template <bool ANY, int SLDS>
 __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
-  __shared__ uint32_t s_node[STACK_LDS * 256];
-  __shared__ float s_t[STACK_LDS * 256];
-  Stack st{s_node, s_t, W.spill, threadIdx.x, 256, STACK_LDS, STACK_SPILL, W.spill_stride,
+  __shared__ uint32_t s_node[SLDS * 256];
+  __shared__ float s_t[SLDS * 256];
+  Stack st{s_node, s_t, W.spill, threadIdx.x, 256, SLDS, STACK_TOTAL - SLDS, W.spill_stride,
            (uint64_t)blockIdx.x * 256 + threadIdx.x, W.overflow};
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = ANY ? W.count[0] * W.nl_max : W.count[0];
@@ -272,15 +274,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         const float4 t4 = W.thr[p];
         const v3 thr = mk(t4.x, t4.y, t4.z);
         W.thr_slot[k] = t4;             // the gather's weight for this vertex's NEE
-        const uint4 at = W.S.attr[tri];
-        const uint4 ms = W.S.meshes[at.w];
-        v3 ng, ns, nsi;
+        const float4 *sh = W.S.tshade + 4 * (size_t)tri;
+        const float4 n0 = sh[0];
+        const uint4 ms = W.S.meshes[__float_as_uint(n0.w)];
+        v3 ng = ld3(sh[3]), ns, nsi;
         {
-          const float4 *tp = W.S.tri + 3 * (size_t)tri;
-          const v3 p0 = ld3(tp[0]);
-          ng = normalize(cross(ld3(tp[1]) - p0, ld3(tp[2]) - p0));
           const float w0 = 1.0f - h4.y - h4.z;
-          nsi = normalize(ld3(W.S.vnorm[at.x]) * w0 + ld3(W.S.vnorm[at.y]) * h4.y + ld3(W.S.vnorm[at.z]) * h4.z);
+          nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h4.y + ld3(sh[2]) * h4.z);
         }
         ns = nsi;
         if (dot(ng, d) > 0.0f) ng = -ng;
@@ -466,8 +466,13 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 
 }  // namespace
 
-hipError_t wf_trace_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false>, 256, 0);
+hipError_t wf_trace_occupancy(uint32_t slds, int *blocks_per_cu) {
+  switch (slds) {
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 4>, 256, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 8>, 256, 0);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 16>, 256, 0);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 size_t wf_sort_temp_bytes(uint32_t n) {
@@ -479,9 +484,9 @@ size_t wf_sort_temp_bytes(uint32_t n) {
 
 // One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t refill, uint32_t *keys_sorted, uint32_t *q_sorted, void *sort_temp,
-                                  size_t sort_temp_bytes, hipStream_t s) {
-  if (refill < 1 || refill > 64) return hipErrorInvalidValue;
+                                  uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
+                                  void *sort_temp, size_t sort_temp_bytes, hipStream_t s) {
+  if (refill < 1 || refill > 64 || (slds != 4 && slds != 8 && slds != 16)) return hipErrorInvalidValue;
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
   const uint32_t gs = grid_for((uint64_t)W.n * W.nl_max, 256, trace_grid_cap);
@@ -489,9 +494,17 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
   W.q = nullptr;   // vertex 1: the camera rays in item order (coherent already)
   for (uint32_t b = 0; b < max_depth; b++) {
     W.bounce = b;
-    hipLaunchKernelGGL(k_wf_trace_dyn<false>, dim3(gt), dim3(256), 0, s, W, refill);
+    switch (slds) {
+      case 4: hipLaunchKernelGGL((k_wf_trace_dyn<false, 4>), dim3(gt), dim3(256), 0, s, W, refill); break;
+      case 8: hipLaunchKernelGGL((k_wf_trace_dyn<false, 8>), dim3(gt), dim3(256), 0, s, W, refill); break;
+      default: hipLaunchKernelGGL((k_wf_trace_dyn<false, 16>), dim3(gt), dim3(256), 0, s, W, refill); break;
+    }
     hipLaunchKernelGGL(k_wf_shade, dim3(g), dim3(256), 0, s, W);
-    hipLaunchKernelGGL(k_wf_trace_dyn<true>, dim3(gs), dim3(256), 0, s, W, refill);
+    switch (slds) {
+      case 4: hipLaunchKernelGGL((k_wf_trace_dyn<true, 4>), dim3(gs), dim3(256), 0, s, W, refill); break;
+      case 8: hipLaunchKernelGGL((k_wf_trace_dyn<true, 8>), dim3(gs), dim3(256), 0, s, W, refill); break;
+      default: hipLaunchKernelGGL((k_wf_trace_dyn<true, 16>), dim3(gs), dim3(256), 0, s, W, refill); break;
+    }
     hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
     if (b + 1 == max_depth) break;
     size_t tb = sort_temp_bytes;

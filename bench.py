@@ -53,7 +53,7 @@ def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
                 if 0 <= x0 <= w - 32 and 0 <= y0 <= h - 32:
                     tiles.append((x0, y0))
     rays = 0
-    vis = np.zeros(2, np.uint64)
+    vis = np.zeros(5, np.uint64)
     px = 0
     t1 = time.time()
     for (x0, y0) in tiles:
@@ -64,10 +64,12 @@ def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
         if time.time() - t1 > budget_s:
             break
     dt = time.time() - t1
+    ext = max(int(vis[2]), 1)
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=nthreads, kind="port",
                 sample="%d pixels (%d 32x32 tiles at the centre of camera 0) x %d spp, depth %d: %d rays in %.1f s on %s"
                 % (px, px // 1024, cam["spp"], cam["max_depth"], rays, dt, platform.processor() or platform.machine()),
-                rays=rays, node_visits_per_ray=float(vis[0]) / max(rays, 1), tri_tests_per_ray=float(vis[1]) / max(rays, 1))
+                rays=rays, node_visits_per_ray=float(vis[0]) / max(rays, 1), tri_tests_per_ray=float(vis[1]) / max(rays, 1),
+                ext_share=ext / max(rays, 1), node_visits_per_ext_ray=float(vis[3]) / ext, tri_tests_per_ext_ray=float(vis[4]) / ext)
 
 
 def bench_c3(args):
@@ -82,32 +84,36 @@ def bench_c3(args):
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
     res = {}
-    for name, mode in (("per_camera", lib.VT_RENDER_PER_CAMERA), ("batched", lib.VT_RENDER_BATCHED)):
-        ctx.render(slots, mode=mode)
+    for name, mode, graph in (("per_camera", lib.VT_RENDER_PER_CAMERA, False), ("batched", lib.VT_RENDER_BATCHED, True),
+                              ("wavefront", lib.VT_RENDER_WAVEFRONT, False)):
+        ctx.render(slots, mode=mode, graph=graph)
         ctx.sync()
         t1 = time.perf_counter()
         rays = 0.0
         for _ in range(args.steps):
-            ctx.render(slots, mode=mode)
+            ctx.render(slots, mode=mode, graph=graph)
             st = ctx.stats()
             rays += st["rays"]
         t2 = time.perf_counter()
         res[name] = dict(ms_per_frame=1e3 * (t2 - t1) / args.steps, mrays=rays / (t2 - t1) / 1e6, launches=st["launches"])
         log("%s: %.1f ms/frame, %.1f Mrays/s" % (name, res[name]["ms_per_frame"], res[name]["mrays"]))
-    # identical images from both forms
-    ctx.render(slots, mode=lib.VT_RENDER_BATCHED)
-    a = ctx.readback(5, lib.VT_KIND_LINEAR, cams[5]["width"], cams[5]["height"])
-    ctx.render(slots, mode=lib.VT_RENDER_PER_CAMERA)
-    b = ctx.readback(5, lib.VT_KIND_LINEAR, cams[5]["width"], cams[5]["height"])
+    # identical images from the three forms
+    imgs = []
+    for mode in (lib.VT_RENDER_BATCHED, lib.VT_RENDER_PER_CAMERA, lib.VT_RENDER_WAVEFRONT):
+        ctx.render(slots, mode=mode)
+        imgs.append(ctx.readback(5, lib.VT_KIND_LINEAR, cams[5]["width"], cams[5]["height"]))
     spp_px = sum(c["spp"] * c["width"] * c["height"] for c in cams)
     out = {"metric": METRIC, "config": {"workload": "batching: 24 KB4 cameras 320x240..2048x1536, spp in {1,4,16,64}, depth 5, "
                                                      "2,013,432-tri scene"},
-           "value": round(res["batched"]["mrays"], 2), "unit": "Mrays/s", "steps": args.steps,
+           "value": round(res["wavefront"]["mrays"], 2), "unit": "Mrays/s", "steps": args.steps,
            "batched_ms_per_frame": round(res["batched"]["ms_per_frame"], 2),
+           "wavefront_ms_per_frame": round(res["wavefront"]["ms_per_frame"], 2),
            "per_camera_ms_per_frame": round(res["per_camera"]["ms_per_frame"], 2),
            "speedup_batched_over_per_camera": round(res["per_camera"]["ms_per_frame"] / res["batched"]["ms_per_frame"], 3),
-           "spp_mpix_per_s": round(spp_px / (res["batched"]["ms_per_frame"] * 1e-3) / 1e6, 2),
-           "images_identical": bool(np.array_equal(a, b)), "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
+           "speedup_wavefront_over_per_camera": round(res["per_camera"]["ms_per_frame"] / res["wavefront"]["ms_per_frame"], 3),
+           "spp_mpix_per_s": round(spp_px / (res["wavefront"]["ms_per_frame"] * 1e-3) / 1e6, 2),
+           "images_identical": bool(np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])),
+           "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -188,7 +194,7 @@ def bench_c5(args):
         ctx.set_camera(k, c2)
     fwd, bwd, rays_f, rays_b = [], [], [], []
     for it in range(args.warmup + args.steps):
-        ctx.render(slots)
+        ctx.render(slots, mode=lib.VT_RENDER_WAVEFRONT)
         st = ctx.stats()
         imgs = [ctx.readback(k, lib.VT_KIND_LINEAR, c["width"], c["height"]) for k, c in enumerate(cams)]
         adj = [im - t for im, t in zip(imgs, targets)]
@@ -213,7 +219,7 @@ def bench_c5(args):
                                                      "depth 5; gradients w.r.t. 256 material colours (rgb), 16 emitters (rgb) "
                                                      "and 3 x (8 KB4 intrinsics + 6 pose)"},
            "value": round(float(np.mean(bwd)) / float(np.mean(fwd)), 3), "unit": "x (adjoint / forward GPU time)",
-           "forward_ms": round(float(np.mean(fwd)), 2), "backward_ms": round(float(np.mean(bwd)), 2),
+           "forward_mode": "wavefront", "forward_ms": round(float(np.mean(fwd)), 2), "backward_ms": round(float(np.mean(bwd)), 2),
            "forward_mrays_s": round(float(np.mean(rays_f)) / (float(np.mean(fwd)) * 1e-3) / 1e6, 1),
            "backward_mrays_s": round(float(np.mean(rays_b)) / (float(np.mean(bwd)) * 1e-3) / 1e6, 1),
            "loss_perturbed": loss, "loss_after_gradient_step": loss2, "steps": args.steps,
@@ -257,11 +263,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["batched", "per-camera"], default="batched")
+    ap.add_argument("--mode", choices=["wavefront", "batched", "per-camera"], default="wavefront",
+                    help="wavefront: per-vertex kernels with sorted rays (default); batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
+    ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
                          "copy: counter calibration kernel")
@@ -291,10 +300,11 @@ def main():
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
     sensor = [scenes.default_sensor(cfa=0, frame=0, seed=1006) for _ in cams]
-    mode = lib.VT_RENDER_BATCHED if args.mode == "batched" else lib.VT_RENDER_PER_CAMERA
+    mode = {"wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
+            "per-camera": lib.VT_RENDER_PER_CAMERA}[args.mode]
 
     def frame():
-        ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk)
+        ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk, stack_lds=args.stack, batch=args.batch)
         for k in slots:
             ctx.sensor(k, sensor[k])
 
@@ -302,7 +312,7 @@ def main():
         frame()
         ctx.sync()
         log("warmup %d done" % i)
-    ms_kernel, rays_step, launches = [], [], 0
+    ms_kernel, rays_step, launches, ext_ms, ext_rays, ext_launches = [], [], 0, [], [], 0
     ctx.sync()
     t1 = time.perf_counter()
     for i in range(args.steps):
@@ -311,6 +321,9 @@ def main():
         ms_kernel.append(st["ms_kernel_avg"])
         rays_step.append(st["rays"])
         launches = st["launches"]
+        ext_ms.append(st["ms_ext_avg"])
+        ext_rays.append(st["rays_extension"])
+        ext_launches = st["ext_launches"]
         log("step %d: %.1f ms trace, %.3g rays" % (i, st["ms_render"], st["rays"]))
     ctx.sync()
     t2 = time.perf_counter()
@@ -330,19 +343,31 @@ def main():
         "rays_per_frame": total_rays / args.steps,
         "trace_ms_per_launch": round(float(np.mean(ms_kernel)), 3),
         "trace_launches_per_frame": launches,
+        "extension_trace_launches_per_frame": ext_launches,
     }
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(sc, cams[0], budget_s=args.cpu_budget)
         out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        vn, vt = cpu["node_visits_per_ray"], cpu["tri_tests_per_ray"]
-        b_ray = 32 + 16 + 64 * vn + 48 * vt
-        rays_launch = float(np.mean(rays_step)) / max(launches, 1)
-        achieved = b_ray * rays_launch / (float(np.mean(ms_kernel)) * 1e-3) / 1e9
+        if ext_launches:
+            # the dominant kernel of the wavefront frame: the extension-ray trace (camera and bounce rays)
+            vn, vt = cpu["node_visits_per_ext_ray"], cpu["tri_tests_per_ext_ray"]
+            b_ray = 32 + 16 + 64 * vn + 48 * vt
+            rays_launch = float(np.mean(ext_rays)) / ext_launches
+            ms_launch = float(np.mean(ext_ms))
+            kernel, prof = "k_wf_trace_dyn<false, 8> (camera and extension rays)", "k_wf_trace_c2_full_scale_forward.md"
+        else:
+            vn, vt = cpu["node_visits_per_ray"], cpu["tri_tests_per_ray"]
+            b_ray = 32 + 16 + 64 * vn + 48 * vt
+            rays_launch = float(np.mean(rays_step)) / max(launches, 1)
+            ms_launch = float(np.mean(ms_kernel))
+            kernel, prof = "k_render<256>", "k_render_c2_full_scale_forward.md"
+        achieved = b_ray * rays_launch / (ms_launch * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-                           "kernel": "k_render<256>", "bytes_per_ray": round(b_ray, 1),
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(prof),
+                           "kernel": kernel, "bytes_per_ray": round(b_ray, 1), "rays_per_launch": round(rays_launch),
+                           "ms_per_launch": round(ms_launch, 3),
                            "node_visits_per_ray": round(vn, 2), "tri_tests_per_ray": round(vt, 2)}
     print(json.dumps(out), flush=True)
     ctx.close()

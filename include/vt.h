@@ -139,6 +139,10 @@ typedef struct {
   double rays_backward;  /* rays traced by the last vt_backward (two walks per sample) */
   uint32_t graph_used;   /* 1 if the last vt_render ran as a hipGraph */
   uint32_t graph_builds; /* hipGraph captures since the context was created */
+  double rays_extension; /* wavefront: camera and extension rays of the last vt_render */
+  double rays_shadow;    /* wavefront: shadow rays of the last vt_render */
+  double ms_ext_avg;     /* wavefront: average duration of its extension-trace launches */
+  uint32_t ext_launches; /* wavefront: extension-trace launches of the last vt_render */
 } vt_frame_stats;
 
 /* ---------------------------------------------------------------- lifetime (TwinRenderer.create / destroy,

@@ -207,7 +207,7 @@ def render_tile(sc, cam, x0, y0, w, h, spp_begin=0, spp_end=None, nthreads=8):
     spp_end = cam["spp"] if spp_end is None else spp_end
     out = np.zeros((h, w, 4), dtype=np.float32)
     rays = C.c_uint64(0)
-    vis = np.zeros(2, dtype=np.uint64)
+    vis = np.zeros(5, dtype=np.uint64)
     lib().or_render_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(out), C.byref(rays), ptr(vis),
                          nthreads)
     return out, int(rays.value), vis

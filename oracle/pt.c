@@ -189,6 +189,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
     or_hit h;
     or_traverse(S->nodes, S->tris, o, d, 0, INFINITY, 0, 0, &h);
     (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
+    visits[2]++; visits[3] += h.node_visits; visits[4] += h.tri_tests;
     seq_add(seq, h.tri);
     if (h.tri < 0) break;
     surf_t s;
@@ -299,15 +300,15 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
 
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                     uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads) {
-  uint64_t nr = 0, v0 = 0, v1 = 0;
-#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1) reduction(+ : nr, v0, v1) collapse(2)
+  uint64_t nr = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1) reduction(+ : nr, v0, v1, v2, v3, v4) collapse(2)
   for (uint32_t yy = 0; yy < h; yy++)
     for (uint32_t xx = 0; xx < w; xx++) {
       uint32_t x = x0 + xx, y = y0 + yy;
       pix_t P;
       pix_seeds(C, x, y, &P);
       double acc[3] = {0, 0, 0};
-      uint64_t r = 0, vv[2] = {0, 0};
+      uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0};
       for (uint32_t i = spp_begin; i < spp_end; i++) {
         double L[3];
         sample_radiance(S, C, &P, x, y, i, L, &r, vv, NULL);
@@ -316,10 +317,10 @@ void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t
       double inv = spp_end > spp_begin ? 1.0 / (spp_end - spp_begin) : 0;
       float *o = out + 4 * ((size_t)yy * w + xx);
       o[0] = (float)(acc[0] * inv); o[1] = (float)(acc[1] * inv); o[2] = (float)(acc[2] * inv); o[3] = 1;
-      nr += r; v0 += vv[0]; v1 += vv[1];
+      nr += r; v0 += vv[0]; v1 += vv[1]; v2 += vv[2]; v3 += vv[3]; v4 += vv[4];
     }
   if (rays) *rays = nr;
-  if (visits) { visits[0] = v0; visits[1] = v1; }
+  if (visits) { visits[0] = v0; visits[1] = v1; visits[2] = v2; visits[3] = v3; visits[4] = v4; }
 }
 
 void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -339,7 +340,7 @@ void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32
         pix_seeds(C, x0 + xx, y0 + yy, &P);
         for (uint32_t i = spp_begin; i < spp_end; i++) {
           double L[3];
-          uint64_t r = 0, vv[2] = {0, 0};
+          uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0};
           sample_radiance(S, C, &P, x0 + xx, y0 + yy, i, L, &r, vv, NULL);
           grad_t G = {gm, gl, {a[0] * sc, a[1] * sc, a[2] * sc}, {L[0], L[1], L[2]}};
           sample_radiance(S, C, &P, x0 + xx, y0 + yy, i, L, &r, vv, &G);
@@ -426,7 +427,7 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
             uvf[0] = ua - 0.5f; uvf[1] = va - 0.5f;
           }
           double ray[6], L0[3];
-          uint64_t r = 0, vv[2] = {0, 0}, seq0 = 0;
+          uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0}, seq0 = 0;
           ray_of_theta(C, th0, uvf[0], uvf[1], ray);
           sample_radiance_ex(S, C, &P, x, y, i, L0, &r, vv, NULL, ray, &seq0);
           for (int k = 0; k < 14; k++) {

@@ -147,7 +147,9 @@ typedef struct {
   uint32_t width, height, spp, seed, cam_index, light_samples, max_depth, flags;
 } or_camera;
 /* radiance of pixels [x0, x0+w) x [y0, y0+h) of a camera, spp sub-samples each, mean written as rgba
- * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays. */
+ * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays;
+ * visits (optional, 5 counts): node visits and triangle tests of all rays, then the camera and extension rays
+ * alone (rays, node visits, triangle tests). */
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                     uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads);
 /* path-replay adjoint of the tile: adds d(sum_p adj_p . I_p)/d(material colour) (mat_count*3) and

@@ -54,9 +54,11 @@ struct vt_ctx {
   hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
   std::vector<hipEvent_t> ev_launch;
   uint32_t last_launches = 0;
+  std::vector<hipEvent_t> ev_ext;   // wavefront: around each extension-trace launch of the last frame
+  uint32_t ext_launches = 0;
   bool render_timed = false, sensor_timed = false, backward_timed = false;
   hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
-  DevBuf grads;
+  DevBuf grads, ltot;
   uint32_t n_mat = 0, n_light = 0;
   // launch shape used for vt_render_opts fields left 0 (vt_autotune replaces it)
   uint32_t def_block = DEFAULT_BLOCK, def_stack = STACK_LDS, def_chunk = 4, def_queues = 1;
@@ -246,7 +248,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
-                    &c->dXd, &c->dR, &c->dPart, &c->grads, &c->wf, &c->wf_temp};
+                    &c->dXd, &c->dR, &c->dPart, &c->grads, &c->wf, &c->wf_temp, &c->ltot};
   for (DevBuf *b : bufs) release(*b);
   for (auto &cs : c->cams) {
     release(cs.acc);
@@ -256,6 +258,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.adj);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : c->ev_ext) (void)hipEventDestroy(ev);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1, c->ev_b0, c->ev_b1};
   for (hipEvent_t ev : evs)
@@ -675,12 +678,20 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
   const uint32_t grid_cap = (uint32_t)c->cu_count * 8;
   const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
   const uint32_t refill = opt.chunk ? std::min(64u, opt.chunk) : 16u;
-  for (uint64_t bb = 0; bb < items; bb += P) {
+  const uint64_t batches = (items + P - 1) / P;
+  while (c->ev_ext.size() < 2 * batches * depth) {
+    hipEvent_t ev;
+    VT_HIP(c, hipEventCreate(&ev));
+    c->ev_ext.push_back(ev);
+  }
+  uint64_t bi = 0;
+  for (uint64_t bb = 0; bb < items; bb += P, bi++) {
     W.batch_begin = bb;
     W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
     VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, slds, keys_sorted, q_sorted,
-                                     c->wf_temp.p, c->wf_temp_bytes, c->stream));
+                                     c->wf_temp.p, c->wf_temp_bytes, c->ev_ext.data() + 2 * depth * bi, c->stream));
   }
+  c->ext_launches = (uint32_t)(batches * depth);
   VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
   return VT_OK;
 }
@@ -702,6 +713,7 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   RenderLaunch L = base_launch(c, sh);
   uint32_t launches = 0;
   c->graph_used = false;
+  c->ext_launches = 0;
   if (opt.mode == VT_RENDER_WAVEFRONT) {
     if ((rc = ensure_launch_events(c, 1))) return rc;
     if ((rc = render_wavefront(c, n, opt, items))) return rc;
@@ -962,6 +974,18 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
   VT_HIP(c, hipMemcpy(cnt, c->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
   out->graph_used = c->graph_used ? 1u : 0u;
   out->graph_builds = c->graph_builds;
+  out->ext_launches = c->ext_launches;
+  if (c->ext_launches) {
+    out->rays_extension = (double)cnt[3];
+    out->rays_shadow = (double)cnt[4];
+    double sum = 0;
+    for (uint32_t k = 0; k < c->ext_launches; k++) {
+      float m = 0;
+      VT_HIP(c, hipEventElapsedTime(&m, c->ev_ext[2 * k], c->ev_ext[2 * k + 1]));
+      sum += m;
+    }
+    out->ms_ext_avg = sum / c->ext_launches;
+  }
   uint32_t ov[4];
   VT_HIP(c, hipMemcpy(ov, c->overflow.p, sizeof ov, hipMemcpyDeviceToHost));
   out->rays = (double)cnt[1];
@@ -1072,7 +1096,10 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   float *gm = (float *)c->grads.p;
   VT_HIP(c, hipEventRecord(c->ev_b0, c->stream));
   if (ngc) {
-    // the camera walk first, on its own queue head and ray counter ([6], [7])
+    // the camera walk first, on its own queue head and ray counter ([6], [7]); it leaves each sample's
+    // radiance for the gradient walk, which then replays each path once instead of twice
+    if ((rc = ensure(c, c->ltot, (size_t)L.total_waves * 64 * sizeof(float4)))) return rc;
+    L.ltot = (float4 *)c->ltot.p;
     RenderLaunch Lcam = L;
     Lcam.counters = (unsigned long long *)c->counters.p + 6;
     VT_HIP(c, launch_camgrad(Lcam, gm + ng_pad, c->stream));

@@ -40,6 +40,8 @@ struct RenderLaunch {
   uint32_t queues;             // persistent: 1 global queue, or XCD_COUNT queues over contiguous eighths of
                                // the waves (a block drains its XCD's eighth, then the others)
   uint64_t wave_stride;        // queue wave w runs frame wave w * wave_stride (1; > 1 for tuning probes)
+  float4 *ltot;                // adjoint: each sample's radiance by queue item, written by the camera walk and
+                               // read by the gradient walk in place of its own first walk (null: walk it)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
 hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu);
@@ -75,18 +77,19 @@ struct WfLaunch {
   uint32_t *nee;               // shadow slots used per queue slot
   float4 *sh_o, *sh_d, *sh_c;  // shadow slots [slot * nl_max + j]: origin + tmax, direction, contribution + visible
   float3 bmin, bscale;         // scene box and 512 / extent, for the origin's Morton code
-  unsigned long long *counters;// [1] rays, [2] samples
+  unsigned long long *counters;// [1] rays, [2] samples, [3] extension rays, [4] shadow rays
   uint32_t *overflow;
   uint2 *spill;
   uint64_t spill_stride;
 };
+// ev_ext (optional): 2 * max_depth events, recorded around each extension-trace launch of the batch
 size_t wf_sort_temp_bytes(uint32_t n);
 hipError_t wf_trace_occupancy(uint32_t slds, int *blocks_per_cu);
 // trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
 // fetch of new rays (1..64); slds: LDS stack entries per lane of the trace kernels (4, 8 or 16)
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
                                   uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
-                                  void *sort_temp, size_t sort_temp_bytes, hipStream_t s);
+                                  void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 // shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals
 hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out,

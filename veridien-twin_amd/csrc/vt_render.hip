@@ -469,7 +469,13 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
     if (gs.delta.x != 0.0f || gs.delta.y != 0.0f || gs.delta.z != 0.0f) {
       PixSeeds P;
       pix_seeds(C, it.x, it.y, P);
-      const v3 Ltot = path_walk<false>(Lc.S, C, P, it.x, it.y, it.i, st, nrays);
+      v3 Ltot;
+      if (Lc.ltot) {
+        const float4 l4 = Lc.ltot[wave * 64 + lane];
+        Ltot = mk(l4.x, l4.y, l4.z);
+      } else {
+        Ltot = path_walk<false>(Lc.S, C, P, it.x, it.y, it.i, st, nrays);
+      }
       path_walk<true>(Lc.S, C, P, it.x, it.y, it.i, st, nrays, &gs, Ltot);
     }
   }
@@ -500,7 +506,8 @@ __device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wa
       v3 e1, e2;
       basis(d0, e1, e2);
       float ws[ND];
-      (void)path_walk_tan(Lc.S, C, P, it.i, o0, d0, e1, e2, delta, st, nrays, ws);
+      const v3 Lv = path_walk_tan(Lc.S, C, P, it.i, o0, d0, e1, e2, delta, st, nrays, ws);
+      if (Lc.ltot) Lc.ltot[wave * 64 + lane] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
       const v3 wo = e1 * ws[0] + e2 * ws[1];
       const v3 wd = e1 * ws[2] + e2 * ws[3];
       // direction: rotation r gives d' = d + r x d; intrinsics through d = R dc

@@ -237,7 +237,10 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
     }
   }
   for (int off = 32; off >= 1; off >>= 1) nrays += __shfl_xor(nrays, off, 64);
-  if (lane == 0 && nrays) atomicAdd(&W.counters[1], (unsigned long long)nrays);
+  if (lane == 0 && nrays) {
+    atomicAdd(&W.counters[1], (unsigned long long)nrays);
+    atomicAdd(&W.counters[ANY ? 4 : 3], (unsigned long long)nrays);
+  }
 }
 
 // one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
@@ -485,7 +488,7 @@ size_t wf_sort_temp_bytes(uint32_t n) {
 // One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
                                   uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
-                                  void *sort_temp, size_t sort_temp_bytes, hipStream_t s) {
+                                  void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext, hipStream_t s) {
   if (refill < 1 || refill > 64 || (slds != 4 && slds != 8 && slds != 16)) return hipErrorInvalidValue;
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
@@ -494,10 +497,18 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
   W.q = nullptr;   // vertex 1: the camera rays in item order (coherent already)
   for (uint32_t b = 0; b < max_depth; b++) {
     W.bounce = b;
+    if (ev_ext) {
+      hipError_t e = hipEventRecord(ev_ext[2 * b], s);
+      if (e != hipSuccess) return e;
+    }
     switch (slds) {
       case 4: hipLaunchKernelGGL((k_wf_trace_dyn<false, 4>), dim3(gt), dim3(256), 0, s, W, refill); break;
       case 8: hipLaunchKernelGGL((k_wf_trace_dyn<false, 8>), dim3(gt), dim3(256), 0, s, W, refill); break;
       default: hipLaunchKernelGGL((k_wf_trace_dyn<false, 16>), dim3(gt), dim3(256), 0, s, W, refill); break;
+    }
+    if (ev_ext) {
+      hipError_t e = hipEventRecord(ev_ext[2 * b + 1], s);
+      if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_wf_shade, dim3(g), dim3(256), 0, s, W);
     switch (slds) {

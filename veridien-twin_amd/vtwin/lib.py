@@ -61,7 +61,8 @@ class SensorParams(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("rays", f64), ("samples", f64), ("ms_render", f64), ("ms_sensor", f64), ("ms_kernel_avg", f64),
                 ("launches", u32), ("stack_overflows", u32), ("ms_backward", f64), ("rays_backward", f64),
-                ("graph_used", u32), ("graph_builds", u32)]
+                ("graph_used", u32), ("graph_builds", u32), ("rays_extension", f64), ("rays_shadow", f64),
+                ("ms_ext_avg", f64), ("ext_launches", u32)]
 
 
 class Grads(C.Structure):

@@ -246,6 +246,25 @@ def bench_copy(args):
                       "check": float(y[-1].item())}), flush=True)
 
 
+def mall_bandwidth(mib=64, iters=50):
+    """Copy bandwidth with a working set that stays in the 256 MB Infinity Cache (MALL): y = x over two mib-MiB
+    buffers, iters times, bytes read + written per second (HIP events on torch's stream)."""
+    import torch
+
+    n = (mib << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2.0 * 4 * n * iters / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
 def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
@@ -364,8 +383,12 @@ def main():
             ms_launch = float(np.mean(ms_kernel))
             kernel, prof = "k_render<256>", "k_render_c2_full_scale_forward.md"
         achieved = b_ray * rays_launch / (ms_launch * 1e-3) / 1e9
+        mall = mall_bandwidth()
+        traffic = traffic_from_profile(prof)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(prof),
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "mall_peak_measured": round(mall, 1), "frac_vs_mall": round(achieved / mall, 4),
+                           "traffic_GBs": round(traffic / (ms_launch * 1e-3) / 1e9, 1) if traffic else None,
                            "kernel": kernel, "bytes_per_ray": round(b_ray, 1), "rays_per_launch": round(rays_launch),
                            "ms_per_launch": round(ms_launch, 3),
                            "node_visits_per_ray": round(vn, 2), "tri_tests_per_ray": round(vt, 2)}

@@ -658,7 +658,7 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
   float ext[3];
   for (int k = 0; k < 3; k++) ext[k] = std::max(c->bmax[k] - c->bmin[k], 1e-6f);
   W.bmin = make_float3(c->bmin[0], c->bmin[1], c->bmin[2]);
-  W.bscale = make_float3(512.0f / ext[0], 512.0f / ext[1], 512.0f / ext[2]);
+  W.bscale = make_float3(128.0f / ext[0], 128.0f / ext[1], 128.0f / ext[2]);
   W.counters = (unsigned long long *)c->counters.p;
   W.overflow = (uint32_t *)c->overflow.p;
   W.spill = (uint2 *)c->spill.p;

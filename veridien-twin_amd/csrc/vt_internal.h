@@ -76,7 +76,7 @@ struct WfLaunch {
   float4 *thr_slot;            // the throughput at this vertex per slot (weight of its NEE in the gather)
   uint32_t *nee;               // shadow slots used per queue slot
   float4 *sh_o, *sh_d, *sh_c;  // shadow slots [slot * nl_max + j]: origin + tmax, direction, contribution + visible
-  float3 bmin, bscale;         // scene box and 512 / extent, for the origin's Morton code
+  float3 bmin, bscale;         // scene box and 128 / extent, for the origin's Morton code
   unsigned long long *counters;// [1] rays, [2] samples, [3] extension rays, [4] shadow rays
   uint32_t *overflow;
   uint2 *spill;

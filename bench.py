@@ -293,7 +293,7 @@ def main():
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
                          "copy: counter calibration kernel")
-    ap.add_argument("--inner", type=int, default=32, help="BF16 sweeps per refinement step (c4)")
+    ap.add_argument("--inner", type=int, default=0, help="BF16 sweeps per refinement step (c4; 0: adaptive)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box); run --gpus 1",

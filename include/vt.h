@@ -256,7 +256,8 @@ typedef struct {
   uint32_t method;        /* VT_DENSE_* */
   double tol;             /* stop when ||e + F x - x||_F <= tol ||e||_F (FP64 check) */
   uint32_t max_sweeps;
-  uint32_t inner_sweeps;  /* BF16 sweeps per FP32 refinement step */
+  uint32_t inner_sweeps;  /* BF16 sweeps per FP32 refinement step (0: until the correction changes by less
+                             than 2^-9 of the residual, BF16's resolution) */
 } vt_dense_opts;
 typedef struct {
   double residual;        /* final relative residual, FP64 */

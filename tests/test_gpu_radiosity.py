@@ -89,14 +89,17 @@ def _dense_problem(n, r, seed=5):
 
 
 # the solution is returned in binary32, so the FP64 solve is checked at binary32 rounding (6e-8)
-@pytest.mark.parametrize("method,tol,err", [(lib.VT_DENSE_FP64, 1e-12, 1e-7), (lib.VT_DENSE_FP32, 1e-6, 2e-5),
-                                            (lib.VT_DENSE_BF16_REFINE, 1e-6, 1e-5)])
-def test_dense_solves_match_direct_solve(ctx, method, tol, err):
+@pytest.mark.parametrize("method,tol,err,inner", [(lib.VT_DENSE_FP64, 1e-12, 1e-7, 0), (lib.VT_DENSE_FP32, 1e-6, 2e-5, 0),
+                                                  (lib.VT_DENSE_BF16_REFINE, 1e-6, 1e-5, 24),
+                                                  (lib.VT_DENSE_BF16_REFINE, 1e-6, 1e-5, 0)])
+def test_dense_solves_match_direct_solve(ctx, method, tol, err, inner):
+    """FP64 and FP32 (MFMA) Jacobi, BF16 MFMA sweeps with FP32-residual refinement at a fixed and at the
+    adaptive inner count, against LAPACK on a 1024-node form-factor-like system with 48 right-hand sides."""
     n, r = 1024, 48
     F, e = _dense_problem(n, r)
     want = np.linalg.solve(np.eye(n) - F.astype(np.float64), e.astype(np.float64))
     ctx.dense_upload(F, e)
-    x, res = ctx.dense_solve(n, r, method, tol=tol, max_sweeps=3000, inner_sweeps=24)
+    x, res = ctx.dense_solve(n, r, method, tol=tol, max_sweeps=3000, inner_sweeps=inner)
     rel = np.linalg.norm(x - want) / np.linalg.norm(want)
     print("method %d: sweeps %d, refinements %d, FP64 residual %.2e, error vs direct %.2e" % (method, res["sweeps"], res["refinements"], res["residual"], rel))
     assert res["residual"] <= max(tol, 2e-7) * 1.5

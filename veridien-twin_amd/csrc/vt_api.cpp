@@ -1360,10 +1360,18 @@ extern "C" int vt_dense_solve(vt_ctx *c, const vt_dense_opts *o, float *x_out, v
   int rc;
   Norms nm;
   VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
+  // x transposed (RHS-major) for the MFMA sweeps
+  DevBuf xT32;
+  if ((rc = ensure(c, xT32, (size_t)rp * n * 4))) return rc;
+  float *XT = (float *)xT32.p;
+  struct Free {
+    DevBuf &b;
+    ~Free() { release(b); }
+  } free_xT{xT32};
   if (o->method == VT_DENSE_FP32) {
     VT_HIP(c, hipMemcpyAsync(X, E, m * 4, hipMemcpyDeviceToDevice, c->stream));
     for (uint32_t s = 0; s < maxs; s++) {
-      VT_HIP(c, launch_sweep_fp32(F, n, rp, X, E, Y, c->stream));
+      VT_HIP(c, launch_sweep_fp32_mfma(F, n, rp, X, XT, E, Y, c->stream));
       VT_HIP(c, launch_resid_f32(Y, X, E, nullptr, m, part, c->stream));
       std::swap(X, Y);
       res->sweeps++;
@@ -1376,35 +1384,47 @@ extern "C" int vt_dense_solve(vt_ctx *c, const vt_dense_opts *o, float *x_out, v
       if ((rc = ensure(c, c->dF16, (size_t)n * n * 2))) return rc;
       VT_HIP(c, launch_convert_bf16(F, (uint16_t *)c->dF16.p, (uint64_t)n * n, c->stream));
     }
-    DevBuf xT, D;
+    DevBuf xT, D0, D1;
     if ((rc = ensure(c, xT, m * 2))) return rc;
-    if ((rc = ensure(c, D, m * 4))) return rc;
-    float *Dp = (float *)D.p;
-    const uint32_t inner = o->inner_sweeps ? o->inner_sweeps : 32;
+    if ((rc = ensure(c, D0, m * 4))) return rc;
+    if ((rc = ensure(c, D1, m * 4))) return rc;
+    float *Da = (float *)D0.p, *Db = (float *)D1.p;
+    // inner sweeps: a fixed count, or (0) until the correction changes by less than BF16's resolution
+    const uint32_t inner = o->inner_sweeps ? o->inner_sweeps : 64;
+    const bool adaptive = o->inner_sweeps == 0;
     VT_HIP(c, hipMemsetAsync(X, 0, m * 4, c->stream));
     bool ok = true;
     hipError_t er = hipSuccess;
     while (res->sweeps < maxs) {
       // FP32 residual r = e + F x - x
-      if ((er = launch_sweep_fp32(F, n, rp, X, E, Y, c->stream)) != hipSuccess) break;
+      if ((er = launch_sweep_fp32_mfma(F, n, rp, X, XT, E, Y, c->stream)) != hipSuccess) break;
       if ((er = launch_sub(Y, X, R, m, c->stream)) != hipSuccess) break;
       if ((er = launch_resid_f32(Y, X, E, nullptr, m, part, c->stream)) != hipSuccess) break;
       res->sweeps++;
       if ((rc = read_norms(c, nm))) { ok = false; break; }
       if (std::sqrt(nm.res2) <= o->tol * std::sqrt(nm.e2)) break;
       // correction d = r + F d by BF16 sweeps from d = r, then x += d
-      if ((er = hipMemcpyAsync(Dp, R, m * 4, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) break;
+      if ((er = hipMemcpyAsync(Da, R, m * 4, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) break;
       for (uint32_t k = 0; k < inner && er == hipSuccess; k++) {
-        er = launch_x_to_bf16T(Dp, n, rp, (uint16_t *)xT.p, c->stream);
-        if (er == hipSuccess) er = launch_sweep_bf16_T((const uint16_t *)c->dF16.p, (const uint16_t *)xT.p, n, rp, R, Dp, c->stream);
+        er = launch_x_to_bf16T(Da, n, rp, (uint16_t *)xT.p, c->stream);
+        if (er == hipSuccess) er = launch_sweep_bf16_T((const uint16_t *)c->dF16.p, (const uint16_t *)xT.p, n, rp, R, Db, c->stream);
         res->sweeps++;
+        std::swap(Da, Db);
+        if (adaptive && er == hipSuccess && k >= 1) {
+          // |d_new - d_old| against |r|: below 2^-9 (BF16's unit roundoff) the sweeps cannot improve d
+          if ((er = launch_resid_f32(Da, Db, R, nullptr, m, part, c->stream)) != hipSuccess) break;
+          Norms dn;
+          if ((rc = read_norms(c, dn))) { ok = false; break; }
+          if (std::sqrt(dn.res2) <= 0.002 * std::sqrt(dn.e2)) break;
+        }
       }
-      if (er != hipSuccess) break;
-      if ((er = launch_add(X, Dp, m, c->stream)) != hipSuccess) break;
+      if (!ok || er != hipSuccess) break;
+      if ((er = launch_add(X, Da, m, c->stream)) != hipSuccess) break;
       res->refinements++;
     }
     release(xT);
-    release(D);
+    release(D0);
+    release(D1);
     if (er != hipSuccess) return hip_fail(c, er, "bf16 refinement");
     if (!ok) return rc;
   } else if (o->method == VT_DENSE_FP64) {

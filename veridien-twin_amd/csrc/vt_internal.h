@@ -134,8 +134,9 @@ hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s);
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s);
 
 // dense node radiosity: x, y, e are n x rp (row-major, RHS fastest); K split four ways, sums added into y
-hipError_t launch_sweep_fp32(const float *F32, uint32_t n, uint32_t rp, const float *x, const float *e, float *y,
-                             hipStream_t s);
+// FP32 on MFMA (v_mfma_f32_16x16x4_f32): xT is scratch of rp * n floats; n a multiple of 16, rp <= 48
+hipError_t launch_sweep_fp32_mfma(const float *F32, uint32_t n, uint32_t rp, const float *x, float *xT,
+                                  const float *e, float *y, hipStream_t s);
 hipError_t launch_sweep_fp64(const float *F32, uint32_t n, uint32_t rp, const double *x, const float *e, double *y,
                              hipStream_t s);
 // BF16 F on MFMA with FP32 accumulation; xT is x rounded to BF16 and transposed (rp x n); n % 128 == 0

@@ -89,6 +89,7 @@ hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s) {
 // y = e + F x over a 128-row x 48-RHS block; K is split KSPLIT ways and the partial sums are added to y
 // (which the host first fills with e).
 constexpr int DM = 128, DK = 32, DR = 48, KSPLIT = 4;
+constexpr int SWEEP_KSPLIT = 8;   // K split of the MFMA sweeps
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_vec(const float *F, uint32_t n, uint32_t rp, const T *x, T *y) {
@@ -146,13 +147,6 @@ __global__ void k_fill_e(const float *e, T *y, uint64_t n) {
   if (i < n) y[i] = (T)e[i];
 }
 
-hipError_t launch_sweep_fp32(const float *F32, uint32_t n, uint32_t rp, const float *x, const float *e, float *y,
-                             hipStream_t s) {
-  const uint64_t m = (uint64_t)n * rp;
-  hipLaunchKernelGGL(k_fill_e<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
-  hipLaunchKernelGGL(k_sweep_vec<float>, dim3((n + DM - 1) / DM, KSPLIT), dim3(256), 0, s, F32, n, rp, x, y);
-  return hipGetLastError();
-}
 
 // ------------------------------------------------------------------ BF16 MFMA sweep
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -184,13 +178,14 @@ __global__ void k_x_to_bf16T(const float *x, uint32_t n, uint32_t rp, uint16_t *
 }
 
 // One wave: 64 rows (four 16-row MFMA tiles) x 48 RHS (three 16-column tiles); a block of 4 waves covers 256
-// rows. K advances 32 per step: every lane loads 16 B of each A row tile and 16 B of each B column tile.
+// rows. K advances 32 per step: every lane loads 16 B of each A row tile and 16 B of each B column tile; K is
+// split SWEEP_KSPLIT ways (more waves in flight for the HBM stream of F).
 constexpr int BM_WAVE = 64, BWAVES = 4;
 __global__ __launch_bounds__(256) void k_sweep_bf16(const uint16_t *F16, const uint16_t *xT, uint32_t n,
                                                     uint32_t rp, float *y) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t row0 = blockIdx.x * (BM_WAVE * BWAVES) + wv * BM_WAVE;
-  const uint32_t kper = (n + KSPLIT - 1) / KSPLIT;
+  const uint32_t kper = (n / SWEEP_KSPLIT + 31) / 32 * 32;
   const uint32_t kbeg = blockIdx.y * kper, kend = min(n, kbeg + kper);
   const int li = lane & 15, lk = (lane >> 4) * 8;
   f32x4 acc[4][3];
@@ -198,26 +193,41 @@ __global__ __launch_bounds__(256) void k_sweep_bf16(const uint16_t *F16, const u
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 3; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  for (uint32_t k0 = kbeg; k0 < kend; k0 += 32) {
-    bf16x8 A[4], B[3];
+  auto load = [&](uint32_t k0, uint4 (&A)[4], uint4 (&B)[3]) {
 #pragma unroll
     for (int a = 0; a < 4; a++) {
       const uint32_t r = row0 + 16 * a + li;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (r < n && k0 + lk + 8 <= kend) v = *reinterpret_cast<const uint4 *>(F16 + (size_t)r * n + k0 + lk);
-      A[a] = __builtin_bit_cast(bf16x8, v);
+      A[a] = (r < n && k0 + lk + 8 <= kend) ? *reinterpret_cast<const uint4 *>(F16 + (size_t)r * n + k0 + lk)
+                                            : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
     for (int b = 0; b < 3; b++) {
       const uint32_t c = 16 * b + li;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (c < rp && k0 + lk + 8 <= kend) v = *reinterpret_cast<const uint4 *>(xT + (size_t)c * n + k0 + lk);
-      B[b] = __builtin_bit_cast(bf16x8, v);
+      B[b] = (c < rp && k0 + lk + 8 <= kend) ? *reinterpret_cast<const uint4 *>(xT + (size_t)c * n + k0 + lk)
+                                             : make_uint4(0, 0, 0, 0);
     }
+  };
+  auto step = [&](const uint4 (&A)[4], const uint4 (&B)[3]) {
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int b = 0; b < 3; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[b], acc[a][b], 0, 0, 0);
+      for (int b = 0; b < 3; b++)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, A[a]),
+                                                             __builtin_bit_cast(bf16x8, B[b]), acc[a][b], 0, 0, 0);
+  };
+  // the next step's operands are loaded before this step's MFMAs (two register sets)
+  uint4 A0[4], B0[3], A1[4], B1[3];
+  uint32_t k0 = kbeg;
+  if (k0 < kend) load(k0, A0, B0);
+  while (k0 < kend) {
+    const uint32_t k1 = k0 + 32;
+    if (k1 < kend) load(k1, A1, B1);
+    step(A0, B0);
+    if (k1 >= kend) break;
+    const uint32_t k2 = k1 + 32;
+    if (k2 < kend) load(k2, A0, B0);
+    step(A1, B1);
+    k0 = k2;
   }
   // D layout: lane holds column (lane & 15), rows 4 * (lane >> 4) + v
 #pragma unroll
@@ -234,6 +244,97 @@ __global__ __launch_bounds__(256) void k_sweep_bf16(const uint16_t *F16, const u
     }
 }
 
+// ------------------------------------------------------------------ FP32 sweep on MFMA
+// y = e + F x with F and x in binary32 on v_mfma_f32_16x16x4_f32 (FP32 inputs, FP32 accumulation). One wave:
+// 64 rows (four 16-row tiles) x 48 RHS (three 16-column tiles). A K step of 16: lane (i = lane & 15,
+// q = lane >> 4) loads 4 consecutive F entries F[row i][k0 + 4q .. 4q + 3] of each row tile and the matching
+// 4 entries of xT (x transposed, RHS-major) of each column tile; MFMA m pairs the m-th of them, so the 16 k of
+// the step are summed in the order (q, m) -> k0 + 4q + m, the same pairing on both operands. The next step's
+// operands are loaded before this step's MFMAs (two register sets), and K is split SWEEP_KSPLIT ways.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_sweep_f32_mfma(const float *F, const float *xT, uint32_t n, uint32_t rp,
+                                                        float *y) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t row0 = blockIdx.x * 256 + wv * 64;
+  const uint32_t kper = (n / SWEEP_KSPLIT + 15) / 16 * 16;
+  const uint32_t kbeg = blockIdx.y * kper, kend = min(n, kbeg + kper);
+  const int li = lane & 15, lq = (lane >> 4) * 4;
+  f32x4v acc[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) acc[a][b] = (f32x4v){0.f, 0.f, 0.f, 0.f};
+  auto load = [&](uint32_t k0, float4 (&A)[4], float4 (&B)[3]) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const uint32_t r = row0 + 16 * a + li;
+      A[a] = (r < n && k0 + lq + 4 <= kend) ? *reinterpret_cast<const float4 *>(F + (size_t)r * n + k0 + lq)
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const uint32_t c = 16 * b + li;
+      B[b] = (c < rp && k0 + lq + 4 <= kend) ? *reinterpret_cast<const float4 *>(xT + (size_t)c * n + k0 + lq)
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  float4 A0[4], B0[3], A1[4], B1[3];
+  auto step = [&](const float4 (&A)[4], const float4 (&B)[3]) {
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a].x, B[b].x, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a].y, B[b].y, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a].z, B[b].z, acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a].w, B[b].w, acc[a][b], 0, 0, 0);
+      }
+  };
+  uint32_t k0 = kbeg;
+  if (k0 < kend) load(k0, A0, B0);
+  while (k0 < kend) {
+    const uint32_t k1 = k0 + 16;
+    if (k1 < kend) load(k1, A1, B1);
+    step(A0, B0);
+    if (k1 >= kend) break;
+    const uint32_t k2 = k1 + 16;
+    if (k2 < kend) load(k2, A0, B0);
+    step(A1, B1);
+    k0 = k2;
+  }
+  // D layout: lane holds column (lane & 15), rows 4 * (lane >> 4) + v
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const uint32_t c = 16 * b + li;
+      if (c >= rp) continue;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t r = row0 + 16 * a + 4 * (lane >> 4) + v;
+        if (r < n) atomicAdd(&y[(size_t)r * rp + c], acc[a][b][v]);
+      }
+    }
+}
+
+// x (n x rp) -> xT (rp x n), binary32
+__global__ void k_x_to_f32T(const float *x, uint32_t n, uint32_t rp, float *xT) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)n * rp) return;
+  const uint32_t k = (uint32_t)(i / rp), c = (uint32_t)(i % rp);
+  xT[(size_t)c * n + k] = x[i];
+}
+
+hipError_t launch_sweep_fp32_mfma(const float *F32, uint32_t n, uint32_t rp, const float *x, float *xT,
+                                  const float *e, float *y, hipStream_t s) {
+  if (n % 16 || rp > 48) return hipErrorInvalidValue;
+  const uint64_t m = (uint64_t)n * rp;
+  hipLaunchKernelGGL(k_x_to_f32T, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, n, rp, xT);
+  hipLaunchKernelGGL(k_fill_e<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
+  hipLaunchKernelGGL(k_sweep_f32_mfma, dim3((n + 255) / 256, SWEEP_KSPLIT), dim3(256), 0, s, F32, xT, n, rp, y);
+  return hipGetLastError();
+}
+
 hipError_t launch_convert_bf16(const float *src, uint16_t *dst, uint64_t n, hipStream_t s) {
   uint64_t th = (n + 3) / 4;
   hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, s, src, dst, n);
@@ -244,8 +345,8 @@ hipError_t launch_sweep_bf16_T(const uint16_t *F16, const uint16_t *xT, uint32_t
                                float *y, hipStream_t s) {
   const uint64_t m = (uint64_t)n * rp;
   hipLaunchKernelGGL(k_fill_e<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, e, y, m);
-  hipLaunchKernelGGL(k_sweep_bf16, dim3((n + BM_WAVE * BWAVES - 1) / (BM_WAVE * BWAVES), KSPLIT), dim3(256), 0, s,
-                     F16, xT, n, rp, y);
+  hipLaunchKernelGGL(k_sweep_bf16, dim3((n + BM_WAVE * BWAVES - 1) / (BM_WAVE * BWAVES), SWEEP_KSPLIT), dim3(256), 0,
+                     s, F16, xT, n, rp, y);
   return hipGetLastError();
 }
 hipError_t launch_x_to_bf16T(const float *x, uint32_t n, uint32_t rp, uint16_t *xT, hipStream_t s) {

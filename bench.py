@@ -83,6 +83,8 @@ def bench_c3(args):
     for k, c in enumerate(cams):
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
+    tuned = ctx.autotune(slots)        # untimed: launch shapes of both forms for this frame
+    log("autotune: %s" % tuned)
     res = {}
     for name, mode, graph in (("per_camera", lib.VT_RENDER_PER_CAMERA, False), ("batched", lib.VT_RENDER_BATCHED, True),
                               ("wavefront", lib.VT_RENDER_WAVEFRONT, False)):
@@ -113,6 +115,7 @@ def bench_c3(args):
            "speedup_wavefront_over_per_camera": round(res["per_camera"]["ms_per_frame"] / res["wavefront"]["ms_per_frame"], 3),
            "spp_mpix_per_s": round(spp_px / (res["wavefront"]["ms_per_frame"] * 1e-3) / 1e6, 2),
            "images_identical": bool(np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])),
+           "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
            "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -246,25 +249,6 @@ def bench_copy(args):
                       "check": float(y[-1].item())}), flush=True)
 
 
-def mall_bandwidth(mib=64, iters=50):
-    """Copy bandwidth with a working set that stays in the 256 MB Infinity Cache (MALL): y = x over two mib-MiB
-    buffers, iters times, bytes read + written per second (HIP events on torch's stream)."""
-    import torch
-
-    n = (mib << 20) // 4
-    x = torch.ones(n, device="cuda", dtype=torch.float32)
-    y = torch.empty_like(x)
-    for _ in range(5):
-        y.copy_(x)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(iters):
-        y.copy_(x)
-    e1.record()
-    torch.cuda.synchronize()
-    return 2.0 * 4 * n * iters / (e0.elapsed_time(e1) * 1e-3) / 1e9
-
-
 def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
@@ -282,8 +266,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["wavefront", "batched", "per-camera"], default="wavefront",
-                    help="wavefront: per-vertex kernels with sorted rays (default); batched: one persistent launch")
+    ap.add_argument("--mode", choices=["auto", "wavefront", "batched", "per-camera"], default="auto",
+                    help="auto (default): the autotuner times both forms on a probe of the frame (untimed) and the "
+                         "faster runs; wavefront: per-vertex kernels with sorted rays; batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--block", type=int, default=0)
@@ -319,8 +304,13 @@ def main():
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
     sensor = [scenes.default_sensor(cfa=0, frame=0, seed=1006) for _ in cams]
-    mode = {"wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
+    mode = {"auto": lib.VT_RENDER_AUTO, "wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
             "per-camera": lib.VT_RENDER_PER_CAMERA}[args.mode]
+    tuned = None
+    if args.mode == "auto":
+        t0 = time.time()
+        tuned = ctx.autotune(slots)
+        log("autotune (%.1f s): %s" % (time.time() - t0, tuned))
 
     def frame():
         ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk, stack_lds=args.stack, batch=args.batch)
@@ -357,12 +347,14 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded mulberry32 scene)",
         "config": {"workload": "full-scale forward: 2,013,432-tri seeded dome scene, 256 materials, 16 lights; "
                                "3 KB4 cameras 1920x1080, 64 spp, depth 5, NEE; RGGB sensor, shot + 2 DN read noise, 12 bit",
-                   "cameras": len(cams), "spp": 64, "depth": 5, "mode": args.mode},
+                   "cameras": len(cams), "spp": 64, "depth": 5, "mode": args.mode,
+                   "form": "wavefront" if ext_launches else ("per-camera" if args.mode == "per-camera" else "batched")},
         "spp_mpix_per_s": round(spp_px / dt / 1e6, 2),
         "rays_per_frame": total_rays / args.steps,
         "trace_ms_per_launch": round(float(np.mean(ms_kernel)), 3),
         "trace_launches_per_frame": launches,
         "extension_trace_launches_per_frame": ext_launches,
+        "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()} if tuned else None,
     }
     cpu = None
     if not args.no_cpu:
@@ -383,7 +375,7 @@ def main():
             ms_launch = float(np.mean(ms_kernel))
             kernel, prof = "k_render<256>", "k_render_c2_full_scale_forward.md"
         achieved = b_ray * rays_launch / (ms_launch * 1e-3) / 1e9
-        mall = mall_bandwidth()
+        mall = ctx.copy_bandwidth(64 << 20)   # 2 x 64 MiB: resident in the 256 MB Infinity Cache
         traffic = traffic_from_profile(prof)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -81,12 +81,13 @@ typedef struct {
   float scale;           /* output scale (CamU.expo.z), usually 1 */
 } vt_camera;
 
-enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2 };
+enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2, VT_RENDER_AUTO = 3 };
 typedef struct {
   uint32_t mode;         /* VT_RENDER_BATCHED: every listed camera in one persistent launch over a global ray
                             queue; VT_RENDER_PER_CAMERA: one plain launch per camera; VT_RENDER_WAVEFRONT: the
                             frame's paths in batches advanced one vertex at a time (trace, shade, shadow, gather
-                            kernels), live paths radix-sorted by ray direction octant and origin between vertices */
+                            kernels), live paths radix-sorted by ray direction and origin between vertices;
+                            VT_RENDER_AUTO: the form vt_autotune found fastest (batched until it has run) */
   uint32_t spp_begin;    /* sub-sample range [spp_begin, spp_end); spp_end 0 = each camera's spp */
   uint32_t spp_end;
   uint32_t accumulate;   /* 0: clear the cameras' sums first; 1: add to them (progressive ranges) */
@@ -102,13 +103,16 @@ typedef struct {
   uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 16,777,216) */
 } vt_render_opts;
 
-/* Result of vt_autotune: the chosen launch shape (also made the context's default) and the probe timings. */
+/* Result of vt_autotune: the chosen launch shapes (also made the context's defaults) and the probe timings. */
 typedef struct {
-  uint32_t block, stack_lds, chunk, queues;
-  uint32_t candidates;   /* launch shapes timed */
-  double probe_mrays_best;    /* Mrays/s of the chosen shape on the probe */
+  uint32_t block, stack_lds, chunk, queues;   /* persistent batched form */
+  uint32_t candidates;   /* launch shapes timed (both forms) */
+  double probe_mrays_best;    /* Mrays/s of the chosen persistent shape on the probe */
   double probe_mrays_default; /* Mrays/s of the built-in default (256 threads, 16 LDS entries, chunk 4, 1 queue) */
   double probe_fraction;      /* share of the frame's waves one probe traces */
+  uint32_t mode;              /* the faster form: VT_RENDER_BATCHED or VT_RENDER_WAVEFRONT (used by VT_RENDER_AUTO) */
+  uint32_t wf_batch, wf_stack_lds, wf_refill;  /* wavefront form: paths per batch, LDS stack, refill threshold */
+  double probe_mrays_wavefront;               /* Mrays/s of the chosen wavefront shape on the same probe */
 } vt_tune_result;
 
 /* Sensor parameters, the 256-B sensor uniform (gpu/sensor.js:158-177), raw-DN part. */
@@ -173,11 +177,12 @@ int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
 /* ---------------------------------------------------------------- per frame */
 /* renderCamera / renderFrame (gpu/renderer.js:974-996): enqueue the listed cameras' path tracing. */
 int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts);
-/* Autotuner (no reference counterpart; the reference fixes its workgroup shape): times the batched launch of
- * the listed cameras over block {64, 128, 256} x LDS stack {8, 16, 24} x chunk {1, 4, 16} x queues {1, 8} on a
- * probe of every `probe_stride`-th wave of the frame (0 = 32), picks the fastest in rays per second and makes it
- * the context's default for fields of vt_render_opts left 0. opts: the sample range (as for vt_render). The
- * listed cameras' accumulated images are cleared. Blocks until done. */
+/* Autotuner (no reference counterpart; the reference fixes its workgroup shape): times the persistent batched
+ * launch of the listed cameras over block {64, 128, 256} x LDS stack {8, 16, 24} x chunk {1, 4, 16} x queues
+ * {1, 8}, and the wavefront form over batch {4M, 16M} x LDS stack {4, 8, 16} x refill {8, 16, 32}, each on a probe
+ * of every `probe_stride`-th wave of the frame (0 = 32); picks the fastest of each form in rays per second, makes
+ * them the context's defaults for fields of vt_render_opts left 0, and the faster form the one VT_RENDER_AUTO
+ * runs. opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
 int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts, uint32_t probe_stride,
                 vt_tune_result *out);
 /* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */
@@ -214,6 +219,11 @@ typedef struct {
 } vt_grads;
 int vt_backward(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const float *const *adj, const vt_render_opts *opts,
                 vt_grads *out);
+
+/* Copy bandwidth: y = x over two `bytes`-sized device buffers, `iters` times, read + written bytes per second
+ * (GB/s). With bytes well under 256 MB the working set stays in the Infinity Cache; the roofline of the trace
+ * kernels is reported against it as well as against HBM. No reference counterpart. */
+int vt_copy_bandwidth(vt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbs);
 
 /* ---------------------------------------------------------------- test kernels (the reference's TESTS entry
  * points, gpu/wgsl/kernels.wgsl.js:487-551, that tests/gpu_test.html compared against the JS) */

@@ -62,6 +62,10 @@ struct vt_ctx {
   uint32_t n_mat = 0, n_light = 0;
   // launch shape used for vt_render_opts fields left 0 (vt_autotune replaces it)
   uint32_t def_block = DEFAULT_BLOCK, def_stack = STACK_LDS, def_chunk = 4, def_queues = 1;
+  // the tuned frame form (VT_RENDER_AUTO) and the wavefront's batch, LDS stack and refill threshold
+  uint32_t def_mode = VT_RENDER_BATCHED;
+  uint64_t def_wf_batch = 1ull << 24;
+  uint32_t def_wf_stack = 8, def_wf_refill = 16;
   int occ[3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (block 64/128/256, LDS depth)
   // scene box (for the wavefront sort key)
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
@@ -608,9 +612,22 @@ int build_graph(vt_ctx *c, uint32_t n, bool accumulate, const RenderLaunch &L) {
   return VT_OK;
 }
 
+struct WfParams {
+  uint64_t batch;     // paths per batch
+  uint32_t slds;      // LDS stack entries of the trace kernels (4, 8, 16)
+  uint32_t refill;    // idle lanes that make a wave fetch rays
+  uint64_t stride;    // 1, or > 1: trace every stride-th wave of the frame (tuning probes)
+};
+
+WfParams wf_params(const vt_ctx *c, const vt_render_opts &opt) {
+  return WfParams{opt.batch ? opt.batch : c->def_wf_batch, opt.stack_lds ? opt.stack_lds : c->def_wf_stack,
+                  opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1};
+}
+
 // the wavefront frame: buffers for one batch, the batches in order, one event pair around all of them
-int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t items) {
-  uint64_t P = opt.batch ? opt.batch : (1ull << 24);
+int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfParams &wp, uint64_t frame_items) {
+  const uint64_t items = (frame_items / 64 + wp.stride - 1) / wp.stride * 64;
+  uint64_t P = wp.batch;
   P = std::min<uint64_t>((P + 63) / 64 * 64, std::max<uint64_t>(items, 64));
   if (P > (1ull << 31)) return fail(c, VT_ERR_ARG, "render: wavefront batch of %llu paths", (unsigned long long)P);
   uint32_t nl = 1, depth = 1;
@@ -671,13 +688,14 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, uint64_t 
       VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
     }
   VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
-  const uint32_t slds = opt.stack_lds ? opt.stack_lds : 8;
+  const uint32_t slds = wp.slds;
   if (slds != 4 && slds != 8 && slds != 16) return fail(c, VT_ERR_ARG, "render: wavefront stack_lds must be 4, 8 or 16");
+  W.wave_stride = wp.stride;
   int occ = 0;
   VT_HIP(c, wf_trace_occupancy(slds, &occ));
   const uint32_t grid_cap = (uint32_t)c->cu_count * 8;
   const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
-  const uint32_t refill = opt.chunk ? std::min(64u, opt.chunk) : 16u;
+  const uint32_t refill = wp.refill;
   const uint64_t batches = (items + P - 1) / P;
   while (c->ev_ext.size() < 2 * batches * depth) {
     hipEvent_t ev;
@@ -703,11 +721,13 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   if (!c->has_scene) return fail(c, VT_ERR_STATE, "render: no scene uploaded");
   vt_render_opts opt{};
   if (o) opt = *o;
+  if (opt.mode == VT_RENDER_AUTO) opt.mode = c->def_mode;
   const Shape sh = shape_of(c, opt);
   int rc = opt.mode == VT_RENDER_WAVEFRONT ? VT_OK : check_shape(c, sh);   // the wavefront checks its own
   if (rc) return rc;
+  if (opt.graph && opt.mode == VT_RENDER_WAVEFRONT) opt.graph = 0;   // the wavefront form is not captured
   if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
-  if (opt.mode > VT_RENDER_WAVEFRONT) return fail(c, VT_ERR_ARG, "render: mode %u", opt.mode);
+  if (opt.mode > VT_RENDER_AUTO) return fail(c, VT_ERR_ARG, "render: mode %u", opt.mode);
   uint64_t items = 0;
   if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
   RenderLaunch L = base_launch(c, sh);
@@ -716,7 +736,7 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   c->ext_launches = 0;
   if (opt.mode == VT_RENDER_WAVEFRONT) {
     if ((rc = ensure_launch_events(c, 1))) return rc;
-    if ((rc = render_wavefront(c, n, opt, items))) return rc;
+    if ((rc = render_wavefront(c, n, opt, wf_params(c, opt), items))) return rc;
     launches = 1;
   } else if (opt.mode == VT_RENDER_BATCHED) {
     if ((rc = ensure_launch_events(c, 1))) return rc;
@@ -868,6 +888,50 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   c->def_stack = best.stack;
   c->def_chunk = best.chunk;
   c->def_queues = best.queues;
+  // the wavefront form on the same probe: batch {4M, 16M} paths x LDS stack {4, 8, 16} x refill {8, 16, 32}
+  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride};
+  double m_wf = 0;
+  auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
+    vt_render_opts po = opt;
+    po.mode = VT_RENDER_WAVEFRONT;
+    po.accumulate = 0;
+    int r = render_wavefront(c, n, po, wp, items);
+    if (r) return r;
+    VT_HIP(c, hipEventSynchronize(c->ev_launch[1]));
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_launch[0], c->ev_launch[1]));
+    unsigned long long cnt[2];
+    VT_HIP(c, hipMemcpy(cnt, c->counters.p, sizeof cnt, hipMemcpyDeviceToHost));
+    mrays = ms > 0 ? (double)cnt[1] / (ms * 1e-3) / 1e6 : 0.0;
+    return VT_OK;
+  };
+  {
+    double warm = 0;
+    if ((rc = probe_wf(wbest, warm))) return rc;
+  }
+  const uint64_t batches[] = {1ull << 22, 1ull << 24};
+  const uint32_t wstacks[] = {4, 8, 16}, refills[] = {8, 16, 32};
+  for (uint64_t bsz : batches)
+    for (uint32_t st : wstacks)
+      for (uint32_t rf : refills) {
+        const WfParams wp{bsz, st, rf, stride};
+        double m = 0;
+        if ((rc = probe_wf(wp, m))) return rc;
+        tried++;
+        if (m > m_wf) {
+          m_wf = m;
+          wbest = wp;
+        }
+      }
+  {
+    double m2w = 0;
+    if ((rc = probe_wf(wbest, m2w))) return rc;
+    m_wf = m2w;
+  }
+  c->def_wf_batch = wbest.batch;
+  c->def_wf_stack = wbest.slds;
+  c->def_wf_refill = wbest.refill;
+  c->def_mode = m_wf > m_best ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
   drop_graph(c);
   // the probes left partial sums behind: clear the listed cameras
   for (uint32_t a = 0; a < n; a++) {
@@ -884,6 +948,11 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   out->candidates = tried;
   out->probe_mrays_best = m_best;
   out->probe_mrays_default = m_def;
+  out->mode = c->def_mode;
+  out->wf_batch = (uint32_t)wbest.batch;
+  out->wf_stack_lds = wbest.slds;
+  out->wf_refill = wbest.refill;
+  out->probe_mrays_wavefront = m_wf;
   out->probe_fraction = (double)probe_waves / (double)std::max<uint64_t>(1, frame_waves);
   return VT_OK;
 }
@@ -1113,6 +1182,34 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
   std::memcpy(out->materials, g.data(), 12 * (size_t)c->n_mat);
   if (c->n_light) std::memcpy(out->lights, g.data() + 3 * (size_t)c->n_mat, 12 * (size_t)c->n_light);
   if (ngc) std::memcpy(out->cameras, g.data() + ng_pad, 4 * ngc);
+  return VT_OK;
+}
+
+// ====================================================================== bandwidth probe
+extern "C" int vt_copy_bandwidth(vt_ctx *c, uint64_t bytes, uint32_t iters, double *gbs) {
+  VT_LIVE(c);
+  if (!gbs || bytes < 4096 || iters == 0) return fail(c, VT_ERR_ARG, "copy_bandwidth: bytes >= 4096, iters > 0");
+  VT_HIP(c, hipSetDevice(c->device));
+  DevBuf a, b;
+  int rc = ensure(c, a, bytes);
+  if (!rc) rc = ensure(c, b, bytes);
+  struct Free {
+    DevBuf &x, &y;
+    ~Free() { release(x); release(y); }
+  } fr{a, b};
+  if (rc) return rc;
+  const uint64_t n = bytes / 16;
+  const uint32_t grid = (uint32_t)c->cu_count * 8;
+  VT_HIP(c, launch_zero_f4((float4 *)a.p, n, c->stream));
+  for (int w = 0; w < 3; w++) VT_HIP(c, launch_copy_f4((const float4 *)a.p, (float4 *)b.p, n, grid, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
+  for (uint32_t k = 0; k < iters; k++) VT_HIP(c, launch_copy_f4((const float4 *)a.p, (float4 *)b.p, n, grid, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_s1, c->stream));
+  VT_HIP(c, hipEventSynchronize(c->ev_s1));
+  float ms = 0;
+  VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
+  *gbs = 2.0 * 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
+  c->sensor_timed = false;
   return VT_OK;
 }
 

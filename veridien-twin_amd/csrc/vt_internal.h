@@ -58,9 +58,10 @@ struct WfLaunch {
   SceneDev S;
   const DevCam *cams;
   uint32_t ncam;
-  uint64_t batch_begin;
+  uint64_t batch_begin;        // in items of the (strided) queue
   uint32_t n;
   uint32_t bounce;
+  uint64_t wave_stride;        // queue wave w is frame wave w * wave_stride (1; > 1 for tuning probes)
   uint32_t nl_max;             // shadow slots per queue slot (the largest light_samples of the frame)
   // path state, by path id (0..n)
   uint32_t *cam;               // camera of the path (index in cams)
@@ -91,6 +92,7 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
                                   uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
                                   void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
+hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid, hipStream_t s);
 // shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals
 hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out,
                               hipStream_t s);

@@ -670,6 +670,16 @@ hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4
   return hipGetLastError();
 }
 
+// y = x over n float4 (a plain copy: the bandwidth probe)
+__global__ void k_copy_f4(const float4 *__restrict__ x, float4 *__restrict__ y, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_f4, dim3(grid), dim3(256), 0, s, x, y, n);
+  return hipGetLastError();
+}
+
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;

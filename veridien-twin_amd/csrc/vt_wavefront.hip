@@ -30,6 +30,12 @@ __device__ __forceinline__ ItemPix item_pix(const DevCam &C, uint64_t item) {
   return r;
 }
 
+// frame item of path p: batch items are 64-item waves of the frame queue, every wave_stride-th one (1: all)
+__device__ __forceinline__ uint64_t path_item(const WfLaunch &W, uint32_t p) {
+  const uint64_t q = W.batch_begin + p;
+  return (q >> 6) * W.wave_stride * 64ull + (q & 63ull);
+}
+
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {   // 9 bits -> every third bit of 27
   v &= 0x1ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -72,7 +78,7 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
     W.count[3] = 0;
   }
   for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < W.n; p += gridDim.x * 256) {
-    const uint64_t item = W.batch_begin + p;
+    const uint64_t item = path_item(W, p);
     uint32_t c = 0;
     while (c + 1 < W.ncam && item >= W.cams[c + 1].item_begin) c++;
     const DevCam &C = W.cams[c];
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
       const float4 h4 = W.hit[k];
       const uint32_t tri = __float_as_uint(h4.w);
       if (tri != MISS && b < Dp) {
-        const ItemPix ip = item_pix(C, W.batch_begin + p);
+        const ItemPix ip = item_pix(C, path_item(W, p));
         PixSeeds P;
         pix_seeds(C, ip.x, ip.y, P);
         const uint32_t i = ip.i, hs = P.hs;
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(256) void k_wf_accum(WfLaunch W) {
   uint32_t nsamp = 0;
   for (uint32_t w = (blockIdx.x * 256 + threadIdx.x) / 64; w < waves; w += gridDim.x * 4) {
     const uint32_t p = w * 64 + lane;
-    const uint64_t item = W.batch_begin + p;
+    const uint64_t item = path_item(W, p);
     const DevCam &C = W.cams[W.cam[p]];
     const ItemPix ip = item_pix(C, item);
     const float4 L4 = W.L[p];

@@ -159,9 +159,10 @@ export class TwinRenderer {
     const names = opts.cameras || Object.keys(this.cameras);
     const cams = names.map((n) => this._slot(n));
     cams.forEach((c) => this._pack(c, opts));
+    // batched frames run in the form autotune() found fastest (the persistent launch, as a hipGraph, until then)
     const batched = dflt(opts.batched, this.opts.batched);
     native.render(this.ctx, cams.map((c) => c.index),
-      { mode: batched ? 0 : 1, graph: batched && dflt(opts.graph, this.opts.graph) ? 1 : 0 });
+      { mode: batched ? 3 : 1, graph: batched && dflt(opts.graph, this.opts.graph) ? 1 : 0 });
     const out = {};
     for (const c of cams) out[c.name] = { linear: { camera: c.name, width: c.size[0], height: c.size[1] } };
     if (opts.sensor) for (const c of cams) out[c.name].raw = this.sensor(c.name, opts.sensorOpts || {});

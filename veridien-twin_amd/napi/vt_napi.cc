@@ -352,11 +352,13 @@ void job_done(napi_env env, napi_status, void *data) {
   } else if (j->kind == -3) {
     napi_create_object(env, &res);
     const char *names[] = {"block", "stackLds", "chunk", "queues", "candidates", "probeMraysBest",
-                           "probeMraysDefault", "probeFraction"};
+                           "probeMraysDefault", "probeFraction", "mode", "wfBatch", "wfStackLds", "wfRefill",
+                           "probeMraysWavefront"};
     double vals[] = {(double)j->tune.block, (double)j->tune.stack_lds, (double)j->tune.chunk, (double)j->tune.queues,
                      (double)j->tune.candidates, j->tune.probe_mrays_best, j->tune.probe_mrays_default,
-                     j->tune.probe_fraction};
-    for (int k = 0; k < 8; k++) {
+                     j->tune.probe_fraction, (double)j->tune.mode, (double)j->tune.wf_batch,
+                     (double)j->tune.wf_stack_lds, (double)j->tune.wf_refill, j->tune.probe_mrays_wavefront};
+    for (int k = 0; k < 13; k++) {
       napi_value v;
       napi_create_double(env, vals[k], &v);
       napi_set_named_property(env, res, names[k], v);

@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "libvt.so")
 
 VT_OK, VT_ERR_ARG, VT_ERR_SIZE, VT_ERR_HIP, VT_ERR_LOST, VT_ERR_STATE, VT_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
-VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA, VT_RENDER_WAVEFRONT = 0, 1, 2
+VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA, VT_RENDER_WAVEFRONT, VT_RENDER_AUTO = 0, 1, 2, 3
 VT_KIND_LINEAR, VT_KIND_RAW, VT_KIND_RI = 0, 1, 2
 VT_DENSE_FP32, VT_DENSE_BF16_REFINE, VT_DENSE_FP64 = 0, 1, 2
 
@@ -17,7 +17,7 @@ EXPORTS = [
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
-    "vt_update_lights", "vt_autotune",
+    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
@@ -48,7 +48,8 @@ class RenderOpts(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("block", u32), ("stack_lds", u32), ("chunk", u32), ("queues", u32), ("candidates", u32),
-                ("probe_mrays_best", f64), ("probe_mrays_default", f64), ("probe_fraction", f64)]
+                ("probe_mrays_best", f64), ("probe_mrays_default", f64), ("probe_fraction", f64), ("mode", u32),
+                ("wf_batch", u32), ("wf_stack_lds", u32), ("wf_refill", u32), ("probe_mrays_wavefront", f64)]
 
 
 class SensorParams(C.Structure):
@@ -121,6 +122,7 @@ def load():
         "vt_update_materials": (i32, [vp, vp, u32]),
         "vt_update_lights": (i32, [vp, vp, u32]),
         "vt_autotune": (i32, [vp, vp, u32, vp, u32, vp]),
+        "vt_copy_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -211,6 +213,11 @@ class Context:
         o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block, stack_lds, queues,
                        1 if graph else 0, batch)
         self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
+
+    def copy_bandwidth(self, nbytes, iters=50):
+        g = C.c_double(0)
+        self.check(self.L.vt_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
 
     def autotune(self, slots, spp_begin=0, spp_end=0, probe_stride=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)

@@ -65,7 +65,22 @@ typedef struct {
   const float *lights; uint32_t light_count;
   const float *emit; uint32_t emit_count;
   float emit_area;       /* total emitter area (m^2), SceneU.L0.w (gpu/renderer.js:646) */
+  /* posed bodies (the reference's BLAS, gpu/renderer.js:538-571): tri_body[k] = the body triangle k belongs to
+   * (0xffffffff: the static scene); a body's vertices are in its own frame and vt_set_instances places it. With
+   * bodies, nodes must be NULL (every BVH is built here); every body and the static scene need triangles, and
+   * emitter triangles stay static. tri_body may be NULL when body_count is 0. */
+  const uint32_t *tri_body; uint32_t body_count;
 } vt_scene;
+
+#define VT_MAX_BODIES 24
+/* Pose of body i over the exposure (setPoses, gpu/renderer.js:619-633): rotation as a unit quaternion
+ * (x, y, z, w) and position, at shutter open (q0, p0) and close (q1, p1); a sub-sample at exposure time t sees
+ * slerp(q0, q1, t), lerp(p0, p1, t). hidden = 1 leaves the body out (a body without a pose). */
+typedef struct {
+  float q0[4], q1[4];
+  float p0[3], p1[3];
+  uint32_t hidden;
+} vt_instance;
 
 /* One tracking camera, CamU (gpu/wgsl/common.wgsl.js:79-90; gpu/renderer.js:888-898). */
 typedef struct {
@@ -79,6 +94,8 @@ typedef struct {
   uint32_t max_depth;    /* path vertices, >= 1 */
   uint32_t flags;        /* reserved, 0 */
   float scale;           /* output scale (CamU.expo.z), usually 1 */
+  float shutter[2];      /* exposure interval on the pose timeline (CamU.expo.xy), within [0, 1]: sub-sample i
+                            is at time shutter[0] + (shutter[1] - shutter[0]) frac(s2 + i / golden ratio) */
 } vt_camera;
 
 enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2, VT_RENDER_AUTO = 3 };
@@ -167,6 +184,10 @@ int vt_build_bvh(const float *pos, uint32_t nvert, const uint32_t *idx, uint32_t
 int vt_upload_scene(vt_ctx *ctx, const vt_scene *scene);
 /* CamU packing of _packCam (gpu/renderer.js:888-898) for camera slot `cam` (< VT_MAX_CAMERAS). */
 int vt_set_camera(vt_ctx *ctx, uint32_t cam, const vt_camera *camera);
+/* setPoses (gpu/renderer.js:619-633, _writeSceneU 640-705): one pose record per body of the uploaded scene
+ * (count = body_count); also derives each body's swept world box and bounding capsule for culling. Bodies start
+ * hidden after an upload. */
+int vt_set_instances(vt_ctx *ctx, const vt_instance *instances, uint32_t count);
 
 /* Replace the material table or the light radiances of the uploaded scene in place (same counts as the upload),
  * e.g. between the steps of a calibration loop; the geometry and BVH stay. No reference counterpart: the

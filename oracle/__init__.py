@@ -123,14 +123,19 @@ class Scene(C.Structure):
                 ("verts", C.c_void_p), ("vert_count", C.c_uint32), ("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
                 ("materials", C.c_void_p), ("mat_count", C.c_uint32), ("lights", C.c_void_p),
                 ("light_count", C.c_uint32), ("emit", C.c_void_p), ("emit_count", C.c_uint32),
-                ("emit_area", C.c_float)]
+                ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32)]
+
+
+class Body(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("tri_count", C.c_uint32), ("q0", C.c_float * 4),
+                ("q1", C.c_float * 4), ("p0", C.c_float * 3), ("p1", C.c_float * 3), ("hidden", C.c_uint32)]
 
 
 class Camera(C.Structure):
     _fields_ = [("R", C.c_float * 9), ("pos", C.c_float * 3), ("mode", C.c_uint32), ("lens", C.c_float * 9),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32),
                 ("cam_index", C.c_uint32), ("light_samples", C.c_uint32), ("max_depth", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("shutter", C.c_float * 2)]
 
 
 # ---------------------------------------------------------------------------- helpers
@@ -179,13 +184,49 @@ def brute(tris, count, rays, nthreads=8):
     return hits
 
 
+def _group_bvh(sc, members):
+    """BVH2 (this oracle's builder) over the listed rows of sc["tris"], and those rows in its slot order."""
+    tris = np.asarray(sc["tris"], np.float32).reshape(-1, 16)
+    idx = tris[members].view(np.uint32)[:, [3, 7, 11]]
+    pos = np.asarray(sc["verts"], np.float32).reshape(-1, 8)[:, :3]
+    b = build_bvh(pos, idx)
+    return b["nodes"], np.ascontiguousarray(tris[members[b["order"]]].reshape(-1))
+
+
 def scene_struct(sc):
-    """sc: dict of the upload arrays (nodes, tris, verts, meshes, materials, lights, emit, emit_area)."""
-    keep = {k: np.ascontiguousarray(sc[k]) for k in ("nodes", "tris", "verts", "meshes", "materials", "lights", "emit")}
+    """sc: dict of the upload arrays (nodes, tris, verts, meshes, materials, lights, emit, emit_area). With posed
+    bodies (tri_body, body_count, poses: one dict(q0, p0[, q1, p1]) or None per body) every group's BVH is built
+    here: the static scene over the rows whose tri_body is 0xffffffff, then each body over its own rows."""
+    keep = {k: np.ascontiguousarray(sc[k]) for k in ("tris", "verts", "meshes", "materials", "lights", "emit")}
+    nb = int(sc.get("body_count", 0))
+    bodies = None
+    if nb:
+        tb = np.asarray(sc["tri_body"], np.uint32)
+        keep["nodes"], keep["tris"] = _group_bvh(sc, np.nonzero(tb == 0xFFFFFFFF)[0])
+        bodies = (Body * nb)()
+        poses = sc.get("poses") or [None] * nb
+        for b in range(nb):
+            n, t = _group_bvh(sc, np.nonzero(tb == b)[0])
+            keep["bn%d" % b], keep["bt%d" % b] = n, t
+            B = bodies[b]
+            B.nodes, B.tris, B.tri_count = ptr(n), ptr(t), t.size // 16
+            P = poses[b]
+            if P is None:
+                B.q0[:] = B.q1[:] = [0.0, 0.0, 0.0, 1.0]
+                B.hidden = 1
+            else:
+                B.q0[:] = [float(x) for x in P["q0"]]
+                B.q1[:] = [float(x) for x in P.get("q1", P["q0"])]
+                B.p0[:] = [float(x) for x in P["p0"]]
+                B.p1[:] = [float(x) for x in P.get("p1", P["p0"])]
+        keep["bodies"] = bodies
+    else:
+        keep["nodes"] = np.ascontiguousarray(sc["nodes"])
     s = Scene(ptr(keep["nodes"]), keep["nodes"].size // 16, ptr(keep["tris"]), keep["tris"].size // 16,
               ptr(keep["verts"]), keep["verts"].size // 8, ptr(keep["meshes"]), keep["meshes"].size // 4,
               ptr(keep["materials"]), keep["materials"].size // 16, ptr(keep["lights"]), keep["lights"].size // 4,
-              ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]))
+              ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]),
+              C.cast(bodies, C.c_void_p) if nb else None, nb)
     s._keep = keep
     return s
 
@@ -198,6 +239,7 @@ def camera_struct(cam):
     c.lens[:] = [float(x) for x in cam["lens"]]
     for k in ("width", "height", "spp", "seed", "cam_index", "light_samples", "max_depth", "flags"):
         setattr(c, k, int(cam.get(k, 0)))
+    c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
     return c
 
 

@@ -28,6 +28,7 @@
 static const float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
 static const float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
 static const float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
+static const float GOLD_F = 0.6180339887498949f;   /* sub-sample time dimension (kernels.wgsl.js:44) */
 
 static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 static inline void norm3(double *a) {
@@ -81,8 +82,76 @@ typedef struct {
   uint32_t vi[3], mesh, mat, flags, light;
 } surf_t;
 
-static void surface(const or_scene *S, const or_hit *h, const double o[3], const double d[3], surf_t *s) {
-  const float *r = S->tris + (size_t)h->tri * 16;
+/* ------------------------------------------------------------------ posed bodies
+ * pose of a body at exposure time t: slerp of the quaternions (the shorter arc, normalised lerp when the ends
+ * nearly agree: gpu/renderer.js:85-91 slerpJS, common.wgsl.js:125-133) and lerp of the positions; R holds the
+ * rotation columns (common.wgsl.js:117-123 quatToMat) */
+typedef struct { double R[9], p[3]; } pose_t;
+
+static void body_pose(const or_body *B, double t, pose_t *P) {
+  double a[4], b[4], q[4], c = 0;
+  for (int k = 0; k < 4; k++) { a[k] = B->q0[k]; b[k] = B->q1[k]; }
+  double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+  double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + b[3] * b[3]);
+  for (int k = 0; k < 4; k++) { a[k] /= la; b[k] /= lb; c += a[k] * b[k]; }
+  if (c < 0) { for (int k = 0; k < 4; k++) b[k] = -b[k]; c = -c; }
+  if (c > 0.9995) {
+    double l = 0;
+    for (int k = 0; k < 4; k++) { q[k] = a[k] + (b[k] - a[k]) * t; l += q[k] * q[k]; }
+    l = sqrt(l);
+    for (int k = 0; k < 4; k++) q[k] /= l;
+  } else {
+    double th = acos(c), sn = sin(th);
+    for (int k = 0; k < 4; k++) q[k] = (a[k] * sin((1 - t) * th) + b[k] * sin(t * th)) / sn;
+  }
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+                       2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+                       2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)};
+  memcpy(P->R, R, sizeof R);
+  for (int k = 0; k < 3; k++) P->p[k] = B->p0[k] + ((double)B->p1[k] - B->p0[k]) * t;
+}
+static void rot_apply(const double R[9], const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = R[k] * v[0] + R[3 + k] * v[1] + R[6 + k] * v[2];
+}
+static void rot_apply_t(const double R[9], const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = R[3 * k] * v[0] + R[3 * k + 1] * v[1] + R[3 * k + 2] * v[2];
+}
+
+/* traceSceneSkip (trace.wgsl.js:144-164) without its culling tests (box and capsule only skip work): the static
+ * BVH, then every posed body in its own frame at time t; a rigid pose keeps ray lengths, so t carries over.
+ * h->inst names the body of the hit (-1: static); *P receives its pose. Visits add up over every BVH walked. */
+static void trace_scene(const or_scene *S, const double o[3], const double d[3], double tmin, double tmax, int any,
+                        double time, or_hit *h, pose_t *P) {
+  or_traverse(S->nodes, S->tris, o, d, tmin, tmax, any, 0, h);
+  h->inst = -1;
+  if (any && h->tri >= 0) return;
+  for (uint32_t b = 0; b < S->body_count; b++) {
+    const or_body *B = &S->bodies[b];
+    if (B->hidden) continue;
+    pose_t Q;
+    body_pose(B, time, &Q);
+    double ol[3], dl[3], rel[3] = {o[0] - Q.p[0], o[1] - Q.p[1], o[2] - Q.p[2]};
+    rot_apply_t(Q.R, rel, ol);
+    rot_apply_t(Q.R, d, dl);
+    const double best = h->tri >= 0 ? h->t : tmax;
+    or_hit hb;
+    or_traverse(B->nodes, B->tris, ol, dl, tmin, best, any, 0, &hb);
+    const uint64_t nv = h->node_visits + hb.node_visits, nt = h->tri_tests + hb.tri_tests;
+    if (hb.tri >= 0 && hb.t < best) {
+      *h = hb;
+      h->inst = (int32_t)b;
+      if (P) *P = Q;
+    }
+    h->node_visits = nv;
+    h->tri_tests = nt;
+    if (any && h->tri >= 0) return;
+  }
+}
+
+static void surface(const or_scene *S, const or_hit *h, const pose_t *P, const double o[3], const double d[3],
+                    surf_t *s) {
+  const float *r = (h->inst >= 0 ? S->bodies[h->inst].tris : S->tris) + (size_t)h->tri * 16;
   const uint32_t *ru = (const uint32_t *)r;
   s->vi[0] = ru[3]; s->vi[1] = ru[7]; s->vi[2] = ru[11];
   s->mesh = ru[12];
@@ -97,6 +166,13 @@ static void surface(const or_scene *S, const or_hit *h, const double o[3], const
     for (int j = 0; j < 3; j++) s->ns[k] += s->b[j] * S->verts[8 * (size_t)s->vi[j] + 4 + k];
   }
   norm3(s->ns);
+  if (h->inst >= 0) {
+    double a[3];
+    rot_apply(P->R, s->ng, a);
+    memcpy(s->ng, a, sizeof a);
+    rot_apply(P->R, s->ns, a);
+    memcpy(s->ns, a, sizeof a);
+  }
   if (dot3(s->ng, d) > 0) for (int k = 0; k < 3; k++) s->ng[k] = -s->ng[k];
   if (dot3(s->ns, s->ng) < 0) for (int k = 0; k < 3; k++) s->ns[k] = -s->ns[k];
   double md[3] = {-d[0], -d[1], -d[2]};
@@ -182,18 +258,21 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
     cam_dir(C, P, x, y, i, d);
   }
   double thr[3] = {1, 1, 1}, L[3] = {0, 0, 0}, pB = 0;
+  const float tk = or_kr(i, GOLD_F, P->s[2]);
+  const double time = (double)(C->shutter[0] + (C->shutter[1] - C->shutter[0]) * tk);
+  pose_t pose;
   const uint32_t nl = C->light_samples < 1 ? 1 : C->light_samples > 16 ? 16 : C->light_samples;
   const uint32_t D = C->max_depth < 1 ? 1 : C->max_depth;
   const double A = S->emit_area;
   for (uint32_t b = 0; b < D; b++) {
     or_hit h;
-    or_traverse(S->nodes, S->tris, o, d, 0, INFINITY, 0, 0, &h);
+    trace_scene(S, o, d, 0, INFINITY, 0, time, &h, &pose);
     (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
     visits[2]++; visits[3] += h.node_visits; visits[4] += h.tri_tests;
     seq_add(seq, h.tri);
     if (h.tri < 0) break;
     surf_t s;
-    surface(S, &h, o, d, &s);
+    surface(S, &h, &pose, o, d, &s);
     if (s.flags & F_EMITTER) {
       double ne[3], md[3] = {-d[0], -d[1], -d[2]};
       emitter_normal(S, &s, ne);
@@ -243,7 +322,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double so[3];
       offset_ray(s.p, s.ng, wi, so);
       or_hit sh;
-      or_traverse(S->nodes, S->tris, so, wi, 0, dist - 2e-5, 1, 0, &sh);
+      trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, &sh, NULL);
       (*rays)++; visits[0] += sh.node_visits; visits[1] += sh.tri_tests;
       seq_add(seq, sh.tri >= 0 ? 0 : -1);
       if (sh.tri >= 0) continue;

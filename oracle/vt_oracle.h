@@ -57,7 +57,7 @@ void or_pack_triangles(const float *pos, const uint32_t *idx, const uint32_t *or
 /* gpu/bvh.js:189-206; returns t or +inf, writes u, v */
 double or_intersect_tri(const float *T, uint32_t k, const double o[3], const double d[3], double tmin,
                         double tmax, double *u, double *v);
-typedef struct { double t, u, v; int64_t tri; uint64_t node_visits, tri_tests; } or_hit;
+typedef struct { double t, u, v; int64_t tri; uint64_t node_visits, tri_tests; int32_t inst; } or_hit;
 /* gpu/bvh.js:224-250 (closest or any hit), counting node visits and triangle tests */
 void or_traverse(const float *nodes, const float *T, const double o[3], const double d[3], double tmin,
                  double tmax, int any_hit, uint32_t root, or_hit *out);
@@ -139,12 +139,23 @@ typedef struct {
   const float *lights; uint32_t light_count;   /* 4 floats per light: radiance rgb, - */
   const float *emit; uint32_t emit_count;      /* 16 floats per emitter triangle: v0|cdf, e1|area, e2|light, n|- */
   float emit_area;
+  const struct or_body *bodies; uint32_t body_count;  /* posed bodies (may be 0) */
 } or_scene;
+/* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
+ * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose
+ * positions and normals for these vertices are in the body frame), and its pose at shutter open (q0, p0) and
+ * close (q1, p1), quaternions (x, y, z, w). At exposure time t it sits at slerp(q0, q1, t), lerp(p0, p1, t). */
+typedef struct or_body {
+  const float *nodes; const float *tris; uint32_t tri_count;
+  float q0[4], q1[4], p0[3], p1[3]; uint32_t hidden;
+} or_body;
 typedef struct {
   float R[9];            /* columns: camera x, y, z axes in the world frame */
   float pos[3]; uint32_t mode;
   float lens[9];         /* fx fy cx cy k1 k2 k3 k4 alpha */
   uint32_t width, height, spp, seed, cam_index, light_samples, max_depth, flags;
+  float shutter[2];      /* exposure interval on the pose timeline: sub-sample i is at
+                            shutter[0] + (shutter[1] - shutter[0]) kr(i, 1/golden ratio, s2) */
 } or_camera;
 /* radiance of pixels [x0, x0+w) x [y0, y0+h) of a camera, spp sub-samples each, mean written as rgba
  * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays;

@@ -10,10 +10,13 @@ const u32 = (name) => { const b = fs.readFileSync(path.join(dir, name + '.bin'))
 
 async function main() {
   const { TwinRenderer } = await import(path.resolve(man.module));
-  const scene = { nodes: f32('nodes'), tris: f32('tris'), verts: f32('verts'), meshes: u32('meshes'), materials: f32('materials'),
+  const scene = { tris: f32('tris'), verts: f32('verts'), meshes: u32('meshes'), materials: f32('materials'),
     lights: f32('lights'), emit: f32('emit'), emitArea: man.emitArea };
+  if (man.bodies) Object.assign(scene, { triBody: u32('triBody'), bodies: man.bodies });
+  else scene.nodes = f32('nodes');
   const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed });
-  const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0 } });
+  if (man.poses0) r.setPoses(man.poses0, man.poses1 || null);
+  const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0 }, shutter: man.shutter });
   await out.done;
   for (const c of man.cameras) {
     const lin = await r.readback(c.name, 'linear');

@@ -1,0 +1,200 @@
+"""Posed bodies (the reference's BLAS instances, gpu/scene_gpu.js:313-356; setPoses gpu/renderer.js:619-633;
+traceSceneSkip gpu/wgsl/trace.wgsl.js:144-164): every body has a BVH of its own in its own frame and is placed at
+each sub-sample's exposure time by slerp / lerp of its shutter-open and shutter-close poses.
+
+Oracle pins (CPU): a body posed at (R, t) renders as the same geometry moved into the world by (R, t) and traced
+as static triangles, where R comes from Rodrigues' formula (not from a quaternion); a moving body seen at one
+exposure time (a shutter interval of zero length) renders as the scene placed at the rotation angle and position
+interpolated by hand; a hidden body renders as the scene without it.
+GPU parity: the HIP path against the oracle on the same samples, for a static pose, a moving body over the whole
+exposure and a hidden body, in the persistent and the wavefront forms; the material adjoint with bodies.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import scenes
+
+Z, X = (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)
+A0, A1 = 0.35, 1.25            # body 0 turns about z from A0 to A1 over the exposure
+P0, P1 = (0.75, -1.45, 0.3), (0.95, -1.2, 0.35)
+B1 = (X, 0.4, (-0.75, -1.35, 0.45))
+
+
+@pytest.fixture(scope="module")
+def parts():
+    return scenes.posed_parts()
+
+
+def _poses(moving):
+    q0 = scenes.quat_axis(Z, A0)
+    p = [dict(q0=q0, p0=P0)]
+    if moving:
+        p[0].update(q1=scenes.quat_axis(Z, A1), p1=P1)
+    p.append(dict(q0=scenes.quat_axis(B1[0], B1[1]), p0=B1[2]))
+    return p
+
+
+def _placed(parts, t=0.0, moving=False, drop=None):
+    """The scene with both bodies moved into the world at exposure time t (by hand: angle and position lerp)."""
+    ang = A0 + (A1 - A0) * t if moving else A0
+    pos = np.asarray(P0) + (np.asarray(P1) - np.asarray(P0)) * t if moving else np.asarray(P0)
+    place = {0: (scenes.rot_axis(Z, ang), pos), 1: (scenes.rot_axis(B1[0], B1[1]), np.asarray(B1[2]))}
+    kept = [p for p in parts[0] if drop is None or p[4] != drop]
+    return scenes.posed_scene(kept, *parts[1:], place=place)
+
+
+def _posed(parts, poses):
+    sc = scenes.posed_scene(*parts)
+    sc["poses"] = poses
+    return sc
+
+
+def _compare(a, b, rel_bar, close_bar=0.99):
+    a = a[:, :, 0:3].astype(np.float64)
+    b = b[:, :, 0:3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    return rel, close
+
+
+# ---------------------------------------------------------------------------- oracle pins (CPU)
+
+def test_oracle_static_pose_is_the_placed_geometry(parts):
+    cam = scenes.posed_camera(48, 36, spp=4)
+    a, _, _ = oracle.render_tile(_posed(parts, _poses(False)), cam, 0, 0, 48, 36)
+    b, _, _ = oracle.render_tile(_placed(parts), cam, 0, 0, 48, 36)
+    rel, close = _compare(a, b, 1e-6)
+    print("static pose vs placed: rel %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-6 and close > 0.999
+
+
+@pytest.mark.parametrize("t", [0.25, 0.5, 1.0])
+def test_oracle_moving_body_at_one_time(parts, t):
+    """shutter [t, t]: every sub-sample sees slerp(q0, q1, t), lerp(p0, p1, t): the scene placed at the angle
+    A0 + (A1 - A0) t about z (both ends share the axis, so slerp turns linearly in angle)."""
+    cam = scenes.posed_camera(48, 36, spp=4, shutter=(t, t))
+    a, _, _ = oracle.render_tile(_posed(parts, _poses(True)), cam, 0, 0, 48, 36)
+    b, _, _ = oracle.render_tile(_placed(parts, t, moving=True), cam, 0, 0, 48, 36)
+    rel, close = _compare(a, b, 1e-6)
+    print("t=%.2f moving vs placed: rel %.3g, close %.4f" % (t, rel, close))
+    assert rel < 1e-6 and close > 0.999
+
+
+def test_oracle_hidden_body_is_absent(parts):
+    cam = scenes.posed_camera(48, 36, spp=4)
+    p = _poses(False)
+    p[1] = None
+    a, _, _ = oracle.render_tile(_posed(parts, p), cam, 0, 0, 48, 36)
+    b, _, _ = oracle.render_tile(_placed(parts, drop=1), cam, 0, 0, 48, 36)
+    rel, close = _compare(a, b, 1e-6)
+    assert rel < 1e-6 and close > 0.999
+    # both bodies are in view: taking either away changes more than 3 % of the pixels
+    c, _, _ = oracle.render_tile(_placed(parts), cam, 0, 0, 48, 36)
+    for drop in (0, 1):
+        d, _, _ = oracle.render_tile(_placed(parts, drop=drop), cam, 0, 0, 48, 36)
+        assert 1.0 - _compare(c, d, 0)[1] > 0.03
+
+
+def test_oracle_motion_blur_lies_between_the_ends(parts):
+    """Over the whole exposure the image is the time average: each pixel's blurred value sits within the range
+    of the sharp images at the sampled times, in the mean over 4x4 blocks (a sanity bound, not a closed form)."""
+    cam = scenes.posed_camera(32, 24, spp=16)
+    blur, _, _ = oracle.render_tile(_posed(parts, _poses(True)), cam, 0, 0, 32, 24)
+    ends = [oracle.render_tile(_placed(parts, t, moving=True), dict(cam, shutter=(t, t)), 0, 0, 32, 24)[0]
+            for t in (0.0, 0.5, 1.0)]
+    blk = lambda im: im[:, :, 0:3].astype(np.float64).reshape(6, 4, 8, 4, 3).mean(axis=(1, 3))
+    lo = np.minimum.reduce([blk(e) for e in ends])
+    hi = np.maximum.reduce([blk(e) for e in ends])
+    b = blk(blur)
+    assert np.mean((b >= lo - 0.05 * hi - 1e-3) & (b <= hi * 1.05 + 1e-3)) > 0.97
+    assert np.linalg.norm(b - blk(ends[0])) > 0   # the body does move
+
+
+# ---------------------------------------------------------------------------- GPU parity
+
+@pytest.fixture(scope="module")
+def ctx():
+    from vtwin import lib
+    c = lib.Context(0)
+    yield c
+    c.close()
+
+
+def _gpu(ctx, sc, cam, poses, **kw):
+    from vtwin import lib
+    ctx.upload_scene(sc)
+    ctx.set_instances(poses)
+    ctx.set_camera(0, cam)
+    ctx.render([0], **kw)
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    st = ctx.stats()
+    assert st["stack_overflows"] == 0
+    return img, st
+
+
+CASES = {
+    "static": (False, None, (0.0, 1.0)),
+    "moving": (True, None, (0.0, 1.0)),
+    "moving_half": (True, None, (0.2, 0.7)),
+    "hidden": (True, 1, (0.0, 1.0)),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", ["persistent", "wavefront"])
+def test_gpu_posed_bodies_match_oracle(ctx, parts, case, mode):
+    from vtwin import lib
+    moving, hide, shutter = CASES[case]
+    poses = _poses(moving)
+    if hide is not None:
+        poses[hide] = None
+    sc = _posed(parts, poses)
+    cam = scenes.posed_camera(96, 72, spp=8, shutter=shutter)
+    print("posed bodies: case %s, mode %s" % (case, mode), flush=True)
+    m = lib.VT_RENDER_WAVEFRONT if mode == "wavefront" else lib.VT_RENDER_BATCHED
+    img, st = _gpu(ctx, sc, cam, poses, mode=m)
+    assert st["samples"] == 96 * 72 * 8
+    ref, _, _ = oracle.render_tile(sc, cam, 0, 0, 96, 72)
+    rel, close = _compare(img, ref, 1e-3)
+    print("GPU vs oracle: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
+def test_gpu_set_instances_validates(ctx, parts):
+    from vtwin import lib
+    sc = _posed(parts, _poses(False))
+    ctx.upload_scene(sc)
+    with pytest.raises(lib.VTError):
+        ctx.set_instances(_poses(False)[:1])          # one pose for two bodies
+    bad = _poses(False)
+    bad[0] = dict(q0=(0.0, 0.0, 0.0, 0.0), p0=(0, 0, 0))
+    with pytest.raises(lib.VTError):
+        ctx.set_instances(bad)
+    cam = scenes.posed_camera(16, 12, spp=1, shutter=(0.0, 1.5))
+    with pytest.raises(lib.VTError):
+        ctx.set_camera(0, cam)
+
+
+@pytest.mark.gpu
+def test_gpu_material_adjoint_with_bodies(ctx, parts):
+    """The path-replay adjoint replays the same posed paths: material and light gradients against the oracle's
+    adjoint of the same samples (the oracle adjoint is pinned by finite differences elsewhere)."""
+    from vtwin import lib
+    poses = _poses(True)
+    sc = _posed(parts, poses)
+    cam = scenes.posed_camera(48, 36, spp=4)
+    _gpu(ctx, sc, cam, poses)
+    rng = np.random.default_rng(5)
+    adj = rng.uniform(-1.0, 1.0, size=(36, 48, 4)).astype(np.float32)
+    n_mat, n_light = sc["materials"].size // 16, sc["lights"].size // 4
+    gm, gl = ctx.backward([0], [adj], n_mat, n_light)
+    om, ol = oracle.backward_tile(sc, cam, 0, 0, 48, 36, adj)
+    print("posed adjoint: materials max rel %.2e, lights max rel %.2e" % (
+        np.max(np.abs(gm - om)) / np.max(np.abs(om)), np.max(np.abs(gl - ol)) / np.max(np.abs(ol))))
+    assert np.allclose(gl, ol, rtol=1e-3, atol=1e-4 * np.max(np.abs(ol)))
+    assert np.allclose(gm, om, rtol=1e-3, atol=1e-4 * np.max(np.abs(om)))

@@ -30,7 +30,7 @@ def test_module_exports_the_renderer_api():
                 "methods: Object.getOwnPropertyNames(p).sort(), create: typeof m.TwinRenderer.create})); })" % MODULE)
     d = json.loads(out)
     assert {"TwinRenderer", "DEFAULTS", "PRESETS", "OPTION_BITS", "lensParams", "fromPrepared", "RenderFunction",
-            "rotateColumns"} <= set(d["exp"])
+            "rotateColumns", "quatFromMat", "STATIC_BODY"} <= set(d["exp"])
     for m in ("renderCamera", "renderFrame", "sensor", "readback", "setPoses", "project", "timings", "destroy", "idle",
               "backward", "autotune", "updateMaterials", "updateLights", "setCameraParams"):
         assert m in d["methods"], m
@@ -139,3 +139,89 @@ def test_js_rotate_columns_is_rodrigues():
     K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
     M = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
     assert np.allclose(got, M @ np.asarray(R, float).reshape(3, 3).T, atol=1e-7)
+
+
+@needs_node
+def test_js_quat_from_mat_matches_scipy():
+    """quatFromMat (every branch of the largest-component choice) against scipy's Rotation.as_quat (x, y, z, w),
+    up to the sign of the whole quaternion."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(11)
+    mats = [Rotation.random(random_state=3 + k).as_matrix() for k in range(12)]
+    for ax in np.eye(3):                                  # half turns: trace -1, the x, y, z branches
+        mats.append(scenes.rot_axis(ax, np.pi))
+    mats.append(np.eye(3))
+    cols = []
+    for M in mats:
+        T = np.eye(4)
+        T[:3, :3] = M
+        T[:3, 3] = rng.uniform(-1, 1, 3)
+        cols.append([float(x) for x in T.T.reshape(-1)])   # column-major
+    out = _node("import('%s').then(m => console.log(JSON.stringify(%s.map((c) => m.quatFromMat(c)))))" % (MODULE, json.dumps(cols)))
+    got = np.array(json.loads(out))
+    for q, M in zip(got, mats):
+        ref = Rotation.from_matrix(M).as_quat()
+        assert min(np.abs(q - ref).max(), np.abs(q + ref).max()) < 1e-12
+        assert abs(np.linalg.norm(q) - 1) < 1e-14
+
+
+@needs_node
+def test_from_prepared_bodies():
+    """fromPrepared with the reference's BLAS list: triangle ranges become triBody, the caller BVH is dropped (every
+    group's BVH is built at upload), and the bodies keep their keys and kinds."""
+    S = ("{nodes: new Float32Array(16), tris: new Float32Array(16 * 5), verts: new Float32Array(24), "
+         "meshTable: [{mat: 0, flags: 1}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, roughness: 0.5}], "
+         "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2}, {body: 'base', kind: 'fixed', triOff: 4, triCount: 1}]}")
+    out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({tb: Array.from(u.triBody), "
+                "nodes: u.nodes === undefined, bodies: u.bodies})); })" % (MODULE, S))
+    d = json.loads(out)
+    assert d["tb"] == [0xFFFFFFFF, 0, 0, 0xFFFFFFFF, 1] and d["nodes"]
+    assert d["bodies"] == [{"key": "arm", "kind": "body"}, {"key": "base", "kind": "fixed"}]
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_posed_frame_equals_c_abi_frame(tmp_path):
+    """setPoses with 4x4 body poses (rotations from Rodrigues' formula, one body moving over the exposure) through
+    TwinRenderer, against the same frame through the C ABI with the poses as quaternions."""
+    parts = scenes.posed_parts()
+    sc = scenes.posed_scene(*parts)
+    cam = scenes.posed_camera(96, 72, spp=4)
+    for k in ("tris", "verts", "materials", "lights", "emit"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    np.ascontiguousarray(sc["tri_body"], np.uint32).tofile(tmp_path / "triBody.bin")
+
+    def T(R, t):
+        M = np.eye(4)
+        M[:3, :3], M[:3, 3] = R, t
+        return [float(x) for x in M.T.reshape(-1)]
+    z, x = (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)
+    poses0 = {"bodies": {"box": T(scenes.rot_axis(z, 0.35), (0.75, -1.45, 0.3)),
+                         "can": T(scenes.rot_axis(x, 0.4), (-0.75, -1.35, 0.45))}}
+    poses1 = {"bodies": {"box": T(scenes.rot_axis(z, 1.25), (0.95, -1.2, 0.35)),
+                         "can": T(scenes.rot_axis(x, 0.4), (-0.75, -1.35, 0.45))}}
+    sp = scenes.default_sensor()
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"],
+               bodies=["box", "can"], poses0=poses0, poses1=poses1, shutter=[0.0, 1.0],
+               cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
+                             pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
+               sensor=dict(m=[float(v) for v in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
+                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
+                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    js_lin = np.fromfile(tmp_path / "CAM1.linear.bin", np.float32).reshape(cam["height"], cam["width"], 4)
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    ctx.set_instances([dict(q0=scenes.quat_axis(z, 0.35), p0=(0.75, -1.45, 0.3), q1=scenes.quat_axis(z, 1.25),
+                            p1=(0.95, -1.2, 0.35)), dict(q0=scenes.quat_axis(x, 0.4), p0=(-0.75, -1.35, 0.45))])
+    cam["cam_index"] = 0
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    ctx.close()
+    a, b = js_lin[:, :, :3].astype(np.float64), lin[:, :, :3].astype(np.float64)
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)

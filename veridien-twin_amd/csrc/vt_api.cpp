@@ -42,7 +42,9 @@ struct vt_ctx {
   int cu_count = 0;
   // scene
   bool has_scene = false;
-  DevBuf nodes, tri, tshade, meshes, mats, lights, emit;
+  DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst;
+  std::vector<uint32_t> body_root;   // BLAS root node of each body
+  std::vector<float> body_box;       // each body's box in its own frame (min xyz, max xyz)
   SceneDev S{};
   uint32_t tri_count = 0, node4_count = 0;
   // launch support
@@ -151,6 +153,47 @@ int upload(vt_ctx *c, DevBuf &b, const void *src, size_t bytes) {
   return VT_OK;
 }
 
+float bits_f(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// host pose of an instance at exposure time t, the device's formulas in FP64 (slerp, quaternion columns)
+struct HostPose {
+  double c[3][3];   // columns
+  double p[3];
+};
+HostPose host_pose(const vt_instance &I, double t) {
+  double a[4], b[4];
+  for (int k = 0; k < 4; k++) { a[k] = I.q0[k]; b[k] = I.q1[k]; }
+  double c = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+  if (c < 0) {
+    for (double &x : b) x = -x;
+    c = -c;
+  }
+  double q[4];
+  if (c > 0.9995) {
+    double l = 0;
+    for (int k = 0; k < 4; k++) { q[k] = a[k] + (b[k] - a[k]) * t; l += q[k] * q[k]; }
+    for (double &x : q) x /= std::sqrt(l);
+  } else {
+    const double th = std::acos(c), sn = std::sin(th);
+    for (int k = 0; k < 4; k++) q[k] = (a[k] * std::sin((1 - t) * th) + b[k] * std::sin(t * th)) / sn;
+  }
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  HostPose P;
+  const double m[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w)},
+                          {2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w)},
+                          {2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)}};
+  std::memcpy(P.c, m, sizeof m);
+  for (int k = 0; k < 3; k++) P.p[k] = I.p0[k] + (I.p1[k] - I.p0[k]) * t;
+  return P;
+}
+void host_apply(const HostPose &P, const double v[3], double out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = P.c[0][k] * v[0] + P.c[1][k] * v[1] + P.c[2][k] * v[2] + P.p[k];
+}
+
 // grid for the trace kernels: every CU filled to its occupancy, never more lanes than spill slots
 uint32_t max_blocks(vt_ctx *c, uint32_t block) { return (uint32_t)(c->spill_lanes / block); }
 
@@ -248,7 +291,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit,
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
@@ -317,13 +360,80 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
     std::memcpy(&lid, s->emit + 16 * (size_t)e + 11, 4);
     if (lid >= s->light_count) return fail(c, VT_ERR_ARG, "scene: emitter %u light out of range", e);
   }
+  if (s->body_count > VT_MAX_BODIES) return fail(c, VT_ERR_ARG, "scene: more than %d bodies", VT_MAX_BODIES);
+  if (s->body_count && (!s->tri_body || s->nodes))
+    return fail(c, VT_ERR_ARG, "scene: bodies need tri_body and no caller BVH (nodes = NULL)");
+  for (uint32_t k = 0; s->body_count && k < T; k++)
+    if (s->tri_body[k] != 0xffffffffu) {
+      if (s->tri_body[k] >= s->body_count) return fail(c, VT_ERR_ARG, "scene: triangle %u body out of range", k);
+      // emitters are sampled from the world-space emitter table, so they stay in the static scene
+      if (s->meshes[4 * (size_t)tw[16 * (size_t)k + 12] + 1] & VT_MESH_EMITTER)
+        return fail(c, VT_ERR_ARG, "scene: triangle %u of a body is an emitter", k);
+    }
   // BVH: the caller's (reference layout) or built here; slot order is the tris order
   std::vector<float> nodes2;
   std::vector<float> tris_sorted;
   const float *tr = s->tris;
   uint32_t n2 = s->node_count;
   const float *np = s->nodes;
-  if (!np) {
+  std::vector<Node4> n4;
+  std::vector<uint32_t> body_root(s->body_count, 0);
+  std::vector<float> body_box(6 * (size_t)s->body_count, 0.0f);
+  if (s->body_count) {
+    // one BVH per group (the static scene, then each body) over that group's triangles, built and collapsed
+    // separately and concatenated: node refs offset by the group's first node, leaf slots by its first slot
+    std::vector<float> pos((size_t)V * 3);
+    for (uint32_t v = 0; v < V; v++)
+      for (int k = 0; k < 3; k++) pos[3 * v + k] = s->verts[8 * (size_t)v + k];
+    tris_sorted.resize((size_t)T * 16);
+    uint32_t slot_base = 0;
+    for (uint32_t g = 0; g <= s->body_count; g++) {
+      const uint32_t want = g == 0 ? 0xffffffffu : g - 1;
+      std::vector<uint32_t> members;
+      for (uint32_t k = 0; k < T; k++)
+        if (s->tri_body[k] == want) members.push_back(k);
+      const uint32_t Tg = (uint32_t)members.size();
+      if (Tg == 0) return fail(c, VT_ERR_ARG, g ? "scene: body %u has no triangles" : "scene: no static triangles", g - 1);
+      std::vector<uint32_t> idx((size_t)Tg * 3), order(Tg);
+      for (uint32_t m = 0; m < Tg; m++)
+        for (int j = 0; j < 3; j++) idx[3 * m + j] = tw[16 * (size_t)members[m] + 4 * j + 3];
+      std::vector<float> ng2((size_t)(Tg + 2) * 16);
+      const int64_t nn = build_bvh2(pos.data(), idx.data(), Tg, 4, 16, ng2.data(), Tg + 2, order.data());
+      if (nn < 0) return fail(c, VT_ERR_ARG, "scene: BVH build failed");
+      std::vector<Node4> g4;
+      try {
+        g4 = collapse_bvh4(ng2.data(), (uint32_t)nn, Tg);
+      } catch (const std::exception &ex) {
+        return fail(c, VT_ERR_ARG, "scene: %s", ex.what());
+      }
+      const uint32_t node_base = (uint32_t)n4.size();
+      for (Node4 &nd : g4) {
+        for (int ch = 0; ch < 4; ch++) {
+          const uint32_t r = nd.ref[ch];
+          if (r == EMPTY_REF) continue;
+          if (r & LEAF_BIT) nd.ref[ch] = (r & ~0x03ffffffu) | ((r & 0x03ffffffu) + slot_base);
+          else nd.ref[ch] = r + node_base;
+        }
+        n4.push_back(nd);
+      }
+      for (uint32_t m = 0; m < Tg; m++)
+        std::memcpy(&tris_sorted[16 * (size_t)(slot_base + m)], s->tris + 16 * (size_t)members[order[m]], 64);
+      if (g > 0) {
+        body_root[g - 1] = node_base;
+        float *bx = &body_box[6 * (size_t)(g - 1)];
+        for (int k = 0; k < 3; k++) { bx[k] = INFINITY; bx[3 + k] = -INFINITY; }
+        for (uint32_t m = 0; m < Tg; m++)
+          for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+              const float x = pos[3 * (size_t)idx[3 * m + j] + k];
+              bx[k] = std::min(bx[k], x);
+              bx[3 + k] = std::max(bx[3 + k], x);
+            }
+      }
+      slot_base += Tg;
+    }
+    tr = tris_sorted.data();
+  } else if (!np) {
     std::vector<float> pos((size_t)V * 3);
     for (uint32_t v = 0; v < V; v++)
       for (int k = 0; k < 3; k++) pos[3 * v + k] = s->verts[8 * (size_t)v + k];
@@ -341,11 +451,12 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   } else if (n2 == 0) {
     return fail(c, VT_ERR_ARG, "scene: node_count is 0");
   }
-  std::vector<Node4> n4;
-  try {
-    n4 = collapse_bvh4(np, n2, T);
-  } catch (const std::exception &ex) {
-    return fail(c, VT_ERR_ARG, "scene: %s", ex.what());
+  if (!s->body_count) {
+    try {
+      n4 = collapse_bvh4(np, n2, T);
+    } catch (const std::exception &ex) {
+      return fail(c, VT_ERR_ARG, "scene: %s", ex.what());
+    }
   }
   const uint32_t *trw = reinterpret_cast<const uint32_t *>(tr);
   std::vector<float4> tri((size_t)T * 3), vn(V);
@@ -396,6 +507,21 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.emit = (const float4 *)c->emit.p;
   c->S.emit_count = s->emit_count;
   c->S.root = 0;
+  c->body_root = body_root;
+  c->body_box = body_box;
+  c->S.inst_count = s->body_count;
+  if (s->body_count) {
+    // every body starts hidden until vt_set_instances gives it a pose
+    std::vector<float4> rec((size_t)INST_WORDS * s->body_count, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (uint32_t b = 0; b < s->body_count; b++) {
+      rec[INST_WORDS * b + 2].w = bits_f(body_root[b]);
+      rec[INST_WORDS * b + 3].w = bits_f(INST_HIDDEN);
+    }
+    if ((rc = upload(c, c->inst, rec.data(), rec.size() * sizeof(float4)))) return rc;
+    c->S.inst = (const float4 *)c->inst.p;
+  } else {
+    c->S.inst = nullptr;
+  }
   c->S.emit_area = s->emit_area > 0 ? s->emit_area : 1e-12f;
   for (int k = 0; k < 3; k++) {
     c->bmin[k] = INFINITY;
@@ -444,6 +570,8 @@ extern "C" int vt_set_camera(vt_ctx *c, uint32_t cam, const vt_camera *k) {
   if (k->spp == 0) return fail(c, VT_ERR_ARG, "camera %u: spp is 0", cam);
   if (k->mode > 1) return fail(c, VT_ERR_ARG, "camera %u: mode %u is not kb4 (0) or pinhole (1)", cam, k->mode);
   if (!(k->lens[0] > 0) || !(k->lens[1] > 0)) return fail(c, VT_ERR_ARG, "camera %u: focal lengths must be > 0", cam);
+  if (!(k->shutter[0] >= 0.0f && k->shutter[0] <= 1.0f && k->shutter[1] >= 0.0f && k->shutter[1] <= 1.0f))
+    return fail(c, VT_ERR_ARG, "camera %u: shutter times must lie in [0, 1]", cam);
   VT_HIP(c, hipSetDevice(c->device));
   CamSlot &s = c->cams[cam];
   const size_t px = (size_t)k->width * k->height;
@@ -462,6 +590,84 @@ extern "C" int vt_set_camera(vt_ctx *c, uint32_t cam, const vt_camera *k) {
     s.sensed = false;
     s.spp_total = 0;
   }
+  return VT_OK;
+}
+
+extern "C" int vt_set_instances(vt_ctx *c, const vt_instance *in, uint32_t count) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "set_instances: no scene uploaded");
+  const uint32_t B = (uint32_t)c->body_root.size();
+  if (count != B || (B && !in)) return fail(c, VT_ERR_SIZE, "set_instances: %u bodies in the scene", B);
+  if (!B) return VT_OK;
+  std::vector<float4> rec((size_t)INST_WORDS * B);
+  for (uint32_t b = 0; b < B; b++) {
+    vt_instance I = in[b];
+    for (float *q : {I.q0, I.q1}) {
+      const double l = std::sqrt((double)q[0] * q[0] + (double)q[1] * q[1] + (double)q[2] * q[2] + (double)q[3] * q[3]);
+      if (!(l > 1e-12) || !std::isfinite(l)) return fail(c, VT_ERR_ARG, "set_instances: body %u has no unit quaternion", b);
+      for (int k = 0; k < 4; k++) q[k] = (float)(q[k] / l);
+    }
+    bool moving = false;
+    for (int k = 0; k < 4; k++) moving |= I.q0[k] != I.q1[k];
+    for (int k = 0; k < 3; k++) moving |= I.p0[k] != I.p1[k];
+    const uint32_t flags = (moving ? INST_MOVING : 0u) | (I.hidden ? INST_HIDDEN : 0u);
+    float4 *r = &rec[INST_WORDS * (size_t)b];
+    r[0] = make_float4(I.q0[0], I.q0[1], I.q0[2], I.q0[3]);
+    r[1] = make_float4(I.q1[0], I.q1[1], I.q1[2], I.q1[3]);
+    r[2] = make_float4(I.p0[0], I.p0[1], I.p0[2], bits_f(c->body_root[b]));
+    r[3] = make_float4(I.p1[0], I.p1[1], I.p1[2], bits_f(flags));
+    // Culling bounds that hold at every exposure time. Between shutter open and close a body point y (body frame)
+    // moves on R(t) y + p(t): R(t) turns about one axis by at most phi (the angle between q0 and q1), p(t) is
+    // linear. The world box is the box of the body's corners posed at t = k/8, padded by the largest gap between
+    // a circular arc of angle phi/8 and its chord, rad (1 - cos(phi/16)), rad = |y| bound over the box.
+    const float *bx = &c->body_box[6 * (size_t)b];
+    double ctr[3], half[3];
+    for (int k = 0; k < 3; k++) { ctr[k] = 0.5 * (bx[k] + bx[3 + k]); half[k] = 0.5 * (bx[3 + k] - bx[k]); }
+    const double halfdiag = std::sqrt(half[0] * half[0] + half[1] * half[1] + half[2] * half[2]);
+    const double rad = std::sqrt(ctr[0] * ctr[0] + ctr[1] * ctr[1] + ctr[2] * ctr[2]) + halfdiag;
+    double qd = 0, dp = 0;
+    for (int k = 0; k < 4; k++) qd += (double)I.q0[k] * I.q1[k];
+    for (int k = 0; k < 3; k++) dp += ((double)I.p1[k] - I.p0[k]) * ((double)I.p1[k] - I.p0[k]);
+    const double phi = moving ? 2.0 * std::acos(std::min(1.0, std::fabs(qd))) : 0.0;
+    dp = std::sqrt(dp);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const int nt = moving ? 9 : 1;
+    for (int ti = 0; ti < nt; ti++) {
+      const HostPose P = host_pose(I, moving ? ti / 8.0 : 0.0);
+      for (int cn = 0; cn < 8; cn++) {
+        const double v[3] = {cn & 1 ? bx[3] : bx[0], cn & 2 ? bx[4] : bx[1], cn & 4 ? bx[5] : bx[2]};
+        double w[3];
+        host_apply(P, v, w);
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], w[k]); hi[k] = std::max(hi[k], w[k]); }
+      }
+    }
+    const double scale = std::max({std::fabs(lo[0]), std::fabs(lo[1]), std::fabs(lo[2]), std::fabs(hi[0]),
+                                   std::fabs(hi[1]), std::fabs(hi[2]), 1e-3});
+    const double pad = 1.01 * rad * (1.0 - std::cos(phi / 16.0)) + 4e-6 * scale + 1e-6;
+    r[4] = make_float4((float)(lo[0] - pad), (float)(lo[1] - pad), (float)(lo[2] - pad), 0.f);
+    r[5] = make_float4((float)(hi[0] + pad), (float)(hi[1] + pad), (float)(hi[2] + pad), 0.f);
+    // Bounding capsule of the box along its longest axis, posed at mid-exposure; every point stays within
+    // 2 sin(phi / 4) rad + |p1 - p0| / 2 of its mid-exposure place, which pads the radius.
+    const int ax = half[0] >= half[1] ? (half[0] >= half[2] ? 0 : 2) : (half[1] >= half[2] ? 1 : 2);
+    double ea[3], eb[3];
+    for (int k = 0; k < 3; k++) { ea[k] = ctr[k]; eb[k] = ctr[k]; }
+    ea[ax] -= half[ax];
+    eb[ax] += half[ax];
+    double rr = 0;
+    for (int k = 0; k < 3; k++)
+      if (k != ax) rr += half[k] * half[k];
+    rr = std::sqrt(rr);
+    const HostPose Pm = host_pose(I, moving ? 0.5 : 0.0);
+    double wa[3], wb[3];
+    host_apply(Pm, ea, wa);
+    host_apply(Pm, eb, wb);
+    const double motion = moving ? 1.01 * (2.0 * std::sin(phi / 4.0) * rad + 0.5 * dp) : 0.0;
+    r[6] = make_float4((float)wa[0], (float)wa[1], (float)wa[2], (float)(rr * 1.0001 + motion + 4e-6 * scale + 1e-6));
+    r[7] = make_float4((float)wb[0], (float)wb[1], (float)wb[2], 0.f);
+  }
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(c->inst.p, rec.data(), rec.size() * sizeof(float4), hipMemcpyHostToDevice));
   return VT_OK;
 }
 
@@ -538,6 +744,8 @@ int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opt
     d.acc = (float4 *)s.acc.p;
     d.adj = nullptr;
     d.scale = k.scale > 0 ? k.scale : 1.0f;
+    d.shutter0 = k.shutter[0];
+    d.shutter1 = k.shutter[1];
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -1137,6 +1345,8 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
     d.acc = (float4 *)s.acc.p;
     d.adj = (const float4 *)s.adj.p;
     d.scale = k.scale > 0 ? k.scale : 1.0f;
+    d.shutter0 = k.shutter[0];
+    d.shutter1 = k.shutter[1];
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));

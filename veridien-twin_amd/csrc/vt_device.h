@@ -68,6 +68,7 @@ struct DevCam {
   float4 *acc;                       // per-pixel radiance sum (rgb), sample count (a)
   const float4 *adj;                 // adjoint image dLoss/dI (rgb) for the backward pass, else null
   float scale;                       // output scale (the image is acc * scale / spp)
+  float shutter0, shutter1;          // exposure interval on the pose timeline (CamU.expo.xy)
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)
@@ -100,6 +101,13 @@ __device__ __forceinline__ void pix_seeds(const DevCam &C, uint32_t x, uint32_t 
   P.hs = hash3(x, y, C.seed * 64u + C.cam_index);
 #pragma unroll
   for (int k = 0; k < 9; k++) P.s[k] = u01(pcg(P.hs + (uint32_t)k));
+}
+
+constexpr float GOLD_F = 0.6180339887498949f;
+// the sub-sample's time in the exposure (kernels.wgsl.js:44): every ray of its path sees the bodies at that time
+__device__ __forceinline__ float sample_time(const DevCam &C, const PixSeeds &P, uint32_t i) {
+  const float k = kr(i, GOLD_F, P.s[2]);
+  return C.shutter0 + (C.shutter1 - C.shutter0) * k;
 }
 
 __device__ __forceinline__ v3 camera_dir(const DevCam &C, const PixSeeds &P, uint32_t x, uint32_t y, uint32_t i) {
@@ -301,11 +309,15 @@ struct SceneDev {
   const float4 *emit;       // 4 float4 per emitter triangle
   uint32_t emit_count, root;
   float emit_area;
+  const float4 *inst;       // posed bodies, INST_WORDS float4 each (layout at inst_pose below)
+  uint32_t inst_count;
 };
 
+constexpr uint32_t NO_INST = 0xffffffffu;
 struct Hit {
   float t, u, v;
   uint32_t tri;
+  uint32_t inst;     // posed body of the hit, NO_INST for the static scene
 };
 
 // per-lane traversal stack: SLDS entries in LDS, the rest spilled to a per-lane global area
@@ -417,14 +429,15 @@ __device__ __forceinline__ uint32_t pop_next(const Stack &st, uint32_t &sp, floa
 }
 
 template <bool ANY>
-__device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, Hit &h, Stack &st) {
+__device__ __forceinline__ bool traverse(const SceneDev &S, uint32_t root, v3 o, v3 d, float tmin, float tmax, Hit &h,
+                                         Stack &st) {
   const v3 inv = safe_inv(d);
   const v3 oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const RayW R = ray_w(o, d);
   float best = tmax;
   bool found = false;
   uint32_t sp = 0;
-  uint32_t ref = S.root;
+  uint32_t ref = root;
   uint32_t leaf = TRAV_DONE;      // postponed leaf
   while (ref != TRAV_DONE || leaf != TRAV_DONE) {
     // node phase
@@ -496,6 +509,125 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, v3 o, v3 d, float tm
       }
     }
   }
+  return found;
+}
+
+// ------------------------------------------------------------------ posed bodies (trace.wgsl.js:122-164)
+// Instance record, INST_WORDS float4: [0] q0 and [1] q1 (unit quaternions x y z w at shutter open and close),
+// [2] p0.xyz + BLAS root (bits), [3] p1.xyz + flags (bits: 1 moving, 2 hidden), [4] world box min, [5] world box
+// max (swept over the exposure), [6] bounding capsule a.xyz + radius, [7] capsule b.xyz.
+constexpr int INST_WORDS = 8;
+constexpr uint32_t INST_MOVING = 1u, INST_HIDDEN = 2u;
+
+struct Pose {
+  v3 c0, c1, c2;    // rotation columns
+  v3 p;
+};
+// quaternion (x, y, z, w) -> rotation columns (common.wgsl.js:117-123 quatToMat)
+__device__ __forceinline__ void quat_cols(float4 q, v3 &c0, v3 &c1, v3 &c2) {
+  const float x = q.x, y = q.y, z = q.z, w = q.w;
+  c0 = mk(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + z * w), 2.0f * (x * z - y * w));
+  c1 = mk(2.0f * (x * y - z * w), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * w));
+  c2 = mk(2.0f * (x * z + y * w), 2.0f * (y * z - x * w), 1.0f - 2.0f * (x * x + y * y));
+}
+// the shorter-arc spherical interpolation, normalised lerp when the ends nearly agree (common.wgsl.js:125-133)
+__device__ __forceinline__ float4 slerp4(float4 a, float4 b, float t) {
+  float c = a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+  if (c < 0.0f) {
+    b = make_float4(-b.x, -b.y, -b.z, -b.w);
+    c = -c;
+  }
+  if (c > 0.9995f) {
+    float4 m = make_float4(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t, a.w + (b.w - a.w) * t);
+    const float il = 1.0f / sqrtf(m.x * m.x + m.y * m.y + m.z * m.z + m.w * m.w);
+    return make_float4(m.x * il, m.y * il, m.z * il, m.w * il);
+  }
+  const float th = acosf(c), s = sinf(th);
+  const float wa = sinf((1.0f - t) * th) / s, wb = sinf(t * th) / s;
+  return make_float4(a.x * wa + b.x * wb, a.y * wa + b.y * wb, a.z * wa + b.z * wb, a.w * wa + b.w * wb);
+}
+// pose of instance i at exposure time t (common.wgsl.js:137-143 instPose)
+__device__ __forceinline__ Pose inst_pose(const SceneDev &S, uint32_t i, float t) {
+  const float4 *I = S.inst + INST_WORDS * (size_t)i;
+  const float4 p0 = I[2], p1 = I[3];
+  Pose P;
+  if (__float_as_uint(p1.w) & INST_MOVING) {
+    quat_cols(slerp4(I[0], I[1], t), P.c0, P.c1, P.c2);
+    P.p = mk(p0.x + (p1.x - p0.x) * t, p0.y + (p1.y - p0.y) * t, p0.z + (p1.z - p0.z) * t);
+  } else {
+    quat_cols(I[0], P.c0, P.c1, P.c2);
+    P.p = mk(p0.x, p0.y, p0.z);
+  }
+  return P;
+}
+__device__ __forceinline__ v3 rot(const Pose &P, v3 v) { return P.c0 * v.x + P.c1 * v.y + P.c2 * v.z; }
+__device__ __forceinline__ v3 rot_t(const Pose &P, v3 v) { return mk(dot(P.c0, v), dot(P.c1, v), dot(P.c2, v)); }
+
+// does the segment o + t d, t in [t0, t1], pass within r of the segment a-b (trace.wgsl.js:122-136)
+__device__ __forceinline__ bool seg_near_capsule(v3 o, v3 d, float t0, float t1, v3 a, v3 b, float r) {
+  const v3 p0 = o + d * t0, u = d * (t1 - t0), v = b - a, w = p0 - a;
+  const float A = dot(u, u), B = dot(u, v), Cc = dot(v, v), D = dot(u, w), E = dot(v, w);
+  const float den = A * Cc - B * B;
+  float s = 0.0f;
+  if (den > 1e-20f * A * Cc) s = fminf(fmaxf((B * E - Cc * D) / den, 0.0f), 1.0f);
+  float t = Cc > 1e-30f ? (B * s + E) / Cc : 0.0f;
+  if (t < 0.0f) {
+    t = 0.0f;
+    s = fminf(fmaxf(-D / fmaxf(A, 1e-30f), 0.0f), 1.0f);
+  } else if (t > 1.0f) {
+    t = 1.0f;
+    s = fminf(fmaxf((B - D) / fmaxf(A, 1e-30f), 0.0f), 1.0f);
+  }
+  const v3 q = (p0 + u * s) - (a + v * t);
+  return dot(q, q) <= r * r;
+}
+
+__device__ __forceinline__ float box_entry(float4 lo, float4 hi, v3 o, v3 inv, float tmin, float tmax) {
+  const float x0 = (lo.x - o.x) * inv.x, x1 = (hi.x - o.x) * inv.x;
+  const float y0 = (lo.y - o.y) * inv.y, y1 = (hi.y - o.y) * inv.y;
+  const float z0 = (lo.z - o.z) * inv.z, z1 = (hi.z - o.z) * inv.z;
+  const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+  const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+  return a <= b ? a : INFINITY;
+}
+
+// The static scene, then every visible posed body (world box, bounding capsule, then its BLAS in its own frame at
+// the ray's exposure time), as traceSceneSkip (trace.wgsl.js:144-164) does. h.inst names the body of the hit.
+// the posed bodies after the static scene: `found` and h carry the static result in and the final one out
+template <bool ANY>
+__device__ __forceinline__ void trace_instances(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time,
+                                                Hit &h, bool &found, Stack &st) {
+  float best = found ? h.t : tmax;
+  const v3 inv = safe_inv(d);
+  for (uint32_t i = 0; i < S.inst_count; i++) {
+    const float4 *I = S.inst + INST_WORDS * (size_t)i;
+    const float4 p0 = I[2], p1 = I[3];
+    if (__float_as_uint(p1.w) & INST_HIDDEN) continue;
+    if (box_entry(I[4], I[5], o, inv, tmin, best) >= best) continue;
+    const float4 ca = I[6], cb = I[7];
+    if (!seg_near_capsule(o, d, tmin, fminf(best, 1e3f), ld3(ca), ld3(cb), ca.w)) continue;
+    const Pose P = inst_pose(S, i, time);
+    const v3 ol = rot_t(P, o - P.p), dl = rot_t(P, d);
+    Hit hl;
+    if (traverse<ANY>(S, __float_as_uint(p0.w), ol, dl, tmin, best, hl, st)) {
+      h = hl;
+      h.inst = i;
+      best = hl.t;
+      found = true;
+      if (ANY) return;
+    }
+  }
+}
+
+template <bool ANY>
+__device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time, Hit &h,
+                                            Stack &st) {
+  bool found = traverse<ANY>(S, S.root, o, d, tmin, tmax, h, st);
+  if (found) {
+    h.inst = NO_INST;
+    if (ANY) return true;
+  }
+  if (S.inst_count) trace_instances<ANY>(S, o, d, tmin, tmax, time, h, found, st);
   return found;
 }
 

@@ -457,7 +457,7 @@ __global__ __launch_bounds__(DEFAULT_BLOCK) void k_formfactor(SceneDev S, uint32
         const float rr = sqrtf(dot(dd, dd));
         dd = dd * (1.0f / rr);
         Hit h;
-        if (!traverse<true>(S, o, dd, 0.0f, rr - 1e-4f, h, st)) f = ci * cj * pj.w / (PI_F * r2);
+        if (!traverse<true>(S, S.root, o, dd, 0.0f, rr - 1e-4f, h, st)) f = ci * cj * pj.w / (PI_F * r2);
       }
     }
     F[q] = f;

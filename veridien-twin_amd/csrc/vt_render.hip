@@ -38,6 +38,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
   // (the NEE term and its MIS weight, the derivative pieces), and the material record is re-read from the
   // cached table after it, so only the ray and the path's small state stay live across the traversals.
   const uint32_t hs = P.hs;
+  const float time = sample_time(C, P, i);
   v3 o = mk(C.pos[0], C.pos[1], C.pos[2]);
   v3 d = camera_dir(C, P, x, y, i);
   v3 thr = mk(1.f, 1.f, 1.f), L = mk(0.f, 0.f, 0.f);
@@ -49,7 +50,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
   for (uint32_t b = 0; b < D; b++) {
     Hit h;
     nrays++;
-    if (!traverse<false>(S, o, d, 0.0f, INFINITY, h, st)) break;
+    if (!trace_scene<false>(S, o, d, 0.0f, INFINITY, time, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
@@ -57,6 +58,11 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
     {
       const float w0 = 1.0f - h.u - h.v;
       nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h.u + ld3(sh[2]) * h.v);
+    }
+    if (h.inst != NO_INST) {
+      const Pose Pb = inst_pose(S, h.inst, time);
+      ng = rot(Pb, ng);
+      nsi = rot(Pb, nsi);
     }
     ns = nsi;
     if (dot(ng, d) > 0.0f) ng = -ng;
@@ -128,7 +134,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
         }
         Hit sh;
         nrays++;
-        if (traverse<true>(S, so, wi, 0.0f, tmax, sh, st)) continue;
+        if (trace_scene<true>(S, so, wi, 0.0f, tmax, time, sh, st)) continue;
         Ln = Ln + contrib;
         if (GRAD) {
           grad_add(gs->light, lid, gl_part);
@@ -240,6 +246,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
                                             v3 o0, v3 d0, v3 e1, v3 e2, v3 delta, Stack &st, uint32_t &nrays,
                                             float ws[ND]) {
   const uint32_t hs = P.hs;
+  const float time = sample_time(C, P, i);
   DV o = dv(o0), d = dv(d0);
   o.x.g[0] = e1.x; o.y.g[0] = e1.y; o.z.g[0] = e1.z;
   o.x.g[1] = e2.x; o.y.g[1] = e2.y; o.z.g[1] = e2.z;
@@ -256,12 +263,22 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
   for (uint32_t b = 0; b < Dp; b++) {
     Hit h;
     nrays++;
-    if (!traverse<false>(S, val(o), val(d), 0.0f, INFINITY, h, st)) break;
+    if (!trace_scene<false>(S, val(o), val(d), 0.0f, INFINITY, time, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
     const float4 *tp = S.tri + 3 * (size_t)h.tri;
-    const v3 p0 = ld3(tp[0]), e1 = ld3(tp[1]) - p0, e2 = ld3(tp[2]) - p0;
+    v3 p0 = ld3(tp[0]), e1 = ld3(tp[1]) - p0, e2 = ld3(tp[2]) - p0;
+    v3 nl0 = ld3(n0), nl1 = ld3(sh[1]), nl2 = ld3(sh[2]), ngl = ld3(sh[3]);
+    if (h.inst != NO_INST) {
+      // a posed body: its triangle and normals in the world at the sample's time (the pose does not depend on the
+      // camera)
+      const Pose Pb = inst_pose(S, h.inst, time);
+      p0 = rot(Pb, p0) + Pb.p;
+      e1 = rot(Pb, e1);
+      e2 = rot(Pb, e2);
+      nl0 = rot(Pb, nl0); nl1 = rot(Pb, nl1); nl2 = rot(Pb, nl2); ngl = rot(Pb, ngl);
+    }
     const v3 nt = cross(e1, e2);
     // the hit on the triangle's plane and its barycentrics, as functions of the ray
     const D t = ddot(p0 - o, nt) / ddot(d, nt);
@@ -270,8 +287,8 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
     const float inn = 1.0f / dot(nt, nt);
     const D bu = ddot(dcross(q, dv(e2)), nt) * inn;
     const D bv = ddot(dcross(dv(e1), q), nt) * inn;
-    v3 ng = ld3(sh[3]);
-    const DV nsi = dnormalize(ld3(n0) * (1.0f - bu - bv) + ld3(sh[1]) * bu + ld3(sh[2]) * bv);
+    v3 ng = ngl;
+    const DV nsi = dnormalize(nl0 * (1.0f - bu - bv) + nl1 * bu + nl2 * bv);
     DV ns = nsi;
     const v3 dval = val(d);
     if (dot(ng, dval) > 0.0f) ng = -ng;
@@ -336,7 +353,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
         const v3 so = val(p) + ng * (dot(ng, wiv) > 0.0f ? EPS_OFF : -EPS_OFF);
         Hit sh;
         nrays++;
-        if (traverse<true>(S, so, wiv, 0.0f, dist.v - 2e-5f, sh, st)) continue;
+        if (trace_scene<true>(S, so, wiv, 0.0f, dist.v - 2e-5f, time, sh, st)) continue;
         Ln = Ln + contrib;
       }
     }
@@ -734,8 +751,8 @@ __global__ __launch_bounds__(DEFAULT_BLOCK) void k_trace_test(TestLaunch Lt, con
   for (uint32_t k = blockIdx.x * DEFAULT_BLOCK + threadIdx.x; k < n; k += gridDim.x * DEFAULT_BLOCK) {
     const float *r = rays + 8 * (size_t)k;
     Hit h;
-    bool hit = any_hit ? traverse<true>(Lt.S, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st)
-                       : traverse<false>(Lt.S, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st);
+    bool hit = any_hit ? traverse<true>(Lt.S, Lt.S.root, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st)
+                       : traverse<false>(Lt.S, Lt.S.root, mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]), r[3], r[7], h, st);
     float *o = hits + 4 * (size_t)k;
     o[0] = hit ? h.t : INFINITY;
     o[1] = __uint_as_float(hit ? h.tri : MISS);

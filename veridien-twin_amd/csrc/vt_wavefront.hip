@@ -85,13 +85,15 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
     const ItemPix ip = item_pix(C, item);
     W.cam[p] = c;
     v3 d = mk(0.f, 0.f, 1.f);
+    float time = 0.0f;
     if (ip.valid) {
       PixSeeds P;
       pix_seeds(C, ip.x, ip.y, P);
       d = camera_dir(C, P, ip.x, ip.y, ip.i);
+      time = sample_time(C, P, ip.i);
     }
     W.o[p] = make_float4(C.pos[0], C.pos[1], C.pos[2], 0.0f);   // w: pdf of the BSDF sample that led here
-    W.d[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    W.d[p] = make_float4(d.x, d.y, d.z, time);                   // w: the sample's time in the exposure
     W.thr[p] = make_float4(1.f, 1.f, 1.f, ip.valid ? 1.0f : 0.0f);   // w: 1 while the path is alive
     W.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
@@ -119,9 +121,10 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
   v3 inv = mk(0.f, 0.f, 0.f), oi = mk(0.f, 0.f, 0.f);
   RayW R;
   R.o = mk(0.f, 0.f, 0.f); R.kx = 0; R.ky = 1; R.kz = 2; R.Sx = 0.f; R.Sy = 0.f; R.Sz = 0.f;
-  float best = 0.0f;
+  float best = 0.0f, tmax0 = 0.0f, time = 0.0f;
+  v3 dir = mk(0.f, 0.f, 1.f);
   bool found = false;
-  Hit h{0.f, 0.f, 0.f, 0u};
+  Hit h{0.f, 0.f, 0.f, 0u, NO_INST};
   uint32_t sp = 0, ref = TRAV_DONE, leaf = TRAV_DONE;
   uint32_t nrays = 0;
   for (;;) {
@@ -145,6 +148,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
             o = mk(o4.x, o4.y, o4.z);
             d = mk(d4.x, d4.y, d4.z);
             tmax = o4.w;
+            time = d4.w;
           } else {
             const uint32_t p = W.q ? W.q[k] : k;
             ok = W.thr[p].w > 0.0f;
@@ -152,13 +156,16 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
             o = mk(o4.x, o4.y, o4.z);
             d = mk(d4.x, d4.y, d4.z);
             tmax = INFINITY;
+            time = d4.w;
             if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
           }
           if (ok) {
             inv = safe_inv(d);
             oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
             R = ray_w(o, d);
+            dir = d;
             best = tmax;
+            tmax0 = tmax;
             found = false;
             sp = 0;
             ref = W.S.root;
@@ -242,10 +249,15 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
         }
         const bool done = ref == TRAV_DONE && leaf == TRAV_DONE;
         if (done) {
+          h.inst = NO_INST;
+          // then the posed bodies (few, small: walked one after the other with the lane's own stack)
+          if (W.S.inst_count && !(ANY && found)) trace_instances<ANY>(W.S, R.o, dir, 0.0f, tmax0, time, h, found, st);
           if (ANY) {
             if (!found) W.sh_c[k].w = 1.0f;
           } else {
-            W.hit[k] = found ? make_float4(h.t, h.u, h.v, __uint_as_float(h.tri))
+            // the hit's body rides in the triangle word's top bits (slots are < 2^26)
+            const uint32_t bits = h.tri | (h.inst == NO_INST ? 0u : (h.inst + 1u) << 26);
+            W.hit[k] = found ? make_float4(h.t, h.u, h.v, __uint_as_float(bits))
                              : make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
           }
           active = false;
@@ -283,7 +295,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
       const uint32_t Dp = C.depth < 1 ? 1 : C.depth;
       uint32_t nee = 0;   // shadow slots written
       const float4 h4 = W.hit[k];
-      const uint32_t tri = __float_as_uint(h4.w);
+      const uint32_t bits = __float_as_uint(h4.w);
+      const uint32_t tri = bits == MISS ? MISS : (bits & 0x03ffffffu);
+      const uint32_t inst = bits == MISS ? NO_INST : (bits >> 26) - 1u;   // NO_INST when the top bits are 0
       if (tri != MISS && b < Dp) {
         const ItemPix ip = item_pix(C, path_item(W, p));
         PixSeeds P;
@@ -291,7 +305,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         const uint32_t i = ip.i, hs = P.hs;
         const float4 o4 = W.o[p], d4 = W.d[p];
         const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-        const float pB = o4.w;
+        const float pB = o4.w, time = d4.w;
         const float4 t4 = W.thr[p];
         const v3 thr = mk(t4.x, t4.y, t4.z);
         W.thr_slot[k] = t4;             // the gather's weight for this vertex's NEE
@@ -302,6 +316,11 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         {
           const float w0 = 1.0f - h4.y - h4.z;
           nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h4.y + ld3(sh[2]) * h4.z);
+        }
+        if (inst != NO_INST) {
+          const Pose Pb = inst_pose(W.S, inst, time);
+          ng = rot(Pb, ng);
+          nsi = rot(Pb, nsi);
         }
         ns = nsi;
         if (dot(ng, d) > 0.0f) ng = -ng;
@@ -363,7 +382,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               // slots in j order; the gather adds the visible ones in that order
               const size_t s = (size_t)k * W.nl_max + nee;
               W.sh_o[s] = make_float4(so.x, so.y, so.z, dist - 2e-5f);
-              W.sh_d[s] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+              W.sh_d[s] = make_float4(wi.x, wi.y, wi.z, time);
               W.sh_c[s] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
               nee++;
             }
@@ -389,7 +408,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
                 const v3 no = pt + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
                 W.thr[p] = make_float4(tn.x, tn.y, tn.z, 1.0f);
                 W.o[p] = make_float4(no.x, no.y, no.z, pdf);
-                W.d[p] = make_float4(l.x, l.y, l.z, 0.0f);
+                W.d[p] = make_float4(l.x, l.y, l.z, time);
                 W.key_next[k] = ray_key(no, l, W);
                 W.q_next[k] = p;
                 live = true;

@@ -8,6 +8,8 @@ import { createRequire } from 'module';
 
 const require = createRequire(import.meta.url);
 const native = require('../napi/vt_napi.node');
+/** triBody entry of a static-scene triangle */
+export const STATIC_BODY = 0xffffffff;
 
 export const OPTION_BITS = Object.freeze({ NO_MIS: 128 });
 export const DEFAULTS = Object.freeze({
@@ -75,11 +77,36 @@ export function fromPrepared(S) {
     eu[16 * k + 11] = 0;
   });
   const L0 = em.L0 || [0, 0, 0];
-  return {
+  const out = {
     nodes: S.nodes, tris: S.tris, verts: S.verts, meshes, materials,
     lights: new Float32Array([L0[0], L0[1], L0[2], 0]), emit: em.tris.length ? emit : new Float32Array(0),
     emitArea: em.area || 0,
   };
+  // bodies (S.blas: contiguous triangle ranges in their own frames): the library builds every BVH itself
+  if (S.blas && S.blas.length) {
+    const triBody = new Uint32Array(S.tris.length / 16).fill(STATIC_BODY);
+    S.blas.forEach((b, i) => triBody.fill(i, b.triOff, b.triOff + b.triCount));
+    Object.assign(out, { nodes: undefined, triBody, bodies: S.blas.map((b) => ({ key: b.body, kind: b.kind })) });
+  }
+  return out;
+}
+
+/** Unit quaternion (x, y, z, w) of the rotation in the top-left 3x3 of a column-major 4x4 (Shepperd's method:
+ * the largest of 1 + trace and the three 1 + 2 r_ii - trace picks the component to divide by). */
+export function quatFromMat(m) {
+  const r = (i, j) => m[4 * j + i];
+  const tr = r(0, 0) + r(1, 1) + r(2, 2);
+  const cand = [1 + tr, 1 + 2 * r(0, 0) - tr, 1 + 2 * r(1, 1) - tr, 1 + 2 * r(2, 2) - tr];
+  let k = 0;
+  for (let i = 1; i < 4; i++) if (cand[i] > cand[k]) k = i;
+  const h = 0.5 * Math.sqrt(cand[k]), f = 0.25 / h;
+  let q;
+  if (k === 0) q = [(r(2, 1) - r(1, 2)) * f, (r(0, 2) - r(2, 0)) * f, (r(1, 0) - r(0, 1)) * f, h];
+  else if (k === 1) q = [h, (r(0, 1) + r(1, 0)) * f, (r(0, 2) + r(2, 0)) * f, (r(2, 1) - r(1, 2)) * f];
+  else if (k === 2) q = [(r(0, 1) + r(1, 0)) * f, h, (r(1, 2) + r(2, 1)) * f, (r(0, 2) - r(2, 0)) * f];
+  else q = [(r(0, 2) + r(2, 0)) * f, (r(1, 2) + r(2, 1)) * f, h, (r(1, 0) - r(0, 1)) * f];
+  const l = Math.hypot(...q);
+  return q.map((x) => x / l);
 }
 
 export class TwinRenderer {
@@ -101,8 +128,9 @@ export class TwinRenderer {
     if (!src || !src.tris) throw new Error('scene: pass the upload arrays or {prepared}');
     this.ctx = native.create(opts.device || 0);
     this.device = { hip: opts.device || 0 };
-    native.uploadScene(this.ctx, src);
-    this.stats = { triangles: src.tris.length / 16 };
+    this.bodies = src.triBody ? (src.bodies || []).map((b) => (typeof b === 'string' ? { key: b, kind: 'body' } : b)) : [];
+    native.uploadScene(this.ctx, { ...src, bodyCount: this.bodies.length });
+    this.stats = { triangles: src.tris.length / 16, bodies: this.bodies.length };
     this.materials = Float32Array.from(src.materials);
     this.lights = Float32Array.from(src.lights);
     this.cameras = {};
@@ -110,6 +138,7 @@ export class TwinRenderer {
     this.frame = 0;
     this.poses0 = null;
     this.poses1 = null;
+    if (this.bodies.length) this.setPoses(null);
   }
 
   _addCamera(c, index) {
@@ -135,14 +164,25 @@ export class TwinRenderer {
     native.setCamera(this.ctx, cam.index, {
       R: cam.R, pos: cam.pos, lens: cam.lens, mode: cam.mode, width: cam.size[0], height: cam.size[1], spp,
       seed: dflt(o.seed, this.opts.seed), camIndex: cam.index, lightSamples: dflt(o.lightSamples, this.opts.lightSamples),
-      maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1),
+      maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1), shutter: o.shutter || [0, 1],
     });
   }
 
-  /** poses.bodies[key] = column-major T_D_body (gpu/renderer.js:619-633). Posed bodies are not in this build. */
+  /**
+   * poses.bodies[key] = column-major 4x4 T_D_body at shutter open (poses0) and close (poses1, default poses0)
+   * (gpu/renderer.js:619-633). Fixed bodies keep the identity; a body without a pose is hidden. A sub-sample at
+   * exposure time t (opts.shutter of a render, default [0, 1]) sees slerp / lerp of the two poses.
+   */
   setPoses(poses0, poses1 = null) {
-    const b = (poses0 && poses0.bodies) || {};
-    if (Object.keys(b).length) throw new Error('posed bodies (BLAS instances) are not supported by this build');
+    const b0 = (poses0 && poses0.bodies) || {}, b1 = ((poses1 || poses0) && (poses1 || poses0).bodies) || {};
+    for (const k of Object.keys(b0)) if (!this.bodies.some((b) => b.key === k)) throw new Error(`unknown body ${k}`);
+    this.instances = this.bodies.map((bl) => {
+      if (bl.kind === 'fixed') return { q0: [0, 0, 0, 1], q1: [0, 0, 0, 1], p0: [0, 0, 0], p1: [0, 0, 0] };
+      const T0 = b0[bl.key], T1 = b1[bl.key] || T0;
+      if (!T0) return null;
+      return { q0: quatFromMat(T0), q1: quatFromMat(T1), p0: [T0[12], T0[13], T0[14]], p1: [T1[12], T1[13], T1[14]] };
+    });
+    if (this.bodies.length) native.setInstances(this.ctx, this.instances);
     this.poses0 = poses0; this.poses1 = poses1 || poses0;
   }
 

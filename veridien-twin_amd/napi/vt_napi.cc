@@ -188,6 +188,12 @@ napi_value UploadScene(napi_env env, napi_callback_info info) {
   s.emit = typed<float>(env, argv[1], "emit", &n, napi_float32_array);
   s.emit_count = (uint32_t)(n / 16);
   s.emit_area = (float)num(env, argv[1], "emitArea", 0);
+  s.tri_body = typed<uint32_t>(env, argv[1], "triBody", &n, napi_uint32_array);
+  s.body_count = s.tri_body ? (uint32_t)num(env, argv[1], "bodyCount", 0) : 0;
+  if (s.tri_body && n != s.tri_count) {
+    napi_throw_range_error(env, nullptr, "triBody needs one entry per triangle");
+    return nullptr;
+  }
   bool pending = false;
   napi_is_exception_pending(env, &pending);
   if (pending) return nullptr;
@@ -219,8 +225,52 @@ napi_value SetCamera(napi_env env, napi_callback_info info) {
   k.light_samples = (uint32_t)num(env, argv[2], "lightSamples", 1);
   k.max_depth = (uint32_t)num(env, argv[2], "maxDepth", 1);
   k.scale = (float)num(env, argv[2], "scale", 1.0);
+  k.shutter[0] = 0.0f;
+  k.shutter[1] = 1.0f;
+  napi_value sh;
+  if (get_prop(env, argv[2], "shutter", &sh) && !float_list(env, argv[2], "shutter", k.shutter, 2)) {
+    napi_throw_type_error(env, nullptr, "camera shutter must be [t0, t1]");
+    return nullptr;
+  }
   int rc = vt_set_camera(c, slot, &k);
   if (rc) return throw_vt(env, c, rc, "vt_set_camera");
+  return nullptr;
+}
+
+// setInstances(ctx, [{q0, q1, p0, p1} | null, ...]): one entry per body, null = hidden
+napi_value SetInstances(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t len = 0;
+  if (argc < 2 || napi_get_array_length(env, argv[1], &len) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "setInstances(ctx, poses: Array)");
+    return nullptr;
+  }
+  std::vector<vt_instance> v(len ? len : 1);
+  for (uint32_t i = 0; i < len; i++) {
+    napi_value e;
+    CHECK_NAPI(napi_get_element(env, argv[1], i, &e));
+    napi_valuetype t;
+    napi_typeof(env, e, &t);
+    vt_instance &I = v[i];
+    I = vt_instance{};
+    I.q0[3] = I.q1[3] = 1.0f;
+    if (t == napi_null || t == napi_undefined) {
+      I.hidden = 1;
+      continue;
+    }
+    if (!float_list(env, e, "q0", I.q0, 4) || !float_list(env, e, "p0", I.p0, 3)) {
+      napi_throw_type_error(env, nullptr, "pose needs q0[4] and p0[3]");
+      return nullptr;
+    }
+    if (!float_list(env, e, "q1", I.q1, 4)) std::memcpy(I.q1, I.q0, sizeof I.q0);
+    if (!float_list(env, e, "p1", I.p1, 3)) std::memcpy(I.p1, I.p0, sizeof I.p0);
+  }
+  int rc = vt_set_instances(c, v.data(), len);
+  if (rc) return throw_vt(env, c, rc, "vt_set_instances");
   return nullptr;
 }
 
@@ -568,7 +618,7 @@ napi_value Init(napi_env env, napi_value exports) {
     const char *name;
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
-             {"setCamera", SetCamera}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
+             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights}};
   for (auto &f : fns) {

@@ -17,7 +17,7 @@ EXPORTS = [
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
-    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth",
+    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
@@ -32,13 +32,21 @@ class Scene(C.Structure):
     _fields_ = [("nodes", vp), ("node_count", u32), ("tris", vp), ("tri_count", u32), ("verts", vp),
                 ("vert_count", u32), ("meshes", vp), ("mesh_count", u32), ("materials", vp),
                 ("material_count", u32), ("lights", vp), ("light_count", u32), ("emit", vp), ("emit_count", u32),
-                ("emit_area", f32)]
+                ("emit_area", f32), ("tri_body", vp), ("body_count", u32)]
+
+
+MAX_BODIES = 24
+STATIC_BODY = 0xFFFFFFFF
+
+
+class Instance(C.Structure):
+    _fields_ = [("q0", f32 * 4), ("q1", f32 * 4), ("p0", f32 * 3), ("p1", f32 * 3), ("hidden", u32)]
 
 
 class Camera(C.Structure):
     _fields_ = [("R", f32 * 9), ("pos", f32 * 3), ("mode", u32), ("lens", f32 * 9), ("width", u32), ("height", u32),
                 ("spp", u32), ("seed", u32), ("cam_index", u32), ("light_samples", u32), ("max_depth", u32),
-                ("flags", u32), ("scale", f32)]
+                ("flags", u32), ("scale", f32), ("shutter", f32 * 2)]
 
 
 class RenderOpts(C.Structure):
@@ -123,6 +131,7 @@ def load():
         "vt_update_lights": (i32, [vp, vp, u32]),
         "vt_autotune": (i32, [vp, vp, u32, vp, u32, vp]),
         "vt_copy_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
+        "vt_set_instances": (i32, [vp, vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -188,13 +197,32 @@ class Context:
 
     # ---------------------------------------------------------------- scene / cameras
     def upload_scene(self, sc, with_nodes=True):
+        """Upload a scene dict; with sc["tri_body"] (body index per triangle, STATIC_BODY for the static scene)
+        and sc["body_count"] the bodies get BVHs of their own and are placed by set_instances."""
         a = {k: np.ascontiguousarray(sc[k]) for k in ("tris", "verts", "meshes", "materials", "lights", "emit")}
-        nodes = np.ascontiguousarray(sc["nodes"]) if with_nodes else None
+        nb = int(sc.get("body_count", 0))
+        tb = np.ascontiguousarray(sc["tri_body"], dtype=np.uint32) if nb else None
+        nodes = np.ascontiguousarray(sc["nodes"]) if with_nodes and not nb else None
         s = Scene(ptr(nodes), 0 if nodes is None else nodes.size // 16, ptr(a["tris"]), a["tris"].size // 16,
                   ptr(a["verts"]), a["verts"].size // 8, ptr(a["meshes"]), a["meshes"].size // 4,
                   ptr(a["materials"]), a["materials"].size // 16, ptr(a["lights"]), a["lights"].size // 4,
-                  ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]))
+                  ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]), ptr(tb), nb)
         self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
+
+    def set_instances(self, poses):
+        """vt_set_instances: poses[i] = None (hidden) or dict(q0, p0[, q1, p1]) with quaternions (x, y, z, w)."""
+        arr = (Instance * max(1, len(poses)))()
+        for i, P in enumerate(poses):
+            if P is None:
+                arr[i].q0[:] = [0, 0, 0, 1]
+                arr[i].q1[:] = [0, 0, 0, 1]
+                arr[i].hidden = 1
+                continue
+            arr[i].q0[:] = [float(x) for x in P["q0"]]
+            arr[i].q1[:] = [float(x) for x in P.get("q1", P["q0"])]
+            arr[i].p0[:] = [float(x) for x in P["p0"]]
+            arr[i].p1[:] = [float(x) for x in P.get("p1", P["p0"])]
+        self.check(self.L.vt_set_instances(self.h, arr, len(poses)))
 
     def set_camera(self, slot, cam):
         c = Camera()
@@ -205,6 +233,7 @@ class Context:
         for k in ("width", "height", "spp", "seed", "cam_index", "light_samples", "max_depth", "flags"):
             setattr(c, k, int(cam.get(k, 0)))
         c.scale = float(cam.get("scale", 1.0))
+        c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
 
     def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,

@@ -242,11 +242,15 @@ def fresnel_average(eta, n=4096):
     return float(2 * np.sum(w * F * mu) / (3 * n))
 
 
-def pack_scene(P, N, I, tri_mesh, meshes, materials, lights, bvh="vt"):
-    """Upload arrays (reference layout) with the BVH built natively (bvh='vt') or by the oracle (bvh=callable)."""
+def pack_scene(P, N, I, tri_mesh, meshes, materials, lights, bvh="vt", tri_body=None, body_count=0):
+    """Upload arrays (reference layout) with the BVH built natively (bvh='vt') or by the oracle (bvh=callable).
+    With posed bodies (tri_body: body per triangle, STATIC_BODY for the static scene) the triangles keep their
+    order and no BVH is packed: every group's BVH is built at upload."""
     from . import lib
 
-    if bvh == "vt":
+    if tri_body is not None:
+        nodes, order = None, np.arange(I.shape[0])
+    elif bvh == "vt":
         nodes, order = lib.build_bvh(P, I)
     else:
         nodes, order = bvh(P, I)
@@ -290,9 +294,12 @@ def pack_scene(P, N, I, tri_mesh, meshes, materials, lights, bvh="vt"):
         emit_area = float(area.sum())
     else:
         emit_area = 0.0
-    return dict(nodes=nodes, tris=tris.reshape(-1), verts=verts.reshape(-1), meshes=meshes.reshape(-1),
-                materials=np.asarray(materials, np.float32).reshape(-1), lights=np.asarray(lights, np.float32).reshape(-1),
-                emit=emit.reshape(-1), emit_area=emit_area, order=order, pos=P, idx=I, tri_mesh=tri_mesh)
+    out = dict(nodes=nodes, tris=tris.reshape(-1), verts=verts.reshape(-1), meshes=meshes.reshape(-1),
+               materials=np.asarray(materials, np.float32).reshape(-1), lights=np.asarray(lights, np.float32).reshape(-1),
+               emit=emit.reshape(-1), emit_area=emit_area, order=order, pos=P, idx=I, tri_mesh=tri_mesh)
+    if tri_body is not None:
+        out.update(tri_body=np.asarray(tri_body, np.uint32), body_count=int(body_count))
+    return out
 
 
 def kb4_lens(rng, w, h, f):
@@ -339,6 +346,71 @@ def default_sensor(cfa=0, read_noise_dn=2.0, exposure=2e-3, gain=1.0, seed=1006,
 
 
 # ---------------------------------------------------------------------------- the configurations
+
+STATIC_BODY = 0xFFFFFFFF
+
+
+def quat_axis(axis, ang):
+    """Unit quaternion (x, y, z, w) of the rotation by ang about axis."""
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    return np.concatenate([a * math.sin(ang / 2), [math.cos(ang / 2)]])
+
+
+def posed_parts():
+    """The c1 room (dome, a fixed box, four ceiling emitters) and two bodies in their own frames: a box
+    (0.5 x 0.3 x 0.4 m, body 0) and a cylinder (r 0.15 m, 0.6 m tall, body 1), 16 Lambertian materials."""
+    rs, rm, rl = Rng(2001), Rng(2002), Rng(2003)
+    parts = []
+    P, N, I = dome(3.0, 40, 48, pole=True)
+    parts.append((P, N, I, rs.int(0, 15, n=I.shape[0]), None))
+    P, N, I = box((-0.5, -1.0, 0.0), (0.5, 1.0, 0.6))
+    parts.append((P, N, I, rs.int(0, 15, n=I.shape[0]), None))
+    P, N, I = box((-0.25, -0.15, -0.2), (0.25, 0.15, 0.2), g=2)
+    parts.append((P, N, I, rs.int(0, 15, n=I.shape[0]), 0))
+    P, N, I = cylinder(0.15, 0.6, 24, 3)
+    parts.append((P, N, I, rs.int(0, 15, n=I.shape[0]), 1))
+    mats = [material_record(KIND_LAMBERT, rm.uniform(0.2, 0.8, n=3)) for _ in range(16)]
+    mats.append(material_record(KIND_LAMBERT, (0.0, 0.0, 0.0)))
+    meshes = [[m, MESH_STATIC, 0, 0] for m in range(16)]
+    lights = []
+    for k, (x, y) in enumerate(((1.0, 1.0), (-1.0, 1.0), (-1.0, -1.0), (1.0, -1.0))):
+        L = rl.uniform(5.0, 20.0)
+        lights.append([L, L, L, 0.0])
+        meshes.append([16, MESH_EMITTER, k, 0])
+        P, N, I = quad((x, y, 2.4), 0.6, (0.0, 0.0, -1.0))
+        parts.append((P, N, I, np.full(I.shape[0], 16 + k), None))
+    return parts, meshes, mats, lights
+
+
+def posed_scene(parts, meshes, mats, lights, place=None):
+    """Scene of posed_parts: bodies kept in their own frames (place=None, uploaded with tri_body), or every body
+    moved into the world by place[b] = (R, t) as static geometry (the reference scene for a posed one)."""
+    B = Builder()
+    tb = []
+    for P, N, I, M, body in parts:
+        if body is not None and place is not None:
+            R, t = place[body]
+            P = np.asarray(P, float) @ np.asarray(R).T + np.asarray(t)
+            N = np.asarray(N, float) @ np.asarray(R).T
+        B.add(P, N, I, M)
+        tb.append(np.full(np.asarray(I).reshape(-1, 3).shape[0], STATIC_BODY if body is None or place is not None
+                          else body, np.uint32))
+    P, N, I, M = B.arrays()
+    if place is not None:
+        return pack_scene(P, N, I, M, meshes, mats, lights)
+    nb = 1 + max(b for *_, b in parts if b is not None)
+    return pack_scene(P, N, I, M, meshes, mats, lights, tri_body=np.concatenate(tb), body_count=nb)
+
+
+def posed_camera(w=96, h=72, spp=8, depth=3, shutter=(0.0, 1.0)):
+    rc = Rng(2004)
+    lens = kb4_lens(rc, w, h, 45.0 * w / 96)
+    pos = (0.2, -2.4, 1.3)
+    cam = camera(look_at(pos, (0.0, 0.0, 0.5)), pos, lens, w, h, spp, 0, depth, seed=2005)
+    cam["shutter"] = tuple(shutter)
+    return cam
+
 
 def c1(bvh="vt"):
     rs, rm, rl, rc = Rng(1001), Rng(1002), Rng(1003), Rng(1004)

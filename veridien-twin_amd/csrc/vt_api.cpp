@@ -68,7 +68,7 @@ struct vt_ctx {
   uint32_t def_mode = VT_RENDER_BATCHED;
   uint64_t def_wf_batch = 1ull << 24;
   uint32_t def_wf_stack = 8, def_wf_refill = 16;
-  int occ[3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (block 64/128/256, LDS depth)
+  int occ[2][3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (bodies, block 64/128/256, LDS depth)
   // scene box (for the wavefront sort key)
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   // wavefront buffers: paths per batch and shadow slots per path they are sized for
@@ -695,9 +695,10 @@ Shape shape_of(const vt_ctx *c, const vt_render_opts &o) {
 // blocks of a persistent grid: every CU filled to the kernel's occupancy, within the spill slots
 int persistent_grid(vt_ctx *c, const Shape &sh, uint64_t waves, uint32_t &grid) {
   const int bi = sh.block == 64 ? 0 : (sh.block == 128 ? 1 : 2);
-  int &occ = c->occ[bi][sh.stack];
+  const bool inst = c->S.inst_count != 0;
+  int &occ = c->occ[inst][bi][sh.stack];
   if (occ == 0) {
-    VT_HIP(c, render_occupancy(sh.block, sh.stack, &occ));
+    VT_HIP(c, render_occupancy(sh.block, sh.stack, inst, &occ));
     if (occ < 1) occ = 1;
   }
   const uint64_t need = (waves * 64 + sh.block - 1) / sh.block;
@@ -900,7 +901,7 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   if (slds != 4 && slds != 8 && slds != 16) return fail(c, VT_ERR_ARG, "render: wavefront stack_lds must be 4, 8 or 16");
   W.wave_stride = wp.stride;
   int occ = 0;
-  VT_HIP(c, wf_trace_occupancy(slds, &occ));
+  VT_HIP(c, wf_trace_occupancy(slds, c->S.inst_count != 0, &occ));
   const uint32_t grid_cap = (uint32_t)c->cu_count * 8;
   const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
   const uint32_t refill = wp.refill;

@@ -619,7 +619,8 @@ __device__ __forceinline__ void trace_instances(const SceneDev &S, v3 o, v3 d, f
   }
 }
 
-template <bool ANY>
+// INST = false: kernels built for scenes without posed bodies (the instance walk and its registers compiled out)
+template <bool ANY, bool INST>
 __device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time, Hit &h,
                                             Stack &st) {
   bool found = traverse<ANY>(S, S.root, o, d, tmin, tmax, h, st);
@@ -627,7 +628,7 @@ __device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float
     h.inst = NO_INST;
     if (ANY) return true;
   }
-  if (S.inst_count) trace_instances<ANY>(S, o, d, tmin, tmax, time, h, found, st);
+  if (INST && S.inst_count) trace_instances<ANY>(S, o, d, tmin, tmax, time, h, found, st);
   return found;
 }
 

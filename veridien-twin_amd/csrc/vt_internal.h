@@ -44,7 +44,7 @@ struct RenderLaunch {
                                // read by the gradient walk in place of its own first walk (null: walk it)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
-hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu);
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
@@ -85,7 +85,7 @@ struct WfLaunch {
 };
 // ev_ext (optional): 2 * max_depth events, recorded around each extension-trace launch of the batch
 size_t wf_sort_temp_bytes(uint32_t n);
-hipError_t wf_trace_occupancy(uint32_t slds, int *blocks_per_cu);
+hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu);
 // trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
 // fetch of new rays (1..64); slds: LDS stack entries per lane of the trace kernels (4, 8 or 16)
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,

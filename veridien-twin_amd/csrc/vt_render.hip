@@ -30,7 +30,7 @@ __device__ __forceinline__ float ratio(float d, float a) { return a > 0.0f ? d /
 // term's derivative with respect to the lights and the material colours is added to the sink, sampling and
 // MIS weights detached; a continuation factor a = f cos / pdf at vertex k scales everything after it, whose sum
 // is Ltot - (what was accumulated up to k), so dL/dc gets (da / a) * (Ltot - L_acc) there.
-template <bool GRAD>
+template <bool GRAD, bool INST>
 __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x, uint32_t y,
                                         uint32_t i, Stack &st, uint32_t &nrays, GradSink *gs = nullptr,
                                         v3 Ltot = v3{0.f, 0.f, 0.f}) {
@@ -50,7 +50,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
   for (uint32_t b = 0; b < D; b++) {
     Hit h;
     nrays++;
-    if (!trace_scene<false>(S, o, d, 0.0f, INFINITY, time, h, st)) break;
+    if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
@@ -59,7 +59,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
       const float w0 = 1.0f - h.u - h.v;
       nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h.u + ld3(sh[2]) * h.v);
     }
-    if (h.inst != NO_INST) {
+    if (INST && h.inst != NO_INST) {
       const Pose Pb = inst_pose(S, h.inst, time);
       ng = rot(Pb, ng);
       nsi = rot(Pb, nsi);
@@ -134,7 +134,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
         }
         Hit sh;
         nrays++;
-        if (trace_scene<true>(S, so, wi, 0.0f, tmax, time, sh, st)) continue;
+        if (trace_scene<true, INST>(S, so, wi, 0.0f, tmax, time, sh, st)) continue;
         Ln = Ln + contrib;
         if (GRAD) {
           grad_add(gs->light, lid, gl_part);
@@ -173,9 +173,10 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
   return L;
 }
 
+template <bool INST>
 __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
                                             uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
-  return path_walk<false>(S, C, P, x, y, i, st, nrays);
+  return path_walk<false, INST>(S, C, P, x, y, i, st, nrays);
 }
 
 // ------------------------------------------------------------------ camera adjoint (interior term)
@@ -242,6 +243,7 @@ __device__ __forceinline__ void camdir_jac(const DevCam &C, float u, float v, v3
 // random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken
 // on the value. Tangents: origin along e1, e2, direction along e1, e2 (e1, e2 perpendicular to d0). Returns
 // the radiance value; ws gets the four tangents of sum_ch delta_ch L_ch.
+template <bool INST>
 __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t i,
                                             v3 o0, v3 d0, v3 e1, v3 e2, v3 delta, Stack &st, uint32_t &nrays,
                                             float ws[ND]) {
@@ -263,14 +265,14 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
   for (uint32_t b = 0; b < Dp; b++) {
     Hit h;
     nrays++;
-    if (!trace_scene<false>(S, val(o), val(d), 0.0f, INFINITY, time, h, st)) break;
+    if (!trace_scene<false, INST>(S, val(o), val(d), 0.0f, INFINITY, time, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
     const float4 *tp = S.tri + 3 * (size_t)h.tri;
     v3 p0 = ld3(tp[0]), e1 = ld3(tp[1]) - p0, e2 = ld3(tp[2]) - p0;
     v3 nl0 = ld3(n0), nl1 = ld3(sh[1]), nl2 = ld3(sh[2]), ngl = ld3(sh[3]);
-    if (h.inst != NO_INST) {
+    if (INST && h.inst != NO_INST) {
       // a posed body: its triangle and normals in the world at the sample's time (the pose does not depend on the
       // camera)
       const Pose Pb = inst_pose(S, h.inst, time);
@@ -353,7 +355,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
         const v3 so = val(p) + ng * (dot(ng, wiv) > 0.0f ? EPS_OFF : -EPS_OFF);
         Hit sh;
         nrays++;
-        if (trace_scene<true>(S, so, wiv, 0.0f, dist.v - 2e-5f, time, sh, st)) continue;
+        if (trace_scene<true, INST>(S, so, wiv, 0.0f, dist.v - 2e-5f, time, sh, st)) continue;
         Ln = Ln + contrib;
       }
     }
@@ -396,6 +398,7 @@ __device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
 
 // one 64-item wave of the queue
 // nrays_acc / nsamp_acc: the lane's running counts, added to the launch counters once when the wave leaves
+template <bool INST>
 __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, Stack &st, uint32_t &nrays_acc,
                                          uint32_t &nsamp_acc) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -416,7 +419,7 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
   if (valid) {
     PixSeeds P;
     pix_seeds(C, x, y, P);
-    L = path_radiance(Lc.S, C, P, x, y, i, st, nrays);
+    L = path_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
   }
   // per-pixel sums: aligned power-of-two runs reduce by butterfly; other run shapes add atomically
   const bool pow2 = (nspp & (nspp - 1)) == 0 && nspp <= 64;
@@ -474,6 +477,7 @@ __device__ __forceinline__ WaveItem wave_item(const RenderLaunch &Lc, uint64_t w
 }
 
 // backward wave: replay each sample's path twice (its value, then the gradient walk)
+template <bool INST>
 __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wave, Stack &st, GradSink &gs) {
   const uint32_t lane = threadIdx.x & 63u;
   const WaveItem it = wave_item(Lc, wave, lane);
@@ -491,9 +495,9 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
         const float4 l4 = Lc.ltot[wave * 64 + lane];
         Ltot = mk(l4.x, l4.y, l4.z);
       } else {
-        Ltot = path_walk<false>(Lc.S, C, P, it.x, it.y, it.i, st, nrays);
+        Ltot = path_walk<false, INST>(Lc.S, C, P, it.x, it.y, it.i, st, nrays);
       }
-      path_walk<true>(Lc.S, C, P, it.x, it.y, it.i, st, nrays, &gs, Ltot);
+      path_walk<true, INST>(Lc.S, C, P, it.x, it.y, it.i, st, nrays, &gs, Ltot);
     }
   }
   uint32_t rs = nrays;
@@ -503,6 +507,7 @@ __device__ __forceinline__ void run_wave_bwd(const RenderLaunch &Lc, uint64_t wa
 
 // camera wave: the tangent walk of each sample, contracted with the ray's derivative with respect to its
 // camera's intrinsics and pose; the wave's sums go to scam[camera * CAM_PARAMS + k]
+template <bool INST>
 __device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wave, Stack &st, float *scam) {
   const uint32_t lane = threadIdx.x & 63u;
   const WaveItem it = wave_item(Lc, wave, lane);
@@ -523,7 +528,7 @@ __device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wa
       v3 e1, e2;
       basis(d0, e1, e2);
       float ws[ND];
-      const v3 Lv = path_walk_tan(Lc.S, C, P, it.i, o0, d0, e1, e2, delta, st, nrays, ws);
+      const v3 Lv = path_walk_tan<INST>(Lc.S, C, P, it.i, o0, d0, e1, e2, delta, st, nrays, ws);
       if (Lc.ltot) Lc.ltot[wave * 64 + lane] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
       const v3 wo = e1 * ws[0] + e2 * ws[1];
       const v3 wd = e1 * ws[2] + e2 * ws[3];
@@ -552,7 +557,7 @@ __device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wa
   if (lane == 0) atomicAdd(&Lc.counters[1], (unsigned long long)rs);
 }
 
-template <int BLOCK>
+template <int BLOCK, bool INST>
 __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
                                                     uint32_t n_light, uint32_t use_lds) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
     base = __shfl(base, 0, 64);
     if (base >= Lc.total_waves) break;
     const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
-    for (uint64_t w = base; w < end; w++) run_wave_bwd(Lc, w, st, gs);
+    for (uint64_t w = base; w < end; w++) run_wave_bwd<INST>(Lc, w, st, gs);
   }
   __syncthreads();
   if (use_lds) {
@@ -583,7 +588,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat
   }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool INST>
 __global__ __launch_bounds__(BLOCK, 2) void k_camgrad(RenderLaunch Lc, float *gcam) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
   __shared__ float s_t[STACK_LDS * BLOCK];
@@ -598,7 +603,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_camgrad(RenderLaunch Lc, float *gc
     base = __shfl(base, 0, 64);
     if (base >= Lc.total_waves) break;
     const uint64_t end = base + Lc.chunk < Lc.total_waves ? base + Lc.chunk : Lc.total_waves;
-    for (uint64_t w = base; w < end; w++) run_wave_cam(Lc, w, st, scam);
+    for (uint64_t w = base; w < end; w++) run_wave_cam<INST>(Lc, w, st, scam);
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK)
@@ -609,18 +614,25 @@ hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, ui
                            hipStream_t s) {
   const size_t lds = (size_t)3 * (n_mat + n_light) * sizeof(float);
   const uint32_t use_lds = lds <= 16384 ? 1u : 0u;
-  hipLaunchKernelGGL(k_backward<256>, dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight, n_mat,
-                     n_light, use_lds);
+  if (L.S.inst_count)
+    hipLaunchKernelGGL((k_backward<256, true>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight, n_mat,
+                       n_light, use_lds);
+  else
+    hipLaunchKernelGGL((k_backward<256, false>), dim3(L.grid), dim3(256), use_lds ? lds : 0, s, L, gmat, glight,
+                       n_mat, n_light, use_lds);
   return hipGetLastError();
 }
 
 hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s) {
   if (L.ncam > MAX_CAMS) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_camgrad<256>, dim3(L.grid), dim3(256), 0, s, L, gcam);
+  if (L.S.inst_count)
+    hipLaunchKernelGGL((k_camgrad<256, true>), dim3(L.grid), dim3(256), 0, s, L, gcam);
+  else
+    hipLaunchKernelGGL((k_camgrad<256, false>), dim3(L.grid), dim3(256), 0, s, L, gcam);
   return hipGetLastError();
 }
 
-template <int BLOCK>
+template <int BLOCK, bool INST>
 __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
   // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
   extern __shared__ uint32_t s_stack[];
@@ -654,14 +666,14 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
         if (base >= q1 - q0) break;
         const uint64_t b = q0 + base;
         const uint64_t end = b + Lc.chunk < q1 ? b + Lc.chunk : q1;
-        for (uint64_t w = b; w < end; w++) run_wave(Lc, w * Lc.wave_stride, st, nrays, nsamp);
+        for (uint64_t w = b; w < end; w++) run_wave<INST>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
       }
     }
   } else {
     // static schedule: the waves of the grid stride over the queue
     const uint64_t stride = (uint64_t)gridDim.x * (BLOCK / 64);
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride)
-      run_wave(Lc, w * Lc.wave_stride, st, nrays, nsamp);
+      run_wave<INST>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
   }
   flush_counts(Lc, nrays, nsamp);
 }
@@ -716,27 +728,38 @@ hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+namespace {
+template <int BLOCK>
+void (*render_kernel(bool inst))(RenderLaunch) {
+  return inst ? k_render<BLOCK, true> : k_render<BLOCK, false>;
+}
+}  // namespace
+
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
   if (L.stack_lds < STACK_LDS_MIN || L.stack_lds > STACK_LDS_MAX || L.wave_stride == 0 ||
       (L.queues != 1 && L.queues != XCD_COUNT) || L.chunk == 0)
     return hipErrorInvalidValue;
   const size_t lds = (size_t)L.stack_lds * L.block * 8;
+  const bool inst = L.S.inst_count != 0;
   switch (L.block) {
-    case 64: hipLaunchKernelGGL(k_render<64>, dim3(L.grid), dim3(64), lds, s, L); break;
-    case 128: hipLaunchKernelGGL(k_render<128>, dim3(L.grid), dim3(128), lds, s, L); break;
-    case 256: hipLaunchKernelGGL(k_render<256>, dim3(L.grid), dim3(256), lds, s, L); break;
+    case 64: hipLaunchKernelGGL(render_kernel<64>(inst), dim3(L.grid), dim3(64), lds, s, L); break;
+    case 128: hipLaunchKernelGGL(render_kernel<128>(inst), dim3(L.grid), dim3(128), lds, s, L); break;
+    case 256: hipLaunchKernelGGL(render_kernel<256>(inst), dim3(L.grid), dim3(256), lds, s, L); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 // resident blocks per CU of the render kernel at this block size and LDS stack depth
-hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, int *blocks_per_cu) {
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, int *blocks_per_cu) {
   const size_t lds = (size_t)stack_lds * block * 8;
   switch (block) {
-    case 64: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<64>, 64, lds);
-    case 128: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<128>, 128, lds);
-    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_render<256>, 256, lds);
+    case 64:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<64>(inst), 64, lds);
+    case 128:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<128>(inst), 128, lds);
+    case 256:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<256>(inst), 256, lds);
     default: return hipErrorInvalidValue;
   }
 }

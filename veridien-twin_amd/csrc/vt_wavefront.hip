@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
 // result into hit[slot]; ANY = true: shadow slots (count[0] * nl_max, head count[3]), visibility into sh_c.w.
 // SLDS: stack entries per lane kept in LDS (the rest of the STACK_TOTAL spill to global memory); a shallow
 // LDS stack lets more waves share a CU, which hides the node fetches better than the deeper stack saves.
-template <bool ANY, int SLDS>
+template <bool ANY, int SLDS, bool INST>
 __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
   __shared__ uint32_t s_node[SLDS * 256];
   __shared__ float s_t[SLDS * 256];
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
         if (done) {
           h.inst = NO_INST;
           // then the posed bodies (few, small: walked one after the other with the lane's own stack)
-          if (W.S.inst_count && !(ANY && found)) trace_instances<ANY>(W.S, R.o, dir, 0.0f, tmax0, time, h, found, st);
+          if (INST && W.S.inst_count && !(ANY && found)) trace_instances<ANY>(W.S, R.o, dir, 0.0f, tmax0, time, h, found, st);
           if (ANY) {
             if (!found) W.sh_c[k].w = 1.0f;
           } else {
@@ -274,6 +274,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
 }
 
 // one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
+template <bool INST>
 __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
   // every slot of the batch gets a key: live paths of this vertex theirs, the rest (and the slots past the
   // live count) the dead key, so the sort of all n slots puts exactly the next vertex's paths first
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
           const float w0 = 1.0f - h4.y - h4.z;
           nsi = normalize(ld3(n0) * w0 + ld3(sh[1]) * h4.y + ld3(sh[2]) * h4.z);
         }
-        if (inst != NO_INST) {
+        if (INST && inst != NO_INST) {
           const Pose Pb = inst_pose(W.S, inst, time);
           ng = rot(Pb, ng);
           nsi = rot(Pb, nsi);
@@ -506,13 +507,25 @@ uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 
 }  // namespace
 
-hipError_t wf_trace_occupancy(uint32_t slds, int *blocks_per_cu) {
+namespace {
+// the trace kernel for an LDS stack depth, with or without the posed-body walk
+template <bool ANY, bool INST>
+void (*trace_kernel(uint32_t slds))(WfLaunch, uint32_t) {
   switch (slds) {
-    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 4>, 256, 0);
-    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 8>, 256, 0);
-    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_wf_trace_dyn<false, 16>, 256, 0);
-    default: return hipErrorInvalidValue;
+    case 4: return k_wf_trace_dyn<ANY, 4, INST>;
+    case 8: return k_wf_trace_dyn<ANY, 8, INST>;
+    default: return k_wf_trace_dyn<ANY, 16, INST>;
   }
+}
+template <bool ANY>
+void (*trace_kernel(uint32_t slds, bool inst))(WfLaunch, uint32_t) {
+  return inst ? trace_kernel<ANY, true>(slds) : trace_kernel<ANY, false>(slds);
+}
+}  // namespace
+
+hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu) {
+  if (slds != 4 && slds != 8 && slds != 16) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<false>(slds, inst), 256, 0);
 }
 
 size_t wf_sort_temp_bytes(uint32_t n) {
@@ -530,6 +543,9 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
   const uint32_t gs = grid_for((uint64_t)W.n * W.nl_max, 256, trace_grid_cap);
+  const bool inst = W.S.inst_count != 0;
+  void (*trace_ext)(WfLaunch, uint32_t) = trace_kernel<false>(slds, inst);
+  void (*trace_shadow)(WfLaunch, uint32_t) = trace_kernel<true>(slds, inst);
   hipLaunchKernelGGL(k_wf_gen, dim3(g), dim3(256), 0, s, W);
   W.q = nullptr;   // vertex 1: the camera rays in item order (coherent already)
   for (uint32_t b = 0; b < max_depth; b++) {
@@ -538,21 +554,14 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
       hipError_t e = hipEventRecord(ev_ext[2 * b], s);
       if (e != hipSuccess) return e;
     }
-    switch (slds) {
-      case 4: hipLaunchKernelGGL((k_wf_trace_dyn<false, 4>), dim3(gt), dim3(256), 0, s, W, refill); break;
-      case 8: hipLaunchKernelGGL((k_wf_trace_dyn<false, 8>), dim3(gt), dim3(256), 0, s, W, refill); break;
-      default: hipLaunchKernelGGL((k_wf_trace_dyn<false, 16>), dim3(gt), dim3(256), 0, s, W, refill); break;
-    }
+    hipLaunchKernelGGL(trace_ext, dim3(gt), dim3(256), 0, s, W, refill);
     if (ev_ext) {
       hipError_t e = hipEventRecord(ev_ext[2 * b + 1], s);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_wf_shade, dim3(g), dim3(256), 0, s, W);
-    switch (slds) {
-      case 4: hipLaunchKernelGGL((k_wf_trace_dyn<true, 4>), dim3(gs), dim3(256), 0, s, W, refill); break;
-      case 8: hipLaunchKernelGGL((k_wf_trace_dyn<true, 8>), dim3(gs), dim3(256), 0, s, W, refill); break;
-      default: hipLaunchKernelGGL((k_wf_trace_dyn<true, 16>), dim3(gs), dim3(256), 0, s, W, refill); break;
-    }
+    if (inst) hipLaunchKernelGGL(k_wf_shade<true>, dim3(g), dim3(256), 0, s, W);
+    else hipLaunchKernelGGL(k_wf_shade<false>, dim3(g), dim3(256), 0, s, W);
+    hipLaunchKernelGGL(trace_shadow, dim3(gs), dim3(256), 0, s, W, refill);
     hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
     if (b + 1 == max_depth) break;
     size_t tb = sort_temp_bytes;

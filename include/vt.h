@@ -70,6 +70,8 @@ typedef struct {
    * bodies, nodes must be NULL (every BVH is built here); every body and the static scene need triangles, and
    * emitter triangles stay static. tri_body may be NULL when body_count is 0. */
   const uint32_t *tri_body; uint32_t body_count;
+  uint32_t shadow_skip;  /* bit i: shadow rays ignore body i (a light's own housing; SceneU skip mask,
+                            gpu/renderer.js:649) */
 } vt_scene;
 
 #define VT_MAX_BODIES 24
@@ -96,6 +98,8 @@ typedef struct {
   float scale;           /* output scale (CamU.expo.z), usually 1 */
   float shutter[2];      /* exposure interval on the pose timeline (CamU.expo.xy), within [0, 1]: sub-sample i
                             is at time shutter[0] + (shutter[1] - shutter[0]) frac(s2 + i / golden ratio) */
+  uint32_t skip_bodies;  /* bit i: this camera's own rays (from its centre) ignore body i, e.g. its lens front
+                            (_skipMask, gpu/renderer.js:881-886) */
 } vt_camera;
 
 enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2, VT_RENDER_AUTO = 3 };
@@ -240,6 +244,12 @@ typedef struct {
 } vt_grads;
 int vt_backward(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const float *const *adj, const vt_render_opts *opts,
                 vt_grads *out);
+
+/* visible (gpu/renderer.js:1238-1258): out[k] = 1 when no surface lies between camera cam's centre and tol short of
+ * world point k (points: n x 3 floats), over the static scene and the posed bodies at exposure time `time` in
+ * [0, 1], the camera's skip_bodies ignored; a point within tol of the centre is visible. The reference walks its
+ * BVH on the CPU; here one any-hit launch answers all n. Blocks until done. */
+int vt_visible(vt_ctx *ctx, uint32_t cam, const float *points, uint32_t n, float tol, float time, uint8_t *out);
 
 /* Copy bandwidth: y = x over two `bytes`-sized device buffers, `iters` times, read + written bytes per second
  * (GB/s). With bytes well under 256 MB the working set stays in the Infinity Cache; the roofline of the trace

@@ -69,6 +69,7 @@ def lib():
             "or_raw_noise_dn": (f64, [f64, f64, f64, f64]),
             "or_render_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
             "or_camera_ray": (None, [vp, u32, u32, u32, vp, vp]),
+            "or_visible": (None, [vp, vp, vp, u32, f64, f64, vp]),
             "or_jacobi_dense": (i32, [vp, vp, u32, u32, f64, i32, vp, vp]),
             "or_jacobi_gather": (None, [u32, vp, vp, u32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
             "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
@@ -123,7 +124,8 @@ class Scene(C.Structure):
                 ("verts", C.c_void_p), ("vert_count", C.c_uint32), ("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
                 ("materials", C.c_void_p), ("mat_count", C.c_uint32), ("lights", C.c_void_p),
                 ("light_count", C.c_uint32), ("emit", C.c_void_p), ("emit_count", C.c_uint32),
-                ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32)]
+                ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32),
+                ("shadow_skip", C.c_uint32)]
 
 
 class Body(C.Structure):
@@ -135,7 +137,7 @@ class Camera(C.Structure):
     _fields_ = [("R", C.c_float * 9), ("pos", C.c_float * 3), ("mode", C.c_uint32), ("lens", C.c_float * 9),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32),
                 ("cam_index", C.c_uint32), ("light_samples", C.c_uint32), ("max_depth", C.c_uint32),
-                ("flags", C.c_uint32), ("shutter", C.c_float * 2)]
+                ("flags", C.c_uint32), ("shutter", C.c_float * 2), ("skip_bodies", C.c_uint32)]
 
 
 # ---------------------------------------------------------------------------- helpers
@@ -226,7 +228,7 @@ def scene_struct(sc):
               ptr(keep["verts"]), keep["verts"].size // 8, ptr(keep["meshes"]), keep["meshes"].size // 4,
               ptr(keep["materials"]), keep["materials"].size // 16, ptr(keep["lights"]), keep["lights"].size // 4,
               ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]),
-              C.cast(bodies, C.c_void_p) if nb else None, nb)
+              C.cast(bodies, C.c_void_p) if nb else None, nb, int(sc.get("shadow_skip", 0)) if nb else 0)
     s._keep = keep
     return s
 
@@ -240,6 +242,7 @@ def camera_struct(cam):
     for k in ("width", "height", "spp", "seed", "cam_index", "light_samples", "max_depth", "flags"):
         setattr(c, k, int(cam.get(k, 0)))
     c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
+    c.skip_bodies = int(cam.get("skip_bodies", 0))
     return c
 
 
@@ -253,6 +256,16 @@ def render_tile(sc, cam, x0, y0, w, h, spp_begin=0, spp_end=None, nthreads=8):
     lib().or_render_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(out), C.byref(rays), ptr(vis),
                          nthreads)
     return out, int(rays.value), vis
+
+
+def visible(sc, cam, points, tol=2e-4, time=0.5):
+    """FP64 visible query (gpu/renderer.js:1238-1258) of world points from the camera centre."""
+    S = scene_struct(sc)
+    Cm = camera_struct(cam)
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(max(1, pts.shape[0]), np.uint8)
+    lib().or_visible(C.byref(S), C.byref(Cm), ptr(pts), pts.shape[0], float(tol), float(time), ptr(out))
+    return out[: pts.shape[0]].astype(bool)
 
 
 def camera_ray(cam, x, y, i):

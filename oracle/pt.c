@@ -118,17 +118,17 @@ static void rot_apply_t(const double R[9], const double v[3], double out[3]) {
   for (int k = 0; k < 3; k++) out[k] = R[3 * k] * v[0] + R[3 * k + 1] * v[1] + R[3 * k + 2] * v[2];
 }
 
-/* traceSceneSkip (trace.wgsl.js:144-164) without its culling tests (box and capsule only skip work): the static
+/* traceSceneSkip (trace.wgsl.js:144-164; skip: bit i = ignore body i) without its culling tests (box and capsule only skip work): the static
  * BVH, then every posed body in its own frame at time t; a rigid pose keeps ray lengths, so t carries over.
  * h->inst names the body of the hit (-1: static); *P receives its pose. Visits add up over every BVH walked. */
 static void trace_scene(const or_scene *S, const double o[3], const double d[3], double tmin, double tmax, int any,
-                        double time, or_hit *h, pose_t *P) {
+                        double time, uint32_t skip, or_hit *h, pose_t *P) {
   or_traverse(S->nodes, S->tris, o, d, tmin, tmax, any, 0, h);
   h->inst = -1;
   if (any && h->tri >= 0) return;
   for (uint32_t b = 0; b < S->body_count; b++) {
     const or_body *B = &S->bodies[b];
-    if (B->hidden) continue;
+    if (B->hidden || ((skip >> b) & 1u)) continue;
     pose_t Q;
     body_pose(B, time, &Q);
     double ol[3], dl[3], rel[3] = {o[0] - Q.p[0], o[1] - Q.p[1], o[2] - Q.p[2]};
@@ -266,7 +266,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
   const double A = S->emit_area;
   for (uint32_t b = 0; b < D; b++) {
     or_hit h;
-    trace_scene(S, o, d, 0, INFINITY, 0, time, &h, &pose);
+    trace_scene(S, o, d, 0, INFINITY, 0, time, b == 0 ? C->skip_bodies : 0u, &h, &pose);
     (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
     visits[2]++; visits[3] += h.node_visits; visits[4] += h.tri_tests;
     seq_add(seq, h.tri);
@@ -322,7 +322,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double so[3];
       offset_ray(s.p, s.ng, wi, so);
       or_hit sh;
-      trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, &sh, NULL);
+      trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, S->shadow_skip, &sh, NULL);
       (*rays)++; visits[0] += sh.node_visits; visits[1] += sh.tri_tests;
       seq_add(seq, sh.tri >= 0 ? 0 : -1);
       if (sh.tri >= 0) continue;
@@ -535,4 +535,19 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
     for (int k = 0; k < 14; k++) grad[k] += g[k];
   }
   if (changed) *changed = nch;
+}
+
+void or_visible(const or_scene *S, const or_camera *C, const float *pts, uint32_t n, double tol, double time,
+                uint8_t *vis) {
+  const double c[3] = {C->pos[0], C->pos[1], C->pos[2]};
+  for (uint32_t k = 0; k < n; k++) {
+    double v[3] = {pts[3 * (size_t)k] - c[0], pts[3 * (size_t)k + 1] - c[1], pts[3 * (size_t)k + 2] - c[2]};
+    const double dist = sqrt(dot3(v, v));
+    vis[k] = 1;
+    if (!(dist > tol)) continue;
+    for (int j = 0; j < 3; j++) v[j] /= dist;
+    or_hit h;
+    trace_scene(S, c, v, 0, dist - tol, 1, time, C->skip_bodies, &h, NULL);
+    vis[k] = h.tri < 0;
+  }
 }

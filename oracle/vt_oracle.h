@@ -140,6 +140,7 @@ typedef struct {
   const float *emit; uint32_t emit_count;      /* 16 floats per emitter triangle: v0|cdf, e1|area, e2|light, n|- */
   float emit_area;
   const struct or_body *bodies; uint32_t body_count;  /* posed bodies (may be 0) */
+  uint32_t shadow_skip;                                /* bit i: shadow rays ignore body i */
 } or_scene;
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose
@@ -156,6 +157,7 @@ typedef struct {
   uint32_t width, height, spp, seed, cam_index, light_samples, max_depth, flags;
   float shutter[2];      /* exposure interval on the pose timeline: sub-sample i is at
                             shutter[0] + (shutter[1] - shutter[0]) kr(i, 1/golden ratio, s2) */
+  uint32_t skip_bodies;  /* bit i: rays from the camera centre ignore body i (_skipMask, gpu/renderer.js:881-886) */
 } or_camera;
 /* radiance of pixels [x0, x0+w) x [y0, y0+h) of a camera, spp sub-samples each, mean written as rgba
  * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays;
@@ -175,6 +177,10 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
                          uint32_t spp_begin, uint32_t spp_end, const float *adj, double rel_step, double *grad,
                          uint64_t *changed, int nthreads);
 /* camera ray of sub-sample i of pixel (x, y): o, d (FP32 sample pattern, FP64 unproject, rounded to FP32) */
+/* visible (gpu/renderer.js:1238-1258): vis[k] = 1 when no surface lies between the camera centre and tol short of
+ * point k (n x 3 floats), the posed bodies at exposure time `time` and the camera's skip_bodies ignored */
+void or_visible(const or_scene *S, const or_camera *C, const float *pts, uint32_t n, double tol, double time,
+                uint8_t *vis);
 void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float o[3], float d[3]);
 
 /* ---------------------------------------------------------------- radiosity */

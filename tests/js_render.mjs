@@ -24,6 +24,10 @@ async function main() {
     const raw = await r.readback(c.name, 'raw');
     fs.writeFileSync(path.join(dir, c.name + '.raw.bin'), Buffer.from(raw.buffer, raw.byteOffset, raw.byteLength));
   }
+  if (man.visiblePoints) {
+    const vis = r.visible(man.cameras[0].name, man.visiblePoints, { tol: 2e-4, time: 0.5 });
+    fs.writeFileSync(path.join(dir, 'visible.json'), JSON.stringify(vis));
+  }
   const t = await r.timings();
   console.log(JSON.stringify({ rays: t.last.rays, overflows: t.last.stackOverflows }));
   r.destroy();

@@ -198,3 +198,96 @@ def test_gpu_material_adjoint_with_bodies(ctx, parts):
         np.max(np.abs(gm - om)) / np.max(np.abs(om)), np.max(np.abs(gl - ol)) / np.max(np.abs(ol))))
     assert np.allclose(gl, ol, rtol=1e-3, atol=1e-4 * np.max(np.abs(ol)))
     assert np.allclose(gm, om, rtol=1e-3, atol=1e-4 * np.max(np.abs(om)))
+
+
+# ---------------------------------------------------------------------------- skip masks and visible()
+
+def _brute_blocked(sc, c, pts, tol):
+    """FP64 Moller-Trumbore over every triangle of a static scene: is the segment from c to tol short of each point
+    blocked? (the definition visible() answers, with no BVH)."""
+    T = np.asarray(sc["tris"], np.float32).reshape(-1, 16).astype(np.float64)
+    v0, e1, e2 = T[:, 0:3], T[:, 4:7], T[:, 8:11]
+    out = []
+    for p in np.asarray(pts, np.float64):
+        v = p - c
+        dist = np.linalg.norm(v)
+        if not dist > tol:
+            out.append(False)
+            continue
+        d = v / dist
+        pv = np.cross(d, e2)
+        det = np.einsum("ij,ij->i", e1, pv)
+        ok = np.abs(det) > 1e-30
+        inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+        s = c - v0
+        u = np.einsum("ij,ij->i", s, pv) * inv
+        q = np.cross(s, e1)
+        w = (q @ d) * inv
+        t = np.einsum("ij,ij->i", e2, q) * inv
+        hit = ok & (u >= 0) & (w >= 0) & (u + w <= 1) & (t > 0) & (t < dist - tol)
+        out.append(bool(hit.any()))
+    return np.array(out)
+
+
+def _probe_points(n=400, seed=7):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([rng.uniform(-1.6, 1.6, n), rng.uniform(-2.2, 1.2, n), rng.uniform(0.0, 1.2, n)])
+
+
+@pytest.mark.parametrize("skip", [0, 1, 2])
+def test_oracle_visible_is_brute_force(parts, skip):
+    """visible() with moving bodies at time 0.5 and the camera's skip mask, against brute force over the scene placed
+    at that time with the skipped bodies left out."""
+    cam = dict(scenes.posed_camera(), skip_bodies=skip)
+    pts = _probe_points()
+    got = oracle.visible(_posed(parts, _poses(True)), cam, pts, tol=2e-4, time=0.5)
+    drop = {0: None, 1: 0, 2: 1}[skip]
+    ref = ~_brute_blocked(_placed(parts, 0.5, moving=True, drop=drop), np.asarray(cam["pos"], np.float64), pts, 2e-4)
+    print("skip %d: %d of %d points visible, %d disagree" % (skip, got.sum(), got.size, np.sum(got != ref)))
+    assert 0.2 < got.mean() < 0.95
+    assert np.sum(got != ref) <= 2       # float32 scene data on grazing segments
+
+
+def test_oracle_skip_masks_remove_the_body_from_those_rays(parts):
+    """Depth 1 (camera rays and their shadow rays only): a body skipped by the camera and by the shadow rays is gone
+    from the image."""
+    cam = dict(scenes.posed_camera(48, 36, spp=4, depth=1), skip_bodies=1)
+    sc = _posed(parts, _poses(False))
+    sc["shadow_skip"] = 1
+    a, _, _ = oracle.render_tile(sc, cam, 0, 0, 48, 36)
+    b, _, _ = oracle.render_tile(_placed(parts, drop=0), cam, 0, 0, 48, 36)
+    rel, close = _compare(a, b, 0)
+    assert rel < 1e-6 and close > 0.999
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["persistent", "wavefront"])
+def test_gpu_skip_masks_match_oracle(ctx, parts, mode):
+    from vtwin import lib
+    poses = _poses(True)
+    sc = _posed(parts, poses)
+    sc["shadow_skip"] = 2
+    cam = dict(scenes.posed_camera(96, 72, spp=8), skip_bodies=1)
+    print("skip masks: mode %s" % mode, flush=True)
+    img, _ = _gpu(ctx, sc, cam, poses, mode=lib.VT_RENDER_WAVEFRONT if mode == "wavefront" else lib.VT_RENDER_BATCHED)
+    ref, _, _ = oracle.render_tile(sc, cam, 0, 0, 96, 72)
+    rel, close = _compare(img, ref, 1e-3)
+    print("GPU vs oracle with skip masks: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("skip", [0, 1])
+def test_gpu_visible_matches_oracle(ctx, parts, skip):
+    poses = _poses(True)
+    sc = _posed(parts, poses)
+    cam = dict(scenes.posed_camera(), skip_bodies=skip)
+    ctx.upload_scene(sc)
+    ctx.set_instances(poses)
+    ctx.set_camera(0, cam)
+    pts = _probe_points(2000, seed=9)
+    for t in (0.0, 0.5, 1.0):
+        got = ctx.visible(0, pts, tol=2e-4, time=t)
+        ref = oracle.visible(sc, cam, pts, tol=2e-4, time=t)
+        print("visible t=%.1f skip %d: %d visible, %d disagree" % (t, skip, ref.sum(), np.sum(got != ref)))
+        assert np.sum(got != ref) <= 4

@@ -171,12 +171,14 @@ def test_from_prepared_bodies():
     group's BVH is built at upload), and the bodies keep their keys and kinds."""
     S = ("{nodes: new Float32Array(16), tris: new Float32Array(16 * 5), verts: new Float32Array(24), "
          "meshTable: [{mat: 0, flags: 1}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, roughness: 0.5}], "
-         "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2}, {body: 'base', kind: 'fixed', triOff: 4, triCount: 1}]}")
+         "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2, owner: 'CAM2'}, {body: 'scope', kind: 'fixed', triOff: 4, "
+         "triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true}}")
     out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({tb: Array.from(u.triBody), "
-                "nodes: u.nodes === undefined, bodies: u.bodies})); })" % (MODULE, S))
+                "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip})); })" % (MODULE, S))
     d = json.loads(out)
     assert d["tb"] == [0xFFFFFFFF, 0, 0, 0xFFFFFFFF, 1] and d["nodes"]
-    assert d["bodies"] == [{"key": "arm", "kind": "body"}, {"key": "base", "kind": "fixed"}]
+    assert d["bodies"] == [{"key": "arm", "kind": "body", "owner": "CAM2"}, {"key": "scope", "kind": "fixed", "owner": None}]
+    assert d["shadowSkip"] == 2          # the scope's bit: shadow rays leave through its housing
 
 
 @pytest.mark.gpu
@@ -187,6 +189,9 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
     parts = scenes.posed_parts()
     sc = scenes.posed_scene(*parts)
     cam = scenes.posed_camera(96, 72, spp=4)
+    cam["skip_bodies"] = 1        # the camera owns body 0 ("box"): its own rays ignore it
+    rng = np.random.default_rng(3)
+    pts = np.column_stack([rng.uniform(-1.6, 1.6, 200), rng.uniform(-2.2, 1.2, 200), rng.uniform(0.0, 1.2, 200)])
     for k in ("tris", "verts", "materials", "lights", "emit"):
         np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
     np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
@@ -203,7 +208,8 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
                          "can": T(scenes.rot_axis(x, 0.4), (-0.75, -1.35, 0.45))}}
     sp = scenes.default_sensor()
     man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"],
-               bodies=["box", "can"], poses0=poses0, poses1=poses1, shutter=[0.0, 1.0],
+               bodies=[{"key": "box", "kind": "body", "owner": "CAM1"}, "can"], poses0=poses0, poses1=poses1,
+               shutter=[0.0, 1.0], visiblePoints=[[float(v) for v in p] for p in pts],
                cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
                              pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
                sensor=dict(m=[float(v) for v in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
@@ -222,6 +228,9 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
     ctx.set_camera(0, cam)
     ctx.render([0])
     lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    vis = ctx.visible(0, pts, tol=2e-4, time=0.5)
     ctx.close()
     a, b = js_lin[:, :, :3].astype(np.float64), lin[:, :, :3].astype(np.float64)
     assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    js_vis = np.array(json.loads((tmp_path / "visible.json").read_text()))
+    assert np.array_equal(js_vis, vis) and 0 < vis.sum() < vis.size

@@ -510,6 +510,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->body_root = body_root;
   c->body_box = body_box;
   c->S.inst_count = s->body_count;
+  c->S.shadow_skip = s->body_count ? s->shadow_skip : 0u;
   if (s->body_count) {
     // every body starts hidden until vt_set_instances gives it a pose
     std::vector<float4> rec((size_t)INST_WORDS * s->body_count, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -747,6 +748,7 @@ int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opt
     d.scale = k.scale > 0 ? k.scale : 1.0f;
     d.shutter0 = k.shutter[0];
     d.shutter1 = k.shutter[1];
+    d.skip = k.skip_bodies;
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -1348,6 +1350,7 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
     d.scale = k.scale > 0 ? k.scale : 1.0f;
     d.shutter0 = k.shutter[0];
     d.shutter1 = k.shutter[1];
+    d.skip = k.skip_bodies;
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -1440,6 +1443,30 @@ extern "C" int vt_trace_test(vt_ctx *c, const float *rays, uint32_t n, int any_h
   VT_HIP(c, launch_trace_test(T, (const float *)c->test_in.p, n, any_hit, (float *)c->test_out.p, c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(hits, c->test_out.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_visible(vt_ctx *c, uint32_t cam, const float *points, uint32_t n, float tol, float time,
+                          uint8_t *out) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "visible: no scene");
+  if (cam >= VT_MAX_CAMERAS || !c->cams[cam].set) return fail(c, VT_ERR_STATE, "visible: camera %u not set", cam);
+  if (n && (!points || !out)) return fail(c, VT_ERR_ARG, "visible: null points or output");
+  if (!(tol >= 0.0f) || !(time >= 0.0f && time <= 1.0f)) return fail(c, VT_ERR_ARG, "visible: tol >= 0, time in [0, 1]");
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 12))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n))) return rc;
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(c->test_in.p, points, (size_t)n * 12, hipMemcpyHostToDevice));
+  TestLaunch T{c->S, (uint32_t *)c->overflow.p, (uint2 *)c->spill.p, c->spill_lanes,
+               (uint32_t)(c->spill_lanes / DEFAULT_BLOCK)};
+  const vt_camera &k = c->cams[cam].c;
+  VT_HIP(c, launch_visible(T, k.pos, k.skip_bodies, tol, time, (const float *)c->test_in.p, n, (uint8_t *)c->test_out.p,
+                           c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(out, c->test_out.p, n, hipMemcpyDeviceToHost));
   return VT_OK;
 }
 

@@ -69,6 +69,7 @@ struct DevCam {
   const float4 *adj;                 // adjoint image dLoss/dI (rgb) for the backward pass, else null
   float scale;                       // output scale (the image is acc * scale / spp)
   float shutter0, shutter1;          // exposure interval on the pose timeline (CamU.expo.xy)
+  uint32_t skip;                     // bodies the camera's own rays ignore (bit per body)
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)
@@ -311,6 +312,7 @@ struct SceneDev {
   float emit_area;
   const float4 *inst;       // posed bodies, INST_WORDS float4 each (layout at inst_pose below)
   uint32_t inst_count;
+  uint32_t shadow_skip;     // bodies shadow rays ignore (bit per body)
 };
 
 constexpr uint32_t NO_INST = 0xffffffffu;
@@ -594,15 +596,16 @@ __device__ __forceinline__ float box_entry(float4 lo, float4 hi, v3 o, v3 inv, f
 // The static scene, then every visible posed body (world box, bounding capsule, then its BLAS in its own frame at
 // the ray's exposure time), as traceSceneSkip (trace.wgsl.js:144-164) does. h.inst names the body of the hit.
 // the posed bodies after the static scene: `found` and h carry the static result in and the final one out
+// skip: bit i set = ignore body i (a camera's own lens front for its camera rays, a light's housing for shadow rays)
 template <bool ANY>
 __device__ __forceinline__ void trace_instances(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time,
-                                                Hit &h, bool &found, Stack &st) {
+                                                uint32_t skip, Hit &h, bool &found, Stack &st) {
   float best = found ? h.t : tmax;
   const v3 inv = safe_inv(d);
   for (uint32_t i = 0; i < S.inst_count; i++) {
     const float4 *I = S.inst + INST_WORDS * (size_t)i;
     const float4 p0 = I[2], p1 = I[3];
-    if (__float_as_uint(p1.w) & INST_HIDDEN) continue;
+    if ((__float_as_uint(p1.w) & INST_HIDDEN) || ((skip >> i) & 1u)) continue;
     if (box_entry(I[4], I[5], o, inv, tmin, best) >= best) continue;
     const float4 ca = I[6], cb = I[7];
     if (!seg_near_capsule(o, d, tmin, fminf(best, 1e3f), ld3(ca), ld3(cb), ca.w)) continue;
@@ -621,14 +624,14 @@ __device__ __forceinline__ void trace_instances(const SceneDev &S, v3 o, v3 d, f
 
 // INST = false: kernels built for scenes without posed bodies (the instance walk and its registers compiled out)
 template <bool ANY, bool INST>
-__device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time, Hit &h,
-                                            Stack &st) {
+__device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float tmin, float tmax, float time,
+                                            uint32_t skip, Hit &h, Stack &st) {
   bool found = traverse<ANY>(S, S.root, o, d, tmin, tmax, h, st);
   if (found) {
     h.inst = NO_INST;
     if (ANY) return true;
   }
-  if (INST && S.inst_count) trace_instances<ANY>(S, o, d, tmin, tmax, time, h, found, st);
+  if (INST && S.inst_count) trace_instances<ANY>(S, o, d, tmin, tmax, time, skip, h, found, st);
   return found;
 }
 

@@ -108,6 +108,8 @@ struct TestLaunch {
 hipError_t launch_trace_test(const TestLaunch &L, const float *rays, uint32_t n, int any_hit, float *hits,
                              hipStream_t s);
 hipError_t launch_lens_test(const float *lens9, const float *uv, uint32_t n, float *rays, hipStream_t s);
+hipError_t launch_visible(const TestLaunch &L, const float c[3], uint32_t skip, float tol, float time, const float *pts,
+                          uint32_t n, uint8_t *out, hipStream_t s);
 hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out, hipStream_t s);
 

@@ -50,7 +50,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
   for (uint32_t b = 0; b < D; b++) {
     Hit h;
     nrays++;
-    if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, h, st)) break;
+    if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, b == 0 ? C.skip : 0u, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
@@ -134,7 +134,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
         }
         Hit sh;
         nrays++;
-        if (trace_scene<true, INST>(S, so, wi, 0.0f, tmax, time, sh, st)) continue;
+        if (trace_scene<true, INST>(S, so, wi, 0.0f, tmax, time, S.shadow_skip, sh, st)) continue;
         Ln = Ln + contrib;
         if (GRAD) {
           grad_add(gs->light, lid, gl_part);
@@ -265,7 +265,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
   for (uint32_t b = 0; b < Dp; b++) {
     Hit h;
     nrays++;
-    if (!trace_scene<false, INST>(S, val(o), val(d), 0.0f, INFINITY, time, h, st)) break;
+    if (!trace_scene<false, INST>(S, val(o), val(d), 0.0f, INFINITY, time, b == 0 ? C.skip : 0u, h, st)) break;
     const float4 *sh = S.tshade + 4 * (size_t)h.tri;
     const float4 n0 = sh[0];
     const uint4 ms = S.meshes[__float_as_uint(n0.w)];
@@ -355,7 +355,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
         const v3 so = val(p) + ng * (dot(ng, wiv) > 0.0f ? EPS_OFF : -EPS_OFF);
         Hit sh;
         nrays++;
-        if (trace_scene<true, INST>(S, so, wiv, 0.0f, dist.v - 2e-5f, time, sh, st)) continue;
+        if (trace_scene<true, INST>(S, so, wiv, 0.0f, dist.v - 2e-5f, time, S.shadow_skip, sh, st)) continue;
         Ln = Ln + contrib;
       }
     }
@@ -789,6 +789,40 @@ hipError_t launch_trace_test(const TestLaunch &L, const float *rays, uint32_t n,
   uint32_t blocks = (n + DEFAULT_BLOCK - 1) / DEFAULT_BLOCK;
   if (blocks > L.max_blocks) blocks = L.max_blocks;
   hipLaunchKernelGGL(k_trace_test, dim3(blocks), dim3(DEFAULT_BLOCK), 0, s, L, rays, n, any_hit, hits);
+  return hipGetLastError();
+}
+
+// visible (gpu/renderer.js:1238-1258): an any-hit query from the camera centre to tol short of each point, over the
+// static scene and the posed bodies at exposure time `time`, the camera's own bodies skipped
+template <bool INST>
+__global__ __launch_bounds__(DEFAULT_BLOCK) void k_visible(TestLaunch Lt, float3 c, uint32_t skip, float tol,
+                                                           float time, const float *pts, uint32_t n, uint8_t *out) {
+  __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
+  __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
+  Stack st{s_node, s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
+           (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, Lt.overflow};
+  const v3 o = mk(c.x, c.y, c.z);
+  for (uint32_t k = blockIdx.x * DEFAULT_BLOCK + threadIdx.x; k < n; k += gridDim.x * DEFAULT_BLOCK) {
+    const v3 v = mk(pts[3 * (size_t)k], pts[3 * (size_t)k + 1], pts[3 * (size_t)k + 2]) - o;
+    const float dist = sqrtf(dot(v, v));
+    uint8_t vis = 1;
+    if (dist > tol) {
+      Hit h;
+      vis = trace_scene<true, INST>(Lt.S, o, v * (1.0f / dist), 0.0f, dist - tol, time, skip, h, st) ? 0 : 1;
+    }
+    out[k] = vis;
+  }
+}
+hipError_t launch_visible(const TestLaunch &L, const float c[3], uint32_t skip, float tol, float time, const float *pts,
+                          uint32_t n, uint8_t *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint32_t blocks = (n + DEFAULT_BLOCK - 1) / DEFAULT_BLOCK;
+  if (blocks > L.max_blocks) blocks = L.max_blocks;
+  const float3 cc = make_float3(c[0], c[1], c[2]);
+  if (L.S.inst_count)
+    hipLaunchKernelGGL(k_visible<true>, dim3(blocks), dim3(DEFAULT_BLOCK), 0, s, L, cc, skip, tol, time, pts, n, out);
+  else
+    hipLaunchKernelGGL(k_visible<false>, dim3(blocks), dim3(DEFAULT_BLOCK), 0, s, L, cc, skip, tol, time, pts, n, out);
   return hipGetLastError();
 }
 

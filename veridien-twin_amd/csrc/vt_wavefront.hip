@@ -122,6 +122,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
   RayW R;
   R.o = mk(0.f, 0.f, 0.f); R.kx = 0; R.ky = 1; R.kz = 2; R.Sx = 0.f; R.Sy = 0.f; R.Sz = 0.f;
   float best = 0.0f, tmax0 = 0.0f, time = 0.0f;
+  uint32_t skip = 0;   // bodies this ray ignores (posed-body kernels only)
   v3 dir = mk(0.f, 0.f, 1.f);
   bool found = false;
   Hit h{0.f, 0.f, 0.f, 0u, NO_INST};
@@ -149,6 +150,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
             d = mk(d4.x, d4.y, d4.z);
             tmax = o4.w;
             time = d4.w;
+            if (INST) skip = W.S.shadow_skip;
           } else {
             const uint32_t p = W.q ? W.q[k] : k;
             ok = W.thr[p].w > 0.0f;
@@ -157,6 +159,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
             d = mk(d4.x, d4.y, d4.z);
             tmax = INFINITY;
             time = d4.w;
+            if (INST) skip = W.bounce == 0 ? W.cams[W.cam[p]].skip : 0u;
             if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
           }
           if (ok) {
@@ -251,7 +254,8 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
         if (done) {
           h.inst = NO_INST;
           // then the posed bodies (few, small: walked one after the other with the lane's own stack)
-          if (INST && W.S.inst_count && !(ANY && found)) trace_instances<ANY>(W.S, R.o, dir, 0.0f, tmax0, time, h, found, st);
+          if (INST && W.S.inst_count && !(ANY && found)) trace_instances<ANY>(W.S, R.o, dir, 0.0f, tmax0, time, skip, h, found,
+                                                                                  st);
           if (ANY) {
             if (!found) W.sh_c[k].w = 1.0f;
           } else {

@@ -86,7 +86,11 @@ export function fromPrepared(S) {
   if (S.blas && S.blas.length) {
     const triBody = new Uint32Array(S.tris.length / 16).fill(STATIC_BODY);
     S.blas.forEach((b, i) => triBody.fill(i, b.triOff, b.triOff + b.triCount));
-    Object.assign(out, { nodes: undefined, triBody, bodies: S.blas.map((b) => ({ key: b.body, kind: b.kind })) });
+    Object.assign(out, { nodes: undefined, triBody,
+      bodies: S.blas.map((b) => ({ key: b.body, kind: b.kind, owner: b.owner || null })) });
+    // shadow rays leave through the scope's housing when the emitter sits at its front (gpu/renderer.js:648-649)
+    const scopeI = S.blas.findIndex((b) => b.body === 'scope');
+    out.shadowSkip = em.shadowSkipsScope && scopeI >= 0 ? (1 << scopeI) >>> 0 : 0;
   }
   return out;
 }
@@ -129,7 +133,7 @@ export class TwinRenderer {
     this.ctx = native.create(opts.device || 0);
     this.device = { hip: opts.device || 0 };
     this.bodies = src.triBody ? (src.bodies || []).map((b) => (typeof b === 'string' ? { key: b, kind: 'body' } : b)) : [];
-    native.uploadScene(this.ctx, { ...src, bodyCount: this.bodies.length });
+    native.uploadScene(this.ctx, { ...src, bodyCount: this.bodies.length, shadowSkip: src.shadowSkip || 0 });
     this.stats = { triangles: src.tris.length / 16, bodies: this.bodies.length };
     this.materials = Float32Array.from(src.materials);
     this.lights = Float32Array.from(src.lights);
@@ -165,7 +169,28 @@ export class TwinRenderer {
       R: cam.R, pos: cam.pos, lens: cam.lens, mode: cam.mode, width: cam.size[0], height: cam.size[1], spp,
       seed: dflt(o.seed, this.opts.seed), camIndex: cam.index, lightSamples: dflt(o.lightSamples, this.opts.lightSamples),
       maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1), shutter: o.shutter || [0, 1],
+      skipBodies: this._skipMask(cam.name),
     });
+  }
+
+  /** Bodies a camera's own rays ignore: those it owns, e.g. its lens front (gpu/renderer.js:881-886). */
+  _skipMask(name) {
+    let m = 0;
+    this.bodies.forEach((b, i) => { if (b.owner && b.owner === name) m |= 1 << i; });
+    return m >>> 0;
+  }
+
+  /**
+   * visible(name, pointsD, {tol, time}) (gpu/renderer.js:1238-1258): per world point, whether nothing lies between
+   * the camera centre and tol short of it (posed bodies at exposure time `time`, the camera's own bodies ignored).
+   * One any-hit launch on the GPU answers the whole list.
+   */
+  visible(name, pointsD, { tol = 2e-4, time = 0.5 } = {}) {
+    const cam = this._slot(name);
+    this._pack(cam, {});
+    const pts = new Float32Array(pointsD.length * 3);
+    pointsD.forEach((p, k) => pts.set([p[0], p[1], p[2]], 3 * k));
+    return Array.from(native.visible(this.ctx, cam.index, pts, tol, time), (v) => v === 1);
   }
 
   /**

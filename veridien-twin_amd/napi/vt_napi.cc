@@ -190,6 +190,7 @@ napi_value UploadScene(napi_env env, napi_callback_info info) {
   s.emit_area = (float)num(env, argv[1], "emitArea", 0);
   s.tri_body = typed<uint32_t>(env, argv[1], "triBody", &n, napi_uint32_array);
   s.body_count = s.tri_body ? (uint32_t)num(env, argv[1], "bodyCount", 0) : 0;
+  s.shadow_skip = (uint32_t)num(env, argv[1], "shadowSkip", 0);
   if (s.tri_body && n != s.tri_count) {
     napi_throw_range_error(env, nullptr, "triBody needs one entry per triangle");
     return nullptr;
@@ -225,6 +226,7 @@ napi_value SetCamera(napi_env env, napi_callback_info info) {
   k.light_samples = (uint32_t)num(env, argv[2], "lightSamples", 1);
   k.max_depth = (uint32_t)num(env, argv[2], "maxDepth", 1);
   k.scale = (float)num(env, argv[2], "scale", 1.0);
+  k.skip_bodies = (uint32_t)num(env, argv[2], "skipBodies", 0);
   k.shutter[0] = 0.0f;
   k.shutter[1] = 1.0f;
   napi_value sh;
@@ -272,6 +274,43 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
   int rc = vt_set_instances(c, v.data(), len);
   if (rc) return throw_vt(env, c, rc, "vt_set_instances");
   return nullptr;
+}
+
+// visible(ctx, cam, points: Float32Array (n x 3), tol, time) -> Uint8Array (1 = visible); synchronous, as the
+// reference's CPU query is
+napi_value Visible(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t cam = 0;
+  double tol = 2e-4, time = 0.5;
+  napi_get_value_uint32(env, argv[1], &cam);
+  bool is = false;
+  if (argc < 5 || napi_is_typedarray(env, argv[2], &is) != napi_ok || !is) {
+    napi_throw_type_error(env, nullptr, "visible(ctx, cam, points: Float32Array, tol, time)");
+    return nullptr;
+  }
+  napi_typedarray_type tt;
+  size_t len, off;
+  void *data;
+  napi_value ab;
+  napi_get_typedarray_info(env, argv[2], &tt, &len, &data, &ab, &off);
+  if (tt != napi_float32_array || len % 3) {
+    napi_throw_type_error(env, nullptr, "visible: points must be a Float32Array of n x 3");
+    return nullptr;
+  }
+  napi_get_value_double(env, argv[3], &tol);
+  napi_get_value_double(env, argv[4], &time);
+  const uint32_t n = (uint32_t)(len / 3);
+  napi_value outab, out;
+  void *dst;
+  CHECK_NAPI(napi_create_arraybuffer(env, n ? n : 1, &dst, &outab));
+  int rc = vt_visible(c, cam, (const float *)data, n, (float)tol, (float)time, (uint8_t *)dst);
+  if (rc) return throw_vt(env, c, rc, "vt_visible");
+  CHECK_NAPI(napi_create_typedarray(env, napi_uint8_array, n, outab, 0, &out));
+  return out;
 }
 
 napi_value Render(napi_env env, napi_callback_info info) {
@@ -618,7 +657,7 @@ napi_value Init(napi_env env, napi_value exports) {
     const char *name;
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
-             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
+             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights}};
   for (auto &f : fns) {

@@ -17,7 +17,7 @@ EXPORTS = [
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
-    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances",
+    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
@@ -32,7 +32,7 @@ class Scene(C.Structure):
     _fields_ = [("nodes", vp), ("node_count", u32), ("tris", vp), ("tri_count", u32), ("verts", vp),
                 ("vert_count", u32), ("meshes", vp), ("mesh_count", u32), ("materials", vp),
                 ("material_count", u32), ("lights", vp), ("light_count", u32), ("emit", vp), ("emit_count", u32),
-                ("emit_area", f32), ("tri_body", vp), ("body_count", u32)]
+                ("emit_area", f32), ("tri_body", vp), ("body_count", u32), ("shadow_skip", u32)]
 
 
 MAX_BODIES = 24
@@ -46,7 +46,7 @@ class Instance(C.Structure):
 class Camera(C.Structure):
     _fields_ = [("R", f32 * 9), ("pos", f32 * 3), ("mode", u32), ("lens", f32 * 9), ("width", u32), ("height", u32),
                 ("spp", u32), ("seed", u32), ("cam_index", u32), ("light_samples", u32), ("max_depth", u32),
-                ("flags", u32), ("scale", f32), ("shutter", f32 * 2)]
+                ("flags", u32), ("scale", f32), ("shutter", f32 * 2), ("skip_bodies", u32)]
 
 
 class RenderOpts(C.Structure):
@@ -132,6 +132,7 @@ def load():
         "vt_autotune": (i32, [vp, vp, u32, vp, u32, vp]),
         "vt_copy_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
         "vt_set_instances": (i32, [vp, vp, u32]),
+        "vt_visible": (i32, [vp, u32, vp, u32, f32, f32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -206,7 +207,8 @@ class Context:
         s = Scene(ptr(nodes), 0 if nodes is None else nodes.size // 16, ptr(a["tris"]), a["tris"].size // 16,
                   ptr(a["verts"]), a["verts"].size // 8, ptr(a["meshes"]), a["meshes"].size // 4,
                   ptr(a["materials"]), a["materials"].size // 16, ptr(a["lights"]), a["lights"].size // 4,
-                  ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]), ptr(tb), nb)
+                  ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]), ptr(tb), nb,
+                  int(sc.get("shadow_skip", 0)))
         self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
 
     def set_instances(self, poses):
@@ -234,7 +236,15 @@ class Context:
             setattr(c, k, int(cam.get(k, 0)))
         c.scale = float(cam.get("scale", 1.0))
         c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
+        c.skip_bodies = int(cam.get("skip_bodies", 0))
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
+
+    def visible(self, slot, points, tol=2e-4, time=0.5):
+        """vt_visible: bool per world point, seen unoccluded from the camera centre (to tol short of the point)."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(max(1, pts.shape[0]), np.uint8)
+        self.check(self.L.vt_visible(self.h, slot, ptr(pts), pts.shape[0], float(tol), float(time), ptr(out)))
+        return out[: pts.shape[0]].astype(bool)
 
     def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,
                stack_lds=0, queues=0, graph=False, batch=0):

@@ -197,6 +197,16 @@ int vt_set_instances(vt_ctx *ctx, const vt_instance *instances, uint32_t count);
  * e.g. between the steps of a calibration loop; the geometry and BVH stay. No reference counterpart: the
  * reference rebuilds its params buffer in _buildBuffers (gpu/renderer.js:271-331). */
 int vt_update_materials(vt_ctx *ctx, const float *materials, uint32_t material_count);
+
+/* Angular emission profile of one light (physics.yaml light.profile; shade.wgsl.js:12-27 profile, :48-52 emitted):
+ * L_e(w) = radiance * p(theta), theta the angle between the outgoing direction w and the light's axis, p linear in
+ * theta between `count` knots (theta ascending, radians; p >= 0), 0 past the last knot; the emitter still emits on
+ * its front side only. count = 0 makes the light Lambertian again. Profiles survive vt_update_lights and are
+ * cleared by vt_upload_scene. The reference has one such light, posed with the scope; here the light's triangles
+ * are static scene geometry. */
+#define VT_MAX_KNOTS 64
+int vt_set_light_profile(vt_ctx *ctx, uint32_t light, const float axis[3], const float *theta, const float *p,
+                         uint32_t count);
 int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
 
 /* ---------------------------------------------------------------- per frame */

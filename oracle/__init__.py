@@ -125,7 +125,7 @@ class Scene(C.Structure):
                 ("materials", C.c_void_p), ("mat_count", C.c_uint32), ("lights", C.c_void_p),
                 ("light_count", C.c_uint32), ("emit", C.c_void_p), ("emit_count", C.c_uint32),
                 ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32),
-                ("shadow_skip", C.c_uint32)]
+                ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p)]
 
 
 class Body(C.Structure):
@@ -229,6 +229,20 @@ def scene_struct(sc):
               ptr(keep["materials"]), keep["materials"].size // 16, ptr(keep["lights"]), keep["lights"].size // 4,
               ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]),
               C.cast(bodies, C.c_void_p) if nb else None, nb, int(sc.get("shadow_skip", 0)) if nb else 0)
+    if sc.get("profiles"):
+        # {light: (axis, theta[], p[])}: the upload layout of vt_set_light_profile
+        nl = keep["lights"].size // 4
+        lp = np.zeros((nl, 4), np.float32)
+        lp[:, 2] = 1.0
+        kn = np.zeros((nl, 64, 2), np.float32)
+        for light, (axis, th, p) in sc["profiles"].items():
+            ax = np.asarray(axis, np.float64)
+            lp[light, :3] = ax / np.linalg.norm(ax)
+            lp[light].view(np.uint32)[3] = len(th)
+            kn[light, : len(th), 0] = th
+            kn[light, : len(th), 1] = p
+        keep["light_prof"], keep["knots"] = lp, kn
+        s.light_prof, s.knots = ptr(lp), ptr(kn)
     s._keep = keep
     return s
 

@@ -180,6 +180,24 @@ static void surface(const or_scene *S, const or_hit *h, const pose_t *P, const d
   for (int k = 0; k < 3; k++) s->p[k] = o[k] + d[k] * h->t;
 }
 
+/* angular profile of light `light` toward the unit direction w leaving the emitter (shade.wgsl.js:48-52 emitted:
+ * p(theta) with theta = acos(w . axis), or_profile over the light's knots); 1 for a Lambertian light */
+static double light_profile(const or_scene *S, uint32_t light, const double w[3]) {
+  if (!S->light_prof) return 1;
+  const float *a = S->light_prof + 4 * (size_t)light;
+  uint32_t cnt;
+  memcpy(&cnt, a + 3, 4);
+  if (cnt == 0) return 1;
+  const double ax[3] = {a[0], a[1], a[2]};
+  const double th = acos(fmin(1.0, fmax(-1.0, dot3(w, ax))));
+  double kt[64], kp[64];
+  for (uint32_t k = 0; k < cnt && k < 64; k++) {
+    kt[k] = S->knots[2 * ((size_t)light * 64 + k)];
+    kp[k] = S->knots[2 * ((size_t)light * 64 + k) + 1];
+  }
+  return or_profile(th, kt, kp, (int)(cnt < 64 ? cnt : 64));
+}
+
 /* outward emitter normal at a hit: the interpolated vertex normal, unflipped (shade.wgsl.js:217-220) */
 static void emitter_normal(const or_scene *S, const surf_t *s, double n[3]) {
   for (int k = 0; k < 3; k++) {
@@ -279,13 +297,14 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double cosE = dot3(md, ne);
       if (cosE > 0) {
         const float *Le = S->lights + 4 * (size_t)s.light;
+        const double pe = light_profile(S, s.light, md);
         double w = 1;
         if (b > 0) {
           double pL = h.t * h.t / (cosE * A);
           w = pB * pB / (pB * pB + nl * nl * pL * pL);
         }
-        for (int k = 0; k < 3; k++) L[k] += thr[k] * Le[k] * w;
-        if (G) for (int k = 0; k < 3; k++) G->light[3 * (size_t)s.light + k] += G->delta[k] * thr[k] * w;
+        for (int k = 0; k < 3; k++) L[k] += thr[k] * Le[k] * pe * w;
+        if (G) for (int k = 0; k < 3; k++) G->light[3 * (size_t)s.light + k] += G->delta[k] * thr[k] * w * pe;
       }
       break;
     }
@@ -314,7 +333,10 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       if (cosS <= 0 || dot3(s.ng, wi) <= 0) continue;
       uint32_t lid;
       memcpy(&lid, e + 11, 4);
-      const float *Le = S->lights + 4 * (size_t)lid;
+      const float *L0 = S->lights + 4 * (size_t)lid;
+      const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
+      const double pe = light_profile(S, lid, mwi);
+      const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
       if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) continue;
       double f[3];
       or_eval_bsdf(&M, s.ns, wi, wo, f);
@@ -338,7 +360,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
         double dc[3];
         eval_bsdf_dc(&M, s.ns, wi, wo, dc);
         for (int k = 0; k < 3; k++) {
-          G->light[3 * (size_t)lid + k] += G->delta[k] * thr[k] / nl * f[k] * kk;
+          G->light[3 * (size_t)lid + k] += G->delta[k] * thr[k] / nl * f[k] * kk * pe;
           G->mat[3 * (size_t)s.mat + k] += G->delta[k] * thr[k] / nl * dc[k] * Le[k] * kk;
         }
       }

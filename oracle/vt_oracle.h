@@ -141,6 +141,9 @@ typedef struct {
   float emit_area;
   const struct or_body *bodies; uint32_t body_count;  /* posed bodies (may be 0) */
   uint32_t shadow_skip;                                /* bit i: shadow rays ignore body i */
+  /* angular emission profiles (optional, NULL = every light Lambertian): per light 4 words (unit axis xyz, knot
+   * count as uint32); knots: 64 (theta, p) pairs per light. L_e(w) = radiance * or_profile(theta(w, axis)). */
+  const float *light_prof; const float *knots;
 } or_scene;
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose

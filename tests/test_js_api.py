@@ -172,13 +172,15 @@ def test_from_prepared_bodies():
     S = ("{nodes: new Float32Array(16), tris: new Float32Array(16 * 5), verts: new Float32Array(24), "
          "meshTable: [{mat: 0, flags: 1}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, roughness: 0.5}], "
          "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2, owner: 'CAM2'}, {body: 'scope', kind: 'fixed', triOff: 4, "
-         "triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true}}")
+         "triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true, axis: [0, 0, -1], "
+         "theta: [0, 0.5], p: [1, 0]}}")
     out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({tb: Array.from(u.triBody), "
-                "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip})); })" % (MODULE, S))
+                "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip, profiles: u.profiles})); })" % (MODULE, S))
     d = json.loads(out)
     assert d["tb"] == [0xFFFFFFFF, 0, 0, 0xFFFFFFFF, 1] and d["nodes"]
     assert d["bodies"] == [{"key": "arm", "kind": "body", "owner": "CAM2"}, {"key": "scope", "kind": "fixed", "owner": None}]
     assert d["shadowSkip"] == 2          # the scope's bit: shadow rays leave through its housing
+    assert d["profiles"] == [{"light": 0, "axis": [0, 0, -1], "theta": [0, 0.5], "p": [1, 0]}]
 
 
 @pytest.mark.gpu

@@ -42,7 +42,7 @@ struct vt_ctx {
   int cu_count = 0;
   // scene
   bool has_scene = false;
-  DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst;
+  DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots;
   std::vector<uint32_t> body_root;   // BLAS root node of each body
   std::vector<float> body_box;       // each body's box in its own frame (min xyz, max xyz)
   SceneDev S{};
@@ -291,7 +291,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst,
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
@@ -504,6 +504,16 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.meshes = (const uint4 *)c->meshes.p;
   c->S.mats = (const float4 *)c->mats.p;
   c->S.lights = (const float4 *)c->lights.p;
+  {
+    // every light Lambertian until vt_set_light_profile gives it a profile
+    const size_t nl = std::max<uint32_t>(1, s->light_count);
+    std::vector<float4> lp(nl, make_float4(0.f, 0.f, 1.f, 0.f));
+    std::vector<float2> kn(nl * MAX_KNOTS, make_float2(0.f, 0.f));
+    if ((rc = upload(c, c->lprof, lp.data(), lp.size() * sizeof(float4)))) return rc;
+    if ((rc = upload(c, c->knots, kn.data(), kn.size() * sizeof(float2)))) return rc;
+    c->S.lprof = (const float4 *)c->lprof.p;
+    c->S.knots = (const float2 *)c->knots.p;
+  }
   c->S.emit = (const float4 *)c->emit.p;
   c->S.emit_count = s->emit_count;
   c->S.root = 0;
@@ -560,6 +570,35 @@ extern "C" int vt_update_lights(vt_ctx *c, const float *lights, uint32_t count) 
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(c->lights.p, lights, (size_t)count * 16, hipMemcpyHostToDevice));
+  return VT_OK;
+}
+
+static_assert(VT_MAX_KNOTS == MAX_KNOTS, "knot table stride");
+
+extern "C" int vt_set_light_profile(vt_ctx *c, uint32_t light, const float axis[3], const float *theta, const float *p,
+                                    uint32_t count) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "light_profile: no scene uploaded");
+  if (light >= c->n_light) return fail(c, VT_ERR_ARG, "light_profile: light %u of %u", light, c->n_light);
+  if (count > VT_MAX_KNOTS) return fail(c, VT_ERR_SIZE, "light_profile: at most %d knots", VT_MAX_KNOTS);
+  float4 rec = make_float4(0.f, 0.f, 1.f, 0.f);
+  std::vector<float2> kn(MAX_KNOTS, make_float2(0.f, 0.f));
+  if (count) {
+    if (!axis || !theta || !p) return fail(c, VT_ERR_ARG, "light_profile: null axis or knots");
+    const double l = std::sqrt((double)axis[0] * axis[0] + (double)axis[1] * axis[1] + (double)axis[2] * axis[2]);
+    if (!(l > 1e-12) || !std::isfinite(l)) return fail(c, VT_ERR_ARG, "light_profile: zero axis");
+    for (uint32_t k = 0; k < count; k++) {
+      if (!std::isfinite(theta[k]) || !std::isfinite(p[k]) || p[k] < 0.0f || (k && theta[k] < theta[k - 1]))
+        return fail(c, VT_ERR_ARG, "light_profile: knots need finite ascending theta and p >= 0 (knot %u)", k);
+      kn[k] = make_float2(theta[k], p[k]);
+    }
+    rec = make_float4((float)(axis[0] / l), (float)(axis[1] / l), (float)(axis[2] / l), bits_f(count));
+  }
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy((float4 *)c->lprof.p + light, &rec, sizeof rec, hipMemcpyHostToDevice));
+  VT_HIP(c, hipMemcpy((float2 *)c->knots.p + (size_t)light * MAX_KNOTS, kn.data(), kn.size() * sizeof(float2),
+                      hipMemcpyHostToDevice));
   return VT_OK;
 }
 

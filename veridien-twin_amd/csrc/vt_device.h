@@ -313,7 +313,31 @@ struct SceneDev {
   const float4 *inst;       // posed bodies, INST_WORDS float4 each (layout at inst_pose below)
   uint32_t inst_count;
   uint32_t shadow_skip;     // bodies shadow rays ignore (bit per body)
+  const float4 *lprof;      // per light: emission axis xyz, w = bits(knot count); knots at light * MAX_KNOTS
+  const float2 *knots;      // (theta, p) knots of the angular emission profiles
 };
+
+constexpr uint32_t MAX_KNOTS = 64;
+// p(theta) of light `light` toward the unit direction w leaving the emitter, theta from the light's axis: linear
+// between knots, 0 past the last one (shade.wgsl.js:12-27); 1 for a light without a profile
+__device__ __forceinline__ float light_profile(const SceneDev &S, uint32_t light, v3 w) {
+  const float4 a = S.lprof[light];
+  const uint32_t cnt = __float_as_uint(a.w);
+  if (cnt == 0) return 1.0f;
+  const float2 *K = S.knots + (size_t)light * MAX_KNOTS;
+  const float th = acosf(fminf(fmaxf(w.x * a.x + w.y * a.y + w.z * a.z, -1.0f), 1.0f));
+  const float2 last = K[cnt - 1];
+  if (th >= last.x) return th == last.x ? last.y : 0.0f;
+  uint32_t lo = 0, hi = cnt - 1;
+  for (int it = 0; it < 8; it++) {
+    if (hi - lo <= 1) break;
+    const uint32_t m = (lo + hi) >> 1;
+    if (K[m].x <= th) lo = m; else hi = m;
+  }
+  const float2 ka = K[lo], kb = K[hi];
+  const float f = fminf(fmaxf((th - ka.x) / fmaxf(kb.x - ka.x, 1e-12f), 0.0f), 1.0f);
+  return ka.y + (kb.y - ka.y) * f;
+}
 
 constexpr uint32_t NO_INST = 0xffffffffu;
 struct Hit {

@@ -72,14 +72,15 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
     if (ms.y & F_EMITTER) {
       const float cosE = -dot(d, nsi);
       if (cosE > 0.0f) {
-        const v3 Le = ld3(S.lights[ms.z]);
+        const float pe = light_profile(S, ms.z, -d);
+        const v3 Le = ld3(S.lights[ms.z]) * pe;
         float w = 1.0f;
         if (b > 0) {
           const float pL = h.t * h.t / (cosE * A);
           w = pB * pB / (pB * pB + fnl * fnl * pL * pL);
         }
         L = L + thr * Le * w;
-        if (GRAD) grad_add(gs->light, ms.z, gs->delta * thr * w);
+        if (GRAD) grad_add(gs->light, ms.z, gs->delta * thr * (w * pe));
       }
       break;
     }
@@ -111,7 +112,8 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
           const float cosS = dot(ns, wi);
           if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
           lid = __float_as_uint(S.emit[4 * e + 2].w);
-          const v3 Le = ld3(S.lights[lid]);
+          const float pe = light_profile(S, lid, -wi);
+          const v3 Le = ld3(S.lights[lid]) * pe;
           if (maxc(Le) <= 0.0f) continue;
           const Mat M = load_mat(S.mats, ms.x);
           const v3 f = eval_bsdf(M, ns, wi, wo);
@@ -126,7 +128,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
           contrib = f * Le * k;
           if (GRAD) {
             const v3 g = gs->delta * thr * (1.0f / fnl);
-            gl_part = g * f * k;
+            gl_part = g * f * (k * pe);
             gm_part = g * eval_bsdf_dc(M, ns, wi, wo) * Le * k;
           }
           so = p + ng * (dot(ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
@@ -239,6 +241,34 @@ __device__ __forceinline__ void camdir_jac(const DevCam &C, float u, float v, v3
   }
 }
 
+// light_profile with the tangents of the direction: theta = acos(w . axis), d theta = -d(w . axis) / sin(theta);
+// the same value as light_profile, zero tangents where p is flat or clamped
+__device__ __forceinline__ D d_light_profile(const SceneDev &S, uint32_t light, const DV &w) {
+  const float4 a = S.lprof[light];
+  const uint32_t cnt = __float_as_uint(a.w);
+  if (cnt == 0) return dk(1.0f);
+  const float2 *K = S.knots + (size_t)light * MAX_KNOTS;
+  const D c = ddot(w, mk(a.x, a.y, a.z));
+  const float cv = fminf(fmaxf(c.v, -1.0f), 1.0f);
+  const float th = acosf(cv);
+  const float2 last = K[cnt - 1];
+  if (th >= last.x) return dk(th == last.x ? last.y : 0.0f);
+  uint32_t lo = 0, hi = cnt - 1;
+  for (int it = 0; it < 8; it++) {
+    if (hi - lo <= 1) break;
+    const uint32_t m = (lo + hi) >> 1;
+    if (K[m].x <= th) lo = m; else hi = m;
+  }
+  const float2 ka = K[lo], kb = K[hi];
+  const float den = fmaxf(kb.x - ka.x, 1e-12f);
+  const float f = (th - ka.x) / den;
+  if (!(f > 0.0f)) return dk(ka.y);
+  if (!(f < 1.0f)) return dk(kb.y);
+  D r = c * ((kb.y - ka.y) / den * (-1.0f / sqrtf(fmaxf(1.0f - cv * cv, 1e-12f))));
+  r.v = ka.y + (kb.y - ka.y) * f;
+  return r;
+}
+
 // The estimator of path_walk with forward-mode tangents of the camera ray (origin o0, direction d0): the same
 // random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken
 // on the value. Tangents: origin along e1, e2, direction along e1, e2 (e1, e2 perpendicular to d0). Returns
@@ -300,13 +330,14 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
       const D cosE = -ddot(d, nsi);
       if (cosE.v > 0.0f) {
         const v3 Le = ld3(S.lights[ms.z]);
+        const D pe = d_light_profile(S, ms.z, -d);
         D w = dk(1.0f);
         if (b > 0) {
           const D pL = t * t / (cosE * A);
           const D pb2 = pB * pB;
           w = pb2 / (pb2 + (fnl * fnl) * (pL * pL));
         }
-        const DV c = thr * Le;
+        const DV c = (thr * Le) * pe;
         L = L + val(c) * w.v;
         Ls = Ls + (c.x * delta.x + c.y * delta.y + c.z * delta.z) * w;
       }
@@ -338,7 +369,8 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
         const v3 wiv = val(wi);
         if (cosS.v <= 0.0f || dot(ng, wiv) <= 0.0f) continue;
         const uint32_t lid = __float_as_uint(S.emit[4 * e + 2].w);
-        const v3 Le = ld3(S.lights[lid]);
+        const D pe = d_light_profile(S, lid, -wi);
+        const v3 Le = ld3(S.lights[lid]) * pe.v;
         if (maxc(Le) <= 0.0f) continue;
         const Mat M = load_mat(S.mats, ms.x);
         const DV f = d_eval_bsdf(M, ns, wi, wo);
@@ -351,7 +383,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
           w = a2 / (a2 + pb * pb);
         }
         const D k = cosS * w / pW;
-        const DV contrib = (f * Le) * k;
+        const DV contrib = (f * ld3(S.lights[lid])) * (k * pe);
         const v3 so = val(p) + ng * (dot(ng, wiv) > 0.0f ? EPS_OFF : -EPS_OFF);
         Hit sh;
         nrays++;

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         if (ms.y & F_EMITTER) {
           const float cosE = -dot(d, nsi);
           if (cosE > 0.0f) {
-            const v3 Le = ld3(W.S.lights[ms.z]);
+            const v3 Le = ld3(W.S.lights[ms.z]) * light_profile(W.S, ms.z, -d);
             float w = 1.0f;
             if (b > 0) {
               const float pL = h4.x * h4.x / (cosE * A);
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               const float cosS = dot(ns, wi);
               if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
               const uint32_t lid = __float_as_uint(W.S.emit[4 * e + 2].w);
-              const v3 Le = ld3(W.S.lights[lid]);
+              const v3 Le = ld3(W.S.lights[lid]) * light_profile(W.S, lid, -wi);
               if (maxc(Le) <= 0.0f) continue;
               const Mat M = load_mat(W.S.mats, ms.x);
               const v3 f = eval_bsdf(M, ns, wi, wo);

@@ -56,8 +56,9 @@ export function projectKB4(L, X) {
 
 /**
  * The reference's prepared scene (gpu/scene_gpu.js:461-469 prepareScene output: nodes, tris, verts, meshTable,
- * materials, emitter) -> this library's upload arrays. The emitter's angular profile p(theta) is not carried:
- * the library's emitters are Lambertian with per-light radiance (one light, L0).
+ * materials, emitter, blas) -> this library's upload arrays: one light (L0) with the emitter's angular profile
+ * p(theta) about its axis, the bodies' triangle ranges as triBody. The emitter triangles stay static (the
+ * reference poses them with the scope).
  */
 export function fromPrepared(S) {
   const meshes = new Uint32Array(S.meshTable.length * 4);
@@ -77,7 +78,10 @@ export function fromPrepared(S) {
     eu[16 * k + 11] = 0;
   });
   const L0 = em.L0 || [0, 0, 0];
+  // the outlet's angular profile L_e = L0 p(theta from the axis) (gpu/scene_gpu.js:205-214, shade.wgsl.js:48-52)
+  const profiles = em.theta && em.p && em.axis ? [{ light: 0, axis: em.axis, theta: em.theta, p: em.p }] : [];
   const out = {
+    profiles,
     nodes: S.nodes, tris: S.tris, verts: S.verts, meshes, materials,
     lights: new Float32Array([L0[0], L0[1], L0[2], 0]), emit: em.tris.length ? emit : new Float32Array(0),
     emitArea: em.area || 0,
@@ -135,6 +139,7 @@ export class TwinRenderer {
     this.bodies = src.triBody ? (src.bodies || []).map((b) => (typeof b === 'string' ? { key: b, kind: 'body' } : b)) : [];
     native.uploadScene(this.ctx, { ...src, bodyCount: this.bodies.length, shadowSkip: src.shadowSkip || 0 });
     this.stats = { triangles: src.tris.length / 16, bodies: this.bodies.length };
+    for (const pr of src.profiles || []) this.setLightProfile(pr.light, pr);
     this.materials = Float32Array.from(src.materials);
     this.lights = Float32Array.from(src.lights);
     this.cameras = {};
@@ -171,6 +176,12 @@ export class TwinRenderer {
       maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1), shutter: o.shutter || [0, 1],
       skipBodies: this._skipMask(cam.name),
     });
+  }
+
+  /** Angular emission profile of a light: {axis, theta (radians, ascending), p}; null restores a Lambertian light. */
+  setLightProfile(light, profile) {
+    native.setLightProfile(this.ctx, light, profile ? { axis: Array.from(profile.axis), theta: Array.from(profile.theta),
+      p: Array.from(profile.p) } : null);
   }
 
   /** Bodies a camera's own rays ignore: those it owns, e.g. its lens front (gpu/renderer.js:881-886). */

@@ -276,6 +276,40 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setLightProfile(ctx, light, {axis: [3], theta: [..], p: [..]} | null)
+napi_value SetLightProfile(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t light = 0;
+  napi_get_value_uint32(env, argv[1], &light);
+  float axis[3] = {0.f, 0.f, 1.f};
+  std::vector<float> th, p;
+  napi_valuetype t = napi_undefined;
+  if (argc > 2) napi_typeof(env, argv[2], &t);
+  if (t == napi_object) {
+    napi_value a, b;
+    uint32_t na = 0, nb = 0;
+    if (!float_list(env, argv[2], "axis", axis, 3) || !get_prop(env, argv[2], "theta", &a) ||
+        !get_prop(env, argv[2], "p", &b) || napi_get_array_length(env, a, &na) != napi_ok ||
+        napi_get_array_length(env, b, &nb) != napi_ok || na != nb || na > VT_MAX_KNOTS) {
+      napi_throw_type_error(env, nullptr, "light profile needs axis[3] and equally long theta[], p[] (<= 64 knots)");
+      return nullptr;
+    }
+    th.resize(na);
+    p.resize(na);
+    if (na && (!float_list(env, argv[2], "theta", th.data(), na) || !float_list(env, argv[2], "p", p.data(), na))) {
+      napi_throw_type_error(env, nullptr, "light profile knots must be numbers");
+      return nullptr;
+    }
+  }
+  int rc = vt_set_light_profile(c, light, axis, th.data(), p.data(), (uint32_t)th.size());
+  if (rc) return throw_vt(env, c, rc, "vt_set_light_profile");
+  return nullptr;
+}
+
 // visible(ctx, cam, points: Float32Array (n x 3), tol, time) -> Uint8Array (1 = visible); synchronous, as the
 // reference's CPU query is
 napi_value Visible(napi_env env, napi_callback_info info) {
@@ -657,7 +691,7 @@ napi_value Init(napi_env env, napi_value exports) {
     const char *name;
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
-             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
+             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights}};
   for (auto &f : fns) {

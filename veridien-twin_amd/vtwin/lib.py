@@ -18,6 +18,7 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
+    "vt_set_light_profile",
 ]
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
@@ -133,6 +134,7 @@ def load():
         "vt_copy_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
         "vt_set_instances": (i32, [vp, vp, u32]),
         "vt_visible": (i32, [vp, u32, vp, u32, f32, f32, vp]),
+        "vt_set_light_profile": (i32, [vp, u32, vp, vp, vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -238,6 +240,19 @@ class Context:
         c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
         c.skip_bodies = int(cam.get("skip_bodies", 0))
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
+
+    def set_light_profile(self, light, axis=None, theta=(), p=()):
+        """vt_set_light_profile: L_e = radiance * p(theta from axis), linear between knots; no knots = Lambertian."""
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        pp = np.ascontiguousarray(p, dtype=np.float32)
+        ax = np.ascontiguousarray(axis if axis is not None else (0, 0, 1), dtype=np.float32)
+        assert th.size == pp.size
+        self.check(self.L.vt_set_light_profile(self.h, light, ptr(ax), ptr(th), ptr(pp), th.size))
+
+    def upload_profiles(self, sc):
+        """The scene dict's profiles ({light: (axis, theta[], p[])}), if any, after upload_scene."""
+        for light, (axis, th, p) in (sc.get("profiles") or {}).items():
+            self.set_light_profile(light, axis, th, p)
 
     def visible(self, slot, points, tol=2e-4, time=0.5):
         """vt_visible: bool per world point, seen unoccluded from the camera centre (to tol short of the point)."""

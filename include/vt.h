@@ -47,7 +47,7 @@ typedef struct {
 /* The prepared scene in the reference's upload layout (gpu/renderer.js:271-331, gpu/scene_gpu.js:461-469):
  *   nodes      BVH2, 16 words per node (gpu/bvh.js:5-8); NULL: the library builds it from tris/verts
  *   tris       16 words per BVH slot: [v0.xyz, i0][e1.xyz, i1][e2.xyz, i2][mesh, flags, -, -]
- *   verts      8 words per vertex: [pos.xyz, -][normal.xyz, -]
+ *   verts      8 words per vertex: [pos.xyz, mesh id (uint32; the node cache's material)][normal.xyz, -]
  *   meshes     4 words per mesh: material, flags (VT_MESH_*), light index (emitters), -
  *   materials  16 floats per material: [c.rgb, alpha][ior, kind, fbar, roughness][tau.rgb, alpha_c]
  *              [fbar_computed, -, -, -]; kind 0 dielectric, 1 conductor, 2 coated conductor, 3 Lambert
@@ -82,6 +82,10 @@ typedef struct {
   float q0[4], q1[4];
   float p0[3], p1[3];
   uint32_t hidden;
+  /* occlusion capsule in the body's frame (segment occ_a-occ_b, radius occ_r; 0 = none) and the effective
+   * reflectance rho_eff of the body: the reference estimator dims a static point's cached indirect light by
+   * 1 - (1 - rho_eff) O(capsule) (capsuleVis, gpu/wgsl/shade.wgsl.js:57-94), the capsule posed at mid-exposure */
+  float occ_a[3], occ_b[3], occ_r, rho_eff;
 } vt_instance;
 
 /* One tracking camera, CamU (gpu/wgsl/common.wgsl.js:79-90; gpu/renderer.js:888-898). */
@@ -94,7 +98,7 @@ typedef struct {
   uint32_t cam_index;    /* camera index in the pixel-seed hash (gpu/wgsl/kernels.wgsl.js:35) */
   uint32_t light_samples;/* emitter samples per path vertex, 1..16 */
   uint32_t max_depth;    /* path vertices, >= 1 */
-  uint32_t flags;        /* reserved, 0 */
+  uint32_t flags;        /* option bits VT_OPT_* (CamU.flags.y, gpu/wgsl/common.wgsl.js:45-58) */
   float scale;           /* output scale (CamU.expo.z), usually 1 */
   float shutter[2];      /* exposure interval on the pose timeline (CamU.expo.xy), within [0, 1]: sub-sample i
                             is at time shutter[0] + (shutter[1] - shutter[0]) frac(s2 + i / golden ratio) */
@@ -278,11 +282,72 @@ int vt_camera_ray_test(vt_ctx *ctx, uint32_t cam, const uint32_t *xyi, uint32_t 
 /* bsdfTest (:507-523): cases of 16 floats (n.xyz, -, l.xyz, -, v.xyz, -, u0 u1 u2, material) ->
  * 8 floats each (f.rgb, pdf(v, l), sampled l.xyz, sampled pdf). */
 int vt_bsdf_test(vt_ctx *ctx, const float *cases, uint32_t n, float *out);
+/* capsuleOcc (gpu/wgsl/shade.wgsl.js:57-81): cases of 16 floats (p.xyz, -, n.xyz, -, a.xyz, r, b.xyz, -) ->
+ * the cosine-weighted occluded fraction, one float each */
+int vt_capsule_test(vt_ctx *ctx, const float *cases, uint32_t n, float *out);
 /* The fused sensor epilogue on caller-given radiance (rgba float32 w*h*4) and RI map (w*h) -> DN (w*h). */
 int vt_sensor_test(vt_ctx *ctx, const vt_sensor_params *params, uint32_t width, uint32_t height,
                    const float *radiance, const float *ri, uint16_t *dn);
 
-/* ---------------------------------------------------------------- node radiosity */
+/* Camera option bits. The reference's own (common.wgsl.js:45-58; they act in the reference estimator):
+ * NO_INDIRECT / DIRECT_ONLY drop the cached indirect light, NO_CAPSULE the bodies' occlusion of it, NORMALS
+ * renders shading normals, NO_MIS leaves direct light to light sampling alone. Added here: REFERENCE selects the
+ * reference's camera estimator (shadeHit, gpu/wgsl/shade.wgsl.js:250-334: first hit, nL light samples, static
+ * surfaces read the node radiosity cache, posed bodies take one BSDF bounce) instead of the depth-D path tracer;
+ * it needs vt_radiosity_build. CULL_SHADOWS skips the shadow rays whose answer the hit triangle's three nodes
+ * give (cachedVisibility, :108-130). */
+#define VT_OPT_NO_INDIRECT 4u
+#define VT_OPT_DIRECT_ONLY 8u
+#define VT_OPT_NO_CAPSULE 32u
+#define VT_OPT_NORMALS 64u
+#define VT_OPT_NO_MIS 128u
+#define VT_OPT_REFERENCE (1u << 16)
+#define VT_OPT_CULL_SHADOWS (1u << 17)
+
+/* ---------------------------------------------------------------- node radiosity on the uploaded scene
+ * The reference's radiosity cache (gather table gpu/renderer.js:456-530, kernels.wgsl.js:271-344; direct light
+ * and Jacobi :348-461, renderer.js:707-852). Nodes are the scene's vertices; a vertex of a static-flagged,
+ * non-body triangle is a static node (its material: that of its first such triangle). Build: gather_k
+ * cosine-distributed rays per static node against the static scene (gatherDir), each entry the nearest vertex of
+ * the hit triangle (0xffffffff for an escape or a back face); a node is active when at most half its rays
+ * escape; entries at inactive nodes move to the hit triangle's highest-barycentric active vertex; nbr_k
+ * same-facing (n.n > 0.9) neighbours within smooth_radius with Gaussian weights (sigma smooth_sigma; 0 = none);
+ * the directional-albedo tables of every material (32 bins, 1024 VNDF Hammersley samples, bsdf.js:210-254). */
+typedef struct {
+  uint32_t gather_k;         /* gather rays per node (64) */
+  uint32_t node_samples;     /* emitter samples per node per update (4) */
+  uint32_t nbr_k;            /* smoothing neighbours (16, at most 32) */
+  float smooth_radius;       /* m (0.006) */
+  float smooth_sigma;        /* m (0.003; 0 = no smoothing) */
+} vt_radiosity_opts;
+typedef struct {
+  uint32_t nodes, static_nodes, active;
+  double ms_build;
+} vt_radiosity_info;
+int vt_radiosity_build(vt_ctx *ctx, const vt_radiosity_opts *opts, vt_radiosity_info *info);
+/* One node update (_ensureRadiosity / converge): with direct != 0, nodeDirect (node_samples emitter samples per
+ * active node at mid-exposure, shadow rays through the static scene and the posed bodies, the capsule visibility
+ * of its indirect light); then `iters` Jacobi steps (jacobiGather + jacobiUpdate, ping-pong source slots), warm
+ * started from the previous state. Enqueue only. */
+int vt_radiosity_update(vt_ctx *ctx, uint32_t direct, uint32_t iters);
+/* nodeSummary (gpu/renderer.js:824-852): the node state (16 floats per node: E_free.rgb + V, slot 1 source,
+ * slot 2 source (w = 1 for a solved node), B_dir.rgb + visibility code) into node_state (NULL: skipped; else
+ * nodes * 64 bytes) and the area-weighted mean irradiance eAvg over the active nodes, which the reference
+ * estimator's bounce from a body to another body reads; eAvg is also stored for rendering. Blocks. */
+int vt_radiosity_state(vt_ctx *ctx, float *node_state, size_t bytes, float e_avg[3]);
+/* The built tables, for inspection (blocks): which = VT_RAD_GATHER (nodes x gather_k uint32), VT_RAD_MISSES
+ * (nodes floats: escaping gather rays), VT_RAD_ACTIVE (active uint32 node ids), VT_RAD_NEIGHBOURS (active x nbr_k
+ * uint32 active indices), VT_RAD_WEIGHTS (active x nbr_k floats), VT_RAD_ALBEDO (materials x 32 bins x 8 floats),
+ * VT_RAD_HH (materials x 4 floats); bytes must equal the table's size. */
+enum { VT_RAD_GATHER = 0, VT_RAD_MISSES = 1, VT_RAD_ACTIVE = 2, VT_RAD_NEIGHBOURS = 3, VT_RAD_WEIGHTS = 4,
+       VT_RAD_ALBEDO = 5, VT_RAD_HH = 6 };
+int vt_radiosity_tables(vt_ctx *ctx, uint32_t which, void *dst, size_t bytes);
+/* The smoothing neighbours of vt_radiosity_build on caller-given nodes (host only, no device): pos and nrm n x 3
+ * floats; idx and w n x k (k <= 32), rows sorted by weight, 0-weight entries unused. */
+int vt_node_neighbours(const float *pos, const float *nrm, uint32_t n, uint32_t k, float radius, float sigma,
+                       uint32_t *idx, float *w);
+
+/* ---------------------------------------------------------------- node radiosity (host-given tables) */
 /* One Jacobi step of the reference's sparse gather form (jacobiGather + jacobiUpdate,
  * gpu/wgsl/kernels.wgsl.js:416-461; _submitJacobi gpu/renderer.js:729-749) on host-given tables. node_state
  * (16 floats per node) is read and written in place. */

@@ -4,6 +4,7 @@ ctypes bindings of liboracle.so (built from the C sources beside this file by `m
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this package, and only as the checker.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -70,6 +71,10 @@ def lib():
             "or_render_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
             "or_camera_ray": (None, [vp, u32, u32, u32, vp, vp]),
             "or_visible": (None, [vp, vp, vp, u32, f64, f64, vp]),
+            "or_gather_build": (None, [vp, vp, vp, u32, u32, vp, i32]),
+            "or_gather_remap": (None, [vp, vp, vp, u32, u32, vp, vp, i32]),
+            "or_node_direct": (None, [vp, vp, vp, vp, vp, u32, u32, vp, vp, i32]),
+            "or_capsule_vis": (f64, [vp, vp, vp]),
             "or_jacobi_dense": (i32, [vp, vp, u32, u32, f64, i32, vp, vp]),
             "or_jacobi_gather": (None, [u32, vp, vp, u32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
             "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
@@ -125,12 +130,16 @@ class Scene(C.Structure):
                 ("materials", C.c_void_p), ("mat_count", C.c_uint32), ("lights", C.c_void_p),
                 ("light_count", C.c_uint32), ("emit", C.c_void_p), ("emit_count", C.c_uint32),
                 ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32),
-                ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p)]
+                ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p),
+                ("node_state", C.c_void_p), ("alb", C.c_void_p), ("hh", C.c_void_p), ("slot", C.c_uint32),
+                ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4)]
 
 
 class Body(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("tri_count", C.c_uint32), ("q0", C.c_float * 4),
-                ("q1", C.c_float * 4), ("p0", C.c_float * 3), ("p1", C.c_float * 3), ("hidden", C.c_uint32)]
+                ("q1", C.c_float * 4), ("p0", C.c_float * 3), ("p1", C.c_float * 3), ("hidden", C.c_uint32),
+                ("occ_a", C.c_float * 3), ("occ_b", C.c_float * 3), ("occ_r", C.c_float), ("rho_eff", C.c_float),
+                ("bc_a", C.c_float * 4), ("bc_b", C.c_float * 4)]
 
 
 class Camera(C.Structure):
@@ -221,6 +230,13 @@ def scene_struct(sc):
                 B.q1[:] = [float(x) for x in P.get("q1", P["q0"])]
                 B.p0[:] = [float(x) for x in P["p0"]]
                 B.p1[:] = [float(x) for x in P.get("p1", P["p0"])]
+                B.occ_a[:] = [float(x) for x in P.get("occ_a", (0, 0, 0))]
+                B.occ_b[:] = [float(x) for x in P.get("occ_b", (0, 0, 0))]
+                B.occ_r = float(P.get("occ_r", 0.0))
+                B.rho_eff = float(P.get("rho_eff", 0.0))
+                ca, cb = body_capsule(t.reshape(-1, 16), P)
+                B.bc_a[:] = [float(x) for x in ca]
+                B.bc_b[:] = [float(x) for x in cb] + [0.0]
         keep["bodies"] = bodies
     else:
         keep["nodes"] = np.ascontiguousarray(sc["nodes"])
@@ -243,8 +259,75 @@ def scene_struct(sc):
             kn[light, : len(th), 1] = p
         keep["light_prof"], keep["knots"] = lp, kn
         s.light_prof, s.knots = ptr(lp), ptr(kn)
+    if sc.get("node_state") is not None:
+        for k in ("node_state", "alb", "hh"):
+            keep[k] = np.ascontiguousarray(sc[k], np.float32)
+        s.node_state, s.alb, s.hh = ptr(keep["node_state"]), ptr(keep["alb"]), ptr(keep["hh"])
+        s.slot = int(sc.get("slot", 1))
+        s.e_avg[:] = [float(x) for x in sc.get("e_avg", (0, 0, 0))]
+        s.emit_c[:] = [float(x) for x in sc.get("emit_c", emitter_centroid(sc))]
     s._keep = keep
     return s
+
+
+def _quat_mat(q):
+    x, y, z, w = np.asarray(q, np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _slerp(a, b, t):
+    a = np.asarray(a, np.float64) / np.linalg.norm(a)
+    b = np.asarray(b, np.float64) / np.linalg.norm(b)
+    c = a @ b
+    if c < 0:
+        b, c = -b, -c
+    if c > 0.9995:
+        q = a + (b - a) * t
+        return q / np.linalg.norm(q)
+    th = math.acos(c)
+    return (a * math.sin((1 - t) * th) + b * math.sin(t * th)) / math.sin(th)
+
+
+def body_capsule(tris, P):
+    """The bounding capsule a posed body is culled with (the library's: its box's longest axis posed at
+    mid-exposure, radius padded by the largest motion of its points over the exposure): (a.xyz + r, b.xyz)."""
+    tr = np.asarray(tris, np.float32).reshape(-1, 16)
+    v0 = tr[:, 0:3].astype(np.float64)
+    pts = np.concatenate([v0, v0 + tr[:, 4:7], v0 + tr[:, 8:11]]).astype(np.float32).astype(np.float64)
+    lo, hi = pts.min(0), pts.max(0)
+    q0 = np.asarray(P["q0"], np.float64)
+    q1 = np.asarray(P.get("q1", P["q0"]), np.float64)
+    p0 = np.asarray(P["p0"], np.float64)
+    p1 = np.asarray(P.get("p1", P["p0"]), np.float64)
+    q0n, q1n = q0 / np.linalg.norm(q0), q1 / np.linalg.norm(q1)
+    moving = bool(np.any(q0n.astype(np.float32) != q1n.astype(np.float32)) or np.any(p0 != p1))
+    ctr, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    rad = np.linalg.norm(ctr) + np.linalg.norm(half)
+    phi = 2 * math.acos(min(1.0, abs(float(q0n @ q1n)))) if moving else 0.0
+    ax = (0 if half[0] >= half[2] else 2) if half[0] >= half[1] else (1 if half[1] >= half[2] else 2)
+    ea, eb = ctr.copy(), ctr.copy()
+    ea[ax] -= half[ax]
+    eb[ax] += half[ax]
+    rr = math.sqrt(sum(half[k] ** 2 for k in range(3) if k != ax))
+    t = 0.5 if moving else 0.0
+    R = _quat_mat(_slerp(q0n, q1n, t))
+    pm = p0 + (p1 - p0) * t
+    wa, wb = R @ ea + pm, R @ eb + pm
+    motion = 1.01 * (2 * math.sin(phi / 4) * rad + 0.5 * np.linalg.norm(p1 - p0)) if moving else 0.0
+    scale = max(np.abs(lo).max(), np.abs(hi).max(), 1e-3)
+    return np.concatenate([wa, [rr * 1.0001 + motion + 4e-6 * scale + 1e-6]]), wb
+
+
+def emitter_centroid(sc):
+    """centroid and radius of the emitter triangles (gpu/scene_gpu.js:418-422)"""
+    e = np.asarray(sc["emit"], np.float32).reshape(-1, 16).astype(np.float64)
+    if e.shape[0] == 0:
+        return (0.0, 0.0, 0.0, 0.0)
+    cen = ((e[:, 0:3] + (e[:, 4:7] + e[:, 8:11]) / 3) * e[:, 7:8]).sum(0) / max(e[:, 7].sum(), 1e-300)
+    pts = np.concatenate([e[:, 0:3], e[:, 0:3] + e[:, 4:7], e[:, 0:3] + e[:, 8:11]])
+    return tuple(cen) + (float(np.linalg.norm(pts - cen, axis=1).max()),)
 
 
 def camera_struct(cam):
@@ -342,3 +425,118 @@ def camera_grad_tile(sc, cam, x0, y0, w, h, adj, rel_step=1e-6, spp_begin=0, spp
     lib().or_camera_grad_tile(C.byref(S), C.byref(Cm), x0, y0, w, h, spp_begin, spp_end, ptr(adj), rel_step, ptr(g),
                               ptr(changed), nthreads)
     return g, int(changed[0])
+
+
+# ---------------------------------------------------------------------------- node radiosity cache
+
+def node_inputs(sc):
+    """Per vertex: npos (pos + static flag), nnorm (unit normal), node material and area, as the library's
+    vt_radiosity_build derives them: static nodes are the vertices of static-flagged triangles of the static scene,
+    the material that of the vertex's own mesh (vertex word 3), a third of every such triangle's area."""
+    verts = np.asarray(sc["verts"], np.float32).reshape(-1, 8)
+    tris = np.asarray(sc["tris"], np.float32).reshape(-1, 16)
+    tu = tris.view(np.uint32)
+    meshes = np.asarray(sc["meshes"], np.uint32).reshape(-1, 4)
+    V = verts.shape[0]
+    sel = (meshes[tu[:, 12], 1] & 1) != 0
+    if int(sc.get("body_count", 0)):
+        sel &= np.asarray(sc["tri_body"], np.uint32) == 0xFFFFFFFF
+    ids = tu[sel][:, [3, 7, 11]]
+    stat = np.zeros(V, bool)
+    stat[ids.reshape(-1)] = True
+    P = verts[:, 0:3].astype(np.float64)
+    a = P[ids[:, 0]]
+    ar = 0.5 * np.linalg.norm(np.cross(P[ids[:, 1]] - a, P[ids[:, 2]] - a), axis=1) / 3
+    area = np.zeros(V)
+    for k in range(3):
+        np.add.at(area, ids[:, k], ar)
+    nmat = np.where(stat, meshes[verts.view(np.uint32)[:, 3] % max(meshes.shape[0], 1), 0], 0).astype(np.uint32)
+    n = verts[:, 4:7].astype(np.float64)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    n = np.where(ln > 0, n / np.where(ln > 0, ln, 1), np.array([0.0, 0.0, 1.0]))
+    npos = np.zeros((V, 4), np.float32)
+    npos[:, 0:3] = verts[:, 0:3]
+    npos[:, 3] = stat
+    nnorm = np.zeros((V, 4), np.float32)
+    nnorm[:, 0:3] = n
+    return dict(npos=npos, nnorm=nnorm, nmat=nmat, static=stat, area=area)
+
+
+def albedo_tables(sc, samples=1024, bins=32):
+    """FP64 albedo tables of every material (albedoTable, gpu/bsdf.js:210-254) in the device layout: per
+    material bins x [spec.rgb, -, diff.rgb, -], and the hemispherical albedo (4 floats per material)."""
+    mats = np.asarray(sc["materials"], np.float32).reshape(-1, 16)
+    alb = np.zeros((mats.shape[0], bins, 8), np.float32)
+    hh = np.zeros((mats.shape[0], 4), np.float32)
+    for m in range(mats.shape[0]):
+        M = mat_from_record(mats[m])
+        spec = np.zeros(bins * 3)
+        diff = np.zeros(bins * 3)
+        h = np.zeros(3)
+        hs = np.zeros(3)
+        lib().or_albedo_table(C.byref(M), bins, samples, ptr(spec), ptr(diff), ptr(h), ptr(hs))
+        alb[m, :, 0:3] = spec.reshape(bins, 3)
+        alb[m, :, 4:7] = diff.reshape(bins, 3)
+        hh[m, 0:3] = h
+    return alb, hh
+
+
+def gather_table(sc, ni, K, nthreads=8):
+    """gatherBuild + gatherCount (kernels.wgsl.js:283-333) over the static scene: (table V x K, misses)."""
+    S = scene_struct(sc)
+    V = ni["npos"].shape[0]
+    g = np.zeros(V * K, np.uint32)
+    lib().or_gather_build(C.byref(S), ptr(ni["npos"]), ptr(ni["nnorm"]), V, K, ptr(g), nthreads)
+    miss = np.where(ni["static"], (g.reshape(V, K) == 0xFFFFFFFF).sum(1), K).astype(np.float32)
+    return g, miss
+
+
+def gather_remap(sc, ni, K, gtable, miss, nthreads=8):
+    S = scene_struct(sc)
+    g = np.ascontiguousarray(gtable, np.uint32).copy()
+    lib().or_gather_remap(C.byref(S), ptr(ni["npos"]), ptr(ni["nnorm"]), ni["npos"].shape[0], K,
+                          ptr(np.ascontiguousarray(miss, np.float32)), ptr(g), nthreads)
+    return g
+
+
+def node_direct(sc, ni, active, samples, alb, state, nthreads=8):
+    """nodeDirect (kernels.wgsl.js:348-406) into state (V x 16, in place)."""
+    S = scene_struct(sc)
+    act = np.ascontiguousarray(active, np.uint32)
+    lib().or_node_direct(C.byref(S), ptr(ni["npos"]), ptr(ni["nnorm"]), ptr(ni["nmat"]), ptr(act), act.size,
+                         samples, ptr(np.ascontiguousarray(alb, np.float32)), ptr(state), nthreads)
+
+
+def jacobi_steps(sc, ni, active, gtable, K, nbr, nbr_w, alb, hh, state, slot, iters):
+    """`iters` steps of jacobiGather + jacobiUpdate (kernels.wgsl.js:416-461), ping-pong source slots; returns the
+    slot holding the last sources."""
+    verts = np.ascontiguousarray(np.asarray(sc["verts"], np.float32).reshape(-1, 8))
+    act = np.ascontiguousarray(active, np.uint32)
+    alb = np.asarray(alb, np.float32)
+    spec = np.ascontiguousarray(alb[:, :, 0:3])
+    diff = np.ascontiguousarray(alb[:, :, 4:7])
+    h3 = np.ascontiguousarray(np.asarray(hh, np.float32)[:, 0:3])
+    nb = np.ascontiguousarray(nbr, np.uint32)
+    nw = np.ascontiguousarray(nbr_w, np.float32)
+    nbr_k = nb.shape[1] if nb.ndim == 2 else 0
+    aux = np.zeros((max(act.size, 1), 4), np.float32)
+    g = np.ascontiguousarray(gtable, np.uint32)
+    for _ in range(iters):
+        lib().or_jacobi_gather(act.size, ptr(act), ptr(g), K, ptr(verts), ptr(ni["nmat"]), ptr(spec), ptr(diff),
+                               alb.shape[1], ptr(h3), ptr(state), slot, ptr(aux))
+        lib().or_jacobi_update(act.size, ptr(act), ptr(nb), ptr(nw), nbr_k, ptr(aux), ptr(ni["nmat"]), ptr(h3),
+                               ptr(state), 3 - slot)
+        slot = 3 - slot
+    return slot
+
+
+def e_avg(ni, active, state, hh):
+    """nodeSummary's eAvg (gpu/renderer.js:824-852): area-weighted E_free V + B_dir / A_hh over the active nodes."""
+    act = np.asarray(active, np.int64)
+    a = ni["area"][act][:, None]
+    st = np.asarray(state, np.float64)
+    h = np.maximum(np.asarray(hh, np.float64)[ni["nmat"][act], 0:3], 1e-3)
+    A = a.sum()
+    E = (a * st[act, 0:3] * st[act, 3:4]).sum(0)
+    Ed = (a * st[act, 12:15] / h).sum(0)
+    return (E + Ed) / A if A > 0 else np.zeros(3)

@@ -25,6 +25,14 @@
 #include <string.h>
 
 #define F_EMITTER 2u
+#define F_STATIC 1u
+#define O_NO_INDIRECT 4u
+#define O_DIRECT_ONLY 8u
+#define O_NO_CAPSULE 32u
+#define O_NORMALS 64u
+#define O_NO_MIS 128u
+#define O_REFERENCE (1u << 16)
+#define O_CULL_SHADOWS (1u << 17)
 static const float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
 static const float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
 static const float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
@@ -399,6 +407,340 @@ static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t
   return sample_radiance_ex(S, C, P, x, y, i, out, rays, visits, G, NULL, NULL);
 }
 
+/* ------------------------------------------------------------------ the reference camera estimator
+ * shadeHit / directLight / secondary / cachedVisibility / capsuleVis (gpu/wgsl/shade.wgsl.js:83-334) on the node
+ * cache of or_scene, in FP64. */
+static void albedo_at(const or_scene *S, uint32_t m, double mu, double out[3]) {
+  double x = mu * 32 - 0.5;
+  if (x < 0) x = 0;
+  if (x > 31) x = 31;
+  int i = (int)floor(x);
+  int j = i + 1 < 31 ? i + 1 : 31;
+  double f = x - i;
+  const float *b = S->alb + (size_t)m * 32 * 8;
+  for (int c = 0; c < 3; c++)
+    out[c] = ((double)b[8 * i + c] * (1 - f) + (double)b[8 * j + c] * f) + ((double)b[8 * i + 4 + c] * (1 - f) + (double)b[8 * j + 4 + c] * f);
+}
+
+static void body_pose_mid(const or_body *B, pose_t *P) { body_pose(B, 0.5, P); }
+
+double or_capsule_vis(const or_scene *S, const double p[3], const double n[3]) {
+  double v = 1;
+  for (uint32_t b = 0; b < S->body_count; b++) {
+    const or_body *B = &S->bodies[b];
+    if (B->hidden || !(B->occ_r > 0)) continue;
+    pose_t Q;
+    body_pose_mid(B, &Q);
+    double a[3], bb[3];
+    const double oa[3] = {B->occ_a[0], B->occ_a[1], B->occ_a[2]}, ob[3] = {B->occ_b[0], B->occ_b[1], B->occ_b[2]};
+    rot_apply(Q.R, oa, a);
+    rot_apply(Q.R, ob, bb);
+    for (int k = 0; k < 3; k++) { a[k] += Q.p[k]; bb[k] += Q.p[k]; }
+    v -= or_capsule_occ(p, n, a, bb, B->occ_r) * (1 - B->rho_eff);
+  }
+  return v < 0 ? 0 : v > 1 ? 1 : v;
+}
+
+/* segment o + t d, t in [0, 1], within r of segment a-b (segNearCapsule, trace.wgsl.js:122-136) */
+static int seg_near(const double o[3], const double d[3], const double a[3], const double b[3], double r) {
+  double v[3], w[3];
+  for (int k = 0; k < 3; k++) { v[k] = b[k] - a[k]; w[k] = o[k] - a[k]; }
+  const double A = dot3(d, d), B = dot3(d, v), Cc = dot3(v, v), D = dot3(d, w), E = dot3(v, w);
+  const double den = A * Cc - B * B;
+  double s = 0;
+  if (den > 1e-20 * A * Cc) s = fmin(fmax((B * E - Cc * D) / den, 0), 1);
+  double t = Cc > 1e-30 ? (B * s + E) / Cc : 0;
+  if (t < 0) { t = 0; s = fmin(fmax(-D / fmax(A, 1e-30), 0), 1); }
+  else if (t > 1) { t = 1; s = fmin(fmax((B - D) / fmax(A, 1e-30), 0), 1); }
+  double q[3];
+  for (int k = 0; k < 3; k++) q[k] = (o[k] + d[k] * s) - (a[k] + v[k] * t);
+  return dot3(q, q) <= r * r;
+}
+
+static double cached_visibility(const or_scene *S, const surf_t *s) {
+  const double a = S->node_state[16 * (size_t)s->vi[0] + 15], b = S->node_state[16 * (size_t)s->vi[1] + 15],
+               c = S->node_state[16 * (size_t)s->vi[2] + 15];
+  if (!(a == b && b == c && (a == 1.0 || a == 0.0))) return -1;
+  const double d[3] = {S->emit_c[0] - s->p[0], S->emit_c[1] - s->p[1], S->emit_c[2] - s->p[2]};
+  for (uint32_t i = 0; i < S->body_count; i++) {
+    const or_body *B = &S->bodies[i];
+    if (B->hidden || ((S->shadow_skip >> i) & 1u)) continue;
+    const double ca[3] = {B->bc_a[0], B->bc_a[1], B->bc_a[2]}, cb[3] = {B->bc_b[0], B->bc_b[1], B->bc_b[2]};
+    if (seg_near(s->p, d, ca, cb, (double)B->bc_a[3] + S->emit_c[3] + 5e-4)) return -1;
+  }
+  return a;
+}
+
+/* renormalised barycentric weights over the solved nodes, then state word k interpolated (nodeWeights,
+ * nodeEfree, nodeLout: shade.wgsl.js:211-231) */
+static void node_interp(const or_scene *S, const surf_t *s, int k, double out[3]) {
+  double w[3], sw = 0;
+  for (int j = 0; j < 3; j++) { w[j] = s->b[j] * S->node_state[16 * (size_t)s->vi[j] + 4 * S->slot + 3]; sw += w[j]; }
+  for (int j = 0; j < 3; j++) w[j] = sw <= 1e-6 ? s->b[j] : w[j] / sw;
+  for (int c = 0; c < 3; c++) {
+    out[c] = 0;
+    for (int j = 0; j < 3; j++) out[c] += w[j] * S->node_state[16 * (size_t)s->vi[j] + 4 * k + c];
+  }
+}
+
+static int is_static(const or_hit *h, const surf_t *s) { return (s->flags & F_STATIC) && h->inst < 0; }
+
+static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], double time,
+                         const double u[3], int mis, double nl, double cached, uint64_t *rays, double out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  double pl[3];
+  const float *e = sample_emitter(S, u, pl);
+  double wi[3] = {pl[0] - s->p[0], pl[1] - s->p[1], pl[2] - s->p[2]};
+  const double d2 = dot3(wi, wi), dist = sqrt(d2);
+  for (int k = 0; k < 3; k++) wi[k] /= dist;
+  const double en[3] = {e[12], e[13], e[14]};
+  const double cosE = -dot3(wi, en);
+  if (cosE <= 0) return;
+  const double cosS = dot3(s->ns, wi);
+  if (cosS <= 0 || dot3(s->ng, wi) <= 0) return;
+  uint32_t lid;
+  memcpy(&lid, e + 11, 4);
+  const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
+  const double pe = light_profile(S, lid, mwi);
+  const float *L0 = S->lights + 4 * (size_t)lid;
+  const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
+  if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) return;
+  double f[3];
+  or_eval_bsdf(M, s->ns, wi, wo, f);
+  if (maxc(f) <= 0) return;
+  if (cached == 0) return;
+  if (cached < 0) {
+    double so[3];
+    offset_ray(s->p, s->ng, wi, so);
+    or_hit sh;
+    trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, S->shadow_skip, &sh, NULL);
+    (*rays)++;
+    if (sh.tri >= 0) return;
+  }
+  const double pW = d2 / (cosE * S->emit_area);
+  double w = 1;
+  if (mis) {
+    const double pb = or_pdf_bsdf(M, s->ns, wo, wi);
+    w = nl * nl * pW * pW / (nl * nl * pW * pW + pb * pb);
+  }
+  for (int k = 0; k < 3; k++) out[k] = f[k] * Le[k] * cosS * w / pW;
+}
+
+static void secondary(const or_scene *S, const double o[3], const double l[3], double time, double pdfB,
+                      uint32_t opts, double nl, uint64_t *rays, double out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  or_hit h;
+  pose_t pose;
+  trace_scene(S, o, l, 0, INFINITY, 0, time, 0u, &h, &pose);
+  (*rays)++;
+  if (h.tri < 0) return;
+  surf_t s;
+  surface(S, &h, &pose, o, l, &s);
+  if (s.flags & F_EMITTER) {
+    if (opts & O_NO_MIS) return;
+    double ne[3];
+    emitter_normal(S, &s, ne);
+    const double ml[3] = {-l[0], -l[1], -l[2]};
+    const double cosE = dot3(ml, ne);
+    if (cosE <= 0) return;
+    const double pe = light_profile(S, s.light, ml);
+    const double pL = h.t * h.t / (cosE * S->emit_area);
+    const double w = pdfB * pdfB / (pdfB * pdfB + nl * nl * pL * pL);
+    for (int k = 0; k < 3; k++) out[k] = S->lights[4 * (size_t)s.light + k] * pe * w;
+    return;
+  }
+  if (opts & (O_DIRECT_ONLY | O_NO_INDIRECT)) return;
+  if (is_static(&h, &s)) {
+    const double ml[3] = {-l[0], -l[1], -l[2]};
+    double mu = dot3(s.ns, ml);
+    mu = mu < 0 ? 0 : mu > 1 ? 1 : mu;
+    double A[3], Ls[3];
+    albedo_at(S, s.mat, mu, A);
+    node_interp(S, &s, (int)S->slot, Ls);
+    for (int k = 0; k < 3; k++) out[k] = A[k] * Ls[k];
+    return;
+  }
+  for (int k = 0; k < 3; k++) out[k] = S->hh[4 * (size_t)s.mat + k] * S->e_avg[k] / M_PI;
+}
+
+static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y, uint32_t i,
+                         double out[3], uint64_t *rays) {
+  out[0] = out[1] = out[2] = 0;
+  const float tk = or_kr(i, GOLD_F, P->s[2]);
+  const double time = (double)(C->shutter[0] + (C->shutter[1] - C->shutter[0]) * tk);
+  double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
+  cam_dir(C, P, x, y, i, d);
+  or_hit h;
+  pose_t pose;
+  trace_scene(S, o, d, 0, INFINITY, 0, time, C->skip_bodies, &h, &pose);
+  (*rays)++;
+  if (h.tri < 0) return;
+  surf_t s;
+  surface(S, &h, &pose, o, d, &s);
+  const double wo[3] = {-d[0], -d[1], -d[2]};
+  const uint32_t opts = C->flags;
+  if (opts & O_NORMALS) {
+    for (int k = 0; k < 3; k++) out[k] = s.ns[k] * 0.5 + 0.5;
+    return;
+  }
+  if (s.flags & F_EMITTER) {
+    double ne[3];
+    emitter_normal(S, &s, ne);
+    if (dot3(wo, ne) <= 0) return;
+    const double pe = light_profile(S, s.light, wo);
+    for (int k = 0; k < 3; k++) out[k] = S->lights[4 * (size_t)s.light + k] * pe;
+    return;
+  }
+  or_mat M;
+  or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
+  const int stat = is_static(&h, &s);
+  const int mis = !stat && !(opts & O_NO_MIS);
+  const uint32_t nl = C->light_samples < 1 ? 1 : C->light_samples > 16 ? 16 : C->light_samples;
+  const double cached = (stat && (opts & O_CULL_SHADOWS)) ? cached_visibility(S, &s) : -1;
+  double L[3] = {0, 0, 0};
+  if (S->emit_count > 0)
+    for (uint32_t j = 0; j < nl; j++) {
+      const uint32_t li = i * nl + j;
+      const double u[3] = {or_kr(li, KL0, P->s[3]), or_kr(li, KL1, P->s[4]), or_kr(li, KL2, P->s[5])};
+      double c[3];
+      direct_light(S, &s, &M, wo, time, u, mis, nl, cached, rays, c);
+      for (int k = 0; k < 3; k++) L[k] += c[k];
+    }
+  for (int k = 0; k < 3; k++) L[k] /= nl;
+  if (stat) {
+    if (!(opts & (O_NO_INDIRECT | O_DIRECT_ONLY))) {
+      double E[3], A[3];
+      node_interp(S, &s, 0, E);
+      const double V = (opts & O_NO_CAPSULE) ? 1.0 : or_capsule_vis(S, s.p, s.ns);
+      double mu = dot3(s.ns, wo);
+      if (mu < 0) mu = 0;
+      albedo_at(S, s.mat, mu, A);
+      for (int k = 0; k < 3; k++) L[k] += E[k] * V / M_PI * A[k];
+    }
+  } else {
+    const double ub[3] = {or_kr(i, KB0, P->s[6]), or_kr(i, KB1, P->s[7]), or_kr(i, KB2, P->s[8])};
+    double l[3];
+    const double pdf = or_sample_bsdf(&M, s.ns, wo, ub, l);
+    if (pdf > 0 && dot3(s.ng, l) > 0) {
+      double f[3], so[3], sec[3];
+      or_eval_bsdf(&M, s.ns, l, wo, f);
+      const double c = dot3(s.ns, l) / pdf;
+      offset_ray(s.p, s.ng, l, so);
+      secondary(S, so, l, time, pdf, opts, nl, rays, sec);
+      for (int k = 0; k < 3; k++) L[k] += f[k] * c * sec[k];
+    }
+  }
+  memcpy(out, L, sizeof L);
+}
+
+/* ------------------------------------------------------------------ node radiosity cache */
+static uint32_t gather_vertex(const or_scene *S, const float *npos, const float *nnorm, uint32_t i, uint32_t j,
+                              uint32_t K, double bb[3], uint32_t vi[3], int *hit) {
+  const double p[3] = {npos[4 * (size_t)i], npos[4 * (size_t)i + 1], npos[4 * (size_t)i + 2]};
+  const float nf[3] = {nnorm[4 * (size_t)i], nnorm[4 * (size_t)i + 1], nnorm[4 * (size_t)i + 2]};
+  float df[3];
+  or_gather_dir(i, j, K, nf, df);
+  const double dir[3] = {df[0], df[1], df[2]};
+  const double o[3] = {p[0] + nf[0] * 2e-6, p[1] + nf[1] * 2e-6, p[2] + nf[2] * 2e-6};
+  or_hit h;
+  or_traverse(S->nodes, S->tris, o, dir, 0, INFINITY, 0, 0, &h);
+  *hit = h.tri >= 0;
+  if (h.tri < 0) return 0xffffffffu;
+  const uint32_t *ru = (const uint32_t *)(S->tris + (size_t)h.tri * 16);
+  vi[0] = ru[3]; vi[1] = ru[7]; vi[2] = ru[11];
+  bb[0] = 1 - h.u - h.v; bb[1] = h.u; bb[2] = h.v;
+  double nv = 0;
+  for (int k = 0; k < 3; k++) {
+    double c = 0;
+    for (int q = 0; q < 3; q++) c += bb[q] * nnorm[4 * (size_t)vi[q] + k];
+    nv += c * dir[k];
+  }
+  if (!(nv < 0)) return 0xffffffffu;
+  if (bb[0] >= bb[1] && bb[0] >= bb[2]) return vi[0];
+  return bb[1] >= bb[2] ? vi[1] : vi[2];
+}
+
+void or_gather_build(const or_scene *S, const float *npos, const float *nnorm, uint32_t V, uint32_t K,
+                     uint32_t *gtable, int nthreads) {
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 64)
+  for (int64_t r = 0; r < (int64_t)V * K; r++) {
+    const uint32_t i = (uint32_t)(r / K), j = (uint32_t)(r % K);
+    double bb[3];
+    uint32_t vi[3];
+    int hit;
+    gtable[r] = npos[4 * (size_t)i + 3] > 0.5f ? gather_vertex(S, npos, nnorm, i, j, K, bb, vi, &hit) : 0xffffffffu;
+  }
+}
+
+void or_gather_remap(const or_scene *S, const float *npos, const float *nnorm, uint32_t V, uint32_t K,
+                     const float *miss, uint32_t *gtable, int nthreads) {
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 64)
+  for (int64_t r = 0; r < (int64_t)V * K; r++) {
+    const uint32_t v0 = gtable[r];
+    if (v0 == 0xffffffffu || miss[v0] <= 0.5 * K) continue;
+    const uint32_t i = (uint32_t)(r / K), j = (uint32_t)(r % K);
+    double bb[3];
+    uint32_t vi[3];
+    int hit;
+    gather_vertex(S, npos, nnorm, i, j, K, bb, vi, &hit);
+    if (!hit) continue;
+    uint32_t best = v0;
+    double bw = -1;
+    for (int q = 0; q < 3; q++)
+      if (miss[vi[q]] <= 0.5 * K && bb[q] > bw) { best = vi[q]; bw = bb[q]; }
+    gtable[r] = best;
+  }
+}
+
+void or_node_direct(const or_scene *S, const float *npos, const float *nnorm, const uint32_t *nmat,
+                    const uint32_t *active, uint32_t n_active, uint32_t samples, const float *alb, float *state,
+                    int nthreads) {
+  or_scene S2 = *S;
+  S2.alb = alb;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 16)
+  for (int64_t k = 0; k < (int64_t)n_active; k++) {
+    const uint32_t i = active[k];
+    const double p[3] = {npos[4 * (size_t)i], npos[4 * (size_t)i + 1], npos[4 * (size_t)i + 2]};
+    const double n[3] = {nnorm[4 * (size_t)i], nnorm[4 * (size_t)i + 1], nnorm[4 * (size_t)i + 2]};
+    const uint32_t m = nmat[i];
+    const uint32_t hs = or_pcg(i * 2246822519u + 3u);
+    const float s0 = or_u01(or_pcg(hs)), s1 = or_u01(or_pcg(hs + 1u)), s2 = or_u01(or_pcg(hs + 2u));
+    double B[3] = {0, 0, 0};
+    uint32_t nlit = 0, nvis = 0;
+    const double o[3] = {p[0] + n[0] * 2e-6, p[1] + n[1] * 2e-6, p[2] + n[2] * 2e-6};
+    for (uint32_t s = 0; s < samples && S->emit_count > 0; s++) {
+      const double u[3] = {or_kr(s, KL0, s0), or_kr(s, KL1, s1), or_kr(s, KL2, s2)};
+      double pl[3];
+      const float *e = sample_emitter(S, u, pl);
+      double wi[3] = {pl[0] - p[0], pl[1] - p[1], pl[2] - p[2]};
+      const double d2 = dot3(wi, wi), dist = sqrt(d2);
+      for (int q = 0; q < 3; q++) wi[q] /= dist;
+      const double en[3] = {e[12], e[13], e[14]};
+      const double cosE = -dot3(wi, en), cosN = dot3(n, wi);
+      if (cosE <= 0 || cosN <= 0) continue;
+      uint32_t lid;
+      memcpy(&lid, e + 11, 4);
+      const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
+      const double pe = light_profile(S, lid, mwi);
+      const float *L0 = S->lights + 4 * (size_t)lid;
+      if (fmax(L0[0], fmax(L0[1], L0[2])) * pe <= 0) continue;
+      nlit++;
+      or_hit sh;
+      trace_scene(S, o, wi, 0, dist - 2e-5, 1, 0.5, S->shadow_skip, &sh, NULL);
+      if (sh.tri >= 0) continue;
+      nvis++;
+      double A[3];
+      albedo_at(&S2, m, cosN, A);
+      for (int q = 0; q < 3; q++) B[q] += L0[q] * pe * (cosE * cosN * S->emit_area / d2) * A[q];
+    }
+    const double inv = 1.0 / (samples ? samples : 1);
+    float *st = state + 16 * (size_t)i;
+    for (int q = 0; q < 3; q++) st[12 + q] = (float)(B[q] * inv);
+    st[15] = nlit == 0 ? -1.0f : nvis == nlit ? 1.0f : nvis == 0 ? 0.0f : 0.5f;
+    st[3] = (float)or_capsule_vis(S, p, n);
+  }
+}
+
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                     uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads) {
   uint64_t nr = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
@@ -412,7 +754,8 @@ void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t
       uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0};
       for (uint32_t i = spp_begin; i < spp_end; i++) {
         double L[3];
-        sample_radiance(S, C, &P, x, y, i, L, &r, vv, NULL);
+        if ((C->flags & O_REFERENCE) && S->node_state) ref_radiance(S, C, &P, x, y, i, L, &r);
+        else sample_radiance(S, C, &P, x, y, i, L, &r, vv, NULL);
         for (int k = 0; k < 3; k++) acc[k] += L[k];
       }
       double inv = spp_end > spp_begin ? 1.0 / (spp_end - spp_begin) : 0;

@@ -144,6 +144,12 @@ typedef struct {
   /* angular emission profiles (optional, NULL = every light Lambertian): per light 4 words (unit axis xyz, knot
    * count as uint32); knots: 64 (theta, p) pairs per light. L_e(w) = radiance * or_profile(theta(w, axis)). */
   const float *light_prof; const float *knots;
+  /* node radiosity cache for the reference estimator (NULL: none): 16 floats per vertex (E_free + V, slot-1 and
+   * slot-2 sources, B_dir + visibility code); albedo tables, 32 bins x 8 floats [spec.rgb, -, diff.rgb, -] per
+   * material; hemispherical albedo, 4 floats per material; the current source slot; eAvg; the emitters'
+   * centroid + radius (shadow culling) */
+  const float *node_state; const float *alb; const float *hh;
+  uint32_t slot; float e_avg[3]; float emit_c[4];
 } or_scene;
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose
@@ -152,6 +158,10 @@ typedef struct {
 typedef struct or_body {
   const float *nodes; const float *tris; uint32_t tri_count;
   float q0[4], q1[4], p0[3], p1[3]; uint32_t hidden;
+  /* occlusion capsule in the body frame (radius occ_r, 0 = none) and rho_eff (capsuleVis); the bounding capsule
+   * in the world (a.xyz + radius, b.xyz) that the shadow culling tests */
+  float occ_a[3], occ_b[3], occ_r, rho_eff;
+  float bc_a[4], bc_b[4];
 } or_body;
 typedef struct {
   float R[9];            /* columns: camera x, y, z axes in the world frame */
@@ -184,6 +194,23 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
  * point k (n x 3 floats), the posed bodies at exposure time `time` and the camera's skip_bodies ignored */
 void or_visible(const or_scene *S, const or_camera *C, const float *pts, uint32_t n, double tol, double time,
                 uint8_t *vis);
+/* ---------------------------------------------------------------- node radiosity cache (the reference estimator's)
+ * nodes are vertices: npos 4 floats each (position, w = 1 for a static node), nnorm 4 floats (unit normal). */
+/* gatherBuild (kernels.wgsl.js:283-306) against the static BVH: gtable[i K + j] = nearest vertex of the hit
+ * triangle or 0xffffffff (escape, back face, or a vertex that is no static node) */
+void or_gather_build(const or_scene *S, const float *npos, const float *nnorm, uint32_t V, uint32_t K,
+                     uint32_t *gtable, int nthreads);
+/* gatherRemap (:308-324): entries at inactive nodes (miss > K/2) move to the hit triangle's highest-barycentric
+ * active vertex */
+void or_gather_remap(const or_scene *S, const float *npos, const float *nnorm, uint32_t V, uint32_t K,
+                     const float *miss, uint32_t *gtable, int nthreads);
+/* nodeDirect (:348-406) of the listed active nodes: state[16 i + 12..15] = B_dir + visibility code,
+ * state[16 i + 3] = capsule visibility (posed bodies at mid-exposure); alb as in or_scene */
+void or_node_direct(const or_scene *S, const float *npos, const float *nnorm, const uint32_t *nmat,
+                    const uint32_t *active, uint32_t n_active, uint32_t samples, const float *alb, float *state,
+                    int nthreads);
+/* capsuleVis (shade.wgsl.js:83-94) at a world point with normal n over the scene's visible bodies */
+double or_capsule_vis(const or_scene *S, const double p[3], const double n[3]);
 void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float o[3], float d[3]);
 
 /* ---------------------------------------------------------------- radiosity */

@@ -14,7 +14,8 @@ async function main() {
     lights: f32('lights'), emit: f32('emit'), emitArea: man.emitArea };
   if (man.bodies) Object.assign(scene, { triBody: u32('triBody'), bodies: man.bodies });
   else scene.nodes = f32('nodes');
-  const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed });
+  const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed,
+    estimator: man.estimator || 'path' });
   if (man.poses0) r.setPoses(man.poses0, man.poses1 || null);
   const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0 }, shutter: man.shutter });
   await out.done;
@@ -27,6 +28,10 @@ async function main() {
   if (man.visiblePoints) {
     const vis = r.visible(man.cameras[0].name, man.visiblePoints, { tol: 2e-4, time: 0.5 });
     fs.writeFileSync(path.join(dir, 'visible.json'), JSON.stringify(vis));
+  }
+  if (man.estimator === 'reference') {
+    const s = await r.nodeSummary();
+    fs.writeFileSync(path.join(dir, 'summary.json'), JSON.stringify({ eAvg: s.eAvg, active: s.activeNodes }));
   }
   const t = await r.timings();
   console.log(JSON.stringify({ rays: t.last.rays, overflows: t.last.stackOverflows }));

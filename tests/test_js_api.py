@@ -32,7 +32,8 @@ def test_module_exports_the_renderer_api():
     assert {"TwinRenderer", "DEFAULTS", "PRESETS", "OPTION_BITS", "lensParams", "fromPrepared", "RenderFunction",
             "rotateColumns", "quatFromMat", "STATIC_BODY"} <= set(d["exp"])
     for m in ("renderCamera", "renderFrame", "sensor", "readback", "setPoses", "project", "timings", "destroy", "idle",
-              "backward", "autotune", "updateMaterials", "updateLights", "setCameraParams"):
+              "backward", "autotune", "updateMaterials", "updateLights", "setCameraParams", "converge", "nodeSummary",
+              "visible", "setLightProfile"):
         assert m in d["methods"], m
     assert d["create"] == "function"
 
@@ -171,14 +172,16 @@ def test_from_prepared_bodies():
     group's BVH is built at upload), and the bodies keep their keys and kinds."""
     S = ("{nodes: new Float32Array(16), tris: new Float32Array(16 * 5), verts: new Float32Array(24), "
          "meshTable: [{mat: 0, flags: 1}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, roughness: 0.5}], "
-         "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2, owner: 'CAM2'}, {body: 'scope', kind: 'fixed', triOff: 4, "
-         "triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true, axis: [0, 0, -1], "
+         "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2, owner: 'CAM2', rhoEff: 0.3, occCapsule: {a: [0, 0, 0], "
+         "b: [0, 0, 0.1], r: 0.01}}, {body: 'scope', kind: 'fixed', triOff: 4, triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true, axis: [0, 0, -1], "
          "theta: [0, 0.5], p: [1, 0]}}")
     out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({tb: Array.from(u.triBody), "
                 "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip, profiles: u.profiles})); })" % (MODULE, S))
     d = json.loads(out)
     assert d["tb"] == [0xFFFFFFFF, 0, 0, 0xFFFFFFFF, 1] and d["nodes"]
-    assert d["bodies"] == [{"key": "arm", "kind": "body", "owner": "CAM2"}, {"key": "scope", "kind": "fixed", "owner": None}]
+    assert d["bodies"] == [{"key": "arm", "kind": "body", "owner": "CAM2", "rhoEff": 0.3,
+                            "occ": {"a": [0, 0, 0], "b": [0, 0, 0.1], "r": 0.01}},
+                           {"key": "scope", "kind": "fixed", "owner": None, "rhoEff": 0, "occ": None}]
     assert d["shadowSkip"] == 2          # the scope's bit: shadow rays leave through its housing
     assert d["profiles"] == [{"light": 0, "axis": [0, 0, -1], "theta": [0, 0.5], "p": [1, 0]}]
 
@@ -236,3 +239,40 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
     assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
     js_vis = np.array(json.loads((tmp_path / "visible.json").read_text()))
     assert np.array_equal(js_vis, vis) and 0 < vis.sum() < vis.size
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_reference_estimator_frame_equals_c_abi(tmp_path):
+    """estimator 'reference' through TwinRenderer (node cache built on first use, direct light + 96 Jacobi steps,
+    shadow culling on, as the reference's DEFAULTS) against the same frame through the C ABI."""
+    sc, cams = scenes.c1()
+    cam = dict(cams[0], spp=4)
+    for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    sp = scenes.default_sensor()
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=1, seed=cam["seed"], estimator="reference",
+               cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
+                             pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
+               sensor=dict(m=[float(v) for v in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
+                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
+                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    js_lin = np.fromfile(tmp_path / "CAM1.linear.bin", np.float32).reshape(cam["height"], cam["width"], 4)
+    summ = json.loads((tmp_path / "summary.json").read_text())
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    info = ctx.radiosity_build()
+    ctx.radiosity_update(True, 96)
+    cam.update(cam_index=0, max_depth=1, flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    _, e = ctx.radiosity_state()
+    ctx.close()
+    assert summ["active"] == info["active"] and np.allclose(summ["eAvg"], e, rtol=1e-6)
+    assert np.array_equal(js_lin, lin)

@@ -1,0 +1,285 @@
+"""The node radiosity cache and the reference camera estimator that reads it (gpu/renderer.js:456-530, 707-852;
+gpu/wgsl/kernels.wgsl.js:271-461; gpu/wgsl/shade.wgsl.js:57-334).
+
+Pins (CPU): the smoothing neighbours against brute force; the whole oracle pipeline (gather table, node direct
+light, Jacobi, reference estimator) inside a closed Lambertian sphere with an emitting cap against the
+infinite-bounce closed form rho Le f / (1 - rho (1 - f)) that a converged node cache must reproduce.
+GPU parity, stage by stage against the oracle fed the GPU's own inputs: albedo tables, gather table and activity,
+node direct light, Jacobi steps, the reference estimator's image (static scene; posed bodies with occlusion
+capsules; option bits; shadow culling); and the sphere's closed form through the GPU pipeline.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import lib, scenes
+
+OBVH = lambda p, i: (lambda b: (b["nodes"], b["order"]))(oracle.build_bvh(p, i))
+
+
+def _brute_neighbours(pos, nrm, k, R, sg):
+    n = pos.shape[0]
+    idx = np.zeros((n, k), np.uint32)
+    w = np.zeros((n, k), np.float32)
+    for a in range(n):
+        d2 = np.sum((pos.astype(np.float64) - pos[a]) ** 2, axis=1)
+        ok = (d2 <= R * R) & (nrm.astype(np.float64) @ nrm[a].astype(np.float64) >= 0.9)
+        cand = np.nonzero(ok)[0]
+        ww = np.exp(-0.5 * d2[cand] / (sg * sg))
+        order = np.lexsort((cand, -ww))[:k]
+        idx[a, : order.size] = cand[order]
+        w[a, : order.size] = ww[order]
+    return idx, w
+
+
+def test_neighbours_match_brute_force():
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0, 0.02, size=(600, 3)).astype(np.float32)
+    pos[:300, 2] = 0.0                                  # a plane and a cloud
+    nrm = np.zeros((600, 3), np.float32)
+    nrm[:, 2] = 1.0
+    nrm[::7] = [0.0, 1.0, 0.0]                          # some face away
+    idx, w = lib.node_neighbours(pos, nrm, 16, 0.006, 0.003)
+    bi, bw = _brute_neighbours(pos, nrm, 16, 0.006, 0.003)
+    assert np.array_equal(idx[bw > 0], bi[bw > 0])
+    assert np.allclose(w, bw, rtol=1e-6, atol=0)
+    i0, w0 = lib.node_neighbours(pos, nrm, 16, 0.006, 0.0)      # no smoothing: every node alone
+    assert np.array_equal(i0[:, 0], np.arange(600)) and np.all(w0[:, 0] == 1) and np.all(w0[:, 1:] == 0)
+
+
+def _sphere(rho=0.6, Le=4.0, n_theta=24, n_phi=48, cap_rows=5):
+    from test_oracle_pins import furnace_sphere
+    sc, cam, _ = furnace_sphere(rho=rho, Le=Le, n_theta=n_theta, n_phi=n_phi, cap_rows=cap_rows, depth=1)
+    P, I, M = sc["pos"], sc["idx"], sc["tri_mesh"]
+    a, b, c = P[I[:, 0]], P[I[:, 1]], P[I[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    f = area[M == 1].sum() / area.sum()
+    cam = dict(cam, flags=lib.VT_OPT_REFERENCE, spp=64)
+    return sc, cam, rho * Le * f / (1 - rho * (1 - f))
+
+
+def _oracle_cache(sc, K, samples, nbr_k=16, radius=0.006, sigma=0.0, iters=0):
+    ni = oracle.node_inputs(sc)
+    alb, hh = oracle.albedo_tables(sc)
+    g, miss = oracle.gather_table(sc, ni, K)
+    active = np.nonzero(ni["static"] & (miss <= 0.5 * K))[0].astype(np.uint32)
+    g = oracle.gather_remap(sc, ni, K, g, miss)
+    pos = ni["npos"][active, :3]
+    nrm = ni["nnorm"][active, :3]
+    nb, nw = lib.node_neighbours(pos, nrm, nbr_k, radius, sigma)
+    st = np.zeros((ni["npos"].shape[0], 16), np.float32)
+    oracle.node_direct(sc, ni, active, samples, alb, st)
+    slot = oracle.jacobi_steps(sc, ni, active, g, K, nb, nw, alb, hh, st, 1, iters)
+    return dict(ni=ni, alb=alb, hh=hh, g=g, miss=miss, active=active, nb=nb, nw=nw, state=st, slot=slot)
+
+
+def test_oracle_reference_estimator_sphere_closed_form():
+    """Converged, the cache carries every bounce: the reference estimator's pixels (NEE + E_free A / pi) equal the
+    infinite-bounce furnace value (gather rays K = 256, 64 emitter samples per node, no smoothing)."""
+    sc, cam, want = _sphere()
+    cc = _oracle_cache(sc, 256, 64, iters=60)
+    sc2 = dict(sc, node_state=cc["state"], alb=cc["alb"], hh=cc["hh"], slot=cc["slot"])
+    img, _, _ = oracle.render_tile(sc2, cam, 16, 16, 32, 32)
+    got = img[:, :, 0].mean()
+    print("reference estimator in the sphere: %.5f, closed form %.5f" % (got, want))
+    assert got == pytest.approx(want, rel=0.02)
+    # the path tracer of the same scene to depth 32 agrees with both
+    pt, _, _ = oracle.render_tile(sc, dict(cam, flags=0, max_depth=32, spp=64), 16, 16, 32, 32)
+    assert pt[:, :, 0].mean() == pytest.approx(want, rel=0.02)
+
+
+# ---------------------------------------------------------------------------- GPU parity
+
+def _gpu_cache(ctx, sc, K=64, samples=4, nbr_k=16, radius=0.006, sigma=0.003, iters=0, poses=None):
+    ctx.upload_scene(sc)
+    if poses is not None:
+        ctx.set_instances(poses)
+    info = ctx.radiosity_build(K, samples, nbr_k, radius, sigma)
+    V = np.asarray(sc["verts"]).size // 8
+    A = info["active"]
+    n_mat = np.asarray(sc["materials"]).size // 16
+    t = dict(info=info,
+             g=ctx.radiosity_tables(0, V * K, np.uint32), miss=ctx.radiosity_tables(1, V, np.float32),
+             active=ctx.radiosity_tables(2, A, np.uint32), nb=ctx.radiosity_tables(3, (A, nbr_k), np.uint32),
+             nw=ctx.radiosity_tables(4, (A, nbr_k), np.float32), alb=ctx.radiosity_tables(5, (n_mat, 32, 8), np.float32),
+             hh=ctx.radiosity_tables(6, (n_mat, 4), np.float32))
+    return t
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lib.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.gpu
+def test_gpu_albedo_tables_match_oracle(ctx):
+    sc, _ = scenes.c1(bvh=OBVH)
+    mats = [scenes.material_record(k, (0.7, 0.5, 0.3), roughness=r, ior=1.45)
+            for k in (scenes.KIND_DIELECTRIC, scenes.KIND_CONDUCTOR, scenes.KIND_COATED, scenes.KIND_LAMBERT)
+            for r in (0.05, 0.3, 0.8)]
+    sc["materials"] = np.concatenate([np.asarray(sc["materials"]).reshape(-1, 16)] + [m[None] for m in mats]).reshape(-1)
+    t = _gpu_cache(ctx, sc)
+    alb, hh = oracle.albedo_tables(sc)
+    err = np.abs(t["alb"] - alb).max()
+    print("albedo tables: max |GPU - FP64| %.2e, hh %.2e" % (err, np.abs(t["hh"] - hh).max()))
+    assert err < 2e-5 and np.abs(t["hh"] - hh).max() < 2e-5
+
+
+@pytest.mark.gpu
+def test_gpu_gather_table_and_cache_match_oracle(ctx):
+    """c1: the gather table and activity, node direct light and 24 Jacobi steps, each stage against the oracle fed
+    the GPU's previous stage."""
+    sc, _ = scenes.c1(bvh=OBVH)
+    K = 64
+    t = _gpu_cache(ctx, sc, K=K)
+    ni = oracle.node_inputs(sc)
+    g0, miss0 = oracle.gather_table(sc, ni, K)
+    # the GPU's table after the remap: compare the oracle's build + remap on the oracle's own counts
+    act0 = np.nonzero(ni["static"] & (miss0 <= 0.5 * K))[0]
+    g1 = oracle.gather_remap(sc, ni, K, g0, miss0)
+    same = np.mean(t["g"] == g1)
+    print("gather entries equal: %.5f; active %d vs %d; misses equal %.5f" % (
+        same, t["active"].size, act0.size, np.mean(t["miss"] == miss0)))
+    assert same > 0.995 and np.mean(t["miss"] == miss0) > 0.99
+    assert abs(t["active"].size - act0.size) <= 0.002 * act0.size + 2
+    # node direct light and Jacobi on the GPU's own tables
+    ctx.radiosity_update(direct=True, iters=0)
+    st_gpu, _ = ctx.radiosity_state(ni["npos"].shape[0])
+    st = np.zeros_like(st_gpu)
+    oracle.node_direct(sc, ni, t["active"], 4, t["alb"], st)
+    a = t["active"]
+    B, Bo = st_gpu[a, 12:15], st[a, 12:15]
+    print("node B_dir: rel err %.2e, visibility codes equal %.5f" % (
+        np.linalg.norm(B - Bo) / np.linalg.norm(Bo), np.mean(st_gpu[a, 15] == st[a, 15])))
+    assert np.linalg.norm(B - Bo) <= 1e-3 * np.linalg.norm(Bo)
+    assert np.mean(st_gpu[a, 15] == st[a, 15]) > 0.995
+    st2 = st_gpu.copy()
+    ctx.radiosity_update(direct=False, iters=24)
+    st_gpu2, e_gpu = ctx.radiosity_state(ni["npos"].shape[0])
+    slot = oracle.jacobi_steps(sc, ni, a, t["g"], K, t["nb"], t["nw"], t["alb"], t["hh"], st2, 1, 24)
+    for k in (0, slot):
+        x, y = st_gpu2[a, 4 * k:4 * k + 3], st2[a, 4 * k:4 * k + 3]
+        print("state word %d: rel err %.2e" % (k, np.linalg.norm(x - y) / np.linalg.norm(y)))
+        assert np.linalg.norm(x - y) <= 1e-5 * np.linalg.norm(y)
+    e = oracle.e_avg(ni, a, st2, t["hh"])
+    assert np.allclose(e_gpu, e, rtol=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [0, lib.VT_OPT_CULL_SHADOWS, lib.VT_OPT_DIRECT_ONLY, lib.VT_OPT_NO_INDIRECT,
+                                  lib.VT_OPT_NORMALS])
+def test_gpu_reference_estimator_matches_oracle(ctx, opts):
+    sc, cams = scenes.c1(bvh=OBVH)
+    t = _gpu_cache(ctx, sc)
+    ctx.radiosity_update(direct=True, iters=16)              # slot 1 -> 16 steps end in slot 1
+    nodes = np.asarray(sc["verts"]).size // 8
+    st, e = ctx.radiosity_state(nodes)                        # also refreshes eAvg for the render
+    cam = dict(cams[0], spp=16, flags=lib.VT_OPT_REFERENCE | opts, light_samples=2)
+    print("reference estimator, options %d" % opts, flush=True)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    x0, y0, w, h = 256, 160, 64, 48
+    sc2 = dict(sc, node_state=st, alb=t["alb"], hh=t["hh"], slot=1, e_avg=e)
+    ref, _, _ = oracle.render_tile(sc2, cam, x0, y0, w, h)
+    a = img[y0:y0 + h, x0:x0 + w, :3].astype(np.float64)
+    b = ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    print("GPU vs oracle reference estimator: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
+def test_gpu_reference_estimator_with_bodies(ctx):
+    """Posed bodies: one BSDF bounce into the cache, the bodies' occlusion capsules dim the static surfaces'
+    cached light, moving over the exposure."""
+    from test_bodies import _poses
+    parts = scenes.posed_parts()
+    sc = scenes.posed_scene(*parts)
+    poses = _poses(True)
+    poses[0].update(occ_a=(-0.25, 0, 0), occ_b=(0.25, 0, 0), occ_r=0.15, rho_eff=0.3)
+    poses[1].update(occ_a=(0, 0, -0.3), occ_b=(0, 0, 0.3), occ_r=0.15, rho_eff=0.5)
+    t = _gpu_cache(ctx, sc, poses=poses)
+    ctx.radiosity_update(direct=True, iters=16)
+    nodes = np.asarray(sc["verts"]).size // 8
+    st, e = ctx.radiosity_state(nodes)            # eAvg: what a body's bounce onto another body reads
+    assert np.all(e > 0)
+    for opts in (0, lib.VT_OPT_NO_CAPSULE, lib.VT_OPT_NO_MIS, lib.VT_OPT_CULL_SHADOWS):
+        cam = dict(scenes.posed_camera(96, 72, spp=8), flags=lib.VT_OPT_REFERENCE | opts)
+        print("reference estimator with bodies, options %d" % opts, flush=True)
+        ctx.set_camera(0, cam)
+        ctx.render([0])
+        img = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+        sc2 = dict(sc, node_state=st, alb=t["alb"], hh=t["hh"], slot=1, e_avg=e, poses=poses)
+        ref, _, _ = oracle.render_tile(sc2, cam, 0, 0, 96, 72)
+        a, b = img[:, :, :3].astype(np.float64), ref[:, :, :3].astype(np.float64)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print("GPU vs oracle: relative RMSE %.3g" % rel)
+        assert rel < 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_reference_estimator_sphere_closed_form(ctx):
+    sc, cam, want = _sphere()
+    _gpu_cache(ctx, sc, K=256, samples=64, sigma=0.0)
+    ctx.radiosity_update(direct=True, iters=60)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 64, 64)
+    got = img[16:48, 16:48, 0].mean()
+    print("GPU reference estimator in the sphere: %.5f, closed form %.5f" % (got, want))
+    assert got == pytest.approx(want, rel=0.02)
+
+
+@pytest.mark.gpu
+def test_gpu_reference_estimator_rejects_misuse(ctx):
+    sc, cams = scenes.c1(bvh=OBVH)
+    ctx.upload_scene(sc)
+    cam = dict(cams[0], spp=1, flags=lib.VT_OPT_REFERENCE)
+    ctx.set_camera(0, cam)
+    with pytest.raises(lib.VTError):
+        ctx.render([0])                                    # no cache built
+    ctx.radiosity_build()
+    with pytest.raises(lib.VTError):
+        ctx.render([0], mode=lib.VT_RENDER_WAVEFRONT)      # the wavefront form runs the path tracer
+    ctx.set_camera(1, dict(cams[0], spp=1, flags=0, cam_index=1))
+    with pytest.raises(lib.VTError):
+        ctx.render([0, 1])                                 # one estimator per frame
+
+
+def _capsule_cases(n=4000, seed=12):
+    rng = np.random.default_rng(seed)
+    c = np.zeros((n, 16), np.float32)
+    a = rng.uniform(-0.5, 0.5, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    L = rng.uniform(0.05, 0.8, n)
+    r = rng.uniform(0.005, 0.2, n)
+    p = a + d * L[:, None] * rng.uniform(-0.5, 1.5, (n, 1)) + rng.normal(size=(n, 3)) * rng.uniform(0.05, 1.0, (n, 1))
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+    c[:, 0:3], c[:, 4:7], c[:, 8:11], c[:, 11], c[:, 12:15] = p, nn, a, r, a + d * L[:, None]
+    return c
+
+
+@pytest.mark.gpu
+def test_gpu_capsule_occlusion_matches_oracle(ctx):
+    """capsuleOcc on random points, normals and capsules (in and out of the horizon, near and far) against the
+    FP64 restatement pinned by the reference's golden values."""
+    c = _capsule_cases()
+    got = ctx.capsule_test(c)
+    cd = c.astype(np.float64)
+    want = np.array([oracle.lib().or_capsule_occ(oracle.ptr(np.ascontiguousarray(x[0:3])), oracle.ptr(np.ascontiguousarray(x[4:7])),
+                                                 oracle.ptr(np.ascontiguousarray(x[8:11])), oracle.ptr(np.ascontiguousarray(x[12:15])), x[11])
+                     for x in cd])
+    err = np.abs(got - want)
+    k = int(np.argmax(err))
+    print("capsule occlusion: max |GPU - FP64| %.2e (case %d: %.6f vs %.6f), mean %.2e, %d nonzero" % (
+        err.max(), k, got[k], want[k], err.mean(), np.sum(want > 0)))
+    assert np.sum(want > 1e-3) > 500
+    assert err.max() < 1e-4

@@ -2,6 +2,7 @@
 // reference's test entry points and node radiosity. Device memory is owned by the context; one HIP stream
 // (default priority) orders every call's work.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,7 +43,18 @@ struct vt_ctx {
   int cu_count = 0;
   // scene
   bool has_scene = false;
-  DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots;
+  DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots, tvid;
+  // host copies the node radiosity build reads: vertices (8 floats each), per slot vertex ids + mesh, meshes,
+  // the static scene's slot count (its slots come first), the emitter records
+  std::vector<float> h_verts;
+  std::vector<uint4> h_tvid;
+  std::vector<uint32_t> h_meshes;
+  std::vector<float> h_emit;
+  uint32_t static_slots = 0;
+  // the node radiosity cache (vt_radiosity_build / update / state)
+  bool rad_built = false;
+  uint32_t rad_K = 0, rad_samples = 0, rad_nbr_k = 0, rad_active = 0, rad_slot = 1;
+  DevBuf g_pos, g_miss, radu, g_area;
   std::vector<uint32_t> body_root;   // BLAS root node of each body
   std::vector<float> body_box;       // each body's box in its own frame (min xyz, max xyz)
   SceneDev S{};
@@ -68,7 +80,7 @@ struct vt_ctx {
   uint32_t def_mode = VT_RENDER_BATCHED;
   uint64_t def_wf_batch = 1ull << 24;
   uint32_t def_wf_stack = 8, def_wf_refill = 16;
-  int occ[2][3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (bodies, block 64/128/256, LDS depth)
+  int occ[2][2][3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (bodies, estimator, block, LDS depth)
   // scene box (for the wavefront sort key)
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
   // wavefront buffers: paths per batch and shadow slots per path they are sized for
@@ -291,7 +303,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots,
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots, &c->tvid, &c->g_pos, &c->g_miss, &c->radu, &c->g_area,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
@@ -378,6 +390,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   const float *np = s->nodes;
   std::vector<Node4> n4;
   std::vector<uint32_t> body_root(s->body_count, 0);
+  uint32_t body_slots = 0, static_count = 0;
   std::vector<float> body_box(6 * (size_t)s->body_count, 0.0f);
   if (s->body_count) {
     // one BVH per group (the static scene, then each body) over that group's triangles, built and collapsed
@@ -393,6 +406,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
       for (uint32_t k = 0; k < T; k++)
         if (s->tri_body[k] == want) members.push_back(k);
       const uint32_t Tg = (uint32_t)members.size();
+      if (g == 0) static_count = Tg;
       if (Tg == 0) return fail(c, VT_ERR_ARG, g ? "scene: body %u has no triangles" : "scene: no static triangles", g - 1);
       std::vector<uint32_t> idx((size_t)Tg * 3), order(Tg);
       for (uint32_t m = 0; m < Tg; m++)
@@ -433,6 +447,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
       slot_base += Tg;
     }
     tr = tris_sorted.data();
+    body_slots = T - static_count;
   } else if (!np) {
     std::vector<float> pos((size_t)V * 3);
     for (uint32_t v = 0; v < V; v++)
@@ -477,21 +492,31 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   if ((rc = upload(c, c->nodes, n4.data(), n4.size() * sizeof(Node4)))) return rc;
   if ((rc = upload(c, c->tri, tri.data(), tri.size() * sizeof(float4)))) return rc;
   {
-    // vertex ids and normals only feed the shading records; they do not stay on the device
-    DevBuf d_attr, d_vn;
-    rc = upload(c, d_attr, attr.data(), attr.size() * sizeof(uint4));
+    // per-slot vertex ids stay on the device (the node cache's lookups); vertex normals only feed the shading
+    // records
+    DevBuf d_vn;
+    rc = upload(c, c->tvid, attr.data(), attr.size() * sizeof(uint4));
     if (!rc) rc = upload(c, d_vn, vn.data(), vn.size() * sizeof(float4));
     if (!rc) rc = ensure(c, c->tshade, (size_t)T * 64);
     if (!rc) {
-      hipError_t e = launch_tri_shading((const float4 *)c->tri.p, (const uint4 *)d_attr.p, (const float4 *)d_vn.p, T,
+      hipError_t e = launch_tri_shading((const float4 *)c->tri.p, (const uint4 *)c->tvid.p, (const float4 *)d_vn.p, T,
                                         (float4 *)c->tshade.p, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) rc = hip_fail(c, e, "triangle shading records");
     }
-    release(d_attr);
     release(d_vn);
     if (rc) return rc;
   }
+  c->h_verts.assign(s->verts, s->verts + 8 * (size_t)V);
+  c->h_tvid = attr;
+  c->h_meshes.assign(s->meshes, s->meshes + 4 * (size_t)s->mesh_count);
+  c->h_emit.assign(s->emit_count ? s->emit : nullptr, s->emit_count ? s->emit + 16 * (size_t)s->emit_count : nullptr);
+  c->static_slots = s->body_count ? (uint32_t)(T - body_slots) : T;
+  c->rad_built = false;
+  c->S.nstate = nullptr;
+  c->S.alb = nullptr;
+  c->S.hh = nullptr;
+  c->S.radu = nullptr;
   if ((rc = upload(c, c->meshes, s->meshes, (size_t)s->mesh_count * 16))) return rc;
   if ((rc = upload(c, c->mats, s->materials, (size_t)s->material_count * 64))) return rc;
   float zero4[4] = {0, 0, 0, 0};
@@ -502,6 +527,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.tri = (const float4 *)c->tri.p;
   c->S.tshade = (const float4 *)c->tshade.p;
   c->S.meshes = (const uint4 *)c->meshes.p;
+  c->S.tvid = (const uint4 *)c->tvid.p;
   c->S.mats = (const float4 *)c->mats.p;
   c->S.lights = (const float4 *)c->lights.p;
   {
@@ -704,6 +730,15 @@ extern "C" int vt_set_instances(vt_ctx *c, const vt_instance *in, uint32_t count
     const double motion = moving ? 1.01 * (2.0 * std::sin(phi / 4.0) * rad + 0.5 * dp) : 0.0;
     r[6] = make_float4((float)wa[0], (float)wa[1], (float)wa[2], (float)(rr * 1.0001 + motion + 4e-6 * scale + 1e-6));
     r[7] = make_float4((float)wb[0], (float)wb[1], (float)wb[2], 0.f);
+    // occlusion capsule of the reference estimator's indirect light, posed at mid-exposure
+    if (!(I.occ_r >= 0.0f) || !std::isfinite(I.occ_r) || !(I.rho_eff >= 0.0f && I.rho_eff <= 1.0f))
+      return fail(c, VT_ERR_ARG, "set_instances: body %u needs occ_r >= 0 and rho_eff in [0, 1]", b);
+    const double oa[3] = {I.occ_a[0], I.occ_a[1], I.occ_a[2]}, ob[3] = {I.occ_b[0], I.occ_b[1], I.occ_b[2]};
+    double woa[3], wob[3];
+    host_apply(Pm, oa, woa);
+    host_apply(Pm, ob, wob);
+    r[8] = make_float4((float)woa[0], (float)woa[1], (float)woa[2], I.occ_r);
+    r[9] = make_float4((float)wob[0], (float)wob[1], (float)wob[2], I.rho_eff);
   }
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));
@@ -733,12 +768,12 @@ Shape shape_of(const vt_ctx *c, const vt_render_opts &o) {
 }
 
 // blocks of a persistent grid: every CU filled to the kernel's occupancy, within the spill slots
-int persistent_grid(vt_ctx *c, const Shape &sh, uint64_t waves, uint32_t &grid) {
+int persistent_grid(vt_ctx *c, const Shape &sh, uint64_t waves, uint32_t &grid, bool ref = false) {
   const int bi = sh.block == 64 ? 0 : (sh.block == 128 ? 1 : 2);
   const bool inst = c->S.inst_count != 0;
-  int &occ = c->occ[inst][bi][sh.stack];
+  int &occ = c->occ[inst][ref][bi][sh.stack];
   if (occ == 0) {
-    VT_HIP(c, render_occupancy(sh.block, sh.stack, inst, &occ));
+    VT_HIP(c, render_occupancy(sh.block, sh.stack, inst, ref, &occ));
     if (occ < 1) occ = 1;
   }
   const uint64_t need = (waves * 64 + sh.block - 1) / sh.block;
@@ -788,6 +823,7 @@ int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opt
     d.shutter0 = k.shutter[0];
     d.shutter1 = k.shutter[1];
     d.skip = k.skip_bodies;
+    d.opts = k.flags;
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -966,14 +1002,33 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
 
 }  // namespace
 
+// the estimator of a frame's cameras: all path tracer, or all the reference estimator (which needs the node cache)
+int frame_estimator(vt_ctx *c, const uint32_t *cams, uint32_t n, bool &ref) {
+  ref = false;
+  if (!cams || n == 0) return VT_OK;
+  for (uint32_t a = 0; a < n; a++) {
+    if (cams[a] >= VT_MAX_CAMERAS || !c->cams[cams[a]].set) return VT_OK;   // stage_frame reports it
+    const bool r = (c->cams[cams[a]].c.flags & VT_OPT_REFERENCE) != 0;
+    if (a && r != ref) return fail(c, VT_ERR_ARG, "render: cameras of one frame use one estimator");
+    ref = r;
+  }
+  if (ref && !c->rad_built) return fail(c, VT_ERR_STATE, "render: the reference estimator needs vt_radiosity_build");
+  return VT_OK;
+}
+
 extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts *o) {
   VT_LIVE(c);
   if (!c->has_scene) return fail(c, VT_ERR_STATE, "render: no scene uploaded");
   vt_render_opts opt{};
   if (o) opt = *o;
-  if (opt.mode == VT_RENDER_AUTO) opt.mode = c->def_mode;
+  bool ref = false;
+  int rc = frame_estimator(c, cams, n, ref);
+  if (rc) return rc;
+  if (ref && opt.mode == VT_RENDER_WAVEFRONT)
+    return fail(c, VT_ERR_ARG, "render: the wavefront form runs the path tracer; the reference estimator runs persistent");
+  if (opt.mode == VT_RENDER_AUTO) opt.mode = ref ? (uint32_t)VT_RENDER_BATCHED : c->def_mode;
   const Shape sh = shape_of(c, opt);
-  int rc = opt.mode == VT_RENDER_WAVEFRONT ? VT_OK : check_shape(c, sh);   // the wavefront checks its own
+  rc = opt.mode == VT_RENDER_WAVEFRONT ? VT_OK : check_shape(c, sh);   // the wavefront checks its own
   if (rc) return rc;
   if (opt.graph && opt.mode == VT_RENDER_WAVEFRONT) opt.graph = 0;   // the wavefront form is not captured
   if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
@@ -981,6 +1036,7 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   uint64_t items = 0;
   if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
   RenderLaunch L = base_launch(c, sh);
+  L.ref = ref ? 1u : 0u;
   uint32_t launches = 0;
   c->graph_used = false;
   c->ext_launches = 0;
@@ -995,7 +1051,7 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
     L.item_base = 0;
     L.total_waves = items / 64;
     L.persistent = 1;
-    if ((rc = persistent_grid(c, sh, L.total_waves, L.grid))) return rc;
+    if ((rc = persistent_grid(c, sh, L.total_waves, L.grid, ref))) return rc;
     if (opt.graph) {
       // key: the launch record, and the sums the graph clears
       std::vector<uint8_t> key(sizeof L + 1 + n * sizeof(DevCam));
@@ -1075,9 +1131,11 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   if (o) opt = *o;
   opt.accumulate = 0;
   const uint64_t stride = probe_stride ? probe_stride : 32;
-  uint64_t items = 0;
-  int rc = stage_frame(c, cams, n, opt, items);
+  bool ref = false;
+  int rc = frame_estimator(c, cams, n, ref);
   if (rc) return rc;
+  uint64_t items = 0;
+  if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
   const uint64_t frame_waves = items / 64;
   const uint64_t probe_waves = (frame_waves + stride - 1) / stride;
   if ((rc = ensure_launch_events(c, 1))) return rc;
@@ -1088,7 +1146,8 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     L.total_waves = probe_waves;
     L.wave_stride = stride;
     L.persistent = 1;
-    int r = persistent_grid(c, sh, probe_waves, L.grid);
+    L.ref = ref ? 1u : 0u;
+    int r = persistent_grid(c, sh, probe_waves, L.grid, ref);
     if (r) return r;
     VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
     VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
@@ -1138,7 +1197,25 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   c->def_stack = best.stack;
   c->def_chunk = best.chunk;
   c->def_queues = best.queues;
-  // the wavefront form on the same probe: batch {4M, 16M} paths x LDS stack {4, 8, 16} x refill {8, 16, 32}
+  // the wavefront form on the same probe: batch {4M, 16M} paths x LDS stack {4, 8, 16} x refill {8, 16, 32};
+  // the reference estimator runs the persistent form only
+  if (ref) {
+    drop_graph(c);
+    for (uint32_t a = 0; a < n; a++) {
+      CamSlot &s = c->cams[cams[a]];
+      VT_HIP(c, launch_zero_f4((float4 *)s.acc.p, (uint64_t)s.c.width * s.c.height, c->stream));
+      s.spp_total = 0;
+      s.rendered = false;
+    }
+    VT_HIP(c, hipStreamSynchronize(c->stream));
+    out->block = best.block; out->stack_lds = best.stack; out->chunk = best.chunk; out->queues = best.queues;
+    out->candidates = tried; out->probe_mrays_best = m_best; out->probe_mrays_default = m_def;
+    out->probe_fraction = (double)probe_waves / (double)std::max<uint64_t>(1, frame_waves);
+    out->mode = VT_RENDER_BATCHED;
+    out->wf_batch = (uint32_t)c->def_wf_batch; out->wf_stack_lds = c->def_wf_stack; out->wf_refill = c->def_wf_refill;
+    out->probe_mrays_wavefront = 0.0;
+    return VT_OK;
+  }
   WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride};
   double m_wf = 0;
   auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
@@ -1341,6 +1418,11 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
 extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const float *const *adj,
                            const vt_render_opts *o, vt_grads *out) {
   VT_LIVE(c);
+  {
+    bool ref = false;
+    if (int rc0 = frame_estimator(c, cams, n, ref)) return rc0;
+    if (ref) return fail(c, VT_ERR_ARG, "backward: the adjoint differentiates the path tracer, not the reference estimator");
+  }
   if (!c->has_scene) return fail(c, VT_ERR_STATE, "backward: no scene uploaded");
   if (!cams || !adj || !out || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "backward: bad arguments");
   if (out->cameras && out->camera_count != n)
@@ -1390,6 +1472,7 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
     d.shutter0 = k.shutter[0];
     d.shutter1 = k.shutter[1];
     d.skip = k.skip_bodies;
+    d.opts = k.flags;
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -1567,6 +1650,21 @@ extern "C" int vt_bsdf_test(vt_ctx *c, const float *cases, uint32_t n, float *ou
   return VT_OK;
 }
 
+extern "C" int vt_capsule_test(vt_ctx *c, const float *cases, uint32_t n, float *out) {
+  VT_LIVE(c);
+  if (n && (!cases || !out)) return VT_ERR_ARG;
+  if (n == 0) return VT_OK;
+  VT_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure(c, c->test_in, (size_t)n * 64))) return rc;
+  if ((rc = ensure(c, c->test_out, (size_t)n * 4))) return rc;
+  VT_HIP(c, hipMemcpy(c->test_in.p, cases, (size_t)n * 64, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_capsule_test((const float *)c->test_in.p, n, (float *)c->test_out.p, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(out, c->test_out.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
 extern "C" int vt_sensor_test(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t h, const float *radiance,
                               const float *ri, uint16_t *dn) {
   VT_LIVE(c);
@@ -1638,6 +1736,313 @@ extern "C" int vt_jacobi_sparse(vt_ctx *c, const vt_gather_tables *t, float *nod
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(node_state, c->g_state.p, N * 64, hipMemcpyDeviceToHost));
   if (aux_out) VT_HIP(c, hipMemcpy(aux_out, c->g_aux.p, A * 16, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+// ====================================================================== node radiosity on the uploaded scene
+namespace {
+
+// smoothing neighbours of every node in a list (_neighbours, gpu/renderer.js:574-616): the same-facing
+// (n.n' >= 0.9) nodes of the list within R, Gaussian weights exp(-d^2 / (2 sigma^2)), the K largest (ties: lower
+// index first), sorted by weight; a uniform grid of cell R finds the candidates. sigma <= 0: each node alone.
+void node_neighbours(const float *pos, const float *nrm, size_t n, uint32_t K, double R, double sg,
+                     std::vector<uint32_t> &nb, std::vector<float> &nw) {
+  nb.assign(n * K, 0u);
+  nw.assign(n * K, 0.0f);
+  if (K == 0) return;
+  if (!(sg > 0)) {
+    for (size_t k = 0; k < n; k++) {
+      nb[k * K] = (uint32_t)k;
+      nw[k * K] = 1.0f;
+    }
+    return;
+  }
+  const int64_t OFF = 1 << 20;
+  auto cell = [&](double x) { return (int64_t)std::floor(x / R) + OFF; };
+  auto key = [](int64_t x, int64_t y, int64_t z) {
+    return ((uint64_t)(x & 0x1fffff) << 42) | ((uint64_t)(y & 0x1fffff) << 21) | (uint64_t)(z & 0x1fffff);
+  };
+  std::vector<std::pair<uint64_t, uint32_t>> order(n);
+  for (size_t k = 0; k < n; k++)
+    order[k] = {key(cell(pos[3 * k]), cell(pos[3 * k + 1]), cell(pos[3 * k + 2])), (uint32_t)k};
+  std::sort(order.begin(), order.end());
+  std::vector<double> cw(K);
+  std::vector<uint32_t> cm(K);
+  for (size_t k = 0; k < n; k++) {
+    const double px = pos[3 * k], py = pos[3 * k + 1], pz = pos[3 * k + 2];
+    const double nx = nrm[3 * k], ny = nrm[3 * k + 1], nz = nrm[3 * k + 2];
+    const int64_t cx = cell(px), cy = cell(py), cz = cell(pz);
+    uint32_t cnt = 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          const uint64_t kk = key(cx + dx, cy + dy, cz + dz);
+          auto lo = std::lower_bound(order.begin(), order.end(), std::make_pair(kk, 0u));
+          for (auto it = lo; it != order.end() && it->first == kk; ++it) {
+            const uint32_t m = it->second;
+            const double ex = pos[3 * m] - px, ey = pos[3 * m + 1] - py, ez = pos[3 * m + 2] - pz;
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            if (d2 > R * R) continue;
+            if (nx * nrm[3 * m] + ny * nrm[3 * m + 1] + nz * nrm[3 * m + 2] < 0.9) continue;
+            const double w = std::exp(-0.5 * d2 / (sg * sg));
+            uint32_t at = cnt < K ? cnt : K;
+            while (at > 0 && (cw[at - 1] < w || (cw[at - 1] == w && cm[at - 1] > m))) at--;
+            if (at >= K) continue;
+            for (uint32_t q = std::min(cnt, K - 1); q > at; q--) {
+              cw[q] = cw[q - 1];
+              cm[q] = cm[q - 1];
+            }
+            cw[at] = w;
+            cm[at] = m;
+            if (cnt < K) cnt++;
+          }
+        }
+    for (uint32_t q = 0; q < cnt; q++) {
+      nb[k * K + q] = cm[q];
+      nw[k * K + q] = (float)cw[q];
+    }
+  }
+}
+
+NodeLaunch node_launch(vt_ctx *c) {
+  NodeLaunch L;
+  std::memset(&L, 0, sizeof L);
+  L.S = c->S;
+  L.npos = (const float4 *)c->g_pos.p;
+  L.nnorm = (const float4 *)c->g_normals.p;
+  L.nmat = (const uint32_t *)c->g_mat.p;
+  L.n_nodes = (uint32_t)(c->h_verts.size() / 8);
+  L.K = c->rad_K;
+  L.node_samples = c->rad_samples;
+  L.n_active = c->rad_active;
+  L.n_mat = c->n_mat;
+  L.gtable = (uint32_t *)c->g_table.p;
+  L.miss = (float *)c->g_miss.p;
+  L.active = (const uint32_t *)c->g_active.p;
+  L.state = (float4 *)c->g_state.p;
+  L.alb = (float4 *)c->g_alb.p;
+  L.hh = (float4 *)c->g_hh.p;
+  L.overflow = (uint32_t *)c->overflow.p;
+  L.spill = (uint2 *)c->spill.p;
+  L.spill_stride = c->spill_lanes;
+  L.max_blocks = (uint32_t)(c->spill_lanes / 256);
+  return L;
+}
+
+
+}  // namespace
+
+extern "C" int vt_node_neighbours(const float *pos, const float *nrm, uint32_t n, uint32_t k, float radius, float sigma,
+                                  uint32_t *idx, float *w) {
+  if (n && (!pos || !nrm || !idx || !w)) return VT_ERR_ARG;
+  if (k == 0 || k > 32 || !(radius > 0)) return VT_ERR_ARG;
+  std::vector<uint32_t> nb;
+  std::vector<float> nw;
+  node_neighbours(pos, nrm, n, k, radius, sigma, nb, nw);
+  std::memcpy(idx, nb.data(), nb.size() * 4);
+  std::memcpy(w, nw.data(), nw.size() * 4);
+  return VT_OK;
+}
+
+extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radiosity_info *info) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "radiosity: no scene uploaded");
+  vt_radiosity_opts op{64, 4, 16, 0.006f, 0.003f};
+  if (o) op = *o;
+  if (op.gather_k < 8 || op.gather_k > 1024 || op.node_samples < 1 || op.node_samples > 1024 || op.nbr_k < 1 ||
+      op.nbr_k > 32 || !(op.smooth_radius > 0) || !(op.smooth_sigma >= 0))
+    return fail(c, VT_ERR_ARG, "radiosity: gather_k 8..1024, node_samples 1..1024, nbr_k 1..32, radius > 0, sigma >= 0");
+  const auto t0 = std::chrono::steady_clock::now();
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t V = c->h_verts.size() / 8;
+  // static nodes: the vertices of static-flagged triangles of the static scene, with a third of every such
+  // triangle's area; a node's material is that of its own mesh (the vertex record's word 3, as the reference's
+  // prepared vertices carry it, gpu/scene_gpu.js:384-387)
+  std::vector<float4> npos(V), nnorm(V);
+  std::vector<uint32_t> nmat(V, 0u);
+  std::vector<float> area(V, 0.0f);
+  std::vector<uint8_t> stat(V, 0);
+  for (uint32_t sl = 0; sl < c->static_slots; sl++) {
+    const uint4 t = c->h_tvid[sl];
+    if (!(c->h_meshes[4 * (size_t)t.w + 1] & VT_MESH_STATIC)) continue;
+    const uint32_t vv[3] = {t.x, t.y, t.z};
+    const float *a = &c->h_verts[8 * (size_t)t.x], *b = &c->h_verts[8 * (size_t)t.y], *q = &c->h_verts[8 * (size_t)t.z];
+    const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+    const double e2[3] = {(double)q[0] - a[0], (double)q[1] - a[1], (double)q[2] - a[2]};
+    const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    const double ar = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    for (uint32_t v : vv) {
+      stat[v] = 1;
+      area[v] += (float)(ar / 3.0);
+    }
+  }
+  const uint32_t n_mesh = (uint32_t)(c->h_meshes.size() / 4);
+  for (size_t v = 0; v < V; v++) {
+    if (!stat[v]) continue;
+    uint32_t mesh;
+    std::memcpy(&mesh, &c->h_verts[8 * v + 3], 4);
+    if (mesh >= n_mesh) return fail(c, VT_ERR_ARG, "radiosity: vertex %zu mesh id %u out of range", v, mesh);
+    nmat[v] = c->h_meshes[4 * (size_t)mesh];
+  }
+  uint32_t n_static = 0;
+  for (size_t v = 0; v < V; v++) {
+    const float *p = &c->h_verts[8 * v];
+    double nx = p[4], ny = p[5], nz = p[6];
+    const double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (l > 0) { nx /= l; ny /= l; nz /= l; } else { nx = 0; ny = 0; nz = 1; }
+    npos[v] = make_float4(p[0], p[1], p[2], stat[v] ? 1.0f : 0.0f);
+    nnorm[v] = make_float4((float)nx, (float)ny, (float)nz, 0.0f);
+    n_static += stat[v];
+  }
+  int rc;
+  if ((rc = upload(c, c->g_pos, npos.data(), V * sizeof(float4)))) return rc;
+  if ((rc = upload(c, c->g_normals, nnorm.data(), V * sizeof(float4)))) return rc;
+  if ((rc = upload(c, c->g_mat, nmat.data(), V * 4))) return rc;
+  if ((rc = upload(c, c->g_area, area.data(), V * 4))) return rc;
+  if ((rc = ensure(c, c->g_table, V * op.gather_k * 4))) return rc;
+  if ((rc = ensure(c, c->g_miss, std::max<size_t>(V, 1) * 4))) return rc;
+  if ((rc = ensure(c, c->g_alb, (size_t)c->n_mat * ALB_BINS * 32))) return rc;
+  if ((rc = ensure(c, c->g_hh, (size_t)c->n_mat * 16))) return rc;
+  if ((rc = ensure(c, c->g_state, std::max<size_t>(V, 1) * 64))) return rc;
+  if ((rc = ensure(c, c->radu, 2 * sizeof(float4)))) return rc;
+  VT_HIP(c, launch_zero_f4((float4 *)c->g_state.p, (uint64_t)V * 4, c->stream));
+  c->rad_K = op.gather_k;
+  c->rad_samples = op.node_samples;
+  c->rad_nbr_k = op.nbr_k;
+  c->rad_active = 0;
+  NodeLaunch L = node_launch(c);
+  VT_HIP(c, launch_albedo_tables(L, c->stream));
+  VT_HIP(c, launch_gather_build(L, c->stream));
+  VT_HIP(c, launch_gather_count(L, c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<float> miss(V);
+  if (V) VT_HIP(c, hipMemcpy(miss.data(), c->g_miss.p, V * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> act;
+  for (size_t v = 0; v < V; v++)
+    if (stat[v] && miss[v] <= 0.5f * (float)op.gather_k) act.push_back((uint32_t)v);
+  const size_t A = act.size();
+  if ((rc = upload(c, c->g_active, act.empty() ? nullptr : act.data(), std::max<size_t>(A, 1) * 4))) return rc;
+  c->rad_active = (uint32_t)A;
+  L = node_launch(c);
+  VT_HIP(c, launch_gather_remap(L, c->stream));
+  // smoothing neighbours among the active nodes, by active index
+  std::vector<float> ap(3 * A), an(3 * A);
+  for (size_t k = 0; k < A; k++)
+    for (int j = 0; j < 3; j++) {
+      ap[3 * k + j] = (&npos[act[k]].x)[j];
+      an[3 * k + j] = (&nnorm[act[k]].x)[j];
+    }
+  std::vector<uint32_t> nb;
+  std::vector<float> nw;
+  node_neighbours(ap.data(), an.data(), A, op.nbr_k, op.smooth_radius, op.smooth_sigma, nb, nw);
+  if ((rc = upload(c, c->g_nbr, nb.empty() ? nullptr : nb.data(), std::max<size_t>(nb.size(), 1) * 4))) return rc;
+  if ((rc = upload(c, c->g_nbrw, nw.empty() ? nullptr : nw.data(), std::max<size_t>(nw.size(), 1) * 4))) return rc;
+  if ((rc = ensure(c, c->g_aux, std::max<size_t>(A, 1) * 16))) return rc;
+  // the emitters' centroid and radius for the shadow culling (gpu/scene_gpu.js:418-422)
+  double cen[3] = {0, 0, 0}, tot = 0, rad = 0;
+  const size_t E = c->h_emit.size() / 16;
+  for (size_t e = 0; e < E; e++) {
+    const float *r = &c->h_emit[16 * e];
+    for (int k = 0; k < 3; k++) cen[k] += ((double)r[k] + ((double)r[4 + k] + r[8 + k]) / 3.0) * r[7];
+    tot += r[7];
+  }
+  for (int k = 0; k < 3; k++) cen[k] /= tot > 0 ? tot : 1.0;
+  for (size_t e = 0; e < E; e++) {
+    const float *r = &c->h_emit[16 * e];
+    for (int q = 0; q < 3; q++) {
+      double d2 = 0;
+      for (int k = 0; k < 3; k++) {
+        const double x = (double)r[k] + (q == 1 ? r[4 + k] : 0.0) + (q == 2 ? r[8 + k] : 0.0) - cen[k];
+        d2 += x * x;
+      }
+      rad = std::max(rad, std::sqrt(d2));
+    }
+  }
+  c->rad_slot = 1;
+  VT_HIP(c, launch_fill_f4((float4 *)c->radu.p, 1, make_float4(0.f, 0.f, 0.f, bits_f(1u)), c->stream));
+  VT_HIP(c, launch_fill_f4((float4 *)c->radu.p + 1, 1, make_float4((float)cen[0], (float)cen[1], (float)cen[2], (float)rad),
+                           c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  c->S.nstate = (const float4 *)c->g_state.p;
+  c->S.alb = (const float4 *)c->g_alb.p;
+  c->S.hh = (const float4 *)c->g_hh.p;
+  c->S.radu = (const float4 *)c->radu.p;
+  c->rad_built = true;
+  drop_graph(c);
+  if (info) {
+    info->nodes = (uint32_t)V;
+    info->static_nodes = n_static;
+    info->active = (uint32_t)A;
+    info->ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return VT_OK;
+}
+
+extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
+  VT_HIP(c, hipSetDevice(c->device));
+  c->S.nstate = (const float4 *)c->g_state.p;     // a scene's instances may have changed S since the build
+  NodeLaunch L = node_launch(c);
+  if (direct) VT_HIP(c, launch_node_direct(L, c->stream));
+  GatherDev G{};
+  G.n_active = c->rad_active; G.gather_k = c->rad_K; G.nbr_k = c->rad_nbr_k; G.bins = ALB_BINS;
+  G.active = (const uint32_t *)c->g_active.p; G.gtable = (const uint32_t *)c->g_table.p;
+  G.node_mat = (const uint32_t *)c->g_mat.p; G.nbr = (const uint32_t *)c->g_nbr.p;
+  G.normals = (const float4 *)c->g_normals.p; G.albedo = (const float4 *)c->g_alb.p; G.hh = (const float4 *)c->g_hh.p;
+  G.nbr_w = (const float *)c->g_nbrw.p; G.node_state = (float4 *)c->g_state.p; G.aux = (float4 *)c->g_aux.p;
+  // ping-pong source slots 1 and 2 (gpu/renderer.js:729-749): a step reads slot_in and writes the other
+  for (uint32_t it = 0; it < iters; it++) {
+    G.slot_in = c->rad_slot;
+    G.slot_out = 3 - c->rad_slot;
+    VT_HIP(c, launch_jacobi_gather(G, c->stream));
+    VT_HIP(c, launch_jacobi_update(G, c->stream));
+    c->rad_slot = 3 - c->rad_slot;
+  }
+  // eAvg (the mean irradiance a body-to-body bounce reads) and the source slot, on the device
+  VT_HIP(c, launch_eavg(L, (const float *)c->g_area.p, (float4 *)c->radu.p, c->rad_slot, c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_radiosity_tables(vt_ctx *c, uint32_t which, void *dst, size_t bytes) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
+  const size_t V = c->h_verts.size() / 8, A = c->rad_active;
+  const DevBuf *src = nullptr;
+  size_t need = 0;
+  switch (which) {
+    case VT_RAD_GATHER: src = &c->g_table; need = V * c->rad_K * 4; break;
+    case VT_RAD_MISSES: src = &c->g_miss; need = V * 4; break;
+    case VT_RAD_ACTIVE: src = &c->g_active; need = A * 4; break;
+    case VT_RAD_NEIGHBOURS: src = &c->g_nbr; need = A * c->rad_nbr_k * 4; break;
+    case VT_RAD_WEIGHTS: src = &c->g_nbrw; need = A * c->rad_nbr_k * 4; break;
+    case VT_RAD_ALBEDO: src = &c->g_alb; need = (size_t)c->n_mat * ALB_BINS * 32; break;
+    case VT_RAD_HH: src = &c->g_hh; need = (size_t)c->n_mat * 16; break;
+    default: return fail(c, VT_ERR_ARG, "radiosity_tables: table %u", which);
+  }
+  if (bytes != need || (need && !dst)) return fail(c, VT_ERR_SIZE, "radiosity_tables: table %u has %zu bytes", which, need);
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  if (need) VT_HIP(c, hipMemcpy(dst, src->p, need, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_radiosity_state(vt_ctx *c, float *node_state, size_t bytes, float e_avg[3]) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
+  const size_t V = c->h_verts.size() / 8;
+  if (node_state && bytes != V * 64) return fail(c, VT_ERR_SIZE, "radiosity_state: %zu bytes for %zu nodes", V * 64, V);
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  if (node_state && V) VT_HIP(c, hipMemcpy(node_state, c->g_state.p, V * 64, hipMemcpyDeviceToHost));
+  float4 ru;
+  VT_HIP(c, hipMemcpy(&ru, c->radu.p, sizeof ru, hipMemcpyDeviceToHost));
+  if (e_avg) {
+    e_avg[0] = ru.x;
+    e_avg[1] = ru.y;
+    e_avg[2] = ru.z;
+  }
   return VT_OK;
 }
 

@@ -70,6 +70,7 @@ struct DevCam {
   float scale;                       // output scale (the image is acc * scale / spp)
   float shutter0, shutter1;          // exposure interval on the pose timeline (CamU.expo.xy)
   uint32_t skip;                     // bodies the camera's own rays ignore (bit per body)
+  uint32_t opts;                     // option bits (VT_OPT_*)
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)
@@ -315,7 +316,16 @@ struct SceneDev {
   uint32_t shadow_skip;     // bodies shadow rays ignore (bit per body)
   const float4 *lprof;      // per light: emission axis xyz, w = bits(knot count); knots at light * MAX_KNOTS
   const float2 *knots;      // (theta, p) knots of the angular emission profiles
+  const uint4 *tvid;        // per slot: vertex ids v0 v1 v2, mesh
+  // node radiosity cache (null until built): 4 float4 per vertex (E_free + V, slot-1 source, slot-2 source,
+  // B_dir + visibility code); albedo tables (per material ALB_BINS x [spec, diff]) and hemispherical albedo;
+  // radu[0] = eAvg.xyz + bits(current source slot), radu[1] = emitter centroid xyz + radius
+  const float4 *nstate;
+  const float4 *alb;
+  const float4 *hh;
+  const float4 *radu;
 };
+constexpr uint32_t ALB_BINS = 32;
 
 constexpr uint32_t MAX_KNOTS = 64;
 // p(theta) of light `light` toward the unit direction w leaving the emitter, theta from the light's axis: linear
@@ -541,8 +551,9 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, uint32_t root, v3 o,
 // ------------------------------------------------------------------ posed bodies (trace.wgsl.js:122-164)
 // Instance record, INST_WORDS float4: [0] q0 and [1] q1 (unit quaternions x y z w at shutter open and close),
 // [2] p0.xyz + BLAS root (bits), [3] p1.xyz + flags (bits: 1 moving, 2 hidden), [4] world box min, [5] world box
-// max (swept over the exposure), [6] bounding capsule a.xyz + radius, [7] capsule b.xyz.
-constexpr int INST_WORDS = 8;
+// max (swept over the exposure), [6] bounding capsule a.xyz + radius, [7] capsule b.xyz, [8] occlusion capsule
+// a.xyz + radius and [9] b.xyz + rho_eff (world frame at mid-exposure; radius 0 = none).
+constexpr int INST_WORDS = 10;
 constexpr uint32_t INST_MOVING = 1u, INST_HIDDEN = 2u;
 
 struct Pose {

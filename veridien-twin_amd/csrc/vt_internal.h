@@ -40,11 +40,12 @@ struct RenderLaunch {
   uint32_t queues;             // persistent: 1 global queue, or XCD_COUNT queues over contiguous eighths of
                                // the waves (a block drains its XCD's eighth, then the others)
   uint64_t wave_stride;        // queue wave w runs frame wave w * wave_stride (1; > 1 for tuning probes)
+  uint32_t ref;                // 1: the cameras use the reference estimator (VT_OPT_REFERENCE)
   float4 *ltot;                // adjoint: each sample's radiance by queue item, written by the camera walk and
                                // read by the gradient walk in place of its own first walk (null: walk it)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
-hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, int *blocks_per_cu);
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, bool ref, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)
 hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, uint32_t n_mat, uint32_t n_light,
@@ -112,6 +113,7 @@ hipError_t launch_visible(const TestLaunch &L, const float c[3], uint32_t skip, 
                           uint32_t n, uint8_t *out, hipStream_t s);
 hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out, hipStream_t s);
+hipError_t launch_capsule_test(const float *cases, uint32_t n, float *out, hipStream_t s);
 
 // the fused sensor epilogue: radiance = acc * inv_spp (rgb), RI map, -> DN
 struct SensorDev {
@@ -134,6 +136,31 @@ struct GatherDev {
   float4 *node_state;
   float4 *aux;
 };
+// the node radiosity cache on the uploaded scene (vt_nodes.hip)
+struct NodeLaunch {
+  SceneDev S;
+  const float4 *npos;        // per vertex: position, w = 1 for a static node
+  const float4 *nnorm;       // per vertex: unit normal
+  const uint32_t *nmat;      // per vertex: material
+  uint32_t n_nodes, K, node_samples, n_active, n_mat;
+  uint32_t *gtable;          // n_nodes * K
+  float *miss;               // per vertex: escaping gather rays (K for a vertex that is no static node)
+  const uint32_t *active;
+  float4 *state;             // 4 per vertex
+  float4 *alb, *hh;          // albedo tables (ALB_BINS x 2 per material), hemispherical albedo
+  uint32_t *overflow;
+  uint2 *spill;
+  uint64_t spill_stride;
+  uint32_t max_blocks;       // spill_stride / 256
+};
+hipError_t launch_albedo_tables(const NodeLaunch &L, hipStream_t s);
+hipError_t launch_fill_f4(float4 *p, uint32_t n, float4 v, hipStream_t s);    // a fill kernel (no memset nodes)
+hipError_t launch_eavg(const NodeLaunch &L, const float *area, float4 *radu, uint32_t slot, hipStream_t s);
+hipError_t launch_gather_build(const NodeLaunch &L, hipStream_t s);
+hipError_t launch_gather_count(const NodeLaunch &L, hipStream_t s);
+hipError_t launch_gather_remap(const NodeLaunch &L, hipStream_t s);
+hipError_t launch_node_direct(const NodeLaunch &L, hipStream_t s);
+
 hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s);
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s);
 

@@ -5,30 +5,11 @@
 //      the BF16 sweep on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), F stored once as BF16 so a
 //      sweep streams 2 B per matrix entry; the form-factor build traces one visibility ray per node pair.
 #include "vt_internal.h"
+#include "vt_nodes.h"
 
 namespace vt {
 
 // ------------------------------------------------------------------ sparse gather Jacobi
-__device__ __forceinline__ v3 gather_dir(uint32_t i, uint32_t j, uint32_t K, v3 n) {
-  v3 t, b;
-  basis(n, t, b);
-  const float rot = u01(pcg(i * 2654435761u + 17u));
-  const float rr = sqrtf(((float)j + 0.5f) / (float)K);
-  const float ph = 2.0f * PI_F * kr(j, 0.6180339887498949f, rot);
-  float sn, cs;
-  sincosf(ph, &sn, &cs);
-  return normalize(t * (rr * cs) + b * (rr * sn) + n * sqrtf(fmaxf(0.0f, 1.0f - rr * rr)));
-}
-__device__ __forceinline__ v3 albedo_at(const float4 *alb, uint32_t bins, uint32_t m, float mu) {
-  float x = fminf(fmaxf(mu * (float)bins - 0.5f, 0.0f), (float)(bins - 1));
-  uint32_t i = (uint32_t)floorf(x);
-  uint32_t j = min(bins - 1, i + 1);
-  float f = x - (float)i;
-  const float4 *b = alb + (size_t)m * bins * 2;
-  v3 s0 = ld3(b[2 * i]), s1 = ld3(b[2 * j]), d0 = ld3(b[2 * i + 1]), d1 = ld3(b[2 * j + 1]);
-  return (s0 * (1.0f - f) + s1 * f) + (d0 * (1.0f - f) + d1 * f);
-}
-
 __global__ void k_jacobi_gather(GatherDev G) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= G.n_active) return;

@@ -5,6 +5,7 @@
 // pixel sum is a butterfly reduction within the wave (deterministic; one writer per pixel).
 #include "vt_dual.h"
 #include "vt_internal.h"
+#include "vt_nodes.h"
 
 namespace vt {
 
@@ -179,6 +180,179 @@ template <bool INST>
 __device__ __forceinline__ v3 path_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
                                             uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
   return path_walk<false, INST>(S, C, P, x, y, i, st, nrays);
+}
+
+// ------------------------------------------------------------------ the reference's camera estimator
+// The hit's surface (surface, trace.wgsl.js:172-195): vertex ids, barycentrics, the outward interpolated normal
+// (nsi, unflipped), the two-sided shading and geometric normals, the point; posed bodies in the world at `time`
+struct Surf {
+  v3 p, ng, ns, nsi, b;
+  uint4 vi;
+  uint4 ms;     // mesh record: material, flags, light, -
+};
+template <bool INST>
+__device__ __forceinline__ Surf surface_at(const SceneDev &S, const Hit &h, v3 o, v3 d, float time) {
+  Surf s;
+  const float4 *sh = S.tshade + 4 * (size_t)h.tri;
+  const float4 n0 = sh[0];
+  s.ms = S.meshes[__float_as_uint(n0.w)];
+  s.vi = S.tvid[h.tri];
+  s.b = mk(1.0f - h.u - h.v, h.u, h.v);
+  v3 ng = ld3(sh[3]);
+  v3 nsi = normalize(ld3(n0) * s.b.x + ld3(sh[1]) * s.b.y + ld3(sh[2]) * s.b.z);
+  if (INST && h.inst != NO_INST) {
+    const Pose Pb = inst_pose(S, h.inst, time);
+    ng = rot(Pb, ng);
+    nsi = rot(Pb, nsi);
+  }
+  v3 ns = nsi;
+  if (dot(ng, d) > 0.0f) ng = -ng;
+  if (dot(ns, ng) < 0.0f) ns = -ns;
+  if (dot(ns, -d) < 1e-4f) ns = ng;
+  s.ng = ng;
+  s.ns = ns;
+  s.nsi = nsi;
+  s.p = o + d * h.t;
+  return s;
+}
+__device__ __forceinline__ uint32_t rad_slot(const SceneDev &S) { return __float_as_uint(S.radu[0].w); }
+__device__ __forceinline__ bool is_static_hit(const Hit &h, const Surf &s) {
+  return (s.ms.y & F_STATIC) && h.inst == NO_INST;
+}
+
+// cachedVisibility (shade.wgsl.js:108-130): 1 or 0 when the hit triangle's three nodes agree that every lit emitter
+// sample is visible or hidden and no visible body's bounding capsule comes near the segment to the emitters'
+// centroid (its radius + 0.5 mm added); -1: trace the shadow rays
+template <bool INST>
+__device__ __forceinline__ float cached_visibility(const SceneDev &S, const Surf &s) {
+  const float a = S.nstate[4 * (size_t)s.vi.x + 3].w, b = S.nstate[4 * (size_t)s.vi.y + 3].w,
+              c = S.nstate[4 * (size_t)s.vi.z + 3].w;
+  if (!(a == b && b == c && (a == 1.0f || a == 0.0f))) return -1.0f;
+  if (INST) {
+    const float4 ec = S.radu[1];
+    const v3 d = ld3(ec) - s.p;
+    for (uint32_t i = 0; i < S.inst_count; i++) {
+      const float4 *I = S.inst + INST_WORDS * (size_t)i;
+      if ((__float_as_uint(I[3].w) & INST_HIDDEN) || ((S.shadow_skip >> i) & 1u)) continue;
+      if (seg_near_capsule(s.p, d, 0.0f, 1.0f, ld3(I[6]), ld3(I[7]), I[6].w + ec.w + 5e-4f)) return -1.0f;
+    }
+  }
+  return a;
+}
+
+// one emitter sample's contribution at the surface (directLight, shade.wgsl.js:160-188)
+template <bool INST>
+__device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float time, float u0,
+                                           float u1, float u2, bool mis, float fnl, float cached, Stack &st,
+                                           uint32_t &nrays) {
+  v3 pl;
+  const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+  v3 wi = pl - s.p;
+  const float d2 = dot(wi, wi), dist = sqrtf(d2);
+  wi = wi * (1.0f / dist);
+  const float cosE = -dot(wi, ld3(S.emit[4 * e + 3]));
+  if (cosE <= 0.0f) return mk(0.f, 0.f, 0.f);
+  const float cosS = dot(s.ns, wi);
+  if (cosS <= 0.0f || dot(s.ng, wi) <= 0.0f) return mk(0.f, 0.f, 0.f);
+  const uint32_t lid = __float_as_uint(S.emit[4 * e + 2].w);
+  const v3 Le = ld3(S.lights[lid]) * light_profile(S, lid, -wi);
+  if (maxc(Le) <= 0.0f) return mk(0.f, 0.f, 0.f);
+  const v3 f = eval_bsdf(M, s.ns, wi, wo);
+  if (maxc(f) <= 0.0f) return mk(0.f, 0.f, 0.f);
+  if (cached == 0.0f) return mk(0.f, 0.f, 0.f);
+  if (cached < 0.0f) {
+    Hit sh;
+    nrays++;
+    const v3 so = s.p + s.ng * (dot(s.ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
+    if (trace_scene<true, INST>(S, so, wi, 0.0f, dist - 2e-5f, time, S.shadow_skip, sh, st)) return mk(0.f, 0.f, 0.f);
+  }
+  const float pW = d2 / (cosE * S.emit_area);
+  float w = 1.0f;
+  if (mis) {
+    const float pb = pdf_bsdf(M, s.ns, wo, wi);
+    w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
+  }
+  return f * Le * (cosS * w / pW);
+}
+
+// radiance along a body's BSDF-sampled secondary ray (secondary, shade.wgsl.js:222-243): the emitters with MIS,
+// the cached source of a static surface toward the ray, or the mean irradiance for another body
+template <bool INST>
+__device__ __forceinline__ v3 secondary(const SceneDev &S, v3 o, v3 l, float time, float pdfB, uint32_t opts, float fnl,
+                                        Stack &st, uint32_t &nrays) {
+  Hit h;
+  nrays++;
+  if (!trace_scene<false, INST>(S, o, l, 0.0f, INFINITY, time, 0u, h, st)) return mk(0.f, 0.f, 0.f);
+  const Surf s = surface_at<INST>(S, h, o, l, time);
+  if (s.ms.y & F_EMITTER) {
+    if (opts & OPT_NO_MIS) return mk(0.f, 0.f, 0.f);
+    const float cosE = -dot(l, s.nsi);
+    if (cosE <= 0.0f) return mk(0.f, 0.f, 0.f);
+    const v3 Le = ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, -l);
+    const float pL = h.t * h.t / (cosE * S.emit_area);
+    return Le * (pdfB * pdfB / (pdfB * pdfB + fnl * fnl * pL * pL));
+  }
+  if (opts & (OPT_DIRECT_ONLY | OPT_NO_INDIRECT)) return mk(0.f, 0.f, 0.f);
+  if (is_static_hit(h, s)) {
+    const float mu = fminf(fmaxf(dot(s.ns, -l), 0.0f), 1.0f);
+    const uint32_t slot = rad_slot(S);
+    return albedo_at(S.alb, ALB_BINS, s.ms.x, mu) * node_interp(S, s.vi, s.b, slot, slot);
+  }
+  return ld3(S.hh[s.ms.x]) * ld3(S.radu[0]) * INV_PI_F;
+}
+
+// shadeHit (shade.wgsl.js:250-334) of a camera ray: an emitter's radiance; else nL light samples (MIS on posed
+// bodies only), then on static surfaces the node cache E_free x capsule visibility x A(mu) / pi, on bodies one
+// BSDF bounce into `secondary`. Light sample j uses kr(i nL + j); the bounce the first BSDF dimensions.
+template <bool INST>
+__device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
+                                           uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
+  const float time = sample_time(C, P, i);
+  const v3 o = mk(C.pos[0], C.pos[1], C.pos[2]);
+  const v3 d = camera_dir(C, P, x, y, i);
+  Hit h;
+  nrays++;
+  if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, C.skip, h, st)) return mk(0.f, 0.f, 0.f);
+  const Surf s = surface_at<INST>(S, h, o, d, time);
+  const v3 wo = -d;
+  const uint32_t opts = C.opts;
+  if (opts & OPT_NORMALS) return s.ns * 0.5f + mk(0.5f, 0.5f, 0.5f);
+  if (s.ms.y & F_EMITTER) {
+    if (dot(wo, s.nsi) <= 0.0f) return mk(0.f, 0.f, 0.f);
+    return ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, wo);
+  }
+  const Mat M = load_mat(S.mats, s.ms.x);
+  const bool stat = is_static_hit(h, s);
+  const bool mis = !stat && !(opts & OPT_NO_MIS);
+  const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+  const float fnl = (float)nl;
+  const float cached = (stat && (opts & OPT_CULL_SHADOWS)) ? cached_visibility<INST>(S, s) : -1.0f;
+  v3 L = mk(0.f, 0.f, 0.f);
+  if (S.emit_count > 0) {
+    for (uint32_t j = 0; j < nl; j++) {
+      const uint32_t li = i * nl + j;
+      L = L + direct_light<INST>(S, s, M, wo, time, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
+                                 mis, fnl, cached, st, nrays);
+    }
+  }
+  L = L * (1.0f / fnl);
+  if (stat) {
+    if (!(opts & (OPT_NO_INDIRECT | OPT_DIRECT_ONLY))) {
+      v3 E = node_interp(S, s.vi, s.b, rad_slot(S), 0);
+      if (INST && !(opts & OPT_NO_CAPSULE)) E = E * capsule_vis(S, s.p, s.ns);
+      const float mu = fmaxf(dot(s.ns, wo), 0.0f);
+      L = L + E * INV_PI_F * albedo_at(S.alb, ALB_BINS, s.ms.x, mu);
+    }
+  } else {
+    v3 l;
+    const float pdf = sample_bsdf(M, s.ns, wo, kr(i, KB0, P.s[6]), kr(i, KB1, P.s[7]), kr(i, KB2, P.s[8]), l);
+    if (pdf > 0.0f && dot(s.ng, l) > 0.0f) {
+      const v3 thr = eval_bsdf(M, s.ns, l, wo) * (dot(s.ns, l) / pdf);
+      const v3 so = s.p + s.ng * (dot(s.ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
+      L = L + thr * secondary<INST>(S, so, l, time, pdf, opts, fnl, st, nrays);
+    }
+  }
+  return L;
 }
 
 // ------------------------------------------------------------------ camera adjoint (interior term)
@@ -430,7 +604,8 @@ __device__ __forceinline__ float wave_sum_xor(float v, uint32_t width) {
 
 // one 64-item wave of the queue
 // nrays_acc / nsamp_acc: the lane's running counts, added to the launch counters once when the wave leaves
-template <bool INST>
+// REF: the launch's cameras use the reference estimator (a launch holds cameras of one estimator)
+template <bool INST, bool REF>
 __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, Stack &st, uint32_t &nrays_acc,
                                          uint32_t &nsamp_acc) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -451,7 +626,8 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
   if (valid) {
     PixSeeds P;
     pix_seeds(C, x, y, P);
-    L = path_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
+    if constexpr (REF) L = ref_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
+    else L = path_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
   }
   // per-pixel sums: aligned power-of-two runs reduce by butterfly; other run shapes add atomically
   const bool pow2 = (nspp & (nspp - 1)) == 0 && nspp <= 64;
@@ -664,7 +840,7 @@ hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BLOCK, bool INST>
+template <int BLOCK, bool INST, bool REF>
 __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
   // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
   extern __shared__ uint32_t s_stack[];
@@ -698,14 +874,14 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
         if (base >= q1 - q0) break;
         const uint64_t b = q0 + base;
         const uint64_t end = b + Lc.chunk < q1 ? b + Lc.chunk : q1;
-        for (uint64_t w = b; w < end; w++) run_wave<INST>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
+        for (uint64_t w = b; w < end; w++) run_wave<INST, REF>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
       }
     }
   } else {
     // static schedule: the waves of the grid stride over the queue
     const uint64_t stride = (uint64_t)gridDim.x * (BLOCK / 64);
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / 64) + wib; w < Lc.total_waves; w += stride)
-      run_wave<INST>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
+      run_wave<INST, REF>(Lc, w * Lc.wave_stride, st, nrays, nsamp);
   }
   flush_counts(Lc, nrays, nsamp);
 }
@@ -762,8 +938,9 @@ hipError_t launch_zero_f4(float4 *p, uint64_t n, hipStream_t s) {
 
 namespace {
 template <int BLOCK>
-void (*render_kernel(bool inst))(RenderLaunch) {
-  return inst ? k_render<BLOCK, true> : k_render<BLOCK, false>;
+void (*render_kernel(bool inst, bool ref))(RenderLaunch) {
+  if (ref) return inst ? k_render<BLOCK, true, true> : k_render<BLOCK, false, true>;
+  return inst ? k_render<BLOCK, true, false> : k_render<BLOCK, false, false>;
 }
 }  // namespace
 
@@ -772,26 +949,26 @@ hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
       (L.queues != 1 && L.queues != XCD_COUNT) || L.chunk == 0)
     return hipErrorInvalidValue;
   const size_t lds = (size_t)L.stack_lds * L.block * 8;
-  const bool inst = L.S.inst_count != 0;
+  const bool inst = L.S.inst_count != 0, ref = L.ref != 0;
   switch (L.block) {
-    case 64: hipLaunchKernelGGL(render_kernel<64>(inst), dim3(L.grid), dim3(64), lds, s, L); break;
-    case 128: hipLaunchKernelGGL(render_kernel<128>(inst), dim3(L.grid), dim3(128), lds, s, L); break;
-    case 256: hipLaunchKernelGGL(render_kernel<256>(inst), dim3(L.grid), dim3(256), lds, s, L); break;
+    case 64: hipLaunchKernelGGL(render_kernel<64>(inst, ref), dim3(L.grid), dim3(64), lds, s, L); break;
+    case 128: hipLaunchKernelGGL(render_kernel<128>(inst, ref), dim3(L.grid), dim3(128), lds, s, L); break;
+    case 256: hipLaunchKernelGGL(render_kernel<256>(inst, ref), dim3(L.grid), dim3(256), lds, s, L); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 // resident blocks per CU of the render kernel at this block size and LDS stack depth
-hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, int *blocks_per_cu) {
+hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, bool ref, int *blocks_per_cu) {
   const size_t lds = (size_t)stack_lds * block * 8;
   switch (block) {
     case 64:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<64>(inst), 64, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<64>(inst, ref), 64, lds);
     case 128:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<128>(inst), 128, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<128>(inst, ref), 128, lds);
     case 256:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<256>(inst), 256, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, render_kernel<256>(inst, ref), 256, lds);
     default: return hipErrorInvalidValue;
   }
 }
@@ -886,6 +1063,19 @@ __global__ void k_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n
 hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n, float *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_camray_test, dim3((n + 255) / 256), dim3(256), 0, s, cam, xyi, n, out);
+  return hipGetLastError();
+}
+
+// capsuleOcc on caller cases: 16 floats each (p.xyz, -, n.xyz, -, a.xyz, r, b.xyz, -) -> occlusion
+__global__ void k_capsule_test(const float *cases, uint32_t n, float *out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float *c = cases + 16 * (size_t)k;
+  out[k] = capsule_occ(mk(c[0], c[1], c[2]), mk(c[4], c[5], c[6]), mk(c[8], c[9], c[10]), mk(c[12], c[13], c[14]), c[11]);
+}
+hipError_t launch_capsule_test(const float *cases, uint32_t n, float *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_capsule_test, dim3((n + 255) / 256), dim3(256), 0, s, cases, n, out);
   return hipGetLastError();
 }
 

@@ -11,14 +11,25 @@ const native = require('../napi/vt_napi.node');
 /** triBody entry of a static-scene triangle */
 export const STATIC_BODY = 0xffffffff;
 
-export const OPTION_BITS = Object.freeze({ NO_MIS: 128 });
+// camera option bits (gpu/wgsl/common.wgsl.js:45-58), plus the estimator choice and the shadow culling
+export const OPTION_BITS = Object.freeze({ NO_INDIRECT: 4, DIRECT_ONLY: 8, NO_CAPSULE: 32, NORMALS: 64, NO_MIS: 128,
+  REFERENCE: 1 << 16, CULL_SHADOWS: 1 << 17 });
 export const DEFAULTS = Object.freeze({
   spp: 16,            // sub-samples per pixel
-  maxDepth: 3,        // path vertices (camera hit = 1)
+  maxDepth: 3,        // path vertices (camera hit = 1), path tracer only
   lightSamples: 1,    // emitter samples per path vertex
   seed: 0,
   batched: true,      // every camera of a frame in one persistent launch
   graph: true,        // batched frames run as a hipGraph, captured once and replayed
+  estimator: 'path',  // 'path': depth-D path tracing; 'reference': the reference's shadeHit over the node cache
+  gatherK: 64,        // node cache (gpu/renderer.js:31-58 DEFAULTS): gather rays per node
+  nodeSamples: 4,     // emitter samples per node per update
+  itersPerFrame: 4,   // warm-started Jacobi steps per pose change
+  itersInitial: 96,   // on the first solve
+  smoothSigma: 0.003, // Gaussian sigma (m) of the node irradiance smoothing; 0 = off
+  smoothRadius: 0.006,
+  nbrK: 16,
+  cullShadows: true,  // skip shadow rays whose answer the three nodes and the bodies' capsules already give
 });
 export const PRESETS = Object.freeze({
   live: { spp: 1, maxDepth: 2, lightSamples: 1 },
@@ -91,7 +102,11 @@ export function fromPrepared(S) {
     const triBody = new Uint32Array(S.tris.length / 16).fill(STATIC_BODY);
     S.blas.forEach((b, i) => triBody.fill(i, b.triOff, b.triOff + b.triCount));
     Object.assign(out, { nodes: undefined, triBody,
-      bodies: S.blas.map((b) => ({ key: b.body, kind: b.kind, owner: b.owner || null })) });
+      bodies: S.blas.map((b) => {
+        const oc = b.occCapsule || b.capsule;   // the occlusion capsule of capsuleVis, in the body frame
+        return { key: b.body, kind: b.kind, owner: b.owner || null, rhoEff: b.rhoEff || 0,
+          occ: oc ? { a: Array.from(oc.a), b: Array.from(oc.b), r: oc.r } : null };
+      }) });
     // shadow rays leave through the scope's housing when the emitter sits at its front (gpu/renderer.js:648-649)
     const scopeI = S.blas.findIndex((b) => b.body === 'scope');
     out.shadowSkip = em.shadowSkipsScope && scopeI >= 0 ? (1 << scopeI) >>> 0 : 0;
@@ -147,7 +162,61 @@ export class TwinRenderer {
     this.frame = 0;
     this.poses0 = null;
     this.poses1 = null;
+    this.posesVersion = 0;
+    this.solvedVersion = -1;
+    this.radiosity = null;
+    this.nodeCount = src.verts.length / 8;
+    this.eAvg = [0, 0, 0];
     if (this.bodies.length) this.setPoses(null);
+  }
+
+  /** camera option bits of a render (gpu/renderer.js:922-933 _options) */
+  _options(o = {}) {
+    let bits = 0;
+    if (o.noIndirect) bits |= OPTION_BITS.NO_INDIRECT;
+    if (o.directOnly) bits |= OPTION_BITS.DIRECT_ONLY;
+    if (o.noCapsule) bits |= OPTION_BITS.NO_CAPSULE;
+    if (o.normals) bits |= OPTION_BITS.NORMALS;
+    if (o.noMIS) bits |= OPTION_BITS.NO_MIS;
+    if (dflt(o.estimator, this.opts.estimator) === 'reference') {
+      bits |= OPTION_BITS.REFERENCE;
+      if (dflt(o.cullShadows, this.opts.cullShadows)) bits |= OPTION_BITS.CULL_SHADOWS;
+    }
+    return bits >>> 0;
+  }
+
+  /**
+   * The node cache for the current poses (_ensureRadiosity, gpu/renderer.js:766-795): built on first use (gather
+   * table, activity, neighbours, albedo tables), then the direct light and itersInitial Jacobi steps; after a
+   * pose change the direct light again and itersPerFrame steps.
+   */
+  _ensureRadiosity() {
+    const o = this.opts;
+    if (!this.radiosity) {
+      this.radiosity = native.radiosityBuild(this.ctx, { gatherK: o.gatherK, nodeSamples: o.nodeSamples, nbrK: o.nbrK,
+        smoothRadius: o.smoothRadius, smoothSigma: o.smoothSigma });
+      this.stats.activeNodes = this.radiosity.active;
+      this.stats.gatherBuildMs = this.radiosity.ms;
+    }
+    if (this.solvedVersion === this.posesVersion) return;
+    native.radiosityUpdate(this.ctx, true, this.solvedVersion < 0 ? o.itersInitial : o.itersPerFrame);
+    this.solvedVersion = this.posesVersion;
+  }
+
+  /** Jacobi steps to convergence (converge, gpu/renderer.js:804-822), then nodeSummary. */
+  async converge(iters = 200) {
+    if (typeof iters !== 'number' || !Number.isFinite(iters)) iters = iters === false ? 0 : 200;
+    this._ensureRadiosity();
+    native.radiosityUpdate(this.ctx, false, iters);
+    return this.nodeSummary();
+  }
+
+  /** The node state and eAvg, the area-weighted mean irradiance (nodeSummary, gpu/renderer.js:824-852). */
+  async nodeSummary() {
+    if (!this.radiosity) this._ensureRadiosity();
+    const r = native.radiosityState(this.ctx, this.nodeCount);
+    this.eAvg = r.eAvg;
+    return { eAvg: r.eAvg, state: r.state, activeNodes: this.radiosity.active };
   }
 
   _addCamera(c, index) {
@@ -174,7 +243,7 @@ export class TwinRenderer {
       R: cam.R, pos: cam.pos, lens: cam.lens, mode: cam.mode, width: cam.size[0], height: cam.size[1], spp,
       seed: dflt(o.seed, this.opts.seed), camIndex: cam.index, lightSamples: dflt(o.lightSamples, this.opts.lightSamples),
       maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1), shutter: o.shutter || [0, 1],
-      skipBodies: this._skipMask(cam.name),
+      skipBodies: this._skipMask(cam.name), options: this._options(o),
     });
   }
 
@@ -213,11 +282,13 @@ export class TwinRenderer {
     const b0 = (poses0 && poses0.bodies) || {}, b1 = ((poses1 || poses0) && (poses1 || poses0).bodies) || {};
     for (const k of Object.keys(b0)) if (!this.bodies.some((b) => b.key === k)) throw new Error(`unknown body ${k}`);
     this.instances = this.bodies.map((bl) => {
-      if (bl.kind === 'fixed') return { q0: [0, 0, 0, 1], q1: [0, 0, 0, 1], p0: [0, 0, 0], p1: [0, 0, 0] };
+      const occ = bl.occ ? { occA: bl.occ.a, occB: bl.occ.b, occR: bl.occ.r, rhoEff: dflt(bl.rhoEff, 0) } : {};
+      if (bl.kind === 'fixed') return { q0: [0, 0, 0, 1], q1: [0, 0, 0, 1], p0: [0, 0, 0], p1: [0, 0, 0], ...occ };
       const T0 = b0[bl.key], T1 = b1[bl.key] || T0;
       if (!T0) return null;
-      return { q0: quatFromMat(T0), q1: quatFromMat(T1), p0: [T0[12], T0[13], T0[14]], p1: [T1[12], T1[13], T1[14]] };
+      return { q0: quatFromMat(T0), q1: quatFromMat(T1), p0: [T0[12], T0[13], T0[14]], p1: [T1[12], T1[13], T1[14]], ...occ };
     });
+    this.posesVersion = (this.posesVersion || 0) + 1;
     if (this.bodies.length) native.setInstances(this.ctx, this.instances);
     this.poses0 = poses0; this.poses1 = poses1 || poses0;
   }
@@ -234,6 +305,7 @@ export class TwinRenderer {
   renderFrame(opts = {}) {
     const names = opts.cameras || Object.keys(this.cameras);
     const cams = names.map((n) => this._slot(n));
+    if (dflt(opts.estimator, this.opts.estimator) === 'reference') this._ensureRadiosity();
     cams.forEach((c) => this._pack(c, opts));
     // batched frames run in the form autotune() found fastest (the persistent launch, as a hipGraph, until then)
     const batched = dflt(opts.batched, this.opts.batched);
@@ -283,6 +355,7 @@ export class TwinRenderer {
   async autotune(opts = {}) {
     const names = opts.cameras || Object.keys(this.cameras);
     const cams = names.map((n) => this._slot(n));
+    if (dflt(opts.estimator, this.opts.estimator) === 'reference') this._ensureRadiosity();
     cams.forEach((c) => this._pack(c, opts));
     this.tuned = await native.autotune(this.ctx, cams.map((c) => c.index), { probeStride: opts.probeStride || 0 });
     return this.tuned;

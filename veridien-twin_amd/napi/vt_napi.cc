@@ -225,6 +225,7 @@ napi_value SetCamera(napi_env env, napi_callback_info info) {
   k.cam_index = (uint32_t)num(env, argv[2], "camIndex", slot);
   k.light_samples = (uint32_t)num(env, argv[2], "lightSamples", 1);
   k.max_depth = (uint32_t)num(env, argv[2], "maxDepth", 1);
+  k.flags = (uint32_t)num(env, argv[2], "options", 0);
   k.scale = (float)num(env, argv[2], "scale", 1.0);
   k.skip_bodies = (uint32_t)num(env, argv[2], "skipBodies", 0);
   k.shutter[0] = 0.0f;
@@ -270,6 +271,11 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
     }
     if (!float_list(env, e, "q1", I.q1, 4)) std::memcpy(I.q1, I.q0, sizeof I.q0);
     if (!float_list(env, e, "p1", I.p1, 3)) std::memcpy(I.p1, I.p0, sizeof I.p0);
+    // occlusion capsule (body frame) and effective reflectance, when the pose carries them
+    if (float_list(env, e, "occA", I.occ_a, 3) && float_list(env, e, "occB", I.occ_b, 3)) {
+      I.occ_r = (float)num(env, e, "occR", 0);
+      I.rho_eff = (float)num(env, e, "rhoEff", 0);
+    }
   }
   int rc = vt_set_instances(c, v.data(), len);
   if (rc) return throw_vt(env, c, rc, "vt_set_instances");
@@ -676,6 +682,83 @@ napi_value update_table(napi_env env, napi_callback_info info, bool mats) {
 napi_value UpdateMaterials(napi_env env, napi_callback_info info) { return update_table(env, info, true); }
 napi_value UpdateLights(napi_env env, napi_callback_info info) { return update_table(env, info, false); }
 
+// radiosityBuild(ctx, {gatherK, nodeSamples, nbrK, smoothRadius, smoothSigma}) -> {nodes, staticNodes, active, ms}
+napi_value RadiosityBuild(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  vt_radiosity_opts o{64, 4, 16, 0.006f, 0.003f};
+  napi_valuetype t = napi_undefined;
+  if (argc > 1) napi_typeof(env, argv[1], &t);
+  if (t == napi_object) {
+    o.gather_k = (uint32_t)num(env, argv[1], "gatherK", o.gather_k);
+    o.node_samples = (uint32_t)num(env, argv[1], "nodeSamples", o.node_samples);
+    o.nbr_k = (uint32_t)num(env, argv[1], "nbrK", o.nbr_k);
+    o.smooth_radius = (float)num(env, argv[1], "smoothRadius", o.smooth_radius);
+    o.smooth_sigma = (float)num(env, argv[1], "smoothSigma", o.smooth_sigma);
+  }
+  vt_radiosity_info inf{};
+  int rc = vt_radiosity_build(c, &o, &inf);
+  if (rc) return throw_vt(env, c, rc, "vt_radiosity_build");
+  napi_value out, v;
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_create_uint32(env, inf.nodes, &v));
+  CHECK_NAPI(napi_set_named_property(env, out, "nodes", v));
+  CHECK_NAPI(napi_create_uint32(env, inf.static_nodes, &v));
+  CHECK_NAPI(napi_set_named_property(env, out, "staticNodes", v));
+  CHECK_NAPI(napi_create_uint32(env, inf.active, &v));
+  CHECK_NAPI(napi_set_named_property(env, out, "active", v));
+  CHECK_NAPI(napi_create_double(env, inf.ms_build, &v));
+  CHECK_NAPI(napi_set_named_property(env, out, "ms", v));
+  return out;
+}
+
+// radiosityUpdate(ctx, direct: bool, iters) (enqueue)
+napi_value RadiosityUpdate(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  bool direct = true;
+  uint32_t iters = 0;
+  if (argc > 1) napi_get_value_bool(env, argv[1], &direct);
+  if (argc > 2) napi_get_value_uint32(env, argv[2], &iters);
+  int rc = vt_radiosity_update(c, direct ? 1u : 0u, iters);
+  if (rc) return throw_vt(env, c, rc, "vt_radiosity_update");
+  return nullptr;
+}
+
+// radiosityState(ctx, nodes) -> {eAvg: [3], state: Float32Array(nodes * 16)}; blocks until the queue is done
+napi_value RadiosityState(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t nodes = 0;
+  if (argc > 1) napi_get_value_uint32(env, argv[1], &nodes);
+  napi_value ab, st, out, ea;
+  void *dst = nullptr;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)nodes * 64, &dst, &ab));
+  float e[3] = {0, 0, 0};
+  int rc = vt_radiosity_state(c, nodes ? (float *)dst : nullptr, (size_t)nodes * 64, e);
+  if (rc) return throw_vt(env, c, rc, "vt_radiosity_state");
+  CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, (size_t)nodes * 16, ab, 0, &st));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &ea));
+  for (uint32_t k = 0; k < 3; k++) {
+    napi_value v;
+    CHECK_NAPI(napi_create_double(env, e[k], &v));
+    CHECK_NAPI(napi_set_element(env, ea, k, v));
+  }
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_set_named_property(env, out, "eAvg", ea));
+  CHECK_NAPI(napi_set_named_property(env, out, "state", st));
+  return out;
+}
+
 napi_value LastError(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
@@ -693,7 +776,9 @@ napi_value Init(napi_env env, napi_value exports) {
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
              {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
-             {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights}};
+             {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
+             {"radiosityBuild", RadiosityBuild}, {"radiosityUpdate", RadiosityUpdate},
+             {"radiosityState", RadiosityState}};
   for (auto &f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

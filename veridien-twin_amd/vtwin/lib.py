@@ -18,8 +18,13 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
-    "vt_set_light_profile",
+    "vt_set_light_profile", "vt_radiosity_build", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
+    "vt_radiosity_tables", "vt_capsule_test",
 ]
+
+# camera option bits (vt_camera.flags)
+VT_OPT_NO_INDIRECT, VT_OPT_DIRECT_ONLY, VT_OPT_NO_CAPSULE, VT_OPT_NORMALS, VT_OPT_NO_MIS = 4, 8, 32, 64, 128
+VT_OPT_REFERENCE, VT_OPT_CULL_SHADOWS = 1 << 16, 1 << 17
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
 CAMERA_PARAMS = 14      # VT_CAMERA_PARAMS: fx fy cx cy k1 k2 k3 k4, rotation rx ry rz, centre tx ty tz
@@ -41,7 +46,17 @@ STATIC_BODY = 0xFFFFFFFF
 
 
 class Instance(C.Structure):
-    _fields_ = [("q0", f32 * 4), ("q1", f32 * 4), ("p0", f32 * 3), ("p1", f32 * 3), ("hidden", u32)]
+    _fields_ = [("q0", f32 * 4), ("q1", f32 * 4), ("p0", f32 * 3), ("p1", f32 * 3), ("hidden", u32),
+                ("occ_a", f32 * 3), ("occ_b", f32 * 3), ("occ_r", f32), ("rho_eff", f32)]
+
+
+class RadiosityOpts(C.Structure):
+    _fields_ = [("gather_k", u32), ("node_samples", u32), ("nbr_k", u32), ("smooth_radius", f32),
+                ("smooth_sigma", f32)]
+
+
+class RadiosityInfo(C.Structure):
+    _fields_ = [("nodes", u32), ("static_nodes", u32), ("active", u32), ("ms_build", f64)]
 
 
 class Camera(C.Structure):
@@ -135,6 +150,12 @@ def load():
         "vt_set_instances": (i32, [vp, vp, u32]),
         "vt_visible": (i32, [vp, u32, vp, u32, f32, f32, vp]),
         "vt_set_light_profile": (i32, [vp, u32, vp, vp, vp, u32]),
+        "vt_radiosity_build": (i32, [vp, vp, vp]),
+        "vt_radiosity_update": (i32, [vp, u32, u32]),
+        "vt_radiosity_state": (i32, [vp, vp, sz, vp]),
+        "vt_node_neighbours": (i32, [vp, vp, u32, u32, f32, f32, vp, vp]),
+        "vt_radiosity_tables": (i32, [vp, u32, vp, sz]),
+        "vt_capsule_test": (i32, [vp, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -226,6 +247,10 @@ class Context:
             arr[i].q1[:] = [float(x) for x in P.get("q1", P["q0"])]
             arr[i].p0[:] = [float(x) for x in P["p0"]]
             arr[i].p1[:] = [float(x) for x in P.get("p1", P["p0"])]
+            arr[i].occ_a[:] = [float(x) for x in P.get("occ_a", (0, 0, 0))]
+            arr[i].occ_b[:] = [float(x) for x in P.get("occ_b", (0, 0, 0))]
+            arr[i].occ_r = float(P.get("occ_r", 0.0))
+            arr[i].rho_eff = float(P.get("rho_eff", 0.0))
         self.check(self.L.vt_set_instances(self.h, arr, len(poses)))
 
     def set_camera(self, slot, cam):
@@ -253,6 +278,34 @@ class Context:
         """The scene dict's profiles ({light: (axis, theta[], p[])}), if any, after upload_scene."""
         for light, (axis, th, p) in (sc.get("profiles") or {}).items():
             self.set_light_profile(light, axis, th, p)
+
+    # ---------------------------------------------------------------- node radiosity cache
+    def radiosity_build(self, gather_k=64, node_samples=4, nbr_k=16, smooth_radius=0.006, smooth_sigma=0.003):
+        o = RadiosityOpts(gather_k, node_samples, nbr_k, smooth_radius, smooth_sigma)
+        info = RadiosityInfo()
+        self.check(self.L.vt_radiosity_build(self.h, C.byref(o), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in RadiosityInfo._fields_}
+
+    def radiosity_update(self, direct=True, iters=0):
+        self.check(self.L.vt_radiosity_update(self.h, 1 if direct else 0, iters))
+
+    def capsule_test(self, cases):
+        cases = np.ascontiguousarray(cases, np.float32).reshape(-1, 16)
+        out = np.zeros(max(1, cases.shape[0]), np.float32)
+        self.check(self.L.vt_capsule_test(self.h, ptr(cases), cases.shape[0], ptr(out)))
+        return out[: cases.shape[0]]
+
+    def radiosity_tables(self, which, shape, dtype):
+        out = np.zeros(shape, dtype)
+        self.check(self.L.vt_radiosity_tables(self.h, which, ptr(out), out.nbytes))
+        return out
+
+    def radiosity_state(self, n_nodes=None):
+        """(node state n x 16 or None, eAvg[3]); n_nodes = the scene's vertex count to download the state."""
+        e = (C.c_float * 3)()
+        st = np.zeros((n_nodes, 16), np.float32) if n_nodes else None
+        self.check(self.L.vt_radiosity_state(self.h, ptr(st), 0 if st is None else st.nbytes, e))
+        return st, np.array(e[:])
 
     def visible(self, slot, points, tol=2e-4, time=0.5):
         """vt_visible: bool per world point, seen unoccluded from the camera centre (to tol short of the point)."""
@@ -406,3 +459,17 @@ def sensor_struct(p):
     s.noise_on = 1 if p["noise_on"] else 0
     s.pupil_exp = float(p.get("pupil_exp", 1.0))
     return s
+
+
+def node_neighbours(pos, nrm, k=16, radius=0.006, sigma=0.003):
+    """vt_node_neighbours (host only): the smoothing neighbour rows of the node radiosity build."""
+    L = load()
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    idx = np.zeros((max(n, 1), k), np.uint32)
+    w = np.zeros((max(n, 1), k), np.float32)
+    rc = L.vt_node_neighbours(ptr(pos), ptr(nrm), n, k, float(radius), float(sigma), ptr(idx), ptr(w))
+    if rc:
+        raise VTError(rc, "vt_node_neighbours")
+    return idx[:n], w[:n]

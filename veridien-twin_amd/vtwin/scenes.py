@@ -268,6 +268,13 @@ def pack_scene(P, N, I, tri_mesh, meshes, materials, lights, bvh="vt", tri_body=
     verts = np.zeros((P.shape[0], 8), np.float32)
     verts[:, 0:3] = P
     verts[:, 4:7] = N
+    # word 3: the vertex's mesh (that of the first triangle using it), as the reference's prepared vertices
+    vm = np.zeros(P.shape[0], np.uint32)
+    flat = I.reshape(-1)
+    tm = np.repeat(np.asarray(tri_mesh, np.uint32), 3)
+    _, idx = np.unique(flat, return_index=True)
+    vm[flat[idx]] = tm[idx]
+    verts.view(np.uint32)[:, 3] = vm
     meshes = np.asarray(meshes, np.uint32).reshape(-1, 4)
     # emitter triangles: every triangle of an emitter mesh, outward normal from the vertex normals
     em_mask = (meshes[tri_mesh, 1] & MESH_EMITTER) != 0

@@ -152,9 +152,38 @@ typedef struct {
   float max_dn, black;
   uint32_t noise_on;
   float pupil_exp;       /* relative-illumination cos exponent (gpu/sensor.js:119) */
+  /* veiling glare (sensor.wgsl.js:87-141, the uniform's glare word, gpu/sensor.js:172-173): fraction k (0 = off),
+   * the Gaussian's sigma in pixels of the 1/F level, the level factor F (1..32) */
+  float glare_fraction, glare_sigma;
+  uint32_t glare_level;
 } vt_sensor_params;
 
-enum { VT_KIND_LINEAR = 0, VT_KIND_RAW = 1, VT_KIND_RI = 2 };
+/* The ISP after the raw DN (sensor.wgsl.js:224-427; the uniform's wb, c0-c2, tone, flags and tone2 words,
+ * gpu/sensor.js:169-176). */
+typedef struct {
+  uint32_t demosaic;     /* 0 bilinear, 1 edge-aware (OpenCV EA), 2 Hamilton-Adams */
+  uint32_t nr;           /* 1: 3x3 bilateral noise reduction (used when nr_sigma_px > 0) */
+  uint32_t tone;         /* 0 sRGB, 1 gamma, 2 linear, 3 sRGB + contrast S-curve */
+  uint32_t chroma420;    /* 1: RGB from 4:2:0 chroma and the YUV planes; 0: 4:4:4 RGB, no YUV planes */
+  float wb[3];           /* white-balance gains */
+  float ccm[9];          /* colour correction matrix rows */
+  float gamma;           /* tone 1 */
+  float s_curve_c;       /* tone 3: contrast exponent c */
+  float black_offset;    /* tone 3: black offset as a fraction of white */
+  float nr_sigma_px;     /* bilateral spatial sigma (px) */
+  float nr_range_scale;  /* bilateral range sigma = scale x the modelled noise */
+  float sharpen_amount;  /* unsharp mask on luma, 0 = off */
+  float sharpen_radius;  /* box radius (px), rounded and clamped to 1..4 */
+} vt_isp_params;
+
+/* readback kinds: LINEAR, RAW and RI as above; the ISP's outputs (gpu/renderer.js:1190-1231):
+ *   VT_KIND_RGB8    uint8 rgb, width*height*3 bytes
+ *   VT_KIND_RGBA8   uint8 rgba (a = 255), width*height*4 bytes
+ *   VT_KIND_YUV420  uint8 planes Y (width*height), Cb, Cr (ceil(width/2)*ceil(height/2) each); 4:2:0 only
+ *   VT_KIND_STATS   uint32[4]: the luma sum of the 8-bit Y, the grey-world sums of round(clamp(c)*1023) r g b
+ *                   (the AE / AWB statistics, gpu/renderer.js:1075-1089) */
+enum { VT_KIND_LINEAR = 0, VT_KIND_RAW = 1, VT_KIND_RI = 2, VT_KIND_RGB8 = 3, VT_KIND_RGBA8 = 4,
+       VT_KIND_YUV420 = 5, VT_KIND_STATS = 6 };
 
 typedef struct {
   double rays;           /* rays traced by the last vt_render (camera, extension and shadow) */
@@ -172,6 +201,7 @@ typedef struct {
   double rays_shadow;    /* wavefront: shadow rays of the last vt_render */
   double ms_ext_avg;     /* wavefront: average duration of its extension-trace launches */
   uint32_t ext_launches; /* wavefront: extension-trace launches of the last vt_render */
+  double ms_isp;         /* GPU time of the last vt_isp */
 } vt_frame_stats;
 
 /* ---------------------------------------------------------------- lifetime (TwinRenderer.create / destroy,
@@ -226,6 +256,10 @@ int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_o
                 vt_tune_result *out);
 /* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */
 int vt_sensor(vt_ctx *ctx, uint32_t cam, const vt_sensor_params *params);
+/* sensor(name) ISP stages after the raw DN (gpu/renderer.js:1091-1099, the demosaicG .. sensorOutput passes of
+ * :1034-1037): demosaic, WB + CCM, AE / AWB statistics, noise reduction, tone, sharpening, BT.601 YCbCr. Uses the
+ * camera's last vt_sensor output and its parameters (black, max DN, noise model, CFA). */
+int vt_isp(vt_ctx *ctx, uint32_t cam, const vt_isp_params *isp);
 /* readback(name, kind) (gpu/renderer.js:1191-1231). Blocks until the data is ready.
  *   VT_KIND_LINEAR  float32 rgba, width*height*16 bytes (mean radiance times scale; a = 1)
  *   VT_KIND_RAW     uint16 DN, width*height*2 bytes (the reference returns these integers as float32)
@@ -288,6 +322,10 @@ int vt_capsule_test(vt_ctx *ctx, const float *cases, uint32_t n, float *out);
 /* The fused sensor epilogue on caller-given radiance (rgba float32 w*h*4) and RI map (w*h) -> DN (w*h). */
 int vt_sensor_test(vt_ctx *ctx, const vt_sensor_params *params, uint32_t width, uint32_t height,
                    const float *radiance, const float *ri, uint16_t *dn);
+/* The ISP on caller-given raw DN (w*h uint16): rgba (w*h*4 bytes), yuv (NULL, or the planes as VT_KIND_YUV420
+ * when chroma420), stats (uint32[4], as VT_KIND_STATS). */
+int vt_isp_test(vt_ctx *ctx, const vt_sensor_params *sensor, const vt_isp_params *isp, uint32_t width,
+                uint32_t height, const uint16_t *dn, uint8_t *rgba, uint8_t *yuv, uint32_t *stats);
 
 /* Camera option bits. The reference's own (common.wgsl.js:45-58; they act in the reference estimator):
  * NO_INDIRECT / DIRECT_ONLY drop the cached indirect light, NO_CAPSULE the bodies' occlusion of it, NORMALS

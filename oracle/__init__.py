@@ -66,6 +66,10 @@ def lib():
             "or_cosf": (f32, [f32]),
             "or_erfa": (f32, [f32]),
             "or_sensor_expose": (None, [vp, vp, vp, vp]),
+            "or_glare_level": (None, [vp, vp, vp, vp]),
+            "or_isp": (None, [vp, vp, vp, vp, vp, vp]),
+            "or_isp_green": (None, [vp, vp, vp]),
+            "or_isp_tone": (f32, [vp, f32]),
             "or_ri_map": (None, [vp, i32, i32, f64, vp]),
             "or_raw_noise_dn": (f64, [f64, f64, f64, f64]),
             "or_render_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
@@ -121,7 +125,17 @@ class Sensor(C.Structure):
                 ("optics_k", C.c_float), ("esf_sigma", C.c_float), ("exposure", C.c_float), ("gain", C.c_float),
                 ("m", C.c_float * 9), ("read_noise", C.c_float), ("full_well", C.c_float), ("dark", C.c_float),
                 ("conv_gain", C.c_float), ("max_dn", C.c_float), ("black", C.c_float), ("noise_on", C.c_float),
-                ("seed", C.c_uint32), ("cam_index", C.c_uint32)]
+                ("seed", C.c_uint32), ("cam_index", C.c_uint32), ("glare_k", C.c_float),
+                ("glare_sigma", C.c_float), ("glare_f", C.c_uint32)]
+
+
+class Isp(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("cfa", C.c_uint32), ("demosaic", C.c_uint32),
+                ("nr", C.c_uint32), ("tone", C.c_uint32), ("chroma420", C.c_uint32), ("max_dn", C.c_float),
+                ("black", C.c_float), ("read_noise", C.c_float), ("conv_gain", C.c_float), ("gain", C.c_float),
+                ("wb", C.c_float * 3), ("ccm", C.c_float * 9), ("nr_sigma", C.c_float), ("nr_scale", C.c_float),
+                ("gamma", C.c_float), ("s_c", C.c_float), ("black_off", C.c_float), ("sharpen_amount", C.c_float),
+                ("sharpen_rad", C.c_int)]
 
 
 class Scene(C.Structure):
@@ -391,7 +405,78 @@ def sensor_struct(p, w, h):
     s.read_noise, s.full_well, s.dark, s.conv_gain = p["read_noise"], p["full_well"], p["dark"], p["conv_gain"]
     s.max_dn, s.black, s.noise_on = p["max_dn"], p["black"], 1.0 if p["noise_on"] else 0.0
     s.seed, s.cam_index = int(p["seed"]), int(p["cam_index"])
+    s.glare_k, s.glare_sigma = p.get("glare_fraction", 0.0), p.get("glare_sigma", 1.0)
+    s.glare_f = int(p.get("glare_level", 16))
     return s
+
+
+def glare_level(params, radiance, ri):
+    """the blurred glare level (gh, gw, 3)"""
+    radiance = np.ascontiguousarray(radiance, dtype=np.float32)
+    ri = np.ascontiguousarray(ri, dtype=np.float32)
+    h, w = ri.shape
+    s = sensor_struct(params, w, h)
+    F = s.glare_f
+    out = np.zeros(((h + F - 1) // F, (w + F - 1) // F, 3), np.float32)
+    lib().or_glare_level(C.byref(s), ptr(radiance), ptr(ri), ptr(out))
+    return out
+
+
+ISP_DEFAULTS = dict(demosaic=0, nr=0, tone=0, chroma420=1, wb=(1.0, 1.0, 1.0),
+                    ccm=(1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0), gamma=2.2, s_curve_c=1.0, black_offset=0.0,
+                    nr_sigma_px=0.4, nr_range_scale=2.0, sharpen_amount=0.0, sharpen_radius=1.0)
+
+
+def isp_struct(sensor, isp, w, h):
+    """or_isp from the raw stage's parameters (cfa, max_dn, black, read_noise, conv_gain, gain) and the ISP's
+    (keys of ISP_DEFAULTS)"""
+    q = dict(ISP_DEFAULTS, **isp)
+    s = Isp()
+    s.w, s.h, s.cfa = w, h, int(sensor["cfa"])
+    s.demosaic, s.tone, s.chroma420 = int(q["demosaic"]), int(q["tone"]), 1 if q["chroma420"] else 0
+    s.nr = 1 if (q["nr"] and q["nr_sigma_px"] > 0) else 0
+    s.max_dn, s.black, s.read_noise = sensor["max_dn"], sensor["black"], sensor["read_noise"]
+    s.conv_gain, s.gain = sensor["conv_gain"], sensor["gain"]
+    s.wb[:] = [float(x) for x in q["wb"]]
+    s.ccm[:] = [float(x) for x in np.asarray(q["ccm"]).reshape(-1)]
+    s.nr_sigma, s.nr_scale, s.gamma = q["nr_sigma_px"], q["nr_range_scale"], q["gamma"]
+    s.s_c, s.black_off, s.sharpen_amount = q["s_curve_c"], q["black_offset"], q["sharpen_amount"]
+    r = np.round(np.float32(q["sharpen_radius"]))
+    s.sharpen_rad = int(min(4, max(1, r)))
+    return s
+
+
+def isp(sensor, isp_params, dn):
+    """the ISP on raw DN (h, w): dict(rgba (h, w, 4) u8, y, cb, cr planes (4:2:0 only), stats u32[4], lin (h, w, 4))"""
+    dn = np.ascontiguousarray(dn, dtype=np.uint16)
+    h, w = dn.shape
+    s = isp_struct(sensor, isp_params, w, h)
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    rgba = np.zeros((h, w, 4), np.uint8)
+    yuv = np.zeros(w * h + 2 * cw * ch, np.uint8)
+    stats = np.zeros(4, np.uint32)
+    lin = np.zeros((h, w, 4), np.float32)
+    lib().or_isp(C.byref(s), ptr(dn), ptr(rgba), ptr(yuv), ptr(stats), ptr(lin))
+    out = dict(rgba=rgba, stats=stats, lin=lin, yuv=yuv)
+    if s.chroma420:
+        out.update(y=yuv[:w * h].reshape(h, w), cb=yuv[w * h:w * h + cw * ch].reshape(ch, cw),
+                   cr=yuv[w * h + cw * ch:].reshape(ch, cw))
+    return out
+
+
+def isp_green(sensor, isp_params, dn):
+    dn = np.ascontiguousarray(dn, dtype=np.uint16)
+    h, w = dn.shape
+    s = isp_struct(sensor, isp_params, w, h)
+    g = np.zeros((h, w), np.float32)
+    lib().or_isp_green(C.byref(s), ptr(dn), ptr(g))
+    return g
+
+
+def isp_tone(isp_params, x):
+    s = isp_struct(dict(cfa=0, max_dn=4095.0, black=0.0, read_noise=0.0, conv_gain=1.0, gain=1.0), isp_params, 1, 1)
+    return np.array([lib().or_isp_tone(C.byref(s), float(v)) for v in np.asarray(x, np.float32).reshape(-1)],
+                    np.float32)
 
 
 def ri_map(L, w, h, pupil_exp=1.0):

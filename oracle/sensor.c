@@ -143,6 +143,49 @@ static inline int cfa_channel(uint32_t x, uint32_t y, uint32_t code) {
   return pat[code & 3][(y & 1u) * 2u + (x & 1u)];
 }
 
+/* veiling glare (sensor.wgsl.js:87-127): block means on the 1/F level (at most 32 x 32 samples per block), then a
+ * Gaussian along rows and then columns of the level, zero outside it, divided by sigma sqrt(2 pi) */
+void or_glare_level(const or_sensor *s, const float *L, const float *ri, float *G) {
+  const int W = (int)s->width, H = (int)s->height, F = (int)s->glare_f;
+  const int gw = (W + F - 1) / F, gh = (H + F - 1) / F, fl = F < 32 ? F : 32;
+  float *T = malloc(sizeof(float) * 3 * (size_t)gw * gh);
+  for (int y = 0; y < gh; y++)
+    for (int x = 0; x < gw; x++) {
+      float a[3] = {0, 0, 0}, n = 0.0f;
+      for (int j = 0; j < fl; j++)
+        for (int i = 0; i < fl; i++) {
+          const int xx = x * F + i, yy = y * F + j;
+          if (xx >= W || yy >= H) continue;
+          const float *p = L + 4 * ((size_t)yy * W + xx);
+          const float q = ri[(size_t)yy * W + xx];
+          for (int c = 0; c < 3; c++) a[c] = a[c] + p[c] * q;
+          n = n + 1.0f;
+        }
+      const float sc = s->optics_k / (n > 1.0f ? n : 1.0f);
+      for (int c = 0; c < 3; c++) G[3 * ((size_t)y * gw + x) + c] = a[c] * sc;
+    }
+  const float sg = s->glare_sigma > 1e-3f ? s->glare_sigma : 1e-3f;
+  const float nrm = sg * 2.5066282746f;
+  for (int pass = 0; pass < 2; pass++) {      /* rows (G -> T), then columns (T -> G) */
+    const float *in = pass == 0 ? G : T;
+    float *out = pass == 0 ? T : G;
+    const int n = pass == 0 ? gw : gh, lim = n < 256 ? n : 256;
+    for (int y = 0; y < gh; y++)
+      for (int x = 0; x < gw; x++) {
+        float a[3] = {0, 0, 0};
+        for (int k = 0; k < lim; k++) {
+          const float d = (float)(k - (pass == 0 ? x : y));
+          const float w = or_expf(-0.5f * d * d / (sg * sg));
+          const float *c = in + 3 * (pass == 0 ? (size_t)y * gw + k : (size_t)k * gw + x);
+          for (int ch = 0; ch < 3; ch++) a[ch] = a[ch] + w * c[ch];
+        }
+        for (int ch = 0; ch < 3; ch++) out[3 * ((size_t)y * gw + x) + ch] = a[ch] / nrm;
+      }
+  }
+  free(T);
+}
+static inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
 void or_sensor_expose(const or_sensor *s, const float *L, const float *ri, uint16_t *dn) {
   const int W = (int)s->width, H = (int)s->height;
   const float sg = s->esf_sigma;
@@ -167,7 +210,15 @@ void or_sensor_expose(const or_sensor *s, const float *L, const float *ri, uint1
       o[0] = a0 * sc; o[1] = a1 * sc; o[2] = a2 * sc;
     }
   const uint32_t hs = or_pcg((s->frame * 0x85ebca6bu) ^ or_pcg(s->seed ^ (s->cam_index * 0xc2b2ae35u)));
-  /* sensorExpose (sensor.wgsl.js:177-208) without glare */
+  float *G = NULL;
+  int gw = 0, gh = 0;
+  if (s->glare_k > 0.0f) {
+    gw = (W + (int)s->glare_f - 1) / (int)s->glare_f;
+    gh = (H + (int)s->glare_f - 1) / (int)s->glare_f;
+    G = malloc(sizeof(float) * 3 * (size_t)gw * gh);
+    or_glare_level(s, L, ri, G);
+  }
+  /* sensorExpose (sensor.wgsl.js:177-208); glare read bilinearly from the level (glareAt :129-141) */
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       float e0 = 0, e1 = 0, e2 = 0, ws = 0;
@@ -179,6 +230,26 @@ void or_sensor_expose(const or_sensor *s, const float *L, const float *ri, uint1
         ws = ws + w;
       }
       e0 = e0 / ws; e1 = e1 / ws; e2 = e2 / ws;
+      if (G) {
+        const float F = (float)s->glare_f;
+        const float px = ((float)x + 0.5f) / F - 0.5f, py = ((float)y + 0.5f) / F - 0.5f;
+        const float fx0 = floorf(px), fy0 = floorf(py), fx = px - fx0, fy = py - fy0;
+        int xs[2] = {(int)fx0, (int)fx0 + 1}, ys[2] = {(int)fy0, (int)fy0 + 1};
+        for (int k = 0; k < 2; k++) {
+          xs[k] = xs[k] < 0 ? 0 : xs[k] > gw - 1 ? gw - 1 : xs[k];
+          ys[k] = ys[k] < 0 ? 0 : ys[k] > gh - 1 ? gh - 1 : ys[k];
+        }
+        float g[3];
+        for (int c = 0; c < 3; c++) {
+          const float a = G[3 * ((size_t)ys[0] * gw + xs[0]) + c], b = G[3 * ((size_t)ys[0] * gw + xs[1]) + c];
+          const float cc = G[3 * ((size_t)ys[1] * gw + xs[0]) + c], d = G[3 * ((size_t)ys[1] * gw + xs[1]) + c];
+          g[c] = mixf(mixf(a, b, fx), mixf(cc, d, fx), fy);
+        }
+        const float k = s->glare_k;
+        e0 = (1.0f - k) * e0 + k * g[0];
+        e1 = (1.0f - k) * e1 + k * g[1];
+        e2 = (1.0f - k) * e2 + k * g[2];
+      }
       int ch = cfa_channel((uint32_t)x, (uint32_t)y, s->cfa);
       const float *row = s->m + 3 * ch;
       float dt = (row[0] * e0 + row[1] * e1) + row[2] * e2;
@@ -191,6 +262,7 @@ void or_sensor_expose(const or_sensor *s, const float *L, const float *ri, uint1
       v = v < 0.0f ? 0.0f : v > s->max_dn ? s->max_dn : v;
       dn[(size_t)y * W + x] = (uint16_t)v;
     }
+  free(G);
   free(A);
 }
 

@@ -121,7 +121,26 @@ typedef struct {
   float read_noise, full_well, dark, conv_gain;
   float max_dn, black, noise_on; uint32_t seed;
   uint32_t cam_index;
+  float glare_k, glare_sigma; uint32_t glare_f; /* veiling glare: fraction (0 off), sigma (level px), level F */
 } or_sensor;
+/* the blurred veiling-glare level (sensor.wgsl.js:87-127): ceil(w/F) x ceil(h/F) x 3 floats */
+void or_glare_level(const or_sensor *s, const float *radiance, const float *ri, float *level);
+
+/* the ISP after the raw DN (sensor.wgsl.js:210-427), isp.c */
+typedef struct {
+  uint32_t w, h, cfa, demosaic, nr, tone, chroma420;   /* demosaic 0 bilinear 1 edge-aware 2 Hamilton-Adams;
+                                                          tone 0 sRGB 1 gamma 2 linear 3 sRGB + S-curve */
+  float max_dn, black, read_noise, conv_gain, gain;    /* from the raw stage */
+  float wb[3], ccm[9];
+  float nr_sigma, nr_scale, gamma, s_c, black_off, sharpen_amount;
+  int sharpen_rad;                                     /* 1..4 */
+} or_isp_p;
+void or_isp_green(const or_isp_p *P, const uint16_t *dn, float *g);
+void or_isp_rb(const or_isp_p *P, const uint16_t *dn, const float *g, float *rgb, uint32_t *stats);
+void or_isp_nr(const or_isp_p *P, const float *src, float *dst);
+float or_isp_tone(const or_isp_p *P, float x);
+void or_isp_out(const or_isp_p *P, const float *src, uint8_t *rgba, uint8_t *yuv, uint32_t *stats);
+void or_isp(const or_isp_p *P, const uint16_t *dn, uint8_t *rgba, uint8_t *yuv, uint32_t *stats, float *lin);
 /* sensorOptics + sensorExpose (sensor.wgsl.js:68-84, 143-208) for one camera: radiance rgba f32 [w*h*4],
  * ri [w*h] -> dn [w*h] (u16). Glare off (fraction 0). */
 void or_sensor_expose(const or_sensor *s, const float *radiance, const float *ri, uint16_t *dn);

@@ -17,13 +17,17 @@ async function main() {
   const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed,
     estimator: man.estimator || 'path' });
   if (man.poses0) r.setPoses(man.poses0, man.poses1 || null);
-  const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0 }, shutter: man.shutter });
+  const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0, exposure_s: man.exposure, gain: 1 }, shutter: man.shutter });
   await out.done;
   for (const c of man.cameras) {
     const lin = await r.readback(c.name, 'linear');
     fs.writeFileSync(path.join(dir, c.name + '.linear.bin'), Buffer.from(lin.buffer, lin.byteOffset, lin.byteLength));
     const raw = await r.readback(c.name, 'raw');
     fs.writeFileSync(path.join(dir, c.name + '.raw.bin'), Buffer.from(raw.buffer, raw.byteOffset, raw.byteLength));
+    const rgb = await r.readback(c.name, 'rgb8');
+    fs.writeFileSync(path.join(dir, c.name + '.rgb8.bin'), Buffer.from(rgb.buffer, rgb.byteOffset, rgb.byteLength));
+    const yuv = await r.readback(c.name, 'yuv420');
+    fs.writeFileSync(path.join(dir, c.name + '.y.bin'), Buffer.from(yuv.y.buffer, yuv.y.byteOffset, yuv.y.byteLength));
   }
   if (man.visiblePoints) {
     const vis = r.visible(man.cameras[0].name, man.visiblePoints, { tol: 2e-4, time: 0.5 });

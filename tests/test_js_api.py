@@ -18,6 +18,13 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(os.path.join(
                                 reason="node or the N-API addon is not available")
 
 
+def _js_sensor(sp):
+    """scenes.default_sensor as sensorParams overrides (gpu/sensor.js field names) for the JavaScript renderer"""
+    return dict(responseMatrix=[[float(v) for v in row] for row in np.asarray(sp["m"])], fNumber=2.0, transmission=0.9,
+                esfSigmaPx=0.7, readNoise=sp["read_noise"], fullWell=sp["full_well"], conversionGain=sp["conv_gain"],
+                adcBits=12, blackLevel=0.0, darkCurrent=0.0, cfa="RGGB", pupilExponent=1.0)
+
+
 def _node(code):
     r = subprocess.run([NODE, "--input-type=module", "-e", code], capture_output=True, text=True, timeout=120, cwd=ROOT)
     assert r.returncode == 0, r.stderr
@@ -81,9 +88,7 @@ def test_js_frame_equals_c_abi_frame(tmp_path):
     man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"],
                cameras=[dict(name="CAM1", lens=[float(x) for x in cam["lens"]], R=[float(x) for x in cam["R"]],
                              pos=[float(x) for x in cam["pos"]], size=[cam["width"], cam["height"]])],
-               sensor=dict(m=[float(x) for x in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
-                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
-                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+               sensor=_js_sensor(sp), exposure=sp["exposure"])
     (tmp_path / "manifest.json").write_text(json.dumps(man))
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
                        timeout=300, cwd=ROOT)
@@ -100,9 +105,16 @@ def test_js_frame_equals_c_abi_frame(tmp_path):
     lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
     ctx.sensor(0, dict(sp, seed=1006, frame=0))
     raw = ctx.readback(0, lib.VT_KIND_RAW, cam["width"], cam["height"])
+    # sensorParams' ISP defaults: bilinear, sRGB, 4:2:0, and NR on ('none' matches the reference's /on/ test)
+    ctx.isp(0, dict(nr=1))
+    rgb = ctx.readback(0, lib.VT_KIND_RGB8, cam["width"], cam["height"])
+    yuv = ctx.readback(0, lib.VT_KIND_YUV420, cam["width"], cam["height"])
     ctx.close()
     assert np.array_equal(js_lin, lin)
     assert np.array_equal(js_raw, raw.astype(np.float32))
+    js_rgb = np.fromfile(tmp_path / "CAM1.rgb8.bin", np.uint8).reshape(cam["height"], cam["width"], 3)
+    assert np.array_equal(js_rgb, rgb)
+    assert np.array_equal(np.fromfile(tmp_path / "CAM1.y.bin", np.uint8).reshape(cam["height"], cam["width"]), yuv["y"])
 
 
 @pytest.mark.gpu
@@ -217,9 +229,7 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
                shutter=[0.0, 1.0], visiblePoints=[[float(v) for v in p] for p in pts],
                cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
                              pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
-               sensor=dict(m=[float(v) for v in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
-                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
-                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+               sensor=_js_sensor(sp), exposure=sp["exposure"])
     (tmp_path / "manifest.json").write_text(json.dumps(man))
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
                        timeout=300, cwd=ROOT)
@@ -255,9 +265,7 @@ def test_js_reference_estimator_frame_equals_c_abi(tmp_path):
     man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=1, seed=cam["seed"], estimator="reference",
                cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
                              pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
-               sensor=dict(m=[float(v) for v in sp["m"].reshape(-1)], opticsK=sp["optics_k"], esfSigma=sp["esf_sigma"],
-                           exposure=sp["exposure"], readNoise=sp["read_noise"], fullWell=sp["full_well"],
-                           convGain=sp["conv_gain"], maxDn=sp["max_dn"], cfa=0))
+               sensor=_js_sensor(sp), exposure=sp["exposure"])
     (tmp_path / "manifest.json").write_text(json.dumps(man))
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
                        timeout=300, cwd=ROOT)

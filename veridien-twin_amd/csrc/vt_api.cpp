@@ -27,6 +27,10 @@ struct CamSlot {
   bool set = false;
   vt_camera c{};
   DevBuf acc, dn, ri, lin, adj;
+  // glare levels; ISP: green plane, linear RGB (two, for NR), 8-bit RGBA, YUV planes, statistics, rgb8 staging
+  DevBuf glare_a, glare_b, gpl, isp_a, isp_b, rgba, yuv, stats, rgb8;
+  vt_sensor_params sp{};      // the last vt_sensor's parameters (the ISP reads black, max DN, noise model, CFA)
+  bool isp_done = false, isp420 = false;
   bool ri_valid = false;
   float ri_pupil = -1.0f;
   uint32_t spp_total = 0;     // samples accumulated in acc (for the mean)
@@ -65,12 +69,12 @@ struct vt_ctx {
   DevCam *h_cams = nullptr;     // pinned staging for the camera table
   hipEvent_t ev_stage = nullptr;
   bool stage_pending = false;
-  hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
+  hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr, ev_i0 = nullptr, ev_i1 = nullptr;
   std::vector<hipEvent_t> ev_launch;
   uint32_t last_launches = 0;
   std::vector<hipEvent_t> ev_ext;   // wavefront: around each extension-trace launch of the last frame
   uint32_t ext_launches = 0;
-  bool render_timed = false, sensor_timed = false, backward_timed = false;
+  bool render_timed = false, sensor_timed = false, backward_timed = false, isp_timed = false;
   hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
   DevBuf grads, ltot;
   uint32_t n_mat = 0, n_light = 0;
@@ -286,7 +290,8 @@ extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
   if (!rc) rc = ensure(c, c->spill, c->spill_lanes * SPILL_ENTRIES * sizeof(uint2));
   if (!rc) rc = ensure(c, c->dcams, sizeof(DevCam) * VT_MAX_CAMERAS);
   if (!rc && hipHostMalloc((void **)&c->h_cams, sizeof(DevCam) * VT_MAX_CAMERAS, hipHostMallocDefault) != hipSuccess) rc = VT_ERR_HIP;
-  hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1, &c->ev_b0, &c->ev_b1};
+  hipEvent_t *evs[] = {&c->ev_stage, &c->ev_r0, &c->ev_r1, &c->ev_s0, &c->ev_s1, &c->ev_b0, &c->ev_b1, &c->ev_i0,
+                       &c->ev_i1};
   for (hipEvent_t *ev : evs)
     if (!rc && hipEventCreate(ev) != hipSuccess) rc = VT_ERR_HIP;
   if (!rc && hipMemsetAsync(c->overflow.p, 0, 64, c->stream) != hipSuccess) rc = VT_ERR_HIP;
@@ -315,11 +320,13 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.ri);
     release(cs.lin);
     release(cs.adj);
+    DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.gpl, &cs.isp_a, &cs.isp_b, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8};
+    for (DevBuf *b : isp) release(*b);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : c->ev_ext) (void)hipEventDestroy(ev);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1, c->ev_b0, c->ev_b1};
+  hipEvent_t evs[] = {c->ev_stage, c->ev_r0, c->ev_r1, c->ev_s0, c->ev_s1, c->ev_b0, c->ev_b1, c->ev_i0, c->ev_i1};
   for (hipEvent_t ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   if (c->h_cams) (void)hipHostFree(c->h_cams);
@@ -1296,7 +1303,50 @@ static int sensor_params(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint3
   P.read_noise = p->read_noise; P.full_well = p->full_well; P.dark = p->dark; P.conv_gain = p->conv_gain;
   P.max_dn = p->max_dn; P.black = p->black; P.noise_on = p->noise_on ? 1u : 0u;
   P.inv_spp = inv_spp;
+  P.glare_k = p->glare_fraction;
+  P.glare_sigma = p->glare_sigma;
+  P.glare_f = p->glare_level;
+  if (!(p->glare_fraction >= 0.0f && p->glare_fraction <= 1.0f))
+    return fail(c, VT_ERR_ARG, "sensor: glare fraction outside [0, 1]");
+  if (p->glare_fraction > 0.0f && (p->glare_level < 1 || p->glare_level > 64))
+    return fail(c, VT_ERR_ARG, "sensor: glare level factor %u outside 1..64", p->glare_level);
   return VT_OK;
+}
+static size_t glare_bytes(uint32_t w, uint32_t h, const SensorDev &P) {
+  if (!(P.glare_k > 0.0f)) return 0;
+  const size_t gw = (w + P.glare_f - 1) / P.glare_f, gh = (h + P.glare_f - 1) / P.glare_f;
+  return gw * gh * sizeof(float4);
+}
+
+// the ISP's uniform words from the raw stage's parameters and the ISP parameters (gpu/sensor.js:169-176)
+static int isp_params(vt_ctx *c, const vt_sensor_params &sp, const vt_isp_params *q, uint32_t w, uint32_t h,
+                      IspDev &P) {
+  if (!q) return fail(c, VT_ERR_ARG, "isp: null parameters");
+  if (q->demosaic > 2) return fail(c, VT_ERR_ARG, "isp: demosaic mode %u", q->demosaic);
+  if (q->tone > 3) return fail(c, VT_ERR_ARG, "isp: tone mode %u", q->tone);
+  if (!(q->sharpen_amount >= 0.0f)) return fail(c, VT_ERR_ARG, "isp: negative sharpening amount");
+  P.w = w; P.h = h; P.cfa = sp.cfa; P.demosaic = q->demosaic;
+  P.nr = (q->nr && q->nr_sigma_px > 0.0f) ? 1u : 0u;
+  P.tone = q->tone; P.chroma420 = q->chroma420 ? 1u : 0u;
+  P.max_dn = sp.max_dn; P.black = sp.black; P.read_noise = sp.read_noise; P.conv_gain = sp.conv_gain;
+  P.gain = sp.gain;
+  std::memcpy(P.wb, q->wb, sizeof P.wb);
+  std::memcpy(P.ccm, q->ccm, sizeof P.ccm);
+  P.nr_sigma = q->nr_sigma_px; P.nr_scale = q->nr_range_scale; P.gamma = q->gamma; P.s_c = q->s_curve_c;
+  P.black_off = q->black_offset; P.sharpen_amount = q->sharpen_amount;
+  const float rr = std::nearbyint(q->sharpen_radius);
+  P.sharpen_rad = rr != rr ? 1 : (int)std::min(4.0f, std::max(1.0f, rr));
+  return VT_OK;
+}
+static int ensure_isp(vt_ctx *c, CamSlot &s, uint32_t w, uint32_t h) {
+  const size_t px = (size_t)w * h, cpx = (size_t)((w + 1) / 2) * ((h + 1) / 2);
+  int rc = ensure(c, s.gpl, px * 4);
+  if (!rc) rc = ensure(c, s.isp_a, px * 16);
+  if (!rc) rc = ensure(c, s.isp_b, px * 16);
+  if (!rc) rc = ensure(c, s.rgba, px * 4);
+  if (!rc) rc = ensure(c, s.yuv, px + 2 * cpx);
+  if (!rc) rc = ensure(c, s.stats, 16);
+  return rc;
 }
 
 extern "C" int vt_sensor(vt_ctx *c, uint32_t cam, const vt_sensor_params *p) {
@@ -1316,11 +1366,37 @@ extern "C" int vt_sensor(vt_ctx *c, uint32_t cam, const vt_sensor_params *p) {
     s.ri_valid = true;
     s.ri_pupil = p->pupil_exp;
   }
+  if (const size_t gb = glare_bytes(s.c.width, s.c.height, P)) {
+    if ((rc = ensure(c, s.glare_a, gb)) || (rc = ensure(c, s.glare_b, gb))) return rc;
+  }
   VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
-  VT_HIP(c, launch_sensor(P, (const float4 *)s.acc.p, (const float *)s.ri.p, (uint16_t *)s.dn.p, c->stream));
+  VT_HIP(c, launch_sensor(P, (const float4 *)s.acc.p, (const float *)s.ri.p, (float4 *)s.glare_a.p,
+                          (float4 *)s.glare_b.p, (uint16_t *)s.dn.p, c->stream));
   VT_HIP(c, hipEventRecord(c->ev_s1, c->stream));
   c->sensor_timed = true;
   s.sensed = true;
+  s.isp_done = false;
+  s.sp = *p;
+  return VT_OK;
+}
+
+extern "C" int vt_isp(vt_ctx *c, uint32_t cam, const vt_isp_params *q) {
+  VT_LIVE(c);
+  if (cam >= VT_MAX_CAMERAS || !c->cams[cam].set) return fail(c, VT_ERR_STATE, "isp: camera %u not set", cam);
+  CamSlot &s = c->cams[cam];
+  if (!s.sensed) return fail(c, VT_ERR_STATE, "isp: camera %u has no raw DN (vt_sensor first)", cam);
+  VT_HIP(c, hipSetDevice(c->device));
+  IspDev P;
+  int rc = isp_params(c, s.sp, q, s.c.width, s.c.height, P);
+  if (!rc) rc = ensure_isp(c, s, s.c.width, s.c.height);
+  if (rc) return rc;
+  VT_HIP(c, hipEventRecord(c->ev_i0, c->stream));
+  VT_HIP(c, launch_isp(P, (const uint16_t *)s.dn.p, (float *)s.gpl.p, (float4 *)s.isp_a.p, (float4 *)s.isp_b.p,
+                       (uchar4 *)s.rgba.p, (uint8_t *)s.yuv.p, (unsigned int *)s.stats.p, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_i1, c->stream));
+  c->isp_timed = true;
+  s.isp_done = true;
+  s.isp420 = P.chroma420 != 0;
   return VT_OK;
 }
 
@@ -1346,6 +1422,24 @@ extern "C" int vt_readback(vt_ctx *c, uint32_t cam, int kind, void *dst, size_t 
     if (bytes != px * 4) return fail(c, VT_ERR_SIZE, "readback ri: %zu bytes, want %zu", bytes, px * 4);
     if (!s.ri_valid) return fail(c, VT_ERR_STATE, "readback ri: no sensor run yet");
     VT_HIP(c, hipMemcpyAsync(dst, s.ri.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  } else if (kind == VT_KIND_RGB8 || kind == VT_KIND_RGBA8 || kind == VT_KIND_YUV420 || kind == VT_KIND_STATS) {
+    if (!s.isp_done) return fail(c, VT_ERR_STATE, "readback: camera %u has no ISP output (vt_isp first)", cam);
+    const size_t cpx = (size_t)((s.c.width + 1) / 2) * ((s.c.height + 1) / 2);
+    const size_t want = kind == VT_KIND_RGB8 ? px * 3 : kind == VT_KIND_RGBA8 ? px * 4
+                        : kind == VT_KIND_YUV420 ? px + 2 * cpx : 16;
+    if (bytes != want) return fail(c, VT_ERR_SIZE, "readback kind %d: %zu bytes, want %zu", kind, bytes, want);
+    if (kind == VT_KIND_RGB8) {
+      int rc = ensure(c, s.rgb8, px * 3);
+      if (rc) return rc;
+      VT_HIP(c, launch_pack_rgb8((const uchar4 *)s.rgba.p, px, (uint8_t *)s.rgb8.p, c->stream));
+      VT_HIP(c, hipMemcpyAsync(dst, s.rgb8.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    } else if (kind == VT_KIND_YUV420) {
+      if (!s.isp420) return fail(c, VT_ERR_STATE, "readback yuv420: the last ISP pass ran 4:4:4");
+      VT_HIP(c, hipMemcpyAsync(dst, s.yuv.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      VT_HIP(c, hipMemcpyAsync(dst, kind == VT_KIND_RGBA8 ? s.rgba.p : s.stats.p, bytes, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
   } else {
     return fail(c, VT_ERR_ARG, "readback: unknown kind %d", kind);
   }
@@ -1404,6 +1498,11 @@ extern "C" int vt_frame_stats_get(vt_ctx *c, vt_frame_stats *out) {
     float ms = 0;
     VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
     out->ms_sensor = ms;
+  }
+  if (c->isp_timed) {
+    float ms = 0;
+    VT_HIP(c, hipEventElapsedTime(&ms, c->ev_i0, c->ev_i1));
+    out->ms_isp = ms;
   }
   if (c->backward_timed) {
     float ms = 0;
@@ -1675,15 +1774,46 @@ extern "C" int vt_sensor_test(vt_ctx *c, const vt_sensor_params *p, uint32_t w, 
   if (rc) return rc;
   P.cam_index = 0;
   const size_t px = (size_t)w * h;
+  // test_aux: the RI map, then the two glare levels (float4-aligned)
+  const size_t gb = glare_bytes(w, h, P), ga = (px * 4 + 255) / 256 * 256;
   if ((rc = ensure(c, c->test_in, px * 16))) return rc;
-  if ((rc = ensure(c, c->test_aux, px * 4))) return rc;
+  if ((rc = ensure(c, c->test_aux, ga + 2 * gb))) return rc;
   if ((rc = ensure(c, c->test_out, px * 2))) return rc;
   VT_HIP(c, hipMemcpy(c->test_in.p, radiance, px * 16, hipMemcpyHostToDevice));
   VT_HIP(c, hipMemcpy(c->test_aux.p, ri, px * 4, hipMemcpyHostToDevice));
-  VT_HIP(c, launch_sensor(P, (const float4 *)c->test_in.p, (const float *)c->test_aux.p, (uint16_t *)c->test_out.p,
-                          c->stream));
+  char *g = (char *)c->test_aux.p + ga;
+  VT_HIP(c, launch_sensor(P, (const float4 *)c->test_in.p, (const float *)c->test_aux.p, (float4 *)g,
+                          (float4 *)(g + gb), (uint16_t *)c->test_out.p, c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(dn, c->test_out.p, px * 2, hipMemcpyDeviceToHost));
+  return VT_OK;
+}
+
+extern "C" int vt_isp_test(vt_ctx *c, const vt_sensor_params *sp, const vt_isp_params *q, uint32_t w, uint32_t h,
+                           const uint16_t *dn, uint8_t *rgba, uint8_t *yuv, uint32_t *stats) {
+  VT_LIVE(c);
+  if (!sp || !dn || !rgba || !stats || w == 0 || h == 0) return fail(c, VT_ERR_ARG, "isp_test: null argument");
+  if (sp->cfa > 3) return fail(c, VT_ERR_ARG, "isp_test: cfa code %u", sp->cfa);
+  VT_HIP(c, hipSetDevice(c->device));
+  IspDev P;
+  int rc = isp_params(c, *sp, q, w, h, P);
+  if (rc) return rc;
+  const size_t px = (size_t)w * h, cpx = (size_t)((w + 1) / 2) * ((h + 1) / 2);
+  // test_in: two float4 planes; test_aux: green plane, then rgba, yuv, stats; test_out: the DN
+  if ((rc = ensure(c, c->test_in, px * 32))) return rc;
+  const size_t o_rgba = (px * 4 + 255) / 256 * 256, o_yuv = o_rgba + (px * 4 + 255) / 256 * 256,
+               o_st = o_yuv + (px + 2 * cpx + 255) / 256 * 256;
+  if ((rc = ensure(c, c->test_aux, o_st + 16))) return rc;
+  if ((rc = ensure(c, c->test_out, px * 2))) return rc;
+  char *aux = (char *)c->test_aux.p;
+  VT_HIP(c, hipMemcpy(c->test_out.p, dn, px * 2, hipMemcpyHostToDevice));
+  VT_HIP(c, launch_isp(P, (const uint16_t *)c->test_out.p, (float *)aux, (float4 *)c->test_in.p,
+                       (float4 *)c->test_in.p + px, (uchar4 *)(aux + o_rgba), (uint8_t *)(aux + o_yuv),
+                       (unsigned int *)(aux + o_st), c->stream));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  VT_HIP(c, hipMemcpy(rgba, aux + o_rgba, px * 4, hipMemcpyDeviceToHost));
+  if (yuv && P.chroma420) VT_HIP(c, hipMemcpy(yuv, aux + o_yuv, px + 2 * cpx, hipMemcpyDeviceToHost));
+  VT_HIP(c, hipMemcpy(stats, aux + o_st, 16, hipMemcpyDeviceToHost));
   return VT_OK;
 }
 

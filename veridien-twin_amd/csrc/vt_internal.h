@@ -123,9 +123,25 @@ struct SensorDev {
   float read_noise, full_well, dark, conv_gain, max_dn, black;
   uint32_t noise_on;
   float inv_spp;
+  float glare_k, glare_sigma;   // veiling glare: fraction (0 = off), sigma (level px)
+  uint32_t glare_f;             // level factor F
 };
-hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, uint16_t *dn, hipStream_t s);
+// glare: two level buffers of ceil(w/F) x ceil(h/F) float4 (unused when glare_k == 0)
+hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, float4 *glare_a, float4 *glare_b,
+                         uint16_t *dn, hipStream_t s);
+hipError_t launch_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb, hipStream_t s);
 hipError_t launch_finalize_linear(const float4 *acc, float scale_over_spp, uint64_t n, float4 *out, hipStream_t s);
+// the ISP after the raw DN
+struct IspDev {
+  uint32_t w, h, cfa, demosaic, nr, tone, chroma420;
+  float max_dn, black, read_noise, conv_gain, gain;
+  float wb[3], ccm[9];
+  float nr_sigma, nr_scale, gamma, s_c, black_off, sharpen_amount;
+  int sharpen_rad;
+};
+// stats: [0] luma sum, [1..3] grey-world sums (x 1023); yuv: Y plane w*h, then Cb and Cr ((w+1)/2 x (h+1)/2)
+hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float *g, float4 *rgb, float4 *tmp, uchar4 *rgba,
+                      uint8_t *yuv, unsigned int *stats, hipStream_t s);
 
 // node radiosity
 struct GatherDev {

@@ -142,7 +142,70 @@ __device__ __forceinline__ int cfa_channel(uint32_t x, uint32_t y, uint32_t code
 
 constexpr int TW = 32, TH = 8, RMAX = 12;
 
-__global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *acc, const float *ri, uint16_t *dn) {
+// ------------------------------------------------------------------ veiling glare on the 1/F level
+// (sensor.wgsl.js:87-141): F x F block means of the irradiance, then a Gaussian along each row and each column of the
+// level (zero outside the sensor, normalised on the infinite line), read back bilinearly in the exposure
+__global__ void k_glare_down(SensorDev P, const float4 *acc, const float *ri, float4 *out) {
+  const int W = (int)P.w, H = (int)P.h, F = (int)P.glare_f;
+  const int gw = (W + F - 1) / F, gh = (H + F - 1) / F;
+  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
+  if (x >= gw || y >= gh) return;
+  const int fl = F < 32 ? F : 32;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, n = 0.0f;
+  for (int j = 0; j < fl; j++)
+    for (int i = 0; i < fl; i++) {
+      const int xx = x * F + i, yy = y * F + j;
+      if (xx >= W || yy >= H) continue;
+      const float4 s = acc[(size_t)yy * W + xx];
+      const float q = ri[(size_t)yy * W + xx];
+      a0 = a0 + (s.x * P.inv_spp) * q;
+      a1 = a1 + (s.y * P.inv_spp) * q;
+      a2 = a2 + (s.z * P.inv_spp) * q;
+      n = n + 1.0f;
+    }
+  const float sc = P.optics_k / fmaxf(n, 1.0f);
+  out[(size_t)y * gw + x] = make_float4(a0 * sc, a1 * sc, a2 * sc, 1.0f);
+}
+template <int DIR>
+__global__ void k_glare_blur(SensorDev P, const float4 *in, float4 *out) {
+  const int W = (int)P.w, H = (int)P.h, F = (int)P.glare_f;
+  const int gw = (W + F - 1) / F, gh = (H + F - 1) / F;
+  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
+  if (x >= gw || y >= gh) return;
+  const float sg = fmaxf(P.glare_sigma, 1e-3f);
+  const int n = DIR == 0 ? gw : gh, lim = n < 256 ? n : 256, at = DIR == 0 ? x : y;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (int k = 0; k < lim; k++) {
+    const float d = (float)(k - at);
+    const float w = vt_expf(-0.5f * d * d / (sg * sg));
+    const float4 c = in[DIR == 0 ? (size_t)y * gw + k : (size_t)k * gw + x];
+    a0 = a0 + w * c.x;
+    a1 = a1 + w * c.y;
+    a2 = a2 + w * c.z;
+  }
+  const float nrm = sg * 2.5066282746f;
+  out[(size_t)y * gw + x] = make_float4(a0 / nrm, a1 / nrm, a2 / nrm, 1.0f);
+}
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ void glare_at(const SensorDev &P, const float4 *G, int x, int y, float &g0, float &g1, float &g2) {
+  const int W = (int)P.w, H = (int)P.h, Fi = (int)P.glare_f;
+  const int gw = (W + Fi - 1) / Fi, gh = (H + Fi - 1) / Fi;
+  const float F = (float)P.glare_f;
+  const float px = ((float)x + 0.5f) / F - 0.5f, py = ((float)y + 0.5f) / F - 0.5f;
+  const float fx0 = floorf(px), fy0 = floorf(py);
+  const int ix = (int)fx0, iy = (int)fy0;
+  const float fx = px - fx0, fy = py - fy0;
+  const int x0 = min(max(ix, 0), gw - 1), x1 = min(max(ix + 1, 0), gw - 1);
+  const int y0 = min(max(iy, 0), gh - 1), y1 = min(max(iy + 1, 0), gh - 1);
+  const float4 a = G[(size_t)y0 * gw + x0], b = G[(size_t)y0 * gw + x1], c = G[(size_t)y1 * gw + x0],
+               d = G[(size_t)y1 * gw + x1];
+  g0 = mixf(mixf(a.x, b.x, fx), mixf(c.x, d.x, fx), fy);
+  g1 = mixf(mixf(a.y, b.y, fx), mixf(c.y, d.y, fx), fy);
+  g2 = mixf(mixf(a.z, b.z, fx), mixf(c.z, d.z, fx), fy);
+}
+
+__global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *acc, const float *ri,
+                                                  const float4 *glare, uint16_t *dn) {
   __shared__ float sA[3][(TH + 2 * RMAX) * TW];
   __shared__ float sw[2 * RMAX + 1];
   const int W = (int)P.w, H = (int)P.h;
@@ -182,7 +245,7 @@ __global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *ac
   __syncthreads();
   const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
   if (x >= W || y >= H) return;
-  // vertical pass + exposure (sensorExpose, glare off)
+  // vertical pass + exposure (sensorExpose)
   float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
   for (int k = -r; k <= r; k++) {
     const int j = (int)threadIdx.y + k + r;
@@ -192,6 +255,14 @@ __global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *ac
     e2 = e2 + w * sA[2][j * TW + threadIdx.x];
   }
   e0 = e0 / wsum; e1 = e1 / wsum; e2 = e2 / wsum;
+  if (P.glare_k > 0.0f) {
+    float g0, g1, g2;
+    glare_at(P, glare, x, y, g0, g1, g2);
+    const float k = P.glare_k;
+    e0 = (1.0f - k) * e0 + k * g0;
+    e1 = (1.0f - k) * e1 + k * g1;
+    e2 = (1.0f - k) * e2 + k * g2;
+  }
   const int ch = cfa_channel((uint32_t)x, (uint32_t)y, P.cfa);
   const float *row = P.m + 3 * ch;
   const float dt = (row[0] * e0 + row[1] * e1) + row[2] * e2;
@@ -206,6 +277,247 @@ __global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *ac
   dn[(size_t)y * W + x] = (uint16_t)v;
 }
 
+// ------------------------------------------------------------------ the ISP after the raw DN
+// (sensor.wgsl.js:210-427). 8-bit outputs equal the host model's bit for bit: the same unfused binary32 order, the
+// library's own exp / log (pow = exp(y log x)).
+__device__ __forceinline__ float isp_pow(float x, float y) { return x > 0.0f ? vt_expf(y * vt_logf(x)) : 0.0f; }
+
+// raw DN mirrored at the borders with the CFA phase kept (reflect by 2) (rawAt :216-223)
+__device__ __forceinline__ float raw_at(const IspDev &P, const uint16_t *dn, int x, int y) {
+  const int W = (int)P.w, H = (int)P.h;
+  int xx = x, yy = y;
+  if (xx < 0) xx = -xx;
+  if (xx >= W) xx = 2 * (W - 1) - xx;
+  if (yy < 0) yy = -yy;
+  if (yy >= H) yy = 2 * (H - 1) - yy;
+  xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
+  yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+  return (float)dn[(size_t)yy * W + xx];
+}
+__device__ __forceinline__ float g_at(const IspDev &P, const float *g, int x, int y) {
+  const int W = (int)P.w, H = (int)P.h;
+  x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+  y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+  return g[(size_t)y * W + x];
+}
+
+// green plane: the sample at G sites; elsewhere the 4-neighbour mean (bilinear), the mean along the smaller green
+// gradient (edge-aware), or Hamilton-Adams with the site's own second derivative (demosaicG :229-256)
+__global__ void k_isp_green(IspDev P, const uint16_t *dn, float *g) {
+  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
+  if (x >= (int)P.w || y >= (int)P.h) return;
+  float v = raw_at(P, dn, x, y);
+  if (cfa_channel((uint32_t)x, (uint32_t)y, P.cfa) != 1) {
+    const float gl = raw_at(P, dn, x - 1, y), gr = raw_at(P, dn, x + 1, y), gu = raw_at(P, dn, x, y - 1),
+                gd = raw_at(P, dn, x, y + 1);
+    if (P.demosaic == 0) {
+      v = 0.25f * (((gl + gr) + gu) + gd);
+    } else if (P.demosaic == 1) {
+      const float dh = fabsf(gl - gr), dv = fabsf(gu - gd);
+      v = dh < dv ? 0.5f * (gl + gr) : (dv < dh ? 0.5f * (gu + gd) : 0.25f * (((gl + gr) + gu) + gd));
+    } else {
+      const float c = v;
+      const float lh = (2.0f * c - raw_at(P, dn, x - 2, y)) - raw_at(P, dn, x + 2, y);
+      const float lv = (2.0f * c - raw_at(P, dn, x, y - 2)) - raw_at(P, dn, x, y + 2);
+      const float dh = fabsf(gl - gr) + fabsf(lh), dv = fabsf(gu - gd) + fabsf(lv);
+      const float gh = 0.5f * (gl + gr) + 0.25f * lh, gv = 0.5f * (gu + gd) + 0.25f * lv;
+      v = dh < dv ? gh : (dv < dh ? gv : 0.5f * (gh + gv));
+      v = v < 0.0f ? 0.0f : (v > P.max_dn ? P.max_dn : v);
+    }
+  }
+  g[(size_t)y * P.w + x] = v;
+}
+
+// red and blue (colour differences over the nearest sites; bilinear modes: the sites' plain means), normalised,
+// the grey-world sums (x 1023, integers), white balance and CCM -> linear camera RGB (demosaicRB :263-319)
+__global__ void k_isp_rb(IspDev P, const uint16_t *dn, const float *g, float4 *rgb, unsigned int *stats) {
+  __shared__ unsigned int sr[64], sg[64], sb[64];
+  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
+  const int li = threadIdx.y * 8 + threadIdx.x;
+  const int W = (int)P.w, H = (int)P.h;
+  sr[li] = 0u;
+  sg[li] = 0u;
+  sb[li] = 0u;
+  if (x < W && y < H) {
+    const int own = cfa_channel((uint32_t)x, (uint32_t)y, P.cfa);
+    const float gv = g_at(P, g, x, y);
+    float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f;
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int xx = min(max(x + dx, 0), W - 1), yy = min(max(y + dy, 0), H - 1);
+        const int ch = cfa_channel((uint32_t)xx, (uint32_t)yy, P.cfa);
+        if (ch == 1) continue;
+        if (ch == own && !(dx == 0 && dy == 0)) continue;
+        const float d = raw_at(P, dn, xx, yy) - g_at(P, g, xx, yy);
+        if (ch == 0) { sx += d; cx += 1.0f; } else { sy += d; cy += 1.0f; }
+      }
+    float r = gv + sx / fmaxf(cx, 1.0f), gg = gv, b = gv + sy / fmaxf(cy, 1.0f);
+    if (P.demosaic <= 1) {
+      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int xx = min(max(x + dx, 0), W - 1), yy = min(max(y + dy, 0), H - 1);
+          const int ch = cfa_channel((uint32_t)xx, (uint32_t)yy, P.cfa);
+          if (ch == own && !(dx == 0 && dy == 0)) continue;
+          if (ch == 1 && own == 1 && !(dx == 0 && dy == 0)) continue;
+          const float v = raw_at(P, dn, xx, yy);
+          if (ch == 0) { s0 += v; c0 += 1.0f; } else if (ch == 1) { s1 += v; c1 += 1.0f; } else { s2 += v; c2 += 1.0f; }
+        }
+      r = s0 / fmaxf(c0, 1.0f);
+      gg = s1 / fmaxf(c1, 1.0f);
+      b = s2 / fmaxf(c2, 1.0f);
+      if (P.demosaic == 1) gg = gv;
+    }
+    const float full = fmaxf(P.max_dn - P.black, 1.0f);
+    float cr = (r - P.black) / full, cg = (gg - P.black) / full, cb = (b - P.black) / full;
+    sr[li] = (unsigned int)rintf(fminf(fmaxf(cr, 0.0f), 1.0f) * 1023.0f);
+    sg[li] = (unsigned int)rintf(fminf(fmaxf(cg, 0.0f), 1.0f) * 1023.0f);
+    sb[li] = (unsigned int)rintf(fminf(fmaxf(cb, 0.0f), 1.0f) * 1023.0f);
+    cr = cr * P.wb[0];
+    cg = cg * P.wb[1];
+    cb = cb * P.wb[2];
+    const float o0 = (P.ccm[0] * cr + P.ccm[1] * cg) + P.ccm[2] * cb;
+    const float o1 = (P.ccm[3] * cr + P.ccm[4] * cg) + P.ccm[5] * cb;
+    const float o2 = (P.ccm[6] * cr + P.ccm[7] * cg) + P.ccm[8] * cb;
+    rgb[(size_t)y * W + x] = make_float4(o0, o1, o2, 1.0f);
+  }
+  __syncthreads();
+  if (li == 0) {
+    unsigned int a = 0, bb = 0, c = 0;
+    for (int k = 0; k < 64; k++) { a += sr[k]; bb += sg[k]; c += sb[k]; }
+    atomicAdd(&stats[1], a);
+    atomicAdd(&stats[2], bb);
+    atomicAdd(&stats[3], c);
+  }
+}
+
+// 3x3 bilateral on linear RGB, range on luma with sigma = scale x the modelled noise at the pixel (sensorNR
+// :321-348)
+__global__ void k_isp_nr(IspDev P, const float4 *src, float4 *dst) {
+  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
+  const int W = (int)P.w, H = (int)P.h;
+  if (x >= W || y >= H) return;
+  const float4 c0 = src[(size_t)y * W + x];
+  const float Y0 = fmaxf((c0.x * 0.2126f + c0.y * 0.7152f) + c0.z * 0.0722f, 0.0f);
+  const float fullE = fmaxf(P.max_dn - P.black, 1.0f) / fmaxf(P.conv_gain * P.gain, 1e-9f);
+  const float sn = sqrtf(Y0 * fullE + P.read_noise * P.read_noise) / fullE;
+  const float rs = fmaxf(P.nr_scale * sn, 1e-6f);
+  const float ss = P.nr_sigma;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, ws = 0.0f;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const float4 c = src[(size_t)min(max(y + dy, 0), H - 1) * W + min(max(x + dx, 0), W - 1)];
+      const float dY = ((c.x - c0.x) * 0.2126f + (c.y - c0.y) * 0.7152f) + (c.z - c0.z) * 0.0722f;
+      const float w = vt_expf(-0.5f * (float)(dx * dx + dy * dy) / (ss * ss) - 0.5f * dY * dY / (rs * rs));
+      ax += w * c.x;
+      ay += w * c.y;
+      az += w * c.z;
+      ws += w;
+    }
+  dst[(size_t)y * W + x] = make_float4(ax / ws, ay / ws, az / ws, 1.0f);
+}
+
+// tone curves (toneC :350-366): sRGB OETF, gamma, linear, or sRGB + a contrast S-curve with a black offset
+__device__ float tone_c(const IspDev &P, float x) {
+  if (P.tone == 3) {
+    const float b = P.black_off;
+    float v = (x - b) / (1.0f - b);
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    const float u = v <= 0.0031308f ? 12.92f * v : 1.055f * isp_pow(v, 1.0f / 2.4f) - 0.055f;
+    const float a = isp_pow(u, P.s_c);
+    return a / fmaxf(a + isp_pow(1.0f - u, P.s_c), 1e-12f);
+  }
+  const float c = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+  if (P.tone == 0) return c <= 0.0031308f ? 12.92f * c : 1.055f * isp_pow(c, 1.0f / 2.4f) - 0.055f;
+  if (P.tone == 1) return isp_pow(c, 1.0f / fmaxf(P.gamma, 1e-3f));
+  return c;
+}
+__device__ __forceinline__ void toned(const IspDev &P, const float4 *src, int x, int y, float &r, float &g, float &b) {
+  const int W = (int)P.w, H = (int)P.h;
+  const float4 c = src[(size_t)min(max(y, 0), H - 1) * W + min(max(x, 0), W - 1)];
+  r = tone_c(P, c.x);
+  g = tone_c(P, c.y);
+  b = tone_c(P, c.z);
+}
+
+// one lane per 2x2 block: tone, optional unsharp mask on luma, BT.601 full-range YCbCr, 4:2:0 chroma (2x2 mean) and
+// RGB from it (or 4:4:4 RGB), Y / Cb / Cr planes, the luma sum for auto exposure (sensorOutput :374-427)
+__global__ void k_isp_out(IspDev P, const float4 *src, uchar4 *rgba, uint8_t *yuv, unsigned int *stats) {
+  __shared__ unsigned int ws[64];
+  const int bx = (blockIdx.x * 8 + threadIdx.x) * 2, by = (blockIdx.y * 8 + threadIdx.y) * 2;
+  const int li = threadIdx.y * 8 + threadIdx.x;
+  const int W = (int)P.w, H = (int)P.h;
+  unsigned int lsum = 0;
+  if (bx < W && by < H) {
+    float Ys[4];
+    float cb = 0.0f, cr = 0.0f, nb = 0.0f;
+    const float amt = P.sharpen_amount;
+    const int rad = P.sharpen_rad;
+    for (int q = 0; q < 4; q++) {
+      const int x = bx + (q & 1), y = by + (q >> 1);
+      if (x >= W || y >= H) { Ys[q] = 0.0f; continue; }
+      float r, g, b;
+      toned(P, src, x, y, r, g, b);
+      if (amt > 0.0f) {
+        float m = 0.0f, n = 0.0f;
+        for (int dy = -rad; dy <= rad; dy++)
+          for (int dx = -rad; dx <= rad; dx++) {
+            float tr, tg, tb;
+            toned(P, src, x + dx, y + dy, tr, tg, tb);
+            m += (tr * 0.299f + tg * 0.587f) + tb * 0.114f;
+            n += 1.0f;
+          }
+        const float Y0 = (r * 0.299f + g * 0.587f) + b * 0.114f;
+        const float d = amt * (Y0 - m / n);
+        r += d; g += d; b += d;
+      }
+      const float r8 = fminf(fmaxf(r, 0.0f), 1.0f) * 255.0f, g8 = fminf(fmaxf(g, 0.0f), 1.0f) * 255.0f,
+                  b8 = fminf(fmaxf(b, 0.0f), 1.0f) * 255.0f;
+      Ys[q] = (0.299f * r8 + 0.587f * g8) + 0.114f * b8;
+      cb += ((128.0f - 0.168736f * r8) - 0.331264f * g8) + 0.5f * b8;
+      cr += ((128.0f + 0.5f * r8) - 0.418688f * g8) - 0.081312f * b8;
+      nb += 1.0f;
+      if (!P.chroma420) {
+        const float y8 = fminf(fmaxf(rintf(Ys[q]), 0.0f), 255.0f);
+        rgba[(size_t)y * W + x] = make_uchar4((uint8_t)rintf(r8), (uint8_t)rintf(g8), (uint8_t)rintf(b8), 255);
+        lsum += (unsigned int)y8;
+      }
+    }
+    if (P.chroma420) {
+      const float Cb = fminf(fmaxf(rintf(cb / nb), 0.0f), 255.0f), Cr = fminf(fmaxf(rintf(cr / nb), 0.0f), 255.0f);
+      const int cw = (W + 1) / 2, ch = (H + 1) / 2;
+      const size_t bi = (size_t)(by / 2) * cw + (bx / 2);
+      for (int q = 0; q < 4; q++) {
+        const int x = bx + (q & 1), y = by + (q >> 1);
+        if (x >= W || y >= H) continue;
+        const float y8 = fminf(fmaxf(rintf(Ys[q]), 0.0f), 255.0f);
+        const float rr = y8 + 1.402f * (Cr - 128.0f);
+        const float gg = (y8 - 0.344136f * (Cb - 128.0f)) - 0.714136f * (Cr - 128.0f);
+        const float bb = y8 + 1.772f * (Cb - 128.0f);
+        rgba[(size_t)y * W + x] = make_uchar4((uint8_t)fminf(fmaxf(rintf(rr), 0.0f), 255.0f),
+                                              (uint8_t)fminf(fmaxf(rintf(gg), 0.0f), 255.0f),
+                                              (uint8_t)fminf(fmaxf(rintf(bb), 0.0f), 255.0f), 255);
+        yuv[(size_t)y * W + x] = (uint8_t)y8;
+        lsum += (unsigned int)y8;
+      }
+      yuv[(size_t)W * H + bi] = (uint8_t)Cb;
+      yuv[(size_t)W * H + (size_t)cw * ch + bi] = (uint8_t)Cr;
+    }
+  }
+  ws[li] = lsum;
+  __syncthreads();
+  if (li == 0) {
+    unsigned int t = 0;
+    for (int k = 0; k < 64; k++) t += ws[k];
+    atomicAdd(&stats[0], t);
+  }
+}
+
+__global__ void k_zero_u32(unsigned int *p, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
+}
+
 __global__ void k_finalize(const float4 *acc, float s, uint64_t n, float4 *out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -215,11 +527,48 @@ __global__ void k_finalize(const float4 *acc, float s, uint64_t n, float4 *out) 
 
 }  // namespace
 
-hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, uint16_t *dn, hipStream_t s) {
+hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, float4 *glare_a, float4 *glare_b,
+                         uint16_t *dn, hipStream_t s) {
+  if (P.glare_k > 0.0f) {
+    const uint32_t F = P.glare_f, gw = (P.w + F - 1) / F, gh = (P.h + F - 1) / F;
+    const dim3 g((gw + 7) / 8, (gh + 7) / 8), b(8, 8);
+    hipLaunchKernelGGL(k_glare_down, g, b, 0, s, P, acc, ri, glare_a);
+    hipLaunchKernelGGL(k_glare_blur<0>, g, b, 0, s, P, (const float4 *)glare_a, glare_b);
+    hipLaunchKernelGGL(k_glare_blur<1>, g, b, 0, s, P, (const float4 *)glare_b, glare_a);
+  }
   dim3 grid((P.w + TW - 1) / TW, (P.h + TH - 1) / TH);
-  hipLaunchKernelGGL(k_sensor, grid, dim3(TW, TH), 0, s, P, acc, ri, dn);
+  hipLaunchKernelGGL(k_sensor, grid, dim3(TW, TH), 0, s, P, acc, ri, (const float4 *)glare_a, dn);
   return hipGetLastError();
 }
+__global__ void k_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uchar4 c = rgba[i];
+  rgb[3 * i] = c.x;
+  rgb[3 * i + 1] = c.y;
+  rgb[3 * i + 2] = c.z;
+}
+hipError_t launch_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_rgb8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rgba, n, rgb);
+  return hipGetLastError();
+}
+hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float *g, float4 *rgb, float4 *tmp, uchar4 *rgba,
+                      uint8_t *yuv, unsigned int *stats, hipStream_t s) {
+  if (P.w == 0 || P.h == 0) return hipSuccess;
+  const dim3 blk(8, 8), grid((P.w + 7) / 8, (P.h + 7) / 8), grid2((P.w + 15) / 16, (P.h + 15) / 16);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, stats, 4u);
+  hipLaunchKernelGGL(k_isp_green, grid, blk, 0, s, P, dn, g);
+  hipLaunchKernelGGL(k_isp_rb, grid, blk, 0, s, P, dn, g, rgb, stats);
+  const float4 *img = rgb;
+  if (P.nr) {
+    hipLaunchKernelGGL(k_isp_nr, grid, blk, 0, s, P, rgb, tmp);
+    img = tmp;
+  }
+  hipLaunchKernelGGL(k_isp_out, grid2, blk, 0, s, P, img, rgba, yuv, stats);
+  return hipGetLastError();
+}
+
 hipError_t launch_finalize_linear(const float4 *acc, float scale_over_spp, uint64_t n, float4 *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc, scale_over_spp, n, out);

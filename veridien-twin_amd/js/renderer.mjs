@@ -3,8 +3,10 @@
 // timings / idle / destroy), backed by the N-API addon napi/vt_napi.node and libvt.so's HIP kernels.
 // There is no WebGPU device: `device` is a HIP device ordinal, and renderCamera returns an opaque handle where
 // the reference returned a GPUTexture (gpu/renderer.js:855-866). readback returns fresh typed arrays as the
-// reference does (gpu/renderer.js:1191-1231): 'linear' Float32Array w*h*4, 'raw' Float32Array DN.
+// reference does (gpu/renderer.js:1191-1231): 'linear' Float32Array w*h*4, 'raw' Float32Array DN, 'rgb8',
+// 'rgba8', 'yuv420' {y, cb, cr}.
 import { createRequire } from 'module';
+import { sensorParams, AutoExposure, updateAWB, nativeSensor, nativeIsp } from './sensor.mjs';
 
 const require = createRequire(import.meta.url);
 const native = require('../napi/vt_napi.node');
@@ -37,7 +39,8 @@ export const PRESETS = Object.freeze({
   judge: { spp: 64, maxDepth: 5, lightSamples: 1 },
 });
 
-const KIND = { linear: 0, raw: 1, ri: 2 };
+const KIND = { linear: 0, raw: 1, ri: 2, rgb8: 3, rgba8: 4, yuv420: 5 };
+const KIND_STATS = 6;
 // Node 12 has no `??`: the same rule as a function
 const dflt = (a, b) => (a === undefined || a === null ? b : a);
 const MESH_EMITTER = 2;
@@ -136,7 +139,8 @@ export class TwinRenderer {
   /**
    * opts: {scene: upload arrays ({nodes?, tris, verts, meshes, materials, lights, emit, emitArea}) or
    * {prepared: the reference's prepareScene output}, cameras: [{name, lens (scene lens block or 9 numbers),
-   * T_D_C (4x4) | {R (9, columns), pos}, size: [w, h]}], device?, preset?, log?, ...DEFAULTS}
+   * T_D_C (4x4) | {R (9, columns), pos}, size: [w, h]}], physics? (physics.yaml's sensor / isp blocks), sensor?
+   * (overrides of sensorParams), device?, preset?, log?, ...DEFAULTS}
    */
   static async create(opts) {
     const r = new TwinRenderer();
@@ -157,6 +161,7 @@ export class TwinRenderer {
     for (const pr of src.profiles || []) this.setLightProfile(pr.light, pr);
     this.materials = Float32Array.from(src.materials);
     this.lights = Float32Array.from(src.lights);
+    this.sensorP = sensorParams(opts.physics || {}, opts.sensor || {});
     this.cameras = {};
     (opts.cameras || []).forEach((c, i) => this._addCamera(c, i));
     this.frame = 0;
@@ -228,7 +233,8 @@ export class TwinRenderer {
       pos = [T[0][3], T[1][3], T[2][3]];
     } else { R = Array.from(c.R); pos = Array.from(c.pos); }
     const size = c.size || (c.lens && c.lens.size) || [c.width, c.height];
-    this.cameras[c.name] = { index, name: c.name, lens, R, pos, size, mode: c.mode || 0 };
+    this.cameras[c.name] = { index, name: c.name, lens, R, pos, size, mode: c.mode || 0, manual: null, awb: null,
+      ae: new AutoExposure(this.sensorP, { exposureUnits: this.sensorP.exposureUnits0 }) };
   }
 
   _slot(name) {
@@ -298,6 +304,7 @@ export class TwinRenderer {
     const cam = this._slot(name);
     this._pack(cam, opts);
     native.render(this.ctx, [cam.index], { mode: 1 });
+    if (opts.exposure !== undefined) this._manualExposure(name, opts.exposure);
     return { camera: name, width: cam.size[0], height: cam.size[1] };
   }
 
@@ -319,25 +326,69 @@ export class TwinRenderer {
     return out;
   }
 
-  /** Radiance -> raw DN (sensorOptics + sensorExpose, one fused kernel). opts: {seed, frame, noise, exposure_s,
-   * gain, params} with params overriding the sensor uniform fields (gpu/sensor.js:158-177). */
-  sensor(name, opts = {}) {
+  /** Manual exposure for a camera ({time_s, gain} or seconds), or back to auto with null / 'auto'
+   * (gpu/renderer.js:1043-1047). */
+  _manualExposure(name, e) {
     const cam = this._slot(name);
-    const p = { ...(this.opts.sensor || {}), ...(opts.params || {}) };
-    const g = p.m || [1.35e7, 0, 0, 0, 1.35e7, 0, 0, 0, 1.35e7];
-    native.sensor(this.ctx, cam.index, {
-      ...p, m: g.flat ? g.flat() : g, frame: dflt(opts.frame, this.frame), seed: dflt(opts.seed, 0),
-      exposure: dflt(opts.exposure_s, dflt(p.exposure, 1e-3)), gain: dflt(opts.gain, dflt(p.gain, 1)), noise: opts.noise === false ? 0 : 1,
-    });
-    return { camera: name, kind: 'raw' };
+    if (e === null || e === 'auto') { cam.manual = null; return; }
+    cam.manual = typeof e === 'number' ? { time_s: e, gain: 1 }
+      : { time_s: dflt(e.time_s, dflt(e.exposure_s, e.time)), gain: dflt(e.gain, 1) };
   }
 
-  /** Fresh host copy: 'linear' Float32Array w*h*4, 'raw' Float32Array of integer DN (gpu/renderer.js:1191-1231). */
-  async readback(name, kind = 'linear') {
+  /** The exposure the next sensor pass of this camera uses: auto exposure's or the manual one
+   * (gpu/renderer.js:1050-1054). */
+  exposureOf(name) {
+    const cam = this._slot(name);
+    if (cam.manual) return { ...cam.manual, auto: false };
+    return { time_s: cam.ae.exposure, gain: cam.ae.gain, gainUnits: cam.ae.gainUnits, units: cam.ae.units, auto: true };
+  }
+
+  /**
+   * Radiance -> 8-bit camera RGB (gpu/renderer.js:1056-1099): the raw stage (optics, glare, exposure, noise,
+   * ADC) and the ISP (demosaic, WB + CCM, NR, tone, sharpening, BT.601 YCbCr). opts: {seed, frame, noise,
+   * exposure_s, gain, params} with params overriding sensorParams fields. The frame's AE / AWB statistics are
+   * read back without blocking and steer the next frame, as the UVC loop does.
+   */
+  sensor(name, opts = {}) {
+    const cam = this._slot(name);
+    const p = { ...this.sensorP, ...(opts.params || {}) };
+    if (p.awb && cam.awb && !(opts.params && opts.params.wb)) p.wb = p.wb.map((g, c) => g * cam.awb[c]);
+    if (opts.noise === false) p.noise = false;
+    const ex = this.exposureOf(name);
+    const [width, height] = cam.size;
+    native.sensor(this.ctx, cam.index, nativeSensor(p, { width, height, frame: dflt(opts.frame, this.frame),
+      seed: dflt(opts.seed, 0), exposure: dflt(opts.exposure_s, ex.time_s), gain: dflt(opts.gain, ex.gain) }));
+    native.isp(this.ctx, cam.index, nativeIsp(p));
+    this._afterSensor(name);
+    return { camera: name, kind: 'rgba8', width, height };
+  }
+
+  /** AE / AWB from the frame's statistics once they arrive (gpu/renderer.js:1075-1089). */
+  _afterSensor(name) {
+    const cam = this.cameras[name], [w, h] = cam.size;
+    cam.pendingStats = native.readback(this.ctx, cam.index, KIND_STATS, w, h).then((st) => {
+      cam.meanLuma = st[0] / (w * h * 255);
+      cam.meanRGB = [st[1], st[2], st[3]].map((x) => x / (w * h * 1023));
+      if (!cam.manual) cam.ae.update(cam.meanLuma);
+      if (this.sensorP.awb) cam.awb = updateAWB(cam.awb, cam.meanRGB, this.sensorP.awbDamping);
+      return st;
+    }).catch(() => null);
+  }
+
+  /** Fresh host copy (gpu/renderer.js:1191-1231): 'linear' Float32Array w*h*4, 'raw' Float32Array of integer DN,
+   * 'rgb8' Uint8Array w*h*3, 'rgba8' Uint8Array w*h*4, 'yuv420' {width, height, y, cb, cr}. */
+  async readback(name, kind = 'rgb8') {
     const cam = this._slot(name);
     if (!(kind in KIND)) throw new Error(`readback kind ${kind} is not supported by this build`);
-    const data = await native.readback(this.ctx, cam.index, KIND[kind], cam.size[0], cam.size[1]);
-    return kind === 'raw' ? Float32Array.from(data) : data;
+    const [w, h] = cam.size;
+    const data = await native.readback(this.ctx, cam.index, KIND[kind], w, h);
+    if (kind === 'raw') return Float32Array.from(data);
+    if (kind === 'yuv420') {
+      const cw = Math.ceil(w / 2), ch = Math.ceil(h / 2);
+      return { width: w, height: h, y: data.slice(0, w * h), cb: data.slice(w * h, w * h + cw * ch),
+        cr: data.slice(w * h + cw * ch, w * h + 2 * cw * ch) };
+    }
+    return data;
   }
 
   /** KB4 projection of D-frame points into a camera (gpu/renderer.js:1261-1265), FP64. */

@@ -2,17 +2,22 @@
 // js/renderer.mjs. Blocking calls (sync, readback, stats, autotune, backward) run as napi_async_work and return
 // Promises, as the reference's queue.onSubmittedWorkDone / mapAsync do (gpu/renderer.js:986-996, 1191-1231);
 // enqueue-only calls (upload, setCamera, render, sensor, updateMaterials, updateLights) return synchronously and
-// throw Error on a bad status.
+// throw Error on a bad status. Calls into libvt.so are serialised by one lock (the pool's jobs included).
 #define NAPI_VERSION 8
 #include <node_api.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/vt.h"
 
 namespace {
+
+// the C ABI wants one host thread per context at a time; the async jobs run on libuv's pool, so every call into
+// libvt.so from this addon (main thread or pool) holds this lock
+std::mutex g_mu;
 
 #define CHECK_NAPI(call)                                                   \
   do {                                                                     \
@@ -117,6 +122,7 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (c) vt_destroy(c);
   return nullptr;
 }
@@ -170,6 +176,7 @@ napi_value UploadScene(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   size_t n;
   vt_scene s{};
@@ -208,6 +215,7 @@ napi_value SetCamera(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t slot = 0;
   napi_get_value_uint32(env, argv[1], &slot);
@@ -246,6 +254,7 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t len = 0;
   if (argc < 2 || napi_get_array_length(env, argv[1], &len) != napi_ok) {
@@ -288,6 +297,7 @@ napi_value SetLightProfile(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t light = 0;
   napi_get_value_uint32(env, argv[1], &light);
@@ -323,6 +333,7 @@ napi_value Visible(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t cam = 0;
   double tol = 2e-4, time = 0.5;
@@ -358,6 +369,7 @@ napi_value Render(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t len = 0;
   napi_get_array_length(env, argv[1], &len);
@@ -389,6 +401,7 @@ napi_value Sensor(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t slot = 0;
   napi_get_value_uint32(env, argv[1], &slot);
@@ -413,8 +426,44 @@ napi_value Sensor(napi_env env, napi_callback_info info) {
   p.black = (float)num(env, o, "black", 0.0);
   p.noise_on = (uint32_t)num(env, o, "noise", 1);
   p.pupil_exp = (float)num(env, o, "pupilExp", 1.0);
+  p.glare_fraction = (float)num(env, o, "glareFraction", 0.0);
+  p.glare_sigma = (float)num(env, o, "glareSigma", 1.0);
+  p.glare_level = (uint32_t)num(env, o, "glareLevel", 16);
   int rc = vt_sensor(c, slot, &p);
   if (rc) return throw_vt(env, c, rc, "vt_sensor");
+  return nullptr;
+}
+
+// isp(ctx, slot, {demosaic, nr, tone, chroma420, wb[3], ccm[9], gamma, sCurveC, blackOffset, nrSigmaPx,
+// nrRangeScale, sharpenAmount, sharpenRadius}) (numeric codes; js/renderer.mjs maps the reference's strings)
+napi_value Isp(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  uint32_t slot = 0;
+  napi_get_value_uint32(env, argv[1], &slot);
+  vt_isp_params q{};
+  napi_value o = argv[2];
+  q.demosaic = (uint32_t)num(env, o, "demosaic", 0);
+  q.nr = (uint32_t)num(env, o, "nr", 0);
+  q.tone = (uint32_t)num(env, o, "tone", 0);
+  q.chroma420 = (uint32_t)num(env, o, "chroma420", 1);
+  if (!float_list(env, o, "wb", q.wb, 3)) q.wb[0] = q.wb[1] = q.wb[2] = 1.0f;
+  if (!float_list(env, o, "ccm", q.ccm, 9)) {
+    for (int k = 0; k < 9; k++) q.ccm[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+  }
+  q.gamma = (float)num(env, o, "gamma", 2.2);
+  q.s_curve_c = (float)num(env, o, "sCurveC", 1.0);
+  q.black_offset = (float)num(env, o, "blackOffset", 0.0);
+  q.nr_sigma_px = (float)num(env, o, "nrSigmaPx", 0.4);
+  q.nr_range_scale = (float)num(env, o, "nrRangeScale", 2.0);
+  q.sharpen_amount = (float)num(env, o, "sharpenAmount", 0.0);
+  q.sharpen_radius = (float)num(env, o, "sharpenRadius", 1.0);
+  int rc = vt_isp(c, slot, &q);
+  if (rc) return throw_vt(env, c, rc, "vt_isp");
   return nullptr;
 }
 
@@ -439,6 +488,7 @@ struct Job {
 
 void job_exec(napi_env, void *data) {
   Job *j = static_cast<Job *>(data);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (j->kind == -1) j->rc = vt_sync(j->c);
   else if (j->kind == -2) j->rc = vt_frame_stats_get(j->c, &j->st);
   else if (j->kind == -3)
@@ -468,11 +518,11 @@ void job_done(napi_env env, napi_status, void *data) {
   } else if (j->kind == -2) {
     napi_create_object(env, &res);
     const char *names[] = {"rays", "samples", "msRender", "msSensor", "msKernelAvg", "launches", "stackOverflows",
-                           "msBackward", "raysBackward", "graphUsed", "graphBuilds"};
+                           "msBackward", "raysBackward", "graphUsed", "graphBuilds", "msIsp"};
     double vals[] = {j->st.rays, j->st.samples, j->st.ms_render, j->st.ms_sensor, j->st.ms_kernel_avg,
                      (double)j->st.launches, (double)j->st.stack_overflows, j->st.ms_backward, j->st.rays_backward,
-                     (double)j->st.graph_used, (double)j->st.graph_builds};
-    for (int k = 0; k < 11; k++) {
+                     (double)j->st.graph_used, (double)j->st.graph_builds, j->st.ms_isp};
+    for (int k = 0; k < 12; k++) {
       napi_value v;
       napi_create_double(env, vals[k], &v);
       napi_set_named_property(env, res, names[k], v);
@@ -512,8 +562,12 @@ void job_done(napi_env env, napi_status, void *data) {
     void *dst;
     napi_create_arraybuffer(env, j->buf.size(), &dst, &ab);
     std::memcpy(dst, j->buf.data(), j->buf.size());
-    napi_typedarray_type t = j->kind == VT_KIND_RAW ? napi_uint16_array : napi_float32_array;
-    size_t elems = j->kind == VT_KIND_RAW ? j->buf.size() / 2 : j->buf.size() / 4;
+    const bool bytes = j->kind == VT_KIND_RGB8 || j->kind == VT_KIND_RGBA8 || j->kind == VT_KIND_YUV420;
+    napi_typedarray_type t = j->kind == VT_KIND_RAW     ? napi_uint16_array
+                             : bytes                    ? napi_uint8_array
+                             : j->kind == VT_KIND_STATS ? napi_uint32_array
+                                                        : napi_float32_array;
+    size_t elems = j->kind == VT_KIND_RAW ? j->buf.size() / 2 : bytes ? j->buf.size() : j->buf.size() / 4;
     napi_create_typedarray(env, t, elems, ab, 0, &res);
     napi_resolve_deferred(env, j->deferred, res);
   }
@@ -539,6 +593,7 @@ napi_value Sync(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
@@ -551,6 +606,7 @@ napi_value Stats(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
@@ -563,6 +619,7 @@ napi_value Readback(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
@@ -573,7 +630,13 @@ napi_value Readback(napi_env env, napi_callback_info info) {
   napi_get_value_uint32(env, argv[4], &j->h);
   j->kind = kind;
   const size_t px = (size_t)j->w * j->h;
-  j->buf.resize(kind == VT_KIND_LINEAR ? px * 16 : kind == VT_KIND_RAW ? px * 2 : px * 4);
+  const size_t cpx = (size_t)((j->w + 1) / 2) * ((j->h + 1) / 2);
+  j->buf.resize(kind == VT_KIND_LINEAR   ? px * 16
+                : kind == VT_KIND_RAW    ? px * 2
+                : kind == VT_KIND_RGB8   ? px * 3
+                : kind == VT_KIND_YUV420 ? px + 2 * cpx
+                : kind == VT_KIND_STATS  ? 16
+                                         : px * 4);
   return queue_job(env, j);
 }
 
@@ -595,6 +658,7 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
@@ -618,6 +682,7 @@ napi_value Backward(napi_env env, napi_callback_info info) {
   napi_value argv[6];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
@@ -663,6 +728,7 @@ napi_value update_table(napi_env env, napi_callback_info info, bool mats) {
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   napi_typedarray_type t;
   size_t len = 0;
@@ -688,6 +754,7 @@ napi_value RadiosityBuild(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   vt_radiosity_opts o{64, 4, 16, 0.006f, 0.003f};
   napi_valuetype t = napi_undefined;
@@ -721,6 +788,7 @@ napi_value RadiosityUpdate(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   bool direct = true;
   uint32_t iters = 0;
@@ -737,6 +805,7 @@ napi_value RadiosityState(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   uint32_t nodes = 0;
   if (argc > 1) napi_get_value_uint32(env, argv[1], &nodes);
@@ -764,6 +833,7 @@ napi_value LastError(napi_env env, napi_callback_info info) {
   napi_value argv[1], out;
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   CHECK_NAPI(napi_create_string_utf8(env, vt_last_error(c), NAPI_AUTO_LENGTH, &out));
   return out;
@@ -774,7 +844,7 @@ napi_value Init(napi_env env, napi_value exports) {
     const char *name;
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
-             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"sync", Sync}, {"stats", Stats},
+             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"isp", Isp}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
              {"radiosityBuild", RadiosityBuild}, {"radiosityUpdate", RadiosityUpdate},

@@ -10,6 +10,7 @@ LIBPATH = os.path.join(PKG, "libvt.so")
 VT_OK, VT_ERR_ARG, VT_ERR_SIZE, VT_ERR_HIP, VT_ERR_LOST, VT_ERR_STATE, VT_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
 VT_RENDER_BATCHED, VT_RENDER_PER_CAMERA, VT_RENDER_WAVEFRONT, VT_RENDER_AUTO = 0, 1, 2, 3
 VT_KIND_LINEAR, VT_KIND_RAW, VT_KIND_RI = 0, 1, 2
+VT_KIND_RGB8, VT_KIND_RGBA8, VT_KIND_YUV420, VT_KIND_STATS = 3, 4, 5, 6
 VT_DENSE_FP32, VT_DENSE_BF16_REFINE, VT_DENSE_FP64 = 0, 1, 2
 
 EXPORTS = [
@@ -19,7 +20,7 @@ EXPORTS = [
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
     "vt_set_light_profile", "vt_radiosity_build", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
-    "vt_radiosity_tables", "vt_capsule_test",
+    "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test",
 ]
 
 # camera option bits (vt_camera.flags)
@@ -80,14 +81,37 @@ class SensorParams(C.Structure):
     _fields_ = [("cfa", u32), ("frame", u32), ("seed", u32), ("optics_k", f32), ("esf_sigma", f32),
                 ("exposure", f32), ("gain", f32), ("m", f32 * 9), ("read_noise", f32), ("full_well", f32),
                 ("dark", f32), ("conv_gain", f32), ("max_dn", f32), ("black", f32), ("noise_on", u32),
-                ("pupil_exp", f32)]
+                ("pupil_exp", f32), ("glare_fraction", f32), ("glare_sigma", f32), ("glare_level", u32)]
+
+
+class IspParams(C.Structure):
+    _fields_ = [("demosaic", u32), ("nr", u32), ("tone", u32), ("chroma420", u32), ("wb", f32 * 3), ("ccm", f32 * 9),
+                ("gamma", f32), ("s_curve_c", f32), ("black_offset", f32), ("nr_sigma_px", f32),
+                ("nr_range_scale", f32), ("sharpen_amount", f32), ("sharpen_radius", f32)]
+
+
+ISP_DEFAULTS = dict(demosaic=0, nr=0, tone=0, chroma420=1, wb=(1.0, 1.0, 1.0),
+                    ccm=(1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0), gamma=2.2, s_curve_c=1.0, black_offset=0.0,
+                    nr_sigma_px=0.4, nr_range_scale=2.0, sharpen_amount=0.0, sharpen_radius=1.0)
+
+
+def isp_struct(p):
+    q = dict(ISP_DEFAULTS, **p)
+    s = IspParams()
+    s.demosaic, s.nr, s.tone, s.chroma420 = int(q["demosaic"]), int(bool(q["nr"])), int(q["tone"]), int(bool(q["chroma420"]))
+    s.wb[:] = [float(x) for x in q["wb"]]
+    s.ccm[:] = [float(x) for x in np.asarray(q["ccm"]).reshape(-1)]
+    s.gamma, s.s_curve_c, s.black_offset = q["gamma"], q["s_curve_c"], q["black_offset"]
+    s.nr_sigma_px, s.nr_range_scale = q["nr_sigma_px"], q["nr_range_scale"]
+    s.sharpen_amount, s.sharpen_radius = q["sharpen_amount"], q["sharpen_radius"]
+    return s
 
 
 class FrameStats(C.Structure):
     _fields_ = [("rays", f64), ("samples", f64), ("ms_render", f64), ("ms_sensor", f64), ("ms_kernel_avg", f64),
                 ("launches", u32), ("stack_overflows", u32), ("ms_backward", f64), ("rays_backward", f64),
                 ("graph_used", u32), ("graph_builds", u32), ("rays_extension", f64), ("rays_shadow", f64),
-                ("ms_ext_avg", f64), ("ext_launches", u32)]
+                ("ms_ext_avg", f64), ("ext_launches", u32), ("ms_isp", f64)]
 
 
 class Grads(C.Structure):
@@ -137,6 +161,8 @@ def load():
         "vt_camera_ray_test": (i32, [vp, u32, vp, u32, vp]),
         "vt_bsdf_test": (i32, [vp, vp, u32, vp]),
         "vt_sensor_test": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "vt_isp": (i32, [vp, u32, vp]),
+        "vt_isp_test": (i32, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
         "vt_jacobi_sparse": (i32, [vp, vp, vp, u32, vp]),
         "vt_dense_upload": (i32, [vp, u32, u32, vp, vp]),
         "vt_dense_formfactors": (i32, [vp, u32, vp, f32, u32, vp]),
@@ -337,14 +363,30 @@ class Context:
         p = sensor_struct(params)
         self.check(self.L.vt_sensor(self.h, slot, C.byref(p)))
 
+    def isp(self, slot, params):
+        q = isp_struct(params)
+        self.check(self.L.vt_isp(self.h, slot, C.byref(q)))
+
     def readback(self, slot, kind, w, h):
+        cw, ch = (w + 1) // 2, (h + 1) // 2
         if kind == VT_KIND_LINEAR:
             out = np.zeros((h, w, 4), np.float32)
         elif kind == VT_KIND_RAW:
             out = np.zeros((h, w), np.uint16)
+        elif kind == VT_KIND_RGB8:
+            out = np.zeros((h, w, 3), np.uint8)
+        elif kind == VT_KIND_RGBA8:
+            out = np.zeros((h, w, 4), np.uint8)
+        elif kind == VT_KIND_YUV420:
+            out = np.zeros(w * h + 2 * cw * ch, np.uint8)
+        elif kind == VT_KIND_STATS:
+            out = np.zeros(4, np.uint32)
         else:
             out = np.zeros((h, w), np.float32)
         self.check(self.L.vt_readback(self.h, slot, kind, ptr(out), out.nbytes))
+        if kind == VT_KIND_YUV420:
+            return dict(y=out[:w * h].reshape(h, w), cb=out[w * h:w * h + cw * ch].reshape(ch, cw),
+                        cr=out[w * h + cw * ch:].reshape(ch, cw))
         return out
 
     def sync(self):
@@ -414,6 +456,25 @@ class Context:
         self.check(self.L.vt_sensor_test(self.h, C.byref(p), w, h, ptr(radiance), ptr(ri), ptr(dn)))
         return dn
 
+    def isp_test(self, sensor, params, dn):
+        """the ISP on caller-given raw DN (h, w): dict(rgba, y/cb/cr (4:2:0), stats)"""
+        dn = np.ascontiguousarray(dn, dtype=np.uint16)
+        h, w = dn.shape
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        sp = sensor_struct(dict(dict(frame=0, seed=0, optics_k=1.0, esf_sigma=0.0, exposure=1.0, m=np.eye(3),
+                                     full_well=1e9, dark=0.0, noise_on=0), **sensor))
+        q = isp_struct(params)
+        rgba = np.zeros((h, w, 4), np.uint8)
+        yuv = np.zeros(w * h + 2 * cw * ch, np.uint8)
+        stats = np.zeros(4, np.uint32)
+        self.check(self.L.vt_isp_test(self.h, C.byref(sp), C.byref(q), w, h, ptr(dn), ptr(rgba), ptr(yuv),
+                                      ptr(stats)))
+        out = dict(rgba=rgba, stats=stats)
+        if q.chroma420:
+            out.update(y=yuv[:w * h].reshape(h, w), cb=yuv[w * h:w * h + cw * ch].reshape(ch, cw),
+                       cr=yuv[w * h + cw * ch:].reshape(ch, cw))
+        return out
+
     # ---------------------------------------------------------------- radiosity
     def jacobi_sparse(self, t, node_state, iters):
         keep = {k: np.ascontiguousarray(v) for k, v in t.items() if isinstance(v, np.ndarray)}
@@ -458,6 +519,9 @@ def sensor_struct(p):
     s.max_dn, s.black = p["max_dn"], p["black"]
     s.noise_on = 1 if p["noise_on"] else 0
     s.pupil_exp = float(p.get("pupil_exp", 1.0))
+    s.glare_fraction = float(p.get("glare_fraction", 0.0))
+    s.glare_sigma = float(p.get("glare_sigma", 1.0))
+    s.glare_level = int(p.get("glare_level", 16))
     return s
 
 

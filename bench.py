@@ -2,9 +2,10 @@
 """Camera-path benchmark on one MI355X: the full-scale forward frame.
 
 Workload (default): the seeded 2,013,432-triangle scene (dome + 100 cylinders + 100 boxes, 256 materials,
-16 quad lights), three KB4 cameras 1920x1080 at 64 spp, paths of depth 5 with NEE, then the fused sensor
-epilogue (RGGB, Poisson shot + 2 DN Gaussian read noise, 12 bit) per camera. One step = one frame: every
-camera traced in one batched persistent launch, then the three sensor launches. Inputs are resident in HBM
+16 quad lights), three KB4 cameras 1920x1080 at 64 spp, paths of depth 5 with NEE, then per camera the fused
+sensor epilogue (RGGB, Poisson shot + 2 DN Gaussian read noise, 12 bit) and the ISP to 8-bit RGB and YUV 4:2:0
+(bilinear demosaic, bilateral NR, sRGB). One step = one frame: every camera traced (the wavefront form by
+default), then the three sensor and ISP passes. Inputs are resident in HBM
 before the timed region; the timed region is exactly `--steps` frames between two device synchronisations.
 
 Prints one JSON line. `value` is Mrays/s (camera + extension + shadow rays counted on the device).
@@ -25,6 +26,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s and spp·Mpix/s per frame on 1 MI355X; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
+# sensorParams' ISP defaults (the reference's 'none' noise-reduction string enables the bilateral)
+ISP = dict(demosaic=0, nr=1, nr_sigma_px=0.4, nr_range_scale=2.0, tone=0, chroma420=1)
 
 
 def log(msg):
@@ -249,6 +252,51 @@ def bench_copy(args):
                       "check": float(y[-1].item())}), flush=True)
 
 
+def bench_sensor(args):
+    """The sensor chain alone on one 1920x1080 camera (rendered once, untimed): the raw stage with glare (the
+    twin's physics settings: glare 0.1 on the 1/16 level, black 168, 12 bit) and the ISP (edge-aware demosaic,
+    bilateral NR, sRGB + S-curve, BT.601 4:2:0), `--steps` passes between two synchronisations. Algorithmic bytes
+    per pixel: 22 (raw: 16 B radiance + 4 B RI in, 2 B DN out) + 64.5 (ISP: DN -> green -> RGB -> NR -> RGBA +
+    YUV)."""
+    import numpy as np
+    from vtwin import lib, scenes
+
+    sc, cams = scenes.c1()
+    cam = dict(cams[0], width=1920, height=1080, spp=4)
+    cam["lens"] = np.array([900.0, 900.0, 959.5, 539.5] + list(cams[0]["lens"][4:]), np.float32)
+    sens = scenes.default_sensor(cfa=0, frame=0, seed=1006)
+    sens.update(black=168.0, glare_fraction=0.1, glare_sigma=0.25 * float(np.hypot(1920, 1080)) / 16.0,
+                glare_level=16)
+    isp = dict(demosaic=1, nr=1, nr_sigma_px=0.4, nr_range_scale=2.0, tone=3, s_curve_c=1.39, black_offset=0.014,
+               chroma420=1)
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    for _ in range(max(1, args.warmup)):
+        ctx.sensor(0, sens)
+        ctx.isp(0, isp)
+    ctx.sync()
+    ms_s, ms_i = [], []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.sensor(0, sens)
+        ctx.isp(0, isp)
+        st = ctx.stats()
+        ms_s.append(st["ms_sensor"])
+        ms_i.append(st["ms_isp"])
+    dt = time.perf_counter() - t1
+    ctx.close()
+    px = 1920 * 1080
+    s_ms, i_ms = float(np.mean(ms_s)), float(np.mean(ms_i))
+    print(json.dumps({"metric": METRIC, "config": {"workload": "sensor chain: one 1920x1080 camera, raw stage with "
+                                                               "glare + ISP (edge-aware, NR, S-curve, 4:2:0)"},
+                      "steps": args.steps, "ms_per_pass_wall": round(1e3 * dt / args.steps, 4),
+                      "raw_stage_ms": round(s_ms, 4), "isp_ms": round(i_ms, 4),
+                      "raw_stage_GBps": round(22 * px / (s_ms * 1e-3) / 1e9, 1),
+                      "isp_GBps": round(64.5 * px / (i_ms * 1e-3) / 1e9, 1), "peak_GBps": HBM_PEAK_GBS}), flush=True)
+
+
 def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
@@ -275,9 +323,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
     ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
-                         "copy: counter calibration kernel")
+                         "copy: counter calibration kernel; sensor: the sensor chain alone")
     ap.add_argument("--inner", type=int, default=0, help="BF16 sweeps per refinement step (c4; 0: adaptive)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,6 +340,8 @@ def main():
         return bench_c5(args)
     if args.config == "copy":
         return bench_copy(args)
+    if args.config == "sensor":
+        return bench_sensor(args)
     import numpy as np
     from vtwin import lib, scenes
 
@@ -316,12 +366,13 @@ def main():
         ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk, stack_lds=args.stack, batch=args.batch)
         for k in slots:
             ctx.sensor(k, sensor[k])
+            ctx.isp(k, ISP)
 
     for i in range(args.warmup):
         frame()
         ctx.sync()
         log("warmup %d done" % i)
-    ms_kernel, rays_step, launches, ext_ms, ext_rays, ext_launches = [], [], 0, [], [], 0
+    ms_kernel, rays_step, launches, ext_ms, ext_rays, ext_launches, isp_ms = [], [], 0, [], [], 0, []
     ctx.sync()
     t1 = time.perf_counter()
     for i in range(args.steps):
@@ -333,6 +384,7 @@ def main():
         ext_ms.append(st["ms_ext_avg"])
         ext_rays.append(st["rays_extension"])
         ext_launches = st["ext_launches"]
+        isp_ms.append(st["ms_isp"])
         log("step %d: %.1f ms trace, %.3g rays" % (i, st["ms_render"], st["rays"]))
     ctx.sync()
     t2 = time.perf_counter()
@@ -346,13 +398,15 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded mulberry32 scene)",
         "config": {"workload": "full-scale forward: 2,013,432-tri seeded dome scene, 256 materials, 16 lights; "
-                               "3 KB4 cameras 1920x1080, 64 spp, depth 5, NEE; RGGB sensor, shot + 2 DN read noise, 12 bit",
+                               "3 KB4 cameras 1920x1080, 64 spp, depth 5, NEE; RGGB sensor, shot + 2 DN read noise, 12 bit; "
+                               "ISP: bilinear demosaic, bilateral NR, sRGB, BT.601 4:2:0",
                    "cameras": len(cams), "spp": 64, "depth": 5, "mode": args.mode,
                    "form": "wavefront" if ext_launches else ("per-camera" if args.mode == "per-camera" else "batched")},
         "spp_mpix_per_s": round(spp_px / dt / 1e6, 2),
         "rays_per_frame": total_rays / args.steps,
         "trace_ms_per_launch": round(float(np.mean(ms_kernel)), 3),
         "trace_launches_per_frame": launches,
+        "isp_ms_per_camera": round(float(np.mean(isp_ms)), 4),
         "extension_trace_launches_per_frame": ext_launches,
         "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()} if tuned else None,
     }

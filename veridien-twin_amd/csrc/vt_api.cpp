@@ -27,8 +27,8 @@ struct CamSlot {
   bool set = false;
   vt_camera c{};
   DevBuf acc, dn, ri, lin, adj;
-  // glare levels; ISP: green plane, linear RGB (two, for NR), 8-bit RGBA, YUV planes, statistics, rgb8 staging
-  DevBuf glare_a, glare_b, gpl, isp_a, isp_b, rgba, yuv, stats, rgb8;
+  // glare levels; ISP: linear RGB, block partial sums, 8-bit RGBA, YUV planes, statistics, rgb8 staging
+  DevBuf glare_a, glare_b, isp_rgb, isp_part, rgba, yuv, stats, rgb8;
   vt_sensor_params sp{};      // the last vt_sensor's parameters (the ISP reads black, max DN, noise model, CFA)
   bool isp_done = false, isp420 = false;
   bool ri_valid = false;
@@ -320,7 +320,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.ri);
     release(cs.lin);
     release(cs.adj);
-    DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.gpl, &cs.isp_a, &cs.isp_b, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8};
+    DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.isp_rgb, &cs.isp_part, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8};
     for (DevBuf *b : isp) release(*b);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
@@ -1340,9 +1340,8 @@ static int isp_params(vt_ctx *c, const vt_sensor_params &sp, const vt_isp_params
 }
 static int ensure_isp(vt_ctx *c, CamSlot &s, uint32_t w, uint32_t h) {
   const size_t px = (size_t)w * h, cpx = (size_t)((w + 1) / 2) * ((h + 1) / 2);
-  int rc = ensure(c, s.gpl, px * 4);
-  if (!rc) rc = ensure(c, s.isp_a, px * 16);
-  if (!rc) rc = ensure(c, s.isp_b, px * 16);
+  int rc = ensure(c, s.isp_rgb, px * 16);
+  if (!rc) rc = ensure(c, s.isp_part, (size_t)isp_partials(w, h) * 16);
   if (!rc) rc = ensure(c, s.rgba, px * 4);
   if (!rc) rc = ensure(c, s.yuv, px + 2 * cpx);
   if (!rc) rc = ensure(c, s.stats, 16);
@@ -1391,7 +1390,7 @@ extern "C" int vt_isp(vt_ctx *c, uint32_t cam, const vt_isp_params *q) {
   if (!rc) rc = ensure_isp(c, s, s.c.width, s.c.height);
   if (rc) return rc;
   VT_HIP(c, hipEventRecord(c->ev_i0, c->stream));
-  VT_HIP(c, launch_isp(P, (const uint16_t *)s.dn.p, (float *)s.gpl.p, (float4 *)s.isp_a.p, (float4 *)s.isp_b.p,
+  VT_HIP(c, launch_isp(P, (const uint16_t *)s.dn.p, (float4 *)s.isp_rgb.p, (uint4 *)s.isp_part.p,
                        (uchar4 *)s.rgba.p, (uint8_t *)s.yuv.p, (unsigned int *)s.stats.p, c->stream));
   VT_HIP(c, hipEventRecord(c->ev_i1, c->stream));
   c->isp_timed = true;
@@ -1799,16 +1798,16 @@ extern "C" int vt_isp_test(vt_ctx *c, const vt_sensor_params *sp, const vt_isp_p
   int rc = isp_params(c, *sp, q, w, h, P);
   if (rc) return rc;
   const size_t px = (size_t)w * h, cpx = (size_t)((w + 1) / 2) * ((h + 1) / 2);
-  // test_in: two float4 planes; test_aux: green plane, then rgba, yuv, stats; test_out: the DN
-  if ((rc = ensure(c, c->test_in, px * 32))) return rc;
-  const size_t o_rgba = (px * 4 + 255) / 256 * 256, o_yuv = o_rgba + (px * 4 + 255) / 256 * 256,
+  // test_in: the RGB plane, then the partial sums; test_aux: rgba, yuv, stats; test_out: the DN
+  if ((rc = ensure(c, c->test_in, px * 16 + (size_t)isp_partials(w, h) * 16))) return rc;
+  const size_t o_rgba = 0, o_yuv = o_rgba + (px * 4 + 255) / 256 * 256,
                o_st = o_yuv + (px + 2 * cpx + 255) / 256 * 256;
   if ((rc = ensure(c, c->test_aux, o_st + 16))) return rc;
   if ((rc = ensure(c, c->test_out, px * 2))) return rc;
   char *aux = (char *)c->test_aux.p;
   VT_HIP(c, hipMemcpy(c->test_out.p, dn, px * 2, hipMemcpyHostToDevice));
-  VT_HIP(c, launch_isp(P, (const uint16_t *)c->test_out.p, (float *)aux, (float4 *)c->test_in.p,
-                       (float4 *)c->test_in.p + px, (uchar4 *)(aux + o_rgba), (uint8_t *)(aux + o_yuv),
+  VT_HIP(c, launch_isp(P, (const uint16_t *)c->test_out.p, (float4 *)c->test_in.p,
+                       (uint4 *)((float4 *)c->test_in.p + px), (uchar4 *)(aux + o_rgba), (uint8_t *)(aux + o_yuv),
                        (unsigned int *)(aux + o_st), c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(rgba, aux + o_rgba, px * 4, hipMemcpyDeviceToHost));

@@ -139,9 +139,11 @@ struct IspDev {
   float nr_sigma, nr_scale, gamma, s_c, black_off, sharpen_amount;
   int sharpen_rad;
 };
-// stats: [0] luma sum, [1..3] grey-world sums (x 1023); yuv: Y plane w*h, then Cb and Cr ((w+1)/2 x (h+1)/2)
-hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float *g, float4 *rgb, float4 *tmp, uchar4 *rgba,
-                      uint8_t *yuv, unsigned int *stats, hipStream_t s);
+// stats: [0] luma sum, [1..3] grey-world sums (x 1023); yuv: Y plane w*h, then Cb and Cr ((w+1)/2 x (h+1)/2);
+// rgb: w*h linear RGB; part: isp_partials(w, h) block partial sums
+uint32_t isp_partials(uint32_t w, uint32_t h);
+hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float4 *rgb, uint4 *part, uchar4 *rgba, uint8_t *yuv,
+                      unsigned int *stats, hipStream_t s);
 
 // node radiosity
 struct GatherDev {

@@ -294,85 +294,98 @@ __device__ __forceinline__ float raw_at(const IspDev &P, const uint16_t *dn, int
   yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
   return (float)dn[(size_t)yy * W + xx];
 }
-__device__ __forceinline__ float g_at(const IspDev &P, const float *g, int x, int y) {
-  const int W = (int)P.w, H = (int)P.h;
-  x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
-  y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-  return g[(size_t)y * W + x];
+
+// One block demosaics a 16 x 16 tile (demosaicG :229-256 and demosaicRB :263-319 fused): the raw tile with a
+// 3-pixel mirrored halo and then the green plane with a 1-pixel halo are staged in LDS, so every DN is read from
+// HBM once and the green plane never leaves the CU. Green: the sample at G sites; elsewhere the 4-neighbour mean
+// (bilinear), the mean along the smaller green gradient (edge-aware), or Hamilton-Adams with the site's own second
+// derivative. Red and blue: colour differences over the nearest sites (Hamilton-Adams) or the sites' plain means,
+// then normalisation, the grey-world sums (x 1023, integers; one partial per block, no atomics), white balance and
+// CCM -> linear camera RGB.
+constexpr int DT = 16, DR = DT + 6, DG = DT + 2;
+
+__device__ __forceinline__ unsigned int wave_sum(unsigned int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// green plane: the sample at G sites; elsewhere the 4-neighbour mean (bilinear), the mean along the smaller green
-// gradient (edge-aware), or Hamilton-Adams with the site's own second derivative (demosaicG :229-256)
-__global__ void k_isp_green(IspDev P, const uint16_t *dn, float *g) {
-  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
-  if (x >= (int)P.w || y >= (int)P.h) return;
-  float v = raw_at(P, dn, x, y);
-  if (cfa_channel((uint32_t)x, (uint32_t)y, P.cfa) != 1) {
-    const float gl = raw_at(P, dn, x - 1, y), gr = raw_at(P, dn, x + 1, y), gu = raw_at(P, dn, x, y - 1),
-                gd = raw_at(P, dn, x, y + 1);
-    if (P.demosaic == 0) {
-      v = 0.25f * (((gl + gr) + gu) + gd);
-    } else if (P.demosaic == 1) {
-      const float dh = fabsf(gl - gr), dv = fabsf(gu - gd);
-      v = dh < dv ? 0.5f * (gl + gr) : (dv < dh ? 0.5f * (gu + gd) : 0.25f * (((gl + gr) + gu) + gd));
-    } else {
-      const float c = v;
-      const float lh = (2.0f * c - raw_at(P, dn, x - 2, y)) - raw_at(P, dn, x + 2, y);
-      const float lv = (2.0f * c - raw_at(P, dn, x, y - 2)) - raw_at(P, dn, x, y + 2);
-      const float dh = fabsf(gl - gr) + fabsf(lh), dv = fabsf(gu - gd) + fabsf(lv);
-      const float gh = 0.5f * (gl + gr) + 0.25f * lh, gv = 0.5f * (gu + gd) + 0.25f * lv;
-      v = dh < dv ? gh : (dv < dh ? gv : 0.5f * (gh + gv));
-      v = v < 0.0f ? 0.0f : (v > P.max_dn ? P.max_dn : v);
+__global__ __launch_bounds__(256) void k_isp_demosaic(IspDev P, const uint16_t *dn, float4 *rgb, uint4 *part) {
+  __shared__ float sR[DR * DR];
+  __shared__ float sG[DG * DG];
+  __shared__ unsigned int sred[3][4];
+  const int W = (int)P.w, H = (int)P.h;
+  const int x0 = blockIdx.x * DT, y0 = blockIdx.y * DT;
+  const int tid = threadIdx.y * DT + threadIdx.x;
+  for (int i = tid; i < DR * DR; i += 256) sR[i] = raw_at(P, dn, x0 - 3 + i % DR, y0 - 3 + i / DR);
+  __syncthreads();
+  // raw_at of any coordinate within 3 of the tile
+#define RAW(X, Y) sR[((Y) - (y0 - 3)) * DR + ((X) - (x0 - 3))]
+  for (int i = tid; i < DG * DG; i += 256) {
+    const int cx = min(max(x0 - 1 + i % DG, 0), W - 1), cy = min(max(y0 - 1 + i / DG, 0), H - 1);
+    float v = RAW(cx, cy);
+    if (cfa_channel((uint32_t)cx, (uint32_t)cy, P.cfa) != 1) {
+      const float gl = RAW(cx - 1, cy), gr = RAW(cx + 1, cy), gu = RAW(cx, cy - 1), gd = RAW(cx, cy + 1);
+      if (P.demosaic == 0) {
+        v = 0.25f * (((gl + gr) + gu) + gd);
+      } else if (P.demosaic == 1) {
+        const float dh = fabsf(gl - gr), dv = fabsf(gu - gd);
+        v = dh < dv ? 0.5f * (gl + gr) : (dv < dh ? 0.5f * (gu + gd) : 0.25f * (((gl + gr) + gu) + gd));
+      } else {
+        const float c = v;
+        const float lh = (2.0f * c - RAW(cx - 2, cy)) - RAW(cx + 2, cy);
+        const float lv = (2.0f * c - RAW(cx, cy - 2)) - RAW(cx, cy + 2);
+        const float dh = fabsf(gl - gr) + fabsf(lh), dv = fabsf(gu - gd) + fabsf(lv);
+        const float gh = 0.5f * (gl + gr) + 0.25f * lh, gv = 0.5f * (gu + gd) + 0.25f * lv;
+        v = dh < dv ? gh : (dv < dh ? gv : 0.5f * (gh + gv));
+        v = v < 0.0f ? 0.0f : (v > P.max_dn ? P.max_dn : v);
+      }
     }
+    sG[i] = v;
   }
-  g[(size_t)y * P.w + x] = v;
-}
-
-// red and blue (colour differences over the nearest sites; bilinear modes: the sites' plain means), normalised,
-// the grey-world sums (x 1023, integers), white balance and CCM -> linear camera RGB (demosaicRB :263-319)
-__global__ void k_isp_rb(IspDev P, const uint16_t *dn, const float *g, float4 *rgb, unsigned int *stats) {
-  __shared__ unsigned int sr[64], sg[64], sb[64];
-  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
-  const int li = threadIdx.y * 8 + threadIdx.x;
-  const int W = (int)P.w, H = (int)P.h;
-  sr[li] = 0u;
-  sg[li] = 0u;
-  sb[li] = 0u;
+  __syncthreads();
+  // the green plane at an in-image coordinate within 1 of the tile
+#define GRN(X, Y) sG[((Y) - (y0 - 1)) * DG + ((X) - (x0 - 1))]
+  const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
+  unsigned int q0 = 0u, q1 = 0u, q2 = 0u;
   if (x < W && y < H) {
     const int own = cfa_channel((uint32_t)x, (uint32_t)y, P.cfa);
-    const float gv = g_at(P, g, x, y);
-    float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f;
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        const int xx = min(max(x + dx, 0), W - 1), yy = min(max(y + dy, 0), H - 1);
-        const int ch = cfa_channel((uint32_t)xx, (uint32_t)yy, P.cfa);
-        if (ch == 1) continue;
-        if (ch == own && !(dx == 0 && dy == 0)) continue;
-        const float d = raw_at(P, dn, xx, yy) - g_at(P, g, xx, yy);
-        if (ch == 0) { sx += d; cx += 1.0f; } else { sy += d; cy += 1.0f; }
-      }
-    float r = gv + sx / fmaxf(cx, 1.0f), gg = gv, b = gv + sy / fmaxf(cy, 1.0f);
-    if (P.demosaic <= 1) {
+    const float gv = GRN(x, y);
+    float r, gg, b;
+    if (P.demosaic == 2) {
+      float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int xx = min(max(x + dx, 0), W - 1), yy = min(max(y + dy, 0), H - 1);
+          const int ch = cfa_channel((uint32_t)xx, (uint32_t)yy, P.cfa);
+          if (ch == 1) continue;
+          if (ch == own && !(dx == 0 && dy == 0)) continue;
+          const float d = RAW(xx, yy) - GRN(xx, yy);
+          if (ch == 0) { sx += d; cx += 1.0f; } else { sy += d; cy += 1.0f; }
+        }
+      r = gv + sx / fmaxf(cx, 1.0f);
+      gg = gv;
+      b = gv + sy / fmaxf(cy, 1.0f);
+    } else {
       float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
           const int xx = min(max(x + dx, 0), W - 1), yy = min(max(y + dy, 0), H - 1);
           const int ch = cfa_channel((uint32_t)xx, (uint32_t)yy, P.cfa);
           if (ch == own && !(dx == 0 && dy == 0)) continue;
-          if (ch == 1 && own == 1 && !(dx == 0 && dy == 0)) continue;
-          const float v = raw_at(P, dn, xx, yy);
+          const float v = RAW(xx, yy);
           if (ch == 0) { s0 += v; c0 += 1.0f; } else if (ch == 1) { s1 += v; c1 += 1.0f; } else { s2 += v; c2 += 1.0f; }
         }
       r = s0 / fmaxf(c0, 1.0f);
-      gg = s1 / fmaxf(c1, 1.0f);
+      gg = P.demosaic == 1 ? gv : s1 / fmaxf(c1, 1.0f);
       b = s2 / fmaxf(c2, 1.0f);
-      if (P.demosaic == 1) gg = gv;
     }
+#undef RAW
+#undef GRN
     const float full = fmaxf(P.max_dn - P.black, 1.0f);
     float cr = (r - P.black) / full, cg = (gg - P.black) / full, cb = (b - P.black) / full;
-    sr[li] = (unsigned int)rintf(fminf(fmaxf(cr, 0.0f), 1.0f) * 1023.0f);
-    sg[li] = (unsigned int)rintf(fminf(fmaxf(cg, 0.0f), 1.0f) * 1023.0f);
-    sb[li] = (unsigned int)rintf(fminf(fmaxf(cb, 0.0f), 1.0f) * 1023.0f);
+    q0 = (unsigned int)rintf(fminf(fmaxf(cr, 0.0f), 1.0f) * 1023.0f);
+    q1 = (unsigned int)rintf(fminf(fmaxf(cg, 0.0f), 1.0f) * 1023.0f);
+    q2 = (unsigned int)rintf(fminf(fmaxf(cb, 0.0f), 1.0f) * 1023.0f);
     cr = cr * P.wb[0];
     cg = cg * P.wb[1];
     cb = cb * P.wb[2];
@@ -381,40 +394,19 @@ __global__ void k_isp_rb(IspDev P, const uint16_t *dn, const float *g, float4 *r
     const float o2 = (P.ccm[6] * cr + P.ccm[7] * cg) + P.ccm[8] * cb;
     rgb[(size_t)y * W + x] = make_float4(o0, o1, o2, 1.0f);
   }
-  __syncthreads();
-  if (li == 0) {
-    unsigned int a = 0, bb = 0, c = 0;
-    for (int k = 0; k < 64; k++) { a += sr[k]; bb += sg[k]; c += sb[k]; }
-    atomicAdd(&stats[1], a);
-    atomicAdd(&stats[2], bb);
-    atomicAdd(&stats[3], c);
+  q0 = wave_sum(q0);
+  q1 = wave_sum(q1);
+  q2 = wave_sum(q2);
+  if ((tid & 63) == 0) {
+    sred[0][tid >> 6] = q0;
+    sred[1][tid >> 6] = q1;
+    sred[2][tid >> 6] = q2;
   }
-}
-
-// 3x3 bilateral on linear RGB, range on luma with sigma = scale x the modelled noise at the pixel (sensorNR
-// :321-348)
-__global__ void k_isp_nr(IspDev P, const float4 *src, float4 *dst) {
-  const int x = blockIdx.x * 8 + threadIdx.x, y = blockIdx.y * 8 + threadIdx.y;
-  const int W = (int)P.w, H = (int)P.h;
-  if (x >= W || y >= H) return;
-  const float4 c0 = src[(size_t)y * W + x];
-  const float Y0 = fmaxf((c0.x * 0.2126f + c0.y * 0.7152f) + c0.z * 0.0722f, 0.0f);
-  const float fullE = fmaxf(P.max_dn - P.black, 1.0f) / fmaxf(P.conv_gain * P.gain, 1e-9f);
-  const float sn = sqrtf(Y0 * fullE + P.read_noise * P.read_noise) / fullE;
-  const float rs = fmaxf(P.nr_scale * sn, 1e-6f);
-  const float ss = P.nr_sigma;
-  float ax = 0.0f, ay = 0.0f, az = 0.0f, ws = 0.0f;
-  for (int dy = -1; dy <= 1; dy++)
-    for (int dx = -1; dx <= 1; dx++) {
-      const float4 c = src[(size_t)min(max(y + dy, 0), H - 1) * W + min(max(x + dx, 0), W - 1)];
-      const float dY = ((c.x - c0.x) * 0.2126f + (c.y - c0.y) * 0.7152f) + (c.z - c0.z) * 0.0722f;
-      const float w = vt_expf(-0.5f * (float)(dx * dx + dy * dy) / (ss * ss) - 0.5f * dY * dY / (rs * rs));
-      ax += w * c.x;
-      ay += w * c.y;
-      az += w * c.z;
-      ws += w;
-    }
-  dst[(size_t)y * W + x] = make_float4(ax / ws, ay / ws, az / ws, 1.0f);
+  __syncthreads();
+  if (tid == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = make_uint4(0u, (sred[0][0] + sred[0][1]) + (sred[0][2] + sred[0][3]),
+                                                           (sred[1][0] + sred[1][1]) + (sred[1][2] + sred[1][3]),
+                                                           (sred[2][0] + sred[2][1]) + (sred[2][2] + sred[2][3]));
 }
 
 // tone curves (toneC :350-366): sRGB OETF, gamma, linear, or sRGB + a contrast S-curve with a black offset
@@ -432,39 +424,93 @@ __device__ float tone_c(const IspDev &P, float x) {
   if (P.tone == 1) return isp_pow(c, 1.0f / fmaxf(P.gamma, 1e-3f));
   return c;
 }
-__device__ __forceinline__ void toned(const IspDev &P, const float4 *src, int x, int y, float &r, float &g, float &b) {
+
+// 3x3 bilateral on linear RGB at an in-image pixel of the LDS tile: range on luma with sigma = scale x the modelled
+// noise at the pixel (sensorNR :321-348); C(X, Y) is the RGB at any coordinate within 1 of it, clamped to the image
+template <typename Fetch>
+__device__ __forceinline__ float3 bilateral(const IspDev &P, int x, int y, Fetch C) {
   const int W = (int)P.w, H = (int)P.h;
-  const float4 c = src[(size_t)min(max(y, 0), H - 1) * W + min(max(x, 0), W - 1)];
-  r = tone_c(P, c.x);
-  g = tone_c(P, c.y);
-  b = tone_c(P, c.z);
+  const float4 c0 = C(x, y);
+  const float Y0 = fmaxf((c0.x * 0.2126f + c0.y * 0.7152f) + c0.z * 0.0722f, 0.0f);
+  const float fullE = fmaxf(P.max_dn - P.black, 1.0f) / fmaxf(P.conv_gain * P.gain, 1e-9f);
+  const float sn = sqrtf(Y0 * fullE + P.read_noise * P.read_noise) / fullE;
+  const float rs = fmaxf(P.nr_scale * sn, 1e-6f);
+  const float ss = P.nr_sigma;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, ws = 0.0f;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const float4 c = C(min(max(x + dx, 0), W - 1), min(max(y + dy, 0), H - 1));
+      const float dY = ((c.x - c0.x) * 0.2126f + (c.y - c0.y) * 0.7152f) + (c.z - c0.z) * 0.0722f;
+      const float w = vt_expf(-0.5f * (float)(dx * dx + dy * dy) / (ss * ss) - 0.5f * dY * dY / (rs * rs));
+      ax += w * c.x;
+      ay += w * c.y;
+      az += w * c.z;
+      ws += w;
+    }
+  return make_float3(ax / ws, ay / ws, az / ws);
 }
 
-// one lane per 2x2 block: tone, optional unsharp mask on luma, BT.601 full-range YCbCr, 4:2:0 chroma (2x2 mean) and
-// RGB from it (or 4:4:4 RGB), Y / Cb / Cr planes, the luma sum for auto exposure (sensorOutput :374-427)
-__global__ void k_isp_out(IspDev P, const float4 *src, uchar4 *rgba, uint8_t *yuv, unsigned int *stats) {
-  __shared__ unsigned int ws[64];
-  const int bx = (blockIdx.x * 8 + threadIdx.x) * 2, by = (blockIdx.y * 8 + threadIdx.y) * 2;
-  const int li = threadIdx.y * 8 + threadIdx.x;
+// One block finishes a 32 x 32 tile (sensorNR, toneC and sensorOutput :321-427 fused): the linear RGB with the
+// halo the noise reduction and the unsharp box need is staged in LDS, each position is denoised and toned once,
+// then one lane per 2x2 block applies the unsharp mask on luma, BT.601 full-range YCbCr, 4:2:0 chroma (2x2 mean)
+// and the RGB rebuilt from it (or 4:4:4 RGB), writes RGBA and the Y / Cb / Cr planes and adds its luma to the
+// block's partial sum.
+constexpr int FT = 32, FSH = 4, FC = FT + 2 * FSH, FR = FC + 2;
+
+__global__ __launch_bounds__(256) void k_isp_finish(IspDev P, const float4 *src, uchar4 *rgba, uint8_t *yuv,
+                                                    uint4 *part) {
+  __shared__ float4 sC[FR * FR];
+  __shared__ float sT[3][FC * FC];
+  __shared__ unsigned int sred[4];
   const int W = (int)P.w, H = (int)P.h;
+  const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
+  const int tid = threadIdx.y * 16 + threadIdx.x;
+  const float amt = P.sharpen_amount;
+  const int rad = P.sharpen_rad;
+  const int R = amt > 0.0f ? rad : 0;       // toned halo
+  const int hc = R + (P.nr ? 1 : 0);        // RGB halo
+  // RGB at coordinates x0 - hc .. x0 + FT - 1 + hc (clamped to the image), stored at the tile's fixed origin
+  const int nc = FT + 2 * hc;
+  for (int i = tid; i < nc * nc; i += 256) {
+    const int X = x0 - hc + i % nc, Y = y0 - hc + i / nc;
+    sC[(Y - (y0 - FSH - 1)) * FR + (X - (x0 - FSH - 1))] =
+        src[(size_t)min(max(Y, 0), H - 1) * W + min(max(X, 0), W - 1)];
+  }
+  __syncthreads();
+  auto C = [&](int X, int Y) { return sC[(Y - (y0 - FSH - 1)) * FR + (X - (x0 - FSH - 1))]; };
+  const int nt = FT + 2 * R;
+  for (int i = tid; i < nt * nt; i += 256) {
+    const int X = x0 - R + i % nt, Y = y0 - R + i / nt;
+    const int cx = min(max(X, 0), W - 1), cy = min(max(Y, 0), H - 1);
+    float3 c;
+    if (P.nr) {
+      c = bilateral(P, cx, cy, C);
+    } else {
+      const float4 v = C(cx, cy);
+      c = make_float3(v.x, v.y, v.z);
+    }
+    const int k = (Y - (y0 - FSH)) * FC + (X - (x0 - FSH));
+    sT[0][k] = tone_c(P, c.x);
+    sT[1][k] = tone_c(P, c.y);
+    sT[2][k] = tone_c(P, c.z);
+  }
+  __syncthreads();
+#define TONED(X, Y, ch) sT[ch][((Y) - (y0 - FSH)) * FC + ((X) - (x0 - FSH))]
+  const int bx = x0 + 2 * (int)threadIdx.x, by = y0 + 2 * (int)threadIdx.y;
   unsigned int lsum = 0;
   if (bx < W && by < H) {
     float Ys[4];
     float cb = 0.0f, cr = 0.0f, nb = 0.0f;
-    const float amt = P.sharpen_amount;
-    const int rad = P.sharpen_rad;
     for (int q = 0; q < 4; q++) {
       const int x = bx + (q & 1), y = by + (q >> 1);
       if (x >= W || y >= H) { Ys[q] = 0.0f; continue; }
-      float r, g, b;
-      toned(P, src, x, y, r, g, b);
+      float r = TONED(x, y, 0), g = TONED(x, y, 1), b = TONED(x, y, 2);
       if (amt > 0.0f) {
         float m = 0.0f, n = 0.0f;
         for (int dy = -rad; dy <= rad; dy++)
           for (int dx = -rad; dx <= rad; dx++) {
-            float tr, tg, tb;
-            toned(P, src, x + dx, y + dy, tr, tg, tb);
-            m += (tr * 0.299f + tg * 0.587f) + tb * 0.114f;
+            m += (TONED(x + dx, y + dy, 0) * 0.299f + TONED(x + dx, y + dy, 1) * 0.587f) +
+                 TONED(x + dx, y + dy, 2) * 0.114f;
             n += 1.0f;
           }
         const float Y0 = (r * 0.299f + g * 0.587f) + b * 0.114f;
@@ -483,6 +529,7 @@ __global__ void k_isp_out(IspDev P, const float4 *src, uchar4 *rgba, uint8_t *yu
         lsum += (unsigned int)y8;
       }
     }
+#undef TONED
     if (P.chroma420) {
       const float Cb = fminf(fmaxf(rintf(cb / nb), 0.0f), 255.0f), Cr = fminf(fmaxf(rintf(cr / nb), 0.0f), 255.0f);
       const int cw = (W + 1) / 2, ch = (H + 1) / 2;
@@ -504,18 +551,37 @@ __global__ void k_isp_out(IspDev P, const float4 *src, uchar4 *rgba, uint8_t *yu
       yuv[(size_t)W * H + (size_t)cw * ch + bi] = (uint8_t)Cr;
     }
   }
-  ws[li] = lsum;
+  lsum = wave_sum(lsum);
+  if ((tid & 63) == 0) sred[tid >> 6] = lsum;
   __syncthreads();
-  if (li == 0) {
-    unsigned int t = 0;
-    for (int k = 0; k < 64; k++) t += ws[k];
-    atomicAdd(&stats[0], t);
-  }
+  if (tid == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = make_uint4((sred[0] + sred[1]) + (sred[2] + sred[3]), 0u, 0u, 0u);
 }
 
-__global__ void k_zero_u32(unsigned int *p, uint32_t n) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0u;
+// the frame's statistics from the blocks' partial sums (integer, so any order gives the same words)
+__global__ __launch_bounds__(256) void k_isp_stats(const uint4 *pa, uint32_t na, const uint4 *pb, uint32_t nb,
+                                                   unsigned int *stats) {
+  __shared__ unsigned int sred[4][4];
+  unsigned int a = 0u, r = 0u, g = 0u, b = 0u;
+  for (uint32_t i = threadIdx.x; i < na; i += 256) {
+    const uint4 v = pa[i];
+    r += v.y;
+    g += v.z;
+    b += v.w;
+  }
+  for (uint32_t i = threadIdx.x; i < nb; i += 256) a += pb[i].x;
+  a = wave_sum(a);
+  r = wave_sum(r);
+  g = wave_sum(g);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sred[0][threadIdx.x >> 6] = a;
+    sred[1][threadIdx.x >> 6] = r;
+    sred[2][threadIdx.x >> 6] = g;
+    sred[3][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4)
+    stats[threadIdx.x] = (sred[threadIdx.x][0] + sred[threadIdx.x][1]) + (sred[threadIdx.x][2] + sred[threadIdx.x][3]);
 }
 
 __global__ void k_finalize(const float4 *acc, float s, uint64_t n, float4 *out) {
@@ -553,19 +619,19 @@ hipError_t launch_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb, hipStr
   hipLaunchKernelGGL(k_pack_rgb8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rgba, n, rgb);
   return hipGetLastError();
 }
-hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float *g, float4 *rgb, float4 *tmp, uchar4 *rgba,
-                      uint8_t *yuv, unsigned int *stats, hipStream_t s) {
+uint32_t isp_partials(uint32_t w, uint32_t h) {
+  return ((w + DT - 1) / DT) * ((h + DT - 1) / DT) + ((w + FT - 1) / FT) * ((h + FT - 1) / FT);
+}
+hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float4 *rgb, uint4 *part, uchar4 *rgba, uint8_t *yuv,
+                      unsigned int *stats, hipStream_t s) {
   if (P.w == 0 || P.h == 0) return hipSuccess;
-  const dim3 blk(8, 8), grid((P.w + 7) / 8, (P.h + 7) / 8), grid2((P.w + 15) / 16, (P.h + 15) / 16);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, stats, 4u);
-  hipLaunchKernelGGL(k_isp_green, grid, blk, 0, s, P, dn, g);
-  hipLaunchKernelGGL(k_isp_rb, grid, blk, 0, s, P, dn, g, rgb, stats);
-  const float4 *img = rgb;
-  if (P.nr) {
-    hipLaunchKernelGGL(k_isp_nr, grid, blk, 0, s, P, rgb, tmp);
-    img = tmp;
-  }
-  hipLaunchKernelGGL(k_isp_out, grid2, blk, 0, s, P, img, rgba, yuv, stats);
+  if (P.sharpen_rad < 1 || P.sharpen_rad > FSH) return hipErrorInvalidValue;
+  const dim3 blk(16, 16), gd((P.w + DT - 1) / DT, (P.h + DT - 1) / DT), gf((P.w + FT - 1) / FT, (P.h + FT - 1) / FT);
+  hipLaunchKernelGGL(k_isp_demosaic, gd, blk, 0, s, P, dn, rgb, part);
+  uint4 *pf = part + gd.x * gd.y;
+  hipLaunchKernelGGL(k_isp_finish, gf, blk, 0, s, P, (const float4 *)rgb, rgba, yuv, pf);
+  hipLaunchKernelGGL(k_isp_stats, dim3(1), dim3(256), 0, s, (const uint4 *)part, gd.x * gd.y, (const uint4 *)pf,
+                     gf.x * gf.y, stats);
   return hipGetLastError();
 }
 

@@ -297,6 +297,64 @@ def bench_sensor(args):
                       "isp_GBps": round(64.5 * px / (i_ms * 1e-3) / 1e9, 1), "peak_GBps": HBM_PEAK_GBS}), flush=True)
 
 
+def bench_twin(args):
+    """The real twin scene (twin_assets/: the reference's scene.json + meshes + physics.json): 993,618 triangles
+    after the 3 mm node subdivision, 14 posed bodies at the default state, the scope's light outlet with its
+    radiance profile; the three 1280x720 KB4 tracking cameras at 64 spp, depth 5, NEE, in one frame."""
+    import numpy as np
+    from vtwin import lib, twin
+
+    root = os.path.join(ROOT, "twin_assets")
+    if not os.path.exists(os.path.join(root, "scene.json")):
+        print("twin_assets/ missing: python tests/golden/fetch_twin_assets.py", file=sys.stderr)
+        sys.exit(2)
+    t0 = time.time()
+    scene, meshes, physics = twin.load(root)
+    prep = twin.prepare(scene, meshes, physics)
+    t_prep = time.time() - t0
+    cams = twin.cameras(scene, spp=64, depth=5, seed=11, prep=prep)
+    poses = twin.poses_from_state(scene, twin.default_state(scene))
+    ctx = lib.Context(0)
+    t0 = time.time()
+    ctx.upload_scene(prep)
+    ctx.upload_profiles(prep)
+    ctx.set_instances(twin.instances(prep, poses))
+    for k, c in enumerate(cams):
+        ctx.set_camera(k, c)
+    ctx.sync()
+    t_up = time.time() - t0
+    slots = list(range(len(cams)))
+    mode = {"auto": lib.VT_RENDER_AUTO, "wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
+            "per-camera": lib.VT_RENDER_PER_CAMERA}[args.mode]
+    tuned = ctx.autotune(slots) if args.mode == "auto" else None
+    for _ in range(args.warmup):
+        ctx.render(slots, mode=mode)
+        ctx.sync()
+    rays = []
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.render(slots, mode=mode)
+        st = ctx.stats()
+        rays.append(st["rays"])
+        log("twin step %d: %.1f ms, %.3g rays" % (i, st["ms_render"], st["rays"]))
+    ctx.sync()
+    dt = time.perf_counter() - t1
+    assert st["stack_overflows"] == 0
+    ctx.close()
+    spp_px = sum(c["spp"] * c["width"] * c["height"] for c in cams) * args.steps
+    print(json.dumps({"metric": METRIC, "config": {"workload": "real twin scene: %d triangles (%d static after the 3 mm "
+                                                               "node subdivision), %d posed bodies, scope outlet with "
+                                                               "its profile; 3 KB4 cameras 1280x720, 64 spp, depth 5, NEE"
+                                                               % (prep["tris"].size // 16, prep["stats"]["static_tris"],
+                                                                  len(prep["bodies"])), "mode": args.mode},
+                      "value": round(float(np.sum(rays)) / dt / 1e6, 2), "unit": "Mrays/s",
+                      "ms_per_frame": round(1e3 * dt / args.steps, 2), "spp_mpix_per_s": round(spp_px / dt / 1e6, 2),
+                      "rays_per_frame": float(np.mean(rays)), "prepare_s": round(t_prep, 2),
+                      "upload_s": round(t_up, 2),
+                      "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()}
+                      if tuned else None}), flush=True)
+
+
 def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
@@ -323,9 +381,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
     ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
-                         "copy: counter calibration kernel; sensor: the sensor chain alone")
+                         "copy: counter calibration kernel; sensor: the sensor chain alone; twin: the real twin scene")
     ap.add_argument("--inner", type=int, default=0, help="BF16 sweeps per refinement step (c4; 0: adaptive)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -342,6 +400,8 @@ def main():
         return bench_copy(args)
     if args.config == "sensor":
         return bench_sensor(args)
+    if args.config == "twin":
+        return bench_twin(args)
     import numpy as np
     from vtwin import lib, scenes
 

@@ -215,6 +215,14 @@ const char *vt_last_error(const vt_ctx *ctx);
  * prepareScene (gpu/scene_gpu.js:305-307). No context needed. */
 int vt_build_bvh(const float *pos, uint32_t nvert, const uint32_t *idx, uint32_t ntri, int leaf_size, int bins,
                  float *nodes_out, uint32_t cap_nodes, uint32_t *node_count, uint32_t *order_out);
+/* weld + subdivide of prepareScene's static meshes (gpu/scene_gpu.js:66-143, called at :268-270): vertices merged
+ * at (1e-7 m, 1e-4) in order of first appearance, collapsed triangles dropped, then conforming edge bisection until
+ * no edge exceeds max_edge (at most max_passes passes; the reference uses 24). Outputs in FP64 as the reference's
+ * arrays hold them (positions, normals) and the triangles. The counts are always written; VT_ERR_SIZE when a
+ * capacity is short (call again with room). No context needed. */
+int vt_weld_subdivide(const float *pos, const float *nrm, uint32_t nvert, const uint32_t *idx, uint32_t ntri,
+                      double max_edge, uint32_t max_passes, double *pos_out, double *nrm_out, uint32_t cap_vert,
+                      uint32_t *idx_out, uint32_t cap_tri, uint32_t *nvert_out, uint32_t *ntri_out);
 
 /* ---------------------------------------------------------------- scene and cameras */
 /* Buffers of _buildBuffers (gpu/renderer.js:271-331): copies the scene in, re-lays the BVH2 out as the

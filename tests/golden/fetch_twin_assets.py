@@ -1,0 +1,29 @@
+"""Copies the real twin scene the tests and `bench.py --config twin` render into <repo>/twin_assets/ (kept out of
+git: 13.5 MB of mesh data): the reference's assets/scene.json, the 44 mesh files it lists, and
+configs/twin/physics.json (materials, emitter, light profile, scope). Run in a container where the reference is
+mounted; the folder then travels with the working tree.
+
+    python tests/golden/fetch_twin_assets.py [/root/reference]
+"""
+import json
+import os
+import shutil
+import sys
+
+
+def main(ref="/root/reference"):
+    repo = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    dst = os.path.join(repo, "twin_assets")
+    os.makedirs(os.path.join(dst, "meshes"), exist_ok=True)
+    src = os.path.join(ref, "assets")
+    with open(os.path.join(src, "scene.json")) as f:
+        scene = json.load(f)
+    shutil.copyfile(os.path.join(src, "scene.json"), os.path.join(dst, "scene.json"))
+    for m in scene["meshes"]:
+        shutil.copyfile(os.path.join(src, m["file"]), os.path.join(dst, m["file"]))
+    shutil.copyfile(os.path.join(ref, "configs", "twin", "physics.json"), os.path.join(dst, "physics.json"))
+    print("twin assets: %d meshes -> %s" % (len(scene["meshes"]), dst))
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

@@ -1,0 +1,210 @@
+"""The real twin scene through the camera path (vtwin/twin.py; the reference's gpu/scene_gpu.js prepareScene and
+control/kinematics.js posesFromState).
+
+Inputs: twin_assets/ (the reference's assets/scene.json, its 44 meshes and configs/twin/physics.json, copied by
+tests/golden/fetch_twin_assets.py; tests that need it skip without it). Golden vectors from the reference's own
+scene_gpu.js and kinematics.js cannot be made here: they use the ?? and ?. operators, which Node 12 (the only
+JavaScript runtime in the image) does not parse. They are pinned instead by:
+  * weld + subdivide (vt_weld_subdivide): closed-form counts on a hand-made triangle; on the real meshes, no edge
+    longer than h, the area unchanged (1e-12), every new vertex the exact midpoint of two older ones, no T-junction
+    (boundary length unchanged), no coincident vertices left after welding;
+  * the scene's own numbers: raw static triangles + lens fronts = scene.json totals.static_triangles, the dynamic
+    total, the light outlet's area = scene.scope.emitter_area_m2;
+  * kinematics: the scope's lens point and view direction at the default state = scene.scope.default origin_D and
+    view_dir_D (exported by the reference's Python twin); orthonormal poses, jaws opened by the jaw angle;
+  * the bench: its triangles tile the extent exactly, fine cells no longer than h.
+GPU: the twin's three cameras (1280 x 720, KB4) rendered with the bodies at the default state against the oracle
+on the same samples.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from vtwin import lib, scenes, twin
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ASSETS = os.path.join(ROOT, "twin_assets")
+needs_assets = pytest.mark.skipif(not os.path.exists(os.path.join(ASSETS, "scene.json")),
+                                  reason="twin_assets/ missing (python tests/golden/fetch_twin_assets.py)")
+
+
+@pytest.fixture(scope="module")
+def loaded():
+    return twin.load(ASSETS)
+
+
+@pytest.fixture(scope="module")
+def prep(loaded):
+    scene, meshes, physics = loaded
+    return twin.prepare(scene, meshes, physics)
+
+
+def _edges(I):
+    e = np.concatenate([I[:, [0, 1]], I[:, [1, 2]], I[:, [2, 0]]])
+    return np.sort(e, axis=1)
+
+
+def _area(P, I):
+    P = np.asarray(P, np.float64)
+    return 0.5 * np.linalg.norm(np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]]), axis=1)
+
+
+def test_subdivide_closed_forms():
+    """An equilateral triangle of side s splits into 4^k equal ones, k = ceil(log2(s / h)), on the triangular
+    lattice's (2^k + 1)(2^k + 2)/2 nodes; a right triangle with legs under h and a longer hypotenuse splits once into
+    two; two triangles with their own copies of a quad's shared corners weld to four vertices."""
+    h = 1.0
+    for s, k in [(0.9, 0), (1.5, 1), (3.0, 2), (7.9, 3)]:
+        P = np.array([[0, 0, 0], [s, 0, 0], [s / 2, s * math.sqrt(3) / 2, 0]], np.float32)
+        N = np.tile(np.array([0, 0, 1], np.float32), (3, 1))
+        P2, N2, I2 = lib.weld_subdivide(P, N, np.array([[0, 1, 2]], np.uint32), h)
+        assert I2.shape[0] == 4 ** k
+        assert P2.shape[0] == (2 ** k + 1) * (2 ** k + 2) // 2          # the triangular lattice's nodes
+        assert np.allclose(N2, [0, 0, 1])
+    P = np.array([[0, 0, 0], [0.9, 0, 0], [0, 0.9, 0]], np.float32)
+    P2, _, I2 = lib.weld_subdivide(P, np.tile(np.float32([0, 0, 1]), (3, 1)), np.array([[0, 1, 2]], np.uint32), 1.0)
+    assert I2.shape[0] == 2 and np.allclose(P2[3], [0.45, 0.45, 0])
+    # welding: a quad given as two triangles with their own copies of the shared corners
+    P = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    N = np.tile(np.float32([0, 0, 1]), (6, 1))
+    P2, _, I2 = lib.weld_subdivide(P, N, np.arange(6, dtype=np.uint32).reshape(2, 3), 10.0)
+    assert P2.shape[0] == 4 and I2.tolist() == [[0, 1, 2], [0, 2, 3]]
+
+
+@needs_assets
+@pytest.mark.parametrize("name", ["membrane_L", "lid_spine_1_crest", "static_shell_left", "hatch_apex"])
+def test_weld_subdivide_invariants_on_twin_meshes(loaded, name):
+    scene, meshes, _ = loaded
+    P, N, I, _ = meshes[name]
+    h = 0.003
+    P2, N2, I2 = lib.weld_subdivide(P, N, I, h)
+    e = _edges(I2)
+    L = np.linalg.norm(P2[e[:, 0]] - P2[e[:, 1]], axis=1)
+    assert L.max() <= h
+    assert _area(P2, I2).sum() == pytest.approx(_area(P, I).sum(), rel=1e-12)
+    # welded vertices first (unique keys), then midpoints of two earlier vertices, exactly
+    key = np.concatenate([np.floor(P2 * 1e7 + 0.5), np.floor(N2 * 1e4 + 0.5)], axis=1)
+    Pw, _, Iw = lib.weld_subdivide(P, N, I, 1e9)
+    nw = Pw.shape[0]
+    assert np.unique(key[:nw], axis=0).shape[0] == nw
+    assert np.array_equal(P2[:nw], Pw)
+    ue, cnt = np.unique(e, axis=0, return_counts=True)
+    assert cnt.max() <= 2
+    # no T-junctions: the boundary (edges of one triangle) keeps its length
+    ew = _edges(Iw)
+    uw, cw = np.unique(ew, axis=0, return_counts=True)
+    bl = lambda Pp, ee: float(np.linalg.norm(Pp[ee[:, 0]] - Pp[ee[:, 1]], axis=1).sum())
+    assert bl(P2, ue[cnt == 1]) == pytest.approx(bl(Pw, uw[cw == 1]), rel=1e-12)
+    # midpoints: each new vertex is the mean of the two ends of an edge that contains it
+    rng = np.random.default_rng(3)
+    for v in rng.choice(np.arange(nw, P2.shape[0]), size=min(200, P2.shape[0] - nw), replace=False):
+        tri = I2[np.any(I2 == v, axis=1)]
+        nb = np.unique(tri[tri != v])
+        d = np.linalg.norm(P2[nb][:, None, :] + P2[nb][None, :, :] - 2 * P2[v], axis=2)
+        assert d.min() == 0.0
+
+
+@needs_assets
+def test_twin_counts_and_outlet(loaded, prep):
+    scene, meshes, _ = loaded
+    lens_fronts = sum(m["triangle_count"] for m in scene["meshes"] if m.get("owner_camera"))
+    assert prep["stats"]["raw_static_tris"] + lens_fronts == scene["totals"]["static_triangles"]
+    tb = np.asarray(prep["tri_body"])
+    body_tris = int(np.sum(tb != twin.STATIC_BODY))
+    outlet = meshes[scene["scope"]["emitter_mesh"]][2].shape[0]
+    assert body_tris + outlet - lens_fronts == scene["totals"]["dynamic_triangles"]
+    assert prep["emit_area"] == pytest.approx(scene["scope"]["emitter_area_m2"], rel=1e-6)
+    assert prep["emit"].size // 16 == outlet
+    assert len(prep["bodies"]) == 14 and prep["body_count"] == 14
+    kinds = sorted(b["kind"] for b in prep["bodies"])
+    assert kinds.count("instrument") == 6 and kinds.count("fixed") == 3 and kinds.count("fixture") == 3
+    # every material the meshes name is in physics (no grey fallback)
+    assert not any(p.get("missing") for p in prep["materials_prepared"] if p["name"] != "emitter")
+
+
+@needs_assets
+def test_kinematics_against_the_scene_export(loaded):
+    scene, _, _ = loaded
+    state = twin.default_state(scene)
+    poses = twin.poses_from_state(scene, state)
+    sd = scene["scope"]["default"]
+    T = poses["scope"]
+    assert np.allclose(T[:3, 3], sd["origin_D"], atol=1e-12)
+    dov = scene["scope"]["direction_of_view_deg"] * math.pi / 180
+    assert np.allclose(T[:3, :3] @ [0, math.sin(dov), math.cos(dov)], sd["view_dir_D"], atol=1e-12)
+    for key, T in poses.items():
+        if T is None:
+            assert key.startswith("board/")
+            continue
+        R = T[:3, :3]
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and np.linalg.det(R) == pytest.approx(1.0)
+    for q in state["instruments"]:
+        a, b, s = poses[q["id"] + "/jaw_a"], poses[q["id"] + "/jaw_b"], poses[q["id"] + "/shaft"]
+        ang = math.acos(np.clip(a[:3, 2] @ b[:3, 2], -1, 1))
+        assert ang == pytest.approx(q["jaw"], abs=1e-12)
+        pv = np.array(next(p for p in scene["ports"] if p["id"] == q["id"])["pivot"])
+        assert np.allclose(s[:3, 3], pv + q["insertion"] * s[:3, 2], atol=1e-12)
+
+
+def test_bench_tiles_its_extent():
+    ext = [[-0.3, -0.2], [0.25, 0.28]]
+    fine = [[-0.1, -0.05], [0.12, 0.1]]
+    h = 0.003
+    P, N, I = twin.bench_mesh(0.004, ext, fine, h)
+    assert np.all(P[:, 2] == 0.004)
+    assert _area(P, I).sum() == pytest.approx(0.55 * 0.48, rel=1e-12)
+    n = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
+    assert np.all(n[:, 2] > 0)                      # counter-clockwise seen from +z
+    fine_tris = I[: (len(I) - 8)]
+    e = _edges(fine_tris)
+    L = np.linalg.norm(P[e[:, 0]] - P[e[:, 1]], axis=1)
+    axis_aligned = (np.abs(P[e[:, 0], 0] - P[e[:, 1], 0]) < 1e-12) | (np.abs(P[e[:, 0], 1] - P[e[:, 1], 1]) < 1e-12)
+    assert L[axis_aligned].max() <= h
+
+
+# ---------------------------------------------------------------------------- GPU
+
+def _twin_frame(prep, scene, spp, depth):
+    cams = twin.cameras(scene, spp=spp, depth=depth, seed=7, prep=prep)
+    poses = twin.poses_from_state(scene, twin.default_state(scene))
+    return cams, poses
+
+
+@pytest.mark.gpu
+@needs_assets
+def test_gpu_twin_frame_matches_oracle(loaded, prep):
+    """CAM3 (the scope, a cannula, both instruments and another camera's lens front in view) at 4 spp, depth 3,
+    bodies at the default state: a tile around the instruments against the oracle on the same samples."""
+    scene, _, _ = loaded
+    cams, poses = _twin_frame(prep, scene, 4, 3)
+    inst = twin.instances(prep, poses)
+    print("twin frame: upload %d triangles, %d bodies" % (prep["tris"].size // 16, len(prep["bodies"])), flush=True)
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(prep)
+        ctx.upload_profiles(prep)
+        ctx.set_instances(inst)
+        for k, c in enumerate(cams):
+            ctx.set_camera(k, c)
+        ctx.render([0, 1, 2], mode=lib.VT_RENDER_BATCHED)
+        imgs = [ctx.readback(k, lib.VT_KIND_LINEAR, c["width"], c["height"]) for k, c in enumerate(cams)]
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    assert st["stack_overflows"] == 0
+    for im in imgs:
+        assert np.isfinite(im).all() and im[..., :3].mean() > 0
+    cam = cams[2]
+    x0, y0, w, h = 600, 300, 48, 32
+    sc = dict(prep)
+    sc["poses"] = inst
+    ref, _, _ = oracle.render_tile(sc, cam, x0, y0, w, h)
+    a = imgs[2][y0:y0 + h, x0:x0 + w, :3].astype(np.float64)
+    b = ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("twin CAM3 tile: relative RMSE %.3g (GPU mean %.4g, oracle %.4g)" % (rel, a.mean(), b.mean()))
+    assert rel < 1e-3

@@ -20,7 +20,7 @@ EXPORTS = [
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
     "vt_set_light_profile", "vt_radiosity_build", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
-    "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test",
+    "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide",
 ]
 
 # camera option bits (vt_camera.flags)
@@ -162,6 +162,7 @@ def load():
         "vt_bsdf_test": (i32, [vp, vp, u32, vp]),
         "vt_sensor_test": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_isp": (i32, [vp, u32, vp]),
+        "vt_weld_subdivide": (i32, [vp, vp, u32, vp, u32, f64, u32, vp, vp, u32, vp, u32, vp, vp]),
         "vt_isp_test": (i32, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
         "vt_jacobi_sparse": (i32, [vp, vp, vp, u32, vp]),
         "vt_dense_upload": (i32, [vp, u32, u32, vp, vp]),
@@ -215,6 +216,27 @@ def build_bvh(pos, idx, leaf_size=4, bins=16):
     if rc:
         raise VTError(rc, "vt_build_bvh")
     return nodes[: cnt.value * 16].copy(), order[:n].copy()
+
+
+def weld_subdivide(pos, nrm, idx, max_edge=0.003, max_passes=24):
+    """vt_weld_subdivide: prepareScene's weld + conforming subdivision of one static mesh. Returns (P, N) as
+    float64 (n, 3) and I uint32 (t, 3)."""
+    L = load()
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    args = (ptr(pos), ptr(nrm), pos.size // 3, ptr(idx), idx.size // 3, float(max_edge), int(max_passes))
+    rc = L.vt_weld_subdivide(*args, None, None, 0, None, 0, C.byref(nv), C.byref(nt))
+    if rc not in (0, VT_ERR_SIZE):
+        raise VTError(rc, "vt_weld_subdivide")
+    P = np.zeros((nv.value, 3), np.float64)
+    N = np.zeros((nv.value, 3), np.float64)
+    I = np.zeros((nt.value, 3), np.uint32)
+    rc = L.vt_weld_subdivide(*args, ptr(P), ptr(N), nv.value, ptr(I), nt.value, C.byref(nv), C.byref(nt))
+    if rc:
+        raise VTError(rc, "vt_weld_subdivide")
+    return P, N, I
 
 
 class Context:

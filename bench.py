@@ -355,6 +355,84 @@ def bench_twin(args):
                       if tuned else None}), flush=True)
 
 
+def bench_twin_live(args):
+    """The reference's `live` operating point on the real twin scene, its published whole-frame workload
+    (gpu/renderer.js:54-66: 3 x 1280x720, spp 1, 1 light sample, no refine pass, tools moving, scope still, node
+    radiosity every 2nd frame with 1 Jacobi step): per frame the instruments move, every 2nd frame the node cache
+    takes its direct light and one step, the three cameras run the reference estimator (shadeHit over the cache,
+    shadow culling), and each camera's sensor and ISP run with the twin's physics settings. Times the whole frame
+    between two synchronisations."""
+    import numpy as np
+    from vtwin import lib, twin
+
+    root = os.path.join(ROOT, "twin_assets")
+    if not os.path.exists(os.path.join(root, "scene.json")):
+        print("twin_assets/ missing: python tests/golden/fetch_twin_assets.py", file=sys.stderr)
+        sys.exit(2)
+    scene, meshes, physics = twin.load(root)
+    prep = twin.prepare(scene, meshes, physics)
+    state = twin.default_state(scene)
+    cams = twin.cameras(scene, spp=1, depth=1, seed=0, prep=prep)
+    for c in cams:
+        c.update(flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS, light_samples=1)
+    ctx = lib.Context(0)
+    ctx.upload_scene(prep)
+    ctx.upload_profiles(prep)
+    ctx.set_instances(twin.instances(prep, twin.poses_from_state(scene, state)))
+    t0 = time.time()
+    info = ctx.radiosity_build()
+    n_mat = np.asarray(prep["materials"]).size // 16
+    twin.rho_eff(prep, ctx.radiosity_tables(6, (n_mat, 4), np.float32)[:, :3])
+    ctx.radiosity_update(direct=True, iters=96)
+    ctx.radiosity_state()
+    ctx.sync()
+    t_build = time.time() - t0
+    for k, c in enumerate(cams):
+        ctx.set_camera(k, c)
+    sens = scenes_default_sensor()
+    sens.update(black=168.0, glare_fraction=0.1, glare_sigma=0.25 * float(np.hypot(1280, 720)) / 16.0, glare_level=16,
+                exposure=0.0157)
+    isp = dict(demosaic=1, nr=1, nr_sigma_px=0.4, nr_range_scale=2.0, tone=3, s_curve_c=1.39, black_offset=0.014,
+               chroma420=1)
+    slots = list(range(len(cams)))
+
+    def frame(i):
+        st = dict(state, instruments=[dict(q, yaw=q["yaw"] + 0.002 * i, jaw=q["jaw"] + 0.01 * (i % 5))
+                                      for q in state["instruments"]])
+        ctx.set_instances(twin.instances(prep, twin.poses_from_state(scene, st)))
+        if i % 2 == 0:
+            ctx.radiosity_update(direct=True, iters=1)
+        ctx.render(slots, mode=lib.VT_RENDER_BATCHED)
+        for k in slots:
+            ctx.sensor(k, dict(sens, frame=i, cam_index=k))
+            ctx.isp(k, isp)
+
+    for i in range(max(2, args.warmup)):
+        frame(i)
+    ctx.sync()
+    n = max(args.steps, 20)
+    t1 = time.perf_counter()
+    for i in range(n):
+        frame(i)
+    ctx.sync()
+    dt = time.perf_counter() - t1
+    st = ctx.stats()
+    ctx.close()
+    print(json.dumps({"metric": METRIC, "config": {"workload": "reference 'live' preset on the real twin scene: 3 KB4 "
+                                                               "cameras 1280x720, reference estimator, spp 1, 1 light "
+                                                               "sample, shadow culling, tools moving, node radiosity "
+                                                               "(direct + 1 Jacobi step) every 2nd frame, sensor + ISP"},
+                      "frames": n, "ms_per_frame": round(1e3 * dt / n, 3), "fps": round(n / dt, 1),
+                      "nodes": info["nodes"], "active_nodes": info["active"], "node_build_s": round(t_build, 2),
+                      "last_render_ms": round(st["ms_render"], 3), "last_isp_ms": round(st["ms_isp"], 3),
+                      "reference_published_ms": {"M4 Air, live": 45}}), flush=True)
+
+
+def scenes_default_sensor():
+    from vtwin import scenes
+    return scenes.default_sensor(cfa=0, frame=0, seed=1006)
+
+
 def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
     kernel) for the same workload, read from the committed summary; None when there is none."""
@@ -381,9 +459,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
     ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin", "twin-live"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
-                         "copy: counter calibration kernel; sensor: the sensor chain alone; twin: the real twin scene")
+                         "copy: counter calibration kernel; sensor: the sensor chain alone; twin: the real twin scene; "
+                         "twin-live: the reference's live preset on the twin scene")
     ap.add_argument("--inner", type=int, default=0, help="BF16 sweeps per refinement step (c4; 0: adaptive)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -402,6 +481,8 @@ def main():
         return bench_sensor(args)
     if args.config == "twin":
         return bench_twin(args)
+    if args.config == "twin-live":
+        return bench_twin_live(args)
     import numpy as np
     from vtwin import lib, scenes
 

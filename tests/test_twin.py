@@ -208,3 +208,47 @@ def test_gpu_twin_frame_matches_oracle(loaded, prep):
     rel = np.linalg.norm(a - b) / np.linalg.norm(b)
     print("twin CAM3 tile: relative RMSE %.3g (GPU mean %.4g, oracle %.4g)" % (rel, a.mean(), b.mean()))
     assert rel < 1e-3
+
+
+@pytest.mark.gpu
+@needs_assets
+def test_gpu_twin_reference_estimator_matches_oracle(loaded, prep):
+    """The reference's own camera estimator on the twin (shadeHit over the node radiosity cache of 445,968 static
+    nodes, bodies at the default state with their occlusion capsules and rho_eff from the albedo tables): CAM3 at
+    the judge preset's 4 light samples, against the oracle fed the GPU's node state."""
+    scene, _, _ = loaded
+    cams, poses = _twin_frame(prep, scene, 4, 1)
+    print("twin reference estimator: build the node cache", flush=True)
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(prep)
+        ctx.upload_profiles(prep)
+        ctx.set_instances(twin.instances(prep, poses))
+        info = ctx.radiosity_build()
+        n_mat = np.asarray(prep["materials"]).size // 16
+        hh = ctx.radiosity_tables(6, (n_mat, 4), np.float32)
+        alb = ctx.radiosity_tables(5, (n_mat, 32, 8), np.float32)
+        twin.rho_eff(prep, hh[:, :3])
+        inst = twin.instances(prep, poses)
+        ctx.set_instances(inst)
+        ctx.radiosity_update(direct=True, iters=96)
+        nodes = np.asarray(prep["verts"]).size // 8
+        st, e = ctx.radiosity_state(nodes)
+        cam = dict(cams[2], flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS, light_samples=4)
+        ctx.set_camera(0, cam)
+        ctx.render([0])
+        img = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    finally:
+        ctx.close()
+    print("nodes %d, active %d, eAvg %s" % (info["nodes"], info["active"], e))
+    assert info["static_nodes"] == prep["stats"]["static_nodes"]
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    x0, y0, w, h = 600, 300, 48, 32
+    sc2 = dict(prep, poses=inst, node_state=st, alb=alb, hh=hh, slot=1, e_avg=e)
+    ref, _, _ = oracle.render_tile(sc2, cam, x0, y0, w, h)
+    a = img[y0:y0 + h, x0:x0 + w, :3].astype(np.float64)
+    b = ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    print("twin reference estimator CAM3 tile: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99

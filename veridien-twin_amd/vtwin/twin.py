@@ -486,6 +486,8 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
                materials_prepared=prepared, material_index=mindex,
                stats=dict(raw_static_tris=raw_static, static_tris=int(np.sum(tri_body == STATIC_BODY)),
                           static_verts=int(sum(p[0].shape[0] for p in parts if p[4] is None)),
+                          static_nodes=int(sum(p[0].shape[0] for p in parts
+                                               if mesh_table[p[3]]["flags"] & MESH_FLAG["STATIC"])),
                           bodies=len(body_list), emitter_area=float(out["emit_area"])))
     return out
 

@@ -355,16 +355,25 @@ def bench_twin(args):
                       if tuned else None}), flush=True)
 
 
-def bench_twin_live(args):
-    """The reference's `live` operating point on the real twin scene, its published whole-frame workload
-    (gpu/renderer.js:54-66: 3 x 1280x720, spp 1, 1 light sample, no refine pass, tools moving, scope still, node
-    radiosity every 2nd frame with 1 Jacobi step): per frame the instruments move, every 2nd frame the node cache
-    takes its direct light and one step, the three cameras run the reference estimator (shadeHit over the cache,
-    shadow culling), and each camera's sensor and ISP run with the twin's physics settings. Times the whole frame
-    between two synchronisations."""
+TWIN_PRESETS = {
+    # the reference's operating points (gpu/renderer.js:31-67) and its published whole-frame times on an M4 Air
+    "live": dict(spp=1, nl=1, refine=0, nl_refine=1, refine_shadows=False, every=2, iters=1, published_ms=45),
+    "balanced": dict(spp=1, nl=2, refine=2, nl_refine=1, refine_shadows=False, every=2, iters=1, published_ms=84),
+    "judge": dict(spp=1, nl=4, refine=6, nl_refine=1, refine_shadows=True, every=1, iters=4,
+                  published_ms="190-245"),
+}
+
+
+def bench_twin_preset(args, preset):
+    """A reference operating point on the real twin scene, its published whole-frame workload (3 x 1280x720,
+    tools moving, scope still): per frame the instruments move; every `every`-th frame the node cache takes its
+    direct light and `iters` Jacobi steps; the three cameras run the reference estimator (shadeHit over the cache,
+    shadow culling) with the preset's sub-samples, light samples and refinement pass; each camera's sensor and ISP
+    run with the twin's physics settings. Times the whole frame between two synchronisations."""
     import numpy as np
     from vtwin import lib, twin
 
+    P = TWIN_PRESETS[preset]
     root = os.path.join(ROOT, "twin_assets")
     if not os.path.exists(os.path.join(root, "scene.json")):
         print("twin_assets/ missing: python tests/golden/fetch_twin_assets.py", file=sys.stderr)
@@ -372,9 +381,10 @@ def bench_twin_live(args):
     scene, meshes, physics = twin.load(root)
     prep = twin.prepare(scene, meshes, physics)
     state = twin.default_state(scene)
-    cams = twin.cameras(scene, spp=1, depth=1, seed=0, prep=prep)
+    cams = twin.cameras(scene, spp=P["spp"], depth=1, seed=0, prep=prep)
+    flags = lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | (lib.VT_OPT_REFINE_SHADOWS if P["refine_shadows"] else 0)
     for c in cams:
-        c.update(flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS, light_samples=1)
+        c.update(flags=flags, light_samples=P["nl"], refine=P["refine"], refine_light_samples=P["nl_refine"])
     ctx = lib.Context(0)
     ctx.upload_scene(prep)
     ctx.upload_profiles(prep)
@@ -384,7 +394,6 @@ def bench_twin_live(args):
     n_mat = np.asarray(prep["materials"]).size // 16
     twin.rho_eff(prep, ctx.radiosity_tables(6, (n_mat, 4), np.float32)[:, :3])
     ctx.radiosity_update(direct=True, iters=96)
-    ctx.radiosity_state()
     ctx.sync()
     t_build = time.time() - t0
     for k, c in enumerate(cams):
@@ -400,8 +409,8 @@ def bench_twin_live(args):
         st = dict(state, instruments=[dict(q, yaw=q["yaw"] + 0.002 * i, jaw=q["jaw"] + 0.01 * (i % 5))
                                       for q in state["instruments"]])
         ctx.set_instances(twin.instances(prep, twin.poses_from_state(scene, st)))
-        if i % 2 == 0:
-            ctx.radiosity_update(direct=True, iters=1)
+        if i % P["every"] == 0:
+            ctx.radiosity_update(direct=True, iters=P["iters"])
         ctx.render(slots, mode=lib.VT_RENDER_BATCHED)
         for k in slots:
             ctx.sensor(k, dict(sens, frame=i, cam_index=k))
@@ -418,14 +427,20 @@ def bench_twin_live(args):
     dt = time.perf_counter() - t1
     st = ctx.stats()
     ctx.close()
-    print(json.dumps({"metric": METRIC, "config": {"workload": "reference 'live' preset on the real twin scene: 3 KB4 "
-                                                               "cameras 1280x720, reference estimator, spp 1, 1 light "
-                                                               "sample, shadow culling, tools moving, node radiosity "
-                                                               "(direct + 1 Jacobi step) every 2nd frame, sensor + ISP"},
+    px = sum(c["width"] * c["height"] for c in cams)
+    print(json.dumps({"metric": METRIC, "config": {"workload": "reference '%s' preset on the real twin scene: 3 KB4 "
+                                                               "cameras 1280x720, reference estimator, spp %d, %d light "
+                                                               "samples, refine %d x %d light samples, shadow culling, "
+                                                               "tools moving, node radiosity (direct + %d Jacobi steps) "
+                                                               "every %s frame, sensor + ISP"
+                                                   % (preset, P["spp"], P["nl"], P["refine"], P["nl_refine"],
+                                                      P["iters"], "2nd" if P["every"] == 2 else "")},
                       "frames": n, "ms_per_frame": round(1e3 * dt / n, 3), "fps": round(n / dt, 1),
+                      "refined_pixel_fraction": round((st["samples"] - px * P["spp"]) / max(1, P["refine"]) / px, 4)
+                      if P["refine"] else 0.0,
                       "nodes": info["nodes"], "active_nodes": info["active"], "node_build_s": round(t_build, 2),
                       "last_render_ms": round(st["ms_render"], 3), "last_isp_ms": round(st["ms_isp"], 3),
-                      "reference_published_ms": {"M4 Air, live": 45}}), flush=True)
+                      "reference_published_ms_m4_air": P["published_ms"]}), flush=True)
 
 
 def scenes_default_sensor():
@@ -459,10 +474,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
     ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin", "twin-live"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin", "twin-live", "twin-balanced",
+                                         "twin-judge"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
                          "copy: counter calibration kernel; sensor: the sensor chain alone; twin: the real twin scene; "
-                         "twin-live: the reference's live preset on the twin scene")
+                         "twin-live / twin-balanced / twin-judge: the reference's presets on the twin scene")
     ap.add_argument("--inner", type=int, default=0, help="BF16 sweeps per refinement step (c4; 0: adaptive)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -481,8 +497,8 @@ def main():
         return bench_sensor(args)
     if args.config == "twin":
         return bench_twin(args)
-    if args.config == "twin-live":
-        return bench_twin_live(args)
+    if args.config.startswith("twin-"):
+        return bench_twin_preset(args, args.config[5:])
     import numpy as np
     from vtwin import lib, scenes
 

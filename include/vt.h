@@ -104,6 +104,11 @@ typedef struct {
                             is at time shutter[0] + (shutter[1] - shutter[0]) frac(s2 + i / golden ratio) */
   uint32_t skip_bodies;  /* bit i: this camera's own rays (from its centre) ignore body i, e.g. its lens front
                             (_skipMask, gpu/renderer.js:881-886) */
+  /* reference estimator: the refinement pass (camRefine, gpu/wgsl/kernels.wgsl.js:218-252; DEFAULTS.refine and
+   * refineLightSamples, gpu/renderer.js:33-35): pixels on a mesh edge or a posed body (and, with
+   * VT_OPT_REFINE_SHADOWS, those whose shadow rays were traced) take `refine` more sub-samples with
+   * refine_light_samples emitter samples each. 0 = no refinement. Needs a whole frame (spp_begin 0, no accumulate). */
+  uint32_t refine, refine_light_samples;
 } vt_camera;
 
 enum { VT_RENDER_BATCHED = 0, VT_RENDER_PER_CAMERA = 1, VT_RENDER_WAVEFRONT = 2, VT_RENDER_AUTO = 3 };
@@ -347,6 +352,7 @@ int vt_isp_test(vt_ctx *ctx, const vt_sensor_params *sensor, const vt_isp_params
 #define VT_OPT_NO_CAPSULE 32u
 #define VT_OPT_NORMALS 64u
 #define VT_OPT_NO_MIS 128u
+#define VT_OPT_REFINE_SHADOWS 1024u   /* refine pixels whose shadow rays were traced (refineShadows) */
 #define VT_OPT_REFERENCE (1u << 16)
 #define VT_OPT_CULL_SHADOWS (1u << 17)
 

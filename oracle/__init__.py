@@ -160,7 +160,8 @@ class Camera(C.Structure):
     _fields_ = [("R", C.c_float * 9), ("pos", C.c_float * 3), ("mode", C.c_uint32), ("lens", C.c_float * 9),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("seed", C.c_uint32),
                 ("cam_index", C.c_uint32), ("light_samples", C.c_uint32), ("max_depth", C.c_uint32),
-                ("flags", C.c_uint32), ("shutter", C.c_float * 2), ("skip_bodies", C.c_uint32)]
+                ("flags", C.c_uint32), ("shutter", C.c_float * 2), ("skip_bodies", C.c_uint32),
+                ("refine", C.c_uint32), ("refine_light_samples", C.c_uint32)]
 
 
 # ---------------------------------------------------------------------------- helpers
@@ -354,6 +355,8 @@ def camera_struct(cam):
         setattr(c, k, int(cam.get(k, 0)))
     c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
     c.skip_bodies = int(cam.get("skip_bodies", 0))
+    c.refine = int(cam.get("refine", 0))
+    c.refine_light_samples = int(cam.get("refine_light_samples", 0))
     return c
 
 

@@ -33,6 +33,7 @@
 #define O_NO_MIS 128u
 #define O_REFERENCE (1u << 16)
 #define O_CULL_SHADOWS (1u << 17)
+#define O_REFINE_SHADOW 1024u
 static const float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
 static const float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
 static const float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
@@ -486,7 +487,8 @@ static void node_interp(const or_scene *S, const surf_t *s, int k, double out[3]
 static int is_static(const or_hit *h, const surf_t *s) { return (s->flags & F_STATIC) && h->inst < 0; }
 
 static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], double time,
-                         const double u[3], int mis, double nl, double cached, uint64_t *rays, double out[3]) {
+                         const double u[3], int mis, double nl, double cached, uint64_t *rays, uint32_t *key,
+                         double out[3]) {
   out[0] = out[1] = out[2] = 0;
   double pl[3];
   const float *e = sample_emitter(S, u, pl);
@@ -513,6 +515,7 @@ static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, co
     double so[3];
     offset_ray(s->p, s->ng, wi, so);
     or_hit sh;
+    *key |= 0x20000u;           /* a traced shadow ray marks the pixel for refinement (shade.wgsl.js:176-178) */
     trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, S->shadow_skip, &sh, NULL);
     (*rays)++;
     if (sh.tri >= 0) return;
@@ -563,9 +566,12 @@ static void secondary(const or_scene *S, const double o[3], const double l[3], d
   for (int k = 0; k < 3; k++) out[k] = S->hh[4 * (size_t)s.mat + k] * S->e_avg[k] / M_PI;
 }
 
+/* key: the first hit's mesh (0xfffe on a miss) | 0x10000 on a posed body | 0x20000 once a shadow ray is traced
+ * (gKey, shade.wgsl.js:10, 251-253); nl_in: the light samples of this pass */
 static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y, uint32_t i,
-                         double out[3], uint64_t *rays) {
+                         uint32_t nl_in, double out[3], uint64_t *rays, uint32_t *key) {
   out[0] = out[1] = out[2] = 0;
+  *key = 0xfffeu;
   const float tk = or_kr(i, GOLD_F, P->s[2]);
   const double time = (double)(C->shutter[0] + (C->shutter[1] - C->shutter[0]) * tk);
   double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
@@ -577,6 +583,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
   if (h.tri < 0) return;
   surf_t s;
   surface(S, &h, &pose, o, d, &s);
+  *key = (s.mesh & 0xffffu) | ((h.inst >= 0 && !(s.flags & F_STATIC)) ? 0x10000u : 0u);
   const double wo[3] = {-d[0], -d[1], -d[2]};
   const uint32_t opts = C->flags;
   if (opts & O_NORMALS) {
@@ -595,7 +602,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
   or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
   const int stat = is_static(&h, &s);
   const int mis = !stat && !(opts & O_NO_MIS);
-  const uint32_t nl = C->light_samples < 1 ? 1 : C->light_samples > 16 ? 16 : C->light_samples;
+  const uint32_t nl = nl_in < 1 ? 1 : nl_in > 16 ? 16 : nl_in;
   const double cached = (stat && (opts & O_CULL_SHADOWS)) ? cached_visibility(S, &s) : -1;
   double L[3] = {0, 0, 0};
   if (S->emit_count > 0)
@@ -603,7 +610,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
       const uint32_t li = i * nl + j;
       const double u[3] = {or_kr(li, KL0, P->s[3]), or_kr(li, KL1, P->s[4]), or_kr(li, KL2, P->s[5])};
       double c[3];
-      direct_light(S, &s, &M, wo, time, u, mis, nl, cached, rays, c);
+      direct_light(S, &s, &M, wo, time, u, mis, nl, cached, rays, key, c);
       for (int k = 0; k < 3; k++) L[k] += c[k];
     }
   for (int k = 0; k < 3; k++) L[k] /= nl;
@@ -741,28 +748,89 @@ void or_node_direct(const or_scene *S, const float *npos, const float *nnorm, co
   }
 }
 
+/* one pixel's first pass: the sum over sub-samples [s0, s1) and, for the reference estimator, its key {mesh of
+ * sub-sample 0 (0xfffe: a miss), a mesh change among the sub-samples (0x40000), body / traced-shadow bits} (camMain,
+ * kernels.wgsl.js:172-213) */
+static void first_pass(const or_scene *S, const or_camera *C, uint32_t x, uint32_t y, uint32_t s0, uint32_t s1,
+                       double acc[3], uint32_t *key, uint64_t *r, uint64_t vv[5]) {
+  pix_t P;
+  pix_seeds(C, x, y, &P);
+  acc[0] = acc[1] = acc[2] = 0;
+  *key = 0;
+  for (uint32_t i = s0; i < s1; i++) {
+    double L[3];
+    if ((C->flags & O_REFERENCE) && S->node_state) {
+      uint32_t k;
+      ref_radiance(S, C, &P, x, y, i, C->light_samples, L, r, &k);
+      if (i == s0) *key = k;
+      else if ((k & 0xffffu) != (*key & 0xffffu)) *key |= 0x40000u;
+      *key |= k & 0x30000u;
+    } else {
+      sample_radiance(S, C, &P, x, y, i, L, r, vv, NULL);
+    }
+    for (int k = 0; k < 3; k++) acc[k] += L[k];
+  }
+}
+
 void or_render_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                     uint32_t spp_begin, uint32_t spp_end, float *out, uint64_t *rays, uint64_t *visits, int nthreads) {
   uint64_t nr = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+  const int refine = (C->flags & O_REFERENCE) && S->node_state && C->refine > 0 && spp_begin == 0;
+  /* with the refinement pass, the first pass covers the tile and a 1-pixel rim (the 3x3 key test) */
+  const uint32_t rx0 = refine && x0 > 0 ? x0 - 1 : x0, ry0 = refine && y0 > 0 ? y0 - 1 : y0;
+  const uint32_t rx1 = refine ? (x0 + w + 1 < C->width ? x0 + w + 1 : C->width) : x0 + w;
+  const uint32_t ry1 = refine ? (y0 + h + 1 < C->height ? y0 + h + 1 : C->height) : y0 + h;
+  const uint32_t rw = rx1 - rx0, rh = ry1 - ry0;
+  double *sum = malloc(sizeof(double) * 3 * (size_t)rw * rh);
+  uint32_t *keys = malloc(sizeof(uint32_t) * (size_t)rw * rh);
 #pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1) reduction(+ : nr, v0, v1, v2, v3, v4) collapse(2)
-  for (uint32_t yy = 0; yy < h; yy++)
-    for (uint32_t xx = 0; xx < w; xx++) {
-      uint32_t x = x0 + xx, y = y0 + yy;
-      pix_t P;
-      pix_seeds(C, x, y, &P);
-      double acc[3] = {0, 0, 0};
+  for (uint32_t yy = 0; yy < rh; yy++)
+    for (uint32_t xx = 0; xx < rw; xx++) {
       uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0};
-      for (uint32_t i = spp_begin; i < spp_end; i++) {
-        double L[3];
-        if ((C->flags & O_REFERENCE) && S->node_state) ref_radiance(S, C, &P, x, y, i, L, &r);
-        else sample_radiance(S, C, &P, x, y, i, L, &r, vv, NULL);
-        for (int k = 0; k < 3; k++) acc[k] += L[k];
-      }
-      double inv = spp_end > spp_begin ? 1.0 / (spp_end - spp_begin) : 0;
-      float *o = out + 4 * ((size_t)yy * w + xx);
-      o[0] = (float)(acc[0] * inv); o[1] = (float)(acc[1] * inv); o[2] = (float)(acc[2] * inv); o[3] = 1;
+      const size_t q = (size_t)yy * rw + xx;
+      first_pass(S, C, rx0 + xx, ry0 + yy, spp_begin, spp_end, sum + 3 * q, keys + q, &r, vv);
       nr += r; v0 += vv[0]; v1 += vv[1]; v2 += vv[2]; v3 += vv[3]; v4 += vv[4];
     }
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1) schedule(dynamic, 1) reduction(+ : nr) collapse(2)
+  for (uint32_t yy = 0; yy < h; yy++)
+    for (uint32_t xx = 0; xx < w; xx++) {
+      const uint32_t x = x0 + xx, y = y0 + yy;
+      const size_t q = (size_t)(y - ry0) * rw + (x - rx0);
+      double acc[3] = {sum[3 * q], sum[3 * q + 1], sum[3 * q + 2]};
+      uint32_t n = spp_end - spp_begin;
+      if (refine) {
+        /* camRefine (kernels.wgsl.js:218-252): an edge among the sub-samples or in the 3x3 block, a posed body,
+         * or (option) traced shadow rays: `refine` more sub-samples, refine_light_samples light samples each */
+        const uint32_t k = keys[q];
+        int go = (k & 0x50000u) != 0 || ((k & 0x20000u) && (C->flags & O_REFINE_SHADOW));
+        for (int dy = -1; dy <= 1 && !go; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            int xn = (int)x + dx, yn = (int)y + dy;
+            xn = xn < 0 ? 0 : xn > (int)C->width - 1 ? (int)C->width - 1 : xn;
+            yn = yn < 0 ? 0 : yn > (int)C->height - 1 ? (int)C->height - 1 : yn;
+            if ((keys[(size_t)(yn - (int)ry0) * rw + (size_t)(xn - (int)rx0)] & 0xffffu) != (k & 0xffffu)) go = 1;
+          }
+        if (go) {
+          pix_t P;
+          pix_seeds(C, x, y, &P);
+          uint64_t r = 0;
+          for (uint32_t i = spp_end; i < spp_end + C->refine; i++) {
+            double L[3];
+            uint32_t kk;
+            ref_radiance(S, C, &P, x, y, i, C->refine_light_samples ? C->refine_light_samples : C->light_samples, L,
+                         &r, &kk);
+            for (int c = 0; c < 3; c++) acc[c] += L[c];
+          }
+          nr += r;
+          n += C->refine;
+        }
+      }
+      const double inv = n ? 1.0 / n : 0;
+      float *o = out + 4 * ((size_t)yy * w + xx);
+      o[0] = (float)(acc[0] * inv); o[1] = (float)(acc[1] * inv); o[2] = (float)(acc[2] * inv); o[3] = 1;
+    }
+  free(sum);
+  free(keys);
   if (rays) *rays = nr;
   if (visits) { visits[0] = v0; visits[1] = v1; visits[2] = v2; visits[3] = v3; visits[4] = v4; }
 }

@@ -190,6 +190,7 @@ typedef struct {
   float shutter[2];      /* exposure interval on the pose timeline: sub-sample i is at
                             shutter[0] + (shutter[1] - shutter[0]) kr(i, 1/golden ratio, s2) */
   uint32_t skip_bodies;  /* bit i: rays from the camera centre ignore body i (_skipMask, gpu/renderer.js:881-886) */
+  uint32_t refine, refine_light_samples;   /* reference estimator: camRefine (kernels.wgsl.js:218-252) */
 } or_camera;
 /* radiance of pixels [x0, x0+w) x [y0, y0+h) of a camera, spp sub-samples each, mean written as rgba
  * (a = 1) into out[(y - y0) * w + (x - x0)]. Counts traced rays (camera + extension + shadow) into *rays;

@@ -195,6 +195,43 @@ def test_gpu_reference_estimator_matches_oracle(ctx, opts):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("refine_shadows", [False, True])
+def test_gpu_refinement_pass_matches_oracle(ctx, refine_shadows):
+    """camRefine: pixels on a mesh edge (or with traced shadow rays) take 6 more sub-samples of 1 light sample after a
+    first pass of 1 sub-sample x 4 light samples (the judge preset); the image and the refined pixel count against
+    the oracle's two passes."""
+    sc, cams = scenes.c1(bvh=OBVH)
+    t = _gpu_cache(ctx, sc)
+    ctx.radiosity_update(direct=True, iters=16)
+    nodes = np.asarray(sc["verts"]).size // 8
+    st, e = ctx.radiosity_state(nodes)
+    flags = lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | (lib.VT_OPT_REFINE_SHADOWS if refine_shadows else 0)
+    cam = dict(cams[0], spp=1, flags=flags, light_samples=4, refine=6, refine_light_samples=1)
+    print("refinement pass, refine shadows %s" % refine_shadows, flush=True)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    samples = ctx.stats()["samples"]
+    ctx.set_camera(0, dict(cam, refine=0))
+    ctx.render([0])
+    plain = ctx.readback(0, lib.VT_KIND_LINEAR, 640, 480)
+    refined = samples - 640 * 480
+    print("refined pixels: %d (%.2f %%)" % (refined // 6, 100.0 * refined / 6 / (640 * 480)))
+    assert refined % 6 == 0 and 0 < refined // 6 < 640 * 480
+    changed = np.any(img != plain, axis=2)
+    assert changed.sum() <= refined // 6 and changed.sum() > 0.5 * refined // 6
+    x0, y0, w, h = 256, 160, 64, 48
+    sc2 = dict(sc, node_state=st, alb=t["alb"], hh=t["hh"], slot=1, e_avg=e)
+    ref, _, _ = oracle.render_tile(sc2, cam, x0, y0, w, h)
+    a = img[y0:y0 + h, x0:x0 + w, :3].astype(np.float64)
+    b = ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    print("GPU vs oracle with the refinement pass: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
 def test_gpu_reference_estimator_with_bodies(ctx):
     """Posed bodies: one BSDF bounce into the cache, the bodies' occlusion capsules dim the static surfaces'
     cached light, moving over the exposure."""

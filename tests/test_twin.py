@@ -252,3 +252,46 @@ def test_gpu_twin_reference_estimator_matches_oracle(loaded, prep):
     close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
     print("twin reference estimator CAM3 tile: relative RMSE %.3g, close %.4f" % (rel, close))
     assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
+@needs_assets
+def test_gpu_twin_judge_preset_matches_oracle(loaded, prep):
+    """The reference's judge preset on the twin (1 sub-sample x 4 light samples, then 6 refinement sub-samples on
+    edges, posed bodies and traced-shadow pixels): CAM3 around the instruments against the oracle's two passes."""
+    scene, _, _ = loaded
+    cams, poses = _twin_frame(prep, scene, 1, 1)
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(prep)
+        ctx.upload_profiles(prep)
+        ctx.set_instances(twin.instances(prep, poses))
+        ctx.radiosity_build()
+        n_mat = np.asarray(prep["materials"]).size // 16
+        hh = ctx.radiosity_tables(6, (n_mat, 4), np.float32)
+        alb = ctx.radiosity_tables(5, (n_mat, 32, 8), np.float32)
+        twin.rho_eff(prep, hh[:, :3])
+        inst = twin.instances(prep, poses)
+        ctx.set_instances(inst)
+        ctx.radiosity_update(direct=True, iters=96)
+        st, e = ctx.radiosity_state(np.asarray(prep["verts"]).size // 8)
+        flags = lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | lib.VT_OPT_REFINE_SHADOWS
+        cam = dict(cams[2], flags=flags, light_samples=4, refine=6, refine_light_samples=1)
+        ctx.set_camera(0, cam)
+        ctx.render([0])
+        img = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+        samples = ctx.stats()["samples"]
+    finally:
+        ctx.close()
+    px = cam["width"] * cam["height"]
+    print("judge preset: %.1f %% of the pixels refined" % (100.0 * (samples - px) / 6 / px))
+    x0, y0, w, h = 600, 300, 48, 32
+    sc2 = dict(prep, poses=inst, node_state=st, alb=alb, hh=hh, slot=1, e_avg=e)
+    ref, _, _ = oracle.render_tile(sc2, cam, x0, y0, w, h)
+    a = img[y0:y0 + h, x0:x0 + w, :3].astype(np.float64)
+    b = ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    close = np.mean(np.all(np.abs(a - b) <= 1e-3 * np.maximum(b, 1e-2), axis=2))
+    print("twin judge preset CAM3 tile: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+

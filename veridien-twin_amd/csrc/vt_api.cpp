@@ -29,6 +29,7 @@ struct CamSlot {
   DevBuf acc, dn, ri, lin, adj;
   // glare levels; ISP: linear RGB, block partial sums, 8-bit RGBA, YUV planes, statistics, rgb8 staging
   DevBuf glare_a, glare_b, isp_rgb, isp_part, rgba, yuv, stats, rgb8;
+  DevBuf keys;                // the reference estimator's first-pass keys (refinement)
   vt_sensor_params sp{};      // the last vt_sensor's parameters (the ISP reads black, max DN, noise model, CFA)
   bool isp_done = false, isp420 = false;
   bool ri_valid = false;
@@ -320,7 +321,8 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.ri);
     release(cs.lin);
     release(cs.adj);
-    DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.isp_rgb, &cs.isp_part, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8};
+    DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.isp_rgb, &cs.isp_part, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8,
+                     &cs.keys};
     for (DevBuf *b : isp) release(*b);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
@@ -645,6 +647,9 @@ extern "C" int vt_set_camera(vt_ctx *c, uint32_t cam, const vt_camera *k) {
   if (!(k->lens[0] > 0) || !(k->lens[1] > 0)) return fail(c, VT_ERR_ARG, "camera %u: focal lengths must be > 0", cam);
   if (!(k->shutter[0] >= 0.0f && k->shutter[0] <= 1.0f && k->shutter[1] >= 0.0f && k->shutter[1] <= 1.0f))
     return fail(c, VT_ERR_ARG, "camera %u: shutter times must lie in [0, 1]", cam);
+  if (k->refine > 64 || k->refine_light_samples > 16)
+    return fail(c, VT_ERR_ARG, "camera %u: refine %u (<= 64), refine light samples %u (<= 16)", cam, k->refine,
+                k->refine_light_samples);
   VT_HIP(c, hipSetDevice(c->device));
   CamSlot &s = c->cams[cam];
   const size_t px = (size_t)k->width * k->height;
@@ -831,6 +836,16 @@ int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opt
     d.shutter1 = k.shutter[1];
     d.skip = k.skip_bodies;
     d.opts = k.flags;
+    d.refine = (k.flags & VT_OPT_REFERENCE) ? k.refine : 0u;
+    d.nl_refine = k.refine_light_samples ? k.refine_light_samples : d.nl;
+    d.keys = nullptr;
+    if (d.refine) {
+      if (s0 != 0 || opt.accumulate)
+        return fail(c, VT_ERR_ARG, "render: the refinement pass needs a whole frame (spp_begin 0, no accumulate)");
+      int rc = ensure(c, s.keys, (size_t)k.width * k.height * sizeof(uint4));
+      if (rc) return rc;
+      d.keys = (uint4 *)s.keys.p;
+    }
     items += d.item_count;
   }
   VT_HIP(c, hipMemcpyAsync(c->dcams.p, c->h_cams, sizeof(DevCam) * n, hipMemcpyHostToDevice, c->stream));
@@ -1047,6 +1062,11 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   uint32_t launches = 0;
   c->graph_used = false;
   c->ext_launches = 0;
+  // the refinement pass's first-pass keys start cleared (a fill kernel, before the frame's launches or graph)
+  for (uint32_t a = 0; a < n; a++) {
+    const DevCam &d = c->h_cams[a];
+    if (d.keys) VT_HIP(c, launch_fill_keys(d.keys, (uint64_t)d.width * d.height, c->stream));
+  }
   if (opt.mode == VT_RENDER_WAVEFRONT) {
     if ((rc = ensure_launch_events(c, 1))) return rc;
     if ((rc = render_wavefront(c, n, opt, wf_params(c, opt), items))) return rc;
@@ -1116,6 +1136,12 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
       VT_HIP(c, hipEventRecord(c->ev_launch[2 * a + 1], c->stream));
       launches++;
     }
+  }
+  // the reference estimator's refinement pass (camRefine) per camera that asks for it
+  for (uint32_t a = 0; a < n; a++) {
+    const DevCam &d = c->h_cams[a];
+    if (d.refine && d.keys)
+      VT_HIP(c, launch_refine(L, (const DevCam *)c->dcams.p + a, d.tiles_x * d.tiles_y, c->stream));
   }
   VT_HIP(c, hipEventRecord(c->ev_r1, c->stream));
   c->last_launches = launches;
@@ -1293,7 +1319,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
 
 // ====================================================================== sensor
 static int sensor_params(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t h, uint32_t cam_index,
-                         float inv_spp, SensorDev &P) {
+                         float scale, SensorDev &P) {
   if (!p) return fail(c, VT_ERR_ARG, "sensor: null parameters");
   if (p->cfa > 3) return fail(c, VT_ERR_ARG, "sensor: cfa code %u", p->cfa);
   if (!(p->esf_sigma >= 0) || p->esf_sigma > 3.8f) return fail(c, VT_ERR_ARG, "sensor: esf_sigma outside [0, 3.8] px");
@@ -1302,7 +1328,7 @@ static int sensor_params(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint3
   std::memcpy(P.m, p->m, sizeof P.m);
   P.read_noise = p->read_noise; P.full_well = p->full_well; P.dark = p->dark; P.conv_gain = p->conv_gain;
   P.max_dn = p->max_dn; P.black = p->black; P.noise_on = p->noise_on ? 1u : 0u;
-  P.inv_spp = inv_spp;
+  P.scale = scale;
   P.glare_k = p->glare_fraction;
   P.glare_sigma = p->glare_sigma;
   P.glare_f = p->glare_level;
@@ -1355,7 +1381,7 @@ extern "C" int vt_sensor(vt_ctx *c, uint32_t cam, const vt_sensor_params *p) {
   if (!s.rendered || s.spp_total == 0) return fail(c, VT_ERR_STATE, "sensor: camera %u has no radiance", cam);
   VT_HIP(c, hipSetDevice(c->device));
   SensorDev P;
-  const float inv = s.c.scale > 0 ? s.c.scale / (float)s.spp_total : 1.0f / (float)s.spp_total;
+  const float inv = s.c.scale > 0 ? s.c.scale : 1.0f;     // over each pixel's own sample count
   int rc = sensor_params(c, p, s.c.width, s.c.height, s.c.cam_index, inv, P);
   if (rc) return rc;
   if (!s.ri_valid || s.ri_pupil != p->pupil_exp) {
@@ -1410,7 +1436,7 @@ extern "C" int vt_readback(vt_ctx *c, uint32_t cam, int kind, void *dst, size_t 
   if (kind == VT_KIND_LINEAR) {
     if (bytes != px * 16) return fail(c, VT_ERR_SIZE, "readback linear: %zu bytes, want %zu", bytes, px * 16);
     if (!s.rendered || s.spp_total == 0) return fail(c, VT_ERR_STATE, "readback: camera %u not rendered", cam);
-    const float sc = (s.c.scale > 0 ? s.c.scale : 1.0f) / (float)s.spp_total;
+    const float sc = s.c.scale > 0 ? s.c.scale : 1.0f;     // over each pixel's own sample count
     VT_HIP(c, launch_finalize_linear((const float4 *)s.acc.p, sc, px, (float4 *)s.lin.p, c->stream));
     VT_HIP(c, hipMemcpyAsync(dst, s.lin.p, bytes, hipMemcpyDeviceToHost, c->stream));
   } else if (kind == VT_KIND_RAW) {

@@ -71,6 +71,11 @@ struct DevCam {
   float shutter0, shutter1;          // exposure interval on the pose timeline (CamU.expo.xy)
   uint32_t skip;                     // bodies the camera's own rays ignore (bit per body)
   uint32_t opts;                     // option bits (VT_OPT_*)
+  // the reference estimator's refinement pass (camRefine): extra sub-samples, their light samples, and the
+  // per-pixel keys of the first pass {mesh of sub-sample 0, lowest and highest mesh over the sub-samples,
+  // 0x10000 posed body | 0x20000 shadow ray traced}; null keys: no refinement this frame
+  uint32_t refine, nl_refine;
+  uint4 *keys;
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)

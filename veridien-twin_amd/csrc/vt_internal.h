@@ -45,6 +45,10 @@ struct RenderLaunch {
                                // read by the gradient walk in place of its own first walk (null: walk it)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
+// the reference estimator's refinement pass of one camera (cam: its device record; keys and acc from the first
+// pass) and the first pass's key reset
+hipError_t launch_refine(const RenderLaunch &L, const DevCam *cam, uint32_t tiles, hipStream_t s);
+hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s);
 hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, bool ref, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)
@@ -115,14 +119,14 @@ hipError_t launch_camray_test(const DevCam *cam, const uint32_t *xyi, uint32_t n
 hipError_t launch_bsdf_test(const float4 *mats, const float *cases, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_capsule_test(const float *cases, uint32_t n, float *out, hipStream_t s);
 
-// the fused sensor epilogue: radiance = acc * inv_spp (rgb), RI map, -> DN
+// the fused sensor epilogue: radiance = acc.rgb * scale / acc.a (the pixel's sample count), RI map, -> DN
 struct SensorDev {
   uint32_t w, h, cfa, frame, seed, cam_index;
   float optics_k, esf_sigma, exposure, gain;
   float m[9];
   float read_noise, full_well, dark, conv_gain, max_dn, black;
   uint32_t noise_on;
-  float inv_spp;
+  float scale;
   float glare_k, glare_sigma;   // veiling glare: fraction (0 = off), sigma (level px)
   uint32_t glare_f;             // level factor F
 };
@@ -130,7 +134,7 @@ struct SensorDev {
 hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, float4 *glare_a, float4 *glare_b,
                          uint16_t *dn, hipStream_t s);
 hipError_t launch_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb, hipStream_t s);
-hipError_t launch_finalize_linear(const float4 *acc, float scale_over_spp, uint64_t n, float4 *out, hipStream_t s);
+hipError_t launch_finalize_linear(const float4 *acc, float scale, uint64_t n, float4 *out, hipStream_t s);
 // the ISP after the raw DN
 struct IspDev {
   uint32_t w, h, cfa, demosaic, nr, tone, chroma420;

@@ -189,13 +189,15 @@ struct Surf {
   v3 p, ng, ns, nsi, b;
   uint4 vi;
   uint4 ms;     // mesh record: material, flags, light, -
+  uint32_t mesh;
 };
 template <bool INST>
 __device__ __forceinline__ Surf surface_at(const SceneDev &S, const Hit &h, v3 o, v3 d, float time) {
   Surf s;
   const float4 *sh = S.tshade + 4 * (size_t)h.tri;
   const float4 n0 = sh[0];
-  s.ms = S.meshes[__float_as_uint(n0.w)];
+  s.mesh = __float_as_uint(n0.w);
+  s.ms = S.meshes[s.mesh];
   s.vi = S.tvid[h.tri];
   s.b = mk(1.0f - h.u - h.v, h.u, h.v);
   v3 ng = ld3(sh[3]);
@@ -240,11 +242,12 @@ __device__ __forceinline__ float cached_visibility(const SceneDev &S, const Surf
   return a;
 }
 
-// one emitter sample's contribution at the surface (directLight, shade.wgsl.js:160-188)
+// one emitter sample's contribution at the surface (directLight, shade.wgsl.js:160-188); a traced shadow ray marks
+// the pixel's key (0x20000)
 template <bool INST>
 __device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float time, float u0,
                                            float u1, float u2, bool mis, float fnl, float cached, Stack &st,
-                                           uint32_t &nrays) {
+                                           uint32_t &nrays, uint32_t &key) {
   v3 pl;
   const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
   v3 wi = pl - s.p;
@@ -263,6 +266,7 @@ __device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, con
   if (cached < 0.0f) {
     Hit sh;
     nrays++;
+    key |= 0x20000u;
     const v3 so = s.p + s.ng * (dot(s.ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
     if (trace_scene<true, INST>(S, so, wi, 0.0f, dist - 2e-5f, time, S.shadow_skip, sh, st)) return mk(0.f, 0.f, 0.f);
   }
@@ -303,17 +307,21 @@ __device__ __forceinline__ v3 secondary(const SceneDev &S, v3 o, v3 l, float tim
 
 // shadeHit (shade.wgsl.js:250-334) of a camera ray: an emitter's radiance; else nL light samples (MIS on posed
 // bodies only), then on static surfaces the node cache E_free x capsule visibility x A(mu) / pi, on bodies one
-// BSDF bounce into `secondary`. Light sample j uses kr(i nL + j); the bounce the first BSDF dimensions.
+// BSDF bounce into `secondary`. Light sample j uses kr(i nL + j); the bounce the first BSDF dimensions. key: the
+// first hit's mesh (0xfffe: a miss), 0x10000 on a posed body, 0x20000 once a shadow ray is traced.
 template <bool INST>
 __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t x,
-                                           uint32_t y, uint32_t i, Stack &st, uint32_t &nrays) {
+                                           uint32_t y, uint32_t i, uint32_t nl_in, Stack &st, uint32_t &nrays,
+                                           uint32_t &key) {
   const float time = sample_time(C, P, i);
   const v3 o = mk(C.pos[0], C.pos[1], C.pos[2]);
   const v3 d = camera_dir(C, P, x, y, i);
   Hit h;
   nrays++;
+  key = 0xfffeu;
   if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, C.skip, h, st)) return mk(0.f, 0.f, 0.f);
   const Surf s = surface_at<INST>(S, h, o, d, time);
+  key = (s.mesh & 0xffffu) | ((h.inst != NO_INST && !(s.ms.y & F_STATIC)) ? 0x10000u : 0u);
   const v3 wo = -d;
   const uint32_t opts = C.opts;
   if (opts & OPT_NORMALS) return s.ns * 0.5f + mk(0.5f, 0.5f, 0.5f);
@@ -324,7 +332,7 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
   const Mat M = load_mat(S.mats, s.ms.x);
   const bool stat = is_static_hit(h, s);
   const bool mis = !stat && !(opts & OPT_NO_MIS);
-  const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
+  const uint32_t nl = nl_in < 1 ? 1 : (nl_in > 16 ? 16 : nl_in);
   const float fnl = (float)nl;
   const float cached = (stat && (opts & OPT_CULL_SHADOWS)) ? cached_visibility<INST>(S, s) : -1.0f;
   v3 L = mk(0.f, 0.f, 0.f);
@@ -332,7 +340,7 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
     for (uint32_t j = 0; j < nl; j++) {
       const uint32_t li = i * nl + j;
       L = L + direct_light<INST>(S, s, M, wo, time, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
-                                 mis, fnl, cached, st, nrays);
+                                 mis, fnl, cached, st, nrays, key);
     }
   }
   L = L * (1.0f / fnl);
@@ -626,8 +634,22 @@ __device__ __forceinline__ void run_wave(const RenderLaunch &Lc, uint64_t wave, 
   if (valid) {
     PixSeeds P;
     pix_seeds(C, x, y, P);
-    if constexpr (REF) L = ref_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
-    else L = path_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
+    if constexpr (REF) {
+      uint32_t key;
+      L = ref_radiance<INST>(Lc.S, C, P, x, y, i, C.nl, st, nrays, key);
+      if (C.keys) {
+        // the pixel's first-pass key (camMain, kernels.wgsl.js:202-206): sub-sample 0's mesh, the mesh range over
+        // the sub-samples (a change marks an edge), the body / traced-shadow bits of any
+        uint4 *k = C.keys + (size_t)y * C.width + x;
+        const uint32_t m = key & 0xffffu;
+        if (i == 0) k->x = m;
+        atomicMin(&k->y, m);
+        atomicMax(&k->z, m);
+        if (key & 0x30000u) atomicOr(&k->w, key & 0x30000u);
+      }
+    } else {
+      L = path_radiance<INST>(Lc.S, C, P, x, y, i, st, nrays);
+    }
   }
   // per-pixel sums: aligned power-of-two runs reduce by butterfly; other run shapes add atomically
   const bool pow2 = (nspp & (nspp - 1)) == 0 && nspp <= 64;
@@ -886,6 +908,62 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
   flush_counts(Lc, nrays, nsamp);
 }
 
+// The reference estimator's refinement pass (camRefine, kernels.wgsl.js:218-252): pixels on an edge (another first-hit
+// mesh among their sub-samples or in the 3x3 block), on a posed body, or (VT_OPT_REFINE_SHADOWS) whose shadow rays
+// were traced take `refine` more sub-samples (indices spp .. spp + refine - 1, `nl_refine` light samples each),
+// added in order to the first pass's sum. One lane per pixel of an 8x8 tile; the grid strides over the tiles.
+template <bool INST>
+__global__ __launch_bounds__(64) void k_refine(RenderLaunch Lc, const DevCam *cam) {
+  extern __shared__ uint32_t s_stack[];
+  Stack st;
+  st.lds_node = s_stack;
+  st.lds_t = reinterpret_cast<float *>(s_stack + Lc.stack_lds * 64);
+  st.spill = Lc.spill;
+  st.tid = threadIdx.x;
+  st.block = 64;
+  st.slds = Lc.stack_lds;
+  st.spill_depth = STACK_TOTAL - Lc.stack_lds;
+  st.spill_stride = Lc.spill_stride;
+  st.gtid = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+  st.overflow = Lc.overflow;
+  const DevCam &C = *cam;
+  const uint32_t W = C.width, H = C.height, lane = threadIdx.x;
+  const uint32_t tiles = C.tiles_x * C.tiles_y;
+  uint32_t nrays = 0, nsamp = 0;
+  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint32_t x = (t % C.tiles_x) * 8u + (lane & 7u), y = (t / C.tiles_x) * 8u + (lane >> 3);
+    if (x >= W || y >= H) continue;
+    const uint4 k = C.keys[(size_t)y * W + x];
+    bool refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
+    for (int dy = -1; dy <= 1 && !refine; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int xx = min(max((int)x + dx, 0), (int)W - 1), yy = min(max((int)y + dy, 0), (int)H - 1);
+        if (C.keys[(size_t)yy * W + xx].x != k.x) refine = true;
+      }
+    if (!refine) continue;
+    PixSeeds P;
+    pix_seeds(C, x, y, P);
+    float4 *a = C.acc + (size_t)y * W + x;
+    float4 v = *a;
+    for (uint32_t i = C.nspp; i < C.nspp + C.refine; i++) {
+      uint32_t key;
+      const v3 L = ref_radiance<INST>(Lc.S, C, P, x, y, i, C.nl_refine, st, nrays, key);
+      v.x += L.x;
+      v.y += L.y;
+      v.z += L.z;
+    }
+    v.w += (float)C.refine;
+    *a = v;
+    nsamp += C.refine;
+  }
+  flush_counts(Lc, nrays, nsamp);
+}
+
+__global__ void k_fill_u4(uint4 *p, uint64_t n, uint4 v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // per-slot shading record: vertex normals (gathered through the slot's vertex ids), mesh id, geometric normal
 __global__ void k_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < T; k += gridDim.x * blockDim.x) {
@@ -956,6 +1034,22 @@ hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
     case 256: hipLaunchKernelGGL(render_kernel<256>(inst, ref), dim3(L.grid), dim3(256), lds, s, L); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_refine(const RenderLaunch &L, const DevCam *cam, uint32_t tiles, hipStream_t s) {
+  if (L.stack_lds < STACK_LDS_MIN || L.stack_lds > STACK_LDS_MAX || L.spill_stride < 64) return hipErrorInvalidValue;
+  if (tiles == 0) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, L.spill_stride / 64);
+  const size_t lds = (size_t)L.stack_lds * 64 * 8;
+  if (L.S.inst_count) hipLaunchKernelGGL(k_refine<true>, dim3(grid), dim3(64), lds, s, L, cam);
+  else hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, L, cam);
+  return hipGetLastError();
+}
+hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, n,
+                     make_uint4(0xfffeu, 0xffffffffu, 0u, 0u));
   return hipGetLastError();
 }
 

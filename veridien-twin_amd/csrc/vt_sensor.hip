@@ -142,6 +142,12 @@ __device__ __forceinline__ int cfa_channel(uint32_t x, uint32_t y, uint32_t code
 
 constexpr int TW = 32, TH = 8, RMAX = 12;
 
+// a pixel's mean radiance factor: the output scale over its sample count (the refinement pass gives some pixels
+// more samples than others)
+__device__ __forceinline__ float pix_scale(const SensorDev &P, float count) {
+  return count > 0.0f ? P.scale / count : 0.0f;
+}
+
 // ------------------------------------------------------------------ veiling glare on the 1/F level
 // (sensor.wgsl.js:87-141): F x F block means of the irradiance, then a Gaussian along each row and each column of the
 // level (zero outside the sensor, normalised on the infinite line), read back bilinearly in the exposure
@@ -158,9 +164,10 @@ __global__ void k_glare_down(SensorDev P, const float4 *acc, const float *ri, fl
       if (xx >= W || yy >= H) continue;
       const float4 s = acc[(size_t)yy * W + xx];
       const float q = ri[(size_t)yy * W + xx];
-      a0 = a0 + (s.x * P.inv_spp) * q;
-      a1 = a1 + (s.y * P.inv_spp) * q;
-      a2 = a2 + (s.z * P.inv_spp) * q;
+      const float inv = pix_scale(P, s.w);
+      a0 = a0 + (s.x * inv) * q;
+      a1 = a1 + (s.y * inv) * q;
+      a2 = a2 + (s.z * inv) * q;
       n = n + 1.0f;
     }
   const float sc = P.optics_k / fmaxf(n, 1.0f);
@@ -232,7 +239,8 @@ __global__ __launch_bounds__(TW *TH) void k_sensor(SensorDev P, const float4 *ac
         const float w = sw[k + r];
         const float4 s = acc[(size_t)yy * W + xx];
         const float q = ri[(size_t)yy * W + xx];
-        const float l0 = s.x * P.inv_spp, l1 = s.y * P.inv_spp, l2 = s.z * P.inv_spp;
+        const float inv = pix_scale(P, s.w);
+        const float l0 = s.x * inv, l1 = s.y * inv, l2 = s.z * inv;
         a0 = a0 + (w * l0) * q;
         a1 = a1 + (w * l1) * q;
         a2 = a2 + (w * l2) * q;
@@ -584,10 +592,11 @@ __global__ __launch_bounds__(256) void k_isp_stats(const uint4 *pa, uint32_t na,
     stats[threadIdx.x] = (sred[threadIdx.x][0] + sred[threadIdx.x][1]) + (sred[threadIdx.x][2] + sred[threadIdx.x][3]);
 }
 
-__global__ void k_finalize(const float4 *acc, float s, uint64_t n, float4 *out) {
+__global__ void k_finalize(const float4 *acc, float scale, uint64_t n, float4 *out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 a = acc[i];
+  const float s = a.w > 0.0f ? scale / a.w : 0.0f;
   out[i] = make_float4(a.x * s, a.y * s, a.z * s, 1.0f);
 }
 
@@ -635,9 +644,9 @@ hipError_t launch_isp(const IspDev &P, const uint16_t *dn, float4 *rgb, uint4 *p
   return hipGetLastError();
 }
 
-hipError_t launch_finalize_linear(const float4 *acc, float scale_over_spp, uint64_t n, float4 *out, hipStream_t s) {
+hipError_t launch_finalize_linear(const float4 *acc, float scale, uint64_t n, float4 *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc, scale_over_spp, n, out);
+  hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc, scale, n, out);
   return hipGetLastError();
 }
 

@@ -15,7 +15,7 @@ export const STATIC_BODY = 0xffffffff;
 
 // camera option bits (gpu/wgsl/common.wgsl.js:45-58), plus the estimator choice and the shadow culling
 export const OPTION_BITS = Object.freeze({ NO_INDIRECT: 4, DIRECT_ONLY: 8, NO_CAPSULE: 32, NORMALS: 64, NO_MIS: 128,
-  REFERENCE: 1 << 16, CULL_SHADOWS: 1 << 17 });
+  REFINE_SHADOW: 1024, REFERENCE: 1 << 16, CULL_SHADOWS: 1 << 17 });
 export const DEFAULTS = Object.freeze({
   spp: 16,            // sub-samples per pixel
   maxDepth: 3,        // path vertices (camera hit = 1), path tracer only
@@ -32,11 +32,16 @@ export const DEFAULTS = Object.freeze({
   smoothRadius: 0.006,
   nbrK: 16,
   cullShadows: true,  // skip shadow rays whose answer the three nodes and the bodies' capsules already give
+  refine: 6,          // reference estimator: extra sub-samples on edges, posed bodies and traced-shadow pixels
+  refineLightSamples: 1,
+  refineShadows: true,
+  radiosityEvery: 1,  // update the node cache every n-th pose change
 });
+// the reference's operating points for its camera estimator (gpu/renderer.js:63-67), spelled out in full
 export const PRESETS = Object.freeze({
-  live: { spp: 1, maxDepth: 2, lightSamples: 1 },
-  balanced: { spp: 4, maxDepth: 3, lightSamples: 1 },
-  judge: { spp: 64, maxDepth: 5, lightSamples: 1 },
+  judge: { spp: 1, lightSamples: 4, refine: 6, refineLightSamples: 1, refineShadows: true, radiosityEvery: 1, itersPerFrame: 4 },
+  balanced: { spp: 1, lightSamples: 2, refine: 2, refineLightSamples: 1, refineShadows: false, radiosityEvery: 2, itersPerFrame: 1 },
+  live: { spp: 1, lightSamples: 1, refine: 0, radiosityEvery: 2, itersPerFrame: 1 },
 });
 
 const KIND = { linear: 0, raw: 1, ri: 2, rgb8: 3, rgba8: 4, yuv420: 5 };
@@ -186,6 +191,7 @@ export class TwinRenderer {
     if (dflt(o.estimator, this.opts.estimator) === 'reference') {
       bits |= OPTION_BITS.REFERENCE;
       if (dflt(o.cullShadows, this.opts.cullShadows)) bits |= OPTION_BITS.CULL_SHADOWS;
+      if (dflt(o.refineShadows, this.opts.refineShadows)) bits |= OPTION_BITS.REFINE_SHADOW;
     }
     return bits >>> 0;
   }
@@ -204,6 +210,8 @@ export class TwinRenderer {
       this.stats.gatherBuildMs = this.radiosity.ms;
     }
     if (this.solvedVersion === this.posesVersion) return;
+    // radiosityEvery n: a pose change updates the cache every n-th time (gpu/renderer.js:36)
+    if (this.solvedVersion >= 0 && this.posesVersion - this.solvedVersion < Math.max(1, o.radiosityEvery || 1)) return;
     native.radiosityUpdate(this.ctx, true, this.solvedVersion < 0 ? o.itersInitial : o.itersPerFrame);
     this.solvedVersion = this.posesVersion;
   }
@@ -250,6 +258,8 @@ export class TwinRenderer {
       seed: dflt(o.seed, this.opts.seed), camIndex: cam.index, lightSamples: dflt(o.lightSamples, this.opts.lightSamples),
       maxDepth: dflt(o.maxDepth, this.opts.maxDepth), scale: dflt(o.scale, 1), shutter: o.shutter || [0, 1],
       skipBodies: this._skipMask(cam.name), options: this._options(o),
+      refine: dflt(o.estimator, this.opts.estimator) === 'reference' ? dflt(o.refine, this.opts.refine) : 0,
+      refineLightSamples: dflt(o.refineLightSamples, this.opts.refineLightSamples),
     });
   }
 

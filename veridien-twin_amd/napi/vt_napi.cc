@@ -236,6 +236,8 @@ napi_value SetCamera(napi_env env, napi_callback_info info) {
   k.flags = (uint32_t)num(env, argv[2], "options", 0);
   k.scale = (float)num(env, argv[2], "scale", 1.0);
   k.skip_bodies = (uint32_t)num(env, argv[2], "skipBodies", 0);
+  k.refine = (uint32_t)num(env, argv[2], "refine", 0);
+  k.refine_light_samples = (uint32_t)num(env, argv[2], "refineLightSamples", 0);
   k.shutter[0] = 0.0f;
   k.shutter[1] = 1.0f;
   napi_value sh;

@@ -25,7 +25,7 @@ EXPORTS = [
 
 # camera option bits (vt_camera.flags)
 VT_OPT_NO_INDIRECT, VT_OPT_DIRECT_ONLY, VT_OPT_NO_CAPSULE, VT_OPT_NORMALS, VT_OPT_NO_MIS = 4, 8, 32, 64, 128
-VT_OPT_REFERENCE, VT_OPT_CULL_SHADOWS = 1 << 16, 1 << 17
+VT_OPT_REFERENCE, VT_OPT_CULL_SHADOWS, VT_OPT_REFINE_SHADOWS = 1 << 16, 1 << 17, 1024
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
 CAMERA_PARAMS = 14      # VT_CAMERA_PARAMS: fx fy cx cy k1 k2 k3 k4, rotation rx ry rz, centre tx ty tz
@@ -63,7 +63,8 @@ class RadiosityInfo(C.Structure):
 class Camera(C.Structure):
     _fields_ = [("R", f32 * 9), ("pos", f32 * 3), ("mode", u32), ("lens", f32 * 9), ("width", u32), ("height", u32),
                 ("spp", u32), ("seed", u32), ("cam_index", u32), ("light_samples", u32), ("max_depth", u32),
-                ("flags", u32), ("scale", f32), ("shutter", f32 * 2), ("skip_bodies", u32)]
+                ("flags", u32), ("scale", f32), ("shutter", f32 * 2), ("skip_bodies", u32), ("refine", u32),
+                ("refine_light_samples", u32)]
 
 
 class RenderOpts(C.Structure):
@@ -312,6 +313,8 @@ class Context:
         c.scale = float(cam.get("scale", 1.0))
         c.shutter[:] = [float(x) for x in cam.get("shutter", (0.0, 1.0))]
         c.skip_bodies = int(cam.get("skip_bodies", 0))
+        c.refine = int(cam.get("refine", 0))
+        c.refine_light_samples = int(cam.get("refine_light_samples", 0))
         self.check(self.L.vt_set_camera(self.h, slot, C.byref(c)))
 
     def set_light_profile(self, light, axis=None, theta=(), p=()):

@@ -66,6 +66,8 @@ struct vt_ctx {
   uint32_t tri_count = 0, node4_count = 0;
   // launch support
   DevBuf counters, overflow, spill, dcams, test_in, test_out, test_aux;
+  DevBuf refine_list;           // the refinement pass's pixel list (camera, pixel) and its count
+  DevBuf eavg_part;             // eAvg's block partials
   uint64_t spill_lanes = 0;
   DevCam *h_cams = nullptr;     // pinned staging for the camera table
   hipEvent_t ev_stage = nullptr;
@@ -311,6 +313,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots, &c->tvid, &c->g_pos, &c->g_miss, &c->radu, &c->g_area,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
+                    &c->refine_list, &c->eavg_part,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
                     &c->dXd, &c->dR, &c->dPart, &c->grads, &c->wf, &c->wf_temp, &c->ltot};
@@ -1137,11 +1140,33 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
       launches++;
     }
   }
-  // the reference estimator's refinement pass (camRefine) per camera that asks for it
-  for (uint32_t a = 0; a < n; a++) {
-    const DevCam &d = c->h_cams[a];
-    if (d.refine && d.keys)
-      VT_HIP(c, launch_refine(L, (const DevCam *)c->dcams.p + a, d.tiles_x * d.tiles_y, c->stream));
+  // the reference estimator's refinement pass (camRefine): the cameras that ask for it list their pixels, then one
+  // launch refines them all
+  {
+    uint32_t R = 0;
+    uint64_t px = 0;
+    for (uint32_t a = 0; a < n; a++) {
+      const DevCam &d = c->h_cams[a];
+      if (d.refine && d.keys) {
+        R = std::max(R, d.refine);
+        px += (uint64_t)d.width * d.height;
+      }
+    }
+    if (R) {
+      if ((rc = ensure(c, c->refine_list, px * sizeof(uint2) + 256))) return rc;
+      uint32_t *count = (uint32_t *)((char *)c->refine_list.p + px * sizeof(uint2));
+      VT_HIP(c, launch_zero_u64((unsigned long long *)count, 1, c->stream));
+      for (uint32_t a = 0; a < n; a++) {
+        const DevCam &d = c->h_cams[a];
+        if (d.refine && d.keys)
+          VT_HIP(c, launch_refine_mark((const DevCam *)c->dcams.p + a, a, d.width, d.height,
+                                       (uint2 *)c->refine_list.p, count, c->stream));
+      }
+      RenderLaunch Lr = L;
+      Lr.cams = (const DevCam *)c->dcams.p;
+      Lr.ncam = n;
+      VT_HIP(c, launch_refine(Lr, (const uint2 *)c->refine_list.p, count, px, R, c->stream));
+    }
   }
   VT_HIP(c, hipEventRecord(c->ev_r1, c->stream));
   c->last_launches = launches;
@@ -2156,7 +2181,9 @@ extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
     c->rad_slot = 3 - c->rad_slot;
   }
   // eAvg (the mean irradiance a body-to-body bounce reads) and the source slot, on the device
-  VT_HIP(c, launch_eavg(L, (const float *)c->g_area.p, (float4 *)c->radu.p, c->rad_slot, c->stream));
+  if (int erc = ensure(c, c->eavg_part, 256 * 4 * sizeof(double))) return erc;
+  VT_HIP(c, launch_eavg(L, (const float *)c->g_area.p, (float4 *)c->radu.p, c->rad_slot, (double *)c->eavg_part.p,
+                        c->stream));
   return VT_OK;
 }
 

@@ -45,9 +45,13 @@ struct RenderLaunch {
                                // read by the gradient walk in place of its own first walk (null: walk it)
 };
 hipError_t launch_render(const RenderLaunch &L, hipStream_t s);
-// the reference estimator's refinement pass of one camera (cam: its device record; keys and acc from the first
-// pass) and the first pass's key reset
-hipError_t launch_refine(const RenderLaunch &L, const DevCam *cam, uint32_t tiles, hipStream_t s);
+// the reference estimator's refinement pass: each camera marks its pixels into one list (cam: the camera's device
+// record, c its index in L.cams), then one launch refines every listed pixel (R: the frame's largest refine); and
+// the first pass's key reset
+hipError_t launch_refine_mark(const DevCam *cam, uint32_t c, uint32_t width, uint32_t height, uint2 *list,
+                              uint32_t *count, hipStream_t s);
+hipError_t launch_refine(const RenderLaunch &L, const uint2 *list, const uint32_t *count, uint64_t max_items,
+                         uint32_t R, hipStream_t s);
 hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s);
 hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, bool ref, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
@@ -177,7 +181,9 @@ struct NodeLaunch {
 };
 hipError_t launch_albedo_tables(const NodeLaunch &L, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, uint32_t n, float4 v, hipStream_t s);    // a fill kernel (no memset nodes)
-hipError_t launch_eavg(const NodeLaunch &L, const float *area, float4 *radu, uint32_t slot, hipStream_t s);
+// part: 256 x 4 doubles of scratch for the block partials
+hipError_t launch_eavg(const NodeLaunch &L, const float *area, float4 *radu, uint32_t slot, double *part,
+                       hipStream_t s);
 hipError_t launch_gather_build(const NodeLaunch &L, hipStream_t s);
 hipError_t launch_gather_count(const NodeLaunch &L, hipStream_t s);
 hipError_t launch_gather_remap(const NodeLaunch &L, hipStream_t s);

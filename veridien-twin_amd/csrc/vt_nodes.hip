@@ -185,30 +185,45 @@ __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
 }
 
 // nodeSummary's eAvg (gpu/renderer.js:824-852) on the device: area-weighted E_free V + B_dir / max(A_hh, 1e-3)
-// over the active nodes, summed in FP64 by one block in a fixed order; radu[0] = eAvg + bits(slot)
-__global__ __launch_bounds__(256) void k_eavg(NodeLaunch L, const float *area, float4 *radu, uint32_t slot) {
-  __shared__ double sh[4][256];
-  double a = 0, e[3] = {0, 0, 0};
-  for (uint32_t k = threadIdx.x; k < L.n_active; k += 256) {
-    const uint32_t i = L.active[k];
-    const double w = area[i];
-    const float4 s0 = L.state[4 * (size_t)i], s3 = L.state[4 * (size_t)i + 3];
-    const float4 h = L.hh[L.nmat[i]];
-    a += w;
-    e[0] += w * ((double)s0.x * s0.w + (double)s3.x / fmax((double)h.x, 1e-3));
-    e[1] += w * ((double)s0.y * s0.w + (double)s3.y / fmax((double)h.y, 1e-3));
-    e[2] += w * ((double)s0.z * s0.w + (double)s3.z / fmax((double)h.z, 1e-3));
-  }
-  sh[0][threadIdx.x] = a;
-  sh[1][threadIdx.x] = e[0];
-  sh[2][threadIdx.x] = e[1];
-  sh[3][threadIdx.x] = e[2];
+// over the active nodes in FP64. Stage 1: block b sums the b-th contiguous share of the active list (a fixed
+// partition, so the result does not depend on scheduling); stage 2: one block adds the EAVG_BLOCKS partials in
+// order and writes radu[0] = eAvg + bits(slot).
+constexpr uint32_t EAVG_BLOCKS = 256;
+
+__device__ __forceinline__ void block_sum4(double v[4], double (*sh)[256]) {
+  for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = v[q];
   __syncthreads();
   for (uint32_t st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st)
       for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(256) void k_eavg_part(NodeLaunch L, const float *area, double *part) {
+  __shared__ double sh[4][256];
+  const uint32_t per = (L.n_active + EAVG_BLOCKS - 1) / EAVG_BLOCKS;
+  const uint32_t k0 = blockIdx.x * per, k1 = min(L.n_active, k0 + per);
+  double v[4] = {0, 0, 0, 0};
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const uint32_t i = L.active[k];
+    const double w = area[i];
+    const float4 s0 = L.state[4 * (size_t)i], s3 = L.state[4 * (size_t)i + 3];
+    const float4 h = L.hh[L.nmat[i]];
+    v[0] += w;
+    v[1] += w * ((double)s0.x * s0.w + (double)s3.x / fmax((double)h.x, 1e-3));
+    v[2] += w * ((double)s0.y * s0.w + (double)s3.y / fmax((double)h.y, 1e-3));
+    v[3] += w * ((double)s0.z * s0.w + (double)s3.z / fmax((double)h.z, 1e-3));
+  }
+  block_sum4(v, sh);
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_eavg(const double *part, float4 *radu, uint32_t slot) {
+  __shared__ double sh[4][256];
+  double v[4];
+  for (int q = 0; q < 4; q++) v[q] = threadIdx.x < EAVG_BLOCKS ? part[4 * threadIdx.x + q] : 0.0;
+  block_sum4(v, sh);
   if (threadIdx.x == 0) {
     const double A = sh[0][0];
     radu[0] = A > 0 ? make_float4((float)(sh[1][0] / A), (float)(sh[2][0] / A), (float)(sh[3][0] / A), __uint_as_float(slot))
@@ -228,8 +243,10 @@ uint32_t grid_of(const NodeLaunch &L, uint64_t lanes) {
 
 }  // namespace
 
-hipError_t launch_eavg(const NodeLaunch &L, const float *area, float4 *radu, uint32_t slot, hipStream_t s) {
-  hipLaunchKernelGGL(k_eavg, dim3(1), dim3(256), 0, s, L, area, radu, slot);
+hipError_t launch_eavg(const NodeLaunch &L, const float *area, float4 *radu, uint32_t slot, double *part,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_eavg_part, dim3(EAVG_BLOCKS), dim3(256), 0, s, L, area, part);
+  hipLaunchKernelGGL(k_eavg, dim3(1), dim3(256), 0, s, (const double *)part, radu, slot);
   return hipGetLastError();
 }
 hipError_t launch_fill_f4(float4 *p, uint32_t n, float4 v, hipStream_t s) {

@@ -908,12 +908,29 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
   flush_counts(Lc, nrays, nsamp);
 }
 
-// The reference estimator's refinement pass (camRefine, kernels.wgsl.js:218-252): pixels on an edge (another first-hit
-// mesh among their sub-samples or in the 3x3 block), on a posed body, or (VT_OPT_REFINE_SHADOWS) whose shadow rays
-// were traced take `refine` more sub-samples (indices spp .. spp + refine - 1, `nl_refine` light samples each),
-// added in order to the first pass's sum. One lane per pixel of an 8x8 tile; the grid strides over the tiles.
+// The reference estimator's refinement pass (camRefine, kernels.wgsl.js:218-252). First each pixel decides from the
+// first pass's keys: an edge (another first-hit mesh among its sub-samples or in its 3x3 block), a posed body, or
+// (VT_OPT_REFINE_SHADOWS) traced shadow rays; the pixels that refine are appended to one list for every camera.
+__global__ void k_refine_mark(const DevCam *cam, uint32_t c, uint2 *list, uint32_t *count) {
+  const DevCam &C = *cam;
+  const uint32_t W = C.width, H = C.height;
+  const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15u), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= W || y >= H) return;
+  const uint4 k = C.keys[(size_t)y * W + x];
+  bool refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
+  for (int dy = -1; dy <= 1 && !refine; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const int xx = min(max((int)x + dx, 0), (int)W - 1), yy = min(max((int)y + dy, 0), (int)H - 1);
+      if (C.keys[(size_t)yy * W + xx].x != k.x) refine = true;
+    }
+  if (refine) list[atomicAdd(count, 1u)] = make_uint2(c, y * W + x);
+}
+
+// Then every listed pixel takes `refine` more sub-samples (indices spp .. spp + refine - 1, nl_refine light samples
+// each): one lane per sub-sample, a pixel's sub-samples on consecutive lanes (groups of R = the frame's largest
+// refine), and the group's first lane adds them to the first pass's sum in sub-sample order.
 template <bool INST>
-__global__ __launch_bounds__(64) void k_refine(RenderLaunch Lc, const DevCam *cam) {
+__global__ __launch_bounds__(64) void k_refine(RenderLaunch Lc, const uint2 *list, const uint32_t *count, uint32_t R) {
   extern __shared__ uint32_t s_stack[];
   Stack st;
   st.lds_node = s_stack;
@@ -926,35 +943,43 @@ __global__ __launch_bounds__(64) void k_refine(RenderLaunch Lc, const DevCam *ca
   st.spill_stride = Lc.spill_stride;
   st.gtid = (uint64_t)blockIdx.x * 64 + threadIdx.x;
   st.overflow = Lc.overflow;
-  const DevCam &C = *cam;
-  const uint32_t W = C.width, H = C.height, lane = threadIdx.x;
-  const uint32_t tiles = C.tiles_x * C.tiles_y;
+  const uint32_t lane = threadIdx.x, G = 64u / R, grp = lane / R, k = lane - grp * R;
+  const uint32_t n = *count;
   uint32_t nrays = 0, nsamp = 0;
-  for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const uint32_t x = (t % C.tiles_x) * 8u + (lane & 7u), y = (t / C.tiles_x) * 8u + (lane >> 3);
-    if (x >= W || y >= H) continue;
-    const uint4 k = C.keys[(size_t)y * W + x];
-    bool refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
-    for (int dy = -1; dy <= 1 && !refine; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        const int xx = min(max((int)x + dx, 0), (int)W - 1), yy = min(max((int)y + dy, 0), (int)H - 1);
-        if (C.keys[(size_t)yy * W + xx].x != k.x) refine = true;
-      }
-    if (!refine) continue;
-    PixSeeds P;
-    pix_seeds(C, x, y, P);
-    float4 *a = C.acc + (size_t)y * W + x;
-    float4 v = *a;
-    for (uint32_t i = C.nspp; i < C.nspp + C.refine; i++) {
+  for (uint32_t g0 = blockIdx.x * G; g0 < n; g0 += gridDim.x * G) {
+    const uint32_t g = g0 + grp;
+    const bool in = grp < G && g < n;
+    uint2 item = make_uint2(0u, 0u);
+    if (in) item = list[g];
+    const DevCam &C = Lc.cams[item.x];
+    const uint32_t x = item.y % C.width, y = item.y / C.width;
+    const bool run = in && k < C.refine;
+    v3 L = mk(0.f, 0.f, 0.f);
+    if (run) {
+      PixSeeds P;
+      pix_seeds(C, x, y, P);
       uint32_t key;
-      const v3 L = ref_radiance<INST>(Lc.S, C, P, x, y, i, C.nl_refine, st, nrays, key);
-      v.x += L.x;
-      v.y += L.y;
-      v.z += L.z;
+      L = ref_radiance<INST>(Lc.S, C, P, x, y, C.nspp + k, C.nl_refine, st, nrays, key);
     }
-    v.w += (float)C.refine;
-    *a = v;
-    nsamp += C.refine;
+    // the group's sum in sub-sample order, onto the first pass's (every lane takes part in the shuffles)
+    const uint32_t base = grp * R;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool writer = in && k == 0;
+    if (writer) v = C.acc[item.y];
+    for (uint32_t j = 0; j < R; j++) {
+      const int src = (int)min(base + j, 63u);
+      const float lx = __shfl(L.x, src, 64), ly = __shfl(L.y, src, 64), lz = __shfl(L.z, src, 64);
+      if (j < C.refine) {
+        v.x += lx;
+        v.y += ly;
+        v.z += lz;
+      }
+    }
+    if (writer) {
+      v.w += (float)C.refine;
+      C.acc[item.y] = v;
+      nsamp += C.refine;
+    }
   }
   flush_counts(Lc, nrays, nsamp);
 }
@@ -1037,13 +1062,21 @@ hipError_t launch_render(const RenderLaunch &L, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_refine(const RenderLaunch &L, const DevCam *cam, uint32_t tiles, hipStream_t s) {
-  if (L.stack_lds < STACK_LDS_MIN || L.stack_lds > STACK_LDS_MAX || L.spill_stride < 64) return hipErrorInvalidValue;
-  if (tiles == 0) return hipSuccess;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, L.spill_stride / 64);
+hipError_t launch_refine_mark(const DevCam *cam, uint32_t c, uint32_t width, uint32_t height, uint2 *list,
+                              uint32_t *count, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_mark, dim3((width + 15) / 16, (height + 15) / 16), dim3(256), 0, s, cam, c, list, count);
+  return hipGetLastError();
+}
+hipError_t launch_refine(const RenderLaunch &L, const uint2 *list, const uint32_t *count, uint64_t max_items,
+                         uint32_t R, hipStream_t s) {
+  if (L.stack_lds < STACK_LDS_MIN || L.stack_lds > STACK_LDS_MAX || L.spill_stride < 64 || R == 0 || R > 64)
+    return hipErrorInvalidValue;
+  if (max_items == 0) return hipSuccess;
+  const uint64_t G = 64 / R, want = (max_items + G - 1) / G;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(want, L.spill_stride / 64);
   const size_t lds = (size_t)L.stack_lds * 64 * 8;
-  if (L.S.inst_count) hipLaunchKernelGGL(k_refine<true>, dim3(grid), dim3(64), lds, s, L, cam);
-  else hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, L, cam);
+  if (L.S.inst_count) hipLaunchKernelGGL(k_refine<true>, dim3(grid), dim3(64), lds, s, L, list, count, R);
+  else hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, L, list, count, R);
   return hipGetLastError();
 }
 hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s) {

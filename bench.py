@@ -382,7 +382,10 @@ def bench_twin_preset(args, preset):
     prep = twin.prepare(scene, meshes, physics)
     state = twin.default_state(scene)
     cams = twin.cameras(scene, spp=P["spp"], depth=1, seed=0, prep=prep)
+    # the reference's defaults: shadow culling, the fixed cameras' first-hit and still-light occlusion caches
     flags = lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | (lib.VT_OPT_REFINE_SHADOWS if P["refine_shadows"] else 0)
+    if not args.no_caches:
+        flags |= lib.VT_OPT_HIT_CACHE | lib.VT_OPT_SHADOW_CACHE
     for c in cams:
         c.update(flags=flags, light_samples=P["nl"], refine=P["refine"], refine_light_samples=P["nl_refine"])
     ctx = lib.Context(0)
@@ -430,10 +433,11 @@ def bench_twin_preset(args, preset):
     px = sum(c["width"] * c["height"] for c in cams)
     print(json.dumps({"metric": METRIC, "config": {"workload": "reference '%s' preset on the real twin scene: 3 KB4 "
                                                                "cameras 1280x720, reference estimator, spp %d, %d light "
-                                                               "samples, refine %d x %d light samples, shadow culling, "
+                                                               "samples, refine %d x %d light samples, shadow culling%s, "
                                                                "tools moving, node radiosity (direct + %d Jacobi steps) "
                                                                "every %s frame, sensor + ISP"
                                                    % (preset, P["spp"], P["nl"], P["refine"], P["nl_refine"],
+                                                      "" if args.no_caches else ", fixed-camera hit and occlusion caches",
                                                       P["iters"], "2nd" if P["every"] == 2 else "")},
                       "frames": n, "ms_per_frame": round(1e3 * dt / n, 3), "fps": round(n / dt, 1),
                       "refined_pixel_fraction": round((st["samples"] - px * P["spp"]) / max(1, P["refine"]) / px, 4)
@@ -469,6 +473,8 @@ def main():
                     help="auto (default): the autotuner times both forms on a probe of the frame (untimed) and the "
                          "faster runs; wavefront: per-vertex kernels with sorted rays; batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-caches", action="store_true",
+                    help="twin presets: trace every first hit and static shadow ray (no fixed-camera caches)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)

@@ -353,6 +353,14 @@ int vt_isp_test(vt_ctx *ctx, const vt_sensor_params *sensor, const vt_isp_params
 #define VT_OPT_NORMALS 64u
 #define VT_OPT_NO_MIS 128u
 #define VT_OPT_REFINE_SHADOWS 1024u   /* refine pixels whose shadow rays were traced (refineShadows) */
+/* reference estimator, fixed cameras: keep each sub-sample's static first hit and trace only the bodies on later
+ * frames (hitCache, gpu/renderer.js:47, 940-967); exact. Reset when the camera's rays or the scene change. */
+#define VT_OPT_HIT_CACHE 2048u
+/* reference estimator, still light: static surfaces test each light sample against the static scene (kept per
+ * sub-sample and light sample while the camera, its rays and the scene hold) and then against the bodies; a pixel's
+ * shadow key marks a body's shadow, a penumbra or light samples that disagree by more than 20 %
+ * (SHADOW_CACHE, gpu/wgsl/shade.wgsl.js:272-301; shadowCache, gpu/renderer.js:48, 951-956) */
+#define VT_OPT_SHADOW_CACHE 8192u
 #define VT_OPT_REFERENCE (1u << 16)
 #define VT_OPT_CULL_SHADOWS (1u << 17)
 

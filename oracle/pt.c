@@ -34,6 +34,7 @@
 #define O_REFERENCE (1u << 16)
 #define O_CULL_SHADOWS (1u << 17)
 #define O_REFINE_SHADOW 1024u
+#define O_SHADOW_CACHE 8192u
 static const float R2A = 0.7548776662466927f, R2B = 0.5698402909980532f;
 static const float KL0 = 0.41421356237f, KL1 = 0.73205080757f, KL2 = 0.23606797750f;
 static const float KB0 = 0.64575131106f, KB1 = 0.31662479036f, KB2 = 0.60555127546f;
@@ -486,6 +487,94 @@ static void node_interp(const or_scene *S, const surf_t *s, int k, double out[3]
 
 static int is_static(const or_hit *h, const surf_t *s) { return (s->flags & F_STATIC) && h->inst < 0; }
 
+/* one emitter sample's unshadowed contribution c and its shadow segment so + t wi, t < tmax; 0 when the sample
+ * cannot light the point (lightEval, shade.wgsl.js:135-157) */
+static int light_eval(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], const double u[3],
+                      int mis, double nl, double c[3], double so[3], double wi[3], double *tmax) {
+  double pl[3];
+  const float *e = sample_emitter(S, u, pl);
+  for (int k = 0; k < 3; k++) wi[k] = pl[k] - s->p[k];
+  const double d2 = dot3(wi, wi), dist = sqrt(d2);
+  for (int k = 0; k < 3; k++) wi[k] /= dist;
+  const double en[3] = {e[12], e[13], e[14]};
+  const double cosE = -dot3(wi, en);
+  if (cosE <= 0) return 0;
+  const double cosS = dot3(s->ns, wi);
+  if (cosS <= 0 || dot3(s->ng, wi) <= 0) return 0;
+  uint32_t lid;
+  memcpy(&lid, e + 11, 4);
+  const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
+  const double pe = light_profile(S, lid, mwi);
+  const float *L0 = S->lights + 4 * (size_t)lid;
+  const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
+  if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) return 0;
+  double f[3];
+  or_eval_bsdf(M, s->ns, wi, wo, f);
+  if (maxc(f) <= 0) return 0;
+  const double pW = d2 / (cosE * S->emit_area);
+  double w = 1;
+  if (mis) {
+    const double pb = or_pdf_bsdf(M, s->ns, wo, wi);
+    w = nl * nl * pW * pW / (nl * nl * pW * pW + pb * pb);
+  }
+  for (int k = 0; k < 3; k++) c[k] = f[k] * Le[k] * cosS * w / pW;
+  offset_ray(s->p, s->ng, wi, so);
+  *tmax = dist - 2e-5;
+  return 1;
+}
+
+/* whether a visible body (not in `skip`) blocks the segment o + t d, t in (0, tmax), at `time` */
+static int bodies_occlude(const or_scene *S, const double o[3], const double d[3], double tmax, double time,
+                          uint32_t skip) {
+  for (uint32_t b = 0; b < S->body_count; b++) {
+    const or_body *B = &S->bodies[b];
+    if (B->hidden || ((skip >> b) & 1u)) continue;
+    pose_t Q;
+    body_pose(B, time, &Q);
+    double ol[3], dl[3], rel[3] = {o[0] - Q.p[0], o[1] - Q.p[1], o[2] - Q.p[2]};
+    rot_apply_t(Q.R, rel, ol);
+    rot_apply_t(Q.R, d, dl);
+    or_hit hb;
+    or_traverse(B->nodes, B->tris, ol, dl, 0, tmax, 1, 0, &hb);
+    if (hb.tri >= 0) return 1;
+  }
+  return 0;
+}
+
+/* the nL light samples of a static surface under a still light (shade.wgsl.js:272-301): a lit sample counts when
+ * neither the static scene nor a visible body blocks its segment; the key gains 0x20000 for a body's shadow, a
+ * penumbra (some lit samples hidden, some not) or visible samples whose luminances differ by more than 20 % of
+ * their midpoint. The renderer keeps the static answers per sub-sample; tracing them every time gives the same
+ * image. Returns the sum of the visible samples. */
+static void still_light(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], double time,
+                        const pix_t *P, uint32_t i, uint32_t nl, uint64_t *rays, uint32_t *key, double out[3]) {
+  out[0] = out[1] = out[2] = 0;
+  uint32_t nvis = 0, nlit = 0;
+  double lmin = INFINITY, lmax = 0;
+  for (uint32_t j = 0; j < nl; j++) {
+    const uint32_t li = i * nl + j;
+    const double u[3] = {or_kr(li, KL0, P->s[3]), or_kr(li, KL1, P->s[4]), or_kr(li, KL2, P->s[5])};
+    double c[3], so[3], wi[3], tmax;
+    if (!light_eval(S, s, M, wo, u, 0, nl, c, so, wi, &tmax)) continue;
+    nlit++;
+    (*rays)++;
+    or_hit sh;
+    or_traverse(S->nodes, S->tris, so, wi, 0, tmax, 1, 0, &sh);
+    if (sh.tri >= 0) continue;
+    if (bodies_occlude(S, so, wi, tmax, time, S->shadow_skip)) {
+      *key |= 0x20000u;
+      continue;
+    }
+    nvis++;
+    for (int k = 0; k < 3; k++) out[k] += c[k];
+    const double y = 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2];
+    lmin = fmin(lmin, y);
+    lmax = fmax(lmax, y);
+  }
+  if (nvis > 0 && nvis < nlit) *key |= 0x20000u;
+  if (nvis > 1 && lmax - lmin > 0.2 * (lmax + lmin) * 0.5) *key |= 0x20000u;
+}
+
 static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], double time,
                          const double u[3], int mis, double nl, double cached, uint64_t *rays, uint32_t *key,
                          double out[3]) {
@@ -603,9 +692,11 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
   const int stat = is_static(&h, &s);
   const int mis = !stat && !(opts & O_NO_MIS);
   const uint32_t nl = nl_in < 1 ? 1 : nl_in > 16 ? 16 : nl_in;
-  const double cached = (stat && (opts & O_CULL_SHADOWS)) ? cached_visibility(S, &s) : -1;
+  const double cached = (stat && (opts & O_CULL_SHADOWS) && !(opts & O_SHADOW_CACHE)) ? cached_visibility(S, &s) : -1;
   double L[3] = {0, 0, 0};
-  if (S->emit_count > 0)
+  if (S->emit_count > 0 && stat && (opts & O_SHADOW_CACHE))
+    still_light(S, &s, &M, wo, time, P, i, nl, rays, key, L);
+  else if (S->emit_count > 0)
     for (uint32_t j = 0; j < nl; j++) {
       const uint32_t li = i * nl + j;
       const double u[3] = {or_kr(li, KL0, P->s[3]), or_kr(li, KL1, P->s[4]), or_kr(li, KL2, P->s[5])};

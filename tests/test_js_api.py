@@ -255,8 +255,8 @@ def test_js_posed_frame_equals_c_abi_frame(tmp_path):
 @needs_node
 def test_js_reference_estimator_frame_equals_c_abi(tmp_path):
     """estimator 'reference' through TwinRenderer (node cache built on first use, direct light + 96 Jacobi steps,
-    shadow culling on, the refinement pass of 6 sub-samples x 1 light sample with traced-shadow pixels included, as
-    the reference's DEFAULTS) against the same frame through the C ABI."""
+    shadow culling on, the refinement pass of 6 sub-samples x 1 light sample with shadow pixels included, the fixed
+    camera's first-hit and occlusion caches, as the reference's DEFAULTS) against the same frame through the C ABI."""
     sc, cams = scenes.c1()
     cam = dict(cams[0], spp=4)
     for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
@@ -278,7 +278,8 @@ def test_js_reference_estimator_frame_equals_c_abi(tmp_path):
     info = ctx.radiosity_build()
     ctx.radiosity_update(True, 96)
     cam.update(cam_index=0, max_depth=1, refine=6, refine_light_samples=1,
-               flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | lib.VT_OPT_REFINE_SHADOWS)
+               flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | lib.VT_OPT_REFINE_SHADOWS | lib.VT_OPT_HIT_CACHE
+               | lib.VT_OPT_SHADOW_CACHE)
     ctx.set_camera(0, cam)
     ctx.render([0])
     lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])

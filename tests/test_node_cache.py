@@ -90,6 +90,48 @@ def test_oracle_reference_estimator_sphere_closed_form():
     assert pt[:, :, 0].mean() == pytest.approx(want, rel=0.02)
 
 
+def _direct_inputs(sc, poses=None):
+    """Estimator inputs for direct-light frames (VT_OPT_DIRECT_ONLY): the node cache is never read."""
+    alb, hh = oracle.albedo_tables(sc)
+    V = np.asarray(sc["verts"]).size // 8
+    out = dict(sc, node_state=np.zeros((V, 16), np.float32), alb=alb, hh=hh, slot=1, e_avg=np.zeros(3, np.float32))
+    if poses is not None:
+        out["poses"] = poses
+    return out
+
+
+def test_oracle_still_light_pins():
+    """The still-light shading of static surfaces (shade.wgsl.js:272-301) tests each light sample against the static
+    scene, then against the bodies: the visibility of one shadow ray against both, so a frame without the refinement
+    pass equals the traced-shadow estimator's exactly, posed bodies included. Its refinement keys mark body shadows,
+    penumbrae and light samples whose luminances spread by more than 20 %: with one light sample and no bodies none
+    can occur, and refining 'shadow' pixels adds nothing to the pixels on edges (same image, same rays); with four
+    light samples it refines more pixels than edges alone and fewer than every pixel that traced a shadow ray."""
+    from test_bodies import _poses
+    R = lib.VT_OPT_REFERENCE | lib.VT_OPT_DIRECT_ONLY
+    parts = scenes.posed_parts()
+    sc = _direct_inputs(scenes.posed_scene(*parts), _poses(True))
+    cam = dict(scenes.posed_camera(96, 72, spp=4), light_samples=4)
+    a, _, _ = oracle.render_tile(sc, dict(cam, flags=R), 0, 0, 96, 72)
+    b, _, _ = oracle.render_tile(sc, dict(cam, flags=R | lib.VT_OPT_SHADOW_CACHE), 0, 0, 96, 72)
+    assert np.array_equal(a, b) and a[..., :3].max() > 0
+    sc1, cams = scenes.c1(bvh=OBVH)
+    sc1 = _direct_inputs(sc1)
+    for nl in (1, 4):
+        base = dict(cams[0], spp=1, light_samples=nl, refine=4, refine_light_samples=1)
+        run = {}
+        for name, f in (("edges", R), ("traced", R | lib.VT_OPT_REFINE_SHADOWS),
+                        ("still", R | lib.VT_OPT_REFINE_SHADOWS | lib.VT_OPT_SHADOW_CACHE)):
+            run[name] = oracle.render_tile(sc1, dict(base, flags=f), 0, 0, 640, 480)[:2]
+        print("light samples %d: rays edges %d, still light %d, traced shadows %d"
+              % (nl, run["edges"][1], run["still"][1], run["traced"][1]))
+        if nl == 1:
+            assert np.array_equal(run["still"][0], run["edges"][0]) and run["still"][1] == run["edges"][1]
+            assert run["traced"][1] > run["edges"][1]
+        else:
+            assert run["edges"][1] < run["still"][1] < run["traced"][1]
+
+
 # ---------------------------------------------------------------------------- GPU parity
 
 def _gpu_cache(ctx, sc, K=64, samples=4, nbr_k=16, radius=0.006, sigma=0.003, iters=0, poses=None):
@@ -258,6 +300,51 @@ def test_gpu_reference_estimator_with_bodies(ctx):
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         print("GPU vs oracle: relative RMSE %.3g" % rel)
         assert rel < 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_camera_caches_are_exact(ctx):
+    """The fixed camera's caches (hitCache, shadowCache; gpu/renderer.js:940-957): frames that read the static first
+    hits and static occlusion kept from earlier frames -- the bodies moved in between -- equal frames rendered from
+    empty caches bit for bit, with and without the first-hit cache, refinement pass included; the cached frame traces
+    fewer rays; the still-light frame against the oracle."""
+    from test_bodies import _poses
+    parts = scenes.posed_parts()
+    sc = scenes.posed_scene(*parts)
+    pa, pb = _poses(False), _poses(True)
+    for p in (pa, pb):
+        p[0].update(occ_a=(-0.25, 0, 0), occ_b=(0.25, 0, 0), occ_r=0.15, rho_eff=0.3)
+        p[1].update(occ_a=(0, 0, -0.3), occ_b=(0, 0, 0.3), occ_r=0.15, rho_eff=0.5)
+    t = _gpu_cache(ctx, sc, poses=pa)
+    ctx.radiosity_update(direct=True, iters=16)
+    nodes = np.asarray(sc["verts"]).size // 8
+    st, e = ctx.radiosity_state(nodes)
+    still = lib.VT_OPT_REFERENCE | lib.VT_OPT_REFINE_SHADOWS | lib.VT_OPT_SHADOW_CACHE
+    cam = dict(scenes.posed_camera(96, 72, spp=2), light_samples=4, refine=4, refine_light_samples=2)
+    for k, flags in enumerate((still | lib.VT_OPT_HIT_CACHE, still)):
+        print("fixed-camera caches, options %d" % flags, flush=True)
+        ctx.set_instances(pa)
+        kept, fresh_slot = 2 * k, 2 * k + 1
+        ctx.set_camera(kept, dict(cam, flags=flags))
+        ctx.render([kept])
+        first, r1 = ctx.readback(kept, lib.VT_KIND_LINEAR, 96, 72), ctx.stats()["rays"]
+        ctx.render([kept])
+        again, r2 = ctx.readback(kept, lib.VT_KIND_LINEAR, 96, 72), ctx.stats()["rays"]
+        print("rays: %d from empty caches, %d from kept ones" % (r1, r2))
+        assert np.array_equal(first, again) and r2 < r1
+        ctx.set_instances(pb)
+        ctx.render([kept])
+        moved = ctx.readback(kept, lib.VT_KIND_LINEAR, 96, 72)
+        ctx.set_camera(fresh_slot, dict(cam, flags=flags))      # a slot of its own: empty caches
+        ctx.render([fresh_slot])
+        fresh = ctx.readback(fresh_slot, lib.VT_KIND_LINEAR, 96, 72)
+        assert np.array_equal(moved, fresh) and not np.array_equal(moved, first)
+    sc2 = dict(sc, node_state=st, alb=t["alb"], hh=t["hh"], slot=1, e_avg=e, poses=pb)
+    ref, _, _ = oracle.render_tile(sc2, dict(cam, flags=still), 0, 0, 96, 72)
+    a, b = moved[:, :, :3].astype(np.float64), ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("GPU still light vs oracle: relative RMSE %.3g" % rel)
+    assert rel < 1e-3
 
 
 @pytest.mark.gpu

@@ -256,9 +256,12 @@ def test_gpu_twin_reference_estimator_matches_oracle(loaded, prep):
 
 @pytest.mark.gpu
 @needs_assets
-def test_gpu_twin_judge_preset_matches_oracle(loaded, prep):
+@pytest.mark.parametrize("caches", [False, True])
+def test_gpu_twin_judge_preset_matches_oracle(loaded, prep, caches):
     """The reference's judge preset on the twin (1 sub-sample x 4 light samples, then 6 refinement sub-samples on
-    edges, posed bodies and traced-shadow pixels): CAM3 around the instruments against the oracle's two passes."""
+    edges, posed bodies and shadow pixels): CAM3 around the instruments against the oracle's two passes; with the
+    fixed camera's caches (the reference's default: the still-light shading and its keys), a second frame from the
+    kept caches equal to the first."""
     scene, _, _ = loaded
     cams, poses = _twin_frame(prep, scene, 1, 1)
     ctx = lib.Context(0)
@@ -276,11 +279,19 @@ def test_gpu_twin_judge_preset_matches_oracle(loaded, prep):
         ctx.radiosity_update(direct=True, iters=96)
         st, e = ctx.radiosity_state(np.asarray(prep["verts"]).size // 8)
         flags = lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | lib.VT_OPT_REFINE_SHADOWS
+        if caches:
+            flags |= lib.VT_OPT_HIT_CACHE | lib.VT_OPT_SHADOW_CACHE
         cam = dict(cams[2], flags=flags, light_samples=4, refine=6, refine_light_samples=1)
+        print("twin judge preset, caches %s" % caches, flush=True)
         ctx.set_camera(0, cam)
         ctx.render([0])
         img = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
         samples = ctx.stats()["samples"]
+        if caches:
+            r1 = ctx.stats()["rays"]
+            ctx.render([0])
+            assert np.array_equal(ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"]), img)
+            print("rays: %d from empty caches, %d from kept ones" % (r1, ctx.stats()["rays"]))
     finally:
         ctx.close()
     px = cam["width"] * cam["height"]

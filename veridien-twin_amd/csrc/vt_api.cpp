@@ -23,6 +23,19 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// what a camera's static first hits depend on: its rays (pose, lens, size, sample pattern) and the static scene;
+// the occlusion cache adds the light samples per sub-sample and the shading inputs that decide which samples light
+// a point (materials, lights, profiles)
+struct CacheKey {
+  float R[9], pos[3];
+  uint32_t mode;
+  float lens[9];
+  uint32_t width, height, seed, cam_index, nl, nl_refine;
+  uint64_t scene_gen, shade_gen;
+  bool operator==(const CacheKey &o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
+};
+static_assert(sizeof(CacheKey) == 28 * 4 + 2 * 8, "CacheKey has no padding (compared bytewise)");
+
 struct CamSlot {
   bool set = false;
   vt_camera c{};
@@ -30,6 +43,11 @@ struct CamSlot {
   // glare levels; ISP: linear RGB, block partial sums, 8-bit RGBA, YUV planes, statistics, rgb8 staging
   DevBuf glare_a, glare_b, isp_rgb, isp_part, rgba, yuv, stats, rgb8;
   DevBuf keys;                // the reference estimator's first-pass keys (refinement)
+  // the static first-hit and static-occlusion caches of a fixed camera, valid while their keys hold, for the first
+  // `layers` sub-sample indices
+  DevBuf hcache, scache;
+  uint32_t hc_layers = 0, sc_layers = 0;
+  CacheKey hc_key{}, sc_key{};
   vt_sensor_params sp{};      // the last vt_sensor's parameters (the ISP reads black, max DN, noise model, CFA)
   bool isp_done = false, isp420 = false;
   bool ri_valid = false;
@@ -48,6 +66,7 @@ struct vt_ctx {
   int cu_count = 0;
   // scene
   bool has_scene = false;
+  uint64_t scene_gen = 0, shade_gen = 0;   // bumped by a scene upload / a material, light or profile change
   DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots, tvid;
   // host copies the node radiosity build reads: vertices (8 floats each), per slot vertex ids + mesh, meshes,
   // the static scene's slot count (its slots come first), the emitter records
@@ -325,7 +344,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
     release(cs.lin);
     release(cs.adj);
     DevBuf *isp[] = {&cs.glare_a, &cs.glare_b, &cs.isp_rgb, &cs.isp_part, &cs.rgba, &cs.yuv, &cs.stats, &cs.rgb8,
-                     &cs.keys};
+                     &cs.keys, &cs.hcache, &cs.scache};
     for (DevBuf *b : isp) release(*b);
   }
   for (hipEvent_t ev : c->ev_launch) (void)hipEventDestroy(ev);
@@ -586,6 +605,7 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->n_mat = s->material_count;
   c->n_light = s->light_count;
   c->has_scene = true;
+  c->scene_gen++;
   VT_HIP(c, hipMemsetAsync(c->overflow.p, 0, 64, c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   return VT_OK;
@@ -598,6 +618,7 @@ extern "C" int vt_update_materials(vt_ctx *c, const float *materials, uint32_t c
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(c->mats.p, materials, (size_t)count * 64, hipMemcpyHostToDevice));
+  c->shade_gen++;
   return VT_OK;
 }
 
@@ -608,6 +629,7 @@ extern "C" int vt_update_lights(vt_ctx *c, const float *lights, uint32_t count) 
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));
   VT_HIP(c, hipMemcpy(c->lights.p, lights, (size_t)count * 16, hipMemcpyHostToDevice));
+  c->shade_gen++;
   return VT_OK;
 }
 
@@ -637,6 +659,7 @@ extern "C" int vt_set_light_profile(vt_ctx *c, uint32_t light, const float axis[
   VT_HIP(c, hipMemcpy((float4 *)c->lprof.p + light, &rec, sizeof rec, hipMemcpyHostToDevice));
   VT_HIP(c, hipMemcpy((float2 *)c->knots.p + (size_t)light * MAX_KNOTS, kn.data(), kn.size() * sizeof(float2),
                       hipMemcpyHostToDevice));
+  c->shade_gen++;
   return VT_OK;
 }
 
@@ -797,6 +820,50 @@ int persistent_grid(vt_ctx *c, const Shape &sh, uint64_t waves, uint32_t &grid, 
   return VT_OK;
 }
 
+// a fixed camera's first-hit (VT_OPT_HIT_CACHE) and occlusion (VT_OPT_SHADOW_CACHE) caches for a whole frame of
+// d.nspp + d.refine sub-samples: (re)allocated and cleared to empty by a fill kernel on the frame's stream when the
+// key changes or more layers are needed, else kept from the last frame
+int camera_caches(vt_ctx *c, CamSlot &s, DevCam &d) {
+  const vt_camera &k = s.c;
+  CacheKey key;
+  std::memset(&key, 0, sizeof key);
+  std::memcpy(key.R, k.R, sizeof key.R);
+  std::memcpy(key.pos, k.pos, sizeof key.pos);
+  key.mode = k.mode;
+  std::memcpy(key.lens, k.lens, sizeof key.lens);
+  key.width = k.width;
+  key.height = k.height;
+  key.seed = k.seed;
+  key.cam_index = k.cam_index;
+  key.scene_gen = c->scene_gen;
+  const uint32_t layers = d.nspp + d.refine;
+  const uint64_t px = (uint64_t)k.width * k.height;
+  if (k.flags & VT_OPT_HIT_CACHE) {
+    if (!(key == s.hc_key) || layers > s.hc_layers) {
+      const uint64_t n = px * layers;
+      if (int rc = ensure(c, s.hcache, n * sizeof(float4))) return rc;
+      VT_HIP(c, launch_fill_u4((uint4 *)s.hcache.p, n, make_uint4(0u, 0u, 0u, 0u), c->stream));
+      s.hc_key = key;
+      s.hc_layers = layers;
+    }
+    d.hcache = (float4 *)s.hcache.p;
+  }
+  if (k.flags & VT_OPT_SHADOW_CACHE) {
+    key.nl = d.nl;
+    key.nl_refine = d.nl_refine;
+    key.shade_gen = c->shade_gen;
+    if (!(key == s.sc_key) || layers > s.sc_layers) {
+      const uint64_t n = (px * layers + 3) / 4;     // 32-bit words, cleared 16 bytes at a time
+      if (int rc = ensure(c, s.scache, n * sizeof(uint4))) return rc;
+      VT_HIP(c, launch_fill_u4((uint4 *)s.scache.p, n, make_uint4(0u, 0u, 0u, 0u), c->stream));
+      s.sc_key = key;
+      s.sc_layers = layers;
+    }
+    d.scache = (uint32_t *)s.scache.p;
+  }
+  return VT_OK;
+}
+
 // the camera table of a frame into the pinned staging area; returns the frame's queue items
 int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opts &opt, uint64_t &items) {
   if (!cams || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "render: 1..%d cameras", VT_MAX_CAMERAS);
@@ -848,6 +915,11 @@ int stage_frame(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_render_opt
       int rc = ensure(c, s.keys, (size_t)k.width * k.height * sizeof(uint4));
       if (rc) return rc;
       d.keys = (uint4 *)s.keys.p;
+    }
+    if ((k.flags & VT_OPT_REFERENCE) && (k.flags & (VT_OPT_HIT_CACHE | VT_OPT_SHADOW_CACHE)) && s0 == 0 &&
+        !opt.accumulate) {
+      int rc = camera_caches(c, s, d);
+      if (rc) return rc;
     }
     items += d.item_count;
   }

@@ -76,6 +76,13 @@ struct DevCam {
   // 0x10000 posed body | 0x20000 shadow ray traced}; null keys: no refinement this frame
   uint32_t refine, nl_refine;
   uint4 *keys;
+  // the static first-hit cache of a fixed camera (staticFirstHit, kernels.wgsl.js:58-67): per sub-sample layer i
+  // and pixel, (t, triangle, u, v) of the static scene's first hit; t > 0 marks an entry written (a miss keeps
+  // t = 1e30, triangle MISS). Null: no cache.
+  float4 *hcache;
+  // the static occlusion cache of a still light (shadowCache): per sub-sample layer and pixel, bit j = light sample
+  // j hidden by the static scene, bit 31 = written. Null: no cache.
+  uint32_t *scache;
 };
 
 // KB4 inverse, FP32 with a fixed 8 Newton steps (lens.wgsl.js:9-29)

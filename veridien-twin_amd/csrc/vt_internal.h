@@ -53,6 +53,7 @@ hipError_t launch_refine_mark(const DevCam *cam, uint32_t c, uint32_t width, uin
 hipError_t launch_refine(const RenderLaunch &L, const uint2 *list, const uint32_t *count, uint64_t max_items,
                          uint32_t R, hipStream_t s);
 hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s);
+hipError_t launch_fill_u4(uint4 *p, uint64_t n, uint4 v, hipStream_t s);
 hipError_t render_occupancy(uint32_t block, uint32_t stack_lds, bool inst, bool ref, int *blocks_per_cu);
 // path-replay adjoint: gradients of sum_p adj_p . I_p with respect to material colours and light radiances
 // (block 256, persistent queue; L.cams[*].adj set)

@@ -7,7 +7,8 @@ namespace vt {
 
 constexpr uint32_t F_STATIC = 1u;
 constexpr uint32_t OPT_NO_INDIRECT = 4u, OPT_DIRECT_ONLY = 8u, OPT_NO_CAPSULE = 32u, OPT_NORMALS = 64u,
-                   OPT_NO_MIS = 128u, OPT_REFINE_SHADOWS = 1024u, OPT_REFERENCE = 1u << 16,
+                   OPT_NO_MIS = 128u, OPT_REFINE_SHADOWS = 1024u, OPT_SHADOW_CACHE = 8192u,
+                   OPT_REFERENCE = 1u << 16,
                    OPT_CULL_SHADOWS = 1u << 17;
 
 // gather-table ray j of node i (normal n): K cosine-distributed rays, a Fibonacci spiral on the disk rotated per

@@ -242,41 +242,115 @@ __device__ __forceinline__ float cached_visibility(const SceneDev &S, const Surf
   return a;
 }
 
-// one emitter sample's contribution at the surface (directLight, shade.wgsl.js:160-188); a traced shadow ray marks
-// the pixel's key (0x20000)
-template <bool INST>
-__device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float time, float u0,
-                                           float u1, float u2, bool mis, float fnl, float cached, Stack &st,
-                                           uint32_t &nrays, uint32_t &key) {
+// one emitter sample's unshadowed contribution and its shadow segment (lightEval, shade.wgsl.js:135-157): ok false
+// when the sample cannot light the point (back of the emitter or the surface, outside the emission profile, a zero
+// BSDF); the segment starts off the surface on the lit side and ends 20 um short of the emitter
+struct LightEval {
+  v3 c, so, wi;
+  float tmax;
+  bool ok;
+};
+
+__device__ __forceinline__ LightEval light_eval(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float u0,
+                                                float u1, float u2, bool mis, float fnl) {
+  LightEval e;
+  e.ok = false;
   v3 pl;
-  const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+  const uint32_t em = sample_emitter(S, u0, u1, u2, pl);
   v3 wi = pl - s.p;
   const float d2 = dot(wi, wi), dist = sqrtf(d2);
   wi = wi * (1.0f / dist);
-  const float cosE = -dot(wi, ld3(S.emit[4 * e + 3]));
-  if (cosE <= 0.0f) return mk(0.f, 0.f, 0.f);
+  const float cosE = -dot(wi, ld3(S.emit[4 * em + 3]));
+  if (cosE <= 0.0f) return e;
   const float cosS = dot(s.ns, wi);
-  if (cosS <= 0.0f || dot(s.ng, wi) <= 0.0f) return mk(0.f, 0.f, 0.f);
-  const uint32_t lid = __float_as_uint(S.emit[4 * e + 2].w);
+  if (cosS <= 0.0f || dot(s.ng, wi) <= 0.0f) return e;
+  const uint32_t lid = __float_as_uint(S.emit[4 * em + 2].w);
   const v3 Le = ld3(S.lights[lid]) * light_profile(S, lid, -wi);
-  if (maxc(Le) <= 0.0f) return mk(0.f, 0.f, 0.f);
+  if (maxc(Le) <= 0.0f) return e;
   const v3 f = eval_bsdf(M, s.ns, wi, wo);
-  if (maxc(f) <= 0.0f) return mk(0.f, 0.f, 0.f);
-  if (cached == 0.0f) return mk(0.f, 0.f, 0.f);
-  if (cached < 0.0f) {
-    Hit sh;
-    nrays++;
-    key |= 0x20000u;
-    const v3 so = s.p + s.ng * (dot(s.ng, wi) > 0.0f ? EPS_OFF : -EPS_OFF);
-    if (trace_scene<true, INST>(S, so, wi, 0.0f, dist - 2e-5f, time, S.shadow_skip, sh, st)) return mk(0.f, 0.f, 0.f);
-  }
+  if (maxc(f) <= 0.0f) return e;
   const float pW = d2 / (cosE * S.emit_area);
   float w = 1.0f;
   if (mis) {
     const float pb = pdf_bsdf(M, s.ns, wo, wi);
     w = fnl * fnl * pW * pW / (fnl * fnl * pW * pW + pb * pb);
   }
-  return f * Le * (cosS * w / pW);
+  e.c = f * Le * (cosS * w / pW);
+  e.so = s.p + s.ng * EPS_OFF;      // dot(ng, wi) > 0 here: the lit side
+  e.wi = wi;
+  e.tmax = dist - 2e-5f;
+  e.ok = true;
+  return e;
+}
+
+// one emitter sample's contribution at the surface (directLight, shade.wgsl.js:160-188): `cached` 1 / 0 answers the
+// shadow test (cachedVisibility), -1 traces it; a traced shadow ray marks the pixel's key (0x20000)
+template <bool INST>
+__device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float time, float u0,
+                                           float u1, float u2, bool mis, float fnl, float cached, Stack &st,
+                                           uint32_t &nrays, uint32_t &key) {
+  const LightEval e = light_eval(S, s, M, wo, u0, u1, u2, mis, fnl);
+  if (!e.ok || cached == 0.0f) return mk(0.f, 0.f, 0.f);
+  if (cached < 0.0f) {
+    Hit sh;
+    nrays++;
+    key |= 0x20000u;
+    if (trace_scene<true, INST>(S, e.so, e.wi, 0.0f, e.tmax, time, S.shadow_skip, sh, st)) return mk(0.f, 0.f, 0.f);
+  }
+  return e.c;
+}
+
+// the nL light samples of a static surface under a still light (shade.wgsl.js:272-301): each lit sample is tested
+// against the static scene -- the answer kept in the camera's occlusion cache word for this sub-sample and pixel
+// when there is one (bit j, bit 31 = written) -- and, if clear, against the visible bodies. The key's 0x20000 marks
+// a body's shadow, a penumbra (some lit samples hidden, some not) or visible samples whose luminances spread by
+// more than 20 % of their midpoint. Returns the sum of the visible samples.
+template <bool INST>
+__device__ __forceinline__ v3 still_light(const SceneDev &S, const DevCam &C, const PixSeeds &P, const Surf &s,
+                                          const Mat &M, v3 wo, float time, uint32_t x, uint32_t y, uint32_t i,
+                                          uint32_t nl, Stack &st, uint32_t &nrays, uint32_t &key) {
+  uint32_t *const sc = C.scache ? C.scache + ((size_t)i * C.height + y) * C.width + x : nullptr;
+  const uint32_t word = sc ? *sc : 0u;
+  const bool valid = (word & 0x80000000u) != 0u;
+  const bool bodies = INST && S.inst_count;
+  const float fnl = (float)nl;
+  uint32_t nw = 0x80000000u, nvis = 0, nlit = 0;
+  float lmin = INFINITY, lmax = 0.0f;
+  v3 L = mk(0.f, 0.f, 0.f);
+  for (uint32_t j = 0; j < nl; j++) {
+    const uint32_t li = i * nl + j;
+    const LightEval e = light_eval(S, s, M, wo, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
+                                   false, fnl);
+    if (!e.ok) continue;
+    nlit++;
+    Hit sh;
+    bool occ;
+    if (valid) {
+      occ = ((word >> j) & 1u) != 0u;
+    } else {
+      occ = traverse<true>(S, S.root, e.so, e.wi, 0.0f, e.tmax, sh, st);
+      if (occ) nw |= 1u << j;
+    }
+    if (!valid || (bodies && !occ)) nrays++;      // one shadow segment, whichever parts of the scene it met
+    if (occ) continue;
+    if (bodies) {
+      bool hit = false;
+      trace_instances<true>(S, e.so, e.wi, 0.0f, e.tmax, time, S.shadow_skip, sh, hit, st);
+      if (hit) {
+        key |= 0x20000u;
+        continue;
+      }
+    }
+    nvis++;
+    L = L + e.c;
+    const float yl = luminance(e.c);
+    lmin = fminf(lmin, yl);
+    lmax = fmaxf(lmax, yl);
+  }
+  if (sc && !valid) *sc = nw;
+  if (nvis > 0 && nvis < nlit) key |= 0x20000u;
+  if (nvis > 1 && lmax - lmin > 0.2f * (lmax + lmin) * 0.5f) key |= 0x20000u;
+  return L;
 }
 
 // radiance along a body's BSDF-sampled secondary ray (secondary, shade.wgsl.js:222-243): the emitters with MIS,
@@ -319,7 +393,27 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
   Hit h;
   nrays++;
   key = 0xfffeu;
-  if (!trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, C.skip, h, st)) return mk(0.f, 0.f, 0.f);
+  bool found;
+  if (C.hcache) {
+    // the static first hit from the camera's cache (written on the first frame after a change), then the bodies
+    float4 *hc = C.hcache + ((size_t)i * C.height + y) * C.width + x;
+    const float4 c = *hc;
+    if (c.x > 0.0f) {
+      found = __float_as_uint(c.y) != MISS;
+      h.t = c.x;
+      h.tri = __float_as_uint(c.y);
+      h.u = c.z;
+      h.v = c.w;
+    } else {
+      found = traverse<false>(S, S.root, o, d, 0.0f, INFINITY, h, st);
+      *hc = found ? make_float4(h.t, __uint_as_float(h.tri), h.u, h.v) : make_float4(1e30f, __uint_as_float(MISS), 0.f, 0.f);
+    }
+    h.inst = NO_INST;
+    if (INST && S.inst_count) trace_instances<false>(S, o, d, 0.0f, INFINITY, time, C.skip, h, found, st);
+  } else {
+    found = trace_scene<false, INST>(S, o, d, 0.0f, INFINITY, time, C.skip, h, st);
+  }
+  if (!found) return mk(0.f, 0.f, 0.f);
   const Surf s = surface_at<INST>(S, h, o, d, time);
   key = (s.mesh & 0xffffu) | ((h.inst != NO_INST && !(s.ms.y & F_STATIC)) ? 0x10000u : 0u);
   const v3 wo = -d;
@@ -334,13 +428,18 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
   const bool mis = !stat && !(opts & OPT_NO_MIS);
   const uint32_t nl = nl_in < 1 ? 1 : (nl_in > 16 ? 16 : nl_in);
   const float fnl = (float)nl;
-  const float cached = (stat && (opts & OPT_CULL_SHADOWS)) ? cached_visibility<INST>(S, s) : -1.0f;
+  const float cached = (stat && (opts & OPT_CULL_SHADOWS) && !(opts & OPT_SHADOW_CACHE))
+                           ? cached_visibility<INST>(S, s) : -1.0f;
   v3 L = mk(0.f, 0.f, 0.f);
   if (S.emit_count > 0) {
-    for (uint32_t j = 0; j < nl; j++) {
-      const uint32_t li = i * nl + j;
-      L = L + direct_light<INST>(S, s, M, wo, time, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
-                                 mis, fnl, cached, st, nrays, key);
+    if (stat && (opts & OPT_SHADOW_CACHE)) {
+      L = still_light<INST>(S, C, P, s, M, wo, time, x, y, i, nl, st, nrays, key);
+    } else {
+      for (uint32_t j = 0; j < nl; j++) {
+        const uint32_t li = i * nl + j;
+        L = L + direct_light<INST>(S, s, M, wo, time, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
+                                   mis, fnl, cached, st, nrays, key);
+      }
     }
   }
   L = L * (1.0f / fnl);
@@ -1079,6 +1178,12 @@ hipError_t launch_refine(const RenderLaunch &L, const uint2 *list, const uint32_
   else hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, L, list, count, R);
   return hipGetLastError();
 }
+hipError_t launch_fill_u4(uint4 *p, uint64_t n, uint4 v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_keys(uint4 *keys, uint64_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, n,

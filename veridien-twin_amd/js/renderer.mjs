@@ -15,7 +15,7 @@ export const STATIC_BODY = 0xffffffff;
 
 // camera option bits (gpu/wgsl/common.wgsl.js:45-58), plus the estimator choice and the shadow culling
 export const OPTION_BITS = Object.freeze({ NO_INDIRECT: 4, DIRECT_ONLY: 8, NO_CAPSULE: 32, NORMALS: 64, NO_MIS: 128,
-  REFINE_SHADOW: 1024, REFERENCE: 1 << 16, CULL_SHADOWS: 1 << 17 });
+  REFINE_SHADOW: 1024, HIT_CACHE: 2048, SHADOW_CACHE: 8192, REFERENCE: 1 << 16, CULL_SHADOWS: 1 << 17 });
 export const DEFAULTS = Object.freeze({
   spp: 16,            // sub-samples per pixel
   maxDepth: 3,        // path vertices (camera hit = 1), path tracer only
@@ -35,6 +35,8 @@ export const DEFAULTS = Object.freeze({
   refine: 6,          // reference estimator: extra sub-samples on edges, posed bodies and traced-shadow pixels
   refineLightSamples: 1,
   refineShadows: true,
+  hitCache: true,     // fixed cameras: each sub-sample's static first hit is traced once and kept (exact)
+  shadowCache: true,  // still light: each sub-sample's static occlusion per light sample is kept (exact)
   radiosityEvery: 1,  // update the node cache every n-th pose change
 });
 // the reference's operating points for its camera estimator (gpu/renderer.js:63-67), spelled out in full
@@ -190,8 +192,13 @@ export class TwinRenderer {
     if (o.noMIS) bits |= OPTION_BITS.NO_MIS;
     if (dflt(o.estimator, this.opts.estimator) === 'reference') {
       bits |= OPTION_BITS.REFERENCE;
-      if (dflt(o.cullShadows, this.opts.cullShadows)) bits |= OPTION_BITS.CULL_SHADOWS;
+      if (!o.exactShadows && dflt(o.cullShadows, this.opts.cullShadows)) bits |= OPTION_BITS.CULL_SHADOWS;
       if (dflt(o.refineShadows, this.opts.refineShadows)) bits |= OPTION_BITS.REFINE_SHADOW;
+      // the caches of a fixed camera (gpu/renderer.js:944-957): the occlusion cache only with the hit cache
+      if (this.opts.hitCache !== false && o.hitCache !== false) {
+        bits |= OPTION_BITS.HIT_CACHE;
+        if (this.opts.shadowCache !== false && o.shadowCache !== false && !o.exactShadows) bits |= OPTION_BITS.SHADOW_CACHE;
+      }
     }
     return bits >>> 0;
   }

@@ -26,6 +26,7 @@ EXPORTS = [
 # camera option bits (vt_camera.flags)
 VT_OPT_NO_INDIRECT, VT_OPT_DIRECT_ONLY, VT_OPT_NO_CAPSULE, VT_OPT_NORMALS, VT_OPT_NO_MIS = 4, 8, 32, 64, 128
 VT_OPT_REFERENCE, VT_OPT_CULL_SHADOWS, VT_OPT_REFINE_SHADOWS = 1 << 16, 1 << 17, 1024
+VT_OPT_HIT_CACHE, VT_OPT_SHADOW_CACHE = 2048, 8192
 
 vp, u32, i32, f32, f64, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_size_t
 CAMERA_PARAMS = 14      # VT_CAMERA_PARAMS: fx fy cx cy k1 k2 k3 k4, rotation rx ry rz, centre tx ty tz

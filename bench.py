@@ -394,6 +394,7 @@ def bench_twin_preset(args, preset):
     ctx.set_instances(twin.instances(prep, twin.poses_from_state(scene, state)))
     t0 = time.time()
     info = ctx.radiosity_build()
+    interior = ctx.interior_bvh(True)          # the reference builds it with the node cache
     n_mat = np.asarray(prep["materials"]).size // 16
     twin.rho_eff(prep, ctx.radiosity_tables(6, (n_mat, 4), np.float32)[:, :3])
     ctx.radiosity_update(direct=True, iters=96)
@@ -433,7 +434,7 @@ def bench_twin_preset(args, preset):
     px = sum(c["width"] * c["height"] for c in cams)
     print(json.dumps({"metric": METRIC, "config": {"workload": "reference '%s' preset on the real twin scene: 3 KB4 "
                                                                "cameras 1280x720, reference estimator, spp %d, %d light "
-                                                               "samples, refine %d x %d light samples, shadow culling%s, "
+                                                               "samples, refine %d x %d light samples, shadow culling%s, interior tree, "
                                                                "tools moving, node radiosity (direct + %d Jacobi steps) "
                                                                "every %s frame, sensor + ISP"
                                                    % (preset, P["spp"], P["nl"], P["refine"], P["nl_refine"],
@@ -443,6 +444,7 @@ def bench_twin_preset(args, preset):
                       "refined_pixel_fraction": round((st["samples"] - px * P["spp"]) / max(1, P["refine"]) / px, 4)
                       if P["refine"] else 0.0,
                       "nodes": info["nodes"], "active_nodes": info["active"], "node_build_s": round(t_build, 2),
+                      "interior_static_tris": interior,
                       "last_render_ms": round(st["ms_render"], 3), "last_isp_ms": round(st["ms_isp"], 3),
                       "reference_published_ms_m4_air": P["published_ms"]}), flush=True)
 

@@ -385,6 +385,12 @@ typedef struct {
   double ms_build;
 } vt_radiosity_info;
 int vt_radiosity_build(vt_ctx *ctx, const vt_radiosity_opts *opts, vt_radiosity_info *info);
+/* The interior tree (_buildInteriorBVH, gpu/renderer.js:533-571), after vt_radiosity_build: enable 1 builds (once
+ * per node cache) a second static BVH without the triangles whose three nodes' gather rays escape in >= 90 % of
+ * cases (bench triangles always kept) and makes it the root of every later camera, shadow, node and trace walk;
+ * enable 0 returns to the whole scene (the reference's fullBVH). vt_visible always walks the whole scene.
+ * interior_tris (optional): the static triangles the walks now see. */
+int vt_interior_bvh(vt_ctx *ctx, uint32_t enable, uint32_t *interior_tris);
 /* One node update (_ensureRadiosity / converge): with direct != 0, nodeDirect (node_samples emitter samples per
  * active node at mid-exposure, shadow rays through the static scene and the posed bodies, the capsule visibility
  * of its indirect light); then `iters` Jacobi steps (jacobiGather + jacobiUpdate, ping-pong source slots), warm

@@ -1,6 +1,7 @@
 """Copies the real twin scene the tests and `bench.py --config twin` render into <repo>/twin_assets/ (kept out of
 git: 13.5 MB of mesh data): the reference's assets/scene.json, the 44 mesh files it lists, and
-configs/twin/physics.json (materials, emitter, light profile, scope). Run in a container where the reference is
+configs/twin/physics.json (materials, emitter, light profile, scope), and the baked node table of the lite renderer
+(assets/lite.json, assets/lite_nodes.bin). Run in a container where the reference is
 mounted; the folder then travels with the working tree.
 
     python tests/golden/fetch_twin_assets.py [/root/reference]
@@ -22,7 +23,11 @@ def main(ref="/root/reference"):
     for m in scene["meshes"]:
         shutil.copyfile(os.path.join(src, m["file"]), os.path.join(dst, m["file"]))
     shutil.copyfile(os.path.join(ref, "configs", "twin", "physics.json"), os.path.join(dst, "physics.json"))
-    print("twin assets: %d meshes -> %s" % (len(scene["meshes"]), dst))
+    # the lite renderer's baked node table (12,976 nodes x 256 gather hits): real sparse input for the Jacobi step
+    os.makedirs(os.path.join(dst, "lite"), exist_ok=True)
+    for name in ("lite.json", "lite_nodes.bin"):
+        shutil.copyfile(os.path.join(src, name), os.path.join(dst, "lite", name))
+    print("twin assets: %d meshes, the baked node table -> %s" % (len(scene["meshes"]), dst))
 
 
 if __name__ == "__main__":

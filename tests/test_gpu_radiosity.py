@@ -50,9 +50,8 @@ def _tables(N=1500, A=1100, K=64, nbrk=16, nmat=5, bins=32, seed=17):
     return t, st
 
 
-def test_sparse_jacobi_matches_oracle(ctx):
-    t, st = _tables()
-    got, aux = ctx.jacobi_sparse(t, st, 3)
+def _oracle_jacobi(t, st, iters):
+    """The oracle's gather + update steps on the same tables: the new state and the last step's aux."""
     want = st.copy()
     N, A, K = t["n_static"], t["n_active"], t["gather_k"]
     verts = np.zeros((N, 8), np.float32)
@@ -63,13 +62,56 @@ def test_sparse_jacobi_matches_oracle(ctx):
     aux_o = np.zeros((A, 4), np.float32)
     nbr, nbr_w = np.ascontiguousarray(t["nbr"]), np.ascontiguousarray(t["nbr_w"])
     slot = 1
-    for _ in range(3):
+    for _ in range(iters):
         oracle.lib().or_jacobi_gather(A, oracle.ptr(t["active"]), oracle.ptr(t["gtable"]), K, oracle.ptr(verts),
                                       oracle.ptr(t["node_mat"]), oracle.ptr(spec), oracle.ptr(diff), 32, oracle.ptr(hh),
                                       oracle.ptr(want), slot, oracle.ptr(aux_o))
         oracle.lib().or_jacobi_update(A, oracle.ptr(t["active"]), oracle.ptr(nbr), oracle.ptr(nbr_w), t["nbr_k"],
                                       oracle.ptr(aux_o), oracle.ptr(t["node_mat"]), oracle.ptr(hh), oracle.ptr(want), 3 - slot)
         slot = 3 - slot
+    return want, aux_o
+
+
+def test_sparse_jacobi_matches_oracle(ctx):
+    t, st = _tables()
+    got, aux = ctx.jacobi_sparse(t, st, 3)
+    want, aux_o = _oracle_jacobi(t, st, 3)
+    assert np.allclose(aux[:, 0:3], aux_o[:, 0:3], rtol=2e-5, atol=1e-6)
+    assert np.allclose(got, want, rtol=2e-5, atol=1e-6)
+
+
+def test_sparse_jacobi_on_the_baked_table(ctx):
+    """Real sparse input: the lite renderer's baked gather table of the twin (12,976 nodes x 256 hits, 4.3 % misses)
+    with its node normals and mesh ids, smoothing neighbours from its node positions; materials and the starting
+    state seeded. Every node active; 4 steps against the oracle."""
+    import os
+    from vtwin import twin
+    root = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "twin_assets")
+    if not os.path.exists(os.path.join(root, "lite", "lite_nodes.bin")):
+        pytest.skip("twin_assets/lite missing: python tests/golden/fetch_twin_assets.py")
+    L = twin.lite_nodes(root)
+    n, K = L["count"], L["K"]
+    meshes, node_mat = np.unique(L["mesh"], return_inverse=True)
+    nmat, bins = meshes.size, 32
+    r = scenes.Rng(23)
+    normals = np.zeros((n, 4), np.float32)
+    normals[:, 0:3] = L["nrm"]
+    alb = np.zeros((nmat, bins, 2, 4), np.float32)
+    alb[:, :, 0, 0:3] = r.uniform(0.0, 0.2, nmat * bins * 3).reshape(nmat, bins, 3)
+    alb[:, :, 1, 0:3] = r.uniform(0.1, 0.7, nmat * bins * 3).reshape(nmat, bins, 3)
+    hh = np.zeros((nmat, 4), np.float32)
+    hh[:, 0:3] = r.uniform(0.2, 0.8, nmat * 3).reshape(nmat, 3)
+    nbr, nbr_w = lib.node_neighbours(L["pos"], L["nrm"], 16, 0.014, 0.00385)
+    st = np.zeros((n, 16), np.float32)
+    st[:, 3] = r.uniform(0.5, 1.0, n)
+    st[:, 4:7] = r.uniform(0, 2, n * 3).reshape(n, 3)
+    st[:, 12:15] = r.uniform(0, 3, n * 3).reshape(n, 3)
+    t = dict(n_static=n, n_active=n, gather_k=K, nbr_k=16, n_mat=nmat, albedo_bins=bins,
+             active=np.arange(n, dtype=np.uint32), gtable=L["gtable"], normals=normals,
+             node_mat=node_mat.astype(np.uint32), albedo=alb, hh=hh, nbr=nbr, nbr_w=nbr_w)
+    print("baked table: %d nodes x %d hits, %d materials" % (n, K, nmat), flush=True)
+    got, aux = ctx.jacobi_sparse(t, st, 4)
+    want, aux_o = _oracle_jacobi(t, st, 4)
     assert np.allclose(aux[:, 0:3], aux_o[:, 0:3], rtol=2e-5, atol=1e-6)
     assert np.allclose(got, want, rtol=2e-5, atol=1e-6)
 

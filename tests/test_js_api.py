@@ -276,6 +276,7 @@ def test_js_reference_estimator_frame_equals_c_abi(tmp_path):
     ctx = lib.Context(0)
     ctx.upload_scene(sc)
     info = ctx.radiosity_build()
+    ctx.interior_bvh(True)
     ctx.radiosity_update(True, 96)
     cam.update(cam_index=0, max_depth=1, refine=6, refine_light_samples=1,
                flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS | lib.VT_OPT_REFINE_SHADOWS | lib.VT_OPT_HIT_CACHE

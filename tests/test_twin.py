@@ -37,6 +37,21 @@ def loaded():
     return twin.load(ASSETS)
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(ASSETS, "lite", "lite_nodes.bin")),
+                    reason="twin_assets/lite missing (python tests/golden/fetch_twin_assets.py)")
+def test_baked_node_table_loads():
+    """The lite renderer's baked node table read as a gather table: its miss fraction equals the one its generator
+    recorded (lite.json stats.gatherMissFrac), every hit names a node of the table, normals are unit vectors and
+    areas positive."""
+    L = twin.lite_nodes(ASSETS)
+    n, K, g = L["count"], L["K"], L["gtable"]
+    assert g.size == n * K and K == 256
+    miss = g == 0xFFFFFFFF
+    assert miss.mean() == pytest.approx(L["stats"]["gatherMissFrac"], abs=1e-15)
+    assert g[~miss].max() < n
+    assert np.allclose(np.linalg.norm(L["nrm"], axis=1), 1.0, atol=1e-6) and (L["area"] > 0).all()
+
+
 @pytest.fixture(scope="module")
 def prep(loaded):
     scene, meshes, physics = loaded
@@ -306,3 +321,49 @@ def test_gpu_twin_judge_preset_matches_oracle(loaded, prep, caches):
     print("twin judge preset CAM3 tile: relative RMSE %.3g, close %.4f" % (rel, close))
     assert rel < 1e-3 and close > 0.99
 
+
+
+@pytest.mark.gpu
+@needs_assets
+def test_gpu_twin_interior_tree(loaded, prep):
+    """The interior tree (_buildInteriorBVH, gpu/renderer.js:533-571): it keeps exactly the static triangles on the
+    bench or with a node whose gather rays escape in fewer than 90 % of cases (counted here from the miss table),
+    dropping the shell's outer skin; CAM3, inside the shell, sees the same frame through it as through the whole
+    scene but for the few rays that leave through an opening and would meet the skin (under 0.1 % of the pixels,
+    the reference's own frame), and the walk through the smaller tree is cheaper."""
+    scene, _, _ = loaded
+    cams, poses = _twin_frame(prep, scene, 1, 1)
+    tw = np.asarray(prep["tris"], np.float32).view(np.uint32).reshape(-1, 16)
+    static = np.asarray(prep["tri_body"]) == 0xFFFFFFFF
+    vid, mesh = tw[static][:, [3, 7, 11]], tw[static][:, 12]
+    bench = (np.asarray(prep["meshes"]).reshape(-1, 4)[mesh, 1] & 64) != 0
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(prep)
+        ctx.upload_profiles(prep)
+        inst = twin.instances(prep, poses)
+        ctx.set_instances(inst)
+        info = ctx.radiosity_build()
+        miss = ctx.radiosity_tables(1, info["nodes"], np.float32)
+        skin = np.all(miss[vid] >= 0.9 * 64, axis=1) & ~bench
+        n_int = ctx.interior_bvh(True)
+        print("interior tree: %d of %d static triangles (%d skin)" % (n_int, static.sum(), skin.sum()), flush=True)
+        assert n_int == static.sum() - skin.sum() and 0 < skin.sum()
+        ctx.radiosity_update(direct=True, iters=32)
+        cam = dict(cams[2], flags=lib.VT_OPT_REFERENCE | lib.VT_OPT_CULL_SHADOWS, light_samples=4)
+        ctx.set_camera(0, cam)
+        img, ms = [], []
+        for enable in (True, False):
+            assert ctx.interior_bvh(enable) == (n_int if enable else static.sum())
+            for _ in range(3):
+                ctx.render([0])
+            img.append(ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"]))
+            ms.append(ctx.stats()["ms_render"])
+    finally:
+        ctx.close()
+    a, b = img[0][..., :3].astype(np.float64), img[1][..., :3].astype(np.float64)
+    diff = np.any(img[0] != img[1], axis=2).mean()
+    print("interior vs whole tree: %.4f %% of the pixels differ, render %.3f vs %.3f ms" % (100 * diff, ms[0], ms[1]))
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("relative difference %.3g" % rel)
+    assert diff < 1e-3 and rel < 1e-2 and ms[0] < ms[1]

@@ -83,6 +83,8 @@ struct vt_ctx {
   std::vector<float> body_box;       // each body's box in its own frame (min xyz, max xyz)
   SceneDev S{};
   uint32_t tri_count = 0, node4_count = 0;
+  // the interior tree (vt_interior_bvh): its root node and slot count past the scene's own nodes and slots
+  uint32_t interior_root = 0, interior_tris = 0;
   // launch support
   DevBuf counters, overflow, spill, dcams, test_in, test_out, test_aux;
   DevBuf refine_list;           // the refinement pass's pixel list (camera, pixel) and its count
@@ -602,6 +604,8 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
     }
   c->tri_count = T;
   c->node4_count = (uint32_t)n4.size();
+  c->interior_root = 0;
+  c->interior_tris = 0;
   c->n_mat = s->material_count;
   c->n_light = s->light_count;
   c->has_scene = true;
@@ -1805,6 +1809,7 @@ extern "C" int vt_visible(vt_ctx *c, uint32_t cam, const float *points, uint32_t
   VT_HIP(c, hipMemcpy(c->test_in.p, points, (size_t)n * 12, hipMemcpyHostToDevice));
   TestLaunch T{c->S, (uint32_t *)c->overflow.p, (uint2 *)c->spill.p, c->spill_lanes,
                (uint32_t)(c->spill_lanes / DEFAULT_BLOCK)};
+  T.S.root = 0;                                 // the whole static scene, as the reference's CPU walk
   const vt_camera &k = c->cams[cam].c;
   VT_HIP(c, launch_visible(T, k.pos, k.skip_bodies, tol, time, (const float *)c->test_in.p, n, (uint8_t *)c->test_out.p,
                            c->stream));
@@ -2107,6 +2112,12 @@ extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radi
   const auto t0 = std::chrono::steady_clock::now();
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));
+  // the gather rays walk the whole static scene; an interior tree of an earlier build is dropped (its slots stay
+  // unused until vt_interior_bvh builds the next one from the new miss counts)
+  if (c->S.root) c->scene_gen++;
+  c->S.root = 0;
+  c->interior_root = 0;
+  c->interior_tris = 0;
   const size_t V = c->h_verts.size() / 8;
   // static nodes: the vertices of static-flagged triangles of the static scene, with a third of every such
   // triangle's area; a node's material is that of its own mesh (the vertex record's word 3, as the reference's
@@ -2228,6 +2239,116 @@ extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radi
     info->active = (uint32_t)A;
     info->ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  return VT_OK;
+}
+
+// _buildInteriorBVH (gpu/renderer.js:533-571): a second static tree without the triangles whose three nodes see
+// the sky in >= 90 % of their gather rays (the shell's outer skin; bench triangles always stay). Its triangles take
+// new slots past the scene's (copies of their records, so a hit's slot still names its vertices and mesh) and its
+// nodes follow the scene's; bodies keep their trees.
+static int build_interior(vt_ctx *c) {
+  const size_t V = c->h_verts.size() / 8;
+  const uint32_t T = c->tri_count, K = c->rad_K;
+  std::vector<float> miss(V);
+  if (V) VT_HIP(c, hipMemcpy(miss.data(), c->g_miss.p, V * 4, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> keep;
+  for (uint32_t k = 0; k < c->static_slots; k++) {
+    const uint4 t = c->h_tvid[k];
+    const bool bench = (c->h_meshes[4 * (size_t)t.w + 1] & 64u) != 0;    // MESH_FLAG.BENCH
+    const float lim = 0.9f * (float)K;
+    if (!bench && miss[t.x] >= lim && miss[t.y] >= lim && miss[t.z] >= lim) continue;
+    keep.push_back(k);
+  }
+  const uint32_t n = (uint32_t)keep.size();
+  c->interior_root = 0;
+  c->interior_tris = 0;
+  if (n == 0 || n == c->static_slots) return VT_OK;     // nothing to drop
+  if ((uint64_t)T + n >= (1u << 26)) return fail(c, VT_ERR_SIZE, "interior_bvh: more than 2^26 slots");
+  std::vector<float> pos(V * 3);
+  for (size_t v = 0; v < V; v++)
+    for (int k = 0; k < 3; k++) pos[3 * v + k] = c->h_verts[8 * v + k];
+  std::vector<uint32_t> idx((size_t)n * 3), order(n);
+  for (uint32_t m = 0; m < n; m++) {
+    const uint4 t = c->h_tvid[keep[m]];
+    idx[3 * m] = t.x;
+    idx[3 * m + 1] = t.y;
+    idx[3 * m + 2] = t.z;
+  }
+  std::vector<float> ng2((size_t)(n + 2) * 16);
+  const int64_t nn = build_bvh2(pos.data(), idx.data(), n, 4, 16, ng2.data(), n + 2, order.data());
+  if (nn < 0) return fail(c, VT_ERR_ARG, "interior_bvh: BVH build failed");
+  std::vector<Node4> g4;
+  try {
+    g4 = collapse_bvh4(ng2.data(), (uint32_t)nn, n);
+  } catch (const std::exception &ex) {
+    return fail(c, VT_ERR_ARG, "interior_bvh: %s", ex.what());
+  }
+  // the scene's nodes, then the interior tree's, relocated
+  std::vector<Node4> nodes(c->node4_count + g4.size());
+  VT_HIP(c, hipMemcpy(nodes.data(), c->nodes.p, (size_t)c->node4_count * sizeof(Node4), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < g4.size(); q++) {
+    Node4 nd = g4[q];
+    for (int ch = 0; ch < 4; ch++) {
+      const uint32_t r = nd.ref[ch];
+      if (r == EMPTY_REF) continue;
+      if (r & LEAF_BIT) nd.ref[ch] = (r & ~0x03ffffffu) | ((r & 0x03ffffffu) + T);
+      else nd.ref[ch] = r + c->node4_count;
+    }
+    nodes[c->node4_count + q] = nd;
+  }
+  // slot records for T + n slots: the scene's, then copies of the kept triangles in the interior tree's order
+  const size_t S2 = (size_t)T + n;
+  std::vector<uint4> attr(S2);
+  for (uint32_t k = 0; k < T; k++) attr[k] = c->h_tvid[k];
+  for (uint32_t m = 0; m < n; m++) attr[T + m] = c->h_tvid[keep[order[m]]];
+  std::vector<float4> tri(S2 * 3), vn(V);
+  for (size_t k = 0; k < S2; k++) {
+    const uint32_t vid[3] = {attr[k].x, attr[k].y, attr[k].z};
+    for (int j = 0; j < 3; j++) {
+      const float *p = &c->h_verts[8 * (size_t)vid[j]];
+      tri[3 * k + j] = make_float4(p[0], p[1], p[2], 0.0f);
+    }
+  }
+  for (size_t v = 0; v < V; v++) {
+    const float *p = &c->h_verts[8 * v];
+    vn[v] = make_float4(p[4], p[5], p[6], 0.0f);
+  }
+  int rc;
+  if ((rc = upload(c, c->nodes, nodes.data(), nodes.size() * sizeof(Node4)))) return rc;
+  if ((rc = upload(c, c->tri, tri.data(), tri.size() * sizeof(float4)))) return rc;
+  if ((rc = upload(c, c->tvid, attr.data(), attr.size() * sizeof(uint4)))) return rc;
+  DevBuf d_vn;
+  rc = upload(c, d_vn, vn.data(), vn.size() * sizeof(float4));
+  if (!rc) rc = ensure(c, c->tshade, S2 * 64);
+  if (!rc) {
+    hipError_t e = launch_tri_shading((const float4 *)c->tri.p, (const uint4 *)c->tvid.p, (const float4 *)d_vn.p,
+                                      (uint32_t)S2, (float4 *)c->tshade.p, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(c, e, "interior triangle shading records");
+  }
+  release(d_vn);
+  if (rc) return rc;
+  c->S.nodes = (const Node4 *)c->nodes.p;
+  c->S.tri = (const float4 *)c->tri.p;
+  c->S.tvid = (const uint4 *)c->tvid.p;
+  c->S.tshade = (const float4 *)c->tshade.p;
+  c->interior_root = c->node4_count;
+  c->interior_tris = n;
+  return VT_OK;
+}
+
+extern "C" int vt_interior_bvh(vt_ctx *c, uint32_t enable, uint32_t *interior_tris) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "interior_bvh: the node cache is not built");
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  if (enable && !c->interior_tris) {
+    if (int rc = build_interior(c)) return rc;
+  }
+  c->S.root = enable ? c->interior_root : 0u;
+  c->scene_gen++;                 // the fixed cameras' caches hold slots and hits of the other tree
+  drop_graph(c);
+  if (interior_tris) *interior_tris = c->S.root ? c->interior_tris : c->static_slots;
   return VT_OK;
 }
 

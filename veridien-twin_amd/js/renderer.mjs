@@ -37,6 +37,7 @@ export const DEFAULTS = Object.freeze({
   refineShadows: true,
   hitCache: true,     // fixed cameras: each sub-sample's static first hit is traced once and kept (exact)
   shadowCache: true,  // still light: each sub-sample's static occlusion per light sample is kept (exact)
+  interiorBVH: true,  // after the node cache build, walk a static tree without the shell's outer skin
   radiosityEvery: 1,  // update the node cache every n-th pose change
 });
 // the reference's operating points for its camera estimator (gpu/renderer.js:63-67), spelled out in full
@@ -215,6 +216,8 @@ export class TwinRenderer {
         smoothRadius: o.smoothRadius, smoothSigma: o.smoothSigma });
       this.stats.activeNodes = this.radiosity.active;
       this.stats.gatherBuildMs = this.radiosity.ms;
+      // the interior tree for every later walk (_buildInteriorBVH, gpu/renderer.js:529, 533-571)
+      this.stats.interiorTris = o.interiorBVH !== false ? native.interiorBvh(this.ctx, true) : undefined;
     }
     if (this.solvedVersion === this.posesVersion) return;
     // radiosityEvery n: a pose change updates the cache every n-th time (gpu/renderer.js:36)

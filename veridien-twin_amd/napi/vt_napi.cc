@@ -784,6 +784,24 @@ napi_value RadiosityBuild(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// interiorBvh(ctx, enable: bool) -> static triangles the walks see (vt_interior_bvh)
+napi_value InteriorBvh(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  bool enable = true;
+  if (argc > 1) napi_get_value_bool(env, argv[1], &enable);
+  uint32_t n = 0;
+  int rc = vt_interior_bvh(c, enable ? 1u : 0u, &n);
+  if (rc) return throw_vt(env, c, rc, "vt_interior_bvh");
+  napi_value out;
+  CHECK_NAPI(napi_create_uint32(env, n, &out));
+  return out;
+}
+
 // radiosityUpdate(ctx, direct: bool, iters) (enqueue)
 napi_value RadiosityUpdate(napi_env env, napi_callback_info info) {
   size_t argc = 3;
@@ -849,7 +867,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"isp", Isp}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
-             {"radiosityBuild", RadiosityBuild}, {"radiosityUpdate", RadiosityUpdate},
+             {"radiosityBuild", RadiosityBuild}, {"interiorBvh", InteriorBvh}, {"radiosityUpdate", RadiosityUpdate},
              {"radiosityState", RadiosityState}};
   for (auto &f : fns) {
     napi_value v;

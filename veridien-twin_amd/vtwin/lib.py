@@ -19,7 +19,7 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
-    "vt_set_light_profile", "vt_radiosity_build", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
+    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
     "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide",
 ]
 
@@ -180,6 +180,7 @@ def load():
         "vt_visible": (i32, [vp, u32, vp, u32, f32, f32, vp]),
         "vt_set_light_profile": (i32, [vp, u32, vp, vp, vp, u32]),
         "vt_radiosity_build": (i32, [vp, vp, vp]),
+        "vt_interior_bvh": (i32, [vp, u32, vp]),
         "vt_radiosity_update": (i32, [vp, u32, u32]),
         "vt_radiosity_state": (i32, [vp, vp, sz, vp]),
         "vt_node_neighbours": (i32, [vp, vp, u32, u32, f32, f32, vp, vp]),
@@ -337,6 +338,13 @@ class Context:
         info = RadiosityInfo()
         self.check(self.L.vt_radiosity_build(self.h, C.byref(o), C.byref(info)))
         return {k: getattr(info, k) for k, _ in RadiosityInfo._fields_}
+
+    def interior_bvh(self, enable=True):
+        """vt_interior_bvh: the interior tree on (built once per node cache) or off; returns the static triangles
+        the walks see."""
+        n = C.c_uint32(0)
+        self.check(self.L.vt_interior_bvh(self.h, 1 if enable else 0, C.byref(n)))
+        return n.value
 
     def radiosity_update(self, direct=True, iters=0):
         self.check(self.L.vt_radiosity_update(self.h, 1 if direct else 0, iters))

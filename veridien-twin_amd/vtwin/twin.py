@@ -593,3 +593,26 @@ def world_triangles(prep, poses):
         P[v] = P[v] @ T[:3, :3].T + T[:3, 3]
     return P.astype(np.float32), idx[keep], tu[keep, 12]
 
+
+
+def lite_nodes(root):
+    """The lite renderer's baked node table (assets/lite.json `files.nodes`, lite/renderer.js:214-221): per node
+    8 floats (position, area, normal, mesh id as bits) and K uint16 gather hits (0xffff: the ray escaped), here as
+    uint32 ids with 0xffffffff for a miss, the gather table layout of the node radiosity step."""
+    with open(os.path.join(root, "lite", "lite.json")) as f:
+        meta = json.load(f)
+    info = meta["files"]["nodes"]
+    raw = np.fromfile(os.path.join(root, "lite", info["file"]), np.uint8)
+    lay = info["layout"]
+
+    def view(name, dtype):
+        off, count, _ = lay[name]
+        return np.frombuffer(raw[off:off + count * np.dtype(dtype).itemsize].tobytes(), dtype)
+
+    n, K = info["count"], info["K"]
+    rec = view("nodes", np.float32).reshape(n, 8)
+    hits = view("gather", np.uint16).reshape(n, K)
+    g = hits.astype(np.uint32)
+    g[hits == 0xFFFF] = 0xFFFFFFFF
+    return dict(count=n, K=K, pos=rec[:, 0:3].copy(), area=rec[:, 3].copy(), nrm=rec[:, 4:7].copy(),
+                mesh=rec[:, 7].copy().view(np.uint32), gtable=g.reshape(-1), stats=meta["stats"])

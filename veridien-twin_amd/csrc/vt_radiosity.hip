@@ -1,5 +1,6 @@
 // Node radiosity on gfx950.
-//  * The reference's sparse gather Jacobi (gpu/wgsl/kernels.wgsl.js:416-461): one lane per active node.
+//  * The reference's sparse gather Jacobi (gpu/wgsl/kernels.wgsl.js:416-461): one wave per active node for the
+//    gather, one lane per node for the update.
 //  * The dense form x <- e + F x of the precision configuration (n = 32768 nodes, 48 right-hand sides):
 //      FP32 / FP64 sweeps on the vector ALUs (128-row x 48-RHS block tiles, split over K, x staged in LDS);
 //      the BF16 sweep on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), F stored once as BF16 so a
@@ -10,14 +11,18 @@
 namespace vt {
 
 // ------------------------------------------------------------------ sparse gather Jacobi
-__global__ void k_jacobi_gather(GatherDev G) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= G.n_active) return;
+// One wave per active node, lane j taking gather entries j, j + 64, ...: the wave reads the node's table row as
+// one contiguous stretch and has its K dependent fetches (hit normal, material, state) in flight at once; the
+// lanes' sums meet in a fixed butterfly, so a step is deterministic.
+__global__ __launch_bounds__(256) void k_jacobi_gather(GatherDev G) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (k >= G.n_active) return;     // uniform over the wave
   const uint32_t i = G.active[k];
   const v3 n = normalize(ld3(G.normals[i]));
   const uint32_t mi = G.node_mat[i];
   v3 acc = mk(0.f, 0.f, 0.f);
-  for (uint32_t j = 0; j < G.gather_k; j++) {
+  for (uint32_t j = lane; j < G.gather_k; j += 64u) {
     const uint32_t h = G.gtable[(size_t)i * G.gather_k + j];
     if (h == MISS) continue;
     const v3 dir = gather_dir(i, j, G.gather_k, n);
@@ -26,6 +31,13 @@ __global__ void k_jacobi_gather(GatherDev G) {
     const v3 Ls = ld3(G.node_state[4 * (size_t)h + G.slot_in]);
     acc = acc + albedo_at(G.albedo, G.bins, mi, muIn) * albedo_at(G.albedo, G.bins, G.node_mat[h], muOut) * Ls;
   }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    acc.x += __shfl_xor(acc.x, off);
+    acc.y += __shfl_xor(acc.y, off);
+    acc.z += __shfl_xor(acc.z, off);
+  }
+  if (lane) return;
   const float4 hh = G.hh[mi];
   const float s = PI_F / (float)G.gather_k;
   G.aux[k] = make_float4(acc.x * s / fmaxf(hh.x, 1e-4f), acc.y * s / fmaxf(hh.y, 1e-4f), acc.z * s / fmaxf(hh.z, 1e-4f), 0.0f);
@@ -57,7 +69,7 @@ __global__ void k_jacobi_update(GatherDev G) {
 
 hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s) {
   if (G.n_active == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_gather, dim3((G.n_active + 63) / 64), dim3(64), 0, s, G);
+  hipLaunchKernelGGL(k_jacobi_gather, dim3((G.n_active + 3) / 4), dim3(256), 0, s, G);
   return hipGetLastError();
 }
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s) {

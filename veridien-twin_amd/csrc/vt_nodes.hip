@@ -4,7 +4,7 @@
 //    summed by a butterfly; lane 0 writes the bins and the hemispherical albedo.
 //  * k_gather_build / k_gather_remap: one lane per gather ray (node-major), persistent over the rays, the static
 //    BVH walked with the LDS short stack of the trace kernels.
-//  * k_node_direct: one lane per active node, node_samples emitter samples with shadow rays.
+//  * k_node_direct: four lanes per active node sharing its node_samples emitter samples with shadow rays.
 #include "vt_internal.h"
 #include "vt_nodes.h"
 
@@ -144,10 +144,17 @@ __global__ __launch_bounds__(256) void k_gather_remap(NodeLaunch L) {
 // the first reflection of the direct light at each active node, B_dir = sum_s E_s A(mu_s) / samples over
 // node_samples fixed emitter samples with shadow rays through the static scene and the posed bodies at
 // mid-exposure; the visibility code of the pixels' shadow culling; the capsule visibility (nodeDirect :348-406)
+// Four lanes per node, lane q taking samples q, q + 4, ...: four times the shadow rays in flight of a lane per node;
+// the group's sums meet in a fixed butterfly and its first lane writes the node.
+constexpr uint32_t NODE_GROUP = 4;
+
 template <bool INST>
 __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
   NODE_STACK(L);
-  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < L.n_active; k += gridDim.x * 256) {
+  // a group's lanes are consecutive and share every loop trip: lanes is a multiple of NODE_GROUP, strides of 256
+  const uint64_t lanes = (uint64_t)L.n_active * NODE_GROUP;
+  for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < lanes; t += (uint64_t)gridDim.x * 256) {
+    const uint32_t k = (uint32_t)(t / NODE_GROUP), q = (uint32_t)(t % NODE_GROUP);
     const uint32_t i = L.active[k];
     const v3 p = ld3(L.npos[i]), n = ld3(L.nnorm[i]);
     const uint32_t m = L.nmat[i];
@@ -156,7 +163,7 @@ __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
     v3 B = mk(0.f, 0.f, 0.f);
     uint32_t nlit = 0, nvis = 0;
     const v3 o = p + n * EPS_OFF;
-    for (uint32_t s = 0; s < L.node_samples && L.S.emit_count > 0; s++) {
+    for (uint32_t s = q; s < L.node_samples && L.S.emit_count > 0; s += NODE_GROUP) {
       v3 pl;
       const uint32_t e = sample_emitter(L.S, kr(s, KL0, s0), kr(s, KL1, s1), kr(s, KL2, s2), pl);
       v3 wi = pl - p;
@@ -175,6 +182,15 @@ __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
       const v3 ev = Le * (cosE * cosN * L.S.emit_area / d2);
       B = B + ev * albedo_at(L.alb, ALB_BINS, m, cosN);
     }
+#pragma unroll
+    for (int off = NODE_GROUP / 2; off >= 1; off >>= 1) {
+      B.x += __shfl_xor(B.x, off, 64);
+      B.y += __shfl_xor(B.y, off, 64);
+      B.z += __shfl_xor(B.z, off, 64);
+      nlit += __shfl_xor(nlit, off, 64);
+      nvis += __shfl_xor(nvis, off, 64);
+    }
+    if (q) continue;
     const float inv = 1.0f / (float)max(L.node_samples, 1u);
     B = B * inv;
     const float vis = nlit == 0 ? -1.0f : (nvis == nlit ? 1.0f : (nvis == 0 ? 0.0f : 0.5f));
@@ -277,9 +293,9 @@ hipError_t launch_gather_remap(const NodeLaunch &L, hipStream_t s) {
 hipError_t launch_node_direct(const NodeLaunch &L, hipStream_t s) {
   if (L.n_active == 0) return hipSuccess;
   if (L.S.inst_count)
-    hipLaunchKernelGGL(k_node_direct<true>, dim3(grid_of(L, L.n_active)), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(k_node_direct<true>, dim3(grid_of(L, (uint64_t)L.n_active * NODE_GROUP)), dim3(256), 0, s, L);
   else
-    hipLaunchKernelGGL(k_node_direct<false>, dim3(grid_of(L, L.n_active)), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(k_node_direct<false>, dim3(grid_of(L, (uint64_t)L.n_active * NODE_GROUP)), dim3(256), 0, s, L);
   return hipGetLastError();
 }
 

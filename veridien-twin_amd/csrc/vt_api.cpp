@@ -1951,6 +1951,7 @@ extern "C" int vt_jacobi_sparse(vt_ctx *c, const vt_gather_tables *t, float *nod
   if (!t || !node_state || !t->active || !t->gtable || !t->normals || !t->node_mat || !t->albedo || !t->hh)
     return fail(c, VT_ERR_ARG, "jacobi: null table");
   if (t->nbr_k && (!t->nbr || !t->nbr_w)) return fail(c, VT_ERR_ARG, "jacobi: null neighbour table");
+  if (t->nbr_k > 32) return fail(c, VT_ERR_ARG, "jacobi: at most 32 smoothing neighbours");
   if (t->gather_k == 0 || t->albedo_bins == 0 || t->n_mat == 0) return fail(c, VT_ERR_ARG, "jacobi: empty sizes");
   const size_t N = t->n_static, A = t->n_active, K = t->gather_k;
   for (size_t k = 0; k < A; k++)

@@ -1,6 +1,6 @@
 // Node radiosity on gfx950.
 //  * The reference's sparse gather Jacobi (gpu/wgsl/kernels.wgsl.js:416-461): one wave per active node for the
-//    gather, one lane per node for the update.
+//    gather, a group of 8 / 16 / 32 lanes (one per smoothing neighbour) for the update.
 //  * The dense form x <- e + F x of the precision configuration (n = 32768 nodes, 48 right-hand sides):
 //      FP32 / FP64 sweeps on the vector ALUs (128-row x 48-RHS block tiles, split over K, x staged in LDS);
 //      the BF16 sweep on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation), F stored once as BF16 so a
@@ -43,18 +43,40 @@ __global__ __launch_bounds__(256) void k_jacobi_gather(GatherDev G) {
   G.aux[k] = make_float4(acc.x * s / fmaxf(hh.x, 1e-4f), acc.y * s / fmaxf(hh.y, 1e-4f), acc.z * s / fmaxf(hh.z, 1e-4f), 0.0f);
 }
 
-__global__ void k_jacobi_update(GatherDev G) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= G.n_active) return;
-  const uint32_t i = G.active[k];
-  v3 Ef = mk(0.f, 0.f, 0.f);
-  float ws = 0.0f;
-  for (uint32_t j = 0; j < G.nbr_k; j++) {
-    const float w = G.nbr_w[(size_t)k * G.nbr_k + j];
-    if (w <= 0.0f) break;
-    Ef = Ef + ld3(G.aux[G.nbr[(size_t)k * G.nbr_k + j]]) * w;
-    ws += w;
+// One group of GR lanes per active node (GR >= nbr_k), lane j taking neighbour j: a row ends at its first
+// non-positive weight (found by a ballot over the group), the weighted sums meet in a fixed butterfly and the
+// group's first lane writes the node.
+template <uint32_t GR>
+__global__ __launch_bounds__(256) void k_jacobi_update(GatherDev G) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t k = t / GR, j = t % GR;
+  const bool node = k < G.n_active;      // uniform over the group: GR divides 64
+  float w = 0.0f;
+  v3 a = mk(0.f, 0.f, 0.f);
+  if (node && j < G.nbr_k) {
+    w = G.nbr_w[(size_t)k * G.nbr_k + j];
+    if (w > 0.0f) a = ld3(G.aux[G.nbr[(size_t)k * G.nbr_k + j]]);
   }
+  // the row's length: the group's first lane with a non-positive weight (or past nbr_k)
+  const uint64_t stop = __ballot(!(w > 0.0f));
+  const uint32_t gbase = (threadIdx.x & 63u) & ~(GR - 1u);
+  const uint64_t gm = (stop >> gbase) & (GR == 64 ? ~0ull : ((1ull << GR) - 1ull));
+  const uint32_t len = gm ? (uint32_t)__builtin_ctzll(gm) : GR;
+  if (j >= len) {
+    w = 0.0f;
+    a = mk(0.f, 0.f, 0.f);
+  }
+  v3 Ef = a * w;
+  float ws = w;
+#pragma unroll
+  for (uint32_t off = GR / 2; off >= 1; off >>= 1) {
+    Ef.x += __shfl_xor(Ef.x, off, 64);
+    Ef.y += __shfl_xor(Ef.y, off, 64);
+    Ef.z += __shfl_xor(Ef.z, off, 64);
+    ws += __shfl_xor(ws, off, 64);
+  }
+  if (!node || j) return;
+  const uint32_t i = G.active[k];
   if (ws > 0.0f) Ef = Ef * (1.0f / ws);
   else Ef = ld3(G.aux[k]);
   const float4 st = G.node_state[4 * (size_t)i];
@@ -74,7 +96,12 @@ hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s) {
 }
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s) {
   if (G.n_active == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_update, dim3((G.n_active + 63) / 64), dim3(64), 0, s, G);
+  if (G.nbr_k > 32) return hipErrorInvalidValue;
+  const uint32_t gr = G.nbr_k <= 8 ? 8u : G.nbr_k <= 16 ? 16u : 32u;
+  const dim3 grid((unsigned)(((uint64_t)G.n_active * gr + 255) / 256));
+  if (gr == 8) hipLaunchKernelGGL(k_jacobi_update<8>, grid, dim3(256), 0, s, G);
+  else if (gr == 16) hipLaunchKernelGGL(k_jacobi_update<16>, grid, dim3(256), 0, s, G);
+  else hipLaunchKernelGGL(k_jacobi_update<32>, grid, dim3(256), 0, s, G);
   return hipGetLastError();
 }
 

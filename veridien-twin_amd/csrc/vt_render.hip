@@ -1010,19 +1010,32 @@ __global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
 // The reference estimator's refinement pass (camRefine, kernels.wgsl.js:218-252). First each pixel decides from the
 // first pass's keys: an edge (another first-hit mesh among its sub-samples or in its 3x3 block), a posed body, or
 // (VT_OPT_REFINE_SHADOWS) traced shadow rays; the pixels that refine are appended to one list for every camera.
-__global__ void k_refine_mark(const DevCam *cam, uint32_t c, uint2 *list, uint32_t *count) {
+__global__ __launch_bounds__(256) void k_refine_mark(const DevCam *cam, uint32_t c, uint2 *list, uint32_t *count) {
+  // the block's 16 x 16 pixels and a one-pixel rim of sub-sample-0 meshes in LDS (edge-clamped, as the 3x3 test)
+  __shared__ uint32_t sk[18][18];
   const DevCam &C = *cam;
   const uint32_t W = C.width, H = C.height;
-  const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15u), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int x0 = (int)blockIdx.x * 16 - 1, y0 = (int)blockIdx.y * 16 - 1;
+  for (uint32_t q = threadIdx.x; q < 18 * 18; q += 256) {
+    const int xx = min(max(x0 + (int)(q % 18), 0), (int)W - 1), yy = min(max(y0 + (int)(q / 18), 0), (int)H - 1);
+    sk[q / 18][q % 18] = C.keys[(size_t)yy * W + xx].x;
+  }
+  __syncthreads();
+  const uint32_t tx = threadIdx.x & 15u, ty = threadIdx.x >> 4;
+  const uint32_t x = blockIdx.x * 16 + tx, y = blockIdx.y * 16 + ty;
   if (x >= W || y >= H) return;
   const uint4 k = C.keys[(size_t)y * W + x];
   bool refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
-  for (int dy = -1; dy <= 1 && !refine; dy++)
-    for (int dx = -1; dx <= 1; dx++) {
-      const int xx = min(max((int)x + dx, 0), (int)W - 1), yy = min(max((int)y + dy, 0), (int)H - 1);
-      if (C.keys[(size_t)yy * W + xx].x != k.x) refine = true;
-    }
-  if (refine) list[atomicAdd(count, 1u)] = make_uint2(c, y * W + x);
+  for (uint32_t dy = 0; dy < 3; dy++)
+    for (uint32_t dx = 0; dx < 3; dx++) refine |= sk[ty + dy][tx + dx] != k.x;
+  // one append per wave: its first refining lane reserves the wave's entries, each lane takes its rank among them
+  const uint64_t m = __ballot(refine);
+  if (!m) return;
+  const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader, 64);
+  if (refine) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = make_uint2(c, y * W + x);
 }
 
 // Then every listed pixel takes `refine` more sub-samples (indices spp .. spp + refine - 1, nl_refine light samples

@@ -962,7 +962,7 @@ hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s) {
 }
 
 template <int BLOCK, bool INST, bool REF>
-__global__ __launch_bounds__(BLOCK) void k_render(RenderLaunch Lc) {
+__global__ __launch_bounds__(BLOCK, REF ? 4 : 1) void k_render(RenderLaunch Lc) {
   // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
   extern __shared__ uint32_t s_stack[];
   Stack st;
@@ -1042,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_refine_mark(const DevCam *cam, uint32_t
 // each): one lane per sub-sample, a pixel's sub-samples on consecutive lanes (groups of R = the frame's largest
 // refine), and the group's first lane adds them to the first pass's sum in sub-sample order.
 template <bool INST>
-__global__ __launch_bounds__(64) void k_refine(RenderLaunch Lc, const uint2 *list, const uint32_t *count, uint32_t R) {
+__global__ __launch_bounds__(64, 4) void k_refine(RenderLaunch Lc, const uint2 *list, const uint32_t *count, uint32_t R) {
   extern __shared__ uint32_t s_stack[];
   Stack st;
   st.lds_node = s_stack;

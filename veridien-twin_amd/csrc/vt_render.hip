@@ -1013,6 +1013,7 @@ __global__ __launch_bounds__(BLOCK, REF ? 4 : 1) void k_render(RenderLaunch Lc) 
 __global__ __launch_bounds__(256) void k_refine_mark(const DevCam *cam, uint32_t c, uint2 *list, uint32_t *count) {
   // the block's 16 x 16 pixels and a one-pixel rim of sub-sample-0 meshes in LDS (edge-clamped, as the 3x3 test)
   __shared__ uint32_t sk[18][18];
+  __shared__ uint32_t s_wave[4], s_base;
   const DevCam &C = *cam;
   const uint32_t W = C.width, H = C.height;
   const int x0 = (int)blockIdx.x * 16 - 1, y0 = (int)blockIdx.y * 16 - 1;
@@ -1023,19 +1024,28 @@ __global__ __launch_bounds__(256) void k_refine_mark(const DevCam *cam, uint32_t
   __syncthreads();
   const uint32_t tx = threadIdx.x & 15u, ty = threadIdx.x >> 4;
   const uint32_t x = blockIdx.x * 16 + tx, y = blockIdx.y * 16 + ty;
-  if (x >= W || y >= H) return;
-  const uint4 k = C.keys[(size_t)y * W + x];
-  bool refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
-  for (uint32_t dy = 0; dy < 3; dy++)
-    for (uint32_t dx = 0; dx < 3; dx++) refine |= sk[ty + dy][tx + dx] != k.x;
-  // one append per wave: its first refining lane reserves the wave's entries, each lane takes its rank among them
+  bool refine = false;
+  if (x < W && y < H) {
+    const uint4 k = C.keys[(size_t)y * W + x];
+    refine = (k.w & 0x10000u) || k.y != k.z || ((k.w & 0x20000u) && (C.opts & OPT_REFINE_SHADOWS));
+    for (uint32_t dy = 0; dy < 3; dy++)
+      for (uint32_t dx = 0; dx < 3; dx++) refine |= sk[ty + dy][tx + dx] != k.x;
+  }
+  // one append per block (same-address atomics serialise across the chip): each wave counts its refining lanes,
+  // the first thread reserves the block's entries, every lane writes at its rank in block order
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint64_t m = __ballot(refine);
-  if (!m) return;
-  const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader, 64);
-  if (refine) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = make_uint2(c, y * W + x);
+  if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    s_base = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  if (!refine) return;
+  uint32_t at = s_base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  for (uint32_t w = 0; w < wave; w++) at += s_wave[w];
+  list[at] = make_uint2(c, y * W + x);
 }
 
 // Then every listed pixel takes `refine` more sub-samples (indices spp .. spp + refine - 1, nl_refine light samples

@@ -125,6 +125,8 @@ struct vt_ctx {
   CamSlot cams[VT_MAX_CAMERAS];
   // radiosity
   DevBuf g_active, g_table, g_normals, g_mat, g_alb, g_hh, g_nbr, g_nbrw, g_state, g_aux;
+  DevBuf g_w;                 // the gather entries' static weights (null use when rad_weights is false)
+  bool rad_weights = false;
   // dense radiosity
   DevBuf dF32, dF16, dF64, dE, dX, dY, dXd, dR, dPart;
   uint32_t dn_n = 0, dn_r = 0, dn_rp = 0;
@@ -335,7 +337,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots, &c->tvid, &c->g_pos, &c->g_miss, &c->radu, &c->g_area,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->refine_list, &c->eavg_part,
-                    &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr,
+                    &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr, &c->g_w,
                     &c->g_nbrw, &c->g_state, &c->g_aux, &c->dF32, &c->dF16, &c->dF64, &c->dE, &c->dX, &c->dY,
                     &c->dXd, &c->dR, &c->dPart, &c->grads, &c->wf, &c->wf_temp, &c->ltot};
   for (DevBuf *b : bufs) release(*b);
@@ -2223,6 +2225,19 @@ extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radi
       rad = std::max(rad, std::sqrt(d2));
     }
   }
+  // the gather entries' static weights, once per build, when the table stays within 4 GiB (else each step
+  // computes them)
+  c->rad_weights = false;
+  if (A && (uint64_t)A * op.gather_k * 16 <= (4ull << 30)) {
+    if ((rc = ensure(c, c->g_w, (size_t)A * op.gather_k * 16))) return rc;
+    GatherDev G{};
+    G.n_active = (uint32_t)A; G.gather_k = op.gather_k; G.bins = ALB_BINS;
+    G.active = (const uint32_t *)c->g_active.p; G.gtable = (const uint32_t *)c->g_table.p;
+    G.node_mat = (const uint32_t *)c->g_mat.p; G.normals = (const float4 *)c->g_normals.p;
+    G.albedo = (const float4 *)c->g_alb.p; G.gw = (float4 *)c->g_w.p;
+    VT_HIP(c, launch_jacobi_weights(G, c->stream));
+    c->rad_weights = true;
+  }
   c->rad_slot = 1;
   VT_HIP(c, launch_fill_f4((float4 *)c->radu.p, 1, make_float4(0.f, 0.f, 0.f, bits_f(1u)), c->stream));
   VT_HIP(c, launch_fill_f4((float4 *)c->radu.p + 1, 1, make_float4((float)cen[0], (float)cen[1], (float)cen[2], (float)rad),
@@ -2366,6 +2381,7 @@ extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
   G.node_mat = (const uint32_t *)c->g_mat.p; G.nbr = (const uint32_t *)c->g_nbr.p;
   G.normals = (const float4 *)c->g_normals.p; G.albedo = (const float4 *)c->g_alb.p; G.hh = (const float4 *)c->g_hh.p;
   G.nbr_w = (const float *)c->g_nbrw.p; G.node_state = (float4 *)c->g_state.p; G.aux = (float4 *)c->g_aux.p;
+  G.gw = c->rad_weights ? (float4 *)c->g_w.p : nullptr;
   // ping-pong source slots 1 and 2 (gpu/renderer.js:729-749): a step reads slot_in and writes the other
   for (uint32_t it = 0; it < iters; it++) {
     G.slot_in = c->rad_slot;

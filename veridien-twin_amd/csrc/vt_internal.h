@@ -162,6 +162,9 @@ struct GatherDev {
   const float *nbr_w;
   float4 *node_state;
   float4 *aux;
+  // per (active node, gather entry): the static weight A_i(mu_in) A_hit(mu_out) (rgb, 0 for a miss) and the hit
+  // node id (w, as bits); null: each step reads the table and computes the weight
+  float4 *gw;
 };
 // the node radiosity cache on the uploaded scene (vt_nodes.hip)
 struct NodeLaunch {
@@ -191,6 +194,7 @@ hipError_t launch_gather_remap(const NodeLaunch &L, hipStream_t s);
 hipError_t launch_node_direct(const NodeLaunch &L, hipStream_t s);
 
 hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s);
+hipError_t launch_jacobi_weights(const GatherDev &G, hipStream_t s);
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s);
 
 // dense node radiosity: x, y, e are n x rp (row-major, RHS fastest); K split four ways, sums added into y

@@ -14,6 +14,19 @@ namespace vt {
 // One wave per active node, lane j taking gather entries j, j + 64, ...: the wave reads the node's table row as
 // one contiguous stretch and has its K dependent fetches (hit normal, material, state) in flight at once; the
 // lanes' sums meet in a fixed butterfly, so a step is deterministic.
+// the gather entry's static weight: the node's albedo toward the gather direction times the hit node's albedo
+// toward the node (both from the tables), the factor that multiplies the hit's radiance
+__device__ __forceinline__ v3 gather_weight(const GatherDev &G, uint32_t i, uint32_t j, v3 n, uint32_t mi, uint32_t h) {
+  const v3 dir = gather_dir(i, j, G.gather_k, n);
+  const float muIn = sqrtf(fmaxf(0.0f, 1.0f - ((float)j + 0.5f) / (float)G.gather_k));
+  const float muOut = fminf(fmaxf(-dot(dir, normalize(ld3(G.normals[h]))), 0.0f), 1.0f);
+  return albedo_at(G.albedo, G.bins, mi, muIn) * albedo_at(G.albedo, G.bins, G.node_mat[h], muOut);
+}
+
+// WEIGHTS: each entry's hit node and weight come from the 16-B records k_jacobi_weights wrote once per node cache
+// (the geometry and the albedo tables are fixed), in active-node order: a step streams them instead of reading the
+// table row and recomputing the direction and two lookups
+template <bool WEIGHTS>
 __global__ __launch_bounds__(256) void k_jacobi_gather(GatherDev G) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -23,13 +36,13 @@ __global__ __launch_bounds__(256) void k_jacobi_gather(GatherDev G) {
   const uint32_t mi = G.node_mat[i];
   v3 acc = mk(0.f, 0.f, 0.f);
   for (uint32_t j = lane; j < G.gather_k; j += 64u) {
-    const uint32_t h = G.gtable[(size_t)i * G.gather_k + j];
+    float4 rec;
+    if (WEIGHTS) rec = G.gw[(size_t)k * G.gather_k + j];
+    const uint32_t h = WEIGHTS ? __float_as_uint(rec.w) : G.gtable[(size_t)i * G.gather_k + j];
     if (h == MISS) continue;
-    const v3 dir = gather_dir(i, j, G.gather_k, n);
-    const float muIn = sqrtf(fmaxf(0.0f, 1.0f - ((float)j + 0.5f) / (float)G.gather_k));
-    const float muOut = fminf(fmaxf(-dot(dir, normalize(ld3(G.normals[h]))), 0.0f), 1.0f);
     const v3 Ls = ld3(G.node_state[4 * (size_t)h + G.slot_in]);
-    acc = acc + albedo_at(G.albedo, G.bins, mi, muIn) * albedo_at(G.albedo, G.bins, G.node_mat[h], muOut) * Ls;
+    const v3 w = WEIGHTS ? ld3(rec) : gather_weight(G, i, j, n, mi, h);
+    acc = acc + w * Ls;
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -41,6 +54,18 @@ __global__ __launch_bounds__(256) void k_jacobi_gather(GatherDev G) {
   const float4 hh = G.hh[mi];
   const float s = PI_F / (float)G.gather_k;
   G.aux[k] = make_float4(acc.x * s / fmaxf(hh.x, 1e-4f), acc.y * s / fmaxf(hh.y, 1e-4f), acc.z * s / fmaxf(hh.z, 1e-4f), 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_weights(GatherDev G) {
+  const uint64_t total = (uint64_t)G.n_active * G.gather_k;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < total; r += (uint64_t)gridDim.x * 256) {
+    const uint32_t k = (uint32_t)(r / G.gather_k), j = (uint32_t)(r - (uint64_t)k * G.gather_k);
+    const uint32_t i = G.active[k];
+    const uint32_t h = G.gtable[(size_t)i * G.gather_k + j];
+    v3 w = mk(0.f, 0.f, 0.f);
+    if (h != MISS) w = gather_weight(G, i, j, normalize(ld3(G.normals[i])), G.node_mat[i], h);
+    G.gw[r] = make_float4(w.x, w.y, w.z, __uint_as_float(h));
+  }
 }
 
 // One group of GR lanes per active node (GR >= nbr_k), lane j taking neighbour j: a row ends at its first
@@ -91,7 +116,15 @@ __global__ __launch_bounds__(256) void k_jacobi_update(GatherDev G) {
 
 hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s) {
   if (G.n_active == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_gather, dim3((G.n_active + 3) / 4), dim3(256), 0, s, G);
+  if (G.gw) hipLaunchKernelGGL(k_jacobi_gather<true>, dim3((G.n_active + 3) / 4), dim3(256), 0, s, G);
+  else hipLaunchKernelGGL(k_jacobi_gather<false>, dim3((G.n_active + 3) / 4), dim3(256), 0, s, G);
+  return hipGetLastError();
+}
+hipError_t launch_jacobi_weights(const GatherDev &G, hipStream_t s) {
+  if (G.n_active == 0 || !G.gw) return hipSuccess;
+  const uint64_t total = (uint64_t)G.n_active * G.gather_k;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_jacobi_weights, dim3(grid), dim3(256), 0, s, G);
   return hipGetLastError();
 }
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s) {

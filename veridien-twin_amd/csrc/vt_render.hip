@@ -962,7 +962,7 @@ hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s) {
 }
 
 template <int BLOCK, bool INST, bool REF>
-__global__ __launch_bounds__(BLOCK, REF ? 4 : 1) void k_render(RenderLaunch Lc) {
+__global__ __launch_bounds__(BLOCK, 4) void k_render(RenderLaunch Lc) {
   // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
   extern __shared__ uint32_t s_stack[];
   Stack st;

@@ -887,7 +887,7 @@ __device__ __forceinline__ void run_wave_cam(const RenderLaunch &Lc, uint64_t wa
 }
 
 template <int BLOCK, bool INST>
-__global__ __launch_bounds__(BLOCK) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
+__global__ __launch_bounds__(BLOCK, 4) void k_backward(RenderLaunch Lc, float *gmat, float *glight, uint32_t n_mat,
                                                     uint32_t n_light, uint32_t use_lds) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
   __shared__ float s_t[STACK_LDS * BLOCK];

@@ -146,7 +146,7 @@ class Scene(C.Structure):
                 ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32),
                 ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p),
                 ("node_state", C.c_void_p), ("alb", C.c_void_p), ("hh", C.c_void_p), ("slot", C.c_uint32),
-                ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4)]
+                ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4), ("sample_materials", C.c_void_p)]
 
 
 class Body(C.Structure):
@@ -281,6 +281,10 @@ def scene_struct(sc):
         s.slot = int(sc.get("slot", 1))
         s.e_avg[:] = [float(x) for x in sc.get("e_avg", (0, 0, 0))]
         s.emit_c[:] = [float(x) for x in sc.get("emit_c", emitter_centroid(sc))]
+    if sc.get("sample_materials") is not None:
+        keep["sample_materials"] = np.ascontiguousarray(sc["sample_materials"], np.float32)
+        assert keep["sample_materials"].size == keep["materials"].size
+        s.sample_materials = ptr(keep["sample_materials"])
     s._keep = keep
     return s
 

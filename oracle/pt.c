@@ -318,8 +318,10 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       }
       break;
     }
-    or_mat M;
+    or_mat M, Ms;
     or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
+    Ms = M;
+    if (S->sample_materials) or_mat_unpack(S->sample_materials + 16 * (size_t)s.mat, &Ms);
     double wo[3] = {-d[0], -d[1], -d[2]};
     const int last = b + 1 == D;
     double Ln[3] = {0, 0, 0};
@@ -361,7 +363,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double pW = d2 / (cosE * A);
       double w = 1;
       if (!last) {
-        double pb = or_pdf_bsdf(&M, s.ns, wo, wi);
+        double pb = or_pdf_bsdf(&Ms, s.ns, wo, wi);
         w = (double)nl * nl * pW * pW / ((double)nl * nl * pW * pW + pb * pb);
       }
       const double kk = cosS * w / pW;
@@ -381,7 +383,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
     if (b == 0) { ub[0] = or_kr(i, KB0, P->s[6]); ub[1] = or_kr(i, KB1, P->s[7]); ub[2] = or_kr(i, KB2, P->s[8]); }
     else { ub[0] = rnd(P, i, b, 200); ub[1] = rnd(P, i, b, 201); ub[2] = rnd(P, i, b, 202); }
     double l[3];
-    double pdf = or_sample_bsdf(&M, s.ns, wo, ub, l);
+    double pdf = or_sample_bsdf(&Ms, s.ns, wo, ub, l);
     if (!(pdf > 0) || dot3(s.ng, l) <= 0) break;
     double f[3];
     or_eval_bsdf(&M, s.ns, l, wo, f);

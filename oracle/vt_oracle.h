@@ -169,6 +169,11 @@ typedef struct {
    * centroid + radius (shadow culling) */
   const float *node_state; const float *alb; const float *hh;
   uint32_t slot; float e_avg[3]; float emit_c[4];
+  /* material table of the path tracer's sampling (NULL: `materials`): the lobe choice, the BSDF pdf of a
+   * continuation and the MIS weights read it, the BSDF values read `materials`. A finite difference of the
+   * colours in `materials` with this table held at the unperturbed colours is the derivative the adjoint
+   * computes (sampling detached). */
+  const float *sample_materials;
 } or_scene;
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose

@@ -26,10 +26,35 @@ def _setup(spp=8):
     return sc, cam, adj
 
 
-def _loss(sc, cam, adj):
-    x0, y0, w, h = TILE
+def _loss(sc, cam, adj, tile=TILE):
+    x0, y0, w, h = tile
     img, _, _ = oracle.render_tile(sc, cam, x0, y0, w, h)
     return float(np.sum(img[:, :, 0:3].astype(np.float64) * adj[:, :, 0:3]))
+
+
+def _setup_mixed(spp=8):
+    """C1 with Lambert, GGX dielectric, GGX conductor and coated materials (scenes.c1_mixed), the oracle's BVH."""
+    sc, cams = scenes.c1_mixed(bvh=lambda p, i: (lambda b: (b["nodes"], b["order"]))(oracle.build_bvh(p, i)))
+    cam = dict(cams[0])
+    cam["spp"] = spp
+    r = scenes.Rng(99)
+    x0, y0, w, h = TILE
+    adj = np.zeros((h, w, 4), np.float32)
+    adj[:, :, 0:3] = r.uniform(-1, 1, w * h * 3).reshape(h, w, 3)
+    return sc, cam, adj
+
+
+def _fd_sampling_fixed(sc, cam, adj, m, ch, hstep=1e-2, tile=TILE):
+    """Central difference in material m's colour channel ch with the path tracer's sampling (lobe choice, BSDF
+    pdf, MIS weights) held at the unperturbed materials: common random numbers take the same lobes and
+    directions, so the estimate is a polynomial in the colour and the difference is the detached derivative."""
+    base = np.asarray(sc["materials"], np.float32)
+    out = []
+    for sgn in (1, -1):
+        a = base.reshape(-1, 16).copy()
+        a[m, ch] += sgn * hstep
+        out.append(_loss(dict(sc, materials=a.reshape(-1), sample_materials=base), cam, adj, tile))
+    return (out[0] - out[1]) / (2 * hstep)
 
 
 def _with(sc, key, idx, ch, delta):
@@ -61,6 +86,110 @@ def test_oracle_adjoint_matches_finite_differences():
             hstep = 1e-3
             fd = (_loss(_with(sc, "materials", m, ch, hstep), cam, adj) - _loss(_with(sc, "materials", m, ch, -hstep), cam, adj)) / (2 * hstep)
             assert gm[m, ch] == pytest.approx(fd, rel=1e-3, abs=2e-4 * np.max(np.abs(gm)))
+
+
+KIND_NAMES = {scenes.KIND_LAMBERT: "lambert", scenes.KIND_DIELECTRIC: "dielectric rho_d",
+              scenes.KIND_CONDUCTOR: "conductor F0", scenes.KIND_COATED: "coated F0"}
+
+
+def test_oracle_ggx_adjoint_matches_finite_differences():
+    """GGX dielectric (rho_d, whose specular probability moves with it, gpu/bsdf.js:145-148), conductor and coated
+    (F0 inside Schlick) colour gradients of the FP64 path replay against central differences with the sampling
+    held fixed: the two largest components of every kind within 1e-3 relative."""
+    sc, cam, adj = _setup_mixed()
+    x0, y0, w, h = TILE
+    gm, _ = oracle.backward_tile(sc, cam, x0, y0, w, h, adj)
+    kinds = np.asarray(sc["materials"], np.float32).reshape(-1, 16)[:16, 5].astype(int)
+    gmax = np.max(np.abs(gm))
+    for kind, name in KIND_NAMES.items():
+        rows = np.nonzero(kinds == kind)[0]
+        flat = [(abs(gm[m, ch]), m, ch) for m in rows for ch in range(3)]
+        for _, m, ch in sorted(flat, reverse=True)[:2]:
+            fd = _fd_sampling_fixed(sc, cam, adj, m, ch)
+            print("%s: material %d channel %d adjoint %.6g, finite difference %.6g" % (name, m, ch, gm[m, ch], fd))
+            assert gm[m, ch] == pytest.approx(fd, rel=1e-3, abs=2e-5 * gmax)
+
+
+def test_oracle_sampling_table_changes_only_the_sampling():
+    """The sampling table equal to the materials leaves the image bit-identical; a different one leaves the
+    estimate unbiased (same mean within noise) but changes the samples."""
+    sc, cam, adj = _setup_mixed(spp=4)
+    x0, y0, w, h = TILE
+    a, _, _ = oracle.render_tile(sc, cam, x0, y0, w, h)
+    b, _, _ = oracle.render_tile(dict(sc, sample_materials=sc["materials"]), cam, x0, y0, w, h)
+    assert np.array_equal(a, b)
+    other = np.asarray(sc["materials"], np.float32).reshape(-1, 16).copy()
+    other[:16, 0:3] = 0.5
+    c, _, _ = oracle.render_tile(dict(sc, sample_materials=other.reshape(-1)), cam, x0, y0, w, h)
+    assert not np.array_equal(a, c)
+
+
+@pytest.mark.gpu
+def test_gpu_ggx_adjoint_matches_oracle_and_fd():
+    """Every material kind on the GPU: the FP32 path replay against the FP64 oracle's on the same samples, and the
+    largest GGX components against the sampling-fixed finite differences."""
+    sc, cam, adj = _setup_mixed()
+    x0, y0, w, h = TILE
+    full = np.zeros((cam["height"], cam["width"], 4), np.float32)
+    full[y0:y0 + h, x0:x0 + w] = adj
+    n_mat, n_light = sc["materials"].size // 16, sc["lights"].size // 4
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(sc)
+        ctx.set_camera(0, cam)
+        gm, gl = ctx.backward([0], [full], n_mat, n_light)
+        st = ctx.stats()
+    finally:
+        ctx.close()
+    om, ol = oracle.backward_tile(sc, cam, x0, y0, w, h, adj)
+    print("mixed materials, GPU vs FP64 oracle: materials max rel %.2e, lights max rel %.2e" % (
+        np.max(np.abs(gm - om)) / np.max(np.abs(om)), np.max(np.abs(gl - ol)) / np.max(np.abs(ol))))
+    assert np.allclose(gm, om, rtol=1e-3, atol=1e-4 * np.max(np.abs(om)))
+    assert np.allclose(gl, ol, rtol=1e-3, atol=1e-4 * np.max(np.abs(ol)))
+    assert st["stack_overflows"] == 0
+    kinds = np.asarray(sc["materials"], np.float32).reshape(-1, 16)[:16, 5].astype(int)
+    for kind in (scenes.KIND_DIELECTRIC, scenes.KIND_CONDUCTOR, scenes.KIND_COATED):
+        rows = np.nonzero(kinds == kind)[0]
+        m, ch = max(((m, ch) for m in rows for ch in range(3)), key=lambda q: abs(om[q[0], q[1]]))
+        fd = _fd_sampling_fixed(sc, cam, adj, m, ch)
+        assert gm[m, ch] == pytest.approx(fd, rel=1e-3, abs=2e-5 * np.max(np.abs(om))), KIND_NAMES[kind]
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_c5_tile_matches_oracle():
+    """The differentiation configuration's scene and cameras (2,013,432 triangles, 256 Lambert / GGX dielectric /
+    GGX conductor materials, 16 lights, KB4 1280x960, 32 spp, depth 5): material, light and the camera's 14
+    parameter gradients of one tile against the FP64 oracle on the same samples, and the largest material
+    components against sampling-fixed finite differences."""
+    sc, cams = scenes.c5()
+    cam = cams[0]
+    x0, y0, w, h = 608, 448, 48, 32
+    r = scenes.Rng(98)
+    adj = np.zeros((h, w, 4), np.float32)
+    adj[:, :, 0:3] = r.uniform(-1, 1, w * h * 3).reshape(h, w, 3)
+    full = np.zeros((cam["height"], cam["width"], 4), np.float32)
+    full[y0:y0 + h, x0:x0 + w] = adj
+    n_mat, n_light = sc["materials"].size // 16, sc["lights"].size // 4
+    gm, gl, gc, _, _, st = _cam_grads_gpu(sc, cam, full, n_mat, n_light)
+    om, ol = oracle.backward_tile(sc, cam, x0, y0, w, h, adj, nthreads=16)
+    rel_m = np.max(np.abs(gm - om)) / np.max(np.abs(om))
+    rel_l = np.max(np.abs(gl - ol)) / np.max(np.abs(ol))
+    print("c5 tile, GPU vs FP64 oracle: materials max rel %.2e, lights max rel %.2e" % (rel_m, rel_l))
+    assert np.allclose(gm, om, rtol=1e-3, atol=1e-4 * np.max(np.abs(om)))
+    assert np.allclose(gl, ol, rtol=1e-3, atol=1e-4 * np.max(np.abs(ol)))
+    want, changed = oracle.camera_grad_tile(sc, cam, x0, y0, w, h, adj, nthreads=16)
+    scale = np.array([cam["lens"][0]] * 4 + [1.0] * 10)
+    a, b = gc * scale, want * scale
+    print("GPU   ", np.array2string(a, precision=5), "\noracle", np.array2string(b, precision=5), "\nchanged", changed)
+    assert np.all(np.abs(a - b) <= 1e-3 * np.abs(b) + 2e-4 * np.abs(b).max())
+    kinds = np.asarray(sc["materials"], np.float32).reshape(-1, 16)[:256, 5].astype(int)
+    for kind in (scenes.KIND_LAMBERT, scenes.KIND_DIELECTRIC, scenes.KIND_CONDUCTOR):
+        rows = np.nonzero(kinds == kind)[0]
+        m, ch = max(((m, ch) for m in rows for ch in range(3)), key=lambda q: abs(om[q[0], q[1]]))
+        fd = _fd_sampling_fixed(sc, cam, adj, m, ch, tile=(x0, y0, w, h))
+        print("c5 %s: material %d channel %d GPU %.6g oracle %.6g finite difference %.6g" % (
+            KIND_NAMES[kind], m, ch, gm[m, ch], om[m, ch], fd))
+        assert gm[m, ch] == pytest.approx(fd, rel=1e-3, abs=2e-5 * np.max(np.abs(om)))
 
 
 @pytest.mark.gpu

@@ -119,6 +119,33 @@ def test_js_frame_equals_c_abi_frame(tmp_path):
 
 @pytest.mark.gpu
 @needs_node
+def test_js_destroy_with_pending_jobs(tmp_path):
+    """destroy() while readback and timings promises are queued: each settles with its data (the context goes
+    when the last job has run), and a later call on the destroyed context throws instead of touching freed
+    memory."""
+    sc, cams = scenes.c1()
+    cam = dict(cams[0])
+    for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=2, maxDepth=2,
+               cameras=[dict(name="CAM1", lens=[float(x) for x in cam["lens"]], R=[float(x) for x in cam["R"]],
+                             pos=[float(x) for x in cam["pos"]], size=[cam["width"], cam["height"]])])
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_destroy.mjs"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print(out)
+    assert out["states"] == ["fulfilled"] * 3
+    assert out["lengths"] == [out["expect"], out["expect"]]
+    assert out["finite"] and out["mean"] > 0
+    assert "destroyed" in out["afterDestroy"]
+    assert out["secondDestroy"] == "ok"
+
+
+@pytest.mark.gpu
+@needs_node
 def test_js_calibration_lowers_the_loss(tmp_path):
     """The autograd-style RenderFunction drives a small calibration: material colours, KB4 intrinsics and the
     camera centre start perturbed; ten Adam steps on the adjoint's gradients lower the image loss."""

@@ -212,6 +212,50 @@ def test_gpu_gather_table_and_cache_match_oracle(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_node_cache_follows_material_and_light_updates(ctx):
+    """After vt_update_materials the node cache's albedo tables and gather weights are the new materials' (the
+    next Jacobi steps equal the oracle's on the new tables), and the next update relights every node even when
+    asked for Jacobi steps only; after vt_update_lights the direct light scales with the radiance."""
+    sc, _ = scenes.c1_mixed(bvh=OBVH)
+    K = 64
+    t = _gpu_cache(ctx, sc, K=K)
+    ctx.radiosity_update(direct=True, iters=8)                # slot 1 -> 8 steps end in slot 1
+    ni = oracle.node_inputs(sc)
+    a = t["active"]
+    mats = np.asarray(sc["materials"], np.float32).reshape(-1, 16).copy()
+    r = scenes.Rng(77)
+    mats[:16, 0:3] = r.uniform(0.2, 0.9, 48).reshape(16, 3)
+    sc2 = dict(sc, materials=mats.reshape(-1))
+    ctx.update_materials(mats)
+    n_mat = mats.shape[0]
+    alb = ctx.radiosity_tables(5, (n_mat, 32, 8), np.float32)
+    hh = ctx.radiosity_tables(6, (n_mat, 4), np.float32)
+    alb_o, hh_o = oracle.albedo_tables(sc2)
+    assert np.abs(alb - alb_o).max() < 2e-5 and np.abs(hh - hh_o).max() < 2e-5
+    ctx.radiosity_update(direct=False, iters=0)               # stale direct light: relit anyway
+    st_gpu, _ = ctx.radiosity_state(ni["npos"].shape[0])
+    st = np.zeros_like(st_gpu)
+    oracle.node_direct(sc2, ni, a, 4, alb, st)
+    B, Bo = st_gpu[a, 12:15], st[a, 12:15]
+    print("after the material update: B_dir rel err %.2e" % (np.linalg.norm(B - Bo) / np.linalg.norm(Bo)))
+    assert np.linalg.norm(B - Bo) <= 1e-3 * np.linalg.norm(Bo)
+    st0 = st_gpu.copy()
+    ctx.radiosity_update(direct=False, iters=6)
+    st_gpu2, _ = ctx.radiosity_state(ni["npos"].shape[0])
+    slot = oracle.jacobi_steps(sc2, ni, a, t["g"], K, t["nb"], t["nw"], alb, hh, st0, 1, 6)
+    x, y = st_gpu2[a, 4 * slot:4 * slot + 3], st0[a, 4 * slot:4 * slot + 3]
+    print("Jacobi on the new materials: rel err %.2e" % (np.linalg.norm(x - y) / np.linalg.norm(y)))
+    assert np.linalg.norm(x - y) <= 1e-5 * np.linalg.norm(y)
+    # lights x 1.5: the next update relights, B_dir scales exactly with the radiance
+    lights = np.asarray(sc["lights"], np.float32).reshape(-1, 4).copy()
+    ctx.update_lights(lights * np.float32(1.5))
+    ctx.radiosity_update(direct=False, iters=0)
+    st_gpu3, _ = ctx.radiosity_state(ni["npos"].shape[0])
+    assert np.allclose(st_gpu3[a, 12:15], 1.5 * st_gpu2[a, 12:15], rtol=2e-6, atol=1e-12)
+    ctx.update_lights(lights)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("opts", [0, lib.VT_OPT_CULL_SHADOWS, lib.VT_OPT_DIRECT_ONLY, lib.VT_OPT_NO_INDIRECT,
                                   lib.VT_OPT_NORMALS])
 def test_gpu_reference_estimator_matches_oracle(ctx, opts):

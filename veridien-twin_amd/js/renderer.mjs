@@ -219,11 +219,15 @@ export class TwinRenderer {
       // the interior tree for every later walk (_buildInteriorBVH, gpu/renderer.js:529, 533-571)
       this.stats.interiorTris = o.interiorBVH !== false ? native.interiorBvh(this.ctx, true) : undefined;
     }
-    if (this.solvedVersion === this.posesVersion) return;
-    // radiosityEvery n: a pose change updates the cache every n-th time (gpu/renderer.js:36)
-    if (this.solvedVersion >= 0 && this.posesVersion - this.solvedVersion < Math.max(1, o.radiosityEvery || 1)) return;
-    native.radiosityUpdate(this.ctx, true, this.solvedVersion < 0 ? o.itersInitial : o.itersPerFrame);
+    if (this.solvedVersion === this.posesVersion && !this.shadeChanged) return;
+    // radiosityEvery n: a pose change updates the cache every n-th time (gpu/renderer.js:36); a material, light or
+    // profile change always does, with the initial iteration count (the solution moves as a whole)
+    if (!this.shadeChanged && this.solvedVersion >= 0 &&
+        this.posesVersion - this.solvedVersion < Math.max(1, o.radiosityEvery || 1)) return;
+    const fresh = this.solvedVersion < 0 || this.shadeChanged;
+    native.radiosityUpdate(this.ctx, true, fresh ? o.itersInitial : o.itersPerFrame);
     this.solvedVersion = this.posesVersion;
+    this.shadeChanged = false;
   }
 
   /** Jacobi steps to convergence (converge, gpu/renderer.js:804-822), then nodeSummary. */
@@ -277,6 +281,7 @@ export class TwinRenderer {
   setLightProfile(light, profile) {
     native.setLightProfile(this.ctx, light, profile ? { axis: Array.from(profile.axis), theta: Array.from(profile.theta),
       p: Array.from(profile.p) } : null);
+    this.shadeChanged = true;
   }
 
   /** Bodies a camera's own rays ignore: those it owns, e.g. its lens front (gpu/renderer.js:881-886). */
@@ -433,8 +438,16 @@ export class TwinRenderer {
   }
 
   /** Replace the material table (16 floats per material) or the light radiances (4 per light) in place. */
-  updateMaterials(materials) { native.updateMaterials(this.ctx, Float32Array.from(materials)); this.materials = Float32Array.from(materials); }
-  updateLights(lights) { native.updateLights(this.ctx, Float32Array.from(lights)); this.lights = Float32Array.from(lights); }
+  updateMaterials(materials) {
+    native.updateMaterials(this.ctx, Float32Array.from(materials));
+    this.materials = Float32Array.from(materials);
+    this.shadeChanged = true;      // the node cache re-solves before its next reference-estimator frame
+  }
+  updateLights(lights) {
+    native.updateLights(this.ctx, Float32Array.from(lights));
+    this.lights = Float32Array.from(lights);
+    this.shadeChanged = true;
+  }
 
   /** Change a camera's intrinsics (9 lens numbers) and / or pose (R: 9, columns; pos: 3) for the next frame. */
   setCameraParams(name, { lens, R, pos } = {}) {

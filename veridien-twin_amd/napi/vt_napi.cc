@@ -33,13 +33,50 @@ napi_value throw_vt(napi_env env, vt_ctx *c, int rc, const char *what) {
   return nullptr;
 }
 
-vt_ctx *get_ctx(napi_env env, napi_value v) {
+// What the JS external holds: the context and the async jobs queued on it. destroy() stops new calls at once and
+// frees the context when the last queued job has completed (each job's promise still settles); a handle whose
+// external is collected goes the same way. The flags and the count change on the main thread only (the calls, the
+// jobs' completions and the finalizer run there); the pool's job_exec reads c under g_mu.
+struct Handle {
+  vt_ctx *c = nullptr;
+  int pending = 0;          // queued jobs that will use c
+  bool destroyed = false;   // destroy() was called
+  bool collected = false;   // the JS external was garbage-collected
+};
+
+// with g_mu held
+void settle(Handle *h) {
+  if ((h->destroyed || h->collected) && h->pending == 0 && h->c) {
+    vt_destroy(h->c);
+    h->c = nullptr;
+  }
+  if (h->collected && h->pending == 0) delete h;
+}
+
+void finalize_handle(napi_env, void *data, void *) {
+  Handle *h = static_cast<Handle *>(data);
+  std::lock_guard<std::mutex> lk(g_mu);
+  h->collected = true;
+  settle(h);
+}
+
+Handle *get_handle(napi_env env, napi_value v) {
   void *p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_type_error(env, nullptr, "first argument must be a vt context");
     return nullptr;
   }
-  return static_cast<vt_ctx *>(p);
+  return static_cast<Handle *>(p);
+}
+
+vt_ctx *get_ctx(napi_env env, napi_value v) {
+  Handle *h = get_handle(env, v);
+  if (!h) return nullptr;
+  if (h->destroyed || !h->c) {
+    napi_throw_error(env, nullptr, "the vt context was destroyed");
+    return nullptr;
+  }
+  return h->c;
 }
 
 bool get_prop(napi_env env, napi_value obj, const char *name, napi_value *out) {
@@ -112,8 +149,15 @@ napi_value Create(napi_env env, napi_callback_info info) {
   vt_ctx *c = nullptr;
   int rc = vt_create(&ci, &c);
   if (rc) return throw_vt(env, nullptr, rc, "vt_create (no HIP device?)");
+  Handle *h = new Handle();
+  h->c = c;
   napi_value out;
-  CHECK_NAPI(napi_create_external(env, c, nullptr, nullptr, &out));
+  if (napi_create_external(env, h, finalize_handle, nullptr, &out) != napi_ok) {
+    vt_destroy(c);
+    delete h;
+    napi_throw_error(env, nullptr, "napi_create_external failed");
+    return nullptr;
+  }
   return out;
 }
 
@@ -121,9 +165,11 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (c) vt_destroy(c);
+  h->destroyed = true;      // a second destroy() is a no-op
+  settle(h);
   return nullptr;
 }
 
@@ -473,6 +519,7 @@ napi_value Isp(napi_env env, napi_callback_info info) {
 struct Job {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
+  Handle *hd = nullptr;   // counted in hd->pending from queue_job to job_done
   vt_ctx *c = nullptr;
   int kind = -1;          // -1 sync, -2 stats, -3 autotune, -4 backward, >= 0 readback kind
   uint32_t slot = 0, w = 0, h = 0;
@@ -574,6 +621,11 @@ void job_done(napi_env env, napi_status, void *data) {
     napi_resolve_deferred(env, j->deferred, res);
   }
   napi_delete_async_work(env, j->work);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    j->hd->pending--;
+    settle(j->hd);
+  }
   delete j;
 }
 
@@ -586,6 +638,7 @@ napi_value queue_job(napi_env env, Job *j) {
   }
   napi_create_string_utf8(env, "vt", NAPI_AUTO_LENGTH, &name);
   napi_create_async_work(env, nullptr, name, job_exec, job_done, j, &j->work);
+  j->hd->pending++;
   napi_queue_async_work(env, j->work);
   return promise;
 }
@@ -595,10 +648,12 @@ napi_value Sync(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *hd = c ? get_handle(env, argv[0]) : nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
+  j->hd = hd;
   j->kind = -1;
   return queue_job(env, j);
 }
@@ -608,10 +663,12 @@ napi_value Stats(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *hd = c ? get_handle(env, argv[0]) : nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
+  j->hd = hd;
   j->kind = -2;
   return queue_job(env, j);
 }
@@ -621,10 +678,12 @@ napi_value Readback(napi_env env, napi_callback_info info) {
   napi_value argv[5];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *hd = c ? get_handle(env, argv[0]) : nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
+  j->hd = hd;
   int32_t kind = 0;
   napi_get_value_uint32(env, argv[1], &j->slot);
   napi_get_value_int32(env, argv[2], &kind);
@@ -660,10 +719,12 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *hd = c ? get_handle(env, argv[0]) : nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
+  j->hd = hd;
   j->kind = -3;
   if (!slot_list(env, argv[1], j->slots)) {
     delete j;
@@ -684,10 +745,12 @@ napi_value Backward(napi_env env, napi_callback_info info) {
   napi_value argv[6];
   CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   vt_ctx *c = get_ctx(env, argv[0]);
+  Handle *hd = c ? get_handle(env, argv[0]) : nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!c) return nullptr;
   Job *j = new Job();
   j->c = c;
+  j->hd = hd;
   j->kind = -4;
   uint32_t na = 0, nm = 0, nl = 0;
   if (argc < 5 || !slot_list(env, argv[1], j->slots) || napi_get_array_length(env, argv[2], &na) != napi_ok ||

@@ -338,6 +338,18 @@ def camera(R, pos, lens, w, h, spp, cam_index, depth, light_samples=1, seed=0, m
                 seed=seed, cam_index=cam_index, light_samples=light_samples, max_depth=depth, mode=mode, scale=1.0)
 
 
+def window(cam, x0, y0, w, h):
+    """The w x h window of a camera's image at (x0, y0) as a camera of its own: same pose and KB4 lens with the
+    principal point moved by (-x0, -y0), so its pixel (x, y) sees the rays of the parent's pixel (x0 + x, y0 + y).
+    Its sample pattern is its own (the pixel hash takes the window's coordinates)."""
+    c = dict(cam)
+    lens = np.asarray(cam["lens"], np.float32).copy()
+    lens[2] -= np.float32(x0)
+    lens[3] -= np.float32(y0)
+    c.update(lens=lens, width=int(w), height=int(h))
+    return c
+
+
 def default_sensor(cfa=0, read_noise_dn=2.0, exposure=2e-3, gain=1.0, seed=1006, frame=0, noise=True):
     """gpu/sensor.js:17-84 defaults (pitch 3 um, QE [0.45, 0.55, 0.5] at 600/540/460 nm, f/2, T 0.9, ESF 0.7 px,
     full well 10,000 e-, conversion gain 0.33 DN/e-, 12 bit), read noise set to `read_noise_dn` DN."""
@@ -445,6 +457,27 @@ def c1(bvh="vt"):
     return sc, [cam]
 
 
+def c1_mixed(bvh="vt"):
+    """C1's geometry, lights and camera with its 16 materials replaced by four of each kind: Lambert, GGX
+    dielectric (roughness 0.1..0.6), GGX conductor and coated conductor (base roughness 0.2..0.5, clear coat
+    0.05), seeded colours. The adjoint's material terms of every kind show up in one tile."""
+    sc, cams = c1(bvh)
+    rm = Rng(1012)
+    mats = []
+    for m in range(16):
+        kind = (KIND_LAMBERT, KIND_DIELECTRIC, KIND_CONDUCTOR, KIND_COATED)[m % 4]
+        col = rm.uniform(0.25, 0.85, n=3)
+        rough = rm.uniform(0.1, 0.6)
+        if kind == KIND_COATED:
+            mats.append(material_record(kind, col, roughness=rough, tau=rm.uniform(0.7, 1.0, n=3)))
+        else:
+            mats.append(material_record(kind, col, roughness=rough))
+    mats.append(material_record(KIND_LAMBERT, (0.0, 0.0, 0.0)))
+    sc = dict(sc)
+    sc["materials"] = np.concatenate(mats).astype(np.float32)
+    return sc, cams
+
+
 def _objects(rs, B, n_cyl, n_box, mesh_base):
     for k in range(n_cyl):
         r = rs.uniform(0.02, 0.15)
@@ -516,8 +549,8 @@ def ring_cameras(rc, n, sizes, spps, depth, f_at_1920=600.0, first_seed=1005):
     return cams
 
 
-def c2(bvh="vt"):
-    sc = full_scene(bvh)
+def c2(bvh="vt", scene=None):
+    sc = scene if scene is not None else full_scene(bvh)
     rc = Rng(1004)
     cams = ring_cameras(rc, 3, [(1920, 1080)] * 3, [64] * 3, 5)
     return sc, cams

@@ -8,10 +8,13 @@ sensor epilogue (RGGB, Poisson shot + 2 DN Gaussian read noise, 12 bit) and the 
 default), then the three sensor and ISP passes. Inputs are resident in HBM
 before the timed region; the timed region is exactly `--steps` frames between two device synchronisations.
 
-Prints one JSON line. `value` is Mrays/s (camera + extension + shadow rays counted on the device).
-`roofline` prices the trace kernel with the algorithmic bytes of the reference-order BVH2 traversal
-(B_ray = 32 ray + 16 hit + 64 per node visit + 48 per triangle test, visit counts from the CPU reference
-traversal of the same seeded paths on the cpu_baseline sample) against 8 TB/s.
+Prints one JSON line. `value` is Mrays/s (camera + extension + shadow rays counted on the device). `roofline`
+is the dominant kernel (the extension-ray trace) against the roof that binds it, VALU issue: its VALU lane
+operations per launch (counters committed under profiles/ for the same shape) over the launch duration measured
+here; beside it the HBM fraction and the algorithmic figures of the reference-order BVH2 traversal of the same
+seeded paths. After the headline frame the default run adds the rest of the metric on short runs of their own:
+the adjoint / forward time ratio (3 KB4 cameras 1280x960, 32 spp), the batched / per-camera speed-up (24 uneven
+cameras) and BF16 + refinement against FP32 with its error against FP64 (dense radiosity, N = 32,768, 48 RHS).
 """
 import argparse
 import json
@@ -26,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s and spp·Mpix/s per frame on 1 MI355X; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
+# VALU lane operations per second: MI355X's FP32 vector 157.3 TFLOP/s counts an FMA as 2 and a packed FP32 op as 2
+# lanes; the library is built without packed FP32, so 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T lane-op/s
+VALU_PEAK_T = 157.3 / 4
 # sensorParams' ISP defaults (the reference's 'none' noise-reduction string enables the bilateral)
 ISP = dict(demosaic=0, nr=1, nr_sigma_px=0.4, nr_range_scale=2.0, tone=0, chroma420=1)
 
@@ -34,9 +40,20 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16, budget_1t=4.0):
     """The oracle (C restatement of gpu/bvh.js traversal + the shading pieces, FP64) on a bounded sample of the
-    same workload: 32x32 tiles of camera 0 around the image centre at the camera's spp, until ~budget_s."""
+    same workload: 32x32 tiles of camera 0 around the image centre at the camera's spp, until ~budget_s on
+    `nthreads` threads; then the same tiles on one thread for ~budget_1t."""
     import oracle
 
     t0 = time.time()
@@ -68,21 +85,35 @@ def cpu_baseline(sc, cam, budget_s=12.0, nthreads=16):
             break
     dt = time.time() - t1
     ext = max(int(vis[2]), 1)
+    # one thread, same tiles from the centre
+    rays1, t2 = 0, time.time()
+    for (x0, y0) in tiles:
+        _, r, _ = oracle.render_tile(sc, cam, x0, y0, 32, 32, 0, cam["spp"], nthreads=1)
+        rays1 += r
+        if time.time() - t2 > budget_1t:
+            break
+    dt1 = time.time() - t2
+    model = cpu_model()
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=nthreads, kind="port",
-                sample="%d pixels (%d 32x32 tiles at the centre of camera 0) x %d spp, depth %d: %d rays in %.1f s on %s"
-                % (px, px // 1024, cam["spp"], cam["max_depth"], rays, dt, platform.processor() or platform.machine()),
+                sample="%d pixels (%d 32x32 tiles at the centre of camera 0) x %d spp, depth %d: %d rays in %.1f s on "
+                       "%d threads of %s" % (px, px // 1024, cam["spp"], cam["max_depth"], rays, dt, nthreads, model),
+                value_1thread=round(rays1 / dt1 / 1e6, 3), host_cpus=os.cpu_count(), cpu_model=model,
                 rays=rays, node_visits_per_ray=float(vis[0]) / max(rays, 1), tri_tests_per_ray=float(vis[1]) / max(rays, 1),
                 ext_share=ext / max(rays, 1), node_visits_per_ext_ray=float(vis[3]) / ext, tri_tests_per_ext_ray=float(vis[4]) / ext)
 
 
-def bench_c3(args):
-    """Batching: the full-scale scene with 24 uneven cameras, one persistent launch vs one launch per camera."""
+def bench_c3(args, ctx=None, sc=None, steps=None, quiet=False):
+    """Batching: the full-scale scene with 24 uneven cameras, one persistent launch vs one launch per camera. With
+    ctx (the full-scale scene uploaded) and sc, runs on that context and returns the result instead of printing."""
     import numpy as np
     from vtwin import lib, scenes
 
-    sc, cams = scenes.c3()
-    ctx = lib.Context(0)
-    ctx.upload_scene(sc)
+    own = ctx is None
+    steps = steps or args.steps
+    sc, cams = scenes.c3(scene=sc)
+    if own:
+        ctx = lib.Context(0)
+        ctx.upload_scene(sc)
     for k, c in enumerate(cams):
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
@@ -95,12 +126,12 @@ def bench_c3(args):
         ctx.sync()
         t1 = time.perf_counter()
         rays = 0.0
-        for _ in range(args.steps):
+        for _ in range(steps):
             ctx.render(slots, mode=mode, graph=graph)
             st = ctx.stats()
             rays += st["rays"]
         t2 = time.perf_counter()
-        res[name] = dict(ms_per_frame=1e3 * (t2 - t1) / args.steps, mrays=rays / (t2 - t1) / 1e6, launches=st["launches"])
+        res[name] = dict(ms_per_frame=1e3 * (t2 - t1) / steps, mrays=rays / (t2 - t1) / 1e6, launches=st["launches"])
         log("%s: %.1f ms/frame, %.1f Mrays/s" % (name, res[name]["ms_per_frame"], res[name]["mrays"]))
     # identical images from the three forms
     imgs = []
@@ -110,7 +141,7 @@ def bench_c3(args):
     spp_px = sum(c["spp"] * c["width"] * c["height"] for c in cams)
     out = {"metric": METRIC, "config": {"workload": "batching: 24 KB4 cameras 320x240..2048x1536, spp in {1,4,16,64}, depth 5, "
                                                      "2,013,432-tri scene"},
-           "value": round(res["wavefront"]["mrays"], 2), "unit": "Mrays/s", "steps": args.steps,
+           "value": round(res["wavefront"]["mrays"], 2), "unit": "Mrays/s", "steps": steps,
            "batched_ms_per_frame": round(res["batched"]["ms_per_frame"], 2),
            "wavefront_ms_per_frame": round(res["wavefront"]["ms_per_frame"], 2),
            "per_camera_ms_per_frame": round(res["per_camera"]["ms_per_frame"], 2),
@@ -120,11 +151,13 @@ def bench_c3(args):
            "images_identical": bool(np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])),
            "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
            "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
+    if not own:
+        return out
     print(json.dumps(out), flush=True)
     ctx.close()
 
 
-def bench_c4(args):
+def bench_c4(args, quiet=False):
     """Dense node radiosity: 32,768 dome nodes, 48 RHS, Jacobi to 1e-6; FP32 vs BF16 MFMA + FP32 refinement vs FP64."""
     import numpy as np
     from vtwin import lib, scenes
@@ -158,11 +191,13 @@ def bench_c4(args):
     out["fp32_sweep_GBs"] = round(sweep_bytes_fp32 / (out["fp32"]["ms_per_sweep"] * 1e-3) / 1e9, 1)
     out["value"] = out["speedup_bf16_over_fp32"]
     out["unit"] = "x (BF16+refinement vs FP32 solve time)"
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if quiet:
+        return out
+    print(json.dumps(out), flush=True)
 
 
-def bench_c5(args):
+def bench_c5(args, ctx=None, sc=None, steps=None, warmup=None):
     """Differentiation: targets rendered from the true materials, lights and cameras, parameters perturbed (albedo
     +-20 %, emitters +-30 %, KB4 k1..k4 +-0.02, pose +-1 degree / 5 mm); one forward frame and one path-replay backward pass of the loss 1/2 ||I - I_target||^2 over
     three 1280x960 cameras at 32 spp, with the gradients of materials, lights and the cameras' intrinsics and
@@ -170,9 +205,13 @@ def bench_c5(args):
     import numpy as np
     from vtwin import lib, scenes
 
-    sc, cams = scenes.c5()
-    ctx = lib.Context(0)
-    ctx.upload_scene(sc)
+    own = ctx is None
+    steps = steps or args.steps
+    warmup = args.warmup if warmup is None else warmup
+    sc, cams = scenes.c5(scene=sc)
+    if own:
+        ctx = lib.Context(0)
+        ctx.upload_scene(sc)
     for k, c in enumerate(cams):
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
@@ -199,14 +238,14 @@ def bench_c5(args):
         c2["lens"] = lens
         ctx.set_camera(k, c2)
     fwd, bwd, rays_f, rays_b = [], [], [], []
-    for it in range(args.warmup + args.steps):
+    for it in range(warmup + steps):
         ctx.render(slots, mode=lib.VT_RENDER_WAVEFRONT)
         st = ctx.stats()
         imgs = [ctx.readback(k, lib.VT_KIND_LINEAR, c["width"], c["height"]) for k, c in enumerate(cams)]
         adj = [im - t for im, t in zip(imgs, targets)]
         gm, gl, gc = ctx.backward(slots, adj, n_mat, n_light, cameras=True)
         sb = ctx.stats()
-        if it >= args.warmup:
+        if it >= warmup:
             fwd.append(st["ms_render"])
             bwd.append(sb["ms_backward"])
             rays_f.append(st["rays"])
@@ -228,9 +267,14 @@ def bench_c5(args):
            "forward_mode": "wavefront", "forward_ms": round(float(np.mean(fwd)), 2), "backward_ms": round(float(np.mean(bwd)), 2),
            "forward_mrays_s": round(float(np.mean(rays_f)) / (float(np.mean(fwd)) * 1e-3) / 1e6, 1),
            "backward_mrays_s": round(float(np.mean(rays_b)) / (float(np.mean(bwd)) * 1e-3) / 1e6, 1),
-           "loss_perturbed": loss, "loss_after_gradient_step": loss2, "steps": args.steps,
+           "loss_perturbed": loss, "loss_after_gradient_step": loss2, "steps": steps,
            "parameters": int(3 * 256 + 3 * n_light + gc.size),
            "camera_gradients": [[round(float(v), 6) for v in row] for row in gc]}
+    # leave the scene as uploaded
+    ctx.update_materials(sc["materials"])
+    ctx.update_lights(sc["lights"])
+    if not own:
+        return out
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -454,16 +498,21 @@ def scenes_default_sensor():
     return scenes.default_sensor(cfa=0, frame=0, seed=1006)
 
 
-def traffic_from_profile(name="k_render_c2_full_scale_forward.md"):
-    """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
-    kernel) for the same workload, read from the committed summary; None when there is none."""
+def profile_figures(name):
+    """Per-launch counter figures of the dominant kernel, read from its committed rocprofv3 summary (lines
+    `key: value`): traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
+    valu_lane_ops_per_launch (SQ_THREAD_CYCLES_VALU), valu_busy and valu_lane_use (rocprof's VALUBusy and
+    VALUUtilization), l2_hit. Missing keys are absent."""
     path = os.path.join(ROOT, "profiles", name)
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        k, sep, v = line.partition(":")
+        if sep and k.strip() in ("traffic_bytes_per_launch", "valu_lane_ops_per_launch", "valu_busy", "valu_lane_use",
+                                 "l2_hit", "shape"):
+            out[k.strip()] = v.strip() if k.strip() == "shape" else float(v)
+    return out
 
 
 def main():
@@ -475,6 +524,8 @@ def main():
                     help="auto (default): the autotuner times both forms on a probe of the frame (untimed) and the "
                          "faster runs; wavefront: per-vertex kernels with sorted rays; batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the adjoint, batching and dense-radiosity legs of the default run (profiling runs)")
     ap.add_argument("--no-caches", action="store_true",
                     help="twin presets: trace every first hit and static shadow ray (no fixed-camera caches)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -579,32 +630,92 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(sc, cams[0], budget_s=args.cpu_budget)
-        out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "value_1thread",
+                                                   "host_cpus", "cpu_model")}
         if ext_launches:
             # the dominant kernel of the wavefront frame: the extension-ray trace (camera and bounce rays)
             vn, vt = cpu["node_visits_per_ext_ray"], cpu["tri_tests_per_ext_ray"]
-            b_ray = 32 + 16 + 64 * vn + 48 * vt
             rays_launch = float(np.mean(ext_rays)) / ext_launches
             ms_launch = float(np.mean(ext_ms))
             kernel, prof = "k_wf_trace_dyn<false, 8> (camera and extension rays)", "k_wf_trace_c2_full_scale_forward.md"
         else:
             vn, vt = cpu["node_visits_per_ray"], cpu["tri_tests_per_ray"]
-            b_ray = 32 + 16 + 64 * vn + 48 * vt
             rays_launch = float(np.mean(rays_step)) / max(launches, 1)
             ms_launch = float(np.mean(ms_kernel))
             kernel, prof = "k_render<256>", "k_render_c2_full_scale_forward.md"
-        achieved = b_ray * rays_launch / (ms_launch * 1e-3) / 1e9
-        mall = ctx.copy_bandwidth(64 << 20)   # 2 x 64 MiB: resident in the 256 MB Infinity Cache
-        traffic = traffic_from_profile(prof)
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "mall_peak_measured": round(mall, 1), "frac_vs_mall": round(achieved / mall, 4),
-                           "traffic_GBs": round(traffic / (ms_launch * 1e-3) / 1e9, 1) if traffic else None,
-                           "kernel": kernel, "bytes_per_ray": round(b_ray, 1), "rays_per_launch": round(rays_launch),
-                           "ms_per_launch": round(ms_launch, 3),
-                           "node_visits_per_ray": round(vn, 2), "tri_tests_per_ray": round(vt, 2)}
+        out["roofline"] = roofline(ctx, kernel, prof, vn, vt, rays_launch, ms_launch)
+    if not args.no_extras:
+        extras(args, ctx, sc, out)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def roofline(ctx, kernel, prof, vn, vt, rays_launch, ms_launch):
+    """The dominant kernel against the roof that binds it: VALU issue. achieved = the VALU lane operations one
+    launch executes (SQ_THREAD_CYCLES_VALU per launch, from the committed counter summary of the same shape) over
+    the launch's duration measured here with HIP events; the same counters give rocprof's VALUBusy x
+    VALUUtilization. Beside it: HBM (FETCH_SIZE + WRITE_SIZE per launch over the same duration, against 8 TB/s),
+    and the algorithmic figures of the reference-order BVH2 traversal of the same seeded paths (the oracle's
+    visit counts): B_ray = 32 + 16 + 64 V_node + 48 V_tri bytes and 2 x 20 ops per node visit + 30 per triangle
+    test."""
+    f = profile_figures(prof)
+    t = ms_launch * 1e-3
+    b_ray = 32 + 16 + 64 * vn + 48 * vt
+    ops_ray = 2 * 20 * vn + 30 * vt
+    r = {"bound": "valu", "peak": round(VALU_PEAK_T, 2), "unit": "Tlane-op/s", "kernel": kernel,
+         "rays_per_launch": round(rays_launch), "ms_per_launch": round(ms_launch, 3), "profile": "profiles/" + prof}
+    if "valu_lane_ops_per_launch" in f:
+        ach = f["valu_lane_ops_per_launch"] / t / 1e12
+        r.update(achieved=round(ach, 3), frac=round(ach / VALU_PEAK_T, 4))
+    else:
+        r.update(achieved=None, frac=None)
+    if "valu_busy" in f and "valu_lane_use" in f:
+        r.update(valu_busy=f["valu_busy"], valu_lane_use=f["valu_lane_use"],
+                 counter_frac=round(f["valu_busy"] * f["valu_lane_use"], 4))
+    traffic = f.get("traffic_bytes_per_launch")
+    r["traffic"] = traffic
+    if traffic:
+        r.update(hbm_GBs=round(traffic / t / 1e9, 1), hbm_frac=round(traffic / t / 1e9 / HBM_PEAK_GBS, 4))
+    if "l2_hit" in f:
+        r["l2_hit"] = f["l2_hit"]
+    if "shape" in f:
+        r["profile_shape"] = f["shape"]
+    alg_gbs = b_ray * rays_launch / t / 1e9
+    alg_ops = ops_ray * rays_launch / t / 1e12
+    r.update(achieved_algorithmic_GBs=round(alg_gbs, 1), bytes_per_ray=round(b_ray, 1),
+             achieved_algorithmic_Tops=round(alg_ops, 3), algorithmic_ops_frac=round(alg_ops / VALU_PEAK_T, 4),
+             ops_per_ray=round(ops_ray, 1), node_visits_per_ray=round(vn, 2), tri_tests_per_ray=round(vt, 2))
+    mall = ctx.copy_bandwidth(64 << 20)   # 2 x 64 MiB: resident in the 256 MB Infinity Cache
+    r.update(mall_copy_GBs=round(mall, 1))
+    return r
+
+
+def extras(args, ctx, sc, out):
+    """The rest of the metric on the same box and scene, each on a short run (untimed by the frame clock above):
+    the adjoint / forward time ratio (differentiation configuration), the batched / per-camera speed-up (24
+    uneven cameras) and BF16 + refinement against FP32 with its error against FP64 (dense radiosity)."""
+    from vtwin import lib
+    t0 = time.time()
+    c5 = bench_c5(args, ctx=ctx, sc=sc, steps=2, warmup=1)
+    out["adjoint_forward_ratio"] = c5["value"]
+    out["c5"] = {k: c5[k] for k in ("forward_ms", "backward_ms", "forward_mrays_s", "backward_mrays_s", "parameters",
+                                    "loss_perturbed", "loss_after_gradient_step")}
+    log("c5: adjoint / forward %.3f (%.1f s)" % (c5["value"], time.time() - t0))
+    t0 = time.time()
+    c3 = bench_c3(args, ctx=ctx, sc=sc, steps=2)
+    out["batched_over_per_camera"] = c3["speedup_batched_over_per_camera"]
+    out["c3"] = {k: c3[k] for k in ("batched_ms_per_frame", "wavefront_ms_per_frame", "per_camera_ms_per_frame",
+                                    "speedup_batched_over_per_camera", "speedup_wavefront_over_per_camera",
+                                    "spp_mpix_per_s", "images_identical")}
+    log("c3: batched / per-camera %.3f (%.1f s)" % (c3["speedup_batched_over_per_camera"], time.time() - t0))
+    t0 = time.time()
+    c4 = bench_c4(args, quiet=True)
+    out["bf16_over_fp32_solve"] = c4["speedup_bf16_over_fp32"]
+    out["c4"] = {"fp32_ms": c4["fp32"]["ms"], "bf16_refine_ms": c4["bf16_refine"]["ms"],
+                 "fp32_ms_per_sweep": c4["fp32"]["ms_per_sweep"], "bf16_ms_per_sweep": c4["bf16_refine"]["ms_per_sweep"],
+                 "bf16_rel_err_vs_fp64": c4["bf16_refine"]["rel_err_vs_fp64"],
+                 "fp32_rel_err_vs_fp64": c4["fp32"]["rel_err_vs_fp64"], "fp32_sweep_GBs": c4["fp32_sweep_GBs"]}
+    log("c4: bf16 / fp32 %.3f (%.1f s)" % (c4["speedup_bf16_over_fp32"], time.time() - t0))
 
 
 if __name__ == "__main__":

@@ -168,10 +168,14 @@ __device__ __forceinline__ v3 schlick3(v3 f0, float c) {
   float m = pow5(1.0f - fminf(fmaxf(c, 0.0f), 1.0f));
   return f0 + (mk(1.f, 1.f, 1.f) - f0) * m;
 }
-__device__ __forceinline__ float ggx_d(float nh, float a) {
+// GGX D = a^2 / (pi (cos^2 (a^2 - 1) + 1)^2) written as a^2 / (pi (sin^2 + cos^2 a^2)^2) with sin^2 = |n x h|^2:
+// the same function for unit n and h, without the cancellation of 1 - cos^2 at the lobe's peak, where the
+// smoothest materials (a^2 ~ 6e-6) would lose two digits in binary32
+__device__ __forceinline__ float ggx_d(v3 n, v3 h, float nh, float a) {
   if (nh <= 0.0f) return 0.0f;
-  float a2 = a * a;
-  float d = nh * nh * (a2 - 1.0f) + 1.0f;
+  const v3 x = cross(n, h);
+  const float s2 = dot(x, x), a2 = a * a;
+  const float d = s2 + nh * nh * a2;
   return a2 / (PI_F * d * d);
 }
 __device__ __forceinline__ float ggx_lambda(float c, float a) {
@@ -192,14 +196,14 @@ __device__ __forceinline__ v3 eval_bsdf(const Mat &M, v3 n, v3 l, v3 v) {
   v3 h = normalize(l + v);
   float nh = dot(n, h), vh = fmaxf(0.0f, dot(v, h));
   if (M.kind == 2) {
-    float coat = ggx_d(nh, M.alpha_c) * ggx_g2(nl, nv, M.alpha_c) * fresnel_dielectric(vh, M.ior) / (4.0f * nl * nv);
+    float coat = ggx_d(n, h, nh, M.alpha_c) * ggx_g2(nl, nv, M.alpha_c) * fresnel_dielectric(vh, M.ior) / (4.0f * nl * nv);
     float t = (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior));
-    float metal = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+    float metal = ggx_d(n, h, nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
     float e = 1.0f / nl + 1.0f / nv;
     v3 tp = mk(powf(M.tau.x, e), powf(M.tau.y, e), powf(M.tau.z, e));
     return mk(coat, coat, coat) + tp * schlick3(M.c, vh) * (t * metal);
   }
-  float spec = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+  float spec = ggx_d(n, h, nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
   v3 F = M.kind == 1 ? schlick3(M.c, vh) : [&] { float f = fresnel_dielectric(vh, M.ior); return mk(f, f, f); }();
   v3 out = F * spec;
   if (M.kind == 0) {
@@ -222,11 +226,11 @@ __device__ __forceinline__ v3 eval_bsdf_dc(const Mat &M, v3 n, v3 l, v3 v) {
   float nh = dot(n, h), vh = fmaxf(0.0f, dot(v, h));
   float one_m = 1.0f - pow5(1.0f - fminf(fmaxf(vh, 0.0f), 1.0f));
   if (M.kind == 1) {
-    float spec = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+    float spec = ggx_d(n, h, nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
     return mk(spec * one_m, spec * one_m, spec * one_m);
   }
   float t = (1.0f - fresnel_dielectric(nl, M.ior)) * (1.0f - fresnel_dielectric(nv, M.ior));
-  float metal = ggx_d(nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
+  float metal = ggx_d(n, h, nh, M.alpha) * ggx_g2(nl, nv, M.alpha) / (4.0f * nl * nv);
   float e = 1.0f / nl + 1.0f / nv;
   return mk(powf(M.tau.x, e), powf(M.tau.y, e), powf(M.tau.z, e)) * (t * metal * one_m);
 }
@@ -251,10 +255,10 @@ __device__ __forceinline__ float pdf_bsdf(const Mat &M, v3 n, v3 v, v3 l) {
   float ps = spec_prob(M, nv);
   if (M.kind == 2) {
     if (!(vh > 0.0f)) return 0.0f;
-    return ps * ggx_g1(nv, M.alpha_c) * ggx_d(nh, M.alpha_c) / (4.0f * nv) +
-           (1.0f - ps) * ggx_g1(nv, M.alpha) * ggx_d(nh, M.alpha) / (4.0f * nv);
+    return ps * ggx_g1(nv, M.alpha_c) * ggx_d(n, h, nh, M.alpha_c) / (4.0f * nv) +
+           (1.0f - ps) * ggx_g1(nv, M.alpha) * ggx_d(n, h, nh, M.alpha) / (4.0f * nv);
   }
-  float pdf_spec = vh > 0.0f ? ggx_g1(nv, M.alpha) * ggx_d(nh, M.alpha) / (4.0f * nv) : 0.0f;
+  float pdf_spec = vh > 0.0f ? ggx_g1(nv, M.alpha) * ggx_d(n, h, nh, M.alpha) / (4.0f * nv) : 0.0f;
   return ps * pdf_spec + (1.0f - ps) * nl * INV_PI_F;
 }
 // Duff et al. 2017 basis (common.wgsl.js:146-157)

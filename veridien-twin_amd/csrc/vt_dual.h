@@ -146,10 +146,12 @@ __device__ __forceinline__ DV d_schlick3(v3 f0, const D &c) {
   const D m = d_pow5(1.0f - dclamp01(c));
   return {f0.x + (1.0f - f0.x) * m, f0.y + (1.0f - f0.y) * m, f0.z + (1.0f - f0.z) * m};
 }
-__device__ __forceinline__ D d_ggx_d(const D &nh, float a) {
+// ggx_d of vt_device.h: sin^2 of the half vector's angle from |n x h|^2
+__device__ __forceinline__ D d_ggx_d(const DV &n, const DV &h, const D &nh, float a) {
   if (nh.v <= 0.0f) return dk(0.0f);
+  const DV x = dcross(n, h);
   const float a2 = a * a;
-  const D d = nh * nh * (a2 - 1.0f) + 1.0f;
+  const D d = ddot(x, x) + (nh * nh) * a2;
   return a2 * rcp(PI_F * (d * d));
 }
 __device__ __forceinline__ D d_ggx_lambda(const D &c, float a) {
@@ -170,16 +172,16 @@ __device__ __forceinline__ DV d_eval_bsdf(const Mat &M, const DV &n, const DV &l
   const D nh = ddot(n, h), vh = dmax(ddot(v, h), 0.0f);
   const D inv4 = rcp(4.0f * nl * nv);
   if (M.kind == 2) {
-    const D coat = d_ggx_d(nh, M.alpha_c) * d_ggx_g2(nl, nv, M.alpha_c) * d_fresnel_dielectric(vh, M.ior) * inv4;
+    const D coat = d_ggx_d(n, h, nh, M.alpha_c) * d_ggx_g2(nl, nv, M.alpha_c) * d_fresnel_dielectric(vh, M.ior) * inv4;
     const D t = (1.0f - d_fresnel_dielectric(nl, M.ior)) * (1.0f - d_fresnel_dielectric(nv, M.ior));
-    const D metal = d_ggx_d(nh, M.alpha) * d_ggx_g2(nl, nv, M.alpha) * inv4;
+    const D metal = d_ggx_d(n, h, nh, M.alpha) * d_ggx_g2(nl, nv, M.alpha) * inv4;
     const D e = rcp(nl) + rcp(nv);
     const DV tp = {dpow_const_base(M.tau.x, e), dpow_const_base(M.tau.y, e), dpow_const_base(M.tau.z, e)};
     const DV sf = d_schlick3(M.c, vh);
     const D tm = t * metal;
     return {coat + tp.x * sf.x * tm, coat + tp.y * sf.y * tm, coat + tp.z * sf.z * tm};
   }
-  const D spec = d_ggx_d(nh, M.alpha) * d_ggx_g2(nl, nv, M.alpha) * inv4;
+  const D spec = d_ggx_d(n, h, nh, M.alpha) * d_ggx_g2(nl, nv, M.alpha) * inv4;
   DV out;
   if (M.kind == 1) {
     out = d_schlick3(M.c, vh) * spec;
@@ -216,10 +218,10 @@ __device__ __forceinline__ D d_pdf_bsdf(const Mat &M, const DV &n, const DV &v, 
   const D inv4 = rcp(4.0f * nv);
   if (M.kind == 2) {
     if (!(vh.v > 0.0f)) return dk(0.0f);
-    return ps * d_ggx_g1(nv, M.alpha_c) * d_ggx_d(nh, M.alpha_c) * inv4 +
-           (1.0f - ps) * d_ggx_g1(nv, M.alpha) * d_ggx_d(nh, M.alpha) * inv4;
+    return ps * d_ggx_g1(nv, M.alpha_c) * d_ggx_d(n, h, nh, M.alpha_c) * inv4 +
+           (1.0f - ps) * d_ggx_g1(nv, M.alpha) * d_ggx_d(n, h, nh, M.alpha) * inv4;
   }
-  const D pdf_spec = vh.v > 0.0f ? d_ggx_g1(nv, M.alpha) * d_ggx_d(nh, M.alpha) * inv4 : dk(0.0f);
+  const D pdf_spec = vh.v > 0.0f ? d_ggx_g1(nv, M.alpha) * d_ggx_d(n, h, nh, M.alpha) * inv4 : dk(0.0f);
   return ps * pdf_spec + (1.0f - ps) * nl * INV_PI_F;
 }
 __device__ __forceinline__ void d_basis(const DV &n, DV &t, DV &b) {

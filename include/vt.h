@@ -292,7 +292,9 @@ int vt_frame_stats_get(vt_ctx *ctx, vt_frame_stats *out);
  * camera to width*height*4 floats (rgb used). opts: the sample range as for vt_render. Buffers of vt_grads are
  * overwritten; their counts must equal the scene's material and light counts. cameras (optional): one row of
  * VT_CAMERA_PARAMS per listed camera, the derivatives with respect to that camera's intrinsics and pose, by
- * forward-mode tangents of the camera ray carried along the replayed path (same random numbers and triangles;
+ * forward-mode tangents of the camera ray carried along the replayed path with its continuation directions and
+ * pdfs held at their sampled values (sampling detached): every hit re-derived on its triangle's plane as the
+ * vertices move, and the shading, emission and NEE of the moved points (same random numbers and triangles;
  * interior term only: pixel-footprint and silhouette edges excluded). */
 /* Camera parameters of the adjoint, per listed camera: fx fy cx cy k1 k2 k3 k4 (the KB4 intrinsics of
  * vt_camera.lens), then the pose: rx ry rz (a small rotation about the world axes applied to R, R' = exp([r]x) R)

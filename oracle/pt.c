@@ -90,6 +90,7 @@ void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float
 typedef struct {
   double p[3], ng[3], ns[3], b[3];
   uint32_t vi[3], mesh, mat, flags, light;
+  int ns_from_ng;   /* the shading normal fell back to the geometric one (grazing view of the interpolated normal) */
 } surf_t;
 
 /* ------------------------------------------------------------------ posed bodies
@@ -186,7 +187,8 @@ static void surface(const or_scene *S, const or_hit *h, const pose_t *P, const d
   if (dot3(s->ng, d) > 0) for (int k = 0; k < 3; k++) s->ng[k] = -s->ng[k];
   if (dot3(s->ns, s->ng) < 0) for (int k = 0; k < 3; k++) s->ns[k] = -s->ns[k];
   double md[3] = {-d[0], -d[1], -d[2]};
-  if (dot3(s->ns, md) < 1e-4) memcpy(s->ns, s->ng, sizeof(s->ng));
+  s->ns_from_ng = dot3(s->ns, md) < 1e-4;
+  if (s->ns_from_ng) memcpy(s->ns, s->ng, sizeof(s->ng));
   for (int k = 0; k < 3; k++) s->p[k] = o[k] + d[k] * h->t;
 }
 
@@ -275,9 +277,70 @@ static void seq_add(uint64_t *seq, int64_t tri) {
   if (seq) *seq = (*seq ^ (uint64_t)(tri + 2)) * 0x100000001b3ull;
 }
 
+/* A path's trace results, for the camera gradient's interior term: recorded on the unperturbed path (mode 1: each
+ * extension ray's triangle and body, each shadow ray's outcome, tagged by vertex and light sample), then replayed
+ * on the perturbed ones (mode 2): an extension hit is the ray's intersection with the recorded triangle's plane
+ * (barycentrics may leave the triangle) and a shadow ray keeps its recorded outcome, so the derivative holds the
+ * triangles and visibility fixed, as the forward-mode tangents do. A replay whose events stop matching (a branch
+ * on a value flipped) sets `bad` and traces normally from there. */
+#define PATH_EVENTS 256
+typedef struct {
+  int mode, n, bad;
+  double min_cos;    /* record: the smallest |d . n_g| of the path's hits */
+  /* each vertex's continuation direction and pdf (record); a replay continues along them (sampling detached) */
+  int n_cont;
+  double cont_l[16][3], cont_pdf[16];
+  uint32_t tag[PATH_EVENTS];
+  int64_t tri[PATH_EVENTS];
+  int32_t inst[PATH_EVENTS];
+} path_rec_t;
+
+/* the ray's hit on the recorded triangle's plane (no range test on u, v, t) */
+static void plane_hit(const or_scene *S, int64_t tri, int32_t inst, const double o[3], const double d[3], double time,
+                      or_hit *h, pose_t *P) {
+  double ol[3] = {o[0], o[1], o[2]}, dl[3] = {d[0], d[1], d[2]};
+  const float *T = S->tris;
+  if (inst >= 0) {
+    const or_body *B = &S->bodies[inst];
+    body_pose(B, time, P);
+    double rel[3] = {o[0] - P->p[0], o[1] - P->p[1], o[2] - P->p[2]};
+    rot_apply_t(P->R, rel, ol);
+    rot_apply_t(P->R, d, dl);
+    T = B->tris;
+  }
+  const float *b = T + (size_t)tri * 16;
+  const double e1[3] = {b[4], b[5], b[6]}, e2[3] = {b[8], b[9], b[10]};
+  const double pv[3] = {dl[1] * e2[2] - dl[2] * e2[1], dl[2] * e2[0] - dl[0] * e2[2], dl[0] * e2[1] - dl[1] * e2[0]};
+  const double inv = 1 / dot3(e1, pv);
+  const double sv[3] = {ol[0] - b[0], ol[1] - b[1], ol[2] - b[2]};
+  const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2], sv[0] * e1[1] - sv[1] * e1[0]};
+  memset(h, 0, sizeof *h);
+  h->tri = tri;
+  h->inst = inst;
+  h->u = dot3(sv, pv) * inv;
+  h->v = dot3(dl, qv) * inv;
+  h->t = dot3(e2, qv) * inv;
+}
+
+/* one event of the record / replay; returns 1 when the replayed event stands in for tracing */
+static int path_event(path_rec_t *pr, uint32_t tag, int64_t *tri, int32_t *inst) {
+  if (!pr || pr->bad) return 0;
+  if (pr->mode == 1) {
+    if (pr->n >= PATH_EVENTS) { pr->bad = 1; return 0; }
+    pr->tag[pr->n] = tag; pr->tri[pr->n] = *tri; pr->inst[pr->n] = *inst;
+    pr->n++;
+    return 0;
+  }
+  if (pr->n >= PATH_EVENTS || pr->tag[pr->n] != tag) { pr->bad = 1; return 0; }
+  *tri = pr->tri[pr->n];
+  *inst = pr->inst[pr->n];
+  pr->n++;
+  return 1;
+}
+
 static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
                                  uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G,
-                                 const double *ray0, uint64_t *seq) {
+                                 const double *ray0, uint64_t *seq, path_rec_t *pr) {
   double d[3], o[3] = {C->pos[0], C->pos[1], C->pos[2]};
   if (ray0) {
     memcpy(o, ray0, sizeof o);
@@ -294,13 +357,35 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
   const double A = S->emit_area;
   for (uint32_t b = 0; b < D; b++) {
     or_hit h;
-    trace_scene(S, o, d, 0, INFINITY, 0, time, b == 0 ? C->skip_bodies : 0u, &h, &pose);
+    {
+      int64_t rt = 0;
+      int32_t ri = -1;
+      if (pr && pr->mode == 2 && path_event(pr, b << 8, &rt, &ri)) {
+        if (rt >= 0) plane_hit(S, rt, ri, o, d, time, &h, &pose);
+        else { memset(&h, 0, sizeof h); h.tri = -1; h.inst = -1; }
+      } else {
+        trace_scene(S, o, d, 0, INFINITY, 0, time, b == 0 ? C->skip_bodies : 0u, &h, &pose);
+        rt = h.tri; ri = h.inst;
+        if (pr && pr->mode == 1) path_event(pr, b << 8, &rt, &ri);
+      }
+    }
     (*rays)++; visits[0] += h.node_visits; visits[1] += h.tri_tests;
     visits[2]++; visits[3] += h.node_visits; visits[4] += h.tri_tests;
     seq_add(seq, h.tri);
     if (h.tri < 0) break;
     surf_t s;
     surface(S, &h, &pose, o, d, &s);
+    if (pr && pr->mode == 1) {
+      const double c = fabs(dot3(d, s.ng));
+      if (c < pr->min_cos) pr->min_cos = c;
+    }
+    if (pr) {
+      /* discrete choices of the vertex that the replay must reproduce: the shading-normal fallback */
+      int64_t v = s.ns_from_ng;
+      int32_t none = -1;
+      const int64_t rec = v;
+      if (path_event(pr, (b << 8) | 0xfe, &v, &none) && v != rec) pr->bad = 1;
+    }
     if (s.flags & F_EMITTER) {
       double ne[3], md[3] = {-d[0], -d[1], -d[2]};
       emitter_normal(S, &s, ne);
@@ -356,7 +441,19 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double so[3];
       offset_ray(s.p, s.ng, wi, so);
       or_hit sh;
-      trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, S->shadow_skip, &sh, NULL);
+      {
+        int64_t rt = 0;
+        int32_t ri = -1;
+        const uint32_t tag = (b << 8) | (j + 1);
+        if (pr && pr->mode == 2 && path_event(pr, tag, &rt, &ri)) {
+          memset(&sh, 0, sizeof sh);
+          sh.tri = rt;
+        } else {
+          trace_scene(S, so, wi, 0, dist - 2e-5, 1, time, S->shadow_skip, &sh, NULL);
+          rt = sh.tri >= 0 ? 0 : -1;
+          if (pr && pr->mode == 1) path_event(pr, tag, &rt, &ri);
+        }
+      }
       (*rays)++; visits[0] += sh.node_visits; visits[1] += sh.tri_tests;
       seq_add(seq, sh.tri >= 0 ? 0 : -1);
       if (sh.tri >= 0) continue;
@@ -383,7 +480,20 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
     if (b == 0) { ub[0] = or_kr(i, KB0, P->s[6]); ub[1] = or_kr(i, KB1, P->s[7]); ub[2] = or_kr(i, KB2, P->s[8]); }
     else { ub[0] = rnd(P, i, b, 200); ub[1] = rnd(P, i, b, 201); ub[2] = rnd(P, i, b, 202); }
     double l[3];
-    double pdf = or_sample_bsdf(&Ms, s.ns, wo, ub, l);
+    double pdf;
+    if (pr && pr->mode == 2 && !pr->bad) {
+      /* replay: the recorded direction and pdf (sampling detached), whatever the moved vertex would sample */
+      if (b >= (uint32_t)pr->n_cont) { pr->bad = 1; break; }
+      memcpy(l, pr->cont_l[b], sizeof l);
+      pdf = pr->cont_pdf[b];
+    } else {
+      pdf = or_sample_bsdf(&Ms, s.ns, wo, ub, l);
+      if (pr && pr->mode == 1 && b < 16) {   /* pdf 0 too: the replay ends where the record did */
+        memcpy(pr->cont_l[b], l, sizeof l);
+        pr->cont_pdf[b] = pdf;
+        pr->n_cont = (int)b + 1;
+      }
+    }
     if (!(pdf > 0) || dot3(s.ng, l) <= 0) break;
     double f[3];
     or_eval_bsdf(&M, s.ns, l, wo, f);
@@ -408,7 +518,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
 
 static double sample_radiance(const or_scene *S, const or_camera *C, const pix_t *P, uint32_t x, uint32_t y,
                               uint32_t i, double out[3], uint64_t *rays, uint64_t *visits, grad_t *G) {
-  return sample_radiance_ex(S, C, P, x, y, i, out, rays, visits, G, NULL, NULL);
+  return sample_radiance_ex(S, C, P, x, y, i, out, rays, visits, G, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------ the reference camera estimator
@@ -964,11 +1074,14 @@ void or_backward_tile(const or_scene *S, const or_camera *C, uint32_t x0, uint32
 /* ---------------------------------------------------------------- camera-parameter gradients (interior term)
  * theta = (fx, fy, cx, cy, k1..k4, r, t): intrinsics of the KB4 (or pinhole) camera, a rotation r about the
  * world axes applied to R (R' = exp([r]x) R) and the camera centre t. For every sample, central differences
- * in FP64 of its path radiance with the image point, the random numbers and the triangles held fixed: the
- * ray of theta +- h e_k is built in FP64 (gpu/lens.js:54-62 unproject), the path re-traced, and the pair is
- * kept only if both re-traces hit the same triangles with the same shadow-ray outcomes as the unperturbed one
- * (else h is shrunk up to three times; a sample still changing is counted in *changed and left out: that is
- * the boundary term, which is excluded). Gradient of sum_p adj_p . I_p (I = mean over the sample range). */
+ * in FP64 of its radiance with the image point, the random numbers and the triangles held fixed: the ray of
+ * theta +- h e_k is built in FP64 (gpu/lens.js:54-62 unproject) and the recorded path replayed on it: every hit
+ * re-derived on its triangle's plane, shadow rays keeping their outcomes, and each continuation along the recorded
+ * direction with the recorded pdf (sampling detached; resampling would turn the directions with the camera and
+ * feed the change back through every later bounce off curved, interpolated-normal surfaces). A pair whose branches
+ * change (a light sample or the shading-normal fallback) is retried with h shrunk up to three times; one still
+ * changing is counted in *changed and left out (the boundary term, which is excluded). Gradient of
+ * sum_p adj_p . I_p (I = mean over the sample range). */
 static void rodrigues(const double r[3], double M[9]) {
   double th = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
   double kx = 0, ky = 0, kz = 0, c = 1, sn = 0;
@@ -1033,8 +1146,12 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
           }
           double ray[6], L0[3];
           uint64_t r = 0, vv[5] = {0, 0, 0, 0, 0}, seq0 = 0;
+          path_rec_t rec;
+          memset(&rec, 0, sizeof rec);
+          rec.mode = 1; rec.min_cos = 1.0;
           ray_of_theta(C, th0, uvf[0], uvf[1], ray);
-          sample_radiance_ex(S, C, &P, x, y, i, L0, &r, vv, NULL, ray, &seq0);
+          sample_radiance_ex(S, C, &P, x, y, i, L0, &r, vv, NULL, ray, &seq0, &rec);
+          const int want_n = rec.n;
           for (int k = 0; k < 14; k++) {
             double hk = step[k];
             int ok = 0;
@@ -1043,13 +1160,19 @@ void or_camera_grad_tile(const or_scene *S, const or_camera *C, uint32_t x0, uin
               double th[14];
               memcpy(th, th0, sizeof th);
               uint64_t sp = 0, sm = 0;
+              path_rec_t rp = rec, rm = rec;
+              rp.mode = rm.mode = 2;
+              rp.n = rm.n = 0;
+              rp.bad = rm.bad = 0;
               th[k] = th0[k] + hk;
               ray_of_theta(C, th, uvf[0], uvf[1], ray);
-              sample_radiance_ex(S, C, &P, x, y, i, Lp, &r, vv, NULL, ray, &sp);
+              sample_radiance_ex(S, C, &P, x, y, i, Lp, &r, vv, NULL, ray, &sp, &rp);
               th[k] = th0[k] - hk;
               ray_of_theta(C, th, uvf[0], uvf[1], ray);
-              sample_radiance_ex(S, C, &P, x, y, i, Lm, &r, vv, NULL, ray, &sm);
-              ok = sp == seq0 && sm == seq0;
+              sample_radiance_ex(S, C, &P, x, y, i, Lm, &r, vv, NULL, ray, &sm, &rm);
+              /* the replays hold the recorded triangles and shadow outcomes; a pair whose branches changed is retried
+                 with a smaller step */
+              ok = !rec.bad && !rp.bad && !rm.bad && rp.n == want_n && rm.n == want_n;
               if (ok)
                 for (int c = 0; c < 3; c++) g[k] += a[c] * sc * (Lp[c] - Lm[c]) / (2 * hk);
             }

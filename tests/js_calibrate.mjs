@@ -58,7 +58,20 @@ async function main() {
     R = m.rotateColumns(R, [-step[3 * nm + 8], -step[3 * nm + 9], -step[3 * nm + 10]]);
     for (let k = 0; k < 3; k++) p.pos[k] -= step[3 * nm + 11 + k];
   }
-  console.log(JSON.stringify({ losses, fx: p.lens[0], fx_true: fxTrue }));
+  // parameter errors at the start and the end: material colours (mean |relative error|), fx, k1, centre (m)
+  const matErr = (mm) => {
+    let e = 0, n = 0;
+    for (let i = 0; i < nm; i++) for (let c = 0; c < 3; c++) {
+      const t = scene.materials[16 * i + c];
+      if (t > 0) { e += Math.abs(mm[16 * i + c] / t - 1); n++; }
+    }
+    return e / Math.max(n, 1);
+  };
+  const mats0 = Float32Array.from(scene.materials);
+  for (let i = 0; i < nm; i++) for (let c = 0; c < 3; c++) mats0[16 * i + c] *= 1 + 0.2 * Math.sin(7 * i + c);
+  const posErr = (pp) => Math.hypot(pp[0] - cam0.pos[0], pp[1] - cam0.pos[1], pp[2] - cam0.pos[2]);
+  console.log(JSON.stringify({ losses, fx: p.lens[0], fx_true: fxTrue, fx0: fxTrue * 1.01, k1: p.lens[4],
+    k1_true: cam0.lens[4], mat_err0: matErr(mats0), mat_err: matErr(p.mats), pos_err0: 0.003, pos_err: posErr(p.pos) }));
   r.destroy();
 }
 main().catch((e) => { console.error(e); process.exit(1); });

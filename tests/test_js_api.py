@@ -148,7 +148,11 @@ def test_js_destroy_with_pending_jobs(tmp_path):
 @needs_node
 def test_js_calibration_lowers_the_loss(tmp_path):
     """The autograd-style RenderFunction drives a small calibration: material colours, KB4 intrinsics and the
-    camera centre start perturbed; ten Adam steps on the adjoint's gradients lower the image loss."""
+    camera centre start perturbed; ten Adam steps on the adjoint's gradients lower the image loss and bring fx, k1
+    and the camera centre back to their true values. (The camera gradient detaches the continuations' sampling, an
+    unbiased estimate of the expected loss's gradient rather than the exact gradient of this 4-spp loss, so the loss
+    is not driven to the common-random-number minimum; Adam's normalised steps move the colours of materials the
+    camera barely sees by the full step size, so their error is printed, not bounded.)"""
     sc, cams = scenes.c1()
     cam = dict(cams[0])
     for k in ("nodes", "tris", "verts", "materials", "lights", "emit"):
@@ -164,8 +168,11 @@ def test_js_calibration_lowers_the_loss(tmp_path):
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     L = info["losses"]
-    print("calibration losses:", ["%.4g" % x for x in L], "fx", info["fx"], "true", info["fx_true"])
-    assert L[-1] < 0.5 * L[0]
+    print("calibration losses:", ["%.4g" % x for x in L], info)
+    assert L[-1] < 0.8 * L[0] and min(L) < 0.6 * L[0]
+    assert abs(info["fx"] - info["fx_true"]) < 0.3 * abs(info["fx0"] - info["fx_true"])
+    assert abs(info["k1"] - info["k1_true"]) < 0.1 * 0.01
+    assert info["pos_err"] < 0.3 * info["pos_err0"]
 
 
 @needs_node

@@ -551,9 +551,11 @@ __device__ __forceinline__ D d_light_profile(const SceneDev &S, uint32_t light, 
 }
 
 // The estimator of path_walk with forward-mode tangents of the camera ray (origin o0, direction d0): the same
-// random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken
-// on the value. Tangents: origin along e1, e2, direction along e1, e2 (e1, e2 perpendicular to d0). Returns
-// the radiance value; ws gets the four tangents of sum_ch delta_ch L_ch.
+// random numbers, the same triangles (each hit point is re-derived on its triangle's plane), every branch taken on
+// the value, the continuations' directions and pdfs at their sampled values (sampling detached: each vertex moves
+// along its fixed incoming direction, and its shading, emission and NEE follow the moved point). Tangents: origin
+// along e1, e2, direction along e1, e2 (e1, e2 perpendicular to d0). Returns the radiance value; ws gets the four
+// tangents of sum_ch delta_ch L_ch.
 template <bool INST>
 __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, const PixSeeds &P, uint32_t i,
                                             v3 o0, v3 d0, v3 e1, v3 e2, v3 delta, Stack &st, uint32_t &nrays,
@@ -687,17 +689,20 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
       u2 = u01(hash3(hs, i, 256u * b + 202u));
     }
     const Mat M = load_mat(S.mats, ms.x);
-    DV l;
-    const D pdf = d_sample_bsdf(M, ns, wo, u0, u1, u2, l);
-    const v3 lv = val(l);
-    if (!(pdf.v > 0.0f) || dot(ng, lv) <= 0.0f) break;
+    // the continuation's direction and pdf are the sample's values (sampling detached): the vertex moves with the
+    // camera along the fixed world direction, whose weight f cos / pdf follows the moved point's shading normal and
+    // incoming direction; turning the direction with the camera would feed back through every later bounce
+    v3 lv;
+    const float pdf = sample_bsdf(M, val(ns), val(wo), u0, u1, u2, lv);
+    if (!(pdf > 0.0f) || dot(ng, lv) <= 0.0f) break;
+    const DV l = dv(lv);
     const DV f = d_eval_bsdf(M, ns, l, wo);
-    const D cp = ddot(ns, l) / pdf;
+    const D cp = ddot(ns, l) * (1.0f / pdf);
     thr = thr * (f * cp);
     if (maxc(val(thr)) <= 0.0f) break;
     o = p + ng * (dot(ng, lv) > 0.0f ? EPS_OFF : -EPS_OFF);
     d = l;
-    pB = pdf;
+    pB = dk(pdf);
   }
 #pragma unroll
   for (int k = 0; k < ND; k++) ws[k] = Ls.g[k];

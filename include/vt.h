@@ -337,6 +337,17 @@ int vt_capsule_test(vt_ctx *ctx, const float *cases, uint32_t n, float *out);
 /* The fused sensor epilogue on caller-given radiance (rgba float32 w*h*4) and RI map (w*h) -> DN (w*h). */
 int vt_sensor_test(vt_ctx *ctx, const vt_sensor_params *params, uint32_t width, uint32_t height,
                    const float *radiance, const float *ri, uint16_t *dn);
+/* vt_sensor_test on caller DEVICE buffers, allocating nothing. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * radiance (w*h float4 rgba, a = the pixel's sample count), ri (w*h floats) and dn (w*h uint16) are device memory of
+ * the context's device (host memory is refused, VT_ERR_ARG); the work is enqueued on the context's stream (vt_sync
+ * waits for it). The veiling-glare levels, its only scratch, live in a static device array of the library
+ * (VT_SENSOR_GLARE_MAX float4 per level, two levels: 8 MiB held on every GPU the library runs on until the process
+ * ends); a glare level ceil(w/F) x ceil(h/F) above VT_SENSOR_GLARE_MAX is refused with VT_ERR_SIZE. Calls that
+ * overlap on one GPU (two contexts or threads at once) share that array and race. */
+#define VT_SENSOR_GLARE_MAX 262144
+int vt_sensor_raw_device(vt_ctx *ctx, const vt_sensor_params *params, uint32_t width, uint32_t height,
+                         const float *radiance, const float *ri, uint16_t *dn);
 /* The ISP on caller-given raw DN (w*h uint16): rgba (w*h*4 bytes), yuv (NULL, or the planes as VT_KIND_YUV420
  * when chroma420), stats (uint32[4], as VT_KIND_STATS). */
 int vt_isp_test(vt_ctx *ctx, const vt_sensor_params *sensor, const vt_isp_params *isp, uint32_t width,

@@ -154,6 +154,8 @@ extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
     return VT_ERR_HIP;
   }
   e = hipSetDevice(c->device);
+  // the no-allocation raw stage's static glare array on this device (vt_sensor_raw_device)
+  if (e == hipSuccess) e = sensor_scratch_init(c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   hipDeviceProp_t prop;
   if (e == hipSuccess) e = hipGetDeviceProperties(&prop, c->device);

@@ -138,6 +138,14 @@ struct SensorDev {
 // glare: two level buffers of ceil(w/F) x ceil(h/F) float4 (unused when glare_k == 0)
 hipError_t launch_sensor(const SensorDev &P, const float4 *acc, const float *ri, float4 *glare_a, float4 *glare_b,
                          uint16_t *dn, hipStream_t s);
+// the raw stage on caller device buffers with its glare levels in the library's static device array (vt_sensor.hip):
+// sensor_scratch_init resolves the array's address for the current device (once per device, at vt_create);
+// launch_sensor_noalloc refuses a glare level above GLARE_LEVEL_MAX float4 or a device not initialised
+constexpr uint32_t GLARE_LEVEL_MAX = 262144;
+constexpr int MAX_DEVICES = 64;
+hipError_t sensor_scratch_init(int device);
+hipError_t launch_sensor_noalloc(int device, const SensorDev &P, const float4 *acc, const float *ri, uint16_t *dn,
+                                 hipStream_t s);
 hipError_t launch_pack_rgb8(const uchar4 *rgba, uint64_t n, uint8_t *rgb, hipStream_t s);
 hipError_t launch_finalize_linear(const float4 *acc, float scale, uint64_t n, float4 *out, hipStream_t s);
 // the ISP after the raw DN

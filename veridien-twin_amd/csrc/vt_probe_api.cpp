@@ -174,6 +174,56 @@ extern "C" int vt_sensor_test(vt_ctx *c, const vt_sensor_params *p, uint32_t w, 
   return VT_OK;
 }
 
+// a device buffer of ctx's device holding at least `bytes` from p (hipPointerGetAttributes, no allocation)
+static int device_buffer(vt_ctx *c, const void *p, size_t bytes, const char *what) {
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof a);
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, VT_ERR_ARG, "sensor_raw_device: %s is not device memory", what);
+  }
+  if (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)
+    return fail(c, VT_ERR_ARG, "sensor_raw_device: %s is not device memory", what);
+  if (a.device != c->device) return fail(c, VT_ERR_ARG, "sensor_raw_device: %s is on device %d, not %d", what, a.device, c->device);
+  // the allocation holding p must reach p + bytes (allocations whose range HIP cannot report pass unchecked)
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) == hipSuccess && base) {
+    const uintptr_t b = (uintptr_t)base, q = (uintptr_t)p;
+    if (q < b || q - b + bytes > size)
+      return fail(c, VT_ERR_SIZE, "sensor_raw_device: %s needs %zu bytes, its allocation holds %zu from there", what,
+                  bytes, (size_t)(b + size > q ? b + size - q : 0));
+  } else {
+    (void)hipGetLastError();
+  }
+  return VT_OK;
+}
+
+extern "C" int vt_sensor_raw_device(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t h, const float *radiance,
+                                    const float *ri, uint16_t *dn) {
+  VT_LIVE(c);
+  if (!radiance || !ri || !dn || w == 0 || h == 0) return fail(c, VT_ERR_ARG, "sensor_raw_device: null buffer or empty size");
+  if (w > 16384 || h > 16384) return fail(c, VT_ERR_SIZE, "sensor_raw_device: %ux%u above 16384x16384", w, h);
+  SensorDev P;
+  int rc = sensor_params(c, p, w, h, 0, 1.0f, P);
+  if (rc) return rc;
+  P.cam_index = 0;
+  if (P.glare_k > 0.0f) {
+    const uint64_t F = P.glare_f, level = ((uint64_t)(w + F - 1) / F) * ((h + F - 1) / F);
+    if (level > GLARE_LEVEL_MAX)
+      return fail(c, VT_ERR_SIZE, "sensor_raw_device: glare level %llu > %u pixels (raise the level factor)",
+                  (unsigned long long)level, GLARE_LEVEL_MAX);
+  }
+  VT_HIP(c, hipSetDevice(c->device));
+  const size_t px = (size_t)w * h;
+  if ((rc = device_buffer(c, radiance, px * 16, "radiance"))) return rc;
+  if ((rc = device_buffer(c, ri, px * 4, "ri"))) return rc;
+  if ((rc = device_buffer(c, dn, px * 2, "dn"))) return rc;
+  VT_HIP(c, launch_sensor_noalloc(c->device, P, (const float4 *)radiance, ri, dn, c->stream));
+  return VT_OK;
+}
+
 extern "C" int vt_isp_test(vt_ctx *c, const vt_sensor_params *sp, const vt_isp_params *q, uint32_t w, uint32_t h,
                            const uint16_t *dn, uint8_t *rgba, uint8_t *yuv, uint32_t *stats) {
   VT_LIVE(c);

@@ -6,6 +6,8 @@
 // Integer DNs match the host sensor model bit for bit: every binary32 operation is unfused (this file is built
 // with -ffp-contract=off), in the WGSL's order, and exp/log/cos are this library's own routines (published
 // methods, stated below) instead of device intrinsics whose last-ulp behaviour differs from the host's.
+#include <mutex>
+
 #include "vt_internal.h"
 
 namespace vt {
@@ -651,3 +653,45 @@ hipError_t launch_finalize_linear(const float4 *acc, float scale, uint64_t n, fl
 }
 
 }  // namespace vt
+
+// ------------------------------------------------------------------ the raw stage without allocation
+// The two glare levels of vt_sensor_raw_device live in this file-scope device array, GLARE_LEVEL_MAX float4 each
+// (8 MiB; every GPU the library runs on holds its own copy until the process ends). Host code never names the array
+// in a launch or copy: it uses the address hipGetSymbolAddress returned for the device, kept per device ordinal.
+// The glare kernels write every element of a level before reading it, so the array needs no clearing between calls;
+// calls that overlap on one GPU share it and race.
+static __device__ __align__(256) float4 scratch[2 * vt::GLARE_LEVEL_MAX];
+
+namespace {
+std::mutex g_scratch_mu;
+float4 *g_scratch[vt::MAX_DEVICES] = {};
+}  // namespace
+
+// with `device` current (vt_create): the array's address on that device, once
+hipError_t vt::sensor_scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch[device] = static_cast<float4 *>(p);
+  return hipSuccess;
+}
+
+hipError_t vt::launch_sensor_noalloc(int device, const SensorDev &P, const float4 *acc, const float *ri, uint16_t *dn,
+                                     hipStream_t s) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  float4 *base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch[device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  if (P.glare_k > 0.0f) {
+    const uint64_t F = P.glare_f, level = ((P.w + F - 1) / F) * ((P.h + F - 1) / F);
+    if (F < 1 || level > (uint64_t)GLARE_LEVEL_MAX) return hipErrorInvalidValue;
+  }
+  return launch_sensor(P, acc, ri, base, base + GLARE_LEVEL_MAX, dn, s);
+}

@@ -20,7 +20,7 @@ EXPORTS = [
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
     "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
-    "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide",
+    "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide", "vt_sensor_raw_device",
 ]
 
 # camera option bits (vt_camera.flags)
@@ -163,6 +163,7 @@ def load():
         "vt_camera_ray_test": (i32, [vp, u32, vp, u32, vp]),
         "vt_bsdf_test": (i32, [vp, vp, u32, vp]),
         "vt_sensor_test": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "vt_sensor_raw_device": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "vt_isp": (i32, [vp, u32, vp]),
         "vt_weld_subdivide": (i32, [vp, vp, u32, vp, u32, f64, u32, vp, vp, u32, vp, u32, vp, vp]),
         "vt_isp_test": (i32, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
@@ -480,6 +481,13 @@ class Context:
         out = np.zeros((cases.shape[0], 8), np.float32)
         self.check(self.L.vt_bsdf_test(self.h, ptr(cases), cases.shape[0], ptr(out)))
         return out
+
+    def sensor_raw_device(self, params, width, height, radiance_ptr, ri_ptr, dn_ptr):
+        """vt_sensor_raw_device: the raw stage on device buffers given by address (radiance w*h float4, ri w*h
+        float32, dn w*h uint16), enqueued on the context's stream; allocates nothing."""
+        p = sensor_struct(params)
+        self.check(self.L.vt_sensor_raw_device(self.h, C.byref(p), width, height, C.c_void_p(radiance_ptr),
+                                               C.c_void_p(ri_ptr), C.c_void_p(dn_ptr)))
 
     def sensor_test(self, params, radiance, ri):
         radiance = np.ascontiguousarray(radiance, dtype=np.float32)

@@ -146,6 +146,7 @@ typedef struct {
 } vt_tune_result;
 
 /* Sensor parameters, the 256-B sensor uniform (gpu/sensor.js:158-177), raw-DN part. */
+#define VT_MAX_VIGNETTING 8
 typedef struct {
   uint32_t cfa;          /* 0 RGGB, 1 GRBG, 2 GBRG, 3 BGGR */
   uint32_t frame, seed;
@@ -161,6 +162,11 @@ typedef struct {
    * the Gaussian's sigma in pixels of the 1/F level, the level factor F (1..32) */
   float glare_fraction, glare_sigma;
   uint32_t glare_level;
+  /* vignetting V(theta) (physics lens.relative_illumination vignetting_theta_deg / vignetting; gpu/sensor.js:110-114):
+   * linear between `vignetting_count` knots (field angle ascending, degrees), the end values outside; 0 knots: V = 1.
+   * The relative-illumination map is the KB4 term times V. */
+  uint32_t vignetting_count;
+  float vignetting_theta_deg[VT_MAX_VIGNETTING], vignetting[VT_MAX_VIGNETTING];
 } vt_sensor_params;
 
 /* The ISP after the raw DN (sensor.wgsl.js:224-427; the uniform's wb, c0-c2, tone, flags and tone2 words,

@@ -48,6 +48,8 @@ def lib():
             "or_fresnel_dielectric": (f64, [f64, f64]),
             "or_fresnel_average": (f64, [f64, i32]),
             "or_ggx_d": (f64, [f64, f64]),
+            "or_schlick": (f64, [f64, f64]),
+            "or_ggx_lambda": (f64, [f64, f64]),
             "or_ggx_g1": (f64, [f64, f64]),
             "or_ggx_g2": (f64, [f64, f64, f64]),
             "or_eval_bsdf": (None, [vp, vp, vp, vp, vp]),
@@ -70,7 +72,7 @@ def lib():
             "or_isp": (None, [vp, vp, vp, vp, vp, vp]),
             "or_isp_green": (None, [vp, vp, vp]),
             "or_isp_tone": (f32, [vp, f32]),
-            "or_ri_map": (None, [vp, i32, i32, f64, vp]),
+            "or_ri_map": (None, [vp, i32, i32, f64, i32, vp, vp, vp]),
             "or_raw_noise_dn": (f64, [f64, f64, f64, f64]),
             "or_render_tile": (None, [vp, vp, u32, u32, u32, u32, u32, u32, vp, vp, vp, i32]),
             "or_camera_ray": (None, [vp, u32, u32, u32, vp, vp]),
@@ -486,9 +488,14 @@ def isp_tone(isp_params, x):
                     np.float32)
 
 
-def ri_map(L, w, h, pupil_exp=1.0):
+def ri_map(L, w, h, pupil_exp=1.0, vignetting_theta_deg=(), vignetting=()):
+    """gpu/sensor.js:103-124 relativeIlluminationMap: the KB4 term times the vignetting table (degrees)"""
     out = np.zeros((h, w), dtype=np.float32)
-    lib().or_ri_map(C.byref(L), w, h, pupil_exp, ptr(out))
+    vt = np.radians(np.asarray(vignetting_theta_deg, np.float64))
+    vv = np.asarray(vignetting, np.float64)
+    assert vt.size == vv.size
+    lib().or_ri_map(C.byref(L), w, h, float(pupil_exp), int(vt.size), ptr(vt) if vt.size else None,
+                    ptr(vv) if vv.size else None, ptr(out))
     return out
 
 

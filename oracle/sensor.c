@@ -266,7 +266,15 @@ void or_sensor_expose(const or_sensor *s, const float *L, const float *ri, uint1
   free(A);
 }
 
-void or_ri_map(const or_lens *L, int w, int h, double q, float *out) {
+static double vignetting(int vn, const double *vt, const double *vv, double th) {
+  if (vn <= 0) return 1;
+  if (th <= vt[0]) return vv[0];
+  for (int k = 1; k < vn; k++)
+    if (th <= vt[k]) return vv[k - 1] + (vv[k] - vv[k - 1]) * (th - vt[k - 1]) / (vt[k] - vt[k - 1]);
+  return vv[vn - 1];
+}
+
+void or_ri_map(const or_lens *L, int w, int h, double q, int vn, const double *vt, const double *vv, float *out) {
   for (int y = 0; y < h; y++)
     for (int x = 0; x < w; x++) {
       double uv[2] = {x, y}, r[3];
@@ -275,7 +283,7 @@ void or_ri_map(const or_lens *L, int w, int h, double q, float *out) {
       double th = acos(c);
       double ri = 1;
       if (th > 1e-6) ri = pow(cos(th), q) * sin(th) / (or_distort(L, th) * or_distort_deriv(L, th));
-      out[(size_t)y * w + x] = (float)(ri > 0 ? ri : 0);
+      out[(size_t)y * w + x] = (float)((ri > 0 ? ri : 0) * vignetting(vn, vt, vv, th));
     }
 }
 

@@ -145,7 +145,10 @@ void or_isp(const or_isp_p *P, const uint16_t *dn, uint8_t *rgba, uint8_t *yuv, 
  * ri [w*h] -> dn [w*h] (u16). Glare off (fraction 0). */
 void or_sensor_expose(const or_sensor *s, const float *radiance, const float *ri, uint16_t *dn);
 /* gpu/sensor.js:103-124 for the default vignetting table [0, 90 deg] -> [1, 1] */
-void or_ri_map(const or_lens *L, int w, int h, double pupil_exp, float *out);
+/* relative illumination x vignetting per pixel (gpu/sensor.js:103-124): vn knots vt_rad (ascending) / vv, linear
+ * between them, the end values outside; vn 0 = no vignetting */
+void or_ri_map(const or_lens *L, int w, int h, double pupil_exp, int vn, const double *vt_rad, const double *vv,
+               float *out);
 double or_raw_noise_dn(double e, double read_noise, double conv_gain, double gain); /* gpu/sensor.js:180-182 */
 
 /* ---------------------------------------------------------------- path tracer (the build's estimator) */

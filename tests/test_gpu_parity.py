@@ -133,6 +133,24 @@ def test_kb4_unproject_matches_fp64(ctx, c1):
     assert ang.max() < 2e-6
 
 
+def test_kb4_lens_kernel_matches_the_scene_export(ctx):
+    """The FP32 KB4 inverse (8 Newton steps, gpu/wgsl/lens.wgsl.js:9-29) on the three measured lenses of the scene
+    export: the aim target's field angle and the corner field angles the export records, within 2e-6 rad."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kb4_scene_cameras.json")
+    for c in json.load(open(path))["cameras"]:
+        K, D = c["K"], c["D"]
+        lens = np.array([K[0][0], K[1][1], K[0][2], K[1][2]] + list(D) + [K[0][1] / K[0][0]], np.float32)
+        w, h = c["size"]
+        uv = np.array([c["aim_target_px"]] + [[u, v] for u in (-0.5, (w - 1) / 2, w - 0.5)
+                                              for v in (-0.5, (h - 1) / 2, h - 0.5)], np.float32)
+        r = ctx.lens_test(lens, uv).astype(np.float64)
+        th = np.arccos(np.clip(r[:, 2] / np.linalg.norm(r, axis=1), -1, 1))
+        assert abs(th[0] - np.radians(c["aim_target_off_axis_deg"])) < 2e-6, c["name"]
+        assert abs(th[1:].max() - c["theta_corner_rad"]) < 2e-6, c["name"]
+
+
 def test_camera_rays_match_oracle(ctx, c1):
     _, cams = c1
     cam = cams[0]

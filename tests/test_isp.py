@@ -389,3 +389,32 @@ def test_gpu_physics_sensor_chain_on_a_rendered_camera():
     for k in ("y", "cb", "cr"):
         assert np.array_equal(yuv[k], want[k]), k
     print("mean luma %.3f, raw mean %.1f DN" % (stats[0] / (640 * 480 * 255), raw.mean()))
+
+
+@pytest.mark.gpu
+def test_gpu_ri_map_matches_oracle_with_vignetting():
+    """The device's relative-illumination map (vt_sensor's, read back as VT_KIND_RI) for the c1 KB4 camera against
+    the oracle's relativeIlluminationMap with a vignetting table, and again after the table changes (the map is
+    rebuilt for it)."""
+    from vtwin import lib
+    sc, cams = scenes.c1()
+    cam = dict(cams[0], spp=1)
+    lens = cam["lens"]
+    Ls = oracle.lens(*[float(x) for x in lens[:4]], [float(x) for x in lens[4:8]], float(lens[8]))
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(sc)
+        ctx.set_camera(0, cam)
+        ctx.render([0])
+        for vt, vv, q in (([0.0, 20.0, 40.0, 60.0], [1.0, 0.95, 0.7, 0.4], 1.0), ((), (), 1.0),
+                          ([10.0, 50.0], [0.9, 0.5], 0.5)):
+            sens = dict(scenes.default_sensor(), pupil_exp=q, vignetting_theta_deg=vt, vignetting=vv)
+            ctx.sensor(0, sens)
+            ri = ctx.readback(0, lib.VT_KIND_RI, 640, 480)
+            want = oracle.ri_map(Ls, 640, 480, q, vt, vv)
+            err = np.max(np.abs(ri.astype(np.float64) - want) / np.maximum(want, 1e-6))
+            print("RI map, vignetting %s, q %.1f: max relative difference %.2e" % (list(vv), q, err))
+            assert err < 1e-6
+    finally:
+        ctx.close()
+

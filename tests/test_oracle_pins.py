@@ -8,7 +8,7 @@ import pytest
 from scipy import integrate, special, stats
 
 import oracle
-from vtwin import scenes
+from vtwin import lib, scenes
 
 L = oracle.lib()
 
@@ -59,6 +59,77 @@ def test_kb4_round_trip_and_derivative():
             h = 1e-6
             num = (L.or_distort(C.byref(Ls), th + h) - L.or_distort(C.byref(Ls), th - h)) / (2 * h)
             assert L.or_distort_deriv(C.byref(Ls), th) == pytest.approx(num, rel=1e-8)
+
+
+def _scene_cameras():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kb4_scene_cameras.json")
+    return json.load(open(path))["cameras"]
+
+
+def _scene_lens(c):
+    K = c["K"]
+    return oracle.lens(K[0][0], K[1][1], K[0][2], K[1][2], c["D"], K[0][1] / K[0][0])
+
+
+def test_kb4_matches_the_scene_export():
+    """The reference's scene export records, per tracking camera, numbers its own KB4 model produced: the field
+    angle of the aim target pixel, the largest field angle of the image (corners at the pixel edges, maxImageTheta,
+    gpu/lens.js:76-83) and the fold of the mapping. The oracle's unproject reproduces the aim angle and the corner
+    angle of all three measured lenses (k1..k4 all non-zero) to rounding, so every power and coefficient of
+    distort / undistort is pinned; the fold is the root of dtheta_d/dtheta = 1 + 3k1 t^2 + 5k2 t^4 + 7k3 t^6 +
+    9k4 t^8 (or pi/2 for a lens monotone that far), within the exporter's 1e-5, and maxMonotoneTheta's 4096-step
+    scan brackets it."""
+    for c in _scene_cameras():
+        Ls = _lens_from(c)
+        r = _unproject(Ls, c["aim_target_px"])
+        ang = math.degrees(math.acos(r[2] / np.linalg.norm(r)))
+        assert ang == pytest.approx(c["aim_target_off_axis_deg"], abs=1e-10), c["name"]
+        K = c["K"]
+        assert math.hypot(c["aim_target_px"][0] - K[0][2], c["aim_target_px"][1] - K[1][2]) == pytest.approx(
+            c["aim_target_off_principal_px"], abs=1e-9)
+        w, h = c["size"]
+        assert L.or_max_image_theta(C.byref(Ls), w, h) == pytest.approx(c["theta_corner_rad"], abs=1e-12), c["name"]
+        k1, k2, k3, k4 = c["D"]
+        roots = [z.real for z in np.roots([9 * k4, 7 * k3, 5 * k2, 3 * k1, 1.0]) if abs(z.imag) < 1e-12 and z.real > 0]
+        fold = min([math.sqrt(z) for z in roots] + [math.pi / 2])
+        assert fold == pytest.approx(c["theta_fold_rad"], abs=1e-5), c["name"]
+        assert L.or_distort_deriv(C.byref(Ls), fold * (1 - 1e-9)) > 0
+        if fold < math.pi / 2:
+            assert L.or_distort_deriv(C.byref(Ls), fold * (1 + 1e-6)) < 0
+            t = L.or_max_monotone_theta(C.byref(Ls), math.pi)
+            assert t <= c["theta_fold_rad"] < t + math.pi / 4096
+        else:
+            assert L.or_max_monotone_theta(C.byref(Ls), math.pi / 2) == pytest.approx(math.pi / 2)
+
+
+def _lens_from(c):
+    return _scene_lens(c)
+
+
+def test_ri_map_closed_forms_and_vignetting():
+    """relativeIlluminationMap (gpu/sensor.js:103-124): for k = 0 the equidistant lens gives cos^q(t) sin(t) / t at
+    field angle t = radius / f; the vignetting table is read linearly between its knots and held at its ends."""
+    Ls = _lens([0, 0, 0, 0])
+    w, h, f = 640, 480, 300.0
+    vt, vv = [0.0, 20.0, 40.0, 60.0], [1.0, 0.9, 0.6, 0.3]
+    for q in (1.0, 0.5):
+        ri = oracle.ri_map(Ls, w, h, q, vt, vv)
+        ri0 = oracle.ri_map(Ls, w, h, q)
+        for (x, y) in ((319, 239), (500, 100), (10, 470), (639, 0), (330, 250)):
+            t = math.hypot(x - 319.5, y - 239.5) / f
+            want = math.cos(t) ** q * math.sin(t) / t
+            assert ri0[y, x] == pytest.approx(want, rel=1e-6)
+            deg = math.degrees(t)
+            v = np.interp(deg, vt, vv)
+            assert ri[y, x] == pytest.approx(want * v, rel=1e-6), (x, y, deg)
+    # the ends are held: a table starting at 30 degrees gives its first value on the axis
+    ri = oracle.ri_map(Ls, w, h, 1.0, [30.0, 50.0], [0.8, 0.5])
+    assert ri[239, 319] == pytest.approx(0.8 * oracle.ri_map(Ls, w, h, 1.0)[239, 319], rel=1e-6)
+    corner_deg = math.degrees(math.hypot(319.5, 239.5) / f)
+    assert corner_deg > 50
+    assert ri[0, 0] == pytest.approx(0.5 * oracle.ri_map(Ls, w, h, 1.0)[0, 0], rel=1e-6)
 
 
 def test_max_monotone_theta_brackets_the_turning_point():
@@ -175,6 +246,94 @@ def test_ggx_d_normalised():
     for a in (0.05, 0.3, 0.8):
         val = integrate.quad(lambda mu: L.or_ggx_d(mu, a) * mu * 2 * math.pi, 0, 1, limit=400, points=[1.0])[0]
         assert val == pytest.approx(1.0, abs=2e-4)
+
+
+@pytest.mark.parametrize("rough", [0.3, 0.55, 0.8])
+def test_ggx_weak_white_furnace(rough):
+    """The visible-normal distribution integrates to one for every view (Heitz 2014, the weak white furnace):
+    int G1(v) max(0, v.h) D(h) / (n.v) dw_h = 1. It ties the Smith masking G1 = 1 / (1 + Lambda) to D: a different
+    Lambda (a wrong power of tan or of alpha) or an unnormalised D fails it."""
+    a = rough * rough
+    nt, npz = 6000, 256
+    mu = (np.arange(nt) + 0.5) / nt
+    ph = (np.arange(npz) + 0.5) * 2 * math.pi / npz
+    Mu, Ph = np.meshgrid(mu, ph, indexing="ij")
+    s = np.sqrt(1 - Mu * Mu)
+    h = np.stack([s * np.cos(Ph), s * np.sin(Ph), Mu], -1).reshape(-1, 3)
+    dw = (1.0 / nt) * (2 * math.pi / npz)
+    D = np.array([L.or_ggx_d(float(m), a) for m in mu])
+    Dh = np.repeat(D[:, None], npz, 1).reshape(-1)
+    for tv in (0.0, 0.6, 1.2, 1.45):
+        v = np.array([math.sin(tv), 0.0, math.cos(tv)])
+        nv = v[2]
+        g1 = L.or_ggx_g1(nv, a)
+        val = g1 * np.sum(np.maximum(0.0, h @ v) * Dh) * dw / nv
+        assert val == pytest.approx(1.0, abs=2e-3), (rough, tv)
+
+
+def test_ggx_height_correlated_smith_and_schlick_limits():
+    """G2 is the height-correlated form 1 / (1 + Lambda(l) + Lambda(v)) (gpu/bsdf.js:59), so at l = v it equals
+    1 / (2 / G1 - 1), which the separable product G1^2 does not; Lambda vanishes at normal incidence. Schlick's
+    Fresnel is F0 at normal incidence, 1 at grazing and (1 - c)^5 in between (gpu/bsdf.js:44)."""
+    for a in (0.01, 0.09, 0.4):
+        assert L.or_ggx_lambda(1.0, a) == pytest.approx(0.0, abs=1e-15)
+        for c in (0.05, 0.3, 0.7, 0.95):
+            g1 = L.or_ggx_g1(c, a)
+            assert L.or_ggx_g2(c, c, a) == pytest.approx(1 / (2 / g1 - 1), rel=1e-12)
+            if a >= 0.4 and c < 0.5:        # where the two forms differ by more than rounding shows
+                assert abs(L.or_ggx_g2(c, c, a) - g1 * g1) > 1e-3
+    for f0 in (0.04, 0.5, 0.95):
+        assert L.or_schlick(f0, 1.0) == pytest.approx(f0, abs=1e-15)
+        assert L.or_schlick(f0, 0.0) == pytest.approx(1.0, abs=1e-15)
+        assert L.or_schlick(f0, 0.5) == pytest.approx(f0 + (1 - f0) * 0.5 ** 5, abs=1e-15)
+        xs = [L.or_schlick(f0, c) for c in np.linspace(0, 1, 21)]
+        assert all(x >= y for x, y in zip(xs, xs[1:]))
+    # exact dielectric Fresnel: normal incidence ((eta - 1) / (eta + 1))^2, grazing 1
+    for eta in (1.33, 1.5, 2.4):
+        assert L.or_fresnel_dielectric(1.0, eta) == pytest.approx(((eta - 1) / (eta + 1)) ** 2, rel=1e-12)
+        assert L.or_fresnel_dielectric(0.0, eta) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_jacobi_update_delta_field_on_a_grid():
+    """jacobiUpdate's smoothing (kernels.wgsl.js:440-461) on nodes of a 3 mm square grid, sigma 3 mm, radius 6.5 mm:
+    every interior node sees the same 13-node stencil (offsets 0, 3, 4.24 and 6 mm), so for aux = 1 at one node and 0
+    elsewhere each interior neighbour of that node gets E_free = exp(-d^2 / (2 sigma^2)) / S with S the stencil's
+    weight sum from the grid geometry, and the source L = (B_dir + E_free V A_hh) / (pi A_hh). A missing weight, a
+    missing normalisation or a node id taken for an active index fails it."""
+    g = 11
+    xs, ys = np.meshgrid(np.arange(g) * 0.003, np.arange(g) * 0.003, indexing="ij")
+    pos = np.stack([xs.ravel(), ys.ravel(), np.zeros(g * g)], 1).astype(np.float32)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (g * g, 1))
+    n_static = g * g + 5                      # node ids 0..4 are not active: active index != node id
+    active = (np.arange(g * g) + 5).astype(np.uint32)
+    nb, nw = lib.node_neighbours(pos, nrm, 16, 0.0065, 0.003)
+    sig2 = 0.003 ** 2
+    S = 1 + 4 * math.exp(-0.5) + 4 * math.exp(-1.0) + 4 * math.exp(-2.0)
+    centre = (g // 2) * g + g // 2
+    aux = np.zeros((g * g, 4), np.float32)
+    aux[centre, 0:3] = (1.0, 2.0, 0.5)
+    state = np.zeros((n_static, 16), np.float32)
+    state[active, 3] = 0.8                    # V
+    state[active, 12:15] = 0.1                # B_dir
+    node_mat = np.zeros(n_static, np.uint32)
+    hh = np.array([[0.5, 0.25, 0.8]], np.float32)
+    L.or_jacobi_update(g * g, oracle.ptr(active), oracle.ptr(nb), oracle.ptr(nw), 16, oracle.ptr(aux),
+                       oracle.ptr(node_mat), oracle.ptr(np.ascontiguousarray(hh.reshape(-1))), oracle.ptr(state), 2)
+    cx, cy = pos[centre, 0], pos[centre, 1]
+    checked = 0
+    for k in range(g * g):
+        ix, iy = divmod(k, g)
+        if not (2 <= ix < g - 2 and 2 <= iy < g - 2):
+            continue                           # the stencil is whole only two nodes in from the border
+        d2 = float((pos[k, 0] - cx) ** 2 + (pos[k, 1] - cy) ** 2)
+        w = math.exp(-0.5 * d2 / sig2) if d2 <= 0.0065 ** 2 else 0.0
+        want = np.array([1.0, 2.0, 0.5]) * w / S
+        assert np.allclose(state[active[k], 0:3], want, rtol=1e-5, atol=1e-9), k
+        Lw = (0.1 + want * 0.8 * hh[0]) / (math.pi * hh[0])
+        assert np.allclose(state[active[k], 8:11], Lw, rtol=1e-5), k
+        assert state[active[k], 11] == 1.0
+        checked += w > 0
+    assert checked == 13
 
 
 def test_dielectric_directional_albedo_at_most_one():

@@ -94,8 +94,19 @@ void host_apply(const HostPose &P, const double v[3], double out[3]) {
 }
 
 
-// relative illumination per pixel (gpu/sensor.js:103-124), FP64, default vignetting (1 at every angle)
-void ri_map_host(const vt_camera &cam, double q, std::vector<float> &out) {
+// relative illumination per pixel (gpu/sensor.js:103-124), FP64: cos^q sin / (theta_d dtheta_d/dtheta) at the pixel's
+// field angle, times the vignetting table read linearly between its knots and held at its ends
+void ri_map_host(const vt_camera &cam, double q, uint32_t vn, const float *vt_deg, const float *vv,
+                 std::vector<float> &out) {
+  double vt[VT_MAX_VIGNETTING];
+  for (uint32_t k = 0; k < vn; k++) vt[k] = (double)vt_deg[k] * M_PI / 180.0;
+  auto vig = [&](double th) -> double {
+    if (vn == 0) return 1.0;
+    if (th <= vt[0]) return vv[0];
+    for (uint32_t k = 1; k < vn; k++)
+      if (th <= vt[k]) return vv[k - 1] + ((double)vv[k] - vv[k - 1]) * (th - vt[k - 1]) / (vt[k] - vt[k - 1]);
+    return vv[vn - 1];
+  };
   const double fx = cam.lens[0], fy = cam.lens[1], cx = cam.lens[2], cy = cam.lens[3], al = cam.lens[8];
   const double k[4] = {cam.lens[4], cam.lens[5], cam.lens[6], cam.lens[7]};
   auto dist = [&](double th) {
@@ -135,7 +146,7 @@ void ri_map_host(const vt_camera &cam, double q, std::vector<float> &out) {
         if (cam.mode == 0) ri = std::pow(std::cos(th), q) * std::sin(th) / (dist(th) * deriv(th));
         else ri = std::pow(std::cos(th), q + 3);   // pinhole mapping: cos^3 * cos^q
       }
-      out[(size_t)y * w + x] = (float)std::max(0.0, ri);
+      out[(size_t)y * w + x] = (float)(std::max(0.0, ri) * vig(th));
     }
 }
 

@@ -50,6 +50,8 @@ struct CamSlot {
   bool isp_done = false, isp420 = false;
   bool ri_valid = false;
   float ri_pupil = -1.0f;
+  uint32_t ri_vn = 0;                     // the vignetting table the RI map was made with
+  float ri_vt[VT_MAX_VIGNETTING] = {}, ri_vv[VT_MAX_VIGNETTING] = {};
   uint32_t spp_total = 0;     // samples accumulated in acc (for the mean)
   bool rendered = false, sensed = false;
 };
@@ -171,8 +173,10 @@ void host_apply(const HostPose &P, const double v[3], double out[3]);
 // grid for the trace kernels: every CU filled to its occupancy, never more lanes than spill slots
 inline uint32_t max_blocks(vt_ctx *c, uint32_t block) { return (uint32_t)(c->spill_lanes / block); }
 
-// relative illumination per pixel (gpu/sensor.js:103-124), FP64
-void ri_map_host(const vt_camera &cam, double q, std::vector<float> &out);
+// relative illumination per pixel (gpu/sensor.js:103-124), FP64: the KB4 term times the vignetting table
+// (vn knots, field angles in degrees)
+void ri_map_host(const vt_camera &cam, double q, uint32_t vn, const float *vt_deg, const float *vv,
+                 std::vector<float> &out);
 
 // the node cache's material-dependent tables (vt_radiosity_api.cpp): the directional-albedo tables and the gather
 // entries' static weights, recomputed on the stream after a material change

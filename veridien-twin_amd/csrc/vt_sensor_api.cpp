@@ -23,6 +23,12 @@ int vt::sensor_params(vt_ctx *c, const vt_sensor_params *p, uint32_t w, uint32_t
     return fail(c, VT_ERR_ARG, "sensor: glare fraction outside [0, 1]");
   if (p->glare_fraction > 0.0f && (p->glare_level < 1 || p->glare_level > 64))
     return fail(c, VT_ERR_ARG, "sensor: glare level factor %u outside 1..64", p->glare_level);
+  if (p->vignetting_count > VT_MAX_VIGNETTING)
+    return fail(c, VT_ERR_ARG, "sensor: at most %d vignetting knots", VT_MAX_VIGNETTING);
+  for (uint32_t k = 0; k < p->vignetting_count; k++)
+    if (!std::isfinite(p->vignetting_theta_deg[k]) || !(p->vignetting[k] >= 0.0f) ||
+        (k && !(p->vignetting_theta_deg[k] > p->vignetting_theta_deg[k - 1])))
+      return fail(c, VT_ERR_ARG, "sensor: vignetting knots need ascending angles and values >= 0 (knot %u)", k);
   return VT_OK;
 }
 size_t vt::glare_bytes(uint32_t w, uint32_t h, const SensorDev &P) {
@@ -71,12 +77,19 @@ extern "C" int vt_sensor(vt_ctx *c, uint32_t cam, const vt_sensor_params *p) {
   const float inv = s.c.scale > 0 ? s.c.scale : 1.0f;     // over each pixel's own sample count
   int rc = sensor_params(c, p, s.c.width, s.c.height, s.c.cam_index, inv, P);
   if (rc) return rc;
-  if (!s.ri_valid || s.ri_pupil != p->pupil_exp) {
+  const uint32_t vn = p->vignetting_count;
+  const bool same_vig = s.ri_vn == vn &&
+                        std::memcmp(s.ri_vt, p->vignetting_theta_deg, vn * sizeof(float)) == 0 &&
+                        std::memcmp(s.ri_vv, p->vignetting, vn * sizeof(float)) == 0;
+  if (!s.ri_valid || s.ri_pupil != p->pupil_exp || !same_vig) {
     std::vector<float> ri;
-    ri_map_host(s.c, p->pupil_exp, ri);
+    ri_map_host(s.c, p->pupil_exp, vn, p->vignetting_theta_deg, p->vignetting, ri);
     VT_HIP(c, hipMemcpy(s.ri.p, ri.data(), ri.size() * 4, hipMemcpyHostToDevice));
     s.ri_valid = true;
     s.ri_pupil = p->pupil_exp;
+    s.ri_vn = vn;
+    std::memcpy(s.ri_vt, p->vignetting_theta_deg, sizeof s.ri_vt);
+    std::memcpy(s.ri_vv, p->vignetting, sizeof s.ri_vv);
   }
   if (const size_t gb = glare_bytes(s.c.width, s.c.height, P)) {
     if ((rc = ensure(c, s.glare_a, gb)) || (rc = ensure(c, s.glare_b, gb))) return rc;

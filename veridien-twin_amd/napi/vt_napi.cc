@@ -477,6 +477,24 @@ napi_value Sensor(napi_env env, napi_callback_info info) {
   p.glare_fraction = (float)num(env, o, "glareFraction", 0.0);
   p.glare_sigma = (float)num(env, o, "glareSigma", 1.0);
   p.glare_level = (uint32_t)num(env, o, "glareLevel", 16);
+  {
+    napi_value vt, vv;
+    uint32_t nt = 0, nv = 0;
+    if (get_prop(env, o, "vignettingThetaDeg", &vt) && get_prop(env, o, "vignetting", &vv)) {
+      napi_get_array_length(env, vt, &nt);
+      napi_get_array_length(env, vv, &nv);
+      if (nt != nv || nt > VT_MAX_VIGNETTING) {
+        napi_throw_range_error(env, nullptr, "vignettingThetaDeg and vignetting need the same length, at most 8");
+        return nullptr;
+      }
+      if (!float_list(env, o, "vignettingThetaDeg", p.vignetting_theta_deg, nt) ||
+          !float_list(env, o, "vignetting", p.vignetting, nt)) {
+        napi_throw_type_error(env, nullptr, "vignetting tables must hold numbers");
+        return nullptr;
+      }
+      p.vignetting_count = nt;
+    }
+  }
   int rc = vt_sensor(c, slot, &p);
   if (rc) return throw_vt(env, c, rc, "vt_sensor");
   return nullptr;

@@ -83,7 +83,8 @@ class SensorParams(C.Structure):
     _fields_ = [("cfa", u32), ("frame", u32), ("seed", u32), ("optics_k", f32), ("esf_sigma", f32),
                 ("exposure", f32), ("gain", f32), ("m", f32 * 9), ("read_noise", f32), ("full_well", f32),
                 ("dark", f32), ("conv_gain", f32), ("max_dn", f32), ("black", f32), ("noise_on", u32),
-                ("pupil_exp", f32), ("glare_fraction", f32), ("glare_sigma", f32), ("glare_level", u32)]
+                ("pupil_exp", f32), ("glare_fraction", f32), ("glare_sigma", f32), ("glare_level", u32),
+                ("vignetting_count", u32), ("vignetting_theta_deg", f32 * 8), ("vignetting", f32 * 8)]
 
 
 class IspParams(C.Structure):
@@ -564,6 +565,11 @@ def sensor_struct(p):
     s.glare_fraction = float(p.get("glare_fraction", 0.0))
     s.glare_sigma = float(p.get("glare_sigma", 1.0))
     s.glare_level = int(p.get("glare_level", 16))
+    vt, vv = list(p.get("vignetting_theta_deg", ())), list(p.get("vignetting", ()))
+    assert len(vt) == len(vv) <= 8
+    s.vignetting_count = len(vt)
+    s.vignetting_theta_deg[:len(vt)] = [float(x) for x in vt]
+    s.vignetting[:len(vv)] = [float(x) for x in vv]
     return s
 
 

@@ -415,6 +415,16 @@ int vt_interior_bvh(vt_ctx *ctx, uint32_t enable, uint32_t *interior_tris);
  * of its indirect light); then `iters` Jacobi steps (jacobiGather + jacobiUpdate, ping-pong source slots), warm
  * started from the previous state. Enqueue only. */
 int vt_radiosity_update(vt_ctx *ctx, uint32_t direct, uint32_t iters);
+/* The incremental node update of _ensureRadiosity (gpu/renderer.js:756-795: _movedSinceSolve, _submitNodeDirect
+ * with a moved mask; nodeDirect's moved-body test, gpu/wgsl/kernels.wgsl.js:357-366). moved: bit b set for each
+ * body whose pose changed since the last direct pass (bodies < 32). The direct pass then relights only the active
+ * nodes whose segment to the emitter's centre passes within (bounding-capsule radius + emitter radius + 1 mm) of a
+ * moved body's capsule, now or at the last direct pass; every other node keeps its B_dir, visibility code and
+ * capsule visibility. A full pass (every node) runs instead when no direct pass ran since vt_radiosity_build or a
+ * material, light or profile change made the cache stale; moved = 0 runs no direct pass. Then `iters` Jacobi
+ * steps as vt_radiosity_update. ran (optional): 0 no direct pass, 1 incremental, 2 full. Enqueue only. */
+enum { VT_DIRECT_NONE = 0, VT_DIRECT_INCREMENTAL = 1, VT_DIRECT_FULL = 2 };
+int vt_radiosity_update_moved(vt_ctx *ctx, uint32_t moved, uint32_t iters, uint32_t *ran);
 /* nodeSummary (gpu/renderer.js:824-852): the node state (16 floats per node: E_free.rgb + V, slot 1 source,
  * slot 2 source (w = 1 for a solved node), B_dir.rgb + visibility code) into node_state (NULL: skipped; else
  * nodes * 64 bytes) and the area-weighted mean irradiance eAvg over the active nodes, which the reference

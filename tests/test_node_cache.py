@@ -255,6 +255,85 @@ def test_gpu_node_cache_follows_material_and_light_updates(ctx):
     ctx.update_lights(lights)
 
 
+def _quat_mat(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _seg_point_dist(p, q, x):
+    """Distance from each segment p[i] -> q (q one point) to each point x[j]: (len(p), len(x))."""
+    d = q - p
+    t = np.clip(np.einsum("ik,ijk->ij", d, x[None, :, :] - p[:, None, :]) / np.einsum("ik,ik->i", d, d)[:, None], 0, 1)
+    c = p[:, None, :] + t[..., None] * d[:, None, :]
+    return np.linalg.norm(c - x[None], axis=2)
+
+
+@pytest.mark.gpu
+def test_gpu_incremental_node_direct(ctx):
+    """The incremental node update after one body moved (_movedSinceSolve / _submitNodeDirect with a mask,
+    gpu/renderer.js:756-795; nodeDirect's moved-body test, kernels.wgsl.js:357-366), under one small ceiling light.
+    B_dir and the visibility codes equal a full pass at the new pose bit for bit at every active node; nodes kept
+    from the last pass exist (their capsule visibility is the old pose's, as in the reference) and every one of them
+    has its segment to the light's centre farther than the light's radius from the moved body's surface, then and
+    now; the full pass with posed bodies matches the oracle; a stale cache falls back to a full pass, no moved body
+    to none, and a mask past the body count is refused."""
+    from test_bodies import _poses
+    parts, meshes, mats, lights = scenes.posed_parts()
+    P, N, I = scenes.quad((0.0, 0.0, 2.4), 0.3, (0.0, 0.0, -1.0))
+    parts = parts[:4] + [(P, N, I, np.full(I.shape[0], 16), None)]
+    sc = scenes.posed_scene(parts, meshes[:17], mats, [[40.0, 40.0, 40.0, 0.0]])
+    poses = _poses(True)
+    poses[1].update(occ_a=(0, 0, -0.3), occ_b=(0, 0, 0.3), occ_r=0.15, rho_eff=0.5)
+    t = _gpu_cache(ctx, sc, poses=poses)
+    a = t["active"]
+    nodes = np.asarray(sc["verts"]).size // 8
+    assert ctx.radiosity_update_moved(0b10, iters=0) == "full"        # nothing solved since the build
+    stA, _ = ctx.radiosity_state(nodes)
+    assert ctx.radiosity_update_moved(0, iters=0) == "none"
+    assert np.array_equal(ctx.radiosity_state(nodes)[0].view(np.uint32), stA.view(np.uint32))
+    posesB = [dict(p) for p in poses]
+    posesB[1]["p0"] = tuple(np.asarray(poses[1]["p0"]) + (0.2, 0.15, 0.0))
+    ctx.set_instances(posesB)
+    assert ctx.radiosity_update_moved(0b10, iters=0) == "incremental"
+    st_inc, _ = ctx.radiosity_state(nodes)
+    ctx.radiosity_update(direct=True, iters=0)
+    st_full, _ = ctx.radiosity_state(nodes)
+    assert np.array_equal(st_inc[a, 12:16].view(np.uint32), st_full[a, 12:16].view(np.uint32))
+    kept = (st_inc[a, 3] == stA[a, 3]) & (st_full[a, 3] != stA[a, 3])
+    relit = (st_inc[a, 3] == st_full[a, 3]) & (st_full[a, 3] != stA[a, 3])
+    print("active %d: capsule visibility changed at %d, kept %d, relit %d" % (
+        a.size, int(np.sum(st_full[a, 3] != stA[a, 3])), int(kept.sum()), int(relit.sum())))
+    assert kept.sum() > 0 and relit.sum() > 0 and np.all(kept | relit | (st_full[a, 3] == stA[a, 3]))
+    # kept nodes: the segment to the light's centre clears the moved body's surface by the light's radius
+    ni = oracle.node_inputs(sc)
+    Pl = P.astype(np.float64)
+    c = Pl.mean(axis=0)
+    R = np.linalg.norm(Pl - c, axis=1).max()
+    Pb, _, Ib, _, _ = parts[3]
+    Pb = np.asarray(Pb, np.float64)
+    surf = np.concatenate([Pb, Pb[np.asarray(Ib)].mean(axis=1)])
+    for pose in (poses[1], posesB[1]):
+        w = surf @ _quat_mat(pose["q0"]).T + np.asarray(pose["p0"])
+        dmin = _seg_point_dist(ni["npos"][a[kept], :3].astype(np.float64), c, w).min()
+        print("kept nodes: closest approach to the moved body %.4f m (light radius %.4f m)" % (dmin, R))
+        assert dmin > R
+    # the full pass with posed bodies against the oracle
+    st = np.zeros_like(st_full)
+    oracle.node_direct(dict(sc, poses=posesB), ni, a, 4, t["alb"], st)
+    B, Bo = st_full[a, 12:15], st[a, 12:15]
+    print("node B_dir with bodies: rel err %.2e, visibility codes equal %.5f" % (
+        np.linalg.norm(B - Bo) / np.linalg.norm(Bo), np.mean(st_full[a, 15] == st[a, 15])))
+    assert np.linalg.norm(B - Bo) <= 1e-3 * np.linalg.norm(Bo)
+    assert np.mean(st_full[a, 15] == st[a, 15]) > 0.995
+    # a material change makes the cache stale: the next incremental request relights everything
+    ctx.update_materials(np.asarray(sc["materials"], np.float32))
+    assert ctx.radiosity_update_moved(0b01, iters=0) == "full"
+    with pytest.raises(lib.VTError):
+        ctx.radiosity_update_moved(0b100, iters=0)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("opts", [0, lib.VT_OPT_CULL_SHADOWS, lib.VT_OPT_DIRECT_ONLY, lib.VT_OPT_NO_INDIRECT,
                                   lib.VT_OPT_NORMALS])

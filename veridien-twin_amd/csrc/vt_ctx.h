@@ -132,6 +132,10 @@ struct vt_ctx {
   // a material, light or profile change since the node cache's last direct pass: the next vt_radiosity_update
   // relights every active node whatever its `direct` argument says (B_dir holds albedo x radiance)
   bool rad_direct_stale = false;
+  // the bodies' bounding capsules at the node cache's last direct pass (2 float4 per body), which the next
+  // incremental pass compares with; rad_pcaps_valid once a direct pass has run since the build
+  DevBuf rad_pcaps;
+  bool rad_pcaps_valid = false;
   // vt_jacobi_sparse's own copies of the caller's tables (never the node cache's buffers)
   DevBuf s_active, s_table, s_normals, s_mat, s_alb, s_hh, s_nbr, s_nbrw, s_state, s_aux;
   // dense radiosity

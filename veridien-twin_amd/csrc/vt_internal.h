@@ -190,6 +190,10 @@ struct NodeLaunch {
   uint2 *spill;
   uint64_t spill_stride;
   uint32_t max_blocks;       // spill_stride / 256
+  // k_node_direct's incremental pass: bit b = body b moved (0: every active node); pcaps: per body, the
+  // bounding capsule (a.xyz + radius, b.xyz) at the last direct pass
+  uint32_t moved;
+  const float4 *pcaps;
 };
 hipError_t launch_albedo_tables(const NodeLaunch &L, hipStream_t s);
 hipError_t launch_fill_f4(float4 *p, uint32_t n, float4 v, hipStream_t s);    // a fill kernel (no memset nodes)

@@ -148,6 +148,26 @@ __global__ __launch_bounds__(256) void k_gather_remap(NodeLaunch L) {
 // the group's sums meet in a fixed butterfly and its first lane writes the node.
 constexpr uint32_t NODE_GROUP = 4;
 
+// The incremental pass (kernels.wgsl.js:357-366): with a moved-body mask, a node keeps its direct light unless
+// the segment from it to the emitter's centre passes within (capsule radius + emitter radius + 1 mm) of a moved
+// body's bounding capsule, where the body is now or where it was at the last direct pass. Every shadow segment
+// from the node to a point of the emitter lies within the emitter radius of that segment, so a node kept this
+// way sends no shadow ray near the moved bodies, and its samples come out as a full pass would compute them.
+__device__ __forceinline__ bool near_moved(const NodeLaunch &L, v3 p) {
+  const float4 ec = L.S.radu[1];
+  const v3 dE = ld3(ec) - p;
+  const float rr = ec.w + 1e-3f;
+  for (uint32_t b = 0; b < L.S.inst_count; b++) {
+    if (!((L.moved >> b) & 1u)) continue;
+    const float4 ca = L.S.inst[INST_WORDS * (size_t)b + 6], cb = L.S.inst[INST_WORDS * (size_t)b + 7];
+    const float4 pa = L.pcaps[2 * b], pb = L.pcaps[2 * b + 1];
+    if (seg_near_capsule(p, dE, 0.0f, 1.0f, ld3(ca), ld3(cb), ca.w + rr) ||
+        seg_near_capsule(p, dE, 0.0f, 1.0f, ld3(pa), ld3(pb), pa.w + rr))
+      return true;
+  }
+  return false;
+}
+
 template <bool INST>
 __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
   NODE_STACK(L);
@@ -157,6 +177,8 @@ __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
     const uint32_t k = (uint32_t)(t / NODE_GROUP), q = (uint32_t)(t % NODE_GROUP);
     const uint32_t i = L.active[k];
     const v3 p = ld3(L.npos[i]), n = ld3(L.nnorm[i]);
+    // the group's four lanes take the same branch, so the butterfly below stays within live lanes
+    if (INST && L.moved && !near_moved(L, p)) continue;
     const uint32_t m = L.nmat[i];
     const uint32_t hs = pcg(i * 2246822519u + 3u);
     const float s0 = u01(pcg(hs)), s1 = u01(pcg(hs + 1u)), s2 = u01(pcg(hs + 2u));

@@ -301,6 +301,7 @@ extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radi
     c->rad_weights = true;
   }
   c->rad_direct_stale = false;
+  c->rad_pcaps_valid = false;
   c->rad_slot = 1;
   VT_HIP(c, launch_fill_f4((float4 *)c->radu.p, 1, make_float4(0.f, 0.f, 0.f, bits_f(1u)), c->stream));
   VT_HIP(c, launch_fill_f4((float4 *)c->radu.p + 1, 1, make_float4((float)cen[0], (float)cen[1], (float)cen[2], (float)rad),
@@ -439,14 +440,28 @@ int vt::refresh_node_materials(vt_ctx *c) {
   return VT_OK;
 }
 
-extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
-  VT_LIVE(c);
-  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
-  VT_HIP(c, hipSetDevice(c->device));
-  c->S.nstate = (const float4 *)c->g_state.p;     // a scene's instances may have changed S since the build
+namespace {
+
+// nodeDirect over every active node (moved = 0) or over those near the moved bodies, then the bodies' bounding
+// capsules become the snapshot the next incremental pass compares with (_solvedCaps, gpu/renderer.js:786)
+int direct_pass(vt_ctx *c, uint32_t moved) {
+  const uint32_t B = (uint32_t)c->body_root.size();
+  if (int rc = ensure(c, c->rad_pcaps, std::max<size_t>(B, 1) * 2 * sizeof(float4))) return rc;
   NodeLaunch L = node_launch(c);
-  if (direct || c->rad_direct_stale) VT_HIP(c, launch_node_direct(L, c->stream));
+  L.moved = moved;
+  L.pcaps = (const float4 *)c->rad_pcaps.p;
+  VT_HIP(c, launch_node_direct(L, c->stream));
+  // instance words 6 and 7 of each body: its bounding capsule
+  if (B)
+    VT_HIP(c, hipMemcpy2DAsync(c->rad_pcaps.p, 2 * sizeof(float4), (const char *)c->inst.p + 6 * sizeof(float4),
+                               INST_WORDS * sizeof(float4), 2 * sizeof(float4), B, hipMemcpyDeviceToDevice, c->stream));
+  c->rad_pcaps_valid = true;
   c->rad_direct_stale = false;
+  return VT_OK;
+}
+
+// `iters` Jacobi steps from the current source slot, then eAvg and the slot on the device
+int jacobi_steps(vt_ctx *c, uint32_t iters) {
   GatherDev G{};
   G.n_active = c->rad_active; G.gather_k = c->rad_K; G.nbr_k = c->rad_nbr_k; G.bins = ALB_BINS;
   G.active = (const uint32_t *)c->g_active.p; G.gtable = (const uint32_t *)c->g_table.p;
@@ -464,9 +479,38 @@ extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
   }
   // eAvg (the mean irradiance a body-to-body bounce reads) and the source slot, on the device
   if (int erc = ensure(c, c->eavg_part, 256 * 4 * sizeof(double))) return erc;
+  const NodeLaunch L = node_launch(c);
   VT_HIP(c, launch_eavg(L, (const float *)c->g_area.p, (float4 *)c->radu.p, c->rad_slot, (double *)c->eavg_part.p,
                         c->stream));
   return VT_OK;
+}
+
+}  // namespace
+
+extern "C" int vt_radiosity_update(vt_ctx *c, uint32_t direct, uint32_t iters) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
+  VT_HIP(c, hipSetDevice(c->device));
+  c->S.nstate = (const float4 *)c->g_state.p;     // a scene's instances may have changed S since the build
+  if (direct || c->rad_direct_stale)
+    if (int rc = direct_pass(c, 0)) return rc;
+  return jacobi_steps(c, iters);
+}
+
+extern "C" int vt_radiosity_update_moved(vt_ctx *c, uint32_t moved, uint32_t iters, uint32_t *ran) {
+  VT_LIVE(c);
+  if (!c->rad_built) return fail(c, VT_ERR_STATE, "radiosity: not built");
+  const uint32_t B = (uint32_t)c->body_root.size();
+  if (B < 32 && (moved >> B)) return fail(c, VT_ERR_ARG, "radiosity_update_moved: mask 0x%x names bodies past %u", moved, B);
+  VT_HIP(c, hipSetDevice(c->device));
+  c->S.nstate = (const float4 *)c->g_state.p;
+  uint32_t kind = VT_DIRECT_NONE;
+  if (c->rad_direct_stale || (moved && !c->rad_pcaps_valid)) kind = VT_DIRECT_FULL;
+  else if (moved) kind = VT_DIRECT_INCREMENTAL;
+  if (kind != VT_DIRECT_NONE)
+    if (int rc = direct_pass(c, kind == VT_DIRECT_FULL ? 0u : moved)) return rc;
+  if (ran) *ran = kind;
+  return jacobi_steps(c, iters);
 }
 
 extern "C" int vt_radiosity_tables(vt_ctx *c, uint32_t which, void *dst, size_t bytes) {

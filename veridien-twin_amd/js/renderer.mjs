@@ -205,9 +205,29 @@ export class TwinRenderer {
   }
 
   /**
-   * The node cache for the current poses (_ensureRadiosity, gpu/renderer.js:766-795): built on first use (gather
+   * Bodies whose pose record differs from the one of the last node update, as a bit mask (_movedSinceSolve,
+   * gpu/renderer.js:756-769); null when nothing was solved yet.
+   */
+  _movedSinceSolve() {
+    const last = this._solvedInst;
+    if (!last) return null;
+    let mask = 0;
+    (this.instances || []).forEach((I, i) => {
+      const L = last[i];
+      const eq = (a, b, tol) => a.every((x, k) => Math.abs(x - b[k]) < tol);
+      const same = (!I && !L) || (I && L && eq(I.q0, L.q0, 1e-9) && eq(I.q1, L.q1, 1e-9) && eq(I.p0, L.p0, 1e-10) &&
+        eq(I.p1, L.p1, 1e-10));
+      if (!same) mask |= 1 << i;
+    });
+    return mask >>> 0;
+  }
+
+  /**
+   * The node cache for the current poses (_ensureRadiosity, gpu/renderer.js:771-795): built on first use (gather
    * table, activity, neighbours, albedo tables), then the direct light and itersInitial Jacobi steps; after a
-   * pose change the direct light again and itersPerFrame steps.
+   * pose change the direct light of the nodes near the moved bodies (or of every node when the light's body
+   * moved) and itersPerFrame steps; nothing moved, no work. stats.lastRadiosity: 'full', 'incremental' or
+   * 'reused'.
    */
   _ensureRadiosity() {
     const o = this.opts;
@@ -225,7 +245,18 @@ export class TwinRenderer {
     if (!this.shadeChanged && this.solvedVersion >= 0 &&
         this.posesVersion - this.solvedVersion < Math.max(1, o.radiosityEvery || 1)) return;
     const fresh = this.solvedVersion < 0 || this.shadeChanged;
-    native.radiosityUpdate(this.ctx, true, fresh ? o.itersInitial : o.itersPerFrame);
+    const moved = fresh ? null : this._movedSinceSolve();
+    if (moved === null) {
+      native.radiosityUpdate(this.ctx, true, fresh ? o.itersInitial : o.itersPerFrame);
+      this.stats.lastRadiosity = 'full';
+    } else if (moved) {
+      const ran = native.radiosityUpdateMoved(this.ctx, moved, o.itersPerFrame);
+      this.stats.lastRadiosity = ran === 'none' ? 'reused' : ran;
+    } else this.stats.lastRadiosity = 'reused';
+    if (moved === null || moved) {
+      this._solvedInst = (this.instances || []).map((I) => (I ? { q0: [...I.q0], q1: [...I.q1], p0: [...I.p0],
+        p1: [...I.p1] } : null));
+    }
     this.solvedVersion = this.posesVersion;
     this.shadeChanged = false;
   }

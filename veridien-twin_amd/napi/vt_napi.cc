@@ -900,6 +900,25 @@ napi_value RadiosityUpdate(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// radiosityUpdateMoved(ctx, movedMask, iters) -> 'none' | 'incremental' | 'full' (enqueue)
+napi_value RadiosityUpdateMoved(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  uint32_t moved = 0, iters = 0, ran = 0;
+  if (argc > 1) napi_get_value_uint32(env, argv[1], &moved);
+  if (argc > 2) napi_get_value_uint32(env, argv[2], &iters);
+  int rc = vt_radiosity_update_moved(c, moved, iters, &ran);
+  if (rc) return throw_vt(env, c, rc, "vt_radiosity_update_moved");
+  static const char *kinds[] = {"none", "incremental", "full"};
+  napi_value out;
+  CHECK_NAPI(napi_create_string_utf8(env, kinds[ran < 3 ? ran : 0], NAPI_AUTO_LENGTH, &out));
+  return out;
+}
+
 // radiosityState(ctx, nodes) -> {eAvg: [3], state: Float32Array(nodes * 16)}; blocks until the queue is done
 napi_value RadiosityState(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -949,6 +968,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
              {"radiosityBuild", RadiosityBuild}, {"interiorBvh", InteriorBvh}, {"radiosityUpdate", RadiosityUpdate},
+             {"radiosityUpdateMoved", RadiosityUpdateMoved},
              {"radiosityState", RadiosityState}};
   for (auto &f : fns) {
     napi_value v;

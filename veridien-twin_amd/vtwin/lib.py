@@ -19,7 +19,7 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
-    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_state", "vt_node_neighbours",
+    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_radiosity_state", "vt_node_neighbours",
     "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide", "vt_sensor_raw_device",
 ]
 
@@ -184,6 +184,7 @@ def load():
         "vt_radiosity_build": (i32, [vp, vp, vp]),
         "vt_interior_bvh": (i32, [vp, u32, vp]),
         "vt_radiosity_update": (i32, [vp, u32, u32]),
+        "vt_radiosity_update_moved": (i32, [vp, u32, u32, C.POINTER(u32)]),
         "vt_radiosity_state": (i32, [vp, vp, sz, vp]),
         "vt_node_neighbours": (i32, [vp, vp, u32, u32, f32, f32, vp, vp]),
         "vt_radiosity_tables": (i32, [vp, u32, vp, sz]),
@@ -350,6 +351,15 @@ class Context:
 
     def radiosity_update(self, direct=True, iters=0):
         self.check(self.L.vt_radiosity_update(self.h, 1 if direct else 0, iters))
+
+    DIRECT_KINDS = ("none", "incremental", "full")
+
+    def radiosity_update_moved(self, moved, iters=0):
+        """The incremental update (_ensureRadiosity with a moved-body mask); returns the direct pass it ran:
+        'none', 'incremental' or 'full'."""
+        ran = C.c_uint32(0)
+        self.check(self.L.vt_radiosity_update_moved(self.h, int(moved) & 0xffffffff, iters, C.byref(ran)))
+        return self.DIRECT_KINDS[ran.value]
 
     def capsule_test(self, cases):
         cases = np.ascontiguousarray(cases, np.float32).reshape(-1, 16)

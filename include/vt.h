@@ -67,11 +67,18 @@ typedef struct {
   float emit_area;       /* total emitter area (m^2), SceneU.L0.w (gpu/renderer.js:646) */
   /* posed bodies (the reference's BLAS, gpu/renderer.js:538-571): tri_body[k] = the body triangle k belongs to
    * (0xffffffff: the static scene); a body's vertices are in its own frame and vt_set_instances places it. With
-   * bodies, nodes must be NULL (every BVH is built here); every body and the static scene need triangles, and
-   * emitter triangles stay static. tri_body may be NULL when body_count is 0. */
+   * bodies, nodes must be NULL (every BVH is built here); every body and the static scene need triangles. Emitter
+   * triangles lie in the static scene, or all in the body that carries the light (emit_posed). tri_body may be
+   * NULL when body_count is 0. */
   const uint32_t *tri_body; uint32_t body_count;
   uint32_t shadow_skip;  /* bit i: shadow rays ignore body i (a light's own housing; SceneU skip mask,
                             gpu/renderer.js:649) */
+  /* emit_posed 1: the light rides on body emit_body, as the reference's outlet rides on the scope, whose pose
+   * places every emitter sample at its exposure time (shade.wgsl.js:31-47, 55; SceneU.info.y,
+   * gpu/renderer.js:652-653). The emit records, the light axes of vt_set_light_profile and the emitter triangles
+   * are then in that body's frame, and every emitter triangle must belong to that body. 0: the light is static
+   * scene geometry. */
+  uint32_t emit_posed, emit_body;
 } vt_scene;
 
 #define VT_MAX_BODIES 24
@@ -255,8 +262,8 @@ int vt_update_materials(vt_ctx *ctx, const float *materials, uint32_t material_c
  * L_e(w) = radiance * p(theta), theta the angle between the outgoing direction w and the light's axis, p linear in
  * theta between `count` knots (theta ascending, radians; p >= 0), 0 past the last knot; the emitter still emits on
  * its front side only. count = 0 makes the light Lambertian again. Profiles survive vt_update_lights and are
- * cleared by vt_upload_scene. The reference has one such light, posed with the scope; here the light's triangles
- * are static scene geometry. */
+ * cleared by vt_upload_scene. The axis is in the emitter's frame: the world for a static light,
+ * its body's frame for a light that rides on a body (vt_scene.emit_posed). */
 #define VT_MAX_KNOTS 64
 int vt_set_light_profile(vt_ctx *ctx, uint32_t light, const float axis[3], const float *theta, const float *p,
                          uint32_t count);
@@ -420,8 +427,9 @@ int vt_radiosity_update(vt_ctx *ctx, uint32_t direct, uint32_t iters);
  * body whose pose changed since the last direct pass (bodies < 32). The direct pass then relights only the active
  * nodes whose segment to the emitter's centre passes within (bounding-capsule radius + emitter radius + 1 mm) of a
  * moved body's capsule, now or at the last direct pass; every other node keeps its B_dir, visibility code and
- * capsule visibility. A full pass (every node) runs instead when no direct pass ran since vt_radiosity_build or a
- * material, light or profile change made the cache stale; moved = 0 runs no direct pass. Then `iters` Jacobi
+ * capsule visibility. A full pass (every node) runs instead when the light's own body moved (vt_scene.emit_posed),
+ * no direct pass ran since vt_radiosity_build, or a material, light or profile change made the cache stale;
+ * moved = 0 runs no direct pass. Then `iters` Jacobi
  * steps as vt_radiosity_update. ran (optional): 0 no direct pass, 1 incremental, 2 full. Enqueue only. */
 enum { VT_DIRECT_NONE = 0, VT_DIRECT_INCREMENTAL = 1, VT_DIRECT_FULL = 2 };
 int vt_radiosity_update_moved(vt_ctx *ctx, uint32_t moved, uint32_t iters, uint32_t *ran);

@@ -148,7 +148,8 @@ class Scene(C.Structure):
                 ("emit_area", C.c_float), ("bodies", C.c_void_p), ("body_count", C.c_uint32),
                 ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p),
                 ("node_state", C.c_void_p), ("alb", C.c_void_p), ("hh", C.c_void_p), ("slot", C.c_uint32),
-                ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4), ("sample_materials", C.c_void_p)]
+                ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4), ("sample_materials", C.c_void_p),
+                ("emit_posed", C.c_uint32), ("emit_body", C.c_uint32)]
 
 
 class Body(C.Structure):
@@ -262,6 +263,9 @@ def scene_struct(sc):
               ptr(keep["materials"]), keep["materials"].size // 16, ptr(keep["lights"]), keep["lights"].size // 4,
               ptr(keep["emit"]), keep["emit"].size // 16, float(sc["emit_area"]),
               C.cast(bodies, C.c_void_p) if nb else None, nb, int(sc.get("shadow_skip", 0)) if nb else 0)
+    if nb and sc.get("emit_body") is not None:
+        # the light rides on this body: emit records and light axes in its frame
+        s.emit_posed, s.emit_body = 1, int(sc["emit_body"])
     if sc.get("profiles"):
         # {light: (axis, theta[], p[])}: the upload layout of vt_set_light_profile
         nl = keep["lights"].size // 4

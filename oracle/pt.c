@@ -89,6 +89,7 @@ void or_camera_ray(const or_camera *C, uint32_t x, uint32_t y, uint32_t i, float
 
 typedef struct {
   double p[3], ng[3], ns[3], b[3];
+  double nsi[3];    /* the interpolated vertex normal in the world, before any flip (an emitter's outward normal) */
   uint32_t vi[3], mesh, mat, flags, light;
   int ns_from_ng;   /* the shading normal fell back to the geometric one (grazing view of the interpolated normal) */
 } surf_t;
@@ -184,6 +185,7 @@ static void surface(const or_scene *S, const or_hit *h, const pose_t *P, const d
     rot_apply(P->R, s->ns, a);
     memcpy(s->ns, a, sizeof a);
   }
+  memcpy(s->nsi, s->ns, sizeof s->nsi);
   if (dot3(s->ng, d) > 0) for (int k = 0; k < 3; k++) s->ng[k] = -s->ng[k];
   if (dot3(s->ns, s->ng) < 0) for (int k = 0; k < 3; k++) s->ns[k] = -s->ns[k];
   double md[3] = {-d[0], -d[1], -d[2]};
@@ -210,13 +212,38 @@ static double light_profile(const or_scene *S, uint32_t light, const double w[3]
   return or_profile(th, kt, kp, (int)(cnt < 64 ? cnt : 64));
 }
 
-/* outward emitter normal at a hit: the interpolated vertex normal, unflipped (shade.wgsl.js:217-220) */
+/* outward emitter normal at a hit: the interpolated vertex normal in the world, unflipped (shade.wgsl.js:217-220) */
 static void emitter_normal(const or_scene *S, const surf_t *s, double n[3]) {
-  for (int k = 0; k < 3; k++) {
-    n[k] = 0;
-    for (int j = 0; j < 3; j++) n[k] += s->b[j] * S->verts[8 * (size_t)s->vi[j] + 4 + k];
-  }
-  norm3(n);
+  (void)S;
+  memcpy(n, s->nsi, 3 * sizeof(double));
+}
+
+/* The emitter's frame at exposure time t: the pose of the body the light rides on (the reference's outlet on the
+ * scope, shade.wgsl.js:45, 55), 0 (and no pose) for a light in the static scene. */
+static int emit_pose(const or_scene *S, double t, pose_t *P) {
+  if (!S->emit_posed) return 0;
+  body_pose(&S->bodies[S->emit_body], t, P);
+  return 1;
+}
+/* a world direction in the emitter's frame (R^T w), where the light axes are */
+static void emit_local(const or_scene *S, double t, const double w[3], double out[3]) {
+  pose_t P;
+  if (!emit_pose(S, t, &P)) { memcpy(out, w, 3 * sizeof(double)); return; }
+  rot_apply_t(P.R, w, out);
+}
+/* light_profile toward the world direction w at exposure time t */
+static double light_profile_at(const or_scene *S, uint32_t light, const double w[3], double t) {
+  double wl[3];
+  emit_local(S, t, w, wl);
+  return light_profile(S, light, wl);
+}
+/* the emitter centre (emit_c, emitter frame) in the world at exposure time t */
+static void emit_centre(const or_scene *S, double t, double c[3]) {
+  pose_t P;
+  const double cl[3] = {S->emit_c[0], S->emit_c[1], S->emit_c[2]};
+  if (!emit_pose(S, t, &P)) { memcpy(c, cl, sizeof cl); return; }
+  rot_apply(P.R, cl, c);
+  for (int k = 0; k < 3; k++) c[k] += P.p[k];
 }
 
 static void offset_ray(const double p[3], const double ng[3], const double dir[3], double out[3]) {
@@ -224,8 +251,9 @@ static void offset_ray(const double p[3], const double ng[3], const double dir[3
   for (int k = 0; k < 3; k++) out[k] = p[k] + ng[k] * (s * 2e-6);
 }
 
-/* shade.wgsl.js:31-47: area-CDF pick, uniform point; returns the emitter record */
-static const float *sample_emitter(const or_scene *S, const double u[3], double p[3]) {
+/* shade.wgsl.js:31-47: area-CDF pick, uniform point, posed at exposure time t with the light's body; p and the
+ * emitter normal n in the world; returns the emitter record */
+static const float *sample_emitter(const or_scene *S, const double u[3], double t, double p[3], double n[3]) {
   uint32_t lo = 0, hi = S->emit_count - 1;
   for (int it = 0; it < 40; it++) {
     if (lo >= hi) break;
@@ -234,7 +262,18 @@ static const float *sample_emitter(const or_scene *S, const double u[3], double 
   }
   const float *e = S->emit + 16 * (size_t)lo;
   double su = sqrt(u[1]);
-  for (int k = 0; k < 3; k++) p[k] = (double)e[k] + (double)e[4 + k] * (su * (1 - u[2])) + (double)e[8 + k] * (su * u[2]);
+  for (int k = 0; k < 3; k++) {
+    p[k] = (double)e[k] + (double)e[4 + k] * (su * (1 - u[2])) + (double)e[8 + k] * (su * u[2]);
+    n[k] = e[12 + k];
+  }
+  pose_t P;
+  if (emit_pose(S, t, &P)) {
+    double a[3];
+    rot_apply(P.R, p, a);
+    for (int k = 0; k < 3; k++) p[k] = a[k] + P.p[k];
+    rot_apply(P.R, n, a);
+    memcpy(n, a, sizeof a);
+  }
   return e;
 }
 
@@ -392,7 +431,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       double cosE = dot3(md, ne);
       if (cosE > 0) {
         const float *Le = S->lights + 4 * (size_t)s.light;
-        const double pe = light_profile(S, s.light, md);
+        const double pe = light_profile_at(S, s.light, md, time);
         double w = 1;
         if (b > 0) {
           double pL = h.t * h.t / (cosE * A);
@@ -418,12 +457,11 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       } else {
         u[0] = rnd(P, i, b, 3 * j); u[1] = rnd(P, i, b, 3 * j + 1); u[2] = rnd(P, i, b, 3 * j + 2);
       }
-      double pl[3];
-      const float *e = sample_emitter(S, u, pl);
+      double pl[3], en[3];
+      const float *e = sample_emitter(S, u, time, pl, en);
       double wi[3] = {pl[0] - s.p[0], pl[1] - s.p[1], pl[2] - s.p[2]};
       double d2 = dot3(wi, wi), dist = sqrt(d2);
       for (int k = 0; k < 3; k++) wi[k] /= dist;
-      double en[3] = {e[12], e[13], e[14]};
       double cosE = -dot3(wi, en);
       if (cosE <= 0) continue;
       double cosS = dot3(s.ns, wi);
@@ -432,7 +470,7 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       memcpy(&lid, e + 11, 4);
       const float *L0 = S->lights + 4 * (size_t)lid;
       const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
-      const double pe = light_profile(S, lid, mwi);
+      const double pe = light_profile_at(S, lid, mwi, time);
       const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
       if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) continue;
       double f[3];
@@ -571,11 +609,13 @@ static int seg_near(const double o[3], const double d[3], const double a[3], con
   return dot3(q, q) <= r * r;
 }
 
-static double cached_visibility(const or_scene *S, const surf_t *s) {
+static double cached_visibility(const or_scene *S, const surf_t *s, double time) {
   const double a = S->node_state[16 * (size_t)s->vi[0] + 15], b = S->node_state[16 * (size_t)s->vi[1] + 15],
                c = S->node_state[16 * (size_t)s->vi[2] + 15];
   if (!(a == b && b == c && (a == 1.0 || a == 0.0))) return -1;
-  const double d[3] = {S->emit_c[0] - s->p[0], S->emit_c[1] - s->p[1], S->emit_c[2] - s->p[2]};
+  double ec[3];
+  emit_centre(S, time, ec);
+  const double d[3] = {ec[0] - s->p[0], ec[1] - s->p[1], ec[2] - s->p[2]};
   for (uint32_t i = 0; i < S->body_count; i++) {
     const or_body *B = &S->bodies[i];
     if (B->hidden || ((S->shadow_skip >> i) & 1u)) continue;
@@ -602,13 +642,12 @@ static int is_static(const or_hit *h, const surf_t *s) { return (s->flags & F_ST
 /* one emitter sample's unshadowed contribution c and its shadow segment so + t wi, t < tmax; 0 when the sample
  * cannot light the point (lightEval, shade.wgsl.js:135-157) */
 static int light_eval(const or_scene *S, const surf_t *s, const or_mat *M, const double wo[3], const double u[3],
-                      int mis, double nl, double c[3], double so[3], double wi[3], double *tmax) {
-  double pl[3];
-  const float *e = sample_emitter(S, u, pl);
+                      double time, int mis, double nl, double c[3], double so[3], double wi[3], double *tmax) {
+  double pl[3], en[3];
+  const float *e = sample_emitter(S, u, time, pl, en);
   for (int k = 0; k < 3; k++) wi[k] = pl[k] - s->p[k];
   const double d2 = dot3(wi, wi), dist = sqrt(d2);
   for (int k = 0; k < 3; k++) wi[k] /= dist;
-  const double en[3] = {e[12], e[13], e[14]};
   const double cosE = -dot3(wi, en);
   if (cosE <= 0) return 0;
   const double cosS = dot3(s->ns, wi);
@@ -616,7 +655,7 @@ static int light_eval(const or_scene *S, const surf_t *s, const or_mat *M, const
   uint32_t lid;
   memcpy(&lid, e + 11, 4);
   const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
-  const double pe = light_profile(S, lid, mwi);
+  const double pe = light_profile_at(S, lid, mwi, time);
   const float *L0 = S->lights + 4 * (size_t)lid;
   const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
   if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) return 0;
@@ -667,7 +706,7 @@ static void still_light(const or_scene *S, const surf_t *s, const or_mat *M, con
     const uint32_t li = i * nl + j;
     const double u[3] = {or_kr(li, KL0, P->s[3]), or_kr(li, KL1, P->s[4]), or_kr(li, KL2, P->s[5])};
     double c[3], so[3], wi[3], tmax;
-    if (!light_eval(S, s, M, wo, u, 0, nl, c, so, wi, &tmax)) continue;
+    if (!light_eval(S, s, M, wo, u, time, 0, nl, c, so, wi, &tmax)) continue;
     nlit++;
     (*rays)++;
     or_hit sh;
@@ -691,12 +730,11 @@ static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, co
                          const double u[3], int mis, double nl, double cached, uint64_t *rays, uint32_t *key,
                          double out[3]) {
   out[0] = out[1] = out[2] = 0;
-  double pl[3];
-  const float *e = sample_emitter(S, u, pl);
+  double pl[3], en[3];
+  const float *e = sample_emitter(S, u, time, pl, en);
   double wi[3] = {pl[0] - s->p[0], pl[1] - s->p[1], pl[2] - s->p[2]};
   const double d2 = dot3(wi, wi), dist = sqrt(d2);
   for (int k = 0; k < 3; k++) wi[k] /= dist;
-  const double en[3] = {e[12], e[13], e[14]};
   const double cosE = -dot3(wi, en);
   if (cosE <= 0) return;
   const double cosS = dot3(s->ns, wi);
@@ -704,7 +742,7 @@ static void direct_light(const or_scene *S, const surf_t *s, const or_mat *M, co
   uint32_t lid;
   memcpy(&lid, e + 11, 4);
   const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
-  const double pe = light_profile(S, lid, mwi);
+  const double pe = light_profile_at(S, lid, mwi, time);
   const float *L0 = S->lights + 4 * (size_t)lid;
   const double Le[3] = {L0[0] * pe, L0[1] * pe, L0[2] * pe};
   if (fmax(Le[0], fmax(Le[1], Le[2])) <= 0) return;
@@ -747,7 +785,7 @@ static void secondary(const or_scene *S, const double o[3], const double l[3], d
     const double ml[3] = {-l[0], -l[1], -l[2]};
     const double cosE = dot3(ml, ne);
     if (cosE <= 0) return;
-    const double pe = light_profile(S, s.light, ml);
+    const double pe = light_profile_at(S, s.light, ml, time);
     const double pL = h.t * h.t / (cosE * S->emit_area);
     const double w = pdfB * pdfB / (pdfB * pdfB + nl * nl * pL * pL);
     for (int k = 0; k < 3; k++) out[k] = S->lights[4 * (size_t)s.light + k] * pe * w;
@@ -795,7 +833,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
     double ne[3];
     emitter_normal(S, &s, ne);
     if (dot3(wo, ne) <= 0) return;
-    const double pe = light_profile(S, s.light, wo);
+    const double pe = light_profile_at(S, s.light, wo, time);
     for (int k = 0; k < 3; k++) out[k] = S->lights[4 * (size_t)s.light + k] * pe;
     return;
   }
@@ -804,7 +842,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
   const int stat = is_static(&h, &s);
   const int mis = !stat && !(opts & O_NO_MIS);
   const uint32_t nl = nl_in < 1 ? 1 : nl_in > 16 ? 16 : nl_in;
-  const double cached = (stat && (opts & O_CULL_SHADOWS) && !(opts & O_SHADOW_CACHE)) ? cached_visibility(S, &s) : -1;
+  const double cached = (stat && (opts & O_CULL_SHADOWS) && !(opts & O_SHADOW_CACHE)) ? cached_visibility(S, &s, time) : -1;
   double L[3] = {0, 0, 0};
   if (S->emit_count > 0 && stat && (opts & O_SHADOW_CACHE))
     still_light(S, &s, &M, wo, time, P, i, nl, rays, key, L);
@@ -920,18 +958,17 @@ void or_node_direct(const or_scene *S, const float *npos, const float *nnorm, co
     const double o[3] = {p[0] + n[0] * 2e-6, p[1] + n[1] * 2e-6, p[2] + n[2] * 2e-6};
     for (uint32_t s = 0; s < samples && S->emit_count > 0; s++) {
       const double u[3] = {or_kr(s, KL0, s0), or_kr(s, KL1, s1), or_kr(s, KL2, s2)};
-      double pl[3];
-      const float *e = sample_emitter(S, u, pl);
+      double pl[3], en[3];
+      const float *e = sample_emitter(S, u, 0.5, pl, en);
       double wi[3] = {pl[0] - p[0], pl[1] - p[1], pl[2] - p[2]};
       const double d2 = dot3(wi, wi), dist = sqrt(d2);
       for (int q = 0; q < 3; q++) wi[q] /= dist;
-      const double en[3] = {e[12], e[13], e[14]};
-      const double cosE = -dot3(wi, en), cosN = dot3(n, wi);
+          const double cosE = -dot3(wi, en), cosN = dot3(n, wi);
       if (cosE <= 0 || cosN <= 0) continue;
       uint32_t lid;
       memcpy(&lid, e + 11, 4);
       const double mwi[3] = {-wi[0], -wi[1], -wi[2]};
-      const double pe = light_profile(S, lid, mwi);
+      const double pe = light_profile_at(S, lid, mwi, 0.5);
       const float *L0 = S->lights + 4 * (size_t)lid;
       if (fmax(L0[0], fmax(L0[1], L0[2])) * pe <= 0) continue;
       nlit++;

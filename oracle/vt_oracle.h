@@ -177,6 +177,9 @@ typedef struct {
    * colours in `materials` with this table held at the unperturbed colours is the derivative the adjoint
    * computes (sampling detached). */
   const float *sample_materials;
+  /* emit_posed 1: the light rides on body emit_body; the emit records, the light axes and emit_c are in its frame
+   * and every emitter sample is posed with it at its exposure time (shade.wgsl.js:31-47, 55) */
+  uint32_t emit_posed, emit_body;
 } or_scene;
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose

@@ -291,3 +291,176 @@ def test_gpu_visible_matches_oracle(ctx, parts, skip):
         ref = oracle.visible(sc, cam, pts, tol=2e-4, time=t)
         print("visible t=%.1f skip %d: %d visible, %d disagree" % (t, skip, ref.sum(), np.sum(got != ref)))
         assert np.sum(got != ref) <= 4
+
+
+# ---------------------------------------------------------------------------- a light that rides on a body
+# The reference's light outlet rides on the scope: its emitter records and emission axis are in the scope's frame
+# and every sample is posed with the scope at its exposure time (shade.wgsl.js:31-47, 55). Here body 0 carries a
+# light with an angular profile about its +x axis (scenes.posed_light_parts).
+
+@pytest.fixture(scope="module")
+def light_parts():
+    return scenes.posed_light_parts()
+
+
+def _light_posed(light_parts, poses):
+    sc = scenes.posed_scene(*light_parts, emit_body=0)
+    sc["poses"] = poses
+    sc["profiles"] = {0: scenes.BODY_LIGHT_PROFILE}
+    return sc
+
+
+def _light_placed(light_parts, t, moving=True):
+    """The scene with both bodies, the light included, moved into the world at exposure time t, and the light's
+    axis turned with body 0 by hand."""
+    ang = A0 + (A1 - A0) * t if moving else A0
+    pos = np.asarray(P0) + (np.asarray(P1) - np.asarray(P0)) * t if moving else np.asarray(P0)
+    R0 = scenes.rot_axis(Z, ang)
+    place = {0: (R0, pos), 1: (scenes.rot_axis(B1[0], B1[1]), np.asarray(B1[2]))}
+    sc = scenes.posed_scene(*light_parts, place=place)
+    axis, th, p = scenes.BODY_LIGHT_PROFILE
+    sc["profiles"] = {0: (tuple(R0 @ np.asarray(axis)), th, p)}
+    return sc
+
+
+@pytest.mark.parametrize("t", [0.0, 0.6, 1.0])
+def test_oracle_light_on_a_body_is_the_placed_light(light_parts, t):
+    """shutter [t, t]: the light's samples, normals and profile axis posed with body 0 at time t render as the
+    light placed in the world by hand at that time (static emitter records, rotated axis)."""
+    cam = scenes.posed_camera(48, 36, spp=4, shutter=(t, t))
+    a, _, _ = oracle.render_tile(_light_posed(light_parts, _poses(True)), cam, 0, 0, 48, 36)
+    b, _, _ = oracle.render_tile(_light_placed(light_parts, t), cam, 0, 0, 48, 36)
+    rel, close = _compare(a, b, 1e-6)
+    print("t=%.2f light on a body vs placed: rel %.3g, close %.4f, mean %.4g" % (t, rel, close, b[..., :3].mean()))
+    assert b[..., :3].mean() > 1e-3 and rel < 1e-6 and close > 0.999
+
+
+def test_oracle_light_on_a_body_moves_the_lighting(light_parts):
+    """The light turns with body 0 over the exposure: the images at the two ends differ, and the whole exposure
+    averages them (block means between the ends)."""
+    cam = scenes.posed_camera(32, 24, spp=16)
+    sc = _light_posed(light_parts, _poses(True))
+    blur, _, _ = oracle.render_tile(sc, cam, 0, 0, 32, 24)
+    ends = [oracle.render_tile(sc, dict(cam, shutter=(t, t)), 0, 0, 32, 24)[0] for t in (0.0, 0.5, 1.0)]
+    blk = lambda im: im[:, :, 0:3].astype(np.float64).reshape(6, 4, 8, 4, 3).mean(axis=(1, 3))
+    lo = np.minimum.reduce([blk(e) for e in ends])
+    hi = np.maximum.reduce([blk(e) for e in ends])
+    assert np.linalg.norm(blk(ends[0]) - blk(ends[2])) > 0.1 * np.linalg.norm(blk(ends[0]))
+    assert np.mean((blk(blur) >= lo - 0.05 * hi - 1e-3) & (blk(blur) <= hi * 1.05 + 1e-3)) > 0.97
+
+
+def test_oracle_node_direct_with_a_light_on_a_body(light_parts):
+    """nodeDirect samples the light at mid-exposure: the static nodes' B_dir under the posed light equals that
+    under the light placed at t = 0.5."""
+    sc = _light_posed(light_parts, _poses(True))
+    sp = _light_placed(light_parts, 0.5)
+    ni, nj = oracle.node_inputs(sc), oracle.node_inputs(sp)
+    act = np.nonzero(ni["static"])[0][::7].astype(np.uint32)
+    assert np.array_equal(ni["npos"][act], nj["npos"][act])
+    alb, _ = oracle.albedo_tables(sc)
+    st_a = np.zeros((ni["npos"].shape[0], 16), np.float32)
+    st_b = np.zeros_like(st_a)
+    oracle.node_direct(sc, ni, act, 4, alb, st_a)
+    oracle.node_direct(sp, nj, act, 4, alb, st_b)
+    B, Bo = st_a[act, 12:15].astype(np.float64), st_b[act, 12:15].astype(np.float64)
+    print("node B_dir, light on a body vs placed: rel %.3g" % (np.linalg.norm(B - Bo) / np.linalg.norm(Bo)))
+    assert np.linalg.norm(Bo) > 0 and np.linalg.norm(B - Bo) <= 1e-5 * np.linalg.norm(Bo)
+    assert np.mean(st_a[act, 15] == st_b[act, 15]) > 0.999
+
+
+def _gpu_light(ctx, sc, poses):
+    ctx.upload_scene(sc)
+    ctx.set_instances(poses)
+    axis, th, p = sc["profiles"][0]
+    ctx.set_light_profile(0, axis, th, p)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["persistent", "wavefront"])
+@pytest.mark.parametrize("shutter", [(0.0, 1.0), (0.6, 0.6)])
+def test_gpu_light_on_a_body_matches_oracle(ctx, light_parts, mode, shutter):
+    from vtwin import lib
+    poses = _poses(True)
+    sc = _light_posed(light_parts, poses)
+    cam = scenes.posed_camera(96, 72, spp=8, shutter=shutter)
+    print("light on a body: mode %s, shutter %s" % (mode, shutter), flush=True)
+    _gpu_light(ctx, sc, poses)
+    ctx.set_camera(0, cam)
+    ctx.render([0], mode=lib.VT_RENDER_WAVEFRONT if mode == "wavefront" else lib.VT_RENDER_BATCHED)
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+    ref, _, _ = oracle.render_tile(sc, cam, 0, 0, 96, 72)
+    rel, close = _compare(img, ref, 1e-3)
+    print("GPU vs oracle: relative RMSE %.3g, close %.4f" % (rel, close))
+    assert rel < 1e-3 and close > 0.99
+
+
+@pytest.mark.gpu
+def test_gpu_light_on_a_body_reference_estimator(ctx, light_parts):
+    """The node cache and the reference estimator under a light on a moving body: node direct light against the
+    oracle; the estimator (shadow culling toward the light's posed centre, fixed-camera caches) against the oracle
+    fed the GPU's cache; a new pose of the light's body clears the occlusion cache (the next frame equals an
+    uncached one) and makes the incremental node update a full one, while another body's move stays incremental."""
+    from vtwin import lib
+    poses = _poses(True)
+    sc = _light_posed(light_parts, poses)
+    _gpu_light(ctx, sc, poses)
+    info = ctx.radiosity_build(64, 4, 16, 0.006, 0.003)
+    V = np.asarray(sc["verts"]).size // 8
+    act = ctx.radiosity_tables(2, info["active"], np.uint32)
+    alb = ctx.radiosity_tables(5, (17, 32, 8), np.float32)
+    hh = ctx.radiosity_tables(6, (17, 4), np.float32)
+    assert ctx.radiosity_update_moved(0b01, iters=16) == "full"
+    st, e = ctx.radiosity_state(V)
+    ni = oracle.node_inputs(sc)
+    so = np.zeros_like(st)
+    oracle.node_direct(sc, ni, act, 4, alb, so)
+    B, Bo = st[act, 12:15].astype(np.float64), so[act, 12:15].astype(np.float64)
+    print("node B_dir under the light on a body: rel %.3g, codes equal %.5f" % (
+        np.linalg.norm(B - Bo) / np.linalg.norm(Bo), np.mean(st[act, 15] == so[act, 15])))
+    assert np.linalg.norm(B - Bo) <= 1e-3 * np.linalg.norm(Bo) and np.mean(st[act, 15] == so[act, 15]) > 0.995
+    R = lib.VT_OPT_REFERENCE
+    cam = scenes.posed_camera(96, 72, spp=4)
+    sc2 = dict(sc, node_state=st, alb=alb, hh=hh, slot=1, e_avg=e)
+    for opts in (R | lib.VT_OPT_CULL_SHADOWS, R | lib.VT_OPT_HIT_CACHE | lib.VT_OPT_SHADOW_CACHE):
+        c = dict(cam, flags=opts)
+        ctx.set_camera(0, c)
+        ctx.render([0])
+        img = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+        ref, _, _ = oracle.render_tile(sc2, c, 0, 0, 96, 72)
+        rel, close = _compare(img, ref, 1e-3)
+        print("reference estimator, options 0x%x: relative RMSE %.3g, close %.4f" % (opts, rel, close))
+        assert rel < 1e-3 and close > 0.99
+    # camera 0 keeps its caches: the next frame reads them; then the light's body turns further, so the kept
+    # occlusion answers are stale and the frame must equal one from empty caches (a camera slot of its own)
+    first = img
+    ctx.render([0])
+    assert np.array_equal(ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72), first)
+    moved = [dict(p) for p in poses]
+    moved[0]["q0"] = scenes.quat_axis(Z, A0 + 0.4)
+    moved[0]["q1"] = scenes.quat_axis(Z, A1 + 0.4)
+    ctx.set_instances(moved)
+    ctx.render([0])
+    cached = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+    ctx.set_camera(1, dict(cam, flags=opts))
+    ctx.render([1])
+    fresh = ctx.readback(1, lib.VT_KIND_LINEAR, 96, 72)
+    assert np.array_equal(cached, fresh) and not np.array_equal(cached, first)
+    assert ctx.radiosity_update_moved(0b01, iters=0) == "full"
+    moved[1]["p0"] = tuple(np.asarray(B1[2]) + (0.1, 0.0, 0.0))
+    ctx.set_instances(moved)
+    assert ctx.radiosity_update_moved(0b10, iters=0) == "incremental"
+
+
+@pytest.mark.gpu
+def test_gpu_light_on_a_body_validates(ctx, light_parts):
+    """A light that rides on a body needs every emitter triangle in that body; a body light left static is
+    refused, as is a light named on a body past the count."""
+    from vtwin import lib
+    sc = scenes.posed_scene(*light_parts)                    # emitter triangles in body 0, light static
+    with pytest.raises(lib.VTError):
+        ctx.upload_scene(sc)
+    with pytest.raises(lib.VTError):
+        ctx.upload_scene(dict(sc, emit_body=1))              # the light's triangles lie in body 0
+    with pytest.raises(lib.VTError):
+        ctx.upload_scene(dict(sc, emit_body=5))
+    ctx.upload_scene(dict(sc, emit_body=0))

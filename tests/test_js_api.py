@@ -215,20 +215,23 @@ def test_js_quat_from_mat_matches_scipy():
 @needs_node
 def test_from_prepared_bodies():
     """fromPrepared with the reference's BLAS list: triangle ranges become triBody, the caller BVH is dropped (every
-    group's BVH is built at upload), and the bodies keep their keys and kinds."""
+    group's BVH is built at upload), the bodies keep their keys and kinds, and the light rides on the scope."""
     S = ("{nodes: new Float32Array(16), tris: new Float32Array(16 * 5), verts: new Float32Array(24), "
          "meshTable: [{mat: 0, flags: 1}], materials: [{metallic: 0, rho: [0.5, 0.4, 0.3], alpha: 0.25, roughness: 0.5}], "
          "blas: [{body: 'arm', kind: 'body', triOff: 1, triCount: 2, owner: 'CAM2', rhoEff: 0.3, occCapsule: {a: [0, 0, 0], "
-         "b: [0, 0, 0.1], r: 0.01}}, {body: 'scope', kind: 'fixed', triOff: 4, triCount: 1}], emitter: {tris: [], area: 0, L0: [1, 1, 1], shadowSkipsScope: true, axis: [0, 0, -1], "
+         "b: [0, 0, 0.1], r: 0.01}}, {body: 'scope', kind: 'fixed', triOff: 4, triCount: 1}], emitter: {tris: [{v0: [0, 0, 0.1], e1: [0.01, 0, 0], "
+         "e2: [0, 0.01, 0], n: [0, 0, 1], cdf: 1, area: 5e-5}], area: 5e-5, L0: [1, 1, 1], shadowSkipsScope: true, axis: [0, 0, -1], "
          "theta: [0, 0.5], p: [1, 0]}}")
     out = _node("import('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({tb: Array.from(u.triBody), "
-                "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip, profiles: u.profiles})); })" % (MODULE, S))
+                "nodes: u.nodes === undefined, bodies: u.bodies, shadowSkip: u.shadowSkip, profiles: u.profiles, "
+                "emitBody: u.emitBody})); })" % (MODULE, S))
     d = json.loads(out)
     assert d["tb"] == [0xFFFFFFFF, 0, 0, 0xFFFFFFFF, 1] and d["nodes"]
     assert d["bodies"] == [{"key": "arm", "kind": "body", "owner": "CAM2", "rhoEff": 0.3,
                             "occ": {"a": [0, 0, 0], "b": [0, 0, 0.1], "r": 0.01}},
                            {"key": "scope", "kind": "fixed", "owner": None, "rhoEff": 0, "occ": None}]
     assert d["shadowSkip"] == 2          # the scope's bit: shadow rays leave through its housing
+    assert d["emitBody"] == 1            # the outlet's records are in the scope's frame: the light rides on it
     assert d["profiles"] == [{"light": 0, "axis": [0, 0, -1], "theta": [0, 0.5], "p": [1, 0]}]
 
 

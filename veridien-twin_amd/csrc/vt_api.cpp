@@ -279,13 +279,18 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   if (s->body_count > VT_MAX_BODIES) return fail(c, VT_ERR_ARG, "scene: more than %d bodies", VT_MAX_BODIES);
   if (s->body_count && (!s->tri_body || s->nodes))
     return fail(c, VT_ERR_ARG, "scene: bodies need tri_body and no caller BVH (nodes = NULL)");
-  for (uint32_t k = 0; s->body_count && k < T; k++)
-    if (s->tri_body[k] != 0xffffffffu) {
-      if (s->tri_body[k] >= s->body_count) return fail(c, VT_ERR_ARG, "scene: triangle %u body out of range", k);
-      // emitters are sampled from the world-space emitter table, so they stay in the static scene
-      if (s->meshes[4 * (size_t)tw[16 * (size_t)k + 12] + 1] & VT_MESH_EMITTER)
-        return fail(c, VT_ERR_ARG, "scene: triangle %u of a body is an emitter", k);
-    }
+  if (s->emit_posed && s->emit_body >= s->body_count)
+    return fail(c, VT_ERR_ARG, "scene: the light rides on body %u of %u", s->emit_body, s->body_count);
+  // emitter samples come from the emit table in the emitter's frame, so the emitter triangles that rays hit must
+  // lie in that same frame: the static scene, or the light's body
+  const uint32_t emit_group = s->emit_posed ? s->emit_body : 0xffffffffu;
+  for (uint32_t k = 0; k < T; k++) {
+    const uint32_t g = s->body_count ? s->tri_body[k] : 0xffffffffu;
+    if (g != 0xffffffffu && g >= s->body_count) return fail(c, VT_ERR_ARG, "scene: triangle %u body out of range", k);
+    if ((s->meshes[4 * (size_t)tw[16 * (size_t)k + 12] + 1] & VT_MESH_EMITTER) && g != emit_group)
+      return fail(c, VT_ERR_ARG, "scene: emitter triangle %u lies outside the light's %s", k,
+                  s->emit_posed ? "body" : "static scene");
+  }
   // BVH: the caller's (reference layout) or built here; slot order is the tris order
   std::vector<float> nodes2;
   std::vector<float> tris_sorted;
@@ -451,6 +456,8 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->body_box = body_box;
   c->S.inst_count = s->body_count;
   c->S.shadow_skip = s->body_count ? s->shadow_skip : 0u;
+  c->S.emit_body = s->emit_posed ? s->emit_body : 0xffffffffu;
+  c->emit_rec_set = false;
   if (s->body_count) {
     // every body starts hidden until vt_set_instances gives it a pose
     std::vector<float4> rec((size_t)INST_WORDS * s->body_count, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -659,6 +666,14 @@ extern "C" int vt_set_instances(vt_ctx *c, const vt_instance *in, uint32_t count
     host_apply(Pm, ob, wob);
     r[8] = make_float4((float)woa[0], (float)woa[1], (float)woa[2], I.occ_r);
     r[9] = make_float4((float)wob[0], (float)wob[1], (float)wob[2], I.rho_eff);
+  }
+  // a light that rides on a body lights the static scene differently once that body's pose record changes: the
+  // fixed cameras' occlusion caches start over
+  if (c->S.emit_body != 0xffffffffu) {
+    const float4 *r = &rec[INST_WORDS * (size_t)c->S.emit_body];
+    if (!c->emit_rec_set || std::memcmp(c->emit_rec, r, sizeof c->emit_rec) != 0) c->shade_gen++;
+    std::memcpy(c->emit_rec, r, sizeof c->emit_rec);
+    c->emit_rec_set = true;
   }
   VT_HIP(c, hipSetDevice(c->device));
   VT_HIP(c, hipStreamSynchronize(c->stream));

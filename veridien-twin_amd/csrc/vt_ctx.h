@@ -29,10 +29,11 @@ struct CacheKey {
   uint32_t mode;
   float lens[9];
   uint32_t width, height, seed, cam_index, nl, nl_refine;
+  float shutter[2];     // the sub-samples' exposure times (where a light that rides on a body stands)
   uint64_t scene_gen, shade_gen;
   bool operator==(const CacheKey &o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
 };
-static_assert(sizeof(CacheKey) == 28 * 4 + 2 * 8, "CacheKey has no padding (compared bytewise)");
+static_assert(sizeof(CacheKey) == 30 * 4 + 2 * 8, "CacheKey has no padding (compared bytewise)");
 
 struct CamSlot {
   bool set = false;
@@ -69,6 +70,9 @@ struct vt_ctx {
   // scene
   bool has_scene = false;
   uint64_t scene_gen = 0, shade_gen = 0;   // bumped by a scene upload / a material, light or profile change
+                                           // (or a new pose record of the body the light rides on)
+  float4 emit_rec[4];                      // that body's pose words (q0, q1, p0, p1) at the last vt_set_instances
+  bool emit_rec_set = false;
   DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots, tvid;
   // host copies the node radiosity build reads: vertices (8 floats each), per slot vertex ids + mesh, meshes,
   // the static scene's slot count (its slots come first), the emitter records

@@ -330,6 +330,7 @@ struct SceneDev {
   const float4 *inst;       // posed bodies, INST_WORDS float4 each (layout at inst_pose below)
   uint32_t inst_count;
   uint32_t shadow_skip;     // bodies shadow rays ignore (bit per body)
+  uint32_t emit_body;       // the body whose frame holds the emitter records and light axes (NO_INST: the world)
   const float4 *lprof;      // per light: emission axis xyz, w = bits(knot count); knots at light * MAX_KNOTS
   const float2 *knots;      // (theta, p) knots of the angular emission profiles
   const uint4 *tvid;        // per slot: vertex ids v0 v1 v2, mesh
@@ -686,8 +687,30 @@ __device__ __forceinline__ bool trace_scene(const SceneDev &S, v3 o, v3 d, float
   return found;
 }
 
-// emitter sample (shade.wgsl.js:31-47); returns the record index
-__device__ __forceinline__ uint32_t sample_emitter(const SceneDev &S, float ux, float uy, float uz, v3 &p) {
+// The emitter's frame at exposure time t: the pose of the body it rides on (the reference's light rides on the
+// scope: shade.wgsl.js:45, 55, 120), or the world for an emitter in the static scene.
+struct EmitPose {
+  Pose P;
+  bool posed;
+};
+__device__ __forceinline__ EmitPose emitter_pose(const SceneDev &S, float t) {
+  EmitPose E;
+  E.posed = S.emit_body != NO_INST;
+  if (E.posed) E.P = inst_pose(S, S.emit_body, t);
+  return E;
+}
+// a world direction in the emitter's frame, where the light axes of the emission profiles are
+__device__ __forceinline__ v3 emit_local(const EmitPose &E, v3 w) { return E.posed ? rot_t(E.P, w) : w; }
+// the emitter's centre (radu[1].xyz, emitter frame) in the world at the pose E
+__device__ __forceinline__ v3 emit_centre(const SceneDev &S, const EmitPose &E) {
+  const v3 c = ld3(S.radu[1]);
+  return E.posed ? rot(E.P, c) + E.P.p : c;
+}
+
+// emitter sample (shade.wgsl.js:31-47): area-CDF pick and a uniform point on the triangle, posed by E; p and the
+// emitter's normal n in the world; returns the record index
+__device__ __forceinline__ uint32_t sample_emitter(const SceneDev &S, const EmitPose &E, float ux, float uy, float uz,
+                                                   v3 &p, v3 &n) {
   uint32_t lo = 0, hi = S.emit_count - 1;
   for (int it = 0; it < 40; it++) {
     if (lo >= hi) break;
@@ -697,6 +720,11 @@ __device__ __forceinline__ uint32_t sample_emitter(const SceneDev &S, float ux, 
   float4 v0 = S.emit[4 * lo], e1 = S.emit[4 * lo + 1], e2 = S.emit[4 * lo + 2];
   float su = sqrtf(uy);
   p = ld3(v0) + ld3(e1) * (su * (1.0f - uz)) + ld3(e2) * (su * uz);
+  n = ld3(S.emit[4 * lo + 3]);
+  if (E.posed) {
+    p = rot(E.P, p) + E.P.p;
+    n = rot(E.P, n);
+  }
   return lo;
 }
 

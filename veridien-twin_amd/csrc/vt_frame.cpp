@@ -72,6 +72,8 @@ int camera_caches(vt_ctx *c, CamSlot &s, DevCam &d) {
   if (k.flags & VT_OPT_SHADOW_CACHE) {
     key.nl = d.nl;
     key.nl_refine = d.nl_refine;
+    key.shutter[0] = k.shutter[0];
+    key.shutter[1] = k.shutter[1];
     key.shade_gen = c->shade_gen;
     if (!(key == s.sc_key) || layers > s.sc_layers) {
       const uint64_t n = (px * layers + 3) / 4;     // 32-bit words, cleared 16 bytes at a time

@@ -155,7 +155,7 @@ constexpr uint32_t NODE_GROUP = 4;
 // way sends no shadow ray near the moved bodies, and its samples come out as a full pass would compute them.
 __device__ __forceinline__ bool near_moved(const NodeLaunch &L, v3 p) {
   const float4 ec = L.S.radu[1];
-  const v3 dE = ld3(ec) - p;
+  const v3 dE = emit_centre(L.S, emitter_pose(L.S, 0.5f)) - p;
   const float rr = ec.w + 1e-3f;
   for (uint32_t b = 0; b < L.S.inst_count; b++) {
     if (!((L.moved >> b) & 1u)) continue;
@@ -185,17 +185,18 @@ __global__ __launch_bounds__(256) void k_node_direct(NodeLaunch L) {
     v3 B = mk(0.f, 0.f, 0.f);
     uint32_t nlit = 0, nvis = 0;
     const v3 o = p + n * EPS_OFF;
+    const EmitPose E = emitter_pose(L.S, 0.5f);      // mid-exposure, as the shadow rays
     for (uint32_t s = q; s < L.node_samples && L.S.emit_count > 0; s += NODE_GROUP) {
-      v3 pl;
-      const uint32_t e = sample_emitter(L.S, kr(s, KL0, s0), kr(s, KL1, s1), kr(s, KL2, s2), pl);
+      v3 pl, nle;
+      const uint32_t e = sample_emitter(L.S, E, kr(s, KL0, s0), kr(s, KL1, s1), kr(s, KL2, s2), pl, nle);
       v3 wi = pl - p;
       const float d2 = dot(wi, wi), dist = sqrtf(d2);
       wi = wi * (1.0f / dist);
-      const float cosE = -dot(wi, ld3(L.S.emit[4 * e + 3]));
+      const float cosE = -dot(wi, nle);
       const float cosN = dot(n, wi);
       if (cosE <= 0.0f || cosN <= 0.0f) continue;
       const uint32_t lid = __float_as_uint(L.S.emit[4 * e + 2].w);
-      const v3 Le = ld3(L.S.lights[lid]) * light_profile(L.S, lid, -wi);
+      const v3 Le = ld3(L.S.lights[lid]) * light_profile(L.S, lid, emit_local(E, -wi));
       if (maxc(Le) <= 0.0f) continue;
       nlit++;
       Hit sh;

@@ -272,7 +272,7 @@ extern "C" int vt_radiosity_build(vt_ctx *c, const vt_radiosity_opts *o, vt_radi
   if ((rc = upload(c, c->g_nbr, nb.empty() ? nullptr : nb.data(), std::max<size_t>(nb.size(), 1) * 4))) return rc;
   if ((rc = upload(c, c->g_nbrw, nw.empty() ? nullptr : nw.data(), std::max<size_t>(nw.size(), 1) * 4))) return rc;
   if ((rc = ensure(c, c->g_aux, std::max<size_t>(A, 1) * 16))) return rc;
-  // the emitters' centroid and radius for the shadow culling (gpu/scene_gpu.js:418-422)
+  // the emitters' centroid and radius in the emitter's frame, for the shadow culling (gpu/scene_gpu.js:418-422)
   double cen[3] = {0, 0, 0}, tot = 0, rad = 0;
   const size_t E = c->h_emit.size() / 16;
   for (size_t e = 0; e < E; e++) {
@@ -505,7 +505,9 @@ extern "C" int vt_radiosity_update_moved(vt_ctx *c, uint32_t moved, uint32_t ite
   VT_HIP(c, hipSetDevice(c->device));
   c->S.nstate = (const float4 *)c->g_state.p;
   uint32_t kind = VT_DIRECT_NONE;
-  if (c->rad_direct_stale || (moved && !c->rad_pcaps_valid)) kind = VT_DIRECT_FULL;
+  // the light's own body moved: every node sees the light from elsewhere (_movedSinceSolve's scope case)
+  const bool light_moved = c->S.emit_body < 32 && ((moved >> c->S.emit_body) & 1u);
+  if (c->rad_direct_stale || (moved && (!c->rad_pcaps_valid || light_moved))) kind = VT_DIRECT_FULL;
   else if (moved) kind = VT_DIRECT_INCREMENTAL;
   if (kind != VT_DIRECT_NONE)
     if (int rc = direct_pass(c, kind == VT_DIRECT_FULL ? 0u : moved)) return rc;

@@ -73,7 +73,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
     if (ms.y & F_EMITTER) {
       const float cosE = -dot(d, nsi);
       if (cosE > 0.0f) {
-        const float pe = light_profile(S, ms.z, -d);
+        const float pe = light_profile(S, ms.z, emit_local(emitter_pose(S, time), -d));
         const v3 Le = ld3(S.lights[ms.z]) * pe;
         float w = 1.0f;
         if (b > 0) {
@@ -103,17 +103,18 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
         float tmax;
         uint32_t lid;
         {
-          v3 pl;
-          const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+          v3 pl, nl;
+          const EmitPose E = emitter_pose(S, time);
+          const uint32_t e = sample_emitter(S, E, u0, u1, u2, pl, nl);
           wi = pl - p;
           const float d2 = dot(wi, wi), dist = sqrtf(d2);
           wi = wi * (1.0f / dist);
-          const float cosE = -dot(wi, ld3(S.emit[4 * e + 3]));
+          const float cosE = -dot(wi, nl);
           if (cosE <= 0.0f) continue;
           const float cosS = dot(ns, wi);
           if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
           lid = __float_as_uint(S.emit[4 * e + 2].w);
-          const float pe = light_profile(S, lid, -wi);
+          const float pe = light_profile(S, lid, emit_local(E, -wi));
           const v3 Le = ld3(S.lights[lid]) * pe;
           if (maxc(Le) <= 0.0f) continue;
           const Mat M = load_mat(S.mats, ms.x);
@@ -226,13 +227,13 @@ __device__ __forceinline__ bool is_static_hit(const Hit &h, const Surf &s) {
 // sample is visible or hidden and no visible body's bounding capsule comes near the segment to the emitters'
 // centroid (its radius + 0.5 mm added); -1: trace the shadow rays
 template <bool INST>
-__device__ __forceinline__ float cached_visibility(const SceneDev &S, const Surf &s) {
+__device__ __forceinline__ float cached_visibility(const SceneDev &S, const Surf &s, float time) {
   const float a = S.nstate[4 * (size_t)s.vi.x + 3].w, b = S.nstate[4 * (size_t)s.vi.y + 3].w,
               c = S.nstate[4 * (size_t)s.vi.z + 3].w;
   if (!(a == b && b == c && (a == 1.0f || a == 0.0f))) return -1.0f;
   if (INST) {
     const float4 ec = S.radu[1];
-    const v3 d = ld3(ec) - s.p;
+    const v3 d = emit_centre(S, emitter_pose(S, time)) - s.p;
     for (uint32_t i = 0; i < S.inst_count; i++) {
       const float4 *I = S.inst + INST_WORDS * (size_t)i;
       if ((__float_as_uint(I[3].w) & INST_HIDDEN) || ((S.shadow_skip >> i) & 1u)) continue;
@@ -252,20 +253,21 @@ struct LightEval {
 };
 
 __device__ __forceinline__ LightEval light_eval(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float u0,
-                                                float u1, float u2, bool mis, float fnl) {
+                                                float u1, float u2, float time, bool mis, float fnl) {
   LightEval e;
   e.ok = false;
-  v3 pl;
-  const uint32_t em = sample_emitter(S, u0, u1, u2, pl);
+  v3 pl, nl;
+  const EmitPose E = emitter_pose(S, time);
+  const uint32_t em = sample_emitter(S, E, u0, u1, u2, pl, nl);
   v3 wi = pl - s.p;
   const float d2 = dot(wi, wi), dist = sqrtf(d2);
   wi = wi * (1.0f / dist);
-  const float cosE = -dot(wi, ld3(S.emit[4 * em + 3]));
+  const float cosE = -dot(wi, nl);
   if (cosE <= 0.0f) return e;
   const float cosS = dot(s.ns, wi);
   if (cosS <= 0.0f || dot(s.ng, wi) <= 0.0f) return e;
   const uint32_t lid = __float_as_uint(S.emit[4 * em + 2].w);
-  const v3 Le = ld3(S.lights[lid]) * light_profile(S, lid, -wi);
+  const v3 Le = ld3(S.lights[lid]) * light_profile(S, lid, emit_local(E, -wi));
   if (maxc(Le) <= 0.0f) return e;
   const v3 f = eval_bsdf(M, s.ns, wi, wo);
   if (maxc(f) <= 0.0f) return e;
@@ -289,7 +291,7 @@ template <bool INST>
 __device__ __forceinline__ v3 direct_light(const SceneDev &S, const Surf &s, const Mat &M, v3 wo, float time, float u0,
                                            float u1, float u2, bool mis, float fnl, float cached, Stack &st,
                                            uint32_t &nrays, uint32_t &key) {
-  const LightEval e = light_eval(S, s, M, wo, u0, u1, u2, mis, fnl);
+  const LightEval e = light_eval(S, s, M, wo, u0, u1, u2, time, mis, fnl);
   if (!e.ok || cached == 0.0f) return mk(0.f, 0.f, 0.f);
   if (cached < 0.0f) {
     Hit sh;
@@ -320,7 +322,7 @@ __device__ __forceinline__ v3 still_light(const SceneDev &S, const DevCam &C, co
   for (uint32_t j = 0; j < nl; j++) {
     const uint32_t li = i * nl + j;
     const LightEval e = light_eval(S, s, M, wo, kr(li, KL0, P.s[3]), kr(li, KL1, P.s[4]), kr(li, KL2, P.s[5]),
-                                   false, fnl);
+                                   time, false, fnl);
     if (!e.ok) continue;
     nlit++;
     Hit sh;
@@ -366,7 +368,7 @@ __device__ __forceinline__ v3 secondary(const SceneDev &S, v3 o, v3 l, float tim
     if (opts & OPT_NO_MIS) return mk(0.f, 0.f, 0.f);
     const float cosE = -dot(l, s.nsi);
     if (cosE <= 0.0f) return mk(0.f, 0.f, 0.f);
-    const v3 Le = ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, -l);
+    const v3 Le = ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, emit_local(emitter_pose(S, time), -l));
     const float pL = h.t * h.t / (cosE * S.emit_area);
     return Le * (pdfB * pdfB / (pdfB * pdfB + fnl * fnl * pL * pL));
   }
@@ -421,7 +423,7 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
   if (opts & OPT_NORMALS) return s.ns * 0.5f + mk(0.5f, 0.5f, 0.5f);
   if (s.ms.y & F_EMITTER) {
     if (dot(wo, s.nsi) <= 0.0f) return mk(0.f, 0.f, 0.f);
-    return ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, wo);
+    return ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, emit_local(emitter_pose(S, time), wo));
   }
   const Mat M = load_mat(S.mats, s.ms.x);
   const bool stat = is_static_hit(h, s);
@@ -429,7 +431,7 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
   const uint32_t nl = nl_in < 1 ? 1 : (nl_in > 16 ? 16 : nl_in);
   const float fnl = (float)nl;
   const float cached = (stat && (opts & OPT_CULL_SHADOWS) && !(opts & OPT_SHADOW_CACHE))
-                           ? cached_visibility<INST>(S, s) : -1.0f;
+                           ? cached_visibility<INST>(S, s, time) : -1.0f;
   v3 L = mk(0.f, 0.f, 0.f);
   if (S.emit_count > 0) {
     if (stat && (opts & OPT_SHADOW_CACHE)) {
@@ -520,6 +522,12 @@ __device__ __forceinline__ void camdir_jac(const DevCam &C, float u, float v, v3
 #pragma unroll
     for (int p = 0; p < 4; p++) J[p] = (dq[p] - dc * dot(dc, dq[p])) * inv;
   }
+}
+
+// a world direction with tangents in the emitter's frame (the pose does not depend on the camera)
+__device__ __forceinline__ DV d_emit_local(const EmitPose &E, const DV &w) {
+  if (!E.posed) return w;
+  return {ddot(w, E.P.c0), ddot(w, E.P.c1), ddot(w, E.P.c2)};
 }
 
 // light_profile with the tangents of the direction: theta = acos(w . axis), d theta = -d(w . axis) / sin(theta);
@@ -613,7 +621,7 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
       const D cosE = -ddot(d, nsi);
       if (cosE.v > 0.0f) {
         const v3 Le = ld3(S.lights[ms.z]);
-        const D pe = d_light_profile(S, ms.z, -d);
+        const D pe = d_light_profile(S, ms.z, d_emit_local(emitter_pose(S, time), -d));
         D w = dk(1.0f);
         if (b > 0) {
           const D pL = t * t / (cosE * A);
@@ -640,19 +648,20 @@ __device__ __forceinline__ v3 path_walk_tan(const SceneDev &S, const DevCam &C, 
           u1 = u01(hash3(hs, i, 256u * b + 3u * j + 1u));
           u2 = u01(hash3(hs, i, 256u * b + 3u * j + 2u));
         }
-        v3 pl;
-        const uint32_t e = sample_emitter(S, u0, u1, u2, pl);
+        v3 pl, nle;
+        const EmitPose E = emitter_pose(S, time);
+        const uint32_t e = sample_emitter(S, E, u0, u1, u2, pl, nle);
         DV wi = pl - p;
         const D d2 = ddot(wi, wi);
         const D dist = dsqrt(d2);
         wi = wi * rcp(dist);
-        const D cosE = -ddot(wi, ld3(S.emit[4 * e + 3]));
+        const D cosE = -ddot(wi, nle);
         if (cosE.v <= 0.0f) continue;
         const D cosS = ddot(ns, wi);
         const v3 wiv = val(wi);
         if (cosS.v <= 0.0f || dot(ng, wiv) <= 0.0f) continue;
         const uint32_t lid = __float_as_uint(S.emit[4 * e + 2].w);
-        const D pe = d_light_profile(S, lid, -wi);
+        const D pe = d_light_profile(S, lid, d_emit_local(E, -wi));
         const v3 Le = ld3(S.lights[lid]) * pe.v;
         if (maxc(Le) <= 0.0f) continue;
         const Mat M = load_mat(S.mats, ms.x);

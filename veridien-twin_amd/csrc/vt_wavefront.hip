@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         if (ms.y & F_EMITTER) {
           const float cosE = -dot(d, nsi);
           if (cosE > 0.0f) {
-            const v3 Le = ld3(W.S.lights[ms.z]) * light_profile(W.S, ms.z, -d);
+            const v3 Le = ld3(W.S.lights[ms.z]) * light_profile(W.S, ms.z, emit_local(emitter_pose(W.S, time), -d));
             float w = 1.0f;
             if (b > 0) {
               const float pL = h4.x * h4.x / (cosE * A);
@@ -360,17 +360,18 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
                 u1 = u01(hash3(hs, i, 256u * b + 3u * j + 1u));
                 u2 = u01(hash3(hs, i, 256u * b + 3u * j + 2u));
               }
-              v3 pl;
-              const uint32_t e = sample_emitter(W.S, u0, u1, u2, pl);
+              v3 pl, nle;
+              const EmitPose E = emitter_pose(W.S, time);
+              const uint32_t e = sample_emitter(W.S, E, u0, u1, u2, pl, nle);
               v3 wi = pl - pt;
               const float d2 = dot(wi, wi), dist = sqrtf(d2);
               wi = wi * (1.0f / dist);
-              const float cosE = -dot(wi, ld3(W.S.emit[4 * e + 3]));
+              const float cosE = -dot(wi, nle);
               if (cosE <= 0.0f) continue;
               const float cosS = dot(ns, wi);
               if (cosS <= 0.0f || dot(ng, wi) <= 0.0f) continue;
               const uint32_t lid = __float_as_uint(W.S.emit[4 * e + 2].w);
-              const v3 Le = ld3(W.S.lights[lid]) * light_profile(W.S, lid, -wi);
+              const v3 Le = ld3(W.S.lights[lid]) * light_profile(W.S, lid, emit_local(E, -wi));
               if (maxc(Le) <= 0.0f) continue;
               const Mat M = load_mat(W.S.mats, ms.x);
               const v3 f = eval_bsdf(M, ns, wi, wo);

@@ -79,8 +79,8 @@ export function projectKB4(L, X) {
 /**
  * The reference's prepared scene (gpu/scene_gpu.js:461-469 prepareScene output: nodes, tris, verts, meshTable,
  * materials, emitter, blas) -> this library's upload arrays: one light (L0) with the emitter's angular profile
- * p(theta) about its axis, the bodies' triangle ranges as triBody. The emitter triangles stay static (the
- * reference poses them with the scope).
+ * p(theta) about its axis, the bodies' triangle ranges as triBody. The emitter records and axis are in the scope's
+ * frame, as the reference's, and the light rides on the scope body (emitBody; shade.wgsl.js:31-47).
  */
 export function fromPrepared(S) {
   const meshes = new Uint32Array(S.meshTable.length * 4);
@@ -121,6 +121,8 @@ export function fromPrepared(S) {
     // shadow rays leave through the scope's housing when the emitter sits at its front (gpu/renderer.js:648-649)
     const scopeI = S.blas.findIndex((b) => b.body === 'scope');
     out.shadowSkip = em.shadowSkipsScope && scopeI >= 0 ? (1 << scopeI) >>> 0 : 0;
+    // the outlet's triangles are part of the scope's BLAS: the light rides on the scope (SceneU.info.y)
+    if (scopeI >= 0 && em.tris.length) out.emitBody = scopeI;
   }
   return out;
 }

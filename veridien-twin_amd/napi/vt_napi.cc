@@ -244,6 +244,10 @@ napi_value UploadScene(napi_env env, napi_callback_info info) {
   s.tri_body = typed<uint32_t>(env, argv[1], "triBody", &n, napi_uint32_array);
   s.body_count = s.tri_body ? (uint32_t)num(env, argv[1], "bodyCount", 0) : 0;
   s.shadow_skip = (uint32_t)num(env, argv[1], "shadowSkip", 0);
+  // emitBody: the body the light rides on (its frame holds the emit records); absent or negative: static
+  const double eb = num(env, argv[1], "emitBody", -1);
+  s.emit_posed = eb >= 0 ? 1u : 0u;
+  s.emit_body = eb >= 0 ? (uint32_t)eb : 0u;
   if (s.tri_body && n != s.tri_count) {
     napi_throw_range_error(env, nullptr, "triBody needs one entry per triangle");
     return nullptr;

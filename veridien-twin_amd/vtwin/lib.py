@@ -40,7 +40,8 @@ class Scene(C.Structure):
     _fields_ = [("nodes", vp), ("node_count", u32), ("tris", vp), ("tri_count", u32), ("verts", vp),
                 ("vert_count", u32), ("meshes", vp), ("mesh_count", u32), ("materials", vp),
                 ("material_count", u32), ("lights", vp), ("light_count", u32), ("emit", vp), ("emit_count", u32),
-                ("emit_area", f32), ("tri_body", vp), ("body_count", u32), ("shadow_skip", u32)]
+                ("emit_area", f32), ("tri_body", vp), ("body_count", u32), ("shadow_skip", u32),
+                ("emit_posed", u32), ("emit_body", u32)]
 
 
 MAX_BODIES = 24
@@ -276,7 +277,8 @@ class Context:
     # ---------------------------------------------------------------- scene / cameras
     def upload_scene(self, sc, with_nodes=True):
         """Upload a scene dict; with sc["tri_body"] (body index per triangle, STATIC_BODY for the static scene)
-        and sc["body_count"] the bodies get BVHs of their own and are placed by set_instances."""
+        and sc["body_count"] the bodies get BVHs of their own and are placed by set_instances; sc["emit_body"]
+        (optional) names the body the light rides on, whose frame then holds the emit records."""
         a = {k: np.ascontiguousarray(sc[k]) for k in ("tris", "verts", "meshes", "materials", "lights", "emit")}
         nb = int(sc.get("body_count", 0))
         tb = np.ascontiguousarray(sc["tri_body"], dtype=np.uint32) if nb else None
@@ -285,7 +287,8 @@ class Context:
                   ptr(a["verts"]), a["verts"].size // 8, ptr(a["meshes"]), a["meshes"].size // 4,
                   ptr(a["materials"]), a["materials"].size // 16, ptr(a["lights"]), a["lights"].size // 4,
                   ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]), ptr(tb), nb,
-                  int(sc.get("shadow_skip", 0)))
+                  int(sc.get("shadow_skip", 0)), 1 if sc.get("emit_body") is not None else 0,
+                  int(sc.get("emit_body") or 0))
         self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
 
     def set_instances(self, poses):

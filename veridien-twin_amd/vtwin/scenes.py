@@ -402,9 +402,24 @@ def posed_parts():
     return parts, meshes, mats, lights
 
 
-def posed_scene(parts, meshes, mats, lights, place=None):
+# the light of posed_light_parts: its angular profile about the body's +x axis (axis, theta knots, p knots)
+BODY_LIGHT_PROFILE = ((1.0, 0.0, 0.0), (0.0, 0.6, 1.2, 1.5), (1.0, 0.8, 0.3, 0.0))
+
+
+def posed_light_parts():
+    """posed_parts with the light riding on body 0 (as the reference's outlet rides on the scope): a 0.2 m emitting
+    square 2 mm off the box's +x face (mesh 16, light 0, radiance 60) in place of the four ceiling lights."""
+    parts, meshes, mats, lights = posed_parts()
+    parts = parts[:4]
+    P, N, I = quad((0.252, 0.0, 0.0), 0.2, (1.0, 0.0, 0.0))
+    parts.append((P, N, I, np.full(I.shape[0], 16), 0))
+    return parts, meshes[:17], mats, [[60.0, 60.0, 60.0, 0.0]]
+
+
+def posed_scene(parts, meshes, mats, lights, place=None, emit_body=None):
     """Scene of posed_parts: bodies kept in their own frames (place=None, uploaded with tri_body), or every body
-    moved into the world by place[b] = (R, t) as static geometry (the reference scene for a posed one)."""
+    moved into the world by place[b] = (R, t) as static geometry (the reference scene for a posed one).
+    emit_body: the body the light rides on (bodies kept in their frames only)."""
     B = Builder()
     tb = []
     for P, N, I, M, body in parts:
@@ -419,7 +434,10 @@ def posed_scene(parts, meshes, mats, lights, place=None):
     if place is not None:
         return pack_scene(P, N, I, M, meshes, mats, lights)
     nb = 1 + max(b for *_, b in parts if b is not None)
-    return pack_scene(P, N, I, M, meshes, mats, lights, tri_body=np.concatenate(tb), body_count=nb)
+    sc = pack_scene(P, N, I, M, meshes, mats, lights, tri_body=np.concatenate(tb), body_count=nb)
+    if emit_body is not None:
+        sc["emit_body"] = int(emit_body)
+    return sc
 
 
 def posed_camera(w=96, h=72, spp=8, depth=3, shutter=(0.0, 1.0)):

@@ -5,9 +5,10 @@ Follows the reference's scene preparation (gpu/scene_gpu.js:20-472: decodeMesh, 
 meshes by vt_weld_subdivide, benchMesh, materialsFromPhysics, emitterFromPhysics, bodyOfPart, the per-body triangle
 sets, fitCapsule, the emitter records) and its kinematics (control/kinematics.js:97-244: shaftRotation, forward,
 scopeForward, defaultState, posesFromState, cannulaT). Differences, by the library's design: the static triangles
-keep their preparation order and the library builds every BVH at upload (the same binned-SAH trees); the emitter
-(the scope's light outlet) is placed in the world at the scope's pose of the frame and stays static geometry, since
-the library samples emitters from a world-space table; the charuco texture and FDM layer lines are not carried.
+keep their preparation order and the library builds every BVH at upload (the same binned-SAH trees); the charuco
+texture and FDM layer lines are not carried. The emitter (the scope's light outlet) rides on the scope body as the
+reference's does (its records and axis in the scope's frame, posed per sample); a scene without a scope body places
+it in the world at the scope's pose as static geometry.
 """
 import json
 import math
@@ -326,8 +327,8 @@ def poses_from_state(scene, state):
 def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
     """Upload arrays of the twin scene (pack_scene's keys, with tri_body / body_count) plus: bodies (per body key,
     kind, owner, port, capsule and occlusion capsule in the body frame, the area per material for rho_eff),
-    shadow_skip, profiles (the outlet's radiance profile about its world axis), stats. poses: {key: 4x4} placing
-    the scope's light outlet (default: the scene's default state)."""
+    shadow_skip, emit_body, profiles (the outlet's radiance profile about its axis, in the scope's frame), stats.
+    poses: {key: 4x4} placing the light outlet when there is no scope body (default: the scene's default state)."""
     if poses is None:
         poses = poses_from_state(scene, default_state(scene))
     bodies_of = body_of_part(scene)
@@ -368,7 +369,7 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
         mid = add_mesh("bench", bench_mat, MESH_FLAG["STATIC"] | MESH_FLAG["BENCH"])
         parts.append((P2, N2, I2, mid, None))
 
-    # the light outlet: the emitter mesh placed at the scope's pose, static
+    # bodies (bodyOfPart): the light outlet belongs to the scope's body, as in the reference's per-body sets
     emit_parts = []
     body_list, body_index = [], {}
     for m in metas:
@@ -385,21 +386,28 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
     if len(body_list) > 24:
         raise ValueError("too many bodies (%d > 24)" % len(body_list))
     light_idx = 0
-    emit_world = None
+    emit_body = body_index.get("scope") if emit_parts else None
+    if emit_parts and "emitter" not in mindex:
+        mindex["emitter"] = len(recs)
+        recs.append(scenes.material_record(scenes.KIND_DIELECTRIC, (0.02, 0.02, 0.02), roughness=0.05))
+        prepared.append(dict(name="emitter", emitter=True))
+    # the outlet in the emitter's frame (the scope's, or the world at the scope's pose without a scope body)
+    T = np.eye(4)
+    if emit_body is None and poses.get("scope") is not None:
+        T = np.asarray(poses["scope"], np.float64)
+    emit_local = None
+    emit_rows = []
     for m in emit_parts:
         P, N, I, _ = meshes[m["name"]]
-        T = np.asarray(poses.get("scope")) if poses.get("scope") is not None else np.eye(4)
-        Pw = P.astype(np.float64) @ T[:3, :3].T + T[:3, 3]
-        Nw = N.astype(np.float64) @ T[:3, :3].T
-        if "emitter" not in mindex:
-            mindex["emitter"] = len(recs)
-            recs.append(scenes.material_record(scenes.KIND_DIELECTRIC, (0.02, 0.02, 0.02), roughness=0.05))
-            prepared.append(dict(name="emitter", emitter=True))
+        Pe = P.astype(np.float64) @ T[:3, :3].T + T[:3, 3]
+        Ne = N.astype(np.float64) @ T[:3, :3].T
         mid = add_mesh(m["name"], "emitter", MESH_FLAG["EMITTER"] | MESH_FLAG["SCOPE"])
-        parts.append((Pw, Nw, np.asarray(I, np.uint32), mid, None))
-        emit_world = (Pw, Nw, np.asarray(I, np.uint32), T)
+        emit_rows.append((Pe, Ne, np.asarray(I, np.uint32), mid, emit_body))
+        emit_local = (Pe, Ne, np.asarray(I, np.uint32))
+    if emit_body is None:
+        parts.extend(emit_rows)
 
-    # bodies: their meshes concatenated in the body frame, in scene order
+    # bodies: their meshes concatenated in the body frame, in scene order (the scope's outlet after its parts)
     for bi, body in enumerate(body_list):
         for m in body["meshes"]:
             P, N, I, _ = meshes[m["name"]]
@@ -408,6 +416,8 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
                      (MESH_FLAG["SCOPE"] if body["kind"] == "scope" else 0))
             mid = add_mesh(m["name"], m["material"], flags)
             parts.append((P.astype(np.float64), N.astype(np.float64), np.asarray(I, np.uint32), mid, bi))
+        if bi == emit_body:
+            parts.extend(emit_rows)
 
     # capsules and the area per material of each body (its rho_eff needs the materials' albedo tables)
     overview = {m["name"] for m in metas if m.get("overview_only")}
@@ -433,7 +443,7 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
             area[k] = area.get(k, 0.0) + float(a)
         body["area_by_material"] = area
 
-    # one vertex / triangle list: static parts first (static triangles, then the outlet), bodies after
+    # one vertex / triangle list: static parts first, bodies after (the outlet with the scope)
     Pl, Nl, Il, Ml, Bl = [], [], [], [], []
     base = 0
     for P, N, I, mid, bi in parts:
@@ -455,11 +465,11 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
                             np.array([[em["L0"][0], em["L0"][1], em["L0"][2], 0.0]], np.float32),
                             tri_body=tri_body, body_count=len(body_list))
     # the outlet's axis: the scope's +z turned toward +y by the emission angle (physics scope.emission_axis), else
-    # the outlet's mean normal; in the world by the scope's pose
+    # the outlet's mean normal; in the emitter's frame
     profiles = {}
     shadow_skip = 0
-    if emit_world is not None:
-        Pw, Nw, Ie, T = emit_world
+    if emit_local is not None:
+        Pw, Nw, Ie = emit_local
         a, b, c = Pw[Ie[:, 0]], Pw[Ie[:, 1]], Pw[Ie[:, 2]]
         n = np.cross(b - a, c - a)
         area = 0.5 * np.linalg.norm(n, axis=1)
@@ -482,6 +492,8 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
                 ahead = max(ahead, float(np.max((Pb - cen) @ mean_n)))
             if ahead <= 1e-6:
                 shadow_skip = 1 << body_index["scope"]
+    if emit_body is not None:
+        out["emit_body"] = emit_body
     out.update(bodies=body_list, shadow_skip=shadow_skip, profiles=profiles, mesh_table=mesh_table,
                materials_prepared=prepared, material_index=mindex,
                stats=dict(raw_static_tris=raw_static, static_tris=int(np.sum(tri_body == STATIC_BODY)),

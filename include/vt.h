@@ -93,6 +93,15 @@ typedef struct {
    * reflectance rho_eff of the body: the reference estimator dims a static point's cached indirect light by
    * 1 - (1 - rho_eff) O(capsule) (capsuleVis, gpu/wgsl/shade.wgsl.js:57-94), the capsule posed at mid-exposure */
   float occ_a[3], occ_b[3], occ_r, rho_eff;
+  /* cavityCapsule (gpu/scene_gpu.js:474-503, applied at gpu/renderer.js:680-686): only the part of the posed
+   * occlusion capsule inside the cavity counts. With cavity = 1 the mid-exposure capsule is cut at the plane
+   * s = cavity_s0, s(X) = (X - cavity_origin) . cavity_axis (world frame; the axis points into the cavity): an end
+   * with s < cavity_s0 moves along the segment onto the plane, and a capsule wholly below it is dropped;
+   * cavity_r > 0 replaces the radius (a cannula's sheath). The reference's rules, for a body through a port
+   * (origin = the port's pivot, axis = -axis_out): a cannula s0 = 0 and r = cannula_outer_radius, an instrument or
+   * the scope s0 = cannula_toe_insertion; lens fronts and boards have no capsule (occ_r = 0). cavity = 0: uncut. */
+  uint32_t cavity;
+  float cavity_origin[3], cavity_axis[3], cavity_s0, cavity_r;
 } vt_instance;
 
 /* One tracking camera, CamU (gpu/wgsl/common.wgsl.js:79-90; gpu/renderer.js:888-898). */

@@ -156,7 +156,9 @@ class Body(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("tri_count", C.c_uint32), ("q0", C.c_float * 4),
                 ("q1", C.c_float * 4), ("p0", C.c_float * 3), ("p1", C.c_float * 3), ("hidden", C.c_uint32),
                 ("occ_a", C.c_float * 3), ("occ_b", C.c_float * 3), ("occ_r", C.c_float), ("rho_eff", C.c_float),
-                ("bc_a", C.c_float * 4), ("bc_b", C.c_float * 4)]
+                ("bc_a", C.c_float * 4), ("bc_b", C.c_float * 4), ("cavity", C.c_uint32),
+                ("cavity_origin", C.c_float * 3), ("cavity_axis", C.c_float * 3), ("cavity_s0", C.c_float),
+                ("cavity_r", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -252,6 +254,13 @@ def scene_struct(sc):
                 B.occ_b[:] = [float(x) for x in P.get("occ_b", (0, 0, 0))]
                 B.occ_r = float(P.get("occ_r", 0.0))
                 B.rho_eff = float(P.get("rho_eff", 0.0))
+                cav = P.get("cavity")
+                if cav is not None:
+                    B.cavity = 1
+                    B.cavity_origin[:] = [float(x) for x in cav["origin"]]
+                    B.cavity_axis[:] = [float(x) for x in cav["axis"]]
+                    B.cavity_s0 = float(cav["s0"])
+                    B.cavity_r = float(cav.get("r", 0.0))
                 ca, cb = body_capsule(t.reshape(-1, 16), P)
                 B.bc_a[:] = [float(x) for x in ca]
                 B.bc_b[:] = [float(x) for x in cb] + [0.0]

@@ -576,6 +576,26 @@ static void albedo_at(const or_scene *S, uint32_t m, double mu, double out[3]) {
 
 static void body_pose_mid(const or_body *B, pose_t *P) { body_pose(B, 0.5, P); }
 
+/* cavityCapsule (gpu/scene_gpu.js:486-503) on the posed capsule a-b of radius *r: the part with s >= s0 */
+static void cavity_cut(const or_body *B, double a[3], double b[3], double *r) {
+  if (!B->cavity) return;
+  if (B->cavity_r > 0) *r = B->cavity_r;
+  double ax[3] = {B->cavity_axis[0], B->cavity_axis[1], B->cavity_axis[2]};
+  norm3(ax);
+  double sa = 0, sb = 0;
+  for (int k = 0; k < 3; k++) {
+    sa += (a[k] - B->cavity_origin[k]) * ax[k];
+    sb += (b[k] - B->cavity_origin[k]) * ax[k];
+  }
+  const double s0 = B->cavity_s0;
+  if (sa < s0 && sb < s0) { *r = 0; return; }
+  double a0[3], b0[3];
+  memcpy(a0, a, sizeof a0);
+  memcpy(b0, b, sizeof b0);
+  if (sa < s0) for (int k = 0; k < 3; k++) a[k] = a0[k] + (b0[k] - a0[k]) * ((s0 - sa) / (sb - sa));
+  else if (sb < s0) for (int k = 0; k < 3; k++) b[k] = b0[k] + (a0[k] - b0[k]) * ((s0 - sb) / (sa - sb));
+}
+
 double or_capsule_vis(const or_scene *S, const double p[3], const double n[3]) {
   double v = 1;
   for (uint32_t b = 0; b < S->body_count; b++) {
@@ -583,12 +603,14 @@ double or_capsule_vis(const or_scene *S, const double p[3], const double n[3]) {
     if (B->hidden || !(B->occ_r > 0)) continue;
     pose_t Q;
     body_pose_mid(B, &Q);
-    double a[3], bb[3];
+    double a[3], bb[3], r = B->occ_r;
     const double oa[3] = {B->occ_a[0], B->occ_a[1], B->occ_a[2]}, ob[3] = {B->occ_b[0], B->occ_b[1], B->occ_b[2]};
     rot_apply(Q.R, oa, a);
     rot_apply(Q.R, ob, bb);
     for (int k = 0; k < 3; k++) { a[k] += Q.p[k]; bb[k] += Q.p[k]; }
-    v -= or_capsule_occ(p, n, a, bb, B->occ_r) * (1 - B->rho_eff);
+    cavity_cut(B, a, bb, &r);
+    if (!(r > 0)) continue;
+    v -= or_capsule_occ(p, n, a, bb, r) * (1 - B->rho_eff);
   }
   return v < 0 ? 0 : v > 1 ? 1 : v;
 }

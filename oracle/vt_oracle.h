@@ -192,6 +192,10 @@ typedef struct or_body {
    * in the world (a.xyz + radius, b.xyz) that the shadow culling tests */
   float occ_a[3], occ_b[3], occ_r, rho_eff;
   float bc_a[4], bc_b[4];
+  /* cavityCapsule (gpu/scene_gpu.js:486-503): cavity 1 cuts the posed occlusion capsule at s = cavity_s0 along the
+   * unit inward axis from cavity_origin (world); cavity_r > 0 replaces the radius */
+  uint32_t cavity;
+  float cavity_origin[3], cavity_axis[3], cavity_s0, cavity_r;
 } or_body;
 typedef struct {
   float R[9];            /* columns: camera x, y, z axes in the world frame */

@@ -213,6 +213,28 @@ def test_js_quat_from_mat_matches_scipy():
 
 
 @needs_node
+def test_cavity_of_rules():
+    """cavityOf (cavityCapsule's rules, gpu/scene_gpu.js:486-503): a cannula keeps s >= 0 with the sheath radius,
+    a body through the port s >= the toe, flush bodies and bodies without a port get no cut; fromPrepared attaches
+    the cut of each body's port."""
+    port = "{id: 'L', pivot: [0.1, 0.2, 0.3], axis_out: [0, 0, 1], cannula_toe_insertion: 0.099, cannula_outer_radius: 0.00425}"
+    S = ("{tris: new Float32Array(16 * 3), verts: new Float32Array(24), meshTable: [{mat: 0, flags: 1}], "
+         "materials: [{metallic: 0, rho: [0.5, 0.5, 0.5], alpha: 0.3, roughness: 0.5}], bodies: [{key: 'L/jaw', port: 'L'}, "
+         "{key: 'L/cannula', port: 'L'}], blas: [{body: 'L/jaw', kind: 'instrument', triOff: 1, triCount: 1, "
+         "capsule: {a: [0, 0, 0], b: [0, 0, 0.2], r: 0.003}}, {body: 'L/cannula', kind: 'fixture', triOff: 2, triCount: 1, "
+         "capsule: {a: [0, 0, 0], b: [0, 0, 0.1], r: 0.005}}], emitter: {tris: [], area: 0, L0: [1, 1, 1]}}")
+    out = _node("import('%s').then(m => { const P = %s; const u = m.fromPrepared(%s, [P]); console.log(JSON.stringify({"
+                "fix: m.cavityOf('fixture', P), ins: m.cavityOf('instrument', P), sc: m.cavityOf('scope', P), "
+                "lens: m.cavityOf('fixed', P), board: m.cavityOf('board', P), none: m.cavityOf('instrument', null), "
+                "bodies: u.bodies.map((b) => b.cavity || null)})); })" % (MODULE, port, S))
+    d = json.loads(out)
+    assert d["fix"] == {"origin": [0.1, 0.2, 0.3], "axis": [0, 0, -1], "s0": 0, "r": 0.00425}
+    assert d["ins"] == {"origin": [0.1, 0.2, 0.3], "axis": [0, 0, -1], "s0": 0.099, "r": 0} == d["sc"]
+    assert d["lens"] is None and d["board"] is None and d["none"] is None
+    assert d["bodies"] == [d["ins"], d["fix"]]
+
+
+@needs_node
 def test_from_prepared_bodies():
     """fromPrepared with the reference's BLAS list: triangle ranges become triBody, the caller BVH is dropped (every
     group's BVH is built at upload), the bodies keep their keys and kinds, and the light rides on the scope."""

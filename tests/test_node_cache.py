@@ -514,6 +514,89 @@ def _capsule_cases(n=4000, seed=12):
     return c
 
 
+def _cavity_vis(poses, pts, nrm):
+    """or_capsule_vis of the posed_parts bodies at the given poses over points and normals."""
+    sc = dict(scenes.posed_scene(*scenes.posed_parts()), poses=poses)
+    S = oracle.scene_struct(sc)
+    return np.array([oracle.lib().or_capsule_vis(oracle.C.byref(S), oracle.ptr(np.ascontiguousarray(p, np.float64)),
+                                                 oracle.ptr(np.ascontiguousarray(n, np.float64)))
+                     for p, n in zip(pts, nrm)])
+
+
+def test_oracle_cavity_cut_is_the_shortened_capsule():
+    """cavityCapsule (gpu/scene_gpu.js:486-503): body 1's capsule (0, 0, -0.3)-(0, 0, 0.3), r 0.15, posed at the
+    identity, cut at s0 along an axis from an origin: the capsule visibility everywhere equals that of the capsule
+    shortened by hand to the cut point (axis along +z, along -z, and tilted); a cut below both ends changes nothing,
+    one above both removes the capsule, and a cavity radius replaces the radius."""
+    rng = np.random.default_rng(31)
+    pts = rng.uniform(-0.6, 0.6, (300, 3))
+    pts[:, 2] = rng.uniform(-0.5, 0.5, 300)
+    nrm = rng.normal(size=(300, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    base = dict(q0=(0.0, 0.0, 0.0, 1.0), p0=(0.0, 0.0, 0.0), occ_a=(0, 0, -0.3), occ_b=(0, 0, 0.3), occ_r=0.15,
+                rho_eff=0.2)
+    far = dict(q0=(0.0, 0.0, 0.0, 1.0), p0=(9.0, 9.0, 9.0))
+    vis = lambda b1: _cavity_vis([far, b1], pts, nrm)
+    v_full = vis(base)
+    assert np.mean(v_full < 0.999) > 0.3
+    tilt = np.array([0.3, -0.2, 1.0]) / np.linalg.norm([0.3, -0.2, 1.0])
+    o = np.array([0.05, 0.02, -0.04])
+    # the point of the segment where (X - o) . tilt = 0.12, by hand
+    A, B = np.array([0, 0, -0.3]), np.array([0, 0, 0.3])
+    t = (0.12 - (A - o) @ tilt) / ((B - A) @ tilt)
+    X = A + (B - A) * t
+    for cav, a, b in ((dict(origin=(0, 0, 0), axis=(0, 0, 1), s0=0.1), (0, 0, 0.1), (0, 0, 0.3)),
+                      (dict(origin=(0, 0, 0), axis=(0, 0, -2.0), s0=0.1), (0, 0, -0.3), (0, 0, -0.1)),
+                      (dict(origin=tuple(o), axis=tuple(tilt * 3.0), s0=0.12), tuple(X), (0, 0, 0.3))):
+        got = vis(dict(base, cavity=cav))
+        want = vis(dict(base, occ_a=a, occ_b=b))
+        print("cut %s: max |cut - shortened| %.2e, changed vs uncut %.3f" % (
+            cav["axis"], np.abs(got - want).max(), np.mean(got != v_full)))
+        # the hand-cut ends are stored as float32 in the body record, the library's cut point stays in FP64
+        assert np.abs(got - want).max() < 1e-7 and np.mean(got != v_full) > 0.1
+    assert np.array_equal(vis(dict(base, cavity=dict(origin=(0, 0, 0), axis=(0, 0, 1), s0=-0.5))), v_full)
+    assert np.all(vis(dict(base, cavity=dict(origin=(0, 0, 0), axis=(0, 0, 1), s0=0.5))) == 1.0)
+    assert np.array_equal(vis(dict(base, cavity=dict(origin=(0, 0, 0), axis=(0, 0, 1), s0=-0.5, r=0.05))),
+                          vis(dict(base, occ_r=0.05)))
+
+
+@pytest.mark.gpu
+def test_gpu_cavity_cut_matches_oracle(ctx):
+    """The reference estimator with both bodies' occlusion capsules cut to a cavity (a moving box cut across its
+    long axis, the cylinder cut with a sheath radius) against the oracle on the same samples; the cut changes the
+    image."""
+    from test_bodies import _poses
+    parts = scenes.posed_parts()
+    sc = scenes.posed_scene(*parts)
+    poses = _poses(True)
+    poses[0].update(occ_a=(-0.25, 0, 0), occ_b=(0.25, 0, 0), occ_r=0.15, rho_eff=0.3,
+                    cavity=dict(origin=(0.75, -1.45, 0.3), axis=(1.0, 0.2, 0.0), s0=0.02))
+    poses[1].update(occ_a=(0, 0, -0.3), occ_b=(0, 0, 0.3), occ_r=0.15, rho_eff=0.5,
+                    cavity=dict(origin=(-0.75, -1.35, 0.45), axis=(0.0, -0.5, 1.0), s0=-0.05, r=0.1))
+    t = _gpu_cache(ctx, sc, poses=poses)
+    ctx.radiosity_update(direct=True, iters=16)
+    st, e = ctx.radiosity_state(np.asarray(sc["verts"]).size // 8)
+    cam = dict(scenes.posed_camera(96, 72, spp=8), flags=lib.VT_OPT_REFERENCE)
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    img = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+    sc2 = dict(sc, node_state=st, alb=t["alb"], hh=t["hh"], slot=1, e_avg=e, poses=poses)
+    ref, _, _ = oracle.render_tile(sc2, cam, 0, 0, 96, 72)
+    a, b = img[:, :, :3].astype(np.float64), ref[:, :, :3].astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    uncut = [{k: v for k, v in p.items() if k != "cavity"} for p in poses]
+    ctx.set_instances(uncut)
+    ctx.render([0])
+    img2 = ctx.readback(0, lib.VT_KIND_LINEAR, 96, 72)
+    changed = np.mean(np.any(img2 != img, axis=2))
+    print("cavity cut: GPU vs oracle relative RMSE %.3g; %.3f of the pixels change without the cut" % (rel, changed))
+    assert rel < 1e-3 and changed > 0.01
+    with pytest.raises(lib.VTError):
+        bad = [dict(p) for p in poses]
+        bad[0]["cavity"] = dict(origin=(0, 0, 0), axis=(0, 0, 0), s0=0.0)
+        ctx.set_instances(bad)
+
+
 @pytest.mark.gpu
 def test_gpu_capsule_occlusion_matches_oracle(ctx):
     """capsuleOcc on random points, normals and capsules (in and out of the horizon, near and far) against the

@@ -148,6 +148,36 @@ def test_twin_counts_and_outlet(loaded, prep):
 
 
 @needs_assets
+def test_twin_cavity_cuts(loaded, prep):
+    """cavityCapsule's cuts (gpu/scene_gpu.js:486-503) from the scene's ports: each cannula keeps s >= 0 with its
+    sheath radius, each instrument part and the scope s >= its cannula's toe, lens fronts and boards carry no
+    capsule; the cut plane is the port's pivot and inward axis."""
+    scene, _, _ = loaded
+    ports = {p["id"]: p for p in scene["ports"]}
+    inst = twin.instances(prep, twin.poses_from_state(scene, twin.default_state(scene)))
+    kinds = {}
+    for body, rec in zip(prep["bodies"], inst):
+        if rec is None:
+            continue
+        cav = rec.get("cavity")
+        kinds.setdefault(body["kind"], []).append(cav is not None)
+        if body["kind"] in ("fixed", "board"):
+            assert cav is None and rec["occ_r"] == 0.0
+            continue
+        port = ports.get(body.get("port"))
+        if port is None:
+            assert cav is None
+            continue
+        assert np.allclose(cav["origin"], port["pivot"]) and np.allclose(cav["axis"], -np.asarray(port["axis_out"]))
+        if body["kind"] == "fixture":
+            assert cav["s0"] == 0.0 and cav["r"] == pytest.approx(port["cannula_outer_radius"])
+        else:
+            assert cav["s0"] == pytest.approx(port["cannula_toe_insertion"]) and cav["r"] == 0.0
+    print("bodies with a cavity cut, by kind: %s" % {k: sum(v) for k, v in kinds.items()})
+    assert sum(kinds.get("instrument", [])) >= 4 and sum(kinds.get("fixture", [])) == 3 and all(kinds.get("scope", [False]))
+
+
+@needs_assets
 def test_kinematics_against_the_scene_export(loaded):
     scene, _, _ = loaded
     state = twin.default_state(scene)

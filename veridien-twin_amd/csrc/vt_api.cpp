@@ -664,7 +664,31 @@ extern "C" int vt_set_instances(vt_ctx *c, const vt_instance *in, uint32_t count
     double woa[3], wob[3];
     host_apply(Pm, oa, woa);
     host_apply(Pm, ob, wob);
-    r[8] = make_float4((float)woa[0], (float)woa[1], (float)woa[2], I.occ_r);
+    double occ_r = I.occ_r;
+    if (I.cavity) {
+      // the part inside the cavity: s >= s0 along the inward axis (cavityCapsule, gpu/scene_gpu.js:486-503)
+      const double ax[3] = {I.cavity_axis[0], I.cavity_axis[1], I.cavity_axis[2]};
+      const double al = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+      if (!(al > 1e-12) || !std::isfinite(al) || !std::isfinite(I.cavity_s0) || !(I.cavity_r >= 0.0f))
+        return fail(c, VT_ERR_ARG, "set_instances: body %u has a cavity without a finite axis, s0 and r >= 0", b);
+      if (I.cavity_r > 0.0f) occ_r = I.cavity_r;
+      const auto s_of = [&](const double X[3]) {
+        double d = 0;
+        for (int k = 0; k < 3; k++) d += (X[k] - I.cavity_origin[k]) * ax[k];
+        return d / al;
+      };
+      const double s0 = I.cavity_s0, sa = s_of(woa), sb = s_of(wob);
+      if (sa < s0 && sb < s0) {
+        occ_r = 0.0;
+      } else if (sa < s0) {
+        const double t = (s0 - sa) / (sb - sa);
+        for (int k = 0; k < 3; k++) woa[k] += (wob[k] - woa[k]) * t;
+      } else if (sb < s0) {
+        const double t = (s0 - sb) / (sa - sb);
+        for (int k = 0; k < 3; k++) wob[k] += (woa[k] - wob[k]) * t;
+      }
+    }
+    r[8] = make_float4((float)woa[0], (float)woa[1], (float)woa[2], (float)occ_r);
     r[9] = make_float4((float)wob[0], (float)wob[1], (float)wob[2], I.rho_eff);
   }
   // a light that rides on a body lights the static scene differently once that body's pose record changes: the

@@ -82,7 +82,19 @@ export function projectKB4(L, X) {
  * p(theta) about its axis, the bodies' triangle ranges as triBody. The emitter records and axis are in the scope's
  * frame, as the reference's, and the light rides on the scope body (emitBody; shade.wgsl.js:31-47).
  */
-export function fromPrepared(S) {
+/** cavityCapsule's cut (gpu/scene_gpu.js:486-503) for a body of `kind` through `port` (a scene.json ports[]
+ * entry), as vt_instance's cavity fields: the port's pivot and inward axis, s >= 0 with the sheath's radius for a
+ * cannula ('fixture'), s >= the cannula's toe for an instrument part or the scope; null without a port (lens fronts
+ * and boards have no capsule at all). */
+export function cavityOf(kind, port) {
+  if (!port || kind === 'fixed' || kind === 'board') return null;
+  const fixture = kind === 'fixture';
+  const toe = Number(port.cannula_toe_insertion), rr = Number(port.cannula_outer_radius);
+  return { origin: Array.from(port.pivot), axis: Array.from(port.axis_out, (x) => -x),
+    s0: fixture || !Number.isFinite(toe) ? 0 : toe, r: fixture && Number.isFinite(rr) ? rr : 0 };
+}
+
+export function fromPrepared(S, ports = []) {
   const meshes = new Uint32Array(S.meshTable.length * 4);
   S.meshTable.forEach((m, i) => { meshes[4 * i] = m.mat; meshes[4 * i + 1] = m.flags >>> 0; meshes[4 * i + 2] = 0; });
   const materials = new Float32Array(S.materials.length * 16);
@@ -115,8 +127,10 @@ export function fromPrepared(S) {
     Object.assign(out, { nodes: undefined, triBody,
       bodies: S.blas.map((b) => {
         const oc = b.occCapsule || b.capsule;   // the occlusion capsule of capsuleVis, in the body frame
+        const pid = ((S.bodies || []).find((x) => x.key === b.body) || {}).port;
+        const cavity = cavityOf(b.kind, pid ? (ports || []).find((p) => p.id === pid) : null);
         return { key: b.body, kind: b.kind, owner: b.owner || null, rhoEff: b.rhoEff || 0,
-          occ: oc ? { a: Array.from(oc.a), b: Array.from(oc.b), r: oc.r } : null };
+          occ: oc ? { a: Array.from(oc.a), b: Array.from(oc.b), r: oc.r } : null, ...(cavity ? { cavity } : {}) };
       }) });
     // shadow rays leave through the scope's housing when the emitter sits at its front (gpu/renderer.js:648-649)
     const scopeI = S.blas.findIndex((b) => b.body === 'scope');
@@ -148,7 +162,7 @@ export function quatFromMat(m) {
 export class TwinRenderer {
   /**
    * opts: {scene: upload arrays ({nodes?, tris, verts, meshes, materials, lights, emit, emitArea}) or
-   * {prepared: the reference's prepareScene output}, cameras: [{name, lens (scene lens block or 9 numbers),
+   * {prepared: the reference's prepareScene output, ports?: scene.json's ports}, cameras: [{name, lens (scene lens block or 9 numbers),
    * T_D_C (4x4) | {R (9, columns), pos}, size: [w, h]}], physics? (physics.yaml's sensor / isp blocks), sensor?
    * (overrides of sensorParams), device?, preset?, log?, ...DEFAULTS}
    */
@@ -161,7 +175,7 @@ export class TwinRenderer {
   _init(opts) {
     this.opts = { ...DEFAULTS, ...(PRESETS[opts.preset] || {}), ...opts };
     this.log = opts.log || (() => {});
-    const src = opts.scene && opts.scene.prepared ? fromPrepared(opts.scene.prepared) : opts.scene;
+    const src = opts.scene && opts.scene.prepared ? fromPrepared(opts.scene.prepared, opts.scene.ports) : opts.scene;
     if (!src || !src.tris) throw new Error('scene: pass the upload arrays or {prepared}');
     this.ctx = native.create(opts.device || 0);
     this.device = { hip: opts.device || 0 };
@@ -346,7 +360,10 @@ export class TwinRenderer {
     const b0 = (poses0 && poses0.bodies) || {}, b1 = ((poses1 || poses0) && (poses1 || poses0).bodies) || {};
     for (const k of Object.keys(b0)) if (!this.bodies.some((b) => b.key === k)) throw new Error(`unknown body ${k}`);
     this.instances = this.bodies.map((bl) => {
-      const occ = bl.occ ? { occA: bl.occ.a, occB: bl.occ.b, occR: bl.occ.r, rhoEff: dflt(bl.rhoEff, 0) } : {};
+      // capsuleVis's capsule: none for lens fronts and boards, cut to the cavity for a body through a port
+      const flush = bl.kind === 'fixed' || bl.kind === 'board';
+      const occ = bl.occ ? { occA: bl.occ.a, occB: bl.occ.b, occR: flush ? 0 : bl.occ.r, rhoEff: dflt(bl.rhoEff, 0),
+        ...(bl.cavity ? { cavity: bl.cavity } : {}) } : {};
       if (bl.kind === 'fixed') return { q0: [0, 0, 0, 1], q1: [0, 0, 0, 1], p0: [0, 0, 0], p1: [0, 0, 0], ...occ };
       const T0 = b0[bl.key], T1 = b1[bl.key] || T0;
       if (!T0) return null;

@@ -337,6 +337,20 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
       I.occ_r = (float)num(env, e, "occR", 0);
       I.rho_eff = (float)num(env, e, "rhoEff", 0);
     }
+    // the cavity cut of the occlusion capsule: {origin: [3], axis: [3], s0, r}
+    napi_value cav;
+    bool has = false;
+    napi_has_named_property(env, e, "cavity", &has);
+    if (has && napi_get_named_property(env, e, "cavity", &cav) == napi_ok) {
+      napi_valuetype t;
+      napi_typeof(env, cav, &t);
+      if (t == napi_object && float_list(env, cav, "origin", I.cavity_origin, 3) &&
+          float_list(env, cav, "axis", I.cavity_axis, 3)) {
+        I.cavity = 1;
+        I.cavity_s0 = (float)num(env, cav, "s0", 0);
+        I.cavity_r = (float)num(env, cav, "r", 0);
+      }
+    }
   }
   int rc = vt_set_instances(c, v.data(), len);
   if (rc) return throw_vt(env, c, rc, "vt_set_instances");

@@ -50,7 +50,9 @@ STATIC_BODY = 0xFFFFFFFF
 
 class Instance(C.Structure):
     _fields_ = [("q0", f32 * 4), ("q1", f32 * 4), ("p0", f32 * 3), ("p1", f32 * 3), ("hidden", u32),
-                ("occ_a", f32 * 3), ("occ_b", f32 * 3), ("occ_r", f32), ("rho_eff", f32)]
+                ("occ_a", f32 * 3), ("occ_b", f32 * 3), ("occ_r", f32), ("rho_eff", f32),
+                ("cavity", u32), ("cavity_origin", f32 * 3), ("cavity_axis", f32 * 3), ("cavity_s0", f32),
+                ("cavity_r", f32)]
 
 
 class RadiosityOpts(C.Structure):
@@ -292,7 +294,8 @@ class Context:
         self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
 
     def set_instances(self, poses):
-        """vt_set_instances: poses[i] = None (hidden) or dict(q0, p0[, q1, p1]) with quaternions (x, y, z, w)."""
+        """vt_set_instances: poses[i] = None (hidden) or dict(q0, p0[, q1, p1]) with quaternions (x, y, z, w), and
+        optionally occ_a, occ_b, occ_r, rho_eff and cavity = dict(origin, axis, s0[, r])."""
         arr = (Instance * max(1, len(poses)))()
         for i, P in enumerate(poses):
             if P is None:
@@ -308,6 +311,13 @@ class Context:
             arr[i].occ_b[:] = [float(x) for x in P.get("occ_b", (0, 0, 0))]
             arr[i].occ_r = float(P.get("occ_r", 0.0))
             arr[i].rho_eff = float(P.get("rho_eff", 0.0))
+            cav = P.get("cavity")
+            if cav is not None:
+                arr[i].cavity = 1
+                arr[i].cavity_origin[:] = [float(x) for x in cav["origin"]]
+                arr[i].cavity_axis[:] = [float(x) for x in cav["axis"]]
+                arr[i].cavity_s0 = float(cav["s0"])
+                arr[i].cavity_r = float(cav.get("r", 0.0))
         self.check(self.L.vt_set_instances(self.h, arr, len(poses)))
 
     def set_camera(self, slot, cam):

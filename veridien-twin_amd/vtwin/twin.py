@@ -494,6 +494,11 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
                 shadow_skip = 1 << body_index["scope"]
     if emit_body is not None:
         out["emit_body"] = emit_body
+    # the ports the bodies go through: what cavityCapsule cuts the occlusion capsules with
+    out["ports"] = {p["id"]: dict(pivot=[float(x) for x in p["pivot"]], axis_out=[float(x) for x in p["axis_out"]],
+                                  cannula_toe_insertion=p.get("cannula_toe_insertion"),
+                                  cannula_outer_radius=p.get("cannula_outer_radius"))
+                    for p in scene.get("ports") or [] if p.get("pivot") and p.get("axis_out")}
     out.update(bodies=body_list, shadow_skip=shadow_skip, profiles=profiles, mesh_table=mesh_table,
                materials_prepared=prepared, material_index=mindex,
                stats=dict(raw_static_tris=raw_static, static_tris=int(np.sum(tri_body == STATIC_BODY)),
@@ -535,9 +540,33 @@ def rho_eff(prep, hh):
         body["rho_eff"] = r / a if a else 0.0
 
 
+def _finite(x):
+    try:
+        return math.isfinite(float(x))
+    except (TypeError, ValueError):
+        return False
+
+
+def cavity_of(prep, body):
+    """cavityCapsule's cut (gpu/scene_gpu.js:486-503) of a body through a port, as vt_instance's cavity fields: the
+    port's pivot and inward axis (-axis_out); a cannula (fixture) keeps s >= 0 with the sheath's radius
+    cannula_outer_radius, an instrument part or the scope s >= cannula_toe_insertion. None without a port (and for
+    lens fronts and boards, which have no capsule at all)."""
+    port = (prep.get("ports") or {}).get(body.get("port"))
+    if port is None or body["kind"] in ("fixed", "board"):
+        return None
+    fixture = body["kind"] == "fixture"
+    toe = port.get("cannula_toe_insertion")
+    s0 = 0.0 if fixture or not _finite(toe) else float(toe)
+    rr = port.get("cannula_outer_radius")
+    r = float(rr) if fixture and _finite(rr) else 0.0
+    return dict(origin=list(port["pivot"]), axis=[-x for x in port["axis_out"]], s0=s0, r=r)
+
+
 def instances(prep, poses0, poses1=None):
     """vt_set_instances records (lib.Context.set_instances) of the prepared bodies at shutter open and close from
-    posesFromState matrices; a body without a pose is hidden, a camera's lens front stays where the scene has it."""
+    posesFromState matrices; a body without a pose is hidden, a camera's lens front stays where the scene has it;
+    occlusion capsules cut to the cavity (cavity_of)."""
     poses1 = poses0 if poses1 is None else poses1
     out = []
     for body in prep["bodies"]:
@@ -552,7 +581,7 @@ def instances(prep, poses0, poses1=None):
         oc = body["occ_capsule"]
         out.append(dict(q0=quat_from_mat(T0[:3, :3]), p0=T0[:3, 3], q1=quat_from_mat(T1[:3, :3]), p1=T1[:3, 3],
                         occ_a=oc["a"], occ_b=oc["b"], occ_r=0.0 if body["kind"] in ("fixed", "board") else oc["r"],
-                        rho_eff=body.get("rho_eff", 0.0)))
+                        rho_eff=body.get("rho_eff", 0.0), cavity=cavity_of(prep, body)))
     return out
 
 

@@ -459,6 +459,36 @@ int vt_radiosity_tables(vt_ctx *ctx, uint32_t which, void *dst, size_t bytes);
 int vt_node_neighbours(const float *pos, const float *nrm, uint32_t n, uint32_t k, float radius, float sigma,
                        uint32_t *idx, float *w);
 
+/* ---------------------------------------------------------------- the baked node table (the lite renderer)
+ * The lite renderer's indirect light (lite/renderer.js:214-222, 486-517; lite/shaders.js:220-283): a baked table of
+ * K cosine gather rays per node (assets/lite_nodes.bin: 12,976 nodes x K = 256 16-bit hit ids, 0xffff for a ray
+ * that escapes), resident on the device. Per pose change: vt_baked_occlusion with the posed capsules (gatherOcc),
+ * then vt_baked_jacobi with the nodes' reflected direct light (jacobi), warm started. */
+typedef struct {
+  uint32_t nodes, K;          /* nodes < 65535; K a multiple of 4, at most VT_BAKED_MAX_K */
+  const uint16_t *gather;     /* nodes x K hit node ids (0xffff: the ray escapes) */
+  const float *node_a;        /* nodes x 4: position xyz (m), area (m^2) (nodeA, lite/renderer.js:215-217) */
+  const float *node_n;        /* nodes x 4: unit normal xyz, mesh id (uint32 bits) (nodeN) */
+  const float *hh;            /* meshes x 4: hemispherical albedo rgb, - (Mat.hh) */
+  uint32_t meshes;
+} vt_baked_table;
+#define VT_BAKED_MAX_K 1024
+#define VT_BAKED_MAX_CAPS 16
+/* Upload the table; every ray transmits, B, E_dir and E_ind start at 0. Blocks. */
+int vt_baked_upload(vt_ctx *ctx, const vt_baked_table *table);
+/* gatherOcc (lite/shaders.js:234-258): per gather ray the transmission byte 255, or min over the capsules whose
+ * radius reaches the segment from the node (lifted 1 mm along its normal) to the hit node of round(255 rho).
+ * caps: n_caps x 8 floats [a.xyz, radius, b.xyz, rho] (posed, world; the body's mean albedo rho). Enqueue only. */
+int vt_baked_occlusion(vt_ctx *ctx, const float *caps, uint32_t n_caps);
+/* `iters` steps of jacobi (lite/shaders.js:260-283): E_ind = pi/K sum_k t_k B(hit_k), t_k = byte / 255;
+ * B' = (E_dir + hh E_ind) / pi, ping-ponged from the latest B. e_dir: nodes x 4 floats (E_dir A(mu_l) rgb, -) to
+ * upload first, or NULL to keep the last. Enqueue only. */
+int vt_baked_jacobi(vt_ctx *ctx, const float *e_dir, uint32_t iters);
+/* Read back (blocks): VT_BAKED_B the latest B (nodes x 4 floats), VT_BAKED_EIND the last step's E_ind
+ * (nodes x 4 floats), VT_BAKED_OCC the transmission bytes (nodes x K). bytes must equal the size. */
+enum { VT_BAKED_B = 0, VT_BAKED_EIND = 1, VT_BAKED_OCC = 2 };
+int vt_baked_read(vt_ctx *ctx, uint32_t which, void *dst, size_t bytes);
+
 /* ---------------------------------------------------------------- node radiosity (host-given tables) */
 /* One Jacobi step of the reference's sparse gather form (jacobiGather + jacobiUpdate,
  * gpu/wgsl/kernels.wgsl.js:416-461; _submitJacobi gpu/renderer.js:729-749) on host-given tables. node_state

@@ -81,6 +81,8 @@ def lib():
             "or_gather_remap": (None, [vp, vp, vp, u32, u32, vp, vp, i32]),
             "or_node_direct": (None, [vp, vp, vp, vp, vp, u32, u32, vp, vp, i32]),
             "or_capsule_vis": (f64, [vp, vp, vp]),
+            "or_baked_occ": (None, [u32, u32, vp, vp, vp, vp, u32, vp, vp]),
+            "or_baked_jacobi": (None, [u32, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_jacobi_dense": (i32, [vp, vp, u32, u32, f64, i32, vp, vp]),
             "or_jacobi_gather": (None, [u32, vp, vp, u32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
             "or_jacobi_update": (None, [u32, vp, vp, vp, u32, vp, vp, vp, vp, i32]),
@@ -617,6 +619,38 @@ def node_direct(sc, ni, active, samples, alb, state, nthreads=8):
     act = np.ascontiguousarray(active, np.uint32)
     lib().or_node_direct(C.byref(S), ptr(ni["npos"]), ptr(ni["nnorm"]), ptr(ni["nmat"]), ptr(act), act.size,
                          samples, ptr(np.ascontiguousarray(alb, np.float32)), ptr(state), nthreads)
+
+
+def baked_occ(table, caps, with_dist=False):
+    """gatherOcc (lite/shaders.js:234-258) on a baked table dict(nodes, K, gather uint16 (nodes, K), node_a, node_n
+    (nodes, 4) float32 with the mesh id bits in node_n[:, 3], hh (meshes, 4)); caps (n, 8) [a, r, b, rho].
+    Returns the transmission bytes (nodes, K) and, with_dist, each ray's closest capsule distance minus radius."""
+    n, K = int(table["nodes"]), int(table["K"])
+    g = np.ascontiguousarray(table["gather"], np.uint16)
+    a = np.ascontiguousarray(table["node_a"], np.float32)
+    nn = np.ascontiguousarray(table["node_n"], np.float32)
+    cp = np.ascontiguousarray(np.asarray(caps, np.float32).reshape(-1, 8))
+    occ = np.zeros((n, K), np.uint8)
+    dist = np.zeros((n, K), np.float64) if with_dist else None
+    lib().or_baked_occ(n, K, ptr(g), ptr(a), ptr(nn), ptr(cp), cp.shape[0], ptr(occ), ptr(dist) if with_dist else None)
+    return (occ, dist) if with_dist else occ
+
+
+def baked_jacobi(table, occ, e_dir, b, iters):
+    """`iters` baked Jacobi steps (lite/shaders.js:260-283) from B = b; returns (B, E_ind of the last step)."""
+    n, K = int(table["nodes"]), int(table["K"])
+    g = np.ascontiguousarray(table["gather"], np.uint16)
+    nn = np.ascontiguousarray(table["node_n"], np.float32)
+    hh = np.ascontiguousarray(table["hh"], np.float32)
+    o = np.ascontiguousarray(occ, np.uint8)
+    ed = np.ascontiguousarray(e_dir, np.float32)
+    cur = np.ascontiguousarray(b, np.float32).copy()
+    nxt = np.zeros_like(cur)
+    ei = np.zeros((n, 4), np.float32)
+    for _ in range(iters):
+        lib().or_baked_jacobi(n, K, ptr(g), ptr(o), ptr(nn), ptr(hh), ptr(ed), ptr(cur), ptr(nxt), ptr(ei))
+        cur, nxt = nxt, cur
+    return cur, ei
 
 
 def jacobi_steps(sc, ni, active, gtable, K, nbr, nbr_w, alb, hh, state, slot, iters):

@@ -135,3 +135,93 @@ void or_jacobi_update(uint32_t n_active, const uint32_t *active, const uint32_t 
     for (int c = 0; c < 3; c++) st[c] = (float)Ef[c];
   }
 }
+
+/* ---------------------------------------------------------------- the lite renderer's baked node table */
+/* closest distance of the segments p0-p1 and q0-q1 (segSeg, lite/shaders.js:86-104), FP64 */
+static double seg_seg(const double p0[3], const double p1[3], const double q0[3], const double q1[3]) {
+  double d1[3], d2[3], r[3];
+  for (int k = 0; k < 3; k++) { d1[k] = p1[k] - p0[k]; d2[k] = q1[k] - q0[k]; r[k] = p0[k] - q0[k]; }
+  const double a = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2], e = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  const double f = d2[0] * r[0] + d2[1] * r[1] + d2[2] * r[2];
+  double s = 0, t = 0;
+  if (a <= 1e-14) {
+    t = fmin(fmax(f / fmax(e, 1e-14), 0), 1);
+  } else {
+    const double c = d1[0] * r[0] + d1[1] * r[1] + d1[2] * r[2];
+    if (e <= 1e-14) {
+      s = fmin(fmax(-c / a, 0), 1);
+    } else {
+      const double b = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+      const double den = a * e - b * b;
+      if (den > 1e-14) s = fmin(fmax((b * f - c * e) / den, 0), 1);
+      t = (b * s + f) / e;
+      if (t < 0) { t = 0; s = fmin(fmax(-c / a, 0), 1); }
+      else if (t > 1) { t = 1; s = fmin(fmax((b - c) / a, 0), 1); }
+    }
+  }
+  double w2 = 0;
+  for (int k = 0; k < 3; k++) {
+    const double w = (p0[k] + d1[k] * s) - (q0[k] + d2[k] * t);
+    w2 += w * w;
+  }
+  return sqrt(w2);
+}
+
+/* gatherOcc (lite/shaders.js:234-258): per gather ray of node i (from p_i + 1 mm n_i to the hit node) the
+ * transmission byte, 255 or the smallest round(255 rho) of the capsules [a.xyz, r, b.xyz, rho] within r of it;
+ * dist_out (optional): per ray the smallest capsule distance minus that capsule's radius (for near-threshold
+ * bookkeeping; +inf for a miss or no capsule) */
+void or_baked_occ(uint32_t nodes, uint32_t K, const uint16_t *gather, const float *node_a, const float *node_n,
+                  const float *caps, uint32_t n_caps, uint8_t *occ, double *dist_out) {
+  for (size_t i = 0; i < nodes; i++) {
+    const double p[3] = {(double)node_a[4 * i] + 1e-3 * node_n[4 * i], (double)node_a[4 * i + 1] + 1e-3 * node_n[4 * i + 1],
+                         (double)node_a[4 * i + 2] + 1e-3 * node_n[4 * i + 2]};
+    for (size_t k = 0; k < K; k++) {
+      const uint32_t hit = gather[i * K + k];
+      int t = 255;
+      double dm = INFINITY;
+      if (hit != 0xffffu) {
+        const double e[3] = {node_a[4 * (size_t)hit], node_a[4 * (size_t)hit + 1], node_a[4 * (size_t)hit + 2]};
+        for (uint32_t c = 0; c < n_caps; c++) {
+          const float *C = caps + 8 * (size_t)c;
+          const double qa[3] = {C[0], C[1], C[2]}, qb[3] = {C[4], C[5], C[6]};
+          const double d = seg_seg(p, e, qa, qb) - C[3];
+          if (d < dm) dm = d;
+          if (d < 0) {
+            /* the byte as the reference quantises it: a binary32 product, rounded half to even */
+            const float rho = fminf(fmaxf(C[7], 0.0f), 1.0f);
+            const float prod = 255.0f * rho;
+            const int v = (int)nearbyintf(prod);
+            if (v < t) t = v;
+          }
+        }
+      }
+      occ[i * K + k] = (uint8_t)t;
+      if (dist_out) dist_out[i * K + k] = dm;
+    }
+  }
+}
+
+/* one step of the baked Jacobi (lite/shaders.js:260-283): E_ind = pi/K sum_k (byte_k / 255) B_in(hit_k),
+ * B_out = (E_dir + hh(mesh) E_ind) / pi; B, E_dir, E_ind 4 floats per node, hh 4 floats per mesh */
+void or_baked_jacobi(uint32_t nodes, uint32_t K, const uint16_t *gather, const uint8_t *occ, const float *node_n,
+                     const float *hh, const float *e_dir, const float *b_in, float *b_out, float *e_ind) {
+  for (size_t i = 0; i < nodes; i++) {
+    double acc[3] = {0, 0, 0};
+    for (size_t k = 0; k < K; k++) {
+      const uint32_t hit = gather[i * K + k];
+      if (hit == 0xffffu) continue;
+      const double t = occ[i * K + k] / 255.0;
+      for (int c = 0; c < 3; c++) acc[c] += t * b_in[4 * (size_t)hit + c];
+    }
+    uint32_t m;
+    memcpy(&m, node_n + 4 * i + 3, 4);
+    for (int c = 0; c < 3; c++) {
+      const double E = acc[c] * (M_PI / K);
+      e_ind[4 * i + c] = (float)E;
+      b_out[4 * i + c] = (float)(((double)e_dir[4 * i + c] + (double)hh[4 * (size_t)m + c] * E) / M_PI);
+    }
+    e_ind[4 * i + 3] = 0;
+    b_out[4 * i + 3] = 0;
+  }
+}

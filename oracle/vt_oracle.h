@@ -259,6 +259,11 @@ void or_jacobi_gather(uint32_t n_active, const uint32_t *active, const uint32_t 
 void or_jacobi_update(uint32_t n_active, const uint32_t *active, const uint32_t *nbr, const float *nbr_w,
                       uint32_t nbr_k, const float *aux, const uint32_t *node_mat, const float *hh,
                       float *node_state, int slot_out);
+/* the lite renderer's baked node table (lite/shaders.js:234-283): gatherOcc and one Jacobi step */
+void or_baked_occ(uint32_t nodes, uint32_t K, const uint16_t *gather, const float *node_a, const float *node_n,
+                  const float *caps, uint32_t n_caps, uint8_t *occ, double *dist_out);
+void or_baked_jacobi(uint32_t nodes, uint32_t K, const uint16_t *gather, const uint8_t *occ, const float *node_n,
+                     const float *hh, const float *e_dir, const float *b_in, float *b_out, float *e_ind);
 void or_gather_dir(uint32_t i, uint32_t j, uint32_t K, const float n[3], float out[3]);
 
 #ifdef __cplusplus

@@ -142,6 +142,10 @@ struct vt_ctx {
   bool rad_pcaps_valid = false;
   // vt_jacobi_sparse's own copies of the caller's tables (never the node cache's buffers)
   DevBuf s_active, s_table, s_normals, s_mat, s_alb, s_hh, s_nbr, s_nbrw, s_state, s_aux;
+  // the baked node table (vt_baked_*): hits, node records, albedos, capsules, transmission bytes, E_dir, E_ind,
+  // and the two B buffers of the ping-pong (bk_cur holds the latest)
+  DevBuf bk_gather, bk_a, bk_n, bk_hh, bk_caps, bk_occ, bk_edir, bk_eind, bk_b[2];
+  uint32_t bk_nodes = 0, bk_K = 0, bk_meshes = 0, bk_ncaps = 0, bk_cur = 0;
   // dense radiosity
   DevBuf dF32, dF16, dF64, dE, dX, dY, dXd, dR, dPart;
   uint32_t dn_n = 0, dn_r = 0, dn_rp = 0;

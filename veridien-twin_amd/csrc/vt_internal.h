@@ -209,6 +209,21 @@ hipError_t launch_jacobi_gather(const GatherDev &G, hipStream_t s);
 hipError_t launch_jacobi_weights(const GatherDev &G, hipStream_t s);
 hipError_t launch_jacobi_update(const GatherDev &G, hipStream_t s);
 
+// the lite renderer's baked node table on the device (vt_baked.hip)
+struct BakedDev {
+  uint32_t nodes, K, n_caps;
+  const uint16_t *gather;    // nodes x K hit ids, 0xffff: escaped
+  const float4 *node_a;      // position, area
+  const float4 *node_n;      // normal, mesh (bits)
+  const float4 *hh;          // per mesh: hemispherical albedo
+  const float4 *caps;        // 2 per capsule: a.xyz + radius, b.xyz + rho
+  uint32_t *occ;             // nodes x K transmission bytes, 4 per word
+  const float4 *e_dir;       // per node: the reflected direct light E_dir A(mu_l)
+  float4 *e_ind;             // per node: the last step's indirect irradiance
+};
+hipError_t launch_baked_occ(const BakedDev &B, hipStream_t s);
+hipError_t launch_baked_jacobi(const BakedDev &B, const float4 *b_in, float4 *b_out, hipStream_t s);
+
 // dense node radiosity: x, y, e are n x rp (row-major, RHS fastest); K split four ways, sums added into y
 // FP32 on MFMA (v_mfma_f32_16x16x4_f32): xT is scratch of rp * n floats; n a multiple of 16, rp <= 48
 hipError_t launch_sweep_fp32_mfma(const float *F32, uint32_t n, uint32_t rp, const float *x, float *xT,

@@ -42,6 +42,7 @@ struct Handle {
   int pending = 0;          // queued jobs that will use c
   bool destroyed = false;   // destroy() was called
   bool collected = false;   // the JS external was garbage-collected
+  uint32_t baked_nodes = 0; // nodes of the baked table last uploaded (bakedJacobi's size check)
 };
 
 // with g_mu held
@@ -116,7 +117,8 @@ const T *typed(napi_env env, napi_value obj, const char *name, size_t *count, na
   size_t off;
   napi_get_typedarray_info(env, v, &tt, &len, &data, &ab, &off);
   if (tt != want) {
-    std::string m = std::string(name) + (want == napi_float32_array ? " must be a Float32Array" : " must be a Uint32Array");
+    std::string m = std::string(name) + (want == napi_float32_array ? " must be a Float32Array"
+                                         : want == napi_uint16_array ? " must be a Uint16Array" : " must be a Uint32Array");
     napi_throw_type_error(env, nullptr, m.c_str());
     return nullptr;
   }
@@ -966,6 +968,109 @@ napi_value RadiosityState(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// bakedUpload(ctx, {nodes, K, gather: Uint16Array, nodeA: Float32Array, nodeN: Float32Array, hh: Float32Array})
+// (the lite renderer's baked node table; blocks)
+napi_value BakedUpload(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c || argc < 2) return nullptr;
+  vt_baked_table t{};
+  size_t ng = 0, na = 0, nn = 0, nh = 0;
+  t.nodes = (uint32_t)num(env, argv[1], "nodes", 0);
+  t.K = (uint32_t)num(env, argv[1], "K", 0);
+  t.gather = typed<uint16_t>(env, argv[1], "gather", &ng, napi_uint16_array);
+  t.node_a = typed<float>(env, argv[1], "nodeA", &na, napi_float32_array);
+  t.node_n = typed<float>(env, argv[1], "nodeN", &nn, napi_float32_array);
+  t.hh = typed<float>(env, argv[1], "hh", &nh, napi_float32_array);
+  t.meshes = (uint32_t)(nh / 4);
+  if (!t.gather || !t.node_a || !t.node_n || !t.hh || ng != (size_t)t.nodes * t.K || na != 4 * (size_t)t.nodes ||
+      nn != 4 * (size_t)t.nodes) {
+    napi_throw_range_error(env, nullptr, "bakedUpload: gather nodes*K, nodeA and nodeN nodes*4 entries");
+    return nullptr;
+  }
+  int rc = vt_baked_upload(c, &t);
+  if (rc) return throw_vt(env, c, rc, "vt_baked_upload");
+  get_handle(env, argv[0])->baked_nodes = t.nodes;
+  return nullptr;
+}
+
+// bakedOcclusion(ctx, caps: Float32Array(8 per capsule: a.xyz, r, b.xyz, rho)) (enqueue)
+// bakedJacobi(ctx, eDir: Float32Array(nodes * 4) | null, iters) (enqueue)
+napi_value BakedOcclusion(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c || argc < 2) return nullptr;
+  napi_value o;
+  CHECK_NAPI(napi_create_object(env, &o));
+  CHECK_NAPI(napi_set_named_property(env, o, "caps", argv[1]));
+  size_t n = 0;
+  const float *caps = typed<float>(env, o, "caps", &n, napi_float32_array);
+  if (!caps && n) return nullptr;
+  int rc = vt_baked_occlusion(c, caps, (uint32_t)(n / 8));
+  if (rc) return throw_vt(env, c, rc, "vt_baked_occlusion");
+  return nullptr;
+}
+
+napi_value BakedJacobi(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  const float *ed = nullptr;
+  if (argc > 1) {
+    napi_valuetype vt;
+    napi_typeof(env, argv[1], &vt);
+    if (vt != napi_null && vt != napi_undefined) {
+      napi_value o;
+      CHECK_NAPI(napi_create_object(env, &o));
+      CHECK_NAPI(napi_set_named_property(env, o, "eDir", argv[1]));
+      size_t n = 0;
+      ed = typed<float>(env, o, "eDir", &n, napi_float32_array);
+      if (!ed) return nullptr;
+      if (n != 4 * (size_t)get_handle(env, argv[0])->baked_nodes) {
+        napi_throw_range_error(env, nullptr, "bakedJacobi: eDir needs nodes * 4 entries");
+        return nullptr;
+      }
+    }
+  }
+  uint32_t iters = 1;
+  if (argc > 2) napi_get_value_uint32(env, argv[2], &iters);
+  int rc = vt_baked_jacobi(c, ed, iters);
+  if (rc) return throw_vt(env, c, rc, "vt_baked_jacobi");
+  return nullptr;
+}
+
+// bakedRead(ctx, which, nodes, K) -> Float32Array (B, E_ind: nodes * 4) | Uint8Array (bytes: nodes * K); blocks
+napi_value BakedRead(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  uint32_t which = 0, nodes = 0, K = 0;
+  if (argc > 1) napi_get_value_uint32(env, argv[1], &which);
+  if (argc > 2) napi_get_value_uint32(env, argv[2], &nodes);
+  if (argc > 3) napi_get_value_uint32(env, argv[3], &K);
+  const size_t bytes = which == VT_BAKED_OCC ? (size_t)nodes * K : (size_t)nodes * 16;
+  napi_value ab, out;
+  void *dst = nullptr;
+  CHECK_NAPI(napi_create_arraybuffer(env, std::max<size_t>(bytes, 1), &dst, &ab));
+  int rc = vt_baked_read(c, which, dst, bytes);
+  if (rc) return throw_vt(env, c, rc, "vt_baked_read");
+  if (which == VT_BAKED_OCC) CHECK_NAPI(napi_create_typedarray(env, napi_uint8_array, bytes, ab, 0, &out));
+  else CHECK_NAPI(napi_create_typedarray(env, napi_float32_array, bytes / 4, ab, 0, &out));
+  return out;
+}
+
 napi_value LastError(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1], out;
@@ -987,6 +1092,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
              {"radiosityBuild", RadiosityBuild}, {"interiorBvh", InteriorBvh}, {"radiosityUpdate", RadiosityUpdate},
              {"radiosityUpdateMoved", RadiosityUpdateMoved},
+             {"bakedUpload", BakedUpload}, {"bakedOcclusion", BakedOcclusion}, {"bakedJacobi", BakedJacobi},
+             {"bakedRead", BakedRead},
              {"radiosityState", RadiosityState}};
   for (auto &f : fns) {
     napi_value v;

@@ -19,7 +19,7 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
-    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_radiosity_state", "vt_node_neighbours",
+    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_baked_upload", "vt_baked_occlusion", "vt_baked_jacobi", "vt_baked_read", "vt_radiosity_state", "vt_node_neighbours",
     "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide", "vt_sensor_raw_device",
 ]
 
@@ -53,6 +53,10 @@ class Instance(C.Structure):
                 ("occ_a", f32 * 3), ("occ_b", f32 * 3), ("occ_r", f32), ("rho_eff", f32),
                 ("cavity", u32), ("cavity_origin", f32 * 3), ("cavity_axis", f32 * 3), ("cavity_s0", f32),
                 ("cavity_r", f32)]
+
+
+class BakedTable(C.Structure):
+    _fields_ = [("nodes", u32), ("K", u32), ("gather", vp), ("node_a", vp), ("node_n", vp), ("hh", vp), ("meshes", u32)]
 
 
 class RadiosityOpts(C.Structure):
@@ -188,6 +192,10 @@ def load():
         "vt_interior_bvh": (i32, [vp, u32, vp]),
         "vt_radiosity_update": (i32, [vp, u32, u32]),
         "vt_radiosity_update_moved": (i32, [vp, u32, u32, C.POINTER(u32)]),
+        "vt_baked_upload": (i32, [vp, vp]),
+        "vt_baked_occlusion": (i32, [vp, vp, u32]),
+        "vt_baked_jacobi": (i32, [vp, vp, u32]),
+        "vt_baked_read": (i32, [vp, u32, vp, sz]),
         "vt_radiosity_state": (i32, [vp, vp, sz, vp]),
         "vt_node_neighbours": (i32, [vp, vp, u32, u32, f32, f32, vp, vp]),
         "vt_radiosity_tables": (i32, [vp, u32, vp, sz]),
@@ -364,6 +372,33 @@ class Context:
 
     def radiosity_update(self, direct=True, iters=0):
         self.check(self.L.vt_radiosity_update(self.h, 1 if direct else 0, iters))
+
+    # ---------------------------------------------------------------- the baked node table (lite renderer)
+    def baked_upload(self, table):
+        """vt_baked_upload: dict(nodes, K, gather uint16 (nodes, K), node_a, node_n (nodes, 4) float32, hh
+        (meshes, 4) float32)."""
+        self._baked = {k: np.ascontiguousarray(table[k], dt) for k, dt in
+                       (("gather", np.uint16), ("node_a", np.float32), ("node_n", np.float32), ("hh", np.float32))}
+        b = self._baked
+        t = BakedTable(int(table["nodes"]), int(table["K"]), ptr(b["gather"]), ptr(b["node_a"]), ptr(b["node_n"]),
+                       ptr(b["hh"]), b["hh"].size // 4)
+        self.check(self.L.vt_baked_upload(self.h, C.byref(t)))
+        self._baked_shape = (int(table["nodes"]), int(table["K"]))
+
+    def baked_occlusion(self, caps):
+        cp = np.ascontiguousarray(np.asarray(caps, np.float32).reshape(-1, 8))
+        self.check(self.L.vt_baked_occlusion(self.h, ptr(cp), cp.shape[0]))
+
+    def baked_jacobi(self, e_dir=None, iters=1):
+        ed = None if e_dir is None else np.ascontiguousarray(e_dir, np.float32)
+        self.check(self.L.vt_baked_jacobi(self.h, ptr(ed), iters))
+
+    def baked_read(self, which):
+        """0: B (nodes, 4) float32; 1: E_ind (nodes, 4); 2: transmission bytes (nodes, K) uint8."""
+        n, K = self._baked_shape
+        out = np.zeros((n, K), np.uint8) if which == 2 else np.zeros((n, 4), np.float32)
+        self.check(self.L.vt_baked_read(self.h, which, ptr(out), out.nbytes))
+        return out
 
     DIRECT_KINDS = ("none", "incremental", "full")
 

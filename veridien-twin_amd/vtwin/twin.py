@@ -656,4 +656,18 @@ def lite_nodes(root):
     g = hits.astype(np.uint32)
     g[hits == 0xFFFF] = 0xFFFFFFFF
     return dict(count=n, K=K, pos=rec[:, 0:3].copy(), area=rec[:, 3].copy(), nrm=rec[:, 4:7].copy(),
-                mesh=rec[:, 7].copy().view(np.uint32), gtable=g.reshape(-1), stats=meta["stats"])
+                mesh=rec[:, 7].copy().view(np.uint32), gtable=g.reshape(-1), hits=hits.copy(), stats=meta["stats"])
+
+
+def baked_table(L, hh):
+    """The baked table of lite_nodes as vt_baked_upload's arrays (nodeA / nodeN, lite/renderer.js:214-217), with
+    hh (meshes x 3 or 4): the meshes' hemispherical albedos."""
+    n = L["count"]
+    a = np.zeros((n, 4), np.float32)
+    a[:, 0:3], a[:, 3] = L["pos"], L["area"]
+    nn = np.zeros((n, 4), np.float32)
+    nn[:, 0:3] = L["nrm"]
+    nn[:, 3] = L["mesh"].view(np.float32)
+    h = np.zeros((np.asarray(hh).shape[0], 4), np.float32)
+    h[:, 0:3] = np.asarray(hh, np.float32)[:, 0:3]
+    return dict(nodes=n, K=L["K"], gather=L["hits"], node_a=a, node_n=nn, hh=h)

@@ -616,7 +616,8 @@ def main():
         "config": {"workload": "full-scale forward: 2,013,432-tri seeded dome scene, 256 materials, 16 lights; "
                                "3 KB4 cameras 1920x1080, 64 spp, depth 5, NEE; RGGB sensor, shot + 2 DN read noise, 12 bit; "
                                "ISP: bilinear demosaic, bilateral NR, sRGB, BT.601 4:2:0",
-                   "cameras": len(cams), "spp": 64, "depth": 5, "mode": args.mode,
+                   "cameras": len(cams), "resolution": [cams[0]["width"], cams[0]["height"]], "spp": 64, "depth": 5,
+                   "mode": args.mode,
                    "form": "wavefront" if ext_launches else ("per-camera" if args.mode == "per-camera" else "batched")},
         "spp_mpix_per_s": round(spp_px / dt / 1e6, 2),
         "rays_per_frame": total_rays / args.steps,
@@ -695,6 +696,25 @@ def extras(args, ctx, sc, out):
     the adjoint / forward time ratio (differentiation configuration), the batched / per-camera speed-up (24
     uneven cameras) and BF16 + refinement against FP32 with its error against FP64 (dense radiosity)."""
     from vtwin import lib
+    import numpy as np
+    if out["config"]["form"] == "wavefront":
+        # the same frame as one hipGraph (captured on its first use, the bin sort between vertices): its time per
+        # replay and whether its image equals the eager frame's (camera 0, the last timed frame above)
+        t0 = time.time()
+        w0, h0 = out["config"]["resolution"]
+        slots = list(range(out["config"]["cameras"]))
+        eager = ctx.readback(0, lib.VT_KIND_LINEAR, w0, h0)
+        ms = []
+        for i in range(3):
+            ctx.render(slots, mode=lib.VT_RENDER_WAVEFRONT, graph=True)
+            st = ctx.stats()
+            if i:
+                ms.append(st["ms_render"])
+        same = bool(np.array_equal(ctx.readback(0, lib.VT_KIND_LINEAR, w0, h0), eager))
+        out["wavefront_graph"] = {"ms_per_frame": round(float(np.mean(ms)), 2), "graph_used": st["graph_used"],
+                                  "graph_builds": st["graph_builds"], "image_equals_eager": same,
+                                  "mrays_s": round(st["rays"] / (np.mean(ms) * 1e-3) / 1e6, 1)}
+        log("wavefront graph: %.1f ms per frame, image equals eager %s (%.1f s)" % (np.mean(ms), same, time.time() - t0))
     t0 = time.time()
     c5 = bench_c5(args, ctx=ctx, sc=sc, steps=2, warmup=1)
     out["adjoint_forward_ratio"] = c5["value"]

@@ -144,9 +144,14 @@ typedef struct {
                             the 48-entry stack spills to global memory. Wavefront: 4, 8 or 16 (0 = 8) */
   uint32_t queues;       /* batched: 1 global ray queue, or 8 queues (one per XCD, over contiguous eighths of the
                             frame, drained by the blocks of that XCD first); 0 = tuned default */
-  uint32_t graph;        /* batched: 1 = run the frame's launches as a hipGraph, captured on first use and
-                            replayed while the cameras, sample range and launch shape stay the same */
+  uint32_t graph;        /* batched and wavefront: 1 = run the frame's launches as a hipGraph, captured on first
+                            use and replayed while the cameras, sample range and launch shape stay the same (the
+                            wavefront graph does not time its extension traces one by one) */
   uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 16,777,216) */
+  uint32_t sort_bits;    /* wavefront: the sort of the live paths between vertices; 0 = tuned default, 1 = the
+                            radix sort of the whole 29-bit key (direction cell, origin Morton code), 4..13 = an
+                            unstable bin sort by the key's top bits (kernels only; a graph uses it, 12 bits when
+                            the default is the radix sort). The image does not depend on it. */
 } vt_render_opts;
 
 /* Result of vt_autotune: the chosen launch shapes (also made the context's defaults) and the probe timings. */
@@ -159,6 +164,7 @@ typedef struct {
   uint32_t mode;              /* the faster form: VT_RENDER_BATCHED or VT_RENDER_WAVEFRONT (used by VT_RENDER_AUTO) */
   uint32_t wf_batch, wf_stack_lds, wf_refill;  /* wavefront form: paths per batch, LDS stack, refill threshold */
   double probe_mrays_wavefront;               /* Mrays/s of the chosen wavefront shape on the same probe */
+  uint32_t wf_sort_bits;                      /* wavefront: 0 the whole-key radix sort, else bins of that many bits */
 } vt_tune_result;
 
 /* Sensor parameters, the 256-B sensor uniform (gpu/sensor.js:158-177), raw-DN part. */

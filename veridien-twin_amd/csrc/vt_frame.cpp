@@ -227,11 +227,17 @@ struct WfParams {
   uint32_t slds;      // LDS stack entries of the trace kernels (4, 8, 16)
   uint32_t refill;    // idle lanes that make a wave fetch rays
   uint64_t stride;    // 1, or > 1: trace every stride-th wave of the frame (tuning probes)
+  uint32_t sort_bits; // 0: the 29-bit radix sort between vertices; 4..13: the bin sort by the key's top bits
+  bool graph;         // run the frame's batches as a hipGraph
 };
 
+constexpr uint32_t GRAPH_SORT_BITS = 12;   // a captured wavefront frame bins by this many key bits unless told
+
 WfParams wf_params(const vt_ctx *c, const vt_render_opts &opt) {
+  uint32_t sb = opt.sort_bits ? (opt.sort_bits == 1 ? 0u : opt.sort_bits) : c->def_wf_sort;
+  if (opt.graph && sb == 0) sb = GRAPH_SORT_BITS;
   return WfParams{opt.batch ? opt.batch : c->def_wf_batch, opt.stack_lds ? opt.stack_lds : c->def_wf_stack,
-                  opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1};
+                  opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1, sb, opt.graph != 0};
 }
 
 // the wavefront frame: buffers for one batch, the batches in order, one event pair around all of them
@@ -290,16 +296,11 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   W.overflow = (uint32_t *)c->overflow.p;
   W.spill = (uint2 *)c->spill.p;
   W.spill_stride = c->spill_lanes;
-  VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
-  VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
-  if (!opt.accumulate)
-    for (uint32_t a = 0; a < n; a++) {
-      const DevCam &d = c->h_cams[a];
-      VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
-    }
-  VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
   const uint32_t slds = wp.slds;
   if (slds != 4 && slds != 8 && slds != 16) return fail(c, VT_ERR_ARG, "render: wavefront stack_lds must be 4, 8 or 16");
+  if (wp.sort_bits && (wp.sort_bits < 4 || wp.sort_bits > WF_SORT_BITS_MAX))
+    return fail(c, VT_ERR_ARG, "render: wavefront sort_bits must be 1 or 4..%u", WF_SORT_BITS_MAX);
+  if (wp.graph && (wp.stride != 1 || wp.sort_bits == 0)) return fail(c, VT_ERR_ARG, "render: a wavefront graph bins");
   W.wave_stride = wp.stride;
   int occ = 0;
   VT_HIP(c, wf_trace_occupancy(slds, c->S.inst_count != 0, &occ));
@@ -307,20 +308,78 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   const uint32_t trace_cap = std::min<uint32_t>(max_blocks(c, 256), (uint32_t)std::max(1, occ) * c->cu_count);
   const uint32_t refill = wp.refill;
   const uint64_t batches = (items + P - 1) / P;
-  while (c->ev_ext.size() < 2 * batches * depth) {
-    hipEvent_t ev;
-    VT_HIP(c, hipEventCreate(&ev));
-    c->ev_ext.push_back(ev);
+  // the frame's work: clear the sums, then every batch (with the extension traces timed when events are given)
+  auto enqueue = [&](bool timed) -> int {
+    VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
+    if (!opt.accumulate)
+      for (uint32_t a = 0; a < n; a++) {
+        const DevCam &d = c->h_cams[a];
+        VT_HIP(c, launch_zero_f4(d.acc, (uint64_t)d.width * d.height, c->stream));
+      }
+    if (timed) VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+    uint64_t bi = 0;
+    for (uint64_t bb = 0; bb < items; bb += P, bi++) {
+      W.batch_begin = bb;
+      W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
+      VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, slds, wp.sort_bits, keys_sorted, q_sorted,
+                                       c->wf_temp.p, c->wf_temp_bytes,
+                                       timed ? c->ev_ext.data() + 2 * depth * bi : nullptr, c->stream));
+    }
+    return VT_OK;
+  };
+  if (!wp.graph) {
+    while (c->ev_ext.size() < 2 * batches * depth) {
+      hipEvent_t ev;
+      VT_HIP(c, hipEventCreate(&ev));
+      c->ev_ext.push_back(ev);
+    }
+    VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+    if ((rc = enqueue(true))) return rc;
+    c->ext_launches = (uint32_t)(batches * depth);
+    VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+    return VT_OK;
   }
-  uint64_t bi = 0;
-  for (uint64_t bb = 0; bb < items; bb += P, bi++) {
-    W.batch_begin = bb;
-    W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
-    VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, slds, keys_sorted, q_sorted,
-                                     c->wf_temp.p, c->wf_temp_bytes, c->ev_ext.data() + 2 * depth * bi, c->stream));
+  // a graph of the whole frame (kernels only: the fill kernels and the bin sort), captured once and replayed while
+  // its launch records, shape and cameras' sums stay the same; its extension traces are not timed one by one
+  std::vector<uint8_t> key(1 + sizeof W + 8 * sizeof(uint64_t) + n * 2 * sizeof(uint64_t));
+  key[0] = 0x57;    // the wavefront form's graphs, apart from the batched form's
+  W.batch_begin = 0;
+  W.n = (uint32_t)std::min<uint64_t>(P, items);
+  std::memcpy(key.data() + 1, &W, sizeof W);
+  const uint64_t sc[8] = {items, P, depth, grid_cap, trace_cap, refill, slds | ((uint64_t)wp.sort_bits << 8),
+                          opt.accumulate ? 1u : 0u};
+  std::memcpy(key.data() + 1 + sizeof W, sc, sizeof sc);
+  for (uint32_t a = 0; a < n; a++) {
+    const DevCam &d = c->h_cams[a];
+    const uint64_t v[2] = {(uint64_t)(uintptr_t)d.acc, (uint64_t)d.width * d.height};
+    std::memcpy(key.data() + 1 + sizeof W + sizeof sc + a * sizeof v, v, sizeof v);
   }
-  c->ext_launches = (uint32_t)(batches * depth);
+  if (!c->gexec || key != c->gkey) {
+    drop_graph(c);
+    VT_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int erc = enqueue(false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (erc) {
+      if (g) (void)hipGraphDestroy(g);
+      return erc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c->gexec = nullptr;
+      return hip_fail(c, ei, "hipGraphInstantiate");
+    }
+    c->graph_builds++;
+    c->gkey = key;
+  }
+  VT_HIP(c, hipEventRecord(c->ev_r0, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
+  VT_HIP(c, hipGraphLaunch(c->gexec, c->stream));
   VT_HIP(c, hipEventRecord(c->ev_launch[1], c->stream));
+  c->graph_used = true;
+  c->ext_launches = 0;
   return VT_OK;
 }
 
@@ -354,8 +413,8 @@ extern "C" int vt_render(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt_r
   const Shape sh = shape_of(c, opt);
   rc = opt.mode == VT_RENDER_WAVEFRONT ? VT_OK : check_shape(c, sh);   // the wavefront checks its own
   if (rc) return rc;
-  if (opt.graph && opt.mode == VT_RENDER_WAVEFRONT) opt.graph = 0;   // the wavefront form is not captured
-  if (opt.graph && opt.mode != VT_RENDER_BATCHED) return fail(c, VT_ERR_ARG, "render: graph needs the batched mode");
+  if (opt.graph && opt.mode != VT_RENDER_BATCHED && opt.mode != VT_RENDER_WAVEFRONT)
+    return fail(c, VT_ERR_ARG, "render: graph needs the batched or the wavefront form");
   if (opt.mode > VT_RENDER_AUTO) return fail(c, VT_ERR_ARG, "render: mode %u", opt.mode);
   uint64_t items = 0;
   if ((rc = stage_frame(c, cams, n, opt, items))) return rc;
@@ -570,10 +629,11 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     out->probe_fraction = (double)probe_waves / (double)std::max<uint64_t>(1, frame_waves);
     out->mode = VT_RENDER_BATCHED;
     out->wf_batch = (uint32_t)c->def_wf_batch; out->wf_stack_lds = c->def_wf_stack; out->wf_refill = c->def_wf_refill;
+    out->wf_sort_bits = c->def_wf_sort;
     out->probe_mrays_wavefront = 0.0;
     return VT_OK;
   }
-  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride};
+  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride, c->def_wf_sort, false};
   double m_wf = 0;
   auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
     vt_render_opts po = opt;
@@ -598,7 +658,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   for (uint64_t bsz : batches)
     for (uint32_t st : wstacks)
       for (uint32_t rf : refills) {
-        const WfParams wp{bsz, st, rf, stride};
+        const WfParams wp{bsz, st, rf, stride, wbest.sort_bits, false};
         double m = 0;
         if ((rc = probe_wf(wp, m))) return rc;
         tried++;
@@ -607,6 +667,18 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
           wbest = wp;
         }
       }
+  // then the sort between vertices at that shape: the whole key, or bins of its top 10, 12 or 13 bits
+  for (uint32_t sb : {0u, 10u, 12u, 13u}) {
+    WfParams wp = wbest;
+    wp.sort_bits = sb;
+    double m = 0;
+    if ((rc = probe_wf(wp, m))) return rc;
+    tried++;
+    if (m > m_wf) {
+      m_wf = m;
+      wbest = wp;
+    }
+  }
   {
     double m2w = 0;
     if ((rc = probe_wf(wbest, m2w))) return rc;
@@ -615,6 +687,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   c->def_wf_batch = wbest.batch;
   c->def_wf_stack = wbest.slds;
   c->def_wf_refill = wbest.refill;
+  c->def_wf_sort = wbest.sort_bits;
   c->def_mode = m_wf > m_best ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
   drop_graph(c);
   // the probes left partial sums behind: clear the listed cameras
@@ -636,6 +709,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   out->wf_batch = (uint32_t)wbest.batch;
   out->wf_stack_lds = wbest.slds;
   out->wf_refill = wbest.refill;
+  out->wf_sort_bits = wbest.sort_bits;
   out->probe_mrays_wavefront = m_wf;
   out->probe_fraction = (double)probe_waves / (double)std::max<uint64_t>(1, frame_waves);
   return VT_OK;

@@ -98,9 +98,13 @@ size_t wf_sort_temp_bytes(uint32_t n);
 hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu);
 // trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
 // fetch of new rays (1..64); slds: LDS stack entries per lane of the trace kernels (4, 8 or 16)
+// sort_bits: 0 sorts the slots between vertices by their whole 29-bit key (rocPRIM's radix sort, whose memset
+// nodes keep it out of graphs); 4..WF_SORT_BITS_MAX bins them by the key's top bits (kernels only)
+constexpr uint32_t WF_SORT_BITS_MAX = 13;
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
-                                  void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext, hipStream_t s);
+                                  uint32_t refill, uint32_t slds, uint32_t sort_bits, uint32_t *keys_sorted,
+                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext,
+                                  hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid, hipStream_t s);
 // shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals

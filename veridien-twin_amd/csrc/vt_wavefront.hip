@@ -533,17 +533,102 @@ hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<false>(slds, inst), 256, 0);
 }
 
+// ---------------------------------------------------------------- the bin sort between vertices
+// A counting sort of the live slots by the top `bits` bits of their key (direction cell first, then the coarse
+// origin), the dead slots in one last bin: a histogram pass, one scan, and a scatter in which each block reserves
+// its share of every bin with one atomic and ranks its own slots in LDS. It is not stable -- the order within a bin
+// follows the atomics -- which leaves every path's result unchanged (a path's work depends on its own state only)
+// and moves no more than 16 B per slot; it issues kernels only, so the frame can be captured as a graph.
+constexpr uint32_t BIN_SORT_CHUNK = 4096;     // slots per scatter block
+__device__ __forceinline__ uint32_t key_bin(uint32_t key, uint32_t bits) {
+  return key == DEAD_KEY ? (1u << bits) : key >> (29u - bits);
+}
+
+__global__ __launch_bounds__(256) void k_bin_zero(uint32_t *hist, uint32_t nbins) {
+  for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < nbins; b += gridDim.x * 256) hist[b] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_bin_hist(const uint32_t *keys, uint32_t n, uint32_t bits, uint32_t *hist) {
+  extern __shared__ uint32_t s_cnt[];
+  const uint32_t nbins = (1u << bits) + 1u;
+  for (uint32_t b = threadIdx.x; b < nbins; b += 256) s_cnt[b] = 0u;
+  __syncthreads();
+  const uint32_t k0 = blockIdx.x * BIN_SORT_CHUNK, k1 = min(n, k0 + BIN_SORT_CHUNK);
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += 256) atomicAdd(&s_cnt[key_bin(keys[k], bits)], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nbins; b += 256)
+    if (s_cnt[b]) atomicAdd(&hist[b], s_cnt[b]);
+}
+
+// exclusive prefix sum of the bin counts, in place (one block of 1024 lanes, each summing a contiguous run)
+__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t *hist, uint32_t nbins) {
+  __shared__ uint32_t s_part[1024];
+  const uint32_t per = (nbins + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(nbins, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; b++) sum += hist[b];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s_part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0u;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t c = hist[b];
+    hist[b] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t *keys, const uint32_t *vals, uint32_t n,
+                                                     uint32_t bits, uint32_t *cursor, uint32_t *out) {
+  extern __shared__ uint32_t s_cnt[];
+  const uint32_t nbins = (1u << bits) + 1u;
+  for (uint32_t b = threadIdx.x; b < nbins; b += 256) s_cnt[b] = 0u;
+  __syncthreads();
+  const uint32_t k0 = blockIdx.x * BIN_SORT_CHUNK, k1 = min(n, k0 + BIN_SORT_CHUNK);
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += 256) atomicAdd(&s_cnt[key_bin(keys[k], bits)], 1u);
+  __syncthreads();
+  // the block's share of each bin: s_cnt becomes its first position there
+  for (uint32_t b = threadIdx.x; b < nbins; b += 256) {
+    const uint32_t c = s_cnt[b];
+    s_cnt[b] = c ? atomicAdd(&cursor[b], c) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const uint32_t pos = atomicAdd(&s_cnt[key_bin(keys[k], bits)], 1u);
+    out[pos] = vals[k];
+  }
+}
+
+hipError_t launch_bin_sort(const uint32_t *keys, const uint32_t *vals, uint32_t n, uint32_t bits, uint32_t *hist,
+                           uint32_t *out, hipStream_t s) {
+  if (bits < 4 || bits > WF_SORT_BITS_MAX) return hipErrorInvalidValue;
+  const uint32_t nbins = (1u << bits) + 1u;
+  const size_t lds = (size_t)nbins * 4;
+  const uint32_t blocks = std::max<uint32_t>(1, (n + BIN_SORT_CHUNK - 1) / BIN_SORT_CHUNK);
+  hipLaunchKernelGGL(k_bin_zero, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
+  hipLaunchKernelGGL(k_bin_hist, dim3(blocks), dim3(256), lds, s, keys, n, bits, hist);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, hist, nbins);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(blocks), dim3(256), lds, s, keys, vals, n, bits, hist, out);
+  return hipGetLastError();
+}
+
 size_t wf_sort_temp_bytes(uint32_t n) {
   size_t bytes = 0;
   (void)rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
                                   (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
-  return bytes;
+  return std::max<size_t>(bytes, ((size_t)(1u << WF_SORT_BITS_MAX) + 1u) * 4);
 }
 
 // One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
-                                  uint32_t refill, uint32_t slds, uint32_t *keys_sorted, uint32_t *q_sorted,
-                                  void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext, hipStream_t s) {
+                                  uint32_t refill, uint32_t slds, uint32_t sort_bits, uint32_t *keys_sorted,
+                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext,
+                                  hipStream_t s) {
   if (refill < 1 || refill > 64 || (slds != 4 && slds != 8 && slds != 16)) return hipErrorInvalidValue;
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
@@ -569,8 +654,13 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
     hipLaunchKernelGGL(trace_shadow, dim3(gs), dim3(256), 0, s, W, refill);
     hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
     if (b + 1 == max_depth) break;
-    size_t tb = sort_temp_bytes;
-    hipError_t e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
+    hipError_t e;
+    if (sort_bits) {
+      e = launch_bin_sort(W.key_next, W.q_next, W.n, sort_bits, (uint32_t *)sort_temp, q_sorted, s);
+    } else {
+      size_t tb = sort_temp_bytes;
+      e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
+    }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(1), 0, s, W.count);
     W.q = q_sorted;

@@ -458,6 +458,8 @@ napi_value Render(napi_env env, napi_callback_info info) {
     o.stack_lds = (uint32_t)num(env, argv[2], "stackLds", 0);
     o.queues = (uint32_t)num(env, argv[2], "queues", 0);
     o.graph = (uint32_t)num(env, argv[2], "graph", 0);
+    o.batch = (uint32_t)num(env, argv[2], "batch", 0);
+    o.sort_bits = (uint32_t)num(env, argv[2], "sortBits", 0);
   }
   int rc = vt_render(c, slots.data(), len, &o);
   if (rc) return throw_vt(env, c, rc, "vt_render");

@@ -77,13 +77,15 @@ class Camera(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("mode", u32), ("spp_begin", u32), ("spp_end", u32), ("accumulate", u32), ("chunk", u32),
-                ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32), ("batch", u32)]
+                ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32), ("batch", u32),
+                ("sort_bits", u32)]
 
 
 class TuneResult(C.Structure):
     _fields_ = [("block", u32), ("stack_lds", u32), ("chunk", u32), ("queues", u32), ("candidates", u32),
                 ("probe_mrays_best", f64), ("probe_mrays_default", f64), ("probe_fraction", f64), ("mode", u32),
-                ("wf_batch", u32), ("wf_stack_lds", u32), ("wf_refill", u32), ("probe_mrays_wavefront", f64)]
+                ("wf_batch", u32), ("wf_stack_lds", u32), ("wf_refill", u32), ("probe_mrays_wavefront", f64),
+                ("wf_sort_bits", u32)]
 
 
 class SensorParams(C.Structure):
@@ -435,10 +437,10 @@ class Context:
         return out[: pts.shape[0]].astype(bool)
 
     def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,
-               stack_lds=0, queues=0, graph=False, batch=0):
+               stack_lds=0, queues=0, graph=False, batch=0, sort_bits=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
         o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block, stack_lds, queues,
-                       1 if graph else 0, batch)
+                       1 if graph else 0, batch, sort_bits)
         self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
 
     def copy_bandwidth(self, nbytes, iters=50):
@@ -448,7 +450,7 @@ class Context:
 
     def autotune(self, slots, spp_begin=0, spp_end=0, probe_stride=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
-        o = RenderOpts(VT_RENDER_BATCHED, spp_begin, spp_end, 0, 0, 0, 0, 0, 0, 0)
+        o = RenderOpts(VT_RENDER_BATCHED, spp_begin, spp_end, 0, 0, 0, 0, 0, 0, 0, 0)
         r = TuneResult()
         self.check(self.L.vt_autotune(self.h, ptr(arr), arr.size, C.byref(o), probe_stride, C.byref(r)))
         return {k: getattr(r, k) for k, _ in TuneResult._fields_}

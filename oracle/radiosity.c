@@ -7,6 +7,10 @@
  * (gatherDir :271-278), FP64 arithmetic on the FP32 tables. Pinned by the uniform-field identity stated at
  * kernels.wgsl.js:414-415 (aux = pi L_in for a uniform incident field with Lambertian tables) and by the
  * fixed point of a two-node exchange solved in closed form.
+ * or_baked_occ / or_baked_jacobi: the lite renderer's gatherOcc and baked Jacobi (lite/shaders.js:234-283).
+ * Pinned by a bounded numerical minimiser of the segment distance, capsules placed by hand on two facing planes
+ * of nodes, the first step E_dir / pi, the closed box's fixed point e / (pi (1 - rho)) and a GMRES solve of the
+ * linear system on the twin's real baked table.
  */
 #include "vt_oracle.h"
 #include <math.h>
@@ -79,7 +83,10 @@ static void alb(const float *spec, const float *diff, int bins, uint32_t m, doub
 void or_jacobi_gather(uint32_t n_active, const uint32_t *active, const uint32_t *gtable, uint32_t K,
                       const float *verts, const uint32_t *node_mat, const float *alb_spec, const float *alb_diff,
                       int bins, const float *hh, const float *node_state, int slot_in, float *aux) {
-  for (uint32_t k = 0; k < n_active; k++) {
+  /* nodes are independent (each writes its own aux entry): spread over the OpenMP default team */
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t kk = 0; kk < (int64_t)n_active; kk++) {
+    const uint32_t k = (uint32_t)kk;
     uint32_t i = active[k];
     const float *nf = verts + 8 * (size_t)i + 4;
     uint32_t mi = node_mat[i];

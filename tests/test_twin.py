@@ -307,6 +307,73 @@ def test_gpu_twin_reference_estimator_matches_oracle(loaded, prep):
 
 @pytest.mark.gpu
 @needs_assets
+def test_gpu_twin_node_cache_matches_oracle(loaded, prep):
+    """The twin's node cache on the device against the oracle, stage by stage (each oracle stage fed the device's
+    previous one), bodies and the light on the scope at the default state: the gather table rows and miss counts
+    of 3,000 seeded static nodes (the oracle traces only those), the direct light of 6,000 seeded active nodes, and
+    8 Jacobi steps over every active node."""
+    scene, _, _ = loaded
+    cams, poses = _twin_frame(prep, scene, 1, 1)
+    K = 64
+    ctx = lib.Context(0)
+    try:
+        ctx.upload_scene(prep)
+        ctx.upload_profiles(prep)
+        inst = twin.instances(prep, poses)
+        ctx.set_instances(inst)
+        print("twin node cache: build", flush=True)
+        info = ctx.radiosity_build(K)
+        V, A = info["nodes"], info["active"]
+        n_mat = np.asarray(prep["materials"]).size // 16
+        g = ctx.radiosity_tables(0, V * K, np.uint32)
+        miss = ctx.radiosity_tables(1, V, np.float32)
+        act = ctx.radiosity_tables(2, A, np.uint32)
+        nb = ctx.radiosity_tables(3, (A, 16), np.uint32)
+        nw = ctx.radiosity_tables(4, (A, 16), np.float32)
+        alb = ctx.radiosity_tables(5, (n_mat, 32, 8), np.float32)
+        hh = ctx.radiosity_tables(6, (n_mat, 4), np.float32)
+        ctx.radiosity_update(direct=True, iters=0)
+        st_gpu, _ = ctx.radiosity_state(V)
+        ctx.radiosity_update(direct=False, iters=8)
+        st_gpu8, e_gpu = ctx.radiosity_state(V)
+    finally:
+        ctx.close()
+    sc = dict(prep, poses=inst)
+    ni = oracle.node_inputs(sc)
+    rng = np.random.default_rng(77)
+    stat = np.nonzero(ni["static"])[0]
+    sub = np.sort(rng.choice(stat, 3000, replace=False))
+    npos = ni["npos"].copy()
+    keep = np.zeros(V, bool)
+    keep[sub] = True
+    npos[~keep, 3] = 0.0                     # the oracle traces the sampled nodes' gather rays only
+    nis = dict(ni, npos=npos)
+    g0, miss0 = oracle.gather_table(sc, nis, K, nthreads=16)
+    g1 = oracle.gather_remap(sc, nis, K, g0, miss, nthreads=16)      # hit nodes' activity from the device
+    rows = g.reshape(V, K)[sub], g1.reshape(V, K)[sub]
+    same_miss, same_g = np.mean(miss[sub] == miss0[sub]), np.mean(rows[0] == rows[1])
+    print("gather: %d nodes, miss counts equal %.4f, entries equal %.4f" % (sub.size, same_miss, same_g))
+    assert same_miss > 0.99 and same_g > 0.995
+    asub = np.sort(rng.choice(act, 6000, replace=False))
+    so = np.zeros_like(st_gpu)
+    oracle.node_direct(sc, ni, asub, 4, alb, so, nthreads=16)
+    B, Bo = st_gpu[asub, 12:15].astype(np.float64), so[asub, 12:15].astype(np.float64)
+    rel = np.linalg.norm(B - Bo) / np.linalg.norm(Bo)
+    codes = np.mean(st_gpu[asub, 15] == so[asub, 15])
+    print("node direct: %d nodes, B_dir rel err %.2e, visibility codes equal %.4f" % (asub.size, rel, codes))
+    assert np.linalg.norm(Bo) > 0 and rel <= 1e-3 and codes > 0.995
+    st2 = st_gpu.copy()
+    slot = oracle.jacobi_steps(sc, ni, act, g, K, nb, nw, alb, hh, st2, 1, 8)
+    for k in (0, slot):
+        x, y = st_gpu8[act, 4 * k:4 * k + 3].astype(np.float64), st2[act, 4 * k:4 * k + 3].astype(np.float64)
+        r = np.linalg.norm(x - y) / np.linalg.norm(y)
+        print("Jacobi 8 steps, state word %d: rel err %.2e over %d active nodes" % (k, r, act.size))
+        assert r <= 1e-5
+    assert np.allclose(e_gpu, oracle.e_avg(ni, act, st2, hh), rtol=1e-5)
+
+
+@pytest.mark.gpu
+@needs_assets
 @pytest.mark.parametrize("caches", [False, True])
 def test_gpu_twin_judge_preset_matches_oracle(loaded, prep, caches):
     """The reference's judge preset on the twin (1 sub-sample x 4 light samples, then 6 refinement sub-samples on

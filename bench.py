@@ -133,6 +133,18 @@ def bench_c3(args, ctx=None, sc=None, steps=None, quiet=False):
         t2 = time.perf_counter()
         res[name] = dict(ms_per_frame=1e3 * (t2 - t1) / steps, mrays=rays / (t2 - t1) / 1e6, launches=st["launches"])
         log("%s: %.1f ms/frame, %.1f Mrays/s" % (name, res[name]["ms_per_frame"], res[name]["mrays"]))
+    # each camera alone (its own launch): its time and rays. The GPU's throughput when one launch fills it is the
+    # best of these; no batching can beat all rays / that throughput, which bounds the batched speed-up
+    alone = []
+    for k in slots:
+        ctx.render([k], mode=lib.VT_RENDER_PER_CAMERA)
+        ctx.sync()
+        ctx.render([k], mode=lib.VT_RENDER_PER_CAMERA)
+        st = ctx.stats()
+        alone.append((st["ms_render"], st["rays"]))
+    r_full = max(r / (m * 1e-3) for m, r in alone if m > 0)
+    t_sum = sum(m for m, _ in alone)
+    t_floor = sum(r for _, r in alone) / r_full * 1e3
     # identical images from the three forms
     imgs = []
     for mode in (lib.VT_RENDER_BATCHED, lib.VT_RENDER_PER_CAMERA, lib.VT_RENDER_WAVEFRONT):
@@ -150,7 +162,10 @@ def bench_c3(args, ctx=None, sc=None, steps=None, quiet=False):
            "spp_mpix_per_s": round(spp_px / (res["wavefront"]["ms_per_frame"] * 1e-3) / 1e6, 2),
            "images_identical": bool(np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])),
            "autotune": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()},
-           "cameras": [[c["width"], c["height"], c["spp"]] for c in cams]}
+           "cameras": [[c["width"], c["height"], c["spp"]] for c in cams],
+           "alone_ms": [round(m, 3) for m, _ in alone], "alone_mrays_s": [round(r / (m * 1e-3) / 1e6, 1) for m, r in alone],
+           "sum_alone_ms": round(t_sum, 2), "floor_ms_at_best_throughput": round(t_floor, 2),
+           "batching_speedup_bound": round(t_sum / t_floor, 3)}
     if not own:
         return out
     print(json.dumps(out), flush=True)
@@ -453,12 +468,29 @@ def bench_twin_preset(args, preset):
                chroma420=1)
     slots = list(range(len(cams)))
 
+    solved = [twin.instances(prep, twin.poses_from_state(scene, state))]
+    kinds = {}
+
+    def moved_mask(inst):
+        """bodies whose pose record differs from the last node update's (_movedSinceSolve)"""
+        m = 0
+        for b, (x, y) in enumerate(zip(inst, solved[0])):
+            same = (x is None and y is None) or (x is not None and y is not None and all(
+                np.array_equal(np.asarray(x[k]), np.asarray(y[k])) for k in ("q0", "q1", "p0", "p1") if k in x))
+            if not same:
+                m |= 1 << b
+        return m
+
     def frame(i):
         st = dict(state, instruments=[dict(q, yaw=q["yaw"] + 0.002 * i, jaw=q["jaw"] + 0.01 * (i % 5))
                                       for q in state["instruments"]])
-        ctx.set_instances(twin.instances(prep, twin.poses_from_state(scene, st)))
+        inst = twin.instances(prep, twin.poses_from_state(scene, st))
+        ctx.set_instances(inst)
         if i % P["every"] == 0:
-            ctx.radiosity_update(direct=True, iters=P["iters"])
+            # the reference's policy: only tools moved, so the node pass relights the nodes near them
+            kind = ctx.radiosity_update_moved(moved_mask(inst), iters=P["iters"])
+            kinds[kind] = kinds.get(kind, 0) + 1
+            solved[0] = inst
         ctx.render(slots, mode=lib.VT_RENDER_BATCHED)
         for k in slots:
             ctx.sensor(k, dict(sens, frame=i, cam_index=k))
@@ -479,7 +511,7 @@ def bench_twin_preset(args, preset):
     print(json.dumps({"metric": METRIC, "config": {"workload": "reference '%s' preset on the real twin scene: 3 KB4 "
                                                                "cameras 1280x720, reference estimator, spp %d, %d light "
                                                                "samples, refine %d x %d light samples, shadow culling%s, interior tree, "
-                                                               "tools moving, node radiosity (direct + %d Jacobi steps) "
+                                                               "tools moving, node radiosity (incremental direct + %d Jacobi steps) "
                                                                "every %s frame, sensor + ISP"
                                                    % (preset, P["spp"], P["nl"], P["refine"], P["nl_refine"],
                                                       "" if args.no_caches else ", fixed-camera hit and occlusion caches",
@@ -488,6 +520,7 @@ def bench_twin_preset(args, preset):
                       "refined_pixel_fraction": round((st["samples"] - px * P["spp"]) / max(1, P["refine"]) / px, 4)
                       if P["refine"] else 0.0,
                       "nodes": info["nodes"], "active_nodes": info["active"], "node_build_s": round(t_build, 2),
+                      "node_updates": kinds,
                       "interior_static_tris": interior,
                       "last_render_ms": round(st["ms_render"], 3), "last_isp_ms": round(st["ms_isp"], 3),
                       "reference_published_ms_m4_air": P["published_ms"]}), flush=True)
@@ -726,7 +759,8 @@ def extras(args, ctx, sc, out):
     out["batched_over_per_camera"] = c3["speedup_batched_over_per_camera"]
     out["c3"] = {k: c3[k] for k in ("batched_ms_per_frame", "wavefront_ms_per_frame", "per_camera_ms_per_frame",
                                     "speedup_batched_over_per_camera", "speedup_wavefront_over_per_camera",
-                                    "spp_mpix_per_s", "images_identical")}
+                                    "spp_mpix_per_s", "images_identical", "sum_alone_ms",
+                                    "floor_ms_at_best_throughput", "batching_speedup_bound")}
     log("c3: batched / per-camera %.3f (%.1f s)" % (c3["speedup_batched_over_per_camera"], time.time() - t0))
     t0 = time.time()
     c4 = bench_c4(args, quiet=True)

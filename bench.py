@@ -383,6 +383,7 @@ def bench_twin(args):
     ctx.sync()
     t_up = time.time() - t0
     slots = list(range(len(cams)))
+    args.mode = args.mode or "auto"
     mode = {"auto": lib.VT_RENDER_AUTO, "wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
             "per-camera": lib.VT_RENDER_PER_CAMERA}[args.mode]
     tuned = ctx.autotune(slots) if args.mode == "auto" else None
@@ -553,9 +554,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["auto", "wavefront", "batched", "per-camera"], default="auto",
-                    help="auto (default): the autotuner times both forms on a probe of the frame (untimed) and the "
-                         "faster runs; wavefront: per-vertex kernels with sorted rays; batched: one persistent launch")
+    ap.add_argument("--mode", choices=["auto", "wavefront", "batched", "per-camera"], default=None,
+                    help="wavefront (the default of the headline frame): per-vertex kernels with sorted rays, at the shape the committed "
+                         "counter profile was taken at (16,777,216-path batches, 8-entry LDS stack, refill 16); "
+                         "auto (the default of --config twin): the autotuner times both forms on a probe of the "
+                         "frame (untimed) and the faster runs; batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the adjoint, batching and dense-radiosity legs of the default run (profiling runs)")
@@ -603,6 +606,7 @@ def main():
         ctx.set_camera(k, c)
     slots = list(range(len(cams)))
     sensor = [scenes.default_sensor(cfa=0, frame=0, seed=1006) for _ in cams]
+    args.mode = args.mode or "wavefront"
     mode = {"auto": lib.VT_RENDER_AUTO, "wavefront": lib.VT_RENDER_WAVEFRONT, "batched": lib.VT_RENDER_BATCHED,
             "per-camera": lib.VT_RENDER_PER_CAMERA}[args.mode]
     tuned = None

@@ -373,10 +373,14 @@ struct Hit {
   uint32_t inst;     // posed body of the hit, NO_INST for the static scene
 };
 
-// per-lane traversal stack: SLDS entries in LDS, the rest spilled to a per-lane global area
+// per-lane traversal stack: SLDS entries in LDS, the rest spilled to a per-lane global area. The LDS arrays are
+// held as LDS-address-space pointers: as generic pointers, the branch between an LDS and a spill read is merged
+// into one generic (flat) load, which waits on every outstanding memory access of the wave.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) float lds_f32;
 struct Stack {
-  uint32_t *lds_node;   // base of this block's node-index array, [slot * BLOCK + tid]
-  float *lds_t;
+  lds_u32 *lds_node;    // base of this block's node-index array, [slot * BLOCK + tid]
+  lds_f32 *lds_t;
   uint2 *spill;         // [(slot - SLDS) * spill_stride + gtid]
   uint32_t tid, block, slds, spill_depth;
   uint64_t spill_stride, gtid;
@@ -457,9 +461,97 @@ __device__ __forceinline__ bool tri_hit(const RayW &R, v3 p0, v3 p1, v3 p2, floa
   return true;
 }
 
+// The triangles of one leaf in slot order. Returns true when ANY stops at a hit. (Loading two triangles before
+// either test measured the same on the full-scale frame and spilled registers, so the loop stays one at a time.)
+template <bool ANY>
+__device__ __forceinline__ bool leaf_test(const float4 *tri, const RayW &R, uint32_t leaf, float tmin, float &best,
+                                          Hit &h, bool &found) {
+  const uint32_t first = leaf & 0x03ffffffu, end = first + ((leaf >> 26) & 31u) + 1u;
+  for (uint32_t t = first; t < end; t++) {
+    const float4 *tp = tri + 3 * (size_t)t;
+    const float4 a = tp[0], b = tp[1], c = tp[2];
+    float tt, u, v;
+    if (tri_hit(R, ld3(a), ld3(b), ld3(c), tmin, best, tt, u, v)) {
+      best = tt;
+      h.t = tt; h.u = u; h.v = v; h.tri = t;
+      found = true;
+      if (ANY) return true;
+    }
+  }
+  return false;
+}
+
 __device__ __forceinline__ v3 safe_inv(v3 d) {
   return mk(1.0f / copysignf(fmaxf(fabsf(d.x), 1e-20f), d.x), 1.0f / copysignf(fmaxf(fabsf(d.y), 1e-20f), d.y),
             1.0f / copysignf(fmaxf(fabsf(d.z), 1e-20f), d.z));
+}
+
+// Per-ray constants of the four-wide slab test. Along each axis the near plane is the lo plane where the
+// direction is >= 0 (inv > 0) and the hi plane otherwise; fma(x, inv, oi) is monotone in x for a fixed inv, so
+// the near plane's distance is exactly min(t_lo, t_hi) and the far plane's max(t_lo, t_hi): each child's entry
+// and exit are a max3 and a min3 of plane distances, with no min/max per axis. An empty child (lo = +inf,
+// hi = -inf) gives entry +inf > exit -inf; its ref is tested all the same, which keeps the ref load issued with
+// the plane loads rather than after the test (a second round trip to memory per node). `near` packs the byte offsets of
+// the near plane arrays in a Node4, one per byte (x: 0 or 16, y: 32 or 48, z: 64 or 80); a far plane array is at
+// its near one ^ 16. One register instead of three: the trace kernels are bound by memory latency, so resident
+// waves count for more than the three bit-field extracts per node.
+struct Slab {
+  v3 inv, oi;
+  uint32_t near;
+};
+__device__ __forceinline__ Slab make_slab(v3 o, v3 d) {
+  Slab R;
+  R.inv = safe_inv(d);
+  R.oi = mk(-o.x * R.inv.x, -o.y * R.inv.y, -o.z * R.inv.z);
+  R.near = (R.inv.x > 0.0f ? 0u : 16u) | (R.inv.y > 0.0f ? 32u : 48u) << 8 | (R.inv.z > 0.0f ? 64u : 80u) << 16;
+  return R;
+}
+// The node array as a buffer resource: a node's seven 16-B loads then take one 32-bit lane offset each
+// (ref * 128 + the plane's offset) instead of a 64-bit address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_rsrc(const Node4 *nodes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<Node4 *>(nodes), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ float4 ld_node16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+// entry distances of the four children of node `ref` in ascending order (INFINITY: missed, or entered past
+// tfar) and their refs
+__device__ __forceinline__ void node4_test(__amdgpu_buffer_rsrc_t nodes, uint32_t ref, const Slab &R, float tmin,
+                                           float tfar, float tn[4], uint32_t rr[4]) {
+  const uint32_t b = ref << 7;
+  const uint32_t ox = R.near & 0xffu, oy = (R.near >> 8) & 0xffu, oz = R.near >> 16;
+  const float4 nx = ld_node16(nodes, b + ox), fx = ld_node16(nodes, b + (ox ^ 16u));
+  const float4 ny = ld_node16(nodes, b + oy), fy = ld_node16(nodes, b + (oy ^ 16u));
+  const float4 nz = ld_node16(nodes, b + oz), fz = ld_node16(nodes, b + (oz ^ 16u));
+  const uint4 rf = __builtin_bit_cast(uint4, ld_node16(nodes, b + 96u));
+  uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+  const uint32_t rref[4] = {r0, r1, r2, r3};
+  const float pnx[4] = {nx.x, nx.y, nx.z, nx.w}, pfx[4] = {fx.x, fx.y, fx.z, fx.w};
+  const float pny[4] = {ny.x, ny.y, ny.z, ny.w}, pfy[4] = {fy.x, fy.y, fy.z, fy.w};
+  const float pnz[4] = {nz.x, nz.y, nz.z, nz.w}, pfz[4] = {fz.x, fz.y, fz.z, fz.w};
+  float t[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const float a = fmaxf(fmaxf(fmaf(pnx[c], R.inv.x, R.oi.x), fmaf(pny[c], R.inv.y, R.oi.y)),
+                          fmaxf(fmaf(pnz[c], R.inv.z, R.oi.z), tmin));
+    const float e = fminf(fminf(fmaf(pfx[c], R.inv.x, R.oi.x), fmaf(pfy[c], R.inv.y, R.oi.y)),
+                          fminf(fmaf(pfz[c], R.inv.z, R.oi.z), tfar));
+    t[c] = a <= e && rref[c] != EMPTY_REF ? a : INFINITY;
+  }
+  float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+  // ascending (t, ref): a five-comparator network of selects (a stable order: equal entries keep theirs)
+#define VT_CSWAP(ti, tj, ri, rj)                                          \
+  {                                                                       \
+    const bool s = tj < ti;                                               \
+    const float lo = s ? tj : ti, hi = s ? ti : tj;                       \
+    const uint32_t rlo = s ? rj : ri, rhi = s ? ri : rj;                  \
+    ti = lo; tj = hi; ri = rlo; rj = rhi;                                 \
+  }
+  VT_CSWAP(t0, t1, r0, r1) VT_CSWAP(t2, t3, r2, r3) VT_CSWAP(t0, t2, r0, r2) VT_CSWAP(t1, t3, r1, r3)
+  VT_CSWAP(t1, t2, r1, r2)
+#undef VT_CSWAP
+  tn[0] = t0; tn[1] = t1; tn[2] = t2; tn[3] = t3;
+  rr[0] = r0; rr[1] = r1; rr[2] = r2; rr[3] = r3;
 }
 
 // Closest (any_hit = false) or any hit in (tmin, tmax), "while-while" with postponed leaves (Aila and Laine,
@@ -484,8 +576,8 @@ __device__ __forceinline__ uint32_t pop_next(const Stack &st, uint32_t &sp, floa
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const SceneDev &S, uint32_t root, v3 o, v3 d, float tmin, float tmax, Hit &h,
                                          Stack &st) {
-  const v3 inv = safe_inv(d);
-  const v3 oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
+  const Slab B = make_slab(o, d);
+  const __amdgpu_buffer_rsrc_t NR = node_rsrc(S.nodes);
   const RayW R = ray_w(o, d);
   float best = tmax;
   bool found = false;
@@ -495,32 +587,9 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, uint32_t root, v3 o,
   while (ref != TRAV_DONE || leaf != TRAV_DONE) {
     // node phase
     while (ref != TRAV_DONE && !(ref & LEAF_BIT)) {
-      const float4 *np = reinterpret_cast<const float4 *>(S.nodes + ref);
-      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-      const uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
       float tn[4];
-      uint32_t rr[4] = {rf.x, rf.y, rf.z, rf.w};
-      const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
-      const float tfar_lim = best * 1.00000024f;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
-        const float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
-        const float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
-        const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-        const float b = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-        tn[c] = (a <= b && rr[c] != EMPTY_REF) ? a : INFINITY;
-      }
-      // sort the four (t, ref) pairs ascending: a five-comparator network
-#define VT_CSWAP(i, j)                                                   \
-  if (tn[j] < tn[i]) {                                                   \
-    float ft = tn[i]; tn[i] = tn[j]; tn[j] = ft;                         \
-    uint32_t fr = rr[i]; rr[i] = rr[j]; rr[j] = fr;                      \
-  }
-      VT_CSWAP(0, 1) VT_CSWAP(2, 3) VT_CSWAP(0, 2) VT_CSWAP(1, 3) VT_CSWAP(1, 2)
-#undef VT_CSWAP
+      uint32_t rr[4];
+      node4_test(NR, ref, B, tmin, best * 1.00000024f, tn, rr);
       if (tn[0] < INFINITY) {
         if (tn[3] < INFINITY) push(st, sp, rr[3], tn[3]);
         if (tn[2] < INFINITY) push(st, sp, rr[2], tn[2]);
@@ -538,18 +607,7 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, uint32_t root, v3 o,
     }
     // triangle phase: the postponed leaf, then any leaf the node phase stopped on
     while (leaf != TRAV_DONE) {
-      const uint32_t first = leaf & 0x03ffffffu, cnt = ((leaf >> 26) & 31u) + 1u;
-      for (uint32_t k = first; k < first + cnt; k++) {
-        const float4 *tp = S.tri + 3 * (size_t)k;
-        const float4 a = tp[0], b = tp[1], c = tp[2];
-        float t, u, v;
-        if (tri_hit(R, ld3(a), ld3(b), ld3(c), tmin, best, t, u, v)) {
-          best = t;
-          h.t = t; h.u = u; h.v = v; h.tri = k;
-          found = true;
-          if (ANY) break;
-        }
-      }
+      leaf_test<ANY>(S.tri, R, leaf, tmin, best, h, found);
       if (ANY && found) {
         leaf = TRAV_DONE;
         ref = TRAV_DONE;

@@ -252,8 +252,8 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
     depth = std::max(depth, c->h_cams[a].depth);
   }
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-  // per path 84 B of state, per queue slot 36 B, per shadow slot 48 B
-  const size_t bytes = al(P * 4) + 5 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) + 256;
+  // per path 80 B of state, per queue slot 36 B, per shadow slot 48 B
+  const size_t bytes = al(P * 64) + 2 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) + 256;
   int rc;
   if (P > c->wf_paths || nl > c->wf_nl) {
     if ((rc = ensure(c, c->wf, bytes))) return rc;
@@ -272,10 +272,7 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   W.cams = (const DevCam *)c->dcams.p;
   W.ncam = n;
   W.nl_max = nl;
-  W.cam = (uint32_t *)take(P * 4);
-  W.o = (float4 *)take(P * 16);
-  W.d = (float4 *)take(P * 16);
-  W.thr = (float4 *)take(P * 16);
+  W.ps = (float4 *)take(P * 64);
   W.thr_slot = (float4 *)take(P * 16);
   W.L = (float4 *)take(P * 16);
   W.q_next = (uint32_t *)take(P * 4);

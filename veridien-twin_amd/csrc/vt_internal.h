@@ -73,11 +73,11 @@ struct WfLaunch {
   uint32_t bounce;
   uint64_t wave_stride;        // queue wave w is frame wave w * wave_stride (1; > 1 for tuning probes)
   uint32_t nl_max;             // shadow slots per queue slot (the largest light_samples of the frame)
-  // path state, by path id (0..n)
-  uint32_t *cam;               // camera of the path (index in cams)
-  float4 *o;                   // next ray origin; w: pdf of the BSDF sample that chose it
-  float4 *d;                   // next ray direction
-  float4 *thr;                 // throughput (w: alive); the shade kernel moves it past the vertex's continuation
+  // path state, by path id (0..n): one 64-B record per path, so a path's state is one line wherever the sorted
+  // queue sends it: [0] next ray origin, w: pdf of the BSDF sample that chose it; [1] next ray direction, w: the
+  // sample's time in the exposure; [2] throughput, w: alive (the shade kernel moves it past the vertex's
+  // continuation); [3] x: camera of the path (index in cams, bits)
+  float4 *ps;
   float4 *L;                   // radiance so far
   // queue of live paths, by queue slot (0..count[0])
   const uint32_t *q;           // path ids (null: identity, vertex 1)

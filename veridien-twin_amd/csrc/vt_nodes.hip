@@ -89,7 +89,7 @@ __device__ __forceinline__ uint32_t gather_hit(const NodeLaunch &L, uint32_t i, 
 #define NODE_STACK(L)                                                                                          \
   __shared__ uint32_t s_node[STACK_LDS * 256];                                                                 \
   __shared__ float s_t[STACK_LDS * 256];                                                                       \
-  Stack st{s_node, s_t, L.spill, threadIdx.x, 256, STACK_LDS, STACK_SPILL, L.spill_stride,                     \
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, L.spill, threadIdx.x, 256, STACK_LDS, STACK_SPILL, L.spill_stride,                     \
            (uint64_t)blockIdx.x * 256 + threadIdx.x, L.overflow}
 
 __global__ __launch_bounds__(256) void k_gather_build(NodeLaunch L) {

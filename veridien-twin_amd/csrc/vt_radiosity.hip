@@ -490,7 +490,7 @@ __global__ __launch_bounds__(DEFAULT_BLOCK) void k_formfactor(SceneDev S, uint32
                                                               float *F) {
   __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
   __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
-  Stack st{s_node, s_t, spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, spill_stride,
            (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, overflow};
   const uint64_t total = (uint64_t)n * n;
   for (uint64_t q = (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x; q < total;

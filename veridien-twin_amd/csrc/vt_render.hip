@@ -911,7 +911,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_backward(RenderLaunch Lc, float *g
     for (uint32_t k = threadIdx.x; k < ng; k += BLOCK) sgrad[k] = 0.0f;
   }
   __syncthreads();
-  Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
            (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
   GradSink gs{use_lds ? sgrad : gmat, use_lds ? sgrad + 3 * n_mat : glight, v3{0.f, 0.f, 0.f}};
   for (;;) {
@@ -938,7 +938,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_camgrad(RenderLaunch Lc, float *gc
   __shared__ float scam[MAX_CAMS * CAM_PARAMS];
   for (uint32_t k = threadIdx.x; k < Lc.ncam * CAM_PARAMS; k += BLOCK) scam[k] = 0.0f;
   __syncthreads();
-  Stack st{s_node, s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, Lc.spill, threadIdx.x, BLOCK, STACK_LDS, STACK_SPILL, Lc.spill_stride,
            (uint64_t)blockIdx.x * BLOCK + threadIdx.x, Lc.overflow};
   for (;;) {
     unsigned long long base = 0;
@@ -980,8 +980,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_render(RenderLaunch Lc) {
   // the per-lane stacks: [stack_lds * BLOCK] node ids, then [stack_lds * BLOCK] entry distances
   extern __shared__ uint32_t s_stack[];
   Stack st;
-  st.lds_node = s_stack;
-  st.lds_t = reinterpret_cast<float *>(s_stack + Lc.stack_lds * BLOCK);
+  st.lds_node = (lds_u32 *)s_stack;
+  st.lds_t = (lds_f32 *)(s_stack + Lc.stack_lds * BLOCK);
   st.spill = Lc.spill;
   st.tid = threadIdx.x;
   st.block = BLOCK;
@@ -1069,8 +1069,8 @@ template <bool INST>
 __global__ __launch_bounds__(64, 4) void k_refine(RenderLaunch Lc, const uint2 *list, const uint32_t *count, uint32_t R) {
   extern __shared__ uint32_t s_stack[];
   Stack st;
-  st.lds_node = s_stack;
-  st.lds_t = reinterpret_cast<float *>(s_stack + Lc.stack_lds * 64);
+  st.lds_node = (lds_u32 *)s_stack;
+  st.lds_t = (lds_f32 *)(s_stack + Lc.stack_lds * 64);
   st.spill = Lc.spill;
   st.tid = threadIdx.x;
   st.block = 64;
@@ -1247,7 +1247,7 @@ __global__ __launch_bounds__(DEFAULT_BLOCK) void k_trace_test(TestLaunch Lt, con
                                                               int any_hit, float *hits) {
   __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
   __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
-  Stack st{s_node, s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
            (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, Lt.overflow};
   for (uint32_t k = blockIdx.x * DEFAULT_BLOCK + threadIdx.x; k < n; k += gridDim.x * DEFAULT_BLOCK) {
     const float *r = rays + 8 * (size_t)k;
@@ -1277,7 +1277,7 @@ __global__ __launch_bounds__(DEFAULT_BLOCK) void k_visible(TestLaunch Lt, float3
                                                            float time, const float *pts, uint32_t n, uint8_t *out) {
   __shared__ uint32_t s_node[STACK_LDS * DEFAULT_BLOCK];
   __shared__ float s_t[STACK_LDS * DEFAULT_BLOCK];
-  Stack st{s_node, s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, Lt.spill, threadIdx.x, DEFAULT_BLOCK, STACK_LDS, STACK_SPILL, Lt.spill_stride,
            (uint64_t)blockIdx.x * DEFAULT_BLOCK + threadIdx.x, Lt.overflow};
   const v3 o = mk(c.x, c.y, c.z);
   for (uint32_t k = blockIdx.x * DEFAULT_BLOCK + threadIdx.x; k < n; k += gridDim.x * DEFAULT_BLOCK) {

@@ -83,7 +83,6 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
     while (c + 1 < W.ncam && item >= W.cams[c + 1].item_begin) c++;
     const DevCam &C = W.cams[c];
     const ItemPix ip = item_pix(C, item);
-    W.cam[p] = c;
     v3 d = mk(0.f, 0.f, 1.f);
     float time = 0.0f;
     if (ip.valid) {
@@ -92,9 +91,11 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
       d = camera_dir(C, P, ip.x, ip.y, ip.i);
       time = sample_time(C, P, ip.i);
     }
-    W.o[p] = make_float4(C.pos[0], C.pos[1], C.pos[2], 0.0f);   // w: pdf of the BSDF sample that led here
-    W.d[p] = make_float4(d.x, d.y, d.z, time);                   // w: the sample's time in the exposure
-    W.thr[p] = make_float4(1.f, 1.f, 1.f, ip.valid ? 1.0f : 0.0f);   // w: 1 while the path is alive
+    float4 *ps = W.ps + 4 * (size_t)p;
+    ps[0] = make_float4(C.pos[0], C.pos[1], C.pos[2], 0.0f);     // w: pdf of the BSDF sample that led here
+    ps[1] = make_float4(d.x, d.y, d.z, time);                     // w: the sample's time in the exposure
+    ps[2] = make_float4(1.f, 1.f, 1.f, ip.valid ? 1.0f : 0.0f);  // w: 1 while the path is alive
+    ps[3] = make_float4(__uint_as_float(c), 0.f, 0.f, 0.f);
     W.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
@@ -106,11 +107,13 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
 // result into hit[slot]; ANY = true: shadow slots (count[0] * nl_max, head count[3]), visibility into sh_c.w.
 // SLDS: stack entries per lane kept in LDS (the rest of the STACK_TOTAL spill to global memory); a shallow
 // LDS stack lets more waves share a CU, which hides the node fetches better than the deeper stack saves.
+// Built for 6 waves per SIMD (at most 80 VGPRs; 7 and 8 spill and ran slower); the body-walking build keeps its
+// natural register count.
 template <bool ANY, int SLDS, bool INST>
-__global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
+__global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
   __shared__ uint32_t s_node[SLDS * 256];
   __shared__ float s_t[SLDS * 256];
-  Stack st{s_node, s_t, W.spill, threadIdx.x, 256, SLDS, STACK_TOTAL - SLDS, W.spill_stride,
+  Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, W.spill, threadIdx.x, 256, SLDS, STACK_TOTAL - SLDS, W.spill_stride,
            (uint64_t)blockIdx.x * 256 + threadIdx.x, W.overflow};
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t n = ANY ? W.count[0] * W.nl_max : W.count[0];
@@ -118,7 +121,9 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
   bool active = false, exhausted = false;
   uint32_t k = 0;
   // the lane's ray and traversal state
-  v3 inv = mk(0.f, 0.f, 0.f), oi = mk(0.f, 0.f, 0.f);
+  const __amdgpu_buffer_rsrc_t NR = node_rsrc(W.S.nodes);
+  Slab B;
+  B.inv = mk(0.f, 0.f, 0.f); B.oi = mk(0.f, 0.f, 0.f); B.near = 0x402000u;
   RayW R;
   R.o = mk(0.f, 0.f, 0.f); R.kx = 0; R.ky = 1; R.kz = 2; R.Sx = 0.f; R.Sy = 0.f; R.Sz = 0.f;
   float best = 0.0f, tmax0 = 0.0f, time = 0.0f;
@@ -153,18 +158,18 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
             if (INST) skip = W.S.shadow_skip;
           } else {
             const uint32_t p = W.q ? W.q[k] : k;
-            ok = W.thr[p].w > 0.0f;
-            const float4 o4 = W.o[p], d4 = W.d[p];
+            const float4 *ps = W.ps + 4 * (size_t)p;
+            ok = ps[2].w > 0.0f;
+            const float4 o4 = ps[0], d4 = ps[1];
             o = mk(o4.x, o4.y, o4.z);
             d = mk(d4.x, d4.y, d4.z);
             tmax = INFINITY;
             time = d4.w;
-            if (INST) skip = W.bounce == 0 ? W.cams[W.cam[p]].skip : 0u;
+            if (INST) skip = W.bounce == 0 ? W.cams[__float_as_uint(ps[3].x)].skip : 0u;
             if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
           }
           if (ok) {
-            inv = safe_inv(d);
-            oi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
+            B = make_slab(o, d);
             R = ray_w(o, d);
             dir = d;
             best = tmax;
@@ -187,31 +192,9 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
     if (active) {
       for (;;) {
         while (ref != TRAV_DONE && !(ref & LEAF_BIT)) {
-          const float4 *np = reinterpret_cast<const float4 *>(W.S.nodes + ref);
-          const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-          const uint4 rf = reinterpret_cast<const uint4 *>(np)[6];
           float tn[4];
-          uint32_t rr[4] = {rf.x, rf.y, rf.z, rf.w};
-          const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
-          const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
-          const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
-          const float tfar_lim = best * 1.00000024f;
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            const float x0 = fmaf(lox[c], inv.x, oi.x), x1 = fmaf(hix[c], inv.x, oi.x);
-            const float y0 = fmaf(loy[c], inv.y, oi.y), y1 = fmaf(hiy[c], inv.y, oi.y);
-            const float z0 = fmaf(loz[c], inv.z, oi.z), z1 = fmaf(hiz[c], inv.z, oi.z);
-            const float a = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-            const float bb = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tfar_lim));
-            tn[c] = (a <= bb && rr[c] != EMPTY_REF) ? a : INFINITY;
-          }
-#define VT_CSWAP(i, j)                                                   \
-  if (tn[j] < tn[i]) {                                                   \
-    float ft = tn[i]; tn[i] = tn[j]; tn[j] = ft;                         \
-    uint32_t fr = rr[i]; rr[i] = rr[j]; rr[j] = fr;                      \
-  }
-          VT_CSWAP(0, 1) VT_CSWAP(2, 3) VT_CSWAP(0, 2) VT_CSWAP(1, 3) VT_CSWAP(1, 2)
-#undef VT_CSWAP
+          uint32_t rr[4];
+          node4_test(NR, ref, B, 0.0f, best * 1.00000024f, tn, rr);
           if (tn[0] < INFINITY) {
             if (tn[3] < INFINITY) push(st, sp, rr[3], tn[3]);
             if (tn[2] < INFINITY) push(st, sp, rr[2], tn[2]);
@@ -227,18 +210,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
           if (!__any(leaf == TRAV_DONE)) break;
         }
         while (leaf != TRAV_DONE) {
-          const uint32_t first = leaf & 0x03ffffffu, cnt = ((leaf >> 26) & 31u) + 1u;
-          for (uint32_t t = first; t < first + cnt; t++) {
-            const float4 *tp = W.S.tri + 3 * (size_t)t;
-            const float4 a = tp[0], b = tp[1], c = tp[2];
-            float tt, u, v;
-            if (tri_hit(R, ld3(a), ld3(b), ld3(c), 0.0f, best, tt, u, v)) {
-              best = tt;
-              h.t = tt; h.u = u; h.v = v; h.tri = t;
-              found = true;
-              if (ANY) break;
-            }
-          }
+          leaf_test<ANY>(W.S.tri, R, leaf, 0.0f, best, h, found);
           if (ANY && found) {
             leaf = TRAV_DONE;
             ref = TRAV_DONE;
@@ -278,6 +250,7 @@ __global__ __launch_bounds__(256) void k_wf_trace_dyn(WfLaunch W, uint32_t refil
 }
 
 // one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
+// (112 VGPRs, 4 waves per SIMD: bounding it to 6 or 8 waves spills and ran 8 % and 12 % slower)
 template <bool INST>
 __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
   // every slot of the batch gets a key: live paths of this vertex theirs, the rest (and the slots past the
@@ -294,7 +267,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
     }
     if (k < n) {
       const uint32_t p = W.q ? W.q[k] : k;
-      const DevCam &C = W.cams[W.cam[p]];
+      float4 *ps = W.ps + 4 * (size_t)p;
+      const float4 ps3 = ps[3];
+      const DevCam &C = W.cams[__float_as_uint(ps3.x)];
       const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
       const float fnl = (float)nl;
       const uint32_t Dp = C.depth < 1 ? 1 : C.depth;
@@ -308,12 +283,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         PixSeeds P;
         pix_seeds(C, ip.x, ip.y, P);
         const uint32_t i = ip.i, hs = P.hs;
-        const float4 o4 = W.o[p], d4 = W.d[p];
+        const float4 o4 = ps[0], d4 = ps[1];
         const v3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
         const float pB = o4.w, time = d4.w;
-        const float4 t4 = W.thr[p];
+        const float4 t4 = ps[2];
         const v3 thr = mk(t4.x, t4.y, t4.z);
-        W.thr_slot[k] = t4;             // the gather's weight for this vertex's NEE
+        // the gather's weight for this vertex's NEE, w: 1 / light samples (the gather needs no camera record)
+        W.thr_slot[k] = make_float4(t4.x, t4.y, t4.z, 1.0f / fnl);
         const float4 *sh = W.S.tshade + 4 * (size_t)tri;
         const float4 n0 = sh[0];
         const uint4 ms = W.S.meshes[__float_as_uint(n0.w)];
@@ -412,9 +388,11 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               const v3 tn = thr * a;
               if (maxc(tn) > 0.0f) {
                 const v3 no = pt + ng * (dot(ng, l) > 0.0f ? EPS_OFF : -EPS_OFF);
-                W.thr[p] = make_float4(tn.x, tn.y, tn.z, 1.0f);
-                W.o[p] = make_float4(no.x, no.y, no.z, pdf);
-                W.d[p] = make_float4(l.x, l.y, l.z, time);
+                // the whole 64-B record, so the line is written in full
+                ps[0] = make_float4(no.x, no.y, no.z, pdf);
+                ps[1] = make_float4(l.x, l.y, l.z, time);
+                ps[2] = make_float4(tn.x, tn.y, tn.z, 1.0f);
+                ps[3] = ps3;
                 W.key_next[k] = ray_key(no, l, W);
                 W.q_next[k] = p;
                 live = true;
@@ -441,15 +419,13 @@ __global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
     const uint32_t p = W.q ? W.q[k] : k;
     const uint32_t ne = W.nee[k];
     if (ne) {
-      const DevCam &C = W.cams[W.cam[p]];
-      const uint32_t nl = C.nl < 1 ? 1 : (C.nl > 16 ? 16 : C.nl);
       v3 Ln = mk(0.f, 0.f, 0.f);
       for (uint32_t j = 0; j < ne; j++) {
         const float4 c = W.sh_c[(size_t)k * W.nl_max + j];
         if (c.w > 0.0f) Ln = Ln + mk(c.x, c.y, c.z);
       }
       const float4 t4 = W.thr_slot[k];
-      const v3 add = mk(t4.x, t4.y, t4.z) * Ln * (1.0f / (float)nl);
+      const v3 add = mk(t4.x, t4.y, t4.z) * Ln * t4.w;
       float4 Lp = W.L[p];
       Lp.x += add.x; Lp.y += add.y; Lp.z += add.z;
       W.L[p] = Lp;
@@ -473,7 +449,9 @@ __global__ __launch_bounds__(256) void k_wf_accum(WfLaunch W) {
   for (uint32_t w = (blockIdx.x * 256 + threadIdx.x) / 64; w < waves; w += gridDim.x * 4) {
     const uint32_t p = w * 64 + lane;
     const uint64_t item = path_item(W, p);
-    const DevCam &C = W.cams[W.cam[p]];
+    uint32_t c = 0;   // the camera from the item's place in the frame queue, as k_wf_gen finds it
+    while (c + 1 < W.ncam && item >= W.cams[c + 1].item_begin) c++;
+    const DevCam &C = W.cams[c];
     const ItemPix ip = item_pix(C, item);
     const float4 L4 = W.L[p];
     const v3 L = ip.valid ? mk(L4.x, L4.y, L4.z) : mk(0.f, 0.f, 0.f);

@@ -35,6 +35,9 @@ enum {
 #define VT_EMIT_WORDS 16
 #define VT_MESH_STATIC 1u
 #define VT_MESH_EMITTER 2u
+/* the mesh samples the print mask (MESH_FLAG.TEXTURED, gpu/scene_gpu.js:14, 324-325): its rho_d and GGX alpha are
+ * mixed between the ink (material words 8-11) and the paper (words 0-3) by the mask at the hit's texture coordinates */
+#define VT_MESH_TEXTURED 128u
 #define VT_MAX_CAMERAS 64
 
 typedef struct vt_ctx vt_ctx;
@@ -51,7 +54,8 @@ typedef struct {
  *   meshes     4 words per mesh: material, flags (VT_MESH_*), light index (emitters), -
  *   materials  16 floats per material: [c.rgb, alpha][ior, kind, fbar, roughness][tau.rgb, alpha_c]
  *              [fbar_computed, -, -, -]; kind 0 dielectric, 1 conductor, 2 coated conductor, 3 Lambert
- *              (gpu/bsdf.js:68-88 prepareMaterial)
+ *              (gpu/bsdf.js:68-88 prepareMaterial); a printed material (charuco_print) keeps the paper's rho_d
+ *              and alpha in words 0-3 and the ink's rho_d rgb and alpha in words 8-11 (gpu/renderer.js:293)
  *   lights     4 floats per light: radiance rgb, -
  *   emit       16 floats per emitter triangle: [v0.xyz, cdf][e1.xyz, area][e2.xyz, light (u32 bits)][n.xyz, -]
  *              (gpu/renderer.js:303-306, with the light index in place of the shared L0)
@@ -79,6 +83,10 @@ typedef struct {
    * are then in that body's frame, and every emitter triangle must belong to that body. 0: the light is static
    * scene geometry. */
   uint32_t emit_posed, emit_body;
+  /* texture coordinates, 2 floats (u, v) per vertex, or NULL; required when a mesh is VT_MESH_TEXTURED (the
+   * reference carries them in the dynamic vertices' spare words, gpu/scene_gpu.js:390-392). (0, 0) is the
+   * mask's top-left corner. */
+  const float *uvs;
 } vt_scene;
 
 #define VT_MAX_BODIES 24
@@ -282,6 +290,12 @@ int vt_update_materials(vt_ctx *ctx, const float *materials, uint32_t material_c
 #define VT_MAX_KNOTS 64
 int vt_set_light_profile(vt_ctx *ctx, uint32_t light, const float axis[3], const float *theta, const float *p,
                          uint32_t count);
+/* The printed board's mask (_loadBoardTexture, gpu/renderer.js:395-412; sampled in shadeHit,
+ * gpu/wgsl/shade.wgsl.js:258-265): width x height 8-bit texels, row 0 the image's top row, 255 = paper, 0 = ink
+ * (the PNG's red channel). Textured meshes sample it bilinearly with clamp-to-edge addressing, texel (i, j)
+ * centred at ((i + 0.5) / width, (j + 0.5) / height). Until it is set (and after vt_upload_scene) the mask is
+ * one paper texel, as the reference's 1x1 white fallback. At most 16384 x 16384. */
+int vt_set_board_texture(vt_ctx *ctx, const uint8_t *mask, uint32_t width, uint32_t height);
 int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
 
 /* ---------------------------------------------------------------- per frame */

@@ -64,6 +64,7 @@ def lib():
             "or_gauss_legendre": (None, [i32, vp, vp]),
             "or_disc_irradiance": (f64, [vp, vp, vp, vp, f64, f64, i32, i32]),
             "or_expf": (f32, [f32]),
+            "or_board_mask": (f32, [vp, u32, u32, f32, f32]),
             "or_logf": (f32, [f32]),
             "or_cosf": (f32, [f32]),
             "or_erfa": (f32, [f32]),
@@ -151,7 +152,8 @@ class Scene(C.Structure):
                 ("shadow_skip", C.c_uint32), ("light_prof", C.c_void_p), ("knots", C.c_void_p),
                 ("node_state", C.c_void_p), ("alb", C.c_void_p), ("hh", C.c_void_p), ("slot", C.c_uint32),
                 ("e_avg", C.c_float * 3), ("emit_c", C.c_float * 4), ("sample_materials", C.c_void_p),
-                ("emit_posed", C.c_uint32), ("emit_body", C.c_uint32)]
+                ("emit_posed", C.c_uint32), ("emit_body", C.c_uint32), ("uvs", C.c_void_p), ("tex", C.c_void_p),
+                ("tex_w", C.c_uint32), ("tex_h", C.c_uint32)]
 
 
 class Body(C.Structure):
@@ -302,8 +304,27 @@ def scene_struct(sc):
         keep["sample_materials"] = np.ascontiguousarray(sc["sample_materials"], np.float32)
         assert keep["sample_materials"].size == keep["materials"].size
         s.sample_materials = ptr(keep["sample_materials"])
+    if sc.get("uvs") is not None:
+        # the printed board: texture coordinates per vertex and the mask (255 = paper; one paper texel without it)
+        keep["uvs"] = np.ascontiguousarray(sc["uvs"], np.float32)
+        s.uvs = ptr(keep["uvs"])
+        if sc.get("board_mask") is not None:
+            keep["tex"] = np.ascontiguousarray(sc["board_mask"], np.uint8)
+            s.tex, s.tex_h, s.tex_w = ptr(keep["tex"]), keep["tex"].shape[0], keep["tex"].shape[1]
     s._keep = keep
     return s
+
+
+def board_mask(mask, u, v):
+    """or_board_mask at each (u, v): the bilinear clamp-to-edge sample of an (h, w) uint8 mask, in [0, 1]."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    u = np.atleast_1d(np.asarray(u, np.float32))
+    v = np.atleast_1d(np.asarray(v, np.float32))
+    out = np.empty(u.shape, np.float32)
+    f = lib().or_board_mask
+    for k in range(u.size):
+        out.flat[k] = f(ptr(m), m.shape[1], m.shape[0], float(u.flat[k]), float(v.flat[k]))
+    return out
 
 
 def _quat_mat(q):

@@ -194,6 +194,40 @@ static void surface(const or_scene *S, const or_hit *h, const pose_t *P, const d
   for (int k = 0; k < 3; k++) s->p[k] = o[k] + d[k] * h->t;
 }
 
+float or_board_mask(const uint8_t *tex, uint32_t w, uint32_t h, float u, float v) {
+  const float fw = (float)w, fh = (float)h;
+  float x = u * fw - 0.5f, y = v * fh - 0.5f;
+  x = x < -1.0f ? -1.0f : (x > fw ? fw : x);
+  y = y < -1.0f ? -1.0f : (y > fh ? fh : y);
+  const float x0 = floorf(x), y0 = floorf(y);
+  const float ax = x - x0, ay = y - y0;
+  int i0 = (int)x0, i1 = (int)x0 + 1, j0 = (int)y0, j1 = (int)y0 + 1;
+  const int lx = (int)w - 1, ly = (int)h - 1;
+  i0 = i0 < 0 ? 0 : (i0 > lx ? lx : i0); i1 = i1 < 0 ? 0 : (i1 > lx ? lx : i1);
+  j0 = j0 < 0 ? 0 : (j0 > ly ? ly : j0); j1 = j1 < 0 ? 0 : (j1 > ly ? ly : j1);
+  const float s = 1.0f / 255.0f;
+  const float t00 = (float)tex[(size_t)j0 * w + i0] * s, t10 = (float)tex[(size_t)j0 * w + i1] * s;
+  const float t01 = (float)tex[(size_t)j1 * w + i0] * s, t11 = (float)tex[(size_t)j1 * w + i1] * s;
+  const float top = t00 * (1.0f - ax) + t10 * ax, bot = t01 * (1.0f - ax) + t11 * ax;
+  return top * (1.0f - ay) + bot * ay;
+}
+
+/* the material record at a hit, mixed between ink and paper on a textured mesh (shade.wgsl.js:258-265):
+ * c = ink (1 - m) + paper m and alpha likewise, m the mask at the interpolated texture coordinates */
+static void hit_material(const or_scene *S, const surf_t *s, const float *rec, or_mat *M) {
+  or_mat_unpack(rec, M);
+  if (!(s->flags & 128u) || !S->uvs) return;
+  double uv[2] = {0, 0};
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 2; k++) uv[k] += s->b[j] * S->uvs[2 * (size_t)s->vi[j] + k];
+  static const uint8_t paper = 255;
+  const float m = S->tex ? or_board_mask(S->tex, S->tex_w, S->tex_h, (float)uv[0], (float)uv[1])
+                         : or_board_mask(&paper, 1, 1, (float)uv[0], (float)uv[1]);
+  const float k = 1.0f - m;
+  for (int c = 0; c < 3; c++) M->c[c] = (float)((float)M->tau[c] * k + (float)M->c[c] * m);
+  M->alpha = (float)((float)M->alpha_c * k + (float)M->alpha * m);
+}
+
 /* angular profile of light `light` toward the unit direction w leaving the emitter (shade.wgsl.js:48-52 emitted:
  * p(theta) with theta = acos(w . axis), or_profile over the light's knots); 1 for a Lambertian light */
 static double light_profile(const or_scene *S, uint32_t light, const double w[3]) {
@@ -443,9 +477,9 @@ static double sample_radiance_ex(const or_scene *S, const or_camera *C, const pi
       break;
     }
     or_mat M, Ms;
-    or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
+    hit_material(S, &s, S->materials + 16 * (size_t)s.mat, &M);
     Ms = M;
-    if (S->sample_materials) or_mat_unpack(S->sample_materials + 16 * (size_t)s.mat, &Ms);
+    if (S->sample_materials) hit_material(S, &s, S->sample_materials + 16 * (size_t)s.mat, &Ms);
     double wo[3] = {-d[0], -d[1], -d[2]};
     const int last = b + 1 == D;
     double Ln[3] = {0, 0, 0};
@@ -860,7 +894,7 @@ static void ref_radiance(const or_scene *S, const or_camera *C, const pix_t *P, 
     return;
   }
   or_mat M;
-  or_mat_unpack(S->materials + 16 * (size_t)s.mat, &M);
+  hit_material(S, &s, S->materials + 16 * (size_t)s.mat, &M);
   const int stat = is_static(&h, &s);
   const int mis = !stat && !(opts & O_NO_MIS);
   const uint32_t nl = nl_in < 1 ? 1 : nl_in > 16 ? 16 : nl_in;

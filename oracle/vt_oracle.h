@@ -180,7 +180,14 @@ typedef struct {
   /* emit_posed 1: the light rides on body emit_body; the emit records, the light axes and emit_c are in its frame
    * and every emitter sample is posed with it at its exposure time (shade.wgsl.js:31-47, 55) */
   uint32_t emit_posed, emit_body;
+  /* the printed board (gpu/wgsl/shade.wgsl.js:258-265): texture coordinates, 2 floats per vertex (NULL: none), and
+   * the mask, tex_w x tex_h bytes, 255 = paper (NULL: one paper texel). A mesh flagged 128 (TEXTURED) mixes rho_d
+   * and alpha between the ink (material words 8-11) and the paper by the mask at the hit. */
+  const float *uvs; const uint8_t *tex; uint32_t tex_w, tex_h;
 } or_scene;
+/* the print mask at texture coordinates (u, v), bilinear with clamp-to-edge addressing, texel (i, j) centred at
+ * ((i + 0.5) / w, (j + 0.5) / h) (a linear sampler, gpu/renderer.js:398), in binary32, 1 = paper */
+float or_board_mask(const uint8_t *tex, uint32_t w, uint32_t h, float u, float v);
 /* A posed body (the reference's BLAS, gpu/scene_gpu.js:313-356, placed by setPoses gpu/renderer.js:619-633):
  * BVH2 over its triangles in its own frame, slots as in or_scene.tris (vertex ids index or_scene.verts, whose
  * positions and normals for these vertices are in the body frame), and its pose at shutter open (q0, p0) and

@@ -14,8 +14,14 @@ async function main() {
     lights: f32('lights'), emit: f32('emit'), emitArea: man.emitArea };
   if (man.bodies) Object.assign(scene, { triBody: u32('triBody'), bodies: man.bodies });
   else scene.nodes = f32('nodes');
+  if (man.uvs) scene.uvs = f32('uvs');
+  let boardTexture;
+  if (man.boardTexture) {
+    const b = fs.readFileSync(path.join(dir, 'mask.bin'));
+    boardTexture = { mask: new Uint8Array(b.buffer, b.byteOffset, b.byteLength), width: man.boardTexture[0], height: man.boardTexture[1] };
+  }
   const r = await TwinRenderer.create({ scene, cameras: man.cameras, spp: man.spp, maxDepth: man.maxDepth, seed: man.seed,
-    estimator: man.estimator || 'path' });
+    estimator: man.estimator || 'path', boardTexture });
   if (man.poses0) r.setPoses(man.poses0, man.poses1 || null);
   const out = r.renderFrame({ sensor: true, sensorOpts: { params: man.sensor, seed: 1006, frame: 0, exposure_s: man.exposure, gain: 1 }, shutter: man.shutter });
   await out.done;

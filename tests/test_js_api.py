@@ -257,6 +257,67 @@ def test_from_prepared_bodies():
     assert d["profiles"] == [{"light": 0, "axis": [0, 0, -1], "theta": [0, 0.5], "p": [1, 0]}]
 
 
+def test_from_prepared_texture():
+    """A printed material's ink goes to words 8-11 (rho_d, roughInk^2: gpu/renderer.js:293) and the textured
+    meshes' vertices give their texture coordinates from the spare words (u in pos.w, v in normal.w)."""
+    S = ("{nodes: new Float32Array(16), tris: T, verts: V, meshTable: [{mat: 0, flags: 1}, {mat: 1, flags: 136}], "
+         "materials: [{metallic: 0, rho: [0.5, 0.5, 0.5], alpha: 0.25, roughness: 0.5}, {metallic: 0, rho: [0.86, 0.86, 0.85], "
+         "alpha: 0.36, roughness: 0.6, texture: {ink: [0.045, 0.04, 0.05], roughInk: 0.4}}]}")
+    setup = ("const T = new Float32Array(32), V = new Float32Array(48), tu = new Uint32Array(T.buffer); "
+             "tu[3] = 0; tu[7] = 1; tu[11] = 2; tu[12] = 0; tu[19] = 3; tu[23] = 4; tu[27] = 5; tu[28] = 1; "
+             "for (let v = 0; v < 6; v++) { V[8 * v + 3] = 0.1 * v; V[8 * v + 7] = 1 - 0.1 * v; } ")
+    out = _node("%simport('%s').then(m => { const u = m.fromPrepared(%s); console.log(JSON.stringify({uvs: Array.from(u.uvs), "
+                "mat: Array.from(u.materials)})); })" % (setup, MODULE, S))
+    d = json.loads(out)
+    uv = np.array(d["uvs"], np.float32).reshape(6, 2)
+    assert np.allclose(uv[:3], 0) and np.allclose(uv[3:], [[0.3, 0.7], [0.4, 0.6], [0.5, 0.5]])
+    m = np.array(d["mat"], np.float32).reshape(2, 16)
+    assert np.allclose(m[1, :4], [0.86, 0.86, 0.85, 0.36]) and np.allclose(m[1, 8:12], [0.045, 0.04, 0.05, 0.16])
+    assert np.allclose(m[0, 8:12], [1, 1, 1, 0])
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_textured_frame_equals_c_abi_frame(tmp_path):
+    """The printed boards through TwinRenderer (uvs in the scene, boardTexture at create) against the same frame
+    through the C ABI."""
+    sc = scenes.textured_scene()
+    cam = scenes.posed_camera(96, 72, spp=4)
+    for k in ("tris", "verts", "materials", "lights", "emit", "uvs"):
+        np.ascontiguousarray(sc[k], np.float32).tofile(tmp_path / (k + ".bin"))
+    np.ascontiguousarray(sc["meshes"], np.uint32).tofile(tmp_path / "meshes.bin")
+    np.ascontiguousarray(sc["tri_body"], np.uint32).tofile(tmp_path / "triBody.bin")
+    np.ascontiguousarray(sc["board_mask"], np.uint8).tofile(tmp_path / "mask.bin")
+    poses = [dict(q0=(0.0, 0.0, 0.0, 1.0), p0=(-0.55, -1.45, 1.0)), dict(q0=(0.0, 0.0, 0.0, 1.0), p0=(0.9, 0.6, 0.6))]
+
+    def T(t):
+        M = np.eye(4)
+        M[:3, 3] = t
+        return [float(x) for x in M.T.reshape(-1)]
+    sp = scenes.default_sensor()
+    man = dict(module=MODULE, emitArea=sc["emit_area"], spp=4, maxDepth=cam["max_depth"], seed=cam["seed"],
+               bodies=["box", "can"], poses0={"bodies": {"box": T(poses[0]["p0"]), "can": T(poses[1]["p0"])}},
+               uvs=True, boardTexture=[sc["board_mask"].shape[1], sc["board_mask"].shape[0]],
+               cameras=[dict(name="CAM1", lens=[float(v) for v in cam["lens"]], R=[float(v) for v in cam["R"]],
+                             pos=[float(v) for v in cam["pos"]], size=[cam["width"], cam["height"]])],
+               sensor=_js_sensor(sp), exposure=sp["exposure"])
+    (tmp_path / "manifest.json").write_text(json.dumps(man))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js_render.mjs"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    js_lin = np.fromfile(tmp_path / "CAM1.linear.bin", np.float32).reshape(cam["height"], cam["width"], 4)
+    ctx = lib.Context(0)
+    ctx.upload_scene(sc)
+    ctx.set_instances(poses)
+    cam["cam_index"] = 0
+    ctx.set_camera(0, cam)
+    ctx.render([0])
+    lin = ctx.readback(0, lib.VT_KIND_LINEAR, cam["width"], cam["height"])
+    ctx.close()
+    a, b = js_lin[:, :, :3].astype(np.float64), lin[:, :, :3].astype(np.float64)
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+
+
 @pytest.mark.gpu
 @needs_node
 def test_js_posed_frame_equals_c_abi_frame(tmp_path):

@@ -202,7 +202,7 @@ extern "C" void vt_destroy(vt_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots, &c->tvid, &c->g_pos, &c->g_miss, &c->radu, &c->g_area,
+  DevBuf *bufs[] = {&c->nodes, &c->tri, &c->tshade, &c->meshes, &c->mats, &c->lights, &c->emit, &c->inst, &c->lprof, &c->knots, &c->tvid, &c->uv, &c->tex, &c->g_pos, &c->g_miss, &c->radu, &c->g_area,
                     &c->counters, &c->overflow, &c->spill, &c->dcams, &c->test_in, &c->test_out, &c->test_aux,
                     &c->refine_list, &c->eavg_part,
                     &c->g_active, &c->g_table, &c->g_normals, &c->g_mat, &c->g_alb, &c->g_hh, &c->g_nbr, &c->g_w,
@@ -437,6 +437,24 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->S.tshade = (const float4 *)c->tshade.p;
   c->S.meshes = (const uint4 *)c->meshes.p;
   c->S.tvid = (const uint4 *)c->tvid.p;
+  {
+    // texture coordinates of textured meshes, and the print mask: one paper texel until vt_set_board_texture
+    c->has_textured = false;
+    for (uint32_t k = 0; k < s->mesh_count; k++)
+      if (s->meshes[4 * (size_t)k + 1] & VT_MESH_TEXTURED) c->has_textured = true;
+    if (c->has_textured && !s->uvs)
+      return fail(c, VT_ERR_ARG, "scene: a VT_MESH_TEXTURED mesh needs texture coordinates (vt_scene.uvs)");
+    if (s->uvs) {
+      if ((rc = upload(c, c->uv, s->uvs, (size_t)V * 8))) return rc;
+    } else {
+      release(c->uv);
+    }
+    const uint8_t paper = 255;
+    if ((rc = upload(c, c->tex, &paper, 1))) return rc;
+    c->S.uv = s->uvs ? (const float2 *)c->uv.p : nullptr;
+    c->S.tex = (const uint8_t *)c->tex.p;
+    c->S.tex_w = c->S.tex_h = 1;
+  }
   c->S.mats = (const float4 *)c->mats.p;
   c->S.lights = (const float4 *)c->lights.p;
   {
@@ -490,6 +508,20 @@ extern "C" int vt_upload_scene(vt_ctx *c, const vt_scene *s) {
   c->scene_gen++;
   VT_HIP(c, hipMemsetAsync(c->overflow.p, 0, 64, c->stream));
   VT_HIP(c, hipStreamSynchronize(c->stream));
+  return VT_OK;
+}
+
+extern "C" int vt_set_board_texture(vt_ctx *c, const uint8_t *mask, uint32_t width, uint32_t height) {
+  VT_LIVE(c);
+  if (!c->has_scene) return fail(c, VT_ERR_STATE, "board texture: no scene uploaded");
+  if (!mask || width == 0 || height == 0 || width > 16384 || height > 16384)
+    return fail(c, VT_ERR_ARG, "board texture: null mask or size outside 1..16384");
+  VT_HIP(c, hipSetDevice(c->device));
+  VT_HIP(c, hipStreamSynchronize(c->stream));
+  if (int rc = upload(c, c->tex, mask, (size_t)width * height)) return rc;
+  c->S.tex = (const uint8_t *)c->tex.p;
+  c->S.tex_w = width;
+  c->S.tex_h = height;
   return VT_OK;
 }
 

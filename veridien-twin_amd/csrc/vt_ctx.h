@@ -74,6 +74,8 @@ struct vt_ctx {
   float4 emit_rec[4];                      // that body's pose words (q0, q1, p0, p1) at the last vt_set_instances
   bool emit_rec_set = false;
   DevBuf nodes, tri, tshade, meshes, mats, lights, emit, inst, lprof, knots, tvid;
+  DevBuf uv, tex;                          // per-vertex texture coordinates (textured scenes); the print mask
+  bool has_textured = false;               // a mesh of the scene is VT_MESH_TEXTURED
   // host copies the node radiosity build reads: vertices (8 floats each), per slot vertex ids + mesh, meshes,
   // the static scene's slot count (its slots come first), the emitter records
   std::vector<float> h_verts;

@@ -13,6 +13,7 @@ constexpr float INV_PI_F = 0.318309886183791f;
 constexpr uint32_t MISS = 0xffffffffu;
 constexpr float EPS_OFF = 2e-6f;      // ray origin offset along the geometric normal (m), common.wgsl.js:33
 constexpr uint32_t F_EMITTER = 2u;
+constexpr uint32_t F_TEXTURED = 128u;   // the mesh samples the print mask (gpu/wgsl/common.wgsl.js:42)
 constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t EMPTY_REF = 0xffffffffu;
 
@@ -341,8 +342,51 @@ struct SceneDev {
   const float4 *alb;
   const float4 *hh;
   const float4 *radu;
+  // the printed board: texture coordinates per vertex (null without textured meshes) and the mask, 8 bits per
+  // texel, 255 = paper
+  const float2 *uv;
+  const uint8_t *tex;
+  uint32_t tex_w, tex_h;
 };
 constexpr uint32_t ALB_BINS = 32;
+
+// The print mask at texture coordinates (u, v): bilinear between the four nearest texel centres, texel (i, j)
+// centred at ((i + 0.5) / w, (j + 0.5) / h), clamped to the edge texels (textureSampleLevel with a linear,
+// clamp-to-edge sampler, gpu/renderer.js:398 and gpu/wgsl/shade.wgsl.js:262), in binary32 with every product
+// rounded on its own; 1 = paper.
+__device__ __forceinline__ float board_mask(const SceneDev &S, float u, float v) {
+#pragma clang fp contract(off)
+  const float fw = (float)S.tex_w, fh = (float)S.tex_h;
+  const float x = fminf(fmaxf(u * fw - 0.5f, -1.0f), fw), y = fminf(fmaxf(v * fh - 0.5f, -1.0f), fh);
+  const float x0 = floorf(x), y0 = floorf(y);
+  const float ax = x - x0, ay = y - y0;
+  const int last_x = (int)S.tex_w - 1, last_y = (int)S.tex_h - 1;
+  const int i0 = min(max((int)x0, 0), last_x), i1 = min(max((int)x0 + 1, 0), last_x);
+  const int j0 = min(max((int)y0, 0), last_y), j1 = min(max((int)y0 + 1, 0), last_y);
+  const float t00 = (float)S.tex[(size_t)j0 * S.tex_w + i0] * (1.0f / 255.0f);
+  const float t10 = (float)S.tex[(size_t)j0 * S.tex_w + i1] * (1.0f / 255.0f);
+  const float t01 = (float)S.tex[(size_t)j1 * S.tex_w + i0] * (1.0f / 255.0f);
+  const float t11 = (float)S.tex[(size_t)j1 * S.tex_w + i1] * (1.0f / 255.0f);
+  const float top = t00 * (1.0f - ax) + t10 * ax, bot = t01 * (1.0f - ax) + t11 * ax;
+  return top * (1.0f - ay) + bot * ay;
+}
+// The material at a hit of slot `tri` (barycentrics bu, bv): the mesh's record, and on a textured mesh
+// (charuco_print, gpu/wgsl/shade.wgsl.js:258-265) rho_d and alpha mixed between the ink (the record's words 8-11)
+// and the paper by the mask at the interpolated texture coordinates: c = ink (1 - m) + paper m.
+__device__ __forceinline__ Mat hit_mat(const SceneDev &S, uint4 ms, uint32_t tri, float bu, float bv) {
+#pragma clang fp contract(off)
+  Mat M = load_mat(S.mats, ms.x);
+  if (ms.y & F_TEXTURED) {
+    const uint4 vi = S.tvid[tri];
+    const float2 a = S.uv[vi.x], b = S.uv[vi.y], c = S.uv[vi.z];
+    const float w0 = 1.0f - bu - bv;
+    const float u = a.x * w0 + b.x * bu + c.x * bv, v = a.y * w0 + b.y * bu + c.y * bv;
+    const float m = board_mask(S, u, v), k = 1.0f - m;
+    M.c = mk(M.tau.x * k + M.c.x * m, M.tau.y * k + M.c.y * m, M.tau.z * k + M.c.z * m);
+    M.alpha = M.alpha_c * k + M.alpha * m;
+  }
+  return M;
+}
 
 constexpr uint32_t MAX_KNOTS = 64;
 // p(theta) of light `light` toward the unit direction w leaving the emitter, theta from the light's axis: linear

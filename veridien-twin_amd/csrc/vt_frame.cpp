@@ -789,6 +789,9 @@ extern "C" int vt_backward(vt_ctx *c, const uint32_t *cams, uint32_t n, const fl
     if (ref) return fail(c, VT_ERR_ARG, "backward: the adjoint differentiates the path tracer, not the reference estimator");
   }
   if (!c->has_scene) return fail(c, VT_ERR_STATE, "backward: no scene uploaded");
+  if (c->has_textured)
+    return fail(c, VT_ERR_ARG, "backward: printed (textured) materials are not differentiated; upload the scene without "
+                               "VT_MESH_TEXTURED meshes");
   if (!cams || !adj || !out || n == 0 || n > VT_MAX_CAMERAS) return fail(c, VT_ERR_ARG, "backward: bad arguments");
   if (out->cameras && out->camera_count != n)
     return fail(c, VT_ERR_SIZE, "backward: cameras needs %u x %d floats (one row per listed camera)", n, VT_CAMERA_PARAMS);

@@ -117,7 +117,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
           const float pe = light_profile(S, lid, emit_local(E, -wi));
           const v3 Le = ld3(S.lights[lid]) * pe;
           if (maxc(Le) <= 0.0f) continue;
-          const Mat M = load_mat(S.mats, ms.x);
+          const Mat M = hit_mat(S, ms, h.tri, h.u, h.v);
           const v3 f = eval_bsdf(M, ns, wi, wo);
           if (maxc(f) <= 0.0f) continue;
           const float pW = d2 / (cosE * A);
@@ -156,7 +156,7 @@ __device__ __forceinline__ v3 path_walk(const SceneDev &S, const DevCam &C, cons
       u1 = u01(hash3(hs, i, 256u * b + 201u));
       u2 = u01(hash3(hs, i, 256u * b + 202u));
     }
-    const Mat M = load_mat(S.mats, ms.x);
+    const Mat M = hit_mat(S, ms, h.tri, h.u, h.v);
     v3 l;
     const float pdf = sample_bsdf(M, ns, wo, u0, u1, u2, l);
     if (!(pdf > 0.0f) || dot(ng, l) <= 0.0f) break;
@@ -425,7 +425,7 @@ __device__ __forceinline__ v3 ref_radiance(const SceneDev &S, const DevCam &C, c
     if (dot(wo, s.nsi) <= 0.0f) return mk(0.f, 0.f, 0.f);
     return ld3(S.lights[s.ms.z]) * light_profile(S, s.ms.z, emit_local(emitter_pose(S, time), wo));
   }
-  const Mat M = load_mat(S.mats, s.ms.x);
+  const Mat M = hit_mat(S, s.ms, h.tri, h.u, h.v);
   const bool stat = is_static_hit(h, s);
   const bool mis = !stat && !(opts & OPT_NO_MIS);
   const uint32_t nl = nl_in < 1 ? 1 : (nl_in > 16 ? 16 : nl_in);

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               const uint32_t lid = __float_as_uint(W.S.emit[4 * e + 2].w);
               const v3 Le = ld3(W.S.lights[lid]) * light_profile(W.S, lid, emit_local(E, -wi));
               if (maxc(Le) <= 0.0f) continue;
-              const Mat M = load_mat(W.S.mats, ms.x);
+              const Mat M = hit_mat(W.S, ms, tri, h4.y, h4.z);
               const v3 f = eval_bsdf(M, ns, wi, wo);
               if (maxc(f) <= 0.0f) continue;
               const float pW = d2 / (cosE * A);
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               u1 = u01(hash3(hs, i, 256u * b + 201u));
               u2 = u01(hash3(hs, i, 256u * b + 202u));
             }
-            const Mat M = load_mat(W.S.mats, ms.x);
+            const Mat M = hit_mat(W.S, ms, tri, h4.y, h4.z);
             v3 l;
             const float pdf = sample_bsdf(M, ns, wo, u0, u1, u2, l);
             if (pdf > 0.0f && dot(ng, l) > 0.0f) {

@@ -52,6 +52,7 @@ const KIND_STATS = 6;
 // Node 12 has no `??`: the same rule as a function
 const dflt = (a, b) => (a === undefined || a === null ? b : a);
 const MESH_EMITTER = 2;
+const MESH_TEXTURED = 128;   // MESH_FLAG.TEXTURED (gpu/scene_gpu.js:14)
 
 /** gpu/lens.js:8-16: scene lens {model, K, D, size} -> [fx fy cx cy k1..k4 alpha] */
 export function lensParams(lens) {
@@ -101,10 +102,27 @@ export function fromPrepared(S, ports = []) {
   S.materials.forEach((m, i) => {
     const kind = m.coated ? 2 : m.metallic ? 1 : 0;
     const c = kind ? (m.f0 || [0.56, 0.57, 0.58]) : (m.rho || [0.5, 0.5, 0.5]);
-    const tau = m.tau || [1, 1, 1];
+    // words 8-11: the coat (tau, alpha_c), or a printed material's ink (rho_d, roughInk^2; gpu/renderer.js:291-293)
+    const x = m.coated || !m.texture ? [...(m.tau || [1, 1, 1]), m.alphaC || 0]
+      : [...m.texture.ink, m.texture.roughInk * m.texture.roughInk];
     materials.set([c[0], c[1], c[2], m.alpha, m.ior || 1.5, kind, m.fbar || 0, m.roughness,
-      tau[0], tau[1], tau[2], m.alphaC || 0, m.fbarComputed || 0, 0, 0, 0], 16 * i);
+      x[0], x[1], x[2], x[3], m.fbarComputed || 0, 0, 0, 0], 16 * i);
   });
+  // texture coordinates of the textured meshes' vertices: the reference keeps them in a dynamic vertex's spare
+  // words (u in pos.w, v in normal.w; gpu/scene_gpu.js:390-392)
+  let uvs;
+  if (S.meshTable.some((m) => m.flags & MESH_TEXTURED)) {
+    uvs = new Float32Array((S.verts.length / 8) * 2);
+    const tu = new Uint32Array(S.tris.buffer, S.tris.byteOffset, S.tris.length);
+    for (let k = 0; k < S.tris.length / 16; k++) {
+      if (!(S.meshTable[tu[16 * k + 12]].flags & MESH_TEXTURED)) continue;
+      for (const w of [3, 7, 11]) {
+        const v = tu[16 * k + w];
+        uvs[2 * v] = S.verts[8 * v + 3];
+        uvs[2 * v + 1] = S.verts[8 * v + 7];
+      }
+    }
+  }
   const em = S.emitter || { tris: [], area: 0, L0: [0, 0, 0] };
   const emit = new Float32Array(Math.max(1, em.tris.length) * 16), eu = new Uint32Array(emit.buffer);
   em.tris.forEach((t, k) => {
@@ -118,7 +136,7 @@ export function fromPrepared(S, ports = []) {
     profiles,
     nodes: S.nodes, tris: S.tris, verts: S.verts, meshes, materials,
     lights: new Float32Array([L0[0], L0[1], L0[2], 0]), emit: em.tris.length ? emit : new Float32Array(0),
-    emitArea: em.area || 0,
+    emitArea: em.area || 0, ...(uvs ? { uvs } : {}),
   };
   // bodies (S.blas: contiguous triangle ranges in their own frames): the library builds every BVH itself
   if (S.blas && S.blas.length) {
@@ -183,6 +201,10 @@ export class TwinRenderer {
     native.uploadScene(this.ctx, { ...src, bodyCount: this.bodies.length, shadowSkip: src.shadowSkip || 0 });
     this.stats = { triangles: src.tris.length / 16, bodies: this.bodies.length };
     for (const pr of src.profiles || []) this.setLightProfile(pr.light, pr);
+    // the first board's print mask (_loadBoardTexture, gpu/renderer.js:395-412): {mask: Uint8Array, width, height}
+    // decoded by the caller (the PNG's red channel, row 0 the image's top); one paper texel without it
+    const bt = opts.boardTexture || (opts.scene && opts.scene.boardTexture);
+    if (bt) this.setBoardTexture(bt.mask, bt.width, bt.height);
     this.materials = Float32Array.from(src.materials);
     this.lights = Float32Array.from(src.lights);
     this.sensorP = sensorParams(opts.physics || {}, opts.sensor || {});
@@ -329,6 +351,13 @@ export class TwinRenderer {
     native.setLightProfile(this.ctx, light, profile ? { axis: Array.from(profile.axis), theta: Array.from(profile.theta),
       p: Array.from(profile.p) } : null);
     this.shadeChanged = true;
+  }
+
+  /** The printed board's mask (_loadBoardTexture, gpu/renderer.js:395-412): width x height bytes, 255 = paper,
+   * row 0 the image's top. Textured meshes sample it bilinearly (clamp to edge). */
+  setBoardTexture(mask, width, height) {
+    native.setBoardTexture(this.ctx, mask instanceof Uint8Array ? mask : Uint8Array.from(mask), width >>> 0, height >>> 0);
+    this.stats.boardTexture = [width, height];
   }
 
   /** Bodies a camera's own rays ignore: those it owns, e.g. its lens front (gpu/renderer.js:881-886). */

@@ -254,6 +254,12 @@ napi_value UploadScene(napi_env env, napi_callback_info info) {
     napi_throw_range_error(env, nullptr, "triBody needs one entry per triangle");
     return nullptr;
   }
+  // uvs (optional): 2 floats per vertex, for VT_MESH_TEXTURED meshes
+  s.uvs = typed<float>(env, argv[1], "uvs", &n, napi_float32_array);
+  if (s.uvs && n != (size_t)s.vert_count * 2) {
+    napi_throw_range_error(env, nullptr, "uvs needs two floats per vertex");
+    return nullptr;
+  }
   bool pending = false;
   napi_is_exception_pending(env, &pending);
   if (pending) return nullptr;
@@ -360,6 +366,35 @@ napi_value SetInstances(napi_env env, napi_callback_info info) {
 }
 
 // setLightProfile(ctx, light, {axis: [3], theta: [..], p: [..]} | null)
+// setBoardTexture(ctx, mask: Uint8Array (width * height, row 0 the image's top), width, height)
+napi_value SetBoardTexture(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  vt_ctx *c = get_ctx(env, argv[0]);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c) return nullptr;
+  uint32_t w = 0, h = 0;
+  bool ta = false;
+  if (argc < 4 || napi_is_typedarray(env, argv[1], &ta) != napi_ok || !ta ||
+      napi_get_value_uint32(env, argv[2], &w) != napi_ok || napi_get_value_uint32(env, argv[3], &h) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "setBoardTexture(ctx, Uint8Array mask, width, height)");
+    return nullptr;
+  }
+  napi_typedarray_type type;
+  size_t len = 0, off = 0;
+  void *data = nullptr;
+  napi_value ab;
+  CHECK_NAPI(napi_get_typedarray_info(env, argv[1], &type, &len, &data, &ab, &off));
+  if ((type != napi_uint8_array && type != napi_uint8_clamped_array) || len != (size_t)w * h) {
+    napi_throw_range_error(env, nullptr, "the mask needs width * height bytes (Uint8Array)");
+    return nullptr;
+  }
+  int rc = vt_set_board_texture(c, (const uint8_t *)data, w, h);
+  if (rc) return throw_vt(env, c, rc, "vt_set_board_texture");
+  return nullptr;
+}
+
 napi_value SetLightProfile(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -1089,7 +1124,7 @@ napi_value Init(napi_env env, napi_value exports) {
     const char *name;
     napi_callback fn;
   } fns[] = {{"create", Create}, {"destroy", Destroy}, {"buildBVH", BuildBVH}, {"uploadScene", UploadScene},
-             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"render", Render}, {"sensor", Sensor}, {"isp", Isp}, {"sync", Sync}, {"stats", Stats},
+             {"setCamera", SetCamera}, {"setInstances", SetInstances}, {"visible", Visible}, {"setLightProfile", SetLightProfile}, {"setBoardTexture", SetBoardTexture}, {"render", Render}, {"sensor", Sensor}, {"isp", Isp}, {"sync", Sync}, {"stats", Stats},
              {"readback", Readback}, {"lastError", LastError}, {"autotune", Autotune}, {"backward", Backward},
              {"updateMaterials", UpdateMaterials}, {"updateLights", UpdateLights},
              {"radiosityBuild", RadiosityBuild}, {"interiorBvh", InteriorBvh}, {"radiosityUpdate", RadiosityUpdate},

@@ -19,7 +19,7 @@ EXPORTS = [
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
     "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
-    "vt_set_light_profile", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_baked_upload", "vt_baked_occlusion", "vt_baked_jacobi", "vt_baked_read", "vt_radiosity_state", "vt_node_neighbours",
+    "vt_set_light_profile", "vt_set_board_texture", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_baked_upload", "vt_baked_occlusion", "vt_baked_jacobi", "vt_baked_read", "vt_radiosity_state", "vt_node_neighbours",
     "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide", "vt_sensor_raw_device",
 ]
 
@@ -41,7 +41,7 @@ class Scene(C.Structure):
                 ("vert_count", u32), ("meshes", vp), ("mesh_count", u32), ("materials", vp),
                 ("material_count", u32), ("lights", vp), ("light_count", u32), ("emit", vp), ("emit_count", u32),
                 ("emit_area", f32), ("tri_body", vp), ("body_count", u32), ("shadow_skip", u32),
-                ("emit_posed", u32), ("emit_body", u32)]
+                ("emit_posed", u32), ("emit_body", u32), ("uvs", vp)]
 
 
 MAX_BODIES = 24
@@ -162,6 +162,7 @@ def load():
         "vt_last_error": (C.c_char_p, [vp]),
         "vt_build_bvh": (i32, [vp, u32, vp, u32, i32, i32, vp, u32, vp, vp]),
         "vt_upload_scene": (i32, [vp, vp]),
+        "vt_set_board_texture": (i32, [vp, vp, u32, u32]),
         "vt_set_camera": (i32, [vp, u32, vp]),
         "vt_render": (i32, [vp, vp, u32, vp]),
         "vt_sensor": (i32, [vp, u32, vp]),
@@ -295,13 +296,21 @@ class Context:
         nb = int(sc.get("body_count", 0))
         tb = np.ascontiguousarray(sc["tri_body"], dtype=np.uint32) if nb else None
         nodes = np.ascontiguousarray(sc["nodes"]) if with_nodes and not nb else None
+        uvs = np.ascontiguousarray(sc["uvs"], dtype=np.float32) if sc.get("uvs") is not None else None
         s = Scene(ptr(nodes), 0 if nodes is None else nodes.size // 16, ptr(a["tris"]), a["tris"].size // 16,
                   ptr(a["verts"]), a["verts"].size // 8, ptr(a["meshes"]), a["meshes"].size // 4,
                   ptr(a["materials"]), a["materials"].size // 16, ptr(a["lights"]), a["lights"].size // 4,
                   ptr(a["emit"]), a["emit"].size // 16, float(sc["emit_area"]), ptr(tb), nb,
                   int(sc.get("shadow_skip", 0)), 1 if sc.get("emit_body") is not None else 0,
-                  int(sc.get("emit_body") or 0))
+                  int(sc.get("emit_body") or 0), ptr(uvs))
         self.check(self.L.vt_upload_scene(self.h, C.byref(s)))
+        if sc.get("board_mask") is not None:
+            self.set_board_texture(sc["board_mask"])
+
+    def set_board_texture(self, mask):
+        """vt_set_board_texture: the print mask, an (h, w) uint8 array (255 = paper), row 0 the image's top."""
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.check(self.L.vt_set_board_texture(self.h, ptr(m), m.shape[1], m.shape[0]))
 
     def set_instances(self, poses):
         """vt_set_instances: poses[i] = None (hidden) or dict(q0, p0[, q1, p1]) with quaternions (x, y, z, w), and

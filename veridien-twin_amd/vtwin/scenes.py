@@ -21,7 +21,7 @@ import numpy as np
 MULB = np.uint32(0x6D2B79F5)
 
 KIND_DIELECTRIC, KIND_CONDUCTOR, KIND_COATED, KIND_LAMBERT = 0, 1, 2, 3
-MESH_STATIC, MESH_EMITTER = 1, 2
+MESH_STATIC, MESH_EMITTER, MESH_TEXTURED = 1, 2, 128
 
 
 class Rng:
@@ -437,6 +437,58 @@ def posed_scene(parts, meshes, mats, lights, place=None, emit_body=None):
     sc = pack_scene(P, N, I, M, meshes, mats, lights, tri_body=np.concatenate(tb), body_count=nb)
     if emit_body is not None:
         sc["emit_body"] = int(emit_body)
+    return sc
+
+
+def printed_material(paper=(0.85, 0.84, 0.80), ink=(0.04, 0.04, 0.05), rough_paper=0.6, rough_ink=0.35):
+    """A printed dielectric (the reference's charuco_print, gpu/scene_gpu.js:180-192, gpu/renderer.js:293): the
+    paper's rho_d and alpha in words 0-3, the ink's rho_d and alpha (roughness^2) in words 8-11."""
+    rec = material_record(KIND_DIELECTRIC, paper, roughness=rough_paper)
+    rec[8:11] = ink
+    rec[11] = max(float(rough_ink), 0.02) ** 2
+    return rec
+
+
+def board_mask_pattern(w=96, h=72, seed=2011):
+    """A print mask to test with (255 = paper): a 6 x 4 chequer with a horizontal grey ramp over its top sixth and
+    scattered single ink texels, so the bilinear filter meets sharp and smooth transitions alike."""
+    y, x = np.mgrid[0:h, 0:w]
+    m = np.where(((x * 6) // w + (y * 4) // h) % 2 == 0, 255, 0).astype(np.uint8)
+    m[: h // 6] = (x[: h // 6] * 255 // (w - 1)).astype(np.uint8)
+    m.flat[Rng(seed).int(0, w * h - 1, n=60)] = 0
+    return m
+
+
+def quad_uv(P, size, normal, up_hint=(0.0, 1.0, 0.0)):
+    """Texture coordinates of quad()'s corners: u across, v down the image, (0, 0) at the top-left corner."""
+    n = np.asarray(normal, float) / np.linalg.norm(normal)
+    u = np.cross(up_hint, n)
+    if np.linalg.norm(u) < 1e-6:
+        u = np.cross((1.0, 0.0, 0.0), n)
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    c = np.mean(np.asarray(P, float), axis=0)
+    d = np.asarray(P, float) - c
+    return np.stack([d @ u / size + 0.5, 0.5 - d @ v / size], axis=1).astype(np.float32)
+
+
+def textured_scene(mask=None):
+    """posed_parts with two printed boards on the printed material: a static one standing beside the fixed box
+    and one on body 0's -y face (bodies placed as set_instances poses them), with texture coordinates per vertex
+    and a print mask (board_mask_pattern by default)."""
+    parts, meshes, mats, lights = posed_parts()
+    mats = list(mats) + [printed_material()]
+    m_static, m_body = len(meshes), len(meshes) + 1
+    meshes = list(meshes) + [[len(mats) - 1, MESH_STATIC | MESH_TEXTURED, 0, 0], [len(mats) - 1, MESH_TEXTURED, 0, 0]]
+    uv_parts = [np.zeros((np.asarray(P).shape[0], 2), np.float32) for P, *_ in parts]
+    for center, size, normal, mesh, body in (((0.35, -1.12, 0.95), 0.6, (0.0, -1.0, 0.0), m_static, None),
+                                             ((0.0, -0.152, 0.0), 0.28, (0.0, -1.0, 0.0), m_body, 0)):
+        P, N, I = quad(center, size, normal, up_hint=(0.0, 0.0, 1.0))
+        parts.append((P, N, I, np.full(I.shape[0], mesh), body))
+        uv_parts.append(quad_uv(P, size, normal, up_hint=(0.0, 0.0, 1.0)))
+    sc = posed_scene(parts, meshes, mats, lights)
+    sc["uvs"] = np.concatenate(uv_parts).reshape(-1)
+    sc["board_mask"] = board_mask_pattern() if mask is None else mask
     return sc
 
 

@@ -6,7 +6,8 @@ meshes by vt_weld_subdivide, benchMesh, materialsFromPhysics, emitterFromPhysics
 sets, fitCapsule, the emitter records) and its kinematics (control/kinematics.js:97-244: shaftRotation, forward,
 scopeForward, defaultState, posesFromState, cannulaT). Differences, by the library's design: the static triangles
 keep their preparation order and the library builds every BVH at upload (the same binned-SAH trees); the charuco
-texture and FDM layer lines are not carried. The emitter (the scope's light outlet) rides on the scope body as the
+board's print texture rides with its mesh (texture coordinates per vertex, the mask's red channel); FDM layer lines
+are not carried. The emitter (the scope's light outlet) rides on the scope body as the
 reference's does (its records and axis in the scope's frame, posed per sample); a scene without a scope body places
 it in the world at the scope's pose as static geometry.
 """
@@ -62,7 +63,9 @@ def compute_normals(P, I):
 
 
 def load(root):
-    """scene.json, physics.json and every mesh of the scene under `root`."""
+    """scene.json, physics.json and every mesh of the scene under `root`; each board's printed texture decoded into
+    board["mask"] (the PNG's red channel, uint8, row 0 the image's top; None when the file is absent), as
+    _loadBoardTexture reads it (gpu/renderer.js:395-412)."""
     with open(os.path.join(root, "scene.json")) as f:
         scene = json.load(f)
     with open(os.path.join(root, "physics.json")) as f:
@@ -71,6 +74,13 @@ def load(root):
     for m in scene["meshes"]:
         with open(os.path.join(root, m["file"]), "rb") as f:
             meshes[m["name"]] = decode_mesh(m, f.read())
+    for b in scene.get("boards") or []:
+        path = os.path.join(root, b["texture"]) if b.get("texture") else None
+        b["mask"] = None
+        if path and os.path.exists(path):
+            from PIL import Image
+            with Image.open(path) as im:
+                b["mask"] = np.ascontiguousarray(np.asarray(im.convert("RGB"))[:, :, 0])
     return scene, meshes, physics
 
 
@@ -137,8 +147,14 @@ def prepare_material(m):
                     roughness=r, ior=ior, fbar=0.0)
     rho = unwrap(m.get("rho_d_paper" if textured else "rho_d"))
     fa = unwrap(m.get("fresnel_avg"))
-    return dict(kind=scenes.KIND_DIELECTRIC, c=[float(x) for x in (rho if rho is not None else (0.5, 0.5, 0.5))],
-                roughness=r, ior=ior, fbar=None if fa is None else float(fa))
+    out = dict(kind=scenes.KIND_DIELECTRIC, c=[float(x) for x in (rho if rho is not None else (0.5, 0.5, 0.5))],
+               roughness=r, ior=ior, fbar=None if fa is None else float(fa))
+    if textured:
+        # charuco_print: the ink's rho_d and roughness (gpu/scene_gpu.js:192; roughness_ink 0.4 by default)
+        ri = unwrap(m.get("roughness_ink"))
+        out["texture"] = dict(ink=[float(x) for x in unwrap(m.get("rho_d_ink"))],
+                              rough_ink=0.4 if ri is None else float(ri))
+    return out
 
 
 def material_record(pm):
@@ -148,6 +164,10 @@ def material_record(pm):
                                  tau=pm.get("tau", (1.0, 1.0, 1.0)), coat_roughness=pm.get("coat_roughness", 0.05))
     if pm["kind"] == scenes.KIND_DIELECTRIC and pm.get("fbar") is not None:
         rec[6] = pm["fbar"]
+    if pm.get("texture"):
+        # the ink in words 8-11 (gpu/renderer.js:293: ink rgb, roughInk^2)
+        rec[8:11] = pm["texture"]["ink"]
+        rec[11] = pm["texture"]["rough_ink"] ** 2
     return rec
 
 
@@ -344,6 +364,7 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
 
     # static: weld + subdivide each static mesh, then the bench
     parts = []        # (P float64, N, I, mesh id, body)
+    part_uv = {}      # part index -> texture coordinates (textured meshes)
     lo, hi = np.array([np.inf, np.inf]), np.array([-np.inf, -np.inf])
     raw_static = 0
     for m in metas:
@@ -411,11 +432,17 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
     for bi, body in enumerate(body_list):
         for m in body["meshes"]:
             P, N, I, _ = meshes[m["name"]]
+            # a mesh on a printed material with texture coordinates samples the print mask
+            # (gpu/scene_gpu.js:324-325: not emitter, textured material, UVs present)
+            uv = meshes[m["name"]][3]
+            texd = uv is not None and bool(prepared[mindex[m["material"]]].get("texture"))
             flags = ((MESH_FLAG["BOARD"] if body["kind"] == "board" else 0) |
                      (MESH_FLAG["INSTRUMENT"] if body["kind"] == "instrument" else 0) |
-                     (MESH_FLAG["SCOPE"] if body["kind"] == "scope" else 0))
+                     (MESH_FLAG["SCOPE"] if body["kind"] == "scope" else 0) |
+                     (MESH_FLAG["TEXTURED"] if texd else 0))
             mid = add_mesh(m["name"], m["material"], flags)
             parts.append((P.astype(np.float64), N.astype(np.float64), np.asarray(I, np.uint32), mid, bi))
+            part_uv[len(parts) - 1] = uv
         if bi == emit_body:
             parts.extend(emit_rows)
 
@@ -444,9 +471,11 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
         body["area_by_material"] = area
 
     # one vertex / triangle list: static parts first, bodies after (the outlet with the scope)
-    Pl, Nl, Il, Ml, Bl = [], [], [], [], []
+    Pl, Nl, Il, Ml, Bl, UVl = [], [], [], [], [], []
     base = 0
-    for P, N, I, mid, bi in parts:
+    for k, (P, N, I, mid, bi) in enumerate(parts):
+        uv = part_uv.get(k)
+        UVl.append(np.zeros((len(P), 2), np.float32) if uv is None else np.asarray(uv, np.float32).reshape(-1, 2))
         Pl.append(np.asarray(P, np.float64))
         Nl.append(np.asarray(N, np.float64))
         Il.append(np.asarray(I, np.uint32) + base)
@@ -464,6 +493,11 @@ def prepare(scene, meshes, physics, node_spacing=0.003, poses=None):
     out = scenes.pack_scene(P, N, I, tri_mesh, meshes_arr, np.array(recs, np.float32),
                             np.array([[em["L0"][0], em["L0"][1], em["L0"][2], 0.0]], np.float32),
                             tri_body=tri_body, body_count=len(body_list))
+    if any(mt["flags"] & MESH_FLAG["TEXTURED"] for mt in mesh_table):
+        # texture coordinates per vertex and the first board's print mask (one paper texel without it)
+        out["uvs"] = np.concatenate(UVl).reshape(-1)
+        boards = scene.get("boards") or []
+        out["board_mask"] = boards[0].get("mask") if boards else None
     # the outlet's axis: the scope's +z turned toward +y by the emission angle (physics scope.emission_axis), else
     # the outlet's mean normal; in the emitter's frame
     profiles = {}

@@ -569,6 +569,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
     ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
+    ap.add_argument("--shadow-sort", type=int, default=0, help="wavefront: 1 = shadow rays traced in key order, "
+                                                               "2 = in slot order, 0 = the default")
+    ap.add_argument("--queues", type=int, default=0, help="ray queues of the trace launches: 1, or 8 (one per XCD); "
+                                                          "0: the mode's default")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "copy", "sensor", "twin", "twin-live", "twin-balanced",
                                          "twin-judge"], default="c2",
                     help="c2: the headline full-scale frame (default); c3: batching; c4: dense radiosity; c5: adjoint; "
@@ -616,7 +620,8 @@ def main():
         log("autotune (%.1f s): %s" % (time.time() - t0, tuned))
 
     def frame():
-        ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk, stack_lds=args.stack, batch=args.batch)
+        ctx.render(slots, mode=mode, block=args.block, chunk=args.chunk, stack_lds=args.stack, batch=args.batch,
+                   queues=args.queues, shadow_sort=args.shadow_sort)
         for k in slots:
             ctx.sensor(k, sensor[k])
             ctx.isp(k, ISP)

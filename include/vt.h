@@ -151,7 +151,8 @@ typedef struct {
   uint32_t stack_lds;    /* traversal stack entries per lane kept in LDS, 8..24 (0 = tuned default); the rest of
                             the 48-entry stack spills to global memory. Wavefront: 4, 8 or 16 (0 = 8) */
   uint32_t queues;       /* batched: 1 global ray queue, or 8 queues (one per XCD, over contiguous eighths of the
-                            frame, drained by the blocks of that XCD first); 0 = tuned default */
+                            frame, drained by the blocks of that XCD first); wavefront: the same for each trace
+                            launch's rays; 0 = tuned default */
   uint32_t graph;        /* batched and wavefront: 1 = run the frame's launches as a hipGraph, captured on first
                             use and replayed while the cameras, sample range and launch shape stay the same (the
                             wavefront graph does not time its extension traces one by one) */
@@ -160,6 +161,9 @@ typedef struct {
                             radix sort of the whole 29-bit key (direction cell, origin Morton code), 4..13 = an
                             unstable bin sort by the key's top bits (kernels only; a graph uses it, 12 bits when
                             the default is the radix sort). The image does not depend on it. */
+  uint32_t shadow_sort;  /* wavefront: 1 = trace each vertex's shadow rays in the order of their (direction cell,
+                            origin Morton code) keys, sorted like the paths; 2 = in slot order; 0 = tuned default.
+                            The image does not depend on it. */
 } vt_render_opts;
 
 /* Result of vt_autotune: the chosen launch shapes (also made the context's defaults) and the probe timings. */

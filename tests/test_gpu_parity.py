@@ -374,15 +374,19 @@ def test_wavefront_graph_replays_match_eager(ctx, c1):
     for bits in (4, 13):
         img, st = _render_img(ctx, mode=W, sort_bits=bits)
         assert st["graph_used"] == 0 and np.array_equal(img, ref), bits
+    # nor on the order of the shadow rays (slot order, or sorted by key: radix and bins), nor on the trace queues
+    for ss, bits, q in ((1, 1, 1), (2, 1, 8), (1, 12, 8), (1, 1, 8)):
+        img, st = _render_img(ctx, mode=W, sort_bits=bits, shadow_sort=ss, queues=q)
+        assert st["graph_used"] == 0 and np.array_equal(img, ref), (ss, bits, q)
     b0 = ctx.stats()["graph_builds"]
     for k in range(3):
-        img, st = _render_img(ctx, mode=W, graph=True)
+        img, st = _render_img(ctx, mode=W, graph=True, shadow_sort=1)
         assert st["graph_used"] == 1 and st["graph_builds"] == b0 + 1
         assert np.array_equal(img, ref), k
         e, st = _render_img(ctx, spp_begin=4, spp_end=12)
         assert st["graph_used"] == 0 and np.array_equal(e, ref2)
     for k in range(2):
-        img, st = _render_img(ctx, mode=W, graph=True, batch=64 * 1000)
+        img, st = _render_img(ctx, mode=W, graph=True, batch=64 * 1000, shadow_sort=1)
         assert st["graph_builds"] == b0 + 2 + k and np.array_equal(img, ref)
         # the batched form's graph in between: the wavefront one is captured again next time
         img, st = _render_img(ctx, graph=True)
@@ -397,7 +401,7 @@ def test_autotune_picks_a_valid_shape(ctx, c1):
     r = ctx.autotune([0], probe_stride=8)
     print("autotune:", r)
     assert r["block"] in (64, 128, 256) and r["stack_lds"] in (8, 16, 24) and r["chunk"] in (1, 4, 16)
-    assert r["queues"] in (1, 8) and r["candidates"] == 54 + 18 + 4 and r["wf_sort_bits"] in (0, 10, 12, 13)
+    assert r["queues"] in (1, 8) and r["candidates"] == 54 + 18 + 2 + 2 + 4 and r["wf_sort_bits"] in (0, 10, 12, 13)
     assert r["probe_mrays_best"] >= r["probe_mrays_default"] > 0 and r["probe_mrays_wavefront"] > 0
     assert r["mode"] in (lib.VT_RENDER_BATCHED, lib.VT_RENDER_WAVEFRONT)
     assert r["wf_stack_lds"] in (4, 8, 16) and r["wf_refill"] in (8, 16, 32) and r["wf_batch"] in (1 << 22, 1 << 24)

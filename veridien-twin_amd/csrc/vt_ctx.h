@@ -115,7 +115,8 @@ struct vt_ctx {
   // the tuned frame form (VT_RENDER_AUTO) and the wavefront's batch, LDS stack and refill threshold
   uint32_t def_mode = VT_RENDER_BATCHED;
   uint64_t def_wf_batch = 1ull << 24;
-  uint32_t def_wf_stack = 8, def_wf_refill = 16, def_wf_sort = 0;
+  uint32_t def_wf_stack = 8, def_wf_refill = 16, def_wf_sort = 0, def_wf_queues = 1;
+  bool def_wf_shadow_sort = true;
   int occ[2][2][3][STACK_LDS_MAX + 1] = {};   // resident render blocks per CU by (bodies, estimator, block, LDS depth)
   // scene box (for the wavefront sort key)
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};

@@ -229,6 +229,8 @@ struct WfParams {
   uint64_t stride;    // 1, or > 1: trace every stride-th wave of the frame (tuning probes)
   uint32_t sort_bits; // 0: the 29-bit radix sort between vertices; 4..13: the bin sort by the key's top bits
   bool graph;         // run the frame's batches as a hipGraph
+  uint32_t queues;    // trace queues: 1, or one per XCD
+  bool shadow_sort;   // trace the shadow rays in the order of their keys
 };
 
 constexpr uint32_t GRAPH_SORT_BITS = 12;   // a captured wavefront frame bins by this many key bits unless told
@@ -237,7 +239,9 @@ WfParams wf_params(const vt_ctx *c, const vt_render_opts &opt) {
   uint32_t sb = opt.sort_bits ? (opt.sort_bits == 1 ? 0u : opt.sort_bits) : c->def_wf_sort;
   if (opt.graph && sb == 0) sb = GRAPH_SORT_BITS;
   return WfParams{opt.batch ? opt.batch : c->def_wf_batch, opt.stack_lds ? opt.stack_lds : c->def_wf_stack,
-                  opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1, sb, opt.graph != 0};
+                  opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1, sb, opt.graph != 0,
+                  opt.queues ? opt.queues : c->def_wf_queues,
+                  opt.shadow_sort ? opt.shadow_sort == 1 : c->def_wf_shadow_sort};
 }
 
 // the wavefront frame: buffers for one batch, the batches in order, one event pair around all of them
@@ -252,12 +256,14 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
     depth = std::max(depth, c->h_cams[a].depth);
   }
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-  // per path 80 B of state, per queue slot 36 B, per shadow slot 48 B
-  const size_t bytes = al(P * 64) + 2 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) + 256;
+  // per path 80 B of state, per queue slot 36 B, per shadow slot 48 B + 16 B of the shadow sort
+  const size_t bytes = al(P * 64) + 2 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) +
+                       4 * al(P * nl * 4) + al(WF_COUNT_WORDS * 4);
   int rc;
+  if (P * nl > (1ull << 32) - 1) return fail(c, VT_ERR_ARG, "render: wavefront batch of %llu shadow slots", (unsigned long long)(P * nl));
   if (P > c->wf_paths || nl > c->wf_nl) {
     if ((rc = ensure(c, c->wf, bytes))) return rc;
-    const size_t tb = wf_sort_temp_bytes((uint32_t)P);
+    const size_t tb = wf_sort_temp_bytes((uint32_t)(P * nl));
     if ((rc = ensure(c, c->wf_temp, tb))) return rc;
     c->wf_temp_bytes = tb;
     c->wf_paths = P;
@@ -284,7 +290,12 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   W.sh_o = (float4 *)take(P * nl * 16);
   W.sh_d = (float4 *)take(P * nl * 16);
   W.sh_c = (float4 *)take(P * nl * 16);
-  W.count = (uint32_t *)take(16);
+  W.sh_key = (uint32_t *)take(P * nl * 4);
+  W.sh_idx = (uint32_t *)take(P * nl * 4);
+  W.sh_keys_sorted = (uint32_t *)take(P * nl * 4);
+  W.sh_q = wp.shadow_sort ? (uint32_t *)take(P * nl * 4) : nullptr;   // null: the shadow slots in slot order
+  W.count = (uint32_t *)take(WF_COUNT_WORDS * 4);
+  W.xcd_queues = wp.queues;
   float ext[3];
   for (int k = 0; k < 3; k++) ext[k] = std::max(c->bmax[k] - c->bmin[k], 1e-6f);
   W.bmin = make_float3(c->bmin[0], c->bmin[1], c->bmin[2]);
@@ -295,6 +306,8 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   W.spill_stride = c->spill_lanes;
   const uint32_t slds = wp.slds;
   if (slds != 4 && slds != 8 && slds != 16) return fail(c, VT_ERR_ARG, "render: wavefront stack_lds must be 4, 8 or 16");
+  if (wp.queues != 1 && wp.queues != (uint32_t)XCD_COUNT)
+    return fail(c, VT_ERR_ARG, "render: queues must be 1 or %d", XCD_COUNT);
   if (wp.sort_bits && (wp.sort_bits < 4 || wp.sort_bits > WF_SORT_BITS_MAX))
     return fail(c, VT_ERR_ARG, "render: wavefront sort_bits must be 1 or 4..%u", WF_SORT_BITS_MAX);
   if (wp.graph && (wp.stride != 1 || wp.sort_bits == 0)) return fail(c, VT_ERR_ARG, "render: a wavefront graph bins");
@@ -319,7 +332,7 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
       W.batch_begin = bb;
       W.n = (uint32_t)std::min<uint64_t>(P, items - bb);
       VT_HIP(c, launch_wavefront_batch(W, depth, grid_cap, trace_cap, refill, slds, wp.sort_bits, keys_sorted, q_sorted,
-                                       c->wf_temp.p, c->wf_temp_bytes,
+                                       c->wf_temp.p, c->wf_temp_bytes, W.sh_q != nullptr,
                                        timed ? c->ev_ext.data() + 2 * depth * bi : nullptr, c->stream));
     }
     return VT_OK;
@@ -630,7 +643,8 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     out->probe_mrays_wavefront = 0.0;
     return VT_OK;
   }
-  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride, c->def_wf_sort, false};
+  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride, c->def_wf_sort, false, c->def_wf_queues,
+                 c->def_wf_shadow_sort};
   double m_wf = 0;
   auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
     vt_render_opts po = opt;
@@ -655,7 +669,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   for (uint64_t bsz : batches)
     for (uint32_t st : wstacks)
       for (uint32_t rf : refills) {
-        const WfParams wp{bsz, st, rf, stride, wbest.sort_bits, false};
+        const WfParams wp{bsz, st, rf, stride, wbest.sort_bits, false, wbest.queues, wbest.shadow_sort};
         double m = 0;
         if ((rc = probe_wf(wp, m))) return rc;
         tried++;
@@ -664,6 +678,30 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
           wbest = wp;
         }
       }
+  // the shadow rays in slot order or sorted by key
+  for (bool ss : {false, true}) {
+    WfParams wp = wbest;
+    wp.shadow_sort = ss;
+    double m = 0;
+    if ((rc = probe_wf(wp, m))) return rc;
+    tried++;
+    if (m > m_wf) {
+      m_wf = m;
+      wbest = wp;
+    }
+  }
+  // the trace queues at that shape: one head, or one per XCD
+  for (uint32_t q : {1u, (uint32_t)XCD_COUNT}) {
+    WfParams wp = wbest;
+    wp.queues = q;
+    double m = 0;
+    if ((rc = probe_wf(wp, m))) return rc;
+    tried++;
+    if (m > m_wf) {
+      m_wf = m;
+      wbest = wp;
+    }
+  }
   // then the sort between vertices at that shape: the whole key, or bins of its top 10, 12 or 13 bits
   for (uint32_t sb : {0u, 10u, 12u, 13u}) {
     WfParams wp = wbest;
@@ -685,6 +723,8 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   c->def_wf_stack = wbest.slds;
   c->def_wf_refill = wbest.refill;
   c->def_wf_sort = wbest.sort_bits;
+  c->def_wf_queues = wbest.queues;
+  c->def_wf_shadow_sort = wbest.shadow_sort;
   c->def_mode = m_wf > m_best ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
   drop_graph(c);
   // the probes left partial sums behind: clear the listed cameras

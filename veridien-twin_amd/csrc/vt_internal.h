@@ -64,6 +64,7 @@ hipError_t launch_backward(const RenderLaunch &L, float *gmat, float *glight, ui
 hipError_t launch_camgrad(const RenderLaunch &L, float *gcam, hipStream_t s);
 // the wavefront form: one batch of paths (items [batch_begin, batch_begin + n) of the frame queue, n a multiple
 // of 64), one kernel per stage per vertex, live paths radix-sorted by ray_key between vertices
+constexpr uint32_t WF_COUNT_WORDS = 32;   // WfLaunch::count
 struct WfLaunch {
   SceneDev S;
   const DevCam *cams;
@@ -82,11 +83,16 @@ struct WfLaunch {
   // queue of live paths, by queue slot (0..count[0])
   const uint32_t *q;           // path ids (null: identity, vertex 1)
   uint32_t *q_next, *key_next; // next vertex: path id and sort key per slot (DEAD key: ended)
-  uint32_t *count;             // [0] live slots of this vertex, [1] of the next, [2] [3] trace queue heads
+  uint32_t *count;             // [0] live slots of this vertex, [1] of the next, [2] [3] trace queue heads,
+                               // [8..15] [16..23] the per-XCD heads of the extension and shadow traces
+  uint32_t xcd_queues;         // trace queues: 1 (one head) or 8 (one per XCD over contiguous eighths)
   float4 *hit;                 // t, u, v, triangle (bits) per slot
   float4 *thr_slot;            // the throughput at this vertex per slot (weight of its NEE in the gather)
   uint32_t *nee;               // shadow slots used per queue slot
   float4 *sh_o, *sh_d, *sh_c;  // shadow slots [slot * nl_max + j]: origin + tmax, direction, contribution + visible
+  // the shadow sort (sh_q non-null): per shadow slot its ray's key (the dead key for an unused slot) and its own
+  // index, and the slot indices sorted by key; count[4] = the live shadow slots, traced in that order
+  uint32_t *sh_key, *sh_idx, *sh_keys_sorted, *sh_q;
   float3 bmin, bscale;         // scene box and 128 / extent, for the origin's Morton code
   unsigned long long *counters;// [1] rays, [2] samples, [3] extension rays, [4] shadow rays
   uint32_t *overflow;
@@ -103,8 +109,8 @@ hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu);
 constexpr uint32_t WF_SORT_BITS_MAX = 13;
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
                                   uint32_t refill, uint32_t slds, uint32_t sort_bits, uint32_t *keys_sorted,
-                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext,
-                                  hipStream_t s);
+                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, bool sort_shadows,
+                                  hipEvent_t *ev_ext, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid, hipStream_t s);
 // shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals

@@ -71,12 +71,9 @@ __device__ __forceinline__ uint32_t wave_count(bool pred) {
 
 // ---------------------------------------------------------------- kernels
 __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    W.count[0] = W.n;
-    W.count[1] = 0;
-    W.count[2] = 0;
-    W.count[3] = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x < WF_COUNT_WORDS) W.count[threadIdx.x] = threadIdx.x == 0 ? W.n : 0u;
+  if (W.sh_q)
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < W.n * W.nl_max; s += gridDim.x * 256) W.sh_idx[s] = s;
   for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < W.n; p += gridDim.x * 256) {
     const uint64_t item = path_item(W, p);
     uint32_t c = 0;
@@ -116,8 +113,16 @@ __global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, 
   Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, W.spill, threadIdx.x, 256, SLDS, STACK_TOTAL - SLDS, W.spill_stride,
            (uint64_t)blockIdx.x * 256 + threadIdx.x, W.overflow};
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t n = ANY ? W.count[0] * W.nl_max : W.count[0];
+  // shadow rays: every slot of the vertex in slot order, or (sh_q) the count[4] live ones in key order
+  const uint32_t n = ANY ? (W.sh_q ? W.count[4] : W.count[0] * W.nl_max) : W.count[0];
   uint32_t *head = &W.count[ANY ? 3 : 2];
+  // per-XCD queues (xcd_queues = 8): queue x is the x-th contiguous eighth of the rays, drained first by the
+  // blocks of XCD x (consecutive blocks go to consecutive XCDs), so an XCD's L2 holds the nodes of fewer
+  // direction cells; a wave whose queue is used up moves on to the next one
+  const uint32_t nq = W.xcd_queues;
+  uint32_t *qhead = &W.count[ANY ? 16 : 8];
+  const uint32_t seg = (n + nq - 1) / nq;
+  uint32_t qi = blockIdx.x % nq, tried = 0;   // lane 0's queue and how many it found used up
   bool active = false, exhausted = false;
   uint32_t k = 0;
   // the lane's ray and traversal state
@@ -138,18 +143,43 @@ __global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, 
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle >= refill) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(head, nidle);
+      uint32_t base = 0, cnt = 0;
+      if (lane == 0) {
+        if (nq == 1) {
+          base = atomicAdd(head, nidle);
+          cnt = base < n ? min(nidle, n - base) : 0u;
+          if (base + nidle >= n) tried = 1;
+        } else {
+          while (tried < nq) {
+            const uint32_t s0 = qi * seg, s1 = min(n, s0 + seg);
+            const uint32_t b = s0 < s1 ? atomicAdd(&qhead[qi], nidle) : 0u;
+            if (s0 + b < s1) {
+              base = s0 + b;
+              cnt = min(nidle, s1 - base);
+              break;
+            }
+            qi = qi + 1 == nq ? 0u : qi + 1;
+            tried++;
+          }
+        }
+      }
       base = __shfl(base, 0, 64);
-      if (base + nidle >= n) exhausted = true;
+      cnt = __shfl(cnt, 0, 64);
+      if (__shfl(tried, 0, 64) >= nq || cnt == 0) exhausted = true;
       if (!active) {
-        k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-        if (k < n) {
+        const uint32_t r = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        k = base + r;
+        if (r < cnt) {
           v3 o, d;
           float tmax;
           bool ok;
           if (ANY) {
-            ok = k % W.nl_max < W.nee[k / W.nl_max];
+            if (W.sh_q) {
+              k = W.sh_q[k];    // the slot the result belongs to
+              ok = true;
+            } else {
+              ok = k % W.nl_max < W.nee[k / W.nl_max];
+            }
             const float4 o4 = W.sh_o[k], d4 = W.sh_d[k];
             o = mk(o4.x, o4.y, o4.z);
             d = mk(d4.x, d4.y, d4.z);
@@ -258,9 +288,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
   const uint32_t n = W.count[0];
   const uint32_t b = W.bounce;
   const float A = W.S.emit_area;
+  uint32_t wave_live = 0;   // the wave's live paths of the next vertex, over the grid-stride loop
+  uint32_t lane_nee = 0;    // the lane's shadow slots written (the shadow sort's live count)
   for (uint32_t k0 = blockIdx.x * 256; k0 < W.n; k0 += gridDim.x * 256) {
     const uint32_t k = k0 + threadIdx.x;
     bool live = false;
+    uint32_t nee_k = 0;   // shadow slots this slot wrote
     if (k >= n && k < W.n) {
       W.key_next[k] = DEAD_KEY;
       W.q_next[k] = k;
@@ -366,6 +399,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
               W.sh_o[s] = make_float4(so.x, so.y, so.z, dist - 2e-5f);
               W.sh_d[s] = make_float4(wi.x, wi.y, wi.z, time);
               W.sh_c[s] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+              if (W.sh_q) W.sh_key[s] = ray_key(so, wi, W);
               nee++;
             }
           }
@@ -402,13 +436,26 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
         }
       }
       W.nee[k] = nee;
+      nee_k = nee;
       if (!live) {
         W.key_next[k] = DEAD_KEY;
         W.q_next[k] = p;
       }
     }
-    const uint32_t nlive = wave_count(live);
-    if ((threadIdx.x & 63u) == 0 && nlive) atomicAdd(&W.count[1], nlive);
+    wave_live += wave_count(live);
+    if (W.sh_q) {
+      // the unused shadow slots sort last; the live ones are counted
+      if (k < W.n)
+        for (uint32_t j = nee_k; j < W.nl_max; j++) W.sh_key[(size_t)k * W.nl_max + j] = DEAD_KEY;
+      lane_nee += nee_k;
+    }
+  }
+  // one atomic per wave for the whole grid-stride loop: same-address atomics serialise across the XCDs (one per
+  // wave and iteration made this kernel take twice as long)
+  if ((threadIdx.x & 63u) == 0 && wave_live) atomicAdd(&W.count[1], wave_live);
+  if (W.sh_q) {
+    for (int off = 32; off >= 1; off >>= 1) lane_nee += __shfl_xor(lane_nee, off, 64);
+    if ((threadIdx.x & 63u) == 0 && lane_nee) atomicAdd(&W.count[4], lane_nee);
   }
 }
 
@@ -435,10 +482,9 @@ __global__ __launch_bounds__(256) void k_wf_gather(WfLaunch W) {
 
 // after the sort: the next vertex's live count
 __global__ void k_wf_swap_counts(uint32_t *count) {
-  count[0] = count[1];
-  count[1] = 0;
-  count[2] = 0;
-  count[3] = 0;
+  const uint32_t next = count[1];
+  __syncthreads();
+  if (threadIdx.x < WF_COUNT_WORDS) count[threadIdx.x] = threadIdx.x == 0 ? next : 0u;
 }
 
 // per-pixel sums of the batch: the butterfly of run_wave over each 64-item wave
@@ -605,8 +651,10 @@ size_t wf_sort_temp_bytes(uint32_t n) {
 // One batch: generate, then per vertex trace / shade / shadow / gather, and a sort between vertices.
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,
                                   uint32_t refill, uint32_t slds, uint32_t sort_bits, uint32_t *keys_sorted,
-                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, hipEvent_t *ev_ext,
-                                  hipStream_t s) {
+                                  uint32_t *q_sorted, void *sort_temp, size_t sort_temp_bytes, bool sort_shadows,
+                                  hipEvent_t *ev_ext, hipStream_t s) {
+  if (!sort_shadows) W.sh_q = nullptr;
+  const uint32_t nsh = W.n * W.nl_max;
   if (refill < 1 || refill > 64 || (slds != 4 && slds != 8 && slds != 16)) return hipErrorInvalidValue;
   const uint32_t g = grid_for(W.n, 256, grid_cap);
   const uint32_t gt = grid_for(W.n, 256, trace_grid_cap);
@@ -629,6 +677,18 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
     }
     if (inst) hipLaunchKernelGGL(k_wf_shade<true>, dim3(g), dim3(256), 0, s, W);
     else hipLaunchKernelGGL(k_wf_shade<false>, dim3(g), dim3(256), 0, s, W);
+    if (W.sh_q) {
+      // the shadow slots by their rays' keys, so a wave traces shadow rays that start close together and head the
+      // same way (the visibility of a slot does not depend on the order)
+      hipError_t e;
+      if (sort_bits) {
+        e = launch_bin_sort(W.sh_key, W.sh_idx, nsh, sort_bits, (uint32_t *)sort_temp, W.sh_q, s);
+      } else {
+        size_t tb = sort_temp_bytes;
+        e = rocprim::radix_sort_pairs(sort_temp, tb, W.sh_key, W.sh_keys_sorted, W.sh_idx, W.sh_q, nsh, 0, 32, s);
+      }
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(trace_shadow, dim3(gs), dim3(256), 0, s, W, refill);
     hipLaunchKernelGGL(k_wf_gather, dim3(g), dim3(256), 0, s, W);
     if (b + 1 == max_depth) break;
@@ -640,7 +700,7 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
       e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(1), 0, s, W.count);
+    hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(WF_COUNT_WORDS), 0, s, W.count);
     W.q = q_sorted;
   }
   hipLaunchKernelGGL(k_wf_accum, dim3(grid_for(W.n, 256, grid_cap)), dim3(256), 0, s, W);

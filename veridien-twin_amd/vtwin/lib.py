@@ -78,7 +78,7 @@ class Camera(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("mode", u32), ("spp_begin", u32), ("spp_end", u32), ("accumulate", u32), ("chunk", u32),
                 ("block", u32), ("stack_lds", u32), ("queues", u32), ("graph", u32), ("batch", u32),
-                ("sort_bits", u32)]
+                ("sort_bits", u32), ("shadow_sort", u32)]
 
 
 class TuneResult(C.Structure):
@@ -446,10 +446,10 @@ class Context:
         return out[: pts.shape[0]].astype(bool)
 
     def render(self, slots, mode=VT_RENDER_BATCHED, spp_begin=0, spp_end=0, accumulate=False, chunk=0, block=0,
-               stack_lds=0, queues=0, graph=False, batch=0, sort_bits=0):
+               stack_lds=0, queues=0, graph=False, batch=0, sort_bits=0, shadow_sort=0):
         arr = np.ascontiguousarray(slots, dtype=np.uint32)
         o = RenderOpts(mode, spp_begin, spp_end, 1 if accumulate else 0, chunk, block, stack_lds, queues,
-                       1 if graph else 0, batch, sort_bits)
+                       1 if graph else 0, batch, sort_bits, shadow_sort)
         self.check(self.L.vt_render(self.h, ptr(arr), arr.size, C.byref(o)))
 
     def copy_bandwidth(self, nbytes, iters=50):

@@ -158,7 +158,7 @@ typedef struct {
                             wavefront graph does not time its extension traces one by one) */
   uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 16,777,216) */
   uint32_t sort_bits;    /* wavefront: the sort of the live paths between vertices; 0 = tuned default, 1 = the
-                            radix sort of the whole 29-bit key (direction cell, origin Morton code), 4..13 = an
+                            radix sort of the whole 23-bit key (direction cell, origin Morton code), 4..13 = an
                             unstable bin sort by the key's top bits (kernels only; a graph uses it, 12 bits when
                             the default is the radix sort). The image does not depend on it. */
   uint32_t shadow_sort;  /* wavefront: 1 = trace each vertex's shadow rays in the order of their (direction cell,

@@ -227,7 +227,7 @@ struct WfParams {
   uint32_t slds;      // LDS stack entries of the trace kernels (4, 8, 16)
   uint32_t refill;    // idle lanes that make a wave fetch rays
   uint64_t stride;    // 1, or > 1: trace every stride-th wave of the frame (tuning probes)
-  uint32_t sort_bits; // 0: the 29-bit radix sort between vertices; 4..13: the bin sort by the key's top bits
+  uint32_t sort_bits; // 0: the 23-bit radix sort between vertices; 4..13: the bin sort by the key's top bits
   bool graph;         // run the frame's batches as a hipGraph
   uint32_t queues;    // trace queues: 1, or one per XCD
   bool shadow_sort;   // trace the shadow rays in the order of their keys

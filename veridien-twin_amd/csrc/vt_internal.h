@@ -104,7 +104,7 @@ size_t wf_sort_temp_bytes(uint32_t n);
 hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu);
 // trace_grid_cap: resident blocks of the persistent trace kernels; refill: idle lanes of a wave that trigger a
 // fetch of new rays (1..64); slds: LDS stack entries per lane of the trace kernels (4, 8 or 16)
-// sort_bits: 0 sorts the slots between vertices by their whole 29-bit key (rocPRIM's radix sort, whose memset
+// sort_bits: 0 sorts the slots between vertices by their whole 23-bit key (rocPRIM's radix sort, whose memset
 // nodes keep it out of graphs); 4..WF_SORT_BITS_MAX bins them by the key's top bits (kernels only)
 constexpr uint32_t WF_SORT_BITS_MAX = 13;
 hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_cap, uint32_t trace_grid_cap,

@@ -12,6 +12,11 @@ namespace vt {
 namespace {
 
 constexpr uint32_t DEAD_KEY = 0xffffffffu;
+// sort key: 8 bits of direction cell, then KEY_MORTON_BITS bits per axis of the origin's Morton code; the dead
+// key's bit KEY_BITS puts ended paths last, so a radix sort over KEY_BITS + 1 bits orders everything. Five bits
+// per axis make 24 sorted bits, three radix passes instead of four: 1.4 % more frame throughput on the full-scale
+// frame than seven bits, whose finer origin order traced 1-4 % faster but sorted 29 ms per frame slower.
+constexpr uint32_t KEY_MORTON_BITS = 5, KEY_BITS = 8 + 3 * KEY_MORTON_BITS;
 
 // the sample of frame item `item`, given its camera
 struct ItemPix {
@@ -44,8 +49,8 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {   // 9 bits -> every t
   v = (v | (v << 2)) & 0x09249249u;
   return v;
 }
-// sort key of a ray: its direction's cell in a 16 x 16 octahedral map (8 bits), then a 21-bit Morton code of
-// its origin in the scene box (7 bits per axis): rays of a wave head the same way first, start nearby second
+// sort key of a ray: its direction's cell in a 16 x 16 octahedral map (8 bits), then the Morton code of its origin
+// in the scene box at KEY_MORTON_BITS bits per axis: rays of a wave head the same way first, start nearby second
 // (direction-major keys traced 8 % faster than origin-major ones, and 16 x 16 cells than 2 x 2 x 2 octants)
 __device__ __forceinline__ uint32_t ray_key(v3 o, v3 d, const WfLaunch &W) {
   const float s = 1.0f / (fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
@@ -62,7 +67,7 @@ __device__ __forceinline__ uint32_t ray_key(v3 o, v3 d, const WfLaunch &W) {
   const float fy = fminf(fmaxf((o.y - W.bmin.y) * W.bscale.y, 0.0f), 127.0f);
   const float fz = fminf(fmaxf((o.z - W.bmin.z) * W.bscale.z, 0.0f), 127.0f);
   const uint32_t m = spread3((uint32_t)fx) | (spread3((uint32_t)fy) << 1) | (spread3((uint32_t)fz) << 2);
-  return ((cy * 16u + cx) << 21) | m;
+  return ((cy * 16u + cx) << (3 * KEY_MORTON_BITS)) | (m >> (3 * (7 - KEY_MORTON_BITS)));
 }
 
 __device__ __forceinline__ uint32_t wave_count(bool pred) {
@@ -123,6 +128,7 @@ __global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, 
   uint32_t *qhead = &W.count[ANY ? 16 : 8];
   const uint32_t seg = (n + nq - 1) / nq;
   uint32_t qi = blockIdx.x % nq, tried = 0;   // lane 0's queue and how many it found used up
+
   bool active = false, exhausted = false;
   uint32_t k = 0;
   // the lane's ray and traversal state
@@ -138,6 +144,51 @@ __global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, 
   Hit h{0.f, 0.f, 0.f, 0u, NO_INST};
   uint32_t sp = 0, ref = TRAV_DONE, leaf = TRAV_DONE;
   uint32_t nrays = 0;
+  // start lane's ray `kk` (a shadow ray through the sorted indices when sh_q): its slot in k, its traversal state
+  auto start_ray = [&](uint32_t kk) {
+    k = kk;
+    v3 o, d;
+    float tmax;
+    bool ok;
+    if (ANY) {
+      if (W.sh_q) {
+        k = W.sh_q[k];    // the slot the result belongs to
+        ok = true;
+      } else {
+        ok = k % W.nl_max < W.nee[k / W.nl_max];
+      }
+      const float4 o4 = W.sh_o[k], d4 = W.sh_d[k];
+      o = mk(o4.x, o4.y, o4.z);
+      d = mk(d4.x, d4.y, d4.z);
+      tmax = o4.w;
+      time = d4.w;
+      if (INST) skip = W.S.shadow_skip;
+    } else {
+      const uint32_t p = W.q ? W.q[k] : k;
+      const float4 *ps = W.ps + 4 * (size_t)p;
+      ok = ps[2].w > 0.0f;
+      const float4 o4 = ps[0], d4 = ps[1];
+      o = mk(o4.x, o4.y, o4.z);
+      d = mk(d4.x, d4.y, d4.z);
+      tmax = INFINITY;
+      time = d4.w;
+      if (INST) skip = W.bounce == 0 ? W.cams[__float_as_uint(ps[3].x)].skip : 0u;
+      if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
+    }
+    if (ok) {
+      B = make_slab(o, d);
+      R = ray_w(o, d);
+      dir = d;
+      best = tmax;
+      tmax0 = tmax;
+      found = false;
+      sp = 0;
+      ref = W.S.root;
+      leaf = TRAV_DONE;
+      active = true;
+      nrays++;
+    }
+  };
   for (;;) {
     // refill: the wave's idle lanes take consecutive queue entries
     const uint64_t idle = __ballot(!active);
@@ -169,49 +220,7 @@ __global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, 
       if (!active) {
         const uint32_t r = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         k = base + r;
-        if (r < cnt) {
-          v3 o, d;
-          float tmax;
-          bool ok;
-          if (ANY) {
-            if (W.sh_q) {
-              k = W.sh_q[k];    // the slot the result belongs to
-              ok = true;
-            } else {
-              ok = k % W.nl_max < W.nee[k / W.nl_max];
-            }
-            const float4 o4 = W.sh_o[k], d4 = W.sh_d[k];
-            o = mk(o4.x, o4.y, o4.z);
-            d = mk(d4.x, d4.y, d4.z);
-            tmax = o4.w;
-            time = d4.w;
-            if (INST) skip = W.S.shadow_skip;
-          } else {
-            const uint32_t p = W.q ? W.q[k] : k;
-            const float4 *ps = W.ps + 4 * (size_t)p;
-            ok = ps[2].w > 0.0f;
-            const float4 o4 = ps[0], d4 = ps[1];
-            o = mk(o4.x, o4.y, o4.z);
-            d = mk(d4.x, d4.y, d4.z);
-            tmax = INFINITY;
-            time = d4.w;
-            if (INST) skip = W.bounce == 0 ? W.cams[__float_as_uint(ps[3].x)].skip : 0u;
-            if (!ok) W.hit[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(MISS));
-          }
-          if (ok) {
-            B = make_slab(o, d);
-            R = ray_w(o, d);
-            dir = d;
-            best = tmax;
-            tmax0 = tmax;
-            found = false;
-            sp = 0;
-            ref = W.S.root;
-            leaf = TRAV_DONE;
-            active = true;
-            nrays++;
-          }
-        }
+        if (r < cnt) start_ray(k);
       }
     }
     if (!__any(active)) {
@@ -565,7 +574,7 @@ hipError_t wf_trace_occupancy(uint32_t slds, bool inst, int *blocks_per_cu) {
 // and moves no more than 16 B per slot; it issues kernels only, so the frame can be captured as a graph.
 constexpr uint32_t BIN_SORT_CHUNK = 4096;     // slots per scatter block
 __device__ __forceinline__ uint32_t key_bin(uint32_t key, uint32_t bits) {
-  return key == DEAD_KEY ? (1u << bits) : key >> (29u - bits);
+  return key == DEAD_KEY ? (1u << bits) : key >> (KEY_BITS - bits);
 }
 
 __global__ __launch_bounds__(256) void k_bin_zero(uint32_t *hist, uint32_t nbins) {
@@ -644,7 +653,7 @@ hipError_t launch_bin_sort(const uint32_t *keys, const uint32_t *vals, uint32_t 
 size_t wf_sort_temp_bytes(uint32_t n) {
   size_t bytes = 0;
   (void)rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                  (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
+                                  (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, KEY_BITS + 1);
   return std::max<size_t>(bytes, ((size_t)(1u << WF_SORT_BITS_MAX) + 1u) * 4);
 }
 
@@ -685,7 +694,7 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
         e = launch_bin_sort(W.sh_key, W.sh_idx, nsh, sort_bits, (uint32_t *)sort_temp, W.sh_q, s);
       } else {
         size_t tb = sort_temp_bytes;
-        e = rocprim::radix_sort_pairs(sort_temp, tb, W.sh_key, W.sh_keys_sorted, W.sh_idx, W.sh_q, nsh, 0, 32, s);
+        e = rocprim::radix_sort_pairs(sort_temp, tb, W.sh_key, W.sh_keys_sorted, W.sh_idx, W.sh_q, nsh, 0, KEY_BITS + 1, s);
       }
       if (e != hipSuccess) return e;
     }
@@ -697,7 +706,7 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
       e = launch_bin_sort(W.key_next, W.q_next, W.n, sort_bits, (uint32_t *)sort_temp, q_sorted, s);
     } else {
       size_t tb = sort_temp_bytes;
-      e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, 32, s);
+      e = rocprim::radix_sort_pairs(sort_temp, tb, W.key_next, keys_sorted, W.q_next, q_sorted, W.n, 0, KEY_BITS + 1, s);
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_wf_swap_counts, dim3(1), dim3(WF_COUNT_WORDS), 0, s, W.count);

@@ -202,6 +202,7 @@ def bench_c4(args, quiet=False):
     for name in ("fp32", "bf16_refine"):
         out[name]["rel_err_vs_fp64"] = float(np.linalg.norm(sols[name] - ref) / np.linalg.norm(ref))
     out["speedup_bf16_over_fp32"] = round(out["fp32"]["ms"] / out["bf16_refine"]["ms"], 3)
+    out["hbm_read_ceiling_GBs"] = round(ctx.read_bandwidth(2 << 30), 1)   # streaming reads of 2 GiB, the BF16 F's size
     sweep_bytes_fp32 = 4.0 * n * n + 2 * 4 * 48 * n
     out["fp32_sweep_GBs"] = round(sweep_bytes_fp32 / (out["fp32"]["ms_per_sweep"] * 1e-3) / 1e9, 1)
     out["value"] = out["speedup_bf16_over_fp32"]
@@ -777,7 +778,8 @@ def extras(args, ctx, sc, out):
     out["c4"] = {"fp32_ms": c4["fp32"]["ms"], "bf16_refine_ms": c4["bf16_refine"]["ms"],
                  "fp32_ms_per_sweep": c4["fp32"]["ms_per_sweep"], "bf16_ms_per_sweep": c4["bf16_refine"]["ms_per_sweep"],
                  "bf16_rel_err_vs_fp64": c4["bf16_refine"]["rel_err_vs_fp64"],
-                 "fp32_rel_err_vs_fp64": c4["fp32"]["rel_err_vs_fp64"], "fp32_sweep_GBs": c4["fp32_sweep_GBs"]}
+                 "fp32_rel_err_vs_fp64": c4["fp32"]["rel_err_vs_fp64"], "fp32_sweep_GBs": c4["fp32_sweep_GBs"],
+                 "hbm_read_ceiling_GBs": c4["hbm_read_ceiling_GBs"]}
     log("c4: bf16 / fp32 %.3f (%.1f s)" % (c4["speedup_bf16_over_fp32"], time.time() - t0))
 
 

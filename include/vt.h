@@ -364,6 +364,11 @@ int vt_visible(vt_ctx *ctx, uint32_t cam, const float *points, uint32_t n, float
  * (GB/s). With bytes well under 256 MB the working set stays in the Infinity Cache; the roofline of the trace
  * kernels is reported against it as well as against HBM. No reference counterpart. */
 int vt_copy_bandwidth(vt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbs);
+/* Read bandwidth: one `bytes`-sized device buffer read `iters` times by a grid-stride kernel (16-B loads, four in
+ * flight per lane, nothing written), bytes read per second (GB/s). With bytes well over 256 MB this is the
+ * streaming-read rate HBM delivers to a kernel, the practical ceiling of read-bound kernels such as the dense
+ * sweeps. No reference counterpart. */
+int vt_read_bandwidth(vt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbs);
 
 /* ---------------------------------------------------------------- test kernels (the reference's TESTS entry
  * points, gpu/wgsl/kernels.wgsl.js:487-551, that tests/gpu_test.html compared against the JS) */

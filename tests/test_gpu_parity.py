@@ -429,3 +429,15 @@ def test_wavefront_matches_persistent(ctx, c1, batch):
     # a second frame through the same buffers gives the same image
     img2, _ = _render_img(ctx, mode=lib.VT_RENDER_WAVEFRONT, batch=batch)
     assert np.array_equal(img, img2)
+
+
+def test_bandwidth_probes(ctx):
+    """The copy and read bandwidth probes report a positive rate below the HBM nameplate for buffers far larger
+    than the Infinity Cache, and reject sizes under one page and zero iterations."""
+    rd = ctx.read_bandwidth(1 << 30, iters=5)
+    cp = ctx.copy_bandwidth(1 << 30, iters=5)
+    print("read %.0f GB/s, copy %.0f GB/s" % (rd, cp))
+    assert 500 < rd < 8000 and 500 < cp < 8000
+    for bad in ((1024, 5), (1 << 20, 0)):
+        with pytest.raises(lib.VTError):
+            ctx.read_bandwidth(*bad)

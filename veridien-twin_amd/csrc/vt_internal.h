@@ -113,6 +113,7 @@ hipError_t launch_wavefront_batch(WfLaunch W, uint32_t max_depth, uint32_t grid_
                                   hipEvent_t *ev_ext, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t s);
 hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid, hipStream_t s);
+hipError_t launch_read_u4(const uint4 *x, uint64_t n, uint32_t grid, uint32_t *sink, hipStream_t s);
 // shading records of every triangle slot (SceneDev::tshade) from the slot's vertices, vertex ids and normals
 hipError_t launch_tri_shading(const float4 *tri, const uint4 *attr, const float4 *vnorm, uint32_t T, float4 *out,
                               hipStream_t s);

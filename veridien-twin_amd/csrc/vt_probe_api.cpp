@@ -32,6 +32,33 @@ extern "C" int vt_copy_bandwidth(vt_ctx *c, uint64_t bytes, uint32_t iters, doub
   return VT_OK;
 }
 
+extern "C" int vt_read_bandwidth(vt_ctx *c, uint64_t bytes, uint32_t iters, double *gbs) {
+  VT_LIVE(c);
+  if (!gbs || bytes < 4096 || iters == 0) return fail(c, VT_ERR_ARG, "read_bandwidth: bytes >= 4096, iters > 0");
+  VT_HIP(c, hipSetDevice(c->device));
+  DevBuf a, sink;
+  int rc = ensure(c, a, bytes);
+  if (!rc) rc = ensure(c, sink, 256);
+  struct Free {
+    DevBuf &x, &y;
+    ~Free() { release(x); release(y); }
+  } fr{a, sink};
+  if (rc) return rc;
+  const uint64_t n = bytes / 16;
+  const uint32_t grid = (uint32_t)c->cu_count * 16;
+  VT_HIP(c, launch_zero_f4((float4 *)a.p, n, c->stream));   // zeros fold to 0, so the sink is never written
+  for (int w = 0; w < 3; w++) VT_HIP(c, launch_read_u4((const uint4 *)a.p, n, grid, (uint32_t *)sink.p, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_s0, c->stream));
+  for (uint32_t k = 0; k < iters; k++) VT_HIP(c, launch_read_u4((const uint4 *)a.p, n, grid, (uint32_t *)sink.p, c->stream));
+  VT_HIP(c, hipEventRecord(c->ev_s1, c->stream));
+  VT_HIP(c, hipEventSynchronize(c->ev_s1));
+  float ms = 0;
+  VT_HIP(c, hipEventElapsedTime(&ms, c->ev_s0, c->ev_s1));
+  *gbs = 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
+  c->sensor_timed = false;
+  return VT_OK;
+}
+
 // ====================================================================== test entry points
 extern "C" int vt_trace_test(vt_ctx *c, const float *rays, uint32_t n, int any_hit, float *hits) {
   VT_LIVE(c);

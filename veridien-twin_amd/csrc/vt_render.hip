@@ -1156,6 +1156,28 @@ hipError_t launch_copy_f4(const float4 *x, float4 *y, uint64_t n, uint32_t grid,
   return hipGetLastError();
 }
 
+// reads n uint4 once (grid-stride, four 16-B loads in flight per lane) and folds them into one word, written only
+// when it equals a value the probe's fill never produces: the read-bandwidth probe
+__global__ void k_read_u4(const uint4 *__restrict__ x, uint64_t n, uint32_t *sink) {
+  uint32_t f = 0;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x * 4 + threadIdx.x; b < n; b += step) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint64_t i = b + (uint64_t)u * blockDim.x;
+      v[u] = i < n ? x[i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) f ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  if (f == 0x9e3779b9u) sink[0] = f;
+}
+hipError_t launch_read_u4(const uint4 *x, uint64_t n, uint32_t grid, uint32_t *sink, hipStream_t s) {
+  hipLaunchKernelGGL(k_read_u4, dim3(grid), dim3(256), 0, s, x, n, sink);
+  return hipGetLastError();
+}
+
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;

@@ -18,7 +18,7 @@ EXPORTS = [
     "vt_sensor", "vt_readback", "vt_sync", "vt_frame_stats_get", "vt_trace_test", "vt_lens_test",
     "vt_camera_ray_test", "vt_bsdf_test", "vt_sensor_test", "vt_jacobi_sparse", "vt_dense_upload",
     "vt_dense_formfactors", "vt_dense_solve", "vt_dense_download_F", "vt_backward", "vt_update_materials",
-    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_set_instances", "vt_visible",
+    "vt_update_lights", "vt_autotune", "vt_copy_bandwidth", "vt_read_bandwidth", "vt_set_instances", "vt_visible",
     "vt_set_light_profile", "vt_set_board_texture", "vt_radiosity_build", "vt_interior_bvh", "vt_radiosity_update", "vt_radiosity_update_moved", "vt_baked_upload", "vt_baked_occlusion", "vt_baked_jacobi", "vt_baked_read", "vt_radiosity_state", "vt_node_neighbours",
     "vt_radiosity_tables", "vt_capsule_test", "vt_isp", "vt_isp_test", "vt_weld_subdivide", "vt_sensor_raw_device",
 ]
@@ -188,6 +188,7 @@ def load():
         "vt_update_lights": (i32, [vp, vp, u32]),
         "vt_autotune": (i32, [vp, vp, u32, vp, u32, vp]),
         "vt_copy_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
+        "vt_read_bandwidth": (i32, [vp, C.c_uint64, u32, vp]),
         "vt_set_instances": (i32, [vp, vp, u32]),
         "vt_visible": (i32, [vp, u32, vp, u32, f32, f32, vp]),
         "vt_set_light_profile": (i32, [vp, u32, vp, vp, vp, u32]),
@@ -455,6 +456,11 @@ class Context:
     def copy_bandwidth(self, nbytes, iters=50):
         g = C.c_double(0)
         self.check(self.L.vt_copy_bandwidth(self.h, nbytes, iters, C.byref(g)))
+        return g.value
+
+    def read_bandwidth(self, nbytes, iters=20):
+        g = C.c_double(0)
+        self.check(self.L.vt_read_bandwidth(self.h, nbytes, iters, C.byref(g)))
         return g.value
 
     def autotune(self, slots, spp_begin=0, spp_end=0, probe_stride=0):

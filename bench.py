@@ -557,7 +557,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["auto", "wavefront", "batched", "per-camera"], default=None,
                     help="wavefront (the default of the headline frame): per-vertex kernels with sorted rays, at the shape the committed "
-                         "counter profile was taken at (16,777,216-path batches, 8-entry LDS stack, refill 16); "
+                         "counter profile was taken at (the default batch, 8-entry LDS stack, refill 16); "
                          "auto (the default of --config twin): the autotuner times both forms on a probe of the "
                          "frame (untimed) and the faster runs; batched: one persistent launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
@@ -569,7 +569,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the mode's default)")
-    ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: 16,777,216)")
+    ap.add_argument("--batch", type=int, default=0, help="wavefront: paths per batch (0: the library's default, 134,217,728 paths within an eighth of the "
+                         "device's memory)")
     ap.add_argument("--shadow-sort", type=int, default=0, help="wavefront: 1 = shadow rays traced in key order, "
                                                                "2 = in slot order, 0 = the default")
     ap.add_argument("--queues", type=int, default=0, help="ray queues of the trace launches: 1, or 8 (one per XCD); "

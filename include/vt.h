@@ -156,7 +156,8 @@ typedef struct {
   uint32_t graph;        /* batched and wavefront: 1 = run the frame's launches as a hipGraph, captured on first
                             use and replayed while the cameras, sample range and launch shape stay the same (the
                             wavefront graph does not time its extension traces one by one) */
-  uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 16,777,216) */
+  uint32_t batch;        /* wavefront: paths per batch, rounded up to 64 (0 = 134,217,728, halved until the batch's
+                            buffers fit an eighth of the device's memory and its shadow slots 32 bits) */
   uint32_t sort_bits;    /* wavefront: the sort of the live paths between vertices; 0 = tuned default, 1 = the
                             radix sort of the whole 23-bit key (direction cell, origin Morton code), 4..13 = an
                             unstable bin sort by the key's top bits (kernels only; a graph uses it, 12 bits when
@@ -307,8 +308,9 @@ int vt_update_lights(vt_ctx *ctx, const float *lights, uint32_t light_count);
 int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts);
 /* Autotuner (no reference counterpart; the reference fixes its workgroup shape): times the persistent batched
  * launch of the listed cameras over block {64, 128, 256} x LDS stack {8, 16, 24} x chunk {1, 4, 16} x queues
- * {1, 8}, and the wavefront form over batch {4M, 16M} x LDS stack {4, 8, 16} x refill {8, 16, 32}, each on a probe
- * of every `probe_stride`-th wave of the frame (0 = 32); picks the fastest of each form in rays per second, makes
+ * {1, 8}, and the wavefront form at its default batch over LDS stack {4, 8, 16} x refill {8, 16, 32}, shadow rays
+ * in slot or key order, one or eight trace queues and four sorts between vertices, each on a probe of every
+ * `probe_stride`-th wave of the frame (0 = 32); picks the fastest of each form in rays per second, makes
  * them the context's defaults for fields of vt_render_opts left 0, and the faster form the one VT_RENDER_AUTO
  * runs. opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
 int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts, uint32_t probe_stride,

@@ -175,6 +175,7 @@ extern "C" int vt_create(const vt_create_info *info, vt_ctx **out) {
     return VT_ERR_HIP;
   }
   c->cu_count = prop.multiProcessorCount;
+  c->wf_budget = (uint64_t)prop.totalGlobalMem / 8;
   int occ = 0;
   // persistent grids: up to 8 resident 256-thread blocks per CU (LDS and VGPRs usually allow fewer)
   c->spill_lanes = (uint64_t)c->cu_count * 8 * 256;

@@ -231,6 +231,7 @@ struct WfParams {
   bool graph;         // run the frame's batches as a hipGraph
   uint32_t queues;    // trace queues: 1, or one per XCD
   bool shadow_sort;   // trace the shadow rays in the order of their keys
+  bool fit_batch;     // the batch is the default: shrink it to the memory budget instead of failing
 };
 
 constexpr uint32_t GRAPH_SORT_BITS = 12;   // a captured wavefront frame bins by this many key bits unless told
@@ -241,7 +242,7 @@ WfParams wf_params(const vt_ctx *c, const vt_render_opts &opt) {
   return WfParams{opt.batch ? opt.batch : c->def_wf_batch, opt.stack_lds ? opt.stack_lds : c->def_wf_stack,
                   opt.chunk ? std::min(64u, opt.chunk) : c->def_wf_refill, 1, sb, opt.graph != 0,
                   opt.queues ? opt.queues : c->def_wf_queues,
-                  opt.shadow_sort ? opt.shadow_sort == 1 : c->def_wf_shadow_sort};
+                  opt.shadow_sort ? opt.shadow_sort == 1 : c->def_wf_shadow_sort, opt.batch == 0};
 }
 
 // the wavefront frame: buffers for one batch, the batches in order, one event pair around all of them
@@ -257,8 +258,15 @@ int render_wavefront(vt_ctx *c, uint32_t n, const vt_render_opts &opt, const WfP
   }
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   // per path 80 B of state, per queue slot 36 B, per shadow slot 48 B + 16 B of the shadow sort
-  const size_t bytes = al(P * 64) + 2 * al(P * 16) + 4 * al(P * 4) + al(P * 16) + al(P * 4) + 3 * al(P * nl * 16) +
-                       4 * al(P * nl * 4) + al(WF_COUNT_WORDS * 4);
+  auto batch_bytes = [&](uint64_t p) {
+    return al(p * 64) + 2 * al(p * 16) + 4 * al(p * 4) + al(p * 16) + al(p * 4) + 3 * al(p * nl * 16) +
+           4 * al(p * nl * 4) + al(WF_COUNT_WORDS * 4);
+  };
+  // the default batch: halved (in multiples of 64) until the buffers fit the budget and the shadow slots 32 bits
+  if (wp.fit_batch)
+    while (P > (1u << 20) && ((c->wf_budget && batch_bytes(P) > c->wf_budget) || P * nl > (1ull << 32) - 1))
+      P = (P / 2 + 63) / 64 * 64;
+  const size_t bytes = batch_bytes(P);
   int rc;
   if (P * nl > (1ull << 32) - 1) return fail(c, VT_ERR_ARG, "render: wavefront batch of %llu shadow slots", (unsigned long long)(P * nl));
   if (P > c->wf_paths || nl > c->wf_nl) {
@@ -623,8 +631,9 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   c->def_stack = best.stack;
   c->def_chunk = best.chunk;
   c->def_queues = best.queues;
-  // the wavefront form on the same probe: batch {4M, 16M} paths x LDS stack {4, 8, 16} x refill {8, 16, 32};
-  // the reference estimator runs the persistent form only
+  // the wavefront form on the same probe: LDS stack {4, 8, 16} x refill {8, 16, 32} at the default batch (the
+  // probe covers a few percent of the frame, too few paths to show what a larger batch gains, and the gain only
+  // grows with the batch); the reference estimator runs the persistent form only
   if (ref) {
     drop_graph(c);
     for (uint32_t a = 0; a < n; a++) {
@@ -644,7 +653,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     return VT_OK;
   }
   WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride, c->def_wf_sort, false, c->def_wf_queues,
-                 c->def_wf_shadow_sort};
+                 c->def_wf_shadow_sort, true};
   double m_wf = 0;
   auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
     vt_render_opts po = opt;
@@ -664,20 +673,18 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     double warm = 0;
     if ((rc = probe_wf(wbest, warm))) return rc;
   }
-  const uint64_t batches[] = {1ull << 22, 1ull << 24};
   const uint32_t wstacks[] = {4, 8, 16}, refills[] = {8, 16, 32};
-  for (uint64_t bsz : batches)
-    for (uint32_t st : wstacks)
-      for (uint32_t rf : refills) {
-        const WfParams wp{bsz, st, rf, stride, wbest.sort_bits, false, wbest.queues, wbest.shadow_sort};
-        double m = 0;
-        if ((rc = probe_wf(wp, m))) return rc;
-        tried++;
-        if (m > m_wf) {
-          m_wf = m;
-          wbest = wp;
-        }
+  for (uint32_t st : wstacks)
+    for (uint32_t rf : refills) {
+      const WfParams wp{wbest.batch, st, rf, stride, wbest.sort_bits, false, wbest.queues, wbest.shadow_sort, true};
+      double m = 0;
+      if ((rc = probe_wf(wp, m))) return rc;
+      tried++;
+      if (m > m_wf) {
+        m_wf = m;
+        wbest = wp;
       }
+    }
   // the shadow rays in slot order or sorted by key
   for (bool ss : {false, true}) {
     WfParams wp = wbest;

@@ -312,7 +312,8 @@ int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opt
  * in slot or key order, one or eight trace queues and four sorts between vertices, each on a probe of every
  * `probe_stride`-th wave of the frame (0 = 32); picks the fastest of each form in rays per second, makes
  * them the context's defaults for fields of vt_render_opts left 0, and the faster form the one VT_RENDER_AUTO
- * runs. opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
+ * runs, judged on one more probe of each winner eight times larger (the wavefront's rate grows with the paths
+ * it sorts together; out->mode reports the pick, the probe_mrays fields the shape probes). opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
 int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts, uint32_t probe_stride,
                 vt_tune_result *out);
 /* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */

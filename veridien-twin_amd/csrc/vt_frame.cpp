@@ -573,15 +573,16 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   const uint64_t frame_waves = items / 64;
   const uint64_t probe_waves = (frame_waves + stride - 1) / stride;
   if ((rc = ensure_launch_events(c, 1))) return rc;
-  auto probe = [&](const Shape &sh, double &mrays) -> int {
+  auto probe_at = [&](const Shape &sh, uint64_t strd, double &mrays) -> int {
+    const uint64_t pw = (frame_waves + strd - 1) / strd;
     RenderLaunch L = base_launch(c, sh);
     L.cams = (const DevCam *)c->dcams.p;
     L.ncam = n;
-    L.total_waves = probe_waves;
-    L.wave_stride = stride;
+    L.total_waves = pw;
+    L.wave_stride = strd;
     L.persistent = 1;
     L.ref = ref ? 1u : 0u;
-    int r = persistent_grid(c, sh, probe_waves, L.grid, ref);
+    int r = persistent_grid(c, sh, pw, L.grid, ref);
     if (r) return r;
     VT_HIP(c, launch_zero_u64((unsigned long long *)c->counters.p, 16, c->stream));
     VT_HIP(c, hipEventRecord(c->ev_launch[0], c->stream));
@@ -595,6 +596,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     mrays = ms > 0 ? (double)cnt[1] / (ms * 1e-3) / 1e6 : 0.0;
     return VT_OK;
   };
+  auto probe = [&](const Shape &sh, double &mrays) { return probe_at(sh, stride, mrays); };
   const Shape def{(uint32_t)DEFAULT_BLOCK, (uint32_t)STACK_LDS, 4u, 1u};
   double m_def = 0, warm = 0;
   if ((rc = probe(def, warm))) return rc;          // first touch of the scene and the caches
@@ -726,13 +728,25 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     if ((rc = probe_wf(wbest, m2w))) return rc;
     m_wf = m2w;
   }
+  // the form: each form's winner once more on a probe eight times larger (a quarter of the frame at the default
+  // stride). The wavefront form sorts a batch's paths together, so its rate grows with the paths it is given,
+  // and the shape probes hold too few (the real twin frame: batched ahead on the small probe, the wavefront 5 %
+  // ahead on the whole frame).
+  const uint64_t dstride = std::max<uint64_t>(1, stride / 8);
+  double mb_d = 0, mw_d = 0;
+  if ((rc = probe_at(best, dstride, mb_d))) return rc;
+  {
+    WfParams wd = wbest;
+    wd.stride = dstride;
+    if ((rc = probe_wf(wd, mw_d))) return rc;
+  }
   c->def_wf_batch = wbest.batch;
   c->def_wf_stack = wbest.slds;
   c->def_wf_refill = wbest.refill;
   c->def_wf_sort = wbest.sort_bits;
   c->def_wf_queues = wbest.queues;
   c->def_wf_shadow_sort = wbest.shadow_sort;
-  c->def_mode = m_wf > m_best ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
+  c->def_mode = mw_d > mb_d ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
   drop_graph(c);
   // the probes left partial sums behind: clear the listed cameras
   for (uint32_t a = 0; a < n; a++) {

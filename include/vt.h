@@ -173,7 +173,7 @@ typedef struct {
   uint32_t candidates;   /* launch shapes timed (both forms) */
   double probe_mrays_best;    /* Mrays/s of the chosen persistent shape on the probe */
   double probe_mrays_default; /* Mrays/s of the built-in default (256 threads, 16 LDS entries, chunk 4, 1 queue) */
-  double probe_fraction;      /* share of the frame's waves one probe traces */
+  double probe_fraction;      /* share of the frame's waves one persistent-form probe traces (wavefront: 8x) */
   uint32_t mode;              /* the faster form: VT_RENDER_BATCHED or VT_RENDER_WAVEFRONT (used by VT_RENDER_AUTO) */
   uint32_t wf_batch, wf_stack_lds, wf_refill;  /* wavefront form: paths per batch, LDS stack, refill threshold */
   double probe_mrays_wavefront;               /* Mrays/s of the chosen wavefront shape on the same probe */
@@ -312,8 +312,9 @@ int vt_render(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opt
  * in slot or key order, one or eight trace queues and four sorts between vertices, each on a probe of every
  * `probe_stride`-th wave of the frame (0 = 32); picks the fastest of each form in rays per second, makes
  * them the context's defaults for fields of vt_render_opts left 0, and the faster form the one VT_RENDER_AUTO
- * runs, judged on one more probe of each winner eight times larger (the wavefront's rate grows with the paths
- * it sorts together; out->mode reports the pick, the probe_mrays fields the shape probes). opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
+ * runs. The wavefront's probes cover eight times more waves than the persistent form's (its rate grows with the
+ * paths it sorts together), and the form is judged at that size (out->mode; probe_mrays_best and _default are
+ * the persistent form's small probes). opts: the sample range (as for vt_render). The listed cameras' accumulated images are cleared. Blocks. */
 int vt_autotune(vt_ctx *ctx, const uint32_t *cams, uint32_t n, const vt_render_opts *opts, uint32_t probe_stride,
                 vt_tune_result *out);
 /* sensor(name) raw stage (gpu/renderer.js:1091-1099; sensorOptics + sensorExpose fused into one kernel). */

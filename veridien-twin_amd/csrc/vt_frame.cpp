@@ -654,7 +654,11 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     out->probe_mrays_wavefront = 0.0;
     return VT_OK;
   }
-  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, stride, c->def_wf_sort, false, c->def_wf_queues,
+  // the wavefront form's probes cover eight times more of the frame than the persistent form's (a quarter at the
+  // default stride): its rate grows with the paths it sorts together, and on small probes it favours shapes
+  // (a 16-entry LDS stack) that lose on the whole frame
+  const uint64_t dstride = std::max<uint64_t>(1, stride / 8);
+  WfParams wbest{c->def_wf_batch, c->def_wf_stack, c->def_wf_refill, dstride, c->def_wf_sort, false, c->def_wf_queues,
                  c->def_wf_shadow_sort, true};
   double m_wf = 0;
   auto probe_wf = [&](const WfParams &wp, double &mrays) -> int {
@@ -678,7 +682,7 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
   const uint32_t wstacks[] = {4, 8, 16}, refills[] = {8, 16, 32};
   for (uint32_t st : wstacks)
     for (uint32_t rf : refills) {
-      const WfParams wp{wbest.batch, st, rf, stride, wbest.sort_bits, false, wbest.queues, wbest.shadow_sort, true};
+      const WfParams wp{wbest.batch, st, rf, dstride, wbest.sort_bits, false, wbest.queues, wbest.shadow_sort, true};
       double m = 0;
       if ((rc = probe_wf(wp, m))) return rc;
       tried++;
@@ -728,25 +732,17 @@ extern "C" int vt_autotune(vt_ctx *c, const uint32_t *cams, uint32_t n, const vt
     if ((rc = probe_wf(wbest, m2w))) return rc;
     m_wf = m2w;
   }
-  // the form: each form's winner once more on a probe eight times larger (a quarter of the frame at the default
-  // stride). The wavefront form sorts a batch's paths together, so its rate grows with the paths it is given,
-  // and the shape probes hold too few (the real twin frame: batched ahead on the small probe, the wavefront 5 %
-  // ahead on the whole frame).
-  const uint64_t dstride = std::max<uint64_t>(1, stride / 8);
-  double mb_d = 0, mw_d = 0;
+  // the form: the persistent winner once more at the wavefront probes' size, against the wavefront winner (the
+  // real twin frame: the persistent form ahead on a 1/32 probe, the wavefront 5 % ahead on the whole frame)
+  double mb_d = 0;
   if ((rc = probe_at(best, dstride, mb_d))) return rc;
-  {
-    WfParams wd = wbest;
-    wd.stride = dstride;
-    if ((rc = probe_wf(wd, mw_d))) return rc;
-  }
   c->def_wf_batch = wbest.batch;
   c->def_wf_stack = wbest.slds;
   c->def_wf_refill = wbest.refill;
   c->def_wf_sort = wbest.sort_bits;
   c->def_wf_queues = wbest.queues;
   c->def_wf_shadow_sort = wbest.shadow_sort;
-  c->def_mode = mw_d > mb_d ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
+  c->def_mode = m_wf > mb_d ? (uint32_t)VT_RENDER_WAVEFRONT : (uint32_t)VT_RENDER_BATCHED;
   drop_graph(c);
   // the probes left partial sums behind: clear the listed cameras
   for (uint32_t a = 0; a < n; a++) {

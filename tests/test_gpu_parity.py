@@ -441,3 +441,28 @@ def test_bandwidth_probes(ctx):
     for bad in ((1024, 5), (1 << 20, 0)):
         with pytest.raises(lib.VTError):
             ctx.read_bandwidth(*bad)
+
+
+def test_wavefront_default_batch_within_memory_budget(ctx, c1):
+    """The default wavefront batch (2^27 paths) is halved until its buffers fit an eighth of the device memory:
+    1280x960 at 64 spp with 16 light samples per vertex (78.6 M paths at 1,156 B each, 91 GB as one batch) renders
+    in smaller batches, with the image of explicit 4 M-path batches (at 64 spp a pixel's samples are one wave,
+    summed by one butterfly and written once, and batches hold whole waves) and the same sample and ray counts."""
+    _, cams = c1
+    w, h = 1280, 960
+    ctx.set_camera(0, dict(cams[0], width=w, height=h, spp=64, light_samples=16))
+
+    def frame(**kw):
+        ctx.render([0], mode=lib.VT_RENDER_WAVEFRONT, **kw)
+        st = ctx.stats()
+        assert st["stack_overflows"] == 0
+        return ctx.readback(0, lib.VT_KIND_LINEAR, w, h), st
+
+    try:
+        ref, st0 = frame(batch=1 << 22)
+        img, st = frame()
+    finally:
+        ctx.set_camera(0, cams[0])
+    assert st["samples"] == st0["samples"] > 0
+    assert st["rays"] == st0["rays"]
+    assert np.array_equal(img, ref)

@@ -7,7 +7,7 @@
   render forms give bit-identical images for every camera; a tile of the smallest and of a 2048x1536 camera
   against the oracle.
 * The device's ray counter against the oracle's count of the same frame.
-* Convergence (the north_star bar: radiance within 1 % relative RMSE of a 4096-spp FP64 CPU oracle): seeded 16x16
+* Convergence (the accuracy bar: radiance within 1 % relative RMSE of a 4096-spp FP64 CPU oracle): seeded 16x16
   windows of the C1 and full-scale cameras at 4096 spp on the same samples, and independent sample sets compared
   through their estimated standard errors (a chi-square test that sees a bias the noise would hide).
 

@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle on the same seeded inputs.
 
-Bars (north_star): primary-hit triangle ids bit-exact except a stated fraction of grazing or edge rays;
+Bars: primary-hit triangle ids bit-exact except a stated fraction of grazing or edge rays;
 barycentrics within 1e-5; sensor integers bit-exact given identical radiance and noise draws; radiance within
 1 % relative RMSE.
 """

@@ -113,10 +113,12 @@ __global__ __launch_bounds__(256) void k_wf_gen(WfLaunch W) {
 // result into hit[slot]; ANY = true: shadow slots (count[0] * nl_max, head count[3]), visibility into sh_c.w.
 // SLDS: stack entries per lane kept in LDS (the rest of the STACK_TOTAL spill to global memory); a shallow
 // LDS stack lets more waves share a CU, which hides the node fetches better than the deeper stack saves.
-// Built for 6 waves per SIMD (at most 80 VGPRs; 7 and 8 spill and ran slower); the body-walking build keeps its
-// natural register count.
+// Built for 6 waves per SIMD (at most 80 VGPRs): 7 and 8 spill and ran slower. The body-walking build (posed
+// bodies: the instance transforms on top) spills some 50 registers at that bound and still ran the real twin frame
+// 22 % faster than at its natural 135 VGPRs and 3 waves (1,637 -> 2,003 Mrays/s; 4 and 5 waves 1,859 and 1,954,
+// 7 and 8 waves 1,504 and 1,317); a 16-entry LDS stack keeps it at 3 waves (six blocks do not fit the LDS).
 template <bool ANY, int SLDS, bool INST>
-__global__ __launch_bounds__(256, INST ? 1 : 6) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
+__global__ __launch_bounds__(256, 6) void k_wf_trace_dyn(WfLaunch W, uint32_t refill) {
   __shared__ uint32_t s_node[SLDS * 256];
   __shared__ float s_t[SLDS * 256];
   Stack st{(lds_u32 *)s_node, (lds_f32 *)s_t, W.spill, threadIdx.x, 256, SLDS, STACK_TOTAL - SLDS, W.spill_stride,

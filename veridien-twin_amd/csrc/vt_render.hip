@@ -931,8 +931,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_backward(RenderLaunch Lc, float *g
   }
 }
 
+// Built for 4 waves per SIMD: the tangent-carrying walk spills about half its registers there and still ran the
+// differentiation frame's camera walk faster than at 2 waves (backward 957 -> 807 ms; 1 wave 1,428 ms, 3 waves
+// 815 ms, 5 and 6 waves 1,430 ms)
 template <int BLOCK, bool INST>
-__global__ __launch_bounds__(BLOCK, 2) void k_camgrad(RenderLaunch Lc, float *gcam) {
+__global__ __launch_bounds__(BLOCK, 4) void k_camgrad(RenderLaunch Lc, float *gcam) {
   __shared__ uint32_t s_node[STACK_LDS * BLOCK];
   __shared__ float s_t[STACK_LDS * BLOCK];
   __shared__ float scam[MAX_CAMS * CAM_PARAMS];

@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256, 6) void k_wf_trace_dyn(WfLaunch W, uint32_t re
 }
 
 // one vertex of path_walk<false>: emitter hit, NEE samples (their shadow rays left to k_wf_shadow), continuation
-// (112 VGPRs, 4 waves per SIMD: bounding it to 6 or 8 waves spills and ran 8 % and 12 % slower)
+// (112-125 VGPRs, 4 waves per SIMD: bounding it to 5, 6 or 8 waves spills and ran 5 %, 8 % and 12 % slower)
 template <bool INST>
 __global__ __launch_bounds__(256) void k_wf_shade(WfLaunch W) {
   // every slot of the batch gets a key: live paths of this vertex theirs, the rest (and the slots past the
